@@ -1,0 +1,457 @@
+"""Seeded synthetic transmission grids and scenario sets.
+
+These generate the inputs of the power-flow workloads the library is measured
+on: a meshed grid (a random recursive spanning tree plus chords between buses
+that are two or three tree hops apart), pi-model branches with a share of
+off-nominal transformers, one slack bus, PV buses with voltage set-points and
+PQ loads.  Everything is a pure function of its seed.
+
+The array conventions are those of the reference's explicit-array ("tuple")
+mode, ``(Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq)``
+(gpusim2grid ``acpf_nr/gpu_facade.py:180-199``): Ybus is a scipy CSR complex
+matrix with sorted column indices, V and Sbus are complex128 per-unit vectors,
+index arrays are int32.  Per-scenario injections are ``(S, n)`` float64 MW /
+MVAr matrices with a system base ``sn_mva`` (``injection_sweep_session.cu:235-247``
+divides them by ``sn_mva``).  Branch data follow ``set_branch_data``
+(``python_bindings.cpp:1745-2166``): from/to bus and the four pi-model
+admittances ``yff, yft, ytf, ytt``.
+"""
+from __future__ import annotations
+
+import dataclasses
+import heapq
+
+import numpy as np
+import scipy.sparse as sp
+
+SN_MVA = 100.0
+
+
+@dataclasses.dataclass
+class Grid:
+    """A solved-topology grid in the tuple-mode conventions."""
+
+    n_bus: int
+    branch_from: np.ndarray  # int32 [m]
+    branch_to: np.ndarray  # int32 [m]
+    r: np.ndarray
+    x: np.ndarray
+    b: np.ndarray
+    tap: np.ndarray  # 1.0 for lines
+    yff: np.ndarray  # complex128 [m]
+    yft: np.ndarray
+    ytf: np.ndarray
+    ytt: np.ndarray
+    Ybus: sp.csr_matrix
+    slack: np.ndarray  # int32 [1]
+    slack_weights: np.ndarray  # float64 [1]
+    pv: np.ndarray  # int32, sorted
+    pq: np.ndarray  # int32, sorted
+    vset: np.ndarray  # float64 [n]; set-point at PV/slack buses, 1.0 elsewhere
+    p_load: np.ndarray  # float64 [n] pu
+    q_load: np.ndarray  # float64 [n] pu
+    p_gen: np.ndarray  # float64 [n] pu (PV buses only)
+    sn_mva: float = SN_MVA
+    draw: int = 0  # rejected draws before this one (see make_grid)
+
+    @property
+    def n_branch(self) -> int:
+        return int(self.branch_from.size)
+
+    @property
+    def Sbus(self) -> np.ndarray:
+        return (self.p_gen - self.p_load) - 1j * self.q_load
+
+    def flat_start(self) -> np.ndarray:
+        """|V| = set-point at PV/slack buses and 1 elsewhere, all angles 0."""
+        return self.vset.astype(np.complex128).copy()
+
+    def as_tuple(self, V=None):
+        """``(Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq)``."""
+        return (self.Ybus, self.flat_start() if V is None else V, self.Sbus,
+                self.slack.copy(), self.slack_weights.copy(), self.pv.copy(),
+                self.pq.copy())
+
+    def branch_data(self):
+        """``set_branch_data`` arguments (without bus_vn_kv, sn_mva)."""
+        return (self.branch_from, self.branch_to, self.yff, self.yft, self.ytf,
+                self.ytt)
+
+
+def _recursive_tree(n: int, rng: np.random.Generator) -> np.ndarray:
+    """Random recursive tree on n nodes, as an (n-1, 2) edge array.
+
+    Node i (in a random labelling) attaches to a uniformly drawn earlier node,
+    which gives the shallow (depth ~ e ln n), hub-and-feeder shape of a
+    transmission grid.
+    """
+    perm = rng.permutation(n)
+    parent = np.array([rng.integers(0, i) for i in range(1, n)], dtype=np.int64)
+    return np.stack([perm[1:], perm[parent]], axis=1).astype(np.int64)
+
+
+def _prufer_tree(n: int, rng: np.random.Generator) -> np.ndarray:
+    """Uniformly random labelled tree on n nodes, as an (n-1, 2) edge array."""
+    if n == 1:
+        return np.zeros((0, 2), dtype=np.int64)
+    if n == 2:
+        return np.array([[0, 1]], dtype=np.int64)
+    seq = rng.integers(0, n, size=n - 2)
+    degree = np.ones(n, dtype=np.int64)
+    np.add.at(degree, seq, 1)
+    leaves = [i for i in range(n) if degree[i] == 1]
+    heapq.heapify(leaves)
+    edges = []
+    for v in seq:
+        leaf = heapq.heappop(leaves)
+        edges.append((leaf, int(v)))
+        degree[v] -= 1
+        if degree[v] == 1:
+            heapq.heappush(leaves, int(v))
+    u = heapq.heappop(leaves)
+    w = heapq.heappop(leaves)
+    edges.append((u, w))
+    return np.asarray(edges, dtype=np.int64)
+
+
+def _tree_chords(n: int, tree: np.ndarray, n_chords: int,
+                 rng: np.random.Generator) -> np.ndarray:
+    """Chords between buses two or three tree hops apart (no duplicates)."""
+    adj = [[] for _ in range(n)]
+    for u, v in tree:
+        adj[u].append(int(v))
+        adj[v].append(int(u))
+    existing = {(min(u, v), max(u, v)) for u, v in tree}
+    chords = []
+    tries = 0
+    while len(chords) < n_chords and tries < 200 * max(n_chords, 1):
+        tries += 1
+        u = int(rng.integers(n))
+        hops = int(rng.integers(2, 4))
+        prev, cur = -1, u
+        ok = True
+        for _ in range(hops):
+            nxt = [w for w in adj[cur] if w != prev]
+            if not nxt:
+                ok = False
+                break
+            prev, cur = cur, nxt[int(rng.integers(len(nxt)))]
+        if not ok or cur == u:
+            continue
+        key = (min(u, cur), max(u, cur))
+        if key in existing:
+            continue
+        existing.add(key)
+        chords.append(key)
+    return np.asarray(chords, dtype=np.int64).reshape(-1, 2)
+
+
+def build_ybus(n: int, f: np.ndarray, t: np.ndarray, yff, yft, ytf, ytt) -> sp.csr_matrix:
+    """Bus admittance matrix from pi-model branch admittances (sorted CSR)."""
+    rows = np.concatenate([f, f, t, t, np.arange(n)])
+    cols = np.concatenate([f, t, f, t, np.arange(n)])
+    vals = np.concatenate([yff, yft, ytf, ytt, np.zeros(n, dtype=np.complex128)])
+    Y = sp.coo_matrix((vals, (rows, cols)), shape=(n, n)).tocsr()
+    Y.sum_duplicates()
+    Y.sort_indices()
+    return Y
+
+
+def _hop_cells(n: int, f: np.ndarray, t: np.ndarray, sources: np.ndarray) -> np.ndarray:
+    """For every bus, the index (into ``sources``) of its nearest source in tree hops."""
+    A = sp.coo_matrix((np.ones(len(f)), (f, t)), shape=(n, n))
+    A = (A + A.T).tocsr()
+    _, _, src = sp.csgraph.dijkstra(A, directed=False, indices=sources,
+                                    unweighted=True, min_only=True,
+                                    return_predecessors=True)
+    pos = np.full(n, -1, dtype=np.int64)
+    pos[sources] = np.arange(len(sources))
+    return pos[src]
+
+
+def flat_start_converges(grid: "Grid", max_iter: int = 10, tol: float = 1e-8,
+                         load_levels=(1.0, 0.7, 1.35), n_rows: int = 4) -> bool:
+    """Acceptance test of a drawn grid: does plain NR converge from a flat start?
+
+    A scipy Newton-Raphson on the MATPOWER-form Jacobian, run at every load
+    multiplier of ``load_levels`` (loads and generation scaled together) and on
+    ``n_rows`` rows of :func:`injection_scenarios`, used only to reject
+    topologies whose operating point NR cannot reach from a flat start across
+    the load range the scenario sets span.
+    """
+    import scipy.sparse.linalg as spla
+    Y = grid.Ybus
+    pvpq = np.sort(np.concatenate([grid.pv, grid.pq]))
+    pq = grid.pq
+    n1 = len(pvpq)
+    cases = [grid.Sbus * level for level in load_levels]
+    if n_rows:
+        p_mw, q_mvar = injection_scenarios(grid, n_rows, seed=10_000 + grid.draw)
+        cases += list((p_mw + 1j * q_mvar) / grid.sn_mva)
+    for S in cases:
+        V = grid.flat_start()
+        ok = False
+        for _ in range(max_iter + 1):
+            I = Y @ V
+            mis = V * np.conj(I) - S
+            F = np.concatenate([mis[pvpq].real, mis[pq].imag])
+            r = np.abs(F).max()
+            if not np.isfinite(r):
+                break
+            if r < tol:
+                ok = True
+                break
+            dV = sp.diags(V)
+            dSa = (1j * dV @ (sp.diags(I) - Y @ dV).conj()).tocsr()
+            dSm = (dV @ (Y @ sp.diags(V / np.abs(V))).conj()
+                   + sp.diags(np.conj(I) * V / np.abs(V))).tocsr()
+            J = sp.bmat([[dSa[pvpq][:, pvpq].real, dSm[pvpq][:, pq].real],
+                         [dSa[pq][:, pvpq].imag, dSm[pq][:, pq].imag]], format="csc")
+            dx = spla.spsolve(J, -F)
+            Va = np.angle(V)
+            Vm = np.abs(V)
+            Va[pvpq] += dx[:n1]
+            Vm[pq] += dx[n1:]
+            V = Vm * np.exp(1j * Va)
+        if not ok:
+            return False
+    return True
+
+
+def _draw_grid(n: int, n_branch: int, n_pv: int, rng: np.random.Generator,
+               load_scale: float) -> Grid:
+    tree = _recursive_tree(n, rng)
+    chords = _tree_chords(n, tree, max(int(n_branch) - (n - 1), 0), rng)
+    edges = np.concatenate([tree, chords], axis=0)
+    # orient and shuffle branches so from/to carry no structure
+    flip = rng.random(len(edges)) < 0.5
+    edges[flip] = edges[flip][:, ::-1]
+    edges = edges[rng.permutation(len(edges))]
+    m = len(edges)
+    f = edges[:, 0].astype(np.int32)
+    t = edges[:, 1].astype(np.int32)
+
+    r = rng.uniform(0.005, 0.05, m)
+    x = r * rng.uniform(3.0, 10.0, m)
+    bch = rng.uniform(0.0, 0.05, m)
+    is_trafo = rng.random(m) < 0.2
+    tap = np.where(is_trafo, rng.uniform(0.95, 1.05, m), 1.0)
+    ys = 1.0 / (r + 1j * x)
+    ytt = ys + 0.5j * bch
+    yff = ytt / (tap * tap)
+    yft = -ys / tap
+    ytf = -ys / tap
+    Y = build_ybus(n, f, t, yff, yft, ytf, ytt)
+
+    deg = np.bincount(np.concatenate([f, t]), minlength=n)
+    slack = int(np.argmax(deg))
+    others = np.array([i for i in range(n) if i != slack], dtype=np.int64)
+    n_pv = min(max(int(n_pv), 0), n - 1)
+    pv = np.sort(rng.choice(others, size=n_pv, replace=False)).astype(np.int32)
+    is_pv = np.zeros(n, dtype=bool)
+    is_pv[pv] = True
+    pq = np.array([i for i in others if not is_pv[i]], dtype=np.int32)
+
+    vset = np.ones(n)
+    vset[pv] = rng.uniform(0.98, 1.05, n_pv)
+    vset[slack] = rng.uniform(0.98, 1.05)
+
+    p_load = np.zeros(n)
+    q_load = np.zeros(n)
+    p_load[others] = rng.uniform(0.1, 1.0, n - 1) * load_scale
+    pf = rng.uniform(0.9, 0.98, n - 1)
+    q_load[others] = p_load[others] * np.tan(np.arccos(pf))
+    p_gen = np.zeros(n)
+    if n_pv > 0:
+        # every bus is served by its nearest generator (in hops); a PV unit
+        # produces 1.05 x the load of its own cell, the slack covers its cell
+        # and the losses
+        src = np.concatenate([[slack], pv]).astype(np.int64)
+        cell = _hop_cells(n, f, t, src)
+        served = np.zeros(len(src))
+        np.add.at(served, cell, p_load)
+        p_gen[pv] = 1.05 * served[1:]
+
+    return Grid(n_bus=n, branch_from=f, branch_to=t, r=r, x=x, b=bch, tap=tap,
+                yff=yff, yft=yft, ytf=ytf, ytt=ytt, Ybus=Y,
+                slack=np.array([slack], dtype=np.int32),
+                slack_weights=np.array([1.0]), pv=pv, pq=pq, vset=vset,
+                p_load=p_load, q_load=q_load, p_gen=p_gen)
+
+
+def make_grid(n_bus: int, n_branch: int, pv_frac: float | None = None, *,
+              n_pv: int | None = None, seed: int = 0,
+              load_scale: float = 0.05, accept: bool = True,
+              max_draws: int = 64, first_draw: int = 0) -> Grid:
+    """Seeded synthetic grid.
+
+    Topology: a random recursive spanning tree (bus i attaches to a uniformly
+    drawn earlier bus, which gives a transmission grid's shallow hub-and-feeder
+    shape) plus chords between buses two or three tree hops apart,
+    ``n_branch`` branches in total.  Branches: r ~ U(0.005, 0.05),
+    x = r U(3, 10), b ~ U(0, 0.05); 20 % are transformers with tap
+    ~ U(0.95, 1.05) on the from side.  The slack is the highest-degree bus;
+    PV buses (``n_pv`` or ``round(pv_frac n)``) have Vset ~ U(0.98, 1.05).
+    Every non-slack bus carries a load P ~ U(0.1, 1.0) x ``load_scale`` pu
+    with Q = P tan(acos(pf)), pf ~ U(0.9, 0.98).  Generation is spread over
+    the PV buses regionally: each bus is served by its nearest generator and a
+    PV unit produces 1.05 x the load it serves.
+
+    With ``accept`` the grid is redrawn (same seed, next draw) until plain NR
+    converges from a flat start at 0.7, 1 and 1.35 x its loads
+    (:func:`flat_start_converges`); ``grid.draw`` records the draw index.
+    ``first_draw`` starts the search later (the workload constructors below
+    pass the draw their search ends on, so they skip the rejected draws).
+    """
+    n = int(n_bus)
+    if n_pv is None:
+        n_pv = int(round((pv_frac or 0.0) * n))
+    for draw in range(int(first_draw), int(first_draw) + max_draws):
+        rng = np.random.default_rng([int(seed), draw])
+        g = _draw_grid(n, n_branch, n_pv, rng, load_scale)
+        g.draw = draw
+        if not accept or flat_start_converges(g):
+            return g
+    raise RuntimeError(f"no convergent grid in {max_draws} draws (n={n}, seed={seed})")
+
+
+def injection_scenarios(grid: Grid, n_scen: int, seed: int = 1,
+                        sigma_row: float = 0.1, sigma_bus: float = 0.05):
+    """Per-row load scaling with proportional PV redispatch.
+
+    Row s scales the load of bus i by ``a_s c_{s,i}`` with a_s ~ lognormal(0,
+    sigma_row) and c_{s,i} ~ lognormal(0, sigma_bus); the PV generation is
+    scaled by the row's total-load ratio.  Returns ``(p_mw, q_mvar)``, each
+    ``(n_scen, n_bus)`` net injections in MW / MVAr (generation minus load),
+    the arrays ``set_injections(p_mw, q_mvar, sn_mva)`` takes.
+    """
+    rng = np.random.default_rng(seed)
+    n = grid.n_bus
+    a = rng.lognormal(0.0, sigma_row, size=(n_scen, 1))
+    c = rng.lognormal(0.0, sigma_bus, size=(n_scen, n))
+    scale = a * c
+    pl = grid.p_load[None, :] * scale
+    ql = grid.q_load[None, :] * scale
+    ratio = pl.sum(axis=1, keepdims=True) / max(grid.p_load.sum(), 1e-300)
+    pg = grid.p_gen[None, :] * ratio
+    p_mw = (pg - pl) * grid.sn_mva
+    q_mvar = (-ql) * grid.sn_mva
+    return p_mw, q_mvar
+
+
+def bridges(n: int, f: np.ndarray, t: np.ndarray) -> np.ndarray:
+    """Boolean mask of the branches whose removal disconnects the graph.
+
+    Iterative Tarjan low-link over the multigraph (a branch parallel to another
+    is never a bridge).
+    """
+    m = len(f)
+    adj_ptr = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(adj_ptr, np.asarray(f) + 1, 1)
+    np.add.at(adj_ptr, np.asarray(t) + 1, 1)
+    adj_ptr = np.cumsum(adj_ptr)
+    adj_to = np.empty(2 * m, dtype=np.int64)
+    adj_e = np.empty(2 * m, dtype=np.int64)
+    head = adj_ptr[:-1].copy()
+    for e in range(m):
+        u, v = int(f[e]), int(t[e])
+        adj_to[head[u]] = v
+        adj_e[head[u]] = e
+        head[u] += 1
+        adj_to[head[v]] = u
+        adj_e[head[v]] = e
+        head[v] += 1
+    disc = np.full(n, -1, dtype=np.int64)
+    low = np.zeros(n, dtype=np.int64)
+    is_bridge = np.zeros(m, dtype=bool)
+    timer = 0
+    for root in range(n):
+        if disc[root] >= 0:
+            continue
+        disc[root] = low[root] = timer
+        timer += 1
+        stack = [(root, -1, adj_ptr[root])]
+        while stack:
+            u, pe, it = stack[-1]
+            if it < adj_ptr[u + 1]:
+                stack[-1] = (u, pe, it + 1)
+                v, e = int(adj_to[it]), int(adj_e[it])
+                if e == pe:
+                    continue
+                if disc[v] < 0:
+                    disc[v] = low[v] = timer
+                    timer += 1
+                    stack.append((v, e, adj_ptr[v]))
+                else:
+                    low[u] = min(low[u], disc[v])
+            else:
+                stack.pop()
+                if stack:
+                    p = stack[-1][0]
+                    low[p] = min(low[p], low[u])
+                    if low[u] > disc[p]:
+                        is_bridge[pe] = True
+    return is_bridge
+
+
+def random_outages(grid: Grid, n_rows: int, max_k: int = 2, seed: int = 3):
+    """0..max_k random non-bridge branch outages per row that keep the grid connected.
+
+    Returns a list of int lists.  A row's set is re-drawn until removing all of
+    its branches leaves a connected graph.
+    """
+    rng = np.random.default_rng(seed)
+    nb = ~bridges(grid.n_bus, grid.branch_from, grid.branch_to)
+    cand = np.flatnonzero(nb)
+    out = []
+    for _ in range(n_rows):
+        k = int(rng.integers(0, max_k + 1))
+        while True:
+            ids = sorted(rng.choice(cand, size=k, replace=False).tolist()) if k else []
+            if k <= 1 or _connected_without(grid, ids):
+                break
+        out.append(ids)
+    return out
+
+
+def _connected_without(grid: Grid, ids) -> bool:
+    keep = np.ones(grid.n_branch, dtype=bool)
+    keep[list(ids)] = False
+    f = grid.branch_from[keep]
+    t = grid.branch_to[keep]
+    A = sp.coo_matrix((np.ones(len(f)), (f, t)), shape=(grid.n_bus, grid.n_bus))
+    ncomp, _ = sp.csgraph.connected_components(A, directed=False)
+    return ncomp == 1
+
+
+# ---------------------------------------------------------------------------
+# The measured workloads (grid seed 0; scenarios seed 1).  Each constructor
+# starts at the draw the acceptance search of make_grid(..., seed=0) ends on,
+# so it costs one draw; tests/test_synthetic.py re-runs the search.
+# ---------------------------------------------------------------------------
+
+ACCEPTED_DRAW = {14: 0, 118: 0, 2000: 0, 6500: 22}
+
+
+def grid_c0(seed: int = 0) -> Grid:
+    """14 buses, 20 branches, 1 slack + 4 PV (single solve)."""
+    return make_grid(14, 20, n_pv=4, seed=seed, accept=seed != 0,
+                     first_draw=ACCEPTED_DRAW[14] if seed == 0 else 0)
+
+
+def grid_118(seed: int = 0) -> Grid:
+    """118 buses, 180 branches, 15 % PV (small test grid)."""
+    return make_grid(118, 180, pv_frac=0.15, seed=seed, accept=seed != 0,
+                     first_draw=ACCEPTED_DRAW[118] if seed == 0 else 0)
+
+
+def grid_c1(seed: int = 0) -> Grid:
+    """2,000 buses, 3,000 branches, 15 % PV (N-1 screen; learned warm start)."""
+    return make_grid(2000, 3000, pv_frac=0.15, seed=seed, accept=seed != 0,
+                     first_draw=ACCEPTED_DRAW[2000] if seed == 0 else 0)
+
+
+def grid_c2(seed: int = 0) -> Grid:
+    """6,500 buses, 9,000 branches, 12 % PV (injection sweep, 16,384 rows)."""
+    return make_grid(6500, 9000, pv_frac=0.12, seed=seed, accept=seed != 0,
+                     first_draw=ACCEPTED_DRAW[6500] if seed == 0 else 0)

@@ -1,0 +1,212 @@
+"""The oracle: CPU FP64 Newton-Raphson power flow (test infrastructure only).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg import this module, as the checker.  The product package never does.
+
+Two independent restatements of the reference's bare ``[pvpq | pq]``
+Newton-Raphson live here:
+
+* ``libnr_oracle.so`` (``nr_oracle.c``): the J layout of
+  ``src/_cpp/acpf_nr.cu:145-236``, mismatch/J/update of
+  ``src/_cpp/acpf_nr_kernels.cu:493-788``, status rules of
+  ``src/_cpp/contingency/row_status.hpp:28-50``, with its own sparse LU
+  (minimum degree + partial pivoting) and POSIX threads over rows;
+* numpy/scipy helpers in MATPOWER form (the reference's own test helpers
+  ``tests/python/_jacobian_ref.py:26-55`` and ``tests/python/conftest.py:326-338``),
+  used to pin the C oracle.
+
+Pinning (tests/test_oracle.py): the C mismatch equals the numpy closed form;
+the C Jacobian equals the MATPOWER-form Jacobian and central finite
+differences of F; the C sparse solve equals scipy's SuperLU; the NR converges
+quadratically and its solutions satisfy the power-flow equations.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+import scipy.sparse as sp
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def build() -> None:
+    """Compile ``libnr_oracle.so`` with gcc (in place)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libnr_oracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.orc_jpattern_size.restype = ctypes.c_int
+        L.orc_jpattern_size.argtypes = [ctypes.c_int, _i32p, _i32p, ctypes.c_int, _i32p,
+                                        ctypes.c_int, _i32p, ctypes.POINTER(ctypes.c_int)]
+        L.orc_jpattern.restype = None
+        L.orc_jpattern.argtypes = [ctypes.c_int, _i32p, _i32p, ctypes.c_int, _i32p, ctypes.c_int,
+                                   _i32p, _i32p, _i32p, _i32p]
+        L.orc_f_and_j.restype = ctypes.c_double
+        L.orc_f_and_j.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, ctypes.c_int, _i32p,
+                                  ctypes.c_int, _i32p, _f64p, _f64p, _f64p, ctypes.c_void_p]
+        L.orc_sparse_solve.restype = ctypes.c_int
+        L.orc_sparse_solve.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+        L.orc_nr_batch.restype = None
+        L.orc_nr_batch.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, ctypes.c_longlong,
+                                   ctypes.c_int, _i32p, ctypes.c_int, _i32p, ctypes.c_int, _f64p,
+                                   _f64p, ctypes.c_longlong, ctypes.c_int, ctypes.c_double, _i32p,
+                                   _i32p, _f64p, ctypes.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _csr(Y):
+    Y = sp.csr_matrix(Y)
+    Y.sort_indices()
+    return (Y.indptr.astype(np.int32), Y.indices.astype(np.int32),
+            np.ascontiguousarray(Y.data.astype(np.complex128)))
+
+
+def _cplx(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.complex128)).view(np.float64)
+
+
+def jpattern(Ybus, pv, pq):
+    """Reference-layout J CSR (Jp, Ji) and the four scatter maps [4, nY]."""
+    Yp, Yi, _ = _csr(Ybus)
+    n = Yp.size - 1
+    pv = np.ascontiguousarray(pv, dtype=np.int32)
+    pq = np.ascontiguousarray(pq, dtype=np.int32)
+    d = ctypes.c_int()
+    nJ = lib().orc_jpattern_size(n, Yp, Yi, pv.size, pv, pq.size, pq, ctypes.byref(d))
+    Jp = np.empty(d.value + 1, np.int32)
+    Ji = np.empty(nJ, np.int32)
+    maps = np.empty(4 * Yi.size, np.int32)
+    lib().orc_jpattern(n, Yp, Yi, pv.size, pv, pq.size, pq, Jp, Ji, maps)
+    return Jp, Ji, maps.reshape(4, -1)
+
+
+def f_and_j(Ybus, V, Sbus, pv, pq, with_j=True):
+    """(F, J csr, ||F||inf) at V, in the reference layout."""
+    Yp, Yi, Yx = _csr(Ybus)
+    n = Yp.size - 1
+    pv = np.ascontiguousarray(pv, dtype=np.int32)
+    pq = np.ascontiguousarray(pq, dtype=np.int32)
+    Jp, Ji, _ = jpattern(Ybus, pv, pq)
+    d = Jp.size - 1
+    F = np.empty(d)
+    Jx = np.empty(Ji.size)
+    r = lib().orc_f_and_j(n, Yp, Yi, _cplx(Yx), pv.size, pv, pq.size, pq, _cplx(V).copy(),
+                          _cplx(Sbus).copy(), F, Jx.ctypes.data if with_j else None)
+    J = sp.csr_matrix((Jx, Ji, Jp), shape=(d, d)) if with_j else None
+    return F, J, r
+
+
+def sparse_solve(A, b):
+    """x = A^-1 b with the oracle's own sparse LU."""
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    x = np.empty(A.shape[0])
+    rc = lib().orc_sparse_solve(A.shape[0], A.indptr.astype(np.int32), A.indices.astype(np.int32),
+                                np.ascontiguousarray(A.data, dtype=np.float64),
+                                np.ascontiguousarray(b, dtype=np.float64), x)
+    if rc != 0:
+        raise ArithmeticError("oracle LU: structurally or numerically singular")
+    return x
+
+
+def nr_batch(Ybus, V0, Sbus, pv, pq, max_iter=10, tol=1e-8, n_threads=1, Yx_rows=None):
+    """Plain FP64 NR over rows.
+
+    V0 : (n,) shared start or (rows, n); Sbus : (n,) shared or (rows, n);
+    Yx_rows : optional (rows, nY) per-row Ybus values on Ybus's pattern.
+    Returns (V [rows, n], status [rows], iters [rows], residual [rows]).
+    Status: 0 Converged, 1 MaxIter, 2 Diverged.
+    """
+    Yp, Yi, Yx = _csr(Ybus)
+    n = Yp.size - 1
+    S = np.atleast_2d(np.asarray(Sbus, dtype=np.complex128))
+    rows = max(S.shape[0], np.atleast_2d(V0).shape[0],
+               1 if Yx_rows is None else np.atleast_2d(Yx_rows).shape[0])
+    V = np.array(np.broadcast_to(np.atleast_2d(np.asarray(V0, dtype=np.complex128)), (rows, n)))
+    s_stride = n if S.shape[0] > 1 else 0
+    if Yx_rows is None:
+        Yall, y_stride = Yx, 0
+    else:
+        Yall = np.ascontiguousarray(np.atleast_2d(Yx_rows), dtype=np.complex128)
+        y_stride = Yall.shape[1]
+    pv = np.ascontiguousarray(pv, dtype=np.int32)
+    pq = np.ascontiguousarray(pq, dtype=np.int32)
+    status = np.empty(rows, np.int32)
+    iters = np.empty(rows, np.int32)
+    res = np.empty(rows)
+    Vf = np.ascontiguousarray(V).view(np.float64)
+    lib().orc_nr_batch(n, Yp, Yi, _cplx(Yall), y_stride, pv.size, pv, pq.size, pq, rows, Vf,
+                       _cplx(np.ascontiguousarray(S)), s_stride, int(max_iter), float(tol), status,
+                       iters, res, int(n_threads))
+    return Vf.view(np.complex128).reshape(rows, n), status, iters, res
+
+
+# ---------------------------------------------------------------------------
+# numpy / scipy restatements (MATPOWER form) used to pin the C oracle
+# ---------------------------------------------------------------------------
+
+def mismatch_np(Ybus, V, Sbus, pv, pq):
+    """F = -[Re(mis)(pvpq sorted) ; Im(mis)(pq)], mis = V conj(Y V) - Sbus."""
+    pvpq = np.sort(np.concatenate([pv, pq]))
+    mis = V * np.conj(Ybus @ V) - Sbus
+    return -np.concatenate([mis[pvpq].real, mis[np.asarray(pq)].imag])
+
+
+def jacobian_np(Ybus, V, pv, pq):
+    """MATPOWER dS/dVa, dS/dVm restricted to [pvpq | pq] (dense)."""
+    Ybus = sp.csr_matrix(Ybus)
+    pvpq = np.sort(np.concatenate([pv, pq]))
+    pq = np.asarray(pq)
+    I = Ybus @ V
+    dV = sp.diags(V)
+    dSa = (1j * dV @ (sp.diags(I) - Ybus @ dV).conj()).toarray()
+    dSm = (dV @ (Ybus @ sp.diags(V / np.abs(V))).conj() + sp.diags(np.conj(I) * V / np.abs(V))).toarray()
+    return np.block([[dSa[np.ix_(pvpq, pvpq)].real, dSm[np.ix_(pvpq, pq)].real],
+                     [dSa[np.ix_(pq, pvpq)].imag, dSm[np.ix_(pq, pq)].imag]])
+
+
+def polar_update_np(V, dx, pv, pq):
+    """theta += dx (pvpq), then |V| += dx (pq): the reference's polar update."""
+    pvpq = np.sort(np.concatenate([pv, pq]))
+    Va = np.angle(V)
+    Vm = np.abs(V)
+    Va[pvpq] += dx[:pvpq.size]
+    Vm[np.asarray(pq)] += dx[pvpq.size:]
+    return Vm * np.exp(1j * Va)
+
+
+def outage_ybus(grid, branch_ids):
+    """Ybus rebuilt from the branch list without ``branch_ids`` (closed form)."""
+    from gpusim2grid_amd.synthetic import build_ybus
+    keep = np.ones(grid.n_branch, dtype=bool)
+    keep[list(branch_ids)] = False
+    Y = build_ybus(grid.n_bus, grid.branch_from[keep], grid.branch_to[keep], grid.yff[keep],
+                   grid.yft[keep], grid.ytf[keep], grid.ytt[keep])
+    return Y
+
+
+def ybus_values_on_pattern(Ybase, Y):
+    """Values of Y on the (superset) pattern of Ybase, zeros where Y has none."""
+    Ybase = sp.csr_matrix(Ybase)
+    Ybase.sort_indices()
+    Yd = sp.csr_matrix(Y)
+    out = np.zeros(Ybase.nnz, np.complex128)
+    for i in range(Ybase.shape[0]):
+        cols = Ybase.indices[Ybase.indptr[i]:Ybase.indptr[i + 1]]
+        row = Yd[i].toarray().ravel()
+        out[Ybase.indptr[i]:Ybase.indptr[i + 1]] = row[cols]
+    return out
