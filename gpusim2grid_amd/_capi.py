@@ -1,0 +1,251 @@
+"""ctypes binding of the C ABI (``include/gpusim2grid_amd.h``).
+
+This is the binding a Python caller without the pybind11 module would add;
+the parity tests drive the kernels through it.  Errors raise ``ValueError``
+(invalid argument) or ``RuntimeError`` (runtime / HIP failure), as the
+reference maps ``std::invalid_argument`` / ``std::runtime_error``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpusim2grid_amd.so")
+
+G2G_OK, G2G_ERR_INVALID, G2G_ERR_RUNTIME, G2G_ERR_HIP = 0, 1, 2, 3
+FP64, MIXED = 0, 1
+CHUNKED, CONTINUOUS = 0, 1
+CONVERGED, MAXITER, DIVERGED, NOTSIMULATED = 0, 1, 2, 3
+ARR = dict(jp=0, ji=1, maps=2, pvpq=3, perm=4, colp=5, coli=6, prog_f=7, prog_b=8, apos=9, fpos=10,
+           uoff=11)
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int32) for k in (
+        "n", "nY", "d", "nJ", "npvpq", "npq", "slack", "nnz", "nLU", "n_updates", "max_c", "height",
+        "slots_f", "slots_b", "a_len", "u_len")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("n_converged", ctypes.c_int32),
+                ("n_maxiter", ctypes.c_int32), ("n_diverged", ctypes.c_int32),
+                ("ms_total", ctypes.c_double), ("ms_mismatch", ctypes.c_double),
+                ("ms_factor", ctypes.c_double), ("ms_backward", ctypes.c_double),
+                ("bytes_factor", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_LIB = None
+_P = ctypes.c_void_p
+
+
+def lib():
+    """Loads ``libgpusim2grid_amd.so`` (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m gpusim2grid_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    i32, i64, dbl = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    sig = {
+        "g2g_last_error": (ctypes.c_char_p, []),
+        "g2g_version": (ctypes.c_int, []),
+        "g2g_plan_create": (ctypes.c_int, [i32, _P, _P, _P, i32, _P, i32, _P, i32, ctypes.POINTER(_P)]),
+        "g2g_lu_plan_create": (ctypes.c_int, [i32, _P, _P, _P, ctypes.POINTER(_P)]),
+        "g2g_plan_destroy": (None, [_P]),
+        "g2g_plan_get_info": (ctypes.c_int, [_P, ctypes.POINTER(PlanInfo)]),
+        "g2g_plan_get_array": (ctypes.c_int, [_P, i32, _P, i64, ctypes.POINTER(i64)]),
+        "g2g_batch_create": (ctypes.c_int, [_P, i32, i32, i32, _P, ctypes.POINTER(_P)]),
+        "g2g_batch_destroy": (None, [_P]),
+        "g2g_batch_set_v": (ctypes.c_int, [_P, _P, i64, i32, i32, i32]),
+        "g2g_batch_set_sbus": (ctypes.c_int, [_P, _P, i64, i32, i32, i32]),
+        "g2g_batch_set_ybus_edits": (ctypes.c_int, [_P, _P, _P, _P, i32]),
+        "g2g_batch_set_active_rows": (ctypes.c_int, [_P, i32]),
+        "g2g_batch_solve": (ctypes.c_int, [_P, i32, dbl, i32, ctypes.POINTER(SolveStats)]),
+        "g2g_batch_get_v": (ctypes.c_int, [_P, _P, i64, i32]),
+        "g2g_batch_get_results": (ctypes.c_int, [_P, _P, _P, _P]),
+        "g2g_batch_eval": (ctypes.c_int, [_P, _P, _P, _P]),
+        "g2g_lu_solve_batch": (ctypes.c_int, [_P, i32, i32, _P, _P, _P, dbl, _P, _P]),
+        "g2g_batch_device_arrays": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def exported_symbols():
+    """Names of the g2g_* entry points the header declares."""
+    hdr = os.path.join(os.path.dirname(HERE), "include", "gpusim2grid_amd.h")
+    import re
+    with open(hdr) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(g2g_[a-z0-9_]+)\s*\(", text)))
+
+
+def _check(rc):
+    if rc != G2G_OK:
+        msg = lib().g2g_last_error().decode()
+        if rc == G2G_ERR_INVALID:
+            raise ValueError(msg)
+        raise RuntimeError(msg)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+class Plan:
+    """Row-independent tables of one grid (or one raw matrix pattern)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def from_grid(cls, Ybus, pv, pq, slack):
+        import scipy.sparse as sp
+        Y = sp.csr_matrix(Ybus)
+        Y.sort_indices()
+        Yp, Yi, Yx = _i32(Y.indptr), _i32(Y.indices), _c128(Y.data)
+        pv, pq = _i32(pv), _i32(pq)
+        h = _P()
+        _check(lib().g2g_plan_create(Y.shape[0], _ptr(Yp), _ptr(Yi), _ptr(Yx), pv.size, _ptr(pv), pq.size,
+                                     _ptr(pq), int(np.atleast_1d(slack)[0]), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_matrix(cls, A, order=None):
+        import scipy.sparse as sp
+        A = sp.csr_matrix(A)
+        A.sort_indices()
+        Ap, Ai = _i32(A.indptr), _i32(A.indices)
+        o = None if order is None else _i32(order)
+        h = _P()
+        _check(lib().g2g_lu_plan_create(A.shape[0], _ptr(Ap), _ptr(Ai), _ptr(o), ctypes.byref(h)))
+        return cls(h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().g2g_plan_destroy(self._h)
+            self._h = None
+
+    def info(self):
+        inf = PlanInfo()
+        _check(lib().g2g_plan_get_info(self._h, ctypes.byref(inf)))
+        return inf.as_dict()
+
+    def array(self, name):
+        n = ctypes.c_int64()
+        _check(lib().g2g_plan_get_array(self._h, ARR[name], None, 0, ctypes.byref(n)))
+        out = np.empty(n.value, np.int32)
+        _check(lib().g2g_plan_get_array(self._h, ARR[name], _ptr(out), out.size, ctypes.byref(n)))
+        return out
+
+    def lu_solve(self, vals, rhs, precision=FP64, pivot_eps=None, stream=None):
+        vals = np.ascontiguousarray(np.atleast_2d(vals), dtype=np.float64)
+        rhs = np.ascontiguousarray(np.atleast_2d(rhs), dtype=np.float64)
+        rows = vals.shape[0]
+        x = np.empty_like(rhs)
+        cl = np.empty(rows, np.int32)
+        eps = (1e-5 if precision == MIXED else 1e-12) if pivot_eps is None else pivot_eps
+        _check(lib().g2g_lu_solve_batch(self._h, rows, precision, _ptr(vals), _ptr(rhs), _ptr(x), eps, _ptr(cl),
+                                        stream))
+        return x, cl
+
+
+class Batch:
+    """Device state of many power flows of one plan."""
+
+    def __init__(self, plan: Plan, rows: int, precision=FP64, per_row_ybus=False, stream=None):
+        self.plan = plan
+        self.rows = int(rows)
+        self.n = plan.info()["n"]
+        h = _P()
+        _check(lib().g2g_batch_create(plan._h, self.rows, precision, int(bool(per_row_ybus)), stream,
+                                      ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().g2g_batch_destroy(self._h)
+            self._h = None
+
+    def set_v(self, V, row0=0):
+        V = _c128(V)
+        rows = 1 if V.ndim == 1 else V.shape[0]
+        ld = 0 if V.ndim == 1 else V.shape[1]
+        if V.ndim == 1:
+            rows = self.rows - row0
+        _check(lib().g2g_batch_set_v(self._h, _ptr(V), ld, rows, row0, 0))
+
+    def set_sbus(self, S, row0=0):
+        S = _c128(S)
+        ld = 0 if S.ndim == 1 else S.shape[1]
+        rows = self.rows - row0 if S.ndim == 1 else S.shape[0]
+        _check(lib().g2g_batch_set_sbus(self._h, _ptr(S), ld, rows, row0, 0))
+
+    def set_v_device(self, ptr, ld, rows, row0=0):
+        _check(lib().g2g_batch_set_v(self._h, _P(ptr), ld, rows, row0, 1))
+
+    def set_sbus_device(self, ptr, ld, rows, row0=0):
+        _check(lib().g2g_batch_set_sbus(self._h, _P(ptr), ld, rows, row0, 1))
+
+    def set_ybus_edits(self, row_ptr, entry, delta):
+        row_ptr, entry, delta = _i32(row_ptr), _i32(entry), _c128(delta)
+        _check(lib().g2g_batch_set_ybus_edits(self._h, _ptr(row_ptr), _ptr(entry), _ptr(delta),
+                                              row_ptr.size - 1))
+
+    def set_active_rows(self, k):
+        _check(lib().g2g_batch_set_active_rows(self._h, int(k)))
+
+    def solve(self, max_iter=10, tol=1e-8, scheduling=CONTINUOUS):
+        st = SolveStats()
+        _check(lib().g2g_batch_solve(self._h, int(max_iter), float(tol), int(scheduling), ctypes.byref(st)))
+        return st.as_dict()
+
+    def get_v(self):
+        V = np.empty((self.rows, self.n), np.complex128)
+        _check(lib().g2g_batch_get_v(self._h, _ptr(V), self.n, 0))
+        return V
+
+    def get_v_device(self, ptr, ld):
+        _check(lib().g2g_batch_get_v(self._h, _P(ptr), ld, 1))
+
+    def results(self):
+        st = np.empty(self.rows, np.int32)
+        it = np.empty(self.rows, np.int32)
+        res = np.empty(self.rows)
+        _check(lib().g2g_batch_get_results(self._h, _ptr(st), _ptr(it), _ptr(res)))
+        return st, it, res
+
+    def eval(self, with_j=True):
+        inf = self.plan.info()
+        F = np.empty((self.rows, inf["d"]))
+        J = np.empty((self.rows, inf["nJ"])) if with_j else None
+        r = np.empty(self.rows)
+        _check(lib().g2g_batch_eval(self._h, _ptr(F), _ptr(J), _ptr(r)))
+        return F, J, r
+
+    def device_arrays(self):
+        v, s = _P(), _P()
+        _check(lib().g2g_batch_device_arrays(self._h, ctypes.byref(v), ctypes.byref(s)))
+        return v.value, s.value

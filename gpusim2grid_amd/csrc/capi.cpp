@@ -1,0 +1,729 @@
+// C ABI of the batched Newton-Raphson power flow (see include/gpusim2grid_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/gpusim2grid_amd.h"
+#include "device_common.hpp"
+#include "plan.hpp"
+
+namespace g2g {
+
+// kernels (kernels.hip)
+template <typename Real, bool PerRowY, bool WithJ>
+__global__ void k_mismatch(MismatchArgs<Real> a);
+template <typename Real>
+__global__ void k_factor(FactorArgs<Real> a);
+template <typename Real, bool Update>
+__global__ void k_backward(BackwardArgs<Real> a);
+template <typename T>
+__global__ void k_rows_to_tiles(const T* in, long long ld, int rows, int m, int row_offset, T* out, int broadcast);
+template <typename T>
+__global__ void k_tiles_to_rows(const T* in, int rows, int m, T* out, long long ld);
+__global__ void k_ybus_broadcast(const double2* Y, int nY, int rows_pad, double2* Yrow);
+__global__ void k_ybus_edits(const int* row_ptr, const int* ent, const double2* delta, int rows, int nY, double2* Yrow);
+template <typename Real>
+__global__ void k_gather_astream(const double* vals, long long ldv, const double* rhs, long long ldb, int rows,
+                                 int nnz, int d, const int* a_pos, const int* f_pos, int a_len, Real* A);
+__global__ void k_row_absmax(const double* vals, long long ldv, int rows, int nnz, double* out);
+__global__ void k_fill_int(int* p, int v, long long n);
+__global__ void k_count_tiles(const int* status, int rows_pad, int* tile_active);
+
+}  // namespace g2g
+
+using namespace g2g;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(call)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess) throw HipError(std::string(#call) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+template <typename F>
+int guarded(F&& f)
+{
+    try {
+        f();
+        return G2G_OK;
+    } catch (const std::invalid_argument& e) {
+        g_err = e.what();
+        return G2G_ERR_INVALID;
+    } catch (const HipError& e) {
+        g_err = e.what();
+        return G2G_ERR_HIP;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return G2G_ERR_RUNTIME;
+    }
+}
+
+template <typename T>
+T* dalloc(size_t count)
+{
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    HIPCHK(hipMalloc(&p, count * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+template <typename T>
+T* dupload(const std::vector<T>& v)
+{
+    T* p = dalloc<T>(v.size());
+    if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+}
+
+void dfree(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+inline unsigned blocks_for(long long n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
+
+}  // namespace
+
+struct g2g_plan_s {
+    bool grid = false;
+    NrPlan nr;           // grid plans
+    SymbolicLU raw;      // raw matrix plans
+    int nnz = 0;
+    int device = 0;
+    int* d_prog_f = nullptr;
+    int* d_prog_b = nullptr;
+    int* d_k1_bus = nullptr;
+    int* d_k1_ent = nullptr;
+    double2* d_Y = nullptr;
+    int* d_perm = nullptr;
+    int* d_apos = nullptr;
+    int* d_fpos = nullptr;
+    bool uploaded = false;
+    const SymbolicLU& lu() const { return grid ? nr.lu : raw; }
+    ~g2g_plan_s()
+    {
+        dfree(d_prog_f); dfree(d_prog_b); dfree(d_k1_bus); dfree(d_k1_ent);
+        dfree(d_Y); dfree(d_perm); dfree(d_apos); dfree(d_fpos);
+    }
+    // device copies of the tables, made on first use (plans can be built and
+    // inspected on a host without a GPU)
+    void upload()
+    {
+        if (uploaded) return;
+        HIPCHK(hipGetDevice(&device));
+        const SymbolicLU& s = lu();
+        if (s.max_c > kMaxC)
+            throw std::runtime_error("pivot column of " + std::to_string(s.max_c) + " entries exceeds the kernels' " +
+                                     std::to_string(kMaxC));
+        d_prog_f = dupload(s.prog_f);
+        d_prog_b = dupload(s.prog_b);
+        d_perm = dupload(s.perm);
+        d_apos = dupload(s.a_pos_of_entry);
+        d_fpos = dupload(s.f_pos);
+        if (grid) {
+            d_k1_bus = dupload(nr.k1_bus);
+            d_k1_ent = dupload(nr.k1_ent);
+            std::vector<double2> y(static_cast<size_t>(nr.nY));
+            for (int k = 0; k < nr.nY; ++k) y[k] = make_double2(nr.Yx[2 * k], nr.Yx[2 * k + 1]);
+            d_Y = dupload(y);
+        }
+        uploaded = true;
+    }
+};
+
+struct g2g_batch_s {
+    g2g_plan_s* plan = nullptr;
+    int rows = 0, rows_pad = 0, T = 0, precision = G2G_FP64, per_row_y = 0, active_rows = 0;
+    hipStream_t stream = nullptr;
+    double2* V = nullptr;
+    double2* S = nullptr;
+    double2* Yrow = nullptr;
+    void* A = nullptr;
+    void* U = nullptr;
+    double* res = nullptr;
+    double* jmax = nullptr;
+    int* status = nullptr;
+    int* iters = nullptr;
+    int* tile_active = nullptr;
+    int* n_active = nullptr;
+    int* clamped = nullptr;
+    int* h_active = nullptr;  // pinned
+    std::vector<hipEvent_t> ev;
+    ~g2g_batch_s()
+    {
+        dfree(V); dfree(S); dfree(Yrow); dfree(A); dfree(U); dfree(res); dfree(jmax);
+        dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped);
+        if (h_active) (void)hipHostFree(h_active);
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+    size_t real_size() const { return precision == G2G_MIXED ? sizeof(float) : sizeof(double); }
+};
+
+namespace {
+
+template <typename Real>
+size_t factor_lds(const SymbolicLU& s)
+{
+    return static_cast<size_t>(s.slots_f + kMaxC) * kLanes * sizeof(Real);
+}
+
+template <typename Real>
+size_t backward_lds(const SymbolicLU& s)
+{
+    return static_cast<size_t>(std::max(s.slots_b, 1)) * kLanes * sizeof(Real);
+}
+
+size_t lds_limit()
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    int v = 0;
+    HIPCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    return static_cast<size_t>(v);
+}
+
+template <typename Real>
+void launch_factor(const g2g_plan_s* P, const Real* A, Real* U, const int* tile_active, const double* jmax,
+                   int* clamped, int rows, int T, double eps, hipStream_t st)
+{
+    const SymbolicLU& s = P->lu();
+    const size_t lds = factor_lds<Real>(s);
+    if (lds > lds_limit()) throw std::runtime_error("factorization needs " + std::to_string(lds) + " bytes of LDS");
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_factor<Real>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    FactorArgs<Real> a{A, U, P->d_prog_f, tile_active, jmax, clamped, s.d, s.a_len, s.u_len, s.slots_f, rows, eps};
+    k_factor<Real><<<T, kLanes, lds, st>>>(a);
+    HIPCHK(hipGetLastError());
+}
+
+template <typename Real, bool Update>
+void launch_backward(const g2g_plan_s* P, const Real* U, const int* tile_active, const int* status, double2* V,
+                     Real* X, int n, int rows, int T, hipStream_t st)
+{
+    const SymbolicLU& s = P->lu();
+    const size_t lds = backward_lds<Real>(s);
+    if (lds > lds_limit()) throw std::runtime_error("backward substitution needs too much LDS");
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_backward<Real, Update>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    BackwardArgs<Real> a{U, P->d_prog_b, tile_active, status, V, X, P->d_perm, s.d, s.u_len, n, rows};
+    k_backward<Real, Update><<<T, kLanes, lds, st>>>(a);
+    HIPCHK(hipGetLastError());
+}
+
+template <typename Real>
+void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double tol, int freeze)
+{
+    const NrPlan& P = b->plan->nr;
+    MismatchArgs<Real> a{b->plan->d_k1_bus, b->plan->d_k1_ent, P.k1_nbus, b->plan->d_Y, b->Yrow, b->V, b->S,
+                         static_cast<Real*>(b->A), P.n, P.nY, P.lu.a_len, b->res, b->jmax, b->status, b->iters,
+                         b->tile_active, b->n_active, b->active_rows, iter, max_iter, tol, freeze};
+    const dim3 grid(b->T), block(kLanes * kMisWaves);
+    if (b->per_row_y) {
+        if (with_j) k_mismatch<Real, true, true><<<grid, block, 0, b->stream>>>(a);
+        else k_mismatch<Real, true, false><<<grid, block, 0, b->stream>>>(a);
+    } else {
+        if (with_j) k_mismatch<Real, false, true><<<grid, block, 0, b->stream>>>(a);
+        else k_mismatch<Real, false, false><<<grid, block, 0, b->stream>>>(a);
+    }
+    HIPCHK(hipGetLastError());
+}
+
+void reset_rows(g2g_batch_s* b)
+{
+    // status: active for [0, active_rows), not simulated beyond
+    std::vector<int> st(static_cast<size_t>(b->rows_pad), kNotSimulated);
+    std::fill(st.begin(), st.begin() + b->active_rows, kActive);
+    HIPCHK(hipMemcpyAsync(b->status, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    k_fill_int<<<blocks_for(b->rows_pad, 256), 256, 0, b->stream>>>(b->iters, 0, b->rows_pad);
+    k_count_tiles<<<b->T, kLanes, 0, b->stream>>>(b->status, b->rows_pad, b->tile_active);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(b->stream));  // st is a host temporary
+}
+
+template <typename Real>
+void solve_impl(g2g_batch_s* b, int max_iter, double tol, int sched, g2g_solve_stats* stats)
+{
+    const int freeze = sched == G2G_SCHED_CONTINUOUS ? 1 : 0;
+    reset_rows(b);
+    const int need = 3 * (max_iter + 1) + 2;
+    while (static_cast<int>(b->ev.size()) < 2 * need) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        b->ev.push_back(e);
+    }
+    int nev = 0;
+    auto mark = [&]() {
+        HIPCHK(hipEventRecord(b->ev[nev], b->stream));
+        return nev++;
+    };
+    std::vector<std::pair<int, int>> ph[3];
+    const int e_begin = mark();
+    const double eps = b->precision == G2G_MIXED ? 1e-5 : 1e-12;  // own_lu/batched_lu.cuh:50-56
+    int it = 0;
+    for (;; ++it) {
+        k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
+        int e0 = mark();
+        launch_mismatch<Real>(b, true, it, max_iter, tol, freeze);
+        ph[0].push_back({e0, mark()});
+        if (it >= max_iter) break;
+        HIPCHK(hipMemcpyAsync(b->h_active, b->n_active, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        if (*b->h_active == 0) break;
+        e0 = mark();
+        launch_factor<Real>(b->plan, static_cast<const Real*>(b->A), static_cast<Real*>(b->U), b->tile_active,
+                            b->jmax, b->clamped, b->active_rows, b->T, eps, b->stream);
+        ph[1].push_back({e0, mark()});
+        e0 = mark();
+        launch_backward<Real, true>(b->plan, static_cast<const Real*>(b->U), b->tile_active, b->status, b->V,
+                                    nullptr, b->plan->nr.n, b->active_rows, b->T, b->stream);
+        ph[2].push_back({e0, mark()});
+    }
+    const int e_end = mark();
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->iterations = it;
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, b->ev[e_begin], b->ev[e_end]));
+        stats->ms_total = ms;
+        double* dst[3] = {&stats->ms_mismatch, &stats->ms_factor, &stats->ms_backward};
+        for (int k = 0; k < 3; ++k)
+            for (auto& pr : ph[k]) {
+                HIPCHK(hipEventElapsedTime(&ms, b->ev[pr.first], b->ev[pr.second]));
+                *dst[k] += ms;
+            }
+        const SymbolicLU& s = b->plan->lu();
+        stats->bytes_factor = static_cast<int64_t>(s.a_len + s.u_len) * static_cast<int64_t>(b->real_size()) *
+                              static_cast<int64_t>(b->active_rows);
+        std::vector<int> st(static_cast<size_t>(b->active_rows));
+        if (!st.empty())
+            HIPCHK(hipMemcpy(st.data(), b->status, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (int v : st) {
+            stats->n_converged += v == kConverged;
+            stats->n_maxiter += v == kMaxIter;
+            stats->n_diverged += v == kDiverged;
+        }
+    }
+}
+
+// rows x m complex (row stride ld complex; ld = 0 broadcasts) -> tiled [T][m][64]
+void to_tiles(const double* src, long long ld, int rows, int m, int row0, int on_device, double2* dst,
+              hipStream_t st)
+{
+    const double2* dsrc = reinterpret_cast<const double2*>(src);
+    double2* tmp = nullptr;
+    if (!on_device) {
+        const size_t cnt = ld == 0 ? static_cast<size_t>(m) : static_cast<size_t>(rows - 1) * ld + m;
+        tmp = dalloc<double2>(cnt);
+        HIPCHK(hipMemcpyAsync(tmp, src, cnt * sizeof(double2), hipMemcpyHostToDevice, st));
+        dsrc = tmp;
+    }
+    const long long total = static_cast<long long>(rows) * m;
+    k_rows_to_tiles<double2><<<blocks_for(total, 256), 256, 0, st>>>(dsrc, ld, rows, m, row0, dst, ld == 0 ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    if (tmp) {
+        HIPCHK(hipStreamSynchronize(st));
+        dfree(tmp);
+    }
+}
+
+void check_rows(const g2g_batch_s* b, int rows, int row0)
+{
+    if (rows < 0 || row0 < 0 || row0 + rows > b->rows) throw std::invalid_argument("row range outside the batch");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* g2g_last_error(void) { return g_err.c_str(); }
+
+int g2g_version(void) { return 100; }
+
+int g2g_plan_create(int32_t n, const int32_t* Yp, const int32_t* Yi, const double* Yx, int32_t npv,
+                    const int32_t* pv, int32_t npq, const int32_t* pq, int32_t slack, g2g_plan_t* out)
+{
+    return guarded([&] {
+        if (!out || !Yp || !Yi || !Yx || (npv && !pv) || (npq && !pq)) throw std::invalid_argument("null argument");
+        auto p = std::make_unique<g2g_plan_s>();
+        p->grid = true;
+        p->nr = build_nr_plan(n, Yp, Yi, Yx, npv, pv, npq, pq, slack);
+        p->nnz = p->nr.nJ;
+        *out = p.release();
+    });
+}
+
+int g2g_lu_plan_create(int32_t d, const int32_t* Ap, const int32_t* Ai, const int32_t* order, g2g_plan_t* out)
+{
+    return guarded([&] {
+        if (!out || !Ap || !Ai) throw std::invalid_argument("null argument");
+        auto p = std::make_unique<g2g_plan_s>();
+        p->grid = false;
+        std::vector<int> ord;
+        if (order) ord.assign(order, order + d);
+        p->raw = build_symbolic(d, Ap, Ai, ord);
+        p->nnz = Ap[d];
+        *out = p.release();
+    });
+}
+
+void g2g_plan_destroy(g2g_plan_t plan) { delete plan; }
+
+int g2g_plan_get_info(g2g_plan_t P, g2g_plan_info* info)
+{
+    return guarded([&] {
+        if (!P || !info) throw std::invalid_argument("null argument");
+        const SymbolicLU& s = P->lu();
+        std::memset(info, 0, sizeof(*info));
+        info->d = s.d;
+        info->nnz = P->nnz;
+        info->slack = -1;
+        if (P->grid) {
+            info->n = P->nr.n;
+            info->nY = P->nr.nY;
+            info->nJ = P->nr.nJ;
+            info->npvpq = P->nr.npvpq;
+            info->npq = P->nr.npq;
+            info->slack = P->nr.slack;
+        }
+        info->nLU = s.nLU;
+        info->n_updates = s.n_updates;
+        info->max_c = s.max_c;
+        info->height = s.height;
+        info->slots_f = s.slots_f;
+        info->slots_b = s.slots_b;
+        info->a_len = s.a_len;
+        info->u_len = s.u_len;
+    });
+}
+
+int g2g_plan_get_array(g2g_plan_t P, int32_t which, int32_t* out, int64_t capacity, int64_t* length)
+{
+    return guarded([&] {
+        if (!P) throw std::invalid_argument("null plan");
+        const SymbolicLU& s = P->lu();
+        std::vector<int> tmp;
+        const std::vector<int>* v = nullptr;
+        switch (which) {
+        case G2G_ARR_JP: if (!P->grid) throw std::invalid_argument("not a grid plan"); v = &P->nr.Jp; break;
+        case G2G_ARR_JI: if (!P->grid) throw std::invalid_argument("not a grid plan"); v = &P->nr.Ji; break;
+        case G2G_ARR_MAPS:
+            if (!P->grid) throw std::invalid_argument("not a grid plan");
+            for (int b = 0; b < 4; ++b) tmp.insert(tmp.end(), P->nr.map[b].begin(), P->nr.map[b].end());
+            v = &tmp;
+            break;
+        case G2G_ARR_PVPQ: if (!P->grid) throw std::invalid_argument("not a grid plan"); v = &P->nr.pvpq; break;
+        case G2G_ARR_PERM: v = &s.perm; break;
+        case G2G_ARR_COLP: v = &s.colp; break;
+        case G2G_ARR_COLI: v = &s.coli; break;
+        case G2G_ARR_PROG_F: v = &s.prog_f; break;
+        case G2G_ARR_PROG_B: v = &s.prog_b; break;
+        case G2G_ARR_APOS: v = &s.a_pos_of_entry; break;
+        case G2G_ARR_FPOS: v = &s.f_pos; break;
+        case G2G_ARR_UOFF: v = &s.u_off; break;
+        default: throw std::invalid_argument("unknown array id");
+        }
+        if (length) *length = static_cast<int64_t>(v->size());
+        if (out) {
+            if (capacity < static_cast<int64_t>(v->size())) throw std::invalid_argument("output buffer too small");
+            std::copy(v->begin(), v->end(), out);
+        }
+    });
+}
+
+int g2g_batch_create(g2g_plan_t P, int32_t rows, int32_t precision, int32_t per_row_ybus, void* stream,
+                     g2g_batch_t* out)
+{
+    return guarded([&] {
+        if (!P || !out) throw std::invalid_argument("null argument");
+        if (!P->grid) throw std::invalid_argument("a batch needs a grid plan");
+        if (rows <= 0) throw std::invalid_argument("rows must be positive");
+        if (precision != G2G_FP64 && precision != G2G_MIXED) throw std::invalid_argument("unknown precision");
+        P->upload();
+        auto b = std::make_unique<g2g_batch_s>();
+        b->plan = P;
+        b->rows = b->active_rows = rows;
+        b->T = (rows + kLanes - 1) / kLanes;
+        b->rows_pad = b->T * kLanes;
+        b->precision = precision;
+        b->per_row_y = per_row_ybus ? 1 : 0;
+        b->stream = static_cast<hipStream_t>(stream);
+        const NrPlan& nr = P->nr;
+        const size_t rp = static_cast<size_t>(b->rows_pad);
+        b->V = dalloc<double2>(rp * nr.n);
+        b->S = dalloc<double2>(rp * nr.n);
+        HIPCHK(hipMemsetAsync(b->S, 0, rp * nr.n * sizeof(double2), b->stream));
+        // padding lanes: |V| = 1 keeps every kernel finite on them
+        {
+            std::vector<double2> ones(static_cast<size_t>(nr.n), make_double2(1.0, 0.0));
+            double2* d1 = dupload(ones);
+            k_rows_to_tiles<double2><<<blocks_for(static_cast<long long>(b->rows_pad) * nr.n, 256), 256, 0,
+                                       b->stream>>>(d1, 0, b->rows_pad, nr.n, 0, b->V, 1);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(b->stream));
+            dfree(d1);
+        }
+        if (b->per_row_y) {
+            b->Yrow = dalloc<double2>(rp * nr.nY);
+            k_ybus_broadcast<<<blocks_for(static_cast<long long>(rp) * nr.nY, 256), 256, 0, b->stream>>>(
+                P->d_Y, nr.nY, b->rows_pad, b->Yrow);
+            HIPCHK(hipGetLastError());
+        }
+        b->A = dalloc<unsigned char>(rp * nr.lu.a_len * b->real_size());
+        b->U = dalloc<unsigned char>(rp * nr.lu.u_len * b->real_size());
+        b->res = dalloc<double>(rp);
+        b->jmax = dalloc<double>(rp);
+        b->status = dalloc<int>(rp);
+        b->iters = dalloc<int>(rp);
+        b->clamped = dalloc<int>(rp);
+        b->tile_active = dalloc<int>(b->T);
+        b->n_active = dalloc<int>(1);
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->h_active), sizeof(int), hipHostMallocDefault));
+        k_fill_int<<<blocks_for(rp, 256), 256, 0, b->stream>>>(b->status, kNotSimulated, rp);
+        k_fill_int<<<blocks_for(rp, 256), 256, 0, b->stream>>>(b->iters, 0, rp);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(b->stream));
+        *out = b.release();
+    });
+}
+
+void g2g_batch_destroy(g2g_batch_t b) { delete b; }
+
+int g2g_batch_set_v(g2g_batch_t b, const double* V, int64_t ld, int32_t rows, int32_t row0, int32_t on_device)
+{
+    return guarded([&] {
+        if (!b || !V) throw std::invalid_argument("null argument");
+        check_rows(b, rows, row0);
+        to_tiles(V, ld, rows, b->plan->nr.n, row0, on_device, b->V, b->stream);
+    });
+}
+
+int g2g_batch_set_sbus(g2g_batch_t b, const double* S, int64_t ld, int32_t rows, int32_t row0, int32_t on_device)
+{
+    return guarded([&] {
+        if (!b || !S) throw std::invalid_argument("null argument");
+        check_rows(b, rows, row0);
+        to_tiles(S, ld, rows, b->plan->nr.n, row0, on_device, b->S, b->stream);
+    });
+}
+
+int g2g_batch_set_ybus_edits(g2g_batch_t b, const int32_t* row_ptr, const int32_t* entry, const double* delta,
+                             int32_t rows)
+{
+    return guarded([&] {
+        if (!b || !row_ptr) throw std::invalid_argument("null argument");
+        if (!b->per_row_y) throw std::invalid_argument("the batch was created without per-row Ybus values");
+        check_rows(b, rows, 0);
+        const NrPlan& nr = b->plan->nr;
+        const int ne = row_ptr[rows];
+        for (int r = 0; r < rows; ++r)
+            if (row_ptr[r + 1] < row_ptr[r]) throw std::invalid_argument("row_ptr is not monotone");
+        for (int q = 0; q < ne; ++q)
+            if (entry[q] < 0 || entry[q] >= nr.nY) throw std::invalid_argument("edit entry out of range");
+        k_ybus_broadcast<<<blocks_for(static_cast<long long>(b->rows_pad) * nr.nY, 256), 256, 0, b->stream>>>(
+            b->plan->d_Y, nr.nY, b->rows_pad, b->Yrow);
+        if (ne > 0) {
+            std::vector<int> rp(row_ptr, row_ptr + rows + 1), en(entry, entry + ne);
+            std::vector<double2> de(static_cast<size_t>(ne));
+            for (int q = 0; q < ne; ++q) de[q] = make_double2(delta[2 * q], delta[2 * q + 1]);
+            int* d_rp = dupload(rp);
+            int* d_en = dupload(en);
+            double2* d_de = dupload(de);
+            k_ybus_edits<<<blocks_for(rows, 256), 256, 0, b->stream>>>(d_rp, d_en, d_de, rows, nr.nY, b->Yrow);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(b->stream));
+            dfree(d_rp);
+            dfree(d_en);
+            dfree(d_de);
+        }
+    });
+}
+
+int g2g_batch_set_active_rows(g2g_batch_t b, int32_t active_rows)
+{
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        if (active_rows < 0 || active_rows > b->rows) throw std::invalid_argument("active_rows outside the batch");
+        b->active_rows = active_rows;
+    });
+}
+
+int g2g_batch_solve(g2g_batch_t b, int32_t max_iter, double tol, int32_t scheduling, g2g_solve_stats* stats)
+{
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        if (max_iter < 0) throw std::invalid_argument("max_iter must be >= 0");
+        if (!(tol > 0.0)) throw std::invalid_argument("tol must be positive");
+        if (scheduling != G2G_SCHED_CHUNKED && scheduling != G2G_SCHED_CONTINUOUS)
+            throw std::invalid_argument("unknown scheduling");
+        if (b->precision == G2G_MIXED) solve_impl<float>(b, max_iter, tol, scheduling, stats);
+        else solve_impl<double>(b, max_iter, tol, scheduling, stats);
+    });
+}
+
+int g2g_batch_get_v(g2g_batch_t b, double* V, int64_t ld, int32_t on_device)
+{
+    return guarded([&] {
+        if (!b || !V) throw std::invalid_argument("null argument");
+        const int n = b->plan->nr.n;
+        if (ld < n) throw std::invalid_argument("ld smaller than n");
+        const long long total = static_cast<long long>(b->rows) * n;
+        if (on_device) {
+            k_tiles_to_rows<double2><<<blocks_for(total, 256), 256, 0, b->stream>>>(
+                b->V, b->rows, n, reinterpret_cast<double2*>(V), ld);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(b->stream));
+        } else {
+            double2* tmp = dalloc<double2>(static_cast<size_t>(b->rows) * ld);
+            k_tiles_to_rows<double2><<<blocks_for(total, 256), 256, 0, b->stream>>>(b->V, b->rows, n, tmp, ld);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(V, tmp, static_cast<size_t>(b->rows) * ld * sizeof(double2),
+                                  hipMemcpyDeviceToHost, b->stream));
+            HIPCHK(hipStreamSynchronize(b->stream));
+            dfree(tmp);
+        }
+    });
+}
+
+int g2g_batch_get_results(g2g_batch_t b, int32_t* status, int32_t* iters, double* residual)
+{
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        HIPCHK(hipStreamSynchronize(b->stream));
+        const size_t r = static_cast<size_t>(b->rows);
+        if (status) HIPCHK(hipMemcpy(status, b->status, r * sizeof(int), hipMemcpyDeviceToHost));
+        if (iters) HIPCHK(hipMemcpy(iters, b->iters, r * sizeof(int), hipMemcpyDeviceToHost));
+        if (residual) HIPCHK(hipMemcpy(residual, b->res, r * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+int g2g_batch_eval(g2g_batch_t b, double* F, double* J, double* residual)
+{
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        const NrPlan& nr = b->plan->nr;
+        const size_t rp = static_cast<size_t>(b->rows_pad);
+        // evaluate every row as active without touching the solve's statuses
+        int* saved = dalloc<int>(rp);
+        int* saved_it = dalloc<int>(rp);
+        HIPCHK(hipMemcpyAsync(saved, b->status, rp * sizeof(int), hipMemcpyDeviceToDevice, b->stream));
+        HIPCHK(hipMemcpyAsync(saved_it, b->iters, rp * sizeof(int), hipMemcpyDeviceToDevice, b->stream));
+        reset_rows(b);
+        k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
+        if (b->precision == G2G_MIXED) launch_mismatch<float>(b, J != nullptr, 0, 1 << 30, 1e-300, 0);
+        else launch_mismatch<double>(b, J != nullptr, 0, 1 << 30, 1e-300, 0);
+        const size_t rs = b->real_size();
+        std::vector<unsigned char> host(rp * nr.lu.a_len * rs);
+        HIPCHK(hipMemcpyAsync(host.data(), b->A, host.size(), hipMemcpyDeviceToHost, b->stream));
+        if (residual) HIPCHK(hipMemcpyAsync(residual, b->res, b->rows * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipMemcpyAsync(b->status, saved, rp * sizeof(int), hipMemcpyDeviceToDevice, b->stream));
+        HIPCHK(hipMemcpyAsync(b->iters, saved_it, rp * sizeof(int), hipMemcpyDeviceToDevice, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        dfree(saved);
+        dfree(saved_it);
+        auto at = [&](int r, int pos) -> double {
+            const size_t i = ((static_cast<size_t>(r / kLanes) * nr.lu.a_len) + pos) * kLanes + (r % kLanes);
+            if (rs == sizeof(float)) return reinterpret_cast<const float*>(host.data())[i];
+            return reinterpret_cast<const double*>(host.data())[i];
+        };
+        for (int r = 0; r < b->rows; ++r) {
+            if (F)
+                for (int e = 0; e < nr.d; ++e) F[static_cast<size_t>(r) * nr.d + e] = at(r, nr.lu.f_pos[e]);
+            if (J)
+                for (int q = 0; q < nr.nJ; ++q) J[static_cast<size_t>(r) * nr.nJ + q] = at(r, nr.lu.a_pos_of_entry[q]);
+        }
+    });
+}
+
+int g2g_lu_solve_batch(g2g_plan_t P, int32_t rows, int32_t precision, const double* vals, const double* rhs,
+                       double* x, double pivot_eps, int32_t* clamped, void* stream)
+{
+    return guarded([&] {
+        if (!P || !vals || !rhs || !x) throw std::invalid_argument("null argument");
+        if (rows <= 0) throw std::invalid_argument("rows must be positive");
+        if (precision != G2G_FP64 && precision != G2G_MIXED) throw std::invalid_argument("unknown precision");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        P->upload();
+        const SymbolicLU& s = P->lu();
+        const int T = (rows + kLanes - 1) / kLanes;
+        const size_t rp = static_cast<size_t>(T) * kLanes;
+        const int nnz = P->nnz, d = s.d;
+        const size_t rs = precision == G2G_MIXED ? sizeof(float) : sizeof(double);
+        double* d_vals = dalloc<double>(static_cast<size_t>(rows) * nnz);
+        double* d_rhs = dalloc<double>(static_cast<size_t>(rows) * d);
+        double* d_x = dalloc<double>(static_cast<size_t>(rows) * d);
+        void* A = dalloc<unsigned char>(rp * s.a_len * rs);
+        void* U = dalloc<unsigned char>(rp * s.u_len * rs);
+        void* X = dalloc<unsigned char>(rp * d * rs);
+        double* jm = dalloc<double>(rp);
+        int* ta = dalloc<int>(T);
+        int* cl = dalloc<int>(rp);
+        try {
+            HIPCHK(hipMemcpyAsync(d_vals, vals, static_cast<size_t>(rows) * nnz * sizeof(double), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(d_rhs, rhs, static_cast<size_t>(rows) * d * sizeof(double), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemsetAsync(A, 0, rp * s.a_len * rs, st));
+            HIPCHK(hipMemsetAsync(jm, 0, rp * sizeof(double), st));
+            k_fill_int<<<blocks_for(T, 256), 256, 0, st>>>(ta, kLanes, T);
+            const long long total = static_cast<long long>(rows) * (nnz + d);
+            k_row_absmax<<<rows, 256, 0, st>>>(d_vals, nnz, rows, nnz, jm);
+            if (precision == G2G_MIXED) {
+                k_gather_astream<float><<<blocks_for(total, 256), 256, 0, st>>>(
+                    d_vals, nnz, d_rhs, d, rows, nnz, d, P->d_apos, P->d_fpos, s.a_len, static_cast<float*>(A));
+                launch_factor<float>(P, static_cast<const float*>(A), static_cast<float*>(U), ta, jm, cl, rows, T,
+                                     pivot_eps, st);
+                launch_backward<float, false>(P, static_cast<const float*>(U), ta, nullptr, nullptr,
+                                              static_cast<float*>(X), 0, rows, T, st);
+                // widen the solution
+                std::vector<float> hx(rp * d);
+                HIPCHK(hipMemcpyAsync(hx.data(), X, hx.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                for (int r = 0; r < rows; ++r)
+                    for (int e = 0; e < d; ++e)
+                        x[static_cast<size_t>(r) * d + e] =
+                            hx[((static_cast<size_t>(r / kLanes) * d) + e) * kLanes + (r % kLanes)];
+            } else {
+                k_gather_astream<double><<<blocks_for(total, 256), 256, 0, st>>>(
+                    d_vals, nnz, d_rhs, d, rows, nnz, d, P->d_apos, P->d_fpos, s.a_len, static_cast<double*>(A));
+                launch_factor<double>(P, static_cast<const double*>(A), static_cast<double*>(U), ta, jm, cl, rows, T,
+                                      pivot_eps, st);
+                launch_backward<double, false>(P, static_cast<const double*>(U), ta, nullptr, nullptr,
+                                               static_cast<double*>(X), 0, rows, T, st);
+                k_tiles_to_rows<double><<<blocks_for(static_cast<long long>(rows) * d, 256), 256, 0, st>>>(
+                    static_cast<const double*>(X), rows, d, d_x, d);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(x, d_x, static_cast<size_t>(rows) * d * sizeof(double), hipMemcpyDeviceToHost, st));
+            }
+            if (clamped) HIPCHK(hipMemcpyAsync(clamped, cl, rows * sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+        } catch (...) {
+            (void)hipStreamSynchronize(st);
+            dfree(d_vals); dfree(d_rhs); dfree(d_x); dfree(A); dfree(U); dfree(X); dfree(jm); dfree(ta); dfree(cl);
+            throw;
+        }
+        dfree(d_vals); dfree(d_rhs); dfree(d_x); dfree(A); dfree(U); dfree(X); dfree(jm); dfree(ta); dfree(cl);
+    });
+}
+
+int g2g_batch_device_arrays(g2g_batch_t b, void** V_tiles, void** S_tiles)
+{
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        if (V_tiles) *V_tiles = b->V;
+        if (S_tiles) *S_tiles = b->S;
+    });
+}
+
+}  // extern "C"

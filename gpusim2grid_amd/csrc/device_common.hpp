@@ -1,0 +1,57 @@
+// Kernel argument bundles and constants (device side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "program_format.hpp"
+
+namespace g2g {
+
+template <typename Real>
+struct MismatchArgs {
+    const int* k1_bus;      // 6 ints per bus (mismatch program)
+    const int* k1_ent;      // 6 ints per Ybus entry
+    int nbus;
+    const double2* Y;       // shared Ybus values [nY]
+    const double2* Yrow;    // per-row Ybus values [T][nY][64] (outage batches)
+    const double2* V;       // [T][n][64]
+    const double2* S;       // [T][n][64]
+    Real* A;                // A stream [T][a_len][64]
+    int n, nY, a_len;
+    double* res;            // [rows_pad] ||F||inf
+    double* jmax;           // [rows_pad] max |J| (pivot clamp scale)
+    int* status;            // [rows_pad]
+    int* iters;             // [rows_pad]
+    int* tile_active;       // [T]
+    int* n_active;          // rows still iterating (accumulated)
+    int rows;
+    int iter, max_iter;
+    double tol;
+    int freeze;             // 1: a converged row stops (continuous); 0: fixed count (chunked)
+};
+
+template <typename Real>
+struct FactorArgs {
+    const Real* A;
+    Real* U;                // U stream [T][u_len][64]
+    const int* prog;
+    const int* tile_active;
+    const double* jmax;     // nullable: scale 1
+    int* clamped;           // nullable: clamped pivots per row
+    int d, a_len, u_len, slots, rows;
+    double pivot_eps;
+};
+
+template <typename Real>
+struct BackwardArgs {
+    const Real* U;
+    const int* prog;
+    const int* tile_active;
+    const int* status;      // NR mode: only active rows update V
+    double2* V;             // NR mode
+    Real* X;                // raw mode: solution [T][d][64] in input labels
+    const int* perm;        // raw mode: pivot -> input index
+    int d, u_len, n, rows;
+};
+
+}  // namespace g2g

@@ -1,0 +1,602 @@
+// Host-side symbolic layer: see plan.hpp.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <numeric>
+#include <set>
+#include <unordered_map>
+#include <utility>
+
+namespace g2g {
+
+namespace {
+
+// Symmetric adjacency (no self loops, sorted) of a square CSR pattern.
+void symmetric_adjacency(int d, const int* Ap, const int* Ai, std::vector<int>& adjp,
+                         std::vector<int>& adji)
+{
+    std::vector<std::vector<int>> nb(static_cast<size_t>(d));
+    for (int r = 0; r < d; ++r)
+        for (int p = Ap[r]; p < Ap[r + 1]; ++p) {
+            const int c = Ai[p];
+            if (c < 0 || c >= d) throw std::invalid_argument("CSR column index out of range");
+            if (c == r) continue;
+            nb[r].push_back(c);
+            nb[c].push_back(r);
+        }
+    adjp.assign(static_cast<size_t>(d) + 1, 0);
+    adji.clear();
+    for (int r = 0; r < d; ++r) {
+        auto& v = nb[r];
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        adji.insert(adji.end(), v.begin(), v.end());
+        adjp[r + 1] = static_cast<int>(adji.size());
+    }
+}
+
+// Column structures of L for a symmetric pattern given as adjacency in pivot
+// labels (etree-based symbolic Cholesky).  Returns parent[].
+std::vector<int> symbolic_structs(int d, const std::vector<int>& adjp, const std::vector<int>& adji,
+                                  std::vector<int>& colp, std::vector<int>& coli)
+{
+    std::vector<int> parent(static_cast<size_t>(d), -1);
+    std::vector<std::vector<int>> children(static_cast<size_t>(d));
+    std::vector<int> mark(static_cast<size_t>(d), -1);
+    colp.assign(static_cast<size_t>(d) + 1, 0);
+    coli.clear();
+    std::vector<int> tmp;
+    for (int k = 0; k < d; ++k) {
+        tmp.clear();
+        mark[k] = k;
+        for (int p = adjp[k]; p < adjp[k + 1]; ++p) {
+            const int i = adji[p];
+            if (i > k && mark[i] != k) { mark[i] = k; tmp.push_back(i); }
+        }
+        for (int c : children[k])
+            for (int p = colp[c]; p < colp[c + 1]; ++p) {
+                const int i = coli[p];
+                if (i > k && mark[i] != k) { mark[i] = k; tmp.push_back(i); }
+            }
+        std::sort(tmp.begin(), tmp.end());
+        coli.insert(coli.end(), tmp.begin(), tmp.end());
+        colp[k + 1] = static_cast<int>(coli.size());
+        if (!tmp.empty()) {
+            parent[k] = tmp.front();
+            children[tmp.front()].push_back(k);
+        }
+    }
+    return parent;
+}
+
+}  // namespace
+
+std::vector<int> min_degree_order(int n, const std::vector<int>& adjp, const std::vector<int>& adji,
+                                  const std::vector<int>& weight)
+{
+    std::vector<std::vector<int>> nb(static_cast<size_t>(n));
+    for (int v = 0; v < n; ++v) nb[v].assign(adji.begin() + adjp[v], adji.begin() + adjp[v + 1]);
+    std::vector<long long> deg(static_cast<size_t>(n), 0);
+    auto degree = [&](int v) {
+        long long s = 0;
+        for (int u : nb[v]) s += weight[u];
+        return s;
+    };
+    std::set<std::pair<long long, int>> pq;
+    for (int v = 0; v < n; ++v) {
+        deg[v] = degree(v);
+        pq.insert({deg[v], v});
+    }
+    std::vector<char> done(static_cast<size_t>(n), 0);
+    std::vector<int> order;
+    order.reserve(static_cast<size_t>(n));
+    std::vector<int> merged;
+    while (!pq.empty()) {
+        const int v = pq.begin()->second;
+        pq.erase(pq.begin());
+        done[v] = 1;
+        order.push_back(v);
+        const std::vector<int> nv = nb[v];
+        for (int u : nv) {
+            // N(u) <- N(u) U N(v) \ {u, v}
+            merged.clear();
+            std::set_union(nb[u].begin(), nb[u].end(), nv.begin(), nv.end(), std::back_inserter(merged));
+            nb[u].clear();
+            for (int w : merged)
+                if (w != u && w != v && !done[w]) nb[u].push_back(w);
+            pq.erase({deg[u], u});
+            deg[u] = degree(u);
+            pq.insert({deg[u], u});
+        }
+        nb[v].clear();
+    }
+    return order;
+}
+
+std::vector<int> rcm_order(int n, const std::vector<int>& adjp, const std::vector<int>& adji)
+{
+    std::vector<int> order;
+    order.reserve(static_cast<size_t>(n));
+    std::vector<char> seen(static_cast<size_t>(n), 0);
+    auto deg = [&](int v) { return adjp[v + 1] - adjp[v]; };
+    auto bfs_last = [&](int s, std::vector<int>& lvl) {
+        lvl.assign(static_cast<size_t>(n), -1);
+        std::vector<int> q{s};
+        lvl[s] = 0;
+        int last = s;
+        for (size_t h = 0; h < q.size(); ++h) {
+            const int v = q[h];
+            last = v;
+            for (int p = adjp[v]; p < adjp[v + 1]; ++p)
+                if (lvl[adji[p]] < 0) { lvl[adji[p]] = lvl[v] + 1; q.push_back(adji[p]); }
+        }
+        // among the deepest level, the smallest degree
+        int best = last;
+        for (int v : q)
+            if (lvl[v] == lvl[last] && deg(v) < deg(best)) best = v;
+        return best;
+    };
+    std::vector<int> lvl;
+    for (int s0 = 0; s0 < n; ++s0) {
+        if (seen[s0]) continue;
+        // pseudo-peripheral start of this component
+        int s = bfs_last(s0, lvl);
+        s = bfs_last(s, lvl);
+        std::vector<int> q{s};
+        seen[s] = 1;
+        std::vector<int> nbr;
+        for (size_t h = 0; h < q.size(); ++h) {
+            const int v = q[h];
+            nbr.clear();
+            for (int p = adjp[v]; p < adjp[v + 1]; ++p)
+                if (!seen[adji[p]]) { seen[adji[p]] = 1; nbr.push_back(adji[p]); }
+            std::sort(nbr.begin(), nbr.end(), [&](int a, int b) { return deg(a) != deg(b) ? deg(a) < deg(b) : a < b; });
+            q.insert(q.end(), nbr.begin(), nbr.end());
+        }
+        order.insert(order.end(), q.begin(), q.end());
+    }
+    std::reverse(order.begin(), order.end());
+    return order;
+}
+
+namespace {
+
+// Static slot allocator: lowest free slot first.
+struct SlotAlloc {
+    std::set<int> free_;
+    int hi = 0;
+    int get()
+    {
+        if (!free_.empty()) { int s = *free_.begin(); free_.erase(free_.begin()); return s; }
+        return hi++;
+    }
+    void put(int s) { free_.insert(s); }
+};
+
+inline long long key(int i, int j, int d) { return static_cast<long long>(i) * d + j; }
+
+void build_backward(SymbolicLU& s, const std::vector<int>& bus_of, const std::vector<int>& kind_of,
+                    const std::vector<int>& partner);
+
+}  // namespace
+
+SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> order)
+{
+    if (d <= 0) throw std::invalid_argument("matrix dimension must be positive");
+    SymbolicLU s;
+    s.d = d;
+    std::vector<int> adjp, adji;
+    symmetric_adjacency(d, Ap, Ai, adjp, adji);
+    if (order.empty()) order = min_degree_order(d, adjp, adji, std::vector<int>(static_cast<size_t>(d), 1));
+    if (static_cast<int>(order.size()) != d) throw std::invalid_argument("order has the wrong length");
+    {
+        std::vector<char> seen(static_cast<size_t>(d), 0);
+        for (int v : order) {
+            if (v < 0 || v >= d || seen[v]) throw std::invalid_argument("order is not a permutation");
+            seen[v] = 1;
+        }
+    }
+    // adjacency in the labels of `order`
+    auto relabel = [&](const std::vector<int>& perm, std::vector<int>& pp, std::vector<int>& pi) {
+        std::vector<int> inv(static_cast<size_t>(d));
+        for (int k = 0; k < d; ++k) inv[perm[k]] = k;
+        pp.assign(static_cast<size_t>(d) + 1, 0);
+        pi.clear();
+        std::vector<int> row;
+        for (int k = 0; k < d; ++k) {
+            const int v = perm[k];
+            row.clear();
+            for (int p = adjp[v]; p < adjp[v + 1]; ++p) row.push_back(inv[adji[p]]);
+            std::sort(row.begin(), row.end());
+            pi.insert(pi.end(), row.begin(), row.end());
+            pp[k + 1] = static_cast<int>(pi.size());
+        }
+    };
+    std::vector<int> pp, pi, colp, coli;
+    relabel(order, pp, pi);
+    std::vector<int> parent = symbolic_structs(d, pp, pi, colp, coli);
+    // postorder of the elimination forest.  Children are visited in
+    // decreasing (peak - out) order (Liu's rule for the smallest working
+    // stack): out_c = c_c^2 + c_c live update/rhs slots a finished subtree
+    // leaves behind, peak_c the most it holds while running.
+    std::vector<std::vector<int>> children(static_cast<size_t>(d));
+    std::vector<int> roots;
+    for (int k = 0; k < d; ++k) (parent[k] >= 0 ? children[parent[k]] : roots).push_back(k);
+    {
+        std::vector<long long> out(static_cast<size_t>(d)), peak(static_cast<size_t>(d));
+        for (int k = 0; k < d; ++k) {  // children have smaller labels than parents
+            const long long ck = colp[k + 1] - colp[k];
+            out[k] = ck * ck + ck;
+            auto& ch = children[k];
+            std::sort(ch.begin(), ch.end(), [&](int a, int b) {
+                const long long x = peak[a] - out[a], y = peak[b] - out[b];
+                return x != y ? x > y : a < b;
+            });
+            long long held = 0, pk = 0;
+            for (int c : ch) {
+                pk = std::max(pk, held + peak[c]);
+                held += out[c];
+            }
+            peak[k] = std::max(pk, held + out[k]);
+        }
+        std::sort(roots.begin(), roots.end(), [&](int a, int b) {
+            const long long x = peak[a] - out[a], y = peak[b] - out[b];
+            return x != y ? x > y : a < b;
+        });
+    }
+    std::vector<int> post;
+    post.reserve(static_cast<size_t>(d));
+    for (int r : roots) {
+        std::vector<std::pair<int, size_t>> st{{r, 0}};
+        while (!st.empty()) {
+            auto& top = st.back();
+            if (top.second < children[top.first].size()) {
+                const int c = children[top.first][top.second++];
+                st.push_back({c, 0});
+            } else {
+                post.push_back(top.first);
+                st.pop_back();
+            }
+        }
+    }
+    s.perm.resize(static_cast<size_t>(d));
+    for (int k = 0; k < d; ++k) s.perm[k] = order[post[k]];
+    s.iperm.resize(static_cast<size_t>(d));
+    for (int k = 0; k < d; ++k) s.iperm[s.perm[k]] = k;
+    relabel(s.perm, pp, pi);
+    parent = symbolic_structs(d, pp, pi, s.colp, s.coli);
+    {
+        std::vector<int> h(static_cast<size_t>(d), 0);
+        for (int k = 0; k < d; ++k)
+            if (parent[k] >= 0) h[parent[k]] = std::max(h[parent[k]], h[k] + 1);
+        s.height = d ? 1 + *std::max_element(h.begin(), h.end()) : 0;
+    }
+    s.nLU = d + 2 * static_cast<int>(s.coli.size());
+
+    // which (row, col) pivot-label entries are present in the input A
+    std::unordered_map<long long, int> a_entry;  // key -> input CSR position
+    a_entry.reserve(static_cast<size_t>(Ap[d]) * 2);
+    for (int r = 0; r < d; ++r)
+        for (int p = Ap[r]; p < Ap[r + 1]; ++p)
+            a_entry[key(s.iperm[r], s.iperm[Ai[p]], d)] = p;
+    s.a_pos_of_entry.assign(static_cast<size_t>(Ap[d]), -1);
+    s.f_pos.assign(static_cast<size_t>(d), -1);
+    s.nA = 0;
+
+    // ---- factorization + forward program -----------------------------------
+    SlotAlloc sa;
+    std::unordered_map<long long, int> slot_of;  // live update targets
+    std::vector<int> bslot(static_cast<size_t>(d), -1);
+    s.prog_f_off.assign(static_cast<size_t>(d) + 1, 0);
+    s.u_off.assign(static_cast<size_t>(d) + 1, 0);
+    int apos = 0, upos = 0;
+    auto take = [&](int i, int j) -> int {  // slot of a consumed entry (freed), or kNoSlot
+        auto it = slot_of.find(key(i, j, d));
+        if (it == slot_of.end()) return kNoSlot;
+        const int sl = it->second;
+        slot_of.erase(it);
+        sa.put(sl);
+        return sl;
+    };
+    auto a_present = [&](int i, int j) -> int {
+        auto it = a_entry.find(key(i, j, d));
+        return it == a_entry.end() ? -1 : it->second;
+    };
+    for (int k = 0; k < d; ++k) {
+        s.prog_f_off[k] = static_cast<int>(s.prog_f.size());
+        s.u_off[k] = upos;
+        const int c0 = s.colp[k], c = s.colp[k + 1] - c0;
+        s.max_c = std::max(s.max_c, c);
+        s.prog_f.push_back(c);
+        // A stream: F(k)
+        s.f_pos[s.perm[k]] = apos++;
+        // b(k)
+        int bref = kNoSlot;
+        if (bslot[k] >= 0) { bref = bslot[k]; sa.put(bref); bslot[k] = -1; }
+        s.prog_f.push_back(bref);
+        // diagonal
+        {
+            const int pa = a_present(k, k);
+            const int sl = take(k, k);
+            int ref = sl;
+            if (pa >= 0) { ref |= kInA; s.a_pos_of_entry[pa] = apos++; ++s.nA; }
+            s.prog_f.push_back(ref);
+        }
+        for (int side = 0; side < 2; ++side)       // 0: L(i,k), 1: U(k,j)
+            for (int p = c0; p < c0 + c; ++p) {
+                const int o = s.coli[p];
+                const int i = side == 0 ? o : k, j = side == 0 ? k : o;
+                const int pa = a_present(i, j);
+                const int sl = take(i, j);
+                int ref = sl;
+                if (pa >= 0) { ref |= kInA; s.a_pos_of_entry[pa] = apos++; ++s.nA; }
+                s.prog_f.push_back(ref);
+            }
+        upos += c + 2;
+        // rank-1 update targets (i, j) for i, j in struct_k
+        for (int pa_ = c0; pa_ < c0 + c; ++pa_)
+            for (int pb = c0; pb < c0 + c; ++pb) {
+                const int i = s.coli[pa_], j = s.coli[pb];
+                const long long kk = key(i, j, d);
+                auto it = slot_of.find(kk);
+                if (it == slot_of.end()) {
+                    const int sl = sa.get();
+                    slot_of[kk] = sl;
+                    s.prog_f.push_back(sl | kFirst);
+                } else {
+                    s.prog_f.push_back(it->second);
+                }
+            }
+        s.n_updates += c * c;
+        // forward substitution targets b(i), i in struct_k
+        for (int p = c0; p < c0 + c; ++p) {
+            const int i = s.coli[p];
+            if (bslot[i] < 0) {
+                bslot[i] = sa.get();
+                s.prog_f.push_back(bslot[i] | kFirst);
+            } else {
+                s.prog_f.push_back(bslot[i]);
+            }
+        }
+    }
+    s.prog_f_off[d] = static_cast<int>(s.prog_f.size());
+    s.u_off[d] = upos;
+    s.a_len = apos;
+    s.u_len = upos;
+    s.slots_f = sa.hi;
+    if (!slot_of.empty()) throw std::logic_error("symbolic: update targets left unconsumed");
+    if (s.slots_f > kSlotMask - 1) throw std::runtime_error("symbolic: too many live update slots");
+    build_backward(s, {}, {}, {});
+    return s;
+}
+
+namespace {
+
+// Backward program (pivots d-1 .. 0) with the fused bus update.  `bus_of`,
+// `kind_of` (0 theta, 1 |V|) and `partner` (pivot label of the bus's other
+// unknown or -1) are per pivot; bus_of < 0 disables updates (raw solves).
+void build_backward(SymbolicLU& s, const std::vector<int>& bus_of, const std::vector<int>& kind_of,
+                    const std::vector<int>& partner)
+{
+    const int d = s.d;
+    const bool upd = !bus_of.empty();
+    // last use (smallest pivot label in whose struct j appears)
+    std::vector<int> last(static_cast<size_t>(d), d);
+    for (int k = 0; k < d; ++k)
+        for (int p = s.colp[k]; p < s.colp[k + 1]; ++p) last[s.coli[p]] = std::min(last[s.coli[p]], k);
+    if (upd)
+        for (int k = 0; k < d; ++k)
+            if (partner[k] >= 0 && partner[k] < k) last[k] = std::min(last[k], partner[k]);
+    SlotAlloc sa;
+    std::vector<int> xslot(static_cast<size_t>(d), -1);
+    std::vector<int> to_free;
+    s.prog_b.clear();
+    for (int k = d - 1; k >= 0; --k) {
+        const int c0 = s.colp[k], c = s.colp[k + 1] - c0;
+        s.prog_b.push_back(c);
+        to_free.clear();
+        for (int p = c0; p < c0 + c; ++p) {
+            const int j = s.coli[p];
+            if (xslot[j] < 0) throw std::logic_error("backward: x(j) read before it is computed");
+            s.prog_b.push_back(xslot[j]);
+            if (last[j] == k) to_free.push_back(xslot[j]);
+        }
+        int op = 0, bus = -1, pslot = kNoSlot;
+        if (upd && bus_of[k] >= 0) {
+            const int q = partner[k];
+            if (q < 0) {
+                op = kOpThetaSelf;
+                bus = bus_of[k];
+            } else if (q > k) {  // the partner was computed earlier: this pivot completes the bus
+                op = kind_of[k] == 0 ? kOpThetaSelfVmSlot : kOpVmSelfThetaSlot;
+                bus = bus_of[k];
+                pslot = xslot[q];
+                if (pslot < 0) throw std::logic_error("backward: partner x has no slot");
+                if (last[q] == k) to_free.push_back(pslot);
+            }
+        }
+        // every read of this pivot happens before x(k) is stored, so a slot
+        // freed here may hold x(k)
+        std::sort(to_free.begin(), to_free.end());
+        to_free.erase(std::unique(to_free.begin(), to_free.end()), to_free.end());
+        for (int sl : to_free) sa.put(sl);
+        int out = kNoSlot;
+        if (last[k] < k) {
+            out = sa.get();
+            xslot[k] = out;
+        }
+        s.prog_b.push_back(out);
+        s.prog_b.push_back(op);
+        s.prog_b.push_back(bus);
+        s.prog_b.push_back(pslot);
+    }
+    s.slots_b = sa.hi;
+}
+
+}  // namespace
+
+NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int npv, const int* pv,
+                     int npq, const int* pq, int slack)
+{
+    if (n <= 1) throw std::invalid_argument("the grid needs at least two buses");
+    NrPlan P;
+    P.n = n;
+    P.nY = Yp[n];
+    P.npv = npv;
+    P.npq = npq;
+    P.npvpq = npv + npq;
+    P.d = P.npvpq + npq;
+    P.slack = slack;
+    P.Yp.assign(Yp, Yp + n + 1);
+    P.Yi.assign(Yi, Yi + P.nY);
+    if (Yx) P.Yx.assign(Yx, Yx + 2 * static_cast<size_t>(P.nY));
+    for (int i = 0; i < n; ++i) {
+        if (Yp[i + 1] < Yp[i]) throw std::invalid_argument("Ybus indptr is not monotone");
+        for (int p = Yp[i]; p < Yp[i + 1]; ++p) {
+            if (Yi[p] < 0 || Yi[p] >= n) throw std::invalid_argument("Ybus column index out of range");
+            if (p > Yp[i] && Yi[p] <= Yi[p - 1])
+                throw std::invalid_argument("Ybus columns must be sorted and unique per row");
+        }
+    }
+    std::vector<int> kind(static_cast<size_t>(n), -1);  // 0 pv, 1 pq
+    for (int k = 0; k < npv; ++k) {
+        if (pv[k] < 0 || pv[k] >= n || kind[pv[k]] >= 0) throw std::invalid_argument("bad or repeated pv bus");
+        kind[pv[k]] = 0;
+    }
+    for (int k = 0; k < npq; ++k) {
+        if (pq[k] < 0 || pq[k] >= n || kind[pq[k]] >= 0) throw std::invalid_argument("bad or repeated pq bus");
+        kind[pq[k]] = 1;
+    }
+    if (slack < 0 || slack >= n || kind[slack] >= 0) throw std::invalid_argument("bad slack bus");
+    for (int i = 0; i < n; ++i)
+        if (i != slack && kind[i] < 0) throw std::invalid_argument("every non-slack bus must be pv or pq");
+    // reference layout (acpf_nr.cu:645-647, 678-685): sorted pvpq, pq as given
+    P.pvpq.assign(pv, pv + npv);
+    P.pvpq.insert(P.pvpq.end(), pq, pq + npq);
+    std::sort(P.pvpq.begin(), P.pvpq.end());
+    P.pq.assign(pq, pq + npq);
+    P.row_p.assign(static_cast<size_t>(n), -1);
+    P.row_q.assign(static_cast<size_t>(n), -1);
+    for (int k = 0; k < P.npvpq; ++k) P.row_p[P.pvpq[k]] = k;
+    for (int k = 0; k < npq; ++k) P.row_q[pq[k]] = P.npvpq + k;
+    // reference J CSR: row r holds one coefficient per (Ybus entry, block),
+    // columns ascending
+    const int d = P.d;
+    struct Ent { int col, yk, blk; };
+    std::vector<std::vector<Ent>> rows(static_cast<size_t>(d));
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 2; ++a) {
+            const int r = a == 0 ? P.row_p[i] : P.row_q[i];
+            if (r < 0) continue;
+            for (int p = Yp[i]; p < Yp[i + 1]; ++p) {
+                const int j = Yi[p];
+                if (P.row_p[j] >= 0) rows[r].push_back({P.row_p[j], p, a == 0 ? 0 : 2});
+                if (P.row_q[j] >= 0) rows[r].push_back({P.row_q[j], p, a == 0 ? 1 : 3});
+            }
+        }
+    P.Jp.assign(static_cast<size_t>(d) + 1, 0);
+    for (int b = 0; b < 4; ++b) P.map[b].assign(static_cast<size_t>(P.nY), -1);
+    for (int r = 0; r < d; ++r) {
+        auto& v = rows[r];
+        std::sort(v.begin(), v.end(), [](const Ent& x, const Ent& y) { return x.col < y.col; });
+        for (const Ent& e : v) {
+            P.map[e.blk][e.yk] = static_cast<int>(P.Ji.size());
+            P.Ji.push_back(e.col);
+        }
+        P.Jp[r + 1] = static_cast<int>(P.Ji.size());
+    }
+    P.nJ = P.Jp[d];
+
+    // bus-level minimum degree (slack excluded), weight = unknowns per bus
+    std::vector<int> bus_ids, bus_idx(static_cast<size_t>(n), -1);
+    for (int i = 0; i < n; ++i)
+        if (i != slack) { bus_idx[i] = static_cast<int>(bus_ids.size()); bus_ids.push_back(i); }
+    const int nb = static_cast<int>(bus_ids.size());
+    std::vector<int> badjp(static_cast<size_t>(nb) + 1, 0), badji, w(static_cast<size_t>(nb));
+    for (int b = 0; b < nb; ++b) {
+        const int i = bus_ids[b];
+        w[b] = kind[i] == 1 ? 2 : 1;
+        for (int p = Yp[i]; p < Yp[i + 1]; ++p) {
+            const int j = Yi[p];
+            if (j != i && j != slack) badji.push_back(bus_idx[j]);
+        }
+        badjp[b + 1] = static_cast<int>(badji.size());
+    }
+    // symmetrise the bus graph (Ybus patterns are symmetric, this guards others)
+    {
+        std::vector<std::vector<int>> nbv(static_cast<size_t>(nb));
+        for (int b = 0; b < nb; ++b)
+            for (int p = badjp[b]; p < badjp[b + 1]; ++p) { nbv[b].push_back(badji[p]); nbv[badji[p]].push_back(b); }
+        badji.clear();
+        for (int b = 0; b < nb; ++b) {
+            auto& v = nbv[b];
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            badji.insert(badji.end(), v.begin(), v.end());
+            badjp[b + 1] = static_cast<int>(badji.size());
+        }
+    }
+    std::vector<int> border = min_degree_order(nb, badjp, badji, w);
+    std::vector<int> order;
+    order.reserve(static_cast<size_t>(d));
+    for (int b : border) {
+        const int i = bus_ids[b];
+        order.push_back(P.row_p[i]);
+        if (P.row_q[i] >= 0) order.push_back(P.row_q[i]);
+    }
+    P.lu = build_symbolic(d, P.Jp.data(), P.Ji.data(), order);
+
+    // backward program with the fused update
+    {
+        std::vector<int> bus_of(static_cast<size_t>(d), -1), kind_of(static_cast<size_t>(d), 0),
+            partner(static_cast<size_t>(d), -1);
+        for (int i = 0; i < n; ++i) {
+            if (i == slack) continue;
+            const int kt = P.lu.iperm[P.row_p[i]];
+            bus_of[kt] = i;
+            kind_of[kt] = 0;
+            if (P.row_q[i] >= 0) {
+                const int kv = P.lu.iperm[P.row_q[i]];
+                bus_of[kv] = i;
+                kind_of[kv] = 1;
+                partner[kt] = kv;
+                partner[kv] = kt;
+            }
+        }
+        build_backward(P.lu, bus_of, kind_of, partner);
+    }
+
+    // mismatch/Jacobian program: buses in RCM order of the Ybus graph
+    {
+        std::vector<int> yadjp(static_cast<size_t>(n) + 1, 0), yadji;
+        for (int i = 0; i < n; ++i) {
+            for (int p = Yp[i]; p < Yp[i + 1]; ++p)
+                if (Yi[p] != i) yadji.push_back(Yi[p]);
+            yadjp[i + 1] = static_cast<int>(yadji.size());
+        }
+        std::vector<int> ro = rcm_order(n, yadjp, yadji);
+        P.k1_bus.clear();
+        P.k1_ent.clear();
+        for (int i : ro) {
+            if (i == slack) continue;
+            const int e0 = static_cast<int>(P.k1_ent.size() / 6);
+            int diag = -1;
+            for (int p = Yp[i]; p < Yp[i + 1]; ++p) {
+                const int j = Yi[p];
+                if (j == i) diag = p;
+                int a[4];
+                for (int b = 0; b < 4; ++b) a[b] = P.map[b][p] >= 0 ? P.lu.a_pos_of_entry[P.map[b][p]] : -1;
+                P.k1_ent.insert(P.k1_ent.end(), {j, p, a[0], a[1], a[2], a[3]});
+            }
+            if (diag < 0) throw std::invalid_argument("Ybus has no diagonal entry for a non-slack bus");
+            const int e1 = static_cast<int>(P.k1_ent.size() / 6);
+            const int fp = P.lu.f_pos[P.row_p[i]];
+            const int fq = P.row_q[i] >= 0 ? P.lu.f_pos[P.row_q[i]] : -1;
+            P.k1_bus.insert(P.k1_bus.end(), {i, e0, e1, fp, fq, diag});
+        }
+        P.k1_nbus = static_cast<int>(P.k1_bus.size() / 6);
+    }
+    return P;
+}
+
+}  // namespace g2g

@@ -1,0 +1,101 @@
+// Host-side symbolic layer of the batched Newton-Raphson power flow.
+//
+// Everything here runs once per grid pattern on the CPU and produces the
+// shared (row-independent) tables the device kernels stream through:
+//
+//   * the reference's Jacobian layout (rows/cols [P of sorted pvpq | Q of pq],
+//     gpusim2grid src/_cpp/acpf_nr.cu:145-236) and its scatter maps;
+//   * a fill-reducing pivot order (bus-level minimum degree, each bus's
+//     unknowns kept adjacent), relabelled by a postorder of the elimination
+//     tree so that partial updates live only briefly;
+//   * the factorization program: per pivot, where each value of its row and
+//     column comes from (the value stream and/or an LDS update slot) and which
+//     slots its rank-1 update feeds (replaces the reference's left-looking
+//     items and contribution lists, own_lu/lu_plan.cpp:88-285);
+//   * the backward-substitution program, including the fused polar V update;
+//   * the mismatch/Jacobian program (buses in reverse Cuthill-McKee order).
+//
+// Stream layout ("tiled interleaved"): a per-row array of L values is stored
+// as [row / 64][position][row % 64], so one wavefront (64 rows) reads or
+// writes 64 consecutive elements per position.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "program_format.hpp"
+
+namespace g2g {
+
+struct SymbolicLU {
+    int d = 0;
+    int nA = 0;         // pattern entries of the (symmetrised) input present in A
+    int nLU = 0;        // L + U + diagonal entries
+    int n_updates = 0;  // sum over pivots of c_k^2 (rank-1 update terms)
+    int max_c = 0;      // longest off-diagonal column
+    int height = 0;     // elimination tree height
+    std::vector<int> perm;     // pivot k -> input index
+    std::vector<int> iperm;    // input index -> pivot k
+    std::vector<int> colp, coli;  // struct of L(:,k) (pivot labels > k, sorted), CSR over k
+
+    // factorization + forward substitution program (one record per pivot):
+    //   c, bref(b(k) slot or none), dref, lref[c], uref[c], upd[c*c], bupd[c]
+    std::vector<int> prog_f;
+    std::vector<int> prog_f_off;  // [d+1]
+    int slots_f = 0;
+    // A stream: per pivot [F(k)] [diag] [L present...] [U present...]
+    int a_len = 0;                 // A-stream length (d + nA)
+    std::vector<int> a_pos_of_entry;  // input CSR position -> A-stream position
+    std::vector<int> f_pos;           // input index (equation) -> A-stream position of its rhs
+    // U stream: per pivot [y(k)] [U(k,k)] [U(k,j)...]
+    int u_len = 0;
+    std::vector<int> u_off;        // [d+1] start of pivot k's U record
+    // backward substitution program (one record per pivot, pivots d-1 .. 0):
+    //   c, xref[c] (slot of x(j) for j in struct_k), out (slot for x(k) or none),
+    //   op, and for op != 0 two further ints (see below)
+    std::vector<int> prog_b;
+    int slots_b = 0;
+};
+
+// Builds the symbolic factorization of a square CSR pattern (structurally
+// symmetrised).  `order` is the pivot order (a permutation of 0..d-1) before
+// the postorder relabelling; empty = minimum degree on the pattern itself.
+// `group` (optional, size d): unknowns with the same group id are eliminated
+// as one supervariable by the minimum-degree order.
+SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> order = {});
+
+// Minimum degree order of an undirected graph given by symmetric adjacency
+// lists (CSR, no self loops) with vertex weights; returns the elimination order.
+std::vector<int> min_degree_order(int n, const std::vector<int>& adjp, const std::vector<int>& adji,
+                                  const std::vector<int>& weight);
+
+// Reverse Cuthill-McKee order of an undirected graph (CSR adjacency).
+std::vector<int> rcm_order(int n, const std::vector<int>& adjp, const std::vector<int>& adji);
+
+// Backward-program records hold [c, xref[c], out, op, bus, partner slot]; a
+// nonzero op (program_format.hpp) completes the bus: with the partner slot's
+// x it applies V <- V ((|V| + dVm) / |V|) exp(j dVa).
+
+struct NrPlan {
+    int n = 0, nY = 0, npv = 0, npq = 0, npvpq = 0, d = 0, nJ = 0, slack = -1;
+    std::vector<int> Yp, Yi;
+    std::vector<double> Yx;  // complex, interleaved
+    std::vector<int> pvpq, pq;           // reference layout
+    std::vector<int> row_p, row_q;       // per bus: reference J row of P / Q, -1 if none
+    std::vector<int> Jp, Ji;             // reference J CSR
+    std::vector<int> map[4];             // per Ybus entry: J CSR position (11,12,21,22) or -1
+    SymbolicLU lu;
+    // mismatch/Jacobian program: buses in RCM order (slack excluded);
+    // k1_bus[i] = {bus, entry begin, entry end, fpos_p, fpos_q, diag entry}
+    std::vector<int> k1_bus;             // 6 ints per bus
+    // per Ybus entry (in k1 order): {col j, ybus k, a11, a12, a21, a22}
+    std::vector<int> k1_ent;             // 6 ints per entry
+    int k1_nbus = 0;
+};
+
+NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int npv, const int* pv,
+                     int npq, const int* pq, int slack);
+
+}  // namespace g2g

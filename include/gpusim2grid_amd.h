@@ -1,0 +1,161 @@
+/*
+ * gpusim2grid_amd.h -- C ABI of the MI355X batched Newton-Raphson power flow.
+ *
+ * Plain C types only (int32 indices, float64 values, complex numbers as
+ * interleaved (re, im) float64 pairs, opaque handles, a HIP stream passed as
+ * void*).  Every function returns G2G_OK or an error code; g2g_last_error()
+ * gives the message of the last failure on the calling thread.  Per-row
+ * numerical failures are statuses, never errors (gpusim2grid CLAUDE.md:172).
+ *
+ * The reference binds its solvers through pybind11 (src/_cpp/python_bindings.cpp)
+ * onto C++ sessions; the seams these entry points replace are cited per
+ * function.  Handles are not thread-safe: one stream per batch, like the
+ * reference's sessions (src/_cpp/acpf_nr_state.cuh:29).
+ */
+#ifndef GPUSIM2GRID_AMD_H
+#define GPUSIM2GRID_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define G2G_OK 0
+#define G2G_ERR_INVALID 1 /* bad argument (Python: ValueError)            */
+#define G2G_ERR_RUNTIME 2 /* unsupported case or internal failure         */
+#define G2G_ERR_HIP 3     /* HIP runtime error                              */
+
+/* precision of the Jacobian, the factorization and the triangular solves;
+ * the mismatch, the convergence test and the V state are always FP64 */
+#define G2G_FP64 0
+#define G2G_MIXED 1
+
+/* schedules (src/_cpp/contingency/row_status.hpp:28-31) */
+#define G2G_SCHED_CHUNKED 0    /* every row runs exactly max_iter updates  */
+#define G2G_SCHED_CONTINUOUS 1 /* a row stops once ||F||inf < tol           */
+
+/* row status (src/_cpp/contingency/row_status.hpp:33-43) */
+#define G2G_CONVERGED 0
+#define G2G_MAXITER 1
+#define G2G_DIVERGED 2
+#define G2G_NOTSIMULATED 3
+
+/* arrays g2g_plan_get_array can copy out (test and integration introspection) */
+#define G2G_ARR_JP 0        /* reference J CSR indptr [d+1]                   */
+#define G2G_ARR_JI 1        /* reference J CSR indices [nJ]                   */
+#define G2G_ARR_MAPS 2      /* J position of each Ybus entry, blocks 11,12,21,22 [4 nY] */
+#define G2G_ARR_PVPQ 3      /* sorted pvpq [npvpq]                            */
+#define G2G_ARR_PERM 4      /* pivot -> reference unknown [d]                 */
+#define G2G_ARR_COLP 5      /* L column structure indptr [d+1] (pivot labels) */
+#define G2G_ARR_COLI 6      /* L column structure indices                     */
+#define G2G_ARR_PROG_F 7    /* factorization program                          */
+#define G2G_ARR_PROG_B 8    /* backward program                               */
+#define G2G_ARR_APOS 9      /* A-stream position of each input CSR entry [nnz] */
+#define G2G_ARR_FPOS 10     /* A-stream position of each rhs entry [d]        */
+#define G2G_ARR_UOFF 11     /* U-stream record start per pivot [d+1]          */
+
+typedef struct g2g_plan_s *g2g_plan_t;
+typedef struct g2g_batch_s *g2g_batch_t;
+
+typedef struct {
+    int32_t n, nY, d, nJ, npvpq, npq, slack; /* grid (0 / -1 for a raw matrix plan)   */
+    int32_t nnz;                             /* input pattern entries (nJ for grids)  */
+    int32_t nLU, n_updates, max_c, height;   /* symbolic factorization                */
+    int32_t slots_f, slots_b, a_len, u_len;  /* program resources                     */
+} g2g_plan_info;
+
+typedef struct {
+    int32_t iterations;     /* Newton iterations launched                        */
+    int32_t n_converged, n_maxiter, n_diverged;
+    double ms_total;        /* GPU time of the whole solve (hipEvents)           */
+    double ms_mismatch;     /* mismatch + Jacobian fill                          */
+    double ms_factor;       /* factorization + forward substitution              */
+    double ms_backward;     /* backward substitution + V update                  */
+    int64_t bytes_factor;   /* algorithmic HBM bytes of one factorization launch */
+} g2g_solve_stats;
+
+const char *g2g_last_error(void);
+int g2g_version(void);
+
+/* Builds every row-independent table of a grid: the reference's [pvpq | pq]
+ * Jacobian pattern and scatter maps (replaces build_J_structure,
+ * src/_cpp/acpf_nr.cu:145-236, and the host part of AcPfNrState's
+ * constructor, acpf_nr.cu:567-806), the fill-reducing pivot order and the
+ * factorization / substitution programs (replaces cuDSS ANALYSIS,
+ * src/_cpp/contingency/strategies/cudss_batch_solver.cuh:339-440, and the own
+ * LU's build_lu_plan, src/_cpp/own_lu/lu_plan.cpp:88-285).  Ybus is CSR with
+ * sorted unique columns; Yx holds 2*nY doubles; pv/pq/slack are bus indices.
+ * Uploads the tables to the current HIP device. */
+int g2g_plan_create(int32_t n, const int32_t *Yp, const int32_t *Yi, const double *Yx, int32_t npv,
+                    const int32_t *pv, int32_t npq, const int32_t *pq, int32_t slack, g2g_plan_t *out);
+
+/* Plan of a bare square matrix pattern for the raw batched LU (replaces
+ * CudssBatchSolver::initialize, cudss_batch_solver.cuh:273-440, behind
+ * benchmark_cudss_batch_raw, src/_cpp/raw_cudss_solve.hpp:105-121).  `order`
+ * (nullable) fixes the elimination order before the postorder relabelling. */
+int g2g_lu_plan_create(int32_t d, const int32_t *Ap, const int32_t *Ai, const int32_t *order, g2g_plan_t *out);
+
+void g2g_plan_destroy(g2g_plan_t plan);
+int g2g_plan_get_info(g2g_plan_t plan, g2g_plan_info *info);
+int g2g_plan_get_array(g2g_plan_t plan, int32_t which, int32_t *out, int64_t capacity, int64_t *length);
+
+/* Device state of `rows` power flows sharing a plan (replaces BatchPfDriver's
+ * allocations, src/_cpp/contingency/batch_pf_driver.cu:122-330).
+ * per_row_ybus = 1 keeps one Ybus value vector per row (outage batches).
+ * stream: hipStream_t (NULL = default stream). */
+int g2g_batch_create(g2g_plan_t plan, int32_t rows, int32_t precision, int32_t per_row_ybus, void *stream,
+                     g2g_batch_t *out);
+void g2g_batch_destroy(g2g_batch_t batch);
+
+/* Start voltages: `rows` complex rows of n values, row stride ld (complex
+ * numbers; ld = 0 broadcasts one row), written to rows [row0, row0 + rows).
+ * on_device = 1 when V points to device memory. */
+int g2g_batch_set_v(g2g_batch_t batch, const double *V, int64_t ld, int32_t rows, int32_t row0, int32_t on_device);
+/* Per-unit injections Sbus, same conventions (the reference's per-row Sbus,
+ * src/_cpp/injection_sweep_session.cu:235-247; ld = 0: shared, as a
+ * contingency batch's Sbus with stride 0, acpf_nr_kernels.cuh:30-41). */
+int g2g_batch_set_sbus(g2g_batch_t batch, const double *S, int64_t ld, int32_t rows, int32_t row0,
+                       int32_t on_device);
+/* Outage value edits: row r subtracts delta[q] (complex) from Ybus entry
+ * entry[q] for q in [row_ptr[r], row_ptr[r+1]) (replaces the flat patches and
+ * apply_contingencies_kernel, contingency_analysis_helper.cpp:163 and
+ * acpf_nr_kernels.cu:328-348).  Needs per_row_ybus; resets every row to the
+ * plan's base values first.  Host arrays. */
+int g2g_batch_set_ybus_edits(g2g_batch_t batch, const int32_t *row_ptr, const int32_t *entry, const double *delta,
+                             int32_t rows);
+/* Rows [0, active_rows) take part in the next solve; the others are left
+ * untouched and reported NotSimulated. */
+int g2g_batch_set_active_rows(g2g_batch_t batch, int32_t active_rows);
+
+/* Newton-Raphson on every row (replaces run_nr_loop, src/_cpp/contingency/driver.cuh:136-259,
+ * with the chunked status rule batch_pf_driver.cu:1361-1377 and the
+ * continuous one :2895-2913).  A row converges when ||F||inf < tol (per
+ * unit); its iteration count is the number of Newton updates applied. */
+int g2g_batch_solve(g2g_batch_t batch, int32_t max_iter, double tol, int32_t scheduling, g2g_solve_stats *stats);
+
+/* Results.  V: rows x n complex, row stride ld (complex numbers). */
+int g2g_batch_get_v(g2g_batch_t batch, double *V, int64_t ld, int32_t on_device);
+int g2g_batch_get_results(g2g_batch_t batch, int32_t *status, int32_t *iters, double *residual);
+
+/* Evaluates, at the current V, F (reference row layout, rows x d) and, when
+ * J != NULL, the Jacobian values in the reference CSR order (rows x nJ), on
+ * the device kernels; residual (nullable) gets ||F||inf.  Host arrays. */
+int g2g_batch_eval(g2g_batch_t batch, double *F, double *J, double *residual);
+
+/* Raw batched LU solve of `rows` matrices on a g2g_lu_plan_create pattern:
+ * vals rows x nnz (CSR order), rhs rows x d, x rows x d (host arrays);
+ * pivots below pivot_eps * max|A_row| are clamped (own_lu/batched_lu.cu:45-53)
+ * and counted per row in `clamped` (nullable). */
+int g2g_lu_solve_batch(g2g_plan_t plan, int32_t rows, int32_t precision, const double *vals, const double *rhs,
+                       double *x, double pivot_eps, int32_t *clamped, void *stream);
+
+/* Device pointer of the batch's tiled Sbus array [ceil(rows/64)][n][64]
+ * complex (bench and zero-copy callers fill it directly). */
+int g2g_batch_device_arrays(g2g_batch_t batch, void **V_tiles, void **S_tiles);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUSIM2GRID_AMD_H */

@@ -1,0 +1,105 @@
+"""The oracle pinned against closed forms, scipy and finite differences."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from gpusim2grid_amd import synthetic as syn
+from oracle import oracle as orc
+
+
+@pytest.fixture(scope="module", params=["c0", "118"])
+def grid(request):
+    return syn.grid_c0() if request.param == "c0" else syn.grid_118()
+
+
+def _perturbed(g, seed=0):
+    rng = np.random.default_rng(seed)
+    return g.flat_start() * np.exp(1j * rng.normal(0, 0.05, g.n_bus)) * (1 + rng.normal(0, 0.02, g.n_bus))
+
+
+def test_mismatch_closed_form(grid):
+    V = _perturbed(grid)
+    F, _, r = orc.f_and_j(grid.Ybus, V, grid.Sbus, grid.pv, grid.pq, with_j=False)
+    Fn = orc.mismatch_np(grid.Ybus, V, grid.Sbus, grid.pv, grid.pq)
+    np.testing.assert_allclose(F, Fn, rtol=0, atol=1e-13)
+    assert r == pytest.approx(np.abs(Fn).max(), rel=1e-14)
+
+
+def test_jacobian_matpower_form(grid):
+    V = _perturbed(grid, 1)
+    _, J, _ = orc.f_and_j(grid.Ybus, V, grid.Sbus, grid.pv, grid.pq)
+    Jn = orc.jacobian_np(grid.Ybus, V, grid.pv, grid.pq)
+    np.testing.assert_allclose(J.toarray(), Jn, rtol=0, atol=1e-11)
+    # every structural MATPOWER entry is in the CSR pattern
+    assert ((Jn != 0) <= (J.toarray() != 0)).all()
+
+
+def test_jacobian_finite_differences(grid):
+    """J = -dF/dx by central differences (F = -mismatch)."""
+    V = _perturbed(grid, 2)
+    pvpq = np.sort(np.concatenate([grid.pv, grid.pq]))
+    _, J, _ = orc.f_and_j(grid.Ybus, V, grid.Sbus, grid.pv, grid.pq)
+    J = J.toarray()
+    h = 1e-6
+    d = J.shape[0]
+    cols = np.random.default_rng(3).choice(d, size=min(d, 12), replace=False)
+    for c in cols:
+        dx = np.zeros(d)
+        dx[c] = h
+        Fp = orc.mismatch_np(grid.Ybus, orc.polar_update_np(V, dx, grid.pv, grid.pq), grid.Sbus, grid.pv, grid.pq)
+        Fm = orc.mismatch_np(grid.Ybus, orc.polar_update_np(V, -dx, grid.pv, grid.pq), grid.Sbus, grid.pv, grid.pq)
+        np.testing.assert_allclose(-(Fp - Fm) / (2 * h), J[:, c], rtol=1e-5, atol=1e-6)
+    assert pvpq.size + grid.pq.size == d
+
+
+def test_sparse_solve_vs_superlu(grid):
+    V = _perturbed(grid, 4)
+    F, J, _ = orc.f_and_j(grid.Ybus, V, grid.Sbus, grid.pv, grid.pq)
+    x = orc.sparse_solve(J, F)
+    xs = spla.spsolve(J.tocsc(), F)
+    np.testing.assert_allclose(x, xs, rtol=1e-10, atol=1e-12)
+
+
+def test_sparse_solve_needs_pivoting():
+    """A matrix with a zero diagonal: partial pivoting must still solve it."""
+    A = sp.csr_matrix(np.array([[0.0, 2.0, 0.0], [1.0, 0.0, 3.0], [0.0, 4.0, 5.0]]))
+    b = np.array([1.0, 2.0, 3.0])
+    np.testing.assert_allclose(orc.sparse_solve(A, b), np.linalg.solve(A.toarray(), b), rtol=1e-14)
+
+
+def test_nr_converges_quadratically_and_solves_the_equations(grid):
+    V, st, it, res = orc.nr_batch(grid.Ybus, grid.flat_start(), grid.Sbus, grid.pv, grid.pq)
+    assert st[0] == 0 and res[0] < 1e-8
+    # a solution of the power-flow equations at PV/PQ buses and set-points kept
+    Fn = orc.mismatch_np(grid.Ybus, V[0], grid.Sbus, grid.pv, grid.pq)
+    assert np.abs(Fn).max() < 1e-8
+    np.testing.assert_allclose(np.abs(V[0][grid.pv]), grid.vset[grid.pv], rtol=1e-12)
+    np.testing.assert_allclose(V[0][grid.slack], grid.flat_start()[grid.slack], rtol=1e-15)
+    # iteration counting: the residual history is quadratic, and max_iter caps
+    V1, st1, it1, res1 = orc.nr_batch(grid.Ybus, grid.flat_start(), grid.Sbus, grid.pv, grid.pq,
+                                      max_iter=int(it[0]) - 1)
+    assert st1[0] == 1 and it1[0] == it[0] - 1 and res1[0] >= 1e-8
+
+
+def test_nr_status_rules():
+    g = syn.grid_118()
+    S = np.stack([g.Sbus, g.Sbus * np.nan, g.Sbus])
+    V, st, it, res = orc.nr_batch(g.Ybus, g.flat_start(), S, g.pv, g.pq, max_iter=10)
+    assert list(st) == [0, 2, 0] and it[1] == 0 and np.isnan(res[1])
+    # already converged at V0: zero iterations
+    V2, st2, it2, _ = orc.nr_batch(g.Ybus, V[0], g.Sbus, g.pv, g.pq)
+    assert st2[0] == 0 and it2[0] == 0
+
+
+def test_outage_ybus_closed_form():
+    g = syn.grid_118()
+    l = 5
+    Y = orc.outage_ybus(g, [l])
+    f, t = g.branch_from[l], g.branch_to[l]
+    D = (g.Ybus - Y).tocoo()
+    got = {(int(i), int(j)): v for i, j, v in zip(D.row, D.col, D.data) if abs(v) > 1e-14}
+    want = {(f, f): g.yff[l], (f, t): g.yft[l], (t, f): g.ytf[l], (t, t): g.ytt[l]}
+    assert set(got) == set(want)
+    for k, v in want.items():
+        assert abs(got[k] - v) < 1e-12

@@ -1,0 +1,166 @@
+"""Host symbolic layer (CPU): J layout, ordering, programs, C ABI loading."""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import program_interp as pi
+from gpusim2grid_amd import _capi
+from gpusim2grid_amd import synthetic as syn
+from oracle import oracle as orc
+
+
+def test_library_exports_every_declared_symbol():
+    L = _capi.lib()
+    names = _capi.exported_symbols()
+    assert len(names) >= 19
+    for name in names:
+        assert hasattr(L, name), name
+    assert L.g2g_version() >= 100
+
+
+@pytest.fixture(scope="module", params=["c0", "118", "c1"])
+def grid(request):
+    return {"c0": syn.grid_c0, "118": syn.grid_118, "c1": syn.grid_c1}[request.param]()
+
+
+def test_jacobian_layout_matches_oracle(grid):
+    P = _capi.Plan.from_grid(grid.Ybus, grid.pv, grid.pq, grid.slack)
+    Jp, Ji, maps = orc.jpattern(grid.Ybus, grid.pv, grid.pq)
+    np.testing.assert_array_equal(P.array("jp"), Jp)
+    np.testing.assert_array_equal(P.array("ji"), Ji)
+    np.testing.assert_array_equal(P.array("maps").reshape(4, -1), maps)
+    np.testing.assert_array_equal(P.array("pvpq"), np.sort(np.concatenate([grid.pv, grid.pq])))
+    inf = P.info()
+    assert inf["d"] == grid.pv.size + 2 * grid.pq.size
+    # strictly increasing columns per row (acpf_nr.cu:226-235)
+    for r in range(inf["d"]):
+        assert np.all(np.diff(Ji[Jp[r]:Jp[r + 1]]) > 0)
+
+
+def _dense_symbolic_fill(A_pattern, perm):
+    """Brute-force symbolic elimination of the symmetrised permuted pattern."""
+    B = (A_pattern + A_pattern.T)[np.ix_(perm, perm)] != 0
+    d = B.shape[0]
+    B = B.copy()
+    for k in range(d):
+        nz = np.flatnonzero(B[k + 1:, k]) + k + 1
+        B[np.ix_(nz, nz)] = True
+    return int(np.tril(B, -1).sum()), B
+
+
+def test_symbolic_fill_equals_brute_force():
+    g = syn.grid_118()
+    P = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+    inf = P.info()
+    Jp, Ji = P.array("jp"), P.array("ji")
+    A = sp.csr_matrix((np.ones(Ji.size), Ji, Jp), shape=(inf["d"], inf["d"])).toarray()
+    perm = P.array("perm")
+    nL, B = _dense_symbolic_fill(A, perm)
+    assert inf["nLU"] == inf["d"] + 2 * nL
+    colp, coli = P.array("colp"), P.array("coli")
+    for k in range(inf["d"]):
+        np.testing.assert_array_equal(coli[colp[k]:colp[k + 1]], np.flatnonzero(B[k + 1:, k]) + k + 1)
+    # minimum degree reduces the fill against the natural order
+    nL_nat, _ = _dense_symbolic_fill(A, np.arange(inf["d"]))
+    assert nL <= nL_nat
+
+
+def _perturbed(g, seed=0):
+    rng = np.random.default_rng(seed)
+    return g.flat_start() * np.exp(1j * rng.normal(0, 0.05, g.n_bus)) * (1 + rng.normal(0, 0.02, g.n_bus))
+
+
+def test_programs_factor_like_superlu_natural(grid):
+    """The static-pivot LU under the plan's order equals SuperLU's factors of
+    P A P^T with no pivoting (natural order, diag_pivot_thresh=0)."""
+    if grid.n_bus > 200:
+        pytest.skip("dense comparison on small grids only")
+    P = _capi.Plan.from_grid(grid.Ybus, grid.pv, grid.pq, grid.slack)
+    inf = P.info()
+    F, J, _ = orc.f_and_j(grid.Ybus, _perturbed(grid), grid.Sbus, grid.pv, grid.pq)
+    A = pi.astream_from_csr(inf, P.array("apos"), P.array("fpos"), J.data[None], F[None])
+    U, cl = pi.factor_forward(inf, P.array("prog_f"), A)
+    assert cl[0] == 0
+    perm = P.array("perm")
+    Jperm = J.toarray()[np.ix_(perm, perm)]
+    lu = spla.splu(sp.csc_matrix(Jperm), permc_spec="NATURAL", diag_pivot_thresh=0.0,
+                   options=dict(SymmetricMode=True))
+    np.testing.assert_array_equal(lu.perm_r, np.arange(inf["d"]))
+    Uref = lu.U.toarray()
+    uoff = P.array("uoff")
+    colp, coli = P.array("colp"), P.array("coli")
+    for k in range(inf["d"]):
+        rec = U[0, uoff[k]:uoff[k + 1]]
+        assert rec[1] == pytest.approx(Uref[k, k], rel=1e-12)
+        np.testing.assert_allclose(rec[2:], Uref[k, coli[colp[k]:colp[k + 1]]], rtol=1e-11, atol=1e-12)
+
+
+def test_raw_program_solves(grid):
+    F, J, _ = orc.f_and_j(grid.Ybus, _perturbed(grid, 1), grid.Sbus, grid.pv, grid.pq)
+    P = _capi.Plan.from_matrix(J)
+    inf = P.info()
+    rows = 3
+    vals = np.stack([J.data * (1 + 0.01 * k) for k in range(rows)])
+    rhs = np.stack([F * (k + 1) for k in range(rows)])
+    A = pi.astream_from_csr(inf, P.array("apos"), P.array("fpos"), vals, rhs)
+    U, _ = pi.factor_forward(inf, P.array("prog_f"), A)
+    x = pi.backward(inf, P.array("prog_b"), U, perm=P.array("perm"))
+    for k in range(rows):
+        Jk = sp.csr_matrix((vals[k], J.indices, J.indptr), shape=J.shape)
+        xs = spla.spsolve(Jk.tocsc(), rhs[k])
+        np.testing.assert_allclose(x[k], xs, rtol=1e-10, atol=1e-10 * np.abs(xs).max())
+
+
+def test_nr_program_applies_the_polar_update(grid):
+    V = _perturbed(grid, 2)
+    P = _capi.Plan.from_grid(grid.Ybus, grid.pv, grid.pq, grid.slack)
+    inf = P.info()
+    F, J, _ = orc.f_and_j(grid.Ybus, V, grid.Sbus, grid.pv, grid.pq)
+    A = pi.astream_from_csr(inf, P.array("apos"), P.array("fpos"), J.data[None], F[None])
+    U, _ = pi.factor_forward(inf, P.array("prog_f"), A)
+    Vn = pi.backward(inf, P.array("prog_b"), U, V=V[None])[0]
+    dx = spla.spsolve(J.tocsc(), F)
+    np.testing.assert_allclose(Vn, orc.polar_update_np(V, dx, grid.pv, grid.pq), rtol=0, atol=1e-10)
+
+
+def test_saddle_matrix_clamp_is_counted():
+    """gpusim2grid tests/python/test_own_lu.py:128-145: a 3x3 saddle whose first
+    pivot is 0 in the identity order is clamped (and counted)."""
+    A = sp.csr_matrix((np.array([0., 1., 2., 1., 1., 0., 2., 0., 1.]), np.array([0, 1, 2] * 3),
+                       np.array([0, 3, 6, 9])), shape=(3, 3))
+    P = _capi.Plan.from_matrix(A, order=[0, 1, 2])
+    inf = P.info()
+    np.testing.assert_array_equal(P.array("perm"), [0, 1, 2])
+    b = np.array([1.0, 2.0, 3.0])
+    Ast = pi.astream_from_csr(inf, P.array("apos"), P.array("fpos"), A.data[None], b[None])
+    U, cl = pi.factor_forward(inf, P.array("prog_f"), Ast, pivot_eps=1e-12, scale=np.array([2.0]))
+    assert cl[0] == 1
+    x = pi.backward(inf, P.array("prog_b"), U, perm=P.array("perm"))[0]
+    r = np.abs(A @ x - b).max() / np.abs(b).max()
+    assert r > 1e-8  # the clamp is visible
+
+
+def test_invalid_inputs_raise_value_error():
+    g = syn.grid_c0()
+    with pytest.raises(ValueError):
+        _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, [g.pv[0]])  # slack also PV
+    with pytest.raises(ValueError):
+        _capi.Plan.from_grid(g.Ybus, g.pv[:-1], g.pq, g.slack)  # a bus without a type
+    Y = g.Ybus.tolil()
+    Y[g.pq[0], g.pq[0]] = 0
+    Y = sp.csr_matrix(Y)
+    Y.eliminate_zeros()
+    with pytest.raises(ValueError):
+        _capi.Plan.from_grid(Y, g.pv, g.pq, g.slack)  # no diagonal entry
+
+
+def test_slots_stay_small_on_the_large_grids():
+    """The postorder keeps the live update set tiny (LDS budget of the kernels)."""
+    for g in (syn.grid_c1(), syn.grid_c2()):
+        inf = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack).info()
+        assert inf["max_c"] <= 16
+        assert (inf["slots_f"] + 16) * 64 * 8 <= 160 * 1024
+        assert inf["nLU"] < 1.2 * inf["nJ"]
