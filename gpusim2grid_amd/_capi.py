@@ -37,7 +37,8 @@ class SolveStats(ctypes.Structure):
                 ("n_maxiter", ctypes.c_int32), ("n_diverged", ctypes.c_int32),
                 ("ms_total", ctypes.c_double), ("ms_mismatch", ctypes.c_double),
                 ("ms_factor", ctypes.c_double), ("ms_backward", ctypes.c_double),
-                ("bytes_factor", ctypes.c_int64)]
+                ("bytes_factor", ctypes.c_int64), ("rows_factored", ctypes.c_int64),
+                ("n_factor_launches", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -20,8 +20,10 @@ template <typename Real, bool PerRowY, bool WithJ>
 __global__ void k_mismatch(MismatchArgs<Real> a);
 template <typename Real>
 __global__ void k_factor(FactorArgs<Real> a);
-template <typename Real, bool Update>
+template <typename Real>
 __global__ void k_backward(BackwardArgs<Real> a);
+template <typename Real>
+__global__ void k_update(UpdateArgs<Real> a);
 template <typename T>
 __global__ void k_rows_to_tiles(const T* in, long long ld, int rows, int m, int row_offset, T* out, int broadcast);
 template <typename T>
@@ -95,6 +97,22 @@ void dfree(void* p)
 
 inline unsigned blocks_for(long long n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
 
+// device buffers freed together at scope exit
+struct DeviceScratch {
+    std::vector<void*> p;
+    ~DeviceScratch()
+    {
+        for (void* q : p) dfree(q);
+    }
+    template <typename T>
+    T* get(size_t n)
+    {
+        T* q = dalloc<T>(n);
+        p.push_back(q);
+        return q;
+    }
+};
+
 }  // namespace
 
 struct g2g_plan_s {
@@ -111,12 +129,15 @@ struct g2g_plan_s {
     int* d_perm = nullptr;
     int* d_apos = nullptr;
     int* d_fpos = nullptr;
+    int* d_upd = nullptr;     // per bus {bus, theta unknown, |V| unknown}
+    int n_upd = 0;
+    int prog_f_stride = 0, prog_b_stride = 0;
     bool uploaded = false;
     const SymbolicLU& lu() const { return grid ? nr.lu : raw; }
     ~g2g_plan_s()
     {
         dfree(d_prog_f); dfree(d_prog_b); dfree(d_k1_bus); dfree(d_k1_ent);
-        dfree(d_Y); dfree(d_perm); dfree(d_apos); dfree(d_fpos);
+        dfree(d_Y); dfree(d_perm); dfree(d_apos); dfree(d_fpos); dfree(d_upd);
     }
     // device copies of the tables, made on first use (plans can be built and
     // inspected on a host without a GPU)
@@ -128,8 +149,15 @@ struct g2g_plan_s {
         if (s.max_c > kMaxC)
             throw std::runtime_error("pivot column of " + std::to_string(s.max_c) + " entries exceeds the kernels' " +
                                      std::to_string(kMaxC));
-        d_prog_f = dupload(s.prog_f);
-        d_prog_b = dupload(s.prog_b);
+        // programs padded to whole prefetch chunks
+        auto padded = [](const std::vector<int>& v, int& stride) {
+            stride = static_cast<int>((v.size() + kProgChunk - 1) / kProgChunk * kProgChunk);
+            std::vector<int> out(v);
+            out.resize(static_cast<size_t>(stride), 0);
+            return out;
+        };
+        d_prog_f = dupload(padded(s.prog_f, prog_f_stride));
+        d_prog_b = dupload(padded(s.prog_b, prog_b_stride));
         d_perm = dupload(s.perm);
         d_apos = dupload(s.a_pos_of_entry);
         d_fpos = dupload(s.f_pos);
@@ -139,6 +167,13 @@ struct g2g_plan_s {
             std::vector<double2> y(static_cast<size_t>(nr.nY));
             for (int k = 0; k < nr.nY; ++k) y[k] = make_double2(nr.Yx[2 * k], nr.Yx[2 * k + 1]);
             d_Y = dupload(y);
+            std::vector<int> upd;
+            for (int i = 0; i < nr.n; ++i) {
+                if (i == nr.slack) continue;
+                upd.insert(upd.end(), {i, nr.row_p[i], nr.row_q[i]});
+            }
+            n_upd = static_cast<int>(upd.size() / 3);
+            d_upd = dupload(upd);
         }
         uploaded = true;
     }
@@ -153,6 +188,8 @@ struct g2g_batch_s {
     double2* Yrow = nullptr;
     void* A = nullptr;
     void* U = nullptr;
+    void* X = nullptr;         // solved Newton step [T][d][64]
+    int a_stride = 0, u_stride = 0;
     double* res = nullptr;
     double* jmax = nullptr;
     int* status = nullptr;
@@ -164,7 +201,7 @@ struct g2g_batch_s {
     std::vector<hipEvent_t> ev;
     ~g2g_batch_s()
     {
-        dfree(V); dfree(S); dfree(Yrow); dfree(A); dfree(U); dfree(res); dfree(jmax);
+        dfree(V); dfree(S); dfree(Yrow); dfree(A); dfree(U); dfree(X); dfree(res); dfree(jmax);
         dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped);
         if (h_active) (void)hipHostFree(h_active);
         for (auto e : ev) (void)hipEventDestroy(e);
@@ -175,15 +212,28 @@ struct g2g_batch_s {
 namespace {
 
 template <typename Real>
+constexpr int chunk_of() { return sizeof(Real) == sizeof(double) ? kChunkF64 : kChunkF32; }
+
+inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+template <typename Real>
+size_t ring_lds()
+{
+    return static_cast<size_t>(chunk_of<Real>()) * kRingSlots * kLanes * sizeof(Real) +
+           static_cast<size_t>(kProgChunk) * kProgSlots * sizeof(int);
+}
+
+// stream rings + slots (plus the zero and trash slots) [+ two column scratch rows]
+template <typename Real>
 size_t factor_lds(const SymbolicLU& s)
 {
-    return static_cast<size_t>(s.slots_f + kMaxC) * kLanes * sizeof(Real);
+    return ring_lds<Real>() + static_cast<size_t>(s.slots_f + 2 + 2 * kMaxC) * kLanes * sizeof(Real);
 }
 
 template <typename Real>
 size_t backward_lds(const SymbolicLU& s)
 {
-    return static_cast<size_t>(std::max(s.slots_b, 1)) * kLanes * sizeof(Real);
+    return ring_lds<Real>() + static_cast<size_t>(s.slots_b + 2) * kLanes * sizeof(Real);
 }
 
 size_t lds_limit()
@@ -196,30 +246,41 @@ size_t lds_limit()
 }
 
 template <typename Real>
-void launch_factor(const g2g_plan_s* P, const Real* A, Real* U, const int* tile_active, const double* jmax,
-                   int* clamped, int rows, int T, double eps, hipStream_t st)
+void launch_factor(const g2g_plan_s* P, const Real* A, Real* U, int a_stride, int u_stride, const int* tile_active,
+                   const double* jmax, int* clamped, int rows, int T, double eps, hipStream_t st)
 {
     const SymbolicLU& s = P->lu();
     const size_t lds = factor_lds<Real>(s);
     if (lds > lds_limit()) throw std::runtime_error("factorization needs " + std::to_string(lds) + " bytes of LDS");
     HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_factor<Real>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    FactorArgs<Real> a{A, U, P->d_prog_f, tile_active, jmax, clamped, s.d, s.a_len, s.u_len, s.slots_f, rows, eps};
+    FactorArgs<Real> a{A, U, P->d_prog_f, tile_active, jmax, clamped, s.d, a_stride, u_stride,
+                       P->prog_f_stride, s.slots_f, rows, eps};
     k_factor<Real><<<T, kLanes, lds, st>>>(a);
     HIPCHK(hipGetLastError());
 }
 
-template <typename Real, bool Update>
-void launch_backward(const g2g_plan_s* P, const Real* U, const int* tile_active, const int* status, double2* V,
-                     Real* X, int n, int rows, int T, hipStream_t st)
+template <typename Real>
+void launch_backward(const g2g_plan_s* P, const Real* U, int u_stride, const int* tile_active, Real* X, int rows,
+                     int T, hipStream_t st)
 {
     const SymbolicLU& s = P->lu();
     const size_t lds = backward_lds<Real>(s);
     if (lds > lds_limit()) throw std::runtime_error("backward substitution needs too much LDS");
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_backward<Real, Update>),
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_backward<Real>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    BackwardArgs<Real> a{U, P->d_prog_b, tile_active, status, V, X, P->d_perm, s.d, s.u_len, n, rows};
-    k_backward<Real, Update><<<T, kLanes, lds, st>>>(a);
+    BackwardArgs<Real> a{U, P->d_prog_b, tile_active, X, P->d_perm, s.d, s.u_len, u_stride, P->prog_b_stride, rows};
+    k_backward<Real><<<T, kLanes, lds, st>>>(a);
+    HIPCHK(hipGetLastError());
+}
+
+template <typename Real>
+void launch_update(g2g_batch_s* b)
+{
+    const g2g_plan_s* P = b->plan;
+    UpdateArgs<Real> a{static_cast<const Real*>(b->X), P->d_upd, P->n_upd, b->tile_active, b->status, b->V,
+                       P->nr.d, P->nr.n, b->active_rows};
+    k_update<Real><<<b->T, kLanes * kMisWaves, 0, b->stream>>>(a);
     HIPCHK(hipGetLastError());
 }
 
@@ -228,7 +289,7 @@ void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double
 {
     const NrPlan& P = b->plan->nr;
     MismatchArgs<Real> a{b->plan->d_k1_bus, b->plan->d_k1_ent, P.k1_nbus, b->plan->d_Y, b->Yrow, b->V, b->S,
-                         static_cast<Real*>(b->A), P.n, P.nY, P.lu.a_len, b->res, b->jmax, b->status, b->iters,
+                         static_cast<Real*>(b->A), P.n, P.nY, b->a_stride, b->res, b->jmax, b->status, b->iters,
                          b->tile_active, b->n_active, b->active_rows, iter, max_iter, tol, freeze};
     const dim3 grid(b->T), block(kLanes * kMisWaves);
     if (b->per_row_y) {
@@ -273,6 +334,7 @@ void solve_impl(g2g_batch_s* b, int max_iter, double tol, int sched, g2g_solve_s
     const int e_begin = mark();
     const double eps = b->precision == G2G_MIXED ? 1e-5 : 1e-12;  // own_lu/batched_lu.cuh:50-56
     int it = 0;
+    long long rows_factored = 0;
     for (;; ++it) {
         k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
         int e0 = mark();
@@ -283,12 +345,14 @@ void solve_impl(g2g_batch_s* b, int max_iter, double tol, int sched, g2g_solve_s
         HIPCHK(hipStreamSynchronize(b->stream));
         if (*b->h_active == 0) break;
         e0 = mark();
-        launch_factor<Real>(b->plan, static_cast<const Real*>(b->A), static_cast<Real*>(b->U), b->tile_active,
-                            b->jmax, b->clamped, b->active_rows, b->T, eps, b->stream);
+        launch_factor<Real>(b->plan, static_cast<const Real*>(b->A), static_cast<Real*>(b->U), b->a_stride,
+                            b->u_stride, b->tile_active, b->jmax, b->clamped, b->active_rows, b->T, eps, b->stream);
         ph[1].push_back({e0, mark()});
+        rows_factored += *b->h_active;
         e0 = mark();
-        launch_backward<Real, true>(b->plan, static_cast<const Real*>(b->U), b->tile_active, b->status, b->V,
-                                    nullptr, b->plan->nr.n, b->active_rows, b->T, b->stream);
+        launch_backward<Real>(b->plan, static_cast<const Real*>(b->U), b->u_stride, b->tile_active,
+                              static_cast<Real*>(b->X), b->active_rows, b->T, b->stream);
+        launch_update<Real>(b);
         ph[2].push_back({e0, mark()});
     }
     const int e_end = mark();
@@ -308,6 +372,8 @@ void solve_impl(g2g_batch_s* b, int max_iter, double tol, int sched, g2g_solve_s
         const SymbolicLU& s = b->plan->lu();
         stats->bytes_factor = static_cast<int64_t>(s.a_len + s.u_len) * static_cast<int64_t>(b->real_size()) *
                               static_cast<int64_t>(b->active_rows);
+        stats->rows_factored = rows_factored;
+        stats->n_factor_launches = static_cast<int32_t>(ph[1].size());
         std::vector<int> st(static_cast<size_t>(b->active_rows));
         if (!st.empty())
             HIPCHK(hipMemcpy(st.data(), b->status, st.size() * sizeof(int), hipMemcpyDeviceToHost));
@@ -343,6 +409,53 @@ void to_tiles(const double* src, long long ld, int rows, int m, int row0, int on
 void check_rows(const g2g_batch_s* b, int rows, int row0)
 {
     if (rows < 0 || row0 < 0 || row0 + rows > b->rows) throw std::invalid_argument("row range outside the batch");
+}
+
+// Raw batched solve of `rows` matrices on a plan's pattern (host arrays).
+template <typename Real>
+void raw_solve(g2g_plan_s* P, int rows, const double* vals, const double* rhs, double* x, double pivot_eps,
+               int32_t* clamped, void* stream)
+{
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    P->upload();
+    const SymbolicLU& s = P->lu();
+    const int T = (rows + kLanes - 1) / kLanes;
+    const size_t rp = static_cast<size_t>(T) * kLanes;
+    const int nnz = P->nnz, d = s.d;
+    const int a_stride = round_up(s.a_len, chunk_of<Real>()), u_stride = round_up(s.u_len, chunk_of<Real>());
+    DeviceScratch B;
+    double* d_vals = B.get<double>(static_cast<size_t>(rows) * nnz);
+    double* d_rhs = B.get<double>(static_cast<size_t>(rows) * d);
+    Real* A = B.get<Real>(rp * a_stride);
+    Real* U = B.get<Real>(rp * u_stride);
+    Real* X = B.get<Real>(rp * d);
+    double* jm = B.get<double>(rp);
+    int* ta = B.get<int>(T);
+    int* cl = B.get<int>(rp);
+    try {
+        HIPCHK(hipMemcpyAsync(d_vals, vals, static_cast<size_t>(rows) * nnz * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_rhs, rhs, static_cast<size_t>(rows) * d * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemsetAsync(A, 0, rp * a_stride * sizeof(Real), st));
+        HIPCHK(hipMemsetAsync(jm, 0, rp * sizeof(double), st));
+        k_fill_int<<<blocks_for(T, 256), 256, 0, st>>>(ta, kLanes, T);
+        k_row_absmax<<<rows, 256, 0, st>>>(d_vals, nnz, rows, nnz, jm);
+        const long long total = static_cast<long long>(rows) * (nnz + d);
+        k_gather_astream<Real><<<blocks_for(total, 256), 256, 0, st>>>(d_vals, nnz, d_rhs, d, rows, nnz, d,
+                                                                        P->d_apos, P->d_fpos, a_stride, A);
+        HIPCHK(hipGetLastError());
+        launch_factor<Real>(P, A, U, a_stride, u_stride, ta, jm, cl, rows, T, pivot_eps, st);
+        launch_backward<Real>(P, U, u_stride, ta, X, rows, T, st);
+        std::vector<Real> hx(rp * d);
+        HIPCHK(hipMemcpyAsync(hx.data(), X, hx.size() * sizeof(Real), hipMemcpyDeviceToHost, st));
+        if (clamped) HIPCHK(hipMemcpyAsync(clamped, cl, rows * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int r = 0; r < rows; ++r)
+            for (int e = 0; e < d; ++e)
+                x[static_cast<size_t>(r) * d + e] = hx[((static_cast<size_t>(r / kLanes) * d) + e) * kLanes + (r % kLanes)];
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        throw;
+    }
 }
 
 }  // namespace
@@ -482,8 +595,13 @@ int g2g_batch_create(g2g_plan_t P, int32_t rows, int32_t precision, int32_t per_
                 P->d_Y, nr.nY, b->rows_pad, b->Yrow);
             HIPCHK(hipGetLastError());
         }
-        b->A = dalloc<unsigned char>(rp * nr.lu.a_len * b->real_size());
-        b->U = dalloc<unsigned char>(rp * nr.lu.u_len * b->real_size());
+        const int ch = precision == G2G_MIXED ? kChunkF32 : kChunkF64;
+        b->a_stride = round_up(nr.lu.a_len, ch);
+        b->u_stride = round_up(nr.lu.u_len, ch);
+        b->A = dalloc<unsigned char>(rp * b->a_stride * b->real_size());
+        b->U = dalloc<unsigned char>(rp * b->u_stride * b->real_size());
+        b->X = dalloc<unsigned char>(rp * nr.d * b->real_size());
+        HIPCHK(hipMemsetAsync(b->A, 0, rp * b->a_stride * b->real_size(), b->stream));
         b->res = dalloc<double>(rp);
         b->jmax = dalloc<double>(rp);
         b->status = dalloc<int>(rp);
@@ -626,7 +744,7 @@ int g2g_batch_eval(g2g_batch_t b, double* F, double* J, double* residual)
         if (b->precision == G2G_MIXED) launch_mismatch<float>(b, J != nullptr, 0, 1 << 30, 1e-300, 0);
         else launch_mismatch<double>(b, J != nullptr, 0, 1 << 30, 1e-300, 0);
         const size_t rs = b->real_size();
-        std::vector<unsigned char> host(rp * nr.lu.a_len * rs);
+        std::vector<unsigned char> host(rp * b->a_stride * rs);
         HIPCHK(hipMemcpyAsync(host.data(), b->A, host.size(), hipMemcpyDeviceToHost, b->stream));
         if (residual) HIPCHK(hipMemcpyAsync(residual, b->res, b->rows * sizeof(double), hipMemcpyDeviceToHost, b->stream));
         HIPCHK(hipMemcpyAsync(b->status, saved, rp * sizeof(int), hipMemcpyDeviceToDevice, b->stream));
@@ -635,7 +753,7 @@ int g2g_batch_eval(g2g_batch_t b, double* F, double* J, double* residual)
         dfree(saved);
         dfree(saved_it);
         auto at = [&](int r, int pos) -> double {
-            const size_t i = ((static_cast<size_t>(r / kLanes) * nr.lu.a_len) + pos) * kLanes + (r % kLanes);
+            const size_t i = ((static_cast<size_t>(r / kLanes) * b->a_stride) + pos) * kLanes + (r % kLanes);
             if (rs == sizeof(float)) return reinterpret_cast<const float*>(host.data())[i];
             return reinterpret_cast<const double*>(host.data())[i];
         };
@@ -654,66 +772,9 @@ int g2g_lu_solve_batch(g2g_plan_t P, int32_t rows, int32_t precision, const doub
     return guarded([&] {
         if (!P || !vals || !rhs || !x) throw std::invalid_argument("null argument");
         if (rows <= 0) throw std::invalid_argument("rows must be positive");
-        if (precision != G2G_FP64 && precision != G2G_MIXED) throw std::invalid_argument("unknown precision");
-        hipStream_t st = static_cast<hipStream_t>(stream);
-        P->upload();
-        const SymbolicLU& s = P->lu();
-        const int T = (rows + kLanes - 1) / kLanes;
-        const size_t rp = static_cast<size_t>(T) * kLanes;
-        const int nnz = P->nnz, d = s.d;
-        const size_t rs = precision == G2G_MIXED ? sizeof(float) : sizeof(double);
-        double* d_vals = dalloc<double>(static_cast<size_t>(rows) * nnz);
-        double* d_rhs = dalloc<double>(static_cast<size_t>(rows) * d);
-        double* d_x = dalloc<double>(static_cast<size_t>(rows) * d);
-        void* A = dalloc<unsigned char>(rp * s.a_len * rs);
-        void* U = dalloc<unsigned char>(rp * s.u_len * rs);
-        void* X = dalloc<unsigned char>(rp * d * rs);
-        double* jm = dalloc<double>(rp);
-        int* ta = dalloc<int>(T);
-        int* cl = dalloc<int>(rp);
-        try {
-            HIPCHK(hipMemcpyAsync(d_vals, vals, static_cast<size_t>(rows) * nnz * sizeof(double), hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(d_rhs, rhs, static_cast<size_t>(rows) * d * sizeof(double), hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemsetAsync(A, 0, rp * s.a_len * rs, st));
-            HIPCHK(hipMemsetAsync(jm, 0, rp * sizeof(double), st));
-            k_fill_int<<<blocks_for(T, 256), 256, 0, st>>>(ta, kLanes, T);
-            const long long total = static_cast<long long>(rows) * (nnz + d);
-            k_row_absmax<<<rows, 256, 0, st>>>(d_vals, nnz, rows, nnz, jm);
-            if (precision == G2G_MIXED) {
-                k_gather_astream<float><<<blocks_for(total, 256), 256, 0, st>>>(
-                    d_vals, nnz, d_rhs, d, rows, nnz, d, P->d_apos, P->d_fpos, s.a_len, static_cast<float*>(A));
-                launch_factor<float>(P, static_cast<const float*>(A), static_cast<float*>(U), ta, jm, cl, rows, T,
-                                     pivot_eps, st);
-                launch_backward<float, false>(P, static_cast<const float*>(U), ta, nullptr, nullptr,
-                                              static_cast<float*>(X), 0, rows, T, st);
-                // widen the solution
-                std::vector<float> hx(rp * d);
-                HIPCHK(hipMemcpyAsync(hx.data(), X, hx.size() * sizeof(float), hipMemcpyDeviceToHost, st));
-                HIPCHK(hipStreamSynchronize(st));
-                for (int r = 0; r < rows; ++r)
-                    for (int e = 0; e < d; ++e)
-                        x[static_cast<size_t>(r) * d + e] =
-                            hx[((static_cast<size_t>(r / kLanes) * d) + e) * kLanes + (r % kLanes)];
-            } else {
-                k_gather_astream<double><<<blocks_for(total, 256), 256, 0, st>>>(
-                    d_vals, nnz, d_rhs, d, rows, nnz, d, P->d_apos, P->d_fpos, s.a_len, static_cast<double*>(A));
-                launch_factor<double>(P, static_cast<const double*>(A), static_cast<double*>(U), ta, jm, cl, rows, T,
-                                      pivot_eps, st);
-                launch_backward<double, false>(P, static_cast<const double*>(U), ta, nullptr, nullptr,
-                                               static_cast<double*>(X), 0, rows, T, st);
-                k_tiles_to_rows<double><<<blocks_for(static_cast<long long>(rows) * d, 256), 256, 0, st>>>(
-                    static_cast<const double*>(X), rows, d, d_x, d);
-                HIPCHK(hipGetLastError());
-                HIPCHK(hipMemcpyAsync(x, d_x, static_cast<size_t>(rows) * d * sizeof(double), hipMemcpyDeviceToHost, st));
-            }
-            if (clamped) HIPCHK(hipMemcpyAsync(clamped, cl, rows * sizeof(int), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-        } catch (...) {
-            (void)hipStreamSynchronize(st);
-            dfree(d_vals); dfree(d_rhs); dfree(d_x); dfree(A); dfree(U); dfree(X); dfree(jm); dfree(ta); dfree(cl);
-            throw;
-        }
-        dfree(d_vals); dfree(d_rhs); dfree(d_x); dfree(A); dfree(U); dfree(X); dfree(jm); dfree(ta); dfree(cl);
+        if (precision == G2G_MIXED) raw_solve<float>(P, rows, vals, rhs, x, pivot_eps, clamped, stream);
+        else if (precision == G2G_FP64) raw_solve<double>(P, rows, vals, rhs, x, pivot_eps, clamped, stream);
+        else throw std::invalid_argument("unknown precision");
     });
 }
 

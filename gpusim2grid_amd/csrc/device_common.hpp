@@ -16,8 +16,8 @@ struct MismatchArgs {
     const double2* Yrow;    // per-row Ybus values [T][nY][64] (outage batches)
     const double2* V;       // [T][n][64]
     const double2* S;       // [T][n][64]
-    Real* A;                // A stream [T][a_len][64]
-    int n, nY, a_len;
+    Real* A;                // A stream [T][a_stride][64]
+    int n, nY, a_stride;
     double* res;            // [rows_pad] ||F||inf
     double* jmax;           // [rows_pad] max |J| (pivot clamp scale)
     int* status;            // [rows_pad]
@@ -32,26 +32,35 @@ struct MismatchArgs {
 
 template <typename Real>
 struct FactorArgs {
-    const Real* A;
-    Real* U;                // U stream [T][u_len][64]
-    const int* prog;
+    const Real* A;          // A stream [T][a_stride][64]
+    Real* U;                // U stream [T][u_stride][64]
+    const int* prog;        // padded to prog_stride words
     const int* tile_active;
     const double* jmax;     // nullable: scale 1
     int* clamped;           // nullable: clamped pivots per row
-    int d, a_len, u_len, slots, rows;
+    int d, a_stride, u_stride, prog_stride, slots, rows;
     double pivot_eps;
 };
 
 template <typename Real>
 struct BackwardArgs {
     const Real* U;
-    const int* prog;
+    const int* prog;        // padded to prog_stride words
     const int* tile_active;
-    const int* status;      // NR mode: only active rows update V
-    double2* V;             // NR mode
-    Real* X;                // raw mode: solution [T][d][64] in input labels
-    const int* perm;        // raw mode: pivot -> input index
-    int d, u_len, n, rows;
+    Real* X;                // solution [T][d][64] in input labels
+    const int* perm;        // pivot -> input index
+    int d, u_len, u_stride, prog_stride, rows;
+};
+
+template <typename Real>
+struct UpdateArgs {
+    const Real* X;          // solved step [T][d][64] (reference unknown labels)
+    const int* upd;         // per bus {bus, theta unknown, |V| unknown or -1}
+    int nbus;
+    const int* tile_active;
+    const int* status;
+    double2* V;
+    int d, n, rows;
 };
 
 }  // namespace g2g

@@ -16,13 +16,19 @@
 //                   LDS slots, forward substitution fused (the reference's
 //                   own_lu/batched_lu.cu:56-84 left-looking items and
 //                   :86-98 forward solve).
-//   k_backward      U x = y in reverse postorder with x(j) kept in LDS slots;
-//                   the polar V update is fused (acpf_nr_kernels.cu:728-788).
+//   k_backward      U x = y in reverse postorder with x(j) kept in LDS slots
+//                   (own_lu/batched_lu.cu:100-109's backward solve).
+//   k_update        polar V update of every bus (acpf_nr_kernels.cu:728-788).
+//
+// k_factor and k_backward read their program and value streams through LDS
+// rings filled ahead of use (streams.hpp): a wavefront's walk is sequential,
+// so its HBM traffic is a pure prefetched stream.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 
 #include "device_common.hpp"
+#include "streams.hpp"
 
 namespace g2g {
 
@@ -103,10 +109,10 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_mismatch(MismatchArgs<Rea
                     const double inv = 1.0 / sqrt(fma(vj.x, vj.x, vj.y * vj.y));
                     const int a11 = en[2], a12 = en[3], a21 = en[4], a22 = en[5];
                     const double j11 = w.y, j21 = -w.x, j12 = w.x * inv, j22 = w.y * inv;
-                    if (a11 >= 0) a.A[tidx(tile, a.a_len, a11, lane)] = static_cast<Real>(j11);
-                    if (a12 >= 0) a.A[tidx(tile, a.a_len, a12, lane)] = static_cast<Real>(j12);
-                    if (a21 >= 0) a.A[tidx(tile, a.a_len, a21, lane)] = static_cast<Real>(j21);
-                    if (a22 >= 0) a.A[tidx(tile, a.a_len, a22, lane)] = static_cast<Real>(j22);
+                    if (a11 >= 0) a.A[tidx(tile, a.a_stride, a11, lane)] = static_cast<Real>(j11);
+                    if (a12 >= 0) a.A[tidx(tile, a.a_stride, a12, lane)] = static_cast<Real>(j12);
+                    if (a21 >= 0) a.A[tidx(tile, a.a_stride, a21, lane)] = static_cast<Real>(j21);
+                    if (a22 >= 0) a.A[tidx(tile, a.a_stride, a22, lane)] = static_cast<Real>(j22);
                     jmax = fmax(jmax, fmax(fmax(fabs(j11), fabs(j12)), fmax(fabs(j21), fabs(j22))));
                 }
             }
@@ -115,11 +121,11 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_mismatch(MismatchArgs<Rea
             const double2 sb = S[tidx(tile, a.n, i, lane)];
             const double fpv = -(sc.x - sb.x);
             rmax = nan_max(rmax, fpv);
-            a.A[tidx(tile, a.a_len, fp, lane)] = static_cast<Real>(fpv);
+            a.A[tidx(tile, a.a_stride, fp, lane)] = static_cast<Real>(fpv);
             if (fq >= 0) {
                 const double fqv = -(sc.y - sb.y);
                 rmax = nan_max(rmax, fqv);
-                a.A[tidx(tile, a.a_len, fq, lane)] = static_cast<Real>(fqv);
+                a.A[tidx(tile, a.a_stride, fq, lane)] = static_cast<Real>(fqv);
             }
             if (WithJ) {
                 const int* en = a.k1_ent + 6 * dent;
@@ -130,10 +136,10 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_mismatch(MismatchArgs<Rea
                 const double inv = 1.0 / sqrt(fma(vi.x, vi.x, vi.y * vi.y));
                 const double j11 = -z.y, j21 = z.x, j12 = u.x * inv, j22 = u.y * inv;
                 const int a11 = en[2], a12 = en[3], a21 = en[4], a22 = en[5];
-                if (a11 >= 0) a.A[tidx(tile, a.a_len, a11, lane)] = static_cast<Real>(j11);
-                if (a12 >= 0) a.A[tidx(tile, a.a_len, a12, lane)] = static_cast<Real>(j12);
-                if (a21 >= 0) a.A[tidx(tile, a.a_len, a21, lane)] = static_cast<Real>(j21);
-                if (a22 >= 0) a.A[tidx(tile, a.a_len, a22, lane)] = static_cast<Real>(j22);
+                if (a11 >= 0) a.A[tidx(tile, a.a_stride, a11, lane)] = static_cast<Real>(j11);
+                if (a12 >= 0) a.A[tidx(tile, a.a_stride, a12, lane)] = static_cast<Real>(j12);
+                if (a21 >= 0) a.A[tidx(tile, a.a_stride, a21, lane)] = static_cast<Real>(j21);
+                if (a22 >= 0) a.A[tidx(tile, a.a_stride, a22, lane)] = static_cast<Real>(j22);
                 jmax = fmax(jmax, fmax(fmax(fabs(j11), fabs(j12)), fmax(fabs(j21), fabs(j22))));
             }
         }
@@ -177,149 +183,336 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_mismatch(MismatchArgs<Rea
 // ---------------------------------------------------------------------------
 
 template <typename Real>
-__global__ __launch_bounds__(kLanes) void k_factor(FactorArgs<Real> a)
+struct StreamShape;
+template <>
+struct StreamShape<double> {
+    static constexpr int kCh = kChunkF64, kNch = kRingSlots;
+};
+template <>
+struct StreamShape<float> {
+    static constexpr int kCh = kChunkF32, kNch = kRingSlots;
+};
+using ProgStream = RingStream<int, kProgChunk, kProgSlots, false, false>;
+
+// 16-bit field q of a record whose refs start at word `head`
+template <int Head, int NW>
+__device__ __forceinline__ unsigned ref_of(const v4u (&w)[NW / 4], int q)
 {
+    const int word = Head + q / 2;
+    const unsigned v = w[word / 4][word % 4];
+    return (q & 1) ? (v >> 16) : (v & 0xFFFFu);
+}
+
+template <int NW>
+__device__ __forceinline__ void read_record(const ProgStream& P, int p, v4u (&w)[NW / 4])
+{
+    const v4u* base = reinterpret_cast<const v4u*>(P.ring);
+#pragma unroll
+    for (int i = 0; i < NW / 4; ++i) w[i] = base[((p >> 2) + i) & (ProgStream::kRing / 4 - 1)];
+}
+
+template <typename Real>
+struct FactorState {
+    Real* sl;        // slot row 0 of this lane: slot s at sl[s * 64]
+    Real* U;         // U stream of this lane
+    Real eps;
+    int clamped;
+    int p, ap, up;
+};
+
+// One pivot with C off-diagonal entries: every slot read of the pivot is
+// issued before any of its slot writes (sources, then old values of the
+// update targets), so the wavefront waits on LDS once per pivot.
+template <int C, typename Real, typename AS>
+__device__ __forceinline__ void factor_pivot(AS& A, ProgStream& P, FactorState<Real>& st)
+{
+    constexpr int NW = frec_words(C);
+    P.ensure(st.p + NW - 1);
+    v4u w[NW / 4];
+    read_record<NW>(P, st.p, w);
+    A.ensure(st.ap);
+    const Real* ring = A.ring + A.lane;
+    auto aval = [&](int i) { return ring[static_cast<size_t>((st.ap + i) & (AS::kRing - 1)) * kLanes]; };
+    auto slot = [&](unsigned ref) -> Real& { return st.sl[static_cast<size_t>(ref & kRefSlot) * kLanes]; };
+    Real dv = aval(1) + slot(ref_of<1, NW>(w, 0));
+    Real fk = aval(0) + slot(ref_of<1, NW>(w, 1));
+    Real lv[C > 0 ? C : 1], uv[C > 0 ? C : 1];
+#pragma unroll
+    for (int q = 0; q < C; ++q) lv[q] = aval(2 + q) + slot(ref_of<1, NW>(w, 2 + q));
+#pragma unroll
+    for (int q = 0; q < C; ++q) uv[q] = aval(2 + C + q) + slot(ref_of<1, NW>(w, 2 + C + q));
+    // old values of the first update row and of the right-hand-side targets
+    Real t0[C > 0 ? C : 1], t1[C > 0 ? C : 1], tb[C > 0 ? C : 1];
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+        const unsigned ref = ref_of<1, NW>(w, 2 + 2 * C + q);
+        const Real old = slot(ref);
+        t0[q] = (ref & kRefFirst) ? Real(0) : old;
+    }
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+        const unsigned ref = ref_of<1, NW>(w, 2 + 2 * C + C * C + q);
+        const Real old = slot(ref);
+        tb[q] = (ref & kRefFirst) ? Real(0) : old;
+    }
+    st.ap += 2 + 2 * C;
+    if (fabs(dv) < st.eps) {  // static-pivot clamp (own_lu/batched_lu.cu:45-53); NaN passes
+        dv = dv < Real(0) ? -st.eps : st.eps;
+        ++st.clamped;
+    }
+    const Real inv = Real(1) / dv;
+#pragma unroll
+    for (int q = 0; q < C; ++q) lv[q] *= inv;
+    // U record: y(k), U(k,k), U(k,j)...
+    st.U[static_cast<size_t>(st.up) * kLanes] = fk;
+    st.U[static_cast<size_t>(st.up + 1) * kLanes] = dv;
+#pragma unroll
+    for (int q = 0; q < C; ++q) st.U[static_cast<size_t>(st.up + 2 + q) * kLanes] = uv[q];
+    st.up += 2 + C;
+    // update rows, software-pipelined: row a+1's old values are read before
+    // row a is written (all targets are distinct slots)
+#pragma unroll
+    for (int a = 0; a < C; ++a) {
+        Real(&cur)[C > 0 ? C : 1] = (a & 1) ? t1 : t0;
+        Real(&nxt)[C > 0 ? C : 1] = (a & 1) ? t0 : t1;
+        if (a + 1 < C) {
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const unsigned ref = ref_of<1, NW>(w, 2 + 2 * C + (a + 1) * C + q);
+                const Real old = slot(ref);
+                nxt[q] = (ref & kRefFirst) ? Real(0) : old;
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < C; ++b) slot(ref_of<1, NW>(w, 2 + 2 * C + a * C + b)) = fma(-lv[a], uv[b], cur[b]);
+    }
+#pragma unroll
+    for (int a = 0; a < C; ++a) slot(ref_of<1, NW>(w, 2 + 2 * C + C * C + a)) = fma(-lv[a], fk, tb[a]);
+    st.p += NW;
+}
+
+// Any column length (up to kMaxC), one field at a time: the rare long columns.
+template <typename Real, typename AS>
+__device__ void factor_pivot_any(int c, AS& A, ProgStream& P, FactorState<Real>& st, Real* lsc, Real* usc)
+{
+    const int nw = frec_words(c);
+    P.ensure(st.p + nw - 1);
+    auto ref = [&](int q) -> unsigned {
+        const unsigned v = static_cast<unsigned>(P.ring[(st.p + 1 + q / 2) & (ProgStream::kRing - 1)]);
+        return (q & 1) ? (v >> 16) : (v & 0xFFFFu);
+    };
+    auto slot = [&](unsigned r) -> Real& { return st.sl[static_cast<size_t>(r & kRefSlot) * kLanes]; };
+    A.ensure(st.ap);
+    const Real* ring = A.ring + A.lane;
+    auto aval = [&](int i) { return ring[static_cast<size_t>((st.ap + i) & (AS::kRing - 1)) * kLanes]; };
+    Real dv = aval(1) + slot(ref(0));
+    const Real fk = aval(0) + slot(ref(1));
+    for (int q = 0; q < c; ++q) lsc[q * kLanes] = aval(2 + q) + slot(ref(2 + q));
+    for (int q = 0; q < c; ++q) usc[q * kLanes] = aval(2 + c + q) + slot(ref(2 + c + q));
+    st.ap += 2 + 2 * c;
+    if (fabs(dv) < st.eps) {
+        dv = dv < Real(0) ? -st.eps : st.eps;
+        ++st.clamped;
+    }
+    const Real inv = Real(1) / dv;
+    for (int q = 0; q < c; ++q) lsc[q * kLanes] *= inv;
+    st.U[static_cast<size_t>(st.up) * kLanes] = fk;
+    st.U[static_cast<size_t>(st.up + 1) * kLanes] = dv;
+    for (int q = 0; q < c; ++q) st.U[static_cast<size_t>(st.up + 2 + q) * kLanes] = usc[q * kLanes];
+    st.up += 2 + c;
+    for (int a = 0; a < c; ++a)
+        for (int b = 0; b < c; ++b) {
+            const unsigned r = ref(2 + 2 * c + a * c + b);
+            const Real old = (r & kRefFirst) ? Real(0) : slot(r);
+            slot(r) = fma(-lsc[a * kLanes], usc[b * kLanes], old);
+        }
+    for (int a = 0; a < c; ++a) {
+        const unsigned r = ref(2 + 2 * c + c * c + a);
+        const Real old = (r & kRefFirst) ? Real(0) : slot(r);
+        slot(r) = fma(-lsc[a * kLanes], fk, old);
+    }
+    st.p += nw;
+}
+
+template <int C, typename Real, typename AS>
+__device__ __forceinline__ bool factor_dispatch(int c, AS& A, ProgStream& P, FactorState<Real>& st)
+{
+    if constexpr (C > kFastC) {
+        return false;
+    } else {
+        if (c == C) {
+            factor_pivot<C>(A, P, st);
+            return true;
+        }
+        return factor_dispatch<C + 1>(c, A, P, st);
+    }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_factor(FactorArgs<Real> a)
+{
+    using Sh = StreamShape<Real>;
+    using AStream = RingStream<Real, Sh::kCh, Sh::kNch, false, true>;
     const int lane = threadIdx.x;
     const size_t tile = blockIdx.x;
     if (a.tile_active[tile] <= 0) return;
     const int row = static_cast<int>(tile) * kLanes + lane;
-    extern __shared__ unsigned char g2g_smem[];
-    Real* sl = reinterpret_cast<Real*>(g2g_smem);          // [slots][64]
-    Real* lsc = sl + static_cast<size_t>(a.slots) * kLanes;  // [kMaxC][64] L column scratch
-    const Real* A = a.A + tile * static_cast<size_t>(a.a_len) * kLanes + lane;
-    Real* U = a.U + tile * static_cast<size_t>(a.u_len) * kLanes + lane;
-    const int* prog = a.prog;
-    const Real eps = static_cast<Real>(a.pivot_eps) * static_cast<Real>(a.jmax ? a.jmax[row] : 1.0);
-    int clamped = 0;
-    int p = 0, ap = 0, up = 0;
+    extern __shared__ __align__(16) unsigned char g2g_smem[];
+    Real* aring = reinterpret_cast<Real*>(g2g_smem);
+    int* pring = reinterpret_cast<int*>(aring + AStream::kRing * kLanes);
+    Real* sl = reinterpret_cast<Real*>(pring + ProgStream::kRing);  // [slots + 2][64]
+    Real* lsc = sl + static_cast<size_t>(a.slots + 2) * kLanes;     // [kMaxC][64]
+    Real* usc = lsc + static_cast<size_t>(kMaxC) * kLanes;          // [kMaxC][64]
+    sl[static_cast<size_t>(a.slots) * kLanes + lane] = Real(0);     // the zero slot
+    AStream A;
+    A.init(a.A + tile * static_cast<size_t>(a.a_stride) * kLanes, a.a_stride / Sh::kCh, aring, lane);
+    ProgStream P;
+    P.init(a.prog, a.prog_stride / kProgChunk, pring, lane);
+    FactorState<Real> st;
+    st.sl = sl + lane;
+    st.U = a.U + tile * static_cast<size_t>(a.u_stride) * kLanes + lane;
+    st.eps = static_cast<Real>(a.pivot_eps) * static_cast<Real>(a.jmax ? a.jmax[row] : 1.0);
+    st.clamped = 0;
+    st.p = st.ap = st.up = 0;
+    int c = prog_at(P, 0) & 0xFFFF;
     for (int k = 0; k < a.d; ++k) {
-        const int c = prog[p];
-        Real fk = A[static_cast<size_t>(ap++) * kLanes];
-        const int bref = prog[p + 1];
-        if (bref != kNoSlot) fk += sl[bref * kLanes + lane];
-        const int dref = prog[p + 2];
-        Real dv = Real(0);
-        if (dref & kInA) dv = A[static_cast<size_t>(ap++) * kLanes];
-        if ((dref & kSlotMask) != kNoSlot) dv += sl[(dref & kSlotMask) * kLanes + lane];
-        if (fabs(dv) < eps) {  // static-pivot clamp (own_lu/batched_lu.cu:45-53); NaN passes
-            dv = dv < Real(0) ? -eps : eps;
-            ++clamped;
-        }
-        const Real inv = Real(1) / dv;
-        p += 3;
-        Real u[kMaxC];
-#pragma unroll
-        for (int q = 0; q < kMaxC; ++q) {
-            if (q < c) {
-                const int ref = prog[p + q];
-                Real v = Real(0);
-                if (ref & kInA) v = A[static_cast<size_t>(ap++) * kLanes];
-                if ((ref & kSlotMask) != kNoSlot) v += sl[(ref & kSlotMask) * kLanes + lane];
-                lsc[q * kLanes + lane] = v * inv;
-            }
-        }
-        p += c;
-#pragma unroll
-        for (int q = 0; q < kMaxC; ++q) {
-            u[q] = Real(0);
-            if (q < c) {
-                const int ref = prog[p + q];
-                Real v = Real(0);
-                if (ref & kInA) v = A[static_cast<size_t>(ap++) * kLanes];
-                if ((ref & kSlotMask) != kNoSlot) v += sl[(ref & kSlotMask) * kLanes + lane];
-                u[q] = v;
-            }
-        }
-        p += c;
-        // U record: y(k), U(k,k), U(k,j)...
-        U[static_cast<size_t>(up++) * kLanes] = fk;
-        U[static_cast<size_t>(up++) * kLanes] = dv;
-#pragma unroll
-        for (int q = 0; q < kMaxC; ++q)
-            if (q < c) U[static_cast<size_t>(up++) * kLanes] = u[q];
-        // rank-1 update of the trailing entries, then of the right-hand side
-        for (int r = 0; r < c; ++r) {
-            const Real l = lsc[r * kLanes + lane];
-#pragma unroll
-            for (int q = 0; q < kMaxC; ++q) {
-                if (q < c) {
-                    const int ref = prog[p + q];
-                    Real* t = sl + (ref & kSlotMask) * kLanes + lane;
-                    const Real v = -l * u[q];
-                    if (ref & kFirst) *t = v;
-                    else *t += v;
-                }
-            }
-            p += c;
-        }
-        for (int r = 0; r < c; ++r) {
-            const int ref = prog[p + r];
-            Real* t = sl + (ref & kSlotMask) * kLanes + lane;
-            const Real v = -lsc[r * kLanes + lane] * fk;
-            if (ref & kFirst) *t = v;
-            else *t += v;
-        }
-        p += c;
+        // the header of this record announces the next record's column length
+        const int c_next = (prog_at(P, st.p) >> 16) & 0xFFFF;
+        if (!factor_dispatch<0>(c, A, P, st)) factor_pivot_any(c, A, P, st, lsc + lane, usc + lane);
+        c = c_next;
     }
-    if (a.clamped && row < a.rows) a.clamped[row] = clamped;
+    if (a.clamped && row < a.rows) a.clamped[row] = st.clamped;
 }
 
 // ---------------------------------------------------------------------------
-// k_backward: U x = y, reverse postorder; fused polar update or raw output
+// k_backward: U x = y in reverse postorder; x written in input labels
 // ---------------------------------------------------------------------------
 
-template <typename Real, bool Update>
-__global__ __launch_bounds__(kLanes) void k_backward(BackwardArgs<Real> a)
+template <typename Real>
+struct BackState {
+    Real* sl;
+    Real* X;
+    int p, up;
+};
+
+template <int C, typename Real, typename US>
+__device__ __forceinline__ void backward_pivot(US& U, ProgStream& P, BackState<Real>& st)
 {
+    constexpr int NW = brec_words(C);
+    P.ensure(st.p + NW - 1);
+    v4u w[NW / 4];
+    read_record<NW>(P, st.p, w);
+    st.up -= C + 2;
+    U.ensure(st.up + C + 1);  // the record is consumed from its top
+    const Real* ring = U.ring + U.lane;
+    auto uval = [&](int i) { return ring[static_cast<size_t>((st.up + i) & (US::kRing - 1)) * kLanes]; };
+    auto slot = [&](unsigned ref) -> Real& { return st.sl[static_cast<size_t>(ref & kRefSlot) * kLanes]; };
+    Real xj[C > 0 ? C : 1];
+#pragma unroll
+    for (int q = 0; q < C; ++q) xj[q] = slot(ref_of<2, NW>(w, q));
+    Real s = uval(0);
+#pragma unroll
+    for (int q = 0; q < C; ++q) s = fma(-uval(2 + q), xj[q], s);
+    const Real x = s / uval(1);
+    const int perm_k = static_cast<int>(w[0][1]);
+    st.X[static_cast<size_t>(perm_k) * kLanes] = x;
+    slot(ref_of<2, NW>(w, C)) = x;
+    st.p += NW;
+}
+
+template <typename Real, typename US>
+__device__ void backward_pivot_any(int c, US& U, ProgStream& P, BackState<Real>& st)
+{
+    const int nw = brec_words(c);
+    P.ensure(st.p + nw - 1);
+    auto ref = [&](int q) -> unsigned {
+        const unsigned v = static_cast<unsigned>(P.ring[(st.p + 2 + q / 2) & (ProgStream::kRing - 1)]);
+        return (q & 1) ? (v >> 16) : (v & 0xFFFFu);
+    };
+    auto slot = [&](unsigned r) -> Real& { return st.sl[static_cast<size_t>(r & kRefSlot) * kLanes]; };
+    st.up -= c + 2;
+    U.ensure(st.up + c + 1);
+    const Real* ring = U.ring + U.lane;
+    auto uval = [&](int i) { return ring[static_cast<size_t>((st.up + i) & (US::kRing - 1)) * kLanes]; };
+    Real s = uval(0);
+    for (int q = 0; q < c; ++q) s = fma(-uval(2 + q), slot(ref(q)), s);
+    const Real x = s / uval(1);
+    const int perm_k = P.ring[(st.p + 1) & (ProgStream::kRing - 1)];
+    st.X[static_cast<size_t>(perm_k) * kLanes] = x;
+    slot(ref(c)) = x;
+    st.p += nw;
+}
+
+template <int C, typename Real, typename US>
+__device__ __forceinline__ bool backward_dispatch(int c, US& U, ProgStream& P, BackState<Real>& st)
+{
+    if constexpr (C > kFastC) {
+        return false;
+    } else {
+        if (c == C) {
+            backward_pivot<C>(U, P, st);
+            return true;
+        }
+        return backward_dispatch<C + 1>(c, U, P, st);
+    }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_backward(BackwardArgs<Real> a)
+{
+    using Sh = StreamShape<Real>;
+    using UStream = RingStream<Real, Sh::kCh, Sh::kNch, true, true>;
     const int lane = threadIdx.x;
     const size_t tile = blockIdx.x;
     if (a.tile_active[tile] <= 0) return;
+    extern __shared__ __align__(16) unsigned char g2g_smem[];
+    Real* uring = reinterpret_cast<Real*>(g2g_smem);
+    int* pring = reinterpret_cast<int*>(uring + UStream::kRing * kLanes);
+    Real* sl = reinterpret_cast<Real*>(pring + ProgStream::kRing);  // [slots + 2][64]
+    UStream U;
+    U.init(a.U + tile * static_cast<size_t>(a.u_stride) * kLanes, a.u_stride / Sh::kCh, uring, lane);
+    ProgStream P;
+    P.init(a.prog, a.prog_stride / kProgChunk, pring, lane);
+    BackState<Real> st;
+    st.sl = sl + lane;
+    st.X = a.X + tile * static_cast<size_t>(a.d) * kLanes + lane;
+    st.p = 0;
+    st.up = a.u_len;
+    int c = prog_at(P, 0) & 0xFFFF;
+    for (int q = 0; q < a.d; ++q) {
+        const int c_next = (prog_at(P, st.p) >> 16) & 0xFFFF;
+        if (!backward_dispatch<0>(c, U, P, st)) backward_pivot_any(c, U, P, st);
+        c = c_next;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_update: polar V update of every bus from the solved step (one lane per row)
+//   V <- V ((|V| + dVm) / |V|) exp(j dVa)   (acpf_nr_kernels.cu:728-788)
+// ---------------------------------------------------------------------------
+
+template <typename Real>
+__global__ __launch_bounds__(kLanes* kMisWaves) void k_update(UpdateArgs<Real> a)
+{
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = threadIdx.x / kLanes;
+    const size_t tile = blockIdx.x;
+    if (a.tile_active[tile] <= 0) return;
     const int row = static_cast<int>(tile) * kLanes + lane;
-    const bool live = row < a.rows && (!Update || a.status[row] == kActive);
-    extern __shared__ unsigned char g2g_smem[];
-    Real* sl = reinterpret_cast<Real*>(g2g_smem);  // [slots][64]
-    const Real* U = a.U + tile * static_cast<size_t>(a.u_len) * kLanes + lane;
-    const int* prog = a.prog;
-    int p = 0, up = a.u_len;
-    for (int k = a.d - 1; k >= 0; --k) {
-        const int c = prog[p];
-        up -= c + 2;
-        Real s = U[static_cast<size_t>(up) * kLanes];
-        const Real piv = U[static_cast<size_t>(up + 1) * kLanes];
-        for (int q = 0; q < c; ++q) {
-            const Real xj = sl[prog[p + 1 + q] * kLanes + lane];
-            s = fma(-U[static_cast<size_t>(up + 2 + q) * kLanes], xj, s);
-        }
-        p += 1 + c;
-        const Real x = s / piv;
-        const int out = prog[p], op = prog[p + 1], bus = prog[p + 2], ps = prog[p + 3];
-        p += 4;
-        if (Update) {
-            if (op != 0) {
-                double dva, dvm = 0.0;
-                if (op == kOpThetaSelf) {
-                    dva = static_cast<double>(x);
-                } else if (op == kOpThetaSelfVmSlot) {
-                    dva = static_cast<double>(x);
-                    dvm = static_cast<double>(sl[ps * kLanes + lane]);
-                } else {
-                    dvm = static_cast<double>(x);
-                    dva = static_cast<double>(sl[ps * kLanes + lane]);
-                }
-                if (live) {
-                    double2* vp = a.V + tidx(tile, a.n, bus, lane);
-                    const double2 v = *vp;
-                    const double m = sqrt(fma(v.x, v.x, v.y * v.y));
-                    const double sc = (m + dvm) / m;
-                    double sn, cs;
-                    sincos(dva, &sn, &cs);
-                    *vp = make_double2(sc * fma(v.x, cs, -v.y * sn), sc * fma(v.x, sn, v.y * cs));
-                }
-            }
-        } else {
-            if (row < a.rows) a.X[tidx(tile, a.d, a.perm[k], lane)] = x;
-        }
-        if (out != kNoSlot) sl[out * kLanes + lane] = x;
+    const bool live = row < a.rows && a.status[row] == kActive;
+    const int b0 = static_cast<int>((static_cast<long long>(a.nbus) * wave) / kMisWaves);
+    const int b1 = static_cast<int>((static_cast<long long>(a.nbus) * (wave + 1)) / kMisWaves);
+    for (int bb = b0; bb < b1; ++bb) {
+        const int bus = a.upd[3 * bb], kt = a.upd[3 * bb + 1], kv = a.upd[3 * bb + 2];
+        const double dva = static_cast<double>(a.X[tidx(tile, a.d, kt, lane)]);
+        const double dvm = kv >= 0 ? static_cast<double>(a.X[tidx(tile, a.d, kv, lane)]) : 0.0;
+        double2* vp = a.V + tidx(tile, a.n, bus, lane);
+        const double2 v = *vp;
+        const double m = sqrt(fma(v.x, v.x, v.y * v.y));
+        const double sc = (m + dvm) / m;
+        double sn, cs;
+        sincos(dva, &sn, &cs);
+        if (live) *vp = make_double2(sc * fma(v.x, cs, -v.y * sn), sc * fma(v.x, sn, v.y * cs));
     }
 }
 
@@ -385,7 +578,7 @@ __global__ void k_ybus_edits(const int* __restrict__ row_ptr, const int* __restr
 template <typename Real>
 __global__ void k_gather_astream(const double* __restrict__ vals, long long ldv, const double* __restrict__ rhs,
                                  long long ldb, int rows, int nnz, int d, const int* __restrict__ a_pos,
-                                 const int* __restrict__ f_pos, int a_len, Real* __restrict__ A)
+                                 const int* __restrict__ f_pos, int a_stride, Real* __restrict__ A)
 {
     const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const long long per = static_cast<long long>(nnz) + d;
@@ -397,10 +590,10 @@ __global__ void k_gather_astream(const double* __restrict__ vals, long long ldv,
     const int lane = r % kLanes;
     if (k < nnz) {
         const int pos = a_pos[k];
-        if (pos >= 0) A[tidx(tile, a_len, pos, lane)] = static_cast<Real>(vals[static_cast<long long>(r) * ldv + k]);
+        if (pos >= 0) A[tidx(tile, a_stride, pos, lane)] = static_cast<Real>(vals[static_cast<long long>(r) * ldv + k]);
     } else {
         const int e = k - nnz;
-        A[tidx(tile, a_len, f_pos[e], lane)] = static_cast<Real>(rhs[static_cast<long long>(r) * ldb + e]);
+        A[tidx(tile, a_stride, f_pos[e], lane)] = static_cast<Real>(rhs[static_cast<long long>(r) * ldb + e]);
     }
 }
 
@@ -451,10 +644,10 @@ template __global__ void k_mismatch<float, false, false>(MismatchArgs<float>);
 template __global__ void k_mismatch<float, true, false>(MismatchArgs<float>);
 template __global__ void k_factor<double>(FactorArgs<double>);
 template __global__ void k_factor<float>(FactorArgs<float>);
-template __global__ void k_backward<double, true>(BackwardArgs<double>);
-template __global__ void k_backward<double, false>(BackwardArgs<double>);
-template __global__ void k_backward<float, true>(BackwardArgs<float>);
-template __global__ void k_backward<float, false>(BackwardArgs<float>);
+template __global__ void k_backward<double>(BackwardArgs<double>);
+template __global__ void k_backward<float>(BackwardArgs<float>);
+template __global__ void k_update<double>(UpdateArgs<double>);
+template __global__ void k_update<float>(UpdateArgs<float>);
 template __global__ void k_rows_to_tiles<double2>(const double2*, long long, int, int, int, double2*, int);
 template __global__ void k_rows_to_tiles<double>(const double*, long long, int, int, int, double*, int);
 template __global__ void k_tiles_to_rows<double2>(const double2*, int, int, double2*, long long);

@@ -175,8 +175,7 @@ struct SlotAlloc {
 
 inline long long key(int i, int j, int d) { return static_cast<long long>(i) * d + j; }
 
-void build_backward(SymbolicLU& s, const std::vector<int>& bus_of, const std::vector<int>& kind_of,
-                    const std::vector<int>& partner);
+void build_backward(SymbolicLU& s);
 
 }  // namespace
 
@@ -284,67 +283,61 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
     s.nA = 0;
 
     // ---- factorization + forward program -----------------------------------
+    // Slots are assigned first (refs as slot | kFirst, -1 = none), then packed
+    // into 16-bit fields once the slot count -- the zero slot's index -- is known.
     SlotAlloc sa;
     std::unordered_map<long long, int> slot_of;  // live update targets
     std::vector<int> bslot(static_cast<size_t>(d), -1);
-    s.prog_f_off.assign(static_cast<size_t>(d) + 1, 0);
+    std::vector<std::vector<int>> refs(static_cast<size_t>(d));
     s.u_off.assign(static_cast<size_t>(d) + 1, 0);
     int apos = 0, upos = 0;
-    auto take = [&](int i, int j) -> int {  // slot of a consumed entry (freed), or kNoSlot
+    auto take = [&](int i, int j) -> int {  // slot of a consumed entry (freed), or -1
         auto it = slot_of.find(key(i, j, d));
-        if (it == slot_of.end()) return kNoSlot;
+        if (it == slot_of.end()) return -1;
         const int sl = it->second;
         slot_of.erase(it);
         sa.put(sl);
         return sl;
     };
-    auto a_present = [&](int i, int j) -> int {
+    auto place = [&](int i, int j) {  // dense A stream: every entry has a position
         auto it = a_entry.find(key(i, j, d));
-        return it == a_entry.end() ? -1 : it->second;
+        if (it != a_entry.end()) {
+            s.a_pos_of_entry[it->second] = apos;
+            ++s.nA;
+        }
+        ++apos;
     };
     for (int k = 0; k < d; ++k) {
-        s.prog_f_off[k] = static_cast<int>(s.prog_f.size());
         s.u_off[k] = upos;
         const int c0 = s.colp[k], c = s.colp[k + 1] - c0;
         s.max_c = std::max(s.max_c, c);
-        s.prog_f.push_back(c);
-        // A stream: F(k)
+        std::vector<int>& r = refs[k];
+        r.reserve(static_cast<size_t>((c + 1) * (c + 2)));
+        // A stream: F(k), diagonal, L column, U row
         s.f_pos[s.perm[k]] = apos++;
-        // b(k)
-        int bref = kNoSlot;
+        place(k, k);
+        for (int p = c0; p < c0 + c; ++p) place(s.coli[p], k);
+        for (int p = c0; p < c0 + c; ++p) place(k, s.coli[p]);
+        // sources: diagonal, b(k), L entries, U entries (their slots are freed:
+        // every read of a pivot happens before its writes)
+        r.push_back(take(k, k));
+        int bref = -1;
         if (bslot[k] >= 0) { bref = bslot[k]; sa.put(bref); bslot[k] = -1; }
-        s.prog_f.push_back(bref);
-        // diagonal
-        {
-            const int pa = a_present(k, k);
-            const int sl = take(k, k);
-            int ref = sl;
-            if (pa >= 0) { ref |= kInA; s.a_pos_of_entry[pa] = apos++; ++s.nA; }
-            s.prog_f.push_back(ref);
-        }
-        for (int side = 0; side < 2; ++side)       // 0: L(i,k), 1: U(k,j)
-            for (int p = c0; p < c0 + c; ++p) {
-                const int o = s.coli[p];
-                const int i = side == 0 ? o : k, j = side == 0 ? k : o;
-                const int pa = a_present(i, j);
-                const int sl = take(i, j);
-                int ref = sl;
-                if (pa >= 0) { ref |= kInA; s.a_pos_of_entry[pa] = apos++; ++s.nA; }
-                s.prog_f.push_back(ref);
-            }
+        r.push_back(bref);
+        for (int p = c0; p < c0 + c; ++p) r.push_back(take(s.coli[p], k));
+        for (int p = c0; p < c0 + c; ++p) r.push_back(take(k, s.coli[p]));
         upos += c + 2;
-        // rank-1 update targets (i, j) for i, j in struct_k
+        // rank-1 update targets (i, j), i, j in struct_k, row-major
         for (int pa_ = c0; pa_ < c0 + c; ++pa_)
             for (int pb = c0; pb < c0 + c; ++pb) {
-                const int i = s.coli[pa_], j = s.coli[pb];
-                const long long kk = key(i, j, d);
+                const long long kk = key(s.coli[pa_], s.coli[pb], d);
                 auto it = slot_of.find(kk);
                 if (it == slot_of.end()) {
                     const int sl = sa.get();
                     slot_of[kk] = sl;
-                    s.prog_f.push_back(sl | kFirst);
+                    r.push_back(sl | kFirst);
                 } else {
-                    s.prog_f.push_back(it->second);
+                    r.push_back(it->second);
                 }
             }
         s.n_updates += c * c;
@@ -353,84 +346,98 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
             const int i = s.coli[p];
             if (bslot[i] < 0) {
                 bslot[i] = sa.get();
-                s.prog_f.push_back(bslot[i] | kFirst);
+                r.push_back(bslot[i] | kFirst);
             } else {
-                s.prog_f.push_back(bslot[i]);
+                r.push_back(bslot[i]);
             }
         }
     }
-    s.prog_f_off[d] = static_cast<int>(s.prog_f.size());
     s.u_off[d] = upos;
     s.a_len = apos;
     s.u_len = upos;
     s.slots_f = sa.hi;
     if (!slot_of.empty()) throw std::logic_error("symbolic: update targets left unconsumed");
-    if (s.slots_f > kSlotMask - 1) throw std::runtime_error("symbolic: too many live update slots");
-    build_backward(s, {}, {}, {});
+    if (s.slots_f + 2 > kRefSlot) throw std::runtime_error("symbolic: too many live update slots");
+    // pack: header word (c | record words << 16), then 16-bit refs, records
+    // padded to 4 words; a missing source reads the zero slot
+    s.prog_f.clear();
+    s.prog_f_off.assign(static_cast<size_t>(d) + 1, 0);
+    for (int k = 0; k < d; ++k) {
+        s.prog_f_off[k] = static_cast<int>(s.prog_f.size());
+        const int c = s.colp[k + 1] - s.colp[k];
+        const int c_next = k + 1 < d ? s.colp[k + 2] - s.colp[k + 1] : 0;
+        const std::vector<int>& r = refs[k];
+        const int words = frec_words(c);
+        s.prog_f.push_back(c | (c_next << 16));
+        std::vector<int> packed(static_cast<size_t>(words - 1), 0);
+        for (size_t q = 0; q < r.size(); ++q) {
+            int v = r[q];
+            unsigned f = v < 0 ? static_cast<unsigned>(s.slots_f) : static_cast<unsigned>(v & kSlotMask) |
+                                                                        ((v & kFirst) ? kRefFirst : 0u);
+            packed[q / 2] |= static_cast<int>(f << (16 * (q & 1)));
+        }
+        s.prog_f.insert(s.prog_f.end(), packed.begin(), packed.end());
+    }
+    s.prog_f_off[d] = static_cast<int>(s.prog_f.size());
+    build_backward(s);
     return s;
 }
 
 namespace {
 
-// Backward program (pivots d-1 .. 0) with the fused bus update.  `bus_of`,
-// `kind_of` (0 theta, 1 |V|) and `partner` (pivot label of the bus's other
-// unknown or -1) are per pivot; bus_of < 0 disables updates (raw solves).
-void build_backward(SymbolicLU& s, const std::vector<int>& bus_of, const std::vector<int>& kind_of,
-                    const std::vector<int>& partner)
+// Backward program, one record per pivot in processing order d-1 .. 0:
+// header word (c | record words << 16), perm[k], then 16-bit refs: the slots
+// of x(j), j in struct_k, and the slot x(k) is kept in (the trash slot when
+// no later pivot reads it).
+void build_backward(SymbolicLU& s)
 {
     const int d = s.d;
-    const bool upd = !bus_of.empty();
-    // last use (smallest pivot label in whose struct j appears)
-    std::vector<int> last(static_cast<size_t>(d), d);
+    std::vector<int> last(static_cast<size_t>(d), d);  // smallest pivot reading x(j)
     for (int k = 0; k < d; ++k)
         for (int p = s.colp[k]; p < s.colp[k + 1]; ++p) last[s.coli[p]] = std::min(last[s.coli[p]], k);
-    if (upd)
-        for (int k = 0; k < d; ++k)
-            if (partner[k] >= 0 && partner[k] < k) last[k] = std::min(last[k], partner[k]);
     SlotAlloc sa;
     std::vector<int> xslot(static_cast<size_t>(d), -1);
-    std::vector<int> to_free;
-    s.prog_b.clear();
+    std::vector<std::vector<int>> recs;
+    recs.reserve(static_cast<size_t>(d));
     for (int k = d - 1; k >= 0; --k) {
         const int c0 = s.colp[k], c = s.colp[k + 1] - c0;
-        s.prog_b.push_back(c);
-        to_free.clear();
+        std::vector<int> r;
+        std::vector<int> to_free;
         for (int p = c0; p < c0 + c; ++p) {
             const int j = s.coli[p];
             if (xslot[j] < 0) throw std::logic_error("backward: x(j) read before it is computed");
-            s.prog_b.push_back(xslot[j]);
+            r.push_back(xslot[j]);
             if (last[j] == k) to_free.push_back(xslot[j]);
-        }
-        int op = 0, bus = -1, pslot = kNoSlot;
-        if (upd && bus_of[k] >= 0) {
-            const int q = partner[k];
-            if (q < 0) {
-                op = kOpThetaSelf;
-                bus = bus_of[k];
-            } else if (q > k) {  // the partner was computed earlier: this pivot completes the bus
-                op = kind_of[k] == 0 ? kOpThetaSelfVmSlot : kOpVmSelfThetaSlot;
-                bus = bus_of[k];
-                pslot = xslot[q];
-                if (pslot < 0) throw std::logic_error("backward: partner x has no slot");
-                if (last[q] == k) to_free.push_back(pslot);
-            }
         }
         // every read of this pivot happens before x(k) is stored, so a slot
         // freed here may hold x(k)
-        std::sort(to_free.begin(), to_free.end());
-        to_free.erase(std::unique(to_free.begin(), to_free.end()), to_free.end());
         for (int sl : to_free) sa.put(sl);
-        int out = kNoSlot;
+        int out = -1;
         if (last[k] < k) {
             out = sa.get();
             xslot[k] = out;
         }
-        s.prog_b.push_back(out);
-        s.prog_b.push_back(op);
-        s.prog_b.push_back(bus);
-        s.prog_b.push_back(pslot);
+        r.push_back(out);
+        recs.push_back(std::move(r));
     }
     s.slots_b = sa.hi;
+    if (s.slots_b + 2 > kRefSlot) throw std::runtime_error("symbolic: too many live solution slots");
+    s.prog_b.clear();
+    for (int q = 0; q < d; ++q) {
+        const int k = d - 1 - q;
+        const int c = s.colp[k + 1] - s.colp[k];
+        const int c_next = k > 0 ? s.colp[k] - s.colp[k - 1] : 0;
+        const std::vector<int>& r = recs[q];
+        const int words = brec_words(c);
+        s.prog_b.push_back(c | (c_next << 16));
+        s.prog_b.push_back(s.perm[k]);
+        std::vector<int> packed(static_cast<size_t>(words - 2), 0);
+        for (size_t i = 0; i < r.size(); ++i) {
+            const unsigned f = r[i] < 0 ? static_cast<unsigned>(s.slots_b + 1) : static_cast<unsigned>(r[i]);
+            packed[i / 2] |= static_cast<int>(f << (16 * (i & 1)));
+        }
+        s.prog_b.insert(s.prog_b.end(), packed.begin(), packed.end());
+    }
 }
 
 }  // namespace
@@ -545,26 +552,6 @@ NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int 
         if (P.row_q[i] >= 0) order.push_back(P.row_q[i]);
     }
     P.lu = build_symbolic(d, P.Jp.data(), P.Ji.data(), order);
-
-    // backward program with the fused update
-    {
-        std::vector<int> bus_of(static_cast<size_t>(d), -1), kind_of(static_cast<size_t>(d), 0),
-            partner(static_cast<size_t>(d), -1);
-        for (int i = 0; i < n; ++i) {
-            if (i == slack) continue;
-            const int kt = P.lu.iperm[P.row_p[i]];
-            bus_of[kt] = i;
-            kind_of[kt] = 0;
-            if (P.row_q[i] >= 0) {
-                const int kv = P.lu.iperm[P.row_q[i]];
-                bus_of[kv] = i;
-                kind_of[kv] = 1;
-                partner[kt] = kv;
-                partner[kv] = kt;
-            }
-        }
-        build_backward(P.lu, bus_of, kind_of, partner);
-    }
 
     // mismatch/Jacobian program: buses in RCM order of the Ybus graph
     {

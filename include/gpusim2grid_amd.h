@@ -72,7 +72,9 @@ typedef struct {
     double ms_mismatch;     /* mismatch + Jacobian fill                          */
     double ms_factor;       /* factorization + forward substitution              */
     double ms_backward;     /* backward substitution + V update                  */
-    int64_t bytes_factor;   /* algorithmic HBM bytes of one factorization launch */
+    int64_t bytes_factor;   /* A- and U-stream bytes of one factorization launch  */
+    int64_t rows_factored;  /* rows summed over the factorization launches       */
+    int32_t n_factor_launches;
 } g2g_solve_stats;
 
 const char *g2g_last_error(void);
