@@ -38,7 +38,8 @@ class SolveStats(ctypes.Structure):
                 ("ms_total", ctypes.c_double), ("ms_mismatch", ctypes.c_double),
                 ("ms_factor", ctypes.c_double), ("ms_backward", ctypes.c_double),
                 ("bytes_factor", ctypes.c_int64), ("rows_factored", ctypes.c_int64),
-                ("n_factor_launches", ctypes.c_int32)]
+                ("n_factor_launches", ctypes.c_int32), ("n_requeued", ctypes.c_int32),
+                ("ms_warm_start", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -77,6 +78,19 @@ def lib():
         "g2g_batch_eval": (ctypes.c_int, [_P, _P, _P, _P]),
         "g2g_lu_solve_batch": (ctypes.c_int, [_P, i32, i32, _P, _P, _P, dbl, _P, _P]),
         "g2g_batch_device_arrays": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+        "g2g_batch_solve_warm": (ctypes.c_int, [_P, _P, _P, i32, i32, dbl, i32, dbl, ctypes.POINTER(SolveStats)]),
+        "g2g_mlp_create": (ctypes.c_int, [i32, i32, i32, ctypes.c_uint64, _P, ctypes.POINTER(_P)]),
+        "g2g_mlp_destroy": (None, [_P]),
+        "g2g_mlp_num_params": (i64, [_P]),
+        "g2g_mlp_param_offsets": (ctypes.c_int, [_P, _P]),
+        "g2g_mlp_get_params": (ctypes.c_int, [_P, _P, i64]),
+        "g2g_mlp_set_params": (ctypes.c_int, [_P, _P, i64]),
+        "g2g_mlp_set_normalization": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+        "g2g_mlp_set_buses": (ctypes.c_int, [_P, _P, _P, i32]),
+        "g2g_mlp_forward": (ctypes.c_int, [_P, _P, i32, _P, i32]),
+        "g2g_mlp_loss_grad": (ctypes.c_int, [_P, _P, _P, i32, _P, _P, i32]),
+        "g2g_mlp_train_step": (ctypes.c_int, [_P, _P, _P, i32, ctypes.c_float, _P, i32]),
+        "g2g_gemm_bf16": (ctypes.c_int, [i32, i32, i32, i32, i32, _P, i64, _P, i64, _P, i64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -223,6 +237,19 @@ class Batch:
         _check(lib().g2g_batch_solve(self._h, int(max_iter), float(tol), int(scheduling), ctypes.byref(st)))
         return st.as_dict()
 
+    def solve_warm(self, mlp, k_warm=3, growth=2.0, max_iter=10, tol=1e-8, mask=None):
+        """Warm-started solve with flat-start fallback (g2g_batch_solve_warm);
+        the batch's V (set_v) is the fallback start."""
+        st = SolveStats()
+        n_mask = 0
+        m = None
+        if mask is not None:
+            m = np.ascontiguousarray(mask, dtype=np.uint8)
+            n_mask = m.shape[1]
+        _check(lib().g2g_batch_solve_warm(self._h, mlp._h, _ptr(m), n_mask, int(k_warm), float(growth),
+                                          int(max_iter), float(tol), ctypes.byref(st)))
+        return st.as_dict()
+
     def get_v(self):
         V = np.empty((self.rows, self.n), np.complex128)
         _check(lib().g2g_batch_get_v(self._h, _ptr(V), self.n, 0))
@@ -250,3 +277,106 @@ class Batch:
         v, s = _P(), _P()
         _check(lib().g2g_batch_device_arrays(self._h, ctypes.byref(v), ctypes.byref(s)))
         return v.value, s.value
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Mlp:
+    """The warm-start MLP (in -> H -> H -> out, GELU; bf16 MFMA, fp32 master
+    weights and Adam state on the device)."""
+
+    NAMES = ("W1", "b1", "W2", "b2", "W3", "b3")
+
+    def __init__(self, n_in, n_hidden, n_out, seed=0, stream=None):
+        self.n_in, self.H, self.n_out = int(n_in), int(n_hidden), int(n_out)
+        h = _P()
+        _check(lib().g2g_mlp_create(self.n_in, self.H, self.n_out, int(seed), stream, ctypes.byref(h)))
+        self._h = h
+        self.n_params = int(lib().g2g_mlp_num_params(h))
+        off = np.empty(6, np.int64)
+        _check(lib().g2g_mlp_param_offsets(h, _ptr(off)))
+        self.offsets = off
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().g2g_mlp_destroy(self._h)
+            self._h = None
+
+    def shapes(self):
+        n_in, H, n_out = self.n_in, self.H, self.n_out
+        return {"W1": (n_in, H), "b1": (H,), "W2": (H, H), "b2": (H,), "W3": (H, n_out), "b3": (n_out,)}
+
+    def unflatten(self, flat):
+        sh = self.shapes()
+        return {k: flat[o:o + int(np.prod(sh[k]))].reshape(sh[k]) for k, o in zip(self.NAMES, self.offsets)}
+
+    def flatten(self, params):
+        flat = np.zeros(self.n_params, np.float32)
+        for k, o in zip(self.NAMES, self.offsets):
+            v = np.asarray(params[k], np.float32).ravel()
+            flat[o:o + v.size] = v
+        return flat
+
+    def get_params(self):
+        out = np.empty(self.n_params, np.float32)
+        _check(lib().g2g_mlp_get_params(self._h, _ptr(out), out.size))
+        return self.unflatten(out)
+
+    def set_params(self, params):
+        flat = self.flatten(params)
+        _check(lib().g2g_mlp_set_params(self._h, _ptr(flat), flat.size))
+
+    def set_normalization(self, in_mu, in_sigma, out_mu, out_sigma):
+        a = [_f32(x) for x in (in_mu, in_sigma, out_mu, out_sigma)]
+        _check(lib().g2g_mlp_set_normalization(self._h, *[_ptr(x) for x in a]))
+
+    def set_buses(self, kind, vfix):
+        kind = _i32(kind)
+        vfix = np.ascontiguousarray(vfix, dtype=np.float64)
+        _check(lib().g2g_mlp_set_buses(self._h, _ptr(kind), _ptr(vfix), kind.size))
+
+    def forward(self, X):
+        X = _f32(X)
+        Y = np.empty((X.shape[0], self.n_out), np.float32)
+        _check(lib().g2g_mlp_forward(self._h, _ptr(X), X.shape[0], _ptr(Y), 0))
+        return Y
+
+    def forward_device(self, x_ptr, M, y_ptr):
+        _check(lib().g2g_mlp_forward(self._h, _P(x_ptr), int(M), _P(y_ptr), 1))
+
+    def loss_grad(self, X, T):
+        X, T = _f32(X), _f32(T)
+        loss = np.empty(1, np.float32)
+        g = np.empty(self.n_params, np.float32)
+        _check(lib().g2g_mlp_loss_grad(self._h, _ptr(X), _ptr(T), X.shape[0], _ptr(loss), _ptr(g), 0))
+        return float(loss[0]), self.unflatten(g)
+
+    def train_step(self, X, T, lr):
+        X, T = _f32(X), _f32(T)
+        loss = np.empty(1, np.float32)
+        _check(lib().g2g_mlp_train_step(self._h, _ptr(X), _ptr(T), X.shape[0], float(lr), _ptr(loss), 0))
+        return float(loss[0])
+
+    def train_step_device(self, x_ptr, t_ptr, M, lr):
+        loss = np.empty(1, np.float32)
+        _check(lib().g2g_mlp_train_step(self._h, _P(x_ptr), _P(t_ptr), int(M), float(lr), _ptr(loss), 1))
+        return float(loss[0])
+
+
+def gemm_bf16(A, B, trans_a=False, trans_b=False):
+    """C = op(A) op(B) on the MLP's bf16 MFMA kernel; A and B are uint16 arrays
+    of bf16 bit patterns (2-D, row-major, as stored)."""
+    A = np.ascontiguousarray(A, dtype=np.uint16)
+    B = np.ascontiguousarray(B, dtype=np.uint16)
+    M = A.shape[1] if trans_a else A.shape[0]
+    K = A.shape[0] if trans_a else A.shape[1]
+    N = B.shape[0] if trans_b else B.shape[1]
+    kb = B.shape[1] if trans_b else B.shape[0]
+    if kb != K:
+        raise ValueError("inner dimensions differ")
+    C = np.empty((M, N), np.float32)
+    _check(lib().g2g_gemm_bf16(int(trans_a), int(trans_b), M, N, K, _ptr(A), A.shape[1], _ptr(B), B.shape[1],
+                               _ptr(C), N))
+    return C

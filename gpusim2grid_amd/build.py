@@ -44,7 +44,8 @@ def _run(cmd, verbose):
 def build_lib(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for src, lang in (("kernels.hip", "hip"), ("capi.cpp", "hip"), ("plan.cpp", "c++"), ("mlp.hip", "hip")):
+    for src, lang in (("kernels.hip", "hip"), ("capi.cpp", "hip"), ("plan.cpp", "c++"), ("mlp.hip", "hip"),
+                      ("mlp_capi.cpp", "hip")):
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/gpusim2grid_amd.h"
+#include "capi_common.hpp"
 #include "device_common.hpp"
 #include "plan.hpp"
 
@@ -36,84 +37,18 @@ __global__ void k_gather_astream(const double* vals, long long ldv, const double
 __global__ void k_row_absmax(const double* vals, long long ldv, int rows, int nnz, double* out);
 __global__ void k_fill_int(int* p, int v, long long n);
 __global__ void k_count_tiles(const int* status, int rows_pad, int* tile_active);
+__global__ void k_requeue_v(const int* status, int rows, int n, const double2* V0, double2* V);
+__global__ void k_requeue_rows(int* status, int* iters, int* iters_warm, int rows, int* n_requeued);
+__global__ void k_add_int(int* p, const int* add, int n);
+
+// learned warm start (mlp_capi.cpp)
+void mlp_warm_start_tiles(g2g_mlp_s* p, const double2* S_tiles, int M, int n, const unsigned char* d_mask,
+                          int n_mask, double2* V_tiles, hipStream_t st);
 
 }  // namespace g2g
 
 using namespace g2g;
 
-namespace {
-
-thread_local std::string g_err;
-
-struct HipError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-#define HIPCHK(call)                                                                                    \
-    do {                                                                                                \
-        hipError_t e_ = (call);                                                                         \
-        if (e_ != hipSuccess) throw HipError(std::string(#call) + ": " + hipGetErrorString(e_));       \
-    } while (0)
-
-template <typename F>
-int guarded(F&& f)
-{
-    try {
-        f();
-        return G2G_OK;
-    } catch (const std::invalid_argument& e) {
-        g_err = e.what();
-        return G2G_ERR_INVALID;
-    } catch (const HipError& e) {
-        g_err = e.what();
-        return G2G_ERR_HIP;
-    } catch (const std::exception& e) {
-        g_err = e.what();
-        return G2G_ERR_RUNTIME;
-    }
-}
-
-template <typename T>
-T* dalloc(size_t count)
-{
-    void* p = nullptr;
-    if (count == 0) count = 1;
-    HIPCHK(hipMalloc(&p, count * sizeof(T)));
-    return static_cast<T*>(p);
-}
-
-template <typename T>
-T* dupload(const std::vector<T>& v)
-{
-    T* p = dalloc<T>(v.size());
-    if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    return p;
-}
-
-void dfree(void* p)
-{
-    if (p) (void)hipFree(p);
-}
-
-inline unsigned blocks_for(long long n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
-
-// device buffers freed together at scope exit
-struct DeviceScratch {
-    std::vector<void*> p;
-    ~DeviceScratch()
-    {
-        for (void* q : p) dfree(q);
-    }
-    template <typename T>
-    T* get(size_t n)
-    {
-        T* q = dalloc<T>(n);
-        p.push_back(q);
-        return q;
-    }
-};
-
-}  // namespace
 
 struct g2g_plan_s {
     bool grid = false;
@@ -197,12 +132,17 @@ struct g2g_batch_s {
     int* tile_active = nullptr;
     int* n_active = nullptr;
     int* clamped = nullptr;
+    int* error = nullptr;     // stream kernels' give-up flag
     int* h_active = nullptr;  // pinned
+    double2* V0 = nullptr;    // warm solves: the caller's start voltages, kept for re-queued rows
+    int* iters_warm = nullptr;
+    int* n_requeued = nullptr;
     std::vector<hipEvent_t> ev;
     ~g2g_batch_s()
     {
         dfree(V); dfree(S); dfree(Yrow); dfree(A); dfree(U); dfree(X); dfree(res); dfree(jmax);
-        dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped);
+        dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped); dfree(error);
+        dfree(V0); dfree(iters_warm); dfree(n_requeued);
         if (h_active) (void)hipHostFree(h_active);
         for (auto e : ev) (void)hipEventDestroy(e);
     }
@@ -217,9 +157,13 @@ constexpr int chunk_of() { return sizeof(Real) == sizeof(double) ? kChunkF64 : k
 inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 template <typename Real>
+constexpr int ring_slots_of() { return sizeof(Real) == sizeof(double) ? kRingSlotsF64 : kRingSlotsF32; }
+
+// control blocks + value ring + program ring (kernels.hip's LDS layout)
+template <typename Real>
 size_t ring_lds()
 {
-    return static_cast<size_t>(chunk_of<Real>()) * kRingSlots * kLanes * sizeof(Real) +
+    return 256 + static_cast<size_t>(chunk_of<Real>()) * ring_slots_of<Real>() * kLanes * sizeof(Real) +
            static_cast<size_t>(kProgChunk) * kProgSlots * sizeof(int);
 }
 
@@ -247,30 +191,30 @@ size_t lds_limit()
 
 template <typename Real>
 void launch_factor(const g2g_plan_s* P, const Real* A, Real* U, int a_stride, int u_stride, const int* tile_active,
-                   const double* jmax, int* clamped, int rows, int T, double eps, hipStream_t st)
+                   const double* jmax, int* clamped, int* error, int rows, int T, double eps, hipStream_t st)
 {
     const SymbolicLU& s = P->lu();
     const size_t lds = factor_lds<Real>(s);
     if (lds > lds_limit()) throw std::runtime_error("factorization needs " + std::to_string(lds) + " bytes of LDS");
     HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_factor<Real>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    FactorArgs<Real> a{A, U, P->d_prog_f, tile_active, jmax, clamped, s.d, a_stride, u_stride,
+    FactorArgs<Real> a{A, U, P->d_prog_f, tile_active, jmax, clamped, error, s.d, a_stride, u_stride,
                        P->prog_f_stride, s.slots_f, rows, eps};
-    k_factor<Real><<<T, kLanes, lds, st>>>(a);
+    k_factor<Real><<<T, kLanes * kStreamWaves, lds, st>>>(a);
     HIPCHK(hipGetLastError());
 }
 
 template <typename Real>
-void launch_backward(const g2g_plan_s* P, const Real* U, int u_stride, const int* tile_active, Real* X, int rows,
-                     int T, hipStream_t st)
+void launch_backward(const g2g_plan_s* P, const Real* U, int u_stride, const int* tile_active, Real* X, int* error,
+                     int rows, int T, hipStream_t st)
 {
     const SymbolicLU& s = P->lu();
     const size_t lds = backward_lds<Real>(s);
     if (lds > lds_limit()) throw std::runtime_error("backward substitution needs too much LDS");
     HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_backward<Real>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    BackwardArgs<Real> a{U, P->d_prog_b, tile_active, X, P->d_perm, s.d, s.u_len, u_stride, P->prog_b_stride, rows};
-    k_backward<Real><<<T, kLanes, lds, st>>>(a);
+    BackwardArgs<Real> a{U, P->d_prog_b, tile_active, X, error, s.d, s.u_len, u_stride, P->prog_b_stride, rows};
+    k_backward<Real><<<T, kLanes * kStreamWaves, lds, st>>>(a);
     HIPCHK(hipGetLastError());
 }
 
@@ -285,12 +229,13 @@ void launch_update(g2g_batch_s* b)
 }
 
 template <typename Real>
-void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double tol, int freeze)
+void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double tol, int freeze,
+                     double growth = 0.0)
 {
     const NrPlan& P = b->plan->nr;
     MismatchArgs<Real> a{b->plan->d_k1_bus, b->plan->d_k1_ent, P.k1_nbus, b->plan->d_Y, b->Yrow, b->V, b->S,
                          static_cast<Real*>(b->A), P.n, P.nY, b->a_stride, b->res, b->jmax, b->status, b->iters,
-                         b->tile_active, b->n_active, b->active_rows, iter, max_iter, tol, freeze};
+                         b->tile_active, b->n_active, b->active_rows, iter, max_iter, tol, freeze, growth};
     const dim3 grid(b->T), block(kLanes * kMisWaves);
     if (b->per_row_y) {
         if (with_j) k_mismatch<Real, true, true><<<grid, block, 0, b->stream>>>(a);
@@ -314,75 +259,167 @@ void reset_rows(g2g_batch_s* b)
     HIPCHK(hipStreamSynchronize(b->stream));  // st is a host temporary
 }
 
+// per-solve event bookkeeping (phase times summed over every launch)
+struct SolveTimer {
+    g2g_batch_s* b;
+    int nev = 0;
+    std::vector<std::pair<int, int>> ph[4];  // mismatch, factor, backward + update, warm start
+    explicit SolveTimer(g2g_batch_s* bb, int max_launches) : b(bb)
+    {
+        while (static_cast<int>(b->ev.size()) < 2 * max_launches + 4) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            b->ev.push_back(e);
+        }
+    }
+    int mark()
+    {
+        if (nev >= static_cast<int>(b->ev.size())) throw std::runtime_error("solve event pool exhausted");
+        HIPCHK(hipEventRecord(b->ev[nev], b->stream));
+        return nev++;
+    }
+    double ms(int k) const
+    {
+        double t = 0.0;
+        for (auto& pr : ph[k]) {
+            float m = 0.f;
+            HIPCHK(hipEventElapsedTime(&m, b->ev[pr.first], b->ev[pr.second]));
+            t += m;
+        }
+        return t;
+    }
+};
+
+// Newton iterations on the batch's active rows until none is left or the
+// budget is spent; returns the iterations launched.
+template <typename Real>
+int nr_loop(g2g_batch_s* b, int max_iter, double tol, int freeze, double growth, SolveTimer& tm,
+            long long& rows_factored)
+{
+    const double eps = b->precision == G2G_MIXED ? 1e-5 : 1e-12;  // own_lu/batched_lu.cuh:50-56
+    int it = 0;
+    for (;; ++it) {
+        k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
+        int e0 = tm.mark();
+        launch_mismatch<Real>(b, true, it, max_iter, tol, freeze, growth);
+        tm.ph[0].push_back({e0, tm.mark()});
+        if (it >= max_iter) break;
+        HIPCHK(hipMemcpyAsync(b->h_active, b->n_active, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        if (*b->h_active == 0) break;
+        e0 = tm.mark();
+        launch_factor<Real>(b->plan, static_cast<const Real*>(b->A), static_cast<Real*>(b->U), b->a_stride,
+                            b->u_stride, b->tile_active, b->jmax, b->clamped, b->error, b->active_rows, b->T, eps,
+                            b->stream);
+        tm.ph[1].push_back({e0, tm.mark()});
+        rows_factored += *b->h_active;
+        e0 = tm.mark();
+        launch_backward<Real>(b->plan, static_cast<const Real*>(b->U), b->u_stride, b->tile_active,
+                              static_cast<Real*>(b->X), b->error, b->active_rows, b->T, b->stream);
+        launch_update<Real>(b);
+        tm.ph[2].push_back({e0, tm.mark()});
+    }
+    return it;
+}
+
+void check_stream_error(g2g_batch_s* b)
+{
+    int err = 0;
+    HIPCHK(hipMemcpyAsync(&err, b->error, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (err) throw std::runtime_error("a stream wavefront gave up waiting for its data (results invalid)");
+}
+
+void fill_stats(g2g_batch_s* b, const SolveTimer& tm, int e_begin, int e_end, int iterations,
+                long long rows_factored, g2g_solve_stats* stats)
+{
+    if (!stats) return;
+    std::memset(stats, 0, sizeof(*stats));
+    stats->iterations = iterations;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev[e_begin], b->ev[e_end]));
+    stats->ms_total = ms;
+    stats->ms_mismatch = tm.ms(0);
+    stats->ms_factor = tm.ms(1);
+    stats->ms_backward = tm.ms(2);
+    stats->ms_warm_start = tm.ms(3);
+    const SymbolicLU& s = b->plan->lu();
+    stats->bytes_factor = static_cast<int64_t>(s.a_len + s.u_len) * static_cast<int64_t>(b->real_size()) *
+                          static_cast<int64_t>(b->active_rows);
+    stats->rows_factored = rows_factored;
+    stats->n_factor_launches = static_cast<int32_t>(tm.ph[1].size());
+    std::vector<int> st(static_cast<size_t>(b->active_rows));
+    if (!st.empty()) HIPCHK(hipMemcpy(st.data(), b->status, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int v : st) {
+        stats->n_converged += v == kConverged;
+        stats->n_maxiter += v == kMaxIter;
+        stats->n_diverged += v == kDiverged;
+    }
+}
+
 template <typename Real>
 void solve_impl(g2g_batch_s* b, int max_iter, double tol, int sched, g2g_solve_stats* stats)
 {
     const int freeze = sched == G2G_SCHED_CONTINUOUS ? 1 : 0;
     reset_rows(b);
-    const int need = 3 * (max_iter + 1) + 2;
-    while (static_cast<int>(b->ev.size()) < 2 * need) {
-        hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
-        b->ev.push_back(e);
-    }
-    int nev = 0;
-    auto mark = [&]() {
-        HIPCHK(hipEventRecord(b->ev[nev], b->stream));
-        return nev++;
-    };
-    std::vector<std::pair<int, int>> ph[3];
-    const int e_begin = mark();
-    const double eps = b->precision == G2G_MIXED ? 1e-5 : 1e-12;  // own_lu/batched_lu.cuh:50-56
-    int it = 0;
+    SolveTimer tm(b, 3 * (max_iter + 1) + 2);
+    const int e_begin = tm.mark();
     long long rows_factored = 0;
-    for (;; ++it) {
-        k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
-        int e0 = mark();
-        launch_mismatch<Real>(b, true, it, max_iter, tol, freeze);
-        ph[0].push_back({e0, mark()});
-        if (it >= max_iter) break;
-        HIPCHK(hipMemcpyAsync(b->h_active, b->n_active, sizeof(int), hipMemcpyDeviceToHost, b->stream));
-        HIPCHK(hipStreamSynchronize(b->stream));
-        if (*b->h_active == 0) break;
-        e0 = mark();
-        launch_factor<Real>(b->plan, static_cast<const Real*>(b->A), static_cast<Real*>(b->U), b->a_stride,
-                            b->u_stride, b->tile_active, b->jmax, b->clamped, b->active_rows, b->T, eps, b->stream);
-        ph[1].push_back({e0, mark()});
-        rows_factored += *b->h_active;
-        e0 = mark();
-        launch_backward<Real>(b->plan, static_cast<const Real*>(b->U), b->u_stride, b->tile_active,
-                              static_cast<Real*>(b->X), b->active_rows, b->T, b->stream);
-        launch_update<Real>(b);
-        ph[2].push_back({e0, mark()});
+    const int it = nr_loop<Real>(b, max_iter, tol, freeze, 0.0, tm, rows_factored);
+    const int e_end = tm.mark();
+    check_stream_error(b);
+    fill_stats(b, tm, e_begin, e_end, it, rows_factored, stats);
+}
+
+// Learned warm start with fallback: the MLP predicts every active row's V,
+// Newton runs at most k_warm updates on it (a row whose ||F||inf grows by
+// more than `growth` is dropped at once), and the rows left unconverged
+// restart from the caller's start voltages with the full budget.  A row's
+// iteration count is the sum of the updates of both passes.
+template <typename Real>
+void solve_warm_impl(g2g_batch_s* b, g2g_mlp_s* mlp, const unsigned char* mask, int n_mask, int k_warm,
+                     double growth, int max_iter, double tol, g2g_solve_stats* stats)
+{
+    const NrPlan& nr = b->plan->nr;
+    const size_t nv = static_cast<size_t>(b->rows_pad) * nr.n;
+    if (!b->V0) {
+        b->V0 = dalloc<double2>(nv);
+        b->iters_warm = dalloc<int>(static_cast<size_t>(b->rows_pad));
+        b->n_requeued = dalloc<int>(1);
     }
-    const int e_end = mark();
+    DeviceScratch tmp;
+    unsigned char* d_mask = nullptr;
+    if (n_mask > 0) {
+        d_mask = tmp.get<unsigned char>(static_cast<size_t>(b->active_rows) * n_mask);
+        HIPCHK(hipMemcpyAsync(d_mask, mask, static_cast<size_t>(b->active_rows) * n_mask, hipMemcpyHostToDevice,
+                              b->stream));
+    }
+    reset_rows(b);
+    SolveTimer tm(b, 3 * (k_warm + max_iter + 2) + 4);
+    const int e_begin = tm.mark();
+    HIPCHK(hipMemcpyAsync(b->V0, b->V, nv * sizeof(double2), hipMemcpyDeviceToDevice, b->stream));
+    mlp_warm_start_tiles(mlp, b->S, b->active_rows, nr.n, d_mask, n_mask, b->V, b->stream);
+    tm.ph[3].push_back({e_begin, tm.mark()});
+    long long rows_factored = 0;
+    int it = nr_loop<Real>(b, k_warm, tol, 1, growth, tm, rows_factored);
+    // re-queue
+    k_fill_int<<<1, 64, 0, b->stream>>>(b->n_requeued, 0, 1);
+    k_requeue_v<<<blocks_for(static_cast<long long>(nv), 256), 256, 0, b->stream>>>(b->status, b->active_rows,
+                                                                                    nr.n, b->V0, b->V);
+    k_requeue_rows<<<blocks_for(b->active_rows, 256), 256, 0, b->stream>>>(b->status, b->iters, b->iters_warm,
+                                                                           b->active_rows, b->n_requeued);
+    k_count_tiles<<<b->T, kLanes, 0, b->stream>>>(b->status, b->rows_pad, b->tile_active);
+    HIPCHK(hipGetLastError());
+    int requeued = 0;
+    HIPCHK(hipMemcpyAsync(&requeued, b->n_requeued, sizeof(int), hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
-    if (stats) {
-        std::memset(stats, 0, sizeof(*stats));
-        stats->iterations = it;
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, b->ev[e_begin], b->ev[e_end]));
-        stats->ms_total = ms;
-        double* dst[3] = {&stats->ms_mismatch, &stats->ms_factor, &stats->ms_backward};
-        for (int k = 0; k < 3; ++k)
-            for (auto& pr : ph[k]) {
-                HIPCHK(hipEventElapsedTime(&ms, b->ev[pr.first], b->ev[pr.second]));
-                *dst[k] += ms;
-            }
-        const SymbolicLU& s = b->plan->lu();
-        stats->bytes_factor = static_cast<int64_t>(s.a_len + s.u_len) * static_cast<int64_t>(b->real_size()) *
-                              static_cast<int64_t>(b->active_rows);
-        stats->rows_factored = rows_factored;
-        stats->n_factor_launches = static_cast<int32_t>(ph[1].size());
-        std::vector<int> st(static_cast<size_t>(b->active_rows));
-        if (!st.empty())
-            HIPCHK(hipMemcpy(st.data(), b->status, st.size() * sizeof(int), hipMemcpyDeviceToHost));
-        for (int v : st) {
-            stats->n_converged += v == kConverged;
-            stats->n_maxiter += v == kMaxIter;
-            stats->n_diverged += v == kDiverged;
-        }
-    }
+    if (requeued > 0) it += nr_loop<Real>(b, max_iter, tol, 1, 0.0, tm, rows_factored);
+    k_add_int<<<blocks_for(b->active_rows, 256), 256, 0, b->stream>>>(b->iters, b->iters_warm, b->active_rows);
+    HIPCHK(hipGetLastError());
+    const int e_end = tm.mark();
+    check_stream_error(b);
+    fill_stats(b, tm, e_begin, e_end, it, rows_factored, stats);
+    if (stats) stats->n_requeued = requeued;
 }
 
 // rows x m complex (row stride ld complex; ld = 0 broadcasts) -> tiled [T][m][64]
@@ -432,7 +469,9 @@ void raw_solve(g2g_plan_s* P, int rows, const double* vals, const double* rhs, d
     double* jm = B.get<double>(rp);
     int* ta = B.get<int>(T);
     int* cl = B.get<int>(rp);
+    int* err = B.get<int>(1);
     try {
+        HIPCHK(hipMemsetAsync(err, 0, sizeof(int), st));
         HIPCHK(hipMemcpyAsync(d_vals, vals, static_cast<size_t>(rows) * nnz * sizeof(double), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(d_rhs, rhs, static_cast<size_t>(rows) * d * sizeof(double), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemsetAsync(A, 0, rp * a_stride * sizeof(Real), st));
@@ -443,12 +482,15 @@ void raw_solve(g2g_plan_s* P, int rows, const double* vals, const double* rhs, d
         k_gather_astream<Real><<<blocks_for(total, 256), 256, 0, st>>>(d_vals, nnz, d_rhs, d, rows, nnz, d,
                                                                         P->d_apos, P->d_fpos, a_stride, A);
         HIPCHK(hipGetLastError());
-        launch_factor<Real>(P, A, U, a_stride, u_stride, ta, jm, cl, rows, T, pivot_eps, st);
-        launch_backward<Real>(P, U, u_stride, ta, X, rows, T, st);
+        launch_factor<Real>(P, A, U, a_stride, u_stride, ta, jm, cl, err, rows, T, pivot_eps, st);
+        launch_backward<Real>(P, U, u_stride, ta, X, err, rows, T, st);
+        int herr = 0;
+        HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
         std::vector<Real> hx(rp * d);
         HIPCHK(hipMemcpyAsync(hx.data(), X, hx.size() * sizeof(Real), hipMemcpyDeviceToHost, st));
         if (clamped) HIPCHK(hipMemcpyAsync(clamped, cl, rows * sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        if (herr) throw std::runtime_error("a stream wavefront gave up waiting for its data (results invalid)");
         for (int r = 0; r < rows; ++r)
             for (int e = 0; e < d; ++e)
                 x[static_cast<size_t>(r) * d + e] = hx[((static_cast<size_t>(r / kLanes) * d) + e) * kLanes + (r % kLanes)];
@@ -462,7 +504,7 @@ void raw_solve(g2g_plan_s* P, int rows, const double* vals, const double* rhs, d
 
 extern "C" {
 
-const char* g2g_last_error(void) { return g_err.c_str(); }
+const char* g2g_last_error(void) { return last_error().c_str(); }
 
 int g2g_version(void) { return 100; }
 
@@ -609,6 +651,8 @@ int g2g_batch_create(g2g_plan_t P, int32_t rows, int32_t precision, int32_t per_
         b->clamped = dalloc<int>(rp);
         b->tile_active = dalloc<int>(b->T);
         b->n_active = dalloc<int>(1);
+        b->error = dalloc<int>(1);
+        HIPCHK(hipMemsetAsync(b->error, 0, sizeof(int), b->stream));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->h_active), sizeof(int), hipHostMallocDefault));
         k_fill_int<<<blocks_for(rp, 256), 256, 0, b->stream>>>(b->status, kNotSimulated, rp);
         k_fill_int<<<blocks_for(rp, 256), 256, 0, b->stream>>>(b->iters, 0, rp);
@@ -689,6 +733,21 @@ int g2g_batch_solve(g2g_batch_t b, int32_t max_iter, double tol, int32_t schedul
             throw std::invalid_argument("unknown scheduling");
         if (b->precision == G2G_MIXED) solve_impl<float>(b, max_iter, tol, scheduling, stats);
         else solve_impl<double>(b, max_iter, tol, scheduling, stats);
+    });
+}
+
+int g2g_batch_solve_warm(g2g_batch_t b, g2g_mlp_t mlp, const uint8_t* mask, int32_t n_mask, int32_t k_warm,
+                         double growth, int32_t max_iter, double tol, g2g_solve_stats* stats)
+{
+    return guarded([&] {
+        if (!b || !mlp) throw std::invalid_argument("null argument");
+        if (n_mask < 0 || (n_mask > 0 && !mask)) throw std::invalid_argument("mask missing");
+        if (k_warm < 0 || max_iter < 0) throw std::invalid_argument("iteration budgets must be >= 0");
+        if (!(tol > 0.0)) throw std::invalid_argument("tol must be positive");
+        if (growth < 0.0 || growth != growth) throw std::invalid_argument("growth must be >= 0 (0: off)");
+        if (b->precision == G2G_MIXED)
+            solve_warm_impl<float>(b, mlp, mask, n_mask, k_warm, growth, max_iter, tol, stats);
+        else solve_warm_impl<double>(b, mlp, mask, n_mask, k_warm, growth, max_iter, tol, stats);
     });
 }
 

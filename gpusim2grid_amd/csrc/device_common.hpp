@@ -28,6 +28,7 @@ struct MismatchArgs {
     int iter, max_iter;
     double tol;
     int freeze;             // 1: a converged row stops (continuous); 0: fixed count (chunked)
+    double growth;          // > 0: a row whose ||F||inf exceeds growth x its previous one is rejected
 };
 
 template <typename Real>
@@ -38,6 +39,7 @@ struct FactorArgs {
     const int* tile_active;
     const double* jmax;     // nullable: scale 1
     int* clamped;           // nullable: clamped pivots per row
+    int* error;             // set when a wavefront gave up waiting on its stream
     int d, a_stride, u_stride, prog_stride, slots, rows;
     double pivot_eps;
 };
@@ -48,7 +50,7 @@ struct BackwardArgs {
     const int* prog;        // padded to prog_stride words
     const int* tile_active;
     Real* X;                // solution [T][d][64] in input labels
-    const int* perm;        // pivot -> input index
+    int* error;             // set when a wavefront gave up waiting on its stream
     int d, u_len, u_stride, prog_stride, rows;
 };
 

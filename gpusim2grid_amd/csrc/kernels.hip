@@ -157,11 +157,13 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_mismatch(MismatchArgs<Rea
     }
     int still = 0;
     if (active && row < a.rows) {
+        const double prev = a.res[row];
         a.res[row] = r;
         if (WithJ) a.jmax[row] = jm;
         int st = kActive;
         if (!isfinite(r)) st = kDiverged;
         else if (a.freeze && r < a.tol) st = kConverged;
+        else if (a.growth > 0.0 && a.iter > 0 && r > a.growth * prev) st = kRejected;
         else if (a.iter >= a.max_iter) st = r < a.tol ? kConverged : kMaxIter;
         if (st != kActive) {
             a.status[row] = st;
@@ -180,21 +182,27 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_mismatch(MismatchArgs<Rea
 
 // ---------------------------------------------------------------------------
 // k_factor: right-looking static-pivot LU + forward substitution
+//
+// Workgroup of three wavefronts over one tile of 64 rows: wave 0 walks the
+// program (LDS only, plus the U-stream stores), wave 1 streams the tile's A
+// values into an LDS ring, wave 2 streams the program words (streams.hpp).
 // ---------------------------------------------------------------------------
 
 template <typename Real>
 struct StreamShape;
 template <>
 struct StreamShape<double> {
-    static constexpr int kCh = kChunkF64, kNch = kRingSlots;
+    static constexpr int kCh = kChunkF64, kNch = kRingSlotsF64;
 };
 template <>
 struct StreamShape<float> {
-    static constexpr int kCh = kChunkF32, kNch = kRingSlots;
+    static constexpr int kCh = kChunkF32, kNch = kRingSlotsF32;
 };
-using ProgStream = RingStream<int, kProgChunk, kProgSlots, false, false>;
+using ProgRing = Ring<int, kProgChunk, kProgSlots, false, false>;
+using ProgReader = ProgRing::Reader;
+constexpr int kCtlBytes = 256;  // the two streams' control blocks
 
-// 16-bit field q of a record whose refs start at word `head`
+// 16-bit field q of a record whose refs start at word `Head`
 template <int Head, int NW>
 __device__ __forceinline__ unsigned ref_of(const v4u (&w)[NW / 4], int q)
 {
@@ -203,12 +211,19 @@ __device__ __forceinline__ unsigned ref_of(const v4u (&w)[NW / 4], int q)
     return (q & 1) ? (v >> 16) : (v & 0xFFFFu);
 }
 
+// NW program words from position p (4-aligned) as 16-byte broadcast reads
 template <int NW>
-__device__ __forceinline__ void read_record(const ProgStream& P, int p, v4u (&w)[NW / 4])
+__device__ __forceinline__ void read_record(const ProgReader& P, int p, v4u (&w)[NW / 4])
 {
     const v4u* base = reinterpret_cast<const v4u*>(P.ring);
 #pragma unroll
-    for (int i = 0; i < NW / 4; ++i) w[i] = base[((p >> 2) + i) & (ProgStream::kRing / 4 - 1)];
+    for (int i = 0; i < NW / 4; ++i) w[i] = base[((p >> 2) + i) & (ProgRing::kRing / 4 - 1)];
+}
+
+__device__ __forceinline__ int prog_word(ProgReader& P, int p)
+{
+    P.window(p, p);
+    return __builtin_amdgcn_readfirstlane(P.at(p));
 }
 
 template <typename Real>
@@ -221,18 +236,18 @@ struct FactorState {
 };
 
 // One pivot with C off-diagonal entries: every slot read of the pivot is
-// issued before any of its slot writes (sources, then old values of the
-// update targets), so the wavefront waits on LDS once per pivot.
-template <int C, typename Real, typename AS>
-__device__ __forceinline__ void factor_pivot(AS& A, ProgStream& P, FactorState<Real>& st)
+// issued before its slot writes (sources, then the old values of the update
+// targets, pipelined one row ahead), so LDS latency is paid a few times per
+// pivot, not once per entry.
+template <int C, typename Real, typename AR>
+__device__ __forceinline__ void factor_pivot(AR& A, ProgReader& P, FactorState<Real>& st)
 {
     constexpr int NW = frec_words(C);
-    P.ensure(st.p + NW - 1);
+    P.window(st.p, st.p + NW - 1);
     v4u w[NW / 4];
     read_record<NW>(P, st.p, w);
-    A.ensure(st.ap);
-    const Real* ring = A.ring + A.lane;
-    auto aval = [&](int i) { return ring[static_cast<size_t>((st.ap + i) & (AS::kRing - 1)) * kLanes]; };
+    A.window(st.ap, st.ap + 1 + 2 * C);
+    auto aval = [&](int i) { return A.at(st.ap + i); };
     auto slot = [&](unsigned ref) -> Real& { return st.sl[static_cast<size_t>(ref & kRefSlot) * kLanes]; };
     Real dv = aval(1) + slot(ref_of<1, NW>(w, 0));
     Real fk = aval(0) + slot(ref_of<1, NW>(w, 1));
@@ -269,8 +284,8 @@ __device__ __forceinline__ void factor_pivot(AS& A, ProgStream& P, FactorState<R
 #pragma unroll
     for (int q = 0; q < C; ++q) st.U[static_cast<size_t>(st.up + 2 + q) * kLanes] = uv[q];
     st.up += 2 + C;
-    // update rows, software-pipelined: row a+1's old values are read before
-    // row a is written (all targets are distinct slots)
+    // update rows: row a+1's old values are read before row a is written
+    // (all targets are distinct slots)
 #pragma unroll
     for (int a = 0; a < C; ++a) {
         Real(&cur)[C > 0 ? C : 1] = (a & 1) ? t1 : t0;
@@ -292,19 +307,18 @@ __device__ __forceinline__ void factor_pivot(AS& A, ProgStream& P, FactorState<R
 }
 
 // Any column length (up to kMaxC), one field at a time: the rare long columns.
-template <typename Real, typename AS>
-__device__ void factor_pivot_any(int c, AS& A, ProgStream& P, FactorState<Real>& st, Real* lsc, Real* usc)
+template <typename Real, typename AR>
+__device__ void factor_pivot_any(int c, AR& A, ProgReader& P, FactorState<Real>& st, Real* lsc, Real* usc)
 {
     const int nw = frec_words(c);
-    P.ensure(st.p + nw - 1);
+    P.window(st.p, st.p + nw - 1);
     auto ref = [&](int q) -> unsigned {
-        const unsigned v = static_cast<unsigned>(P.ring[(st.p + 1 + q / 2) & (ProgStream::kRing - 1)]);
+        const unsigned v = static_cast<unsigned>(P.at(st.p + 1 + q / 2));
         return (q & 1) ? (v >> 16) : (v & 0xFFFFu);
     };
     auto slot = [&](unsigned r) -> Real& { return st.sl[static_cast<size_t>(r & kRefSlot) * kLanes]; };
-    A.ensure(st.ap);
-    const Real* ring = A.ring + A.lane;
-    auto aval = [&](int i) { return ring[static_cast<size_t>((st.ap + i) & (AS::kRing - 1)) * kLanes]; };
+    A.window(st.ap, st.ap + 1 + 2 * c);
+    auto aval = [&](int i) { return A.at(st.ap + i); };
     Real dv = aval(1) + slot(ref(0));
     const Real fk = aval(0) + slot(ref(1));
     for (int q = 0; q < c; ++q) lsc[q * kLanes] = aval(2 + q) + slot(ref(2 + q));
@@ -334,8 +348,8 @@ __device__ void factor_pivot_any(int c, AS& A, ProgStream& P, FactorState<Real>&
     st.p += nw;
 }
 
-template <int C, typename Real, typename AS>
-__device__ __forceinline__ bool factor_dispatch(int c, AS& A, ProgStream& P, FactorState<Real>& st)
+template <int C, typename Real, typename AR>
+__device__ __forceinline__ bool factor_dispatch(int c, AR& A, ProgReader& P, FactorState<Real>& st)
 {
     if constexpr (C > kFastC) {
         return false;
@@ -349,43 +363,64 @@ __device__ __forceinline__ bool factor_dispatch(int c, AS& A, ProgStream& P, Fac
 }
 
 template <typename Real>
-__global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_factor(FactorArgs<Real> a)
+__global__ __launch_bounds__(kLanes * kStreamWaves) void k_factor(FactorArgs<Real> a)
 {
     using Sh = StreamShape<Real>;
-    using AStream = RingStream<Real, Sh::kCh, Sh::kNch, false, true>;
-    const int lane = threadIdx.x;
+    using ARing = Ring<Real, Sh::kCh, Sh::kNch, false, true>;
+    static_assert(sizeof(typename ARing::Ctl) + sizeof(ProgRing::Ctl) <= kCtlBytes, "control blocks");
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = threadIdx.x / kLanes;
     const size_t tile = blockIdx.x;
     if (a.tile_active[tile] <= 0) return;
-    const int row = static_cast<int>(tile) * kLanes + lane;
     extern __shared__ __align__(16) unsigned char g2g_smem[];
-    Real* aring = reinterpret_cast<Real*>(g2g_smem);
-    int* pring = reinterpret_cast<int*>(aring + AStream::kRing * kLanes);
-    Real* sl = reinterpret_cast<Real*>(pring + ProgStream::kRing);  // [slots + 2][64]
-    Real* lsc = sl + static_cast<size_t>(a.slots + 2) * kLanes;     // [kMaxC][64]
-    Real* usc = lsc + static_cast<size_t>(kMaxC) * kLanes;          // [kMaxC][64]
-    sl[static_cast<size_t>(a.slots) * kLanes + lane] = Real(0);     // the zero slot
-    AStream A;
-    A.init(a.A + tile * static_cast<size_t>(a.a_stride) * kLanes, a.a_stride / Sh::kCh, aring, lane);
-    ProgStream P;
-    P.init(a.prog, a.prog_stride / kProgChunk, pring, lane);
+    auto* ctlA = reinterpret_cast<typename ARing::Ctl*>(g2g_smem);
+    auto* ctlP = reinterpret_cast<ProgRing::Ctl*>(g2g_smem + sizeof(typename ARing::Ctl));
+    Real* aring = reinterpret_cast<Real*>(g2g_smem + kCtlBytes);
+    int* pring = reinterpret_cast<int*>(g2g_smem + kCtlBytes + ARing::kLdsBytes);
+    Real* sl = reinterpret_cast<Real*>(g2g_smem + kCtlBytes + ARing::kLdsBytes + ProgRing::kLdsBytes);
+    if (wave == 0) {
+        ARing::init_ctl(ctlA, lane);
+        ProgRing::init_ctl(ctlP, lane);
+    }
+    __syncthreads();
+    const Real* Atile = a.A + tile * static_cast<size_t>(a.a_stride) * kLanes;
+    if (wave >= 1 && wave <= kValueProducers) {
+        ARing::produce(Atile, a.a_stride / Sh::kCh, aring, ctlA, lane, wave - 1, kValueProducers);
+        return;
+    }
+    if (wave == kValueProducers + 1) {
+        ProgRing::produce(a.prog, a.prog_stride / kProgChunk, pring, ctlP, lane, 0, 1);
+        return;
+    }
+    // wave 0: the program walk
+    const int row = static_cast<int>(tile) * kLanes + lane;
+    Real* lsc = sl + static_cast<size_t>(a.slots + 2) * kLanes;  // [kMaxC][64]
+    Real* usc = lsc + static_cast<size_t>(kMaxC) * kLanes;       // [kMaxC][64]
+    sl[static_cast<size_t>(a.slots) * kLanes + lane] = Real(0);  // the zero slot
+    typename ARing::Reader A;
+    A.init(aring, ctlA, a.a_stride / Sh::kCh, lane);
+    ProgReader P;
+    P.init(pring, ctlP, a.prog_stride / kProgChunk, lane);
     FactorState<Real> st;
     st.sl = sl + lane;
     st.U = a.U + tile * static_cast<size_t>(a.u_stride) * kLanes + lane;
     st.eps = static_cast<Real>(a.pivot_eps) * static_cast<Real>(a.jmax ? a.jmax[row] : 1.0);
     st.clamped = 0;
     st.p = st.ap = st.up = 0;
-    int c = prog_at(P, 0) & 0xFFFF;
+    int c = prog_word(P, 0) & 0xFFFF;
     for (int k = 0; k < a.d; ++k) {
         // the header of this record announces the next record's column length
-        const int c_next = (prog_at(P, st.p) >> 16) & 0xFFFF;
+        const int c_next = (prog_word(P, st.p) >> 16) & 0xFFFF;
         if (!factor_dispatch<0>(c, A, P, st)) factor_pivot_any(c, A, P, st, lsc + lane, usc + lane);
         c = c_next;
     }
     if (a.clamped && row < a.rows) a.clamped[row] = st.clamped;
+    if ((A.failed || P.failed) && lane == 0) atomicOr(a.error, 1);
 }
 
 // ---------------------------------------------------------------------------
 // k_backward: U x = y in reverse postorder; x written in input labels
+// (same three-wavefront layout: walker, U-stream producer, program producer)
 // ---------------------------------------------------------------------------
 
 template <typename Real>
@@ -395,17 +430,16 @@ struct BackState {
     int p, up;
 };
 
-template <int C, typename Real, typename US>
-__device__ __forceinline__ void backward_pivot(US& U, ProgStream& P, BackState<Real>& st)
+template <int C, typename Real, typename UR>
+__device__ __forceinline__ void backward_pivot(UR& U, ProgReader& P, BackState<Real>& st)
 {
     constexpr int NW = brec_words(C);
-    P.ensure(st.p + NW - 1);
+    P.window(st.p, st.p + NW - 1);
     v4u w[NW / 4];
     read_record<NW>(P, st.p, w);
     st.up -= C + 2;
-    U.ensure(st.up + C + 1);  // the record is consumed from its top
-    const Real* ring = U.ring + U.lane;
-    auto uval = [&](int i) { return ring[static_cast<size_t>((st.up + i) & (US::kRing - 1)) * kLanes]; };
+    U.window(st.up + C + 1, st.up);  // the record is consumed from its top
+    auto uval = [&](int i) { return U.at(st.up + i); };
     auto slot = [&](unsigned ref) -> Real& { return st.sl[static_cast<size_t>(ref & kRefSlot) * kLanes]; };
     Real xj[C > 0 ? C : 1];
 #pragma unroll
@@ -420,31 +454,30 @@ __device__ __forceinline__ void backward_pivot(US& U, ProgStream& P, BackState<R
     st.p += NW;
 }
 
-template <typename Real, typename US>
-__device__ void backward_pivot_any(int c, US& U, ProgStream& P, BackState<Real>& st)
+template <typename Real, typename UR>
+__device__ void backward_pivot_any(int c, UR& U, ProgReader& P, BackState<Real>& st)
 {
     const int nw = brec_words(c);
-    P.ensure(st.p + nw - 1);
+    P.window(st.p, st.p + nw - 1);
     auto ref = [&](int q) -> unsigned {
-        const unsigned v = static_cast<unsigned>(P.ring[(st.p + 2 + q / 2) & (ProgStream::kRing - 1)]);
+        const unsigned v = static_cast<unsigned>(P.at(st.p + 2 + q / 2));
         return (q & 1) ? (v >> 16) : (v & 0xFFFFu);
     };
     auto slot = [&](unsigned r) -> Real& { return st.sl[static_cast<size_t>(r & kRefSlot) * kLanes]; };
     st.up -= c + 2;
-    U.ensure(st.up + c + 1);
-    const Real* ring = U.ring + U.lane;
-    auto uval = [&](int i) { return ring[static_cast<size_t>((st.up + i) & (US::kRing - 1)) * kLanes]; };
+    U.window(st.up + c + 1, st.up);
+    auto uval = [&](int i) { return U.at(st.up + i); };
     Real s = uval(0);
     for (int q = 0; q < c; ++q) s = fma(-uval(2 + q), slot(ref(q)), s);
     const Real x = s / uval(1);
-    const int perm_k = P.ring[(st.p + 1) & (ProgStream::kRing - 1)];
+    const int perm_k = P.at(st.p + 1);
     st.X[static_cast<size_t>(perm_k) * kLanes] = x;
     slot(ref(c)) = x;
     st.p += nw;
 }
 
-template <int C, typename Real, typename US>
-__device__ __forceinline__ bool backward_dispatch(int c, US& U, ProgStream& P, BackState<Real>& st)
+template <int C, typename Real, typename UR>
+__device__ __forceinline__ bool backward_dispatch(int c, UR& U, ProgReader& P, BackState<Real>& st)
 {
     if constexpr (C > kFastC) {
         return false;
@@ -458,32 +491,51 @@ __device__ __forceinline__ bool backward_dispatch(int c, US& U, ProgStream& P, B
 }
 
 template <typename Real>
-__global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_backward(BackwardArgs<Real> a)
+__global__ __launch_bounds__(kLanes * kStreamWaves) void k_backward(BackwardArgs<Real> a)
 {
     using Sh = StreamShape<Real>;
-    using UStream = RingStream<Real, Sh::kCh, Sh::kNch, true, true>;
-    const int lane = threadIdx.x;
+    using URing = Ring<Real, Sh::kCh, Sh::kNch, true, true>;
+    static_assert(sizeof(typename URing::Ctl) + sizeof(ProgRing::Ctl) <= kCtlBytes, "control blocks");
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = threadIdx.x / kLanes;
     const size_t tile = blockIdx.x;
     if (a.tile_active[tile] <= 0) return;
     extern __shared__ __align__(16) unsigned char g2g_smem[];
-    Real* uring = reinterpret_cast<Real*>(g2g_smem);
-    int* pring = reinterpret_cast<int*>(uring + UStream::kRing * kLanes);
-    Real* sl = reinterpret_cast<Real*>(pring + ProgStream::kRing);  // [slots + 2][64]
-    UStream U;
-    U.init(a.U + tile * static_cast<size_t>(a.u_stride) * kLanes, a.u_stride / Sh::kCh, uring, lane);
-    ProgStream P;
-    P.init(a.prog, a.prog_stride / kProgChunk, pring, lane);
+    auto* ctlU = reinterpret_cast<typename URing::Ctl*>(g2g_smem);
+    auto* ctlP = reinterpret_cast<ProgRing::Ctl*>(g2g_smem + sizeof(typename URing::Ctl));
+    Real* uring = reinterpret_cast<Real*>(g2g_smem + kCtlBytes);
+    int* pring = reinterpret_cast<int*>(g2g_smem + kCtlBytes + URing::kLdsBytes);
+    Real* sl = reinterpret_cast<Real*>(g2g_smem + kCtlBytes + URing::kLdsBytes + ProgRing::kLdsBytes);
+    if (wave == 0) {
+        URing::init_ctl(ctlU, lane);
+        ProgRing::init_ctl(ctlP, lane);
+    }
+    __syncthreads();
+    if (wave >= 1 && wave <= kValueProducers) {
+        URing::produce(a.U + tile * static_cast<size_t>(a.u_stride) * kLanes, a.u_stride / Sh::kCh, uring, ctlU,
+                       lane, wave - 1, kValueProducers);
+        return;
+    }
+    if (wave == kValueProducers + 1) {
+        ProgRing::produce(a.prog, a.prog_stride / kProgChunk, pring, ctlP, lane, 0, 1);
+        return;
+    }
+    typename URing::Reader U;
+    U.init(uring, ctlU, a.u_stride / Sh::kCh, lane);
+    ProgReader P;
+    P.init(pring, ctlP, a.prog_stride / kProgChunk, lane);
     BackState<Real> st;
     st.sl = sl + lane;
     st.X = a.X + tile * static_cast<size_t>(a.d) * kLanes + lane;
     st.p = 0;
     st.up = a.u_len;
-    int c = prog_at(P, 0) & 0xFFFF;
+    int c = prog_word(P, 0) & 0xFFFF;
     for (int q = 0; q < a.d; ++q) {
-        const int c_next = (prog_at(P, st.p) >> 16) & 0xFFFF;
+        const int c_next = (prog_word(P, st.p) >> 16) & 0xFFFF;
         if (!backward_dispatch<0>(c, U, P, st)) backward_pivot_any(c, U, P, st);
         c = c_next;
     }
+    if ((U.failed || P.failed) && lane == 0) atomicOr(a.error, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -628,6 +680,49 @@ __global__ void k_count_tiles(const int* __restrict__ status, int rows_pad, int*
     const int row = t * kLanes + lane;
     const unsigned long long m = __ballot(row < rows_pad && status[row] == kActive);
     if (lane == 0) tile_active[t] = __popcll(m);
+}
+
+// Re-queue of the rows a warm-started pass left unconverged: their V goes back
+// to the flat start V0 (one thread per tiled complex value) ...
+__global__ void k_requeue_v(const int* __restrict__ status, int rows, int n, const double2* __restrict__ V0,
+                            double2* __restrict__ V)
+{
+    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long T = (rows + kLanes - 1) / kLanes;
+    if (gid >= T * n * kLanes) return;
+    const int lane = static_cast<int>(gid & (kLanes - 1));
+    const long long tile = gid / (static_cast<long long>(n) * kLanes);
+    const int row = static_cast<int>(tile) * kLanes + lane;
+    if (row >= rows) return;
+    const int st = status[row];
+    if (st == kMaxIter || st == kDiverged || st == kRejected) V[gid] = V0[gid];
+}
+
+// ... then their status becomes active again, with the warm pass's update
+// count moved to iters_warm
+__global__ void k_requeue_rows(int* __restrict__ status, int* __restrict__ iters, int* __restrict__ iters_warm,
+                               int rows, int* __restrict__ n_requeued)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    int re = 0;
+    if (row < rows) {
+        const int st = status[row];
+        re = st == kMaxIter || st == kDiverged || st == kRejected;
+        iters_warm[row] = iters[row];
+        if (re) {
+            status[row] = kActive;
+            iters[row] = 0;
+        }
+    }
+    const unsigned long long m = __ballot(re);
+    if ((threadIdx.x & (kLanes - 1)) == 0 && m) atomicAdd(n_requeued, __popcll(m));
+}
+
+// iters[r] += add[r]
+__global__ void k_add_int(int* __restrict__ p, const int* __restrict__ add, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] += add[i];
 }
 
 // ---------------------------------------------------------------------------

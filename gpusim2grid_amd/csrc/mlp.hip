@@ -1,0 +1,300 @@
+// Learned warm start: the MLP's training and inference kernels (gfx950).
+//
+// A 3-layer MLP (in -> hidden -> hidden -> out, GELU) maps a row's per-bus
+// injections (and outage mask) to its voltage angles and magnitudes.  Every
+// product runs on the bf16 matrix cores with fp32 accumulation
+// (v_mfma_f32_32x32x16_bf16): one tiled GEMM kernel, templated on the operand
+// layouts (each operand row- or column-major, which covers the forward
+// X W, the backward dY W^T and the weight gradient X^T dY) and on a fused
+// epilogue (bias + GELU with the pre-activation kept for the backward pass,
+// bias only, GELU derivative).  Weights are kept in fp32 (Adam state) with a
+// bf16 copy the GEMMs read.
+//
+// The gpusim2grid reference has no learned component; this follows the
+// model the reference's paper benchmarks time as "PFdelta" MLPs
+// (20260831_neurips_paper/scripts/README.md, ml_injection.py) only in kind.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mlp.hpp"
+
+namespace g2g {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBM = 128, kBN = 128, kBK = 32;
+constexpr int kPadK = kBK + 8;  // LDS row pitch (bf16): 80 bytes, spreads the 16-byte reads over banks
+
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+
+__device__ __forceinline__ float gelu_grad(float x)
+{
+    const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
+    const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
+    return cdf + x * pdf;
+}
+
+// A (M x K) and B (K x N) tiles of one K step, staged k-contiguous in LDS:
+// sA[m][k], sB[n][k].
+template <bool TA, bool TB>
+__device__ __forceinline__ void load_tiles(const GemmArgs& g, int m0, int n0, int k0, __bf16 (*sA)[kPadK],
+                                           __bf16 (*sB)[kPadK])
+{
+    const int t = threadIdx.x;  // 256 threads
+    const __bf16 zero = static_cast<__bf16>(0.0f);
+    // ---- A: 128 x 32 ------------------------------------------------------
+    if (!TA) {  // element (m, k) at A[m * lda + k]
+        const int m = t >> 1, kh = (t & 1) * 16;
+        const int gm = m0 + m, gk = k0 + kh;
+        const __bf16* src = g.A + static_cast<long long>(gm) * g.lda + gk;
+        if (gm < g.M && gk + 16 <= g.K && (reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+            const u32x4 v0 = reinterpret_cast<const u32x4*>(src)[0];
+            const u32x4 v1 = reinterpret_cast<const u32x4*>(src)[1];
+            *reinterpret_cast<u32x4*>(&sA[m][kh]) = v0;
+            *reinterpret_cast<u32x4*>(&sA[m][kh + 8]) = v1;
+        } else {
+            for (int i = 0; i < 16; ++i) sA[m][kh + i] = (gm < g.M && gk + i < g.K) ? src[i] : zero;
+        }
+    } else {  // element (m, k) at A[k * lda + m]
+        const int k = t >> 3, mh = (t & 7) * 16;
+        const int gk = k0 + k, gm = m0 + mh;
+        const __bf16* src = g.A + static_cast<long long>(gk) * g.lda + gm;
+        for (int i = 0; i < 16; ++i) sA[mh + i][k] = (gk < g.K && gm + i < g.M) ? src[i] : zero;
+    }
+    // ---- B: 32 x 128 ------------------------------------------------------
+    if (TB) {  // element (k, n) at B[n * ldb + k]
+        const int n = t >> 1, kh = (t & 1) * 16;
+        const int gn = n0 + n, gk = k0 + kh;
+        const __bf16* src = g.B + static_cast<long long>(gn) * g.ldb + gk;
+        if (gn < g.N && gk + 16 <= g.K && (reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+            const u32x4 v0 = reinterpret_cast<const u32x4*>(src)[0];
+            const u32x4 v1 = reinterpret_cast<const u32x4*>(src)[1];
+            *reinterpret_cast<u32x4*>(&sB[n][kh]) = v0;
+            *reinterpret_cast<u32x4*>(&sB[n][kh + 8]) = v1;
+        } else {
+            for (int i = 0; i < 16; ++i) sB[n][kh + i] = (gn < g.N && gk + i < g.K) ? src[i] : zero;
+        }
+    } else {  // element (k, n) at B[k * ldb + n]
+        const int k = t >> 3, nh = (t & 7) * 16;
+        const int gk = k0 + k, gn = n0 + nh;
+        const __bf16* src = g.B + static_cast<long long>(gk) * g.ldb + gn;
+        for (int i = 0; i < 16; ++i) sB[nh + i][k] = (gk < g.K && gn + i < g.N) ? src[i] : zero;
+    }
+}
+
+// C = A B, 128 x 128 output tile per workgroup of 4 waves (each 64 x 64 as
+// 2 x 2 MFMA 32 x 32 tiles), K in steps of 32 through LDS.
+template <bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
+{
+    __shared__ __align__(16) __bf16 sA[kBM][kPadK];
+    __shared__ __align__(16) __bf16 sB[kBN][kPadK];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+    const int lr = lane & 31, lk = (lane >> 5) * 8;
+    for (int k0 = 0; k0 < g.K; k0 += kBK) {
+        load_tiles<TA, TB>(g, m0, n0, k0, sA, sB);
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kBK; kk += 16) {
+            bf16x8 fa[2], fb[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                fa[i] = *reinterpret_cast<const bf16x8*>(&sA[wm * 64 + i * 32 + lr][kk + lk]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                fb[j] = *reinterpret_cast<const bf16x8*>(&sB[wn * 64 + j * 32 + lr][kk + lk]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // epilogue: lane holds column lane % 32 and rows 8 (r / 4) + 4 (lane / 32) + r % 4
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int n = n0 + wn * 64 + j * 32 + lr;
+                if (m >= g.M || n >= g.N) continue;
+                const float v = acc[i][j][r];
+                const long long mc = static_cast<long long>(m);
+                if (EPI == kEpiF32) {
+                    g.C[mc * g.ldc + n] = v;
+                } else if (EPI == kEpiBiasF32) {
+                    g.C[mc * g.ldc + n] = v + g.bias[n];
+                } else if (EPI == kEpiBiasGeluBf16) {
+                    const float z = v + g.bias[n];
+                    g.Z[mc * g.ldz + n] = z;
+                    g.H[mc * g.ldh + n] = static_cast<__bf16>(gelu(z));
+                } else {  // kEpiGeluBwd: dZ = (dH) * gelu'(Z), fp32 and bf16
+                    const float dz = v * gelu_grad(g.Z[mc * g.ldz + n]);
+                    g.C[mc * g.ldc + n] = dz;
+                    g.H[mc * g.ldh + n] = static_cast<__bf16>(dz);
+                }
+            }
+}
+
+// column sums of an fp32 M x N matrix (bias gradients), one thread per column
+__global__ void k_colsum(const float* __restrict__ X, long long ldx, int M, int N, float* __restrict__ out)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float s = 0.0f;
+    for (int m = 0; m < M; ++m) s += X[static_cast<long long>(m) * ldx + n];
+    out[n] = s;
+}
+
+// MSE loss gradient dY = 2 (Y - T) / (M N) (fp32 and bf16) and the loss sum
+__global__ void k_mse_grad(const float* __restrict__ Y, const float* __restrict__ T, long long n_el, float scale,
+                           float* __restrict__ dY, __bf16* __restrict__ dYb, float* __restrict__ loss)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    float e = 0.0f;
+    if (i < n_el) {
+        const float d = Y[i] - T[i];
+        e = d * d;
+        dY[i] = 2.0f * scale * d;
+        dYb[i] = static_cast<__bf16>(2.0f * scale * d);
+    }
+    // wave reduction, one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, e * scale);
+}
+
+// Adam on fp32 master weights, refreshing the bf16 copy (nullable)
+__global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                       const float* __restrict__ g, long long n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, __bf16* __restrict__ wb)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = static_cast<__bf16>(wi);
+}
+
+__global__ void k_f32_to_bf16(const float* __restrict__ x, long long n, __bf16* __restrict__ y)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = static_cast<__bf16>(x[i]);
+}
+
+// Features of a row: per bus (P - mu) / sigma, (Q - mu) / sigma from its
+// complex Sbus row, then the (optional) 0/1 outage mask -> bf16 [M][F]
+__global__ void k_features(const double2* __restrict__ S, long long lds, int M, int n, const float* __restrict__ mu,
+                           const float* __restrict__ sigma, const unsigned char* __restrict__ mask, int n_mask,
+                           __bf16* __restrict__ X, long long ldx)
+{
+    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int F = 2 * n + n_mask;
+    if (gid >= static_cast<long long>(M) * F) return;
+    const int r = static_cast<int>(gid / F);
+    const int f = static_cast<int>(gid - static_cast<long long>(r) * F);
+    float v;
+    if (f < 2 * n) {
+        const double2 s = S[static_cast<long long>(r) * lds + (f >> 1)];
+        const double x = (f & 1) ? s.y : s.x;
+        v = static_cast<float>((x - mu[f]) / sigma[f]);
+    } else {
+        v = static_cast<float>(mask[static_cast<long long>(r) * n_mask + (f - 2 * n)]);
+    }
+    X[static_cast<long long>(r) * ldx + f] = static_cast<__bf16>(v);
+}
+
+// Start voltages from the network output Y (normalised Va, Vm per bus):
+// V = Vm exp(j Va) with Vm held at the set-point where it is fixed (PV, slack)
+// and Va = 0 at the slack.
+__global__ void k_warm_v(const float* __restrict__ Y, long long ldy, int M, int n, const float* __restrict__ mu,
+                         const float* __restrict__ sigma, const double* __restrict__ vfix,
+                         const int* __restrict__ kind, double2* __restrict__ V, long long ldv)
+{
+    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= static_cast<long long>(M) * n) return;
+    const int r = static_cast<int>(gid / n);
+    const int b = static_cast<int>(gid - static_cast<long long>(r) * n);
+    const float* y = Y + static_cast<long long>(r) * ldy;
+    double va = static_cast<double>(y[2 * b]) * sigma[2 * b] + mu[2 * b];
+    double vm = static_cast<double>(y[2 * b + 1]) * sigma[2 * b + 1] + mu[2 * b + 1];
+    if (kind[b] != 1) vm = vfix[b];  // PV / slack: the set-point
+    if (kind[b] == 2) va = 0.0;      // slack
+    double sn, cs;
+    sincos(va, &sn, &cs);
+    V[static_cast<long long>(r) * ldv + b] = make_double2(vm * cs, vm * sn);
+}
+
+// tiled-layout variants: a batch's Sbus / V are stored [row / 64][bus][row % 64]
+__device__ __forceinline__ long long tiled(int r, int n, int b)
+{
+    return (static_cast<long long>(r >> 6) * n + b) * 64 + (r & 63);
+}
+
+__global__ void k_features_tiles(const double2* __restrict__ S_tiles, int M, int n, const float* __restrict__ mu,
+                                 const float* __restrict__ sigma, const unsigned char* __restrict__ mask, int n_mask,
+                                 __bf16* __restrict__ X, long long ldx)
+{
+    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int F = 2 * n + n_mask;
+    if (gid >= static_cast<long long>(M) * F) return;
+    const int r = static_cast<int>(gid / F);
+    const int f = static_cast<int>(gid - static_cast<long long>(r) * F);
+    float v;
+    if (f < 2 * n) {
+        const double2 s = S_tiles[tiled(r, n, f >> 1)];
+        const double x = (f & 1) ? s.y : s.x;
+        v = static_cast<float>((x - mu[f]) / sigma[f]);
+    } else {
+        v = static_cast<float>(mask[static_cast<long long>(r) * n_mask + (f - 2 * n)]);
+    }
+    X[static_cast<long long>(r) * ldx + f] = static_cast<__bf16>(v);
+}
+
+__global__ void k_warm_v_tiles(const float* __restrict__ Y, long long ldy, int M, int n, const float* __restrict__ mu,
+                               const float* __restrict__ sigma, const double* __restrict__ vfix,
+                               const int* __restrict__ kind, double2* __restrict__ V_tiles, int row0)
+{
+    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= static_cast<long long>(M) * n) return;
+    const int r = static_cast<int>(gid / n);
+    const int b = static_cast<int>(gid - static_cast<long long>(r) * n);
+    const float* y = Y + static_cast<long long>(r) * ldy;
+    double va = static_cast<double>(y[2 * b]) * sigma[2 * b] + mu[2 * b];
+    double vm = static_cast<double>(y[2 * b + 1]) * sigma[2 * b + 1] + mu[2 * b + 1];
+    if (kind[b] != 1) vm = vfix[b];
+    if (kind[b] == 2) va = 0.0;
+    double sn, cs;
+    sincos(va, &sn, &cs);
+    V_tiles[tiled(row0 + r, n, b)] = make_double2(vm * cs, vm * sn);
+}
+
+template __global__ void k_gemm<false, false, kEpiF32>(GemmArgs);
+template __global__ void k_gemm<false, false, kEpiBiasF32>(GemmArgs);
+template __global__ void k_gemm<false, false, kEpiBiasGeluBf16>(GemmArgs);
+template __global__ void k_gemm<false, true, kEpiF32>(GemmArgs);
+template __global__ void k_gemm<false, true, kEpiGeluBwd>(GemmArgs);
+template __global__ void k_gemm<true, false, kEpiF32>(GemmArgs);
+template __global__ void k_gemm<true, true, kEpiF32>(GemmArgs);
+
+}  // namespace g2g

@@ -9,17 +9,20 @@
 namespace g2g {
 
 constexpr int kLanes = 64;          // rows per wavefront (one lane per batch row)
-constexpr int kMisWaves = 4;        // wavefronts per tile in the mismatch kernel
+constexpr int kMisWaves = 16;       // wavefronts per tile in the mismatch and update kernels
 constexpr int kMaxC = 16;           // longest off-diagonal L column the kernels unroll for
 
 constexpr int kFastC = 10;          // column lengths with a specialised kernel path
 
 // stream prefetch geometry (streams.hpp): positions per chunk and ring slots
 constexpr int kChunkF64 = 8;        // 4 KB chunks of a wavefront's FP64 stream
+constexpr int kRingSlotsF64 = 8;    // 32 KB ring
 constexpr int kChunkF32 = 16;       // 4 KB chunks of a wavefront's FP32 stream
-constexpr int kRingSlots = 16;      // 64 KB ring per wavefront
+constexpr int kRingSlotsF32 = 16;   // 64 KB ring
+constexpr int kValueProducers = 6;  // wavefronts streaming a value stream (one chunk in flight each)
 constexpr int kProgChunk = 512;     // program words per chunk (2 KB)
-constexpr int kProgSlots = 2;
+constexpr int kProgSlots = 4;       // 8 KB ring
+constexpr int kStreamWaves = 2 + kValueProducers;  // walker + value producers + program producer
 
 // Programs (one record per pivot, 4-word aligned, length fixed by c):
 //   factorization  [c | c_next << 16] then 16-bit refs
@@ -44,5 +47,7 @@ constexpr int kConverged = 0;
 constexpr int kMaxIter = 1;
 constexpr int kDiverged = 2;
 constexpr int kNotSimulated = 3;
+// internal: a warm-started row whose mismatch grew (re-queued to a flat start)
+constexpr int kRejected = 4;
 
 }  // namespace g2g

@@ -2,11 +2,18 @@
 //
 // A wavefront walks its factorization / substitution program strictly in
 // order, so every global input it needs -- the program words and its tile's
-// value stream -- is read sequentially.  RingStream keeps the next chunks of
-// such a stream in an LDS ring: two register stages hold the chunks in flight
-// (loaded with 16-byte-per-lane coalesced loads) and are written into the
-// ring once the chunk two slots behind has been consumed, so a read at the
-// consumption point finds its data in LDS instead of waiting on HBM.
+// value stream -- is read sequentially.  Each such stream gets an LDS ring of
+// NCH chunks filled by W producer wavefronts of the same workgroup: producer
+// w loads chunks w, w + W, w + 2W, ... (one chunk in flight per producer, so
+// no load is carried across loop iterations and its waits are exact), writes
+// each into its ring slot and tags the slot with the chunk number.  The
+// consumer wavefront (the one walking the program) only reads LDS: it waits
+// for the tags of the chunks it needs and publishes `lo`, the oldest chunk it
+// may still read, which frees the older slots for the producers.
+//
+// Every wait is bounded: after kSpinLimit polls a wave sets the abort flag
+// and leaves, and the other waves leave once they see it, so the grid always
+// drains (the kernel then reports an error).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -18,110 +25,150 @@ namespace g2g {
 // 16-byte register type (a native vector, so arrays of it stay in VGPRs)
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
-// T: element type.  CH: positions per chunk.  NCH: ring slots (>= 2).
+constexpr int kSpinLimit = 1 << 22;  // polls (each after s_sleep 2) before a wave gives up
+
+// Flow-control flags.  The LDS executes a wavefront's DS instructions in
+// order and is coherent across the workgroup, so a producer's ring writes
+// land before its later tag write, and a consumer's ring reads issued after
+// it saw a tag see the data.  The flags are therefore relaxed atomics fenced
+// against compiler reordering only: release/acquire at workgroup scope would
+// also wait for every outstanding global access of the wave (the consumer's
+// U-stream stores, the producer's loads), which is exactly the latency the
+// rings are there to hide.
+__device__ __forceinline__ int lds_load_acquire(const int* p)
+{
+    const int v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    return v;
+}
+
+__device__ __forceinline__ void lds_store_release(int* p, int v)
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <int NCH>
+struct RingCtl {
+    int lo;          // oldest chunk the consumer may still read
+    int abort;       // set by a wave that gave up waiting
+    int tag[NCH];    // chunk number + 1 held by each slot (0: empty)
+};
+
+// T: element type.  CH: positions per chunk.  NCH: ring slots.
 // PerLane: element (pos, lane) stored [pos][64] (a tile's value stream); else
 // one element per position shared by every lane (the program).
 // Reverse: the stream is consumed from its end towards position 0.
 // The global buffer must be readable up to nch * CH positions.
 template <typename T, int CH, int NCH, bool Reverse, bool PerLane>
-struct RingStream {
+struct Ring {
     static constexpr int kElems = PerLane ? CH * kLanes : CH;
     static constexpr int kBytes = kElems * static_cast<int>(sizeof(T));
     static constexpr int kV4 = kBytes / 16 / kLanes;  // 16-byte loads per lane per chunk
     static_assert(kV4 >= 1 && kBytes % (16 * kLanes) == 0, "chunk must be a whole number of wave loads");
     static_assert((CH & (CH - 1)) == 0 && (NCH & (NCH - 1)) == 0, "powers of two");
     static constexpr int kRing = CH * NCH;
+    static constexpr int kLdsBytes = kBytes * NCH;
+    using Ctl = RingCtl<NCH>;
 
-    const v4u* g = nullptr;  // stream base (chunk 0)
-    T* ring = nullptr;         // LDS, NCH chunks
-    int nch = 0;               // chunks in the stream
-    int cur = 0;               // current chunk, in consumption order
-    int lane = 0;
-    v4u s0[kV4], s1[kV4];
+    // global chunk of consumption-order chunk c
+    __device__ static __forceinline__ int phys(int c, int nch) { return Reverse ? nch - 1 - c : c; }
+    // consumption-order chunk of global position p
+    __device__ static __forceinline__ int chunk_of(int p, int nch) { return Reverse ? nch - 1 - p / CH : p / CH; }
 
-    __device__ __forceinline__ int phys(int c) const { return Reverse ? nch - 1 - c : c; }
-
-    __device__ __forceinline__ void load(v4u (&st)[kV4], int c)
+    __device__ static void init_ctl(Ctl* ctl, int lane)
     {
-        if (c < nch) {
-            const v4u* src = g + static_cast<size_t>(phys(c)) * (kBytes / 16) + lane;
+        if (lane == 0) {
+            ctl->lo = 0;
+            ctl->abort = 0;
+        }
+        for (int i = lane; i < NCH; i += kLanes) ctl->tag[i] = 0;
+    }
+
+    // ---- producer wavefront w of W ---------------------------------------
+    __device__ static void produce(const void* base, int nch, T* ring, Ctl* ctl, int lane, int w, int W)
+    {
+        const v4u* g = static_cast<const v4u*>(base);
+        v4u* r = reinterpret_cast<v4u*>(ring);
+        int lo = 0;
+        for (int c = w; c < nch; c += W) {
+            v4u st[kV4];
+            const v4u* src = g + static_cast<size_t>(phys(c, nch)) * (kBytes / 16) + lane;
 #pragma unroll
             for (int i = 0; i < kV4; ++i) st[i] = src[i * kLanes];
-        }
-    }
-
-    __device__ __forceinline__ void store(const v4u (&st)[kV4], int c)
-    {
-        if (c < nch) {
-            v4u* dst = reinterpret_cast<v4u*>(ring) + static_cast<size_t>(phys(c) & (NCH - 1)) * (kBytes / 16) + lane;
+            // the slot of chunk c is free once the consumer moved past c - NCH
+            int spins = 0;
+            while (c - lo >= NCH) {
+                lo = lds_load_acquire(&ctl->lo);
+                if (c - lo < NCH) break;
+                if (lds_load_acquire(&ctl->abort) || ++spins > kSpinLimit) {
+                    lds_store_release(&ctl->abort, 1);
+                    return;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            // data slot by global chunk (what Reader::at addresses), tag by
+            // consumption-order chunk (what Reader::wait checks)
+            v4u* dst = r + static_cast<size_t>(phys(c, nch) & (NCH - 1)) * (kBytes / 16) + lane;
 #pragma unroll
             for (int i = 0; i < kV4; ++i) dst[i * kLanes] = st[i];
+            lds_store_release(&ctl->tag[c & (NCH - 1)], c + 1);
         }
     }
 
-    __device__ __forceinline__ void init(const void* base, int n_chunks, T* lds, int lane_id)
-    {
-        g = static_cast<const v4u*>(base);
-        nch = n_chunks;
-        ring = lds;
-        lane = lane_id;
-        cur = 0;
-#pragma unroll
-        for (int c = 0; c < NCH - 1; ++c) {
-            load(s0, c);
-            store(s0, c);
-        }
-        if ((NCH - 1) & 1) {
-            load(s1, NCH - 1);
-            load(s0, NCH);
-        } else {
-            load(s0, NCH - 1);
-            load(s1, NCH);
-        }
-    }
+    // ---- consumer wavefront ----------------------------------------------
+    struct Reader {
+        const T* ring;   // ring base (+ this lane for per-lane streams)
+        Ctl* ctl;
+        int nch;
+        int ready;       // chunks [lo, ready) known to be in the ring
+        int lo;          // published lo
+        bool failed;
 
-    // chunk `cur` is done: the chunk NCH-1 ahead enters the ring, the one
-    // after the other stage starts loading
-    __device__ __forceinline__ void advance()
-    {
-        const int w = cur + NCH - 1;
-        // the distinct markers keep the two stage paths from being merged
-        // into one path over a pointer select, which would force the stages
-        // into scratch memory
-        if (w & 1) {
-            asm volatile("; ring stage 1 begin");
-            store(s1, w);
-            load(s1, w + 2);
-            asm volatile("; ring stage 1 end");
-        } else {
-            asm volatile("; ring stage 0 begin");
-            store(s0, w);
-            load(s0, w + 2);
-            asm volatile("; ring stage 0 end");
+        __device__ __forceinline__ void init(T* r, Ctl* c, int n, int lane)
+        {
+            ring = PerLane ? r + lane : r;
+            ctl = c;
+            nch = n;
+            ready = 0;
+            lo = 0;
+            failed = false;
         }
-        ++cur;
-    }
 
-    // makes position p (global index) readable; p must not go backwards in
-    // consumption order by more than the current chunk
-    __device__ __forceinline__ void ensure(int p)
-    {
-        const int c = Reverse ? nch - 1 - p / CH : p / CH;
-        while (cur < c) advance();
-    }
+        // makes the positions from `first` to `last` (consumption order, at
+        // most (NCH - 2) * CH apart) readable; frees the chunks before first's
+        __device__ __forceinline__ void window(int first, int last)
+        {
+            const int cf = chunk_of(first, nch);
+            const int cl = chunk_of(last, nch);
+            if (cf - 1 > lo) {
+                lo = cf - 1;
+                lds_store_release(&ctl->lo, lo);
+            }
+            if (ready < cf) ready = cf;  // chunks before the window are not needed
+            while (ready <= cl) wait(ready++);
+        }
 
-    __device__ __forceinline__ T get(int p)
-    {
-        ensure(p);
-        return PerLane ? ring[static_cast<size_t>(p & (kRing - 1)) * kLanes + lane] : ring[p & (kRing - 1)];
-    }
+        __device__ void wait(int c)
+        {
+            if (failed) return;
+            int spins = 0;
+            const int* tag = &ctl->tag[c & (NCH - 1)];
+            while (lds_load_acquire(tag) != c + 1) {
+                if (lds_load_acquire(&ctl->abort) || ++spins > kSpinLimit) {
+                    lds_store_release(&ctl->abort, 1);
+                    failed = true;  // the results are discarded
+                    return;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+
+        __device__ __forceinline__ T at(int p) const
+        {
+            return PerLane ? ring[static_cast<size_t>(p & (kRing - 1)) * kLanes] : ring[p & (kRing - 1)];
+        }
+    };
 };
-
-// A program word, made wave-uniform.
-template <typename S>
-__device__ __forceinline__ int prog_at(S& s, int p)
-{
-    return __builtin_amdgcn_readfirstlane(s.get(p));
-}
 
 }  // namespace g2g

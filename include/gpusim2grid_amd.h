@@ -57,6 +57,7 @@ extern "C" {
 
 typedef struct g2g_plan_s *g2g_plan_t;
 typedef struct g2g_batch_s *g2g_batch_t;
+typedef struct g2g_mlp_s *g2g_mlp_t;
 
 typedef struct {
     int32_t n, nY, d, nJ, npvpq, npq, slack; /* grid (0 / -1 for a raw matrix plan)   */
@@ -75,6 +76,8 @@ typedef struct {
     int64_t bytes_factor;   /* A- and U-stream bytes of one factorization launch  */
     int64_t rows_factored;  /* rows summed over the factorization launches       */
     int32_t n_factor_launches;
+    int32_t n_requeued;     /* warm solves: rows restarted from the caller's V   */
+    double ms_warm_start;   /* warm solves: MLP inference                        */
 } g2g_solve_stats;
 
 const char *g2g_last_error(void);
@@ -137,6 +140,18 @@ int g2g_batch_set_active_rows(g2g_batch_t batch, int32_t active_rows);
 int g2g_batch_solve(g2g_batch_t batch, int32_t max_iter, double tol, int32_t scheduling, g2g_solve_stats *stats);
 
 /* Results.  V: rows x n complex, row stride ld (complex numbers). */
+/* Learned warm start with fallback (no counterpart in the reference, whose
+ * rows start from the base case or a caller's V, python_bindings.cpp:2722-2736).
+ * The MLP (g2g_mlp_*, below) maps each active row's Sbus (and its outage
+ * mask: host, active_rows x n_mask bytes, 0/1; n_mask = 0 for none) to start
+ * voltages; continuous Newton runs at most k_warm updates from there; a row
+ * whose ||F||inf exceeds growth x its previous value (growth > 0) or that is
+ * not converged after k_warm updates restarts from the batch's V as set by
+ * g2g_batch_set_v, with max_iter updates.  Statuses follow the second pass;
+ * iteration counts add both passes. */
+int g2g_batch_solve_warm(g2g_batch_t batch, g2g_mlp_t mlp, const uint8_t *mask, int32_t n_mask, int32_t k_warm,
+                         double growth, int32_t max_iter, double tol, g2g_solve_stats *stats);
+
 int g2g_batch_get_v(g2g_batch_t batch, double *V, int64_t ld, int32_t on_device);
 int g2g_batch_get_results(g2g_batch_t batch, int32_t *status, int32_t *iters, double *residual);
 
@@ -155,6 +170,39 @@ int g2g_lu_solve_batch(g2g_plan_t plan, int32_t rows, int32_t precision, const d
 /* Device pointer of the batch's tiled Sbus array [ceil(rows/64)][n][64]
  * complex (bench and zero-copy callers fill it directly). */
 int g2g_batch_device_arrays(g2g_batch_t batch, void **V_tiles, void **S_tiles);
+
+/* ---- Learned warm start (no counterpart in the reference, whose ML models are
+ * random-init timing stand-ins,
+ * 20260831_neurips_paper/scripts/ml_injection.py).  A 3-layer GELU MLP,
+ * in -> H -> H -> out, bf16 MFMA products with fp32 accumulation, fp32
+ * master weights and Adam state on the device.  Matrices are row-major
+ * float32 (inputs M x n_in, outputs M x n_out); on_device = 1 when they point
+ * to device memory.  Widths must be multiples of 8. */
+int g2g_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_out, uint64_t seed, void *stream, g2g_mlp_t *out);
+void g2g_mlp_destroy(g2g_mlp_t mlp);
+int64_t g2g_mlp_num_params(g2g_mlp_t mlp);
+/* offsets of W1 [n_in x H], b1, W2 [H x H], b2, W3 [H x n_out], b3 in the
+ * flat parameter vector (6 values) */
+int g2g_mlp_param_offsets(g2g_mlp_t mlp, int64_t *offsets);
+int g2g_mlp_get_params(g2g_mlp_t mlp, float *out, int64_t capacity);
+int g2g_mlp_set_params(g2g_mlp_t mlp, const float *in, int64_t n);
+/* feature and output normalisation: x_feat = (x - mu) / sigma, y = mu + sigma * out */
+int g2g_mlp_set_normalization(g2g_mlp_t mlp, const float *in_mu, const float *in_sigma, const float *out_mu,
+                              const float *out_sigma);
+/* bus types for warm starts: kind 0 PV, 1 PQ, 2 slack; vfix = |V| setpoint of PV/slack buses */
+int g2g_mlp_set_buses(g2g_mlp_t mlp, const int32_t *kind, const double *vfix, int32_t n);
+int g2g_mlp_forward(g2g_mlp_t mlp, const float *X, int32_t M, float *Y, int32_t on_device);
+/* mean-squared-error loss over all M x n_out outputs and its gradient (flat,
+ * num_params floats; loss and grads nullable) */
+int g2g_mlp_loss_grad(g2g_mlp_t mlp, const float *X, const float *T, int32_t M, float *loss, float *grads,
+                      int32_t on_device);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on a minibatch */
+int g2g_mlp_train_step(g2g_mlp_t mlp, const float *X, const float *T, int32_t M, float lr, float *loss,
+                       int32_t on_device);
+/* C = op(A) op(B) with bf16 operands (raw bit patterns) and fp32 result, on the
+ * MLP's MFMA kernel (test entry; host arrays) */
+int g2g_gemm_bf16(int32_t trans_a, int32_t trans_b, int32_t M, int32_t N, int32_t K, const uint16_t *A, int64_t lda,
+                  const uint16_t *B, int64_t ldb, float *C, int64_t ldc);
 
 #ifdef __cplusplus
 }
