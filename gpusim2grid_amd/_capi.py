@@ -61,6 +61,7 @@ def lib():
     sig = {
         "g2g_last_error": (ctypes.c_char_p, []),
         "g2g_version": (ctypes.c_int, []),
+        "g2g_set_device": (ctypes.c_int, [i32]),
         "g2g_plan_create": (ctypes.c_int, [i32, _P, _P, _P, i32, _P, i32, _P, i32, ctypes.POINTER(_P)]),
         "g2g_lu_plan_create": (ctypes.c_int, [i32, _P, _P, _P, ctypes.POINTER(_P)]),
         "g2g_plan_destroy": (None, [_P]),
@@ -78,6 +79,7 @@ def lib():
         "g2g_batch_eval": (ctypes.c_int, [_P, _P, _P, _P]),
         "g2g_lu_solve_batch": (ctypes.c_int, [_P, i32, i32, _P, _P, _P, dbl, _P, _P]),
         "g2g_batch_device_arrays": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+        "g2g_batch_warm_start": (ctypes.c_int, [_P, _P, _P, i32]),
         "g2g_batch_solve_warm": (ctypes.c_int, [_P, _P, _P, i32, i32, dbl, i32, dbl, ctypes.POINTER(SolveStats)]),
         "g2g_mlp_create": (ctypes.c_int, [i32, i32, i32, ctypes.c_uint64, _P, ctypes.POINTER(_P)]),
         "g2g_mlp_destroy": (None, [_P]),
@@ -249,6 +251,11 @@ class Batch:
         _check(lib().g2g_batch_solve_warm(self._h, mlp._h, _ptr(m), n_mask, int(k_warm), float(growth),
                                           int(max_iter), float(tol), ctypes.byref(st)))
         return st.as_dict()
+
+    def warm_start(self, mlp, mask=None):
+        """Writes the MLP's start voltages as the batch's V."""
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        _check(lib().g2g_batch_warm_start(self._h, mlp._h, _ptr(m), 0 if m is None else m.shape[1]))
 
     def get_v(self):
         V = np.empty((self.rows, self.n), np.complex128)

@@ -1,0 +1,1144 @@
+// _gpusim2grid: the reference's pybind11 module names (src/_cpp/python_bindings.cpp)
+// as a C++ layer over the C ABI of include/gpusim2grid_amd.h.
+//
+// Sessions own a plan (the grid's symbolic layer) and device batches; every
+// solve runs the HIP kernels behind g2g_batch_solve.  The array ("tuple")
+// mode of the reference is what is mirrored: the bare [pvpq | pq] system
+// (README.md:51-58), no lightsim2grid bridge.
+#include <pybind11/complex.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gpusim2grid_amd.h"
+
+namespace py = pybind11;
+using cplx = std::complex<double>;
+using CArr = py::array_t<cplx, py::array::c_style | py::array::forcecast>;
+using DArr = py::array_t<double, py::array::c_style | py::array::forcecast>;
+using IArr = py::array_t<int32_t, py::array::c_style | py::array::forcecast>;
+
+namespace {
+
+void check(int rc)
+{
+    if (rc == G2G_OK) return;
+    if (rc == G2G_ERR_INVALID) throw py::value_error(g2g_last_error());
+    throw std::runtime_error(g2g_last_error());
+}
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ---- enums (python_bindings.cpp:1294-1361,1576-1654; row_status.hpp:28-43) ----
+enum class RowStatus : int { Converged = 0, MaxIter = 1, Diverged = 2, NotSimulated = 3 };
+enum class BatchScheduling : int { Chunked = 0, Continuous = 1 };
+enum class LinearSolver : int { Cudss = 0, OwnLu = 1 };
+enum class ReorderingAlg : int { Default = 0, BtfColamd, Colamd, Amd, NestedDissection, None };
+enum class MatchingAlg : int { None = 0, MaxDiagCount, MaxMinDiag, MaxMinDiagAlt, MaxDiagSum, MaxDiagProduct, Auto };
+enum class PivotEpsilonAlg : int { Default = 0, Scaled, Static };
+enum class ContingencySolverType : int { DirectRefactorEvery = 0, DirectIter0Only, DirectRefactorEveryN };
+enum class Precision : int { Fp64 = G2G_FP64, Mixed = G2G_MIXED };
+
+// ---- timings (timing_utils.hpp:38-53,131-171,348-400) ------------------------
+struct TimingEntry {
+    double gpu_ms = 0.0;
+    double wall_ms = 0.0;
+};
+
+struct AcPfTimings {
+    double t_init_ms = 0.0;     // plan build + device upload
+    double t_analyze_ms = 0.0;  // symbolic analysis (ordering, programs)
+    TimingEntry t_mismatch;     // fused SpMV + F + J fill + residual
+    TimingEntry t_refactorize;  // factorization + forward substitution
+    TimingEntry t_solve;        // backward substitution + V update
+    double t_total_ms = 0.0;
+    double t_copy_v_to_host_ms = 0.0;
+    int nb_iter = 0;
+    int n_refactorize = 0;
+    bool converged = false;
+    double residual = 0.0;
+};
+
+struct BatchTimings {
+    double t_base_case_ms = 0.0;
+    double t_analysis_ms = 0.0;
+    double t_alloc_ms = 0.0;
+    double t_preprocess_ms = 0.0;   // outage edits, connectivity filter
+    TimingEntry t_mismatch;
+    TimingEntry t_refactorize;
+    TimingEntry t_solve;
+    TimingEntry t_warm_start;
+    double t_run_ms = 0.0;          // wall time of the last run()
+    double t_copy_V_to_host_ms = 0.0;
+    int n_chunks = 0;
+    int n_rows = 0;
+    int n_converged = 0;
+    int n_requeued = 0;
+    long long rows_factored = 0;
+    double total_ms() const { return t_run_ms; }
+};
+
+// ---- grid core ------------------------------------------------------------
+struct Csr {
+    int n = 0;
+    std::vector<int32_t> p, i;
+    std::vector<double> x;  // interleaved complex
+};
+
+Csr csr_from(py::object Y)
+{
+    py::object sp = py::module_::import("scipy.sparse");
+    py::object C = sp.attr("csr_matrix")(Y);
+    C.attr("sum_duplicates")();
+    C.attr("sort_indices")();
+    py::tuple shape = C.attr("shape");
+    const int n = shape[0].cast<int>();
+    if (shape[1].cast<int>() != n) throw py::value_error("Ybus must be square");
+    Csr c;
+    c.n = n;
+    IArr ip = C.attr("indptr").cast<IArr>();
+    IArr ii = C.attr("indices").cast<IArr>();
+    CArr xx = C.attr("data").cast<CArr>();
+    c.p.assign(ip.data(), ip.data() + ip.size());
+    c.i.assign(ii.data(), ii.data() + ii.size());
+    c.x.resize(2 * static_cast<size_t>(xx.size()));
+    std::memcpy(c.x.data(), xx.data(), c.x.size() * sizeof(double));
+    return c;
+}
+
+std::vector<int32_t> ivec(const IArr& a) { return std::vector<int32_t>(a.data(), a.data() + a.size()); }
+
+struct Grid {
+    Csr Y;
+    std::vector<int32_t> pv, pq;
+    int slack = -1;
+    g2g_plan_t plan = nullptr;
+    g2g_plan_info info{};
+    double t_analyze_ms = 0.0;
+
+    Grid(py::object Ybus, const IArr& slack_ids, const IArr& pv_, const IArr& pq_)
+    {
+        Y = csr_from(Ybus);
+        pv = ivec(pv_);
+        pq = ivec(pq_);
+        if (slack_ids.size() < 1) throw py::value_error("slack_ids is empty");
+        if (slack_ids.size() > 1)
+            throw py::value_error("several slack buses (distributed slack) need the lightsim2grid ledger, which the "
+                                  "array mode does not carry; pass one slack");
+        slack = slack_ids.data()[0];
+        const double t0 = now_ms();
+        check(g2g_plan_create(Y.n, Y.p.data(), Y.i.data(), Y.x.data(), static_cast<int32_t>(pv.size()), pv.data(),
+                              static_cast<int32_t>(pq.size()), pq.data(), slack, &plan));
+        t_analyze_ms = now_ms() - t0;
+        check(g2g_plan_get_info(plan, &info));
+    }
+    ~Grid() { g2g_plan_destroy(plan); }
+    Grid(const Grid&) = delete;
+    Grid& operator=(const Grid&) = delete;
+
+    int n() const { return Y.n; }
+    std::vector<int32_t> array(int which) const
+    {
+        int64_t len = 0;
+        check(g2g_plan_get_array(plan, which, nullptr, 0, &len));
+        std::vector<int32_t> out(static_cast<size_t>(len));
+        check(g2g_plan_get_array(plan, which, out.data(), len, &len));
+        return out;
+    }
+    // CSR position of Ybus entry (r, c), -1 if absent
+    int64_t pos(int r, int c) const
+    {
+        const auto b = Y.i.begin() + Y.p[r], e = Y.i.begin() + Y.p[r + 1];
+        const auto it = std::lower_bound(b, e, c);
+        return (it != e && *it == c) ? static_cast<int64_t>(it - Y.i.begin()) : -1;
+    }
+};
+
+// device batch owned by a session
+struct Batch {
+    g2g_batch_t h = nullptr;
+    int rows = 0;
+    int precision = G2G_FP64;
+    int per_row_y = 0;
+    Batch(const Grid& g, int rows_, int precision_, int per_row_y_)
+        : rows(rows_), precision(precision_), per_row_y(per_row_y_)
+    {
+        check(g2g_batch_create(g.plan, rows, precision, per_row_y, nullptr, &h));
+    }
+    ~Batch() { g2g_batch_destroy(h); }
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+};
+
+void add(TimingEntry& t, double gpu_ms) { t.gpu_ms += gpu_ms; t.wall_ms += gpu_ms; }
+
+// ---- the solve of many rows, chunked by batch_size -----------------------
+struct RowJob {
+    const cplx* S = nullptr;       // rows x n, or one shared row (S_stride 0)
+    int64_t S_stride = 0;
+    const cplx* V0 = nullptr;      // rows x n, or one shared row
+    int64_t V_stride = 0;
+    // per-row Ybus edits (CSR-like, nullable)
+    const std::vector<int32_t>* edit_ptr = nullptr;
+    const std::vector<int32_t>* edit_ent = nullptr;
+    const std::vector<cplx>* edit_delta = nullptr;
+    std::vector<char> skip;        // rows not simulated (islanding trips)
+    int max_iter = 10;
+    double tol = 1e-8;
+    int sched = G2G_SCHED_CONTINUOUS;
+    uintptr_t mlp = 0;             // warm start (0: none)
+    int k_warm = 4;
+    double growth = 2.0;
+};
+
+struct RowResults {
+    std::vector<cplx> V;
+    std::vector<int32_t> status, iters;
+    std::vector<double> res;
+};
+
+void run_rows(const Grid& g, std::unique_ptr<Batch>& batch, int precision, int batch_size, int n_rows,
+              const RowJob& job, RowResults& out, BatchTimings& tm)
+{
+    const int n = g.n();
+    out.V.assign(static_cast<size_t>(n_rows) * n, cplx(std::nan(""), std::nan("")));
+    out.status.assign(n_rows, static_cast<int32_t>(RowStatus::NotSimulated));
+    out.iters.assign(n_rows, 0);
+    out.res.assign(n_rows, std::nan(""));
+    // simulated rows, in order
+    std::vector<int> live;
+    for (int r = 0; r < n_rows; ++r)
+        if (job.skip.empty() || !job.skip[r]) live.push_back(r);
+    if (live.empty()) return;
+    const int cap = std::max(1, std::min<int>(batch_size > 0 ? batch_size : static_cast<int>(live.size()),
+                                              static_cast<int>(live.size())));
+    const int per_row_y = job.edit_ptr ? 1 : 0;
+    if (!batch || batch->rows < cap || batch->precision != precision || batch->per_row_y != per_row_y) {
+        const double t0 = now_ms();
+        batch.reset();
+        batch = std::make_unique<Batch>(g, cap, precision, per_row_y);
+        tm.t_alloc_ms += now_ms() - t0;
+    }
+    const int B = batch->rows;
+    std::vector<cplx> Sbuf(static_cast<size_t>(B) * n), Vbuf(static_cast<size_t>(B) * n);
+    std::vector<int32_t> st(B), it(B);
+    std::vector<double> rs(B);
+    std::vector<int32_t> eptr, eent;
+    std::vector<cplx> edel;
+    for (size_t c0 = 0; c0 < live.size(); c0 += B) {
+        const int m = static_cast<int>(std::min<size_t>(B, live.size() - c0));
+        for (int k = 0; k < m; ++k) {
+            const int r = live[c0 + k];
+            std::copy_n(job.S + r * job.S_stride, n, Sbuf.begin() + static_cast<size_t>(k) * n);
+            std::copy_n(job.V0 + r * job.V_stride, n, Vbuf.begin() + static_cast<size_t>(k) * n);
+        }
+        check(g2g_batch_set_active_rows(batch->h, m));
+        check(g2g_batch_set_sbus(batch->h, reinterpret_cast<const double*>(Sbuf.data()), n, m, 0, 0));
+        check(g2g_batch_set_v(batch->h, reinterpret_cast<const double*>(Vbuf.data()), n, m, 0, 0));
+        if (per_row_y) {
+            eptr.assign(1, 0);
+            eent.clear();
+            edel.clear();
+            for (int k = 0; k < m; ++k) {
+                const int r = live[c0 + k];
+                for (int q = (*job.edit_ptr)[r]; q < (*job.edit_ptr)[r + 1]; ++q) {
+                    eent.push_back((*job.edit_ent)[q]);
+                    edel.push_back((*job.edit_delta)[q]);
+                }
+                eptr.push_back(static_cast<int32_t>(eent.size()));
+            }
+            check(g2g_batch_set_ybus_edits(batch->h, eptr.data(), eent.data(),
+                                           reinterpret_cast<const double*>(edel.data()), m));
+        }
+        g2g_solve_stats s{};
+        if (job.mlp)
+            check(g2g_batch_solve_warm(batch->h, reinterpret_cast<g2g_mlp_t>(job.mlp), nullptr, 0, job.k_warm,
+                                       job.growth, job.max_iter, job.tol, &s));
+        else check(g2g_batch_solve(batch->h, job.max_iter, job.tol, job.sched, &s));
+        add(tm.t_mismatch, s.ms_mismatch);
+        add(tm.t_refactorize, s.ms_factor);
+        add(tm.t_solve, s.ms_backward);
+        add(tm.t_warm_start, s.ms_warm_start);
+        tm.rows_factored += s.rows_factored;
+        tm.n_requeued += s.n_requeued;
+        ++tm.n_chunks;
+        const double t0 = now_ms();
+        check(g2g_batch_get_v(batch->h, reinterpret_cast<double*>(Vbuf.data()), n, 0));
+        tm.t_copy_V_to_host_ms += now_ms() - t0;
+        check(g2g_batch_get_results(batch->h, st.data(), it.data(), rs.data()));
+        for (int k = 0; k < m; ++k) {
+            const int r = live[c0 + k];
+            std::copy_n(Vbuf.begin() + static_cast<size_t>(k) * n, n, out.V.begin() + static_cast<size_t>(r) * n);
+            out.status[r] = st[k];
+            out.iters[r] = it[k];
+            out.res[r] = rs[k];
+        }
+    }
+    tm.n_rows = n_rows;
+    tm.n_converged = static_cast<int>(std::count(out.status.begin(), out.status.end(), 0));
+}
+
+template <typename T>
+py::array_t<T> to_array(const std::vector<T>& v)
+{
+    py::array_t<T> a(static_cast<py::ssize_t>(v.size()));
+    std::copy(v.begin(), v.end(), a.mutable_data());
+    return a;
+}
+
+void unsupported_options(int ir_n_steps, bool scaling_max_voltage_change)
+{
+    if (ir_n_steps != 0)
+        throw py::value_error("ir_n_steps > 0 is not supported by this build (Newton's own iteration corrects the "
+                              "linear solve; the mixed-precision mode keeps F in FP64)");
+    if (scaling_max_voltage_change)
+        throw py::value_error("scaling_max_voltage_change (step damping) is not supported by this build");
+}
+
+void set_device(int device)
+{
+    if (device >= 0) check(g2g_set_device(device));
+}
+
+// ---- AcPfNrSession (python_bindings.cpp:1369-1560) -------------------------
+class AcPfNrSession {
+public:
+    AcPfNrSession(py::object Ybus, CArr Vinit, CArr Sbus, IArr slack_ids, DArr /*slack_weights*/, IArr pv, IArr pq,
+                  int max_iter, double tol, int device, bool presolved_v, int ir_n_steps,
+                  bool scaling_max_voltage_change, Precision precision)
+        : max_iter_(max_iter), tol_(tol), precision_(static_cast<int>(precision))
+    {
+        unsupported_options(ir_n_steps, scaling_max_voltage_change);
+        set_device(device);
+        const double t0 = now_ms();
+        grid_ = std::make_unique<Grid>(Ybus, slack_ids, pv, pq);
+        const int n = grid_->n();
+        if (Vinit.size() != n || Sbus.size() != n) throw py::value_error("Vinit / Sbus must have n_bus entries");
+        S_.assign(Sbus.data(), Sbus.data() + n);
+        V_.assign(Vinit.data(), Vinit.data() + n);
+        batch_ = std::make_unique<Batch>(*grid_, 1, precision_, 0);
+        timings_.t_analyze_ms = grid_->t_analyze_ms;
+        timings_.t_init_ms = now_ms() - t0;
+        if (presolved_v) {
+            upload();
+            double r = 0.0;
+            check(g2g_batch_eval(batch_->h, nullptr, nullptr, &r));
+            if (!(r < tol_))
+                throw std::runtime_error("presolved_v: ||F(Vinit)||inf = " + std::to_string(r) +
+                                         " is not below tol");
+            timings_.converged = true;
+            timings_.residual = r;
+        } else {
+            solve(max_iter_, tol_);
+        }
+    }
+
+    AcPfTimings resolve(CArr Vinit, py::object Sbus, int max_iter, double tol)
+    {
+        const int n = grid_->n();
+        if (Vinit.size() != n) throw py::value_error("Vinit must have n_bus entries");
+        V_.assign(Vinit.data(), Vinit.data() + n);
+        if (!Sbus.is_none()) {
+            CArr s = Sbus.cast<CArr>();
+            if (s.size() != n) throw py::value_error("Sbus must have n_bus entries");
+            S_.assign(s.data(), s.data() + n);
+        }
+        py::gil_scoped_release nogil;
+        solve(max_iter < 0 ? max_iter_ : max_iter, tol < 0 ? tol_ : tol);
+        return timings_;
+    }
+
+    CArr get_v()
+    {
+        const double t0 = now_ms();
+        CArr out(grid_->n());
+        check(g2g_batch_get_v(batch_->h, reinterpret_cast<double*>(out.mutable_data()), grid_->n(), 0));
+        timings_.t_copy_v_to_host_ms = now_ms() - t0;
+        return out;
+    }
+
+    DArr get_F()
+    {
+        DArr F(grid_->info.d);
+        check(g2g_batch_eval(batch_->h, F.mutable_data(), nullptr, nullptr));
+        return F;
+    }
+
+    py::tuple get_J()
+    {
+        std::vector<double> F(grid_->info.d), J(grid_->info.nJ);
+        check(g2g_batch_eval(batch_->h, F.data(), J.data(), nullptr));
+        return py::make_tuple(to_array(grid_->array(G2G_ARR_JP)), to_array(grid_->array(G2G_ARR_JI)), to_array(J));
+    }
+
+    const AcPfTimings& timings() const { return timings_; }
+    int n_pvpq() const { return grid_->info.npvpq; }
+    int n_pq() const { return grid_->info.npq; }
+    int dim_J() const { return grid_->info.d; }
+    py::array_t<int32_t> pvpq() const { return to_array(grid_->array(G2G_ARR_PVPQ)); }
+    py::array_t<int32_t> pq() const { return to_array(grid_->pq); }
+    py::array_t<int32_t> p_row_of_bus() const
+    {
+        std::vector<int32_t> r(grid_->n(), -1);
+        const auto pvpq = grid_->array(G2G_ARR_PVPQ);
+        for (size_t k = 0; k < pvpq.size(); ++k) r[pvpq[k]] = static_cast<int32_t>(k);
+        return to_array(r);
+    }
+    py::array_t<int32_t> q_row_of_bus() const
+    {
+        std::vector<int32_t> r(grid_->n(), -1);
+        for (size_t k = 0; k < grid_->pq.size(); ++k) r[grid_->pq[k]] = grid_->info.npvpq + static_cast<int32_t>(k);
+        return to_array(r);
+    }
+
+private:
+    void upload()
+    {
+        const int n = grid_->n();
+        check(g2g_batch_set_sbus(batch_->h, reinterpret_cast<const double*>(S_.data()), n, 1, 0, 0));
+        check(g2g_batch_set_v(batch_->h, reinterpret_cast<const double*>(V_.data()), n, 1, 0, 0));
+    }
+    void solve(int max_iter, double tol)
+    {
+        upload();
+        g2g_solve_stats s{};
+        const double t0 = now_ms();
+        check(g2g_batch_solve(batch_->h, max_iter, tol, G2G_SCHED_CONTINUOUS, &s));
+        int32_t st = 0, it = 0;
+        double r = 0.0;
+        check(g2g_batch_get_results(batch_->h, &st, &it, &r));
+        timings_.t_mismatch = {s.ms_mismatch, s.ms_mismatch};
+        timings_.t_refactorize = {s.ms_factor, s.ms_factor};
+        timings_.t_solve = {s.ms_backward, s.ms_backward};
+        timings_.t_total_ms = now_ms() - t0;
+        timings_.nb_iter = it;
+        timings_.n_refactorize = s.n_factor_launches;
+        timings_.converged = st == 0;
+        timings_.residual = r;
+    }
+
+    std::unique_ptr<Grid> grid_;
+    std::unique_ptr<Batch> batch_;
+    std::vector<cplx> S_, V_;
+    int max_iter_;
+    double tol_;
+    int precision_;
+    AcPfTimings timings_;
+};
+
+// ---- batch sessions (Injection sweep / contingency analysis / scenario sweep)
+class BatchSession {
+public:
+    BatchSession(py::object Ybus, CArr Vinit, CArr Sbus, IArr slack_ids, IArr pv, IArr pq, int batch_size,
+                 int nb_iter, int max_iter_base, double tol_base, int device, bool presolved_v, int ir_n_steps,
+                 bool scaling_max_voltage_change, Precision precision)
+        : batch_size_(batch_size), nb_iter_(nb_iter), precision_(static_cast<int>(precision))
+    {
+        unsupported_options(ir_n_steps, scaling_max_voltage_change);
+        if (batch_size <= 0) throw py::value_error("batch_size must be positive");
+        if (nb_iter < 0) throw py::value_error("nb_iter must be >= 0");
+        set_device(device);
+        const double t0 = now_ms();
+        grid_ = std::make_unique<Grid>(Ybus, slack_ids, pv, pq);
+        timings_.t_analysis_ms = grid_->t_analyze_ms;
+        const int n = grid_->n();
+        if (Vinit.size() != n || Sbus.size() != n) throw py::value_error("Vinit / Sbus must have n_bus entries");
+        S_base_.assign(Sbus.data(), Sbus.data() + n);
+        V_base_.assign(Vinit.data(), Vinit.data() + n);
+        // the base case, solved once (or checked, when presolved)
+        {
+            py::gil_scoped_release nogil;
+            Batch b(*grid_, 1, G2G_FP64, 0);
+            check(g2g_batch_set_sbus(b.h, reinterpret_cast<const double*>(S_base_.data()), n, 1, 0, 0));
+            check(g2g_batch_set_v(b.h, reinterpret_cast<const double*>(V_base_.data()), n, 1, 0, 0));
+            int32_t st = 0, it = 0;
+            double r = 0.0;
+            if (presolved_v) {
+                check(g2g_batch_eval(b.h, nullptr, nullptr, &r));
+                if (!(r < tol_base))
+                    throw std::runtime_error("presolved_v: ||F(Vinit)||inf = " + std::to_string(r) +
+                                             " is not below tol_base");
+            } else {
+                check(g2g_batch_solve(b.h, max_iter_base, tol_base, G2G_SCHED_CONTINUOUS, nullptr));
+                check(g2g_batch_get_results(b.h, &st, &it, &r));
+                if (st != 0)
+                    throw std::runtime_error("the base case did not converge (residual " + std::to_string(r) + ")");
+                check(g2g_batch_get_v(b.h, reinterpret_cast<double*>(V_base_.data()), n, 0));
+            }
+        }
+        timings_.t_base_case_ms = now_ms() - t0;
+    }
+    virtual ~BatchSession() = default;
+
+    // settings (mutable, next run; CLAUDE.md:130-138)
+    int batch_size = 0;
+    int get_batch_size() const { return batch_size_; }
+    void set_batch_size(int v)
+    {
+        if (v <= 0) throw py::value_error("batch_size must be positive");
+        batch_size_ = v;
+    }
+    int get_nb_iter() const { return nb_iter_; }
+    void set_nb_iter(int v)
+    {
+        if (v < 0) throw py::value_error("nb_iter must be >= 0");
+        nb_iter_ = v;
+    }
+    BatchScheduling get_scheduling() const { return sched_; }
+    void set_scheduling(BatchScheduling s) { sched_ = s; }
+    double get_tol() const { return tol_; }
+    void set_tol(double t)
+    {
+        if (!(t > 0)) throw py::value_error("tol must be positive");
+        tol_ = t;
+    }
+    Precision get_precision() const { return static_cast<Precision>(precision_); }
+    void set_precision(Precision p) { precision_ = static_cast<int>(p); }
+    void set_warm_start(uintptr_t mlp, int k_warm, double growth)
+    {
+        if (k_warm < 0 || growth < 0) throw py::value_error("k_warm and growth must be >= 0");
+        mlp_ = mlp;
+        k_warm_ = k_warm;
+        growth_ = growth;
+    }
+    void clear_warm_start() { mlp_ = 0; }
+
+    void run()
+    {
+        const int rows = n_rows();
+        if (rows <= 0) throw std::runtime_error("nothing to run: set the rows first");
+        RowJob job;
+        prepare(job);
+        job.max_iter = nb_iter_;
+        job.tol = tol_;
+        job.sched = sched_ == BatchScheduling::Chunked ? G2G_SCHED_CHUNKED : G2G_SCHED_CONTINUOUS;
+        job.mlp = mlp_;
+        job.k_warm = k_warm_;
+        job.growth = growth_;
+        BatchTimings t = timings_;
+        t.t_mismatch = t.t_refactorize = t.t_solve = t.t_warm_start = TimingEntry{};
+        t.n_chunks = 0;
+        t.rows_factored = 0;
+        t.n_requeued = 0;
+        t.t_copy_V_to_host_ms = 0.0;
+        const double t0 = now_ms();
+        run_rows(*grid_, batch_, precision_, batch_size_, rows, job, res_, t);
+        t.t_run_ms = now_ms() - t0;
+        timings_ = t;
+        ran_ = true;
+    }
+
+    CArr get_V_results() const
+    {
+        need_run();
+        CArr a(static_cast<py::ssize_t>(res_.V.size()));
+        std::copy(res_.V.begin(), res_.V.end(), a.mutable_data());
+        return a;
+    }
+    py::array_t<double> get_residuals() const { need_run(); return to_array(res_.res); }
+    py::array_t<int32_t> get_row_iterations() const { need_run(); return to_array(res_.iters); }
+    py::array_t<int32_t> get_row_status() const { need_run(); return to_array(res_.status); }
+    const BatchTimings& get_timings() const { return timings_; }
+    int n_bus() const { return grid_->n(); }
+    int dim_J() const { return grid_->info.d; }
+    CArr v_base() const
+    {
+        CArr a(grid_->n());
+        std::copy(V_base_.begin(), V_base_.end(), a.mutable_data());
+        return a;
+    }
+    virtual int n_rows() const = 0;
+
+protected:
+    virtual void prepare(RowJob& job) = 0;
+    void need_run() const
+    {
+        if (!ran_) throw std::runtime_error("call run() first");
+    }
+
+    std::unique_ptr<Grid> grid_;
+    std::unique_ptr<Batch> batch_;
+    std::vector<cplx> S_base_, V_base_;
+    int batch_size_;
+    int nb_iter_;
+    int precision_;
+    BatchScheduling sched_ = BatchScheduling::Chunked;
+    double tol_ = 1e-8;
+    uintptr_t mlp_ = 0;
+    int k_warm_ = 4;
+    double growth_ = 2.0;
+    RowResults res_;
+    BatchTimings timings_;
+    bool ran_ = false;
+};
+
+// branch data and outage edits (contingency_analysis_helper.cpp:23-83,131-163)
+struct Branches {
+    std::vector<int32_t> f, t;
+    std::vector<cplx> yff, yft, ytf, ytt;
+    std::vector<char> bridge;
+    bool set = false;
+
+    int m() const { return static_cast<int>(f.size()); }
+
+    void assign(const IArr& bf, const IArr& bt, const CArr& ff, const CArr& ft, const CArr& tf, const CArr& tt,
+                int n)
+    {
+        const auto m = bf.size();
+        if (bt.size() != m || ff.size() != m || ft.size() != m || tf.size() != m || tt.size() != m)
+            throw py::value_error("branch arrays differ in length");
+        f = ivec(bf);
+        t = ivec(bt);
+        for (py::ssize_t l = 0; l < m; ++l)
+            if (f[l] < 0 || f[l] >= n || t[l] < 0 || t[l] >= n) throw py::value_error("branch bus out of range");
+        yff.assign(ff.data(), ff.data() + m);
+        yft.assign(ft.data(), ft.data() + m);
+        ytf.assign(tf.data(), tf.data() + m);
+        ytt.assign(tt.data(), tt.data() + m);
+        bridge = find_bridges(n);
+        set = true;
+    }
+
+    // Tarjan low-link over the multigraph (a branch parallel to another is no bridge)
+    std::vector<char> find_bridges(int n) const
+    {
+        std::vector<std::vector<std::pair<int, int>>> adj(n);
+        for (int l = 0; l < m(); ++l) {
+            if (f[l] == t[l]) continue;
+            adj[f[l]].push_back({t[l], l});
+            adj[t[l]].push_back({f[l], l});
+        }
+        std::vector<int> disc(n, -1), low(n, 0);
+        std::vector<char> br(m(), 0);
+        int timer = 0;
+        struct Frame { int v, pe; size_t k; };
+        for (int s = 0; s < n; ++s) {
+            if (disc[s] >= 0) continue;
+            std::vector<Frame> st{{s, -1, 0}};
+            disc[s] = low[s] = timer++;
+            while (!st.empty()) {
+                Frame& fr = st.back();
+                if (fr.k < adj[fr.v].size()) {
+                    auto [w, l] = adj[fr.v][fr.k++];
+                    if (l == fr.pe) continue;
+                    if (disc[w] < 0) {
+                        disc[w] = low[w] = timer++;
+                        st.push_back({w, l, 0});
+                    } else {
+                        low[fr.v] = std::min(low[fr.v], disc[w]);
+                    }
+                } else {
+                    const Frame done = fr;
+                    st.pop_back();
+                    if (!st.empty()) {
+                        Frame& p = st.back();
+                        low[p.v] = std::min(low[p.v], low[done.v]);
+                        if (low[done.v] > disc[p.v]) br[done.pe] = 1;
+                    }
+                }
+            }
+        }
+        return br;
+    }
+
+    // does removing `out` disconnect the graph? (union-find over the rest)
+    bool islands(const std::vector<int>& out, int n) const
+    {
+        if (out.empty()) return false;
+        if (out.size() == 1) return bridge[out[0]] != 0;
+        std::vector<char> gone(m(), 0);
+        for (int l : out) gone[l] = 1;
+        std::vector<int> par(n);
+        std::iota(par.begin(), par.end(), 0);
+        auto find = [&](int x) {
+            while (par[x] != x) x = par[x] = par[par[x]];
+            return x;
+        };
+        int comps = n;
+        for (int l = 0; l < m(); ++l) {
+            if (gone[l]) continue;
+            const int a = find(f[l]), b = find(t[l]);
+            if (a != b) {
+                par[a] = b;
+                --comps;
+            }
+        }
+        return comps > 1;
+    }
+
+    // value edits of one outage, merged by CSR position
+    void edits(const Grid& g, const std::vector<int>& out, std::vector<int32_t>& ent, std::vector<cplx>& del) const
+    {
+        std::vector<std::pair<int64_t, cplx>> e;
+        auto put = [&](int r, int c, cplx v) {
+            const int64_t p = g.pos(r, c);
+            if (p < 0) throw py::value_error("a tripped branch's entry is not in Ybus's pattern");
+            e.push_back({p, v});
+        };
+        for (int l : out) {
+            if (l < 0 || l >= m()) throw py::value_error("branch id out of range");
+            put(f[l], f[l], yff[l]);
+            put(t[l], t[l], ytt[l]);
+            if (f[l] != t[l]) {
+                put(f[l], t[l], yft[l]);
+                put(t[l], f[l], ytf[l]);
+            }
+        }
+        std::sort(e.begin(), e.end(), [](auto& a, auto& b) { return a.first < b.first; });
+        for (size_t k = 0; k < e.size(); ++k) {
+            if (!ent.empty() && k > 0 && e[k].first == e[k - 1].first) {
+                del.back() += e[k].second;
+                continue;
+            }
+            ent.push_back(static_cast<int32_t>(e[k].first));
+            del.push_back(e[k].second);
+        }
+    }
+};
+
+// InjectionSweepSession (python_bindings.cpp:2167-2423)
+class InjectionSweepSession : public BatchSession {
+public:
+    using BatchSession::BatchSession;
+    void set_injections(DArr p_mw, DArr q_mvar, double sn_mva)
+    {
+        if (p_mw.ndim() != 2 || q_mvar.ndim() != 2 || p_mw.shape(0) != q_mvar.shape(0) ||
+            p_mw.shape(1) != q_mvar.shape(1))
+            throw py::value_error("p_mw and q_mvar must be (n_scenarios, n_bus) arrays of one shape");
+        if (p_mw.shape(1) != n_bus()) throw py::value_error("injection rows must have n_bus entries");
+        if (!(sn_mva > 0)) throw py::value_error("sn_mva must be positive");
+        const py::ssize_t rows = p_mw.shape(0), n = p_mw.shape(1);
+        S_.resize(static_cast<size_t>(rows * n));
+        for (py::ssize_t i = 0; i < rows * n; ++i) S_[i] = cplx(p_mw.data()[i], q_mvar.data()[i]) / sn_mva;
+        rows_ = static_cast<int>(rows);
+        ran_ = false;
+    }
+    void set_sbus(CArr S)
+    {
+        if (S.ndim() != 2 || S.shape(1) != n_bus()) throw py::value_error("Sbus must be (n_scenarios, n_bus)");
+        S_.assign(S.data(), S.data() + S.size());
+        rows_ = static_cast<int>(S.shape(0));
+        ran_ = false;
+    }
+    // per-row start voltages (None: the base case), as ScenarioSweep's set_v_init_from
+    void set_v_init_from(py::object V)
+    {
+        if (V.is_none()) {
+            V0_.clear();
+            return;
+        }
+        CArr a = V.cast<CArr>();
+        if (a.size() % n_bus()) throw py::value_error("V must hold whole rows of n_bus entries");
+        V0_.assign(a.data(), a.data() + a.size());
+    }
+    int n_rows() const override { return rows_; }
+    int n_scenarios() const { return rows_; }
+
+protected:
+    void prepare(RowJob& job) override
+    {
+        const int n = n_bus();
+        job.S = S_.data();
+        job.S_stride = n;
+        start(job, n);
+    }
+    void start(RowJob& job, int n)
+    {
+        if (V0_.empty()) {
+            job.V0 = V_base_.data();
+            job.V_stride = 0;
+        } else if (V0_.size() == static_cast<size_t>(n)) {
+            job.V0 = V0_.data();
+            job.V_stride = 0;
+        } else {
+            if (V0_.size() != static_cast<size_t>(rows_) * n)
+                throw py::value_error("set_v_init_from: one row or one per scenario");
+            job.V0 = V0_.data();
+            job.V_stride = n;
+        }
+    }
+    std::vector<cplx> S_, V0_;
+    int rows_ = 0;
+};
+
+// ContingencyAnalysisSession (python_bindings.cpp:1745-2166)
+class ContingencyAnalysisSession : public BatchSession {
+public:
+    using BatchSession::BatchSession;
+    void set_branch_data(IArr bf, IArr bt, CArr yff, CArr yft, CArr ytf, CArr ytt, DArr /*bus_vn_kv*/,
+                         double /*sn_mva*/)
+    {
+        br_.assign(bf, bt, yff, yft, ytf, ytt, n_bus());
+    }
+    void build_contingencies(const std::vector<std::vector<int>>& ids)
+    {
+        if (!br_.set) throw std::runtime_error("call set_branch_data() first");
+        const double t0 = now_ms();
+        eptr_.assign(1, 0);
+        eent_.clear();
+        edel_.clear();
+        skip_.assign(ids.size(), 0);
+        for (size_t r = 0; r < ids.size(); ++r) {
+            skip_[r] = br_.islands(ids[r], n_bus()) ? 1 : 0;
+            if (!skip_[r]) br_.edits(*grid_, ids[r], eent_, edel_);
+            eptr_.push_back(static_cast<int32_t>(eent_.size()));
+        }
+        ctg_ = ids;
+        timings_.t_preprocess_ms = now_ms() - t0;
+        ran_ = false;
+    }
+    int n_rows() const override { return static_cast<int>(ctg_.size()); }
+    int n_contingencies() const { return n_rows(); }
+    int n_branches() const { return br_.m(); }
+
+protected:
+    void prepare(RowJob& job) override
+    {
+        job.S = S_base_.data();
+        job.S_stride = 0;
+        job.V0 = V_base_.data();
+        job.V_stride = 0;
+        job.edit_ptr = &eptr_;
+        job.edit_ent = &eent_;
+        job.edit_delta = &edel_;
+        job.skip = skip_;
+    }
+    Branches br_;
+    std::vector<std::vector<int>> ctg_;
+    std::vector<int32_t> eptr_, eent_;
+    std::vector<cplx> edel_;
+    std::vector<char> skip_;
+};
+
+// ScenarioSweepSession (python_bindings.cpp:2424-3207): per-row injections and
+// per-row branch trips together, optional per-row start voltages
+class ScenarioSweepSession : public InjectionSweepSession {
+public:
+    using InjectionSweepSession::InjectionSweepSession;
+    void set_branch_data(IArr bf, IArr bt, CArr yff, CArr yft, CArr ytf, CArr ytt, DArr /*bus_vn_kv*/,
+                         double /*sn_mva*/)
+    {
+        br_.assign(bf, bt, yff, yft, ytf, ytt, n_bus());
+    }
+    void set_topology(const std::vector<std::vector<int>>& ids)
+    {
+        if (!br_.set) throw std::runtime_error("call set_branch_data() first");
+        eptr_.assign(1, 0);
+        eent_.clear();
+        edel_.clear();
+        skip_.assign(ids.size(), 0);
+        for (size_t r = 0; r < ids.size(); ++r) {
+            skip_[r] = br_.islands(ids[r], n_bus()) ? 1 : 0;
+            if (!skip_[r]) br_.edits(*grid_, ids[r], eent_, edel_);
+            eptr_.push_back(static_cast<int32_t>(eent_.size()));
+        }
+        topo_ = true;
+        ran_ = false;
+    }
+    void clear_topology()
+    {
+        topo_ = false;
+        skip_.clear();
+    }
+    void clear_v_init() { V0_.clear(); }
+    bool has_v_init() const { return !V0_.empty(); }
+    int n_branches() const { return br_.m(); }
+    py::array_t<int32_t> get_disconnected() const
+    {
+        std::vector<int32_t> d(skip_.begin(), skip_.end());
+        return to_array(d);
+    }
+
+protected:
+    void prepare(RowJob& job) override
+    {
+        InjectionSweepSession::prepare(job);
+        if (topo_) {
+            if (eptr_.size() != static_cast<size_t>(rows_) + 1)
+                throw py::value_error("set_topology: one entry per scenario");
+            job.edit_ptr = &eptr_;
+            job.edit_ent = &eent_;
+            job.edit_delta = &edel_;
+            job.skip = skip_;
+        }
+    }
+    Branches br_;
+    bool topo_ = false;
+    std::vector<int32_t> eptr_, eent_;
+    std::vector<cplx> edel_;
+    std::vector<char> skip_;
+};
+
+}  // namespace
+
+PYBIND11_MODULE(_gpusim2grid, m)
+{
+    m.doc() = "MI355X batched Newton-Raphson power flow (HIP kernels behind the g2g C ABI)";
+
+    py::enum_<RowStatus>(m, "RowStatus")
+        .value("Converged", RowStatus::Converged)
+        .value("MaxIter", RowStatus::MaxIter)
+        .value("Diverged", RowStatus::Diverged)
+        .value("NotSimulated", RowStatus::NotSimulated);
+    py::enum_<BatchScheduling>(m, "BatchScheduling")
+        .value("Chunked", BatchScheduling::Chunked)
+        .value("Continuous", BatchScheduling::Continuous);
+    py::enum_<LinearSolver>(m, "LinearSolver")
+        .value("Cudss", LinearSolver::Cudss, "accepted for compatibility: the batched LU of this build runs")
+        .value("OwnLu", LinearSolver::OwnLu);
+    py::enum_<ReorderingAlg>(m, "ReorderingAlg")
+        .value("Default", ReorderingAlg::Default)
+        .value("BtfColamd", ReorderingAlg::BtfColamd)
+        .value("Colamd", ReorderingAlg::Colamd)
+        .value("Amd", ReorderingAlg::Amd)
+        .value("NestedDissection", ReorderingAlg::NestedDissection)
+        .value("None_", ReorderingAlg::None);
+    py::enum_<MatchingAlg>(m, "MatchingAlg")
+        .value("None_", MatchingAlg::None)
+        .value("MaxDiagCount", MatchingAlg::MaxDiagCount)
+        .value("MaxMinDiag", MatchingAlg::MaxMinDiag)
+        .value("MaxMinDiagAlt", MatchingAlg::MaxMinDiagAlt)
+        .value("MaxDiagSum", MatchingAlg::MaxDiagSum)
+        .value("MaxDiagProduct", MatchingAlg::MaxDiagProduct)
+        .value("Auto", MatchingAlg::Auto);
+    py::enum_<PivotEpsilonAlg>(m, "PivotEpsilonAlg")
+        .value("Default", PivotEpsilonAlg::Default)
+        .value("Scaled", PivotEpsilonAlg::Scaled)
+        .value("Static", PivotEpsilonAlg::Static);
+    py::enum_<ContingencySolverType>(m, "ContingencySolverType")
+        .value("DirectRefactorEvery", ContingencySolverType::DirectRefactorEvery)
+        .value("DirectIter0Only", ContingencySolverType::DirectIter0Only)
+        .value("DirectRefactorEveryN", ContingencySolverType::DirectRefactorEveryN);
+    py::enum_<Precision>(m, "Precision").value("Fp64", Precision::Fp64).value("Mixed", Precision::Mixed);
+
+    py::class_<TimingEntry>(m, "TimingEntry")
+        .def_readonly("gpu_ms", &TimingEntry::gpu_ms)
+        .def_readonly("wall_ms", &TimingEntry::wall_ms);
+    py::class_<AcPfTimings>(m, "AcPfTimings")
+        .def_readonly("t_init_ms", &AcPfTimings::t_init_ms)
+        .def_readonly("t_analyze_ms", &AcPfTimings::t_analyze_ms)
+        .def_readonly("t_mismatch", &AcPfTimings::t_mismatch)
+        .def_readonly("t_refactorize", &AcPfTimings::t_refactorize)
+        .def_readonly("t_solve", &AcPfTimings::t_solve)
+        .def_readonly("t_total_ms", &AcPfTimings::t_total_ms)
+        .def_readonly("t_copy_v_to_host_ms", &AcPfTimings::t_copy_v_to_host_ms)
+        .def_readonly("nb_iter", &AcPfTimings::nb_iter)
+        .def_readonly("n_refactorize", &AcPfTimings::n_refactorize)
+        .def_readonly("converged", &AcPfTimings::converged)
+        .def_readonly("residual", &AcPfTimings::residual);
+    py::class_<BatchTimings>(m, "BatchTimings")
+        .def_readonly("t_base_case_ms", &BatchTimings::t_base_case_ms)
+        .def_readonly("t_analysis_ms", &BatchTimings::t_analysis_ms)
+        .def_readonly("t_alloc_ms", &BatchTimings::t_alloc_ms)
+        .def_readonly("t_preprocess_ms", &BatchTimings::t_preprocess_ms)
+        .def_readonly("t_mismatch", &BatchTimings::t_mismatch)
+        .def_readonly("t_refactorize", &BatchTimings::t_refactorize)
+        .def_readonly("t_solve", &BatchTimings::t_solve)
+        .def_readonly("t_warm_start", &BatchTimings::t_warm_start)
+        .def_readonly("t_run_ms", &BatchTimings::t_run_ms)
+        .def_readonly("t_copy_V_to_host_ms", &BatchTimings::t_copy_V_to_host_ms)
+        .def_readonly("n_chunks", &BatchTimings::n_chunks)
+        .def_readonly("n_rows", &BatchTimings::n_rows)
+        .def_readonly("n_converged", &BatchTimings::n_converged)
+        .def_readonly("n_requeued", &BatchTimings::n_requeued)
+        .def_readonly("rows_factored", &BatchTimings::rows_factored)
+        .def_property_readonly("t_total_ms", &BatchTimings::total_ms);
+    m.attr("ContingencyTimings") = m.attr("BatchTimings");
+
+    py::class_<AcPfNrSession, std::shared_ptr<AcPfNrSession>>(m, "AcPfNrSession")
+        .def(py::init([](py::object Ybus, CArr Vinit, CArr Sbus, IArr slack_ids, DArr slack_weights, IArr pv,
+                         IArr pq, int max_iter, double tol, int device, bool presolved_v, ReorderingAlg,
+                         MatchingAlg, PivotEpsilonAlg, int ir_n_steps, bool, bool scaling, double, double,
+                         Precision precision) {
+                 return std::make_shared<AcPfNrSession>(Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq,
+                                                        max_iter, tol, device, presolved_v, ir_n_steps, scaling,
+                                                        precision);
+             }),
+             py::arg("Ybus"), py::arg("Vinit"), py::arg("Sbus"), py::arg("slack_ids"), py::arg("slack_weights"),
+             py::arg("pv"), py::arg("pq"), py::arg("max_iter"), py::arg("tol"), py::arg("device") = -1,
+             py::arg("presolved_v") = false, py::arg("reordering_alg") = ReorderingAlg::Default,
+             py::arg("matching_alg") = MatchingAlg::None, py::arg("pivot_epsilon_alg") = PivotEpsilonAlg::Default,
+             py::arg("ir_n_steps") = 0, py::arg("debug_base_case") = false,
+             py::arg("scaling_max_voltage_change") = false, py::arg("max_dVa") = 0.5, py::arg("max_dVm") = 0.1,
+             py::arg("precision") = Precision::Fp64)
+        .def_property_readonly("timings", &AcPfNrSession::timings)
+        .def("resolve", &AcPfNrSession::resolve, py::arg("Vinit"), py::arg("Sbus") = py::none(),
+             py::arg("max_iter") = -1, py::arg("tol") = -1.0)
+        .def("get_v", &AcPfNrSession::get_v)
+        .def("get_F", &AcPfNrSession::get_F)
+        .def("get_J", &AcPfNrSession::get_J)
+        .def_property_readonly("n_pvpq", &AcPfNrSession::n_pvpq)
+        .def_property_readonly("n_pq", &AcPfNrSession::n_pq)
+        .def_property_readonly("dim_J", &AcPfNrSession::dim_J)
+        .def_property_readonly("pvpq", &AcPfNrSession::pvpq)
+        .def_property_readonly("pq", &AcPfNrSession::pq)
+        .def_property_readonly("p_row_of_bus", &AcPfNrSession::p_row_of_bus)
+        .def_property_readonly("q_row_of_bus", &AcPfNrSession::q_row_of_bus)
+        .def_property_readonly("theta_col_of_bus", &AcPfNrSession::p_row_of_bus)
+        .def_property_readonly("vm_col_of_bus", &AcPfNrSession::q_row_of_bus);
+
+    m.def(
+        "acpf_nr_gpu",
+        [](py::array_t<cplx> out, py::object Ybus, CArr V, CArr Sbus, IArr slack_ids, DArr slack_weights, IArr pv,
+           IArr pq, int max_iter, double tol, int nb_solve, int device) {
+            if (nb_solve < 1) throw py::value_error("nb_solve must be >= 1");
+            AcPfNrSession s(Ybus, V, Sbus, slack_ids, slack_weights, pv, pq, max_iter, tol, device, false, 0, false,
+                            Precision::Fp64);
+            for (int k = 1; k < nb_solve; ++k) s.resolve(V, py::none(), -1, -1.0);
+            CArr v = s.get_v();
+            if (out.ndim() != 1 || out.shape(0) != v.shape(0))
+                throw py::value_error("out must be a complex128 array of n_bus entries");
+            std::copy(v.data(), v.data() + v.size(), out.mutable_data());
+            return s.timings();
+        },
+        py::arg("out"), py::arg("ac_Ybus"), py::arg("V"), py::arg("Sbus"), py::arg("slack_ids"),
+        py::arg("slack_weights"), py::arg("pv"), py::arg("pq"), py::arg("max_iter"), py::arg("tol"),
+        py::arg("nb_solve"), py::arg("device") = -1);
+
+    auto bind_batch = [](auto& c) {
+        using S = typename std::remove_reference_t<decltype(c)>::type;
+        c.def("run", &S::run, py::call_guard<py::gil_scoped_release>())
+            .def("get_V_results", &S::get_V_results)
+            .def("get_residuals", &S::get_residuals)
+            .def("get_row_iterations", &S::get_row_iterations)
+            .def("get_row_status", &S::get_row_status)
+            .def("get_timings", &S::get_timings)
+            .def_property_readonly("timings", &S::get_timings)
+            .def_property_readonly("n_bus", &S::n_bus)
+            .def_property_readonly("dim_J", &S::dim_J)
+            .def("get_v_base", &S::v_base)
+            .def_property("batch_size", &S::get_batch_size, &S::set_batch_size)
+            .def_property("nb_iter", &S::get_nb_iter, &S::set_nb_iter)
+            .def_property("scheduling", &S::get_scheduling, &S::set_scheduling)
+            .def_property("tol", &S::get_tol, &S::set_tol)
+            .def_property("precision", &S::get_precision, &S::set_precision)
+            .def("set_warm_start", &S::set_warm_start, py::arg("mlp_handle"), py::arg("k_warm") = 4,
+                 py::arg("growth") = 2.0)
+            .def("clear_warm_start", &S::clear_warm_start);
+    };
+    auto batch_init = [](auto* tag) {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        return py::init([](py::object Ybus, CArr Vinit, CArr Sbus, IArr slack_ids, DArr, IArr pv, IArr pq,
+                           int batch_size, int nb_iter, int max_iter_base, double tol_base, int device,
+                           bool presolved_v, ReorderingAlg, MatchingAlg, PivotEpsilonAlg, int ir_n_steps, bool,
+                           bool scaling, double, double, Precision precision) {
+            return std::make_shared<S>(Ybus, Vinit, Sbus, slack_ids, pv, pq, batch_size, nb_iter, max_iter_base,
+                                       tol_base, device, presolved_v, ir_n_steps, scaling, precision);
+        });
+    };
+#define G2G_BATCH_ARGS                                                                                          \
+    py::arg("Ybus"), py::arg("Vinit"), py::arg("Sbus"), py::arg("slack_ids"), py::arg("slack_weights"),         \
+        py::arg("pv"), py::arg("pq"), py::arg("batch_size"), py::arg("nb_iter"), py::arg("max_iter_base") = 10, \
+        py::arg("tol_base") = 1e-6, py::arg("device") = -1, py::arg("presolved_v") = false,                     \
+        py::arg("reordering_alg") = ReorderingAlg::Default, py::arg("matching_alg") = MatchingAlg::None,        \
+        py::arg("pivot_epsilon_alg") = PivotEpsilonAlg::Default, py::arg("ir_n_steps") = 0,                     \
+        py::arg("debug_base_case") = false, py::arg("scaling_max_voltage_change") = false,                      \
+        py::arg("max_dVa") = 0.5, py::arg("max_dVm") = 0.1, py::arg("precision") = Precision::Fp64
+
+    py::class_<InjectionSweepSession, std::shared_ptr<InjectionSweepSession>> inj(m, "InjectionSweepSession");
+    inj.def(batch_init(static_cast<InjectionSweepSession*>(nullptr)), G2G_BATCH_ARGS)
+        .def("set_injections", &InjectionSweepSession::set_injections, py::arg("p_mw"), py::arg("q_mvar"),
+             py::arg("sn_mva"))
+        .def("set_sbus", &InjectionSweepSession::set_sbus, py::arg("Sbus"))
+        .def("set_v_init_from", &InjectionSweepSession::set_v_init_from, py::arg("V"))
+        .def_property_readonly("n_scenarios", &InjectionSweepSession::n_scenarios);
+    bind_batch(inj);
+
+    py::class_<ContingencyAnalysisSession, std::shared_ptr<ContingencyAnalysisSession>> ca(
+        m, "ContingencyAnalysisSession");
+    ca.def(batch_init(static_cast<ContingencyAnalysisSession*>(nullptr)), G2G_BATCH_ARGS)
+        .def("set_branch_data", &ContingencyAnalysisSession::set_branch_data, py::arg("branch_from"),
+             py::arg("branch_to"), py::arg("yff_eff"), py::arg("yft_eff"), py::arg("ytf_eff"), py::arg("ytt_eff"),
+             py::arg("bus_vn_kv"), py::arg("sn_mva"))
+        .def("build_contingencies", &ContingencyAnalysisSession::build_contingencies,
+             py::arg("branch_ids_per_ctg"))
+        .def_property_readonly("n_contingencies", &ContingencyAnalysisSession::n_contingencies)
+        .def_property_readonly("n_branches", &ContingencyAnalysisSession::n_branches);
+    bind_batch(ca);
+
+    py::class_<ScenarioSweepSession, std::shared_ptr<ScenarioSweepSession>> ss(m, "ScenarioSweepSession");
+    ss.def(batch_init(static_cast<ScenarioSweepSession*>(nullptr)), G2G_BATCH_ARGS)
+        .def("set_branch_data", &ScenarioSweepSession::set_branch_data, py::arg("branch_from"),
+             py::arg("branch_to"), py::arg("yff_eff"), py::arg("yft_eff"), py::arg("ytf_eff"), py::arg("ytt_eff"),
+             py::arg("bus_vn_kv"), py::arg("sn_mva"))
+        .def("set_injections", &ScenarioSweepSession::set_injections, py::arg("p_mw"), py::arg("q_mvar"),
+             py::arg("sn_mva"))
+        .def("set_sbus", &ScenarioSweepSession::set_sbus, py::arg("Sbus"))
+        .def("set_topology", &ScenarioSweepSession::set_topology, py::arg("branch_ids_per_row"))
+        .def("clear_topology", &ScenarioSweepSession::clear_topology)
+        .def("set_v_init_from", &ScenarioSweepSession::set_v_init_from, py::arg("V"))
+        .def("clear_v_init", &ScenarioSweepSession::clear_v_init)
+        .def_property_readonly("has_v_init", &ScenarioSweepSession::has_v_init)
+        .def("get_disconnected", &ScenarioSweepSession::get_disconnected)
+        .def_property_readonly("n_scenarios", &ScenarioSweepSession::n_scenarios)
+        .def_property_readonly("n_branches", &ScenarioSweepSession::n_branches);
+    bind_batch(ss);
+#undef G2G_BATCH_ARGS
+
+    m.def(
+        "acpf_nr_gpu_injection",
+        [](py::object Ybus, CArr Vinit, CArr Sbus, IArr slack_ids, DArr slack_weights, IArr pv, IArr pq, DArr p_mw,
+           DArr q_mvar, double sn_mva, int batch_size, int nb_iter, int max_iter_base, double tol_base,
+           ContingencySolverType, int) {
+            auto s = std::make_shared<InjectionSweepSession>(Ybus, Vinit, Sbus, slack_ids, pv, pq, batch_size,
+                                                             nb_iter, max_iter_base, tol_base, -1, false, 0, false,
+                                                             Precision::Fp64);
+            s->set_injections(p_mw, q_mvar, sn_mva);
+            {
+                py::gil_scoped_release nogil;
+                s->run();
+            }
+            return py::make_tuple(s->get_V_results(), s->get_residuals(), s->get_timings());
+        },
+        py::arg("Ybus"), py::arg("Vinit"), py::arg("Sbus"), py::arg("slack_ids"), py::arg("slack_weights"),
+        py::arg("pv"), py::arg("pq"), py::arg("p_mw"), py::arg("q_mvar"), py::arg("sn_mva"), py::arg("batch_size"),
+        py::arg("nb_iter"), py::arg("max_iter_base") = 10, py::arg("tol_base") = 1e-6,
+        py::arg("strategy_type") = ContingencySolverType::DirectRefactorEvery, py::arg("refactor_period") = 1);
+
+    m.def(
+        "warmup",
+        [](int device) {
+            set_device(device);
+            // one tiny solve loads the kernels' code objects
+            py::object sp = py::module_::import("scipy.sparse");
+            std::vector<cplx> y{cplx(1, -10), cplx(-1, 10), cplx(-1, 10), cplx(1, -10)};
+            py::array_t<cplx> data(4);
+            std::copy(y.begin(), y.end(), data.mutable_data());
+            py::array_t<int32_t> ind(std::vector<py::ssize_t>{4});
+            py::array_t<int32_t> ptr(std::vector<py::ssize_t>{3});
+            const int32_t ii[4] = {0, 1, 0, 1}, pp[3] = {0, 2, 4};
+            std::copy(ii, ii + 4, ind.mutable_data());
+            std::copy(pp, pp + 3, ptr.mutable_data());
+            py::object Y = sp.attr("csr_matrix")(py::make_tuple(data, ind, ptr), py::make_tuple(2, 2));
+            CArr V(2), S(2);
+            V.mutable_data()[0] = V.mutable_data()[1] = cplx(1, 0);
+            S.mutable_data()[0] = cplx(0, 0);
+            S.mutable_data()[1] = cplx(-0.1, -0.05);
+            IArr sl(1), pv(0), pq(1);
+            sl.mutable_data()[0] = 0;
+            pq.mutable_data()[0] = 1;
+            DArr w(1);
+            w.mutable_data()[0] = 1.0;
+            AcPfNrSession s(Y, V, S, sl, w, pv, pq, 10, 1e-8, device, false, 0, false, Precision::Fp64);
+            AcPfNrSession s2(Y, V, S, sl, w, pv, pq, 10, 1e-8, device, false, 0, false, Precision::Mixed);
+        },
+        py::arg("device") = -1);
+    m.attr("is_fp32") = false;
+    m.attr("have_ls2g_bridge") = false;
+    m.attr("abi_version") = g2g_version();
+}

@@ -508,6 +508,16 @@ const char* g2g_last_error(void) { return last_error().c_str(); }
 
 int g2g_version(void) { return 100; }
 
+int g2g_set_device(int32_t device)
+{
+    return guarded([&] {
+        int count = 0;
+        HIPCHK(hipGetDeviceCount(&count));
+        if (device >= count) throw std::invalid_argument("device ordinal out of range");
+        if (device >= 0) HIPCHK(hipSetDevice(device));
+    });
+}
+
 int g2g_plan_create(int32_t n, const int32_t* Yp, const int32_t* Yi, const double* Yx, int32_t npv,
                     const int32_t* pv, int32_t npq, const int32_t* pq, int32_t slack, g2g_plan_t* out)
 {
@@ -748,6 +758,23 @@ int g2g_batch_solve_warm(g2g_batch_t b, g2g_mlp_t mlp, const uint8_t* mask, int3
         if (b->precision == G2G_MIXED)
             solve_warm_impl<float>(b, mlp, mask, n_mask, k_warm, growth, max_iter, tol, stats);
         else solve_warm_impl<double>(b, mlp, mask, n_mask, k_warm, growth, max_iter, tol, stats);
+    });
+}
+
+int g2g_batch_warm_start(g2g_batch_t b, g2g_mlp_t mlp, const uint8_t* mask, int32_t n_mask)
+{
+    return guarded([&] {
+        if (!b || !mlp) throw std::invalid_argument("null argument");
+        if (n_mask < 0 || (n_mask > 0 && !mask)) throw std::invalid_argument("mask missing");
+        DeviceScratch tmp;
+        unsigned char* d_mask = nullptr;
+        if (n_mask > 0) {
+            d_mask = tmp.get<unsigned char>(static_cast<size_t>(b->active_rows) * n_mask);
+            HIPCHK(hipMemcpyAsync(d_mask, mask, static_cast<size_t>(b->active_rows) * n_mask,
+                                  hipMemcpyHostToDevice, b->stream));
+        }
+        mlp_warm_start_tiles(mlp, b->S, b->active_rows, b->plan->nr.n, d_mask, n_mask, b->V, b->stream);
+        HIPCHK(hipStreamSynchronize(b->stream));
     });
 }
 

@@ -193,8 +193,6 @@ int g2g_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_out, uint64_t seed,
     return guarded([&] {
         if (!out) throw std::invalid_argument("null argument");
         if (n_in <= 0 || n_hidden <= 0 || n_out <= 0) throw std::invalid_argument("layer widths must be positive");
-        if (n_in % 8 || n_hidden % 8 || n_out % 8)
-            throw std::invalid_argument("layer widths must be multiples of 8 (16-byte bf16 rows)");
         auto p = std::make_unique<g2g_mlp_s>();
         p->n_in = n_in;
         p->H = n_hidden;

@@ -1,0 +1,332 @@
+"""The reference's Python facades in array ("tuple") mode.
+
+``AcPfGPU`` (``src/gpusim2grid/acpf_nr/gpu_facade.py``), ``InjectionSweepGPU``
+(``injection_sweep/gpu_facade.py``), ``ContingencyAnalysisGPU``
+(``contingency_analysis/gpu_facade.py``) and ``ScenarioSweepGPU``
+(``scenario_sweep/gpu_facade.py``) take ``grid=(Ybus, Vinit, Sbus, slack_ids,
+slack_weights, pv, pq)`` and drive the ``_gpusim2grid`` sessions, whose solves
+run the HIP kernels.  Settings take effect on the next run, as in the
+reference (``CLAUDE.md:130-138``).  A lightsim2grid grid object is not
+accepted: the bridge is out of scope.
+
+``compute()`` returns the voltages as a DLPack capsule of a device tensor
+(the reference's ``compute`` contract, ``injection_sweep/gpu_facade.py:515-524``),
+or, with ``as_numpy=True``, a host array.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _gpusim2grid as _cpp
+
+__all__ = ["AcPfGPU", "InjectionSweepGPU", "ContingencyAnalysisGPU", "ScenarioSweepGPU", "RowStatus"]
+
+RowStatus = _cpp.RowStatus
+
+_SCHED = {"chunked": _cpp.BatchScheduling.Chunked, "continuous": _cpp.BatchScheduling.Continuous}
+
+
+def _precision(precision):
+    """'fp64' (default) or 'mixed' (FP32 Jacobian / LU / solves, FP64 mismatch,
+    test and state).  The reference validates the name against its compiled
+    real type (``_ls2g_utils._validate_precision``); here precision is chosen
+    per session."""
+    if precision is None or precision == "fp64":
+        return _cpp.Precision.Fp64
+    if precision == "mixed":
+        return _cpp.Precision.Mixed
+    if precision == "fp32":
+        raise ValueError("precision='fp32': this build keeps the state in FP64; use 'mixed' for FP32 linear algebra")
+    raise ValueError(f"unknown precision {precision!r} (expected 'fp64' or 'mixed')")
+
+
+def _device(device):
+    """None | int | 'cuda' | 'cuda:N' -> ordinal (-1: current device)."""
+    if device is None or device == "cuda":
+        return -1
+    if isinstance(device, str):
+        if device.startswith("cuda:"):
+            return int(device[5:])
+        raise ValueError(f"unknown device {device!r}")
+    return int(device)
+
+
+def _tuple(grid):
+    if not isinstance(grid, (tuple, list)) or len(grid) != 7:
+        raise TypeError("grid must be the array tuple (Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq); "
+                        "lightsim2grid grid objects need the bridge, which this build does not have")
+    Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq = grid
+    return (Ybus, np.ascontiguousarray(Vinit, dtype=complex), np.ascontiguousarray(Sbus, dtype=complex),
+            np.ascontiguousarray(np.atleast_1d(slack_ids), dtype=np.int32),
+            np.ascontiguousarray(np.atleast_1d(slack_weights), dtype=np.float64),
+            np.ascontiguousarray(pv, dtype=np.int32), np.ascontiguousarray(pq, dtype=np.int32))
+
+
+def _check_common(ir_n_steps, scaling_max_voltage_change, use_bridge):
+    if use_bridge:
+        raise ValueError("use_bridge=True requires a lightsim2grid grid object, not an explicit-array tuple.")
+    if ir_n_steps:
+        raise ValueError("ir_n_steps > 0 is not supported by this build")
+    if scaling_max_voltage_change:
+        raise ValueError("scaling_max_voltage_change is not supported by this build")
+
+
+def _to_device_capsule(V):
+    """(rows, n) complex host array -> DLPack capsule of a device tensor."""
+    import torch
+    from torch.utils import dlpack
+    return dlpack.to_dlpack(torch.from_numpy(np.ascontiguousarray(V)).to("cuda"))
+
+
+class _HostResult:
+    """Lazy host view of a result (the reference's ``.to_numpy()`` holders)."""
+
+    def __init__(self, a):
+        self._a = a
+
+    def to_numpy(self):
+        return self._a
+
+
+class AcPfGPU:
+    """Single-system AC Newton-Raphson on the GPU (the reference's
+    ``acpf_nr/gpu_facade.py`` ``AcPfGPU``, tuple mode)."""
+
+    def __init__(self, grid, *, precision="fp64", max_iter=10, tol=1e-8, device=None, init_from_n_powerflow=True,
+                 use_bridge=None, reordering_alg="default", matching_alg="none", pivot_epsilon_alg="default",
+                 ir_n_steps=0, debug_base_case=False, scaling_max_voltage_change=None, max_dVa=None, max_dVm=None,
+                 use_distributed_slack=True):
+        _check_common(ir_n_steps, scaling_max_voltage_change, use_bridge)
+        Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq = _tuple(grid)
+        self._vm_fixed = None
+        self._s = _cpp.AcPfNrSession(Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq, int(max_iter), float(tol),
+                                     _device(device), presolved_v=bool(init_from_n_powerflow),
+                                     precision=_precision(precision))
+
+    def solve(self):
+        """The solved complex voltage vector (host copy)."""
+        return self._s.get_v()
+
+    def resolve(self, Vinit, Sbus=None, *, max_iter=None, tol=None):
+        """Solve again from ``Vinit`` (and new ``Sbus``) reusing the plan."""
+        self._vm_setpoints()
+        return self._s.resolve(np.ascontiguousarray(Vinit, dtype=complex),
+                               None if Sbus is None else np.ascontiguousarray(Sbus, dtype=complex),
+                               -1 if max_iter is None else int(max_iter), -1.0 if tol is None else float(tol))
+
+    def ac_pf(self, Vinit, max_iter, tol):
+        """``LSGrid.ac_pf``'s signature: set-points written into |V| of the
+        Vm-fixed buses, then :meth:`resolve`; empty array if not converged."""
+        mask, vm = self._vm_setpoints()
+        V = np.array(Vinit, dtype=complex)
+        V[mask] = vm[mask] * np.exp(1j * np.angle(V[mask]))
+        t = self.resolve(V, max_iter=max_iter, tol=tol)
+        return self._s.get_v() if t.converged else np.zeros(0, dtype=complex)
+
+    def _vm_setpoints(self):
+        if self._vm_fixed is None:
+            mask = np.asarray(self._s.vm_col_of_bus) < 0
+            self._vm_fixed = (mask, np.abs(self._s.get_v()))
+        return self._vm_fixed
+
+    def get_F(self):
+        return self._s.get_F()
+
+    def get_J(self):
+        """The Jacobian at the current V as a scipy CSR matrix."""
+        import scipy.sparse as sp
+        indptr, indices, data = self._s.get_J()
+        d = self._s.dim_J
+        return sp.csr_matrix((data, indices, indptr), shape=(d, d))
+
+    @property
+    def timings(self):
+        return self._s.timings
+
+    @property
+    def session(self):
+        return self._s
+
+
+class _BatchFacade:
+    _session_cls = None
+
+    def __init__(self, grid, *, init_from_n_powerflow=True, precision="fp64", nb_iter=4, max_iter_base=10,
+                 tol_base=1e-8, device=None, use_bridge=None, ir_n_steps=0, scaling_max_voltage_change=None,
+                 scheduling=None, nb_iter_per_round=1, tol=None, linear_solver="own_lu", **_ignored):
+        _check_common(ir_n_steps, scaling_max_voltage_change, use_bridge)
+        if nb_iter_per_round != 1:
+            raise ValueError("nb_iter_per_round: rows are checked after every iteration in this build (1)")
+        if linear_solver not in ("cudss", "own_lu"):
+            raise ValueError(f"unknown linear_solver {linear_solver!r}")
+        Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq = _tuple(grid)
+        self._s = self._session_cls(Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq, 100, int(nb_iter),
+                                    int(max_iter_base), float(tol_base), _device(device),
+                                    presolved_v=bool(init_from_n_powerflow), precision=_precision(precision))
+        self.scheduling = scheduling or "chunked"
+        if tol is not None:
+            self.tol = tol
+        self._last_residuals = None
+        self._warm = None
+
+    # settings ---------------------------------------------------------------
+    @property
+    def nb_iter(self):
+        return self._s.nb_iter
+
+    @nb_iter.setter
+    def nb_iter(self, v):
+        self._s.nb_iter = int(v)
+
+    @property
+    def scheduling(self):
+        return "chunked" if self._s.scheduling == _cpp.BatchScheduling.Chunked else "continuous"
+
+    @scheduling.setter
+    def scheduling(self, v):
+        if v not in _SCHED:
+            raise ValueError(f"unknown scheduling {v!r} (expected 'chunked' or 'continuous')")
+        self._s.scheduling = _SCHED[v]
+
+    @property
+    def tol(self):
+        return self._s.tol
+
+    @tol.setter
+    def tol(self, v):
+        self._s.tol = float(v)
+
+    @property
+    def precision(self):
+        return "fp64" if self._s.precision == _cpp.Precision.Fp64 else "mixed"
+
+    @precision.setter
+    def precision(self, v):
+        self._s.precision = _precision(v)
+
+    def set_warm_start(self, warm, k_warm=4, growth=2.0):
+        """Learned warm start (``gpusim2grid_amd.warmstart.WarmStart``) with
+        flat-start fallback on the next runs; None switches it off."""
+        self._warm = warm
+        if warm is None:
+            self._s.clear_warm_start()
+        else:
+            self._s.set_warm_start(warm.mlp._h.value, int(k_warm), float(growth))
+
+    # running ----------------------------------------------------------------
+    def compute(self, batch_size=512, as_numpy=False):
+        """Solve every row; DLPack (n_rows, n_bus) complex of a device tensor
+        (``as_numpy``: host array).  Residuals go to :meth:`last_residuals`."""
+        self._s.batch_size = int(batch_size)
+        self._s.run()
+        self._last_residuals = self._s.get_residuals()
+        V = self._s.get_V_results().reshape(-1, self._s.n_bus)
+        return V if as_numpy else _to_device_capsule(V)
+
+    def last_residuals(self):
+        if self._last_residuals is None:
+            raise RuntimeError("Call compute() before last_residuals().")
+        return self._last_residuals
+
+    @property
+    def V_results(self):
+        return _HostResult(self._s.get_V_results().reshape(-1, self._s.n_bus))
+
+    def get_row_status(self):
+        return self._s.get_row_status()
+
+    def get_row_iterations(self):
+        return self._s.get_row_iterations()
+
+    def converged(self, tol=None):
+        r = self.last_residuals()
+        return r < (self.tol if tol is None else tol)
+
+    @property
+    def timings(self):
+        return self._s.timings
+
+    @property
+    def n_bus(self):
+        return self._s.n_bus
+
+    @property
+    def solver(self):
+        return self._s
+
+
+class InjectionSweepGPU(_BatchFacade):
+    """Many injection profiles on one grid (``injection_sweep/gpu_facade.py``)."""
+
+    _session_cls = _cpp.InjectionSweepSession
+
+    def set_injections(self, p_mw, q_mvar, sn_mva):
+        """(n_scenarios, n_bus) MW / MVAr per bus; Sbus = (p + jq) / sn_mva."""
+        self._s.set_injections(np.ascontiguousarray(p_mw, dtype=float), np.ascontiguousarray(q_mvar, dtype=float),
+                               float(sn_mva))
+
+    def set_v_init_from(self, V):
+        """Start voltages: one row, one per scenario, or None (the base case)."""
+        self._s.set_v_init_from(None if V is None else np.ascontiguousarray(V, dtype=complex))
+
+    @property
+    def n_scenarios(self):
+        return self._s.n_scenarios
+
+
+class ContingencyAnalysisGPU(_BatchFacade):
+    """N-k branch outages as value edits on the shared Ybus pattern
+    (``contingency_analysis/gpu_facade.py``)."""
+
+    _session_cls = _cpp.ContingencyAnalysisSession
+
+    def __init__(self, grid, **kw):
+        super().__init__(grid, **kw)
+        self._ctg = []
+
+    def set_branch_data(self, branch_from, branch_to, yff_eff, yft_eff, ytf_eff, ytt_eff, bus_vn_kv=None,
+                        sn_mva=100.0):
+        n = self._s.n_bus
+        vn = np.ones(n) if bus_vn_kv is None else np.ascontiguousarray(bus_vn_kv, dtype=float)
+        self._s.set_branch_data(np.ascontiguousarray(branch_from, dtype=np.int32),
+                                np.ascontiguousarray(branch_to, dtype=np.int32),
+                                *[np.ascontiguousarray(y, dtype=complex) for y in (yff_eff, yft_eff, ytf_eff,
+                                                                                   ytt_eff)],
+                                vn, float(sn_mva))
+
+    def add_contingencies_by_branch_id(self, branch_ids_per_ctg):
+        """One contingency per list of tripped branch ids."""
+        self._ctg.extend([list(map(int, c)) for c in branch_ids_per_ctg])
+        self._s.build_contingencies(self._ctg)
+
+    def add_contingency(self, branch_ids=()):
+        self.add_contingencies_by_branch_id([list(branch_ids)])
+
+    def get_contingencies(self):
+        return [list(c) for c in self._ctg]
+
+    @property
+    def n_contingencies(self):
+        return self._s.n_contingencies
+
+    @property
+    def n_branches(self):
+        return self._s.n_branches
+
+
+class ScenarioSweepGPU(InjectionSweepGPU):
+    """Per-row injections and per-row branch trips together
+    (``scenario_sweep/gpu_facade.py``)."""
+
+    _session_cls = _cpp.ScenarioSweepSession
+
+    def set_branch_data(self, branch_from, branch_to, yff_eff, yft_eff, ytf_eff, ytt_eff, bus_vn_kv=None,
+                        sn_mva=100.0):
+        ContingencyAnalysisGPU.set_branch_data(self, branch_from, branch_to, yff_eff, yft_eff, ytf_eff, ytt_eff,
+                                               bus_vn_kv, sn_mva)
+
+    def set_topology(self, branch_ids_per_scenario):
+        self._s.set_topology([list(map(int, c)) for c in branch_ids_per_scenario])
+
+    def get_disconnected(self):
+        return self._s.get_disconnected().astype(bool)

@@ -1,0 +1,136 @@
+"""Learned warm start: training the MLP on solver solutions and solving from
+its predictions with a flat-start fallback.
+
+The reference starts every row from the base case's converged V or from a
+caller's V (``src/_cpp/python_bindings.cpp:2722-2736``); it has no learned
+start.  Here an MLP maps a row's per-bus injections (P, Q per unit, plus an
+optional 0/1 outage mask) to its voltage angles and magnitudes.  The device
+side is ``g2g_mlp_*`` / ``g2g_batch_solve_warm``; this module prepares the
+training data on the device (torch tensors used as device buffers), keeps the
+normalisation, and drives the training loop.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _capi
+
+PV, PQ, SLACK = 0, 1, 2
+
+
+def bus_kinds(n, pv, pq, slack):
+    kind = np.full(n, PQ, np.int32)
+    kind[np.asarray(pv, np.int64)] = PV
+    kind[np.atleast_1d(np.asarray(slack, np.int64))] = SLACK
+    return kind
+
+
+class WarmStart:
+    """An MLP warm start for one grid.
+
+    n: buses; pv, pq, slack: bus indices; vset: |V| set-points (the flat
+    start's magnitudes, used at PV and slack buses); n_mask: outage-mask
+    width (0: injections only).
+    """
+
+    def __init__(self, n, pv, pq, slack, vset, n_hidden=1024, n_mask=0, seed=0, stream=None):
+        self.n = int(n)
+        self.n_mask = int(n_mask)
+        self.n_in = 2 * self.n + self.n_mask
+        self.n_out = 2 * self.n
+        self.mlp = _capi.Mlp(self.n_in, n_hidden, self.n_out, seed=seed, stream=stream)
+        self.kind = bus_kinds(self.n, pv, pq, slack)
+        self.vset = np.asarray(vset, np.float64)
+        self.mlp.set_buses(self.kind, self.vset)
+        self.in_mu = self.in_sigma = self.out_mu = self.out_sigma = None
+
+    # -- data ------------------------------------------------------------
+    @staticmethod
+    def _interleave(a, b):
+        out = a.new_empty((a.shape[0], 2 * a.shape[1]))
+        out[:, 0::2] = a
+        out[:, 1::2] = b
+        return out
+
+    def features(self, S, mask=None):
+        """S: complex128 tensor [rows, n] (device) -> raw features [rows, n_in] float64."""
+        F = self._interleave(S.real, S.imag)
+        if self.n_mask:
+            import torch
+            F = torch.cat([F, mask.to(F.dtype)], dim=1)
+        return F
+
+    def targets(self, V):
+        return self._interleave(torch_angle(V), V.abs())
+
+    def fit_normalization(self, S, V, mask=None):
+        """Feature / target mean and spread from training rows (device tensors)."""
+        F = self.features(S, mask)
+        T = self.targets(V)
+
+        def ms(x):
+            mu = x.mean(0)
+            sd = x.std(0)
+            sd = sd.where(sd > 1e-9, sd.new_ones(()))
+            return mu.float().cpu().numpy(), sd.float().cpu().numpy()
+
+        self.in_mu, self.in_sigma = ms(F)
+        if self.n_mask:  # the mask enters raw (0/1)
+            self.in_mu[2 * self.n:] = 0.0
+            self.in_sigma[2 * self.n:] = 1.0
+        self.out_mu, self.out_sigma = ms(T)
+        self.mlp.set_normalization(self.in_mu, self.in_sigma, self.out_mu, self.out_sigma)
+
+    def _normalised(self, S, V, mask=None):
+        import torch
+        dev = S.device
+        X = ((self.features(S, mask) - torch.as_tensor(self.in_mu, device=dev, dtype=torch.float64))
+             / torch.as_tensor(self.in_sigma, device=dev, dtype=torch.float64)).float().contiguous()
+        T = ((self.targets(V) - torch.as_tensor(self.out_mu, device=dev, dtype=torch.float64))
+             / torch.as_tensor(self.out_sigma, device=dev, dtype=torch.float64)).float().contiguous()
+        return X, T
+
+    # -- training ----------------------------------------------------------
+    def train(self, S, V, mask=None, epochs=20, batch=1024, lr=1e-3, seed=0, log=None):
+        """Adam on (features(S), targets(V)); S, V: device complex128 tensors of
+        converged training rows.  Returns the per-epoch mean losses."""
+        import torch
+        if self.in_mu is None:
+            self.fit_normalization(S, V, mask)
+        X, T = self._normalised(S, V, mask)
+        rows = X.shape[0]
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        hist = []
+        for ep in range(epochs):
+            perm = torch.randperm(rows, generator=g).to(X.device)
+            Xs, Ts = X[perm].contiguous(), T[perm].contiguous()
+            torch.cuda.synchronize()
+            tot, cnt = 0.0, 0
+            for r0 in range(0, rows - batch + 1, batch):
+                xp = Xs.data_ptr() + r0 * self.n_in * 4
+                tp = Ts.data_ptr() + r0 * self.n_out * 4
+                tot += self.mlp.train_step_device(xp, tp, batch, lr)
+                cnt += 1
+            hist.append(tot / max(cnt, 1))
+            if log:
+                log(f"warm-start epoch {ep}: loss {hist[-1]:.3e}")
+        return hist
+
+    def predict_v(self, S, mask=None):
+        """Start voltages predicted for S (host complex [rows, n]) -> host complex."""
+        import torch
+        St = torch.as_tensor(np.atleast_2d(S), device="cuda")
+        X, _ = self._normalised(St, St.new_ones(St.shape), mask)
+        Y = torch.empty((X.shape[0], self.n_out), device="cuda", dtype=torch.float32)
+        torch.cuda.synchronize()
+        self.mlp.forward_device(X.data_ptr(), X.shape[0], Y.data_ptr())
+        Yd = Y.double().cpu().numpy() * self.out_sigma + self.out_mu
+        va, vm = Yd[:, 0::2], Yd[:, 1::2]
+        vm[:, self.kind != PQ] = self.vset[self.kind != PQ]
+        va[:, self.kind == SLACK] = 0.0
+        return vm * np.exp(1j * va)
+
+
+def torch_angle(V):
+    import torch
+    return torch.atan2(V.imag, V.real)

@@ -82,6 +82,9 @@ typedef struct {
 
 const char *g2g_last_error(void);
 int g2g_version(void);
+/* Makes `device` the calling thread's current device (-1: leave it), as the
+ * reference's `device` arguments do (acpf_nr.cu:23-39). */
+int g2g_set_device(int32_t device);
 
 /* Builds every row-independent table of a grid: the reference's [pvpq | pq]
  * Jacobian pattern and scatter maps (replaces build_J_structure,
@@ -152,6 +155,10 @@ int g2g_batch_solve(g2g_batch_t batch, int32_t max_iter, double tol, int32_t sch
 int g2g_batch_solve_warm(g2g_batch_t batch, g2g_mlp_t mlp, const uint8_t *mask, int32_t n_mask, int32_t k_warm,
                          double growth, int32_t max_iter, double tol, g2g_solve_stats *stats);
 
+/* The MLP's start voltages for the active rows, written as the batch's V
+ * (no counterpart in the reference; same mask convention as above). */
+int g2g_batch_warm_start(g2g_batch_t batch, g2g_mlp_t mlp, const uint8_t *mask, int32_t n_mask);
+
 int g2g_batch_get_v(g2g_batch_t batch, double *V, int64_t ld, int32_t on_device);
 int g2g_batch_get_results(g2g_batch_t batch, int32_t *status, int32_t *iters, double *residual);
 
@@ -177,7 +184,7 @@ int g2g_batch_device_arrays(g2g_batch_t batch, void **V_tiles, void **S_tiles);
  * in -> H -> H -> out, bf16 MFMA products with fp32 accumulation, fp32
  * master weights and Adam state on the device.  Matrices are row-major
  * float32 (inputs M x n_in, outputs M x n_out); on_device = 1 when they point
- * to device memory.  Widths must be multiples of 8. */
+ * to device memory. */
 int g2g_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_out, uint64_t seed, void *stream, g2g_mlp_t *out);
 void g2g_mlp_destroy(g2g_mlp_t mlp);
 int64_t g2g_mlp_num_params(g2g_mlp_t mlp);

@@ -47,7 +47,7 @@ def _mlp(n_in, H, n_out, seed=0):
     return net, {k: np.asarray(v, np.float32).astype(np.float64) for k, v in p.items()}
 
 
-@pytest.mark.parametrize("n_in,H,n_out,m", [(48, 64, 40, 200), (136, 256, 96, 70)])
+@pytest.mark.parametrize("n_in,H,n_out,m", [(48, 64, 40, 200), (136, 256, 96, 70), (236, 128, 236, 33)])
 def test_mlp_forward_against_oracle(n_in, H, n_out, m):
     _say(f"mlp forward {n_in}-{H}-{n_out} m={m}")
     net, p = _mlp(n_in, H, n_out)
@@ -98,6 +98,6 @@ def test_mlp_training_reduces_loss():
 
 def test_mlp_rejects_bad_widths():
     with pytest.raises(ValueError):
-        _capi.Mlp(10, 64, 8)
-    with pytest.raises(ValueError):
         _capi.Mlp(0, 64, 8)
+    with pytest.raises(ValueError):
+        _capi.Mlp(16, -1, 8)

@@ -1,0 +1,144 @@
+"""Learned warm start with flat-start fallback, through the C ABI, against
+the CPU oracle's flat-start solutions."""
+import sys
+
+import numpy as np
+import pytest
+
+from gpusim2grid_amd import _capi
+from gpusim2grid_amd import synthetic as syn
+from gpusim2grid_amd.warmstart import WarmStart
+from oracle import mlp_ref as M
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _sbus(g, rows, seed):
+    p, q = syn.injection_scenarios(g, rows, seed=seed)
+    return (p + 1j * q) / g.sn_mva
+
+
+@pytest.fixture(scope="module")
+def trained():
+    import torch
+    g = syn.grid_118()
+    rows = 4096
+    _say("warm: solving the training set")
+    S = _sbus(g, rows, seed=2)
+    plan = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+    B = _capi.Batch(plan, rows)
+    B.set_sbus(S)
+    B.set_v(g.flat_start())
+    B.solve(max_iter=10, tol=1e-8)
+    st, _, _ = B.results()
+    V = B.get_v()
+    ok = st == 0
+    ws = WarmStart(g.n_bus, g.pv, g.pq, g.slack, np.abs(g.flat_start()), n_hidden=256, seed=0)
+    _say("warm: training")
+    hist = ws.train(torch.as_tensor(S[ok], device="cuda"), torch.as_tensor(V[ok], device="cuda"), epochs=40,
+                    batch=256, lr=1e-3)
+    return g, ws, hist
+
+
+def test_training_converges(trained):
+    _, _, hist = trained
+    assert hist[-1] < 0.05 * hist[0], hist
+
+
+def test_warm_start_kernel_matches_oracle_forward(trained):
+    g, ws, _ = trained
+    rows = 100
+    S = _sbus(g, rows, seed=7)
+    plan = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+    B = _capi.Batch(plan, rows)
+    B.set_sbus(S)
+    B.warm_start(ws.mlp)
+    Vw = B.get_v()
+    p = {k: v.astype(np.float64) for k, v in ws.mlp.get_params().items()}
+    X = M.features(S, ws.in_mu.astype(np.float64), ws.in_sigma.astype(np.float64))
+    Y, _ = M.forward(p, X, bf16_ops=True)
+    Vref = M.warm_v(Y, ws.out_mu.astype(np.float64), ws.out_sigma.astype(np.float64), ws.kind, ws.vset)
+    # bf16 products: a few 1e-3 of the output spread
+    tol = 0.02 * float(np.max(ws.out_sigma)) + 1e-6
+    assert np.max(np.abs(Vw - Vref)) < tol
+    # fixed magnitudes and the slack angle are exact
+    fixed = ws.kind != 1
+    np.testing.assert_allclose(np.abs(Vw[:, fixed]), np.broadcast_to(ws.vset[fixed], (rows, fixed.sum())),
+                               rtol=1e-14)
+    assert np.all(np.angle(Vw[:, g.slack]) == 0.0)
+    # the host prediction path agrees with the device one
+    Vh = ws.predict_v(S)
+    assert np.max(np.abs(Vh - Vw)) < 1e-5
+
+
+@pytest.mark.parametrize("prec", [_capi.FP64, _capi.MIXED])
+def test_warm_solve_matches_flat_start_oracle(trained, prec):
+    g, ws, _ = trained
+    rows = 1000
+    _say(f"warm solve prec={prec}")
+    S = _sbus(g, rows, seed=1)
+    plan = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+    B = _capi.Batch(plan, rows, precision=prec)
+    B.set_sbus(S)
+    B.set_v(g.flat_start())
+    stats = B.solve_warm(ws.mlp, k_warm=4, growth=2.0, max_iter=10, tol=1e-8)
+    st, it, res = B.results()
+    V = B.get_v()
+    Vo, sto, ito, _ = orc.nr_batch(g.Ybus, g.flat_start(), S, g.pv, g.pq, 10, 1e-8, n_threads=16)
+    np.testing.assert_array_equal(st, sto)
+    conv = st == 0
+    assert conv.all()
+    # every converged row satisfies the test, on the oracle's own mismatch
+    F = orc.mismatch_np(g.Ybus, V, S, g.pv, g.pq)
+    assert np.max(np.abs(F)) < 1e-8
+    assert np.max(res) < 1e-8
+    # same solution as the flat start (to the conditioning of the test)
+    assert np.max(np.abs(V - Vo)) < 1e-6
+    # fewer Newton updates than from the flat start
+    assert it.mean() < ito.mean(), (it.mean(), ito.mean())
+    assert stats["n_converged"] == rows
+
+
+def test_untrained_mlp_falls_back_to_flat_start():
+    g = syn.grid_118()
+    rows = 300
+    _say("warm fallback (untrained)")
+    S = _sbus(g, rows, seed=3)
+    ws = WarmStart(g.n_bus, g.pv, g.pq, g.slack, np.abs(g.flat_start()), n_hidden=64, seed=5)
+    n2 = 2 * g.n_bus
+    # normalisation that turns the random network's outputs into a poor start:
+    # angles spread by one radian
+    ws.in_mu = np.zeros(n2, np.float32)
+    ws.in_sigma = np.ones(n2, np.float32)
+    ws.out_mu = np.tile(np.array([0.0, 1.0], np.float32), g.n_bus)
+    ws.out_sigma = np.tile(np.array([1.0, 0.05], np.float32), g.n_bus)
+    ws.mlp.set_normalization(ws.in_mu, ws.in_sigma, ws.out_mu, ws.out_sigma)
+    plan = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+    B = _capi.Batch(plan, rows)
+    B.set_sbus(S)
+    B.set_v(g.flat_start())
+    stats = B.solve_warm(ws.mlp, k_warm=2, growth=2.0, max_iter=10, tol=1e-8)
+    st, it, _ = B.results()
+    V = B.get_v()
+    Vo, sto, ito, _ = orc.nr_batch(g.Ybus, g.flat_start(), S, g.pv, g.pq, 10, 1e-8, n_threads=16)
+    np.testing.assert_array_equal(st, sto)
+    assert stats["n_requeued"] > 0
+    # re-queued rows restart from the flat start: their second pass is the
+    # oracle's flat-start run, iteration for iteration, on top of the warm pass
+    Bf = _capi.Batch(plan, rows)
+    Bf.set_sbus(S)
+    Bf.set_v(g.flat_start())
+    Bf.solve(max_iter=10, tol=1e-8)
+    stf, itf, _ = Bf.results()
+    Vf = Bf.get_v()
+    np.testing.assert_array_equal(itf, ito)
+    re = it > 2
+    assert re.any()
+    np.testing.assert_array_equal(it[re] - itf[re] <= 2, True)
+    assert np.max(np.abs(V[re] - Vf[re])) < 1e-12
