@@ -41,7 +41,14 @@ def parse():
     ap.add_argument("--rows", type=int, default=16384)
     ap.add_argument("--max-iter", type=int, default=10)
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--methods", default="fp64,mixed")
+    ap.add_argument("--methods", default="fp64,mixed,warm")
+    ap.add_argument("--train-rows", type=int, default=16384, help="warm start: solved training scenarios")
+    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--hidden", type=int, default=1024)
+    ap.add_argument("--train-batch", type=int, default=512)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--k-warm", type=int, default=4)
+    ap.add_argument("--growth", type=float, default=2.0)
     ap.add_argument("--cpu-rows", type=int, default=512, help="oracle sample size (0: skip)")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
@@ -52,7 +59,41 @@ def log(args, *msg):
         print(*msg, file=sys.stderr, flush=True)
 
 
-def run_method(args, name, g, S, torch, capi):
+def train_warm(args, g, torch, capi, syn):
+    """Learned warm start: solve fresh training scenarios (seed 2) with the
+    mixed-precision solver, then train the MLP on them -- both outside the
+    timed region, like the symbolic analysis."""
+    from gpusim2grid_amd.warmstart import WarmStart
+    t0 = time.perf_counter()
+    rows = args.train_rows
+    p, q = syn.injection_scenarios(g, rows, seed=2)
+    S = torch.from_numpy((p + 1j * q) / g.sn_mva).to("cuda")
+    del p, q
+    plan = capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+    B = capi.Batch(plan, rows, precision=capi.MIXED)
+    B.set_sbus_device(S.data_ptr(), S.shape[1], rows)
+    B.set_v(g.flat_start())
+    B.solve(max_iter=args.max_iter, tol=args.tol, scheduling=capi.CONTINUOUS)
+    st, _, _ = B.results()
+    V = torch.empty((rows, g.n_bus), dtype=torch.complex128, device="cuda")
+    torch.cuda.synchronize()
+    B.get_v_device(V.data_ptr(), g.n_bus)
+    del B
+    ok = torch.from_numpy(st == 0).to("cuda")
+    S, V = S[ok].contiguous(), V[ok].contiguous()
+    t_data = time.perf_counter() - t0
+    ws = WarmStart(g.n_bus, g.pv, g.pq, g.slack, np.abs(g.flat_start()), n_hidden=args.hidden, seed=2)
+    t1 = time.perf_counter()
+    hist = ws.train(S, V, epochs=args.epochs, batch=args.train_batch, lr=args.lr,
+                    log=(lambda m: log(args, m)))
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    del S, V
+    return ws, dict(train_rows=int(ok.sum()), epochs=args.epochs, hidden=args.hidden, loss_first=hist[0],
+                    loss_last=hist[-1], data_s=t_data, train_s=t_train)
+
+
+def run_method(args, name, g, S, torch, capi, ws=None):
     prec = capi.FP64 if name == "fp64" else capi.MIXED
     plan = capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
     info = plan.info()
@@ -65,6 +106,9 @@ def run_method(args, name, g, S, torch, capi):
 
     def step():
         B.set_v_device(V0.data_ptr(), 0, args.rows)  # flat start (broadcast), inside the step
+        if ws is not None:
+            return B.solve_warm(ws.mlp, k_warm=args.k_warm, growth=args.growth, max_iter=args.max_iter,
+                                tol=args.tol)
         return B.solve(max_iter=args.max_iter, tol=args.tol, scheduling=capi.CONTINUOUS)
 
     for _ in range(args.warmup):
@@ -87,6 +131,8 @@ def run_method(args, name, g, S, torch, capi):
         max_residual=float(res[st == 0].max()) if conv else None,
         ms_mismatch=np.mean([s["ms_mismatch"] for s in stats]), ms_factor=ms_factor / args.steps,
         ms_backward=np.mean([s["ms_backward"] for s in stats]),
+        ms_warm_start=np.mean([s["ms_warm_start"] for s in stats]),
+        n_requeued=stats[-1]["n_requeued"],
         factor_launch_ms=ms_factor / max(launches, 1),
         factor_alg_bytes_per_launch=alg_bytes / max(launches, 1),
         factor_achieved_gbs=alg_bytes / (ms_factor * 1e-3) / 1e9 if ms_factor > 0 else None,
@@ -141,9 +187,16 @@ def main():
     log(args, f"grid + scenarios {time.perf_counter() - t0:.1f} s")
     methods = {}
     keep = {}
+    warm_info = None
     for name in args.methods.split(","):
         t1 = time.perf_counter()
-        methods[name], V, st = run_method(args, name, g, S, torch, capi)
+        ws = None
+        if name == "warm":
+            ws, warm_info = train_warm(args, g, torch, capi, syn)
+            log(args, "warm-start training", warm_info)
+        methods[name], V, st = run_method(args, name, g, S, torch, capi, ws)
+        if name == "warm":
+            methods[name]["training"] = warm_info
         keep[name] = (V, st)
         log(args, f"{name}: {methods[name]['pfps']:.0f} PF/s ({time.perf_counter() - t1:.1f} s)",
             json.dumps({k: v for k, v in methods[name].items() if k != 'info'}))
@@ -152,6 +205,15 @@ def main():
         V, st = keep.get("fp64", (None, None))
         cpu, parity = cpu_baseline(args, g, S, V, st)
         log(args, "cpu", cpu, parity)
+    # every method's answers against the plain FP64 method's (same rows)
+    if "fp64" in keep:
+        V64, st64 = keep["fp64"]
+        for k, (Vk, stk) in keep.items():
+            if k == "fp64":
+                continue
+            ok = (stk == 0) & (st64 == 0)
+            methods[k]["vs_fp64"] = dict(status_equal=bool(np.array_equal(stk, st64)),
+                                         max_dV=float(np.abs(Vk[ok] - V64[ok]).max()) if ok.any() else None)
     head = methods.get("warm") or methods.get("mixed") or methods.get("fp64")
     fp64 = methods.get("fp64")
     dom = head
@@ -164,7 +226,7 @@ def main():
         ms_per_step=head["ms_per_step"], higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype="f64" if head["method"] == "fp64" else "f32+f64",
         data="synthetic (seeded 6,500-bus grid, 16,384 lognormal injection scenarios)",
-        config=dict(workload="6500-bus injection sweep, flat start, ||dS||inf<1e-8", n_bus=g.n_bus,
+        config=dict(workload="6500-bus injection sweep, 16384 scenarios, ||dS||inf<1e-8", n_bus=g.n_bus,
                     n_branch=g.n_branch, rows=args.rows, method=head["method"], max_iter=args.max_iter),
         ratio_vs_plain_fp64=(head["pfps"] / fp64["pfps"]) if fp64 else None,
         methods={k: {kk: vv for kk, vv in v.items() if kk != "info"} for k, v in methods.items()},

@@ -123,6 +123,12 @@ def test_injection_sweep_against_oracle(scheduling):
     np.testing.assert_array_equal(sw.get_row_status(), sto)
     if scheduling == "continuous":
         np.testing.assert_array_equal(sw.get_row_iterations(), ito)
+    else:
+        # chunked: every row runs exactly nb_iter updates; the oracle does the
+        # same when its test can never pass
+        Vo, _, ito, _ = orc.nr_batch(g.Ybus, Vb, S, g.pv, g.pq, 6, 1e-300, n_threads=8)
+        assert (ito == 6).all()
+        np.testing.assert_array_equal(sw.get_row_iterations(), ito)
     assert np.max(np.abs(V - Vo)) < 1e-9
     assert (sw.last_residuals() < 1e-8).all()
     # batch-size invariance (test_contingency_batch.py:174-222's pattern)

@@ -71,7 +71,7 @@ def test_gpu_matches_fixture(name, prec):
     Y = _ybus(z)
     S = np.atleast_2d(z["S"])
     rows = np.atleast_2d(z["V"]).shape[0]
-    plan = _capi.Plan.from_grid(Y, z["pv"], z["pq"], int(z["slack"]))
+    plan = _capi.Plan.from_grid(Y, z["pv"], z["pq"], int(np.asarray(z["slack"]).reshape(-1)[0]))
     edits = "row_ptr" in z
     B = _capi.Batch(plan, rows, precision=prec, per_row_ybus=edits)
     B.set_sbus(S[0] if S.shape[0] == 1 else S)

@@ -95,8 +95,8 @@ def test_warm_solve_matches_flat_start_oracle(trained, prec):
     conv = st == 0
     assert conv.all()
     # every converged row satisfies the test, on the oracle's own mismatch
-    F = orc.mismatch_np(g.Ybus, V, S, g.pv, g.pq)
-    assert np.max(np.abs(F)) < 1e-8
+    F = max(np.max(np.abs(orc.mismatch_np(g.Ybus, V[r], S[r], g.pv, g.pq))) for r in range(rows))
+    assert F < 1e-8
     assert np.max(res) < 1e-8
     # same solution as the flat start (to the conditioning of the test)
     assert np.max(np.abs(V - Vo)) < 1e-6
