@@ -42,8 +42,8 @@ def parse():
     ap.add_argument("--max-iter", type=int, default=10)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--methods", default="fp64,mixed,warm")
-    ap.add_argument("--train-rows", type=int, default=16384, help="warm start: solved training scenarios")
-    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--train-rows", type=int, default=32768, help="warm start: solved training scenarios")
+    ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--hidden", type=int, default=1024)
     ap.add_argument("--train-batch", type=int, default=512)
     ap.add_argument("--lr", type=float, default=1e-3)

@@ -37,9 +37,10 @@ __global__ void k_gather_astream(const double* vals, long long ldv, const double
 __global__ void k_row_absmax(const double* vals, long long ldv, int rows, int nnz, double* out);
 __global__ void k_fill_int(int* p, int v, long long n);
 __global__ void k_count_tiles(const int* status, int rows_pad, int* tile_active);
-__global__ void k_requeue_v(const int* status, int rows, int n, const double2* V0, double2* V);
-__global__ void k_requeue_rows(int* status, int* iters, int* iters_warm, int rows, int* n_requeued);
-__global__ void k_add_int(int* p, const int* add, int n);
+template <typename T>
+__global__ void k_move_rows(const T* src, T* dst, const int* map, int R, int m, int gather);
+template <typename T>
+__global__ void k_move_vals(const T* src, T* dst, const int* map, int R, int gather, int add);
 
 // learned warm start (mlp_capi.cpp)
 void mlp_warm_start_tiles(g2g_mlp_s* p, const double2* S_tiles, int M, int n, const unsigned char* d_mask,
@@ -134,15 +135,23 @@ struct g2g_batch_s {
     int* clamped = nullptr;
     int* error = nullptr;     // stream kernels' give-up flag
     int* h_active = nullptr;  // pinned
-    double2* V0 = nullptr;    // warm solves: the caller's start voltages, kept for re-queued rows
-    int* iters_warm = nullptr;
-    int* n_requeued = nullptr;
+    // warm solves: the caller's start voltages, and the dense work area of the
+    // re-queued rows (V, Sbus, per-row Ybus, saved per-row results, row map)
+    double2* V0 = nullptr;
+    double2* Vc = nullptr;
+    double2* Sc = nullptr;
+    double2* Yc = nullptr;
+    int* st_save = nullptr;
+    int* it_save = nullptr;
+    double* res_save = nullptr;
+    int* row_map = nullptr;
     std::vector<hipEvent_t> ev;
     ~g2g_batch_s()
     {
         dfree(V); dfree(S); dfree(Yrow); dfree(A); dfree(U); dfree(X); dfree(res); dfree(jmax);
         dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped); dfree(error);
-        dfree(V0); dfree(iters_warm); dfree(n_requeued);
+        dfree(V0); dfree(Vc); dfree(Sc); dfree(Yc); dfree(st_save); dfree(it_save); dfree(res_save);
+        dfree(row_map);
         if (h_active) (void)hipHostFree(h_active);
         for (auto e : ev) (void)hipEventDestroy(e);
     }
@@ -171,7 +180,8 @@ size_t ring_lds()
 template <typename Real>
 size_t factor_lds(const SymbolicLU& s)
 {
-    return ring_lds<Real>() + static_cast<size_t>(s.slots_f + 2 + 2 * kMaxC) * kLanes * sizeof(Real);
+    const int scratch = s.max_c > kFastC ? 2 * kMaxC : 0;  // rows for the generic long-column path
+    return ring_lds<Real>() + static_cast<size_t>(s.slots_f + 2 + scratch) * kLanes * sizeof(Real);
 }
 
 template <typename Real>
@@ -374,18 +384,31 @@ void solve_impl(g2g_batch_s* b, int max_iter, double tol, int sched, g2g_solve_s
 // Learned warm start with fallback: the MLP predicts every active row's V,
 // Newton runs at most k_warm updates on it (a row whose ||F||inf grows by
 // more than `growth` is dropped at once), and the rows left unconverged
-// restart from the caller's start voltages with the full budget.  A row's
+// restart from the caller's start voltages with the full budget.  The
+// restarted rows are first gathered into dense tiles, so the fallback pass
+// costs in proportion to their number, then scattered back.  A row's
 // iteration count is the sum of the updates of both passes.
 template <typename Real>
 void solve_warm_impl(g2g_batch_s* b, g2g_mlp_s* mlp, const unsigned char* mask, int n_mask, int k_warm,
                      double growth, int max_iter, double tol, g2g_solve_stats* stats)
 {
     const NrPlan& nr = b->plan->nr;
-    const size_t nv = static_cast<size_t>(b->rows_pad) * nr.n;
+    const size_t rp = static_cast<size_t>(b->rows_pad);
+    const size_t nv = rp * nr.n;
     if (!b->V0) {
         b->V0 = dalloc<double2>(nv);
-        b->iters_warm = dalloc<int>(static_cast<size_t>(b->rows_pad));
-        b->n_requeued = dalloc<int>(1);
+        b->Vc = dalloc<double2>(nv);
+        b->Sc = dalloc<double2>(nv);
+        if (b->per_row_y) b->Yc = dalloc<double2>(rp * nr.nY);
+        b->st_save = dalloc<int>(rp);
+        b->it_save = dalloc<int>(rp);
+        b->res_save = dalloc<double>(rp);
+        b->row_map = dalloc<int>(rp);
+        // lanes beyond the re-queued rows stay finite (|V| = 1)
+        HIPCHK(hipMemcpyAsync(b->Vc, b->V, nv * sizeof(double2), hipMemcpyDeviceToDevice, b->stream));
+        HIPCHK(hipMemsetAsync(b->Sc, 0, nv * sizeof(double2), b->stream));
+        if (b->Yc)
+            HIPCHK(hipMemcpyAsync(b->Yc, b->Yrow, rp * nr.nY * sizeof(double2), hipMemcpyDeviceToDevice, b->stream));
     }
     DeviceScratch tmp;
     unsigned char* d_mask = nullptr;
@@ -402,24 +425,63 @@ void solve_warm_impl(g2g_batch_s* b, g2g_mlp_s* mlp, const unsigned char* mask, 
     tm.ph[3].push_back({e_begin, tm.mark()});
     long long rows_factored = 0;
     int it = nr_loop<Real>(b, k_warm, tol, 1, growth, tm, rows_factored);
-    // re-queue
-    k_fill_int<<<1, 64, 0, b->stream>>>(b->n_requeued, 0, 1);
-    k_requeue_v<<<blocks_for(static_cast<long long>(nv), 256), 256, 0, b->stream>>>(b->status, b->active_rows,
-                                                                                    nr.n, b->V0, b->V);
-    k_requeue_rows<<<blocks_for(b->active_rows, 256), 256, 0, b->stream>>>(b->status, b->iters, b->iters_warm,
-                                                                           b->active_rows, b->n_requeued);
-    k_count_tiles<<<b->T, kLanes, 0, b->stream>>>(b->status, b->rows_pad, b->tile_active);
-    HIPCHK(hipGetLastError());
-    int requeued = 0;
-    HIPCHK(hipMemcpyAsync(&requeued, b->n_requeued, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    // the rows to restart, in order
+    const int rows = b->active_rows;
+    std::vector<int> st(static_cast<size_t>(rows));
+    HIPCHK(hipMemcpyAsync(st.data(), b->status, st.size() * sizeof(int), hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
-    if (requeued > 0) it += nr_loop<Real>(b, max_iter, tol, 1, 0.0, tm, rows_factored);
-    k_add_int<<<blocks_for(b->active_rows, 256), 256, 0, b->stream>>>(b->iters, b->iters_warm, b->active_rows);
-    HIPCHK(hipGetLastError());
+    std::vector<int> map;
+    for (int r = 0; r < rows; ++r)
+        if (st[r] == kMaxIter || st[r] == kDiverged || st[r] == kRejected) map.push_back(r);
+    const int R = static_cast<int>(map.size());
+    if (R > 0) {
+        hipStream_t s = b->stream;
+        HIPCHK(hipMemcpyAsync(b->row_map, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(b->st_save, b->status, rp * sizeof(int), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(b->it_save, b->iters, rp * sizeof(int), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(b->res_save, b->res, rp * sizeof(double), hipMemcpyDeviceToDevice, s));
+        const unsigned gv = blocks_for(static_cast<long long>(R) * nr.n, 256);
+        k_move_rows<double2><<<gv, 256, 0, s>>>(b->V0, b->Vc, b->row_map, R, nr.n, 1);
+        k_move_rows<double2><<<gv, 256, 0, s>>>(b->S, b->Sc, b->row_map, R, nr.n, 1);
+        if (b->per_row_y)
+            k_move_rows<double2><<<blocks_for(static_cast<long long>(R) * nr.nY, 256), 256, 0, s>>>(
+                b->Yrow, b->Yc, b->row_map, R, nr.nY, 1);
+        HIPCHK(hipGetLastError());
+        std::swap(b->V, b->Vc);
+        std::swap(b->S, b->Sc);
+        if (b->per_row_y) std::swap(b->Yrow, b->Yc);
+        b->active_rows = R;
+        try {
+            reset_rows(b);
+            it += nr_loop<Real>(b, max_iter, tol, 1, 0.0, tm, rows_factored);
+        } catch (...) {
+            std::swap(b->V, b->Vc);
+            std::swap(b->S, b->Sc);
+            if (b->per_row_y) std::swap(b->Yrow, b->Yc);
+            b->active_rows = rows;
+            throw;
+        }
+        std::swap(b->V, b->Vc);
+        std::swap(b->S, b->Sc);
+        if (b->per_row_y) std::swap(b->Yrow, b->Yc);
+        b->active_rows = rows;
+        // results of the restarted rows back to their places
+        k_move_rows<double2><<<gv, 256, 0, s>>>(b->Vc, b->V, b->row_map, R, nr.n, 0);
+        const unsigned gr = blocks_for(R, 256);
+        k_move_vals<int><<<gr, 256, 0, s>>>(b->status, b->st_save, b->row_map, R, 0, 0);
+        k_move_vals<int><<<gr, 256, 0, s>>>(b->iters, b->it_save, b->row_map, R, 0, 1);
+        k_move_vals<double><<<gr, 256, 0, s>>>(b->res, b->res_save, b->row_map, R, 0, 0);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(b->status, b->st_save, rp * sizeof(int), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(b->iters, b->it_save, rp * sizeof(int), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(b->res, b->res_save, rp * sizeof(double), hipMemcpyDeviceToDevice, s));
+        k_count_tiles<<<b->T, kLanes, 0, s>>>(b->status, b->rows_pad, b->tile_active);
+        HIPCHK(hipGetLastError());
+    }
     const int e_end = tm.mark();
     check_stream_error(b);
     fill_stats(b, tm, e_begin, e_end, it, rows_factored, stats);
-    if (stats) stats->n_requeued = requeued;
+    if (stats) stats->n_requeued = R;
 }
 
 // rows x m complex (row stride ld complex; ld = 0 broadcasts) -> tiled [T][m][64]

@@ -202,18 +202,9 @@ using ProgRing = Ring<int, kProgChunk, kProgSlots, false, false>;
 using ProgReader = ProgRing::Reader;
 constexpr int kCtlBytes = 256;  // the two streams' control blocks
 
-// 16-bit field q of a record whose refs start at word `Head`
-template <int Head, int NW>
-__device__ __forceinline__ unsigned ref_of(const v4u (&w)[NW / 4], int q)
-{
-    const int word = Head + q / 2;
-    const unsigned v = w[word / 4][word % 4];
-    return (q & 1) ? (v >> 16) : (v & 0xFFFFu);
-}
-
 // NW program words from position p (4-aligned) as 16-byte broadcast reads
 template <int NW>
-__device__ __forceinline__ void read_record(const ProgReader& P, int p, v4u (&w)[NW / 4])
+__device__ __forceinline__ void read_words(const ProgReader& P, int p, v4u (&w)[NW / 4])
 {
     const v4u* base = reinterpret_cast<const v4u*>(P.ring);
 #pragma unroll
@@ -226,124 +217,164 @@ __device__ __forceinline__ int prog_word(ProgReader& P, int p)
     return __builtin_amdgcn_readfirstlane(P.at(p));
 }
 
+// word q of a record section held in registers
+template <int N>
+__device__ __forceinline__ unsigned word(const v4u (&w)[N], int q)
+{
+    return w[q / 4][q % 4];
+}
+
+// this lane's element of the slot row at byte offset `ref` (in FP32 rows)
+template <typename Real>
+__device__ __forceinline__ Real* slot_at(unsigned char* lane_base, unsigned ref)
+{
+    return reinterpret_cast<Real*>(lane_base + ref * (sizeof(Real) / 4));
+}
+
+// N consecutive stream positions from p (already inside the reader's window):
+// one base address and immediate offsets unless the run wraps the ring
+template <int N, typename RD, typename T>
+__device__ __forceinline__ void load_values(const RD& R, int p, T (&v)[N])
+{
+    const int b = p & (RD::kRingLen - 1);
+    if (b + N <= RD::kRingLen) {
+        const T* q = R.ring + static_cast<size_t>(b) * kLanes;
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = q[i * kLanes];
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = R.at(p + i);
+    }
+}
+
 template <typename Real>
 struct FactorState {
-    Real* sl;        // slot row 0 of this lane: slot s at sl[s * 64]
-    Real* U;         // U stream of this lane
+    unsigned char* lb;  // this lane's slot base
+    Real* U;            // U stream of this lane
     Real eps;
     int clamped;
     int p, ap, up;
 };
 
-// One pivot with C off-diagonal entries: every slot read of the pivot is
-// issued before its slot writes (sources, then the old values of the update
-// targets, pipelined one row ahead), so LDS latency is paid a few times per
-// pivot, not once per entry.
+// One pivot with C off-diagonal entries.  The whole record is read at once;
+// then every slot the pivot reads -- its sources and, for short columns, the
+// old values of all its update targets -- is read in one batch (a pivot's
+// sources and targets never share a slot), so the pivot pays about two LDS
+// round trips.  Sources are cleared after the read; targets accumulate.
+// Longer columns read their update targets row by row, one row ahead of the
+// writes.  The U record stores y(k), 1 / U(k,k) and the U row.
+constexpr int kAllReadC = 5;
+
 template <int C, typename Real, typename AR>
 __device__ __forceinline__ void factor_pivot(AR& A, ProgReader& P, FactorState<Real>& st)
 {
-    constexpr int NW = frec_words(C);
+    constexpr int NS = frec_src_words(C), NW = frec_words(C), NV = 2 + 2 * C;
+    constexpr bool kAll = C <= kAllReadC;
     P.window(st.p, st.p + NW - 1);
+    A.window(st.ap, st.ap + NV - 1);
     v4u w[NW / 4];
-    read_record<NW>(P, st.p, w);
-    A.window(st.ap, st.ap + 1 + 2 * C);
-    auto aval = [&](int i) { return A.at(st.ap + i); };
-    auto slot = [&](unsigned ref) -> Real& { return st.sl[static_cast<size_t>(ref & kRefSlot) * kLanes]; };
-    Real dv = aval(1) + slot(ref_of<1, NW>(w, 0));
-    Real fk = aval(0) + slot(ref_of<1, NW>(w, 1));
+    read_words<NW>(P, st.p, w);
+    Real av[NV];
+    load_values<NV>(A, st.ap, av);
+    auto at = [&](int q) -> Real* { return slot_at<Real>(st.lb, word(w, q)); };
+    Real sv[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sv[q] = *at(1 + q);
+    Real tv[kAll && C > 0 ? C * C : 1], tb[C > 0 ? C : 1];
+    if constexpr (kAll) {
+#pragma unroll
+        for (int i = 0; i < C * C; ++i) tv[i] = *at(NS + i);
+    }
+#pragma unroll
+    for (int a = 0; a < C; ++a) tb[a] = *at(NS + C * C + a);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) *at(1 + q) = Real(0);
+    Real dv = av[1] + sv[0];
+    const Real fk = av[0] + sv[1];
     Real lv[C > 0 ? C : 1], uv[C > 0 ? C : 1];
 #pragma unroll
-    for (int q = 0; q < C; ++q) lv[q] = aval(2 + q) + slot(ref_of<1, NW>(w, 2 + q));
+    for (int q = 0; q < C; ++q) lv[q] = av[2 + q] + sv[2 + q];
 #pragma unroll
-    for (int q = 0; q < C; ++q) uv[q] = aval(2 + C + q) + slot(ref_of<1, NW>(w, 2 + C + q));
-    // old values of the first update row and of the right-hand-side targets
-    Real t0[C > 0 ? C : 1], t1[C > 0 ? C : 1], tb[C > 0 ? C : 1];
-#pragma unroll
-    for (int q = 0; q < C; ++q) {
-        const unsigned ref = ref_of<1, NW>(w, 2 + 2 * C + q);
-        const Real old = slot(ref);
-        t0[q] = (ref & kRefFirst) ? Real(0) : old;
-    }
-#pragma unroll
-    for (int q = 0; q < C; ++q) {
-        const unsigned ref = ref_of<1, NW>(w, 2 + 2 * C + C * C + q);
-        const Real old = slot(ref);
-        tb[q] = (ref & kRefFirst) ? Real(0) : old;
-    }
-    st.ap += 2 + 2 * C;
+    for (int q = 0; q < C; ++q) uv[q] = av[2 + C + q] + sv[2 + C + q];
+    st.ap += NV;
     if (fabs(dv) < st.eps) {  // static-pivot clamp (own_lu/batched_lu.cu:45-53); NaN passes
         dv = dv < Real(0) ? -st.eps : st.eps;
         ++st.clamped;
     }
     const Real inv = Real(1) / dv;
 #pragma unroll
-    for (int q = 0; q < C; ++q) lv[q] *= inv;
-    // U record: y(k), U(k,k), U(k,j)...
-    st.U[static_cast<size_t>(st.up) * kLanes] = fk;
-    st.U[static_cast<size_t>(st.up + 1) * kLanes] = dv;
+    for (int q = 0; q < C; ++q) lv[q] *= -inv;  // negated multipliers
+    Real* u = st.U + static_cast<size_t>(st.up) * kLanes;
+    u[0] = fk;
+    u[kLanes] = inv;
 #pragma unroll
-    for (int q = 0; q < C; ++q) st.U[static_cast<size_t>(st.up + 2 + q) * kLanes] = uv[q];
+    for (int q = 0; q < C; ++q) u[(2 + q) * kLanes] = uv[q];
     st.up += 2 + C;
-    // update rows: row a+1's old values are read before row a is written
-    // (all targets are distinct slots)
+    if constexpr (kAll) {
 #pragma unroll
-    for (int a = 0; a < C; ++a) {
-        Real(&cur)[C > 0 ? C : 1] = (a & 1) ? t1 : t0;
-        Real(&nxt)[C > 0 ? C : 1] = (a & 1) ? t0 : t1;
-        if (a + 1 < C) {
+        for (int a = 0; a < C; ++a)
 #pragma unroll
-            for (int q = 0; q < C; ++q) {
-                const unsigned ref = ref_of<1, NW>(w, 2 + 2 * C + (a + 1) * C + q);
-                const Real old = slot(ref);
-                nxt[q] = (ref & kRefFirst) ? Real(0) : old;
+            for (int b = 0; b < C; ++b) *at(NS + a * C + b) = fma(lv[a], uv[b], tv[a * C + b]);
+    } else {
+        Real t0[C], t1[C];
+#pragma unroll
+        for (int q = 0; q < C; ++q) t0[q] = *at(NS + q);
+#pragma unroll
+        for (int a = 0; a < C; ++a) {
+            Real(&cur)[C] = (a & 1) ? t1 : t0;
+            Real(&nxt)[C] = (a & 1) ? t0 : t1;
+            if (a + 1 < C) {
+#pragma unroll
+                for (int q = 0; q < C; ++q) nxt[q] = *at(NS + (a + 1) * C + q);
             }
-        }
 #pragma unroll
-        for (int b = 0; b < C; ++b) slot(ref_of<1, NW>(w, 2 + 2 * C + a * C + b)) = fma(-lv[a], uv[b], cur[b]);
+            for (int b = 0; b < C; ++b) *at(NS + a * C + b) = fma(lv[a], uv[b], cur[b]);
+        }
     }
 #pragma unroll
-    for (int a = 0; a < C; ++a) slot(ref_of<1, NW>(w, 2 + 2 * C + C * C + a)) = fma(-lv[a], fk, tb[a]);
+    for (int a = 0; a < C; ++a) *at(NS + C * C + a) = fma(lv[a], fk, tb[a]);
     st.p += NW;
 }
 
-// Any column length (up to kMaxC), one field at a time: the rare long columns.
+// Any column length (up to kMaxC), one word at a time: the rare long columns.
 template <typename Real, typename AR>
 __device__ void factor_pivot_any(int c, AR& A, ProgReader& P, FactorState<Real>& st, Real* lsc, Real* usc)
 {
-    const int nw = frec_words(c);
+    const int ns = frec_src_words(c), rw = frec_row_words(c), nw = frec_words(c);
     P.window(st.p, st.p + nw - 1);
-    auto ref = [&](int q) -> unsigned {
-        const unsigned v = static_cast<unsigned>(P.at(st.p + 1 + q / 2));
-        return (q & 1) ? (v >> 16) : (v & 0xFFFFu);
+    auto ref = [&](int q) -> unsigned { return static_cast<unsigned>(P.at(st.p + q)); };
+    auto take = [&](int q) -> Real {
+        Real* s = slot_at<Real>(st.lb, ref(1 + q));
+        const Real v = *s;
+        *s = Real(0);
+        return v;
     };
-    auto slot = [&](unsigned r) -> Real& { return st.sl[static_cast<size_t>(r & kRefSlot) * kLanes]; };
     A.window(st.ap, st.ap + 1 + 2 * c);
     auto aval = [&](int i) { return A.at(st.ap + i); };
-    Real dv = aval(1) + slot(ref(0));
-    const Real fk = aval(0) + slot(ref(1));
-    for (int q = 0; q < c; ++q) lsc[q * kLanes] = aval(2 + q) + slot(ref(2 + q));
-    for (int q = 0; q < c; ++q) usc[q * kLanes] = aval(2 + c + q) + slot(ref(2 + c + q));
+    Real dv = aval(1) + take(0);
+    const Real fk = aval(0) + take(1);
+    for (int q = 0; q < c; ++q) lsc[q * kLanes] = aval(2 + q) + take(2 + q);
+    for (int q = 0; q < c; ++q) usc[q * kLanes] = aval(2 + c + q) + take(2 + c + q);
     st.ap += 2 + 2 * c;
     if (fabs(dv) < st.eps) {
         dv = dv < Real(0) ? -st.eps : st.eps;
         ++st.clamped;
     }
     const Real inv = Real(1) / dv;
-    for (int q = 0; q < c; ++q) lsc[q * kLanes] *= inv;
-    st.U[static_cast<size_t>(st.up) * kLanes] = fk;
-    st.U[static_cast<size_t>(st.up + 1) * kLanes] = dv;
-    for (int q = 0; q < c; ++q) st.U[static_cast<size_t>(st.up + 2 + q) * kLanes] = usc[q * kLanes];
+    for (int q = 0; q < c; ++q) lsc[q * kLanes] *= -inv;
+    Real* u = st.U + static_cast<size_t>(st.up) * kLanes;
+    u[0] = fk;
+    u[kLanes] = inv;
+    for (int q = 0; q < c; ++q) u[(2 + q) * kLanes] = usc[q * kLanes];
     st.up += 2 + c;
     for (int a = 0; a < c; ++a)
         for (int b = 0; b < c; ++b) {
-            const unsigned r = ref(2 + 2 * c + a * c + b);
-            const Real old = (r & kRefFirst) ? Real(0) : slot(r);
-            slot(r) = fma(-lsc[a * kLanes], usc[b * kLanes], old);
+            Real* t = slot_at<Real>(st.lb, ref(ns + a * rw + b));
+            *t = fma(lsc[a * kLanes], usc[b * kLanes], *t);
         }
     for (int a = 0; a < c; ++a) {
-        const unsigned r = ref(2 + 2 * c + c * c + a);
-        const Real old = (r & kRefFirst) ? Real(0) : slot(r);
-        slot(r) = fma(-lsc[a * kLanes], fk, old);
+        Real* t = slot_at<Real>(st.lb, ref(ns + c * rw + a));
+        *t = fma(lsc[a * kLanes], fk, *t);
     }
     st.p += nw;
 }
@@ -394,15 +425,15 @@ __global__ __launch_bounds__(kLanes * kStreamWaves) void k_factor(FactorArgs<Rea
     }
     // wave 0: the program walk
     const int row = static_cast<int>(tile) * kLanes + lane;
-    Real* lsc = sl + static_cast<size_t>(a.slots + 2) * kLanes;  // [kMaxC][64]
+    Real* lsc = sl + static_cast<size_t>(a.slots + 2) * kLanes;  // [kMaxC][64] (long columns only)
     Real* usc = lsc + static_cast<size_t>(kMaxC) * kLanes;       // [kMaxC][64]
-    sl[static_cast<size_t>(a.slots) * kLanes + lane] = Real(0);  // the zero slot
+    for (int s2 = 0; s2 < a.slots + 2; ++s2) sl[static_cast<size_t>(s2) * kLanes + lane] = Real(0);
     typename ARing::Reader A;
     A.init(aring, ctlA, a.a_stride / Sh::kCh, lane);
     ProgReader P;
     P.init(pring, ctlP, a.prog_stride / kProgChunk, lane);
     FactorState<Real> st;
-    st.sl = sl + lane;
+    st.lb = reinterpret_cast<unsigned char*>(sl + lane);
     st.U = a.U + tile * static_cast<size_t>(a.u_stride) * kLanes + lane;
     st.eps = static_cast<Real>(a.pivot_eps) * static_cast<Real>(a.jmax ? a.jmax[row] : 1.0);
     st.clamped = 0;
@@ -420,12 +451,12 @@ __global__ __launch_bounds__(kLanes * kStreamWaves) void k_factor(FactorArgs<Rea
 
 // ---------------------------------------------------------------------------
 // k_backward: U x = y in reverse postorder; x written in input labels
-// (same three-wavefront layout: walker, U-stream producer, program producer)
+// (same three-role layout: walker, U-stream producers, program producer)
 // ---------------------------------------------------------------------------
 
 template <typename Real>
 struct BackState {
-    Real* sl;
+    unsigned char* lb;
     Real* X;
     int p, up;
 };
@@ -436,21 +467,21 @@ __device__ __forceinline__ void backward_pivot(UR& U, ProgReader& P, BackState<R
     constexpr int NW = brec_words(C);
     P.window(st.p, st.p + NW - 1);
     v4u w[NW / 4];
-    read_record<NW>(P, st.p, w);
+    read_words<NW>(P, st.p, w);
     st.up -= C + 2;
     U.window(st.up + C + 1, st.up);  // the record is consumed from its top
-    auto uval = [&](int i) { return U.at(st.up + i); };
-    auto slot = [&](unsigned ref) -> Real& { return st.sl[static_cast<size_t>(ref & kRefSlot) * kLanes]; };
+    Real uv[C + 2];
+    load_values<C + 2>(U, st.up, uv);
     Real xj[C > 0 ? C : 1];
 #pragma unroll
-    for (int q = 0; q < C; ++q) xj[q] = slot(ref_of<2, NW>(w, q));
-    Real s = uval(0);
+    for (int q = 0; q < C; ++q) xj[q] = *slot_at<Real>(st.lb, word(w, 2 + q));
+    Real s = uv[0];
 #pragma unroll
-    for (int q = 0; q < C; ++q) s = fma(-uval(2 + q), xj[q], s);
-    const Real x = s / uval(1);
-    const int perm_k = static_cast<int>(w[0][1]);
+    for (int q = 0; q < C; ++q) s = fma(-uv[2 + q], xj[q], s);
+    const Real x = s * uv[1];  // U(k,k) is stored as its reciprocal
+    const int perm_k = __builtin_amdgcn_readfirstlane(static_cast<int>(word(w, 1)));
     st.X[static_cast<size_t>(perm_k) * kLanes] = x;
-    slot(ref_of<2, NW>(w, C)) = x;
+    *slot_at<Real>(st.lb, word(w, 2 + C)) = x;
     st.p += NW;
 }
 
@@ -459,20 +490,16 @@ __device__ void backward_pivot_any(int c, UR& U, ProgReader& P, BackState<Real>&
 {
     const int nw = brec_words(c);
     P.window(st.p, st.p + nw - 1);
-    auto ref = [&](int q) -> unsigned {
-        const unsigned v = static_cast<unsigned>(P.at(st.p + 2 + q / 2));
-        return (q & 1) ? (v >> 16) : (v & 0xFFFFu);
-    };
-    auto slot = [&](unsigned r) -> Real& { return st.sl[static_cast<size_t>(r & kRefSlot) * kLanes]; };
+    auto ref = [&](int q) -> unsigned { return static_cast<unsigned>(P.at(st.p + q)); };
     st.up -= c + 2;
     U.window(st.up + c + 1, st.up);
     auto uval = [&](int i) { return U.at(st.up + i); };
     Real s = uval(0);
-    for (int q = 0; q < c; ++q) s = fma(-uval(2 + q), slot(ref(q)), s);
-    const Real x = s / uval(1);
+    for (int q = 0; q < c; ++q) s = fma(-uval(2 + q), *slot_at<Real>(st.lb, ref(2 + q)), s);
+    const Real x = s * uval(1);
     const int perm_k = P.at(st.p + 1);
     st.X[static_cast<size_t>(perm_k) * kLanes] = x;
-    slot(ref(c)) = x;
+    *slot_at<Real>(st.lb, ref(2 + c)) = x;
     st.p += nw;
 }
 
@@ -525,7 +552,7 @@ __global__ __launch_bounds__(kLanes * kStreamWaves) void k_backward(BackwardArgs
     ProgReader P;
     P.init(pring, ctlP, a.prog_stride / kProgChunk, lane);
     BackState<Real> st;
-    st.sl = sl + lane;
+    st.lb = reinterpret_cast<unsigned char*>(sl + lane);
     st.X = a.X + tile * static_cast<size_t>(a.d) * kLanes + lane;
     st.p = 0;
     st.up = a.u_len;
@@ -682,47 +709,33 @@ __global__ void k_count_tiles(const int* __restrict__ status, int rows_pad, int*
     if (lane == 0) tile_active[t] = __popcll(m);
 }
 
-// Re-queue of the rows a warm-started pass left unconverged: their V goes back
-// to the flat start V0 (one thread per tiled complex value) ...
-__global__ void k_requeue_v(const int* __restrict__ status, int rows, int n, const double2* __restrict__ V0,
-                            double2* __restrict__ V)
+// Row moves between a batch's tiled arrays [T][m][64] and a dense set of
+// rows: gather dst row i <- src row map[i]; scatter dst row map[i] <- src row i.
+// One thread per (position, i), i fastest, so the dense side is coalesced.
+template <typename T>
+__global__ void k_move_rows(const T* __restrict__ src, T* __restrict__ dst, const int* __restrict__ map, int R,
+                            int m, int gather)
 {
     const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const long long T = (rows + kLanes - 1) / kLanes;
-    if (gid >= T * n * kLanes) return;
-    const int lane = static_cast<int>(gid & (kLanes - 1));
-    const long long tile = gid / (static_cast<long long>(n) * kLanes);
-    const int row = static_cast<int>(tile) * kLanes + lane;
-    if (row >= rows) return;
-    const int st = status[row];
-    if (st == kMaxIter || st == kDiverged || st == kRejected) V[gid] = V0[gid];
+    if (gid >= static_cast<long long>(R) * m) return;
+    const int i = static_cast<int>(gid % R);
+    const int pos = static_cast<int>(gid / R);
+    const int r = map[i];
+    const size_t a = tidx(static_cast<size_t>(r >> 6), static_cast<size_t>(m), static_cast<size_t>(pos), r & 63);
+    const size_t d = tidx(static_cast<size_t>(i >> 6), static_cast<size_t>(m), static_cast<size_t>(pos), i & 63);
+    if (gather) dst[d] = src[a];
+    else dst[a] = src[d];
 }
 
-// ... then their status becomes active again, with the warm pass's update
-// count moved to iters_warm
-__global__ void k_requeue_rows(int* __restrict__ status, int* __restrict__ iters, int* __restrict__ iters_warm,
-                               int rows, int* __restrict__ n_requeued)
-{
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    int re = 0;
-    if (row < rows) {
-        const int st = status[row];
-        re = st == kMaxIter || st == kDiverged || st == kRejected;
-        iters_warm[row] = iters[row];
-        if (re) {
-            status[row] = kActive;
-            iters[row] = 0;
-        }
-    }
-    const unsigned long long m = __ballot(re);
-    if ((threadIdx.x & (kLanes - 1)) == 0 && m) atomicAdd(n_requeued, __popcll(m));
-}
-
-// iters[r] += add[r]
-__global__ void k_add_int(int* __restrict__ p, const int* __restrict__ add, int n)
+// per-row values: gather dst[i] = src[map[i]]; scatter dst[map[i]] = src[i] (+ dst[map[i]] when add)
+template <typename T>
+__global__ void k_move_vals(const T* __restrict__ src, T* __restrict__ dst, const int* __restrict__ map, int R,
+                            int gather, int add)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] += add[i];
+    if (i >= R) return;
+    if (gather) dst[i] = src[map[i]];
+    else dst[map[i]] = add ? dst[map[i]] + src[i] : src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -737,6 +750,9 @@ template __global__ void k_mismatch<float, false, true>(MismatchArgs<float>);
 template __global__ void k_mismatch<float, true, true>(MismatchArgs<float>);
 template __global__ void k_mismatch<float, false, false>(MismatchArgs<float>);
 template __global__ void k_mismatch<float, true, false>(MismatchArgs<float>);
+template __global__ void k_move_rows<double2>(const double2*, double2*, const int*, int, int, int);
+template __global__ void k_move_vals<int>(const int*, int*, const int*, int, int, int);
+template __global__ void k_move_vals<double>(const double*, double*, const int*, int, int, int);
 template __global__ void k_factor<double>(FactorArgs<double>);
 template __global__ void k_factor<float>(FactorArgs<float>);
 template __global__ void k_backward<double>(BackwardArgs<double>);

@@ -291,12 +291,16 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
     std::vector<std::vector<int>> refs(static_cast<size_t>(d));
     s.u_off.assign(static_cast<size_t>(d) + 1, 0);
     int apos = 0, upos = 0;
-    auto take = [&](int i, int j) -> int {  // slot of a consumed entry (freed), or -1
+    // slots a pivot consumes are freed after its targets are placed, so a
+    // pivot's sources and targets never share a slot (the kernels read both
+    // in one batch)
+    std::vector<int> consumed;
+    auto take = [&](int i, int j) -> int {  // slot of a consumed entry, or -1
         auto it = slot_of.find(key(i, j, d));
         if (it == slot_of.end()) return -1;
         const int sl = it->second;
         slot_of.erase(it);
-        sa.put(sl);
+        consumed.push_back(sl);
         return sl;
     };
     auto place = [&](int i, int j) {  // dense A stream: every entry has a position
@@ -318,11 +322,10 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
         place(k, k);
         for (int p = c0; p < c0 + c; ++p) place(s.coli[p], k);
         for (int p = c0; p < c0 + c; ++p) place(k, s.coli[p]);
-        // sources: diagonal, b(k), L entries, U entries (their slots are freed:
-        // every read of a pivot happens before its writes)
+        // sources: diagonal, b(k), L entries, U entries
         r.push_back(take(k, k));
         int bref = -1;
-        if (bslot[k] >= 0) { bref = bslot[k]; sa.put(bref); bslot[k] = -1; }
+        if (bslot[k] >= 0) { bref = bslot[k]; consumed.push_back(bref); bslot[k] = -1; }
         r.push_back(bref);
         for (int p = c0; p < c0 + c; ++p) r.push_back(take(s.coli[p], k));
         for (int p = c0; p < c0 + c; ++p) r.push_back(take(k, s.coli[p]));
@@ -351,15 +354,16 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
                 r.push_back(bslot[i]);
             }
         }
+        for (int sl : consumed) sa.put(sl);
+        consumed.clear();
     }
     s.u_off[d] = upos;
     s.a_len = apos;
     s.u_len = upos;
     s.slots_f = sa.hi;
     if (!slot_of.empty()) throw std::logic_error("symbolic: update targets left unconsumed");
-    if (s.slots_f + 2 > kRefSlot) throw std::runtime_error("symbolic: too many live update slots");
-    // pack: header word (c | record words << 16), then 16-bit refs, records
-    // padded to 4 words; a missing source reads the zero slot
+    // encode: refs as byte offsets; a missing source reads the zero slot
+    const auto enc = [](int slot) { return slot * kRefUnit; };
     s.prog_f.clear();
     s.prog_f_off.assign(static_cast<size_t>(d) + 1, 0);
     for (int k = 0; k < d; ++k) {
@@ -367,16 +371,21 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
         const int c = s.colp[k + 1] - s.colp[k];
         const int c_next = k + 1 < d ? s.colp[k + 2] - s.colp[k + 1] : 0;
         const std::vector<int>& r = refs[k];
-        const int words = frec_words(c);
-        s.prog_f.push_back(c | (c_next << 16));
-        std::vector<int> packed(static_cast<size_t>(words - 1), 0);
-        for (size_t q = 0; q < r.size(); ++q) {
-            int v = r[q];
-            unsigned f = v < 0 ? static_cast<unsigned>(s.slots_f) : static_cast<unsigned>(v & kSlotMask) |
-                                                                        ((v & kFirst) ? kRefFirst : 0u);
-            packed[q / 2] |= static_cast<int>(f << (16 * (q & 1)));
-        }
-        s.prog_f.insert(s.prog_f.end(), packed.begin(), packed.end());
+        const size_t at = s.prog_f.size();
+        s.prog_f.resize(at + static_cast<size_t>(frec_words(c)), enc(s.slots_f));
+        int* w = s.prog_f.data() + at;
+        w[0] = c | (c_next << 16);
+        for (int q = 0; q < 2 + 2 * c; ++q) w[1 + q] = enc(r[q] < 0 ? s.slots_f : (r[q] & kSlotMask));
+        const int rw = frec_row_words(c);
+        int* u = w + frec_src_words(c);
+        for (int a = 0; a < c; ++a)
+            for (int b = 0; b < c; ++b) u[a * rw + b] = enc(r[2 + 2 * c + a * c + b] & kSlotMask);
+        int* fb = u + c * rw;
+        for (int a = 0; a < c; ++a) fb[a] = enc(r[2 + 2 * c + c * c + a] & kSlotMask);
+        // padding of the target rows points at the trash slot
+        for (int a = 0; a < c; ++a)
+            for (int b = c; b < rw; ++b) u[a * rw + b] = enc(s.slots_f + 1);
+        for (int a = c; a < rw; ++a) fb[a] = enc(s.slots_f + 1);
     }
     s.prog_f_off[d] = static_cast<int>(s.prog_f.size());
     build_backward(s);
@@ -421,22 +430,19 @@ void build_backward(SymbolicLU& s)
         recs.push_back(std::move(r));
     }
     s.slots_b = sa.hi;
-    if (s.slots_b + 2 > kRefSlot) throw std::runtime_error("symbolic: too many live solution slots");
+    const auto enc = [](int slot) { return slot * kRefUnit; };
     s.prog_b.clear();
     for (int q = 0; q < d; ++q) {
         const int k = d - 1 - q;
         const int c = s.colp[k + 1] - s.colp[k];
         const int c_next = k > 0 ? s.colp[k] - s.colp[k - 1] : 0;
         const std::vector<int>& r = recs[q];
-        const int words = brec_words(c);
-        s.prog_b.push_back(c | (c_next << 16));
-        s.prog_b.push_back(s.perm[k]);
-        std::vector<int> packed(static_cast<size_t>(words - 2), 0);
-        for (size_t i = 0; i < r.size(); ++i) {
-            const unsigned f = r[i] < 0 ? static_cast<unsigned>(s.slots_b + 1) : static_cast<unsigned>(r[i]);
-            packed[i / 2] |= static_cast<int>(f << (16 * (i & 1)));
-        }
-        s.prog_b.insert(s.prog_b.end(), packed.begin(), packed.end());
+        const size_t at = s.prog_b.size();
+        s.prog_b.resize(at + static_cast<size_t>(brec_words(c)), enc(s.slots_b + 1));
+        int* w = s.prog_b.data() + at;
+        w[0] = c | (c_next << 16);
+        w[1] = s.perm[k];
+        for (size_t i = 0; i < r.size(); ++i) w[2 + i] = enc(r[i] < 0 ? s.slots_b + 1 : r[i]);
     }
 }
 

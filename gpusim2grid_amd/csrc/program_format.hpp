@@ -12,34 +12,40 @@ constexpr int kLanes = 64;          // rows per wavefront (one lane per batch ro
 constexpr int kMisWaves = 16;       // wavefronts per tile in the mismatch and update kernels
 constexpr int kMaxC = 16;           // longest off-diagonal L column the kernels unroll for
 
-constexpr int kFastC = 10;          // column lengths with a specialised kernel path
+constexpr int kFastC = 7;           // column lengths with a specialised kernel path
 
 // stream prefetch geometry (streams.hpp): positions per chunk and ring slots
-constexpr int kChunkF64 = 8;        // 4 KB chunks of a wavefront's FP64 stream
-constexpr int kRingSlotsF64 = 8;    // 32 KB ring
-constexpr int kChunkF32 = 16;       // 4 KB chunks of a wavefront's FP32 stream
-constexpr int kRingSlotsF32 = 16;   // 64 KB ring
+constexpr int kChunkF64 = 16;       // 8 KB chunks of a wavefront's FP64 stream
+constexpr int kRingSlotsF64 = 8;    // 64 KB ring
+constexpr int kChunkF32 = 32;       // 8 KB chunks of a wavefront's FP32 stream
+constexpr int kRingSlotsF32 = 8;    // 64 KB ring
 constexpr int kValueProducers = 6;  // wavefronts streaming a value stream (one chunk in flight each)
 constexpr int kProgChunk = 512;     // program words per chunk (2 KB)
 constexpr int kProgSlots = 4;       // 8 KB ring
 constexpr int kStreamWaves = 2 + kValueProducers;  // walker + value producers + program producer
 
-// Programs (one record per pivot, 4-word aligned, length fixed by c):
-//   factorization  [c | c_next << 16] then 16-bit refs
-//                  diag, b(k), L[c], U[c], update[c*c] (row-major), b-update[c]
-//   backward       [c | c_next << 16] [perm[k]] then 16-bit refs x(j)[c], out
+// Programs (one record per pivot, padded to 4 words so the kernels read a
+// record with 16-byte loads):
+//   factorization  [c | c_next << 16, diag, b(k), L[c], U[c]]   sources
+//                  c rows of c refs                             rank-1 update targets (row-major)
+//                  c refs                                       forward-substitution targets b(i)
+//   backward       [c | c_next << 16, perm[k], x(j)[c], out]
 // c_next is the column length of the following record (0 after the last).
-// A ref is a slot index; kRefFirst marks the first update of a target (store,
-// not accumulate).  Slot S (the slot count) is the zero slot a missing source
-// reads, S + 1 the trash slot an unneeded result goes to.
+// A ref is a 32-bit byte offset, slot * kRefUnit: the offset of the slot's
+// 64-lane row of FP32 values (an FP64 slot sits at twice the offset).  Slot S
+// (the slot count) is the zero slot a missing source reads, S + 1 the trash
+// slot an unneeded result goes to.  The factorization reads a source slot and
+// clears it; an update target accumulates (slots start at zero), so one slot
+// serves a sequence of update targets without a first-update flag.
 constexpr int kSlotMask = 0x7FFF;     // host-side slot index field
-constexpr int kFirst = 1 << 16;       // host-side "first update" flag
-constexpr unsigned kRefSlot = 0x7FFF; // 16-bit ref: slot index bits
-constexpr unsigned kRefFirst = 0x8000;
+constexpr int kFirst = 1 << 16;       // host-side "first update" flag (bookkeeping only)
+constexpr int kRefUnit = 256;         // bytes per FP32 slot row
 
-__host__ __device__ constexpr int record_words(int head, int nrefs) { return (head + (nrefs + 1) / 2 + 3) / 4 * 4; }
-__host__ __device__ constexpr int frec_words(int c) { return record_words(1, (c + 1) * (c + 2)); }
-__host__ __device__ constexpr int brec_words(int c) { return record_words(2, c + 1); }
+__host__ __device__ constexpr int round4(int v) { return (v + 3) / 4 * 4; }
+__host__ __device__ constexpr int frec_src_words(int c) { return 3 + 2 * c; }  // header + sources
+__host__ __device__ constexpr int frec_row_words(int c) { return c; }
+__host__ __device__ constexpr int frec_words(int c) { return round4(frec_src_words(c) + (c + 1) * c); }
+__host__ __device__ constexpr int brec_words(int c) { return round4(3 + c); }
 
 // row status (gpusim2grid contingency/row_status.hpp:33-43) plus "still iterating"
 constexpr int kActive = -1;

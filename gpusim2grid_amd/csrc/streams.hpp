@@ -118,6 +118,7 @@ struct Ring {
 
     // ---- consumer wavefront ----------------------------------------------
     struct Reader {
+        static constexpr int kRingLen = kRing;  // positions the ring holds
         const T* ring;   // ring base (+ this lane for per-lane streams)
         Ctl* ctl;
         int nch;

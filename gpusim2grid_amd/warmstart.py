@@ -78,7 +78,15 @@ class WarmStart:
         if self.n_mask:  # the mask enters raw (0/1)
             self.in_mu[2 * self.n:] = 0.0
             self.in_sigma[2 * self.n:] = 1.0
-        self.out_mu, self.out_sigma = ms(T)
+        # outputs: one scale for all angles and one for all magnitudes, so the
+        # loss weighs every bus by its absolute voltage error (what Newton's
+        # convergence depends on), not by its own spread
+        self.out_mu, _ = ms(T)
+        sd = T.std(0)
+        self.out_sigma = np.empty(self.n_out, np.float32)
+        self.out_sigma[0::2] = float(sd[0::2].pow(2).mean().sqrt())
+        self.out_sigma[1::2] = float(sd[1::2].pow(2).mean().sqrt())
+        self.out_sigma[self.out_sigma <= 1e-12] = 1.0
         self.mlp.set_normalization(self.in_mu, self.in_sigma, self.out_mu, self.out_sigma)
 
     def _normalised(self, S, V, mask=None):
@@ -91,9 +99,11 @@ class WarmStart:
         return X, T
 
     # -- training ----------------------------------------------------------
-    def train(self, S, V, mask=None, epochs=20, batch=1024, lr=1e-3, seed=0, log=None):
-        """Adam on (features(S), targets(V)); S, V: device complex128 tensors of
-        converged training rows.  Returns the per-epoch mean losses."""
+    def train(self, S, V, mask=None, epochs=20, batch=1024, lr=1e-3, lr_final=1e-5, seed=0, log=None):
+        """Adam on (features(S), targets(V)) with a cosine step-size decay from
+        lr to lr_final; S, V: device complex128 tensors of converged training
+        rows.  Returns the per-epoch mean losses."""
+        import math
         import torch
         if self.in_mu is None:
             self.fit_normalization(S, V, mask)
@@ -101,6 +111,8 @@ class WarmStart:
         rows = X.shape[0]
         g = torch.Generator(device="cpu").manual_seed(seed)
         hist = []
+        steps = max(1, epochs * (rows // batch))
+        k = 0
         for ep in range(epochs):
             perm = torch.randperm(rows, generator=g).to(X.device)
             Xs, Ts = X[perm].contiguous(), T[perm].contiguous()
@@ -109,8 +121,10 @@ class WarmStart:
             for r0 in range(0, rows - batch + 1, batch):
                 xp = Xs.data_ptr() + r0 * self.n_in * 4
                 tp = Ts.data_ptr() + r0 * self.n_out * 4
-                tot += self.mlp.train_step_device(xp, tp, batch, lr)
+                lr_k = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * k / steps))
+                tot += self.mlp.train_step_device(xp, tp, batch, lr_k)
                 cnt += 1
+                k += 1
             hist.append(tot / max(cnt, 1))
             if log:
                 log(f"warm-start epoch {ep}: loss {hist[-1]:.3e}")

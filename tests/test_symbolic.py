@@ -94,7 +94,7 @@ def test_programs_factor_like_superlu_natural(grid):
     colp, coli = P.array("colp"), P.array("coli")
     for k in range(inf["d"]):
         rec = U[0, uoff[k]:uoff[k + 1]]
-        assert rec[1] == pytest.approx(Uref[k, k], rel=1e-12)
+        assert rec[1] == pytest.approx(1.0 / Uref[k, k], rel=1e-12)  # stored as the reciprocal
         np.testing.assert_allclose(rec[2:], Uref[k, coli[colp[k]:colp[k + 1]]], rtol=1e-11, atol=1e-12)
 
 
