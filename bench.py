@@ -31,6 +31,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "converged PF/s at ‖ΔS‖∞<1e-8 pu: warm-start+mixed vs plain FP64 NR"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes one k_factor launch moves per factored row, measured with rocprofv3 PMC counters
+# (FETCH_SIZE x 2 + WRITE_SIZE, scaled on a 1 GiB copy) on the 16,384-row sweep
+# (profiles/hbm_traffic_c2_16384.md): 22.48 GB (FP64) and 11.24 GB (FP32) per full launch.
+FACTOR_TRAFFIC_PER_ROW = {"fp64": 22.48e9 / 16384, "f32": 11.24e9 / 16384}
 
 
 def parse():
@@ -47,8 +51,8 @@ def parse():
     ap.add_argument("--hidden", type=int, default=1024)
     ap.add_argument("--train-batch", type=int, default=512)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--k-warm", type=int, default=4)
-    ap.add_argument("--growth", type=float, default=2.0)
+    ap.add_argument("--k-warm", type=int, default=6)
+    ap.add_argument("--growth", type=float, default=10.0)
     ap.add_argument("--cpu-rows", type=int, default=512, help="oracle sample size (0: skip)")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
@@ -136,6 +140,8 @@ def run_method(args, name, g, S, torch, capi, ws=None):
         factor_launch_ms=ms_factor / max(launches, 1),
         factor_alg_bytes_per_launch=alg_bytes / max(launches, 1),
         factor_achieved_gbs=alg_bytes / (ms_factor * 1e-3) / 1e9 if ms_factor > 0 else None,
+        factor_rows_per_launch=sum(s["rows_factored"] for s in stats) / max(launches, 1),
+        real="fp64" if prec == capi.FP64 else "f32",
         info=info,
     )
     V = B.get_v()
@@ -219,7 +225,8 @@ def main():
     dom = head
     roof = dict(bound="hbm", kernel="k_factor",
                 achieved=dom["factor_achieved_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=(dom["factor_achieved_gbs"] or 0.0) / HBM_PEAK_GBS, traffic=None,
+                frac=(dom["factor_achieved_gbs"] or 0.0) / HBM_PEAK_GBS,
+                traffic=FACTOR_TRAFFIC_PER_ROW[dom["real"]] * dom["factor_rows_per_launch"],
                 alg_bytes_per_launch=dom["factor_alg_bytes_per_launch"], launch_ms=dom["factor_launch_ms"])
     line = dict(
         metric=METRIC, value=head["pfps"], unit="PF/s", n_gpus=1, steps=args.steps, warmup=args.warmup,

@@ -80,6 +80,7 @@ def lib():
         "g2g_lu_solve_batch": (ctypes.c_int, [_P, i32, i32, _P, _P, _P, dbl, _P, _P]),
         "g2g_batch_device_arrays": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
         "g2g_batch_warm_start": (ctypes.c_int, [_P, _P, _P, i32]),
+        "g2g_batch_branch_flows": (ctypes.c_int, [_P, i32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
         "g2g_batch_solve_warm": (ctypes.c_int, [_P, _P, _P, i32, i32, dbl, i32, dbl, ctypes.POINTER(SolveStats)]),
         "g2g_mlp_create": (ctypes.c_int, [i32, i32, i32, ctypes.c_uint64, _P, ctypes.POINTER(_P)]),
         "g2g_mlp_destroy": (None, [_P]),
@@ -251,6 +252,24 @@ class Batch:
         _check(lib().g2g_batch_solve_warm(self._h, mlp._h, _ptr(m), n_mask, int(k_warm), float(growth),
                                           int(max_iter), float(tol), ctypes.byref(st)))
         return st.as_dict()
+
+    def branch_flows(self, from_bus, to_bus, yff, yft, ytf, ytt, base_or, base_ex, trips=None):
+        """(or_amps, ex_amps), active rows x branches, at the batch's V."""
+        f, t = _i32(from_bus), _i32(to_bus)
+        m = f.size
+        y4 = np.ascontiguousarray(np.stack([yff, yft, ytf, ytt], axis=1), dtype=np.complex128)
+        bo = np.ascontiguousarray(base_or, dtype=np.float64)
+        be = np.ascontiguousarray(base_ex, dtype=np.float64)
+        tp = tb = None
+        if trips is not None:
+            tp = _i32(np.concatenate([[0], np.cumsum([len(x) for x in trips])]))
+            tb = _i32(np.concatenate([np.asarray(x, np.int64) for x in trips]) if len(trips) else [])
+        rows = self.rows
+        o = np.empty((rows, m))
+        e = np.empty((rows, m))
+        _check(lib().g2g_batch_branch_flows(self._h, m, _ptr(f), _ptr(t), _ptr(y4), _ptr(bo), _ptr(be), _ptr(tp),
+                                            _ptr(tb), _ptr(o), _ptr(e)))
+        return o, e
 
     def warm_start(self, mlp, mask=None):
         """Writes the MLP's start voltages as the batch's V."""

@@ -441,164 +441,21 @@ private:
     AcPfTimings timings_;
 };
 
-// ---- batch sessions (Injection sweep / contingency analysis / scenario sweep)
-class BatchSession {
-public:
-    BatchSession(py::object Ybus, CArr Vinit, CArr Sbus, IArr slack_ids, IArr pv, IArr pq, int batch_size,
-                 int nb_iter, int max_iter_base, double tol_base, int device, bool presolved_v, int ir_n_steps,
-                 bool scaling_max_voltage_change, Precision precision)
-        : batch_size_(batch_size), nb_iter_(nb_iter), precision_(static_cast<int>(precision))
-    {
-        unsupported_options(ir_n_steps, scaling_max_voltage_change);
-        if (batch_size <= 0) throw py::value_error("batch_size must be positive");
-        if (nb_iter < 0) throw py::value_error("nb_iter must be >= 0");
-        set_device(device);
-        const double t0 = now_ms();
-        grid_ = std::make_unique<Grid>(Ybus, slack_ids, pv, pq);
-        timings_.t_analysis_ms = grid_->t_analyze_ms;
-        const int n = grid_->n();
-        if (Vinit.size() != n || Sbus.size() != n) throw py::value_error("Vinit / Sbus must have n_bus entries");
-        S_base_.assign(Sbus.data(), Sbus.data() + n);
-        V_base_.assign(Vinit.data(), Vinit.data() + n);
-        // the base case, solved once (or checked, when presolved)
-        {
-            py::gil_scoped_release nogil;
-            Batch b(*grid_, 1, G2G_FP64, 0);
-            check(g2g_batch_set_sbus(b.h, reinterpret_cast<const double*>(S_base_.data()), n, 1, 0, 0));
-            check(g2g_batch_set_v(b.h, reinterpret_cast<const double*>(V_base_.data()), n, 1, 0, 0));
-            int32_t st = 0, it = 0;
-            double r = 0.0;
-            if (presolved_v) {
-                check(g2g_batch_eval(b.h, nullptr, nullptr, &r));
-                if (!(r < tol_base))
-                    throw std::runtime_error("presolved_v: ||F(Vinit)||inf = " + std::to_string(r) +
-                                             " is not below tol_base");
-            } else {
-                check(g2g_batch_solve(b.h, max_iter_base, tol_base, G2G_SCHED_CONTINUOUS, nullptr));
-                check(g2g_batch_get_results(b.h, &st, &it, &r));
-                if (st != 0)
-                    throw std::runtime_error("the base case did not converge (residual " + std::to_string(r) + ")");
-                check(g2g_batch_get_v(b.h, reinterpret_cast<double*>(V_base_.data()), n, 0));
-            }
-        }
-        timings_.t_base_case_ms = now_ms() - t0;
-    }
-    virtual ~BatchSession() = default;
-
-    // settings (mutable, next run; CLAUDE.md:130-138)
-    int batch_size = 0;
-    int get_batch_size() const { return batch_size_; }
-    void set_batch_size(int v)
-    {
-        if (v <= 0) throw py::value_error("batch_size must be positive");
-        batch_size_ = v;
-    }
-    int get_nb_iter() const { return nb_iter_; }
-    void set_nb_iter(int v)
-    {
-        if (v < 0) throw py::value_error("nb_iter must be >= 0");
-        nb_iter_ = v;
-    }
-    BatchScheduling get_scheduling() const { return sched_; }
-    void set_scheduling(BatchScheduling s) { sched_ = s; }
-    double get_tol() const { return tol_; }
-    void set_tol(double t)
-    {
-        if (!(t > 0)) throw py::value_error("tol must be positive");
-        tol_ = t;
-    }
-    Precision get_precision() const { return static_cast<Precision>(precision_); }
-    void set_precision(Precision p) { precision_ = static_cast<int>(p); }
-    void set_warm_start(uintptr_t mlp, int k_warm, double growth)
-    {
-        if (k_warm < 0 || growth < 0) throw py::value_error("k_warm and growth must be >= 0");
-        mlp_ = mlp;
-        k_warm_ = k_warm;
-        growth_ = growth;
-    }
-    void clear_warm_start() { mlp_ = 0; }
-
-    void run()
-    {
-        const int rows = n_rows();
-        if (rows <= 0) throw std::runtime_error("nothing to run: set the rows first");
-        RowJob job;
-        prepare(job);
-        job.max_iter = nb_iter_;
-        job.tol = tol_;
-        job.sched = sched_ == BatchScheduling::Chunked ? G2G_SCHED_CHUNKED : G2G_SCHED_CONTINUOUS;
-        job.mlp = mlp_;
-        job.k_warm = k_warm_;
-        job.growth = growth_;
-        BatchTimings t = timings_;
-        t.t_mismatch = t.t_refactorize = t.t_solve = t.t_warm_start = TimingEntry{};
-        t.n_chunks = 0;
-        t.rows_factored = 0;
-        t.n_requeued = 0;
-        t.t_copy_V_to_host_ms = 0.0;
-        const double t0 = now_ms();
-        run_rows(*grid_, batch_, precision_, batch_size_, rows, job, res_, t);
-        t.t_run_ms = now_ms() - t0;
-        timings_ = t;
-        ran_ = true;
-    }
-
-    CArr get_V_results() const
-    {
-        need_run();
-        CArr a(static_cast<py::ssize_t>(res_.V.size()));
-        std::copy(res_.V.begin(), res_.V.end(), a.mutable_data());
-        return a;
-    }
-    py::array_t<double> get_residuals() const { need_run(); return to_array(res_.res); }
-    py::array_t<int32_t> get_row_iterations() const { need_run(); return to_array(res_.iters); }
-    py::array_t<int32_t> get_row_status() const { need_run(); return to_array(res_.status); }
-    const BatchTimings& get_timings() const { return timings_; }
-    int n_bus() const { return grid_->n(); }
-    int dim_J() const { return grid_->info.d; }
-    CArr v_base() const
-    {
-        CArr a(grid_->n());
-        std::copy(V_base_.begin(), V_base_.end(), a.mutable_data());
-        return a;
-    }
-    virtual int n_rows() const = 0;
-
-protected:
-    virtual void prepare(RowJob& job) = 0;
-    void need_run() const
-    {
-        if (!ran_) throw std::runtime_error("call run() first");
-    }
-
-    std::unique_ptr<Grid> grid_;
-    std::unique_ptr<Batch> batch_;
-    std::vector<cplx> S_base_, V_base_;
-    int batch_size_;
-    int nb_iter_;
-    int precision_;
-    BatchScheduling sched_ = BatchScheduling::Chunked;
-    double tol_ = 1e-8;
-    uintptr_t mlp_ = 0;
-    int k_warm_ = 4;
-    double growth_ = 2.0;
-    RowResults res_;
-    BatchTimings timings_;
-    bool ran_ = false;
-};
-
 // branch data and outage edits (contingency_analysis_helper.cpp:23-83,131-163)
 struct Branches {
     std::vector<int32_t> f, t;
     std::vector<cplx> yff, yft, ytf, ytt;
+    std::vector<double> base_or, base_ex;  // A per unit current (sn_mva 1e6 / (sqrt(3) vn_kv 1e3))
     std::vector<char> bridge;
     bool set = false;
 
     int m() const { return static_cast<int>(f.size()); }
 
     void assign(const IArr& bf, const IArr& bt, const CArr& ff, const CArr& ft, const CArr& tf, const CArr& tt,
-                int n)
+                const DArr& vn_kv, double sn_mva, int n)
     {
+        if (vn_kv.size() != n) throw py::value_error("bus_vn_kv must have n_bus entries");
+        if (!(sn_mva > 0)) throw py::value_error("sn_mva must be positive");
         const auto m = bf.size();
         if (bt.size() != m || ff.size() != m || ft.size() != m || tf.size() != m || tt.size() != m)
             throw py::value_error("branch arrays differ in length");
@@ -610,6 +467,13 @@ struct Branches {
         yft.assign(ft.data(), ft.data() + m);
         ytf.assign(tf.data(), tf.data() + m);
         ytt.assign(tt.data(), tt.data() + m);
+        base_or.resize(m);
+        base_ex.resize(m);
+        const double* vn = vn_kv.data();
+        for (py::ssize_t l = 0; l < m; ++l) {
+            base_or[l] = sn_mva * 1e6 / (std::sqrt(3.0) * vn[f[l]] * 1e3);
+            base_ex[l] = sn_mva * 1e6 / (std::sqrt(3.0) * vn[t[l]] * 1e3);
+        }
         bridge = find_bridges(n);
         set = true;
     }
@@ -711,6 +575,228 @@ struct Branches {
     }
 };
 
+// ---- batch sessions (Injection sweep / contingency analysis / scenario sweep)
+class BatchSession {
+public:
+    BatchSession(py::object Ybus, CArr Vinit, CArr Sbus, IArr slack_ids, IArr pv, IArr pq, int batch_size,
+                 int nb_iter, int max_iter_base, double tol_base, int device, bool presolved_v, int ir_n_steps,
+                 bool scaling_max_voltage_change, Precision precision)
+        : batch_size_(batch_size), nb_iter_(nb_iter), precision_(static_cast<int>(precision))
+    {
+        unsupported_options(ir_n_steps, scaling_max_voltage_change);
+        if (batch_size <= 0) throw py::value_error("batch_size must be positive");
+        if (nb_iter < 0) throw py::value_error("nb_iter must be >= 0");
+        set_device(device);
+        const double t0 = now_ms();
+        grid_ = std::make_unique<Grid>(Ybus, slack_ids, pv, pq);
+        timings_.t_analysis_ms = grid_->t_analyze_ms;
+        const int n = grid_->n();
+        if (Vinit.size() != n || Sbus.size() != n) throw py::value_error("Vinit / Sbus must have n_bus entries");
+        S_base_.assign(Sbus.data(), Sbus.data() + n);
+        V_base_.assign(Vinit.data(), Vinit.data() + n);
+        // the base case, solved once (or checked, when presolved)
+        {
+            py::gil_scoped_release nogil;
+            Batch b(*grid_, 1, G2G_FP64, 0);
+            check(g2g_batch_set_sbus(b.h, reinterpret_cast<const double*>(S_base_.data()), n, 1, 0, 0));
+            check(g2g_batch_set_v(b.h, reinterpret_cast<const double*>(V_base_.data()), n, 1, 0, 0));
+            int32_t st = 0, it = 0;
+            double r = 0.0;
+            if (presolved_v) {
+                check(g2g_batch_eval(b.h, nullptr, nullptr, &r));
+                if (!(r < tol_base))
+                    throw std::runtime_error("presolved_v: ||F(Vinit)||inf = " + std::to_string(r) +
+                                             " is not below tol_base");
+            } else {
+                check(g2g_batch_solve(b.h, max_iter_base, tol_base, G2G_SCHED_CONTINUOUS, nullptr));
+                check(g2g_batch_get_results(b.h, &st, &it, &r));
+                if (st != 0)
+                    throw std::runtime_error("the base case did not converge (residual " + std::to_string(r) + ")");
+                check(g2g_batch_get_v(b.h, reinterpret_cast<double*>(V_base_.data()), n, 0));
+            }
+        }
+        timings_.t_base_case_ms = now_ms() - t0;
+    }
+    virtual ~BatchSession() = default;
+
+    // settings (mutable, next run; CLAUDE.md:130-138)
+    int batch_size = 0;
+    int get_batch_size() const { return batch_size_; }
+    void set_batch_size(int v)
+    {
+        if (v <= 0) throw py::value_error("batch_size must be positive");
+        batch_size_ = v;
+    }
+    int get_nb_iter() const { return nb_iter_; }
+    void set_nb_iter(int v)
+    {
+        if (v < 0) throw py::value_error("nb_iter must be >= 0");
+        nb_iter_ = v;
+    }
+    BatchScheduling get_scheduling() const { return sched_; }
+    void set_scheduling(BatchScheduling s) { sched_ = s; }
+    double get_tol() const { return tol_; }
+    void set_tol(double t)
+    {
+        if (!(t > 0)) throw py::value_error("tol must be positive");
+        tol_ = t;
+    }
+    Precision get_precision() const { return static_cast<Precision>(precision_); }
+    void set_precision(Precision p) { precision_ = static_cast<int>(p); }
+    void set_warm_start(uintptr_t mlp, int k_warm, double growth)
+    {
+        if (k_warm < 0 || growth < 0) throw py::value_error("k_warm and growth must be >= 0");
+        mlp_ = mlp;
+        k_warm_ = k_warm;
+        growth_ = growth;
+    }
+    void clear_warm_start() { mlp_ = 0; }
+
+    void run()
+    {
+        const int rows = n_rows();
+        if (rows <= 0) throw std::runtime_error("nothing to run: set the rows first");
+        RowJob job;
+        prepare(job);
+        job.max_iter = nb_iter_;
+        job.tol = tol_;
+        job.sched = sched_ == BatchScheduling::Chunked ? G2G_SCHED_CHUNKED : G2G_SCHED_CONTINUOUS;
+        job.mlp = mlp_;
+        job.k_warm = k_warm_;
+        job.growth = growth_;
+        BatchTimings t = timings_;
+        t.t_mismatch = t.t_refactorize = t.t_solve = t.t_warm_start = TimingEntry{};
+        t.n_chunks = 0;
+        t.rows_factored = 0;
+        t.n_requeued = 0;
+        t.t_copy_V_to_host_ms = 0.0;
+        const double t0 = now_ms();
+        run_rows(*grid_, batch_, precision_, batch_size_, rows, job, res_, t);
+        t.t_run_ms = now_ms() - t0;
+        timings_ = t;
+        ran_ = true;
+        flows_ = false;
+    }
+
+    CArr get_V_results() const
+    {
+        need_run();
+        CArr a(static_cast<py::ssize_t>(res_.V.size()));
+        std::copy(res_.V.begin(), res_.V.end(), a.mutable_data());
+        return a;
+    }
+    py::array_t<double> get_residuals() const { need_run(); return to_array(res_.res); }
+    py::array_t<int32_t> get_row_iterations() const { need_run(); return to_array(res_.iters); }
+    py::array_t<int32_t> get_row_status() const { need_run(); return to_array(res_.status); }
+    const BatchTimings& get_timings() const { return timings_; }
+    int n_bus() const { return grid_->n(); }
+    int dim_J() const { return grid_->info.d; }
+    // branch data (python_bindings.cpp:1819-1832)
+    void set_branch_data(IArr bf, IArr bt, CArr yff, CArr yft, CArr ytf, CArr ytt, DArr bus_vn_kv, double sn_mva)
+    {
+        br_.assign(bf, bt, yff, yft, ytf, ytt, bus_vn_kv, sn_mva, n_bus());
+    }
+    int n_branches() const { return br_.m(); }
+
+    // branch currents of every simulated row at its result V, on the device
+    // (python_bindings.cpp:1869-1872, 1915-1922)
+    void compute_flows()
+    {
+        need_run();
+        if (!br_.set) throw std::runtime_error("call set_branch_data() first");
+        const int n = n_bus(), m = br_.m(), rows = n_rows();
+        or_.assign(static_cast<size_t>(rows) * m, std::nan(""));
+        ex_.assign(static_cast<size_t>(rows) * m, std::nan(""));
+        std::vector<cplx> y4(4 * static_cast<size_t>(m));
+        for (int l = 0; l < m; ++l) {
+            y4[4 * l] = br_.yff[l];
+            y4[4 * l + 1] = br_.yft[l];
+            y4[4 * l + 2] = br_.ytf[l];
+            y4[4 * l + 3] = br_.ytt[l];
+        }
+        std::vector<int> live;
+        for (int r = 0; r < rows; ++r)
+            if (res_.status[r] != static_cast<int32_t>(RowStatus::NotSimulated)) live.push_back(r);
+        if (live.empty()) return;
+        const int cap = std::min<int>(std::max(1, batch_size_), static_cast<int>(live.size()));
+        if (!batch_ || batch_->rows < cap) {
+            batch_.reset();
+            batch_ = std::make_unique<Batch>(*grid_, cap, precision_, 0);
+        }
+        const int B = batch_->rows;
+        std::vector<cplx> Vbuf(static_cast<size_t>(B) * n);
+        std::vector<double> o(static_cast<size_t>(B) * m), e(static_cast<size_t>(B) * m);
+        std::vector<int32_t> tp, tb;
+        for (size_t c0 = 0; c0 < live.size(); c0 += B) {
+            const int k = static_cast<int>(std::min<size_t>(B, live.size() - c0));
+            tp.assign(1, 0);
+            tb.clear();
+            for (int q = 0; q < k; ++q) {
+                const int r = live[c0 + q];
+                std::copy_n(res_.V.begin() + static_cast<size_t>(r) * n, n, Vbuf.begin() + static_cast<size_t>(q) * n);
+                if (static_cast<size_t>(r) < trips_.size())
+                    for (int l : trips_[r]) tb.push_back(l);
+                tp.push_back(static_cast<int32_t>(tb.size()));
+            }
+            check(g2g_batch_set_active_rows(batch_->h, k));
+            check(g2g_batch_set_v(batch_->h, reinterpret_cast<const double*>(Vbuf.data()), n, k, 0, 0));
+            check(g2g_batch_branch_flows(batch_->h, m, br_.f.data(), br_.t.data(),
+                                         reinterpret_cast<const double*>(y4.data()), br_.base_or.data(),
+                                         br_.base_ex.data(), tp.data(), tb.empty() ? nullptr : tb.data(), o.data(),
+                                         e.data()));
+            for (int q = 0; q < k; ++q) {
+                const int r = live[c0 + q];
+                std::copy_n(o.begin() + static_cast<size_t>(q) * m, m, or_.begin() + static_cast<size_t>(r) * m);
+                std::copy_n(e.begin() + static_cast<size_t>(q) * m, m, ex_.begin() + static_cast<size_t>(r) * m);
+            }
+        }
+        flows_ = true;
+    }
+    py::array_t<double> get_or_amps() const { return flows(or_); }
+    py::array_t<double> get_ex_amps() const { return flows(ex_); }
+
+    CArr v_base() const
+    {
+        CArr a(grid_->n());
+        std::copy(V_base_.begin(), V_base_.end(), a.mutable_data());
+        return a;
+    }
+    virtual int n_rows() const = 0;
+
+protected:
+    virtual void prepare(RowJob& job) = 0;
+    void need_run() const
+    {
+        if (!ran_) throw std::runtime_error("call run() first");
+    }
+    py::array_t<double> flows(const std::vector<double>& v) const
+    {
+        if (!flows_) throw std::runtime_error("call compute_flows() first");
+        py::array_t<double> a({static_cast<py::ssize_t>(n_rows()), static_cast<py::ssize_t>(br_.m())});
+        std::copy(v.begin(), v.end(), a.mutable_data());
+        return a;
+    }
+    Branches br_;
+    std::vector<std::vector<int>> trips_;  // tripped branches per row
+    std::vector<double> or_, ex_;
+    bool flows_ = false;
+
+    std::unique_ptr<Grid> grid_;
+    std::unique_ptr<Batch> batch_;
+    std::vector<cplx> S_base_, V_base_;
+    int batch_size_;
+    int nb_iter_;
+    int precision_;
+    BatchScheduling sched_ = BatchScheduling::Chunked;
+    double tol_ = 1e-8;
+    uintptr_t mlp_ = 0;
+    int k_warm_ = 4;
+    double growth_ = 2.0;
+    RowResults res_;
+    BatchTimings timings_;
+    bool ran_ = false;
+};
+
 // InjectionSweepSession (python_bindings.cpp:2167-2423)
 class InjectionSweepSession : public BatchSession {
 public:
@@ -780,11 +866,6 @@ protected:
 class ContingencyAnalysisSession : public BatchSession {
 public:
     using BatchSession::BatchSession;
-    void set_branch_data(IArr bf, IArr bt, CArr yff, CArr yft, CArr ytf, CArr ytt, DArr /*bus_vn_kv*/,
-                         double /*sn_mva*/)
-    {
-        br_.assign(bf, bt, yff, yft, ytf, ytt, n_bus());
-    }
     void build_contingencies(const std::vector<std::vector<int>>& ids)
     {
         if (!br_.set) throw std::runtime_error("call set_branch_data() first");
@@ -799,12 +880,12 @@ public:
             eptr_.push_back(static_cast<int32_t>(eent_.size()));
         }
         ctg_ = ids;
+        trips_ = ids;
         timings_.t_preprocess_ms = now_ms() - t0;
         ran_ = false;
     }
     int n_rows() const override { return static_cast<int>(ctg_.size()); }
     int n_contingencies() const { return n_rows(); }
-    int n_branches() const { return br_.m(); }
 
 protected:
     void prepare(RowJob& job) override
@@ -818,7 +899,6 @@ protected:
         job.edit_delta = &edel_;
         job.skip = skip_;
     }
-    Branches br_;
     std::vector<std::vector<int>> ctg_;
     std::vector<int32_t> eptr_, eent_;
     std::vector<cplx> edel_;
@@ -830,11 +910,6 @@ protected:
 class ScenarioSweepSession : public InjectionSweepSession {
 public:
     using InjectionSweepSession::InjectionSweepSession;
-    void set_branch_data(IArr bf, IArr bt, CArr yff, CArr yft, CArr ytf, CArr ytt, DArr /*bus_vn_kv*/,
-                         double /*sn_mva*/)
-    {
-        br_.assign(bf, bt, yff, yft, ytf, ytt, n_bus());
-    }
     void set_topology(const std::vector<std::vector<int>>& ids)
     {
         if (!br_.set) throw std::runtime_error("call set_branch_data() first");
@@ -848,16 +923,17 @@ public:
             eptr_.push_back(static_cast<int32_t>(eent_.size()));
         }
         topo_ = true;
+        trips_ = ids;
         ran_ = false;
     }
     void clear_topology()
     {
         topo_ = false;
         skip_.clear();
+        trips_.clear();
     }
     void clear_v_init() { V0_.clear(); }
     bool has_v_init() const { return !V0_.empty(); }
-    int n_branches() const { return br_.m(); }
     py::array_t<int32_t> get_disconnected() const
     {
         std::vector<int32_t> d(skip_.begin(), skip_.end());
@@ -877,7 +953,6 @@ protected:
             job.skip = skip_;
         }
     }
-    Branches br_;
     bool topo_ = false;
     std::vector<int32_t> eptr_, eent_;
     std::vector<cplx> edel_;
@@ -1029,6 +1104,13 @@ PYBIND11_MODULE(_gpusim2grid, m)
             .def_property("precision", &S::get_precision, &S::set_precision)
             .def("set_warm_start", &S::set_warm_start, py::arg("mlp_handle"), py::arg("k_warm") = 4,
                  py::arg("growth") = 2.0)
+            .def("set_branch_data", &S::set_branch_data, py::arg("branch_from"), py::arg("branch_to"),
+                 py::arg("yff_eff"), py::arg("yft_eff"), py::arg("ytf_eff"), py::arg("ytt_eff"), py::arg("bus_vn_kv"),
+                 py::arg("sn_mva"))
+            .def_property_readonly("n_branches", &S::n_branches)
+            .def("compute_flows", &S::compute_flows, py::call_guard<py::gil_scoped_release>())
+            .def("get_or_amps", &S::get_or_amps)
+            .def("get_ex_amps", &S::get_ex_amps)
             .def("clear_warm_start", &S::clear_warm_start);
     };
     auto batch_init = [](auto* tag) {
@@ -1062,20 +1144,13 @@ PYBIND11_MODULE(_gpusim2grid, m)
     py::class_<ContingencyAnalysisSession, std::shared_ptr<ContingencyAnalysisSession>> ca(
         m, "ContingencyAnalysisSession");
     ca.def(batch_init(static_cast<ContingencyAnalysisSession*>(nullptr)), G2G_BATCH_ARGS)
-        .def("set_branch_data", &ContingencyAnalysisSession::set_branch_data, py::arg("branch_from"),
-             py::arg("branch_to"), py::arg("yff_eff"), py::arg("yft_eff"), py::arg("ytf_eff"), py::arg("ytt_eff"),
-             py::arg("bus_vn_kv"), py::arg("sn_mva"))
         .def("build_contingencies", &ContingencyAnalysisSession::build_contingencies,
              py::arg("branch_ids_per_ctg"))
-        .def_property_readonly("n_contingencies", &ContingencyAnalysisSession::n_contingencies)
-        .def_property_readonly("n_branches", &ContingencyAnalysisSession::n_branches);
+        .def_property_readonly("n_contingencies", &ContingencyAnalysisSession::n_contingencies);
     bind_batch(ca);
 
     py::class_<ScenarioSweepSession, std::shared_ptr<ScenarioSweepSession>> ss(m, "ScenarioSweepSession");
     ss.def(batch_init(static_cast<ScenarioSweepSession*>(nullptr)), G2G_BATCH_ARGS)
-        .def("set_branch_data", &ScenarioSweepSession::set_branch_data, py::arg("branch_from"),
-             py::arg("branch_to"), py::arg("yff_eff"), py::arg("yft_eff"), py::arg("ytf_eff"), py::arg("ytt_eff"),
-             py::arg("bus_vn_kv"), py::arg("sn_mva"))
         .def("set_injections", &ScenarioSweepSession::set_injections, py::arg("p_mw"), py::arg("q_mvar"),
              py::arg("sn_mva"))
         .def("set_sbus", &ScenarioSweepSession::set_sbus, py::arg("Sbus"))
@@ -1085,8 +1160,7 @@ PYBIND11_MODULE(_gpusim2grid, m)
         .def("clear_v_init", &ScenarioSweepSession::clear_v_init)
         .def_property_readonly("has_v_init", &ScenarioSweepSession::has_v_init)
         .def("get_disconnected", &ScenarioSweepSession::get_disconnected)
-        .def_property_readonly("n_scenarios", &ScenarioSweepSession::n_scenarios)
-        .def_property_readonly("n_branches", &ScenarioSweepSession::n_branches);
+        .def_property_readonly("n_scenarios", &ScenarioSweepSession::n_scenarios);
     bind_batch(ss);
 #undef G2G_BATCH_ARGS
 

@@ -37,6 +37,9 @@ __global__ void k_gather_astream(const double* vals, long long ldv, const double
 __global__ void k_row_absmax(const double* vals, long long ldv, int rows, int nnz, double* out);
 __global__ void k_fill_int(int* p, int v, long long n);
 __global__ void k_count_tiles(const int* status, int rows_pad, int* tile_active);
+__global__ void k_branch_flows(const double2* V, int n, const int* from, const int* to, const double2* y4,
+                               const double* base_or, const double* base_ex, int m, int T, double* or_t,
+                               double* ex_t);
 template <typename T>
 __global__ void k_move_rows(const T* src, T* dst, const int* map, int R, int m, int gather);
 template <typename T>
@@ -180,7 +183,7 @@ size_t ring_lds()
 template <typename Real>
 size_t factor_lds(const SymbolicLU& s)
 {
-    const int scratch = s.max_c > kFastC ? 2 * kMaxC : 0;  // rows for the generic long-column path
+    const int scratch = s.max_c > kFastC ? 2 * s.max_c : 0;  // rows for the generic long-column path
     return ring_lds<Real>() + static_cast<size_t>(s.slots_f + 2 + scratch) * kLanes * sizeof(Real);
 }
 
@@ -209,7 +212,7 @@ void launch_factor(const g2g_plan_s* P, const Real* A, Real* U, int a_stride, in
     HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_factor<Real>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     FactorArgs<Real> a{A, U, P->d_prog_f, tile_active, jmax, clamped, error, s.d, a_stride, u_stride,
-                       P->prog_f_stride, s.slots_f, rows, eps};
+                       P->prog_f_stride, s.slots_f, rows, s.max_c, eps};
     k_factor<Real><<<T, kLanes * kStreamWaves, lds, st>>>(a);
     HIPCHK(hipGetLastError());
 }
@@ -837,6 +840,60 @@ int g2g_batch_warm_start(g2g_batch_t b, g2g_mlp_t mlp, const uint8_t* mask, int3
         }
         mlp_warm_start_tiles(mlp, b->S, b->active_rows, b->plan->nr.n, d_mask, n_mask, b->V, b->stream);
         HIPCHK(hipStreamSynchronize(b->stream));
+    });
+}
+
+int g2g_batch_branch_flows(g2g_batch_t b, int32_t m, const int32_t* from, const int32_t* to, const double* y4,
+                           const double* base_or, const double* base_ex, const int32_t* trip_ptr,
+                           const int32_t* trip_branch, double* or_amps, double* ex_amps)
+{
+    return guarded([&] {
+        if (!b || !from || !to || !y4 || !base_or || !base_ex || !or_amps || !ex_amps)
+            throw std::invalid_argument("null argument");
+        if (m <= 0) throw std::invalid_argument("m must be positive");
+        const int n = b->plan->nr.n;
+        for (int l = 0; l < m; ++l)
+            if (from[l] < -1 || from[l] >= n || to[l] < -1 || to[l] >= n)
+                throw std::invalid_argument("branch bus out of range");
+        const int rows = b->active_rows;
+        const int T = (rows + kLanes - 1) / kLanes;
+        DeviceScratch s;
+        int* d_f = s.get<int>(m);
+        int* d_t = s.get<int>(m);
+        double2* d_y = s.get<double2>(4 * static_cast<size_t>(m));
+        double* d_bo = s.get<double>(m);
+        double* d_be = s.get<double>(m);
+        const size_t cnt = static_cast<size_t>(T) * m * kLanes;
+        double* d_or = s.get<double>(cnt);
+        double* d_ex = s.get<double>(cnt);
+        hipStream_t st = b->stream;
+        HIPCHK(hipMemcpyAsync(d_f, from, m * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_t, to, m * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_y, y4, 4 * static_cast<size_t>(m) * sizeof(double2), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_bo, base_or, m * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_be, base_ex, m * sizeof(double), hipMemcpyHostToDevice, st));
+        k_branch_flows<<<blocks_for(static_cast<long long>(cnt), 256), 256, 0, st>>>(
+            b->V, n, d_f, d_t, d_y, d_bo, d_be, m, T, d_or, d_ex);
+        HIPCHK(hipGetLastError());
+        std::vector<double> ho(cnt), he(cnt);
+        HIPCHK(hipMemcpyAsync(ho.data(), d_or, cnt * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(he.data(), d_ex, cnt * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int r = 0; r < rows; ++r)
+            for (int l = 0; l < m; ++l) {
+                const size_t i = (static_cast<size_t>(r / kLanes) * m + l) * kLanes + (r % kLanes);
+                or_amps[static_cast<size_t>(r) * m + l] = ho[i];
+                ex_amps[static_cast<size_t>(r) * m + l] = he[i];
+            }
+        // tripped branches carry no current
+        if (trip_ptr && trip_branch)
+            for (int r = 0; r < rows; ++r)
+                for (int q = trip_ptr[r]; q < trip_ptr[r + 1]; ++q) {
+                    const int l = trip_branch[q];
+                    if (l < 0 || l >= m) throw std::invalid_argument("tripped branch out of range");
+                    or_amps[static_cast<size_t>(r) * m + l] = 0.0;
+                    ex_amps[static_cast<size_t>(r) * m + l] = 0.0;
+                }
     });
 }
 

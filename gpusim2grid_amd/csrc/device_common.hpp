@@ -41,6 +41,7 @@ struct FactorArgs {
     int* clamped;           // nullable: clamped pivots per row
     int* error;             // set when a wavefront gave up waiting on its stream
     int d, a_stride, u_stride, prog_stride, slots, rows;
+    int max_c;              // longest column (sizes the long-column scratch rows)
     double pivot_eps;
 };
 

@@ -425,8 +425,8 @@ __global__ __launch_bounds__(kLanes * kStreamWaves) void k_factor(FactorArgs<Rea
     }
     // wave 0: the program walk
     const int row = static_cast<int>(tile) * kLanes + lane;
-    Real* lsc = sl + static_cast<size_t>(a.slots + 2) * kLanes;  // [kMaxC][64] (long columns only)
-    Real* usc = lsc + static_cast<size_t>(kMaxC) * kLanes;       // [kMaxC][64]
+    Real* lsc = sl + static_cast<size_t>(a.slots + 2) * kLanes;  // [max_c][64] (long columns only)
+    Real* usc = lsc + static_cast<size_t>(a.max_c) * kLanes;     // [max_c][64]
     for (int s2 = 0; s2 < a.slots + 2; ++s2) sl[static_cast<size_t>(s2) * kLanes + lane] = Real(0);
     typename ARing::Reader A;
     A.init(aring, ctlA, a.a_stride / Sh::kCh, lane);
@@ -604,11 +604,12 @@ template <typename T>
 __global__ void k_rows_to_tiles(const T* __restrict__ in, long long ld, int rows, int m, int row_offset,
                                 T* __restrict__ out, int broadcast)
 {
+    // rows fastest: consecutive threads fill consecutive lanes of the tiled output
     const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const long long total = static_cast<long long>(rows) * m;
     if (gid >= total) return;
-    const int r = static_cast<int>(gid / m);
-    const int k = static_cast<int>(gid - static_cast<long long>(r) * m);
+    const int k = static_cast<int>(gid / rows);
+    const int r = static_cast<int>(gid - static_cast<long long>(k) * rows);
     const int dst = r + row_offset;
     const T v = in[(broadcast ? 0 : static_cast<long long>(r) * ld) + k];
     out[tidx(static_cast<size_t>(dst / kLanes), m, k, dst % kLanes)] = v;
@@ -620,8 +621,8 @@ __global__ void k_tiles_to_rows(const T* __restrict__ in, int rows, int m, T* __
     const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const long long total = static_cast<long long>(rows) * m;
     if (gid >= total) return;
-    const int r = static_cast<int>(gid / m);
-    const int k = static_cast<int>(gid - static_cast<long long>(r) * m);
+    const int k = static_cast<int>(gid / rows);
+    const int r = static_cast<int>(gid - static_cast<long long>(k) * rows);
     out[static_cast<long long>(r) * ld + k] = in[tidx(static_cast<size_t>(r / kLanes), m, k, r % kLanes)];
 }
 
@@ -736,6 +737,39 @@ __global__ void k_move_vals(const T* __restrict__ src, T* __restrict__ dst, cons
     if (i >= R) return;
     if (gather) dst[i] = src[map[i]];
     else dst[map[i]] = add ? dst[map[i]] + src[i] : src[i];
+}
+
+// Branch terminal currents on converged V (acpf_nr_kernels.cu:122-182):
+//   I_or = yff V_f + yft V_t,  I_ex = ytf V_f + ytt V_t,  amps = |I| * base current
+// One thread per (tile, branch, lane); outputs tiled [T][m][64].  A side with
+// bus -1 (no terminal) reports 0.
+__global__ void k_branch_flows(const double2* __restrict__ V, int n, const int* __restrict__ from,
+                               const int* __restrict__ to, const double2* __restrict__ y4,
+                               const double* __restrict__ base_or, const double* __restrict__ base_ex, int m,
+                               int T, double* __restrict__ or_t, double* __restrict__ ex_t)
+{
+    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= static_cast<long long>(T) * m * kLanes) return;
+    const int lane = static_cast<int>(gid & (kLanes - 1));
+    const long long tl = gid / kLanes;
+    const int l = static_cast<int>(tl % m);
+    const size_t tile = static_cast<size_t>(tl / m);
+    const int f = from[l], t = to[l];
+    const double2 z = make_double2(0.0, 0.0);
+    const double2 vf = f >= 0 ? V[tidx(tile, n, f, lane)] : z;
+    const double2 vt = t >= 0 ? V[tidx(tile, n, t, lane)] : z;
+    const double2* y = y4 + 4 * static_cast<size_t>(l);
+    double2 io = z, ie = z;
+    if (f >= 0) {
+        const double2 a = cmul(y[0], vf), b = cmul(y[1], vt);
+        io = make_double2(a.x + b.x, a.y + b.y);
+    }
+    if (t >= 0) {
+        const double2 a = cmul(y[2], vf), b = cmul(y[3], vt);
+        ie = make_double2(a.x + b.x, a.y + b.y);
+    }
+    or_t[gid] = sqrt(fma(io.x, io.x, io.y * io.y)) * base_or[l];
+    ex_t[gid] = sqrt(fma(ie.x, ie.x, ie.y * ie.y)) * base_ex[l];
 }
 
 // ---------------------------------------------------------------------------

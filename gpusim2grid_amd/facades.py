@@ -204,6 +204,36 @@ class _BatchFacade:
     def precision(self, v):
         self._s.precision = _precision(v)
 
+    def set_branch_data(self, branch_from, branch_to, yff_eff, yft_eff, ytf_eff, ytt_eff, bus_vn_kv=None,
+                        sn_mva=100.0):
+        """Per-branch pi-model admittances (from / to bus, yff, yft, ytf, ytt),
+        bus nominal voltages (kV; default 1) and the system base (MVA), for
+        outages and :meth:`compute_flows`."""
+        n = self._s.n_bus
+        vn = np.ones(n) if bus_vn_kv is None else np.ascontiguousarray(bus_vn_kv, dtype=float)
+        self._s.set_branch_data(np.ascontiguousarray(branch_from, dtype=np.int32),
+                                np.ascontiguousarray(branch_to, dtype=np.int32),
+                                *[np.ascontiguousarray(y, dtype=complex) for y in (yff_eff, yft_eff, ytf_eff,
+                                                                                   ytt_eff)],
+                                vn, float(sn_mva))
+
+    def compute_flows(self):
+        """Branch currents (A) of every row at its result, on the device;
+        tripped branches carry 0 and rows not simulated NaN."""
+        self._s.compute_flows()
+
+    @property
+    def or_amps(self):
+        return self._s.get_or_amps()
+
+    @property
+    def ex_amps(self):
+        return self._s.get_ex_amps()
+
+    @property
+    def n_branches(self):
+        return self._s.n_branches
+
     def set_warm_start(self, warm, k_warm=4, growth=2.0):
         """Learned warm start (``gpusim2grid_amd.warmstart.WarmStart``) with
         flat-start fallback on the next runs; None switches it off."""
@@ -284,16 +314,6 @@ class ContingencyAnalysisGPU(_BatchFacade):
         super().__init__(grid, **kw)
         self._ctg = []
 
-    def set_branch_data(self, branch_from, branch_to, yff_eff, yft_eff, ytf_eff, ytt_eff, bus_vn_kv=None,
-                        sn_mva=100.0):
-        n = self._s.n_bus
-        vn = np.ones(n) if bus_vn_kv is None else np.ascontiguousarray(bus_vn_kv, dtype=float)
-        self._s.set_branch_data(np.ascontiguousarray(branch_from, dtype=np.int32),
-                                np.ascontiguousarray(branch_to, dtype=np.int32),
-                                *[np.ascontiguousarray(y, dtype=complex) for y in (yff_eff, yft_eff, ytf_eff,
-                                                                                   ytt_eff)],
-                                vn, float(sn_mva))
-
     def add_contingencies_by_branch_id(self, branch_ids_per_ctg):
         """One contingency per list of tripped branch ids."""
         self._ctg.extend([list(map(int, c)) for c in branch_ids_per_ctg])
@@ -309,21 +329,12 @@ class ContingencyAnalysisGPU(_BatchFacade):
     def n_contingencies(self):
         return self._s.n_contingencies
 
-    @property
-    def n_branches(self):
-        return self._s.n_branches
-
 
 class ScenarioSweepGPU(InjectionSweepGPU):
     """Per-row injections and per-row branch trips together
     (``scenario_sweep/gpu_facade.py``)."""
 
     _session_cls = _cpp.ScenarioSweepSession
-
-    def set_branch_data(self, branch_from, branch_to, yff_eff, yft_eff, ytf_eff, ytt_eff, bus_vn_kv=None,
-                        sn_mva=100.0):
-        ContingencyAnalysisGPU.set_branch_data(self, branch_from, branch_to, yff_eff, yft_eff, ytf_eff, ytt_eff,
-                                               bus_vn_kv, sn_mva)
 
     def set_topology(self, branch_ids_per_scenario):
         self._s.set_topology([list(map(int, c)) for c in branch_ids_per_scenario])

@@ -159,6 +159,18 @@ int g2g_batch_solve_warm(g2g_batch_t batch, g2g_mlp_t mlp, const uint8_t *mask, 
  * (no counterpart in the reference; same mask convention as above). */
 int g2g_batch_warm_start(g2g_batch_t batch, g2g_mlp_t mlp, const uint8_t *mask, int32_t n_mask);
 
+/* Branch terminal currents of the active rows at the batch's V (replaces
+ * compute_branch_flows_kernel, acpf_nr_kernels.cu:122-182): for branch l,
+ * I_or = yff V_from + yft V_to, I_ex = ytf V_from + ytt V_to, and
+ * amps = |I| * base (base_or / base_ex per branch, e.g. sn_mva 1e6 /
+ * (sqrt(3) vn_kv 1e3)).  y4 holds 4 m complex values per branch in the order
+ * yff, yft, ytf, ytt; a bus of -1 marks a missing terminal (0 A).  Row r's
+ * tripped branches trip_branch[trip_ptr[r] .. trip_ptr[r+1]) report 0
+ * (trip_ptr / trip_branch nullable).  Outputs rows x m, host arrays. */
+int g2g_batch_branch_flows(g2g_batch_t batch, int32_t m, const int32_t *from, const int32_t *to, const double *y4,
+                           const double *base_or, const double *base_ex, const int32_t *trip_ptr,
+                           const int32_t *trip_branch, double *or_amps, double *ex_amps);
+
 int g2g_batch_get_v(g2g_batch_t batch, double *V, int64_t ld, int32_t on_device);
 int g2g_batch_get_results(g2g_batch_t batch, int32_t *status, int32_t *iters, double *residual);
 

@@ -228,3 +228,32 @@ def test_warmup_and_mixed_precision_session():
                                  n_threads=8)
     np.testing.assert_array_equal(sw.get_row_status(), sto)
     assert np.max(np.abs(V - Vo)) < 1e-7
+
+
+@pytest.mark.gpu
+def test_branch_flows_against_closed_form():
+    from gpusim2grid_amd import ContingencyAnalysisGPU
+    g = syn.grid_118()
+    _say("branch flows 118")
+    br = syn.bridges(g.n_bus, g.branch_from, g.branch_to)
+    ok = np.flatnonzero(~br)[:20]
+    ctg = [[int(l)] for l in ok] + [[int(np.flatnonzero(br)[0])]]
+    vn = np.random.default_rng(0).choice([63.0, 225.0, 400.0], size=g.n_bus)
+    ca = ContingencyAnalysisGPU(_tuple(g), init_from_n_powerflow=False, nb_iter=10, scheduling="continuous")
+    ca.set_branch_data(g.branch_from, g.branch_to, g.yff, g.yft, g.ytf, g.ytt, bus_vn_kv=vn, sn_mva=g.sn_mva)
+    ca.add_contingencies_by_branch_id(ctg)
+    V = ca.compute(batch_size=8, as_numpy=True)
+    ca.compute_flows()
+    o, e = ca.or_amps, ca.ex_amps
+    assert o.shape == (len(ctg), len(g.branch_from))
+    f, t = g.branch_from, g.branch_to
+    bo = g.sn_mva * 1e6 / (np.sqrt(3) * vn[f] * 1e3)
+    be = g.sn_mva * 1e6 / (np.sqrt(3) * vn[t] * 1e3)
+    for r, c in enumerate(ctg[:-1]):
+        Io = np.abs(g.yff * V[r, f] + g.yft * V[r, t]) * bo
+        Ie = np.abs(g.ytf * V[r, f] + g.ytt * V[r, t]) * be
+        Io[c] = 0.0
+        Ie[c] = 0.0
+        np.testing.assert_allclose(o[r], Io, rtol=1e-12, atol=1e-9)
+        np.testing.assert_allclose(e[r], Ie, rtol=1e-12, atol=1e-9)
+    assert np.isnan(o[-1]).all()  # the islanding row is not simulated
