@@ -26,7 +26,7 @@ ARR = dict(jp=0, ji=1, maps=2, pvpq=3, perm=4, colp=5, coli=6, prog_f=7, prog_b=
 class PlanInfo(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int32) for k in (
         "n", "nY", "d", "nJ", "npvpq", "npq", "slack", "nnz", "nLU", "n_updates", "max_c", "height",
-        "slots_f", "slots_b", "a_len", "u_len")]
+        "slots_f", "slots_b", "a_len", "u_len", "n_steps")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

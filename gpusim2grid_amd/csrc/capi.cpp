@@ -226,7 +226,8 @@ void launch_backward(const g2g_plan_s* P, const Real* U, int u_stride, const int
     if (lds > lds_limit()) throw std::runtime_error("backward substitution needs too much LDS");
     HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_backward<Real>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    BackwardArgs<Real> a{U, P->d_prog_b, tile_active, X, error, s.d, s.u_len, u_stride, P->prog_b_stride, rows};
+    BackwardArgs<Real> a{U, P->d_prog_b, tile_active, X, error, s.d, s.u_len, u_stride, P->prog_b_stride, rows,
+                         s.max_c};
     k_backward<Real><<<T, kLanes * kStreamWaves, lds, st>>>(a);
     HIPCHK(hipGetLastError());
 }
@@ -631,6 +632,7 @@ int g2g_plan_get_info(g2g_plan_t P, g2g_plan_info* info)
         }
         info->nLU = s.nLU;
         info->n_updates = s.n_updates;
+        info->n_steps = s.n_steps;
         info->max_c = s.max_c;
         info->height = s.height;
         info->slots_f = s.slots_f;

@@ -53,6 +53,7 @@ struct BackwardArgs {
     Real* X;                // solution [T][d][64] in input labels
     int* error;             // set when a wavefront gave up waiting on its stream
     int d, u_len, u_stride, prog_stride, rows;
+    int max_c;              // longest column (a longer record header means a corrupt walk)
 };
 
 template <typename Real>

@@ -266,7 +266,7 @@ struct FactorState {
 constexpr int kAllReadC = 5;
 
 template <int C, typename Real, typename AR>
-__device__ __forceinline__ void factor_pivot(AR& A, ProgReader& P, FactorState<Real>& st)
+__device__ __forceinline__ int factor_pivot(AR& A, ProgReader& P, FactorState<Real>& st)
 {
     constexpr int NS = frec_src_words(C), NW = frec_words(C), NV = 2 + 2 * C;
     constexpr bool kAll = C <= kAllReadC;
@@ -334,11 +334,132 @@ __device__ __forceinline__ void factor_pivot(AR& A, ProgReader& P, FactorState<R
 #pragma unroll
     for (int a = 0; a < C; ++a) *at(NS + C * C + a) = fma(lv[a], fk, tb[a]);
     st.p += NW;
+    return static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(word(w, 0))) >> 16);  // next record's c
+}
+
+// 2-column supernode (pivots k, k+1 with struct(k) = {k+1} + S, |S| = C): all
+// sources of both pivots are read at once; pivot k is applied to pivot k+1's
+// row, column and diagonal in registers; S x S and b(S) receive one rank-2
+// update.  The U stream gets the two pivots' records, as separate pivots
+// would write them.  The A stream holds pivot k's then pivot k+1's values:
+//   [F(k), a(k,k), a(k+1,k), L(S,k)[C], a(k,k+1), U(k,S)[C],
+//    F(k+1), a(k+1,k+1), L(S,k+1)[C], U(k+1,S)[C]]
+template <int C, typename Real, typename AR>
+__device__ __forceinline__ int factor_pivot2(AR& A, ProgReader& P, FactorState<Real>& st)
+{
+    constexpr int NS = frec2_src_words(C), NW = frec2_words(C), NV = 6 + 4 * C;
+    constexpr bool kAll = C <= 3;
+    P.window(st.p, st.p + NW - 1);
+    A.window(st.ap, st.ap + NV - 1);
+    v4u w[NW / 4];
+    read_words<NW>(P, st.p, w);
+    Real av[NV];
+    load_values<NV>(A, st.ap, av);
+    auto at = [&](int q) -> Real* { return slot_at<Real>(st.lb, word(w, q)); };
+    Real sv[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sv[q] = *at(1 + q);
+    Real tv[kAll && C > 0 ? C * C : 1], tb[C > 0 ? C : 1];
+    if constexpr (kAll) {
+#pragma unroll
+        for (int i = 0; i < C * C; ++i) tv[i] = *at(NS + i);
+    }
+#pragma unroll
+    for (int a = 0; a < C; ++a) tb[a] = *at(NS + C * C + a);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) *at(1 + q) = Real(0);
+    // A-stream order per pivot is [F, diag, ...] and source order [diag, b, ...]
+    constexpr int K1 = 4 + 2 * C;
+    Real x[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        const int sq = (q == 0 || q == K1) ? q + 1 : (q == 1 || q == K1 + 1) ? q - 1 : q;
+        x[q] = av[q] + sv[sq];
+    }
+    // pivot k
+    const Real fk = x[0];
+    Real d1 = x[1];
+    const Real e = x[2];
+    const Real b = x[3 + C];
+    if (fabs(d1) < st.eps) {
+        d1 = d1 < Real(0) ? -st.eps : st.eps;
+        ++st.clamped;
+    }
+    const Real inv1 = Real(1) / d1;
+    const Real l = e * inv1;
+    Real mk[C > 0 ? C : 1], uk[C > 0 ? C : 1], mk1[C > 0 ? C : 1], uk1[C > 0 ? C : 1];
+#pragma unroll
+    for (int i = 0; i < C; ++i) mk[i] = x[3 + i] * inv1;
+#pragma unroll
+    for (int j = 0; j < C; ++j) uk[j] = x[4 + C + j];
+    Real* u = st.U + static_cast<size_t>(st.up) * kLanes;
+    u[0] = fk;
+    u[kLanes] = inv1;
+    u[2 * kLanes] = b;
+#pragma unroll
+    for (int j = 0; j < C; ++j) u[(3 + j) * kLanes] = uk[j];
+    // pivot k+1, updated by pivot k
+    const Real fk1 = fma(-l, fk, x[K1]);
+    Real d2 = fma(-l, b, x[K1 + 1]);
+    if (fabs(d2) < st.eps) {
+        d2 = d2 < Real(0) ? -st.eps : st.eps;
+        ++st.clamped;
+    }
+    const Real inv2 = Real(1) / d2;
+#pragma unroll
+    for (int i = 0; i < C; ++i) mk1[i] = fma(-mk[i], b, x[K1 + 2 + i]) * inv2;
+#pragma unroll
+    for (int j = 0; j < C; ++j) uk1[j] = fma(-l, uk[j], x[K1 + 2 + C + j]);
+    Real* u1 = u + static_cast<size_t>(3 + C) * kLanes;
+    u1[0] = fk1;
+    u1[kLanes] = inv2;
+#pragma unroll
+    for (int j = 0; j < C; ++j) u1[(2 + j) * kLanes] = uk1[j];
+    st.up += 5 + 2 * C;
+    st.ap += NV;
+    // rank-2 update of S x S and b(S)
+    if constexpr (kAll) {
+#pragma unroll
+        for (int a = 0; a < C; ++a)
+#pragma unroll
+            for (int c2 = 0; c2 < C; ++c2)
+                *at(NS + a * C + c2) = fma(-mk1[a], uk1[c2], fma(-mk[a], uk[c2], tv[a * C + c2]));
+    } else {
+        Real t0[C], t1[C];
+#pragma unroll
+        for (int q = 0; q < C; ++q) t0[q] = *at(NS + q);
+#pragma unroll
+        for (int a = 0; a < C; ++a) {
+            Real(&cur)[C] = (a & 1) ? t1 : t0;
+            Real(&nxt)[C] = (a & 1) ? t0 : t1;
+            if (a + 1 < C) {
+#pragma unroll
+                for (int q = 0; q < C; ++q) nxt[q] = *at(NS + (a + 1) * C + q);
+            }
+#pragma unroll
+            for (int c2 = 0; c2 < C; ++c2) *at(NS + a * C + c2) = fma(-mk1[a], uk1[c2], fma(-mk[a], uk[c2], cur[c2]));
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < C; ++a) *at(NS + C * C + a) = fma(-mk1[a], fk1, fma(-mk[a], fk, tb[a]));
+    st.p += NW;
+    return static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(word(w, 0))) >> 16);
+}
+
+template <int C, typename Real, typename AR>
+__device__ __forceinline__ int factor_dispatch2(int c, AR& A, ProgReader& P, FactorState<Real>& st)
+{
+    if constexpr (C > kFastC2) {
+        return -1;
+    } else {
+        if (c == C) return factor_pivot2<C>(A, P, st);
+        return factor_dispatch2<C + 1>(c, A, P, st);
+    }
 }
 
 // Any column length (up to kMaxC), one word at a time: the rare long columns.
 template <typename Real, typename AR>
-__device__ void factor_pivot_any(int c, AR& A, ProgReader& P, FactorState<Real>& st, Real* lsc, Real* usc)
+__device__ int factor_pivot_any(int c, AR& A, ProgReader& P, FactorState<Real>& st, Real* lsc, Real* usc)
 {
     const int ns = frec_src_words(c), rw = frec_row_words(c), nw = frec_words(c);
     P.window(st.p, st.p + nw - 1);
@@ -376,19 +497,19 @@ __device__ void factor_pivot_any(int c, AR& A, ProgReader& P, FactorState<Real>&
         Real* t = slot_at<Real>(st.lb, ref(ns + c * rw + a));
         *t = fma(lsc[a * kLanes], fk, *t);
     }
+    const int c_next = __builtin_amdgcn_readfirstlane(static_cast<int>(ref(0) >> 16));
     st.p += nw;
+    return c_next;
 }
 
+// the pivot of column length c; returns the next record's c (-1: no fast path)
 template <int C, typename Real, typename AR>
-__device__ __forceinline__ bool factor_dispatch(int c, AR& A, ProgReader& P, FactorState<Real>& st)
+__device__ __forceinline__ int factor_dispatch(int c, AR& A, ProgReader& P, FactorState<Real>& st)
 {
     if constexpr (C > kFastC) {
-        return false;
+        return -1;
     } else {
-        if (c == C) {
-            factor_pivot<C>(A, P, st);
-            return true;
-        }
+        if (c == C) return factor_pivot<C>(A, P, st);
         return factor_dispatch<C + 1>(c, A, P, st);
     }
 }
@@ -438,15 +559,38 @@ __global__ __launch_bounds__(kLanes * kStreamWaves) void k_factor(FactorArgs<Rea
     st.eps = static_cast<Real>(a.pivot_eps) * static_cast<Real>(a.jmax ? a.jmax[row] : 1.0);
     st.clamped = 0;
     st.p = st.ap = st.up = 0;
-    int c = prog_word(P, 0) & 0xFFFF;
-    for (int k = 0; k < a.d; ++k) {
-        // the header of this record announces the next record's column length
-        const int c_next = (prog_word(P, st.p) >> 16) & 0xFFFF;
-        if (!factor_dispatch<0>(c, A, P, st)) factor_pivot_any(c, A, P, st, lsc + lane, usc + lane);
-        c = c_next;
+    int h = prog_word(P, 0) & 0xFFFF;
+    bool bad = (h & 0x7FFF) > a.max_c;
+    for (int k = 0; k < a.d && !bad;) {
+        // each record's header announces the next record's column length and
+        // width; a walk whose stream wait gave up stops before it trusts
+        // another record
+        const int c = h & 0x7FFF;
+        int h_next;
+        if (h >> 15) {
+            h_next = factor_dispatch2<0>(c, A, P, st);
+            if (h_next < 0) {  // never emitted: the host pairs columns up to kFastC2 only
+                bad = true;
+                break;
+            }
+            k += 2;
+        } else {
+            h_next = factor_dispatch<0>(c, A, P, st);
+            if (h_next < 0) h_next = factor_pivot_any(c, A, P, st, lsc + lane, usc + lane);
+            k += 1;
+        }
+        h = h_next & 0xFFFF;
+        if (A.failed || P.failed || ((h & 0x7FFF) > a.max_c && k < a.d)) {
+            bad = true;
+            break;
+        }
     }
     if (a.clamped && row < a.rows) a.clamped[row] = st.clamped;
-    if ((A.failed || P.failed) && lane == 0) atomicOr(a.error, 1);
+    if (bad) {  // release the producers at once
+        if (lane == 0) atomicOr(a.error, 1);
+        lds_store_release(&ctlA->abort, 1);
+        lds_store_release(&ctlP->abort, 1);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -458,11 +602,11 @@ template <typename Real>
 struct BackState {
     unsigned char* lb;
     Real* X;
-    int p, up;
+    int p, up, d;
 };
 
 template <int C, typename Real, typename UR>
-__device__ __forceinline__ void backward_pivot(UR& U, ProgReader& P, BackState<Real>& st)
+__device__ __forceinline__ int backward_pivot(UR& U, ProgReader& P, BackState<Real>& st)
 {
     constexpr int NW = brec_words(C);
     P.window(st.p, st.p + NW - 1);
@@ -480,13 +624,60 @@ __device__ __forceinline__ void backward_pivot(UR& U, ProgReader& P, BackState<R
     for (int q = 0; q < C; ++q) s = fma(-uv[2 + q], xj[q], s);
     const Real x = s * uv[1];  // U(k,k) is stored as its reciprocal
     const int perm_k = __builtin_amdgcn_readfirstlane(static_cast<int>(word(w, 1)));
-    st.X[static_cast<size_t>(perm_k) * kLanes] = x;
+    if (static_cast<unsigned>(perm_k) < static_cast<unsigned>(st.d)) st.X[static_cast<size_t>(perm_k) * kLanes] = x;
     *slot_at<Real>(st.lb, word(w, 2 + C)) = x;
     st.p += NW;
+    return static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(word(w, 0))) >> 16);
+}
+
+// Backward 2-column supernode (pivots k, k+1; |S| = C): x(k+1) from its U row,
+// then x(k) with x(k+1) in registers; the U records of the pair are adjacent:
+//   [y(k), 1/U(k,k), U(k,k+1), U(k,S)[C], y(k+1), 1/U(k+1,k+1), U(k+1,S)[C]]
+template <int C, typename Real, typename UR>
+__device__ __forceinline__ int backward_pivot2(UR& U, ProgReader& P, BackState<Real>& st)
+{
+    constexpr int NW = brec2_words(C), NU = 5 + 2 * C;
+    P.window(st.p, st.p + NW - 1);
+    v4u w[NW / 4];
+    read_words<NW>(P, st.p, w);
+    st.up -= NU;
+    U.window(st.up + NU - 1, st.up);
+    Real uv[NU];
+    load_values<NU>(U, st.up, uv);
+    Real xj[C > 0 ? C : 1];
+#pragma unroll
+    for (int q = 0; q < C; ++q) xj[q] = *slot_at<Real>(st.lb, word(w, 3 + q));
+    Real s2 = uv[3 + C];
+#pragma unroll
+    for (int q = 0; q < C; ++q) s2 = fma(-uv[5 + C + q], xj[q], s2);
+    const Real x2 = s2 * uv[4 + C];
+    Real s1 = fma(-uv[2], x2, uv[0]);
+#pragma unroll
+    for (int q = 0; q < C; ++q) s1 = fma(-uv[3 + q], xj[q], s1);
+    const Real x1 = s1 * uv[1];
+    const int perm_k1 = __builtin_amdgcn_readfirstlane(static_cast<int>(word(w, 1)));
+    const int perm_k = __builtin_amdgcn_readfirstlane(static_cast<int>(word(w, 2)));
+    if (static_cast<unsigned>(perm_k1) < static_cast<unsigned>(st.d)) st.X[static_cast<size_t>(perm_k1) * kLanes] = x2;
+    if (static_cast<unsigned>(perm_k) < static_cast<unsigned>(st.d)) st.X[static_cast<size_t>(perm_k) * kLanes] = x1;
+    *slot_at<Real>(st.lb, word(w, 3 + C)) = x2;
+    *slot_at<Real>(st.lb, word(w, 4 + C)) = x1;
+    st.p += NW;
+    return static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(word(w, 0))) >> 16);
+}
+
+template <int C, typename Real, typename UR>
+__device__ __forceinline__ int backward_dispatch2(int c, UR& U, ProgReader& P, BackState<Real>& st)
+{
+    if constexpr (C > kFastC2) {
+        return -1;
+    } else {
+        if (c == C) return backward_pivot2<C>(U, P, st);
+        return backward_dispatch2<C + 1>(c, U, P, st);
+    }
 }
 
 template <typename Real, typename UR>
-__device__ void backward_pivot_any(int c, UR& U, ProgReader& P, BackState<Real>& st)
+__device__ int backward_pivot_any(int c, UR& U, ProgReader& P, BackState<Real>& st)
 {
     const int nw = brec_words(c);
     P.window(st.p, st.p + nw - 1);
@@ -498,21 +689,20 @@ __device__ void backward_pivot_any(int c, UR& U, ProgReader& P, BackState<Real>&
     for (int q = 0; q < c; ++q) s = fma(-uval(2 + q), *slot_at<Real>(st.lb, ref(2 + q)), s);
     const Real x = s * uval(1);
     const int perm_k = P.at(st.p + 1);
-    st.X[static_cast<size_t>(perm_k) * kLanes] = x;
+    if (static_cast<unsigned>(perm_k) < static_cast<unsigned>(st.d)) st.X[static_cast<size_t>(perm_k) * kLanes] = x;
     *slot_at<Real>(st.lb, ref(2 + c)) = x;
+    const int c_next = __builtin_amdgcn_readfirstlane(static_cast<int>(ref(0) >> 16));
     st.p += nw;
+    return c_next;
 }
 
 template <int C, typename Real, typename UR>
-__device__ __forceinline__ bool backward_dispatch(int c, UR& U, ProgReader& P, BackState<Real>& st)
+__device__ __forceinline__ int backward_dispatch(int c, UR& U, ProgReader& P, BackState<Real>& st)
 {
     if constexpr (C > kFastC) {
-        return false;
+        return -1;
     } else {
-        if (c == C) {
-            backward_pivot<C>(U, P, st);
-            return true;
-        }
+        if (c == C) return backward_pivot<C>(U, P, st);
         return backward_dispatch<C + 1>(c, U, P, st);
     }
 }
@@ -556,13 +746,35 @@ __global__ __launch_bounds__(kLanes * kStreamWaves) void k_backward(BackwardArgs
     st.X = a.X + tile * static_cast<size_t>(a.d) * kLanes + lane;
     st.p = 0;
     st.up = a.u_len;
-    int c = prog_word(P, 0) & 0xFFFF;
-    for (int q = 0; q < a.d; ++q) {
-        const int c_next = (prog_word(P, st.p) >> 16) & 0xFFFF;
-        if (!backward_dispatch<0>(c, U, P, st)) backward_pivot_any(c, U, P, st);
-        c = c_next;
+    st.d = a.d;
+    int h = prog_word(P, 0) & 0xFFFF;
+    bool bad = (h & 0x7FFF) > a.max_c;
+    for (int q = 0; q < a.d && !bad;) {
+        const int c = h & 0x7FFF;
+        int h_next;
+        if (h >> 15) {
+            h_next = backward_dispatch2<0>(c, U, P, st);
+            if (h_next < 0) {
+                bad = true;
+                break;
+            }
+            q += 2;
+        } else {
+            h_next = backward_dispatch<0>(c, U, P, st);
+            if (h_next < 0) h_next = backward_pivot_any(c, U, P, st);
+            q += 1;
+        }
+        h = h_next & 0xFFFF;
+        if (U.failed || P.failed || ((h & 0x7FFF) > a.max_c && q < a.d)) {
+            bad = true;
+            break;
+        }
     }
-    if ((U.failed || P.failed) && lane == 0) atomicOr(a.error, 1);
+    if (bad) {
+        if (lane == 0) atomicOr(a.error, 1);
+        lds_store_release(&ctlU->abort, 1);
+        lds_store_release(&ctlP->abort, 1);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -311,81 +311,106 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
         }
         ++apos;
     };
-    for (int k = 0; k < d; ++k) {
-        s.u_off[k] = upos;
+    // Steps of the walk: a single pivot, or a 2-column supernode (k, k+1) when
+    // struct(k) = {k+1} + struct(k+1) -- the P and Q unknowns of one bus,
+    // eliminated together.  A supernode reads the sources of both pivots,
+    // applies pivot k to pivot k+1's row and column in registers, and updates
+    // struct(k+1) x struct(k+1) once with a rank-2 term.  The A and U streams
+    // keep their per-pivot layout.
+    struct Step {
+        int k, w, c;  // first pivot, width, |struct| of the last pivot
+        std::vector<int> r;
+    };
+    std::vector<Step> steps;
+    steps.reserve(static_cast<size_t>(d));
+    s.step_w.assign(static_cast<size_t>(d), 1);
+    auto alloc_target = [&](int i, int j, std::vector<int>& r) {
+        const long long kk = key(i, j, d);
+        auto it = slot_of.find(kk);
+        if (it == slot_of.end()) {
+            const int sl = sa.get();
+            slot_of[kk] = sl;
+            r.push_back(sl | kFirst);
+        } else {
+            r.push_back(it->second);
+        }
+    };
+    auto alloc_b = [&](int i, std::vector<int>& r) {
+        if (bslot[i] < 0) {
+            bslot[i] = sa.get();
+            r.push_back(bslot[i] | kFirst);
+        } else {
+            r.push_back(bslot[i]);
+        }
+    };
+    auto sources = [&](int k, std::vector<int>& r) {  // diagonal, b(k), L(struct,k), U(k,struct)
         const int c0 = s.colp[k], c = s.colp[k + 1] - c0;
-        s.max_c = std::max(s.max_c, c);
-        std::vector<int>& r = refs[k];
-        r.reserve(static_cast<size_t>((c + 1) * (c + 2)));
-        // A stream: F(k), diagonal, L column, U row
         s.f_pos[s.perm[k]] = apos++;
         place(k, k);
         for (int p = c0; p < c0 + c; ++p) place(s.coli[p], k);
         for (int p = c0; p < c0 + c; ++p) place(k, s.coli[p]);
-        // sources: diagonal, b(k), L entries, U entries
         r.push_back(take(k, k));
         int bref = -1;
         if (bslot[k] >= 0) { bref = bslot[k]; consumed.push_back(bref); bslot[k] = -1; }
         r.push_back(bref);
         for (int p = c0; p < c0 + c; ++p) r.push_back(take(s.coli[p], k));
         for (int p = c0; p < c0 + c; ++p) r.push_back(take(k, s.coli[p]));
+        s.u_off[k] = upos;
         upos += c + 2;
-        // rank-1 update targets (i, j), i, j in struct_k, row-major
-        for (int pa_ = c0; pa_ < c0 + c; ++pa_)
-            for (int pb = c0; pb < c0 + c; ++pb) {
-                const long long kk = key(s.coli[pa_], s.coli[pb], d);
-                auto it = slot_of.find(kk);
-                if (it == slot_of.end()) {
-                    const int sl = sa.get();
-                    slot_of[kk] = sl;
-                    r.push_back(sl | kFirst);
-                } else {
-                    r.push_back(it->second);
-                }
-            }
-        s.n_updates += c * c;
-        // forward substitution targets b(i), i in struct_k
-        for (int p = c0; p < c0 + c; ++p) {
-            const int i = s.coli[p];
-            if (bslot[i] < 0) {
-                bslot[i] = sa.get();
-                r.push_back(bslot[i] | kFirst);
-            } else {
-                r.push_back(bslot[i]);
-            }
+    };
+    for (int k = 0; k < d; ++k) {
+        const int c0 = s.colp[k], c = s.colp[k + 1] - c0;
+        s.max_c = std::max(s.max_c, c);
+        bool pair = false;
+        if (k + 1 < d && c >= 1 && s.coli[c0] == k + 1) {
+            const int c1 = s.colp[k + 2] - s.colp[k + 1];
+            pair = c1 == c - 1 && c1 <= kFastC2 &&
+                   std::equal(s.coli.begin() + c0 + 1, s.coli.begin() + c0 + c, s.coli.begin() + s.colp[k + 1]);
         }
+        Step st{k, pair ? 2 : 1, pair ? c - 1 : c, {}};
+        std::vector<int>& r = st.r;
+        sources(k, r);
+        const int last = pair ? k + 1 : k;
+        if (pair) sources(k + 1, r);
+        const int l0 = s.colp[last], cl = s.colp[last + 1] - l0;
+        s.max_c = std::max(s.max_c, cl);
+        // update targets (i, j) in struct(last) x struct(last), row-major, then b(i)
+        for (int pa_ = l0; pa_ < l0 + cl; ++pa_)
+            for (int pb = l0; pb < l0 + cl; ++pb) alloc_target(s.coli[pa_], s.coli[pb], r);
+        for (int p = l0; p < l0 + cl; ++p) alloc_b(s.coli[p], r);
+        s.n_updates += pair ? 2 * cl * cl : cl * cl;
         for (int sl : consumed) sa.put(sl);
         consumed.clear();
+        s.step_w[k] = pair ? 2 : 1;
+        if (pair) s.step_w[k + 1] = 0;
+        steps.push_back(std::move(st));
+        if (pair) ++k;
     }
     s.u_off[d] = upos;
     s.a_len = apos;
     s.u_len = upos;
     s.slots_f = sa.hi;
+    s.n_steps = static_cast<int>(steps.size());
     if (!slot_of.empty()) throw std::logic_error("symbolic: update targets left unconsumed");
     // encode: refs as byte offsets; a missing source reads the zero slot
     const auto enc = [](int slot) { return slot * kRefUnit; };
     s.prog_f.clear();
     s.prog_f_off.assign(static_cast<size_t>(d) + 1, 0);
-    for (int k = 0; k < d; ++k) {
-        s.prog_f_off[k] = static_cast<int>(s.prog_f.size());
-        const int c = s.colp[k + 1] - s.colp[k];
-        const int c_next = k + 1 < d ? s.colp[k + 2] - s.colp[k + 1] : 0;
-        const std::vector<int>& r = refs[k];
+    for (size_t q = 0; q < steps.size(); ++q) {
+        const Step& st = steps[q];
+        s.prog_f_off[st.k] = static_cast<int>(s.prog_f.size());
+        if (st.w == 2) s.prog_f_off[st.k + 1] = s.prog_f_off[st.k];
+        const int c = st.c;
+        const int next = q + 1 < steps.size() ? rec_head(steps[q + 1].c, steps[q + 1].w) : 0;
+        const int ns = st.w == 2 ? frec2_src_words(c) : frec_src_words(c);
         const size_t at = s.prog_f.size();
-        s.prog_f.resize(at + static_cast<size_t>(frec_words(c)), enc(s.slots_f));
+        s.prog_f.resize(at + static_cast<size_t>(st.w == 2 ? frec2_words(c) : frec_words(c)), enc(s.slots_f));
         int* w = s.prog_f.data() + at;
-        w[0] = c | (c_next << 16);
-        for (int q = 0; q < 2 + 2 * c; ++q) w[1 + q] = enc(r[q] < 0 ? s.slots_f : (r[q] & kSlotMask));
-        const int rw = frec_row_words(c);
-        int* u = w + frec_src_words(c);
-        for (int a = 0; a < c; ++a)
-            for (int b = 0; b < c; ++b) u[a * rw + b] = enc(r[2 + 2 * c + a * c + b] & kSlotMask);
-        int* fb = u + c * rw;
-        for (int a = 0; a < c; ++a) fb[a] = enc(r[2 + 2 * c + c * c + a] & kSlotMask);
-        // padding of the target rows points at the trash slot
-        for (int a = 0; a < c; ++a)
-            for (int b = c; b < rw; ++b) u[a * rw + b] = enc(s.slots_f + 1);
-        for (int a = c; a < rw; ++a) fb[a] = enc(s.slots_f + 1);
+        w[0] = rec_head(c, st.w) | (next << 16);
+        const std::vector<int>& r = st.r;
+        const int nsrc = ns - 1;
+        for (int q2 = 0; q2 < nsrc; ++q2) w[1 + q2] = enc(r[q2] < 0 ? s.slots_f : (r[q2] & kSlotMask));
+        for (int i = 0; i < c * c + c; ++i) w[ns + i] = enc(r[nsrc + i] & kSlotMask);
     }
     s.prog_f_off[d] = static_cast<int>(s.prog_f.size());
     build_backward(s);
@@ -406,11 +431,21 @@ void build_backward(SymbolicLU& s)
         for (int p = s.colp[k]; p < s.colp[k + 1]; ++p) last[s.coli[p]] = std::min(last[s.coli[p]], k);
     SlotAlloc sa;
     std::vector<int> xslot(static_cast<size_t>(d), -1);
-    std::vector<std::vector<int>> recs;
-    recs.reserve(static_cast<size_t>(d));
-    for (int k = d - 1; k >= 0; --k) {
-        const int c0 = s.colp[k], c = s.colp[k + 1] - c0;
+    struct Rec {
+        int k, w, c;  // first pivot of the step, width, |struct| of its last pivot
         std::vector<int> r;
+    };
+    std::vector<Rec> recs;
+    recs.reserve(static_cast<size_t>(d));
+    // steps in reverse: a 2-column supernode (k, k+1) solves x(k+1), then x(k)
+    // with x(k+1) in registers
+    for (int kk = d - 1; kk >= 0; --kk) {
+        const bool pair = s.step_w[kk] == 0;  // kk is the second pivot of a pair
+        const int k = pair ? kk - 1 : kk;
+        const int last_p = pair ? kk : k;  // the pivot whose struct the step reads
+        const int c0 = s.colp[last_p], c = s.colp[last_p + 1] - c0;
+        Rec rec{k, pair ? 2 : 1, c, {}};
+        std::vector<int>& r = rec.r;
         std::vector<int> to_free;
         for (int p = c0; p < c0 + c; ++p) {
             const int j = s.coli[p];
@@ -418,31 +453,42 @@ void build_backward(SymbolicLU& s)
             r.push_back(xslot[j]);
             if (last[j] == k) to_free.push_back(xslot[j]);
         }
-        // every read of this pivot happens before x(k) is stored, so a slot
-        // freed here may hold x(k)
+        // every read of this step happens before its results are stored, so a
+        // slot freed here may hold one of them
         for (int sl : to_free) sa.put(sl);
-        int out = -1;
-        if (last[k] < k) {
-            out = sa.get();
-            xslot[k] = out;
-        }
-        r.push_back(out);
-        recs.push_back(std::move(r));
+        auto out_slot = [&](int j) {
+            int out = -1;
+            if (last[j] < k) {
+                out = sa.get();
+                xslot[j] = out;
+            }
+            r.push_back(out);
+        };
+        if (pair) out_slot(k + 1);  // x(k+1): read by k in registers; a slot only for earlier pivots
+        out_slot(k);
+        recs.push_back(std::move(rec));
+        kk = k;
     }
     s.slots_b = sa.hi;
     const auto enc = [](int slot) { return slot * kRefUnit; };
     s.prog_b.clear();
-    for (int q = 0; q < d; ++q) {
-        const int k = d - 1 - q;
-        const int c = s.colp[k + 1] - s.colp[k];
-        const int c_next = k > 0 ? s.colp[k] - s.colp[k - 1] : 0;
-        const std::vector<int>& r = recs[q];
+    for (size_t q = 0; q < recs.size(); ++q) {
+        const Rec& rec = recs[q];
+        const int c = rec.c;
+        const int next = q + 1 < recs.size() ? rec_head(recs[q + 1].c, recs[q + 1].w) : 0;
         const size_t at = s.prog_b.size();
-        s.prog_b.resize(at + static_cast<size_t>(brec_words(c)), enc(s.slots_b + 1));
+        const int nw = rec.w == 2 ? brec2_words(c) : brec_words(c);
+        s.prog_b.resize(at + static_cast<size_t>(nw), enc(s.slots_b + 1));
         int* w = s.prog_b.data() + at;
-        w[0] = c | (c_next << 16);
-        w[1] = s.perm[k];
-        for (size_t i = 0; i < r.size(); ++i) w[2 + i] = enc(r[i] < 0 ? s.slots_b + 1 : r[i]);
+        w[0] = rec_head(c, rec.w) | (next << 16);
+        if (rec.w == 2) {
+            w[1] = s.perm[rec.k + 1];
+            w[2] = s.perm[rec.k];
+            for (size_t i = 0; i < rec.r.size(); ++i) w[3 + i] = enc(rec.r[i] < 0 ? s.slots_b + 1 : rec.r[i]);
+        } else {
+            w[1] = s.perm[rec.k];
+            for (size_t i = 0; i < rec.r.size(); ++i) w[2 + i] = enc(rec.r[i] < 0 ? s.slots_b + 1 : rec.r[i]);
+        }
     }
 }
 

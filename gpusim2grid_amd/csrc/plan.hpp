@@ -34,6 +34,8 @@ struct SymbolicLU {
     int nA = 0;         // pattern entries of the (symmetrised) input present in A
     int nLU = 0;        // L + U + diagonal entries
     int n_updates = 0;  // sum over pivots of c_k^2 (rank-1 update terms)
+    int n_steps = 0;    // walk steps of the factorization (pivots, 2-column supernodes)
+    std::vector<int> step_w;  // per pivot: 1 single, 2 first of a 2-column supernode, 0 its second
     int max_c = 0;      // longest off-diagonal column
     int height = 0;     // elimination tree height
     std::vector<int> perm;     // pivot k -> input index

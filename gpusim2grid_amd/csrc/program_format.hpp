@@ -26,9 +26,10 @@ constexpr int kStreamWaves = 2 + kValueProducers;  // walker + value producers +
 
 // Programs (one record per pivot, padded to 4 words so the kernels read a
 // record with 16-byte loads):
-//   factorization  [c | c_next << 16, diag, b(k), L[c], U[c]]   sources
-//                  c rows of c refs                             rank-1 update targets (row-major)
-//                  c refs                                       forward-substitution targets b(i)
+//   factorization  [head | next head << 16, diag, b(k), L[c], U[c]]   sources
+//                  c rows of c refs                                   rank-1 update targets (row-major)
+//                  c refs                                             forward-substitution targets b(i)
+//                  (2-column supernodes: see frec2_words)
 //   backward       [c | c_next << 16, perm[k], x(j)[c], out]
 // c_next is the column length of the following record (0 after the last).
 // A ref is a 32-bit byte offset, slot * kRefUnit: the offset of the slot's
@@ -45,7 +46,18 @@ __host__ __device__ constexpr int round4(int v) { return (v + 3) / 4 * 4; }
 __host__ __device__ constexpr int frec_src_words(int c) { return 3 + 2 * c; }  // header + sources
 __host__ __device__ constexpr int frec_row_words(int c) { return c; }
 __host__ __device__ constexpr int frec_words(int c) { return round4(frec_src_words(c) + (c + 1) * c); }
+// 2-column supernode (pivots k, k+1; c = |struct(k+1)|): header, the sources of
+// pivot k (2 + 2 (c + 1)) and of pivot k+1 (2 + 2c), then c rows of c update
+// targets and the c forward-substitution targets
+__host__ __device__ constexpr int frec2_src_words(int c) { return 7 + 4 * c; }
+__host__ __device__ constexpr int frec2_words(int c) { return round4(frec2_src_words(c) + (c + 1) * c); }
+// record header field: c | (width - 1) << 15; a header word holds this
+// record's field and, in its high half, the next record's
+__host__ __device__ constexpr int rec_head(int c, int w) { return c | ((w - 1) << 15); }
+constexpr int kFastC2 = 6;          // longest struct(k+1) of a 2-column supernode
 __host__ __device__ constexpr int brec_words(int c) { return round4(3 + c); }
+// backward 2-column supernode: [head, perm[k+1], perm[k], x(S)[c], out(k+1), out(k)]
+__host__ __device__ constexpr int brec2_words(int c) { return round4(5 + c); }
 
 // row status (gpusim2grid contingency/row_status.hpp:33-43) plus "still iterating"
 constexpr int kActive = -1;

@@ -35,11 +35,13 @@ constexpr int kSpinLimit = 1 << 22;  // polls (each after s_sleep 2) before a wa
 // also wait for every outstanding global access of the wave (the consumer's
 // U-stream stores, the producer's loads), which is exactly the latency the
 // rings are there to hide.
+// (every flag is a wave-uniform value: read into a scalar register, so the
+// polling loops branch on the scalar unit instead of masking lanes)
 __device__ __forceinline__ int lds_load_acquire(const int* p)
 {
     const int v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    return v;
+    return __builtin_amdgcn_readfirstlane(v);
 }
 
 __device__ __forceinline__ void lds_store_release(int* p, int v)
@@ -147,7 +149,37 @@ struct Ring {
                 lds_store_release(&ctl->lo, lo);
             }
             if (ready < cf) ready = cf;  // chunks before the window are not needed
-            while (ready <= cl) wait(ready++);
+            if (ready > cl) return;
+            if (cl - ready < 3) {
+                wait_upto3(cl);
+            } else {
+                while (ready <= cl) wait(ready++);
+            }
+        }
+
+        // chunks [ready, cl], at most three: their tags are read together, so
+        // one LDS round trip covers the common case
+        __device__ void wait_upto3(int cl)
+        {
+            if (failed) return;
+            int spins = 0;
+            for (;;) {
+                const int r = ready;
+                const int t0 = lds_load_acquire(&ctl->tag[r & (NCH - 1)]);
+                const int t1 = lds_load_acquire(&ctl->tag[(r + 1) & (NCH - 1)]);
+                const int t2 = lds_load_acquire(&ctl->tag[(r + 2) & (NCH - 1)]);
+                const bool ok = t0 == r + 1 && (cl < r + 1 || t1 == r + 2) && (cl < r + 2 || t2 == r + 3);
+                if (ok) {
+                    ready = cl + 1;
+                    return;
+                }
+                if (lds_load_acquire(&ctl->abort) || ++spins > kSpinLimit) {
+                    lds_store_release(&ctl->abort, 1);
+                    failed = true;  // the results are discarded
+                    return;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
         }
 
         __device__ void wait(int c)

@@ -64,6 +64,7 @@ typedef struct {
     int32_t nnz;                             /* input pattern entries (nJ for grids)  */
     int32_t nLU, n_updates, max_c, height;   /* symbolic factorization                */
     int32_t slots_f, slots_b, a_len, u_len;  /* program resources                     */
+    int32_t n_steps;                         /* factorization walk steps (pivots and 2-column supernodes) */
 } g2g_plan_info;
 
 typedef struct {
