@@ -152,6 +152,127 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
             }
 }
 
+// Forward GEMM with both operands K-contiguous: C = A Bt^T, A: M x K (row
+// m at A + m lda), Bt: N x K (row n at B + n ldb), K a multiple of 8 and rows
+// 16-byte aligned.  128 x 128 output tile per workgroup of 4 waves (each
+// 64 x 64 as 2 x 2 MFMA 32x32x16 tiles); K in steps of 64 through a
+// double-buffered LDS stage, the next step's global loads in flight while
+// the current one is multiplied.
+constexpr int kFK = 64, kFPad = 8;
+
+__device__ __forceinline__ void kc_load(const __bf16* base, long long ld, int rows, int r0, int k0, int K, int t,
+                                        u32x4 (&reg)[4])
+{
+    const int kc = (t & 7) * 8;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = r0 + (t >> 3) + 32 * i;
+        const int k = k0 + kc;
+        if (r < rows && k < K) reg[i] = *reinterpret_cast<const u32x4*>(base + static_cast<long long>(r) * ld + k);
+        else reg[i] = u32x4{0u, 0u, 0u, 0u};
+    }
+}
+
+__device__ __forceinline__ void kc_store(__bf16 (*s)[kFK + kFPad], int t, const u32x4 (&reg)[4])
+{
+    const int kc = (t & 7) * 8;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) *reinterpret_cast<u32x4*>(&s[(t >> 3) + 32 * i][kc]) = reg[i];
+}
+
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm_kc(GemmArgs g)
+{
+    __shared__ __align__(16) __bf16 sA[2][kBM][kFK + kFPad];
+    __shared__ __align__(16) __bf16 sB[2][kBN][kFK + kFPad];
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = t >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+    const int lr = lane & 31, lk = (lane >> 5) * 8;
+    u32x4 ra[4], rb[4];
+    kc_load(g.A, g.lda, g.M, m0, 0, g.K, t, ra);
+    kc_load(g.B, g.ldb, g.N, n0, 0, g.K, t, rb);
+    kc_store(sA[0], t, ra);
+    kc_store(sB[0], t, rb);
+    __syncthreads();
+    int cur = 0;
+    for (int k0 = 0; k0 < g.K; k0 += kFK) {
+        const bool more = k0 + kFK < g.K;
+        if (more) {
+            kc_load(g.A, g.lda, g.M, m0, k0 + kFK, g.K, t, ra);
+            kc_load(g.B, g.ldb, g.N, n0, k0 + kFK, g.K, t, rb);
+        }
+#pragma unroll
+        for (int kk = 0; kk < kFK; kk += 16) {
+            bf16x8 fa[2], fb[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                fa[i] = *reinterpret_cast<const bf16x8*>(&sA[cur][wm * 64 + i * 32 + lr][kk + lk]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                fb[j] = *reinterpret_cast<const bf16x8*>(&sB[cur][wn * 64 + j * 32 + lr][kk + lk]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) {
+            kc_store(sA[cur ^ 1], t, ra);
+            kc_store(sB[cur ^ 1], t, rb);
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int n = n0 + wn * 64 + j * 32 + lr;
+                if (m >= g.M || n >= g.N) continue;
+                const float v = acc[i][j][r];
+                const long long mc = static_cast<long long>(m);
+                if (EPI == kEpiF32) {
+                    g.C[mc * g.ldc + n] = v;
+                } else if (EPI == kEpiBiasF32) {
+                    g.C[mc * g.ldc + n] = v + g.bias[n];
+                } else if (EPI == kEpiBiasGeluBf16) {
+                    const float z = v + g.bias[n];
+                    if (g.Z) g.Z[mc * g.ldz + n] = z;
+                    g.H[mc * g.ldh + n] = static_cast<__bf16>(gelu(z));
+                }
+            }
+}
+
+// Wt (N x K, bf16) = transpose of W (K x N, fp32), 32 x 32 tiles through LDS
+__global__ void k_transpose_bf16(const float* __restrict__ W, int K, int N, __bf16* __restrict__ Wt)
+{
+    __shared__ float tile[32][33];
+    const int k0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        const int k = k0 + r, n = n0 + tx;
+        tile[r][tx] = (k < K && n < N) ? W[static_cast<long long>(k) * N + n] : 0.0f;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int n = n0 + r, k = k0 + tx;
+        if (n < N && k < K) Wt[static_cast<long long>(n) * K + k] = static_cast<__bf16>(tile[tx][r]);
+    }
+}
+
 // column sums of an fp32 M x N matrix (bias gradients), one thread per column
 __global__ void k_colsum(const float* __restrict__ X, long long ldx, int M, int N, float* __restrict__ out)
 {
@@ -289,6 +410,9 @@ __global__ void k_warm_v_tiles(const float* __restrict__ Y, long long ldy, int M
     V_tiles[tiled(row0 + r, n, b)] = make_double2(vm * cs, vm * sn);
 }
 
+template __global__ void k_gemm_kc<kEpiF32>(GemmArgs);
+template __global__ void k_gemm_kc<kEpiBiasF32>(GemmArgs);
+template __global__ void k_gemm_kc<kEpiBiasGeluBf16>(GemmArgs);
 template __global__ void k_gemm<false, false, kEpiF32>(GemmArgs);
 template __global__ void k_gemm<false, false, kEpiBiasF32>(GemmArgs);
 template __global__ void k_gemm<false, false, kEpiBiasGeluBf16>(GemmArgs);

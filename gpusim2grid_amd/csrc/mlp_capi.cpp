@@ -25,6 +25,8 @@ struct g2g_mlp_s {
     float* v = nullptr;
     float* g = nullptr;
     __bf16* wb = nullptr;  // bf16 copy of all parameters (the GEMMs read the weight parts)
+    __bf16* wt = nullptr;  // bf16 transposed weights (N x K) for the K-contiguous forward GEMM
+    bool kc = false;       // every width a multiple of 8: the forward uses k_gemm_kc
     int step = 0;
     // normalisation (features: z-score per input; outputs: per output)
     float* in_mu = nullptr;
@@ -57,6 +59,23 @@ struct g2g_mlp_s {
     __bf16* W1b() const { return wb + o_W1; }
     __bf16* W2b() const { return wb + o_W2; }
     __bf16* W3b() const { return wb + o_W3; }
+    __bf16* W1t() const { return wt + o_W1; }
+    __bf16* W2t() const { return wt + o_W2; }
+    __bf16* W3t() const { return wt + o_W3; }
+
+    // refresh the transposed bf16 weights from the fp32 masters
+    void transpose_weights(hipStream_t st)
+    {
+        if (!kc) return;
+        auto tr = [&](size_t off, int K, int N) {
+            const dim3 grid((N + 31) / 32, (K + 31) / 32);
+            k_transpose_bf16<<<grid, 256, 0, st>>>(w + off, K, N, wt + off);
+        };
+        tr(o_W1, n_in, H);
+        tr(o_W2, H, H);
+        tr(o_W3, H, n_out);
+        HIPCHK(hipGetLastError());
+    }
 
     void workspace(int M)
     {
@@ -112,9 +131,29 @@ void stage_x(g2g_mlp_s* p, const float* X, int M, int on_device)
     if (!on_device) HIPCHK(hipStreamSynchronize(p->stream));
 }
 
-// forward of the staged X into p->Y
-void forward(g2g_mlp_s* p, int M)
+template <int EPI>
+void gemm_kc(hipStream_t st, const __bf16* A, long long lda, const __bf16* Bt, long long ldb, int M, int N, int K,
+             float* C, long long ldc, __bf16* Hh, long long ldh, float* Z, long long ldz, const float* bias)
 {
+    GemmArgs a{A, lda, Bt, ldb, M, N, K, C, ldc, Hh, ldh, Z, ldz, bias};
+    const dim3 grid((N + 127) / 128, (M + 127) / 128);
+    k_gemm_kc<EPI><<<grid, 256, 0, st>>>(a);
+    HIPCHK(hipGetLastError());
+}
+
+// forward of the staged X into p->Y; keep_z: the pre-activations the backward
+// pass needs (inference skips them)
+void forward(g2g_mlp_s* p, int M, bool keep_z = true)
+{
+    if (p->kc) {
+        gemm_kc<kEpiBiasGeluBf16>(p->stream, p->X, p->n_in, p->W1t(), p->n_in, M, p->H, p->n_in, nullptr, 0, p->H1,
+                                  p->H, keep_z ? p->Z1 : nullptr, p->H, p->w + p->o_b1);
+        gemm_kc<kEpiBiasGeluBf16>(p->stream, p->H1, p->H, p->W2t(), p->H, M, p->H, p->H, nullptr, 0, p->H2, p->H,
+                                  keep_z ? p->Z2 : nullptr, p->H, p->w + p->o_b2);
+        gemm_kc<kEpiBiasF32>(p->stream, p->H2, p->H, p->W3t(), p->H, M, p->n_out, p->H, p->Y, p->n_out, nullptr, 0,
+                             nullptr, 0, p->w + p->o_b3);
+        return;
+    }
     gemm<false, false, kEpiBiasGeluBf16>(p->stream, p->X, p->n_in, p->W1b(), p->H, M, p->H, p->n_in, nullptr, 0,
                                          p->H1, p->H, p->Z1, p->H, p->w + p->o_b1);
     gemm<false, false, kEpiBiasGeluBf16>(p->stream, p->H1, p->H, p->W2b(), p->H, M, p->H, p->H, nullptr, 0, p->H2,
@@ -176,7 +215,7 @@ void mlp_warm_start_tiles(g2g_mlp_s* p, const double2* S_tiles, int M, int n, co
     k_features_tiles<<<blocks_for(nf, 256), 256, 0, st>>>(S_tiles, M, n, p->in_mu, p->in_sigma, d_mask, n_mask, p->X,
                                                           p->n_in);
     HIPCHK(hipGetLastError());
-    forward(p, M);
+    forward(p, M, false);
     const long long nv = static_cast<long long>(M) * n;
     k_warm_v_tiles<<<blocks_for(nv, 256), 256, 0, st>>>(p->Y, p->n_out, M, n, p->out_mu, p->out_sigma, p->vfix,
                                                         p->kind, V_tiles, 0);
@@ -212,6 +251,8 @@ int g2g_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_out, uint64_t seed,
         p->v = p->own.get<float>(o);
         p->g = p->own.get<float>(o);
         p->wb = p->own.get<__bf16>(o);
+        p->kc = n_in % 8 == 0 && n_hidden % 8 == 0 && n_out % 8 == 0;
+        if (p->kc) p->wt = p->own.get<__bf16>(o);
         // Glorot-uniform weights, zero biases (seeded)
         std::vector<float> h(o, 0.0f);
         std::mt19937_64 rng(seed);
@@ -228,6 +269,7 @@ int g2g_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_out, uint64_t seed,
         HIPCHK(hipMemset(p->v, 0, o * sizeof(float)));
         k_f32_to_bf16<<<blocks_for(static_cast<long long>(o), 256), 256>>>(p->w, static_cast<long long>(o), p->wb);
         HIPCHK(hipGetLastError());
+        p->transpose_weights(nullptr);
         HIPCHK(hipDeviceSynchronize());
         *out = p.release();
     });
@@ -276,6 +318,7 @@ int g2g_mlp_set_params(g2g_mlp_t p, const float* in, int64_t n)
         HIPCHK(hipMemcpy(p->w, in, p->n_params * sizeof(float), hipMemcpyHostToDevice));
         k_f32_to_bf16<<<blocks_for(static_cast<long long>(n), 256), 256, 0, p->stream>>>(p->w, n, p->wb);
         HIPCHK(hipGetLastError());
+        p->transpose_weights(p->stream);
         HIPCHK(hipStreamSynchronize(p->stream));
     });
 }
@@ -316,7 +359,7 @@ int g2g_mlp_forward(g2g_mlp_t p, const float* X, int32_t M, float* Y, int32_t on
         if (M <= 0) throw std::invalid_argument("M must be positive");
         p->workspace(M);
         stage_x(p, X, M, on_device);
-        forward(p, M);
+        forward(p, M, false);
         HIPCHK(hipMemcpyAsync(Y, p->Y, static_cast<size_t>(M) * p->n_out * sizeof(float),
                               on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, p->stream));
         HIPCHK(hipStreamSynchronize(p->stream));
@@ -359,6 +402,7 @@ int g2g_mlp_train_step(g2g_mlp_t p, const float* X, const float* T, int32_t M, f
         const long long n = static_cast<long long>(p->n_params);
         k_adam<<<blocks_for(n, 256), 256, 0, p->stream>>>(p->w, p->m, p->v, p->g, n, lr, b1, b2, 1e-8f, c1, c2, p->wb);
         HIPCHK(hipGetLastError());
+        p->transpose_weights(p->stream);
         if (loss) {
             HIPCHK(hipMemcpyAsync(loss, p->loss, sizeof(float), hipMemcpyDeviceToHost, p->stream));
             HIPCHK(hipStreamSynchronize(p->stream));
@@ -381,6 +425,8 @@ int g2g_gemm_bf16(int32_t trans_a, int32_t trans_b, int32_t M, int32_t N, int32_
         HIPCHK(hipMemcpy(dA, A, na * 2, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(dB, B, nb * 2, hipMemcpyHostToDevice));
         if (!trans_a && !trans_b) gemm<false, false, kEpiF32>(nullptr, dA, lda, dB, ldb, M, N, K, dC, ldc);
+        else if (!trans_a && trans_b && K % 8 == 0 && lda % 8 == 0 && ldb % 8 == 0)
+            gemm_kc<kEpiF32>(nullptr, dA, lda, dB, ldb, M, N, K, dC, ldc, nullptr, 0, nullptr, 0, nullptr);
         else if (!trans_a && trans_b) gemm<false, true, kEpiF32>(nullptr, dA, lda, dB, ldb, M, N, K, dC, ldc);
         else if (trans_a && !trans_b) gemm<true, false, kEpiF32>(nullptr, dA, lda, dB, ldb, M, N, K, dC, ldc);
         else gemm<true, true, kEpiF32>(nullptr, dA, lda, dB, ldb, M, N, K, dC, ldc);

@@ -220,7 +220,8 @@ int g2g_mlp_loss_grad(g2g_mlp_t mlp, const float *X, const float *T, int32_t M, 
 int g2g_mlp_train_step(g2g_mlp_t mlp, const float *X, const float *T, int32_t M, float lr, float *loss,
                        int32_t on_device);
 /* C = op(A) op(B) with bf16 operands (raw bit patterns) and fp32 result, on the
- * MLP's MFMA kernel (test entry; host arrays) */
+ * MLP's MFMA kernels (test entry; host arrays).  op(A) = A, op(B) = B^T with
+ * K, lda and ldb multiples of 8 runs the forward pass's K-contiguous kernel. */
 int g2g_gemm_bf16(int32_t trans_a, int32_t trans_b, int32_t M, int32_t N, int32_t K, const uint16_t *A, int64_t lda,
                   const uint16_t *B, int64_t ldb, float *C, int64_t ldc);
 
