@@ -150,29 +150,30 @@ struct Ring {
             }
             if (ready < cf) ready = cf;  // chunks before the window are not needed
             if (ready > cl) return;
-            if (cl - ready < 3) {
-                wait_upto3(cl);
-            } else {
-                while (ready <= cl) wait(ready++);
-            }
+            wait_for(cl);
         }
 
-        // chunks [ready, cl], at most three: their tags are read together, so
-        // one LDS round trip covers the common case
-        __device__ void wait_upto3(int cl)
+        // Makes chunks up to cl readable.  One poll reads every slot's tag at
+        // once and advances `ready` over all consecutive chunks already
+        // written, so a poll also covers the chunks the next windows need.
+        __device__ void wait_for(int cl)
         {
             if (failed) return;
             int spins = 0;
             for (;;) {
-                const int r = ready;
-                const int t0 = lds_load_acquire(&ctl->tag[r & (NCH - 1)]);
-                const int t1 = lds_load_acquire(&ctl->tag[(r + 1) & (NCH - 1)]);
-                const int t2 = lds_load_acquire(&ctl->tag[(r + 2) & (NCH - 1)]);
-                const bool ok = t0 == r + 1 && (cl < r + 1 || t1 == r + 2) && (cl < r + 2 || t2 == r + 3);
-                if (ok) {
-                    ready = cl + 1;
-                    return;
+                int t[NCH];
+#pragma unroll
+                for (int i = 0; i < NCH; ++i) t[i] = lds_load_acquire(&ctl->tag[i]);
+#pragma unroll
+                for (int i = 0; i < NCH; ++i) {
+                    // chunk ready + i sits in slot (ready + i) & (NCH - 1)
+                    int tv = t[0];
+#pragma unroll
+                    for (int q = 1; q < NCH; ++q) tv = ((ready & (NCH - 1)) == q) ? t[q] : tv;
+                    if (tv != ready + 1 || ready >= lo + NCH) break;
+                    ++ready;
                 }
+                if (ready > cl) return;
                 if (lds_load_acquire(&ctl->abort) || ++spins > kSpinLimit) {
                     lds_store_release(&ctl->abort, 1);
                     failed = true;  // the results are discarded
