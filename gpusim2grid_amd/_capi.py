@@ -77,7 +77,8 @@ def lib():
         "g2g_batch_get_v": (ctypes.c_int, [_P, _P, i64, i32]),
         "g2g_batch_get_results": (ctypes.c_int, [_P, _P, _P, _P]),
         "g2g_batch_eval": (ctypes.c_int, [_P, _P, _P, _P]),
-        "g2g_lu_solve_batch": (ctypes.c_int, [_P, i32, i32, _P, _P, _P, dbl, _P, _P]),
+        "g2g_lu_solve_batch": (ctypes.c_int, [_P, i32, i32, _P, _P, _P, dbl, i32, _P, _P]),
+        "g2g_batch_set_step_control": (ctypes.c_int, [_P, i32, i32, dbl, dbl]),
         "g2g_batch_device_arrays": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
         "g2g_batch_warm_start": (ctypes.c_int, [_P, _P, _P, i32]),
         "g2g_batch_branch_flows": (ctypes.c_int, [_P, i32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
@@ -178,15 +179,15 @@ class Plan:
         _check(lib().g2g_plan_get_array(self._h, ARR[name], _ptr(out), out.size, ctypes.byref(n)))
         return out
 
-    def lu_solve(self, vals, rhs, precision=FP64, pivot_eps=None, stream=None):
+    def lu_solve(self, vals, rhs, precision=FP64, pivot_eps=None, ir_steps=0, stream=None):
         vals = np.ascontiguousarray(np.atleast_2d(vals), dtype=np.float64)
         rhs = np.ascontiguousarray(np.atleast_2d(rhs), dtype=np.float64)
         rows = vals.shape[0]
         x = np.empty_like(rhs)
         cl = np.empty(rows, np.int32)
         eps = (1e-5 if precision == MIXED else 1e-12) if pivot_eps is None else pivot_eps
-        _check(lib().g2g_lu_solve_batch(self._h, rows, precision, _ptr(vals), _ptr(rhs), _ptr(x), eps, _ptr(cl),
-                                        stream))
+        _check(lib().g2g_lu_solve_batch(self._h, rows, precision, _ptr(vals), _ptr(rhs), _ptr(x), eps,
+                                        int(ir_steps), _ptr(cl), stream))
         return x, cl
 
 
@@ -234,6 +235,11 @@ class Batch:
 
     def set_active_rows(self, k):
         _check(lib().g2g_batch_set_active_rows(self._h, int(k)))
+
+    def set_step_control(self, ir_steps=0, scaling=False, max_dva=0.5, max_dvm=0.1):
+        """Refinement rounds per Newton step and per-row step scaling of the next solves."""
+        _check(lib().g2g_batch_set_step_control(self._h, int(ir_steps), 1 if scaling else 0, float(max_dva),
+                                                float(max_dvm)))
 
     def solve(self, max_iter=10, tol=1e-8, scheduling=CONTINUOUS):
         st = SolveStats()

@@ -187,6 +187,21 @@ struct Batch {
 
 void add(TimingEntry& t, double gpu_ms) { t.gpu_ms += gpu_ms; t.wall_ms += gpu_ms; }
 
+// refinement rounds and step scaling of a session's solves (ir_n_steps,
+// scaling_max_voltage_change, max_dVa, max_dVm: python_bindings.cpp:1369-1560)
+struct StepControl {
+    int ir_steps = 0;
+    bool scaling = false;
+    double max_dva = 0.5, max_dvm = 0.1;
+    StepControl() = default;
+    StepControl(int ir, bool sc, double dva, double dvm) : ir_steps(ir), scaling(sc), max_dva(dva), max_dvm(dvm)
+    {
+        if (ir < 0) throw py::value_error("ir_n_steps must be >= 0");
+        if (sc && !(dva > 0.0 && dvm > 0.0)) throw py::value_error("max_dVa and max_dVm must be positive");
+    }
+    void apply(g2g_batch_t h) const { check(g2g_batch_set_step_control(h, ir_steps, scaling ? 1 : 0, max_dva, max_dvm)); }
+};
+
 // ---- the solve of many rows, chunked by batch_size -----------------------
 struct RowJob {
     const cplx* S = nullptr;       // rows x n, or one shared row (S_stride 0)
@@ -204,6 +219,7 @@ struct RowJob {
     uintptr_t mlp = 0;             // warm start (0: none)
     int k_warm = 4;
     double growth = 2.0;
+    StepControl step;
 };
 
 struct RowResults {
@@ -248,6 +264,7 @@ void run_rows(const Grid& g, std::unique_ptr<Batch>& batch, int precision, int b
             std::copy_n(job.V0 + r * job.V_stride, n, Vbuf.begin() + static_cast<size_t>(k) * n);
         }
         check(g2g_batch_set_active_rows(batch->h, m));
+        job.step.apply(batch->h);
         check(g2g_batch_set_sbus(batch->h, reinterpret_cast<const double*>(Sbuf.data()), n, m, 0, 0));
         check(g2g_batch_set_v(batch->h, reinterpret_cast<const double*>(Vbuf.data()), n, m, 0, 0));
         if (per_row_y) {
@@ -301,14 +318,6 @@ py::array_t<T> to_array(const std::vector<T>& v)
     return a;
 }
 
-void unsupported_options(int ir_n_steps, bool scaling_max_voltage_change)
-{
-    if (ir_n_steps != 0)
-        throw py::value_error("ir_n_steps > 0 is not supported by this build (Newton's own iteration corrects the "
-                              "linear solve; the mixed-precision mode keeps F in FP64)");
-    if (scaling_max_voltage_change)
-        throw py::value_error("scaling_max_voltage_change (step damping) is not supported by this build");
-}
 
 void set_device(int device)
 {
@@ -319,11 +328,9 @@ void set_device(int device)
 class AcPfNrSession {
 public:
     AcPfNrSession(py::object Ybus, CArr Vinit, CArr Sbus, IArr slack_ids, DArr /*slack_weights*/, IArr pv, IArr pq,
-                  int max_iter, double tol, int device, bool presolved_v, int ir_n_steps,
-                  bool scaling_max_voltage_change, Precision precision)
+                  int max_iter, double tol, int device, bool presolved_v, StepControl step, Precision precision)
         : max_iter_(max_iter), tol_(tol), precision_(static_cast<int>(precision))
     {
-        unsupported_options(ir_n_steps, scaling_max_voltage_change);
         set_device(device);
         const double t0 = now_ms();
         grid_ = std::make_unique<Grid>(Ybus, slack_ids, pv, pq);
@@ -332,6 +339,7 @@ public:
         S_.assign(Sbus.data(), Sbus.data() + n);
         V_.assign(Vinit.data(), Vinit.data() + n);
         batch_ = std::make_unique<Batch>(*grid_, 1, precision_, 0);
+        step.apply(batch_->h);
         timings_.t_analyze_ms = grid_->t_analyze_ms;
         timings_.t_init_ms = now_ms() - t0;
         if (presolved_v) {
@@ -579,11 +587,10 @@ struct Branches {
 class BatchSession {
 public:
     BatchSession(py::object Ybus, CArr Vinit, CArr Sbus, IArr slack_ids, IArr pv, IArr pq, int batch_size,
-                 int nb_iter, int max_iter_base, double tol_base, int device, bool presolved_v, int ir_n_steps,
-                 bool scaling_max_voltage_change, Precision precision)
-        : batch_size_(batch_size), nb_iter_(nb_iter), precision_(static_cast<int>(precision))
+                 int nb_iter, int max_iter_base, double tol_base, int device, bool presolved_v, StepControl step,
+                 Precision precision)
+        : batch_size_(batch_size), nb_iter_(nb_iter), precision_(static_cast<int>(precision)), step_(step)
     {
-        unsupported_options(ir_n_steps, scaling_max_voltage_change);
         if (batch_size <= 0) throw py::value_error("batch_size must be positive");
         if (nb_iter < 0) throw py::value_error("nb_iter must be >= 0");
         set_device(device);
@@ -598,6 +605,7 @@ public:
         {
             py::gil_scoped_release nogil;
             Batch b(*grid_, 1, G2G_FP64, 0);
+            step_.apply(b.h);
             check(g2g_batch_set_sbus(b.h, reinterpret_cast<const double*>(S_base_.data()), n, 1, 0, 0));
             check(g2g_batch_set_v(b.h, reinterpret_cast<const double*>(V_base_.data()), n, 1, 0, 0));
             int32_t st = 0, it = 0;
@@ -664,6 +672,7 @@ public:
         job.mlp = mlp_;
         job.k_warm = k_warm_;
         job.growth = growth_;
+        job.step = step_;
         BatchTimings t = timings_;
         t.t_mismatch = t.t_refactorize = t.t_solve = t.t_warm_start = TimingEntry{};
         t.n_chunks = 0;
@@ -787,6 +796,7 @@ protected:
     int batch_size_;
     int nb_iter_;
     int precision_;
+    StepControl step_;
     BatchScheduling sched_ = BatchScheduling::Chunked;
     double tol_ = 1e-8;
     uintptr_t mlp_ = 0;
@@ -1038,11 +1048,11 @@ PYBIND11_MODULE(_gpusim2grid, m)
     py::class_<AcPfNrSession, std::shared_ptr<AcPfNrSession>>(m, "AcPfNrSession")
         .def(py::init([](py::object Ybus, CArr Vinit, CArr Sbus, IArr slack_ids, DArr slack_weights, IArr pv,
                          IArr pq, int max_iter, double tol, int device, bool presolved_v, ReorderingAlg,
-                         MatchingAlg, PivotEpsilonAlg, int ir_n_steps, bool, bool scaling, double, double,
-                         Precision precision) {
+                         MatchingAlg, PivotEpsilonAlg, int ir_n_steps, bool, bool scaling, double max_dva,
+                         double max_dvm, Precision precision) {
                  return std::make_shared<AcPfNrSession>(Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq,
-                                                        max_iter, tol, device, presolved_v, ir_n_steps, scaling,
-                                                        precision);
+                                                        max_iter, tol, device, presolved_v,
+                                                        StepControl(ir_n_steps, scaling, max_dva, max_dvm), precision);
              }),
              py::arg("Ybus"), py::arg("Vinit"), py::arg("Sbus"), py::arg("slack_ids"), py::arg("slack_weights"),
              py::arg("pv"), py::arg("pq"), py::arg("max_iter"), py::arg("tol"), py::arg("device") = -1,
@@ -1072,7 +1082,7 @@ PYBIND11_MODULE(_gpusim2grid, m)
         [](py::array_t<cplx> out, py::object Ybus, CArr V, CArr Sbus, IArr slack_ids, DArr slack_weights, IArr pv,
            IArr pq, int max_iter, double tol, int nb_solve, int device) {
             if (nb_solve < 1) throw py::value_error("nb_solve must be >= 1");
-            AcPfNrSession s(Ybus, V, Sbus, slack_ids, slack_weights, pv, pq, max_iter, tol, device, false, 0, false,
+            AcPfNrSession s(Ybus, V, Sbus, slack_ids, slack_weights, pv, pq, max_iter, tol, device, false, StepControl(),
                             Precision::Fp64);
             for (int k = 1; k < nb_solve; ++k) s.resolve(V, py::none(), -1, -1.0);
             CArr v = s.get_v();
@@ -1118,9 +1128,10 @@ PYBIND11_MODULE(_gpusim2grid, m)
         return py::init([](py::object Ybus, CArr Vinit, CArr Sbus, IArr slack_ids, DArr, IArr pv, IArr pq,
                            int batch_size, int nb_iter, int max_iter_base, double tol_base, int device,
                            bool presolved_v, ReorderingAlg, MatchingAlg, PivotEpsilonAlg, int ir_n_steps, bool,
-                           bool scaling, double, double, Precision precision) {
+                           bool scaling, double max_dva, double max_dvm, Precision precision) {
             return std::make_shared<S>(Ybus, Vinit, Sbus, slack_ids, pv, pq, batch_size, nb_iter, max_iter_base,
-                                       tol_base, device, presolved_v, ir_n_steps, scaling, precision);
+                                       tol_base, device, presolved_v, StepControl(ir_n_steps, scaling, max_dva, max_dvm),
+                                       precision);
         });
     };
 #define G2G_BATCH_ARGS                                                                                          \
@@ -1170,7 +1181,7 @@ PYBIND11_MODULE(_gpusim2grid, m)
            DArr q_mvar, double sn_mva, int batch_size, int nb_iter, int max_iter_base, double tol_base,
            ContingencySolverType, int) {
             auto s = std::make_shared<InjectionSweepSession>(Ybus, Vinit, Sbus, slack_ids, pv, pq, batch_size,
-                                                             nb_iter, max_iter_base, tol_base, -1, false, 0, false,
+                                                             nb_iter, max_iter_base, tol_base, -1, false, StepControl(),
                                                              Precision::Fp64);
             s->set_injections(p_mw, q_mvar, sn_mva);
             {
@@ -1208,8 +1219,8 @@ PYBIND11_MODULE(_gpusim2grid, m)
             pq.mutable_data()[0] = 1;
             DArr w(1);
             w.mutable_data()[0] = 1.0;
-            AcPfNrSession s(Y, V, S, sl, w, pv, pq, 10, 1e-8, device, false, 0, false, Precision::Fp64);
-            AcPfNrSession s2(Y, V, S, sl, w, pv, pq, 10, 1e-8, device, false, 0, false, Precision::Mixed);
+            AcPfNrSession s(Y, V, S, sl, w, pv, pq, 10, 1e-8, device, false, StepControl(), Precision::Fp64);
+            AcPfNrSession s2(Y, V, S, sl, w, pv, pq, 10, 1e-8, device, false, StepControl(), Precision::Mixed);
         },
         py::arg("device") = -1);
     m.attr("is_fp32") = false;

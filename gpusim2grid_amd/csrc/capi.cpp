@@ -25,6 +25,10 @@ template <typename Real>
 __global__ void k_backward(BackwardArgs<Real> a);
 template <typename Real>
 __global__ void k_update(UpdateArgs<Real> a);
+template <typename Real>
+__global__ void k_lin_residual(RefineArgs<Real> a);
+template <typename Real>
+__global__ void k_axpy_tiles(const Real* Xs, Real* X, const int* tile_active, int d);
 template <typename T>
 __global__ void k_rows_to_tiles(const T* in, long long ld, int rows, int m, int row_offset, T* out, int broadcast);
 template <typename T>
@@ -70,13 +74,16 @@ struct g2g_plan_s {
     int* d_fpos = nullptr;
     int* d_upd = nullptr;     // per bus {bus, theta unknown, |V| unknown}
     int n_upd = 0;
+    std::vector<int> raw_p, raw_i;  // a raw plan's matrix pattern (CSR)
+    int* d_Jp = nullptr;      // the matrix CSR in unknown labels (refinement residuals)
+    int* d_Ji = nullptr;
     int prog_f_stride = 0, prog_b_stride = 0;
     bool uploaded = false;
     const SymbolicLU& lu() const { return grid ? nr.lu : raw; }
     ~g2g_plan_s()
     {
         dfree(d_prog_f); dfree(d_prog_b); dfree(d_k1_bus); dfree(d_k1_ent);
-        dfree(d_Y); dfree(d_perm); dfree(d_apos); dfree(d_fpos); dfree(d_upd);
+        dfree(d_Y); dfree(d_perm); dfree(d_apos); dfree(d_fpos); dfree(d_upd); dfree(d_Jp); dfree(d_Ji);
     }
     // device copies of the tables, made on first use (plans can be built and
     // inspected on a host without a GPU)
@@ -100,6 +107,8 @@ struct g2g_plan_s {
         d_perm = dupload(s.perm);
         d_apos = dupload(s.a_pos_of_entry);
         d_fpos = dupload(s.f_pos);
+        d_Jp = dupload(grid ? nr.Jp : raw_p);
+        d_Ji = dupload(grid ? nr.Ji : raw_i);
         if (grid) {
             d_k1_bus = dupload(nr.k1_bus);
             d_k1_ent = dupload(nr.k1_ent);
@@ -148,13 +157,18 @@ struct g2g_batch_s {
     int* it_save = nullptr;
     double* res_save = nullptr;
     int* row_map = nullptr;
+    // step control (settings of the next solve)
+    int ir_steps = 0;                 // refinement steps of every Newton step
+    double max_dva = 0.0, max_dvm = 0.0;  // step scaling bounds (max_dva <= 0: off)
+    void* Bs = nullptr;               // refinement: saved rhs and step [T][d][64]
+    void* Xs = nullptr;
     std::vector<hipEvent_t> ev;
     ~g2g_batch_s()
     {
         dfree(V); dfree(S); dfree(Yrow); dfree(A); dfree(U); dfree(X); dfree(res); dfree(jmax);
         dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped); dfree(error);
         dfree(V0); dfree(Vc); dfree(Sc); dfree(Yc); dfree(st_save); dfree(it_save); dfree(res_save);
-        dfree(row_map);
+        dfree(row_map); dfree(Bs); dfree(Xs);
         if (h_active) (void)hipHostFree(h_active);
         for (auto e : ev) (void)hipEventDestroy(e);
     }
@@ -232,12 +246,34 @@ void launch_backward(const g2g_plan_s* P, const Real* U, int u_stride, const int
     HIPCHK(hipGetLastError());
 }
 
+// `steps` rounds of x += (LU)^-1 (b - A x) on the tiles' A streams: the
+// residual overwrites the A stream's right-hand side, and the factors are
+// recomputed from the unchanged matrix values (bitwise the same factors).
+template <typename Real>
+void refine(const g2g_plan_s* P, Real* A, Real* U, Real* X, Real* Bs, Real* Xs, int a_stride, int u_stride,
+            const int* tile_active, const double* jmax, int* clamped, int* error, int rows, int T, double eps,
+            int steps, hipStream_t st)
+{
+    const SymbolicLU& s = P->lu();
+    for (int k = 0; k < steps; ++k) {
+        RefineArgs<Real> ra{A, X, Bs, Xs, P->d_Jp, P->d_Ji, P->d_apos, P->d_fpos, tile_active, s.d, a_stride,
+                            k == 0 ? 1 : 0};
+        k_lin_residual<Real><<<T, kLanes * kMisWaves, 0, st>>>(ra);
+        HIPCHK(hipGetLastError());
+        launch_factor<Real>(P, A, U, a_stride, u_stride, tile_active, jmax, clamped, error, rows, T, eps, st);
+        launch_backward<Real>(P, U, u_stride, tile_active, X, error, rows, T, st);
+        const int per_tile = s.d * kLanes;
+        k_axpy_tiles<Real><<<dim3(std::min(64, (per_tile + 255) / 256), T), 256, 0, st>>>(Xs, X, tile_active, s.d);
+        HIPCHK(hipGetLastError());
+    }
+}
+
 template <typename Real>
 void launch_update(g2g_batch_s* b)
 {
     const g2g_plan_s* P = b->plan;
     UpdateArgs<Real> a{static_cast<const Real*>(b->X), P->d_upd, P->n_upd, b->tile_active, b->status, b->V,
-                       P->nr.d, P->nr.n, b->active_rows};
+                       P->nr.d, P->nr.n, b->active_rows, b->max_dva, b->max_dvm};
     k_update<Real><<<b->T, kLanes * kMisWaves, 0, b->stream>>>(a);
     HIPCHK(hipGetLastError());
 }
@@ -311,6 +347,11 @@ int nr_loop(g2g_batch_s* b, int max_iter, double tol, int freeze, double growth,
             long long& rows_factored)
 {
     const double eps = b->precision == G2G_MIXED ? 1e-5 : 1e-12;  // own_lu/batched_lu.cuh:50-56
+    if (b->ir_steps > 0 && !b->Bs) {
+        const size_t bytes = static_cast<size_t>(b->rows_pad) * b->plan->nr.d * b->real_size();
+        b->Bs = dalloc<unsigned char>(bytes);
+        b->Xs = dalloc<unsigned char>(bytes);
+    }
     int it = 0;
     for (;; ++it) {
         k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
@@ -330,6 +371,11 @@ int nr_loop(g2g_batch_s* b, int max_iter, double tol, int freeze, double growth,
         e0 = tm.mark();
         launch_backward<Real>(b->plan, static_cast<const Real*>(b->U), b->u_stride, b->tile_active,
                               static_cast<Real*>(b->X), b->error, b->active_rows, b->T, b->stream);
+        if (b->ir_steps > 0)
+            refine<Real>(b->plan, static_cast<Real*>(b->A), static_cast<Real*>(b->U), static_cast<Real*>(b->X),
+                         static_cast<Real*>(b->Bs), static_cast<Real*>(b->Xs), b->a_stride, b->u_stride,
+                         b->tile_active, b->jmax, b->clamped, b->error, b->active_rows, b->T, eps, b->ir_steps,
+                         b->stream);
         launch_update<Real>(b);
         tm.ph[2].push_back({e0, tm.mark()});
     }
@@ -517,7 +563,7 @@ void check_rows(const g2g_batch_s* b, int rows, int row0)
 // Raw batched solve of `rows` matrices on a plan's pattern (host arrays).
 template <typename Real>
 void raw_solve(g2g_plan_s* P, int rows, const double* vals, const double* rhs, double* x, double pivot_eps,
-               int32_t* clamped, void* stream)
+               int ir_steps, int32_t* clamped, void* stream)
 {
     hipStream_t st = static_cast<hipStream_t>(stream);
     P->upload();
@@ -550,6 +596,11 @@ void raw_solve(g2g_plan_s* P, int rows, const double* vals, const double* rhs, d
         HIPCHK(hipGetLastError());
         launch_factor<Real>(P, A, U, a_stride, u_stride, ta, jm, cl, err, rows, T, pivot_eps, st);
         launch_backward<Real>(P, U, u_stride, ta, X, err, rows, T, st);
+        if (ir_steps > 0) {
+            Real* Bs = B.get<Real>(rp * d);
+            Real* Xs = B.get<Real>(rp * d);
+            refine<Real>(P, A, U, X, Bs, Xs, a_stride, u_stride, ta, jm, cl, err, rows, T, pivot_eps, ir_steps, st);
+        }
         int herr = 0;
         HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
         std::vector<Real> hx(rp * d);
@@ -607,6 +658,8 @@ int g2g_lu_plan_create(int32_t d, const int32_t* Ap, const int32_t* Ai, const in
         if (order) ord.assign(order, order + d);
         p->raw = build_symbolic(d, Ap, Ai, ord);
         p->nnz = Ap[d];
+        p->raw_p.assign(Ap, Ap + d + 1);
+        p->raw_i.assign(Ai, Ai + Ap[d]);
         *out = p.release();
     });
 }
@@ -800,6 +853,19 @@ int g2g_batch_set_active_rows(g2g_batch_t b, int32_t active_rows)
     });
 }
 
+int g2g_batch_set_step_control(g2g_batch_t b, int32_t ir_steps, int32_t scaling, double max_dva, double max_dvm)
+{
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        if (ir_steps < 0) throw std::invalid_argument("ir_steps must be >= 0");
+        if (scaling && !(max_dva > 0.0 && max_dvm > 0.0))
+            throw std::invalid_argument("step scaling needs positive max_dva and max_dvm");
+        b->ir_steps = ir_steps;
+        b->max_dva = scaling ? max_dva : 0.0;
+        b->max_dvm = scaling ? max_dvm : 0.0;
+    });
+}
+
 int g2g_batch_solve(g2g_batch_t b, int32_t max_iter, double tol, int32_t scheduling, g2g_solve_stats* stats)
 {
     return guarded([&] {
@@ -974,13 +1040,14 @@ int g2g_batch_eval(g2g_batch_t b, double* F, double* J, double* residual)
 }
 
 int g2g_lu_solve_batch(g2g_plan_t P, int32_t rows, int32_t precision, const double* vals, const double* rhs,
-                       double* x, double pivot_eps, int32_t* clamped, void* stream)
+                       double* x, double pivot_eps, int32_t ir_steps, int32_t* clamped, void* stream)
 {
     return guarded([&] {
         if (!P || !vals || !rhs || !x) throw std::invalid_argument("null argument");
         if (rows <= 0) throw std::invalid_argument("rows must be positive");
-        if (precision == G2G_MIXED) raw_solve<float>(P, rows, vals, rhs, x, pivot_eps, clamped, stream);
-        else if (precision == G2G_FP64) raw_solve<double>(P, rows, vals, rhs, x, pivot_eps, clamped, stream);
+        if (ir_steps < 0) throw std::invalid_argument("ir_steps must be >= 0");
+        if (precision == G2G_MIXED) raw_solve<float>(P, rows, vals, rhs, x, pivot_eps, ir_steps, clamped, stream);
+        else if (precision == G2G_FP64) raw_solve<double>(P, rows, vals, rhs, x, pivot_eps, ir_steps, clamped, stream);
         else throw std::invalid_argument("unknown precision");
     });
 }

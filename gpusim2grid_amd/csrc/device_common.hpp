@@ -65,6 +65,21 @@ struct UpdateArgs {
     const int* status;
     double2* V;
     int d, n, rows;
+    double max_dva, max_dvm;  // step scaling bounds; max_dva <= 0: off
+};
+
+template <typename Real>
+struct RefineArgs {
+    Real* A;                // A stream [T][a_stride][64]: J values read, rhs overwritten
+    const Real* X;          // current solution [T][d][64] (unknown labels)
+    Real* Bs;               // saved rhs [T][d][64]
+    Real* Xs;               // copy of X [T][d][64]
+    const int* Jp;          // CSR of the matrix in unknown labels
+    const int* Ji;
+    const int* apos;        // A-stream position of each CSR entry
+    const int* fpos;        // A-stream position of each rhs entry
+    const int* tile_active;
+    int d, a_stride, first;
 };
 
 }  // namespace g2g

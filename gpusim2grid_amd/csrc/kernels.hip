@@ -793,10 +793,43 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_update(UpdateArgs<Real> a
     const bool live = row < a.rows && a.status[row] == kActive;
     const int b0 = static_cast<int>((static_cast<long long>(a.nbus) * wave) / kMisWaves);
     const int b1 = static_cast<int>((static_cast<long long>(a.nbus) * (wave + 1)) / kMisWaves);
+    // Step scaling (acpf_nr_kernels.cu:677-723): alpha = min(1, max_dva /
+    // max|dVa|, max_dvm / max|dVm|) per row, the maxima NaN-sticky so that a
+    // non-finite step is left unscaled for the status rule to catch.
+    double alpha = 1.0;
+    if (a.max_dva > 0.0) {
+        __shared__ double s_mt[kMisWaves][kLanes];
+        __shared__ double s_mv[kMisWaves][kLanes];
+        double mt = 0.0, mv = 0.0;
+        for (int bb = b0; bb < b1; ++bb) {
+            const int kt = a.upd[3 * bb + 1], kv = a.upd[3 * bb + 2];
+            const double at = fabs(static_cast<double>(a.X[tidx(tile, a.d, kt, lane)]));
+            if (!(mt >= at)) mt = at;
+            if (kv >= 0) {
+                const double av = fabs(static_cast<double>(a.X[tidx(tile, a.d, kv, lane)]));
+                if (!(mv >= av)) mv = av;
+            }
+        }
+        s_mt[wave][lane] = mt;
+        s_mv[wave][lane] = mv;
+        __syncthreads();
+        mt = 0.0;
+        mv = 0.0;
+        for (int w = 0; w < kMisWaves; ++w) {
+            if (!(mt >= s_mt[w][lane])) mt = s_mt[w][lane];
+            if (!(mv >= s_mv[w][lane])) mv = s_mv[w][lane];
+        }
+        if (mt > a.max_dva) alpha = fmin(alpha, a.max_dva / mt);
+        if (mv > a.max_dvm) alpha = fmin(alpha, a.max_dvm / mv);
+    }
     for (int bb = b0; bb < b1; ++bb) {
         const int bus = a.upd[3 * bb], kt = a.upd[3 * bb + 1], kv = a.upd[3 * bb + 2];
-        const double dva = static_cast<double>(a.X[tidx(tile, a.d, kt, lane)]);
-        const double dvm = kv >= 0 ? static_cast<double>(a.X[tidx(tile, a.d, kv, lane)]) : 0.0;
+        double dva = static_cast<double>(a.X[tidx(tile, a.d, kt, lane)]);
+        double dvm = kv >= 0 ? static_cast<double>(a.X[tidx(tile, a.d, kv, lane)]) : 0.0;
+        if (alpha != 1.0) {
+            dva *= alpha;
+            dvm *= alpha;
+        }
         double2* vp = a.V + tidx(tile, a.n, bus, lane);
         const double2 v = *vp;
         const double m = sqrt(fma(v.x, v.x, v.y * v.y));
@@ -808,10 +841,58 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_update(UpdateArgs<Real> a
 }
 
 // ---------------------------------------------------------------------------
-// layout transforms and value edits
+// Iterative refinement of the Newton step (own_lu/batched_lu.cu:239-260,391-411):
+//   r = b - J x  written over the A stream's right-hand side, so that the next
+//   k_factor / k_backward pass (the same factors, recomputed) solves J d = r;
+//   then x += d.  J is read from the A stream through the CSR's positions; x is
+//   the step in unknown labels.  On the first step b is saved from the A
+//   stream, x is copied to Xs on every step.
 // ---------------------------------------------------------------------------
 
-// rows-major [rows][m] (row stride ld elements) <-> tiled [T][m][64]
+template <typename Real>
+__global__ __launch_bounds__(kLanes* kMisWaves) void k_lin_residual(RefineArgs<Real> a)
+{
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = threadIdx.x / kLanes;
+    const size_t tile = blockIdx.x;
+    if (a.tile_active[tile] <= 0) return;
+    const int i0 = static_cast<int>((static_cast<long long>(a.d) * wave) / kMisWaves);
+    const int i1 = static_cast<int>((static_cast<long long>(a.d) * (wave + 1)) / kMisWaves);
+    Real* A = a.A + tile * static_cast<size_t>(a.a_stride) * kLanes + lane;
+    const Real* X = a.X + tile * static_cast<size_t>(a.d) * kLanes + lane;
+    Real* Bs = a.Bs + tile * static_cast<size_t>(a.d) * kLanes + lane;
+    Real* Xs = a.Xs + tile * static_cast<size_t>(a.d) * kLanes + lane;
+    for (int i = i0; i < i1; ++i) {
+        const size_t fp = static_cast<size_t>(a.fpos[i]) * kLanes;
+        Real s;
+        if (a.first) {
+            s = A[fp];
+            Bs[static_cast<size_t>(i) * kLanes] = s;
+        } else {
+            s = Bs[static_cast<size_t>(i) * kLanes];
+        }
+        for (int e = a.Jp[i]; e < a.Jp[i + 1]; ++e) {
+            const int ap = a.apos[e];
+            if (ap >= 0) s = fma(-A[static_cast<size_t>(ap) * kLanes], X[static_cast<size_t>(a.Ji[e]) * kLanes], s);
+        }
+        A[fp] = s;
+        Xs[static_cast<size_t>(i) * kLanes] = X[static_cast<size_t>(i) * kLanes];
+    }
+}
+
+template <typename Real>
+__global__ void k_axpy_tiles(const Real* __restrict__ Xs, Real* __restrict__ X, const int* __restrict__ tile_active,
+                             int d)
+{
+    const size_t tile = blockIdx.y;
+    if (tile_active[tile] <= 0) return;
+    const size_t m = static_cast<size_t>(d) * kLanes;
+    const size_t base = tile * m;
+    for (size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < m;
+         k += static_cast<size_t>(gridDim.x) * blockDim.x)
+        X[base + k] += Xs[base + k];
+}
+
 template <typename T>
 __global__ void k_rows_to_tiles(const T* __restrict__ in, long long ld, int rows, int m, int row_offset,
                                 T* __restrict__ out, int broadcast)
@@ -1005,6 +1086,10 @@ template __global__ void k_backward<double>(BackwardArgs<double>);
 template __global__ void k_backward<float>(BackwardArgs<float>);
 template __global__ void k_update<double>(UpdateArgs<double>);
 template __global__ void k_update<float>(UpdateArgs<float>);
+template __global__ void k_lin_residual<double>(RefineArgs<double>);
+template __global__ void k_lin_residual<float>(RefineArgs<float>);
+template __global__ void k_axpy_tiles<double>(const double*, double*, const int*, int);
+template __global__ void k_axpy_tiles<float>(const float*, float*, const int*, int);
 template __global__ void k_rows_to_tiles<double2>(const double2*, long long, int, int, int, double2*, int);
 template __global__ void k_rows_to_tiles<double>(const double*, long long, int, int, int, double*, int);
 template __global__ void k_tiles_to_rows<double2>(const double2*, int, int, double2*, long long);

@@ -15,6 +15,8 @@ or, with ``as_numpy=True``, a host array.
 """
 from __future__ import annotations
 
+import operator
+
 import numpy as np
 
 from . import _gpusim2grid as _cpp
@@ -62,13 +64,25 @@ def _tuple(grid):
             np.ascontiguousarray(pv, dtype=np.int32), np.ascontiguousarray(pq, dtype=np.int32))
 
 
-def _check_common(ir_n_steps, scaling_max_voltage_change, use_bridge):
+def _check_common(ir_n_steps, scaling_max_voltage_change, max_dVa, max_dVm, use_bridge):
+    """Validated step options as session keywords.  ``ir_n_steps`` follows
+    contingency_analysis/__init__.py:157-167; ``scaling_max_voltage_change=None``
+    inherits from the grid, and an array tuple carries no configuration, so it
+    means off; the thresholds default to 0.5 rad / 0.1 pu
+    (acpf_nr/gpu_facade.py:196-198)."""
     if use_bridge:
         raise ValueError("use_bridge=True requires a lightsim2grid grid object, not an explicit-array tuple.")
-    if ir_n_steps:
-        raise ValueError("ir_n_steps > 0 is not supported by this build")
-    if scaling_max_voltage_change:
-        raise ValueError("scaling_max_voltage_change is not supported by this build")
+    if isinstance(ir_n_steps, bool):
+        raise TypeError("ir_n_steps must be an int, got bool")
+    try:
+        n = operator.index(ir_n_steps)
+    except TypeError:
+        raise TypeError(f"ir_n_steps must be an int, got {type(ir_n_steps).__name__}") from None
+    if n < 0:
+        raise ValueError(f"ir_n_steps must be >= 0, got {n}")
+    return dict(ir_n_steps=n, scaling_max_voltage_change=bool(scaling_max_voltage_change),
+                max_dVa=0.5 if max_dVa is None else float(max_dVa),
+                max_dVm=0.1 if max_dVm is None else float(max_dVm))
 
 
 def _to_device_capsule(V):
@@ -96,12 +110,12 @@ class AcPfGPU:
                  use_bridge=None, reordering_alg="default", matching_alg="none", pivot_epsilon_alg="default",
                  ir_n_steps=0, debug_base_case=False, scaling_max_voltage_change=None, max_dVa=None, max_dVm=None,
                  use_distributed_slack=True):
-        _check_common(ir_n_steps, scaling_max_voltage_change, use_bridge)
+        step = _check_common(ir_n_steps, scaling_max_voltage_change, max_dVa, max_dVm, use_bridge)
         Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq = _tuple(grid)
         self._vm_fixed = None
         self._s = _cpp.AcPfNrSession(Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq, int(max_iter), float(tol),
                                      _device(device), presolved_v=bool(init_from_n_powerflow),
-                                     precision=_precision(precision))
+                                     precision=_precision(precision), **step)
 
     def solve(self):
         """The solved complex voltage vector (host copy)."""
@@ -153,8 +167,9 @@ class _BatchFacade:
 
     def __init__(self, grid, *, init_from_n_powerflow=True, precision="fp64", nb_iter=4, max_iter_base=10,
                  tol_base=1e-8, device=None, use_bridge=None, ir_n_steps=0, scaling_max_voltage_change=None,
+                 max_dVa=None, max_dVm=None,
                  scheduling=None, nb_iter_per_round=1, tol=None, linear_solver="own_lu", **_ignored):
-        _check_common(ir_n_steps, scaling_max_voltage_change, use_bridge)
+        step = _check_common(ir_n_steps, scaling_max_voltage_change, max_dVa, max_dVm, use_bridge)
         if nb_iter_per_round != 1:
             raise ValueError("nb_iter_per_round: rows are checked after every iteration in this build (1)")
         if linear_solver not in ("cudss", "own_lu"):
@@ -162,7 +177,8 @@ class _BatchFacade:
         Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq = _tuple(grid)
         self._s = self._session_cls(Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq, 100, int(nb_iter),
                                     int(max_iter_base), float(tol_base), _device(device),
-                                    presolved_v=bool(init_from_n_powerflow), precision=_precision(precision))
+                                    presolved_v=bool(init_from_n_powerflow), precision=_precision(precision),
+                                    **step)
         self.scheduling = scheduling or "chunked"
         if tol is not None:
             self.tol = tol

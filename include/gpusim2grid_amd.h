@@ -143,6 +143,15 @@ int g2g_batch_set_active_rows(g2g_batch_t batch, int32_t active_rows);
  * unit); its iteration count is the number of Newton updates applied. */
 int g2g_batch_solve(g2g_batch_t batch, int32_t max_iter, double tol, int32_t scheduling, g2g_solve_stats *stats);
 
+/* Step control of the next solves (AcPfNrSession's ir_n_steps and
+ * scaling_max_voltage_change / max_dVa / max_dVm, python_bindings.cpp:1369-1560).
+ * ir_steps: rounds of iterative refinement of every Newton step, in the
+ * factorization's precision (BatchedLu::refine, own_lu/batched_lu.cu:391-411).
+ * scaling: each row's step is multiplied by alpha = min(1, max_dva / max|dVa|,
+ * max_dvm / max|dVm|) before the update (reduce_step_norms_kernel /
+ * apply_step_scale_kernel, acpf_nr_kernels.cu:677-723; driver.cuh:225-235). */
+int g2g_batch_set_step_control(g2g_batch_t batch, int32_t ir_steps, int32_t scaling, double max_dva, double max_dvm);
+
 /* Results.  V: rows x n complex, row stride ld (complex numbers). */
 /* Learned warm start with fallback (no counterpart in the reference, whose
  * rows start from the base case or a caller's V, python_bindings.cpp:2722-2736).
@@ -183,9 +192,12 @@ int g2g_batch_eval(g2g_batch_t batch, double *F, double *J, double *residual);
 /* Raw batched LU solve of `rows` matrices on a g2g_lu_plan_create pattern:
  * vals rows x nnz (CSR order), rhs rows x d, x rows x d (host arrays);
  * pivots below pivot_eps * max|A_row| are clamped (own_lu/batched_lu.cu:45-53)
- * and counted per row in `clamped` (nullable). */
+ * and counted per row in `clamped` (nullable).  ir_steps rounds of iterative
+ * refinement x += (LU)^-1 (rhs - A x) follow, in the solve's precision
+ * (BatchedLu::refine, own_lu/batched_lu.cu:391-411; solve_cudss_raw's
+ * ir_n_steps, python_bindings.cpp:3827). */
 int g2g_lu_solve_batch(g2g_plan_t plan, int32_t rows, int32_t precision, const double *vals, const double *rhs,
-                       double *x, double pivot_eps, int32_t *clamped, void *stream);
+                       double *x, double pivot_eps, int32_t ir_steps, int32_t *clamped, void *stream);
 
 /* Device pointer of the batch's tiled Sbus array [ceil(rows/64)][n][64]
  * complex (bench and zero-copy callers fill it directly). */

@@ -14,7 +14,8 @@
  *     (src/_cpp/acpf_nr_kernels.cu:493-554, fill_FP/fill_FQ);
  *   - Jacobian values dS/dVa, dS/dVm per Ybus entry
  *     (src/_cpp/acpf_nr_kernels.cu:559-654, fill_J_kernel);
- *   - polar update, theta first then |V| (acpf_nr_kernels.cu:728-788);
+ *   - polar update, theta first then |V| (acpf_nr_kernels.cu:728-788), with
+ *     the optional per-row step scaling (acpf_nr_kernels.cu:677-723);
  *   - convergence: strict ||F||inf < tol in per unit, NaN-sticky
  *     (acpf_nr_kernels.cu:1076-1128; contingency/row_status.hpp:28-50);
  *     a row's iteration count is the number of Newton updates applied before
@@ -472,9 +473,31 @@ static void orc_solver_free(orc_solver *o)
     free(o->I); free(o->F); free(o->Jx); free(o->dx); free(o->w);
 }
 
+/* Step scaling (acpf_nr_kernels.cu:677-723): dx *= alpha, alpha = min(1,
+ * max_dva / max|dtheta|, max_dvm / max|dVm|), maxima NaN-sticky (a NaN
+ * maximum leaves alpha at 1).  max_dva <= 0: off. */
+static void orc_scale_step(const orc_sys *s, double *dx, double max_dva, double max_dvm)
+{
+    if (!(max_dva > 0.0)) return;
+    double mt = 0.0, mv = 0.0;
+    for (int k = 0; k < s->npvpq; ++k) {
+        const double a = fabs(dx[k]);
+        if (!(mt >= a)) mt = a;
+    }
+    for (int k = 0; k < s->npq; ++k) {
+        const double a = fabs(dx[s->npvpq + k]);
+        if (!(mv >= a)) mv = a;
+    }
+    double alpha = 1.0;
+    if (mt > max_dva && max_dva / mt < alpha) alpha = max_dva / mt;
+    if (mv > max_dvm && max_dvm / mv < alpha) alpha = max_dvm / mv;
+    if (alpha == 1.0) return;
+    for (int k = 0; k < s->d; ++k) dx[k] *= alpha;
+}
+
 /* One row: V (in/out, interleaved complex), S. Returns status. */
 static int orc_solver_run(orc_solver *o, const double *Yx, double *V, const double *S, int max_iter,
-                          double tol, int *iters, double *res)
+                          double tol, double max_dva, double max_dvm, int *iters, double *res)
 {
     orc_sys *s = &o->s;
     s->Yx = Yx;
@@ -489,6 +512,7 @@ static int orc_solver_run(orc_solver *o, const double *Yx, double *V, const doub
         orc_jacobian(s, V, o->I, o->Jx);
         if (orc_lu_factor(&o->lu, o->Jx) != 0) { *res = NAN; return 2; }
         orc_lu_solve(&o->lu, o->F, o->dx, o->w);
+        orc_scale_step(s, o->dx, max_dva, max_dvm);
         /* theta of pvpq buses, then |V| of pq buses */
         for (int k = 0; k < s->npvpq; ++k) {
             const int b = s->pvpq[k];
@@ -570,7 +594,7 @@ typedef struct {
     int n; const int *Yp, *Yi; const double *Yx; long long y_stride; /* 0: shared */
     int npv; const int *pv; int npq; const int *pq;
     double *V; long long v_stride; const double *S; long long s_stride;
-    int max_iter; double tol;
+    int max_iter; double tol; double max_dva, max_dvm;
     int *status; int *iters; double *res;
     int row0, row1;
 } orc_job;
@@ -584,7 +608,7 @@ static void *orc_worker(void *arg)
         const double *Yx = j->Yx + (size_t)r * (size_t)j->y_stride;
         j->status[r] = orc_solver_run(&o, Yx, j->V + (size_t)r * (size_t)j->v_stride,
                                       j->S + (size_t)r * (size_t)j->s_stride, j->max_iter, j->tol,
-                                      &j->iters[r], &j->res[r]);
+                                      j->max_dva, j->max_dvm, &j->iters[r], &j->res[r]);
     }
     orc_solver_free(&o);
     return NULL;
@@ -593,11 +617,12 @@ static void *orc_worker(void *arg)
 /* Solves n_rows power flows. V: [rows, n] complex in/out (interleaved re,im);
  * S: [rows, n] complex, or shared when s_stride == 0; Yx: [rows, nY] complex,
  * or shared when y_stride == 0 (strides counted in complex numbers).
- * Runs on n_threads POSIX threads. */
+ * Runs on n_threads POSIX threads.  max_dva > 0 turns on step scaling
+ * with bounds max_dva (angles, rad) and max_dvm (magnitudes, pu). */
 void orc_nr_batch(int n, const int *Yp, const int *Yi, const double *Yx, long long y_stride, int npv,
                   const int *pv, int npq, const int *pq, int n_rows, double *V, const double *S,
                   long long s_stride, int max_iter, double tol, int *status, int *iters, double *res,
-                  int n_threads)
+                  int n_threads, double max_dva, double max_dvm)
 {
     if (n_threads < 1) n_threads = 1;
     if (n_threads > n_rows) n_threads = n_rows > 0 ? n_rows : 1;
@@ -608,7 +633,7 @@ void orc_nr_batch(int n, const int *Yp, const int *Yi, const double *Yx, long lo
         j->n = n; j->Yp = Yp; j->Yi = Yi; j->Yx = Yx; j->y_stride = 2 * y_stride;
         j->npv = npv; j->pv = pv; j->npq = npq; j->pq = pq;
         j->V = V; j->v_stride = 2LL * n; j->S = S; j->s_stride = 2 * s_stride;
-        j->max_iter = max_iter; j->tol = tol;
+        j->max_iter = max_iter; j->tol = tol; j->max_dva = max_dva; j->max_dvm = max_dvm;
         j->status = status; j->iters = iters; j->res = res;
         j->row0 = (int)((long long)n_rows * t / n_threads);
         j->row1 = (int)((long long)n_rows * (t + 1) / n_threads);

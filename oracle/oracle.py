@@ -63,7 +63,7 @@ def lib():
         L.orc_nr_batch.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, ctypes.c_longlong,
                                    ctypes.c_int, _i32p, ctypes.c_int, _i32p, ctypes.c_int, _f64p,
                                    _f64p, ctypes.c_longlong, ctypes.c_int, ctypes.c_double, _i32p,
-                                   _i32p, _f64p, ctypes.c_int]
+                                   _i32p, _f64p, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         _LIB = L
     return _LIB
 
@@ -123,8 +123,9 @@ def sparse_solve(A, b):
     return x
 
 
-def nr_batch(Ybus, V0, Sbus, pv, pq, max_iter=10, tol=1e-8, n_threads=1, Yx_rows=None):
-    """Plain FP64 NR over rows.
+def nr_batch(Ybus, V0, Sbus, pv, pq, max_iter=10, tol=1e-8, n_threads=1, Yx_rows=None, max_dva=0.0,
+             max_dvm=0.0):
+    """Plain FP64 NR over rows (step scaling when max_dva > 0).
 
     V0 : (n,) shared start or (rows, n); Sbus : (n,) shared or (rows, n);
     Yx_rows : optional (rows, nY) per-row Ybus values on Ybus's pattern.
@@ -151,7 +152,7 @@ def nr_batch(Ybus, V0, Sbus, pv, pq, max_iter=10, tol=1e-8, n_threads=1, Yx_rows
     Vf = np.ascontiguousarray(V).view(np.float64)
     lib().orc_nr_batch(n, Yp, Yi, _cplx(Yall), y_stride, pv.size, pv, pq.size, pq, rows, Vf,
                        _cplx(np.ascontiguousarray(S)), s_stride, int(max_iter), float(tol), status,
-                       iters, res, int(n_threads))
+                       iters, res, int(n_threads), float(max_dva), float(max_dvm))
     return Vf.view(np.complex128).reshape(rows, n), status, iters, res
 
 

@@ -44,9 +44,17 @@ def test_tuple_mode_only():
     t = F._tuple(_tuple(g))
     assert t[1].dtype == complex and t[5].dtype == np.int32
     with pytest.raises(ValueError):
-        F._check_common(0, None, True)
+        F._check_common(0, None, None, None, True)
+    # ir_n_steps validation (contingency_analysis/__init__.py:157-167)
     with pytest.raises(ValueError):
-        F._check_common(1, None, None)
+        F._check_common(-1, None, None, None, None)
+    with pytest.raises(TypeError):
+        F._check_common(True, None, None, None, None)
+    with pytest.raises(TypeError):
+        F._check_common(1.5, None, None, None, None)
+    assert F._check_common(2, None, None, None, None) == dict(ir_n_steps=2, scaling_max_voltage_change=False,
+                                                            max_dVa=0.5, max_dVm=0.1)
+    assert F._check_common(0, True, 0.2, None, None)["max_dVa"] == 0.2
 
 
 def test_module_surface():
