@@ -99,8 +99,10 @@ def train_warm(args, g, torch, capi, syn):
 
 def run_method(args, name, g, S, torch, capi, ws=None):
     prec = capi.FP64 if name == "fp64" else capi.MIXED
+    t_plan = time.perf_counter()
     plan = capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
     info = plan.info()
+    t_plan = time.perf_counter() - t_plan
     B = capi.Batch(plan, args.rows, precision=prec)
     # injections resident in HBM: one upload before any timing
     S_dev = torch.from_numpy(np.ascontiguousarray(S)).to("cuda")
@@ -130,6 +132,7 @@ def run_method(args, name, g, S, torch, capi, ws=None):
     ms_factor = sum(s["ms_factor"] for s in stats)
     out = dict(
         method=name, pfps=conv * args.steps / dt, ms_per_step=1e3 * dt / args.steps,
+        analysis_s=t_plan, pfps_incl_analysis=conv / (dt / args.steps + t_plan),
         converged=conv, rows=args.rows, iterations=stats[-1]["iterations"],
         mean_row_iters=float(it[st == 0].mean()) if conv else None,
         max_residual=float(res[st == 0].max()) if conv else None,
@@ -168,7 +171,18 @@ def cpu_baseline(args, g, S, V_gpu, st_gpu):
         ok = (sto == 0) & (st_gpu[:n] == 0)
         parity = dict(status_equal=bool(np.array_equal(sto, st_gpu[:n])),
                       max_dV=float(np.abs(V_gpu[:n][ok] - Vo[ok]).max()) if ok.any() else None)
-    return dict(value=conv / dt, unit="PF/s", cores=cores, kind="port",
+    cpu = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(value=conv / dt, unit="PF/s", cores=cores, kind="port", per_core=conv / dt / cores,
+                label="own CPU NR (KLU-equivalent), not lightsim2grid", host_cpu=cpu,
+                host_nproc=os.cpu_count(),
                 sample=f"first {n} of the {S.shape[0]} rows, oracle C NR (min-degree + partial-pivot sparse LU), "
                        f"{cores} threads, {dt:.1f} s"), parity
 

@@ -19,9 +19,9 @@ namespace g2g {
 // kernels (kernels.hip)
 template <typename Real, bool PerRowY, bool WithJ>
 __global__ void k_mismatch(MismatchArgs<Real> a);
-template <typename Real>
+template <typename Real, int W>
 __global__ void k_factor(FactorArgs<Real> a);
-template <typename Real>
+template <typename Real, int W>
 __global__ void k_backward(BackwardArgs<Real> a);
 template <typename Real>
 __global__ void k_update(UpdateArgs<Real> a);
@@ -66,6 +66,7 @@ struct g2g_plan_s {
     int device = 0;
     int* d_prog_f = nullptr;
     int* d_prog_b = nullptr;
+    int* d_seg = nullptr;      // per walker segments (SymbolicLU::Seg, 9 ints each)
     int* d_k1_bus = nullptr;
     int* d_k1_ent = nullptr;
     double2* d_Y = nullptr;
@@ -75,6 +76,10 @@ struct g2g_plan_s {
     int* d_upd = nullptr;     // per bus {bus, theta unknown, |V| unknown}
     int n_upd = 0;
     std::vector<int> raw_p, raw_i;  // a raw plan's matrix pattern (CSR)
+    std::vector<int> raw_order;     // and its pivot order (empty: minimum degree)
+    // the one-walker variant, built when a precision's walker kernels do not
+    // fit the LDS
+    std::unique_ptr<g2g_plan_s> single;
     int* d_Jp = nullptr;      // the matrix CSR in unknown labels (refinement residuals)
     int* d_Ji = nullptr;
     int prog_f_stride = 0, prog_b_stride = 0;
@@ -82,7 +87,7 @@ struct g2g_plan_s {
     const SymbolicLU& lu() const { return grid ? nr.lu : raw; }
     ~g2g_plan_s()
     {
-        dfree(d_prog_f); dfree(d_prog_b); dfree(d_k1_bus); dfree(d_k1_ent);
+        dfree(d_prog_f); dfree(d_prog_b); dfree(d_seg); dfree(d_k1_bus); dfree(d_k1_ent);
         dfree(d_Y); dfree(d_perm); dfree(d_apos); dfree(d_fpos); dfree(d_upd); dfree(d_Jp); dfree(d_Ji);
     }
     // device copies of the tables, made on first use (plans can be built and
@@ -104,6 +109,13 @@ struct g2g_plan_s {
         };
         d_prog_f = dupload(padded(s.prog_f, prog_f_stride));
         d_prog_b = dupload(padded(s.prog_b, prog_b_stride));
+        {
+            std::vector<int> sg;
+            for (const auto& g : s.seg)
+                sg.insert(sg.end(), {g.prog_f_off, g.prog_f_len, g.prog_b_off, g.prog_b_len, g.a_off, g.a_len,
+                                     g.u_off, g.u_len, g.n_piv});
+            d_seg = dupload(sg);
+        }
         d_perm = dupload(s.perm);
         d_apos = dupload(s.a_pos_of_entry);
         d_fpos = dupload(s.f_pos);
@@ -185,26 +197,22 @@ inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
 template <typename Real>
 constexpr int ring_slots_of() { return sizeof(Real) == sizeof(double) ? kRingSlotsF64 : kRingSlotsF32; }
 
-// control blocks + value ring + program ring (kernels.hip's LDS layout)
-template <typename Real>
-size_t ring_lds()
-{
-    return 256 + static_cast<size_t>(chunk_of<Real>()) * ring_slots_of<Real>() * kLanes * sizeof(Real) +
-           static_cast<size_t>(kProgChunk) * kProgSlots * sizeof(int);
-}
-
-// stream rings + slots (plus the zero and trash slots) [+ two column scratch rows]
+// walker control areas + flags + value and program rings, then the slots
+// (plus the zero and trash slots) and each walker's long-column scratch rows
+// (kernels.hip's LDS layout)
 template <typename Real>
 size_t factor_lds(const SymbolicLU& s)
 {
-    const int scratch = s.max_c > kFastC ? 2 * s.max_c : 0;  // rows for the generic long-column path
-    return ring_lds<Real>() + static_cast<size_t>(s.slots_f + 2 + scratch) * kLanes * sizeof(Real);
+    const int scratch = s.max_c > kFastC ? 2 * s.max_c * s.walkers : 0;
+    return static_cast<size_t>(walk_ring_bytes(sizeof(Real), s.walkers)) +
+           static_cast<size_t>(s.slots_f + 2 + scratch) * kLanes * sizeof(Real);
 }
 
 template <typename Real>
 size_t backward_lds(const SymbolicLU& s)
 {
-    return ring_lds<Real>() + static_cast<size_t>(s.slots_b + 2) * kLanes * sizeof(Real);
+    return static_cast<size_t>(walk_ring_bytes(sizeof(Real), s.walkers)) +
+           static_cast<size_t>(s.slots_b + 2) * kLanes * sizeof(Real);
 }
 
 size_t lds_limit()
@@ -216,6 +224,39 @@ size_t lds_limit()
     return static_cast<size_t>(v);
 }
 
+// The plan whose walker kernels fit the LDS at this precision: P itself, or
+// its one-walker variant (built once, owned by P).
+g2g_plan_s* plan_fitting(g2g_plan_s* P, bool fp64)
+{
+    const SymbolicLU& s = P->lu();
+    if (s.walkers == 1) return P;
+    const size_t need = fp64 ? std::max(factor_lds<double>(s), backward_lds<double>(s))
+                             : std::max(factor_lds<float>(s), backward_lds<float>(s));
+    if (need <= lds_limit()) return P;
+    if (!P->single) {
+        auto q = std::make_unique<g2g_plan_s>();
+        q->grid = P->grid;
+        q->nnz = P->nnz;
+        if (P->grid) {
+            const NrPlan& nr = P->nr;
+            std::vector<int> pv;
+            std::vector<char> is_pq(static_cast<size_t>(nr.n), 0);
+            for (int b : nr.pq) is_pq[b] = 1;
+            for (int b : nr.pvpq)
+                if (!is_pq[b]) pv.push_back(b);
+            q->nr = build_nr_plan(nr.n, nr.Yp.data(), nr.Yi.data(), nr.Yx.empty() ? nullptr : nr.Yx.data(),
+                                  static_cast<int>(pv.size()), pv.data(), nr.npq, nr.pq.data(), nr.slack, 1);
+        } else {
+            q->raw = build_symbolic(P->raw.d, P->raw_p.data(), P->raw_i.data(), P->raw_order, 1);
+            q->raw_p = P->raw_p;
+            q->raw_i = P->raw_i;
+            q->raw_order = P->raw_order;
+        }
+        P->single = std::move(q);
+    }
+    return P->single.get();
+}
+
 template <typename Real>
 void launch_factor(const g2g_plan_s* P, const Real* A, Real* U, int a_stride, int u_stride, const int* tile_active,
                    const double* jmax, int* clamped, int* error, int rows, int T, double eps, hipStream_t st)
@@ -223,11 +264,13 @@ void launch_factor(const g2g_plan_s* P, const Real* A, Real* U, int a_stride, in
     const SymbolicLU& s = P->lu();
     const size_t lds = factor_lds<Real>(s);
     if (lds > lds_limit()) throw std::runtime_error("factorization needs " + std::to_string(lds) + " bytes of LDS");
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_factor<Real>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     FactorArgs<Real> a{A, U, P->d_prog_f, tile_active, jmax, clamped, error, s.d, a_stride, u_stride,
-                       P->prog_f_stride, s.slots_f, rows, s.max_c, eps};
-    k_factor<Real><<<T, kLanes * kStreamWaves, lds, st>>>(a);
+                       P->prog_f_stride, s.slots_f, rows, s.max_c, eps, P->d_seg};
+    const void* fn = s.walkers == 2 ? reinterpret_cast<const void*>(&k_factor<Real, 2>)
+                                    : reinterpret_cast<const void*>(&k_factor<Real, 1>);
+    HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    if (s.walkers == 2) k_factor<Real, 2><<<T, kLanes * walk_waves(2), lds, st>>>(a);
+    else k_factor<Real, 1><<<T, kLanes * walk_waves(1), lds, st>>>(a);
     HIPCHK(hipGetLastError());
 }
 
@@ -238,11 +281,13 @@ void launch_backward(const g2g_plan_s* P, const Real* U, int u_stride, const int
     const SymbolicLU& s = P->lu();
     const size_t lds = backward_lds<Real>(s);
     if (lds > lds_limit()) throw std::runtime_error("backward substitution needs too much LDS");
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_backward<Real>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     BackwardArgs<Real> a{U, P->d_prog_b, tile_active, X, error, s.d, s.u_len, u_stride, P->prog_b_stride, rows,
-                         s.max_c};
-    k_backward<Real><<<T, kLanes * kStreamWaves, lds, st>>>(a);
+                         s.max_c, P->d_seg, s.n_top};
+    const void* fn = s.walkers == 2 ? reinterpret_cast<const void*>(&k_backward<Real, 2>)
+                                    : reinterpret_cast<const void*>(&k_backward<Real, 1>);
+    HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    if (s.walkers == 2) k_backward<Real, 2><<<T, kLanes * walk_waves(2), lds, st>>>(a);
+    else k_backward<Real, 1><<<T, kLanes * walk_waves(1), lds, st>>>(a);
     HIPCHK(hipGetLastError());
 }
 
@@ -566,6 +611,7 @@ void raw_solve(g2g_plan_s* P, int rows, const double* vals, const double* rhs, d
                int ir_steps, int32_t* clamped, void* stream)
 {
     hipStream_t st = static_cast<hipStream_t>(stream);
+    P = plan_fitting(P, sizeof(Real) == sizeof(double));
     P->upload();
     const SymbolicLU& s = P->lu();
     const int T = (rows + kLanes - 1) / kLanes;
@@ -656,7 +702,8 @@ int g2g_lu_plan_create(int32_t d, const int32_t* Ap, const int32_t* Ai, const in
         p->grid = false;
         std::vector<int> ord;
         if (order) ord.assign(order, order + d);
-        p->raw = build_symbolic(d, Ap, Ai, ord);
+        p->raw = build_symbolic(d, Ap, Ai, ord, default_walkers(d));
+        p->raw_order = ord;
         p->nnz = Ap[d];
         p->raw_p.assign(Ap, Ap + d + 1);
         p->raw_i.assign(Ai, Ai + Ap[d]);
@@ -686,6 +733,8 @@ int g2g_plan_get_info(g2g_plan_t P, g2g_plan_info* info)
         info->nLU = s.nLU;
         info->n_updates = s.n_updates;
         info->n_steps = s.n_steps;
+        info->walkers = s.walkers;
+        info->n_top = s.n_top;
         info->max_c = s.max_c;
         info->height = s.height;
         info->slots_f = s.slots_f;
@@ -719,6 +768,12 @@ int g2g_plan_get_array(g2g_plan_t P, int32_t which, int32_t* out, int64_t capaci
         case G2G_ARR_APOS: v = &s.a_pos_of_entry; break;
         case G2G_ARR_FPOS: v = &s.f_pos; break;
         case G2G_ARR_UOFF: v = &s.u_off; break;
+        case G2G_ARR_SEGS:
+            for (const auto& g : s.seg)
+                tmp.insert(tmp.end(), {g.prog_f_off, g.prog_f_len, g.prog_b_off, g.prog_b_len, g.a_off, g.a_len,
+                                       g.u_off, g.u_len, g.n_piv});
+            v = &tmp;
+            break;
         default: throw std::invalid_argument("unknown array id");
         }
         if (length) *length = static_cast<int64_t>(v->size());
@@ -737,6 +792,7 @@ int g2g_batch_create(g2g_plan_t P, int32_t rows, int32_t precision, int32_t per_
         if (!P->grid) throw std::invalid_argument("a batch needs a grid plan");
         if (rows <= 0) throw std::invalid_argument("rows must be positive");
         if (precision != G2G_FP64 && precision != G2G_MIXED) throw std::invalid_argument("unknown precision");
+        P = plan_fitting(P, precision == G2G_FP64);
         P->upload();
         auto b = std::make_unique<g2g_batch_s>();
         b->plan = P;

@@ -43,6 +43,7 @@ struct FactorArgs {
     int d, a_stride, u_stride, prog_stride, slots, rows;
     int max_c;              // longest column (sizes the long-column scratch rows)
     double pivot_eps;
+    const int* seg;         // per walker: prog_f off/len, prog_b off/len, A off/len, U off/len, pivots
 };
 
 template <typename Real>
@@ -54,6 +55,8 @@ struct BackwardArgs {
     int* error;             // set when a wavefront gave up waiting on its stream
     int d, u_len, u_stride, prog_stride, rows;
     int max_c;              // longest column (a longer record header means a corrupt walk)
+    const int* seg;         // per walker segments (as FactorArgs::seg)
+    int n_top;              // pivots walker 0 walks before the others start
 };
 
 template <typename Real>

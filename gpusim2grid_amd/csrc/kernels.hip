@@ -200,7 +200,6 @@ struct StreamShape<float> {
 };
 using ProgRing = Ring<int, kProgChunk, kProgSlots, false, false>;
 using ProgReader = ProgRing::Reader;
-constexpr int kCtlBytes = 256;  // the two streams' control blocks
 
 // NW program words from position p (4-aligned) as 16-byte broadcast reads
 template <int NW>
@@ -514,54 +513,142 @@ __device__ __forceinline__ int factor_dispatch(int c, AR& A, ProgReader& P, Fact
     }
 }
 
-template <typename Real>
-__global__ __launch_bounds__(kLanes * kStreamWaves) void k_factor(FactorArgs<Real> a)
+// cross-walker flags (LDS): walker w > 0 has finished its first phase; the
+// top part of the backward substitution is done; each walker's clamp counts
+struct WalkFlags {
+    int done[kMaxWalkers];
+    int top_done;
+    int pad;
+    int clamped[kMaxWalkers][kLanes];
+};
+static_assert(sizeof(WalkFlags) <= kWalkFlagBytes, "walker flags");
+
+// bounded wait for an LDS flag another walker sets (false: gave up)
+__device__ __forceinline__ bool wait_flag(int* flag)
 {
-    using Sh = StreamShape<Real>;
-    using ARing = Ring<Real, Sh::kCh, Sh::kNch, false, true>;
-    static_assert(sizeof(typename ARing::Ctl) + sizeof(ProgRing::Ctl) <= kCtlBytes, "control blocks");
+    int spins = 0;
+    while (!lds_load_acquire(flag)) {
+        if (++spins > kSpinLimit) return false;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return true;
+}
+
+// A merge record [kMergeHead | next << 16, n, (dst, src) x n]: dst += src,
+// src = 0 (walker 0, after the other walkers' first phase); returns the
+// next record's head
+template <typename Real>
+__device__ int factor_merge(ProgReader& P, FactorState<Real>& st)
+{
+    P.window(st.p, st.p + 1);
+    const int n = __builtin_amdgcn_readfirstlane(P.at(st.p + 1));
+    const int nw = mrec_words(n > kMergePairs ? kMergePairs : (n < 0 ? 0 : n));
+    P.window(st.p, st.p + nw - 1);
+    for (int i = 0; i < n && i < kMergePairs; ++i) {
+        const unsigned dst = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(st.p + 2 + 2 * i)));
+        const unsigned src = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(st.p + 3 + 2 * i)));
+        Real* pd = slot_at<Real>(st.lb, dst);
+        Real* ps = slot_at<Real>(st.lb, src);
+        const Real v = *ps;
+        *ps = Real(0);
+        *pd += v;
+    }
+    const int next = static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(st.p))) >> 16);
+    st.p += nw;
+    return next;
+}
+
+__device__ __forceinline__ bool head_ok(int h, int max_c) { return h == kMergeHead || (h & 0x7FFF) <= max_c; }
+
+// Workgroup over one tile of 64 rows: W walker wavefronts walk their
+// program segments (LDS only, plus the U-stream stores); value_producers(W)
+// waves stream the walkers' A segments into their LDS rings (split evenly
+// between the walkers) and W waves stream the program segments.
+template <typename Real, int W>
+__global__ __launch_bounds__(kLanes * walk_waves(W)) void k_factor(FactorArgs<Real> a)
+{
+    constexpr int kCh = value_chunk(sizeof(Real)), kNch = value_ring_slots(sizeof(Real), W);
+    using ARing = Ring<Real, kCh, kNch, false, true>;
+    constexpr int kNP = value_producers(W), kP = kNP / W;
+    static_assert(kP * W == kNP, "producers split evenly");
+    static_assert(sizeof(typename ARing::Ctl) + sizeof(ProgRing::Ctl) <= kWalkerCtlBytes, "control blocks");
     const int lane = threadIdx.x & (kLanes - 1);
-    const int wave = threadIdx.x / kLanes;
+    // wave-uniform (scalar) role, so the segment table is read with scalar loads
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kLanes));
     const size_t tile = blockIdx.x;
     if (a.tile_active[tile] <= 0) return;
     extern __shared__ __align__(16) unsigned char g2g_smem[];
-    auto* ctlA = reinterpret_cast<typename ARing::Ctl*>(g2g_smem);
-    auto* ctlP = reinterpret_cast<ProgRing::Ctl*>(g2g_smem + sizeof(typename ARing::Ctl));
-    Real* aring = reinterpret_cast<Real*>(g2g_smem + kCtlBytes);
-    int* pring = reinterpret_cast<int*>(g2g_smem + kCtlBytes + ARing::kLdsBytes);
-    Real* sl = reinterpret_cast<Real*>(g2g_smem + kCtlBytes + ARing::kLdsBytes + ProgRing::kLdsBytes);
+    auto ctlA = [&](int w) { return reinterpret_cast<typename ARing::Ctl*>(g2g_smem + w * kWalkerCtlBytes); };
+    auto ctlP = [&](int w) {
+        return reinterpret_cast<ProgRing::Ctl*>(g2g_smem + w * kWalkerCtlBytes + sizeof(typename ARing::Ctl));
+    };
+    auto* flags = reinterpret_cast<WalkFlags*>(g2g_smem + W * kWalkerCtlBytes);
+    unsigned char* rings = g2g_smem + W * kWalkerCtlBytes + kWalkFlagBytes;
+    auto aring = [&](int w) { return reinterpret_cast<Real*>(rings + w * ARing::kLdsBytes); };
+    auto pring = [&](int w) {
+        return reinterpret_cast<int*>(rings + W * ARing::kLdsBytes + w * ProgRing::kLdsBytes);
+    };
+    Real* sl = reinterpret_cast<Real*>(rings + W * (ARing::kLdsBytes + ProgRing::kLdsBytes));
     if (wave == 0) {
-        ARing::init_ctl(ctlA, lane);
-        ProgRing::init_ctl(ctlP, lane);
+        for (int w = 0; w < W; ++w) {
+            ARing::init_ctl(ctlA(w), lane);
+            ProgRing::init_ctl(ctlP(w), lane);
+        }
+        if (lane < kMaxWalkers) flags->done[lane] = 0;
+        if (lane == 0) flags->top_done = 0;
     }
+    // every slot (and the zero and trash slots) starts at zero
+    for (int i = threadIdx.x; i < (a.slots + 2) * kLanes; i += blockDim.x) sl[i] = Real(0);
     __syncthreads();
     const Real* Atile = a.A + tile * static_cast<size_t>(a.a_stride) * kLanes;
-    if (wave >= 1 && wave <= kValueProducers) {
-        ARing::produce(Atile, a.a_stride / Sh::kCh, aring, ctlA, lane, wave - 1, kValueProducers);
+    if (wave >= W && wave < W + kNP) {
+        const int pw = (wave - W) / kP;
+        const int* sg = a.seg + 9 * pw;
+        ARing::produce(Atile + static_cast<size_t>(sg[4]) * kLanes, (sg[5] + kCh - 1) / kCh, aring(pw), ctlA(pw),
+                       lane, (wave - W) % kP, kP);
         return;
     }
-    if (wave == kValueProducers + 1) {
-        ProgRing::produce(a.prog, a.prog_stride / kProgChunk, pring, ctlP, lane, 0, 1);
+    if (wave >= W + kNP) {
+        const int pw = wave - W - kNP;
+        const int* sg = a.seg + 9 * pw;
+        ProgRing::produce(a.prog + sg[0], sg[1] / kProgChunk, pring(pw), ctlP(pw), lane, 0, 1);
         return;
     }
-    // wave 0: the program walk
+    // walker `wave`: its program segment
+    const int wk = wave;
+    const int* sg = a.seg + 9 * wk;
+    const int n_piv = sg[8];
     const int row = static_cast<int>(tile) * kLanes + lane;
-    Real* lsc = sl + static_cast<size_t>(a.slots + 2) * kLanes;  // [max_c][64] (long columns only)
-    Real* usc = lsc + static_cast<size_t>(a.max_c) * kLanes;     // [max_c][64]
-    for (int s2 = 0; s2 < a.slots + 2; ++s2) sl[static_cast<size_t>(s2) * kLanes + lane] = Real(0);
+    // long-column scratch rows of this walker (long columns only)
+    Real* lsc = sl + static_cast<size_t>(a.slots + 2 + 2 * wk * a.max_c) * kLanes;
+    Real* usc = lsc + static_cast<size_t>(a.max_c) * kLanes;
     typename ARing::Reader A;
-    A.init(aring, ctlA, a.a_stride / Sh::kCh, lane);
+    A.init(aring(wk), ctlA(wk), (sg[5] + kCh - 1) / kCh, lane);
     ProgReader P;
-    P.init(pring, ctlP, a.prog_stride / kProgChunk, lane);
+    P.init(pring(wk), ctlP(wk), sg[1] / kProgChunk, lane);
     FactorState<Real> st;
     st.lb = reinterpret_cast<unsigned char*>(sl + lane);
-    st.U = a.U + tile * static_cast<size_t>(a.u_stride) * kLanes + lane;
+    st.U = a.U + tile * static_cast<size_t>(a.u_stride) * kLanes + static_cast<size_t>(sg[6]) * kLanes + lane;
     st.eps = static_cast<Real>(a.pivot_eps) * static_cast<Real>(a.jmax ? a.jmax[row] : 1.0);
     st.clamped = 0;
     st.p = st.ap = st.up = 0;
-    int h = prog_word(P, 0) & 0xFFFF;
-    bool bad = (h & 0x7FFF) > a.max_c;
-    for (int k = 0; k < a.d && !bad;) {
+    int h = n_piv > 0 ? prog_word(P, 0) & 0xFFFF : 0;  // an empty segment has no record
+    bool bad = !head_ok(h, a.max_c);
+    bool waited = W == 1 || wk != 0;
+    for (int k = 0; k < n_piv && !bad;) {
+        if (h == kMergeHead) {
+            // the other walkers' first phase is over: their pending
+            // contributions to the top entries are final
+            if (!waited) {
+                waited = true;
+                for (int w = 1; w < W; ++w)
+                    if (!wait_flag(&flags->done[w])) bad = true;
+                if (bad) break;
+            }
+            h = factor_merge(P, st) & 0xFFFF;
+            if (P.failed || !head_ok(h, a.max_c)) bad = true;
+            continue;
+        }
         // each record's header announces the next record's column length and
         // width; a walk whose stream wait gave up stops before it trusts
         // another record
@@ -580,17 +667,28 @@ __global__ __launch_bounds__(kLanes * kStreamWaves) void k_factor(FactorArgs<Rea
             k += 1;
         }
         h = h_next & 0xFFFF;
-        if (A.failed || P.failed || ((h & 0x7FFF) > a.max_c && k < a.d)) {
+        if (A.failed || P.failed || (!head_ok(h, a.max_c) && k < n_piv)) {
             bad = true;
             break;
         }
     }
-    if (a.clamped && row < a.rows) a.clamped[row] = st.clamped;
     if (bad) {  // release the producers at once
         if (lane == 0) atomicOr(a.error, 1);
-        lds_store_release(&ctlA->abort, 1);
-        lds_store_release(&ctlP->abort, 1);
+        lds_store_release(&ctlA(wk)->abort, 1);
+        lds_store_release(&ctlP(wk)->abort, 1);
     }
+    if (W > 1) {
+        if (wk != 0) {
+            flags->clamped[wk][lane] = st.clamped;
+            lds_store_release(&flags->done[wk], 1);  // also after a failure: walker 0 must not wait
+            return;
+        }
+        if (!waited)
+            for (int w = 1; w < W; ++w)
+                if (!wait_flag(&flags->done[w]) && lane == 0) atomicOr(a.error, 1);
+        for (int w = 1; w < W; ++w) st.clamped += lds_load_acquire(&flags->clamped[w][lane]);
+    }
+    if (a.clamped && row < a.rows) a.clamped[row] = st.clamped;
 }
 
 // ---------------------------------------------------------------------------
@@ -707,49 +805,79 @@ __device__ __forceinline__ int backward_dispatch(int c, UR& U, ProgReader& P, Ba
     }
 }
 
-template <typename Real>
-__global__ __launch_bounds__(kLanes * kStreamWaves) void k_backward(BackwardArgs<Real> a)
+// Same roles as k_factor.  Walker 0 walks the top part first and then
+// signals the others, which walk their subtrees once the top's x are in
+// their slots.
+template <typename Real, int W>
+__global__ __launch_bounds__(kLanes * walk_waves(W)) void k_backward(BackwardArgs<Real> a)
 {
-    using Sh = StreamShape<Real>;
-    using URing = Ring<Real, Sh::kCh, Sh::kNch, true, true>;
-    static_assert(sizeof(typename URing::Ctl) + sizeof(ProgRing::Ctl) <= kCtlBytes, "control blocks");
+    constexpr int kCh = value_chunk(sizeof(Real)), kNch = value_ring_slots(sizeof(Real), W);
+    using URing = Ring<Real, kCh, kNch, true, true>;
+    constexpr int kNP = value_producers(W), kP = kNP / W;
+    static_assert(kP * W == kNP, "producers split evenly");
+    static_assert(sizeof(typename URing::Ctl) + sizeof(ProgRing::Ctl) <= kWalkerCtlBytes, "control blocks");
     const int lane = threadIdx.x & (kLanes - 1);
-    const int wave = threadIdx.x / kLanes;
+    // wave-uniform (scalar) role, so the segment table is read with scalar loads
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kLanes));
     const size_t tile = blockIdx.x;
     if (a.tile_active[tile] <= 0) return;
     extern __shared__ __align__(16) unsigned char g2g_smem[];
-    auto* ctlU = reinterpret_cast<typename URing::Ctl*>(g2g_smem);
-    auto* ctlP = reinterpret_cast<ProgRing::Ctl*>(g2g_smem + sizeof(typename URing::Ctl));
-    Real* uring = reinterpret_cast<Real*>(g2g_smem + kCtlBytes);
-    int* pring = reinterpret_cast<int*>(g2g_smem + kCtlBytes + URing::kLdsBytes);
-    Real* sl = reinterpret_cast<Real*>(g2g_smem + kCtlBytes + URing::kLdsBytes + ProgRing::kLdsBytes);
+    auto ctlU = [&](int w) { return reinterpret_cast<typename URing::Ctl*>(g2g_smem + w * kWalkerCtlBytes); };
+    auto ctlP = [&](int w) {
+        return reinterpret_cast<ProgRing::Ctl*>(g2g_smem + w * kWalkerCtlBytes + sizeof(typename URing::Ctl));
+    };
+    auto* flags = reinterpret_cast<WalkFlags*>(g2g_smem + W * kWalkerCtlBytes);
+    unsigned char* rings = g2g_smem + W * kWalkerCtlBytes + kWalkFlagBytes;
+    auto uring = [&](int w) { return reinterpret_cast<Real*>(rings + w * URing::kLdsBytes); };
+    auto pring = [&](int w) {
+        return reinterpret_cast<int*>(rings + W * URing::kLdsBytes + w * ProgRing::kLdsBytes);
+    };
+    Real* sl = reinterpret_cast<Real*>(rings + W * (URing::kLdsBytes + ProgRing::kLdsBytes));
     if (wave == 0) {
-        URing::init_ctl(ctlU, lane);
-        ProgRing::init_ctl(ctlP, lane);
+        for (int w = 0; w < W; ++w) {
+            URing::init_ctl(ctlU(w), lane);
+            ProgRing::init_ctl(ctlP(w), lane);
+        }
+        if (lane == 0) flags->top_done = 0;
     }
     __syncthreads();
-    if (wave >= 1 && wave <= kValueProducers) {
-        URing::produce(a.U + tile * static_cast<size_t>(a.u_stride) * kLanes, a.u_stride / Sh::kCh, uring, ctlU,
-                       lane, wave - 1, kValueProducers);
+    const Real* Utile = a.U + tile * static_cast<size_t>(a.u_stride) * kLanes;
+    if (wave >= W && wave < W + kNP) {
+        const int pw = (wave - W) / kP;
+        const int* sg = a.seg + 9 * pw;
+        URing::produce(Utile + static_cast<size_t>(sg[6]) * kLanes, (sg[7] + kCh - 1) / kCh, uring(pw), ctlU(pw),
+                       lane, (wave - W) % kP, kP);
         return;
     }
-    if (wave == kValueProducers + 1) {
-        ProgRing::produce(a.prog, a.prog_stride / kProgChunk, pring, ctlP, lane, 0, 1);
+    if (wave >= W + kNP) {
+        const int pw = wave - W - kNP;
+        const int* sg = a.seg + 9 * pw;
+        ProgRing::produce(a.prog + sg[2], sg[3] / kProgChunk, pring(pw), ctlP(pw), lane, 0, 1);
         return;
     }
+    const int wk = wave;
+    const int* sg = a.seg + 9 * wk;
+    const int n_piv = sg[8];
     typename URing::Reader U;
-    U.init(uring, ctlU, a.u_stride / Sh::kCh, lane);
+    U.init(uring(wk), ctlU(wk), (sg[7] + kCh - 1) / kCh, lane);
     ProgReader P;
-    P.init(pring, ctlP, a.prog_stride / kProgChunk, lane);
+    P.init(pring(wk), ctlP(wk), sg[3] / kProgChunk, lane);
     BackState<Real> st;
     st.lb = reinterpret_cast<unsigned char*>(sl + lane);
     st.X = a.X + tile * static_cast<size_t>(a.d) * kLanes + lane;
     st.p = 0;
-    st.up = a.u_len;
+    st.up = sg[7];
     st.d = a.d;
-    int h = prog_word(P, 0) & 0xFFFF;
-    bool bad = (h & 0x7FFF) > a.max_c;
-    for (int q = 0; q < a.d && !bad;) {
+    bool bad = false;
+    bool signaled = W == 1 || wk != 0;
+    if (W > 1 && wk != 0 && !wait_flag(&flags->top_done)) bad = true;
+    int h = n_piv > 0 && !bad ? prog_word(P, 0) & 0xFFFF : 0;  // an empty segment has no record
+    if ((h & 0x7FFF) > a.max_c) bad = true;
+    for (int q = 0; q < n_piv && !bad;) {
+        if (!signaled && q >= a.n_top) {
+            signaled = true;
+            lds_store_release(&flags->top_done, 1);
+        }
         const int c = h & 0x7FFF;
         int h_next;
         if (h >> 15) {
@@ -765,15 +893,16 @@ __global__ __launch_bounds__(kLanes * kStreamWaves) void k_backward(BackwardArgs
             q += 1;
         }
         h = h_next & 0xFFFF;
-        if (U.failed || P.failed || ((h & 0x7FFF) > a.max_c && q < a.d)) {
+        if (U.failed || P.failed || ((h & 0x7FFF) > a.max_c && q < n_piv)) {
             bad = true;
             break;
         }
     }
+    if (!signaled) lds_store_release(&flags->top_done, 1);  // also after a failure: the others must not wait
     if (bad) {
         if (lane == 0) atomicOr(a.error, 1);
-        lds_store_release(&ctlU->abort, 1);
-        lds_store_release(&ctlP->abort, 1);
+        lds_store_release(&ctlU(wk)->abort, 1);
+        lds_store_release(&ctlP(wk)->abort, 1);
     }
 }
 
@@ -1080,10 +1209,14 @@ template __global__ void k_mismatch<float, true, false>(MismatchArgs<float>);
 template __global__ void k_move_rows<double2>(const double2*, double2*, const int*, int, int, int);
 template __global__ void k_move_vals<int>(const int*, int*, const int*, int, int, int);
 template __global__ void k_move_vals<double>(const double*, double*, const int*, int, int, int);
-template __global__ void k_factor<double>(FactorArgs<double>);
-template __global__ void k_factor<float>(FactorArgs<float>);
-template __global__ void k_backward<double>(BackwardArgs<double>);
-template __global__ void k_backward<float>(BackwardArgs<float>);
+template __global__ void k_factor<double, 1>(FactorArgs<double>);
+template __global__ void k_factor<float, 1>(FactorArgs<float>);
+template __global__ void k_factor<double, 2>(FactorArgs<double>);
+template __global__ void k_factor<float, 2>(FactorArgs<float>);
+template __global__ void k_backward<double, 1>(BackwardArgs<double>);
+template __global__ void k_backward<float, 1>(BackwardArgs<float>);
+template __global__ void k_backward<double, 2>(BackwardArgs<double>);
+template __global__ void k_backward<float, 2>(BackwardArgs<float>);
 template __global__ void k_update<double>(UpdateArgs<double>);
 template __global__ void k_update<float>(UpdateArgs<float>);
 template __global__ void k_lin_residual<double>(RefineArgs<double>);

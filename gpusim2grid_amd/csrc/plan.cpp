@@ -2,6 +2,7 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <numeric>
 #include <set>
 #include <unordered_map>
@@ -70,6 +71,15 @@ std::vector<int> symbolic_structs(int d, const std::vector<int>& adjp, const std
 }
 
 }  // namespace
+
+int default_walkers(int d)
+{
+    if (const char* e = std::getenv("G2G_WALKERS")) {
+        const int w = std::atoi(e);
+        if (w >= 1 && w <= kMaxWalkers) return w;
+    }
+    return d >= kWalkersMinD ? 2 : 1;
+}
 
 std::vector<int> min_degree_order(int n, const std::vector<int>& adjp, const std::vector<int>& adji,
                                   const std::vector<int>& weight)
@@ -175,11 +185,155 @@ struct SlotAlloc {
 
 inline long long key(int i, int j, int d) { return static_cast<long long>(i) * d + j; }
 
-void build_backward(SymbolicLU& s);
+inline int round_up_to(int v, int m) { return (v + m - 1) / m * m; }
+
+// A record of a factorization program: a walk step (single pivot or
+// 2-column supernode) or a merge of another walker's pending values.
+struct Rec {
+    enum { kStep = 0, kMerge = 1 };
+    int kind = kStep;
+    int k = 0, w = 1, c = 0;  // step: first pivot, width, |struct| of its last pivot
+    std::vector<int> r;       // step: sources then targets (slot | kFirst, -1 none); merge: (dst, src) pairs
+};
+
+// One walker's factorization program under construction: its live update
+// targets, slot pool, A/U stream positions and records.  Slots a step
+// consumes are freed after its targets are placed, so a step's sources and
+// targets never share a slot (the kernels read both in one batch).
+struct Emitter {
+    SymbolicLU& s;
+    const std::unordered_map<long long, int>& a_entry;
+    std::unordered_map<long long, int> slot_of;  // live update targets
+    std::vector<int> bslot;                      // live forward-substitution targets
+    SlotAlloc sa;
+    std::vector<int> consumed;
+    std::vector<Rec> recs;
+    int apos = 0, upos = 0, n_piv = 0;
+
+    Emitter(SymbolicLU& s_, const std::unordered_map<long long, int>& ae)
+        : s(s_), a_entry(ae), bslot(static_cast<size_t>(s_.d), -1)
+    {
+    }
+    int take(int i, int j)  // slot of a consumed entry, or -1
+    {
+        auto it = slot_of.find(key(i, j, s.d));
+        if (it == slot_of.end()) return -1;
+        const int sl = it->second;
+        slot_of.erase(it);
+        consumed.push_back(sl);
+        return sl;
+    }
+    void place(int i, int j)  // dense A stream: every entry has a position
+    {
+        auto it = a_entry.find(key(i, j, s.d));
+        if (it != a_entry.end()) {
+            s.a_pos_of_entry[it->second] = apos;
+            ++s.nA;
+        }
+        ++apos;
+    }
+    void alloc_target(int i, int j, std::vector<int>& r)
+    {
+        const long long kk = key(i, j, s.d);
+        auto it = slot_of.find(kk);
+        if (it == slot_of.end()) {
+            const int sl = sa.get();
+            slot_of[kk] = sl;
+            r.push_back(sl | kFirst);
+        } else {
+            r.push_back(it->second);
+        }
+    }
+    void alloc_b(int i, std::vector<int>& r)
+    {
+        if (bslot[i] < 0) {
+            bslot[i] = sa.get();
+            r.push_back(bslot[i] | kFirst);
+        } else {
+            r.push_back(bslot[i]);
+        }
+    }
+    void sources(int k, std::vector<int>& r)  // diagonal, b(k), L(struct,k), U(k,struct)
+    {
+        const int c0 = s.colp[k], c = s.colp[k + 1] - c0;
+        s.f_pos[s.perm[k]] = apos++;
+        place(k, k);
+        for (int p = c0; p < c0 + c; ++p) place(s.coli[p], k);
+        for (int p = c0; p < c0 + c; ++p) place(k, s.coli[p]);
+        r.push_back(take(k, k));
+        int bref = -1;
+        if (bslot[k] >= 0) {
+            bref = bslot[k];
+            consumed.push_back(bref);
+            bslot[k] = -1;
+        }
+        r.push_back(bref);
+        for (int p = c0; p < c0 + c; ++p) r.push_back(take(s.coli[p], k));
+        for (int p = c0; p < c0 + c; ++p) r.push_back(take(k, s.coli[p]));
+        s.u_off[k] = upos;
+        upos += c + 2;
+    }
+    void step(int k)
+    {
+        const int w = s.step_w[k] == 2 ? 2 : 1;
+        const int last = k + w - 1;
+        const int l0 = s.colp[last], cl = s.colp[last + 1] - l0;
+        Rec st;
+        st.k = k;
+        st.w = w;
+        st.c = cl;
+        sources(k, st.r);
+        if (w == 2) sources(k + 1, st.r);
+        // update targets (i, j) in struct(last) x struct(last), row-major, then b(i)
+        for (int pa = l0; pa < l0 + cl; ++pa)
+            for (int pb = l0; pb < l0 + cl; ++pb) alloc_target(s.coli[pa], s.coli[pb], st.r);
+        for (int p = l0; p < l0 + cl; ++p) alloc_b(s.coli[p], st.r);
+        s.n_updates += w == 2 ? 2 * cl * cl : cl * cl;
+        for (int sl : consumed) sa.put(sl);
+        consumed.clear();
+        recs.push_back(std::move(st));
+        n_piv += w;
+    }
+    // appends the records; walker-local slot ids are shifted by `off`, a
+    // missing source reads the zero slot
+    void encode(std::vector<int>& out, int off, int zero) const
+    {
+        const auto enc = [](int slot) { return slot * kRefUnit; };
+        auto head = [](const Rec& x) {
+            return x.kind == Rec::kMerge ? kMergeHead : rec_head(x.c, x.w);
+        };
+        for (size_t q = 0; q < recs.size(); ++q) {
+            const Rec& st = recs[q];
+            const int next = q + 1 < recs.size() ? head(recs[q + 1]) : 0;
+            const size_t at = out.size();
+            if (st.kind == Rec::kMerge) {
+                const int n = static_cast<int>(st.r.size() / 2);
+                out.resize(at + static_cast<size_t>(mrec_words(n)), enc(zero));
+                int* wd = out.data() + at;
+                wd[0] = kMergeHead | (next << 16);
+                wd[1] = n;
+                for (int i = 0; i < 2 * n; ++i) wd[2 + i] = enc(st.r[i]);
+                continue;
+            }
+            const int c = st.c;
+            const int ns = st.w == 2 ? frec2_src_words(c) : frec_src_words(c);
+            out.resize(at + static_cast<size_t>(st.w == 2 ? frec2_words(c) : frec_words(c)), enc(zero));
+            int* wd = out.data() + at;
+            wd[0] = rec_head(c, st.w) | (next << 16);
+            const int nsrc = ns - 1;
+            for (int q2 = 0; q2 < nsrc; ++q2) wd[1 + q2] = enc(st.r[q2] < 0 ? zero : (st.r[q2] & kSlotMask) + off);
+            for (int i = 0; i < c * c + c; ++i) wd[ns + i] = enc((st.r[nsrc + i] & kSlotMask) + off);
+        }
+    }
+};
+
+std::vector<int> partition_steps(const SymbolicLU& s, const std::vector<int>& parent,
+                                 const std::vector<int>& step_first, int W);
+void build_backward(SymbolicLU& s, const std::vector<int>& group, int W);
 
 }  // namespace
 
-SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> order)
+SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> order, int walkers)
 {
     if (d <= 0) throw std::invalid_argument("matrix dimension must be positive");
     SymbolicLU s;
@@ -282,83 +436,16 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
     s.f_pos.assign(static_cast<size_t>(d), -1);
     s.nA = 0;
 
-    // ---- factorization + forward program -----------------------------------
-    // Slots are assigned first (refs as slot | kFirst, -1 = none), then packed
-    // into 16-bit fields once the slot count -- the zero slot's index -- is known.
-    SlotAlloc sa;
-    std::unordered_map<long long, int> slot_of;  // live update targets
-    std::vector<int> bslot(static_cast<size_t>(d), -1);
-    std::vector<std::vector<int>> refs(static_cast<size_t>(d));
-    s.u_off.assign(static_cast<size_t>(d) + 1, 0);
-    int apos = 0, upos = 0;
-    // slots a pivot consumes are freed after its targets are placed, so a
-    // pivot's sources and targets never share a slot (the kernels read both
-    // in one batch)
-    std::vector<int> consumed;
-    auto take = [&](int i, int j) -> int {  // slot of a consumed entry, or -1
-        auto it = slot_of.find(key(i, j, d));
-        if (it == slot_of.end()) return -1;
-        const int sl = it->second;
-        slot_of.erase(it);
-        consumed.push_back(sl);
-        return sl;
-    };
-    auto place = [&](int i, int j) {  // dense A stream: every entry has a position
-        auto it = a_entry.find(key(i, j, d));
-        if (it != a_entry.end()) {
-            s.a_pos_of_entry[it->second] = apos;
-            ++s.nA;
-        }
-        ++apos;
-    };
     // Steps of the walk: a single pivot, or a 2-column supernode (k, k+1) when
     // struct(k) = {k+1} + struct(k+1) -- the P and Q unknowns of one bus,
     // eliminated together.  A supernode reads the sources of both pivots,
     // applies pivot k to pivot k+1's row and column in registers, and updates
     // struct(k+1) x struct(k+1) once with a rank-2 term.  The A and U streams
     // keep their per-pivot layout.
-    struct Step {
-        int k, w, c;  // first pivot, width, |struct| of the last pivot
-        std::vector<int> r;
-    };
-    std::vector<Step> steps;
-    steps.reserve(static_cast<size_t>(d));
     s.step_w.assign(static_cast<size_t>(d), 1);
-    auto alloc_target = [&](int i, int j, std::vector<int>& r) {
-        const long long kk = key(i, j, d);
-        auto it = slot_of.find(kk);
-        if (it == slot_of.end()) {
-            const int sl = sa.get();
-            slot_of[kk] = sl;
-            r.push_back(sl | kFirst);
-        } else {
-            r.push_back(it->second);
-        }
-    };
-    auto alloc_b = [&](int i, std::vector<int>& r) {
-        if (bslot[i] < 0) {
-            bslot[i] = sa.get();
-            r.push_back(bslot[i] | kFirst);
-        } else {
-            r.push_back(bslot[i]);
-        }
-    };
-    auto sources = [&](int k, std::vector<int>& r) {  // diagonal, b(k), L(struct,k), U(k,struct)
-        const int c0 = s.colp[k], c = s.colp[k + 1] - c0;
-        s.f_pos[s.perm[k]] = apos++;
-        place(k, k);
-        for (int p = c0; p < c0 + c; ++p) place(s.coli[p], k);
-        for (int p = c0; p < c0 + c; ++p) place(k, s.coli[p]);
-        r.push_back(take(k, k));
-        int bref = -1;
-        if (bslot[k] >= 0) { bref = bslot[k]; consumed.push_back(bref); bslot[k] = -1; }
-        r.push_back(bref);
-        for (int p = c0; p < c0 + c; ++p) r.push_back(take(s.coli[p], k));
-        for (int p = c0; p < c0 + c; ++p) r.push_back(take(k, s.coli[p]));
-        s.u_off[k] = upos;
-        upos += c + 2;
-    };
-    for (int k = 0; k < d; ++k) {
+    std::vector<int> step_first;
+    s.max_c = 0;
+    for (int k = 0; k < d;) {
         const int c0 = s.colp[k], c = s.colp[k + 1] - c0;
         s.max_c = std::max(s.max_c, c);
         bool pair = false;
@@ -367,135 +454,289 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
             pair = c1 == c - 1 && c1 <= kFastC2 &&
                    std::equal(s.coli.begin() + c0 + 1, s.coli.begin() + c0 + c, s.coli.begin() + s.colp[k + 1]);
         }
-        Step st{k, pair ? 2 : 1, pair ? c - 1 : c, {}};
-        std::vector<int>& r = st.r;
-        sources(k, r);
-        const int last = pair ? k + 1 : k;
-        if (pair) sources(k + 1, r);
-        const int l0 = s.colp[last], cl = s.colp[last + 1] - l0;
-        s.max_c = std::max(s.max_c, cl);
-        // update targets (i, j) in struct(last) x struct(last), row-major, then b(i)
-        for (int pa_ = l0; pa_ < l0 + cl; ++pa_)
-            for (int pb = l0; pb < l0 + cl; ++pb) alloc_target(s.coli[pa_], s.coli[pb], r);
-        for (int p = l0; p < l0 + cl; ++p) alloc_b(s.coli[p], r);
-        s.n_updates += pair ? 2 * cl * cl : cl * cl;
-        for (int sl : consumed) sa.put(sl);
-        consumed.clear();
-        s.step_w[k] = pair ? 2 : 1;
-        if (pair) s.step_w[k + 1] = 0;
-        steps.push_back(std::move(st));
-        if (pair) ++k;
+        step_first.push_back(k);
+        if (pair) {
+            s.step_w[k] = 2;
+            s.step_w[k + 1] = 0;
+            s.max_c = std::max(s.max_c, c - 1);
+            k += 2;
+        } else {
+            k += 1;
+        }
     }
-    s.u_off[d] = upos;
-    s.a_len = apos;
-    s.u_len = upos;
-    s.slots_f = sa.hi;
-    s.n_steps = static_cast<int>(steps.size());
-    if (!slot_of.empty()) throw std::logic_error("symbolic: update targets left unconsumed");
+    s.n_steps = static_cast<int>(step_first.size());
+    s.u_off.assign(static_cast<size_t>(d) + 1, 0);
+    const int W = std::max(1, std::min(walkers, kMaxWalkers));
+    const std::vector<int> group = partition_steps(s, parent, step_first, W);
+
+    Emitter e1(s, a_entry), e0(s, a_entry);
+    std::vector<Emitter*> em{&e0, &e1};
+    // first phase: walker 1's segment first, then walker 0's (whose segment
+    // continues with the top part)
+    int a_off0 = 0, u_off0 = 0;
+    if (W > 1) {
+        for (int k : step_first)
+            if (group[k] == 1) e1.step(k);
+        a_off0 = round_up_to(e1.apos, kSegAlign);
+        u_off0 = round_up_to(e1.upos, kSegAlign);
+    }
+    e0.apos = a_off0;
+    e0.upos = u_off0;
+    for (int k : step_first)
+        if (group[k] == 0) e0.step(k);
+    const int n_phase1_0 = e0.n_piv;
+    int slot_base1 = 0;
+    if (W > 1) {
+        // merge walker 1's pending contributions into walker 0's numbering:
+        // an entry both hold is summed by a merge record, an entry only walker
+        // 1 holds keeps its slot
+        slot_base1 = e0.sa.hi;
+        std::vector<int> merges;
+        std::set<int> adopted;
+        for (const auto& kv : e1.slot_of) {
+            const int g1 = kv.second + slot_base1;
+            auto it = e0.slot_of.find(kv.first);
+            if (it != e0.slot_of.end()) {
+                merges.push_back(it->second);
+                merges.push_back(g1);
+            } else {
+                e0.slot_of[kv.first] = g1;
+                adopted.insert(g1);
+            }
+        }
+        for (int i = 0; i < d; ++i) {
+            if (e1.bslot[i] < 0) continue;
+            const int g1 = e1.bslot[i] + slot_base1;
+            if (e0.bslot[i] >= 0) {
+                merges.push_back(e0.bslot[i]);
+                merges.push_back(g1);
+            } else {
+                e0.bslot[i] = g1;
+                adopted.insert(g1);
+            }
+        }
+        // the merge records (at least one: it carries the wait for walker 1)
+        const size_t npairs = merges.size() / 2;
+        for (size_t q = 0; q == 0 || q < npairs; q += kMergePairs) {
+            Rec m;
+            m.kind = Rec::kMerge;
+            const size_t q1 = std::min(npairs, q + kMergePairs);
+            m.r.assign(merges.begin() + 2 * q, merges.begin() + 2 * q1);
+            e0.recs.push_back(std::move(m));
+            if (npairs == 0) break;
+        }
+        e1.slot_of.clear();
+        // the top part allocates from every slot not holding a live value
+        const int hi1 = e1.sa.hi;
+        e0.sa.hi = slot_base1 + hi1;
+        for (int j = 0; j < hi1; ++j)
+            if (!adopted.count(slot_base1 + j)) e0.sa.put(slot_base1 + j);
+        for (int k : step_first)
+            if (group[k] == W) e0.step(k);
+    }
+    if (e0.n_piv + e1.n_piv != d) throw std::logic_error("symbolic: walkers do not cover every pivot");
+    if (!e0.slot_of.empty()) throw std::logic_error("symbolic: update targets left unconsumed");
+    for (int i = 0; i < d; ++i)
+        if (e0.bslot[i] >= 0) throw std::logic_error("symbolic: forward-substitution targets left unconsumed");
+    s.slots_f = e0.sa.hi;
+    s.a_len = e0.apos;
+    s.u_len = e0.upos;
+    s.u_off[d] = s.u_len;
+    s.walkers = W;
+    s.n_top = W > 1 ? e0.n_piv - n_phase1_0 : 0;
+    s.seg.assign(static_cast<size_t>(W), SymbolicLU::Seg{});
+    s.seg[0].a_off = a_off0;
+    s.seg[0].a_len = e0.apos - a_off0;
+    s.seg[0].u_off = u_off0;
+    s.seg[0].u_len = e0.upos - u_off0;
+    s.seg[0].n_piv = e0.n_piv;
+    if (W > 1) {
+        s.seg[1].a_len = e1.apos;
+        s.seg[1].u_len = e1.upos;
+        s.seg[1].n_piv = e1.n_piv;
+    }
     // encode: refs as byte offsets; a missing source reads the zero slot
-    const auto enc = [](int slot) { return slot * kRefUnit; };
     s.prog_f.clear();
-    s.prog_f_off.assign(static_cast<size_t>(d) + 1, 0);
-    for (size_t q = 0; q < steps.size(); ++q) {
-        const Step& st = steps[q];
-        s.prog_f_off[st.k] = static_cast<int>(s.prog_f.size());
-        if (st.w == 2) s.prog_f_off[st.k + 1] = s.prog_f_off[st.k];
-        const int c = st.c;
-        const int next = q + 1 < steps.size() ? rec_head(steps[q + 1].c, steps[q + 1].w) : 0;
-        const int ns = st.w == 2 ? frec2_src_words(c) : frec_src_words(c);
-        const size_t at = s.prog_f.size();
-        s.prog_f.resize(at + static_cast<size_t>(st.w == 2 ? frec2_words(c) : frec_words(c)), enc(s.slots_f));
-        int* w = s.prog_f.data() + at;
-        w[0] = rec_head(c, st.w) | (next << 16);
-        const std::vector<int>& r = st.r;
-        const int nsrc = ns - 1;
-        for (int q2 = 0; q2 < nsrc; ++q2) w[1 + q2] = enc(r[q2] < 0 ? s.slots_f : (r[q2] & kSlotMask));
-        for (int i = 0; i < c * c + c; ++i) w[ns + i] = enc(r[nsrc + i] & kSlotMask);
+    for (int w = 0; w < W; ++w) {
+        s.seg[w].prog_f_off = static_cast<int>(s.prog_f.size());
+        em[w]->encode(s.prog_f, w == 1 ? slot_base1 : 0, s.slots_f);
+        s.prog_f.resize(static_cast<size_t>(round_up_to(static_cast<int>(s.prog_f.size()), kProgChunk)),
+                        s.slots_f * kRefUnit);
+        s.seg[w].prog_f_len = static_cast<int>(s.prog_f.size()) - s.seg[w].prog_f_off;
     }
-    s.prog_f_off[d] = static_cast<int>(s.prog_f.size());
-    build_backward(s);
+    build_backward(s, group, W);
     return s;
 }
 
 namespace {
 
-// Backward program, one record per pivot in processing order d-1 .. 0:
-// header word (c | record words << 16), perm[k], then 16-bit refs: the slots
-// of x(j), j in struct_k, and the slot x(k) is kept in (the trash slot when
-// no later pivot reads it).
-void build_backward(SymbolicLU& s)
+// Cuts the elimination forest (at step granularity) into W groups of whole
+// subtrees plus a top part (group W) and returns the group of every pivot.
+// A threshold T puts every step whose subtree costs more than T into the top;
+// the remaining maximal subtrees are packed into W bins, largest first into
+// the lightest bin.  The T of the smallest makespan (largest bin + top) wins.
+std::vector<int> partition_steps(const SymbolicLU& s, const std::vector<int>& parent,
+                                 const std::vector<int>& step_first, int W)
+{
+    const int d = s.d;
+    std::vector<int> group(static_cast<size_t>(d), 0);
+    if (W <= 1) return group;
+    const int ns = static_cast<int>(step_first.size());
+    std::vector<int> step_of(static_cast<size_t>(d));
+    for (int q = 0; q < ns; ++q) {
+        const int k = step_first[q];
+        step_of[k] = q;
+        if (s.step_w[k] == 2) step_of[k + 1] = q;
+    }
+    std::vector<int> sparent(static_cast<size_t>(ns), -1);
+    std::vector<double> cost(static_cast<size_t>(ns)), sub(static_cast<size_t>(ns));
+    for (int q = 0; q < ns; ++q) {
+        const int k = step_first[q];
+        const int last = s.step_w[k] == 2 ? k + 1 : k;
+        const int c = s.colp[last + 1] - s.colp[last];
+        cost[q] = 3.0 + c + 0.25 * c * c;  // a step's walk time, roughly
+        if (parent[last] >= 0) sparent[q] = step_of[parent[last]];
+    }
+    for (int q = 0; q < ns; ++q) sub[q] += cost[q];  // children precede parents
+    for (int q = 0; q < ns; ++q)
+        if (sparent[q] >= 0) sub[sparent[q]] += sub[q];
+    std::vector<double> cand(sub);
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    double best = 1e300, bestT = cand.back();
+    const size_t stride = std::max<size_t>(1, cand.size() / 512);
+    for (size_t ci = 0; ci < cand.size(); ci += stride) {
+        const double T = cand[ci];
+        double top = 0.0;
+        std::vector<double> parts;
+        for (int q = 0; q < ns; ++q) {
+            if (sub[q] > T) {
+                top += cost[q];
+            } else if (sparent[q] < 0 || sub[sparent[q]] > T) {
+                parts.push_back(sub[q]);
+            }
+        }
+        std::sort(parts.begin(), parts.end(), std::greater<double>());
+        std::vector<double> bins(static_cast<size_t>(W), 0.0);
+        for (double x : parts) *std::min_element(bins.begin(), bins.end()) += x;
+        const double mk = *std::max_element(bins.begin(), bins.end()) + top;
+        if (mk < best) { best = mk; bestT = T; }
+    }
+    // assign: part roots largest first into the lightest bin, then propagate
+    std::vector<int> sg(static_cast<size_t>(ns), -1);
+    std::vector<std::pair<double, int>> roots;
+    for (int q = 0; q < ns; ++q) {
+        if (sub[q] > bestT) sg[q] = W;
+        else if (sparent[q] < 0 || sub[sparent[q]] > bestT) roots.push_back({sub[q], q});
+    }
+    std::sort(roots.begin(), roots.end(), [](const auto& a, const auto& b) {
+        return a.first != b.first ? a.first > b.first : a.second < b.second;
+    });
+    std::vector<double> bins(static_cast<size_t>(W), 0.0);
+    for (const auto& rt : roots) {
+        const int b = static_cast<int>(std::min_element(bins.begin(), bins.end()) - bins.begin());
+        bins[b] += rt.first;
+        sg[rt.second] = b;
+    }
+    for (int q = ns - 1; q >= 0; --q)  // parents precede children in reverse order
+        if (sg[q] < 0) sg[q] = sg[sparent[q]];
+    for (int k = 0; k < d; ++k) group[k] = sg[step_of[k]];
+    return group;
+}
+
+// Backward programs, one per walker, records in processing order: walker 0
+// walks the top part (last pivot first) and then its subtrees; walker w > 0
+// waits for the top and walks its subtrees.  Header word (c | next << 16),
+// perm[k], then the refs of x(j), j in struct_k, and the slot x(k) is kept in
+// (the trash slot when no later step reads it).  The x of a top pivot is read
+// by every walker and keeps its slot to the end.
+void build_backward(SymbolicLU& s, const std::vector<int>& group, int W)
 {
     const int d = s.d;
     std::vector<int> last(static_cast<size_t>(d), d);  // smallest pivot reading x(j)
     for (int k = 0; k < d; ++k)
         for (int p = s.colp[k]; p < s.colp[k + 1]; ++p) last[s.coli[p]] = std::min(last[s.coli[p]], k);
-    SlotAlloc sa;
-    std::vector<int> xslot(static_cast<size_t>(d), -1);
-    struct Rec {
+    struct BRec {
         int k, w, c;  // first pivot of the step, width, |struct| of its last pivot
         std::vector<int> r;
     };
-    std::vector<Rec> recs;
-    recs.reserve(static_cast<size_t>(d));
-    // steps in reverse: a 2-column supernode (k, k+1) solves x(k+1), then x(k)
-    // with x(k+1) in registers
-    for (int kk = d - 1; kk >= 0; --kk) {
-        const bool pair = s.step_w[kk] == 0;  // kk is the second pivot of a pair
-        const int k = pair ? kk - 1 : kk;
-        const int last_p = pair ? kk : k;  // the pivot whose struct the step reads
-        const int c0 = s.colp[last_p], c = s.colp[last_p + 1] - c0;
-        Rec rec{k, pair ? 2 : 1, c, {}};
-        std::vector<int>& r = rec.r;
-        std::vector<int> to_free;
-        for (int p = c0; p < c0 + c; ++p) {
-            const int j = s.coli[p];
-            if (xslot[j] < 0) throw std::logic_error("backward: x(j) read before it is computed");
-            r.push_back(xslot[j]);
-            if (last[j] == k) to_free.push_back(xslot[j]);
-        }
-        // every read of this step happens before its results are stored, so a
-        // slot freed here may hold one of them
-        for (int sl : to_free) sa.put(sl);
-        auto out_slot = [&](int j) {
-            int out = -1;
-            if (last[j] < k) {
-                out = sa.get();
-                xslot[j] = out;
+    std::vector<std::vector<BRec>> recs(static_cast<size_t>(W));
+    std::vector<int> xslot(static_cast<size_t>(d), -1);
+    // pools: the top part's x slots first, then one pool per walker
+    SlotAlloc top_sa;
+    std::vector<SlotAlloc> pool(static_cast<size_t>(W));
+    auto walk = [&](int wk, bool top_phase, SlotAlloc& sa, int base) {
+        for (int kk = d - 1; kk >= 0; --kk) {
+            const bool pair = s.step_w[kk] == 0;  // kk is the second pivot of a pair
+            const int k = pair ? kk - 1 : kk;
+            kk = k;
+            const int g = group[k];
+            if (top_phase ? g != W : g != wk) continue;
+            const int last_p = pair ? k + 1 : k;
+            const int c0 = s.colp[last_p], c = s.colp[last_p + 1] - c0;
+            BRec rec{k, pair ? 2 : 1, c, {}};
+            std::vector<int> to_free;
+            for (int p = c0; p < c0 + c; ++p) {
+                const int j = s.coli[p];
+                if (xslot[j] < 0) throw std::logic_error("backward: x(j) read before it is computed");
+                rec.r.push_back(xslot[j]);
+                if (last[j] == k && group[j] != W) to_free.push_back(xslot[j]);
             }
-            r.push_back(out);
-        };
-        if (pair) out_slot(k + 1);  // x(k+1): read by k in registers; a slot only for earlier pivots
-        out_slot(k);
-        recs.push_back(std::move(rec));
-        kk = k;
-    }
-    s.slots_b = sa.hi;
-    const auto enc = [](int slot) { return slot * kRefUnit; };
-    s.prog_b.clear();
-    for (size_t q = 0; q < recs.size(); ++q) {
-        const Rec& rec = recs[q];
-        const int c = rec.c;
-        const int next = q + 1 < recs.size() ? rec_head(recs[q + 1].c, recs[q + 1].w) : 0;
-        const size_t at = s.prog_b.size();
-        const int nw = rec.w == 2 ? brec2_words(c) : brec_words(c);
-        s.prog_b.resize(at + static_cast<size_t>(nw), enc(s.slots_b + 1));
-        int* w = s.prog_b.data() + at;
-        w[0] = rec_head(c, rec.w) | (next << 16);
-        if (rec.w == 2) {
-            w[1] = s.perm[rec.k + 1];
-            w[2] = s.perm[rec.k];
-            for (size_t i = 0; i < rec.r.size(); ++i) w[3 + i] = enc(rec.r[i] < 0 ? s.slots_b + 1 : rec.r[i]);
-        } else {
-            w[1] = s.perm[rec.k];
-            for (size_t i = 0; i < rec.r.size(); ++i) w[2 + i] = enc(rec.r[i] < 0 ? s.slots_b + 1 : rec.r[i]);
+            // every read of this step happens before its results are stored,
+            // so a slot freed here may hold one of them
+            for (int sl : to_free) sa.put(sl - base);
+            auto out_slot = [&](int j) {
+                int out = -1;
+                if (last[j] < k) {
+                    out = sa.get() + base;
+                    xslot[j] = out;
+                }
+                rec.r.push_back(out);
+            };
+            if (pair) out_slot(k + 1);  // x(k+1): read by k in registers; a slot only for earlier pivots
+            out_slot(k);
+            recs[static_cast<size_t>(wk)].push_back(std::move(rec));
         }
+    };
+    walk(0, true, top_sa, 0);
+    int base = top_sa.hi;
+    for (int w = 0; w < W; ++w) {
+        walk(w, false, pool[w], base);
+        base += pool[w].hi;
+    }
+    s.slots_b = base;
+    const auto enc = [&](int slot) { return (slot < 0 ? s.slots_b + 1 : slot) * kRefUnit; };
+    s.prog_b.clear();
+    for (int w = 0; w < W; ++w) {
+        s.seg[w].prog_b_off = static_cast<int>(s.prog_b.size());
+        const auto& rv = recs[static_cast<size_t>(w)];
+        for (size_t q = 0; q < rv.size(); ++q) {
+            const BRec& rec = rv[q];
+            const int c = rec.c;
+            const int next = q + 1 < rv.size() ? rec_head(rv[q + 1].c, rv[q + 1].w) : 0;
+            const size_t at = s.prog_b.size();
+            const int nw = rec.w == 2 ? brec2_words(c) : brec_words(c);
+            s.prog_b.resize(at + static_cast<size_t>(nw), enc(-1));
+            int* wd = s.prog_b.data() + at;
+            wd[0] = rec_head(c, rec.w) | (next << 16);
+            if (rec.w == 2) {
+                wd[1] = s.perm[rec.k + 1];
+                wd[2] = s.perm[rec.k];
+                for (size_t i = 0; i < rec.r.size(); ++i) wd[3 + i] = enc(rec.r[i]);
+            } else {
+                wd[1] = s.perm[rec.k];
+                for (size_t i = 0; i < rec.r.size(); ++i) wd[2 + i] = enc(rec.r[i]);
+            }
+        }
+        s.prog_b.resize(static_cast<size_t>(round_up_to(static_cast<int>(s.prog_b.size()), kProgChunk)), enc(-1));
+        s.seg[w].prog_b_len = static_cast<int>(s.prog_b.size()) - s.seg[w].prog_b_off;
     }
 }
 
 }  // namespace
 
 NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int npv, const int* pv,
-                     int npq, const int* pq, int slack)
+                     int npq, const int* pq, int slack, int walkers)
 {
     if (n <= 1) throw std::invalid_argument("the grid needs at least two buses");
     NrPlan P;
@@ -603,7 +844,9 @@ NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int 
         order.push_back(P.row_p[i]);
         if (P.row_q[i] >= 0) order.push_back(P.row_q[i]);
     }
-    P.lu = build_symbolic(d, P.Jp.data(), P.Ji.data(), order);
+    // walkers: two for large systems (the kernels keep one otherwise);
+    // G2G_WALKERS=1|2 overrides
+    P.lu = build_symbolic(d, P.Jp.data(), P.Ji.data(), order, walkers > 0 ? walkers : default_walkers(d));
 
     // mismatch/Jacobian program: buses in RCM order of the Ybus graph
     {

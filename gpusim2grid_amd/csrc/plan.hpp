@@ -54,6 +54,18 @@ struct SymbolicLU {
     // U stream: per pivot [y(k)] [U(k,k)] [U(k,j)...]
     int u_len = 0;
     std::vector<int> u_off;        // [d+1] start of pivot k's U record
+    // walkers (program_format.hpp): per walker, its program, A-stream and
+    // U-stream segments (offsets into prog_f / prog_b in words, into a tile's
+    // streams in positions) and the pivots it walks; walker 0 walks the top
+    // part's n_top pivots after its first phase (factorization) or before it
+    // (backward substitution)
+    struct Seg {
+        int prog_f_off = 0, prog_f_len = 0, prog_b_off = 0, prog_b_len = 0;
+        int a_off = 0, a_len = 0, u_off = 0, u_len = 0, n_piv = 0;
+    };
+    int walkers = 1;
+    int n_top = 0;
+    std::vector<Seg> seg;
     // backward substitution program (one record per pivot, pivots d-1 .. 0):
     //   c, xref[c] (slot of x(j) for j in struct_k), out (slot for x(k) or none),
     //   op, and for op != 0 two further ints (see below)
@@ -64,9 +76,8 @@ struct SymbolicLU {
 // Builds the symbolic factorization of a square CSR pattern (structurally
 // symmetrised).  `order` is the pivot order (a permutation of 0..d-1) before
 // the postorder relabelling; empty = minimum degree on the pattern itself.
-// `group` (optional, size d): unknowns with the same group id are eliminated
-// as one supervariable by the minimum-degree order.
-SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> order = {});
+// `walkers` (1 or 2) splits each tile's walk over that many walker wavefronts.
+SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> order = {}, int walkers = 1);
 
 // Minimum degree order of an undirected graph given by symmetric adjacency
 // lists (CSR, no self loops) with vertex weights; returns the elimination order.
@@ -97,7 +108,12 @@ struct NrPlan {
     int k1_nbus = 0;
 };
 
+// walkers <= 0: default_walkers(d)
+constexpr int kWalkersMinD = 2000;
+// two walkers when d >= kWalkersMinD, else one; G2G_WALKERS=1|2 in the
+// environment overrides
+int default_walkers(int d);
 NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int npv, const int* pv,
-                     int npq, const int* pq, int slack);
+                     int npq, const int* pq, int slack, int walkers = 0);
 
 }  // namespace g2g

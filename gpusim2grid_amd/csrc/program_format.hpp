@@ -22,7 +22,6 @@ constexpr int kRingSlotsF32 = 8;    // 64 KB ring
 constexpr int kValueProducers = 6;  // wavefronts streaming a value stream (one chunk in flight each)
 constexpr int kProgChunk = 512;     // program words per chunk (2 KB)
 constexpr int kProgSlots = 4;       // 8 KB ring
-constexpr int kStreamWaves = 2 + kValueProducers;  // walker + value producers + program producer
 
 // Programs (one record per pivot, padded to 4 words so the kernels read a
 // record with 16-byte loads):
@@ -58,6 +57,43 @@ constexpr int kFastC2 = 6;          // longest struct(k+1) of a 2-column superno
 __host__ __device__ constexpr int brec_words(int c) { return round4(3 + c); }
 // backward 2-column supernode: [head, perm[k+1], perm[k], x(S)[c], out(k+1), out(k)]
 __host__ __device__ constexpr int brec2_words(int c) { return round4(5 + c); }
+
+// Walkers.  A tile's walk can be split over kMaxWalkers walker wavefronts:
+// the host cuts the elimination forest into independent subtrees (each
+// walker's first phase) and a small top part.  Walker 0 then waits for the
+// others, merges their pending contributions to the top entries with merge
+// records and walks the top; the backward substitution walks the top first
+// (walker 0) and then the subtrees in parallel.  Each walker has its own
+// program, A-stream and U-stream segment; segments start at multiples of
+// kSegAlign positions / kProgChunk words.
+constexpr int kMaxWalkers = 2;
+constexpr int kSegAlign = 32;
+// merge record: [kMergeHead | next head << 16, n, (dst ref, src ref) x n]
+// (dst += src, src = 0); the first one also waits for the other walkers
+constexpr int kMergeHead = 0x7FFE;
+constexpr int kMergePairs = 31;
+__host__ __device__ constexpr int mrec_words(int n) { return round4(2 + 2 * n); }
+// walker kernels' LDS layout: per walker a 256-byte control area (value and
+// program ring), then kWalkFlagBytes of cross-walker flags, the W value
+// rings, the W program rings, and the slots
+constexpr int kWalkerCtlBytes = 256;
+constexpr int kWalkFlagBytes = 1024;
+// value-stream producers per workgroup: six for one walker, two per walker
+// for two (eight waves either way, which leaves 256 VGPRs per wave)
+__host__ __device__ constexpr int value_producers(int W) { return W == 1 ? kValueProducers : 2 * W; }
+__host__ __device__ constexpr int walk_waves(int W) { return 2 * W + value_producers(W); }
+// value ring geometry of a walker: positions per chunk and ring slots
+__host__ __device__ constexpr int value_chunk(int real_size) { return real_size == 8 ? kChunkF64 : kChunkF32; }
+__host__ __device__ constexpr int value_ring_slots(int real_size, int W)
+{
+    return W == 1 ? (real_size == 8 ? kRingSlotsF64 : kRingSlotsF32) : 4;
+}
+__host__ __device__ constexpr int walk_ring_bytes(int real_size, int W)
+{
+    return W * (kWalkerCtlBytes + value_chunk(real_size) * value_ring_slots(real_size, W) * kLanes * real_size +
+                kProgChunk * kProgSlots * 4) +
+           kWalkFlagBytes;
+}
 
 // row status (gpusim2grid contingency/row_status.hpp:33-43) plus "still iterating"
 constexpr int kActive = -1;

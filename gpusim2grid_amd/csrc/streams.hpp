@@ -107,7 +107,7 @@ struct Ring {
                     lds_store_release(&ctl->abort, 1);
                     return;
                 }
-                __builtin_amdgcn_s_sleep(2);
+                __builtin_amdgcn_s_sleep(32);
             }
             // data slot by global chunk (what Reader::at addresses), tag by
             // consumption-order chunk (what Reader::wait checks)

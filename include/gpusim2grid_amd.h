@@ -54,6 +54,7 @@ extern "C" {
 #define G2G_ARR_APOS 9      /* A-stream position of each input CSR entry [nnz] */
 #define G2G_ARR_FPOS 10     /* A-stream position of each rhs entry [d]        */
 #define G2G_ARR_UOFF 11     /* U-stream record start per pivot [d+1]          */
+#define G2G_ARR_SEGS 12     /* per walker: prog_f off/len, prog_b off/len, A off/len, U off/len, pivots [9 walkers] */
 
 typedef struct g2g_plan_s *g2g_plan_t;
 typedef struct g2g_batch_s *g2g_batch_t;
@@ -65,6 +66,7 @@ typedef struct {
     int32_t nLU, n_updates, max_c, height;   /* symbolic factorization                */
     int32_t slots_f, slots_b, a_len, u_len;  /* program resources                     */
     int32_t n_steps;                         /* factorization walk steps (pivots and 2-column supernodes) */
+    int32_t walkers, n_top;                  /* walker wavefronts per tile; pivots of the top part */
 } g2g_plan_info;
 
 typedef struct {

@@ -79,17 +79,17 @@ def _interp_refined(A, b, ir_steps, eps):
     apos, fpos = P.array("apos"), P.array("fpos")
     scale = np.array([np.abs(A.data).max()])
     Ast = pi.astream_from_csr(inf, apos, fpos, A.data[None], b[None])
-    U, cl = pi.factor_forward(inf, P.array("prog_f"), Ast, pivot_eps=eps, scale=scale)
-    x = pi.backward(inf, P.array("prog_b"), U)[0]
+    U, cl = pi.factor_forward(P, Ast, pivot_eps=eps, scale=scale)
+    x = pi.backward(P, U)[0]
     for _ in range(ir_steps):
         r = b - A @ x
-        U2, _ = pi.factor_forward(inf, P.array("prog_f"), pi.astream_from_csr(inf, apos, fpos, A.data[None], r[None]),
+        U2, _ = pi.factor_forward(P, pi.astream_from_csr(inf, apos, fpos, A.data[None], r[None]),
                                   pivot_eps=eps, scale=scale)
         # the same factors: the U streams differ only in the y entries
         keep = np.ones(U.shape[1], bool)
         keep[P.array("uoff")[:-1]] = False
         np.testing.assert_array_equal(U2[:, keep], U[:, keep])
-        x = x + pi.backward(inf, P.array("prog_b"), U2)[0]
+        x = x + pi.backward(P, U2)[0]
     return x, cl[0]
 
 

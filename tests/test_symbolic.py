@@ -26,6 +26,13 @@ def grid(request):
     return {"c0": syn.grid_c0, "118": syn.grid_118, "c1": syn.grid_c1}[request.param]()
 
 
+@pytest.fixture(params=[1, 2])
+def walkers(request, monkeypatch):
+    """Plans built with one walker, or with the walk split over two."""
+    monkeypatch.setenv("G2G_WALKERS", str(request.param))
+    return request.param
+
+
 def test_jacobian_layout_matches_oracle(grid):
     P = _capi.Plan.from_grid(grid.Ybus, grid.pv, grid.pq, grid.slack)
     Jp, Ji, maps = orc.jpattern(grid.Ybus, grid.pv, grid.pq)
@@ -73,16 +80,17 @@ def _perturbed(g, seed=0):
     return g.flat_start() * np.exp(1j * rng.normal(0, 0.05, g.n_bus)) * (1 + rng.normal(0, 0.02, g.n_bus))
 
 
-def test_programs_factor_like_superlu_natural(grid):
+def test_programs_factor_like_superlu_natural(grid, walkers):
     """The static-pivot LU under the plan's order equals SuperLU's factors of
     P A P^T with no pivoting (natural order, diag_pivot_thresh=0)."""
     if grid.n_bus > 200:
         pytest.skip("dense comparison on small grids only")
     P = _capi.Plan.from_grid(grid.Ybus, grid.pv, grid.pq, grid.slack)
     inf = P.info()
+    assert inf["walkers"] == walkers
     F, J, _ = orc.f_and_j(grid.Ybus, _perturbed(grid), grid.Sbus, grid.pv, grid.pq)
     A = pi.astream_from_csr(inf, P.array("apos"), P.array("fpos"), J.data[None], F[None])
-    U, cl = pi.factor_forward(inf, P.array("prog_f"), A)
+    U, cl = pi.factor_forward(P, A)
     assert cl[0] == 0
     perm = P.array("perm")
     Jperm = J.toarray()[np.ix_(perm, perm)]
@@ -93,12 +101,12 @@ def test_programs_factor_like_superlu_natural(grid):
     uoff = P.array("uoff")
     colp, coli = P.array("colp"), P.array("coli")
     for k in range(inf["d"]):
-        rec = U[0, uoff[k]:uoff[k + 1]]
+        rec = U[0, uoff[k]:uoff[k] + 2 + colp[k + 1] - colp[k]]
         assert rec[1] == pytest.approx(1.0 / Uref[k, k], rel=1e-12)  # stored as the reciprocal
         np.testing.assert_allclose(rec[2:], Uref[k, coli[colp[k]:colp[k + 1]]], rtol=1e-11, atol=1e-12)
 
 
-def test_raw_program_solves(grid):
+def test_raw_program_solves(grid, walkers):
     F, J, _ = orc.f_and_j(grid.Ybus, _perturbed(grid, 1), grid.Sbus, grid.pv, grid.pq)
     P = _capi.Plan.from_matrix(J)
     inf = P.info()
@@ -106,22 +114,22 @@ def test_raw_program_solves(grid):
     vals = np.stack([J.data * (1 + 0.01 * k) for k in range(rows)])
     rhs = np.stack([F * (k + 1) for k in range(rows)])
     A = pi.astream_from_csr(inf, P.array("apos"), P.array("fpos"), vals, rhs)
-    U, _ = pi.factor_forward(inf, P.array("prog_f"), A)
-    x = pi.backward(inf, P.array("prog_b"), U)
+    U, _ = pi.factor_forward(P, A)
+    x = pi.backward(P, U)
     for k in range(rows):
         Jk = sp.csr_matrix((vals[k], J.indices, J.indptr), shape=J.shape)
         xs = spla.spsolve(Jk.tocsc(), rhs[k])
         np.testing.assert_allclose(x[k], xs, rtol=1e-10, atol=1e-10 * np.abs(xs).max())
 
 
-def test_nr_program_applies_the_polar_update(grid):
+def test_nr_program_applies_the_polar_update(grid, walkers):
     V = _perturbed(grid, 2)
     P = _capi.Plan.from_grid(grid.Ybus, grid.pv, grid.pq, grid.slack)
     inf = P.info()
     F, J, _ = orc.f_and_j(grid.Ybus, V, grid.Sbus, grid.pv, grid.pq)
     A = pi.astream_from_csr(inf, P.array("apos"), P.array("fpos"), J.data[None], F[None])
-    U, _ = pi.factor_forward(inf, P.array("prog_f"), A)
-    x = pi.backward(inf, P.array("prog_b"), U)[0]
+    U, _ = pi.factor_forward(P, A)
+    x = pi.backward(P, U)[0]
     Vn = pi.polar_update(V, x, P.array("pvpq"), grid.pq)
     dx = spla.spsolve(J.tocsc(), F)
     np.testing.assert_allclose(Vn, orc.polar_update_np(V, dx, grid.pv, grid.pq), rtol=0, atol=1e-10)
@@ -137,9 +145,9 @@ def test_saddle_matrix_clamp_is_counted():
     np.testing.assert_array_equal(P.array("perm"), [0, 1, 2])
     b = np.array([1.0, 2.0, 3.0])
     Ast = pi.astream_from_csr(inf, P.array("apos"), P.array("fpos"), A.data[None], b[None])
-    U, cl = pi.factor_forward(inf, P.array("prog_f"), Ast, pivot_eps=1e-12, scale=np.array([2.0]))
+    U, cl = pi.factor_forward(P, Ast, pivot_eps=1e-12, scale=np.array([2.0]))
     assert cl[0] == 1
-    x = pi.backward(inf, P.array("prog_b"), U)[0]
+    x = pi.backward(P, U)[0]
     r = np.abs(A @ x - b).max() / np.abs(b).max()
     assert r > 1e-8  # the clamp is visible
 
@@ -158,10 +166,50 @@ def test_invalid_inputs_raise_value_error():
         _capi.Plan.from_grid(Y, g.pv, g.pq, g.slack)  # no diagonal entry
 
 
-def test_slots_stay_small_on_the_large_grids():
-    """The postorder keeps the live update set tiny (LDS budget of the kernels)."""
+LDS_BYTES = 160 * 1024
+
+
+def walk_lds(inf, real_size):
+    """The factorization kernel's LDS (program_format.hpp walk_ring_bytes + slots)."""
+    W = inf["walkers"]
+    chunk = 16 if real_size == 8 else 32
+    ring_slots = 8 if W == 1 else 4
+    rings = W * (256 + chunk * ring_slots * 64 * real_size + 512 * 4 * 4) + 1024
+    scratch = 2 * inf["max_c"] * W if inf["max_c"] > 7 else 0
+    return rings + (inf["slots_f"] + 2 + scratch) * 64 * real_size
+
+
+def test_slots_stay_small_on_the_large_grids(monkeypatch):
+    """The postorder keeps the live update set small: one walker fits the LDS
+    in FP64, and the 6,500-bus grid's two-walker split fits it in FP32."""
     for g in (syn.grid_c1(), syn.grid_c2()):
+        monkeypatch.setenv("G2G_WALKERS", "1")
         inf = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack).info()
         assert inf["max_c"] <= 16
-        assert (inf["slots_f"] + 2) * 64 * 8 <= 96 * 1024
+        assert walk_lds(inf, 8) <= LDS_BYTES
         assert inf["nLU"] < 1.2 * inf["nJ"]
+    monkeypatch.setenv("G2G_WALKERS", "2")
+    g = syn.grid_c2()
+    inf = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack).info()
+    assert walk_lds(inf, 4) <= LDS_BYTES
+
+
+def test_two_walkers_split_the_tree_evenly(monkeypatch):
+    """The 6,500-bus grid's walk splits into two halves within 5 % of each
+    other and a top part of a few pivots; segments tile the streams."""
+    monkeypatch.setenv("G2G_WALKERS", "2")
+    g = syn.grid_c2()
+    P = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+    inf = P.info()
+    seg = P.array("segs").reshape(-1, 9)
+    assert inf["walkers"] == 2 and seg.shape[0] == 2
+    d = inf["d"]
+    assert seg[:, 8].sum() == d and 0 < inf["n_top"] < 0.01 * d
+    first = seg[:, 8] - np.array([inf["n_top"], 0])
+    assert abs(first[0] - first[1]) < 0.05 * d / 2
+    # walker 1's A and U segments come first, walker 0's start at aligned offsets
+    assert seg[1, 4] == 0 and seg[0, 4] >= seg[1, 5] and seg[0, 4] % 32 == 0
+    assert seg[1, 6] == 0 and seg[0, 6] >= seg[1, 7] and seg[0, 6] % 32 == 0
+    assert seg[0, 4] + seg[0, 5] == inf["a_len"] and seg[0, 6] + seg[0, 7] == inf["u_len"]
+    # program segments are whole prefetch chunks
+    assert (seg[:, [0, 1, 2, 3]] % 512 == 0).all()
