@@ -135,6 +135,7 @@ def run_method(args, name, g, S, torch, capi, ws=None):
         analysis_s=t_plan, pfps_incl_analysis=conv / (dt / args.steps + t_plan),
         converged=conv, rows=args.rows, iterations=stats[-1]["iterations"],
         mean_row_iters=float(it[st == 0].mean()) if conv else None,
+        iters_hist=np.bincount(it[st == 0]).tolist() if conv else None,
         max_residual=float(res[st == 0].max()) if conv else None,
         ms_mismatch=np.mean([s["ms_mismatch"] for s in stats]), ms_factor=ms_factor / args.steps,
         ms_backward=np.mean([s["ms_backward"] for s in stats]),

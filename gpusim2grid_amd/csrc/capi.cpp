@@ -174,6 +174,17 @@ struct g2g_batch_s {
     double max_dva = 0.0, max_dvm = 0.0;  // step scaling bounds (max_dva <= 0: off)
     void* Bs = nullptr;               // refinement: saved rhs and step [T][d][64]
     void* Xs = nullptr;
+    // tail compaction: the full batch's V, Sbus, per-row Ybus, statuses,
+    // iteration counts and residuals while the few rows left iterate in
+    // dense tiles, and the row map (compacted row -> batch row)
+    double2* Vk = nullptr;
+    double2* Sk = nullptr;
+    double2* Yk = nullptr;
+    int* stk = nullptr;
+    int* itk = nullptr;
+    double* resk = nullptr;
+    int* mapk = nullptr;
+    int compact_R = 0, compact_saved = 0;
     std::vector<hipEvent_t> ev;
     ~g2g_batch_s()
     {
@@ -181,6 +192,7 @@ struct g2g_batch_s {
         dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped); dfree(error);
         dfree(V0); dfree(Vc); dfree(Sc); dfree(Yc); dfree(st_save); dfree(it_save); dfree(res_save);
         dfree(row_map); dfree(Bs); dfree(Xs);
+        dfree(Vk); dfree(Sk); dfree(Yk); dfree(stk); dfree(itk); dfree(resk); dfree(mapk);
         if (h_active) (void)hipHostFree(h_active);
         for (auto e : ev) (void)hipEventDestroy(e);
     }
@@ -385,11 +397,91 @@ struct SolveTimer {
     }
 };
 
-// Newton iterations on the batch's active rows until none is left or the
-// budget is spent; returns the iterations launched.
+// Tail compaction.  The LU walk of a tile costs the same whatever number of
+// its rows are active, but the mismatch and update kernels cost in
+// proportion to the active tiles: once few rows are left, they are gathered
+// into dense tiles (V, Sbus, per-row Ybus, iteration counts, residuals), the
+// iterations continue there, and the results are scattered back.
+void compact_rows(g2g_batch_s* b)
+{
+    const NrPlan& nr = b->plan->nr;
+    const size_t rp = static_cast<size_t>(b->rows_pad);
+    hipStream_t s = b->stream;
+    if (!b->Vk) {
+        b->Vk = dalloc<double2>(rp * nr.n);
+        b->Sk = dalloc<double2>(rp * nr.n);
+        if (b->per_row_y) b->Yk = dalloc<double2>(rp * nr.nY);
+        b->stk = dalloc<int>(rp);
+        b->itk = dalloc<int>(rp);
+        b->resk = dalloc<double>(rp);
+        b->mapk = dalloc<int>(rp);
+        // lanes beyond the compacted rows stay finite (|V| = 1)
+        HIPCHK(hipMemcpyAsync(b->Vk, b->V, rp * nr.n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemsetAsync(b->Sk, 0, rp * nr.n * sizeof(double2), s));
+        if (b->Yk) HIPCHK(hipMemcpyAsync(b->Yk, b->Yrow, rp * nr.nY * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    }
+    const int rows = b->active_rows;
+    std::vector<int> st(static_cast<size_t>(rows));
+    HIPCHK(hipMemcpyAsync(st.data(), b->status, st.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<int> map;
+    for (int r = 0; r < rows; ++r)
+        if (st[r] == kActive) map.push_back(r);
+    const int R = static_cast<int>(map.size());
+    HIPCHK(hipMemcpyAsync(b->mapk, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->stk, b->status, rp * sizeof(int), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->itk, b->iters, rp * sizeof(int), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->resk, b->res, rp * sizeof(double), hipMemcpyDeviceToDevice, s));
+    const unsigned gv = blocks_for(static_cast<long long>(R) * nr.n, 256);
+    k_move_rows<double2><<<gv, 256, 0, s>>>(b->V, b->Vk, b->mapk, R, nr.n, 1);
+    k_move_rows<double2><<<gv, 256, 0, s>>>(b->S, b->Sk, b->mapk, R, nr.n, 1);
+    if (b->per_row_y)
+        k_move_rows<double2><<<blocks_for(static_cast<long long>(R) * nr.nY, 256), 256, 0, s>>>(b->Yrow, b->Yk,
+                                                                                               b->mapk, R, nr.nY, 1);
+    k_fill_int<<<blocks_for(rp, 256), 256, 0, s>>>(b->status, kNotSimulated, rp);
+    k_fill_int<<<blocks_for(R, 256), 256, 0, s>>>(b->status, kActive, R);
+    const unsigned gr = blocks_for(R, 256);
+    k_move_vals<int><<<gr, 256, 0, s>>>(b->itk, b->iters, b->mapk, R, 1, 0);
+    k_move_vals<double><<<gr, 256, 0, s>>>(b->resk, b->res, b->mapk, R, 1, 0);
+    HIPCHK(hipGetLastError());
+    std::swap(b->V, b->Vk);
+    std::swap(b->S, b->Sk);
+    if (b->per_row_y) std::swap(b->Yrow, b->Yk);
+    b->compact_saved = rows;
+    b->compact_R = R;
+    b->active_rows = R;
+    k_count_tiles<<<b->T, kLanes, 0, s>>>(b->status, b->rows_pad, b->tile_active);
+    HIPCHK(hipGetLastError());
+}
+
+void uncompact_rows(g2g_batch_s* b)
+{
+    const NrPlan& nr = b->plan->nr;
+    const size_t rp = static_cast<size_t>(b->rows_pad);
+    hipStream_t s = b->stream;
+    const int R = b->compact_R;
+    k_move_rows<double2><<<blocks_for(static_cast<long long>(R) * nr.n, 256), 256, 0, s>>>(b->V, b->Vk, b->mapk, R,
+                                                                                          nr.n, 0);
+    std::swap(b->V, b->Vk);
+    std::swap(b->S, b->Sk);
+    if (b->per_row_y) std::swap(b->Yrow, b->Yk);
+    const unsigned gr = blocks_for(R, 256);
+    k_move_vals<int><<<gr, 256, 0, s>>>(b->status, b->stk, b->mapk, R, 0, 0);
+    k_move_vals<int><<<gr, 256, 0, s>>>(b->iters, b->itk, b->mapk, R, 0, 0);
+    k_move_vals<double><<<gr, 256, 0, s>>>(b->res, b->resk, b->mapk, R, 0, 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(b->status, b->stk, rp * sizeof(int), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->iters, b->itk, rp * sizeof(int), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->res, b->resk, rp * sizeof(double), hipMemcpyDeviceToDevice, s));
+    b->active_rows = b->compact_saved;
+    b->compact_R = 0;
+    k_count_tiles<<<b->T, kLanes, 0, s>>>(b->status, b->rows_pad, b->tile_active);
+    HIPCHK(hipGetLastError());
+}
+
 template <typename Real>
-int nr_loop(g2g_batch_s* b, int max_iter, double tol, int freeze, double growth, SolveTimer& tm,
-            long long& rows_factored)
+int nr_loop_body(g2g_batch_s* b, int max_iter, double tol, int freeze, double growth, SolveTimer& tm,
+                 long long& rows_factored, bool& compacted)
 {
     const double eps = b->precision == G2G_MIXED ? 1e-5 : 1e-12;  // own_lu/batched_lu.cuh:50-56
     if (b->ir_steps > 0 && !b->Bs) {
@@ -407,6 +499,16 @@ int nr_loop(g2g_batch_s* b, int max_iter, double tol, int freeze, double growth,
         HIPCHK(hipMemcpyAsync(b->h_active, b->n_active, sizeof(int), hipMemcpyDeviceToHost, b->stream));
         HIPCHK(hipStreamSynchronize(b->stream));
         if (*b->h_active == 0) break;
+        if (!compacted && *b->h_active * 4 <= b->active_rows && *b->h_active + 4 * kLanes <= b->active_rows) {
+            compact_rows(b);
+            compacted = true;
+            // the A stream was written in the old row order: evaluate again
+            // (same V, same decisions) in the compacted one
+            k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
+            e0 = tm.mark();
+            launch_mismatch<Real>(b, true, it, max_iter, tol, freeze, growth);
+            tm.ph[0].push_back({e0, tm.mark()});
+        }
         e0 = tm.mark();
         launch_factor<Real>(b->plan, static_cast<const Real*>(b->A), static_cast<Real*>(b->U), b->a_stride,
                             b->u_stride, b->tile_active, b->jmax, b->clamped, b->error, b->active_rows, b->T, eps,
@@ -425,6 +527,25 @@ int nr_loop(g2g_batch_s* b, int max_iter, double tol, int freeze, double growth,
         tm.ph[2].push_back({e0, tm.mark()});
     }
     return it;
+}
+
+// Newton iterations on the batch's active rows until none is left or the
+// budget is spent; returns the iterations launched.  When no more than a
+// quarter of the rows (and at most all but four tiles' worth) are still
+// iterating, they are compacted into dense tiles for the rest of the loop.
+template <typename Real>
+int nr_loop(g2g_batch_s* b, int max_iter, double tol, int freeze, double growth, SolveTimer& tm,
+            long long& rows_factored)
+{
+    bool compacted = false;
+    try {
+        const int it = nr_loop_body<Real>(b, max_iter, tol, freeze, growth, tm, rows_factored, compacted);
+        if (compacted) uncompact_rows(b);
+        return it;
+    } catch (...) {
+        if (compacted) uncompact_rows(b);
+        throw;
+    }
 }
 
 void check_stream_error(g2g_batch_s* b)

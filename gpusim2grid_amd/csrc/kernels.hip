@@ -77,11 +77,12 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_mismatch(MismatchArgs<Rea
     const bool active = st0 == kActive;
     double rmax = 0.0, jmax = 0.0;
     if (a.tile_active[tile] > 0) {
-        const int b0 = static_cast<int>((static_cast<long long>(a.nbus) * wave) / kMisWaves);
-        const int b1 = static_cast<int>((static_cast<long long>(a.nbus) * (wave + 1)) / kMisWaves);
+        // the waves take the buses round-robin in the program's (RCM) order,
+        // so they all work in one neighbourhood of the grid at a time and the
+        // neighbours' V they gather stay in cache
         const double2* V = a.V;
         const double2* S = a.S;
-        for (int bb = b0; bb < b1; ++bb) {
+        for (int bb = wave; bb < a.nbus; bb += kMisWaves) {
             const int* bus = a.k1_bus + 6 * bb;
             const int i = bus[0], e0 = bus[1], e1 = bus[2], fp = bus[3], fq = bus[4], dk = bus[5];
             const double2 vi = V[tidx(tile, a.n, i, lane)];

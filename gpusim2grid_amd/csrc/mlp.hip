@@ -323,91 +323,88 @@ __global__ void k_f32_to_bf16(const float* __restrict__ x, long long n, __bf16* 
     if (i < n) y[i] = static_cast<__bf16>(x[i]);
 }
 
-// Features of a row: per bus (P - mu) / sigma, (Q - mu) / sigma from its
-// complex Sbus row, then the (optional) 0/1 outage mask -> bf16 [M][F]
-__global__ void k_features(const double2* __restrict__ S, long long lds, int M, int n, const float* __restrict__ mu,
-                           const float* __restrict__ sigma, const unsigned char* __restrict__ mask, int n_mask,
-                           __bf16* __restrict__ X, long long ldx)
+// Features and start voltages in the batch's tiled layout, [row / 64][bus][row % 64]:
+//   features: per bus (P - mu) / sigma, (Q - mu) / sigma, then the optional
+//             0/1 outage mask -> bf16 [M][F];
+//   start V:  V = Vm exp(j Va) from the network output Y (normalised Va, Vm
+//             per bus), Vm held at the set-point where it is fixed (PV, slack)
+//             and Va = 0 at the slack.
+// Both sides are coalesced: a block moves a patch
+// of one tile's 64 rows by 32 buses through LDS (the tiled side is read or
+// written 64 lanes = 1 KB at a time, the row-major side 64 features of a row
+// at a time).  Grid: (ceil(n / 32), ceil(M / 64)), 256 threads.
+__global__ __launch_bounds__(256) void k_features_tiles_t(const double2* __restrict__ S_tiles, int M, int n,
+                                                          const float* __restrict__ mu,
+                                                          const float* __restrict__ sigma, __bf16* __restrict__ X,
+                                                          long long ldx)
 {
-    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int F = 2 * n + n_mask;
-    if (gid >= static_cast<long long>(M) * F) return;
-    const int r = static_cast<int>(gid / F);
-    const int f = static_cast<int>(gid - static_cast<long long>(r) * F);
-    float v;
-    if (f < 2 * n) {
-        const double2 s = S[static_cast<long long>(r) * lds + (f >> 1)];
-        const double x = (f & 1) ? s.y : s.x;
-        v = static_cast<float>((x - mu[f]) / sigma[f]);
-    } else {
-        v = static_cast<float>(mask[static_cast<long long>(r) * n_mask + (f - 2 * n)]);
+    __shared__ double2 sS[kPatchBuses][64];
+    const int t = threadIdx.x;
+    const int b0 = blockIdx.x * kPatchBuses, tile = blockIdx.y;
+    for (int i = t; i < kPatchBuses * 64; i += 256) {
+        const int bl = i >> 6, lane = i & 63;
+        const int b = b0 + bl;
+        sS[bl][lane] = b < n ? S_tiles[(static_cast<long long>(tile) * n + b) * 64 + lane] : make_double2(0.0, 0.0);
     }
-    X[static_cast<long long>(r) * ldx + f] = static_cast<__bf16>(v);
-}
-
-// Start voltages from the network output Y (normalised Va, Vm per bus):
-// V = Vm exp(j Va) with Vm held at the set-point where it is fixed (PV, slack)
-// and Va = 0 at the slack.
-__global__ void k_warm_v(const float* __restrict__ Y, long long ldy, int M, int n, const float* __restrict__ mu,
-                         const float* __restrict__ sigma, const double* __restrict__ vfix,
-                         const int* __restrict__ kind, double2* __restrict__ V, long long ldv)
-{
-    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gid >= static_cast<long long>(M) * n) return;
-    const int r = static_cast<int>(gid / n);
-    const int b = static_cast<int>(gid - static_cast<long long>(r) * n);
-    const float* y = Y + static_cast<long long>(r) * ldy;
-    double va = static_cast<double>(y[2 * b]) * sigma[2 * b] + mu[2 * b];
-    double vm = static_cast<double>(y[2 * b + 1]) * sigma[2 * b + 1] + mu[2 * b + 1];
-    if (kind[b] != 1) vm = vfix[b];  // PV / slack: the set-point
-    if (kind[b] == 2) va = 0.0;      // slack
-    double sn, cs;
-    sincos(va, &sn, &cs);
-    V[static_cast<long long>(r) * ldv + b] = make_double2(vm * cs, vm * sn);
-}
-
-// tiled-layout variants: a batch's Sbus / V are stored [row / 64][bus][row % 64]
-__device__ __forceinline__ long long tiled(int r, int n, int b)
-{
-    return (static_cast<long long>(r >> 6) * n + b) * 64 + (r & 63);
-}
-
-__global__ void k_features_tiles(const double2* __restrict__ S_tiles, int M, int n, const float* __restrict__ mu,
-                                 const float* __restrict__ sigma, const unsigned char* __restrict__ mask, int n_mask,
-                                 __bf16* __restrict__ X, long long ldx)
-{
-    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int F = 2 * n + n_mask;
-    if (gid >= static_cast<long long>(M) * F) return;
-    const int r = static_cast<int>(gid / F);
-    const int f = static_cast<int>(gid - static_cast<long long>(r) * F);
-    float v;
-    if (f < 2 * n) {
-        const double2 s = S_tiles[tiled(r, n, f >> 1)];
-        const double x = (f & 1) ? s.y : s.x;
-        v = static_cast<float>((x - mu[f]) / sigma[f]);
-    } else {
-        v = static_cast<float>(mask[static_cast<long long>(r) * n_mask + (f - 2 * n)]);
+    __syncthreads();
+    const int rr = t >> 2, q = t & 3;  // row of the patch, quarter of its 64 features
+    const int r = tile * 64 + rr;
+    if (r >= M) return;
+    __bf16* x = X + static_cast<long long>(r) * ldx;
+    for (int k = 0; k < kPatchBuses / 4; ++k) {
+        const int bl = q * (kPatchBuses / 4) + k;
+        const int b = b0 + bl;
+        if (b >= n) break;
+        const double2 v = sS[bl][rr];
+        x[2 * b] = static_cast<__bf16>(static_cast<float>((v.x - mu[2 * b]) / sigma[2 * b]));
+        x[2 * b + 1] = static_cast<__bf16>(static_cast<float>((v.y - mu[2 * b + 1]) / sigma[2 * b + 1]));
     }
-    X[static_cast<long long>(r) * ldx + f] = static_cast<__bf16>(v);
 }
 
-__global__ void k_warm_v_tiles(const float* __restrict__ Y, long long ldy, int M, int n, const float* __restrict__ mu,
-                               const float* __restrict__ sigma, const double* __restrict__ vfix,
-                               const int* __restrict__ kind, double2* __restrict__ V_tiles, int row0)
+// outage-mask features 2n .. 2n + n_mask of each row
+__global__ void k_mask_features(const unsigned char* __restrict__ mask, int M, int n, int n_mask,
+                                __bf16* __restrict__ X, long long ldx)
 {
     const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gid >= static_cast<long long>(M) * n) return;
-    const int r = static_cast<int>(gid / n);
-    const int b = static_cast<int>(gid - static_cast<long long>(r) * n);
-    const float* y = Y + static_cast<long long>(r) * ldy;
-    double va = static_cast<double>(y[2 * b]) * sigma[2 * b] + mu[2 * b];
-    double vm = static_cast<double>(y[2 * b + 1]) * sigma[2 * b + 1] + mu[2 * b + 1];
-    if (kind[b] != 1) vm = vfix[b];
-    if (kind[b] == 2) va = 0.0;
-    double sn, cs;
-    sincos(va, &sn, &cs);
-    V_tiles[tiled(row0 + r, n, b)] = make_double2(vm * cs, vm * sn);
+    if (gid >= static_cast<long long>(M) * n_mask) return;
+    const int r = static_cast<int>(gid / n_mask);
+    const int m = static_cast<int>(gid - static_cast<long long>(r) * n_mask);
+    X[static_cast<long long>(r) * ldx + 2 * n + m] = static_cast<__bf16>(static_cast<float>(mask[gid]));
+}
+
+__global__ __launch_bounds__(256) void k_warm_v_tiles_t(const float* __restrict__ Y, long long ldy, int M, int n,
+                                                        const float* __restrict__ mu,
+                                                        const float* __restrict__ sigma,
+                                                        const double* __restrict__ vfix, const int* __restrict__ kind,
+                                                        double2* __restrict__ V_tiles)
+{
+    __shared__ float2 sY[kPatchBuses][64];
+    const int t = threadIdx.x;
+    const int b0 = blockIdx.x * kPatchBuses, tile = blockIdx.y;
+    {
+        const int rr = t >> 2, q = t & 3;
+        const int r = tile * 64 + rr;
+        const float* y = Y + static_cast<long long>(r) * ldy;
+        for (int k = 0; k < kPatchBuses / 4; ++k) {
+            const int bl = q * (kPatchBuses / 4) + k;
+            const int b = b0 + bl;
+            sY[bl][rr] = (r < M && b < n) ? make_float2(y[2 * b], y[2 * b + 1]) : make_float2(0.f, 0.f);
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < kPatchBuses * 64; i += 256) {
+        const int bl = i >> 6, lane = i & 63;
+        const int b = b0 + bl, r = tile * 64 + lane;
+        if (b >= n || r >= M) continue;
+        const float2 o = sY[bl][lane];
+        double va = static_cast<double>(o.x) * sigma[2 * b] + mu[2 * b];
+        double vm = static_cast<double>(o.y) * sigma[2 * b + 1] + mu[2 * b + 1];
+        if (kind[b] != 1) vm = vfix[b];
+        if (kind[b] == 2) va = 0.0;
+        double sn, cs;
+        sincos(va, &sn, &cs);
+        V_tiles[(static_cast<long long>(tile) * n + b) * 64 + lane] = make_double2(vm * cs, vm * sn);
+    }
 }
 
 template __global__ void k_gemm_kc<kEpiF32>(GemmArgs);

@@ -40,13 +40,11 @@ __global__ void k_mse_grad(const float* Y, const float* T, long long n_el, float
 __global__ void k_adam(float* w, float* m, float* v, const float* g, long long n, float lr, float b1, float b2,
                        float eps, float c1, float c2, __bf16* wb);
 __global__ void k_f32_to_bf16(const float* x, long long n, __bf16* y);
-__global__ void k_features(const double2* S, long long lds, int M, int n, const float* mu, const float* sigma,
-                           const unsigned char* mask, int n_mask, __bf16* X, long long ldx);
-__global__ void k_features_tiles(const double2* S_tiles, int M, int n, const float* mu, const float* sigma,
-                                 const unsigned char* mask, int n_mask, __bf16* X, long long ldx);
-__global__ void k_warm_v(const float* Y, long long ldy, int M, int n, const float* mu, const float* sigma,
-                         const double* vfix, const int* kind, double2* V, long long ldv);
-__global__ void k_warm_v_tiles(const float* Y, long long ldy, int M, int n, const float* mu, const float* sigma,
-                               const double* vfix, const int* kind, double2* V_tiles, int row0);
+constexpr int kPatchBuses = 32;  // buses per block of the tiled-layout conversions
+__global__ void k_features_tiles_t(const double2* S_tiles, int M, int n, const float* mu, const float* sigma,
+                                   __bf16* X, long long ldx);
+__global__ void k_mask_features(const unsigned char* mask, int M, int n, int n_mask, __bf16* X, long long ldx);
+__global__ void k_warm_v_tiles_t(const float* Y, long long ldy, int M, int n, const float* mu, const float* sigma,
+                                 const double* vfix, const int* kind, double2* V_tiles);
 
 }  // namespace g2g

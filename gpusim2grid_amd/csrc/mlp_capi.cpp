@@ -211,14 +211,15 @@ void mlp_warm_start_tiles(g2g_mlp_s* p, const double2* S_tiles, int M, int n, co
     hipStream_t keep = p->stream;
     p->stream = st;
     p->workspace(M);
-    const long long nf = static_cast<long long>(M) * p->n_in;
-    k_features_tiles<<<blocks_for(nf, 256), 256, 0, st>>>(S_tiles, M, n, p->in_mu, p->in_sigma, d_mask, n_mask, p->X,
-                                                          p->n_in);
+    const dim3 patches((n + kPatchBuses - 1) / kPatchBuses, (M + 63) / 64);
+    k_features_tiles_t<<<patches, 256, 0, st>>>(S_tiles, M, n, p->in_mu, p->in_sigma, p->X, p->n_in);
+    if (n_mask > 0)
+        k_mask_features<<<blocks_for(static_cast<long long>(M) * n_mask, 256), 256, 0, st>>>(d_mask, M, n, n_mask,
+                                                                                            p->X, p->n_in);
     HIPCHK(hipGetLastError());
     forward(p, M, false);
-    const long long nv = static_cast<long long>(M) * n;
-    k_warm_v_tiles<<<blocks_for(nv, 256), 256, 0, st>>>(p->Y, p->n_out, M, n, p->out_mu, p->out_sigma, p->vfix,
-                                                        p->kind, V_tiles, 0);
+    k_warm_v_tiles_t<<<patches, 256, 0, st>>>(p->Y, p->n_out, M, n, p->out_mu, p->out_sigma, p->vfix, p->kind,
+                                              V_tiles);
     HIPCHK(hipGetLastError());
     p->stream = keep;
 }

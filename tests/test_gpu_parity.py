@@ -200,3 +200,29 @@ def test_raw_lu_solve_matches_scipy(prec):
         r = np.abs(Jk @ x[k] - rhs[k]).max() / np.abs(rhs[k]).max()
         assert r < tol, (k, r)
     assert (cl == 0).all()
+
+
+@pytest.mark.parametrize("prec", [_capi.FP64, _capi.MIXED])
+def test_tail_compaction_keeps_every_row(prec):
+    """A batch whose rows mostly start converged: after the first update few
+    rows are left, they are compacted into dense tiles and scattered back;
+    statuses, iteration counts and voltages are the oracle's, row by row."""
+    g = syn.grid_118()
+    rows = 1000
+    _say(f"tail compaction prec={prec} rows={rows}")
+    S = _sbus_rows(g, rows, seed=11)
+    Vs, _, _, _ = orc.nr_batch(g.Ybus, g.flat_start(), S, g.pv, g.pq, 10, 1e-12, n_threads=ORACLE_THREADS)
+    V0 = Vs.copy()
+    rng = np.random.default_rng(12)
+    hard = rng.choice(rows, size=90, replace=False)
+    V0[hard] = g.flat_start()
+    V0[hard[:30]] *= np.exp(1j * rng.normal(0, 0.02, (30, g.n_bus)))
+    V, st, it, res, _ = _gpu_solve(g, S, V0, rows, precision=prec)
+    Vo, sto, ito, _ = orc.nr_batch(g.Ybus, V0, S, g.pv, g.pq, 10, 1e-8, n_threads=ORACLE_THREADS)
+    np.testing.assert_array_equal(st, sto)
+    if prec == _capi.FP64:
+        np.testing.assert_array_equal(it, ito)
+        assert np.abs(V - Vo).max() < 1e-8
+    else:
+        assert (it <= ito + 2).all() and np.abs(V - Vo).max() < 1e-6
+    assert (it[hard] > 0).all() and (np.delete(it, hard) == 0).all()
