@@ -15,6 +15,12 @@ injections are resident in HBM before the timed region.  Methods:
 Prints one JSON line.  ``value`` is the converged PF/s of the best method
 measured (the headline method: warm start + mixed precision, else mixed
 precision); every method's figure is in ``methods``.
+
+``--workload c3`` runs the learned-warm-start workload instead: the 2,000-bus
+grid, 16,384 rows of the same load scaling plus 0-2 random non-islanding
+branch outages each (per-row Ybus value edits, and a 0/1 outage mask among
+the network's inputs), the warm start trained on solver-generated solutions
+and a flat-start fallback after 4 iterations.
 """
 from __future__ import annotations
 
@@ -54,8 +60,44 @@ def parse():
     ap.add_argument("--k-warm", type=int, default=6)
     ap.add_argument("--growth", type=float, default=10.0)
     ap.add_argument("--cpu-rows", type=int, default=512, help="oracle sample size (0: skip)")
+    ap.add_argument("--workload", choices=("c2", "c3", "c4"), default="c2")
+    ap.add_argument("--rate", type=float, default=50000.0, help="c4: offered requests per second")
+    ap.add_argument("--requests", type=int, default=100000, help="c4: requests replayed")
+    ap.add_argument("--no-warm", action="store_true", help="c4: flat starts instead of the learned warm start")
     ap.add_argument("--quiet", action="store_true")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.workload == "c3":  # the learned-warm-start workload's defaults
+        for k, v in (("train_rows", 200_000), ("k_warm", 4)):
+            if getattr(args, k) == ap.get_default(k):
+                setattr(args, k, v)
+    return args
+
+
+class Workload:
+    """Grid, scenario rows (Sbus) and, for outage rows, the per-row Ybus edits
+    and the outage mask."""
+
+    def __init__(self, args, syn, rows, seed_s, seed_o):
+        self.name = args.workload
+        self.g = syn.grid_c2() if self.name == "c2" else syn.grid_c1()
+        p, q = syn.injection_scenarios(self.g, rows, seed=seed_s)
+        self.S = (p + 1j * q) / self.g.sn_mva
+        self.outages = None
+        self.edits = None
+        self.mask = None
+        if self.name == "c3":
+            self.outages = syn.random_outages(self.g, rows, max_k=2, seed=seed_o)
+            self.edits = syn.outage_edits(self.g, self.outages)
+            self.mask = syn.outage_mask(self.g, self.outages)
+
+    def describe(self):
+        g = self.g
+        if self.name == "c2":
+            return ("6500-bus injection sweep, 16384 scenarios, ||dS||inf<1e-8",
+                    "synthetic (seeded 6,500-bus grid, 16,384 lognormal injection scenarios)")
+        return ("2000-bus sweep with 0-2 branch outages per row, 16384 rows, learned warm start, ||dS||inf<1e-8",
+                "synthetic (seeded 2,000-bus grid, 16,384 lognormal injection scenarios, 0-2 random "
+                "non-islanding outages per row)")
 
 
 def log(args, *msg):
@@ -63,47 +105,64 @@ def log(args, *msg):
         print(*msg, file=sys.stderr, flush=True)
 
 
-def train_warm(args, g, torch, capi, syn):
-    """Learned warm start: solve fresh training scenarios (seed 2) with the
-    mixed-precision solver, then train the MLP on them -- both outside the
-    timed region, like the symbolic analysis."""
+def train_warm(args, wl, torch, capi, syn):
+    """Learned warm start: solve fresh training scenarios (seed 2; outages seed
+    4) with the mixed-precision solver, then train the MLP on them -- both
+    outside the timed region, like the symbolic analysis."""
     from gpusim2grid_amd.warmstart import WarmStart
+    g = wl.g
     t0 = time.perf_counter()
     rows = args.train_rows
-    p, q = syn.injection_scenarios(g, rows, seed=2)
-    S = torch.from_numpy((p + 1j * q) / g.sn_mva).to("cuda")
-    del p, q
+    tr = Workload(args, syn, rows, seed_s=2, seed_o=4)
     plan = capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
-    B = capi.Batch(plan, rows, precision=capi.MIXED)
-    B.set_sbus_device(S.data_ptr(), S.shape[1], rows)
-    B.set_v(g.flat_start())
-    B.solve(max_iter=args.max_iter, tol=args.tol, scheduling=capi.CONTINUOUS)
-    st, _, _ = B.results()
-    V = torch.empty((rows, g.n_bus), dtype=torch.complex128, device="cuda")
-    torch.cuda.synchronize()
-    B.get_v_device(V.data_ptr(), g.n_bus)
-    del B
-    ok = torch.from_numpy(st == 0).to("cuda")
-    S, V = S[ok].contiguous(), V[ok].contiguous()
+    chunk = min(rows, 32768)
+    S_all, V_all, M_all = [], [], []
+    for r0 in range(0, rows, chunk):
+        r1 = min(rows, r0 + chunk)
+        S = torch.from_numpy(np.ascontiguousarray(tr.S[r0:r1])).to("cuda")
+        B = capi.Batch(plan, r1 - r0, precision=capi.MIXED, per_row_ybus=tr.edits is not None)
+        B.set_sbus_device(S.data_ptr(), S.shape[1], r1 - r0)
+        B.set_v(g.flat_start())
+        if tr.edits is not None:
+            B.set_ybus_edits(*syn.outage_edits(g, tr.outages[r0:r1]))
+        B.solve(max_iter=args.max_iter, tol=args.tol, scheduling=capi.CONTINUOUS)
+        st, _, _ = B.results()
+        V = torch.empty((r1 - r0, g.n_bus), dtype=torch.complex128, device="cuda")
+        torch.cuda.synchronize()
+        B.get_v_device(V.data_ptr(), g.n_bus)
+        del B
+        ok = torch.from_numpy(st == 0).to("cuda")
+        S_all.append(S[ok])
+        V_all.append(V[ok])
+        if tr.mask is not None:
+            M_all.append(torch.from_numpy(tr.mask[r0:r1]).to("cuda")[ok])
+    S = torch.cat(S_all).contiguous()
+    V = torch.cat(V_all).contiguous()
+    mask = torch.cat(M_all).contiguous() if M_all else None
+    del S_all, V_all, M_all, tr
     t_data = time.perf_counter() - t0
-    ws = WarmStart(g.n_bus, g.pv, g.pq, g.slack, np.abs(g.flat_start()), n_hidden=args.hidden, seed=2)
+    ws = WarmStart(g.n_bus, g.pv, g.pq, g.slack, np.abs(g.flat_start()), n_hidden=args.hidden, seed=2,
+                   n_mask=0 if mask is None else g.n_branch)
     t1 = time.perf_counter()
-    hist = ws.train(S, V, epochs=args.epochs, batch=args.train_batch, lr=args.lr,
+    hist = ws.train(S, V, mask=mask, epochs=args.epochs, batch=args.train_batch, lr=args.lr,
                     log=(lambda m: log(args, m)))
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t1
     del S, V
-    return ws, dict(train_rows=int(ok.sum()), epochs=args.epochs, hidden=args.hidden, loss_first=hist[0],
+    return ws, dict(train_rows=int(S.shape[0]), epochs=args.epochs, hidden=args.hidden, loss_first=hist[0],
                     loss_last=hist[-1], data_s=t_data, train_s=t_train)
 
 
-def run_method(args, name, g, S, torch, capi, ws=None):
+def run_method(args, name, wl, torch, capi, ws=None):
+    g, S = wl.g, wl.S
     prec = capi.FP64 if name == "fp64" else capi.MIXED
     t_plan = time.perf_counter()
     plan = capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
     info = plan.info()
     t_plan = time.perf_counter() - t_plan
-    B = capi.Batch(plan, args.rows, precision=prec)
+    B = capi.Batch(plan, args.rows, precision=prec, per_row_ybus=wl.edits is not None)
+    if wl.edits is not None:
+        B.set_ybus_edits(*wl.edits)
     # injections resident in HBM: one upload before any timing
     S_dev = torch.from_numpy(np.ascontiguousarray(S)).to("cuda")
     B.set_sbus_device(S_dev.data_ptr(), S.shape[1], args.rows)
@@ -114,7 +173,7 @@ def run_method(args, name, g, S, torch, capi, ws=None):
         B.set_v_device(V0.data_ptr(), 0, args.rows)  # flat start (broadcast), inside the step
         if ws is not None:
             return B.solve_warm(ws.mlp, k_warm=args.k_warm, growth=args.growth, max_iter=args.max_iter,
-                                tol=args.tol)
+                                tol=args.tol, mask=wl.mask)
         return B.solve(max_iter=args.max_iter, tol=args.tol, scheduling=capi.CONTINUOUS)
 
     for _ in range(args.warmup):
@@ -154,9 +213,13 @@ def run_method(args, name, g, S, torch, capi, ws=None):
     return out, V, st
 
 
-def cpu_baseline(args, g, S, V_gpu, st_gpu):
+def cpu_baseline(args, wl, V_gpu, st_gpu):
     from oracle import oracle as orc
+    g, S = wl.g, wl.S
     n = min(args.cpu_rows, S.shape[0])
+    Yrows = None
+    if wl.outages is not None:
+        Yrows = np.stack([orc.ybus_values_on_pattern(g.Ybus, orc.outage_ybus(g, wl.outages[r])) for r in range(n)])
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
@@ -164,7 +227,7 @@ def cpu_baseline(args, g, S, V_gpu, st_gpu):
     cores = max(1, min(cores, 16))
     t0 = time.perf_counter()
     Vo, sto, ito, reso = orc.nr_batch(g.Ybus, g.flat_start(), S[:n], g.pv, g.pq, max_iter=args.max_iter,
-                                      tol=args.tol, n_threads=cores)
+                                      tol=args.tol, n_threads=cores, Yx_rows=Yrows)
     dt = time.perf_counter() - t0
     conv = int((sto == 0).sum())
     parity = None
@@ -188,6 +251,56 @@ def cpu_baseline(args, g, S, V_gpu, st_gpu):
                        f"{cores} threads, {dt:.1f} s"), parity
 
 
+def run_serving(args, torch, capi, syn):
+    """C4: Poisson arrivals over the 118-, 2,000- and 6,500-bus grids, 0-3
+    tripped branches per request, resident per-grid batches (continuous
+    batching with slot refill), warm start + mixed precision; throughput and
+    p50 / p99 latency from arrival to result."""
+    from gpusim2grid_amd import serving
+    from gpusim2grid_amd.warmstart import WarmStart
+    grids = {"g118": syn.grid_118(), "c1": syn.grid_c1(), "c2": syn.grid_c2()}
+    warm = {}
+    t0 = time.perf_counter()
+    if not args.no_warm:
+        for name, g in grids.items():
+            rows = min(args.train_rows, 16384)
+            p, q = syn.injection_scenarios(g, rows, seed=2)
+            S = (p + 1j * q) / g.sn_mva
+            outs = syn.random_outages(g, rows, max_k=2, seed=4)
+            plan = capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+            B = capi.Batch(plan, rows, precision=capi.MIXED, per_row_ybus=True)
+            B.set_sbus(S)
+            B.set_v(g.flat_start())
+            B.set_ybus_edits(*syn.outage_edits(g, outs))
+            B.solve(max_iter=args.max_iter, tol=args.tol)
+            st, _, _ = B.results()
+            V = B.get_v()
+            del B
+            ok = st == 0
+            ws = WarmStart(g.n_bus, g.pv, g.pq, g.slack, np.abs(g.flat_start()), n_hidden=args.hidden, seed=2,
+                           n_mask=g.n_branch)
+            ws.train(torch.as_tensor(S[ok], device="cuda"), torch.as_tensor(V[ok], device="cuda"),
+                     mask=torch.as_tensor(syn.outage_mask(g, outs)[ok], device="cuda"), epochs=args.epochs,
+                     batch=args.train_batch, lr=args.lr)
+            warm[name] = ws
+            log(args, f"c4: warm start of {name} trained")
+    t_train = time.perf_counter() - t0
+    reqs = serving.poisson_requests(grids, rate=args.rate, n=args.requests, max_outages=3, seed=5)
+    srv = serving.PowerFlowServer(grids, slots={"g118": 4096, "c1": 8192, "c2": 16384}, precision=capi.MIXED,
+                                  warm=warm or None, hbm_budget=200e9)
+    # a short replay first (compiles nothing, but pages every kernel in)
+    srv.run(serving.poisson_requests(grids, rate=args.rate, n=2000, max_outages=3, seed=99), timeout=300)
+    done, rep = srv.run(reqs, timeout=600)
+    line = dict(metric="served power flows per second, Poisson arrivals over 118/2000/6500-bus grids",
+                value=rep["throughput_rps"], unit="PF/s", n_gpus=1, steps=1, warmup=1,
+                ms_per_step=rep["wall_s"] * 1e3, higher_is_better=True, scaling="weak", vs_baseline=None,
+                dtype="f32+f64", data="synthetic (seeded grids, Poisson arrivals, 0-3 outages per request)",
+                config=dict(workload="serving loop, 118/2000/6500-bus grids, Poisson arrivals",
+                            offered_rps=args.rate, requests=args.requests, warm_start=not args.no_warm),
+                serving=rep, train_s=t_train)
+    print(json.dumps(line, default=float))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -201,10 +314,12 @@ def main():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(2)
     capi.lib()  # fails loudly when the HIP library is missing
+    if args.workload == "c4":
+        run_serving(args, torch, capi, syn)
+        return
     t0 = time.perf_counter()
-    g = syn.grid_c2()
-    p, q = syn.injection_scenarios(g, args.rows, seed=1)
-    S = (p + 1j * q) / g.sn_mva
+    wl = Workload(args, syn, args.rows, seed_s=1, seed_o=3)
+    g = wl.g
     log(args, f"grid + scenarios {time.perf_counter() - t0:.1f} s")
     methods = {}
     keep = {}
@@ -213,9 +328,9 @@ def main():
         t1 = time.perf_counter()
         ws = None
         if name == "warm":
-            ws, warm_info = train_warm(args, g, torch, capi, syn)
+            ws, warm_info = train_warm(args, wl, torch, capi, syn)
             log(args, "warm-start training", warm_info)
-        methods[name], V, st = run_method(args, name, g, S, torch, capi, ws)
+        methods[name], V, st = run_method(args, name, wl, torch, capi, ws)
         if name == "warm":
             methods[name]["training"] = warm_info
         keep[name] = (V, st)
@@ -224,7 +339,7 @@ def main():
     cpu, parity = (None, None)
     if args.cpu_rows > 0:
         V, st = keep.get("fp64", (None, None))
-        cpu, parity = cpu_baseline(args, g, S, V, st)
+        cpu, parity = cpu_baseline(args, wl, V, st)
         log(args, "cpu", cpu, parity)
     # every method's answers against the plain FP64 method's (same rows)
     if "fp64" in keep:
@@ -241,14 +356,15 @@ def main():
     roof = dict(bound="hbm", kernel="k_factor",
                 achieved=dom["factor_achieved_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=(dom["factor_achieved_gbs"] or 0.0) / HBM_PEAK_GBS,
-                traffic=FACTOR_TRAFFIC_PER_ROW[dom["real"]] * dom["factor_rows_per_launch"],
+                traffic=(FACTOR_TRAFFIC_PER_ROW[dom["real"]] * dom["factor_rows_per_launch"]
+                         if args.workload == "c2" else None),
                 alg_bytes_per_launch=dom["factor_alg_bytes_per_launch"], launch_ms=dom["factor_launch_ms"])
     line = dict(
         metric=METRIC, value=head["pfps"], unit="PF/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
         ms_per_step=head["ms_per_step"], higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype="f64" if head["method"] == "fp64" else "f32+f64",
-        data="synthetic (seeded 6,500-bus grid, 16,384 lognormal injection scenarios)",
-        config=dict(workload="6500-bus injection sweep, 16384 scenarios, ||dS||inf<1e-8", n_bus=g.n_bus,
+        data=wl.describe()[1],
+        config=dict(workload=wl.describe()[0], n_bus=g.n_bus,
                     n_branch=g.n_branch, rows=args.rows, method=head["method"], max_iter=args.max_iter),
         ratio_vs_plain_fp64=(head["pfps"] / fp64["pfps"]) if fp64 else None,
         methods={k: {kk: vv for kk, vv in v.items() if kk != "info"} for k, v in methods.items()},

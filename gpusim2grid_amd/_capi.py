@@ -79,6 +79,10 @@ def lib():
         "g2g_batch_eval": (ctypes.c_int, [_P, _P, _P, _P]),
         "g2g_lu_solve_batch": (ctypes.c_int, [_P, i32, i32, _P, _P, _P, dbl, i32, _P, _P]),
         "g2g_batch_set_step_control": (ctypes.c_int, [_P, i32, i32, dbl, dbl]),
+        "g2g_batch_stream_begin": (ctypes.c_int, [_P]),
+        "g2g_batch_stream_fill": (ctypes.c_int, [_P, i32, _P, _P, i64, _P, i64, i32, _P, _P, _P]),
+        "g2g_batch_stream_round": (ctypes.c_int, [_P, i32, i32, dbl]),
+        "g2g_batch_stream_take": (ctypes.c_int, [_P, i32, _P, _P, i64, i32]),
         "g2g_batch_device_arrays": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
         "g2g_batch_warm_start": (ctypes.c_int, [_P, _P, _P, i32]),
         "g2g_batch_branch_flows": (ctypes.c_int, [_P, i32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
@@ -235,6 +239,35 @@ class Batch:
 
     def set_active_rows(self, k):
         _check(lib().g2g_batch_set_active_rows(self._h, int(k)))
+
+    # -- streamed rows (continuous batching) --------------------------------
+    def stream_begin(self):
+        """Every slot free; the batch's iteration clock at 0."""
+        _check(lib().g2g_batch_stream_begin(self._h))
+
+    def stream_fill(self, slots, S, V0, edits=None):
+        """Rows S [k, n] (complex host arrays; V0 [k, n] or one shared row) into
+        the given free slots, each with its outage edits (row_ptr, entry, delta)."""
+        slots = _i32(slots)
+        k = slots.size
+        S = np.ascontiguousarray(np.atleast_2d(S), dtype=np.complex128)
+        V0 = np.ascontiguousarray(np.atleast_2d(V0), dtype=np.complex128)
+        ldv = 0 if V0.shape[0] == 1 and k > 1 else V0.shape[1]
+        ep = en = ed = None
+        if edits is not None:
+            ep, en, ed = _i32(edits[0]), _i32(edits[1]), _c128(edits[2])
+        _check(lib().g2g_batch_stream_fill(self._h, k, _ptr(slots), _ptr(S), S.shape[1], _ptr(V0), ldv, 0,
+                                           _ptr(ep), _ptr(en), _ptr(ed)))
+
+    def stream_round(self, k=1, max_iter=10, tol=1e-8):
+        _check(lib().g2g_batch_stream_round(self._h, int(k), int(max_iter), float(tol)))
+
+    def stream_take(self, slots):
+        """V of the given slots (host [k, n]); the slots become free."""
+        slots = _i32(slots)
+        V = np.empty((slots.size, self.n), np.complex128)
+        _check(lib().g2g_batch_stream_take(self._h, slots.size, _ptr(slots), _ptr(V), self.n, 0))
+        return V
 
     def set_step_control(self, ir_steps=0, scaling=False, max_dva=0.5, max_dvm=0.1):
         """Refinement rounds per Newton step and per-row step scaling of the next solves."""

@@ -34,7 +34,14 @@ __global__ void k_rows_to_tiles(const T* in, long long ld, int rows, int m, int 
 template <typename T>
 __global__ void k_tiles_to_rows(const T* in, int rows, int m, T* out, long long ld);
 __global__ void k_ybus_broadcast(const double2* Y, int nY, int rows_pad, double2* Yrow);
-__global__ void k_ybus_edits(const int* row_ptr, const int* ent, const double2* delta, int rows, int nY, double2* Yrow);
+__global__ void k_ybus_edits(const int* row_ptr, const int* ent, const double2* delta, int rows, int nY, double2* Yrow,
+                             const int* slots);
+__global__ void k_ybus_reset_slots(const int* slots, int R, const double2* Y, int nY, double2* Yrow);
+template <typename T>
+__global__ void k_rows_to_slots(const T* src, long long ld, int R, int m, const int* slots, T* dst);
+template <typename T>
+__global__ void k_slots_to_rows(const T* src, int R, int m, const int* slots, T* dst, long long ld);
+__global__ void k_set_slots(const int* slots, int R, int* v, int value);
 template <typename Real>
 __global__ void k_gather_astream(const double* vals, long long ldv, const double* rhs, long long ldb, int rows,
                                  int nnz, int d, const int* a_pos, const int* f_pos, int a_len, Real* A);
@@ -185,6 +192,11 @@ struct g2g_batch_s {
     double* resk = nullptr;
     int* mapk = nullptr;
     int compact_R = 0, compact_saved = 0;
+    // streamed rows (continuous batching): per-row start iteration and the
+    // batch's iteration clock
+    int* it0 = nullptr;
+    int clock = 0;
+    bool streaming = false;
     std::vector<hipEvent_t> ev;
     ~g2g_batch_s()
     {
@@ -192,7 +204,7 @@ struct g2g_batch_s {
         dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped); dfree(error);
         dfree(V0); dfree(Vc); dfree(Sc); dfree(Yc); dfree(st_save); dfree(it_save); dfree(res_save);
         dfree(row_map); dfree(Bs); dfree(Xs);
-        dfree(Vk); dfree(Sk); dfree(Yk); dfree(stk); dfree(itk); dfree(resk); dfree(mapk);
+        dfree(Vk); dfree(Sk); dfree(Yk); dfree(stk); dfree(itk); dfree(resk); dfree(mapk); dfree(it0);
         if (h_active) (void)hipHostFree(h_active);
         for (auto e : ev) (void)hipEventDestroy(e);
     }
@@ -342,7 +354,8 @@ void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double
     const NrPlan& P = b->plan->nr;
     MismatchArgs<Real> a{b->plan->d_k1_bus, b->plan->d_k1_ent, P.k1_nbus, b->plan->d_Y, b->Yrow, b->V, b->S,
                          static_cast<Real*>(b->A), P.n, P.nY, b->a_stride, b->res, b->jmax, b->status, b->iters,
-                         b->tile_active, b->n_active, b->active_rows, iter, max_iter, tol, freeze, growth};
+                         b->tile_active, b->n_active, b->active_rows, iter, max_iter, tol, freeze, growth,
+                         b->streaming ? b->it0 : nullptr};
     const dim3 grid(b->T), block(kLanes * kMisWaves);
     if (b->per_row_y) {
         if (with_j) k_mismatch<Real, true, true><<<grid, block, 0, b->stream>>>(a);
@@ -548,6 +561,37 @@ int nr_loop(g2g_batch_s* b, int max_iter, double tol, int freeze, double growth,
     }
 }
 
+// One round of a streamed batch: k Newton iterations on its active slots,
+// continuous rule, each row against its own iteration count.  A row whose
+// last update brought it below tol is found by the next round's first
+// mismatch (its count is then the number of updates it had).
+template <typename Real>
+void stream_round(g2g_batch_s* b, int k, int max_iter, double tol)
+{
+    const double eps = b->precision == G2G_MIXED ? 1e-5 : 1e-12;  // own_lu/batched_lu.cuh:50-56
+    if (b->ir_steps > 0 && !b->Bs) {
+        const size_t bytes = static_cast<size_t>(b->rows_pad) * b->plan->nr.d * b->real_size();
+        b->Bs = dalloc<unsigned char>(bytes);
+        b->Xs = dalloc<unsigned char>(bytes);
+    }
+    for (int q = 0; q < k; ++q) {
+        k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
+        launch_mismatch<Real>(b, true, b->clock, max_iter, tol, 1, 0.0);
+        launch_factor<Real>(b->plan, static_cast<const Real*>(b->A), static_cast<Real*>(b->U), b->a_stride,
+                            b->u_stride, b->tile_active, b->jmax, b->clamped, b->error, b->active_rows, b->T, eps,
+                            b->stream);
+        launch_backward<Real>(b->plan, static_cast<const Real*>(b->U), b->u_stride, b->tile_active,
+                              static_cast<Real*>(b->X), b->error, b->active_rows, b->T, b->stream);
+        if (b->ir_steps > 0)
+            refine<Real>(b->plan, static_cast<Real*>(b->A), static_cast<Real*>(b->U), static_cast<Real*>(b->X),
+                         static_cast<Real*>(b->Bs), static_cast<Real*>(b->Xs), b->a_stride, b->u_stride,
+                         b->tile_active, b->jmax, b->clamped, b->error, b->active_rows, b->T, eps, b->ir_steps,
+                         b->stream);
+        launch_update<Real>(b);
+        ++b->clock;
+    }
+}
+
 void check_stream_error(g2g_batch_s* b)
 {
     int err = 0;
@@ -587,6 +631,7 @@ template <typename Real>
 void solve_impl(g2g_batch_s* b, int max_iter, double tol, int sched, g2g_solve_stats* stats)
 {
     const int freeze = sched == G2G_SCHED_CONTINUOUS ? 1 : 0;
+    b->streaming = false;
     reset_rows(b);
     SolveTimer tm(b, 3 * (max_iter + 1) + 2);
     const int e_begin = tm.mark();
@@ -611,6 +656,7 @@ void solve_warm_impl(g2g_batch_s* b, g2g_mlp_s* mlp, const unsigned char* mask, 
     const NrPlan& nr = b->plan->nr;
     const size_t rp = static_cast<size_t>(b->rows_pad);
     const size_t nv = rp * nr.n;
+    b->streaming = false;
     if (!b->V0) {
         b->V0 = dalloc<double2>(nv);
         b->Vc = dalloc<double2>(nv);
@@ -1011,7 +1057,8 @@ int g2g_batch_set_ybus_edits(g2g_batch_t b, const int32_t* row_ptr, const int32_
             int* d_rp = dupload(rp);
             int* d_en = dupload(en);
             double2* d_de = dupload(de);
-            k_ybus_edits<<<blocks_for(rows, 256), 256, 0, b->stream>>>(d_rp, d_en, d_de, rows, nr.nY, b->Yrow);
+            k_ybus_edits<<<blocks_for(rows, 256), 256, 0, b->stream>>>(d_rp, d_en, d_de, rows, nr.nY, b->Yrow,
+                                                                       nullptr);
             HIPCHK(hipGetLastError());
             HIPCHK(hipStreamSynchronize(b->stream));
             dfree(d_rp);
@@ -1226,6 +1273,123 @@ int g2g_lu_solve_batch(g2g_plan_t P, int32_t rows, int32_t precision, const doub
         if (precision == G2G_MIXED) raw_solve<float>(P, rows, vals, rhs, x, pivot_eps, ir_steps, clamped, stream);
         else if (precision == G2G_FP64) raw_solve<double>(P, rows, vals, rhs, x, pivot_eps, ir_steps, clamped, stream);
         else throw std::invalid_argument("unknown precision");
+    });
+}
+
+int g2g_batch_stream_begin(g2g_batch_t b)
+{
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        const size_t rp = static_cast<size_t>(b->rows_pad);
+        if (!b->it0) b->it0 = dalloc<int>(rp);
+        k_fill_int<<<blocks_for(rp, 256), 256, 0, b->stream>>>(b->it0, 0, rp);
+        k_fill_int<<<blocks_for(rp, 256), 256, 0, b->stream>>>(b->status, kNotSimulated, rp);
+        k_fill_int<<<blocks_for(rp, 256), 256, 0, b->stream>>>(b->iters, 0, rp);
+        k_count_tiles<<<b->T, kLanes, 0, b->stream>>>(b->status, b->rows_pad, b->tile_active);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(b->stream));
+        b->clock = 0;
+        b->active_rows = b->rows;
+        b->streaming = true;
+    });
+}
+
+int g2g_batch_stream_fill(g2g_batch_t b, int32_t n, const int32_t* slots, const double* S, int64_t lds,
+                          const double* V0, int64_t ldv, int32_t on_device, const int32_t* edit_ptr,
+                          const int32_t* edit_ent, const double* edit_delta)
+{
+    return guarded([&] {
+        if (!b || (n > 0 && (!slots || !S || !V0))) throw std::invalid_argument("null argument");
+        if (!b->streaming) throw std::invalid_argument("call g2g_batch_stream_begin first");
+        if (n <= 0) return;
+        const NrPlan& nr = b->plan->nr;
+        for (int r = 0; r < n; ++r)
+            if (slots[r] < 0 || slots[r] >= b->rows) throw std::invalid_argument("slot outside the batch");
+        if (edit_ptr && !b->per_row_y) throw std::invalid_argument("the batch was created without per-row Ybus values");
+        hipStream_t st = b->stream;
+        DeviceScratch tmp;
+        int* d_slots = tmp.get<int>(n);
+        HIPCHK(hipMemcpyAsync(d_slots, slots, n * sizeof(int), hipMemcpyHostToDevice, st));
+        auto rows_src = [&](const double* src, long long ld) -> const double2* {
+            if (on_device) return reinterpret_cast<const double2*>(src);
+            const size_t cnt = ld == 0 ? static_cast<size_t>(nr.n) : static_cast<size_t>(n - 1) * ld + nr.n;
+            double2* d = tmp.get<double2>(cnt);
+            HIPCHK(hipMemcpyAsync(d, src, cnt * sizeof(double2), hipMemcpyHostToDevice, st));
+            return d;
+        };
+        const unsigned g = blocks_for(static_cast<long long>(n) * nr.n, 256);
+        k_rows_to_slots<double2><<<g, 256, 0, st>>>(rows_src(S, lds), lds, n, nr.n, d_slots, b->S);
+        k_rows_to_slots<double2><<<g, 256, 0, st>>>(rows_src(V0, ldv), ldv, n, nr.n, d_slots, b->V);
+        if (b->per_row_y) {
+            k_ybus_reset_slots<<<blocks_for(static_cast<long long>(n) * nr.nY, 256), 256, 0, st>>>(
+                d_slots, n, b->plan->d_Y, nr.nY, b->Yrow);
+            if (edit_ptr && edit_ptr[n] > 0) {
+                const int ne = edit_ptr[n];
+                for (int q = 0; q < ne; ++q)
+                    if (edit_ent[q] < 0 || edit_ent[q] >= nr.nY) throw std::invalid_argument("edit entry out of range");
+                int* d_rp = tmp.get<int>(n + 1);
+                int* d_en = tmp.get<int>(ne);
+                double2* d_de = tmp.get<double2>(ne);
+                HIPCHK(hipMemcpyAsync(d_rp, edit_ptr, (n + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(d_en, edit_ent, ne * sizeof(int), hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(d_de, edit_delta, ne * sizeof(double2), hipMemcpyHostToDevice, st));
+                k_ybus_edits<<<blocks_for(n, 256), 256, 0, st>>>(d_rp, d_en, d_de, n, nr.nY, b->Yrow, d_slots);
+            }
+        }
+        const unsigned gs = blocks_for(n, 256);
+        k_set_slots<<<gs, 256, 0, st>>>(d_slots, n, b->status, kActive);
+        k_set_slots<<<gs, 256, 0, st>>>(d_slots, n, b->it0, b->clock);
+        k_set_slots<<<gs, 256, 0, st>>>(d_slots, n, b->iters, 0);
+        k_count_tiles<<<b->T, kLanes, 0, st>>>(b->status, b->rows_pad, b->tile_active);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));  // the staging buffers are freed on return
+    });
+}
+
+int g2g_batch_stream_round(g2g_batch_t b, int32_t k, int32_t max_iter, double tol)
+{
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        if (!b->streaming) throw std::invalid_argument("call g2g_batch_stream_begin first");
+        if (k <= 0 || max_iter < 0) throw std::invalid_argument("k must be positive and max_iter >= 0");
+        if (b->precision == G2G_MIXED) stream_round<float>(b, k, max_iter, tol);
+        else stream_round<double>(b, k, max_iter, tol);
+        check_stream_error(b);
+    });
+}
+
+int g2g_batch_stream_take(g2g_batch_t b, int32_t n, const int32_t* slots, double* V, int64_t ldv, int32_t on_device)
+{
+    return guarded([&] {
+        if (!b || (n > 0 && !slots)) throw std::invalid_argument("null argument");
+        if (!b->streaming) throw std::invalid_argument("call g2g_batch_stream_begin first");
+        if (n <= 0) return;
+        const NrPlan& nr = b->plan->nr;
+        for (int r = 0; r < n; ++r)
+            if (slots[r] < 0 || slots[r] >= b->rows) throw std::invalid_argument("slot outside the batch");
+        hipStream_t st = b->stream;
+        DeviceScratch tmp;
+        int* d_slots = tmp.get<int>(n);
+        HIPCHK(hipMemcpyAsync(d_slots, slots, n * sizeof(int), hipMemcpyHostToDevice, st));
+        if (V) {
+            double2* dst = on_device ? reinterpret_cast<double2*>(V) : tmp.get<double2>(static_cast<size_t>(n) * nr.n);
+            const long long ld = on_device ? ldv : nr.n;
+            k_slots_to_rows<double2><<<blocks_for(static_cast<long long>(n) * nr.n, 256), 256, 0, st>>>(
+                b->V, n, nr.n, d_slots, dst, ld);
+            HIPCHK(hipGetLastError());
+            if (!on_device) {
+                std::vector<double2> h(static_cast<size_t>(n) * nr.n);
+                HIPCHK(hipMemcpyAsync(h.data(), dst, h.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                for (int r = 0; r < n; ++r)
+                    std::memcpy(V + 2 * static_cast<size_t>(r) * ldv, h.data() + static_cast<size_t>(r) * nr.n,
+                                nr.n * sizeof(double2));
+            }
+        }
+        k_set_slots<<<blocks_for(n, 256), 256, 0, st>>>(d_slots, n, b->status, kNotSimulated);
+        k_count_tiles<<<b->T, kLanes, 0, st>>>(b->status, b->rows_pad, b->tile_active);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
     });
 }
 

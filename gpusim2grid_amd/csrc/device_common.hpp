@@ -29,6 +29,7 @@ struct MismatchArgs {
     double tol;
     int freeze;             // 1: a converged row stops (continuous); 0: fixed count (chunked)
     double growth;          // > 0: a row whose ||F||inf exceeds growth x its previous one is rejected
+    const int* it0;         // nullable: per row, the iteration its solve started at (streamed rows)
 };
 
 template <typename Real>

@@ -63,6 +63,8 @@ __device__ __forceinline__ double nan_max(double m, double v)
 // k_mismatch: one workgroup per tile, kMisWaves waves split the buses
 // ---------------------------------------------------------------------------
 
+constexpr int kMisBatch = 4;  // Ybus entries of a bus gathered together
+
 template <typename Real, bool PerRowY, bool WithJ>
 __global__ __launch_bounds__(kLanes* kMisWaves) void k_mismatch(MismatchArgs<Real> a)
 {
@@ -86,40 +88,55 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_mismatch(MismatchArgs<Rea
             const int* bus = a.k1_bus + 6 * bb;
             const int i = bus[0], e0 = bus[1], e1 = bus[2], fp = bus[3], fq = bus[4], dk = bus[5];
             const double2 vi = V[tidx(tile, a.n, i, lane)];
+            const double2 sb = S[tidx(tile, a.n, i, lane)];
             double2 I = make_double2(0.0, 0.0);
             double dr = 0.0, di = 0.0;  // diagonal admittance
             int dent = -1;
-            for (int e = e0; e < e1; ++e) {
-                const int* en = a.k1_ent + 6 * e;
-                const int j = en[0], yk = en[1];
-                const double2 y = PerRowY ? a.Yrow[tidx(tile, a.nY, yk, lane)] : a.Y[yk];
-                if (j == i) {
-                    dr = y.x;
-                    di = y.y;
-                    dent = e;
-                    I.x = fma(y.x, vi.x, fma(-y.y, vi.y, I.x));
-                    I.y = fma(y.x, vi.y, fma(y.y, vi.x, I.y));
-                    continue;
+            // entries in groups of kMisBatch: every group's admittances and
+            // neighbour voltages are loaded before any of them is used, so a
+            // wave keeps that many gathers in flight
+            for (int eb = e0; eb < e1; eb += kMisBatch) {
+                int j[kMisBatch];
+                double2 y[kMisBatch], vj[kMisBatch];
+#pragma unroll
+                for (int q = 0; q < kMisBatch; ++q) {
+                    const int e = eb + q < e1 ? eb + q : e1 - 1;
+                    const int* en = a.k1_ent + 6 * e;
+                    j[q] = eb + q < e1 ? en[0] : -1;
+                    const int yk = en[1];
+                    y[q] = PerRowY ? a.Yrow[tidx(tile, a.nY, yk, lane)] : a.Y[yk];
+                    vj[q] = (j[q] >= 0 && j[q] != i) ? V[tidx(tile, a.n, j[q], lane)] : vi;
                 }
-                const double2 vj = V[tidx(tile, a.n, j, lane)];
-                const double2 t = cmul(y, vj);
-                I.x += t.x;
-                I.y += t.y;
-                if (WithJ) {
-                    const double2 w = cmulc(vi, t);  // V_i conj(Y_ij V_j)
-                    const double inv = 1.0 / sqrt(fma(vj.x, vj.x, vj.y * vj.y));
-                    const int a11 = en[2], a12 = en[3], a21 = en[4], a22 = en[5];
-                    const double j11 = w.y, j21 = -w.x, j12 = w.x * inv, j22 = w.y * inv;
-                    if (a11 >= 0) a.A[tidx(tile, a.a_stride, a11, lane)] = static_cast<Real>(j11);
-                    if (a12 >= 0) a.A[tidx(tile, a.a_stride, a12, lane)] = static_cast<Real>(j12);
-                    if (a21 >= 0) a.A[tidx(tile, a.a_stride, a21, lane)] = static_cast<Real>(j21);
-                    if (a22 >= 0) a.A[tidx(tile, a.a_stride, a22, lane)] = static_cast<Real>(j22);
-                    jmax = fmax(jmax, fmax(fmax(fabs(j11), fabs(j12)), fmax(fabs(j21), fabs(j22))));
+#pragma unroll
+                for (int q = 0; q < kMisBatch; ++q) {
+                    if (j[q] < 0) continue;
+                    if (j[q] == i) {
+                        dr = y[q].x;
+                        di = y[q].y;
+                        dent = eb + q;
+                        I.x = fma(y[q].x, vi.x, fma(-y[q].y, vi.y, I.x));
+                        I.y = fma(y[q].x, vi.y, fma(y[q].y, vi.x, I.y));
+                        continue;
+                    }
+                    const double2 t = cmul(y[q], vj[q]);
+                    I.x += t.x;
+                    I.y += t.y;
+                    if (WithJ) {
+                        const int* en = a.k1_ent + 6 * (eb + q);
+                        const double2 w = cmulc(vi, t);  // V_i conj(Y_ij V_j)
+                        const double inv = 1.0 / sqrt(fma(vj[q].x, vj[q].x, vj[q].y * vj[q].y));
+                        const int a11 = en[2], a12 = en[3], a21 = en[4], a22 = en[5];
+                        const double j11 = w.y, j21 = -w.x, j12 = w.x * inv, j22 = w.y * inv;
+                        if (a11 >= 0) a.A[tidx(tile, a.a_stride, a11, lane)] = static_cast<Real>(j11);
+                        if (a12 >= 0) a.A[tidx(tile, a.a_stride, a12, lane)] = static_cast<Real>(j12);
+                        if (a21 >= 0) a.A[tidx(tile, a.a_stride, a21, lane)] = static_cast<Real>(j21);
+                        if (a22 >= 0) a.A[tidx(tile, a.a_stride, a22, lane)] = static_cast<Real>(j22);
+                        jmax = fmax(jmax, fmax(fmax(fabs(j11), fabs(j12)), fmax(fabs(j21), fabs(j22))));
+                    }
                 }
             }
             (void)dk;
             const double2 sc = cmulc(vi, I);  // V conj(I)
-            const double2 sb = S[tidx(tile, a.n, i, lane)];
             const double fpv = -(sc.x - sb.x);
             rmax = nan_max(rmax, fpv);
             a.A[tidx(tile, a.a_stride, fp, lane)] = static_cast<Real>(fpv);
@@ -161,14 +178,15 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_mismatch(MismatchArgs<Rea
         const double prev = a.res[row];
         a.res[row] = r;
         if (WithJ) a.jmax[row] = jm;
+        const int it = a.it0 ? a.iter - a.it0[row] : a.iter;  // updates this row has had
         int st = kActive;
         if (!isfinite(r)) st = kDiverged;
         else if (a.freeze && r < a.tol) st = kConverged;
-        else if (a.growth > 0.0 && a.iter > 0 && r > a.growth * prev) st = kRejected;
-        else if (a.iter >= a.max_iter) st = r < a.tol ? kConverged : kMaxIter;
+        else if (a.growth > 0.0 && it > 0 && r > a.growth * prev) st = kRejected;
+        else if (it >= a.max_iter) st = r < a.tol ? kConverged : kMaxIter;
         if (st != kActive) {
             a.status[row] = st;
-            a.iters[row] = a.iter;
+            a.iters[row] = it;
         }
         still = st == kActive;
     }
@@ -952,21 +970,36 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_update(UpdateArgs<Real> a
         if (mt > a.max_dva) alpha = fmin(alpha, a.max_dva / mt);
         if (mv > a.max_dvm) alpha = fmin(alpha, a.max_dvm / mv);
     }
-    for (int bb = b0; bb < b1; ++bb) {
-        const int bus = a.upd[3 * bb], kt = a.upd[3 * bb + 1], kv = a.upd[3 * bb + 2];
-        double dva = static_cast<double>(a.X[tidx(tile, a.d, kt, lane)]);
-        double dvm = kv >= 0 ? static_cast<double>(a.X[tidx(tile, a.d, kv, lane)]) : 0.0;
-        if (alpha != 1.0) {
-            dva *= alpha;
-            dvm *= alpha;
+    // buses in groups of kUpdBatch, every group's loads issued before its stores
+    constexpr int kUpdBatch = 4;
+    for (int bq = b0; bq < b1; bq += kUpdBatch) {
+        int bus[kUpdBatch];
+        double dva[kUpdBatch], dvm[kUpdBatch];
+        double2 v[kUpdBatch];
+#pragma unroll
+        for (int q = 0; q < kUpdBatch; ++q) {
+            const int bb = bq + q < b1 ? bq + q : b1 - 1;
+            const int kt = a.upd[3 * bb + 1], kv = a.upd[3 * bb + 2];
+            bus[q] = bq + q < b1 ? a.upd[3 * bb] : -1;
+            dva[q] = static_cast<double>(a.X[tidx(tile, a.d, kt, lane)]);
+            dvm[q] = kv >= 0 ? static_cast<double>(a.X[tidx(tile, a.d, kv, lane)]) : 0.0;
+            v[q] = a.V[tidx(tile, a.n, a.upd[3 * bb], lane)];
         }
-        double2* vp = a.V + tidx(tile, a.n, bus, lane);
-        const double2 v = *vp;
-        const double m = sqrt(fma(v.x, v.x, v.y * v.y));
-        const double sc = (m + dvm) / m;
-        double sn, cs;
-        sincos(dva, &sn, &cs);
-        if (live) *vp = make_double2(sc * fma(v.x, cs, -v.y * sn), sc * fma(v.x, sn, v.y * cs));
+#pragma unroll
+        for (int q = 0; q < kUpdBatch; ++q) {
+            if (bus[q] < 0) continue;
+            if (alpha != 1.0) {
+                dva[q] *= alpha;
+                dvm[q] *= alpha;
+            }
+            const double m = sqrt(fma(v[q].x, v[q].x, v[q].y * v[q].y));
+            const double sc = (m + dvm[q]) / m;
+            double sn, cs;
+            sincos(dva[q], &sn, &cs);
+            if (live)
+                a.V[tidx(tile, a.n, bus[q], lane)] =
+                    make_double2(sc * fma(v[q].x, cs, -v[q].y * sn), sc * fma(v[q].x, sn, v[q].y * cs));
+        }
     }
 }
 
@@ -1064,16 +1097,63 @@ __global__ void k_ybus_broadcast(const double2* __restrict__ Y, int nY, int rows
     Yrow[tidx(static_cast<size_t>(tile), nY, k, lane)] = Y[k];
 }
 
+// subtracts row r's edits from the per-row values of batch row slots[r]
+// (slots nullable: row r itself)
 __global__ void k_ybus_edits(const int* __restrict__ row_ptr, const int* __restrict__ ent,
-                             const double2* __restrict__ delta, int rows, int nY, double2* __restrict__ Yrow)
+                             const double2* __restrict__ delta, int rows, int nY, double2* __restrict__ Yrow,
+                             const int* __restrict__ slots)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
+    const int b = slots ? slots[r] : r;
     for (int q = row_ptr[r]; q < row_ptr[r + 1]; ++q) {
-        double2* y = Yrow + tidx(static_cast<size_t>(r / kLanes), nY, ent[q], r % kLanes);
+        double2* y = Yrow + tidx(static_cast<size_t>(b / kLanes), nY, ent[q], b % kLanes);
         const double2 v = *y;
         *y = make_double2(v.x - delta[q].x, v.y - delta[q].y);
     }
+}
+
+// the shared Ybus values into the per-row values of batch rows slots[0..R)
+__global__ void k_ybus_reset_slots(const int* __restrict__ slots, int R, const double2* __restrict__ Y, int nY,
+                                   double2* __restrict__ Yrow)
+{
+    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= static_cast<long long>(R) * nY) return;
+    const int r = static_cast<int>(gid / nY), k = static_cast<int>(gid - static_cast<long long>(r) * nY);
+    const int b = slots[r];
+    Yrow[tidx(static_cast<size_t>(b / kLanes), nY, k, b % kLanes)] = Y[k];
+}
+
+// row-major rows [R][m] (row stride ld; ld = 0 broadcasts row 0) into the
+// tiled rows slots[0..R) of dst [T][m][64]
+template <typename T>
+__global__ void k_rows_to_slots(const T* __restrict__ src, long long ld, int R, int m, const int* __restrict__ slots,
+                                T* __restrict__ dst)
+{
+    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= static_cast<long long>(R) * m) return;
+    const int r = static_cast<int>(gid / m), j = static_cast<int>(gid - static_cast<long long>(r) * m);
+    const int b = slots[r];
+    dst[tidx(static_cast<size_t>(b / kLanes), m, j, b % kLanes)] = src[static_cast<long long>(r) * ld + j];
+}
+
+// tiled rows slots[0..R) of src [T][m][64] into row-major rows [R][m] (stride ld)
+template <typename T>
+__global__ void k_slots_to_rows(const T* __restrict__ src, int R, int m, const int* __restrict__ slots,
+                                T* __restrict__ dst, long long ld)
+{
+    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= static_cast<long long>(R) * m) return;
+    const int r = static_cast<int>(gid / m), j = static_cast<int>(gid - static_cast<long long>(r) * m);
+    const int b = slots[r];
+    dst[static_cast<long long>(r) * ld + j] = src[tidx(static_cast<size_t>(b / kLanes), m, j, b % kLanes)];
+}
+
+// per-slot scalars: v[slots[r]] = value
+__global__ void k_set_slots(const int* __restrict__ slots, int R, int* __restrict__ v, int value)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R) v[slots[r]] = value;
 }
 
 // Gathers CSR values [rows][nnz] (and a rhs [rows][d]) into the A stream of
@@ -1208,6 +1288,8 @@ template __global__ void k_mismatch<float, true, true>(MismatchArgs<float>);
 template __global__ void k_mismatch<float, false, false>(MismatchArgs<float>);
 template __global__ void k_mismatch<float, true, false>(MismatchArgs<float>);
 template __global__ void k_move_rows<double2>(const double2*, double2*, const int*, int, int, int);
+template __global__ void k_rows_to_slots<double2>(const double2*, long long, int, int, const int*, double2*);
+template __global__ void k_slots_to_rows<double2>(const double2*, int, int, const int*, double2*, long long);
 template __global__ void k_move_vals<int>(const int*, int*, const int*, int, int, int);
 template __global__ void k_move_vals<double>(const double*, double*, const int*, int, int, int);
 template __global__ void k_factor<double, 1>(FactorArgs<double>);

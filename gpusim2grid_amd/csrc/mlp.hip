@@ -274,30 +274,44 @@ __global__ void k_transpose_bf16(const float* __restrict__ W, int K, int N, __bf
 }
 
 // column sums of an fp32 M x N matrix (bias gradients), one thread per column
-__global__ void k_colsum(const float* __restrict__ X, long long ldx, int M, int N, float* __restrict__ out)
+// Column sums of X [M][N] accumulated into out (zeroed by the caller): a
+// block sums kColRows rows of 64 columns, four row groups per column reduced
+// in LDS, then one atomic per column.  Grid: (ceil(N / 64), ceil(M / kColRows)).
+__global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ X, long long ldx, int M, int N,
+                                                float* __restrict__ out)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+    __shared__ float part[4][64];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + c;
+    const int m0 = blockIdx.y * kColRows;
+    const int m1 = m0 + kColRows < M ? m0 + kColRows : M;
     float s = 0.0f;
-    for (int m = 0; m < M; ++m) s += X[static_cast<long long>(m) * ldx + n];
-    out[n] = s;
+    if (n < N)
+        for (int m = m0 + g; m < m1; m += 4) s += X[static_cast<long long>(m) * ldx + n];
+    part[g][c] = s;
+    __syncthreads();
+    if (g == 0 && n < N) atomicAdd(&out[n], part[0][c] + part[1][c] + part[2][c] + part[3][c]);
 }
 
-// MSE loss gradient dY = 2 (Y - T) / (M N) (fp32 and bf16) and the loss sum
-__global__ void k_mse_grad(const float* __restrict__ Y, const float* __restrict__ T, long long n_el, float scale,
-                           float* __restrict__ dY, __bf16* __restrict__ dYb, float* __restrict__ loss)
+// MSE loss gradient dY = 2 (Y - T) / (M N) (fp32 and bf16) and the loss sum:
+// grid-stride over the elements, one atomic per block
+__global__ __launch_bounds__(256) void k_mse_grad(const float* __restrict__ Y, const float* __restrict__ T,
+                                                  long long n_el, float scale, float* __restrict__ dY,
+                                                  __bf16* __restrict__ dYb, float* __restrict__ loss)
 {
-    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    __shared__ float part[4];
     float e = 0.0f;
-    if (i < n_el) {
+    for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_el;
+         i += static_cast<long long>(gridDim.x) * blockDim.x) {
         const float d = Y[i] - T[i];
-        e = d * d;
+        e += d * d;
         dY[i] = 2.0f * scale * d;
         dYb[i] = static_cast<__bf16>(2.0f * scale * d);
     }
-    // wave reduction, one atomic per wave
     for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss, e * scale);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, (part[0] + part[1] + part[2] + part[3]) * scale);
 }
 
 // Adam on fp32 master weights, refreshing the bf16 copy (nullable)

@@ -34,6 +34,7 @@ __global__ void k_gemm(GemmArgs g);
 template <int EPI>
 __global__ void k_gemm_kc(GemmArgs g);
 __global__ void k_transpose_bf16(const float* W, int K, int N, __bf16* Wt);
+constexpr int kColRows = 128;  // rows a k_colsum block sums
 __global__ void k_colsum(const float* X, long long ldx, int M, int N, float* out);
 __global__ void k_mse_grad(const float* Y, const float* T, long long n_el, float scale, float* dY, __bf16* dYb,
                            float* loss);

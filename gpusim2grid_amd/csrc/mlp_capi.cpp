@@ -1,6 +1,8 @@
 // C ABI of the learned warm start (see include/gpusim2grid_amd.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -162,28 +164,39 @@ void forward(g2g_mlp_s* p, int M, bool keep_z = true)
                                     nullptr, 0, nullptr, 0, p->w + p->o_b3);
 }
 
+// out[n] = sum over the M rows of X[m][n]
+void colsum(hipStream_t st, const float* X, long long ldx, int M, int N, float* out)
+{
+    HIPCHK(hipMemsetAsync(out, 0, static_cast<size_t>(N) * sizeof(float), st));
+    const dim3 grid((N + 63) / 64, (M + kColRows - 1) / kColRows);
+    k_colsum<<<grid, 256, 0, st>>>(X, ldx, M, N, out);
+    HIPCHK(hipGetLastError());
+}
+
 // loss and gradients (into p->g) of the staged X against targets p->T
 void backward(g2g_mlp_s* p, int M)
 {
     hipStream_t st = p->stream;
     const long long ne = static_cast<long long>(M) * p->n_out;
     HIPCHK(hipMemsetAsync(p->loss, 0, sizeof(float), st));
-    k_mse_grad<<<blocks_for(ne, 256), 256, 0, st>>>(p->Y, p->T, ne, 1.0f / static_cast<float>(ne), p->dY, p->dYb,
+    k_mse_grad<<<std::min<unsigned>(blocks_for(ne, 256), 2048), 256, 0, st>>>(p->Y, p->T, ne,
+                                                                              1.0f / static_cast<float>(ne), p->dY,
+                                                                              p->dYb,
                                                     p->loss);
     HIPCHK(hipGetLastError());
     // layer 3: dW3 = H2^T dY, db3 = sum dY, dZ2 = (dY W3^T) gelu'(Z2)
     gemm<true, false, kEpiF32>(st, p->H2, p->H, p->dYb, p->n_out, p->H, p->n_out, M, p->g + p->o_W3, p->n_out);
-    k_colsum<<<blocks_for(p->n_out, 256), 256, 0, st>>>(p->dY, p->n_out, M, p->n_out, p->g + p->o_b3);
+    colsum(st, p->dY, p->n_out, M, p->n_out, p->g + p->o_b3);
     gemm<false, true, kEpiGeluBwd>(st, p->dYb, p->n_out, p->W3b(), p->n_out, M, p->H, p->n_out, p->dZ2, p->H,
                                    p->dZ2b, p->H, p->Z2, p->H);
     // layer 2
     gemm<true, false, kEpiF32>(st, p->H1, p->H, p->dZ2b, p->H, p->H, p->H, M, p->g + p->o_W2, p->H);
-    k_colsum<<<blocks_for(p->H, 256), 256, 0, st>>>(p->dZ2, p->H, M, p->H, p->g + p->o_b2);
+    colsum(st, p->dZ2, p->H, M, p->H, p->g + p->o_b2);
     gemm<false, true, kEpiGeluBwd>(st, p->dZ2b, p->H, p->W2b(), p->H, M, p->H, p->H, p->dZ1, p->H, p->dZ1b, p->H,
                                    p->Z1, p->H);
     // layer 1
     gemm<true, false, kEpiF32>(st, p->X, p->n_in, p->dZ1b, p->H, p->n_in, p->H, M, p->g + p->o_W1, p->H);
-    k_colsum<<<blocks_for(p->H, 256), 256, 0, st>>>(p->dZ1, p->H, M, p->H, p->g + p->o_b1);
+    colsum(st, p->dZ1, p->H, M, p->H, p->g + p->o_b1);
     HIPCHK(hipGetLastError());
 }
 

@@ -403,15 +403,99 @@ def random_outages(grid: Grid, n_rows: int, max_k: int = 2, seed: int = 3):
     rng = np.random.default_rng(seed)
     nb = ~bridges(grid.n_bus, grid.branch_from, grid.branch_to)
     cand = np.flatnonzero(nb)
+    label = _cut_labels(grid) if max_k == 2 else None
     out = []
     for _ in range(n_rows):
         k = int(rng.integers(0, max_k + 1))
         while True:
             ids = sorted(rng.choice(cand, size=k, replace=False).tolist()) if k else []
-            if k <= 1 or _connected_without(grid, ids):
+            if k <= 1:
+                break
+            # two non-bridges disconnect the grid exactly when they carry the
+            # same cut label
+            if (label[ids[0]] != label[ids[1]]) if label is not None else _connected_without(grid, ids):
                 break
         out.append(ids)
     return out
+
+
+def _cut_labels(grid: Grid) -> np.ndarray:
+    """Random 64-bit cycle-space labels of the branches: a spanning-tree
+    branch gets the XOR of the labels of the non-tree branches that cross its
+    subtree's boundary.  A branch is a bridge when its label is 0, and two
+    branches form a 2-edge cut when their labels are equal (with probability
+    1 - 2^-64 per pair)."""
+    n, f, t = grid.n_bus, np.asarray(grid.branch_from), np.asarray(grid.branch_to)
+    m = len(f)
+    rng = np.random.default_rng(0x5EED)
+    adj = [[] for _ in range(n)]
+    for e in range(m):
+        adj[int(f[e])].append((int(t[e]), e))
+        adj[int(t[e])].append((int(f[e]), e))
+    parent_edge = np.full(n, -1, dtype=np.int64)
+    seen = np.zeros(n, dtype=bool)
+    order = []
+    tree = np.zeros(m, dtype=bool)
+    for s0 in range(n):
+        if seen[s0]:
+            continue
+        seen[s0] = True
+        stack = [s0]
+        while stack:
+            v = stack.pop()
+            order.append(v)
+            for u, e in adj[v]:
+                if not seen[u]:
+                    seen[u] = True
+                    parent_edge[u] = e
+                    tree[e] = True
+                    stack.append(u)
+    label = np.zeros(m, dtype=np.uint64)
+    label[~tree] = rng.integers(1, np.iinfo(np.int64).max, size=int((~tree).sum()), dtype=np.int64).astype(np.uint64)
+    acc = np.zeros(n, dtype=np.uint64)
+    for e in np.flatnonzero(~tree):
+        acc[f[e]] ^= label[e]
+        acc[t[e]] ^= label[e]
+    for v in reversed(order):  # children before parents
+        e = parent_edge[v]
+        if e >= 0:
+            label[e] = acc[v]
+            other = int(f[e]) if int(t[e]) == v else int(t[e])
+            acc[other] ^= acc[v]
+    return label
+
+
+def outage_edits(grid: Grid, rows_branches):
+    """Per-row Ybus value edits of branch trips, the (row_ptr, entry, delta)
+    arrays ``Batch.set_ybus_edits`` takes: tripping branch l (f -> t)
+    subtracts yff, ytt, yft, ytf at the Ybus CSR positions (f,f), (t,t), (f,t),
+    (t,f) (contingency_analysis_helper.cpp:163, build_flat_patches)."""
+    Y = sp.csr_matrix(grid.Ybus)
+    Y.sort_indices()
+
+    def k_of(i, j):
+        lo, hi = Y.indptr[i], Y.indptr[i + 1]
+        return lo + int(np.searchsorted(Y.indices[lo:hi], j))
+
+    row_ptr, ent, delta = [0], [], []
+    for brs in rows_branches:
+        for l in brs:
+            f, t = int(grid.branch_from[l]), int(grid.branch_to[l])
+            for (i, j, v) in ((f, f, grid.yff[l]), (t, t, grid.ytt[l]), (f, t, grid.yft[l]), (t, f, grid.ytf[l])):
+                ent.append(k_of(i, j))
+                delta.append(v)
+        row_ptr.append(len(ent))
+    return (np.array(row_ptr, dtype=np.int32), np.array(ent, dtype=np.int32),
+            np.array(delta, dtype=np.complex128))
+
+
+def outage_mask(grid: Grid, rows_branches) -> np.ndarray:
+    """0/1 uint8 [rows, n_branch]: 1 where the row trips the branch (the
+    learned warm start's mask input)."""
+    m = np.zeros((len(rows_branches), grid.n_branch), dtype=np.uint8)
+    for r, brs in enumerate(rows_branches):
+        m[r, list(brs)] = 1
+    return m
 
 
 def _connected_without(grid: Grid, ids) -> bool:

@@ -134,6 +134,8 @@ class WarmStart:
         """Start voltages predicted for S (host complex [rows, n]) -> host complex."""
         import torch
         St = torch.as_tensor(np.atleast_2d(S), device="cuda")
+        if mask is not None:
+            mask = torch.as_tensor(np.atleast_2d(mask), device="cuda")
         X, _ = self._normalised(St, St.new_ones(St.shape), mask)
         Y = torch.empty((X.shape[0], self.n_out), device="cuda", dtype=torch.float32)
         torch.cuda.synchronize()

@@ -201,6 +201,25 @@ int g2g_batch_eval(g2g_batch_t batch, double *F, double *J, double *residual);
 int g2g_lu_solve_batch(g2g_plan_t plan, int32_t rows, int32_t precision, const double *vals, const double *rhs,
                        double *x, double pivot_eps, int32_t ir_steps, int32_t *clamped, void *stream);
 
+/* ---- Streamed rows (continuous batching with slot refill; replaces the
+ * reference's _solve_continuous refill, contingency/batch_pf_driver.cu:2720-3100,
+ * and slot_schedule.cuh:82-156).  The batch's rows become slots: a caller
+ * fills free slots with new rows (Sbus, start V and, for per-row Ybus
+ * batches, the row's outage edits), runs rounds of k Newton iterations on
+ * every active slot, reads the slots' statuses with g2g_batch_get_results
+ * (G2G_NOTSIMULATED = free, -1 = still iterating) and takes finished slots'
+ * V out, which frees them.  Each row counts its own updates against max_iter;
+ * a row whose last update of a round converged it is reported by the next
+ * round.  Rows are host or device arrays (on_device), row-major n complex
+ * with row strides lds / ldv in complex numbers (0 broadcasts one row). */
+int g2g_batch_stream_begin(g2g_batch_t batch);
+int g2g_batch_stream_fill(g2g_batch_t batch, int32_t n, const int32_t *slots, const double *S, int64_t lds,
+                          const double *V0, int64_t ldv, int32_t on_device, const int32_t *edit_ptr,
+                          const int32_t *edit_entry, const double *edit_delta);
+int g2g_batch_stream_round(g2g_batch_t batch, int32_t k, int32_t max_iter, double tol);
+int g2g_batch_stream_take(g2g_batch_t batch, int32_t n, const int32_t *slots, double *V, int64_t ldv,
+                          int32_t on_device);
+
 /* Device pointer of the batch's tiled Sbus array [ceil(rows/64)][n][64]
  * complex (bench and zero-copy callers fill it directly). */
 int g2g_batch_device_arrays(g2g_batch_t batch, void **V_tiles, void **S_tiles);

@@ -142,3 +142,51 @@ def test_untrained_mlp_falls_back_to_flat_start():
     assert re.any()
     np.testing.assert_array_equal(it[re] - itf[re] <= 2, True)
     assert np.max(np.abs(V[re] - Vf[re])) < 1e-12
+
+
+def test_warm_start_with_outages_gains_over_flat_start():
+    """The learned-warm-start workload in small: the 2,000-bus grid, load
+    scaling plus 0-2 random non-islanding outages per row, the outage mask
+    among the inputs.  Every row ends as the flat-start oracle's does, and the
+    trained network saves Newton updates over the flat start."""
+    import torch
+    g = syn.grid_c1()
+    n_train, rows = 8192, 512
+    _say("warm outages: solving the training set")
+    S = _sbus(g, n_train, seed=2)
+    outs = syn.random_outages(g, n_train, seed=4)
+    plan = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+    B = _capi.Batch(plan, n_train, precision=_capi.MIXED, per_row_ybus=True)
+    B.set_sbus(S)
+    B.set_v(g.flat_start())
+    B.set_ybus_edits(*syn.outage_edits(g, outs))
+    B.solve(max_iter=10, tol=1e-8)
+    st, _, _ = B.results()
+    V = B.get_v()
+    del B
+    ok = st == 0
+    mask = syn.outage_mask(g, outs)
+    ws = WarmStart(g.n_bus, g.pv, g.pq, g.slack, np.abs(g.flat_start()), n_hidden=512, n_mask=g.n_branch, seed=1)
+    _say("warm outages: training")
+    hist = ws.train(torch.as_tensor(S[ok], device="cuda"), torch.as_tensor(V[ok], device="cuda"),
+                    mask=torch.as_tensor(mask[ok], device="cuda"), epochs=20, batch=256, lr=5e-4)
+    assert hist[-1] < 0.5 * hist[0], hist
+    _say("warm outages: solving the test rows")
+    St = _sbus(g, rows, seed=1)
+    outs_t = syn.random_outages(g, rows, seed=3)
+    Bt = _capi.Batch(plan, rows, precision=_capi.MIXED, per_row_ybus=True)
+    Bt.set_sbus(St)
+    Bt.set_v(g.flat_start())
+    Bt.set_ybus_edits(*syn.outage_edits(g, outs_t))
+    stats = Bt.solve_warm(ws.mlp, k_warm=4, growth=2.0, max_iter=10, tol=1e-8, mask=syn.outage_mask(g, outs_t))
+    st, it, res = Bt.results()
+    Vw = Bt.get_v()
+    Yrows = np.stack([orc.ybus_values_on_pattern(g.Ybus, orc.outage_ybus(g, o)) for o in outs_t])
+    Vo, sto, ito, _ = orc.nr_batch(g.Ybus, g.flat_start(), St, g.pv, g.pq, 10, 1e-8, n_threads=16, Yx_rows=Yrows)
+    np.testing.assert_array_equal(st, sto)
+    conv = sto == 0
+    assert conv.mean() > 0.95 and (res[conv] < 1e-8).all()
+    assert np.abs(Vw[conv] - Vo[conv]).max() < 1e-6
+    _say(f"warm outages: mean updates {it[conv].mean():.2f} vs flat {ito[conv].mean():.2f}, "
+         f"re-queued {stats['n_requeued']}")
+    assert it[conv].mean() <= ito[conv].mean() - 0.75
