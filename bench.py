@@ -148,8 +148,9 @@ def train_warm(args, wl, torch, capi, syn):
                     log=(lambda m: log(args, m)))
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t1
+    n_train = int(S.shape[0])
     del S, V
-    return ws, dict(train_rows=int(S.shape[0]), epochs=args.epochs, hidden=args.hidden, loss_first=hist[0],
+    return ws, dict(train_rows=n_train, epochs=args.epochs, hidden=args.hidden, loss_first=hist[0],
                     loss_last=hist[-1], data_s=t_data, train_s=t_train)
 
 

@@ -72,6 +72,7 @@ def lib():
         "g2g_batch_set_v": (ctypes.c_int, [_P, _P, i64, i32, i32, i32]),
         "g2g_batch_set_sbus": (ctypes.c_int, [_P, _P, i64, i32, i32, i32]),
         "g2g_batch_set_ybus_edits": (ctypes.c_int, [_P, _P, _P, _P, i32]),
+        "g2g_batch_set_bus_mask": (ctypes.c_int, [_P, _P, _P, i32]),
         "g2g_batch_set_active_rows": (ctypes.c_int, [_P, i32]),
         "g2g_batch_solve": (ctypes.c_int, [_P, i32, dbl, i32, ctypes.POINTER(SolveStats)]),
         "g2g_batch_get_v": (ctypes.c_int, [_P, _P, i64, i32]),
@@ -80,6 +81,7 @@ def lib():
         "g2g_lu_solve_batch": (ctypes.c_int, [_P, i32, i32, _P, _P, _P, dbl, i32, _P, _P]),
         "g2g_batch_set_step_control": (ctypes.c_int, [_P, i32, i32, dbl, dbl]),
         "g2g_batch_stream_begin": (ctypes.c_int, [_P]),
+        "g2g_batch_solve_adjoint": (ctypes.c_int, [_P, _P, _P]),
         "g2g_batch_stream_fill": (ctypes.c_int, [_P, i32, _P, _P, i64, _P, i64, i32, _P, _P, _P]),
         "g2g_batch_stream_round": (ctypes.c_int, [_P, i32, i32, dbl]),
         "g2g_batch_stream_take": (ctypes.c_int, [_P, i32, _P, _P, i64, i32]),
@@ -237,8 +239,23 @@ class Batch:
         _check(lib().g2g_batch_set_ybus_edits(self._h, _ptr(row_ptr), _ptr(entry), _ptr(delta),
                                               row_ptr.size - 1))
 
+    def set_bus_mask(self, row_ptr, buses=None):
+        """Islanded buses per row (CSR row_ptr / buses); row_ptr None clears the mask."""
+        if row_ptr is None:
+            _check(lib().g2g_batch_set_bus_mask(self._h, None, None, 0))
+            return
+        row_ptr, buses = _i32(row_ptr), _i32(buses if buses is not None else [])
+        _check(lib().g2g_batch_set_bus_mask(self._h, _ptr(row_ptr), _ptr(buses), row_ptr.size - 1))
+
     def set_active_rows(self, k):
         _check(lib().g2g_batch_set_active_rows(self._h, int(k)))
+
+    def solve_adjoint(self, rhs):
+        """x with J^T x = rhs per active row (J at the batch's V); rhs [rows, d]."""
+        rhs = np.ascontiguousarray(np.atleast_2d(rhs), dtype=np.float64)
+        x = np.empty_like(rhs)
+        _check(lib().g2g_batch_solve_adjoint(self._h, _ptr(rhs), _ptr(x)))
+        return x
 
     # -- streamed rows (continuous batching) --------------------------------
     def stream_begin(self):

@@ -213,6 +213,9 @@ struct RowJob {
     const std::vector<int32_t>* edit_ent = nullptr;
     const std::vector<cplx>* edit_delta = nullptr;
     std::vector<char> skip;        // rows not simulated (islanding trips)
+    // per-row islanded buses (CSR-like, nullable; handle_disconnected_grid)
+    const std::vector<int32_t>* mask_ptr = nullptr;
+    const std::vector<int32_t>* mask_bus = nullptr;
     int max_iter = 10;
     double tol = 1e-8;
     int sched = G2G_SCHED_CONTINUOUS;
@@ -254,8 +257,9 @@ void run_rows(const Grid& g, std::unique_ptr<Batch>& batch, int precision, int b
     std::vector<cplx> Sbuf(static_cast<size_t>(B) * n), Vbuf(static_cast<size_t>(B) * n);
     std::vector<int32_t> st(B), it(B);
     std::vector<double> rs(B);
-    std::vector<int32_t> eptr, eent;
+    std::vector<int32_t> eptr, eent, mptr, mbus;
     std::vector<cplx> edel;
+    if (!job.mask_ptr) check(g2g_batch_set_bus_mask(batch->h, nullptr, nullptr, 0));
     for (size_t c0 = 0; c0 < live.size(); c0 += B) {
         const int m = static_cast<int>(std::min<size_t>(B, live.size() - c0));
         for (int k = 0; k < m; ++k) {
@@ -281,6 +285,16 @@ void run_rows(const Grid& g, std::unique_ptr<Batch>& batch, int precision, int b
             }
             check(g2g_batch_set_ybus_edits(batch->h, eptr.data(), eent.data(),
                                            reinterpret_cast<const double*>(edel.data()), m));
+        }
+        if (job.mask_ptr) {
+            mptr.assign(1, 0);
+            mbus.clear();
+            for (int k = 0; k < m; ++k) {
+                const int r = live[c0 + k];
+                for (int q = (*job.mask_ptr)[r]; q < (*job.mask_ptr)[r + 1]; ++q) mbus.push_back((*job.mask_bus)[q]);
+                mptr.push_back(static_cast<int32_t>(mbus.size()));
+            }
+            check(g2g_batch_set_bus_mask(batch->h, mptr.data(), mbus.data(), m));
         }
         g2g_solve_stats s{};
         if (job.mlp)
@@ -385,6 +399,20 @@ public:
         DArr F(grid_->info.d);
         check(g2g_batch_eval(batch_->h, F.mutable_data(), nullptr, nullptr));
         return F;
+    }
+
+    // J^T x = rhs at the solved V (python_bindings.cpp:1525, solve_JT_dlpack);
+    // a host array here, the facade wraps it as a device capsule
+    DArr solve_JT(DArr rhs)
+    {
+        const int d = grid_->info.d;
+        if (rhs.size() != d) throw py::value_error("rhs must have dim_J entries");
+        DArr x(d);
+        {
+            py::gil_scoped_release nogil;
+            check(g2g_batch_solve_adjoint(batch_->h, rhs.data(), x.mutable_data()));
+        }
+        return x;
     }
 
     py::tuple get_J()
@@ -553,6 +581,42 @@ struct Branches {
         return comps > 1;
     }
 
+    // the buses outside the largest connected component once `out` is tripped
+    // (compute_component_masks, contingency_analysis_helper.cpp:657-692: the
+    // largest by bus count, a tie to the component met first in bus order)
+    std::vector<int> off_main(const std::vector<int>& out, int n) const
+    {
+        std::vector<char> gone(m(), 0);
+        for (int l : out) gone[l] = 1;
+        std::vector<int> par(n);
+        std::iota(par.begin(), par.end(), 0);
+        auto find = [&](int x) {
+            while (par[x] != x) x = par[x] = par[par[x]];
+            return x;
+        };
+        for (int l = 0; l < m(); ++l)
+            if (!gone[l]) {
+                const int a = find(f[l]), b = find(t[l]);
+                if (a != b) par[a] = b;
+            }
+        std::vector<int> label(n, -1), count;
+        std::vector<int> comp(n);
+        for (int i = 0; i < n; ++i) {
+            const int r = find(i);
+            if (label[r] < 0) {
+                label[r] = static_cast<int>(count.size());
+                count.push_back(0);
+            }
+            comp[i] = label[r];
+            ++count[comp[i]];
+        }
+        const int main = static_cast<int>(std::max_element(count.begin(), count.end()) - count.begin());
+        std::vector<int> off;
+        for (int i = 0; i < n; ++i)
+            if (comp[i] != main) off.push_back(i);
+        return off;
+    }
+
     // value edits of one outage, merged by CSR position
     void edits(const Grid& g, const std::vector<int>& out, std::vector<int32_t>& ent, std::vector<cplx>& del) const
     {
@@ -659,6 +723,7 @@ public:
         growth_ = growth;
     }
     void clear_warm_start() { mlp_ = 0; }
+    bool handle_disconnected_grid = false;
 
     void run()
     {
@@ -683,6 +748,8 @@ public:
         run_rows(*grid_, batch_, precision_, batch_size_, rows, job, res_, t);
         t.t_run_ms = now_ms() - t0;
         timings_ = t;
+        disc_.assign(static_cast<size_t>(rows), 0);
+        for (size_t r = 0; r < job.skip.size() && r < disc_.size(); ++r) disc_[r] = job.skip[r] ? 1 : 0;
         ran_ = true;
         flows_ = false;
     }
@@ -697,6 +764,9 @@ public:
     py::array_t<double> get_residuals() const { need_run(); return to_array(res_.res); }
     py::array_t<int32_t> get_row_iterations() const { need_run(); return to_array(res_.iters); }
     py::array_t<int32_t> get_row_status() const { need_run(); return to_array(res_.status); }
+    // per row, 1 = its trips split the grid and it was not simulated (empty
+    // before run(); scenario_sweep_session.hpp:867-870)
+    py::array_t<int32_t> get_disconnected() const { return to_array(disc_); }
     const BatchTimings& get_timings() const { return timings_; }
     int n_bus() const { return grid_->n(); }
     int dim_J() const { return grid_->info.d; }
@@ -805,6 +875,52 @@ protected:
     RowResults res_;
     BatchTimings timings_;
     bool ran_ = false;
+    std::vector<int32_t> disc_;
+
+    // Trips per row: the value edits, and for rows they split, the buses off
+    // the largest component (masked under handle_disconnected_grid unless
+    // the slack is among them: then the row is skipped either way).
+    struct Topology {
+        std::vector<int32_t> eptr{0}, eent;
+        std::vector<cplx> edel;
+        std::vector<char> split, strands_slack;
+        std::vector<int32_t> mptr{0}, mbus;
+
+        void build(const Grid& g, const Branches& br, const std::vector<std::vector<int>>& ids)
+        {
+            const int n = g.n();
+            eptr.assign(1, 0);
+            eent.clear();
+            edel.clear();
+            mptr.assign(1, 0);
+            mbus.clear();
+            split.assign(ids.size(), 0);
+            strands_slack.assign(ids.size(), 0);
+            for (size_t r = 0; r < ids.size(); ++r) {
+                br.edits(g, ids[r], eent, edel);
+                eptr.push_back(static_cast<int32_t>(eent.size()));
+                if (br.islands(ids[r], n)) {
+                    split[r] = 1;
+                    const std::vector<int> off = br.off_main(ids[r], n);
+                    if (std::find(off.begin(), off.end(), g.slack) != off.end()) strands_slack[r] = 1;
+                    else mbus.insert(mbus.end(), off.begin(), off.end());
+                }
+                mptr.push_back(static_cast<int32_t>(mbus.size()));
+            }
+        }
+        void apply(RowJob& job, bool handle) const
+        {
+            job.edit_ptr = &eptr;
+            job.edit_ent = &eent;
+            job.edit_delta = &edel;
+            job.skip.assign(split.size(), 0);
+            for (size_t r = 0; r < split.size(); ++r) job.skip[r] = split[r] && (!handle || strands_slack[r]);
+            if (handle && !mbus.empty()) {
+                job.mask_ptr = &mptr;
+                job.mask_bus = &mbus;
+            }
+        }
+    };
 };
 
 // InjectionSweepSession (python_bindings.cpp:2167-2423)
@@ -880,15 +996,7 @@ public:
     {
         if (!br_.set) throw std::runtime_error("call set_branch_data() first");
         const double t0 = now_ms();
-        eptr_.assign(1, 0);
-        eent_.clear();
-        edel_.clear();
-        skip_.assign(ids.size(), 0);
-        for (size_t r = 0; r < ids.size(); ++r) {
-            skip_[r] = br_.islands(ids[r], n_bus()) ? 1 : 0;
-            if (!skip_[r]) br_.edits(*grid_, ids[r], eent_, edel_);
-            eptr_.push_back(static_cast<int32_t>(eent_.size()));
-        }
+        topo_.build(*grid_, br_, ids);
         ctg_ = ids;
         trips_ = ids;
         timings_.t_preprocess_ms = now_ms() - t0;
@@ -904,15 +1012,10 @@ protected:
         job.S_stride = 0;
         job.V0 = V_base_.data();
         job.V_stride = 0;
-        job.edit_ptr = &eptr_;
-        job.edit_ent = &eent_;
-        job.edit_delta = &edel_;
-        job.skip = skip_;
+        topo_.apply(job, handle_disconnected_grid);
     }
     std::vector<std::vector<int>> ctg_;
-    std::vector<int32_t> eptr_, eent_;
-    std::vector<cplx> edel_;
-    std::vector<char> skip_;
+    Topology topo_;
 };
 
 // ScenarioSweepSession (python_bindings.cpp:2424-3207): per-row injections and
@@ -923,15 +1026,7 @@ public:
     void set_topology(const std::vector<std::vector<int>>& ids)
     {
         if (!br_.set) throw std::runtime_error("call set_branch_data() first");
-        eptr_.assign(1, 0);
-        eent_.clear();
-        edel_.clear();
-        skip_.assign(ids.size(), 0);
-        for (size_t r = 0; r < ids.size(); ++r) {
-            skip_[r] = br_.islands(ids[r], n_bus()) ? 1 : 0;
-            if (!skip_[r]) br_.edits(*grid_, ids[r], eent_, edel_);
-            eptr_.push_back(static_cast<int32_t>(eent_.size()));
-        }
+        tp_.build(*grid_, br_, ids);
         topo_ = true;
         trips_ = ids;
         ran_ = false;
@@ -939,34 +1034,24 @@ public:
     void clear_topology()
     {
         topo_ = false;
-        skip_.clear();
+        tp_ = Topology{};
         trips_.clear();
     }
     void clear_v_init() { V0_.clear(); }
     bool has_v_init() const { return !V0_.empty(); }
-    py::array_t<int32_t> get_disconnected() const
-    {
-        std::vector<int32_t> d(skip_.begin(), skip_.end());
-        return to_array(d);
-    }
 
 protected:
     void prepare(RowJob& job) override
     {
         InjectionSweepSession::prepare(job);
         if (topo_) {
-            if (eptr_.size() != static_cast<size_t>(rows_) + 1)
+            if (tp_.eptr.size() != static_cast<size_t>(rows_) + 1)
                 throw py::value_error("set_topology: one entry per scenario");
-            job.edit_ptr = &eptr_;
-            job.edit_ent = &eent_;
-            job.edit_delta = &edel_;
-            job.skip = skip_;
+            tp_.apply(job, handle_disconnected_grid);
         }
     }
     bool topo_ = false;
-    std::vector<int32_t> eptr_, eent_;
-    std::vector<cplx> edel_;
-    std::vector<char> skip_;
+    Topology tp_;
 };
 
 }  // namespace
@@ -1067,6 +1152,7 @@ PYBIND11_MODULE(_gpusim2grid, m)
         .def("get_v", &AcPfNrSession::get_v)
         .def("get_F", &AcPfNrSession::get_F)
         .def("get_J", &AcPfNrSession::get_J)
+        .def("solve_JT", &AcPfNrSession::solve_JT, py::arg("rhs"))
         .def_property_readonly("n_pvpq", &AcPfNrSession::n_pvpq)
         .def_property_readonly("n_pq", &AcPfNrSession::n_pq)
         .def_property_readonly("dim_J", &AcPfNrSession::dim_J)
@@ -1121,7 +1207,9 @@ PYBIND11_MODULE(_gpusim2grid, m)
             .def("compute_flows", &S::compute_flows, py::call_guard<py::gil_scoped_release>())
             .def("get_or_amps", &S::get_or_amps)
             .def("get_ex_amps", &S::get_ex_amps)
-            .def("clear_warm_start", &S::clear_warm_start);
+            .def("clear_warm_start", &S::clear_warm_start)
+            .def_readwrite("handle_disconnected_grid", &S::handle_disconnected_grid)
+            .def("get_disconnected", &S::get_disconnected);
     };
     auto batch_init = [](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
@@ -1170,7 +1258,6 @@ PYBIND11_MODULE(_gpusim2grid, m)
         .def("set_v_init_from", &ScenarioSweepSession::set_v_init_from, py::arg("V"))
         .def("clear_v_init", &ScenarioSweepSession::clear_v_init)
         .def_property_readonly("has_v_init", &ScenarioSweepSession::has_v_init)
-        .def("get_disconnected", &ScenarioSweepSession::get_disconnected)
         .def_property_readonly("n_scenarios", &ScenarioSweepSession::n_scenarios);
     bind_batch(ss);
 #undef G2G_BATCH_ARGS

@@ -42,6 +42,7 @@ __global__ void k_rows_to_slots(const T* src, long long ld, int R, int m, const 
 template <typename T>
 __global__ void k_slots_to_rows(const T* src, int R, int m, const int* slots, T* dst, long long ld);
 __global__ void k_set_slots(const int* slots, int R, int* v, int value);
+__global__ void k_mask_v_nan(const unsigned char* mask, long long total, double2* V);
 template <typename Real>
 __global__ void k_gather_astream(const double* vals, long long ldv, const double* rhs, long long ldb, int rows,
                                  int nnz, int d, const int* a_pos, const int* f_pos, int a_len, Real* A);
@@ -82,6 +83,7 @@ struct g2g_plan_s {
     int* d_fpos = nullptr;
     int* d_upd = nullptr;     // per bus {bus, theta unknown, |V| unknown}
     int n_upd = 0;
+    int* d_k1_ent_t = nullptr;  // k1_ent with the A positions of the transposed entries (adjoint)
     std::vector<int> raw_p, raw_i;  // a raw plan's matrix pattern (CSR)
     std::vector<int> raw_order;     // and its pivot order (empty: minimum degree)
     // the one-walker variant, built when a precision's walker kernels do not
@@ -96,6 +98,33 @@ struct g2g_plan_s {
     {
         dfree(d_prog_f); dfree(d_prog_b); dfree(d_seg); dfree(d_k1_bus); dfree(d_k1_ent);
         dfree(d_Y); dfree(d_perm); dfree(d_apos); dfree(d_fpos); dfree(d_upd); dfree(d_Jp); dfree(d_Ji);
+        dfree(d_k1_ent_t);
+    }
+    // the mismatch program writing J transposed: entry (r, c) of J goes to the
+    // A position of (c, r) (the pattern is structurally symmetric)
+    void upload_transposed()
+    {
+        if (d_k1_ent_t) return;
+        const NrPlan& P = nr;
+        std::vector<int> tpos(static_cast<size_t>(P.nJ), -1);
+        for (int r = 0; r < P.d; ++r)
+            for (int e = P.Jp[r]; e < P.Jp[r + 1]; ++e) {
+                const int c = P.Ji[e];
+                const auto b0 = P.Ji.begin() + P.Jp[c], b1 = P.Ji.begin() + P.Jp[c + 1];
+                const auto it = std::lower_bound(b0, b1, r);
+                if (it == b1 || *it != r) throw std::runtime_error("the Jacobian pattern is not structurally symmetric");
+                tpos[e] = nr.lu.a_pos_of_entry[static_cast<size_t>(it - P.Ji.begin())];
+            }
+        // a k1 entry's a11..a22 are a_pos_of_entry[map]: replace them by the
+        // transposed entries' positions
+        std::vector<int> ent = P.k1_ent;
+        std::vector<int> inv(static_cast<size_t>(nr.lu.a_len + 1), -1);
+        for (int e = 0; e < P.nJ; ++e)
+            if (nr.lu.a_pos_of_entry[e] >= 0) inv[nr.lu.a_pos_of_entry[e]] = e;
+        for (size_t q = 0; q < ent.size(); q += 6)
+            for (int b = 2; b < 6; ++b)
+                if (ent[q + b] >= 0) ent[q + b] = tpos[inv[ent[q + b]]];
+        d_k1_ent_t = dupload(ent);
     }
     // device copies of the tables, made on first use (plans can be built and
     // inspected on a host without a GPU)
@@ -197,6 +226,10 @@ struct g2g_batch_s {
     int* it0 = nullptr;
     int clock = 0;
     bool streaming = false;
+    // islanded buses per row [T][n][64] (nullable), and its compaction / re-queue copies
+    unsigned char* mask = nullptr;
+    unsigned char* maskk = nullptr;
+    unsigned char* maskc = nullptr;
     std::vector<hipEvent_t> ev;
     ~g2g_batch_s()
     {
@@ -205,6 +238,7 @@ struct g2g_batch_s {
         dfree(V0); dfree(Vc); dfree(Sc); dfree(Yc); dfree(st_save); dfree(it_save); dfree(res_save);
         dfree(row_map); dfree(Bs); dfree(Xs);
         dfree(Vk); dfree(Sk); dfree(Yk); dfree(stk); dfree(itk); dfree(resk); dfree(mapk); dfree(it0);
+        dfree(mask); dfree(maskk); dfree(maskc);
         if (h_active) (void)hipHostFree(h_active);
         for (auto e : ev) (void)hipEventDestroy(e);
     }
@@ -349,13 +383,13 @@ void launch_update(g2g_batch_s* b)
 
 template <typename Real>
 void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double tol, int freeze,
-                     double growth = 0.0)
+                     double growth = 0.0, bool transposed = false)
 {
     const NrPlan& P = b->plan->nr;
-    MismatchArgs<Real> a{b->plan->d_k1_bus, b->plan->d_k1_ent, P.k1_nbus, b->plan->d_Y, b->Yrow, b->V, b->S,
+    MismatchArgs<Real> a{b->plan->d_k1_bus, transposed ? b->plan->d_k1_ent_t : b->plan->d_k1_ent, P.k1_nbus, b->plan->d_Y, b->Yrow, b->V, b->S,
                          static_cast<Real*>(b->A), P.n, P.nY, b->a_stride, b->res, b->jmax, b->status, b->iters,
                          b->tile_active, b->n_active, b->active_rows, iter, max_iter, tol, freeze, growth,
-                         b->streaming ? b->it0 : nullptr};
+                         b->streaming ? b->it0 : nullptr, b->mask};
     const dim3 grid(b->T), block(kLanes * kMisWaves);
     if (b->per_row_y) {
         if (with_j) k_mismatch<Real, true, true><<<grid, block, 0, b->stream>>>(a);
@@ -451,6 +485,12 @@ void compact_rows(g2g_batch_s* b)
     if (b->per_row_y)
         k_move_rows<double2><<<blocks_for(static_cast<long long>(R) * nr.nY, 256), 256, 0, s>>>(b->Yrow, b->Yk,
                                                                                                b->mapk, R, nr.nY, 1);
+    if (b->mask) {
+        if (!b->maskk) b->maskk = dalloc<unsigned char>(rp * nr.n);
+        HIPCHK(hipMemsetAsync(b->maskk, 0, rp * nr.n, s));
+        k_move_rows<unsigned char><<<gv, 256, 0, s>>>(b->mask, b->maskk, b->mapk, R, nr.n, 1);
+        std::swap(b->mask, b->maskk);
+    }
     k_fill_int<<<blocks_for(rp, 256), 256, 0, s>>>(b->status, kNotSimulated, rp);
     k_fill_int<<<blocks_for(R, 256), 256, 0, s>>>(b->status, kActive, R);
     const unsigned gr = blocks_for(R, 256);
@@ -478,6 +518,7 @@ void uncompact_rows(g2g_batch_s* b)
     std::swap(b->V, b->Vk);
     std::swap(b->S, b->Sk);
     if (b->per_row_y) std::swap(b->Yrow, b->Yk);
+    if (b->mask) std::swap(b->mask, b->maskk);
     const unsigned gr = blocks_for(R, 256);
     k_move_vals<int><<<gr, 256, 0, s>>>(b->status, b->stk, b->mapk, R, 0, 0);
     k_move_vals<int><<<gr, 256, 0, s>>>(b->iters, b->itk, b->mapk, R, 0, 0);
@@ -592,6 +633,15 @@ void stream_round(g2g_batch_s* b, int k, int max_iter, double tol)
     }
 }
 
+// islanded buses of the solved rows to NaN
+void mask_islanded_v(g2g_batch_s* b)
+{
+    if (!b->mask) return;
+    const long long total = static_cast<long long>(b->rows_pad) * b->plan->nr.n;
+    k_mask_v_nan<<<blocks_for(total, 256), 256, 0, b->stream>>>(b->mask, total, b->V);
+    HIPCHK(hipGetLastError());
+}
+
 void check_stream_error(g2g_batch_s* b)
 {
     int err = 0;
@@ -637,6 +687,7 @@ void solve_impl(g2g_batch_s* b, int max_iter, double tol, int sched, g2g_solve_s
     const int e_begin = tm.mark();
     long long rows_factored = 0;
     const int it = nr_loop<Real>(b, max_iter, tol, freeze, 0.0, tm, rows_factored);
+    mask_islanded_v(b);
     const int e_end = tm.mark();
     check_stream_error(b);
     fill_stats(b, tm, e_begin, e_end, it, rows_factored, stats);
@@ -708,24 +759,29 @@ void solve_warm_impl(g2g_batch_s* b, g2g_mlp_s* mlp, const unsigned char* mask, 
         if (b->per_row_y)
             k_move_rows<double2><<<blocks_for(static_cast<long long>(R) * nr.nY, 256), 256, 0, s>>>(
                 b->Yrow, b->Yc, b->row_map, R, nr.nY, 1);
+        if (b->mask) {
+            if (!b->maskc) b->maskc = dalloc<unsigned char>(rp * nr.n);
+            HIPCHK(hipMemsetAsync(b->maskc, 0, rp * nr.n, s));
+            k_move_rows<unsigned char><<<gv, 256, 0, s>>>(b->mask, b->maskc, b->row_map, R, nr.n, 1);
+        }
         HIPCHK(hipGetLastError());
-        std::swap(b->V, b->Vc);
-        std::swap(b->S, b->Sc);
-        if (b->per_row_y) std::swap(b->Yrow, b->Yc);
+        auto swap_all = [&] {
+            std::swap(b->V, b->Vc);
+            std::swap(b->S, b->Sc);
+            if (b->per_row_y) std::swap(b->Yrow, b->Yc);
+            if (b->mask) std::swap(b->mask, b->maskc);
+        };
+        swap_all();
         b->active_rows = R;
         try {
             reset_rows(b);
             it += nr_loop<Real>(b, max_iter, tol, 1, 0.0, tm, rows_factored);
         } catch (...) {
-            std::swap(b->V, b->Vc);
-            std::swap(b->S, b->Sc);
-            if (b->per_row_y) std::swap(b->Yrow, b->Yc);
+            swap_all();
             b->active_rows = rows;
             throw;
         }
-        std::swap(b->V, b->Vc);
-        std::swap(b->S, b->Sc);
-        if (b->per_row_y) std::swap(b->Yrow, b->Yc);
+        swap_all();
         b->active_rows = rows;
         // results of the restarted rows back to their places
         k_move_rows<double2><<<gv, 256, 0, s>>>(b->Vc, b->V, b->row_map, R, nr.n, 0);
@@ -740,6 +796,7 @@ void solve_warm_impl(g2g_batch_s* b, g2g_mlp_s* mlp, const unsigned char* mask, 
         k_count_tiles<<<b->T, kLanes, 0, s>>>(b->status, b->rows_pad, b->tile_active);
         HIPCHK(hipGetLastError());
     }
+    mask_islanded_v(b);
     const int e_end = tm.mark();
     check_stream_error(b);
     fill_stats(b, tm, e_begin, e_end, it, rows_factored, stats);
@@ -829,6 +886,66 @@ void raw_solve(g2g_plan_s* P, int rows, const double* vals, const double* rhs, d
         throw;
     }
 }
+
+// J^T x = rhs for the rows [0, active_rows) at the batch's V: the mismatch
+// program fills the A stream with J transposed, rhs replaces its right-hand
+// side, and the factorization / backward programs of the (symmetric) pattern
+// solve it.  The solve's statuses and iteration counts are kept.
+template <typename Real>
+void adjoint_impl(g2g_batch_s* b, const double* rhs, double* x)
+{
+    const NrPlan& nr = b->plan->nr;
+    const size_t rp = static_cast<size_t>(b->rows_pad);
+    const int rows = b->active_rows;
+    b->plan->upload_transposed();
+    DeviceScratch tmp;
+    int* saved = tmp.get<int>(rp);
+    int* saved_it = tmp.get<int>(rp);
+    double* saved_res = tmp.get<double>(rp);
+    double* d_rhs = tmp.get<double>(static_cast<size_t>(rows) * nr.d);
+    hipStream_t st = b->stream;
+    HIPCHK(hipMemcpyAsync(saved, b->status, rp * sizeof(int), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(saved_it, b->iters, rp * sizeof(int), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(saved_res, b->res, rp * sizeof(double), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_rhs, rhs, static_cast<size_t>(rows) * nr.d * sizeof(double), hipMemcpyHostToDevice, st));
+    const bool streaming = b->streaming;
+    b->streaming = false;
+    try {
+        reset_rows(b);
+        k_fill_int<<<1, 64, 0, st>>>(b->n_active, 0, 1);
+        launch_mismatch<Real>(b, true, 0, 1 << 30, 1e-300, 0, 0.0, true);
+        k_gather_astream<Real><<<blocks_for(static_cast<long long>(rows) * nr.d, 256), 256, 0, st>>>(
+            nullptr, 0, d_rhs, nr.d, rows, 0, nr.d, b->plan->d_apos, b->plan->d_fpos, b->a_stride,
+            static_cast<Real*>(b->A));
+        HIPCHK(hipGetLastError());
+        // every row of the batch is solved; the clamp scale is |J|'s, which
+        // the transposed fill computed
+        const double eps = b->precision == G2G_MIXED ? 1e-5 : 1e-12;
+        k_count_tiles<<<b->T, kLanes, 0, st>>>(b->status, b->rows_pad, b->tile_active);
+        launch_factor<Real>(b->plan, static_cast<const Real*>(b->A), static_cast<Real*>(b->U), b->a_stride,
+                            b->u_stride, b->tile_active, b->jmax, b->clamped, b->error, rows, b->T, eps, st);
+        launch_backward<Real>(b->plan, static_cast<const Real*>(b->U), b->u_stride, b->tile_active,
+                              static_cast<Real*>(b->X), b->error, rows, b->T, st);
+        std::vector<Real> hx(rp * nr.d);
+        HIPCHK(hipMemcpyAsync(hx.data(), b->X, hx.size() * sizeof(Real), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(b->status, saved, rp * sizeof(int), hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(b->iters, saved_it, rp * sizeof(int), hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(b->res, saved_res, rp * sizeof(double), hipMemcpyDeviceToDevice, st));
+        k_count_tiles<<<b->T, kLanes, 0, st>>>(b->status, b->rows_pad, b->tile_active);
+        HIPCHK(hipStreamSynchronize(st));
+        b->streaming = streaming;
+        check_stream_error(b);
+        for (int r = 0; r < rows; ++r)
+            for (int e = 0; e < nr.d; ++e)
+                x[static_cast<size_t>(r) * nr.d + e] =
+                    hx[((static_cast<size_t>(r / kLanes) * nr.d) + e) * kLanes + (r % kLanes)];
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        b->streaming = streaming;
+        throw;
+    }
+}
+
 
 }  // namespace
 
@@ -1263,6 +1380,15 @@ int g2g_batch_eval(g2g_batch_t b, double* F, double* J, double* residual)
     });
 }
 
+int g2g_batch_solve_adjoint(g2g_batch_t b, const double* rhs, double* x)
+{
+    return guarded([&] {
+        if (!b || !rhs || !x) throw std::invalid_argument("null argument");
+        if (b->precision == G2G_MIXED) adjoint_impl<float>(b, rhs, x);
+        else adjoint_impl<double>(b, rhs, x);
+    });
+}
+
 int g2g_lu_solve_batch(g2g_plan_t P, int32_t rows, int32_t precision, const double* vals, const double* rhs,
                        double* x, double pivot_eps, int32_t ir_steps, int32_t* clamped, void* stream)
 {
@@ -1276,10 +1402,41 @@ int g2g_lu_solve_batch(g2g_plan_t P, int32_t rows, int32_t precision, const doub
     });
 }
 
+int g2g_batch_set_bus_mask(g2g_batch_t b, const int32_t* row_ptr, const int32_t* buses, int32_t rows)
+{
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        const NrPlan& nr = b->plan->nr;
+        const size_t rp = static_cast<size_t>(b->rows_pad);
+        if (!row_ptr) {  // masking off
+            dfree(b->mask);
+            b->mask = nullptr;
+            return;
+        }
+        if (b->streaming) throw std::invalid_argument("streamed rows take no bus mask");
+        if (!buses && row_ptr[rows] > row_ptr[0]) throw std::invalid_argument("null buses");
+        check_rows(b, rows, 0);
+        std::vector<unsigned char> h(rp * nr.n, 0);
+        for (int r = 0; r < rows; ++r) {
+            if (row_ptr[r + 1] < row_ptr[r]) throw std::invalid_argument("row_ptr is not monotone");
+            for (int q = row_ptr[r]; q < row_ptr[r + 1]; ++q) {
+                const int i = buses[q];
+                if (i < 0 || i >= nr.n) throw std::invalid_argument("masked bus out of range");
+                if (i == nr.slack) throw std::invalid_argument("the slack bus cannot be masked");
+                h[((static_cast<size_t>(r / kLanes) * nr.n) + i) * kLanes + (r % kLanes)] = 1;
+            }
+        }
+        if (!b->mask) b->mask = dalloc<unsigned char>(rp * nr.n);
+        HIPCHK(hipMemcpyAsync(b->mask, h.data(), h.size(), hipMemcpyHostToDevice, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+    });
+}
+
 int g2g_batch_stream_begin(g2g_batch_t b)
 {
     return guarded([&] {
         if (!b) throw std::invalid_argument("null batch");
+        if (b->mask) throw std::invalid_argument("streamed rows take no bus mask: clear it first");
         const size_t rp = static_cast<size_t>(b->rows_pad);
         if (!b->it0) b->it0 = dalloc<int>(rp);
         k_fill_int<<<blocks_for(rp, 256), 256, 0, b->stream>>>(b->it0, 0, rp);

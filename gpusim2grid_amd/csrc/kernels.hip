@@ -89,6 +89,9 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_mismatch(MismatchArgs<Rea
             const int i = bus[0], e0 = bus[1], e1 = bus[2], fp = bus[3], fq = bus[4], dk = bus[5];
             const double2 vi = V[tidx(tile, a.n, i, lane)];
             const double2 sb = S[tidx(tile, a.n, i, lane)];
+            // an islanded bus (apply_bus_mask_kernel, acpf_nr_kernels.cu:219-244):
+            // identity rows in J, zero mismatch, out of the residual
+            const bool masked = a.mask && a.mask[tidx(tile, a.n, i, lane)];
             double2 I = make_double2(0.0, 0.0);
             double dr = 0.0, di = 0.0;  // diagonal admittance
             int dent = -1;
@@ -126,7 +129,8 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_mismatch(MismatchArgs<Rea
                         const double2 w = cmulc(vi, t);  // V_i conj(Y_ij V_j)
                         const double inv = 1.0 / sqrt(fma(vj[q].x, vj[q].x, vj[q].y * vj[q].y));
                         const int a11 = en[2], a12 = en[3], a21 = en[4], a22 = en[5];
-                        const double j11 = w.y, j21 = -w.x, j12 = w.x * inv, j22 = w.y * inv;
+                        double j11 = w.y, j21 = -w.x, j12 = w.x * inv, j22 = w.y * inv;
+                        if (masked) j11 = j21 = j12 = j22 = 0.0;
                         if (a11 >= 0) a.A[tidx(tile, a.a_stride, a11, lane)] = static_cast<Real>(j11);
                         if (a12 >= 0) a.A[tidx(tile, a.a_stride, a12, lane)] = static_cast<Real>(j12);
                         if (a21 >= 0) a.A[tidx(tile, a.a_stride, a21, lane)] = static_cast<Real>(j21);
@@ -137,11 +141,11 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_mismatch(MismatchArgs<Rea
             }
             (void)dk;
             const double2 sc = cmulc(vi, I);  // V conj(I)
-            const double fpv = -(sc.x - sb.x);
+            const double fpv = masked ? 0.0 : -(sc.x - sb.x);
             rmax = nan_max(rmax, fpv);
             a.A[tidx(tile, a.a_stride, fp, lane)] = static_cast<Real>(fpv);
             if (fq >= 0) {
-                const double fqv = -(sc.y - sb.y);
+                const double fqv = masked ? 0.0 : -(sc.y - sb.y);
                 rmax = nan_max(rmax, fqv);
                 a.A[tidx(tile, a.a_stride, fq, lane)] = static_cast<Real>(fqv);
             }
@@ -152,7 +156,11 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_mismatch(MismatchArgs<Rea
                 const double2 z = cmulc(vi, make_double2(I.x - t.x, I.y - t.y));
                 const double2 u = cmulc(vi, make_double2(I.x + t.x, I.y + t.y));
                 const double inv = 1.0 / sqrt(fma(vi.x, vi.x, vi.y * vi.y));
-                const double j11 = -z.y, j21 = z.x, j12 = u.x * inv, j22 = u.y * inv;
+                double j11 = -z.y, j21 = z.x, j12 = u.x * inv, j22 = u.y * inv;
+                if (masked) {
+                    j11 = j22 = 1.0;
+                    j12 = j21 = 0.0;
+                }
                 const int a11 = en[2], a12 = en[3], a21 = en[4], a22 = en[5];
                 if (a11 >= 0) a.A[tidx(tile, a.a_stride, a11, lane)] = static_cast<Real>(j11);
                 if (a12 >= 0) a.A[tidx(tile, a.a_stride, a12, lane)] = static_cast<Real>(j12);
@@ -1149,6 +1157,13 @@ __global__ void k_slots_to_rows(const T* __restrict__ src, int R, int m, const i
     dst[static_cast<long long>(r) * ld + j] = src[tidx(static_cast<size_t>(b / kLanes), m, j, b % kLanes)];
 }
 
+// islanded buses' voltages to NaN (mask_V_nan_kernel, acpf_nr_kernels.cu:289-307)
+__global__ void k_mask_v_nan(const unsigned char* __restrict__ mask, long long total, double2* __restrict__ V)
+{
+    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid < total && mask[gid]) V[gid] = make_double2(__builtin_nan(""), __builtin_nan(""));
+}
+
 // per-slot scalars: v[slots[r]] = value
 __global__ void k_set_slots(const int* __restrict__ slots, int R, int* __restrict__ v, int value)
 {
@@ -1288,6 +1303,7 @@ template __global__ void k_mismatch<float, true, true>(MismatchArgs<float>);
 template __global__ void k_mismatch<float, false, false>(MismatchArgs<float>);
 template __global__ void k_mismatch<float, true, false>(MismatchArgs<float>);
 template __global__ void k_move_rows<double2>(const double2*, double2*, const int*, int, int, int);
+template __global__ void k_move_rows<unsigned char>(const unsigned char*, unsigned char*, const int*, int, int, int);
 template __global__ void k_rows_to_slots<double2>(const double2*, long long, int, int, const int*, double2*);
 template __global__ void k_slots_to_rows<double2>(const double2*, int, int, const int*, double2*, long long);
 template __global__ void k_move_vals<int>(const int*, int*, const int*, int, int, int);

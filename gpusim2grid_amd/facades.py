@@ -146,6 +146,15 @@ class AcPfGPU:
     def get_F(self):
         return self._s.get_F()
 
+    def solve_JT_dlpack(self, rhs):
+        """Solve J^T x = rhs at the solved V (the adjoint) -> DLPack capsule of
+        a device float64 vector."""
+        import torch
+        from torch.utils import dlpack
+        r = rhs.cpu().numpy() if hasattr(rhs, "cpu") else np.asarray(rhs)
+        x = self._s.solve_JT(np.ascontiguousarray(r, dtype=np.float64).ravel())
+        return dlpack.to_dlpack(torch.from_numpy(x).to("cuda"))
+
     def get_J(self):
         """The Jacobian at the current V as a scipy CSR matrix."""
         import scipy.sparse as sp
@@ -167,7 +176,7 @@ class _BatchFacade:
 
     def __init__(self, grid, *, init_from_n_powerflow=True, precision="fp64", nb_iter=4, max_iter_base=10,
                  tol_base=1e-8, device=None, use_bridge=None, ir_n_steps=0, scaling_max_voltage_change=None,
-                 max_dVa=None, max_dVm=None,
+                 max_dVa=None, max_dVm=None, handle_disconnected_grid=False,
                  scheduling=None, nb_iter_per_round=1, tol=None, linear_solver="own_lu", **_ignored):
         step = _check_common(ir_n_steps, scaling_max_voltage_change, max_dVa, max_dVm, use_bridge)
         if nb_iter_per_round != 1:
@@ -180,12 +189,29 @@ class _BatchFacade:
                                     presolved_v=bool(init_from_n_powerflow), precision=_precision(precision),
                                     **step)
         self.scheduling = scheduling or "chunked"
+        self._s.handle_disconnected_grid = bool(handle_disconnected_grid)
         if tol is not None:
             self.tol = tol
         self._last_residuals = None
         self._warm = None
 
     # settings ---------------------------------------------------------------
+    @property
+    def handle_disconnected_grid(self):
+        """bool: solve a row whose trips split the grid on its largest connected
+        component (the other buses frozen in J and reported as NaN) instead of
+        skipping it; a split that strands the slack is still skipped
+        (``contingency_analysis/__init__.py:496-508``).  Next run."""
+        return self._s.handle_disconnected_grid
+
+    @handle_disconnected_grid.setter
+    def handle_disconnected_grid(self, value):
+        self._s.handle_disconnected_grid = bool(value)
+
+    def get_disconnected(self):
+        """Per row, True when its trips split the grid and it was not simulated."""
+        return self._s.get_disconnected().astype(bool)
+
     @property
     def nb_iter(self):
         return self._s.nb_iter
@@ -354,6 +380,3 @@ class ScenarioSweepGPU(InjectionSweepGPU):
 
     def set_topology(self, branch_ids_per_scenario):
         self._s.set_topology([list(map(int, c)) for c in branch_ids_per_scenario])
-
-    def get_disconnected(self):
-        return self._s.get_disconnected().astype(bool)

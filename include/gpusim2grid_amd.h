@@ -135,6 +135,15 @@ int g2g_batch_set_sbus(g2g_batch_t batch, const double *S, int64_t ld, int32_t r
  * plan's base values first.  Host arrays. */
 int g2g_batch_set_ybus_edits(g2g_batch_t batch, const int32_t *row_ptr, const int32_t *entry, const double *delta,
                              int32_t rows);
+/* Islanded buses (handle_disconnected_grid; replaces build_mask_entries,
+ * apply_bus_mask_kernel and mask_V_nan_kernel, contingency_analysis_helper.hpp:
+ * 492-505 and acpf_nr_kernels.cu:219-307): row r masks buses[q] for q in
+ * [row_ptr[r], row_ptr[r+1]).  A masked bus's P/Q equations become identity
+ * rows of J with zero mismatch (its voltage stays where it started and it is
+ * left out of the residual) and its result voltage is NaN.  The slack cannot
+ * be masked.  Host arrays; row_ptr = NULL clears the mask.  Not with streamed
+ * rows. */
+int g2g_batch_set_bus_mask(g2g_batch_t batch, const int32_t *row_ptr, const int32_t *buses, int32_t rows);
 /* Rows [0, active_rows) take part in the next solve; the others are left
  * untouched and reported NotSimulated. */
 int g2g_batch_set_active_rows(g2g_batch_t batch, int32_t active_rows);
@@ -190,6 +199,14 @@ int g2g_batch_get_results(g2g_batch_t batch, int32_t *status, int32_t *iters, do
  * J != NULL, the Jacobian values in the reference CSR order (rows x nJ), on
  * the device kernels; residual (nullable) gets ||F||inf.  Host arrays. */
 int g2g_batch_eval(g2g_batch_t batch, double *F, double *J, double *residual);
+
+/* Batched adjoint: J^T x = rhs for every active row, J at the batch's V
+ * (BatchAdjoint / solve_JT_batch, contingency/batch_pf_driver.cuh:177-217,807-830;
+ * AcPfNrSession.solve_JT_dlpack, python_bindings.cpp:1525).  rhs and x are
+ * host arrays rows x d in the reference's J row / column layout.  The J^T
+ * values fill the same factorization programs (the pattern is structurally
+ * symmetric); the solve's statuses are kept. */
+int g2g_batch_solve_adjoint(g2g_batch_t batch, const double *rhs, double *x);
 
 /* Raw batched LU solve of `rows` matrices on a g2g_lu_plan_create pattern:
  * vals rows x nnz (CSR order), rhs rows x d, x rows x d (host arrays);

@@ -156,6 +156,42 @@ def nr_batch(Ybus, V0, Sbus, pv, pq, max_iter=10, tol=1e-8, n_threads=1, Yx_rows
     return Vf.view(np.complex128).reshape(rows, n), status, iters, res
 
 
+def nr_masked(Ybus, V0, Sbus, pv, pq, masks, max_iter=10, tol=1e-8, Yx_rows=None, n_threads=1):
+    """NR of rows whose ``masks[r]`` buses are islanded (handle_disconnected_grid,
+    ``src/_cpp/contingency_analysis_helper.cpp:657-692`` and
+    ``src/_cpp/acpf_nr_kernels.cu:219-307``): a masked bus's equations are
+    identity rows with zero mismatch, so its voltage never moves, it is left
+    out of the residual, and the row's other buses solve as the grid restricted
+    to them (the trips that island it leave no coupling).  Each row is solved
+    as that restricted grid with ``nr_batch``; masked voltages come back NaN.
+    Rows with an empty mask solve on the full grid.  Same returns as nr_batch.
+    """
+    Y = sp.csr_matrix(Ybus)
+    Y.sort_indices()
+    n = Y.shape[0]
+    S = np.atleast_2d(np.asarray(Sbus, dtype=np.complex128))
+    rows = len(masks)
+    V0 = np.broadcast_to(np.atleast_2d(np.asarray(V0, dtype=np.complex128)), (rows, n))
+    S = np.broadcast_to(S, (rows, n))
+    Vout = np.empty((rows, n), np.complex128)
+    st = np.empty(rows, np.int32)
+    it = np.empty(rows, np.int32)
+    res = np.empty(rows)
+    for r in range(rows):
+        Yr = Y if Yx_rows is None else sp.csr_matrix((np.asarray(Yx_rows[r]), Y.indices, Y.indptr), shape=Y.shape)
+        live = np.setdiff1d(np.arange(n), np.asarray(masks[r], dtype=np.int64))
+        new = np.full(n, -1, np.int64)
+        new[live] = np.arange(live.size)
+        sub = sp.csr_matrix(Yr[live][:, live])
+        pvs, pqs = new[np.asarray(pv)], new[np.asarray(pq)]
+        v, s_, i_, q_ = nr_batch(sub, V0[r, live], S[r, live], pvs[pvs >= 0], pqs[pqs >= 0], max_iter, tol,
+                                 n_threads=n_threads)
+        Vout[r] = np.nan
+        Vout[r, live] = v[0]
+        st[r], it[r], res[r] = s_[0], i_[0], q_[0]
+    return Vout, st, it, res
+
+
 # ---------------------------------------------------------------------------
 # numpy / scipy restatements (MATPOWER form) used to pin the C oracle
 # ---------------------------------------------------------------------------
