@@ -10,13 +10,18 @@ __version__ = "0.1.0"
 
 _FACADES = ("AcPfGPU", "InjectionSweepGPU", "ContingencyAnalysisGPU", "ScenarioSweepGPU", "RowStatus")
 
-__all__ = list(_FACADES) + ["warmup"]
+_VIOLATIONS = ("ViolationElementType", "LimitViolationType", "LimitViolation")
+
+__all__ = list(_FACADES) + list(_VIOLATIONS) + ["warmup"]
 
 
 def __getattr__(name):
     if name in _FACADES:
         from . import facades
         return getattr(facades, name)
+    if name in _VIOLATIONS:
+        from . import violations
+        return getattr(violations, name)
     if name == "warmup":
         from ._gpusim2grid import warmup
         return warmup

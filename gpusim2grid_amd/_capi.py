@@ -32,6 +32,24 @@ class PlanInfo(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class LimitTable(ctypes.Structure):
+    """g2g_limit_table"""
+    _fields_ = [("m", ctypes.c_int32), ("from_", ctypes.c_void_p), ("to", ctypes.c_void_p),
+                ("y4", ctypes.c_void_p), ("base_or", ctypes.c_void_p), ("base_ex", ctypes.c_void_p),
+                ("limit1_ka", ctypes.c_void_p), ("limit2_ka", ctypes.c_void_p), ("n_lines", ctypes.c_int32),
+                ("vn_kv", ctypes.c_void_p), ("vmin_kv", ctypes.c_void_p), ("vmax_kv", ctypes.c_void_p)]
+
+
+class Violations(ctypes.Structure):
+    """g2g_violations"""
+    _fields_ = [(k, ctypes.c_void_p) for k in ("count", "truncated", "n_low", "n_high", "n_current",
+                                                "element_type", "element_id", "side", "type", "value",
+                                                "limit")]
+
+
+MAX_VIOLATION_CAPACITY = 1024
+
+
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("n_converged", ctypes.c_int32),
                 ("n_maxiter", ctypes.c_int32), ("n_diverged", ctypes.c_int32),
@@ -88,6 +106,8 @@ def lib():
         "g2g_batch_device_arrays": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
         "g2g_batch_warm_start": (ctypes.c_int, [_P, _P, _P, i32]),
         "g2g_batch_branch_flows": (ctypes.c_int, [_P, i32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+        "g2g_batch_limit_violations": (ctypes.c_int, [_P, ctypes.POINTER(LimitTable), _P, _P, ctypes.c_double,
+                                                      ctypes.c_double, i32, ctypes.POINTER(Violations)]),
         "g2g_batch_solve_warm": (ctypes.c_int, [_P, _P, _P, i32, i32, dbl, i32, dbl, ctypes.POINTER(SolveStats)]),
         "g2g_mlp_create": (ctypes.c_int, [i32, i32, i32, ctypes.c_uint64, _P, ctypes.POINTER(_P)]),
         "g2g_mlp_destroy": (None, [_P]),
@@ -326,6 +346,38 @@ class Batch:
         _check(lib().g2g_batch_branch_flows(self._h, m, _ptr(f), _ptr(t), _ptr(y4), _ptr(bo), _ptr(be), _ptr(tp),
                                             _ptr(tb), _ptr(o), _ptr(e)))
         return o, e
+
+    def limit_violations(self, table, trips=None, tol=1e-6, rel_tol=1e-9, capacity=16, rows=None):
+        """Limit checks of the active rows at the batch's V.  ``table``: dict with
+        from, to, yff, yft, ytf, ytt, base_or, base_ex, limit1_ka, limit2_ka,
+        n_lines, vn_kv, vmin_kv, vmax_kv.  Returns a dict of the C ABI's outputs
+        (records shaped [rows, 3 capacity])."""
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+        keep = dict(fr=_i32(table["from"]), to=_i32(table["to"]),
+                    y4=np.ascontiguousarray(np.stack([table["yff"], table["yft"], table["ytf"], table["ytt"]],
+                                                     axis=1), dtype=np.complex128),
+                    bo=f(table["base_or"]), be=f(table["base_ex"]), l1=f(table["limit1_ka"]),
+                    l2=f(table["limit2_ka"]), vn=f(table["vn_kv"]), vmin=f(table["vmin_kv"]),
+                    vmax=f(table["vmax_kv"]))
+        vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        t = LimitTable(keep["fr"].size, vp(keep["fr"]), vp(keep["to"]), vp(keep["y4"]), vp(keep["bo"]),
+                       vp(keep["be"]), vp(keep["l1"]), vp(keep["l2"]), int(table["n_lines"]), vp(keep["vn"]),
+                       vp(keep["vmin"]), vp(keep["vmax"]))
+        tp = tb = None
+        if trips is not None:
+            tp = _i32(np.concatenate([[0], np.cumsum([len(x) for x in trips])]))
+            tb = _i32(np.concatenate([np.asarray(x, np.int64) for x in trips]) if len(trips) else [])
+        R = self.rows if rows is None else int(rows)
+        K = int(capacity)
+        out = {k: np.zeros(R, np.int32) for k in ("count", "truncated", "n_low", "n_high", "n_current")}
+        for k in ("element_type", "element_id", "side", "type"):
+            out[k] = np.zeros((R, 3 * K), np.int32)
+        for k in ("value", "limit"):
+            out[k] = np.full((R, 3 * K), np.nan)
+        v = Violations(*[vp(out[k]) for k, _ in Violations._fields_])
+        _check(lib().g2g_batch_limit_violations(self._h, ctypes.byref(t), _ptr(tp), _ptr(tb), float(tol),
+                                                float(rel_tol), K, ctypes.byref(v)))
+        return out
 
     def warm_start(self, mlp, mask=None):
         """Writes the MLP's start voltages as the batch's V."""

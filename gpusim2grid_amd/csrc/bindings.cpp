@@ -85,6 +85,7 @@ struct BatchTimings {
     TimingEntry t_warm_start;
     double t_run_ms = 0.0;          // wall time of the last run()
     double t_copy_V_to_host_ms = 0.0;
+    double t_violations_ms = 0.0;   // limit checks (compute_limit_violations)
     int n_chunks = 0;
     int n_rows = 0;
     int n_converged = 0;
@@ -202,6 +203,38 @@ struct StepControl {
     void apply(g2g_batch_t h) const { check(g2g_batch_set_step_control(h, ir_steps, scaling ? 1 : 0, max_dva, max_dvm)); }
 };
 
+// ---- limit violations of the solved rows (compute_limit_violations) ------
+struct ViolJob {
+    g2g_limit_table table{};
+    const std::vector<std::vector<int>>* trips = nullptr;  // tripped branches per row (nullable)
+    double tol = 1e-6;
+    double rel_tol = 1e-9;
+    int K = 16;
+};
+
+// per row: count (-1 not simulated), truncated, true counts per type; records [rows][3K]
+struct ViolResults {
+    int K = 0;
+    std::vector<int32_t> count, truncated, n_low, n_high, n_current, etype, eid, side, type;
+    std::vector<double> value, limit;
+    void reset(int rows, int k)
+    {
+        K = k;
+        const size_t R = static_cast<size_t>(rows), nrec = R * 3 * k;
+        count.assign(R, -1);
+        truncated.assign(R, 0);
+        n_low.assign(R, -1);
+        n_high.assign(R, -1);
+        n_current.assign(R, -1);
+        etype.assign(nrec, 0);
+        eid.assign(nrec, 0);
+        side.assign(nrec, 0);
+        type.assign(nrec, 0);
+        value.assign(nrec, std::nan(""));
+        limit.assign(nrec, std::nan(""));
+    }
+};
+
 // ---- the solve of many rows, chunked by batch_size -----------------------
 struct RowJob {
     const cplx* S = nullptr;       // rows x n, or one shared row (S_stride 0)
@@ -223,13 +256,52 @@ struct RowJob {
     int k_warm = 4;
     double growth = 2.0;
     StepControl step;
+    const ViolJob* viol = nullptr;  // limit checks after each chunk (nullable)
 };
 
 struct RowResults {
     std::vector<cplx> V;
     std::vector<int32_t> status, iters;
     std::vector<double> res;
+    ViolResults viol;
 };
+
+// the limit checks of chunk rows live[c0 .. c0 + m), scattered to their rows
+void check_violations(g2g_batch_t h, const ViolJob& vj, const std::vector<int>& live, size_t c0, int m,
+                      ViolResults& out)
+{
+    const int K = vj.K;
+    const size_t nrec = static_cast<size_t>(m) * 3 * K;
+    std::vector<int32_t> tp(1, 0), tb;
+    for (int k = 0; k < m; ++k) {
+        const int r = live[c0 + k];
+        if (vj.trips && static_cast<size_t>(r) < vj.trips->size())
+            for (int l : (*vj.trips)[r]) tb.push_back(l);
+        tp.push_back(static_cast<int32_t>(tb.size()));
+    }
+    std::vector<int32_t> cnt(m), tr(m), nl(m), nh(m), nc(m), et(nrec), ei(nrec), sd(nrec), ty(nrec);
+    std::vector<double> va(nrec), li(nrec);
+    g2g_violations o{cnt.data(), tr.data(), nl.data(), nh.data(), nc.data(), et.data(), ei.data(), sd.data(),
+                     ty.data(), va.data(), li.data()};
+    check(g2g_batch_limit_violations(h, &vj.table, tp.data(), tb.empty() ? nullptr : tb.data(), vj.tol, vj.rel_tol,
+                                     K, &o));
+    const size_t w = 3 * static_cast<size_t>(K);
+    for (int k = 0; k < m; ++k) {
+        const size_t r = static_cast<size_t>(live[c0 + k]);
+        out.count[r] = cnt[k];
+        out.truncated[r] = tr[k];
+        out.n_low[r] = nl[k];
+        out.n_high[r] = nh[k];
+        out.n_current[r] = nc[k];
+        const size_t a = k * w, b = r * w;
+        std::copy_n(et.begin() + a, w, out.etype.begin() + b);
+        std::copy_n(ei.begin() + a, w, out.eid.begin() + b);
+        std::copy_n(sd.begin() + a, w, out.side.begin() + b);
+        std::copy_n(ty.begin() + a, w, out.type.begin() + b);
+        std::copy_n(va.begin() + a, w, out.value.begin() + b);
+        std::copy_n(li.begin() + a, w, out.limit.begin() + b);
+    }
+}
 
 void run_rows(const Grid& g, std::unique_ptr<Batch>& batch, int precision, int batch_size, int n_rows,
               const RowJob& job, RowResults& out, BatchTimings& tm)
@@ -239,6 +311,8 @@ void run_rows(const Grid& g, std::unique_ptr<Batch>& batch, int precision, int b
     out.status.assign(n_rows, static_cast<int32_t>(RowStatus::NotSimulated));
     out.iters.assign(n_rows, 0);
     out.res.assign(n_rows, std::nan(""));
+    if (job.viol) out.viol.reset(n_rows, job.viol->K);
+    else out.viol = ViolResults{};
     // simulated rows, in order
     std::vector<int> live;
     for (int r = 0; r < n_rows; ++r)
@@ -318,6 +392,11 @@ void run_rows(const Grid& g, std::unique_ptr<Batch>& batch, int precision, int b
             out.status[r] = st[k];
             out.iters[r] = it[k];
             out.res[r] = rs[k];
+        }
+        if (job.viol) {
+            const double tv = now_ms();
+            check_violations(batch->h, *job.viol, live, c0, m, out.viol);
+            tm.t_violations_ms += now_ms() - tv;
         }
     }
     tm.n_rows = n_rows;
@@ -482,10 +561,24 @@ struct Branches {
     std::vector<int32_t> f, t;
     std::vector<cplx> yff, yft, ytf, ytt;
     std::vector<double> base_or, base_ex;  // A per unit current (sn_mva 1e6 / (sqrt(3) vn_kv 1e3))
+    std::vector<double> vn_kv_bus;         // [n] nominal kV
     std::vector<char> bridge;
     bool set = false;
 
     int m() const { return static_cast<int>(f.size()); }
+
+    // yff, yft, ytf, ytt per branch
+    std::vector<cplx> y4() const
+    {
+        std::vector<cplx> y(4 * f.size());
+        for (size_t l = 0; l < f.size(); ++l) {
+            y[4 * l] = yff[l];
+            y[4 * l + 1] = yft[l];
+            y[4 * l + 2] = ytf[l];
+            y[4 * l + 3] = ytt[l];
+        }
+        return y;
+    }
 
     void assign(const IArr& bf, const IArr& bt, const CArr& ff, const CArr& ft, const CArr& tf, const CArr& tt,
                 const DArr& vn_kv, double sn_mva, int n)
@@ -506,6 +599,7 @@ struct Branches {
         base_or.resize(m);
         base_ex.resize(m);
         const double* vn = vn_kv.data();
+        vn_kv_bus.assign(vn, vn + n);
         for (py::ssize_t l = 0; l < m; ++l) {
             base_or[l] = sn_mva * 1e6 / (std::sqrt(3.0) * vn[f[l]] * 1e3);
             base_ex[l] = sn_mva * 1e6 / (std::sqrt(3.0) * vn[t[l]] * 1e3);
@@ -725,6 +819,50 @@ public:
     void clear_warm_start() { mlp_ = 0; }
     bool handle_disconnected_grid = false;
 
+    // compute_limit_violations (contingency_analysis/__init__.py:589-735)
+    bool compute_limit_violations = false;
+    double violation_tol = 1e-6;
+    double get_violation_rel_tol() const { return violation_rel_tol_; }
+    void set_violation_rel_tol(double v)
+    {
+        if (!(v >= 0.0 && v < 1.0)) throw py::value_error("violation_rel_tol must be a real number in [0, 1[");
+        violation_rel_tol_ = v;
+    }
+    int get_violation_capacity() const { return violation_capacity_; }
+    void set_violation_capacity(int k)
+    {
+        if (k < 1 || k > G2G_MAX_VIOLATION_CAPACITY)
+            throw py::value_error("violation_capacity must be in [1, " + std::to_string(G2G_MAX_VIOLATION_CAPACITY) +
+                                  "], got " + std::to_string(k));
+        violation_capacity_ = k;
+    }
+    void set_limits(DArr vmin_kv, DArr vmax_kv, DArr lim1_ka, DArr lim2_ka, int n_lines)
+    {
+        if (!br_.set) throw std::runtime_error("call set_branch_data() first");
+        const int n = n_bus(), m = br_.m();
+        if (vmin_kv.size() != n || vmax_kv.size() != n) throw py::value_error("bus limits must have n_bus entries");
+        if (lim1_ka.size() != m || lim2_ka.size() != m)
+            throw py::value_error("branch limits must have n_branches entries");
+        if (n_lines < 0 || n_lines > m) throw py::value_error("n_lines must be in [0, n_branches]");
+        vmin_.assign(vmin_kv.data(), vmin_kv.data() + n);
+        vmax_.assign(vmax_kv.data(), vmax_kv.data() + n);
+        lim1_.assign(lim1_ka.data(), lim1_ka.data() + m);
+        lim2_.assign(lim2_ka.data(), lim2_ka.data() + m);
+        vn_kv_ = br_.vn_kv_bus;
+        n_lines_ = n_lines;
+        limits_set_ = true;
+    }
+    py::array_t<int32_t> viol_i(const std::vector<int32_t> ViolResults::*f) const
+    {
+        need_violations();
+        return to_array(res_.viol.*f);
+    }
+    py::array_t<double> viol_d(const std::vector<double> ViolResults::*f) const
+    {
+        need_violations();
+        return to_array(res_.viol.*f);
+    }
+
     void run()
     {
         const int rows = n_rows();
@@ -744,6 +882,21 @@ public:
         t.rows_factored = 0;
         t.n_requeued = 0;
         t.t_copy_V_to_host_ms = 0.0;
+        t.t_violations_ms = 0.0;
+        ViolJob vj;
+        std::vector<cplx> y4;
+        if (compute_limit_violations) {
+            if (!limits_set_) throw std::runtime_error("compute_limit_violations: call set_limits() first");
+            y4 = br_.y4();
+            vj.table = g2g_limit_table{br_.m(), br_.f.data(), br_.t.data(), reinterpret_cast<const double*>(y4.data()),
+                                       br_.base_or.data(), br_.base_ex.data(), lim1_.data(), lim2_.data(), n_lines_,
+                                       vn_kv_.data(), vmin_.data(), vmax_.data()};
+            vj.trips = &trips_;
+            vj.tol = violation_tol;
+            vj.rel_tol = violation_rel_tol_;
+            vj.K = violation_capacity_;
+            job.viol = &vj;
+        }
         const double t0 = now_ms();
         run_rows(*grid_, batch_, precision_, batch_size_, rows, job, res_, t);
         t.t_run_ms = now_ms() - t0;
@@ -786,13 +939,7 @@ public:
         const int n = n_bus(), m = br_.m(), rows = n_rows();
         or_.assign(static_cast<size_t>(rows) * m, std::nan(""));
         ex_.assign(static_cast<size_t>(rows) * m, std::nan(""));
-        std::vector<cplx> y4(4 * static_cast<size_t>(m));
-        for (int l = 0; l < m; ++l) {
-            y4[4 * l] = br_.yff[l];
-            y4[4 * l + 1] = br_.yft[l];
-            y4[4 * l + 2] = br_.ytf[l];
-            y4[4 * l + 3] = br_.ytt[l];
-        }
+        const std::vector<cplx> y4 = br_.y4();
         std::vector<int> live;
         for (int r = 0; r < rows; ++r)
             if (res_.status[r] != static_cast<int32_t>(RowStatus::NotSimulated)) live.push_back(r);
@@ -876,6 +1023,18 @@ protected:
     BatchTimings timings_;
     bool ran_ = false;
     std::vector<int32_t> disc_;
+    // limits of compute_limit_violations
+    std::vector<double> vmin_, vmax_, lim1_, lim2_, vn_kv_;
+    int n_lines_ = 0;
+    bool limits_set_ = false;
+    double violation_rel_tol_ = 1e-9;
+    int violation_capacity_ = 16;
+    void need_violations() const
+    {
+        need_run();
+        if (res_.viol.count.empty() && n_rows() > 0)
+            throw std::runtime_error("no limit violations: run() with compute_limit_violations = True");
+    }
 
     // Trips per row: the value edits, and for rows they split, the buses off
     // the largest component (masked under handle_disconnected_grid unless
@@ -1122,6 +1281,7 @@ PYBIND11_MODULE(_gpusim2grid, m)
         .def_readonly("t_warm_start", &BatchTimings::t_warm_start)
         .def_readonly("t_run_ms", &BatchTimings::t_run_ms)
         .def_readonly("t_copy_V_to_host_ms", &BatchTimings::t_copy_V_to_host_ms)
+        .def_readonly("t_violations_ms", &BatchTimings::t_violations_ms)
         .def_readonly("n_chunks", &BatchTimings::n_chunks)
         .def_readonly("n_rows", &BatchTimings::n_rows)
         .def_readonly("n_converged", &BatchTimings::n_converged)
@@ -1209,6 +1369,23 @@ PYBIND11_MODULE(_gpusim2grid, m)
             .def("get_ex_amps", &S::get_ex_amps)
             .def("clear_warm_start", &S::clear_warm_start)
             .def_readwrite("handle_disconnected_grid", &S::handle_disconnected_grid)
+            .def_readwrite("compute_limit_violations", &S::compute_limit_violations)
+            .def_readwrite("violation_tol", &S::violation_tol)
+            .def_property("violation_rel_tol", &S::get_violation_rel_tol, &S::set_violation_rel_tol)
+            .def_property("violation_capacity", &S::get_violation_capacity, &S::set_violation_capacity)
+            .def("set_limits", &S::set_limits, py::arg("bus_vmin_kv"), py::arg("bus_vmax_kv"),
+                 py::arg("branch_limit_a1_ka"), py::arg("branch_limit_a2_ka"), py::arg("n_lines"))
+            .def("get_violation_count", [](const S& s) { return s.viol_i(&ViolResults::count); })
+            .def("get_violation_truncated", [](const S& s) { return s.viol_i(&ViolResults::truncated); })
+            .def("get_violation_count_low_voltage", [](const S& s) { return s.viol_i(&ViolResults::n_low); })
+            .def("get_violation_count_high_voltage", [](const S& s) { return s.viol_i(&ViolResults::n_high); })
+            .def("get_violation_count_current", [](const S& s) { return s.viol_i(&ViolResults::n_current); })
+            .def("get_violation_element_type", [](const S& s) { return s.viol_i(&ViolResults::etype); })
+            .def("get_violation_element_id", [](const S& s) { return s.viol_i(&ViolResults::eid); })
+            .def("get_violation_side", [](const S& s) { return s.viol_i(&ViolResults::side); })
+            .def("get_violation_type", [](const S& s) { return s.viol_i(&ViolResults::type); })
+            .def("get_violation_value", [](const S& s) { return s.viol_d(&ViolResults::value); })
+            .def("get_violation_limit", [](const S& s) { return s.viol_d(&ViolResults::limit); })
             .def("get_disconnected", &S::get_disconnected);
     };
     auto batch_init = [](auto* tag) {

@@ -52,6 +52,8 @@ __global__ void k_count_tiles(const int* status, int rows_pad, int* tile_active)
 __global__ void k_branch_flows(const double2* V, int n, const int* from, const int* to, const double2* y4,
                                const double* base_or, const double* base_ex, int m, int T, double* or_t,
                                double* ex_t);
+__global__ void k_limit_violations(ViolationArgs a);
+size_t limit_violations_lds(int K);
 template <typename T>
 __global__ void k_move_rows(const T* src, T* dst, const int* map, int R, int m, int gather);
 template <typename T>
@@ -1303,6 +1305,114 @@ int g2g_batch_branch_flows(g2g_batch_t b, int32_t m, const int32_t* from, const 
                     or_amps[static_cast<size_t>(r) * m + l] = 0.0;
                     ex_amps[static_cast<size_t>(r) * m + l] = 0.0;
                 }
+    });
+}
+
+int g2g_batch_limit_violations(g2g_batch_t b, const g2g_limit_table* t, const int32_t* trip_ptr,
+                               const int32_t* trip_branch, double tol, double rel_tol, int32_t capacity,
+                               g2g_violations* out)
+{
+    return guarded([&] {
+        if (!b || !t || !out) throw std::invalid_argument("null argument");
+        if (!t->from || !t->to || !t->y4 || !t->base_or || !t->base_ex || !t->limit1_ka || !t->limit2_ka ||
+            !t->vn_kv || !t->vmin_kv || !t->vmax_kv)
+            throw std::invalid_argument("null limit table array");
+        if (!out->count || !out->truncated || !out->n_low || !out->n_high || !out->n_current || !out->element_type ||
+            !out->element_id || !out->side || !out->type || !out->value || !out->limit)
+            throw std::invalid_argument("null output array");
+        if (capacity < 1 || capacity > G2G_MAX_VIOLATION_CAPACITY)
+            throw std::invalid_argument("violation capacity must be in [1, " +
+                                        std::to_string(G2G_MAX_VIOLATION_CAPACITY) + "]");
+        if (!(rel_tol >= 0.0 && rel_tol < 1.0)) throw std::invalid_argument("rel_tol must be in [0, 1[");
+        const int m = t->m, n = b->plan->nr.n;
+        if (m < 0) throw std::invalid_argument("m must be >= 0");
+        if (t->n_lines < 0 || t->n_lines > m) throw std::invalid_argument("n_lines must be in [0, m]");
+        for (int l = 0; l < m; ++l)
+            if (t->from[l] < -1 || t->from[l] >= n || t->to[l] < -1 || t->to[l] >= n)
+                throw std::invalid_argument("branch bus out of range");
+        const int rows = b->active_rows;
+        if (rows <= 0) return;
+        if (trip_ptr) {
+            if (!trip_branch && trip_ptr[rows] > trip_ptr[0]) throw std::invalid_argument("null trip_branch");
+            if (trip_ptr[0] != 0) throw std::invalid_argument("trip_ptr must start at 0");
+            for (int r = 0; r < rows; ++r) {
+                if (trip_ptr[r + 1] < trip_ptr[r]) throw std::invalid_argument("trip_ptr is not monotone");
+                for (int q = trip_ptr[r]; q < trip_ptr[r + 1]; ++q)
+                    if (trip_branch[q] < 0 || trip_branch[q] >= m)
+                        throw std::invalid_argument("tripped branch out of range");
+            }
+        }
+        const int K = capacity;
+        const size_t R = static_cast<size_t>(rows);
+        const size_t nrec = R * 3 * K;
+        const int ntrip = trip_ptr ? trip_ptr[rows] : 0;
+        DeviceScratch s;
+        hipStream_t st = b->stream;
+        auto up = [&](auto* host, size_t cnt) {
+            using T = std::remove_const_t<std::remove_pointer_t<decltype(host)>>;
+            T* d = s.get<T>(std::max<size_t>(cnt, 1));
+            if (cnt) HIPCHK(hipMemcpyAsync(d, host, cnt * sizeof(T), hipMemcpyHostToDevice, st));
+            return d;
+        };
+        ViolationArgs a{};
+        a.V = b->V;
+        a.n = n;
+        a.rows = rows;
+        a.status = b->status;
+        a.res = b->res;
+        a.tol = tol;
+        a.up = 1.0 + rel_tol;
+        a.dn = 1.0 - rel_tol;
+        a.m = m;
+        a.n_lines = t->n_lines;
+        a.from = up(t->from, m);
+        a.to = up(t->to, m);
+        a.y4 = reinterpret_cast<const double2*>(up(t->y4, 8 * static_cast<size_t>(m)));
+        a.base_or = up(t->base_or, m);
+        a.base_ex = up(t->base_ex, m);
+        a.lim1 = up(t->limit1_ka, m);
+        a.lim2 = up(t->limit2_ka, m);
+        a.vn_kv = up(t->vn_kv, n);
+        a.vmin = up(t->vmin_kv, n);
+        a.vmax = up(t->vmax_kv, n);
+        if (trip_ptr) {
+            a.trip_ptr = up(trip_ptr, R + 1);
+            a.trip = up(trip_branch ? trip_branch : trip_ptr, static_cast<size_t>(ntrip));
+        }
+        a.K = K;
+        a.count = s.get<int>(R);
+        a.truncated = s.get<int>(R);
+        a.n_type = s.get<int>(3 * R);
+        a.etype = s.get<int>(nrec);
+        a.eid = s.get<int>(nrec);
+        a.side = s.get<int>(nrec);
+        a.vtype = s.get<int>(nrec);
+        a.value = s.get<double>(nrec);
+        a.limit = s.get<double>(nrec);
+        const size_t lds = limit_violations_lds(K);
+        HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_limit_violations),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        k_limit_violations<<<rows, kLanes, lds, st>>>(a);
+        HIPCHK(hipGetLastError());
+        auto down = [&](auto* host, const auto* dev, size_t cnt) {
+            HIPCHK(hipMemcpyAsync(host, dev, cnt * sizeof(*host), hipMemcpyDeviceToHost, st));
+        };
+        std::vector<int> nt(3 * R);
+        down(out->count, a.count, R);
+        down(out->truncated, a.truncated, R);
+        down(nt.data(), a.n_type, 3 * R);
+        down(out->element_type, a.etype, nrec);
+        down(out->element_id, a.eid, nrec);
+        down(out->side, a.side, nrec);
+        down(out->type, a.vtype, nrec);
+        down(out->value, a.value, nrec);
+        down(out->limit, a.limit, nrec);
+        HIPCHK(hipStreamSynchronize(st));
+        for (size_t r = 0; r < R; ++r) {
+            out->n_low[r] = nt[3 * r];
+            out->n_high[r] = nt[3 * r + 1];
+            out->n_current[r] = nt[3 * r + 2];
+        }
     });
 }
 

@@ -34,6 +34,39 @@ struct MismatchArgs {
                                 // become identity rows: F = 0, J row = unit diagonal)
 };
 
+// limit-violation check of solved rows (one wavefront per row)
+struct ViolationArgs {
+    const double2* V;           // [T][n][64]
+    int n, rows;
+    const int* status;          // [rows_pad]
+    const double* res;          // [rows_pad]
+    double tol;                 // DIVERGENCE above this residual
+    double up, dn;              // 1 + rel_tol, 1 - rel_tol
+    int m, n_lines;
+    const int* from;            // [m] (-1: missing terminal)
+    const int* to;
+    const double2* y4;          // [m][4] yff, yft, ytf, ytt
+    const double* base_or;      // [m] A per unit current
+    const double* base_ex;
+    const double* lim1;         // [m] kA, NaN = none
+    const double* lim2;
+    const double* vn_kv;        // [n]
+    const double* vmin;         // [n] kV, NaN = none
+    const double* vmax;
+    const int* trip_ptr;        // nullable [rows + 1]
+    const int* trip;            // tripped branches per row
+    int K;                      // records kept per type
+    int* count;                 // [rows] records kept (-1: not simulated)
+    int* truncated;             // [rows]
+    int* n_type;                // [rows][3] true counts: low, high, current
+    int* etype;                 // [rows][3K]
+    int* eid;
+    int* side;
+    int* vtype;
+    double* value;
+    double* limit;
+};
+
 template <typename Real>
 struct FactorArgs {
     const Real* A;          // A stream [T][a_stride][64]

@@ -1291,6 +1291,306 @@ __global__ void k_branch_flows(const double2* __restrict__ V, int n, const int* 
 }
 
 // ---------------------------------------------------------------------------
+// k_limit_violations: limit checks of solved rows (check_limit_violations_kernel,
+// contingency/violation_kernels.cuh:80-160 semantics).  One wavefront per row:
+// a row that did not converge gets one DIVERGENCE record; otherwise every
+// branch terminal's current (kA) and every bus voltage (kV) is checked
+// against its limit, the true count of each violation type is kept, and of
+// each type the K most severe records (severity |value/limit - 1|, ties to
+// the element scanned first) are written most severe first, CURRENT then
+// LOW_VOLTAGE then HIGH_VOLTAGE.
+//
+// Scan order: branch l's two sides (keys 2l, 2l + 1), then bus b (key 2m + b).
+// The wave walks the elements 64 at a time: a first pass counts and keeps the
+// first K records of each type; a type with more than K finds the K-th
+// largest severity by an 8-bit-digit radix select over its IEEE bits (eight
+// passes; a severity is >= 0 so its bits order like the values) and a last
+// pass keeps the records above it plus the first ties in scan order.  The
+// kept records are ranked by (severity desc, key asc) in LDS.
+// ---------------------------------------------------------------------------
+
+constexpr int kGrpCurrent = 0, kGrpLow = 1, kGrpHigh = 2;
+// LimitViolationType / ViolationElementType codes (limit_violation_types.hpp:46-52)
+constexpr int kVtLow = 0, kVtHigh = 1, kVtCurrent = 2, kVtDivergence = 4;
+constexpr int kEtBus = 0, kEtLine = 1, kEtTrafo = 2, kEtGrid = 3;
+
+__device__ inline unsigned long long sev_key(double value, double limit)
+{
+    return static_cast<unsigned long long>(__double_as_longlong(fabs(value / limit - 1.0)));
+}
+
+// The records of element e: msk bit 0 = record 0 in CURRENT, bit 1 = record
+// 1 in CURRENT, bit 2 = record 0 in LOW, bit 3 = record 0 in HIGH.  The
+// arithmetic is not contracted, so each pass that re-evaluates an element
+// gets the same severity to the last bit.
+struct ViolRec {
+    int msk;
+    double val[2], lim[2];
+    unsigned long long key[2];
+};
+
+__device__ __forceinline__ ViolRec viol_eval(const ViolationArgs& a, const double2* Vr, const int* trip, int nt,
+                                             int e)
+{
+#pragma clang fp contract(off)
+    ViolRec o;
+    o.msk = 0;
+    if (e < a.m) {
+        for (int q = 0; q < nt; ++q)
+            if (trip[q] == e) return o;  // the pi-model of a tripped branch no longer applies
+        const double l1 = a.lim1[e], l2 = a.lim2[e];
+        if (isnan(l1) && isnan(l2)) return o;
+        const int f = a.from[e], t = a.to[e];
+        const double2 vf = f >= 0 ? Vr[static_cast<size_t>(f) * kLanes] : make_double2(0.0, 0.0);
+        const double2 vt = t >= 0 ? Vr[static_cast<size_t>(t) * kLanes] : make_double2(0.0, 0.0);
+        const double2* y = a.y4 + 4 * static_cast<size_t>(e);
+        const double2 yff = y[0], yft = y[1], ytf = y[2], ytt = y[3];
+        double ior = 0.0, ioi = 0.0, ier = 0.0, iei = 0.0;
+        if (f >= 0) {
+            ior = (yff.x * vf.x - yff.y * vf.y) + (yft.x * vt.x - yft.y * vt.y);
+            ioi = (yff.x * vf.y + yff.y * vf.x) + (yft.x * vt.y + yft.y * vt.x);
+        }
+        if (t >= 0) {
+            ier = (ytf.x * vf.x - ytf.y * vf.y) + (ytt.x * vt.x - ytt.y * vt.y);
+            iei = (ytf.x * vf.y + ytf.y * vf.x) + (ytt.x * vt.y + ytt.y * vt.x);
+        }
+        const double ko = sqrt(ior * ior + ioi * ioi) * a.base_or[e] * 0.001;
+        const double ke = sqrt(ier * ier + iei * iei) * a.base_ex[e] * 0.001;
+        if (!isnan(l1) && ko > l1 * a.up) {
+            o.val[0] = ko;
+            o.lim[0] = l1;
+            o.key[0] = sev_key(ko, l1);
+            o.msk |= 1;
+        }
+        if (!isnan(l2) && ke > l2 * a.up) {
+            o.val[1] = ke;
+            o.lim[1] = l2;
+            o.key[1] = sev_key(ke, l2);
+            o.msk |= 2;
+        }
+        return o;
+    }
+    const int b = e - a.m;
+    const double vmin = a.vmin[b], vmax = a.vmax[b];
+    if (isnan(vmin) && isnan(vmax)) return o;
+    const double2 v = Vr[static_cast<size_t>(b) * kLanes];
+    const double vm = sqrt(v.x * v.x + v.y * v.y) * a.vn_kv[b];  // NaN (islanded bus): no record
+    if (!isnan(vmin) && vm < vmin * a.dn) {
+        o.val[0] = vm;
+        o.lim[0] = vmin;
+        o.key[0] = sev_key(vm, vmin);
+        o.msk = 4;
+    } else if (!isnan(vmax) && vm > vmax * a.up) {
+        o.val[0] = vm;
+        o.lim[0] = vmax;
+        o.key[0] = sev_key(vm, vmax);
+        o.msk = 8;
+    }
+    return o;
+}
+
+// the group of record q of an element with record mask msk (-1: none)
+__device__ inline int viol_group(int msk, int q)
+{
+    if (q == 0) return (msk & 1) ? kGrpCurrent : (msk & 4) ? kGrpLow : (msk & 8) ? kGrpHigh : -1;
+    return (msk & 2) ? kGrpCurrent : -1;
+}
+
+__global__ __launch_bounds__(kLanes) void k_limit_violations(ViolationArgs a)
+{
+    extern __shared__ __align__(16) unsigned char g2g_smem[];
+    const int lane = threadIdx.x;
+    const int r = blockIdx.x;
+    const int K = a.K;
+    unsigned* hist = reinterpret_cast<unsigned*>(g2g_smem);                       // [3][256]
+    unsigned long long* lkey = reinterpret_cast<unsigned long long*>(hist + 3 * 256);  // [3][K]
+    double* lval = reinterpret_cast<double*>(lkey + 3 * K);                        // [3][K]
+    double* llim = lval + 3 * K;                                                   // [3][K]
+    int* lord = reinterpret_cast<int*>(llim + 3 * K);                              // [3][K] scan keys
+    __shared__ unsigned long long s_sel[3];
+    __shared__ int s_need[3];
+
+    const size_t ob = static_cast<size_t>(r) * 3 * K;
+    if (a.status[r] == kNotSimulated) {
+        if (lane == 0) {
+            a.count[r] = -1;
+            a.truncated[r] = 0;
+            for (int g = 0; g < 3; ++g) a.n_type[3 * r + g] = -1;
+        }
+        return;
+    }
+    const double res = a.res[r];
+    if (isnan(res) || res > a.tol) {
+        if (lane == 0) {
+            a.etype[ob] = kEtGrid;
+            a.eid[ob] = -1;
+            a.side[ob] = 0;
+            a.vtype[ob] = kVtDivergence;
+            a.value[ob] = res;
+            a.limit[ob] = a.tol;
+            a.count[r] = 1;
+            a.truncated[r] = 0;
+            for (int g = 0; g < 3; ++g) a.n_type[3 * r + g] = 0;
+        }
+        return;
+    }
+    const double2* Vr = a.V + static_cast<size_t>(r / kLanes) * a.n * kLanes + (r % kLanes);
+    const int* trip = nullptr;
+    int nt = 0;
+    if (a.trip_ptr) {
+        trip = a.trip + a.trip_ptr[r];
+        nt = a.trip_ptr[r + 1] - a.trip_ptr[r];
+    }
+    const int E = a.m + a.n;
+    const unsigned long long lt = __lanemask_lt();
+    auto scan_key = [&](int e, int q) { return e < a.m ? 2 * e + q : 2 * a.m + (e - a.m); };
+    auto put = [&](int g, int slot, unsigned long long k, double v, double l, int order) {
+        lkey[g * K + slot] = k;
+        lval[g * K + slot] = v;
+        llim[g * K + slot] = l;
+        lord[g * K + slot] = order;
+    };
+
+    // pass 1: true counts, and the first K records of each type in scan order
+    int cnt[3] = {0, 0, 0};
+    for (int e0 = 0; e0 < E; e0 += kLanes) {
+        const int e = e0 + lane;
+        ViolRec v;
+        v.msk = 0;
+        if (e < E) v = viol_eval(a, Vr, trip, nt, e);
+        const int g0 = viol_group(v.msk, 0), g1 = viol_group(v.msk, 1);
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+            const unsigned long long b0 = __ballot(g0 == g), b1 = __ballot(g1 == g);
+            if ((b0 | b1) == 0) continue;
+            const int s0 = cnt[g] + __popcll(b0 & lt) + __popcll(b1 & lt);
+            const int s1 = s0 + (g0 == g ? 1 : 0);
+            if (g0 == g && s0 < K) put(g, s0, v.key[0], v.val[0], v.lim[0], scan_key(e, 0));
+            if (g1 == g && s1 < K) put(g, s1, v.key[1], v.val[1], v.lim[1], scan_key(e, 1));
+            cnt[g] += __popcll(b0) + __popcll(b1);
+        }
+    }
+    const bool over0 = cnt[0] > K, over1 = cnt[1] > K, over2 = cnt[2] > K;
+    if (over0 || over1 || over2) {
+        const bool over[3] = {over0, over1, over2};
+        // radix select of the K-th largest severity of each overflowing type
+        unsigned long long prefix[3] = {0, 0, 0}, pmask[3] = {0, 0, 0};
+        int need[3] = {K, K, K};
+        for (int pass = 0; pass < 8; ++pass) {
+            const int shift = 56 - 8 * pass;
+            for (int i = lane; i < 3 * 256; i += kLanes) hist[i] = 0;
+            __syncthreads();
+            for (int e0 = 0; e0 < E; e0 += kLanes) {
+                const int e = e0 + lane;
+                ViolRec v;
+                v.msk = 0;
+                if (e < E) v = viol_eval(a, Vr, trip, nt, e);
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const int g = viol_group(v.msk, q);
+                    if (g >= 0 && over[g] && (v.key[q] & pmask[g]) == prefix[g])
+                        atomicAdd(&hist[g * 256 + static_cast<int>((v.key[q] >> shift) & 255)], 1u);
+                }
+            }
+            __syncthreads();
+            if (lane < 3 && over[lane]) {
+                const int g = lane;
+                unsigned acc = 0;
+                int d = 255;
+                for (; d > 0; --d) {
+                    const unsigned c = hist[g * 256 + d];
+                    if (acc + c >= static_cast<unsigned>(need[g])) break;
+                    acc += c;
+                }
+                s_sel[g] = static_cast<unsigned long long>(d);
+                s_need[g] = need[g] - static_cast<int>(acc);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int g = 0; g < 3; ++g)
+                if (over[g]) {
+                    prefix[g] |= s_sel[g] << shift;
+                    pmask[g] |= 255ull << shift;
+                    need[g] = s_need[g];
+                }
+            __syncthreads();
+        }
+        // collect: above the threshold, plus the first `need` ties in scan order
+        int kept[3] = {0, 0, 0}, ties[3] = {0, 0, 0};
+        for (int e0 = 0; e0 < E; e0 += kLanes) {
+            const int e = e0 + lane;
+            ViolRec v;
+            v.msk = 0;
+            if (e < E) v = viol_eval(a, Vr, trip, nt, e);
+            const int g0 = viol_group(v.msk, 0), g1 = viol_group(v.msk, 1);
+#pragma unroll
+            for (int g = 0; g < 3; ++g) {
+                if (!over[g]) continue;
+                const bool in0 = g0 == g, in1 = g1 == g;
+                const bool eq0 = in0 && v.key[0] == prefix[g], eq1 = in1 && v.key[1] == prefix[g];
+                const unsigned long long t0 = __ballot(eq0), t1 = __ballot(eq1);
+                const int r0 = ties[g] + __popcll(t0 & lt) + __popcll(t1 & lt);
+                const int r1 = r0 + (eq0 ? 1 : 0);
+                const bool k0 = in0 && (v.key[0] > prefix[g] || (eq0 && r0 < need[g]));
+                const bool k1 = in1 && (v.key[1] > prefix[g] || (eq1 && r1 < need[g]));
+                const unsigned long long c0 = __ballot(k0), c1 = __ballot(k1);
+                const int s0 = kept[g] + __popcll(c0 & lt) + __popcll(c1 & lt);
+                const int s1 = s0 + (k0 ? 1 : 0);
+                if (k0) put(g, s0, v.key[0], v.val[0], v.lim[0], scan_key(e, 0));
+                if (k1) put(g, s1, v.key[1], v.val[1], v.lim[1], scan_key(e, 1));
+                kept[g] += __popcll(c0) + __popcll(c1);
+                ties[g] += __popcll(t0) + __popcll(t1);
+            }
+        }
+    }
+    __syncthreads();
+    // rank the kept records of each type and write them out in type order
+    const int k0 = min(cnt[0], K), k1 = min(cnt[1], K), k2 = min(cnt[2], K);
+    const int kk[3] = {k0, k1, k2};
+    const int off[3] = {0, k0, k0 + k1};
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+        const int ng = kk[g];
+        for (int i = lane; i < ng; i += kLanes) {
+            const unsigned long long ki = lkey[g * K + i];
+            const int oi = lord[g * K + i];
+            int rank = 0;
+            for (int j = 0; j < ng; ++j) {
+                const unsigned long long kj = lkey[g * K + j];
+                rank += (kj > ki || (kj == ki && lord[g * K + j] < oi)) ? 1 : 0;
+            }
+            const size_t o = ob + off[g] + rank;
+            if (oi < 2 * a.m) {
+                const int l = oi >> 1;
+                a.etype[o] = l < a.n_lines ? kEtLine : kEtTrafo;
+                a.eid[o] = l < a.n_lines ? l : l - a.n_lines;
+                a.side[o] = (oi & 1) + 1;
+                a.vtype[o] = kVtCurrent;
+            } else {
+                a.etype[o] = kEtBus;
+                a.eid[o] = oi - 2 * a.m;
+                a.side[o] = 0;
+                a.vtype[o] = g == kGrpLow ? kVtLow : kVtHigh;
+            }
+            a.value[o] = lval[g * K + i];
+            a.limit[o] = llim[g * K + i];
+        }
+    }
+    if (lane == 0) {
+        a.count[r] = k0 + k1 + k2;
+        a.truncated[r] = (cnt[0] > K || cnt[1] > K || cnt[2] > K) ? 1 : 0;
+        a.n_type[3 * r + 0] = cnt[kGrpLow];
+        a.n_type[3 * r + 1] = cnt[kGrpHigh];
+        a.n_type[3 * r + 2] = cnt[kGrpCurrent];
+    }
+}
+
+size_t limit_violations_lds(int K)
+{
+    return 3 * 256 * sizeof(unsigned) + 3 * static_cast<size_t>(K) * (8 + 8 + 8 + 4);
+}
+
+// ---------------------------------------------------------------------------
 // explicit instantiations used by the C ABI
 // ---------------------------------------------------------------------------
 

@@ -176,7 +176,7 @@ class _BatchFacade:
 
     def __init__(self, grid, *, init_from_n_powerflow=True, precision="fp64", nb_iter=4, max_iter_base=10,
                  tol_base=1e-8, device=None, use_bridge=None, ir_n_steps=0, scaling_max_voltage_change=None,
-                 max_dVa=None, max_dVm=None, handle_disconnected_grid=False,
+                 max_dVa=None, max_dVm=None, handle_disconnected_grid=False, compute_limit_violations=False,
                  scheduling=None, nb_iter_per_round=1, tol=None, linear_solver="own_lu", **_ignored):
         step = _check_common(ir_n_steps, scaling_max_voltage_change, max_dVa, max_dVm, use_bridge)
         if nb_iter_per_round != 1:
@@ -190,6 +190,7 @@ class _BatchFacade:
                                     **step)
         self.scheduling = scheduling or "chunked"
         self._s.handle_disconnected_grid = bool(handle_disconnected_grid)
+        self._s.compute_limit_violations = bool(compute_limit_violations)
         if tol is not None:
             self.tol = tol
         self._last_residuals = None
@@ -211,6 +212,71 @@ class _BatchFacade:
     def get_disconnected(self):
         """Per row, True when its trips split the grid and it was not simulated."""
         return self._s.get_disconnected().astype(bool)
+
+    # limit violations (contingency_analysis/__init__.py:589-735) ----------------
+    @property
+    def compute_limit_violations(self):
+        """bool: check every solved row against the limits of :meth:`set_limits`
+        right after its chunk's solve.  Next run."""
+        return self._s.compute_limit_violations
+
+    @compute_limit_violations.setter
+    def compute_limit_violations(self, value):
+        self._s.compute_limit_violations = bool(value)
+
+    @property
+    def violation_tol(self):
+        """Residual above which a solved row gets a DIVERGENCE record (1e-6)."""
+        return self._s.violation_tol
+
+    @violation_tol.setter
+    def violation_tol(self, value):
+        self._s.violation_tol = float(value)
+
+    @property
+    def violation_rel_tol(self):
+        """Relative margin a value must clear past its limit (1e-9)."""
+        return self._s.violation_rel_tol
+
+    @violation_rel_tol.setter
+    def violation_rel_tol(self, value):
+        self._s.violation_rel_tol = float(value)
+
+    @property
+    def violation_capacity(self):
+        """Records kept per row and per violation type, the most severe (16)."""
+        return self._s.violation_capacity
+
+    @violation_capacity.setter
+    def violation_capacity(self, value):
+        self._s.violation_capacity = int(value)
+
+    def set_limits(self, bus_vmin_kv, bus_vmax_kv, branch_limit_a1_ka, branch_limit_a2_ka, n_lines):
+        """Per-bus voltage limits (kV) and per-branch current limits (kA, lines
+        then transformers); NaN = none.  ``n_lines`` splits the branch ids into
+        line and transformer ids of the records.  Needs set_branch_data()."""
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+        self._s.set_limits(f(bus_vmin_kv), f(bus_vmax_kv), f(branch_limit_a1_ka), f(branch_limit_a2_ka),
+                           int(n_lines))
+
+    def get_violations(self):
+        """Per row, its list of :class:`LimitViolation`: CURRENT, then LOW_VOLTAGE,
+        then HIGH_VOLTAGE records, each type its most severe first."""
+        from .violations import violations_from_session
+        if not self._s.compute_limit_violations:
+            raise RuntimeError("get_violations() requires compute_limit_violations=True")
+        return violations_from_session(self._s)
+
+    def get_violations_truncated(self):
+        """Per row, True where a type had more violations than violation_capacity."""
+        return self._s.get_violation_truncated().astype(bool)
+
+    def get_violation_counts(self):
+        """The true counts per row ('low_voltage', 'high_voltage', 'current'), -1
+        for a row that was not simulated."""
+        return {"low_voltage": np.asarray(self._s.get_violation_count_low_voltage()),
+                "high_voltage": np.asarray(self._s.get_violation_count_high_voltage()),
+                "current": np.asarray(self._s.get_violation_count_current())}
 
     @property
     def nb_iter(self):

@@ -192,6 +192,50 @@ int g2g_batch_branch_flows(g2g_batch_t batch, int32_t m, const int32_t *from, co
                            const double *base_or, const double *base_ex, const int32_t *trip_ptr,
                            const int32_t *trip_branch, double *or_amps, double *ex_amps);
 
+/* Limit checks of the active rows at the batch's V (compute_limit_violations;
+ * replaces check_limit_violations_kernel, contingency/violation_kernels.cuh:
+ * 80-160, with the record layout of ContingencyAnalysisSession's
+ * get_violation_*, contingency_analysis/__init__.py:616-673).
+ *
+ * A row whose residual is NaN or above tol gets one record {GRID, -1, 0,
+ * DIVERGENCE, residual, tol}.  Otherwise, for each branch l not tripped in the
+ * row: side 1 is a CURRENT violation when |I_or| * base_or * 1e-3 (kA) >
+ * limit1_ka * (1 + rel_tol), side 2 likewise with I_ex / base_ex / limit2_ka;
+ * for each bus b with vm = |V_b| * vn_kv: LOW_VOLTAGE when vm < vmin_kv *
+ * (1 - rel_tol), else HIGH_VOLTAGE when vm > vmax_kv * (1 + rel_tol).  NaN
+ * limits are unset; NaN voltages (islanded buses) never violate.  Each type
+ * keeps its `capacity` most severe records (severity |value / limit - 1|, ties
+ * to the element met first: branch sides, then buses), most severe first, in
+ * the order CURRENT, LOW_VOLTAGE, HIGH_VOLTAGE; element codes BUS 0, LINE 1,
+ * TRAFO 2, GRID 3 (branches l >= n_lines are transformers, numbered
+ * l - n_lines); type codes LOW_VOLTAGE 0, HIGH_VOLTAGE 1, CURRENT 2,
+ * DIVERGENCE 4.  A row reported NotSimulated gets count -1 and type counts -1.
+ * Outputs are host arrays: per row count, truncated (a type had more than
+ * capacity), n_low / n_high / n_current (true counts); records [rows][3 capacity]. */
+#define G2G_MAX_VIOLATION_CAPACITY 1024
+typedef struct g2g_limit_table {
+    int32_t m;               /* branches */
+    const int32_t *from;     /* [m] bus, -1 = missing terminal */
+    const int32_t *to;
+    const double *y4;        /* [m][4] complex yff, yft, ytf, ytt */
+    const double *base_or;   /* [m] A per unit current, origin side */
+    const double *base_ex;   /* [m] extremity side */
+    const double *limit1_ka; /* [m] NaN = none */
+    const double *limit2_ka;
+    int32_t n_lines;         /* branches [0, n_lines) are lines, the rest transformers */
+    const double *vn_kv;     /* [n] */
+    const double *vmin_kv;   /* [n] NaN = none */
+    const double *vmax_kv;
+} g2g_limit_table;
+typedef struct g2g_violations {
+    int32_t *count, *truncated, *n_low, *n_high, *n_current;       /* [rows] */
+    int32_t *element_type, *element_id, *side, *type;             /* [rows][3 capacity] */
+    double *value, *limit;
+} g2g_violations;
+int g2g_batch_limit_violations(g2g_batch_t batch, const g2g_limit_table *table, const int32_t *trip_ptr,
+                               const int32_t *trip_branch, double tol, double rel_tol, int32_t capacity,
+                               g2g_violations *out);
+
 int g2g_batch_get_v(g2g_batch_t batch, double *V, int64_t ld, int32_t on_device);
 int g2g_batch_get_results(g2g_batch_t batch, int32_t *status, int32_t *iters, double *residual);
 

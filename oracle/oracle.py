@@ -192,6 +192,77 @@ def nr_masked(Ybus, V0, Sbus, pv, pq, masks, max_iter=10, tol=1e-8, Yx_rows=None
     return Vout, st, it, res
 
 
+def limit_violations(V, res, status, table, trips=None, tol=1e-6, rel_tol=1e-9, K=16):
+    """Limit checks of solved rows, record by record as the reference's
+    check_limit_violations_kernel defines them
+    (``src/_cpp/contingency/violation_kernels.cuh:80-160``,
+    ``violation_kernels.cu:196-276``) and get_violations orders them
+    (``src/gpusim2grid/contingency_analysis/__init__.py:616-673``).
+
+    V : (rows, n) complex; res, status : (rows,); table : dict as
+    ``_capi.Batch.limit_violations`` takes; trips : per row, tripped branches.
+    Returns per row a dict: count, truncated, n_low, n_high, n_current and
+    ``records``, a list of (element_type, element_id, side, type, value, limit).
+    """
+    f = np.asarray(table["from"])
+    t = np.asarray(table["to"])
+    m = f.size
+    nl = int(table["n_lines"])
+    base_or, base_ex = np.asarray(table["base_or"], float), np.asarray(table["base_ex"], float)
+    l1, l2 = np.asarray(table["limit1_ka"], float), np.asarray(table["limit2_ka"], float)
+    vn, vmin, vmax = (np.asarray(table[k], float) for k in ("vn_kv", "vmin_kv", "vmax_kv"))
+    Y = [np.asarray(table[k], np.complex128) for k in ("yff", "yft", "ytf", "ytt")]
+    up, dn = 1.0 + rel_tol, 1.0 - rel_tol
+    out = []
+    for r in range(V.shape[0]):
+        if status[r] == 3:
+            out.append(dict(count=-1, truncated=0, n_low=-1, n_high=-1, n_current=-1, records=[]))
+            continue
+        if np.isnan(res[r]) or res[r] > tol:
+            out.append(dict(count=1, truncated=0, n_low=0, n_high=0, n_current=0,
+                            records=[(3, -1, 0, 4, float(res[r]), float(tol))]))
+            continue
+        v = V[r]
+        vf = np.where(f >= 0, v[np.maximum(f, 0)], 0)
+        vt = np.where(t >= 0, v[np.maximum(t, 0)], 0)
+        # I = y1 va + y2 vb, in the (re, im) operation order of a complex product
+        cur = lambda y1, va, y2, vb: ((y1.real * va.real - y1.imag * va.imag)  # noqa: E731
+                                      + (y2.real * vb.real - y2.imag * vb.imag),
+                                      (y1.real * va.imag + y1.imag * va.real)
+                                      + (y2.real * vb.imag + y2.imag * vb.real))
+        with np.errstate(invalid="ignore", divide="ignore"):
+            ior, ioi = cur(Y[0], vf, Y[1], vt)
+            ier, iei = cur(Y[2], vf, Y[3], vt)
+            ior, ioi = np.where(f >= 0, ior, 0.0), np.where(f >= 0, ioi, 0.0)
+            ier, iei = np.where(t >= 0, ier, 0.0), np.where(t >= 0, iei, 0.0)
+            ko = np.sqrt(ior * ior + ioi * ioi) * base_or * 0.001
+            ke = np.sqrt(ier * ier + iei * iei) * base_ex * 0.001
+            vm = np.sqrt(v.real * v.real + v.imag * v.imag) * vn
+            tripped = np.zeros(m, bool)
+            if trips is not None:
+                tripped[list(trips[r])] = True
+            groups = {0: [], 1: [], 2: []}  # CURRENT, LOW_VOLTAGE, HIGH_VOLTAGE
+            for side, k, lim in ((1, ko, l1), (2, ke, l2)):
+                hit = ~tripped & ~np.isnan(lim) & (k > lim * up)
+                for l in np.flatnonzero(hit):
+                    et, eid = (1, l) if l < nl else (2, l - nl)
+                    groups[0].append((abs(k[l] / lim[l] - 1.0), 2 * l + side - 1,
+                                      (et, int(eid), side, 2, float(k[l]), float(lim[l]))))
+            low = ~np.isnan(vmin) & (vm < vmin * dn)
+            high = ~low & ~np.isnan(vmax) & (vm > vmax * up)
+            for g, hit, lim, code in ((1, low, vmin, 0), (2, high, vmax, 1)):
+                for b in np.flatnonzero(hit):
+                    groups[g].append((abs(vm[b] / lim[b] - 1.0), 2 * m + b,
+                                      (0, int(b), 0, code, float(vm[b]), float(lim[b]))))
+        recs = []
+        for g in (0, 1, 2):
+            kept = sorted(groups[g], key=lambda x: (-x[0], x[1]))[:K]
+            recs.extend(x[2] for x in kept)
+        out.append(dict(count=len(recs), truncated=int(any(len(groups[g]) > K for g in groups)),
+                        n_low=len(groups[1]), n_high=len(groups[2]), n_current=len(groups[0]), records=recs))
+    return out
+
+
 # ---------------------------------------------------------------------------
 # numpy / scipy restatements (MATPOWER form) used to pin the C oracle
 # ---------------------------------------------------------------------------
