@@ -392,7 +392,7 @@ void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double
                          static_cast<Real*>(b->A), P.n, P.nY, b->a_stride, b->res, b->jmax, b->status, b->iters,
                          b->tile_active, b->n_active, b->active_rows, iter, max_iter, tol, freeze, growth,
                          b->streaming ? b->it0 : nullptr, b->mask};
-    const dim3 grid(b->T), block(kLanes * kMisWaves);
+    const dim3 grid(b->T), block(kLanes * kMismatchWaves);
     if (b->per_row_y) {
         if (with_j) k_mismatch<Real, true, true><<<grid, block, 0, b->stream>>>(a);
         else k_mismatch<Real, true, false><<<grid, block, 0, b->stream>>>(a);

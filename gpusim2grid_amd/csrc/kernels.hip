@@ -64,109 +64,158 @@ __device__ __forceinline__ double nan_max(double m, double v)
 // ---------------------------------------------------------------------------
 
 constexpr int kMisBatch = 4;  // Ybus entries of a bus gathered together
+constexpr int kMisBuses = 1;  // buses a wave works on at once
+
+// The running sums of one bus of k_mismatch.
+struct MisBus {
+    int i, e0, e1, fp, fq;
+    bool ok, masked;
+    double2 vi, sb, I;
+    double dr, di;
+    int dent;
+};
+
+// Adds Ybus entries [eb, eb + kMisBatch) of bus b (y, vj already gathered;
+// j < 0 marks a padding entry): I += Y_ij V_j, and the off-diagonal J values
+// into the A stream.
+template <typename Real, bool WithJ>
+__device__ __forceinline__ void mis_entries(const MismatchArgs<Real>& a, size_t tile, int lane, MisBus& b, int eb,
+                                            const int (&j)[kMisBatch], const double2 (&y)[kMisBatch],
+                                            const double2 (&vj)[kMisBatch], double& jmax)
+{
+#pragma unroll
+    for (int q = 0; q < kMisBatch; ++q) {
+        if (j[q] < 0) continue;
+        if (j[q] == b.i) {
+            b.dr = y[q].x;
+            b.di = y[q].y;
+            b.dent = eb + q;
+            b.I.x = fma(y[q].x, b.vi.x, fma(-y[q].y, b.vi.y, b.I.x));
+            b.I.y = fma(y[q].x, b.vi.y, fma(y[q].y, b.vi.x, b.I.y));
+            continue;
+        }
+        const double2 t = cmul(y[q], vj[q]);
+        b.I.x += t.x;
+        b.I.y += t.y;
+        if (WithJ) {
+            const int* en = a.k1_ent + 6 * (eb + q);
+            const double2 w = cmulc(b.vi, t);  // V_i conj(Y_ij V_j)
+            const double inv = 1.0 / sqrt(fma(vj[q].x, vj[q].x, vj[q].y * vj[q].y));
+            const int a11 = en[2], a12 = en[3], a21 = en[4], a22 = en[5];
+            double j11 = w.y, j21 = -w.x, j12 = w.x * inv, j22 = w.y * inv;
+            if (b.masked) j11 = j21 = j12 = j22 = 0.0;
+            if (a11 >= 0) a.A[tidx(tile, a.a_stride, a11, lane)] = static_cast<Real>(j11);
+            if (a12 >= 0) a.A[tidx(tile, a.a_stride, a12, lane)] = static_cast<Real>(j12);
+            if (a21 >= 0) a.A[tidx(tile, a.a_stride, a21, lane)] = static_cast<Real>(j21);
+            if (a22 >= 0) a.A[tidx(tile, a.a_stride, a22, lane)] = static_cast<Real>(j22);
+            jmax = fmax(jmax, fmax(fmax(fabs(j11), fabs(j12)), fmax(fabs(j21), fabs(j22))));
+        }
+    }
+}
+
+// Gathers entries [eb, eb + kMisBatch) of bus b (clamped to its last entry).
+template <typename Real, bool PerRowY>
+__device__ __forceinline__ void mis_gather(const MismatchArgs<Real>& a, size_t tile, int lane, const MisBus& b,
+                                           int eb, int (&j)[kMisBatch], double2 (&y)[kMisBatch],
+                                           double2 (&vj)[kMisBatch])
+{
+#pragma unroll
+    for (int q = 0; q < kMisBatch; ++q) {
+        const int e = eb + q < b.e1 ? eb + q : b.e1 - 1;
+        const int* en = a.k1_ent + 6 * e;
+        j[q] = eb + q < b.e1 ? en[0] : -1;
+        const int yk = en[1];
+        y[q] = PerRowY ? a.Yrow[tidx(tile, a.nY, yk, lane)] : a.Y[yk];
+        vj[q] = (j[q] >= 0 && j[q] != b.i) ? a.V[tidx(tile, a.n, j[q], lane)] : b.vi;
+    }
+}
 
 template <typename Real, bool PerRowY, bool WithJ>
-__global__ __launch_bounds__(kLanes* kMisWaves) void k_mismatch(MismatchArgs<Real> a)
+__global__ __launch_bounds__(kLanes* kMismatchWaves) void k_mismatch(MismatchArgs<Real> a)
 {
     const int lane = threadIdx.x & (kLanes - 1);
-    const int wave = threadIdx.x / kLanes;
+    // wave-uniform, so the bus and entry tables are read with scalar loads
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kLanes));
     const size_t tile = blockIdx.x;
     const int row = static_cast<int>(tile) * kLanes + lane;
-    __shared__ double s_res[kMisWaves][kLanes];
-    __shared__ double s_jmax[kMisWaves][kLanes];
+    __shared__ double s_res[kMismatchWaves][kLanes];
+    __shared__ double s_jmax[kMismatchWaves][kLanes];
 
     const int st0 = a.status[row];
     const bool active = st0 == kActive;
     double rmax = 0.0, jmax = 0.0;
     if (a.tile_active[tile] > 0) {
-        // the waves take the buses round-robin in the program's (RCM) order,
+        // The waves take the buses round-robin in the program's (RCM) order,
         // so they all work in one neighbourhood of the grid at a time and the
-        // neighbours' V they gather stay in cache
-        const double2* V = a.V;
-        const double2* S = a.S;
-        for (int bb = wave; bb < a.nbus; bb += kMisWaves) {
-            const int* bus = a.k1_bus + 6 * bb;
-            const int i = bus[0], e0 = bus[1], e1 = bus[2], fp = bus[3], fq = bus[4], dk = bus[5];
-            const double2 vi = V[tidx(tile, a.n, i, lane)];
-            const double2 sb = S[tidx(tile, a.n, i, lane)];
-            // an islanded bus (apply_bus_mask_kernel, acpf_nr_kernels.cu:219-244):
-            // identity rows in J, zero mismatch, out of the residual
-            const bool masked = a.mask && a.mask[tidx(tile, a.n, i, lane)];
-            double2 I = make_double2(0.0, 0.0);
-            double dr = 0.0, di = 0.0;  // diagonal admittance
-            int dent = -1;
-            // entries in groups of kMisBatch: every group's admittances and
-            // neighbour voltages are loaded before any of them is used, so a
-            // wave keeps that many gathers in flight
-            for (int eb = e0; eb < e1; eb += kMisBatch) {
-                int j[kMisBatch];
-                double2 y[kMisBatch], vj[kMisBatch];
+        // neighbours' V they gather stay in cache.  A wave works on
+        // kMisBuses buses at once: every load of their first kMisBatch
+        // entries is issued before any is used, which keeps that many
+        // gathers in flight per wave instead of one bus's dependent chain.
+        for (int bb0 = wave; bb0 < a.nbus; bb0 += kMismatchWaves * kMisBuses) {
+            MisBus b[kMisBuses];
 #pragma unroll
-                for (int q = 0; q < kMisBatch; ++q) {
-                    const int e = eb + q < e1 ? eb + q : e1 - 1;
-                    const int* en = a.k1_ent + 6 * e;
-                    j[q] = eb + q < e1 ? en[0] : -1;
-                    const int yk = en[1];
-                    y[q] = PerRowY ? a.Yrow[tidx(tile, a.nY, yk, lane)] : a.Y[yk];
-                    vj[q] = (j[q] >= 0 && j[q] != i) ? V[tidx(tile, a.n, j[q], lane)] : vi;
-                }
+            for (int g = 0; g < kMisBuses; ++g) {
+                const int bb = bb0 + g * kMismatchWaves;
+                b[g].ok = bb < a.nbus;
+                const int* bus = a.k1_bus + 6 * (b[g].ok ? bb : a.nbus - 1);
+                b[g].i = bus[0];
+                b[g].e0 = bus[1];
+                b[g].e1 = bus[2];
+                b[g].fp = bus[3];
+                b[g].fq = bus[4];
+                b[g].vi = a.V[tidx(tile, a.n, b[g].i, lane)];
+                b[g].sb = a.S[tidx(tile, a.n, b[g].i, lane)];
+                // an islanded bus (apply_bus_mask_kernel, acpf_nr_kernels.cu:219-244):
+                // identity rows in J, zero mismatch, out of the residual
+                b[g].masked = a.mask && a.mask[tidx(tile, a.n, b[g].i, lane)];
+                b[g].I = make_double2(0.0, 0.0);
+                b[g].dr = b[g].di = 0.0;
+                b[g].dent = -1;
+            }
+            int j[kMisBuses][kMisBatch];
+            double2 y[kMisBuses][kMisBatch], vj[kMisBuses][kMisBatch];
 #pragma unroll
-                for (int q = 0; q < kMisBatch; ++q) {
-                    if (j[q] < 0) continue;
-                    if (j[q] == i) {
-                        dr = y[q].x;
-                        di = y[q].y;
-                        dent = eb + q;
-                        I.x = fma(y[q].x, vi.x, fma(-y[q].y, vi.y, I.x));
-                        I.y = fma(y[q].x, vi.y, fma(y[q].y, vi.x, I.y));
-                        continue;
-                    }
-                    const double2 t = cmul(y[q], vj[q]);
-                    I.x += t.x;
-                    I.y += t.y;
-                    if (WithJ) {
-                        const int* en = a.k1_ent + 6 * (eb + q);
-                        const double2 w = cmulc(vi, t);  // V_i conj(Y_ij V_j)
-                        const double inv = 1.0 / sqrt(fma(vj[q].x, vj[q].x, vj[q].y * vj[q].y));
-                        const int a11 = en[2], a12 = en[3], a21 = en[4], a22 = en[5];
-                        double j11 = w.y, j21 = -w.x, j12 = w.x * inv, j22 = w.y * inv;
-                        if (masked) j11 = j21 = j12 = j22 = 0.0;
-                        if (a11 >= 0) a.A[tidx(tile, a.a_stride, a11, lane)] = static_cast<Real>(j11);
-                        if (a12 >= 0) a.A[tidx(tile, a.a_stride, a12, lane)] = static_cast<Real>(j12);
-                        if (a21 >= 0) a.A[tidx(tile, a.a_stride, a21, lane)] = static_cast<Real>(j21);
-                        if (a22 >= 0) a.A[tidx(tile, a.a_stride, a22, lane)] = static_cast<Real>(j22);
-                        jmax = fmax(jmax, fmax(fmax(fabs(j11), fabs(j12)), fmax(fabs(j21), fabs(j22))));
-                    }
+            for (int g = 0; g < kMisBuses; ++g) mis_gather<Real, PerRowY>(a, tile, lane, b[g], b[g].e0, j[g], y[g], vj[g]);
+#pragma unroll
+            for (int g = 0; g < kMisBuses; ++g) {
+                if (!b[g].ok) continue;
+                mis_entries<Real, WithJ>(a, tile, lane, b[g], b[g].e0, j[g], y[g], vj[g], jmax);
+                // the rest of a bus with more than kMisBatch entries
+                for (int eb = b[g].e0 + kMisBatch; eb < b[g].e1; eb += kMisBatch) {
+                    int jj[kMisBatch];
+                    double2 yy[kMisBatch], vv[kMisBatch];
+                    mis_gather<Real, PerRowY>(a, tile, lane, b[g], eb, jj, yy, vv);
+                    mis_entries<Real, WithJ>(a, tile, lane, b[g], eb, jj, yy, vv, jmax);
                 }
-            }
-            (void)dk;
-            const double2 sc = cmulc(vi, I);  // V conj(I)
-            const double fpv = masked ? 0.0 : -(sc.x - sb.x);
-            rmax = nan_max(rmax, fpv);
-            a.A[tidx(tile, a.a_stride, fp, lane)] = static_cast<Real>(fpv);
-            if (fq >= 0) {
-                const double fqv = masked ? 0.0 : -(sc.y - sb.y);
-                rmax = nan_max(rmax, fqv);
-                a.A[tidx(tile, a.a_stride, fq, lane)] = static_cast<Real>(fqv);
-            }
-            if (WithJ) {
-                const int* en = a.k1_ent + 6 * dent;
-                const double2 t = make_double2(fma(dr, vi.x, -di * vi.y), fma(dr, vi.y, di * vi.x));
-                // dS/dVa = j V_i conj(I - t); dS/dVm = V_i conj(t + I) / |V_i|
-                const double2 z = cmulc(vi, make_double2(I.x - t.x, I.y - t.y));
-                const double2 u = cmulc(vi, make_double2(I.x + t.x, I.y + t.y));
-                const double inv = 1.0 / sqrt(fma(vi.x, vi.x, vi.y * vi.y));
-                double j11 = -z.y, j21 = z.x, j12 = u.x * inv, j22 = u.y * inv;
-                if (masked) {
-                    j11 = j22 = 1.0;
-                    j12 = j21 = 0.0;
+                const MisBus& c = b[g];
+                const double2 sc = cmulc(c.vi, c.I);  // V conj(I)
+                const double fpv = c.masked ? 0.0 : -(sc.x - c.sb.x);
+                rmax = nan_max(rmax, fpv);
+                a.A[tidx(tile, a.a_stride, c.fp, lane)] = static_cast<Real>(fpv);
+                if (c.fq >= 0) {
+                    const double fqv = c.masked ? 0.0 : -(sc.y - c.sb.y);
+                    rmax = nan_max(rmax, fqv);
+                    a.A[tidx(tile, a.a_stride, c.fq, lane)] = static_cast<Real>(fqv);
                 }
-                const int a11 = en[2], a12 = en[3], a21 = en[4], a22 = en[5];
-                if (a11 >= 0) a.A[tidx(tile, a.a_stride, a11, lane)] = static_cast<Real>(j11);
-                if (a12 >= 0) a.A[tidx(tile, a.a_stride, a12, lane)] = static_cast<Real>(j12);
-                if (a21 >= 0) a.A[tidx(tile, a.a_stride, a21, lane)] = static_cast<Real>(j21);
-                if (a22 >= 0) a.A[tidx(tile, a.a_stride, a22, lane)] = static_cast<Real>(j22);
-                jmax = fmax(jmax, fmax(fmax(fabs(j11), fabs(j12)), fmax(fabs(j21), fabs(j22))));
+                if (WithJ) {
+                    const int* en = a.k1_ent + 6 * c.dent;
+                    const double2 t = make_double2(fma(c.dr, c.vi.x, -c.di * c.vi.y), fma(c.dr, c.vi.y, c.di * c.vi.x));
+                    // dS/dVa = j V_i conj(I - t); dS/dVm = V_i conj(t + I) / |V_i|
+                    const double2 z = cmulc(c.vi, make_double2(c.I.x - t.x, c.I.y - t.y));
+                    const double2 u = cmulc(c.vi, make_double2(c.I.x + t.x, c.I.y + t.y));
+                    const double inv = 1.0 / sqrt(fma(c.vi.x, c.vi.x, c.vi.y * c.vi.y));
+                    double j11 = -z.y, j21 = z.x, j12 = u.x * inv, j22 = u.y * inv;
+                    if (c.masked) {
+                        j11 = j22 = 1.0;
+                        j12 = j21 = 0.0;
+                    }
+                    const int a11 = en[2], a12 = en[3], a21 = en[4], a22 = en[5];
+                    if (a11 >= 0) a.A[tidx(tile, a.a_stride, a11, lane)] = static_cast<Real>(j11);
+                    if (a12 >= 0) a.A[tidx(tile, a.a_stride, a12, lane)] = static_cast<Real>(j12);
+                    if (a21 >= 0) a.A[tidx(tile, a.a_stride, a21, lane)] = static_cast<Real>(j21);
+                    if (a22 >= 0) a.A[tidx(tile, a.a_stride, a22, lane)] = static_cast<Real>(j22);
+                    jmax = fmax(jmax, fmax(fmax(fabs(j11), fabs(j12)), fmax(fabs(j21), fabs(j22))));
+                }
             }
         }
     }
@@ -176,7 +225,7 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_mismatch(MismatchArgs<Rea
     if (wave != 0) return;
     if (a.tile_active[tile] <= 0) return;
     double r = 0.0, jm = 0.0;
-    for (int w = 0; w < kMisWaves; ++w) {
+    for (int w = 0; w < kMismatchWaves; ++w) {
         const double v = s_res[w][lane];
         r = (isnan(r) || isnan(v)) ? __builtin_nan("") : fmax(r, v);
         jm = fmax(jm, s_jmax[w][lane]);
@@ -942,7 +991,8 @@ template <typename Real>
 __global__ __launch_bounds__(kLanes* kMisWaves) void k_update(UpdateArgs<Real> a)
 {
     const int lane = threadIdx.x & (kLanes - 1);
-    const int wave = threadIdx.x / kLanes;
+    // wave-uniform, so the bus tables are read with scalar loads
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kLanes));
     const size_t tile = blockIdx.x;
     if (a.tile_active[tile] <= 0) return;
     const int row = static_cast<int>(tile) * kLanes + lane;
@@ -1024,7 +1074,8 @@ template <typename Real>
 __global__ __launch_bounds__(kLanes* kMisWaves) void k_lin_residual(RefineArgs<Real> a)
 {
     const int lane = threadIdx.x & (kLanes - 1);
-    const int wave = threadIdx.x / kLanes;
+    // wave-uniform, so the CSR is read with scalar loads
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kLanes));
     const size_t tile = blockIdx.x;
     if (a.tile_active[tile] <= 0) return;
     const int i0 = static_cast<int>((static_cast<long long>(a.d) * wave) / kMisWaves);

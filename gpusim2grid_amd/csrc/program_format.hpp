@@ -9,7 +9,8 @@
 namespace g2g {
 
 constexpr int kLanes = 64;          // rows per wavefront (one lane per batch row)
-constexpr int kMisWaves = 16;       // wavefronts per tile in the mismatch and update kernels
+constexpr int kMisWaves = 16;       // wavefronts per tile in the update and refinement kernels
+constexpr int kMismatchWaves = 16;   // wavefronts per tile in the mismatch kernel
 constexpr int kMaxC = 16;           // longest off-diagonal L column the kernels unroll for
 
 constexpr int kFastC = 7;           // column lengths with a specialised kernel path
