@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <numeric>
+#include <memory>
 #include <set>
 #include <unordered_map>
 #include <utility>
@@ -469,53 +470,67 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
     const int W = std::max(1, std::min(walkers, kMaxWalkers));
     const std::vector<int> group = partition_steps(s, parent, step_first, W);
 
-    Emitter e1(s, a_entry), e0(s, a_entry);
-    std::vector<Emitter*> em{&e0, &e1};
-    // first phase: walker 1's segment first, then walker 0's (whose segment
-    // continues with the top part)
-    int a_off0 = 0, u_off0 = 0;
-    if (W > 1) {
+    std::vector<std::unique_ptr<Emitter>> em;
+    for (int w = 0; w < W; ++w) em.push_back(std::make_unique<Emitter>(s, a_entry));
+    Emitter& e0 = *em[0];
+    // first phase: walkers 1 .. W-1 lay out their segments first, each at an
+    // aligned offset, then walker 0's (whose segment continues with the top part)
+    std::vector<int> a_off(static_cast<size_t>(W), 0), u_off(static_cast<size_t>(W), 0);
+    int a_cur = 0, u_cur = 0;
+    for (int w = 1; w < W; ++w) {
+        a_off[w] = a_cur;
+        u_off[w] = u_cur;
+        em[w]->apos = a_cur;
+        em[w]->upos = u_cur;
         for (int k : step_first)
-            if (group[k] == 1) e1.step(k);
-        a_off0 = round_up_to(e1.apos, kSegAlign);
-        u_off0 = round_up_to(e1.upos, kSegAlign);
+            if (group[k] == w) em[w]->step(k);
+        a_cur = round_up_to(em[w]->apos, kSegAlign);
+        u_cur = round_up_to(em[w]->upos, kSegAlign);
     }
-    e0.apos = a_off0;
-    e0.upos = u_off0;
+    a_off[0] = a_cur;
+    u_off[0] = u_cur;
+    e0.apos = a_cur;
+    e0.upos = u_cur;
     for (int k : step_first)
         if (group[k] == 0) e0.step(k);
     const int n_phase1_0 = e0.n_piv;
-    int slot_base1 = 0;
+    // walker w's slots sit after walker 0's and those of walkers 1 .. w-1
+    std::vector<int> slot_base(static_cast<size_t>(W), 0);
     if (W > 1) {
-        // merge walker 1's pending contributions into walker 0's numbering:
-        // an entry both hold is summed by a merge record, an entry only walker
-        // 1 holds keeps its slot
-        slot_base1 = e0.sa.hi;
+        slot_base[1] = e0.sa.hi;
+        for (int w = 2; w < W; ++w) slot_base[w] = slot_base[w - 1] + em[w - 1]->sa.hi;
+        // merge the other walkers' pending contributions into walker 0's
+        // numbering: an entry already held there is summed by a merge record,
+        // an entry held nowhere yet keeps the walker's slot
         std::vector<int> merges;
         std::set<int> adopted;
-        for (const auto& kv : e1.slot_of) {
-            const int g1 = kv.second + slot_base1;
-            auto it = e0.slot_of.find(kv.first);
-            if (it != e0.slot_of.end()) {
-                merges.push_back(it->second);
-                merges.push_back(g1);
-            } else {
-                e0.slot_of[kv.first] = g1;
-                adopted.insert(g1);
+        for (int w = 1; w < W; ++w) {
+            Emitter& ew = *em[w];
+            for (const auto& kv : ew.slot_of) {
+                const int g = kv.second + slot_base[w];
+                auto it = e0.slot_of.find(kv.first);
+                if (it != e0.slot_of.end()) {
+                    merges.push_back(it->second);
+                    merges.push_back(g);
+                } else {
+                    e0.slot_of[kv.first] = g;
+                    adopted.insert(g);
+                }
             }
-        }
-        for (int i = 0; i < d; ++i) {
-            if (e1.bslot[i] < 0) continue;
-            const int g1 = e1.bslot[i] + slot_base1;
-            if (e0.bslot[i] >= 0) {
-                merges.push_back(e0.bslot[i]);
-                merges.push_back(g1);
-            } else {
-                e0.bslot[i] = g1;
-                adopted.insert(g1);
+            for (int i = 0; i < d; ++i) {
+                if (ew.bslot[i] < 0) continue;
+                const int g = ew.bslot[i] + slot_base[w];
+                if (e0.bslot[i] >= 0) {
+                    merges.push_back(e0.bslot[i]);
+                    merges.push_back(g);
+                } else {
+                    e0.bslot[i] = g;
+                    adopted.insert(g);
+                }
             }
+            ew.slot_of.clear();
         }
-        // the merge records (at least one: it carries the wait for walker 1)
+        // the merge records (at least one: it carries the wait for the others)
         const size_t npairs = merges.size() / 2;
         for (size_t q = 0; q == 0 || q < npairs; q += kMergePairs) {
             Rec m;
@@ -525,16 +540,17 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
             e0.recs.push_back(std::move(m));
             if (npairs == 0) break;
         }
-        e1.slot_of.clear();
         // the top part allocates from every slot not holding a live value
-        const int hi1 = e1.sa.hi;
-        e0.sa.hi = slot_base1 + hi1;
-        for (int j = 0; j < hi1; ++j)
-            if (!adopted.count(slot_base1 + j)) e0.sa.put(slot_base1 + j);
+        const int hi = slot_base[W - 1] + em[W - 1]->sa.hi;
+        for (int j = slot_base[1]; j < hi; ++j)
+            if (!adopted.count(j)) e0.sa.put(j);
+        e0.sa.hi = hi;
         for (int k : step_first)
             if (group[k] == W) e0.step(k);
     }
-    if (e0.n_piv + e1.n_piv != d) throw std::logic_error("symbolic: walkers do not cover every pivot");
+    int covered = 0;
+    for (int w = 0; w < W; ++w) covered += em[w]->n_piv;
+    if (covered != d) throw std::logic_error("symbolic: walkers do not cover every pivot");
     if (!e0.slot_of.empty()) throw std::logic_error("symbolic: update targets left unconsumed");
     for (int i = 0; i < d; ++i)
         if (e0.bslot[i] >= 0) throw std::logic_error("symbolic: forward-substitution targets left unconsumed");
@@ -545,21 +561,18 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
     s.walkers = W;
     s.n_top = W > 1 ? e0.n_piv - n_phase1_0 : 0;
     s.seg.assign(static_cast<size_t>(W), SymbolicLU::Seg{});
-    s.seg[0].a_off = a_off0;
-    s.seg[0].a_len = e0.apos - a_off0;
-    s.seg[0].u_off = u_off0;
-    s.seg[0].u_len = e0.upos - u_off0;
-    s.seg[0].n_piv = e0.n_piv;
-    if (W > 1) {
-        s.seg[1].a_len = e1.apos;
-        s.seg[1].u_len = e1.upos;
-        s.seg[1].n_piv = e1.n_piv;
+    for (int w = 0; w < W; ++w) {
+        s.seg[w].a_off = a_off[w];
+        s.seg[w].a_len = em[w]->apos - a_off[w];
+        s.seg[w].u_off = u_off[w];
+        s.seg[w].u_len = em[w]->upos - u_off[w];
+        s.seg[w].n_piv = em[w]->n_piv;
     }
     // encode: refs as byte offsets; a missing source reads the zero slot
     s.prog_f.clear();
     for (int w = 0; w < W; ++w) {
         s.seg[w].prog_f_off = static_cast<int>(s.prog_f.size());
-        em[w]->encode(s.prog_f, w == 1 ? slot_base1 : 0, s.slots_f);
+        em[w]->encode(s.prog_f, slot_base[w], s.slots_f);
         s.prog_f.resize(static_cast<size_t>(round_up_to(static_cast<int>(s.prog_f.size()), kProgChunk)),
                         s.slots_f * kRefUnit);
         s.seg[w].prog_f_len = static_cast<int>(s.prog_f.size()) - s.seg[w].prog_f_off;
