@@ -39,8 +39,8 @@ METRIC = "converged PF/s at ‖ΔS‖∞<1e-8 pu: warm-start+mixed vs plain FP64
 HBM_PEAK_GBS = 8000.0
 # HBM bytes one k_factor launch moves per factored row, measured with rocprofv3 PMC counters
 # (FETCH_SIZE x 2 + WRITE_SIZE, scaled on a 1 GiB copy) on the 16,384-row sweep
-# (profiles/hbm_traffic_c2_16384.md): 22.48 GB (FP64) and 11.24 GB (FP32) per full launch.
-FACTOR_TRAFFIC_PER_ROW = {"fp64": 22.48e9 / 16384, "f32": 11.24e9 / 16384}
+# (profiles/hbm_traffic_c2_16384.md): 22.46 GB (FP64) and 11.24 GB (FP32) per full launch.
+FACTOR_TRAFFIC_PER_ROW = {"fp64": 22.46e9 / 16384, "f32": 11.24e9 / 16384}
 
 
 def parse():
@@ -64,6 +64,9 @@ def parse():
     ap.add_argument("--rate", type=float, default=50000.0, help="c4: offered requests per second")
     ap.add_argument("--requests", type=int, default=100000, help="c4: requests replayed")
     ap.add_argument("--no-warm", action="store_true", help="c4: flat starts instead of the learned warm start")
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="first copy this many GiB device to device (a dispatch of known size that scales "
+                         "rocprofv3 PMC byte counters)")
     ap.add_argument("--quiet", action="store_true")
     args = ap.parse_args()
     if args.workload == "c3":  # the learned-warm-start workload's defaults
@@ -315,6 +318,14 @@ def main():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(2)
     capi.lib()  # fails loudly when the HIP library is missing
+    if args.copy_gib > 0:
+        nb = int(args.copy_gib * (1 << 30))
+        src = torch.empty(nb, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(args, f"copied {nb} bytes device to device")
+        del src, dst
     if args.workload == "c4":
         run_serving(args, torch, capi, syn)
         return

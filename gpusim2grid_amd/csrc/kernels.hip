@@ -322,6 +322,16 @@ __device__ __forceinline__ void load_values(const RD& R, int p, T (&v)[N])
     }
 }
 
+// 1 / pivot: the hardware reciprocal in FP32 (1 ulp; the FP32 factors feed
+// a Newton step the FP64 mismatch corrects), the correctly rounded quotient
+// in FP64
+template <typename Real>
+__device__ __forceinline__ Real pivot_inv(Real d)
+{
+    if constexpr (sizeof(Real) == 4) return __builtin_amdgcn_rcpf(d);
+    else return Real(1) / d;
+}
+
 template <typename Real>
 struct FactorState {
     unsigned char* lb;  // this lane's slot base
@@ -376,7 +386,7 @@ __device__ __forceinline__ int factor_pivot(AR& A, ProgReader& P, FactorState<Re
         dv = dv < Real(0) ? -st.eps : st.eps;
         ++st.clamped;
     }
-    const Real inv = Real(1) / dv;
+    const Real inv = pivot_inv(dv);
 #pragma unroll
     for (int q = 0; q < C; ++q) lv[q] *= -inv;  // negated multipliers
     Real* u = st.U + static_cast<size_t>(st.up) * kLanes;
@@ -460,7 +470,7 @@ __device__ __forceinline__ int factor_pivot2(AR& A, ProgReader& P, FactorState<R
         d1 = d1 < Real(0) ? -st.eps : st.eps;
         ++st.clamped;
     }
-    const Real inv1 = Real(1) / d1;
+    const Real inv1 = pivot_inv(d1);
     const Real l = e * inv1;
     Real mk[C > 0 ? C : 1], uk[C > 0 ? C : 1], mk1[C > 0 ? C : 1], uk1[C > 0 ? C : 1];
 #pragma unroll
@@ -480,7 +490,7 @@ __device__ __forceinline__ int factor_pivot2(AR& A, ProgReader& P, FactorState<R
         d2 = d2 < Real(0) ? -st.eps : st.eps;
         ++st.clamped;
     }
-    const Real inv2 = Real(1) / d2;
+    const Real inv2 = pivot_inv(d2);
 #pragma unroll
     for (int i = 0; i < C; ++i) mk1[i] = fma(-mk[i], b, x[K1 + 2 + i]) * inv2;
 #pragma unroll
@@ -556,7 +566,7 @@ __device__ int factor_pivot_any(int c, AR& A, ProgReader& P, FactorState<Real>& 
         dv = dv < Real(0) ? -st.eps : st.eps;
         ++st.clamped;
     }
-    const Real inv = Real(1) / dv;
+    const Real inv = pivot_inv(dv);
     for (int q = 0; q < c; ++q) lsc[q * kLanes] *= -inv;
     Real* u = st.U + static_cast<size_t>(st.up) * kLanes;
     u[0] = fk;
