@@ -103,6 +103,7 @@ def lib():
         "g2g_batch_stream_fill": (ctypes.c_int, [_P, i32, _P, _P, i64, _P, i64, i32, _P, _P, _P]),
         "g2g_batch_stream_round": (ctypes.c_int, [_P, i32, i32, dbl]),
         "g2g_batch_stream_take": (ctypes.c_int, [_P, i32, _P, _P, i64, i32]),
+        "g2g_batch_stream_warm": (ctypes.c_int, [_P, _P, i32, _P, _P, i32]),
         "g2g_batch_device_arrays": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
         "g2g_batch_warm_start": (ctypes.c_int, [_P, _P, _P, i32]),
         "g2g_batch_branch_flows": (ctypes.c_int, [_P, i32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
@@ -296,13 +297,27 @@ class Batch:
         _check(lib().g2g_batch_stream_fill(self._h, k, _ptr(slots), _ptr(S), S.shape[1], _ptr(V0), ldv, 0,
                                            _ptr(ep), _ptr(en), _ptr(ed)))
 
+    def stream_warm(self, mlp, slots, mask=None):
+        """Start voltages of the given (just filled) slots from the MLP, on the device."""
+        slots = _i32(slots)
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        _check(lib().g2g_batch_stream_warm(self._h, mlp._h, slots.size, _ptr(slots), _ptr(m),
+                                           0 if m is None else m.shape[1]))
+
     def stream_round(self, k=1, max_iter=10, tol=1e-8):
         _check(lib().g2g_batch_stream_round(self._h, int(k), int(max_iter), float(tol)))
 
-    def stream_take(self, slots):
-        """V of the given slots (host [k, n]); the slots become free."""
+    def stream_take(self, slots, out=None):
+        """V of the given slots (host [k, n], or into `out`, a C-contiguous
+        complex128 [>= k, n] array such as a pinned buffer); the slots become free."""
         slots = _i32(slots)
-        V = np.empty((slots.size, self.n), np.complex128)
+        if out is None:
+            V = np.empty((slots.size, self.n), np.complex128)
+        else:
+            if out.dtype != np.complex128 or not out.flags.c_contiguous or out.shape[1] != self.n \
+                    or out.shape[0] < slots.size:
+                raise ValueError("out must be a C-contiguous complex128 [>= k, n] array")
+            V = out[:slots.size]
         _check(lib().g2g_batch_stream_take(self._h, slots.size, _ptr(slots), _ptr(V), self.n, 0))
         return V
 

@@ -177,6 +177,9 @@ struct g2g_plan_s {
     }
 };
 
+// staging buffers of the streamed-row calls
+enum StageBuf { kStSlots, kStS, kStV, kStEdPtr, kStEdEnt, kStEdDelta, kStMask, kStSg, kStVg, kStageBufs };
+
 struct g2g_batch_s {
     g2g_plan_s* plan = nullptr;
     int rows = 0, rows_pad = 0, T = 0, precision = G2G_FP64, per_row_y = 0, active_rows = 0;
@@ -232,6 +235,10 @@ struct g2g_batch_s {
     unsigned char* mask = nullptr;
     unsigned char* maskk = nullptr;
     unsigned char* maskc = nullptr;
+    // grow-only device staging of the streamed-row calls (kept between calls,
+    // so a refill or a take allocates and frees nothing once warm)
+    void* stage[kStageBufs] = {};
+    size_t stage_cap[kStageBufs] = {};
     std::vector<hipEvent_t> ev;
     ~g2g_batch_s()
     {
@@ -241,6 +248,7 @@ struct g2g_batch_s {
         dfree(row_map); dfree(Bs); dfree(Xs);
         dfree(Vk); dfree(Sk); dfree(Yk); dfree(stk); dfree(itk); dfree(resk); dfree(mapk); dfree(it0);
         dfree(mask); dfree(maskk); dfree(maskc);
+        for (void*& q : stage) dfree(q);
         if (h_active) (void)hipHostFree(h_active);
         for (auto e : ev) (void)hipEventDestroy(e);
     }
@@ -1561,6 +1569,30 @@ int g2g_batch_stream_begin(g2g_batch_t b)
     });
 }
 
+}  // extern "C"
+
+namespace {
+// staging buffer i of the batch with room for n elements of T (grown, after
+// the stream drained, when too small)
+template <typename T>
+T* stage_buf(g2g_batch_s* b, int i, size_t n)
+{
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    if (bytes > b->stage_cap[i]) {
+        const size_t cap = std::max(bytes, 2 * b->stage_cap[i]);
+        HIPCHK(hipStreamSynchronize(b->stream));
+        dfree(b->stage[i]);
+        b->stage[i] = nullptr;
+        b->stage_cap[i] = 0;
+        b->stage[i] = dalloc<unsigned char>(cap);
+        b->stage_cap[i] = cap;
+    }
+    return static_cast<T*>(b->stage[i]);
+}
+}  // namespace
+
+extern "C" {
+
 int g2g_batch_stream_fill(g2g_batch_t b, int32_t n, const int32_t* slots, const double* S, int64_t lds,
                           const double* V0, int64_t ldv, int32_t on_device, const int32_t* edit_ptr,
                           const int32_t* edit_ent, const double* edit_delta)
@@ -1574,19 +1606,18 @@ int g2g_batch_stream_fill(g2g_batch_t b, int32_t n, const int32_t* slots, const 
             if (slots[r] < 0 || slots[r] >= b->rows) throw std::invalid_argument("slot outside the batch");
         if (edit_ptr && !b->per_row_y) throw std::invalid_argument("the batch was created without per-row Ybus values");
         hipStream_t st = b->stream;
-        DeviceScratch tmp;
-        int* d_slots = tmp.get<int>(n);
+        int* d_slots = stage_buf<int>(b, kStSlots, n);
         HIPCHK(hipMemcpyAsync(d_slots, slots, n * sizeof(int), hipMemcpyHostToDevice, st));
-        auto rows_src = [&](const double* src, long long ld) -> const double2* {
+        auto rows_src = [&](const double* src, long long ld, int buf) -> const double2* {
             if (on_device) return reinterpret_cast<const double2*>(src);
             const size_t cnt = ld == 0 ? static_cast<size_t>(nr.n) : static_cast<size_t>(n - 1) * ld + nr.n;
-            double2* d = tmp.get<double2>(cnt);
+            double2* d = stage_buf<double2>(b, buf, cnt);
             HIPCHK(hipMemcpyAsync(d, src, cnt * sizeof(double2), hipMemcpyHostToDevice, st));
             return d;
         };
         const unsigned g = blocks_for(static_cast<long long>(n) * nr.n, 256);
-        k_rows_to_slots<double2><<<g, 256, 0, st>>>(rows_src(S, lds), lds, n, nr.n, d_slots, b->S);
-        k_rows_to_slots<double2><<<g, 256, 0, st>>>(rows_src(V0, ldv), ldv, n, nr.n, d_slots, b->V);
+        k_rows_to_slots<double2><<<g, 256, 0, st>>>(rows_src(S, lds, kStS), lds, n, nr.n, d_slots, b->S);
+        k_rows_to_slots<double2><<<g, 256, 0, st>>>(rows_src(V0, ldv, kStV), ldv, n, nr.n, d_slots, b->V);
         if (b->per_row_y) {
             k_ybus_reset_slots<<<blocks_for(static_cast<long long>(n) * nr.nY, 256), 256, 0, st>>>(
                 d_slots, n, b->plan->d_Y, nr.nY, b->Yrow);
@@ -1594,9 +1625,9 @@ int g2g_batch_stream_fill(g2g_batch_t b, int32_t n, const int32_t* slots, const 
                 const int ne = edit_ptr[n];
                 for (int q = 0; q < ne; ++q)
                     if (edit_ent[q] < 0 || edit_ent[q] >= nr.nY) throw std::invalid_argument("edit entry out of range");
-                int* d_rp = tmp.get<int>(n + 1);
-                int* d_en = tmp.get<int>(ne);
-                double2* d_de = tmp.get<double2>(ne);
+                int* d_rp = stage_buf<int>(b, kStEdPtr, n + 1);
+                int* d_en = stage_buf<int>(b, kStEdEnt, ne);
+                double2* d_de = stage_buf<double2>(b, kStEdDelta, ne);
                 HIPCHK(hipMemcpyAsync(d_rp, edit_ptr, (n + 1) * sizeof(int), hipMemcpyHostToDevice, st));
                 HIPCHK(hipMemcpyAsync(d_en, edit_ent, ne * sizeof(int), hipMemcpyHostToDevice, st));
                 HIPCHK(hipMemcpyAsync(d_de, edit_delta, ne * sizeof(double2), hipMemcpyHostToDevice, st));
@@ -1609,7 +1640,7 @@ int g2g_batch_stream_fill(g2g_batch_t b, int32_t n, const int32_t* slots, const 
         k_set_slots<<<gs, 256, 0, st>>>(d_slots, n, b->iters, 0);
         k_count_tiles<<<b->T, kLanes, 0, st>>>(b->status, b->rows_pad, b->tile_active);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(st));  // the staging buffers are freed on return
+        HIPCHK(hipStreamSynchronize(st));  // the caller's host rows may be reused on return
     });
 }
 
@@ -1625,6 +1656,40 @@ int g2g_batch_stream_round(g2g_batch_t b, int32_t k, int32_t max_iter, double to
     });
 }
 
+int g2g_batch_stream_warm(g2g_batch_t b, g2g_mlp_t mlp, int32_t n, const int32_t* slots, const uint8_t* mask,
+                          int32_t n_mask)
+{
+    return guarded([&] {
+        if (!b || !mlp || (n > 0 && !slots)) throw std::invalid_argument("null argument");
+        if (!b->streaming) throw std::invalid_argument("call g2g_batch_stream_begin first");
+        if (n_mask < 0 || (n_mask > 0 && !mask)) throw std::invalid_argument("mask missing");
+        if (n <= 0) return;
+        const int N = b->plan->nr.n;
+        for (int r = 0; r < n; ++r)
+            if (slots[r] < 0 || slots[r] >= b->rows) throw std::invalid_argument("slot outside the batch");
+        hipStream_t st = b->stream;
+        const size_t kp = static_cast<size_t>(n + kLanes - 1) / kLanes * kLanes;
+        int* d_slots = stage_buf<int>(b, kStSlots, n);
+        double2* Sg = stage_buf<double2>(b, kStSg, kp * N);
+        double2* Vg = stage_buf<double2>(b, kStVg, kp * N);
+        HIPCHK(hipMemcpyAsync(d_slots, slots, n * sizeof(int), hipMemcpyHostToDevice, st));
+        unsigned char* d_mask = nullptr;
+        if (n_mask > 0) {
+            d_mask = stage_buf<unsigned char>(b, kStMask, static_cast<size_t>(n) * n_mask);
+            HIPCHK(hipMemcpyAsync(d_mask, mask, static_cast<size_t>(n) * n_mask, hipMemcpyHostToDevice, st));
+        }
+        // the slots' injections into dense tiles, the MLP on them, the start
+        // voltages back into the slots
+        const unsigned g = blocks_for(static_cast<long long>(n) * N, 256);
+        k_move_rows<double2><<<g, 256, 0, st>>>(b->S, Sg, d_slots, n, N, 1);
+        HIPCHK(hipGetLastError());
+        mlp_warm_start_tiles(mlp, Sg, n, N, d_mask, n_mask, Vg, st);
+        k_move_rows<double2><<<g, 256, 0, st>>>(Vg, b->V, d_slots, n, N, 0);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
 int g2g_batch_stream_take(g2g_batch_t b, int32_t n, const int32_t* slots, double* V, int64_t ldv, int32_t on_device)
 {
     return guarded([&] {
@@ -1635,22 +1700,28 @@ int g2g_batch_stream_take(g2g_batch_t b, int32_t n, const int32_t* slots, double
         for (int r = 0; r < n; ++r)
             if (slots[r] < 0 || slots[r] >= b->rows) throw std::invalid_argument("slot outside the batch");
         hipStream_t st = b->stream;
-        DeviceScratch tmp;
-        int* d_slots = tmp.get<int>(n);
+        int* d_slots = stage_buf<int>(b, kStSlots, n);
         HIPCHK(hipMemcpyAsync(d_slots, slots, n * sizeof(int), hipMemcpyHostToDevice, st));
         if (V) {
-            double2* dst = on_device ? reinterpret_cast<double2*>(V) : tmp.get<double2>(static_cast<size_t>(n) * nr.n);
+            double2* dst = on_device ? reinterpret_cast<double2*>(V)
+                                     : stage_buf<double2>(b, kStV, static_cast<size_t>(n) * nr.n);
             const long long ld = on_device ? ldv : nr.n;
             k_slots_to_rows<double2><<<blocks_for(static_cast<long long>(n) * nr.n, 256), 256, 0, st>>>(
                 b->V, n, nr.n, d_slots, dst, ld);
             HIPCHK(hipGetLastError());
             if (!on_device) {
-                std::vector<double2> h(static_cast<size_t>(n) * nr.n);
-                HIPCHK(hipMemcpyAsync(h.data(), dst, h.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
-                HIPCHK(hipStreamSynchronize(st));
-                for (int r = 0; r < n; ++r)
-                    std::memcpy(V + 2 * static_cast<size_t>(r) * ldv, h.data() + static_cast<size_t>(r) * nr.n,
-                                nr.n * sizeof(double2));
+                if (ldv == nr.n) {  // contiguous host rows: one copy (direct DMA when they are pinned)
+                    HIPCHK(hipMemcpyAsync(V, dst, static_cast<size_t>(n) * nr.n * sizeof(double2),
+                                          hipMemcpyDeviceToHost, st));
+                    HIPCHK(hipStreamSynchronize(st));
+                } else {
+                    std::vector<double2> h(static_cast<size_t>(n) * nr.n);
+                    HIPCHK(hipMemcpyAsync(h.data(), dst, h.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
+                    HIPCHK(hipStreamSynchronize(st));
+                    for (int r = 0; r < n; ++r)
+                        std::memcpy(V + 2 * static_cast<size_t>(r) * ldv, h.data() + static_cast<size_t>(r) * nr.n,
+                                    nr.n * sizeof(double2));
+                }
             }
         }
         k_set_slots<<<blocks_for(n, 256), 256, 0, st>>>(d_slots, n, b->status, kNotSimulated);

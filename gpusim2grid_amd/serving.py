@@ -53,7 +53,8 @@ def batch_bytes(plan_info, rows, precision, per_row_ybus):
 class GridState:
     """One topology's resident state: plan, batch of slots, queue."""
 
-    def __init__(self, name, grid, slots, precision, warm=None, max_iter=10, tol=1e-8):
+    def __init__(self, name, grid, slots, precision, warm=None, max_iter=10, tol=1e-8, stage_rows=4096):
+        import torch
         self.name = name
         self.g = grid
         self.plan = _capi.Plan.from_grid(grid.Ybus, grid.pv, grid.pq, grid.slack)
@@ -66,22 +67,25 @@ class GridState:
         self.max_iter = max_iter
         self.tol = tol
         self.flat = grid.flat_start()
+        # pinned host staging of the rows going in and out (direct DMA)
+        self.stage_rows = max(1, min(int(stage_rows), slots))
+        self._pin = [torch.empty((self.stage_rows, grid.n_bus), dtype=torch.complex128, pin_memory=True)
+                     for _ in range(2)]
+        self.S_pin, self.V_pin = (t.numpy() for t in self._pin)
 
     def admit(self):
         """Fills free slots from the queue; returns how many."""
-        k = min(len(self.free), len(self.queue))
+        k = min(len(self.free), len(self.queue), self.stage_rows)
         if k == 0:
             return 0
         reqs = [self.queue.popleft() for _ in range(k)]
         slots = [self.free.pop() for _ in range(k)]
-        S = np.stack([r.S for r in reqs])
+        S = np.stack([r.S for r in reqs], out=self.S_pin[:k])
         edits = syn.outage_edits(self.g, [r.outages for r in reqs])
-        if self.warm is not None:
+        self.batch.stream_fill(slots, S, self.flat[None], edits=edits)
+        if self.warm is not None:  # start voltages predicted on the device, into the slots
             mask = syn.outage_mask(self.g, [r.outages for r in reqs]) if self.warm.n_mask else None
-            V0 = self.warm.predict_v(S, mask=mask)
-        else:
-            V0 = self.flat[None]
-        self.batch.stream_fill(slots, S, V0, edits=edits)
+            self.batch.stream_warm(self.warm.mlp, slots, mask=mask)
         for s, r in zip(slots, reqs):
             self.busy[s] = r
         return k
@@ -92,16 +96,18 @@ class GridState:
     def retire(self, now):
         """Takes the finished slots' results out; returns the finished requests."""
         st, it, _ = self.batch.results()
-        done = [s for s in self.busy if st[s] not in (-1, _capi.NOTSIMULATED)]
-        if not done:
-            return []
-        V = self.batch.stream_take(done)
+        busy = np.fromiter(self.busy.keys(), dtype=np.int64, count=len(self.busy))
+        fin = (st[busy] != -1) & (st[busy] != _capi.NOTSIMULATED)
+        done = busy[fin].tolist()
         out = []
-        for q, s in enumerate(done):
-            r = self.busy.pop(s)
-            r.status, r.iterations, r.V, r.t_done = int(st[s]), int(it[s]), V[q], now
-            self.free.append(s)
-            out.append(r)
+        for c0 in range(0, len(done), self.stage_rows):
+            part = done[c0:c0 + self.stage_rows]
+            V = self.batch.stream_take(part, out=self.V_pin)
+            for q, s in enumerate(part):
+                r = self.busy.pop(s)
+                r.status, r.iterations, r.V, r.t_done = int(st[s]), int(it[s]), V[q].copy(), now
+                self.free.append(s)
+                out.append(r)
         return out
 
 

@@ -465,28 +465,42 @@ def _cut_labels(grid: Grid) -> np.ndarray:
     return label
 
 
+def _branch_positions(grid: Grid) -> np.ndarray:
+    """[m, 4] Ybus CSR positions of each branch's (f,f), (t,t), (f,t), (t,f)
+    entries (computed once per grid)."""
+    pos = getattr(grid, "_branch_pos", None)
+    if pos is not None:
+        return pos
+    Y = sp.csr_matrix(grid.Ybus)
+    Y.sort_indices()
+    f = np.asarray(grid.branch_from, np.int64)
+    t = np.asarray(grid.branch_to, np.int64)
+
+    def k_of(i, j):
+        out = np.empty(i.size, np.int64)
+        for q in range(i.size):
+            lo, hi = Y.indptr[i[q]], Y.indptr[i[q] + 1]
+            out[q] = lo + int(np.searchsorted(Y.indices[lo:hi], j[q]))
+        return out
+
+    pos = np.stack([k_of(f, f), k_of(t, t), k_of(f, t), k_of(t, f)], axis=1).astype(np.int32)
+    grid._branch_pos = pos
+    return pos
+
+
 def outage_edits(grid: Grid, rows_branches):
     """Per-row Ybus value edits of branch trips, the (row_ptr, entry, delta)
     arrays ``Batch.set_ybus_edits`` takes: tripping branch l (f -> t)
     subtracts yff, ytt, yft, ytf at the Ybus CSR positions (f,f), (t,t), (f,t),
     (t,f) (contingency_analysis_helper.cpp:163, build_flat_patches)."""
-    Y = sp.csr_matrix(grid.Ybus)
-    Y.sort_indices()
-
-    def k_of(i, j):
-        lo, hi = Y.indptr[i], Y.indptr[i + 1]
-        return lo + int(np.searchsorted(Y.indices[lo:hi], j))
-
-    row_ptr, ent, delta = [0], [], []
-    for brs in rows_branches:
-        for l in brs:
-            f, t = int(grid.branch_from[l]), int(grid.branch_to[l])
-            for (i, j, v) in ((f, f, grid.yff[l]), (t, t, grid.ytt[l]), (f, t, grid.yft[l]), (t, f, grid.ytf[l])):
-                ent.append(k_of(i, j))
-                delta.append(v)
-        row_ptr.append(len(ent))
-    return (np.array(row_ptr, dtype=np.int32), np.array(ent, dtype=np.int32),
-            np.array(delta, dtype=np.complex128))
+    pos = _branch_positions(grid)
+    counts = np.fromiter((len(b) for b in rows_branches), dtype=np.int64, count=len(rows_branches))
+    ids = (np.fromiter((l for b in rows_branches for l in b), dtype=np.int64, count=int(counts.sum()))
+           if counts.sum() else np.zeros(0, np.int64))
+    y = np.stack([grid.yff, grid.ytt, grid.yft, grid.ytf], axis=1)
+    row_ptr = np.zeros(len(rows_branches) + 1, np.int32)
+    row_ptr[1:] = np.cumsum(4 * counts)
+    return row_ptr, pos[ids].reshape(-1).astype(np.int32), y[ids].reshape(-1).astype(np.complex128)
 
 
 def outage_mask(grid: Grid, rows_branches) -> np.ndarray:

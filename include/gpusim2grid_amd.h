@@ -274,6 +274,11 @@ int g2g_lu_solve_batch(g2g_plan_t plan, int32_t rows, int32_t precision, const d
  * round.  Rows are host or device arrays (on_device), row-major n complex
  * with row strides lds / ldv in complex numbers (0 broadcasts one row). */
 int g2g_batch_stream_begin(g2g_batch_t batch);
+/* The learned start voltages of n freshly filled slots, predicted on the
+ * device from the slots' Sbus (and a host 0/1 outage mask [n][n_mask], or
+ * NULL) and written as their V; no counterpart in the reference. */
+int g2g_batch_stream_warm(g2g_batch_t batch, g2g_mlp_t mlp, int32_t n, const int32_t *slots, const uint8_t *mask,
+                          int32_t n_mask);
 int g2g_batch_stream_fill(g2g_batch_t batch, int32_t n, const int32_t *slots, const double *S, int64_t lds,
                           const double *V0, int64_t ldv, int32_t on_device, const int32_t *edit_ptr,
                           const int32_t *edit_entry, const double *edit_delta);

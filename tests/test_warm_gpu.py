@@ -77,6 +77,32 @@ def test_warm_start_kernel_matches_oracle_forward(trained):
     assert np.max(np.abs(Vh - Vw)) < 1e-5
 
 
+def test_streamed_slots_take_the_device_warm_start(trained):
+    """Slots filled in a streamed batch get, from g2g_batch_stream_warm, the
+    start voltages the whole-batch warm start gives the same rows; the other
+    slots keep their V."""
+    g, ws, _ = trained
+    rows, slots = 70, 256
+    _say("stream warm")
+    S = _sbus(g, rows, seed=12)
+    plan = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+    B = _capi.Batch(plan, rows, precision=_capi.MIXED)
+    B.set_sbus(S)
+    B.warm_start(ws.mlp)
+    Vw = B.get_v()
+    Bs = _capi.Batch(plan, slots, precision=_capi.MIXED, per_row_ybus=True)
+    Bs.stream_begin()
+    sl = list(range(3, 3 + 3 * rows, 3))
+    other = [5, 200]
+    Bs.stream_fill(other, S[:2], g.flat_start()[None], edits=syn.outage_edits(g, [[], []]))
+    Bs.stream_fill(sl, S, g.flat_start()[None], edits=syn.outage_edits(g, [[] for _ in range(rows)]))
+    Bs.stream_warm(ws.mlp, sl)
+    V = Bs.stream_take(sl)
+    np.testing.assert_allclose(V, Vw, rtol=0, atol=1e-12)
+    Vo = Bs.stream_take(other)
+    np.testing.assert_array_equal(Vo, np.broadcast_to(g.flat_start(), Vo.shape))
+
+
 @pytest.mark.parametrize("prec", [_capi.FP64, _capi.MIXED])
 def test_warm_solve_matches_flat_start_oracle(trained, prec):
     g, ws, _ = trained
