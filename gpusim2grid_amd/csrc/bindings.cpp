@@ -1119,6 +1119,36 @@ public:
     }
     int n_rows() const override { return rows_; }
     int n_scenarios() const { return rows_; }
+    // per-row generator voltage set-points (python_bindings.cpp:2232-2243): gen_v
+    // [n_scenarios][n_gen] in per unit, gen_bus[g] the bus generator g holds; at
+    // a Vm-fixed bus (PV or slack) the row's start |V| becomes gen_v (angle
+    // kept); other buses and NaN entries are ignored
+    void set_gen_v(DArr gen_v, IArr gen_bus)
+    {
+        if (gen_v.ndim() != 2 || gen_v.shape(1) != gen_bus.size())
+            throw py::value_error("gen_v must be (n_scenarios, n_gen) with n_gen = len(gen_bus)");
+        for (py::ssize_t g = 0; g < gen_bus.size(); ++g)
+            if (gen_bus.data()[g] < -1 || gen_bus.data()[g] >= n_bus())
+                throw py::value_error("gen_bus entry out of range");
+        gv_.assign(gen_v.data(), gen_v.data() + gen_v.size());
+        gv_rows_ = static_cast<int>(gen_v.shape(0));
+        gb_.assign(gen_bus.data(), gen_bus.data() + gen_bus.size());
+    }
+    void clear_gen_v()
+    {
+        gv_.clear();
+        gb_.clear();
+        gv_rows_ = 0;
+    }
+    py::array_t<bool> is_vm_fixed_bus() const
+    {
+        py::array_t<bool> a(n_bus());
+        bool* f = a.mutable_data();
+        std::fill(f, f + n_bus(), false);
+        for (int b : grid_->pv) f[b] = true;
+        f[grid_->slack] = true;
+        return a;
+    }
 
 protected:
     void prepare(RowJob& job) override
@@ -1129,6 +1159,31 @@ protected:
         start(job, n);
     }
     void start(RowJob& job, int n)
+    {
+        start_v(job, n);
+        if (gb_.empty()) return;
+        if (gv_rows_ != rows_) throw py::value_error("set_gen_v: one row per scenario");
+        // the start voltages with the set-points applied, one row per scenario
+        std::vector<char> fixed(static_cast<size_t>(n), 0);
+        for (int b : grid_->pv) fixed[b] = 1;
+        fixed[grid_->slack] = 1;
+        Vg_.resize(static_cast<size_t>(rows_) * n);
+        const int ng = static_cast<int>(gb_.size());
+        for (int r = 0; r < rows_; ++r) {
+            cplx* v = Vg_.data() + static_cast<size_t>(r) * n;
+            std::copy_n(job.V0 + r * job.V_stride, n, v);
+            for (int g = 0; g < ng; ++g) {
+                const int b = gb_[g];
+                const double vm = gv_[static_cast<size_t>(r) * ng + g];
+                if (b < 0 || !fixed[b] || std::isnan(vm)) continue;
+                const double a = std::abs(v[b]) > 0.0 ? std::arg(v[b]) : 0.0;
+                v[b] = std::polar(vm, a);
+            }
+        }
+        job.V0 = Vg_.data();
+        job.V_stride = n;
+    }
+    void start_v(RowJob& job, int n)
     {
         if (V0_.empty()) {
             job.V0 = V_base_.data();
@@ -1143,7 +1198,10 @@ protected:
             job.V_stride = n;
         }
     }
-    std::vector<cplx> S_, V0_;
+    std::vector<cplx> S_, V0_, Vg_;
+    std::vector<double> gv_;
+    std::vector<int32_t> gb_;
+    int gv_rows_ = 0;
     int rows_ = 0;
 };
 
@@ -1212,6 +1270,161 @@ protected:
     bool topo_ = false;
     Topology tp_;
 };
+
+// a DLPack capsule of a device copy of host complex rows (the reference's
+// *_dlpack exports hand out device memory; the results here live on the host)
+py::object device_capsule(py::array arr)
+{
+    py::object torch = py::module_::import("torch");
+    py::object t = torch.attr("from_numpy")(arr).attr("to")("cuda");
+    torch.attr("cuda").attr("synchronize")();
+    return py::module_::import("torch.utils.dlpack").attr("to_dlpack")(t);
+}
+
+// ---- raw batched LU (raw_cudss_solve.hpp:31-121) --------------------------
+struct RawPlan {
+    g2g_plan_t h = nullptr;
+    g2g_plan_info info{};
+    RawPlan(int dim, const std::vector<int>& indptr, const std::vector<int>& indices)
+    {
+        if (dim <= 0) throw py::value_error("dim must be positive");
+        if (static_cast<int>(indptr.size()) != dim + 1) throw py::value_error("indptr must have dim + 1 entries");
+        if (static_cast<int>(indices.size()) != indptr[dim]) throw py::value_error("indices must have nnz entries");
+        check(g2g_lu_plan_create(dim, indptr.data(), indices.data(), nullptr, &h));
+        check(g2g_plan_get_info(h, &info));
+    }
+    ~RawPlan() { g2g_plan_destroy(h); }
+};
+
+// max over rows of ||A x - b||inf / ||b||inf, and the rows with a non-finite x
+std::pair<double, int> raw_residual(int dim, const std::vector<int>& p, const std::vector<int>& ix,
+                                    const double* vals, const double* b, const double* x, int rows, int nnz)
+{
+    double worst = 0.0;
+    int bad = 0;
+    for (int r = 0; r < rows; ++r) {
+        const double* v = vals + static_cast<size_t>(r) * nnz;
+        const double* br = b + static_cast<size_t>(r) * dim;
+        const double* xr = x + static_cast<size_t>(r) * dim;
+        bool finite = true;
+        double num = 0.0, den = 0.0;
+        for (int i = 0; i < dim; ++i) {
+            if (!std::isfinite(xr[i])) finite = false;
+            double ax = 0.0;
+            for (int q = p[i]; q < p[i + 1]; ++q) ax += v[q] * xr[ix[q]];
+            num = std::max(num, std::fabs(ax - br[i]));
+            den = std::max(den, std::fabs(br[i]));
+        }
+        if (!finite) {
+            ++bad;
+            continue;
+        }
+        worst = std::max(worst, den > 0.0 ? num / den : num);
+    }
+    return {worst, bad};
+}
+
+std::vector<double> solve_raw(int dim, const std::vector<int>& indptr, const std::vector<int>& indices,
+                              const std::vector<double>& data, const std::vector<double>& rhs, int device,
+                              ReorderingAlg, MatchingAlg, PivotEpsilonAlg, int ir_n_steps)
+{
+    set_device(device);
+    if (static_cast<int>(rhs.size()) != dim) throw py::value_error("rhs must have dim entries");
+    if (ir_n_steps < 0) throw py::value_error("ir_n_steps must be >= 0");
+    RawPlan P(dim, indptr, indices);
+    if (data.size() != indices.size()) throw py::value_error("data must have nnz entries");
+    std::vector<double> x(static_cast<size_t>(dim));
+    check(g2g_lu_solve_batch(P.h, 1, G2G_FP64, data.data(), rhs.data(), x.data(), 1e-12, ir_n_steps, nullptr,
+                             nullptr));
+    return x;
+}
+
+struct RawBench {
+    int dim = 0, nnz = 0, batch_size = 0;
+    double context_init_ms = 0.0, analysis_ms = 0.0, factorize_ms = 0.0, refactorize_ms = 0.0, solve_ms = 0.0;
+    int n_refactorize = 0;
+    long long lu_nnz = -1, mem_device_permanent = -1, mem_device_peak = -1, mem_host_permanent = -1,
+              mem_host_peak = -1;
+    double max_rel_residual = 0.0;
+    int n_nonfinite_slots = 0, n_groups = 1;
+    double group_cycle_ms = 0.0;
+    int n_handles = 1;
+    std::string handle_launch = "serial";
+    double handle_setup_ms = 0.0, handle_round_ms = 0.0;
+    int n_active = 0;
+    double active_max_rel_residual = -1.0;
+    long long n_inactive_touched = -1;
+    double inactive_max_rel_residual = -1.0;
+};
+
+RawBench bench_raw(int dim, const std::vector<int>& indptr, const std::vector<int>& indices,
+                   const std::vector<double>& data, int batch_size, int n_refactorize, int device, ReorderingAlg,
+                   MatchingAlg, PivotEpsilonAlg, int ir_n_steps, int n_groups, int n_handles,
+                   const std::string& handle_launch, LinearSolver, int n_active)
+{
+    set_device(device);
+    if (batch_size <= 0) throw py::value_error("batch_size must be positive");
+    if (n_refactorize < 0 || ir_n_steps < 0 || n_active < 0 || n_active > batch_size)
+        throw py::value_error("n_refactorize, ir_n_steps must be >= 0 and n_active in [0, batch_size]");
+    if (n_groups != 1 || n_handles != 1)
+        throw py::value_error("n_groups / n_handles > 1 cycle cuDSS handles; this build's batched LU has none");
+    RawBench R;
+    R.dim = dim;
+    R.nnz = static_cast<int>(indices.size());
+    R.batch_size = batch_size;
+    R.n_refactorize = n_refactorize;
+    R.handle_launch = handle_launch;
+    R.n_active = n_active;
+    double t0 = now_ms();
+    RawPlan P(dim, indptr, indices);
+    R.analysis_ms = now_ms() - t0;
+    R.lu_nnz = P.info.nLU;
+    if (data.size() != indices.size()) throw py::value_error("data must have nnz entries");
+    const int nnz = R.nnz;
+    // every slot holds the same values; rhs = A 1, so x = 1
+    std::vector<double> vals(static_cast<size_t>(batch_size) * nnz), b(static_cast<size_t>(batch_size) * dim, 0.0),
+        x(b.size());
+    for (int r = 0; r < batch_size; ++r) std::copy(data.begin(), data.end(), vals.begin() + static_cast<size_t>(r) * nnz);
+    for (int i = 0; i < dim; ++i) {
+        double s = 0.0;
+        for (int q = indptr[i]; q < indptr[i + 1]; ++q) s += data[q];
+        for (int r = 0; r < batch_size; ++r) b[static_cast<size_t>(r) * dim + i] = s;
+    }
+    auto solve = [&](int rows) {
+        check(g2g_lu_solve_batch(P.h, rows, G2G_FP64, vals.data(), b.data(), x.data(), 1e-12, ir_n_steps, nullptr,
+                                 nullptr));
+    };
+    {
+        py::gil_scoped_release nogil;
+        t0 = now_ms();
+        solve(batch_size);
+        R.factorize_ms = now_ms() - t0;
+        double tot = 0.0;
+        for (int k = 0; k < n_refactorize; ++k) {
+            t0 = now_ms();
+            solve(batch_size);
+            tot += now_ms() - t0;
+        }
+        // one call factors and solves: its time is the refactorization's
+        R.refactorize_ms = n_refactorize ? tot / n_refactorize : 0.0;
+    }
+    auto rr = raw_residual(dim, indptr, indices, vals.data(), b.data(), x.data(), batch_size, nnz);
+    R.max_rel_residual = rr.first;
+    R.n_nonfinite_slots = rr.second;
+    if (n_active > 0) {
+        // the prefix alone, with its values tripled; the other rows are not touched
+        for (size_t q = 0; q < static_cast<size_t>(n_active) * nnz; ++q) vals[q] *= 3.0;
+        for (size_t q = 0; q < static_cast<size_t>(n_active) * dim; ++q) b[q] *= 3.0;
+        std::vector<double> keep(x.begin() + static_cast<size_t>(n_active) * dim, x.end());
+        solve(n_active);
+        R.active_max_rel_residual =
+            raw_residual(dim, indptr, indices, vals.data(), b.data(), x.data(), n_active, nnz).first;
+        long long touched = 0;
+        for (size_t q = 0; q < keep.size(); ++q) touched += keep[q] != x[static_cast<size_t>(n_active) * dim + q];
+        R.n_inactive_touched = touched;
+    }
+    return R;
+}
 
 }  // namespace
 
@@ -1310,6 +1523,7 @@ PYBIND11_MODULE(_gpusim2grid, m)
         .def("resolve", &AcPfNrSession::resolve, py::arg("Vinit"), py::arg("Sbus") = py::none(),
              py::arg("max_iter") = -1, py::arg("tol") = -1.0)
         .def("get_v", &AcPfNrSession::get_v)
+        .def("v_dlpack", [](AcPfNrSession& s) { return device_capsule(s.get_v()); })
         .def("get_F", &AcPfNrSession::get_F)
         .def("get_J", &AcPfNrSession::get_J)
         .def("solve_JT", &AcPfNrSession::solve_JT, py::arg("rhs"))
@@ -1369,6 +1583,12 @@ PYBIND11_MODULE(_gpusim2grid, m)
             .def("get_ex_amps", &S::get_ex_amps)
             .def("clear_warm_start", &S::clear_warm_start)
             .def_readwrite("handle_disconnected_grid", &S::handle_disconnected_grid)
+            .def("v_results_dlpack",
+                 [](const S& s) {
+                     CArr V = s.get_V_results();
+                     return device_capsule(V.reshape({static_cast<py::ssize_t>(s.n_rows()),
+                                                      static_cast<py::ssize_t>(s.n_bus())}));
+                 })
             .def_readwrite("compute_limit_violations", &S::compute_limit_violations)
             .def_readwrite("violation_tol", &S::violation_tol)
             .def_property("violation_rel_tol", &S::get_violation_rel_tol, &S::set_violation_rel_tol)
@@ -1414,6 +1634,9 @@ PYBIND11_MODULE(_gpusim2grid, m)
              py::arg("sn_mva"))
         .def("set_sbus", &InjectionSweepSession::set_sbus, py::arg("Sbus"))
         .def("set_v_init_from", &InjectionSweepSession::set_v_init_from, py::arg("V"))
+        .def("set_gen_v", &InjectionSweepSession::set_gen_v, py::arg("gen_v"), py::arg("gen_bus"))
+        .def("clear_gen_v", &InjectionSweepSession::clear_gen_v)
+        .def_property_readonly("is_vm_fixed_bus", &InjectionSweepSession::is_vm_fixed_bus)
         .def_property_readonly("n_scenarios", &InjectionSweepSession::n_scenarios);
     bind_batch(inj);
 
@@ -1434,6 +1657,9 @@ PYBIND11_MODULE(_gpusim2grid, m)
         .def("clear_topology", &ScenarioSweepSession::clear_topology)
         .def("set_v_init_from", &ScenarioSweepSession::set_v_init_from, py::arg("V"))
         .def("clear_v_init", &ScenarioSweepSession::clear_v_init)
+        .def("set_gen_v", &ScenarioSweepSession::set_gen_v, py::arg("gen_v"), py::arg("gen_bus"))
+        .def("clear_gen_v", &ScenarioSweepSession::clear_gen_v)
+        .def_property_readonly("is_vm_fixed_bus", &ScenarioSweepSession::is_vm_fixed_bus)
         .def_property_readonly("has_v_init", &ScenarioSweepSession::has_v_init)
         .def_property_readonly("n_scenarios", &ScenarioSweepSession::n_scenarios);
     bind_batch(ss);
@@ -1487,6 +1713,47 @@ PYBIND11_MODULE(_gpusim2grid, m)
             AcPfNrSession s2(Y, V, S, sl, w, pv, pq, 10, 1e-8, device, false, StepControl(), Precision::Mixed);
         },
         py::arg("device") = -1);
+    m.def("solve_cudss_raw", &solve_raw, py::call_guard<py::gil_scoped_release>(), py::arg("dim"),
+          py::arg("indptr"), py::arg("indices"), py::arg("data"), py::arg("rhs"), py::arg("device") = -1,
+          py::arg("reordering_alg") = ReorderingAlg::Default, py::arg("matching_alg") = MatchingAlg::None,
+          py::arg("pivot_epsilon_alg") = PivotEpsilonAlg::Default, py::arg("ir_n_steps") = 0,
+          "Solve J dx = rhs for one CSR matrix through the batched LU (plan, factor, solve).");
+    py::class_<RawBench>(m, "CudssBatchBenchResult")
+        .def_readonly("dim", &RawBench::dim)
+        .def_readonly("nnz", &RawBench::nnz)
+        .def_readonly("batch_size", &RawBench::batch_size)
+        .def_readonly("context_init_ms", &RawBench::context_init_ms)
+        .def_readonly("analysis_ms", &RawBench::analysis_ms)
+        .def_readonly("factorize_ms", &RawBench::factorize_ms)
+        .def_readonly("refactorize_ms", &RawBench::refactorize_ms)
+        .def_readonly("solve_ms", &RawBench::solve_ms)
+        .def_readonly("n_refactorize", &RawBench::n_refactorize)
+        .def_readonly("lu_nnz", &RawBench::lu_nnz)
+        .def_readonly("mem_device_permanent", &RawBench::mem_device_permanent)
+        .def_readonly("mem_device_peak", &RawBench::mem_device_peak)
+        .def_readonly("mem_host_permanent", &RawBench::mem_host_permanent)
+        .def_readonly("mem_host_peak", &RawBench::mem_host_peak)
+        .def_readonly("max_rel_residual", &RawBench::max_rel_residual)
+        .def_readonly("n_nonfinite_slots", &RawBench::n_nonfinite_slots)
+        .def_readonly("n_groups", &RawBench::n_groups)
+        .def_readonly("group_cycle_ms", &RawBench::group_cycle_ms)
+        .def_readonly("n_handles", &RawBench::n_handles)
+        .def_readonly("handle_launch", &RawBench::handle_launch)
+        .def_readonly("handle_setup_ms", &RawBench::handle_setup_ms)
+        .def_readonly("handle_round_ms", &RawBench::handle_round_ms)
+        .def_readonly("n_active", &RawBench::n_active)
+        .def_readonly("active_max_rel_residual", &RawBench::active_max_rel_residual)
+        .def_readonly("n_inactive_touched", &RawBench::n_inactive_touched)
+        .def_readonly("inactive_max_rel_residual", &RawBench::inactive_max_rel_residual);
+    m.def("benchmark_cudss_batch_raw", &bench_raw, py::arg("dim"), py::arg("indptr"), py::arg("indices"),
+          py::arg("data"), py::arg("batch_size"), py::arg("n_refactorize") = 4, py::arg("device") = -1,
+          py::arg("reordering_alg") = ReorderingAlg::Default, py::arg("matching_alg") = MatchingAlg::None,
+          py::arg("pivot_epsilon_alg") = PivotEpsilonAlg::Default, py::arg("ir_n_steps") = 0,
+          py::arg("n_groups") = 1, py::arg("n_handles") = 1, py::arg("handle_launch") = "serial",
+          py::arg("linear_solver") = LinearSolver::Cudss, py::arg("n_active") = 0,
+          "Time the batched LU on one CSR matrix replicated batch_size times (rhs = A 1, so x = 1): "
+          "plan (analysis_ms), first call (factorize_ms), then n_refactorize calls (refactorize_ms: "
+          "a call factors and solves; solve_ms stays 0).");
     m.attr("is_fp32") = false;
     m.attr("have_ls2g_bridge") = false;
     m.attr("abi_version") = g2g_version();

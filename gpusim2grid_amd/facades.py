@@ -407,6 +407,14 @@ class InjectionSweepGPU(_BatchFacade):
         """Start voltages: one row, one per scenario, or None (the base case)."""
         self._s.set_v_init_from(None if V is None else np.ascontiguousarray(V, dtype=complex))
 
+    def set_gen_v(self, gen_v, gen_bus):
+        """Per-scenario generator voltage set-points (n_scenarios, n_gen) in per
+        unit, generator g holding bus gen_bus[g]: the start |V| of that bus when
+        it is PV or slack (``injection_sweep/gpu_facade.py:457-511``; array mode
+        names the buses, which the reference reads from its lightsim2grid grid)."""
+        self._s.set_gen_v(np.ascontiguousarray(gen_v, dtype=np.float64),
+                          np.ascontiguousarray(gen_bus, dtype=np.int32))
+
     @property
     def n_scenarios(self):
         return self._s.n_scenarios
