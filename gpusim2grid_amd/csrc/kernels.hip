@@ -52,6 +52,16 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b)
     return make_double2(fma(a.x, b.x, a.y * b.y), fma(a.y, b.x, -a.x * b.y));
 }
 
+// a store of streamed output (the mismatch's A stream, the factorization's U
+// stream) that no later access of the kernel reads: non-temporal, so it does
+// not evict what the kernel reuses from the L2 (the neighbours' V, the
+// program)
+template <typename T>
+__device__ __forceinline__ void stream_store(T* p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
 __device__ __forceinline__ double nan_max(double m, double v)
 {
     // NaN-sticky max of |v| (acpf_nr_kernels.cu:1101-1114)
@@ -104,10 +114,10 @@ __device__ __forceinline__ void mis_entries(const MismatchArgs<Real>& a, size_t 
             const int a11 = en[2], a12 = en[3], a21 = en[4], a22 = en[5];
             double j11 = w.y, j21 = -w.x, j12 = w.x * inv, j22 = w.y * inv;
             if (b.masked) j11 = j21 = j12 = j22 = 0.0;
-            if (a11 >= 0) a.A[tidx(tile, a.a_stride, a11, lane)] = static_cast<Real>(j11);
-            if (a12 >= 0) a.A[tidx(tile, a.a_stride, a12, lane)] = static_cast<Real>(j12);
-            if (a21 >= 0) a.A[tidx(tile, a.a_stride, a21, lane)] = static_cast<Real>(j21);
-            if (a22 >= 0) a.A[tidx(tile, a.a_stride, a22, lane)] = static_cast<Real>(j22);
+            if (a11 >= 0) stream_store(&a.A[tidx(tile, a.a_stride, a11, lane)], static_cast<Real>(j11));
+            if (a12 >= 0) stream_store(&a.A[tidx(tile, a.a_stride, a12, lane)], static_cast<Real>(j12));
+            if (a21 >= 0) stream_store(&a.A[tidx(tile, a.a_stride, a21, lane)], static_cast<Real>(j21));
+            if (a22 >= 0) stream_store(&a.A[tidx(tile, a.a_stride, a22, lane)], static_cast<Real>(j22));
             jmax = fmax(jmax, fmax(fmax(fabs(j11), fabs(j12)), fmax(fabs(j21), fabs(j22))));
         }
     }
@@ -191,11 +201,11 @@ __global__ __launch_bounds__(kLanes* kMismatchWaves) void k_mismatch(MismatchArg
                 const double2 sc = cmulc(c.vi, c.I);  // V conj(I)
                 const double fpv = c.masked ? 0.0 : -(sc.x - c.sb.x);
                 rmax = nan_max(rmax, fpv);
-                a.A[tidx(tile, a.a_stride, c.fp, lane)] = static_cast<Real>(fpv);
+                stream_store(&a.A[tidx(tile, a.a_stride, c.fp, lane)], static_cast<Real>(fpv));
                 if (c.fq >= 0) {
                     const double fqv = c.masked ? 0.0 : -(sc.y - c.sb.y);
                     rmax = nan_max(rmax, fqv);
-                    a.A[tidx(tile, a.a_stride, c.fq, lane)] = static_cast<Real>(fqv);
+                    stream_store(&a.A[tidx(tile, a.a_stride, c.fq, lane)], static_cast<Real>(fqv));
                 }
                 if (WithJ) {
                     const int* en = a.k1_ent + 6 * c.dent;
@@ -210,10 +220,10 @@ __global__ __launch_bounds__(kLanes* kMismatchWaves) void k_mismatch(MismatchArg
                         j12 = j21 = 0.0;
                     }
                     const int a11 = en[2], a12 = en[3], a21 = en[4], a22 = en[5];
-                    if (a11 >= 0) a.A[tidx(tile, a.a_stride, a11, lane)] = static_cast<Real>(j11);
-                    if (a12 >= 0) a.A[tidx(tile, a.a_stride, a12, lane)] = static_cast<Real>(j12);
-                    if (a21 >= 0) a.A[tidx(tile, a.a_stride, a21, lane)] = static_cast<Real>(j21);
-                    if (a22 >= 0) a.A[tidx(tile, a.a_stride, a22, lane)] = static_cast<Real>(j22);
+                    if (a11 >= 0) stream_store(&a.A[tidx(tile, a.a_stride, a11, lane)], static_cast<Real>(j11));
+                    if (a12 >= 0) stream_store(&a.A[tidx(tile, a.a_stride, a12, lane)], static_cast<Real>(j12));
+                    if (a21 >= 0) stream_store(&a.A[tidx(tile, a.a_stride, a21, lane)], static_cast<Real>(j21));
+                    if (a22 >= 0) stream_store(&a.A[tidx(tile, a.a_stride, a22, lane)], static_cast<Real>(j22));
                     jmax = fmax(jmax, fmax(fmax(fabs(j11), fabs(j12)), fmax(fabs(j21), fabs(j22))));
                 }
             }
@@ -390,10 +400,10 @@ __device__ __forceinline__ int factor_pivot(AR& A, ProgReader& P, FactorState<Re
 #pragma unroll
     for (int q = 0; q < C; ++q) lv[q] *= -inv;  // negated multipliers
     Real* u = st.U + static_cast<size_t>(st.up) * kLanes;
-    u[0] = fk;
-    u[kLanes] = inv;
+    stream_store(&u[0], fk);
+    stream_store(&u[kLanes], inv);
 #pragma unroll
-    for (int q = 0; q < C; ++q) u[(2 + q) * kLanes] = uv[q];
+    for (int q = 0; q < C; ++q) stream_store(&u[(2 + q) * kLanes], uv[q]);
     st.up += 2 + C;
     if constexpr (kAll) {
 #pragma unroll
@@ -478,11 +488,11 @@ __device__ __forceinline__ int factor_pivot2(AR& A, ProgReader& P, FactorState<R
 #pragma unroll
     for (int j = 0; j < C; ++j) uk[j] = x[4 + C + j];
     Real* u = st.U + static_cast<size_t>(st.up) * kLanes;
-    u[0] = fk;
-    u[kLanes] = inv1;
-    u[2 * kLanes] = b;
+    stream_store(&u[0], fk);
+    stream_store(&u[kLanes], inv1);
+    stream_store(&u[2 * kLanes], b);
 #pragma unroll
-    for (int j = 0; j < C; ++j) u[(3 + j) * kLanes] = uk[j];
+    for (int j = 0; j < C; ++j) stream_store(&u[(3 + j) * kLanes], uk[j]);
     // pivot k+1, updated by pivot k
     const Real fk1 = fma(-l, fk, x[K1]);
     Real d2 = fma(-l, b, x[K1 + 1]);
@@ -496,10 +506,10 @@ __device__ __forceinline__ int factor_pivot2(AR& A, ProgReader& P, FactorState<R
 #pragma unroll
     for (int j = 0; j < C; ++j) uk1[j] = fma(-l, uk[j], x[K1 + 2 + C + j]);
     Real* u1 = u + static_cast<size_t>(3 + C) * kLanes;
-    u1[0] = fk1;
-    u1[kLanes] = inv2;
+    stream_store(&u1[0], fk1);
+    stream_store(&u1[kLanes], inv2);
 #pragma unroll
-    for (int j = 0; j < C; ++j) u1[(2 + j) * kLanes] = uk1[j];
+    for (int j = 0; j < C; ++j) stream_store(&u1[(2 + j) * kLanes], uk1[j]);
     st.up += 5 + 2 * C;
     st.ap += NV;
     // rank-2 update of S x S and b(S)
@@ -569,9 +579,9 @@ __device__ int factor_pivot_any(int c, AR& A, ProgReader& P, FactorState<Real>& 
     const Real inv = pivot_inv(dv);
     for (int q = 0; q < c; ++q) lsc[q * kLanes] *= -inv;
     Real* u = st.U + static_cast<size_t>(st.up) * kLanes;
-    u[0] = fk;
-    u[kLanes] = inv;
-    for (int q = 0; q < c; ++q) u[(2 + q) * kLanes] = usc[q * kLanes];
+    stream_store(&u[0], fk);
+    stream_store(&u[kLanes], inv);
+    for (int q = 0; q < c; ++q) stream_store(&u[(2 + q) * kLanes], usc[q * kLanes]);
     st.up += 2 + c;
     for (int a = 0; a < c; ++a)
         for (int b = 0; b < c; ++b) {
