@@ -43,6 +43,10 @@ template <typename T>
 __global__ void k_slots_to_rows(const T* src, int R, int m, const int* slots, T* dst, long long ld);
 __global__ void k_set_slots(const int* slots, int R, int* v, int value);
 __global__ void k_mask_v_nan(const unsigned char* mask, long long total, double2* V);
+__global__ void k_restart_v(const int* status, const unsigned char* warm, int T, int n, const double2* V0,
+                            double2* V);
+__global__ void k_restart_rows(int* status, unsigned char* warm, const int* it0, int* budget, int rows, int iter,
+                               int max_iter, int* requeued);
 template <typename Real>
 __global__ void k_gather_astream(const double* vals, long long ldv, const double* rhs, long long ldb, int rows,
                                  int nnz, int d, const int* a_pos, const int* f_pos, int a_len, Real* A);
@@ -199,17 +203,9 @@ struct g2g_batch_s {
     int* n_active = nullptr;
     int* clamped = nullptr;
     int* error = nullptr;     // stream kernels' give-up flag
-    int* h_active = nullptr;  // pinned
-    // warm solves: the caller's start voltages, and the dense work area of the
-    // re-queued rows (V, Sbus, per-row Ybus, saved per-row results, row map)
+    int* h_active = nullptr;  // pinned: rows still iterating, rows restarted
+    // warm solves: the caller's start voltages (the fallback start)
     double2* V0 = nullptr;
-    double2* Vc = nullptr;
-    double2* Sc = nullptr;
-    double2* Yc = nullptr;
-    int* st_save = nullptr;
-    int* it_save = nullptr;
-    double* res_save = nullptr;
-    int* row_map = nullptr;
     // step control (settings of the next solve)
     int ir_steps = 0;                 // refinement steps of every Newton step
     double max_dva = 0.0, max_dvm = 0.0;  // step scaling bounds (max_dva <= 0: off)
@@ -231,10 +227,19 @@ struct g2g_batch_s {
     int* it0 = nullptr;
     int clock = 0;
     bool streaming = false;
-    // islanded buses per row [T][n][64] (nullable), and its compaction / re-queue copies
+    // learned starts with in-place fallback: per-row update budgets, the rows
+    // still on their learned start, the restart count, and compaction copies
+    bool restart_on = false;
+    int restart_max_iter = 0;
+    int* budget = nullptr;
+    unsigned char* warm = nullptr;
+    int* requeued = nullptr;
+    double2* V0k = nullptr;
+    int *it0k = nullptr, *budgetk = nullptr;
+    unsigned char* warmk = nullptr;
+    // islanded buses per row [T][n][64] (nullable), and its compaction copy
     unsigned char* mask = nullptr;
     unsigned char* maskk = nullptr;
-    unsigned char* maskc = nullptr;
     // grow-only device staging of the streamed-row calls (kept between calls,
     // so a refill or a take allocates and frees nothing once warm)
     void* stage[kStageBufs] = {};
@@ -244,10 +249,10 @@ struct g2g_batch_s {
     {
         dfree(V); dfree(S); dfree(Yrow); dfree(A); dfree(U); dfree(X); dfree(res); dfree(jmax);
         dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped); dfree(error);
-        dfree(V0); dfree(Vc); dfree(Sc); dfree(Yc); dfree(st_save); dfree(it_save); dfree(res_save);
-        dfree(row_map); dfree(Bs); dfree(Xs);
+        dfree(V0); dfree(Bs); dfree(Xs);
         dfree(Vk); dfree(Sk); dfree(Yk); dfree(stk); dfree(itk); dfree(resk); dfree(mapk); dfree(it0);
-        dfree(mask); dfree(maskk); dfree(maskc);
+        dfree(mask); dfree(maskk);
+        dfree(budget); dfree(warm); dfree(requeued); dfree(V0k); dfree(it0k); dfree(budgetk); dfree(warmk);
         for (void*& q : stage) dfree(q);
         if (h_active) (void)hipHostFree(h_active);
         for (auto e : ev) (void)hipEventDestroy(e);
@@ -399,7 +404,8 @@ void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double
     MismatchArgs<Real> a{b->plan->d_k1_bus, transposed ? b->plan->d_k1_ent_t : b->plan->d_k1_ent, P.k1_nbus, b->plan->d_Y, b->Yrow, b->V, b->S,
                          static_cast<Real*>(b->A), P.n, P.nY, b->a_stride, b->res, b->jmax, b->status, b->iters,
                          b->tile_active, b->n_active, b->active_rows, iter, max_iter, tol, freeze, growth,
-                         b->streaming ? b->it0 : nullptr, b->mask};
+                         (b->streaming || b->restart_on) ? b->it0 : nullptr, b->mask,
+                         b->restart_on ? b->budget : nullptr, b->restart_on ? b->warm : nullptr};
     const dim3 grid(b->T), block(kLanes * kMismatchWaves);
     if (b->per_row_y) {
         if (with_j) k_mismatch<Real, true, true><<<grid, block, 0, b->stream>>>(a);
@@ -479,11 +485,14 @@ void compact_rows(g2g_batch_s* b)
     }
     const int rows = b->active_rows;
     std::vector<int> st(static_cast<size_t>(rows));
+    std::vector<unsigned char> wm(b->restart_on ? static_cast<size_t>(rows) : 0);
     HIPCHK(hipMemcpyAsync(st.data(), b->status, st.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (b->restart_on) HIPCHK(hipMemcpyAsync(wm.data(), b->warm, wm.size(), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     std::vector<int> map;
+    // the rows still iterating, and those about to restart from their fallback start
     for (int r = 0; r < rows; ++r)
-        if (st[r] == kActive) map.push_back(r);
+        if (st[r] == kActive || (b->restart_on && st[r] == kRejected && wm[r])) map.push_back(r);
     const int R = static_cast<int>(map.size());
     HIPCHK(hipMemcpyAsync(b->mapk, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(b->stk, b->status, rp * sizeof(int), hipMemcpyDeviceToDevice, s));
@@ -502,10 +511,26 @@ void compact_rows(g2g_batch_s* b)
         std::swap(b->mask, b->maskk);
     }
     k_fill_int<<<blocks_for(rp, 256), 256, 0, s>>>(b->status, kNotSimulated, rp);
-    k_fill_int<<<blocks_for(R, 256), 256, 0, s>>>(b->status, kActive, R);
     const unsigned gr = blocks_for(R, 256);
+    k_move_vals<int><<<gr, 256, 0, s>>>(b->stk, b->status, b->mapk, R, 1, 0);
     k_move_vals<int><<<gr, 256, 0, s>>>(b->itk, b->iters, b->mapk, R, 1, 0);
     k_move_vals<double><<<gr, 256, 0, s>>>(b->resk, b->res, b->mapk, R, 1, 0);
+    if (b->restart_on) {
+        if (!b->V0k) {
+            b->V0k = dalloc<double2>(rp * nr.n);
+            b->it0k = dalloc<int>(rp);
+            b->budgetk = dalloc<int>(rp);
+            b->warmk = dalloc<unsigned char>(rp);
+        }
+        k_move_rows<double2><<<gv, 256, 0, s>>>(b->V0, b->V0k, b->mapk, R, nr.n, 1);
+        k_move_vals<int><<<gr, 256, 0, s>>>(b->it0, b->it0k, b->mapk, R, 1, 0);
+        k_move_vals<int><<<gr, 256, 0, s>>>(b->budget, b->budgetk, b->mapk, R, 1, 0);
+        k_move_vals<unsigned char><<<gr, 256, 0, s>>>(b->warm, b->warmk, b->mapk, R, 1, 0);
+        std::swap(b->V0, b->V0k);
+        std::swap(b->it0, b->it0k);
+        std::swap(b->budget, b->budgetk);
+        std::swap(b->warm, b->warmk);
+    }
     HIPCHK(hipGetLastError());
     std::swap(b->V, b->Vk);
     std::swap(b->S, b->Sk);
@@ -529,6 +554,12 @@ void uncompact_rows(g2g_batch_s* b)
     std::swap(b->S, b->Sk);
     if (b->per_row_y) std::swap(b->Yrow, b->Yk);
     if (b->mask) std::swap(b->mask, b->maskk);
+    if (b->restart_on) {  // the fallback tables go back to row order unread
+        std::swap(b->V0, b->V0k);
+        std::swap(b->it0, b->it0k);
+        std::swap(b->budget, b->budgetk);
+        std::swap(b->warm, b->warmk);
+    }
     const unsigned gr = blocks_for(R, 256);
     k_move_vals<int><<<gr, 256, 0, s>>>(b->status, b->stk, b->mapk, R, 0, 0);
     k_move_vals<int><<<gr, 256, 0, s>>>(b->iters, b->itk, b->mapk, R, 0, 0);
@@ -554,14 +585,38 @@ int nr_loop_body(g2g_batch_s* b, int max_iter, double tol, int freeze, double gr
         b->Xs = dalloc<unsigned char>(bytes);
     }
     int it = 0;
+    int restarts = 0;
     for (;; ++it) {
         k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
         int e0 = tm.mark();
         launch_mismatch<Real>(b, true, it, max_iter, tol, freeze, growth);
         tm.ph[0].push_back({e0, tm.mark()});
         if (it >= max_iter) break;
+        if (b->restart_on) {
+            // rows whose learned start failed at this mismatch take their
+            // fallback start now, with max_iter more updates
+            const NrPlan& nr = b->plan->nr;
+            k_restart_v<<<blocks_for(static_cast<long long>(b->T) * nr.n * kLanes, 256), 256, 0, b->stream>>>(
+                b->status, b->warm, b->T, nr.n, b->V0, b->V);
+            k_restart_rows<<<blocks_for(b->active_rows, 256), 256, 0, b->stream>>>(
+                b->status, b->warm, b->it0, b->budget, b->active_rows, it, b->restart_max_iter, b->requeued);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(b->h_active + 1, b->requeued, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+        }
         HIPCHK(hipMemcpyAsync(b->h_active, b->n_active, sizeof(int), hipMemcpyDeviceToHost, b->stream));
         HIPCHK(hipStreamSynchronize(b->stream));
+        if (b->restart_on && b->h_active[1] > restarts) {
+            // evaluate again so the restarted rows' J and F (at their new
+            // start) are in the A stream; the other rows' decisions were made
+            // (no growth test this time) and come out the same
+            restarts = b->h_active[1];
+            k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
+            e0 = tm.mark();
+            launch_mismatch<Real>(b, true, it, max_iter, tol, freeze, 0.0);
+            tm.ph[0].push_back({e0, tm.mark()});
+            HIPCHK(hipMemcpyAsync(b->h_active, b->n_active, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+            HIPCHK(hipStreamSynchronize(b->stream));
+        }
         if (*b->h_active == 0) break;
         if (!compacted && *b->h_active * 4 <= b->active_rows && *b->h_active + 4 * kLanes <= b->active_rows) {
             compact_rows(b);
@@ -703,13 +758,13 @@ void solve_impl(g2g_batch_s* b, int max_iter, double tol, int sched, g2g_solve_s
     fill_stats(b, tm, e_begin, e_end, it, rows_factored, stats);
 }
 
-// Learned warm start with fallback: the MLP predicts every active row's V,
-// Newton runs at most k_warm updates on it (a row whose ||F||inf grows by
-// more than `growth` is dropped at once), and the rows left unconverged
-// restart from the caller's start voltages with the full budget.  The
-// restarted rows are first gathered into dense tiles, so the fallback pass
-// costs in proportion to their number, then scattered back.  A row's
-// iteration count is the sum of the updates of both passes.
+// Learned start with in-place fallback: every row starts from the MLP's
+// prediction with a budget of k_warm updates; a row whose mismatch grows
+// more than `growth`, turns non-finite or is not converged within k_warm
+// takes the caller's start voltages (the V the batch held before the call)
+// at once and gets the full budget of max_iter updates on top, while the
+// other rows carry on, so a failure costs its own iterations only.  A row's
+// iteration count is the sum of its updates from both starts.
 template <typename Real>
 void solve_warm_impl(g2g_batch_s* b, g2g_mlp_s* mlp, const unsigned char* mask, int n_mask, int k_warm,
                      double growth, int max_iter, double tol, g2g_solve_stats* stats)
@@ -718,21 +773,13 @@ void solve_warm_impl(g2g_batch_s* b, g2g_mlp_s* mlp, const unsigned char* mask, 
     const size_t rp = static_cast<size_t>(b->rows_pad);
     const size_t nv = rp * nr.n;
     b->streaming = false;
-    if (!b->V0) {
-        b->V0 = dalloc<double2>(nv);
-        b->Vc = dalloc<double2>(nv);
-        b->Sc = dalloc<double2>(nv);
-        if (b->per_row_y) b->Yc = dalloc<double2>(rp * nr.nY);
-        b->st_save = dalloc<int>(rp);
-        b->it_save = dalloc<int>(rp);
-        b->res_save = dalloc<double>(rp);
-        b->row_map = dalloc<int>(rp);
-        // lanes beyond the re-queued rows stay finite (|V| = 1)
-        HIPCHK(hipMemcpyAsync(b->Vc, b->V, nv * sizeof(double2), hipMemcpyDeviceToDevice, b->stream));
-        HIPCHK(hipMemsetAsync(b->Sc, 0, nv * sizeof(double2), b->stream));
-        if (b->Yc)
-            HIPCHK(hipMemcpyAsync(b->Yc, b->Yrow, rp * nr.nY * sizeof(double2), hipMemcpyDeviceToDevice, b->stream));
+    if (!b->V0) b->V0 = dalloc<double2>(nv);
+    if (!b->budget) {
+        b->budget = dalloc<int>(rp);
+        b->warm = dalloc<unsigned char>(rp);
+        b->requeued = dalloc<int>(1);
     }
+    if (!b->it0) b->it0 = dalloc<int>(rp);
     DeviceScratch tmp;
     unsigned char* d_mask = nullptr;
     if (n_mask > 0) {
@@ -741,73 +788,33 @@ void solve_warm_impl(g2g_batch_s* b, g2g_mlp_s* mlp, const unsigned char* mask, 
                               b->stream));
     }
     reset_rows(b);
-    SolveTimer tm(b, 3 * (k_warm + max_iter + 2) + 4);
+    SolveTimer tm(b, 3 * (k_warm + max_iter + 2) + 8);
     const int e_begin = tm.mark();
-    HIPCHK(hipMemcpyAsync(b->V0, b->V, nv * sizeof(double2), hipMemcpyDeviceToDevice, b->stream));
-    mlp_warm_start_tiles(mlp, b->S, b->active_rows, nr.n, d_mask, n_mask, b->V, b->stream);
+    hipStream_t s = b->stream;
+    HIPCHK(hipMemcpyAsync(b->V0, b->V, nv * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    mlp_warm_start_tiles(mlp, b->S, b->active_rows, nr.n, d_mask, n_mask, b->V, s);
     tm.ph[3].push_back({e_begin, tm.mark()});
+    k_fill_int<<<blocks_for(rp, 256), 256, 0, s>>>(b->it0, 0, rp);
+    k_fill_int<<<blocks_for(rp, 256), 256, 0, s>>>(b->budget, k_warm, rp);
+    HIPCHK(hipMemsetAsync(b->warm, 1, rp, s));
+    k_fill_int<<<1, 64, 0, s>>>(b->requeued, 0, 1);
+    HIPCHK(hipGetLastError());
     long long rows_factored = 0;
-    int it = nr_loop<Real>(b, k_warm, tol, 1, growth, tm, rows_factored);
-    // the rows to restart, in order
-    const int rows = b->active_rows;
-    std::vector<int> st(static_cast<size_t>(rows));
-    HIPCHK(hipMemcpyAsync(st.data(), b->status, st.size() * sizeof(int), hipMemcpyDeviceToHost, b->stream));
-    HIPCHK(hipStreamSynchronize(b->stream));
-    std::vector<int> map;
-    for (int r = 0; r < rows; ++r)
-        if (st[r] == kMaxIter || st[r] == kDiverged || st[r] == kRejected) map.push_back(r);
-    const int R = static_cast<int>(map.size());
-    if (R > 0) {
-        hipStream_t s = b->stream;
-        HIPCHK(hipMemcpyAsync(b->row_map, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(b->st_save, b->status, rp * sizeof(int), hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(b->it_save, b->iters, rp * sizeof(int), hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(b->res_save, b->res, rp * sizeof(double), hipMemcpyDeviceToDevice, s));
-        const unsigned gv = blocks_for(static_cast<long long>(R) * nr.n, 256);
-        k_move_rows<double2><<<gv, 256, 0, s>>>(b->V0, b->Vc, b->row_map, R, nr.n, 1);
-        k_move_rows<double2><<<gv, 256, 0, s>>>(b->S, b->Sc, b->row_map, R, nr.n, 1);
-        if (b->per_row_y)
-            k_move_rows<double2><<<blocks_for(static_cast<long long>(R) * nr.nY, 256), 256, 0, s>>>(
-                b->Yrow, b->Yc, b->row_map, R, nr.nY, 1);
-        if (b->mask) {
-            if (!b->maskc) b->maskc = dalloc<unsigned char>(rp * nr.n);
-            HIPCHK(hipMemsetAsync(b->maskc, 0, rp * nr.n, s));
-            k_move_rows<unsigned char><<<gv, 256, 0, s>>>(b->mask, b->maskc, b->row_map, R, nr.n, 1);
-        }
-        HIPCHK(hipGetLastError());
-        auto swap_all = [&] {
-            std::swap(b->V, b->Vc);
-            std::swap(b->S, b->Sc);
-            if (b->per_row_y) std::swap(b->Yrow, b->Yc);
-            if (b->mask) std::swap(b->mask, b->maskc);
-        };
-        swap_all();
-        b->active_rows = R;
-        try {
-            reset_rows(b);
-            it += nr_loop<Real>(b, max_iter, tol, 1, 0.0, tm, rows_factored);
-        } catch (...) {
-            swap_all();
-            b->active_rows = rows;
-            throw;
-        }
-        swap_all();
-        b->active_rows = rows;
-        // results of the restarted rows back to their places
-        k_move_rows<double2><<<gv, 256, 0, s>>>(b->Vc, b->V, b->row_map, R, nr.n, 0);
-        const unsigned gr = blocks_for(R, 256);
-        k_move_vals<int><<<gr, 256, 0, s>>>(b->status, b->st_save, b->row_map, R, 0, 0);
-        k_move_vals<int><<<gr, 256, 0, s>>>(b->iters, b->it_save, b->row_map, R, 0, 1);
-        k_move_vals<double><<<gr, 256, 0, s>>>(b->res, b->res_save, b->row_map, R, 0, 0);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(b->status, b->st_save, rp * sizeof(int), hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(b->iters, b->it_save, rp * sizeof(int), hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(b->res, b->res_save, rp * sizeof(double), hipMemcpyDeviceToDevice, s));
-        k_count_tiles<<<b->T, kLanes, 0, s>>>(b->status, b->rows_pad, b->tile_active);
-        HIPCHK(hipGetLastError());
+    b->restart_on = true;
+    b->restart_max_iter = max_iter;
+    int it = 0;
+    try {
+        // a row fails by update k_warm at the latest and then has max_iter more
+        it = nr_loop<Real>(b, k_warm + max_iter, tol, 1, growth, tm, rows_factored);
+    } catch (...) {
+        b->restart_on = false;
+        throw;
     }
+    b->restart_on = false;
     mask_islanded_v(b);
     const int e_end = tm.mark();
+    int R = 0;
+    HIPCHK(hipMemcpyAsync(&R, b->requeued, sizeof(int), hipMemcpyDeviceToHost, s));
     check_stream_error(b);
     fill_stats(b, tm, e_begin, e_end, it, rows_factored, stats);
     if (stats) stats->n_requeued = R;
@@ -1133,7 +1140,7 @@ int g2g_batch_create(g2g_plan_t P, int32_t rows, int32_t precision, int32_t per_
         b->n_active = dalloc<int>(1);
         b->error = dalloc<int>(1);
         HIPCHK(hipMemsetAsync(b->error, 0, sizeof(int), b->stream));
-        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->h_active), sizeof(int), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->h_active), 2 * sizeof(int), hipHostMallocDefault));
         k_fill_int<<<blocks_for(rp, 256), 256, 0, b->stream>>>(b->status, kNotSimulated, rp);
         k_fill_int<<<blocks_for(rp, 256), 256, 0, b->stream>>>(b->iters, 0, rp);
         HIPCHK(hipGetLastError());

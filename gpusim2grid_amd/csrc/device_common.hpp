@@ -32,6 +32,9 @@ struct MismatchArgs {
     const int* it0;         // nullable: per row, the iteration its solve started at (streamed rows)
     const unsigned char* mask;  // nullable: [T][n][64], 1 = bus islanded in this row (its equations
                                 // become identity rows: F = 0, J row = unit diagonal)
+    const int* budget;          // nullable: per row, its update budget (else max_iter)
+    const unsigned char* warm;  // nullable: per row, 1 = still on its learned start (a failure
+                                // marks it for a restart instead of ending it)
 };
 
 // limit-violation check of solved rows (one wavefront per row)

@@ -246,16 +246,21 @@ __global__ __launch_bounds__(kLanes* kMismatchWaves) void k_mismatch(MismatchArg
         a.res[row] = r;
         if (WithJ) a.jmax[row] = jm;
         const int it = a.it0 ? a.iter - a.it0[row] : a.iter;  // updates this row has had
+        const int budget = a.budget ? a.budget[row] : a.max_iter;
+        // a row on its learned start that fails (non-finite, growing, or out
+        // of its budget) is marked for a restart rather than ended
+        const bool wrow = a.warm && a.warm[row];
+        const bool growth_on = a.growth > 0.0 && (!a.warm || wrow);
         int st = kActive;
-        if (!isfinite(r)) st = kDiverged;
+        if (!isfinite(r)) st = wrow ? kRejected : kDiverged;
         else if (a.freeze && r < a.tol) st = kConverged;
-        else if (a.growth > 0.0 && it > 0 && r > a.growth * prev) st = kRejected;
-        else if (it >= a.max_iter) st = r < a.tol ? kConverged : kMaxIter;
+        else if (growth_on && it > 0 && r > a.growth * prev) st = kRejected;
+        else if (it >= budget) st = r < a.tol ? kConverged : (wrow ? kRejected : kMaxIter);
         if (st != kActive) {
             a.status[row] = st;
-            a.iters[row] = it;
+            if (!(wrow && st == kRejected)) a.iters[row] = it;
         }
-        still = st == kActive;
+        still = st == kActive || (wrow && st == kRejected);
     }
     // wave-level count of rows still iterating
     const unsigned long long m = __ballot(still);
@@ -1235,6 +1240,32 @@ __global__ void k_mask_v_nan(const unsigned char* __restrict__ mask, long long t
     if (gid < total && mask[gid]) V[gid] = make_double2(__builtin_nan(""), __builtin_nan(""));
 }
 
+// Rows marked for a restart (status kRejected while on their learned start)
+// take their fallback start V0 back, bus by bus ...
+__global__ void k_restart_v(const int* __restrict__ status, const unsigned char* __restrict__ warm, int T, int n,
+                            const double2* __restrict__ V0, double2* __restrict__ V)
+{
+    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= static_cast<long long>(T) * n * kLanes) return;
+    const int lane = static_cast<int>(gid & (kLanes - 1));
+    const long long tp = gid / kLanes;
+    const int row = static_cast<int>(tp / n) * kLanes + lane;
+    if (status[row] == kRejected && warm[row]) V[gid] = V0[gid];
+}
+
+// ... then continue from it in the same iteration: the fallback gets max_iter
+// updates on top of the ones the row had, and the row counts as re-queued.
+__global__ void k_restart_rows(int* __restrict__ status, unsigned char* __restrict__ warm, const int* __restrict__ it0,
+                               int* __restrict__ budget, int rows, int iter, int max_iter, int* __restrict__ requeued)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= rows || status[row] != kRejected || !warm[row]) return;
+    budget[row] = (iter - it0[row]) + max_iter;
+    warm[row] = 0;
+    status[row] = kActive;
+    atomicAdd(requeued, 1);
+}
+
 // per-slot scalars: v[slots[r]] = value
 __global__ void k_set_slots(const int* __restrict__ slots, int R, int* __restrict__ v, int value)
 {
@@ -1679,6 +1710,7 @@ template __global__ void k_rows_to_slots<double2>(const double2*, long long, int
 template __global__ void k_slots_to_rows<double2>(const double2*, int, int, const int*, double2*, long long);
 template __global__ void k_move_vals<int>(const int*, int*, const int*, int, int, int);
 template __global__ void k_move_vals<double>(const double*, double*, const int*, int, int, int);
+template __global__ void k_move_vals<unsigned char>(const unsigned char*, unsigned char*, const int*, int, int, int);
 template __global__ void k_factor<double, 1>(FactorArgs<double>);
 template __global__ void k_factor<float, 1>(FactorArgs<float>);
 template __global__ void k_factor<double, 2>(FactorArgs<double>);
