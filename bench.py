@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--max-iter", type=int, default=10)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--methods", default="fp64,mixed,warm")
-    ap.add_argument("--train-rows", type=int, default=32768, help="warm start: solved training scenarios")
+    ap.add_argument("--train-rows", type=int, default=65536, help="warm start: solved training scenarios")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--hidden", type=int, default=1024)
     ap.add_argument("--train-batch", type=int, default=512)
@@ -70,7 +70,8 @@ def parse():
     ap.add_argument("--quiet", action="store_true")
     args = ap.parse_args()
     if args.workload == "c3":  # the learned-warm-start workload's defaults
-        for k, v in (("train_rows", 200_000), ("k_warm", 4)):
+        # fallback after 4 updates or as soon as the mismatch grows
+        for k, v in (("train_rows", 200_000), ("k_warm", 4), ("growth", 1.0)):
             if getattr(args, k) == ap.get_default(k):
                 setattr(args, k, v)
     return args
@@ -149,11 +150,22 @@ def train_warm(args, wl, torch, capi, syn):
     t1 = time.perf_counter()
     hist = ws.train(S, V, mask=mask, epochs=args.epochs, batch=args.train_batch, lr=args.lr,
                     log=(lambda m: log(args, m)))
+    lr = args.lr
+    if not hist[-1] < 0.5 * hist[0]:
+        # the run collapsed to predicting the mean (Adam at this step size can
+        # blow up on long schedules): train again, from a fresh network, a
+        # third of the step size
+        lr = args.lr / 3
+        log(args, f"warm-start training did not converge (loss {hist[0]:.3f} -> {hist[-1]:.3f}): lr {lr:g}")
+        ws = WarmStart(g.n_bus, g.pv, g.pq, g.slack, np.abs(g.flat_start()), n_hidden=args.hidden, seed=3,
+                       n_mask=0 if mask is None else g.n_branch)
+        hist = ws.train(S, V, mask=mask, epochs=args.epochs, batch=args.train_batch, lr=lr,
+                        log=(lambda m: log(args, m)))
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t1
     n_train = int(S.shape[0])
     del S, V
-    return ws, dict(train_rows=n_train, epochs=args.epochs, hidden=args.hidden, loss_first=hist[0],
+    return ws, dict(train_rows=n_train, epochs=args.epochs, hidden=args.hidden, lr=lr, loss_first=hist[0],
                     loss_last=hist[-1], data_s=t_data, train_s=t_train)
 
 
