@@ -72,6 +72,16 @@ void mlp_warm_start_tiles(g2g_mlp_s* p, const double2* S_tiles, int M, int n, co
 using namespace g2g;
 
 
+// a segment table as the kernels read it: 9 ints per walker
+inline std::vector<int> seg_words(const std::vector<g2g::SymbolicLU::Seg>& segs)
+{
+    std::vector<int> w;
+    for (const auto& g : segs)
+        w.insert(w.end(), {g.prog_f_off, g.prog_f_len, g.prog_b_off, g.prog_b_len, g.a_off, g.a_len, g.u_off, g.u_len,
+                           g.n_piv});
+    return w;
+}
+
 struct g2g_plan_s {
     bool grid = false;
     NrPlan nr;           // grid plans
@@ -81,6 +91,7 @@ struct g2g_plan_s {
     int* d_prog_f = nullptr;
     int* d_prog_b = nullptr;
     int* d_seg = nullptr;      // per walker segments (SymbolicLU::Seg, 9 ints each)
+    int* d_bseg = nullptr;     // per backward walker segments
     int* d_k1_bus = nullptr;
     int* d_k1_ent = nullptr;
     double2* d_Y = nullptr;
@@ -92,9 +103,9 @@ struct g2g_plan_s {
     int* d_k1_ent_t = nullptr;  // k1_ent with the A positions of the transposed entries (adjoint)
     std::vector<int> raw_p, raw_i;  // a raw plan's matrix pattern (CSR)
     std::vector<int> raw_order;     // and its pivot order (empty: minimum degree)
-    // the one-walker variant, built when a precision's walker kernels do not
-    // fit the LDS
-    std::unique_ptr<g2g_plan_s> single;
+    // variants with fewer walkers, built when a precision's walker kernels do
+    // not fit the LDS
+    std::vector<std::unique_ptr<g2g_plan_s>> fitted;
     int* d_Jp = nullptr;      // the matrix CSR in unknown labels (refinement residuals)
     int* d_Ji = nullptr;
     int prog_f_stride = 0, prog_b_stride = 0;
@@ -102,7 +113,7 @@ struct g2g_plan_s {
     const SymbolicLU& lu() const { return grid ? nr.lu : raw; }
     ~g2g_plan_s()
     {
-        dfree(d_prog_f); dfree(d_prog_b); dfree(d_seg); dfree(d_k1_bus); dfree(d_k1_ent);
+        dfree(d_prog_f); dfree(d_prog_b); dfree(d_seg); dfree(d_bseg); dfree(d_k1_bus); dfree(d_k1_ent);
         dfree(d_Y); dfree(d_perm); dfree(d_apos); dfree(d_fpos); dfree(d_upd); dfree(d_Jp); dfree(d_Ji);
         dfree(d_k1_ent_t);
     }
@@ -151,13 +162,8 @@ struct g2g_plan_s {
         };
         d_prog_f = dupload(padded(s.prog_f, prog_f_stride));
         d_prog_b = dupload(padded(s.prog_b, prog_b_stride));
-        {
-            std::vector<int> sg;
-            for (const auto& g : s.seg)
-                sg.insert(sg.end(), {g.prog_f_off, g.prog_f_len, g.prog_b_off, g.prog_b_len, g.a_off, g.a_len,
-                                     g.u_off, g.u_len, g.n_piv});
-            d_seg = dupload(sg);
-        }
+        d_seg = dupload(seg_words(s.seg));
+        d_bseg = dupload(seg_words(s.bseg));
         d_perm = dupload(s.perm);
         d_apos = dupload(s.a_pos_of_entry);
         d_fpos = dupload(s.f_pos);
@@ -267,6 +273,20 @@ constexpr int chunk_of() { return sizeof(Real) == sizeof(double) ? kChunkF64 : k
 
 inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
+// U positions one tile holds: the stream in whole chunks, reaching far enough
+// that every producer, which streams its segment in whole chunks from the
+// segment's own start (a backward set need not start on a chunk boundary),
+// stays inside the tile's range; the last tile's reads then stay inside the
+// buffer
+int u_stride_of(const SymbolicLU& s, int real_size)
+{
+    const int ch = real_size == 8 ? kChunkF64 : kChunkF32;
+    int end = s.u_len;
+    for (const auto& g : s.seg) end = std::max(end, g.u_off + round_up(g.u_len, value_chunk(real_size, s.walkers)));
+    for (const auto& g : s.bseg) end = std::max(end, g.u_off + round_up(g.u_len, value_chunk(real_size, s.bwalkers)));
+    return round_up(end, ch);
+}
+
 template <typename Real>
 constexpr int ring_slots_of() { return sizeof(Real) == sizeof(double) ? kRingSlotsF64 : kRingSlotsF32; }
 
@@ -284,7 +304,7 @@ size_t factor_lds(const SymbolicLU& s)
 template <typename Real>
 size_t backward_lds(const SymbolicLU& s)
 {
-    return static_cast<size_t>(walk_ring_bytes(sizeof(Real), s.walkers)) +
+    return static_cast<size_t>(walk_ring_bytes(sizeof(Real), s.bwalkers)) +
            static_cast<size_t>(s.slots_b + 2) * kLanes * sizeof(Real);
 }
 
@@ -301,33 +321,46 @@ size_t lds_limit()
 // its one-walker variant (built once, owned by P).
 g2g_plan_s* plan_fitting(g2g_plan_s* P, bool fp64)
 {
-    const SymbolicLU& s = P->lu();
-    if (s.walkers == 1) return P;
-    const size_t need = fp64 ? std::max(factor_lds<double>(s), backward_lds<double>(s))
-                             : std::max(factor_lds<float>(s), backward_lds<float>(s));
-    if (need <= lds_limit()) return P;
-    if (!P->single) {
-        auto q = std::make_unique<g2g_plan_s>();
-        q->grid = P->grid;
-        q->nnz = P->nnz;
-        if (P->grid) {
-            const NrPlan& nr = P->nr;
-            std::vector<int> pv;
-            std::vector<char> is_pq(static_cast<size_t>(nr.n), 0);
-            for (int b : nr.pq) is_pq[b] = 1;
-            for (int b : nr.pvpq)
-                if (!is_pq[b]) pv.push_back(b);
-            q->nr = build_nr_plan(nr.n, nr.Yp.data(), nr.Yi.data(), nr.Yx.empty() ? nullptr : nr.Yx.data(),
-                                  static_cast<int>(pv.size()), pv.data(), nr.npq, nr.pq.data(), nr.slack, 1);
-        } else {
-            q->raw = build_symbolic(P->raw.d, P->raw_p.data(), P->raw_i.data(), P->raw_order, 1);
-            q->raw_p = P->raw_p;
-            q->raw_i = P->raw_i;
-            q->raw_order = P->raw_order;
+    const size_t limit = lds_limit();
+    auto fits = [&](const SymbolicLU& s) {
+        const size_t f = fp64 ? factor_lds<double>(s) : factor_lds<float>(s);
+        const size_t b = fp64 ? backward_lds<double>(s) : backward_lds<float>(s);
+        return f <= limit && b <= limit;
+    };
+    if (fits(P->lu())) return P;
+    // the same pattern with fewer walkers: one factorization walker (the
+    // walker split duplicates the slot working set), then one for both
+    const int wb = P->lu().bwalkers;
+    for (const auto& [wf, wbk] : {std::pair<int, int>{1, wb}, std::pair<int, int>{1, 1}}) {
+        g2g_plan_s* q = nullptr;
+        for (const auto& c : P->fitted)
+            if (c->lu().walkers == wf && c->lu().bwalkers == wbk) q = c.get();
+        if (!q) {
+            auto c = std::make_unique<g2g_plan_s>();
+            c->grid = P->grid;
+            c->nnz = P->nnz;
+            if (P->grid) {
+                const NrPlan& nr = P->nr;
+                std::vector<int> pv;
+                std::vector<char> is_pq(static_cast<size_t>(nr.n), 0);
+                for (int b : nr.pq) is_pq[b] = 1;
+                for (int b : nr.pvpq)
+                    if (!is_pq[b]) pv.push_back(b);
+                c->nr = build_nr_plan(nr.n, nr.Yp.data(), nr.Yi.data(), nr.Yx.empty() ? nullptr : nr.Yx.data(),
+                                      static_cast<int>(pv.size()), pv.data(), nr.npq, nr.pq.data(), nr.slack, wf,
+                                      wbk);
+            } else {
+                c->raw = build_symbolic(P->raw.d, P->raw_p.data(), P->raw_i.data(), P->raw_order, wf, wbk);
+                c->raw_p = P->raw_p;
+                c->raw_i = P->raw_i;
+                c->raw_order = P->raw_order;
+            }
+            P->fitted.push_back(std::move(c));
+            q = P->fitted.back().get();
         }
-        P->single = std::move(q);
+        if (fits(q->lu())) return q;
     }
-    return P->single.get();
+    return P->fitted.back().get();  // one walker each: the launch reports what does not fit
 }
 
 template <typename Real>
@@ -355,11 +388,13 @@ void launch_backward(const g2g_plan_s* P, const Real* U, int u_stride, const int
     const size_t lds = backward_lds<Real>(s);
     if (lds > lds_limit()) throw std::runtime_error("backward substitution needs too much LDS");
     BackwardArgs<Real> a{U, P->d_prog_b, tile_active, X, error, s.d, s.u_len, u_stride, P->prog_b_stride, rows,
-                         s.max_c, P->d_seg, s.n_top};
-    const void* fn = s.walkers == 2 ? reinterpret_cast<const void*>(&k_backward<Real, 2>)
-                                    : reinterpret_cast<const void*>(&k_backward<Real, 1>);
+                         s.max_c, P->d_bseg, s.n_top};
+    const void* fn = s.bwalkers == 4   ? reinterpret_cast<const void*>(&k_backward<Real, 4>)
+                     : s.bwalkers == 2 ? reinterpret_cast<const void*>(&k_backward<Real, 2>)
+                                       : reinterpret_cast<const void*>(&k_backward<Real, 1>);
     HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    if (s.walkers == 2) k_backward<Real, 2><<<T, kLanes * walk_waves(2), lds, st>>>(a);
+    if (s.bwalkers == 4) k_backward<Real, 4><<<T, kLanes * walk_waves(4), lds, st>>>(a);
+    else if (s.bwalkers == 2) k_backward<Real, 2><<<T, kLanes * walk_waves(2), lds, st>>>(a);
     else k_backward<Real, 1><<<T, kLanes * walk_waves(1), lds, st>>>(a);
     HIPCHK(hipGetLastError());
 }
@@ -858,7 +893,7 @@ void raw_solve(g2g_plan_s* P, int rows, const double* vals, const double* rhs, d
     const int T = (rows + kLanes - 1) / kLanes;
     const size_t rp = static_cast<size_t>(T) * kLanes;
     const int nnz = P->nnz, d = s.d;
-    const int a_stride = round_up(s.a_len, chunk_of<Real>()), u_stride = round_up(s.u_len, chunk_of<Real>());
+    const int a_stride = round_up(s.a_len, chunk_of<Real>()), u_stride = u_stride_of(s, sizeof(Real));
     DeviceScratch B;
     double* d_vals = B.get<double>(static_cast<size_t>(rows) * nnz);
     double* d_rhs = B.get<double>(static_cast<size_t>(rows) * d);
@@ -1003,7 +1038,7 @@ int g2g_lu_plan_create(int32_t d, const int32_t* Ap, const int32_t* Ai, const in
         p->grid = false;
         std::vector<int> ord;
         if (order) ord.assign(order, order + d);
-        p->raw = build_symbolic(d, Ap, Ai, ord, default_walkers(d));
+        p->raw = build_symbolic(d, Ap, Ai, ord, default_walkers(d), default_bwalkers(d));
         p->raw_order = ord;
         p->nnz = Ap[d];
         p->raw_p.assign(Ap, Ap + d + 1);
@@ -1035,6 +1070,7 @@ int g2g_plan_get_info(g2g_plan_t P, g2g_plan_info* info)
         info->n_updates = s.n_updates;
         info->n_steps = s.n_steps;
         info->walkers = s.walkers;
+        info->bwalkers = s.bwalkers;
         info->n_top = s.n_top;
         info->max_c = s.max_c;
         info->height = s.height;
@@ -1042,6 +1078,8 @@ int g2g_plan_get_info(g2g_plan_t P, g2g_plan_info* info)
         info->slots_b = s.slots_b;
         info->a_len = s.a_len;
         info->u_len = s.u_len;
+        info->u_stride_f64 = u_stride_of(s, 8);
+        info->u_stride_f32 = u_stride_of(s, 4);
     });
 }
 
@@ -1070,9 +1108,11 @@ int g2g_plan_get_array(g2g_plan_t P, int32_t which, int32_t* out, int64_t capaci
         case G2G_ARR_FPOS: v = &s.f_pos; break;
         case G2G_ARR_UOFF: v = &s.u_off; break;
         case G2G_ARR_SEGS:
-            for (const auto& g : s.seg)
-                tmp.insert(tmp.end(), {g.prog_f_off, g.prog_f_len, g.prog_b_off, g.prog_b_len, g.a_off, g.a_len,
-                                       g.u_off, g.u_len, g.n_piv});
+            tmp = seg_words(s.seg);
+            v = &tmp;
+            break;
+        case G2G_ARR_BSEGS:
+            tmp = seg_words(s.bseg);
             v = &tmp;
             break;
         default: throw std::invalid_argument("unknown array id");
@@ -1126,7 +1166,7 @@ int g2g_batch_create(g2g_plan_t P, int32_t rows, int32_t precision, int32_t per_
         }
         const int ch = precision == G2G_MIXED ? kChunkF32 : kChunkF64;
         b->a_stride = round_up(nr.lu.a_len, ch);
-        b->u_stride = round_up(nr.lu.u_len, ch);
+        b->u_stride = u_stride_of(nr.lu, static_cast<int>(b->real_size()));
         b->A = dalloc<unsigned char>(rp * b->a_stride * b->real_size());
         b->U = dalloc<unsigned char>(rp * b->u_stride * b->real_size());
         b->X = dalloc<unsigned char>(rp * nr.d * b->real_size());

@@ -668,7 +668,7 @@ __device__ __forceinline__ bool head_ok(int h, int max_c) { return h == kMergeHe
 template <typename Real, int W>
 __global__ __launch_bounds__(kLanes * walk_waves(W)) void k_factor(FactorArgs<Real> a)
 {
-    constexpr int kCh = value_chunk(sizeof(Real)), kNch = value_ring_slots(sizeof(Real), W);
+    constexpr int kCh = value_chunk(sizeof(Real), W), kNch = value_ring_slots(sizeof(Real), W);
     using ARing = Ring<Real, kCh, kNch, false, true>;
     constexpr int kNP = value_producers(W), kP = kNP / W;
     static_assert(kP * W == kNP, "producers split evenly");
@@ -912,7 +912,7 @@ __device__ __forceinline__ int backward_dispatch(int c, UR& U, ProgReader& P, Ba
 template <typename Real, int W>
 __global__ __launch_bounds__(kLanes * walk_waves(W)) void k_backward(BackwardArgs<Real> a)
 {
-    constexpr int kCh = value_chunk(sizeof(Real)), kNch = value_ring_slots(sizeof(Real), W);
+    constexpr int kCh = value_chunk(sizeof(Real), W), kNch = value_ring_slots(sizeof(Real), W);
     using URing = Ring<Real, kCh, kNch, true, true>;
     constexpr int kNP = value_producers(W), kP = kNP / W;
     static_assert(kP * W == kNP, "producers split evenly");
@@ -1719,6 +1719,8 @@ template __global__ void k_backward<double, 1>(BackwardArgs<double>);
 template __global__ void k_backward<float, 1>(BackwardArgs<float>);
 template __global__ void k_backward<double, 2>(BackwardArgs<double>);
 template __global__ void k_backward<float, 2>(BackwardArgs<float>);
+template __global__ void k_backward<double, 4>(BackwardArgs<double>);
+template __global__ void k_backward<float, 4>(BackwardArgs<float>);
 template __global__ void k_update<double>(UpdateArgs<double>);
 template __global__ void k_update<float>(UpdateArgs<float>);
 template __global__ void k_lin_residual<double>(RefineArgs<double>);

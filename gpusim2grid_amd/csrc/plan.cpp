@@ -77,9 +77,18 @@ int default_walkers(int d)
 {
     if (const char* e = std::getenv("G2G_WALKERS")) {
         const int w = std::atoi(e);
-        if (w >= 1 && w <= kMaxWalkers) return w;
+        if (w >= 1 && w <= kMaxFactorWalkers) return w;
     }
     return d >= kWalkersMinD ? 2 : 1;
+}
+
+int default_bwalkers(int d)
+{
+    if (const char* e = std::getenv("G2G_BWALKERS")) {
+        const int w = std::atoi(e);
+        if (w == 1 || w == 2 || w == 4) return w;
+    }
+    return d >= kWalkersMinD ? 4 : 1;
 }
 
 std::vector<int> min_degree_order(int n, const std::vector<int>& adjp, const std::vector<int>& adji,
@@ -334,7 +343,7 @@ void build_backward(SymbolicLU& s, const std::vector<int>& group, int W);
 
 }  // namespace
 
-SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> order, int walkers)
+SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> order, int walkers, int bwalkers)
 {
     if (d <= 0) throw std::invalid_argument("matrix dimension must be positive");
     SymbolicLU s;
@@ -467,8 +476,16 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
     }
     s.n_steps = static_cast<int>(step_first.size());
     s.u_off.assign(static_cast<size_t>(d) + 1, 0);
-    const int W = std::max(1, std::min(walkers, kMaxWalkers));
-    const std::vector<int> group = partition_steps(s, parent, step_first, W);
+    const int W = std::max(1, std::min(walkers, kMaxFactorWalkers));
+    // G subtree sets (the backward walkers), G / W per factorization walker;
+    // group G is the top part
+    int G = bwalkers > 0 ? std::min(bwalkers, kMaxWalkers) : W;
+    if (G < W || G % W != 0) G = W;
+    const std::vector<int> group = partition_steps(s, parent, step_first, G);
+    const int gpw = G / W;
+    // U-stream range and pivots of every set (and of the top part)
+    std::vector<int> gu_off(static_cast<size_t>(G) + 1, 0), gu_end(static_cast<size_t>(G) + 1, 0),
+        gpiv(static_cast<size_t>(G) + 1, 0);
 
     std::vector<std::unique_ptr<Emitter>> em;
     for (int w = 0; w < W; ++w) em.push_back(std::make_unique<Emitter>(s, a_entry));
@@ -477,13 +494,24 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
     // aligned offset, then walker 0's (whose segment continues with the top part)
     std::vector<int> a_off(static_cast<size_t>(W), 0), u_off(static_cast<size_t>(W), 0);
     int a_cur = 0, u_cur = 0;
+    // a walker walks its sets last to first, so its set of lowest index lies
+    // right before whatever follows in its U segment (walker 0: the top part)
+    auto walk_sets = [&](Emitter& e, int w) {
+        for (int g = (w + 1) * gpw - 1; g >= w * gpw; --g) {
+            gu_off[g] = e.upos;
+            const int p0 = e.n_piv;
+            for (int k : step_first)
+                if (group[k] == g) e.step(k);
+            gu_end[g] = e.upos;
+            gpiv[g] = e.n_piv - p0;
+        }
+    };
     for (int w = 1; w < W; ++w) {
         a_off[w] = a_cur;
         u_off[w] = u_cur;
         em[w]->apos = a_cur;
         em[w]->upos = u_cur;
-        for (int k : step_first)
-            if (group[k] == w) em[w]->step(k);
+        walk_sets(*em[w], w);
         a_cur = round_up_to(em[w]->apos, kSegAlign);
         u_cur = round_up_to(em[w]->upos, kSegAlign);
     }
@@ -491,8 +519,7 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
     u_off[0] = u_cur;
     e0.apos = a_cur;
     e0.upos = u_cur;
-    for (int k : step_first)
-        if (group[k] == 0) e0.step(k);
+    walk_sets(e0, 0);
     const int n_phase1_0 = e0.n_piv;
     // walker w's slots sit after walker 0's and those of walkers 1 .. w-1
     std::vector<int> slot_base(static_cast<size_t>(W), 0);
@@ -545,9 +572,13 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
         for (int j = slot_base[1]; j < hi; ++j)
             if (!adopted.count(j)) e0.sa.put(j);
         e0.sa.hi = hi;
-        for (int k : step_first)
-            if (group[k] == W) e0.step(k);
     }
+    // the top part (walker 0, after the merge)
+    gu_off[G] = e0.upos;
+    for (int k : step_first)
+        if (group[k] == G) e0.step(k);
+    gu_end[G] = e0.upos;
+    gpiv[G] = e0.n_piv - n_phase1_0;
     int covered = 0;
     for (int w = 0; w < W; ++w) covered += em[w]->n_piv;
     if (covered != d) throw std::logic_error("symbolic: walkers do not cover every pivot");
@@ -559,7 +590,18 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
     s.u_len = e0.upos;
     s.u_off[d] = s.u_len;
     s.walkers = W;
-    s.n_top = W > 1 ? e0.n_piv - n_phase1_0 : 0;
+    s.n_top = gpiv[G];
+    s.bwalkers = G;
+    s.bseg.assign(static_cast<size_t>(G), SymbolicLU::Seg{});
+    for (int g = 0; g < G; ++g) {
+        s.bseg[g].u_off = gu_off[g];
+        s.bseg[g].u_len = gu_end[g] - gu_off[g];
+        s.bseg[g].n_piv = gpiv[g];
+    }
+    // set 0's walker also walks the top part, which follows set 0 in the U stream
+    if (gu_end[0] != gu_off[G]) throw std::logic_error("symbolic: the top part does not follow set 0");
+    s.bseg[0].u_len = gu_end[G] - gu_off[0];
+    s.bseg[0].n_piv += gpiv[G];
     s.seg.assign(static_cast<size_t>(W), SymbolicLU::Seg{});
     for (int w = 0; w < W; ++w) {
         s.seg[w].a_off = a_off[w];
@@ -577,7 +619,7 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
                         s.slots_f * kRefUnit);
         s.seg[w].prog_f_len = static_cast<int>(s.prog_f.size()) - s.seg[w].prog_f_off;
     }
-    build_backward(s, group, W);
+    build_backward(s, group, G);
     return s;
 }
 
@@ -721,7 +763,7 @@ void build_backward(SymbolicLU& s, const std::vector<int>& group, int W)
     const auto enc = [&](int slot) { return (slot < 0 ? s.slots_b + 1 : slot) * kRefUnit; };
     s.prog_b.clear();
     for (int w = 0; w < W; ++w) {
-        s.seg[w].prog_b_off = static_cast<int>(s.prog_b.size());
+        s.bseg[w].prog_b_off = static_cast<int>(s.prog_b.size());
         const auto& rv = recs[static_cast<size_t>(w)];
         for (size_t q = 0; q < rv.size(); ++q) {
             const BRec& rec = rv[q];
@@ -742,14 +784,14 @@ void build_backward(SymbolicLU& s, const std::vector<int>& group, int W)
             }
         }
         s.prog_b.resize(static_cast<size_t>(round_up_to(static_cast<int>(s.prog_b.size()), kProgChunk)), enc(-1));
-        s.seg[w].prog_b_len = static_cast<int>(s.prog_b.size()) - s.seg[w].prog_b_off;
+        s.bseg[w].prog_b_len = static_cast<int>(s.prog_b.size()) - s.bseg[w].prog_b_off;
     }
 }
 
 }  // namespace
 
 NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int npv, const int* pv,
-                     int npq, const int* pq, int slack, int walkers)
+                     int npq, const int* pq, int slack, int walkers, int bwalkers)
 {
     if (n <= 1) throw std::invalid_argument("the grid needs at least two buses");
     NrPlan P;
@@ -859,7 +901,8 @@ NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int 
     }
     // walkers: two for large systems (the kernels keep one otherwise);
     // G2G_WALKERS=1|2 overrides
-    P.lu = build_symbolic(d, P.Jp.data(), P.Ji.data(), order, walkers > 0 ? walkers : default_walkers(d));
+    P.lu = build_symbolic(d, P.Jp.data(), P.Ji.data(), order, walkers > 0 ? walkers : default_walkers(d),
+                          bwalkers > 0 ? bwalkers : default_bwalkers(d));
 
     // mismatch/Jacobian program: buses in RCM order of the Ybus graph
     {

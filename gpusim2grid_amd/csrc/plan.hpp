@@ -66,6 +66,12 @@ struct SymbolicLU {
     int walkers = 1;
     int n_top = 0;
     std::vector<Seg> seg;
+    // the backward substitution's walkers (a multiple of the factorization's:
+    // the forest is cut into bwalkers subtree sets, factorization walker w
+    // walks sets w * bwalkers / walkers ... in turn): their program and
+    // U-stream segments; walker 0's covers set 0 and then the top part
+    int bwalkers = 1;
+    std::vector<Seg> bseg;
     // backward substitution program (one record per pivot, pivots d-1 .. 0):
     //   c, xref[c] (slot of x(j) for j in struct_k), out (slot for x(k) or none),
     //   op, and for op != 0 two further ints (see below)
@@ -77,7 +83,10 @@ struct SymbolicLU {
 // symmetrised).  `order` is the pivot order (a permutation of 0..d-1) before
 // the postorder relabelling; empty = minimum degree on the pattern itself.
 // `walkers` (1 or 2) splits each tile's walk over that many walker wavefronts.
-SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> order = {}, int walkers = 1);
+// `bwalkers` (1, 2 or 4, a multiple of `walkers`; <= 0: as `walkers`) does the
+// same for the backward substitution.
+SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> order = {}, int walkers = 1,
+                          int bwalkers = 0);
 
 // Minimum degree order of an undirected graph given by symmetric adjacency
 // lists (CSR, no self loops) with vertex weights; returns the elimination order.
@@ -113,7 +122,9 @@ constexpr int kWalkersMinD = 2000;
 // two walkers when d >= kWalkersMinD, else one; G2G_WALKERS=1|2 in the
 // environment overrides
 int default_walkers(int d);
+// backward walkers: four when d >= kWalkersMinD, else one; G2G_BWALKERS=1|2|4 overrides
+int default_bwalkers(int d);
 NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int npv, const int* pv,
-                     int npq, const int* pq, int slack, int walkers = 0);
+                     int npq, const int* pq, int slack, int walkers = 0, int bwalkers = 0);
 
 }  // namespace g2g

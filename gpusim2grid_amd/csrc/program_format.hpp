@@ -67,7 +67,8 @@ __host__ __device__ constexpr int brec2_words(int c) { return round4(5 + c); }
 // (walker 0) and then the subtrees in parallel.  Each walker has its own
 // program, A-stream and U-stream segment; segments start at multiples of
 // kSegAlign positions / kProgChunk words.
-constexpr int kMaxWalkers = 2;
+constexpr int kMaxWalkers = 4;        // backward substitution
+constexpr int kMaxFactorWalkers = 2;  // factorization (its slots grow with every walker)
 constexpr int kSegAlign = 32;
 // merge record: [kMergeHead | next head << 16, n, (dst ref, src ref) x n]
 // (dst += src, src = 0); the first one also waits for the other walkers
@@ -78,20 +79,24 @@ __host__ __device__ constexpr int mrec_words(int n) { return round4(2 + 2 * n); 
 // program ring), then kWalkFlagBytes of cross-walker flags, the W value
 // rings, the W program rings, and the slots
 constexpr int kWalkerCtlBytes = 256;
-constexpr int kWalkFlagBytes = 1024;
+constexpr int kWalkFlagBytes = 1280;
 // value-stream producers per workgroup: six for one walker, two per walker
 // for two (eight waves either way, which leaves 256 VGPRs per wave)
 __host__ __device__ constexpr int value_producers(int W) { return W == 1 ? kValueProducers : 2 * W; }
 __host__ __device__ constexpr int walk_waves(int W) { return 2 * W + value_producers(W); }
-// value ring geometry of a walker: positions per chunk and ring slots
-__host__ __device__ constexpr int value_chunk(int real_size) { return real_size == 8 ? kChunkF64 : kChunkF32; }
+// value ring geometry of a walker: positions per chunk and ring slots (four
+// walkers stream half-size chunks, so their rings fit the LDS together)
+__host__ __device__ constexpr int value_chunk(int real_size, int W = 1)
+{
+    return (real_size == 8 ? kChunkF64 : kChunkF32) / (W >= 4 ? 2 : 1);
+}
 __host__ __device__ constexpr int value_ring_slots(int real_size, int W)
 {
     return W == 1 ? (real_size == 8 ? kRingSlotsF64 : kRingSlotsF32) : 4;
 }
 __host__ __device__ constexpr int walk_ring_bytes(int real_size, int W)
 {
-    return W * (kWalkerCtlBytes + value_chunk(real_size) * value_ring_slots(real_size, W) * kLanes * real_size +
+    return W * (kWalkerCtlBytes + value_chunk(real_size, W) * value_ring_slots(real_size, W) * kLanes * real_size +
                 kProgChunk * kProgSlots * 4) +
            kWalkFlagBytes;
 }

@@ -55,6 +55,7 @@ extern "C" {
 #define G2G_ARR_FPOS 10     /* A-stream position of each rhs entry [d]        */
 #define G2G_ARR_UOFF 11     /* U-stream record start per pivot [d+1]          */
 #define G2G_ARR_SEGS 12     /* per walker: prog_f off/len, prog_b off/len, A off/len, U off/len, pivots [9 walkers] */
+#define G2G_ARR_BSEGS 13    /* the same per backward-substitution walker */
 
 typedef struct g2g_plan_s *g2g_plan_t;
 typedef struct g2g_batch_s *g2g_batch_t;
@@ -67,6 +68,8 @@ typedef struct {
     int32_t slots_f, slots_b, a_len, u_len;  /* program resources                     */
     int32_t n_steps;                         /* factorization walk steps (pivots and 2-column supernodes) */
     int32_t walkers, n_top;                  /* walker wavefronts per tile; pivots of the top part */
+    int32_t bwalkers;                        /* walker wavefronts per tile of the backward substitution */
+    int32_t u_stride_f64, u_stride_f32;      /* U positions a tile holds in a batch of this precision */
 } g2g_plan_info;
 
 typedef struct {

@@ -187,7 +187,7 @@ def backward(plan, U):
     first (its segment starts with the top part), then the others."""
     info = plan.info()
     prog = np.asarray(plan.array("prog_b"), dtype=np.int64)
-    segs = _segs(plan)
+    segs = plan.array("bsegs").reshape(-1, 9)
     rows = U.shape[0]
     d = info["d"]
     S = info["slots_b"]

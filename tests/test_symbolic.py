@@ -169,29 +169,46 @@ def test_invalid_inputs_raise_value_error():
 LDS_BYTES = 160 * 1024
 
 
+def ring_bytes(W, real_size):
+    """program_format.hpp walk_ring_bytes: per walker a control area, a value
+    ring (four walkers stream half-size chunks) and a program ring, then the
+    cross-walker flags."""
+    chunk = (16 if real_size == 8 else 32) // (2 if W >= 4 else 1)
+    ring_slots = (8 if real_size == 8 else 4) if W == 1 else 4
+    return W * (256 + chunk * ring_slots * 64 * real_size + 512 * 4 * 4) + 1280
+
+
 def walk_lds(inf, real_size):
-    """The factorization kernel's LDS (program_format.hpp walk_ring_bytes + slots)."""
+    """The factorization kernel's LDS: rings, then the slots, the zero and
+    trash slots and the long-column scratch rows."""
     W = inf["walkers"]
-    chunk = 16 if real_size == 8 else 32
-    ring_slots = 8 if W == 1 else 4
-    rings = W * (256 + chunk * ring_slots * 64 * real_size + 512 * 4 * 4) + 1024
     scratch = 2 * inf["max_c"] * W if inf["max_c"] > 7 else 0
-    return rings + (inf["slots_f"] + 2 + scratch) * 64 * real_size
+    return ring_bytes(W, real_size) + (inf["slots_f"] + 2 + scratch) * 64 * real_size
+
+
+def backward_lds(inf, real_size):
+    return ring_bytes(inf["bwalkers"], real_size) + (inf["slots_b"] + 2) * 64 * real_size
 
 
 def test_slots_stay_small_on_the_large_grids(monkeypatch):
-    """The postorder keeps the live update set small: one walker fits the LDS
-    in FP64, and the 6,500-bus grid's two-walker split fits it in FP32."""
+    """The postorder keeps the live update set small: one walker over one
+    backward set fits the LDS in FP64 (the plan every grid falls back to), the
+    6,500-bus grid's two-walker split fits it in FP32, and its four backward
+    walkers fit in either precision."""
+    monkeypatch.setenv("G2G_BWALKERS", "1")
     for g in (syn.grid_c1(), syn.grid_c2()):
         monkeypatch.setenv("G2G_WALKERS", "1")
         inf = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack).info()
-        assert inf["max_c"] <= 16
-        assert walk_lds(inf, 8) <= LDS_BYTES
+        assert inf["max_c"] <= 16 and inf["bwalkers"] == 1
+        assert walk_lds(inf, 8) <= LDS_BYTES and backward_lds(inf, 8) <= LDS_BYTES
         assert inf["nLU"] < 1.2 * inf["nJ"]
     monkeypatch.setenv("G2G_WALKERS", "2")
+    monkeypatch.delenv("G2G_BWALKERS")
     g = syn.grid_c2()
     inf = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack).info()
+    assert inf["bwalkers"] == 4
     assert walk_lds(inf, 4) <= LDS_BYTES
+    assert backward_lds(inf, 4) <= LDS_BYTES and backward_lds(inf, 8) <= LDS_BYTES
 
 
 def test_two_walkers_split_the_tree_evenly(monkeypatch):
@@ -213,3 +230,43 @@ def test_two_walkers_split_the_tree_evenly(monkeypatch):
     assert seg[0, 4] + seg[0, 5] == inf["a_len"] and seg[0, 6] + seg[0, 7] == inf["u_len"]
     # program segments are whole prefetch chunks
     assert (seg[:, [0, 1, 2, 3]] % 512 == 0).all()
+
+
+def test_four_backward_walkers_split_the_walk_evenly(monkeypatch):
+    """The 6,500-bus grid's backward substitution splits into four sets within
+    5 % of each other; their U segments tile the U stream from the last set
+    down, and their program segments are whole prefetch chunks."""
+    monkeypatch.setenv("G2G_WALKERS", "2")
+    g = syn.grid_c2()
+    P = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+    inf = P.info()
+    seg = P.array("bsegs").reshape(-1, 9)
+    assert inf["bwalkers"] == 4 and seg.shape[0] == 4
+    d = inf["d"]
+    assert seg[:, 8].sum() == d
+    assert seg[:, 8].max() - seg[:, 8].min() < 0.05 * d / 4
+    order = np.argsort(seg[:, 6])
+    starts, lens = seg[order, 6], seg[order, 7]
+    assert starts[0] == 0 and (starts[1:] >= starts[:-1] + lens[:-1]).all()
+    assert starts[-1] + lens[-1] == inf["u_len"]
+    assert (seg[:, [2, 3]] % 512 == 0).all()
+
+
+@pytest.mark.parametrize("wf,wb", [(1, 1), (1, 2), (1, 4), (2, 2), (2, 4)])
+def test_u_stride_covers_every_whole_chunk_read(monkeypatch, wf, wb):
+    """Each walker's U producer streams its segment in whole chunks from the
+    segment's start, which for a backward set inside one factorization walker
+    is not chunk-aligned: a batch's U stride reaches past every such read, so
+    the last tile never reads beyond its buffer."""
+    monkeypatch.setenv("G2G_WALKERS", str(wf))
+    monkeypatch.setenv("G2G_BWALKERS", str(wb))
+    for g in (syn.grid_c1(), syn.grid_c2()):
+        P = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+        inf = P.info()
+        assert (inf["walkers"], inf["bwalkers"]) == (wf, wb)
+        for real, chunk, stride in ((8, 16, inf["u_stride_f64"]), (4, 32, inf["u_stride_f32"])):
+            assert stride % chunk == 0 and stride >= inf["u_len"]
+            for name, W in (("segs", wf), ("bsegs", wb)):
+                kch = chunk // (2 if W >= 4 else 1)
+                for s in P.array(name).reshape(-1, 9):
+                    assert s[6] + -(-s[7] // kch) * kch <= stride
