@@ -19,6 +19,8 @@ namespace g2g {
 // kernels (kernels.hip)
 template <typename Real, bool PerRowY, bool WithJ>
 __global__ void k_mismatch(MismatchArgs<Real> a);
+template <typename Real, bool WithJ>
+__global__ void k_mis_status(MismatchArgs<Real> a);
 template <typename Real, int W>
 __global__ void k_factor(FactorArgs<Real> a);
 template <typename Real, int W>
@@ -203,6 +205,10 @@ struct g2g_batch_s {
     int a_stride = 0, u_stride = 0;
     double* res = nullptr;
     double* jmax = nullptr;
+    // workgroups per tile of the mismatch and update kernels, and the
+    // mismatch's per-split residual partials [2][mis_splits][rows_pad]
+    int mis_splits = 1;
+    double* mis_part = nullptr;
     int* status = nullptr;
     int* iters = nullptr;
     int* tile_active = nullptr;
@@ -253,7 +259,7 @@ struct g2g_batch_s {
     std::vector<hipEvent_t> ev;
     ~g2g_batch_s()
     {
-        dfree(V); dfree(S); dfree(Yrow); dfree(A); dfree(U); dfree(X); dfree(res); dfree(jmax);
+        dfree(V); dfree(S); dfree(Yrow); dfree(A); dfree(U); dfree(X); dfree(res); dfree(jmax); dfree(mis_part);
         dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped); dfree(error);
         dfree(V0); dfree(Bs); dfree(Xs);
         dfree(Vk); dfree(Sk); dfree(Yk); dfree(stk); dfree(itk); dfree(resk); dfree(mapk); dfree(it0);
@@ -421,13 +427,27 @@ void refine(const g2g_plan_s* P, Real* A, Real* U, Real* X, Real* Bs, Real* Xs, 
     }
 }
 
+// Workgroups per tile of the mismatch and update kernels (G2G_MIS_SPLITS:
+// 1, 2 or 4; default 4).  A tile's kernels are latency-bound chains over its
+// buses, so splitting the buses over more workgroups shortens a launch on a
+// full batch (two workgroups resident per CU) and on a batch's last few tiles.
+int default_mis_splits()
+{
+    if (const char* e = std::getenv("G2G_MIS_SPLITS")) {
+        const int v = std::atoi(e);
+        if (v == 1 || v == 2 || v == 4) return v;
+    }
+    return 4;
+}
+
 template <typename Real>
 void launch_update(g2g_batch_s* b)
 {
     const g2g_plan_s* P = b->plan;
+    const int splits = b->max_dva > 0.0 ? 1 : b->mis_splits;  // step scaling: the row's maxima in one workgroup
     UpdateArgs<Real> a{static_cast<const Real*>(b->X), P->d_upd, P->n_upd, b->tile_active, b->status, b->V,
-                       P->nr.d, P->nr.n, b->active_rows, b->max_dva, b->max_dvm};
-    k_update<Real><<<b->T, kLanes * kMisWaves, 0, b->stream>>>(a);
+                       P->nr.d, P->nr.n, b->active_rows, b->max_dva, b->max_dvm, splits};
+    k_update<Real><<<b->T * splits, kLanes * kMisWaves, 0, b->stream>>>(a);
     HIPCHK(hipGetLastError());
 }
 
@@ -440,8 +460,9 @@ void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double
                          static_cast<Real*>(b->A), P.n, P.nY, b->a_stride, b->res, b->jmax, b->status, b->iters,
                          b->tile_active, b->n_active, b->active_rows, iter, max_iter, tol, freeze, growth,
                          (b->streaming || b->restart_on) ? b->it0 : nullptr, b->mask,
-                         b->restart_on ? b->budget : nullptr, b->restart_on ? b->warm : nullptr};
-    const dim3 grid(b->T), block(kLanes * kMismatchWaves);
+                         b->restart_on ? b->budget : nullptr, b->restart_on ? b->warm : nullptr,
+                         b->mis_splits, b->mis_part, b->rows_pad};
+    const dim3 grid(b->T * b->mis_splits), block(kLanes * kMismatchWaves);
     if (b->per_row_y) {
         if (with_j) k_mismatch<Real, true, true><<<grid, block, 0, b->stream>>>(a);
         else k_mismatch<Real, true, false><<<grid, block, 0, b->stream>>>(a);
@@ -450,6 +471,11 @@ void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double
         else k_mismatch<Real, false, false><<<grid, block, 0, b->stream>>>(a);
     }
     HIPCHK(hipGetLastError());
+    if (b->mis_splits > 1) {
+        if (with_j) k_mis_status<Real, true><<<b->T, kLanes, 0, b->stream>>>(a);
+        else k_mis_status<Real, false><<<b->T, kLanes, 0, b->stream>>>(a);
+        HIPCHK(hipGetLastError());
+    }
 }
 
 void reset_rows(g2g_batch_s* b)
@@ -1173,6 +1199,8 @@ int g2g_batch_create(g2g_plan_t P, int32_t rows, int32_t precision, int32_t per_
         HIPCHK(hipMemsetAsync(b->A, 0, rp * b->a_stride * b->real_size(), b->stream));
         b->res = dalloc<double>(rp);
         b->jmax = dalloc<double>(rp);
+        b->mis_splits = default_mis_splits();
+        if (b->mis_splits > 1) b->mis_part = dalloc<double>(2 * static_cast<size_t>(b->mis_splits) * rp);
         b->status = dalloc<int>(rp);
         b->iters = dalloc<int>(rp);
         b->clamped = dalloc<int>(rp);

@@ -35,6 +35,11 @@ struct MismatchArgs {
     const int* budget;          // nullable: per row, its update budget (else max_iter)
     const unsigned char* warm;  // nullable: per row, 1 = still on its learned start (a failure
                                 // marks it for a restart instead of ending it)
+    int splits;                 // workgroups per tile (bus ranges); <= 1: one, which also
+                                // applies the status rule
+    double* part;               // splits > 1: per split, ||F||inf then max |J| partials
+                                // [2][splits][part_stride]
+    int part_stride;            // rows_pad
 };
 
 // limit-violation check of solved rows (one wavefront per row)
@@ -108,6 +113,7 @@ struct UpdateArgs {
     double2* V;
     int d, n, rows;
     double max_dva, max_dvm;  // step scaling bounds; max_dva <= 0: off
+    int splits;               // workgroups per tile (bus ranges); > 1 only with step scaling off
 };
 
 template <typename Real>

@@ -140,19 +140,66 @@ __device__ __forceinline__ void mis_gather(const MismatchArgs<Real>& a, size_t t
     }
 }
 
+// The status rule of one row from its ||F||inf (r) and max |J| (jm), run by
+// one wavefront per tile; ends with the tile's count of rows still iterating.
+template <typename Real, bool WithJ>
+__device__ __forceinline__ void mis_status(const MismatchArgs<Real>& a, size_t tile, int lane, double r, double jm)
+{
+    const int row = static_cast<int>(tile) * kLanes + lane;
+    int still = 0;
+    if (a.status[row] == kActive && row < a.rows) {
+        const double prev = a.res[row];
+        a.res[row] = r;
+        if (WithJ) a.jmax[row] = jm;
+        const int it = a.it0 ? a.iter - a.it0[row] : a.iter;  // updates this row has had
+        const int budget = a.budget ? a.budget[row] : a.max_iter;
+        // a row on its learned start that fails (non-finite, growing, or out
+        // of its budget) is marked for a restart rather than ended
+        const bool wrow = a.warm && a.warm[row];
+        const bool growth_on = a.growth > 0.0 && (!a.warm || wrow);
+        int st = kActive;
+        if (!isfinite(r)) st = wrow ? kRejected : kDiverged;
+        else if (a.freeze && r < a.tol) st = kConverged;
+        else if (growth_on && it > 0 && r > a.growth * prev) st = kRejected;
+        else if (it >= budget) st = r < a.tol ? kConverged : (wrow ? kRejected : kMaxIter);
+        if (st != kActive) {
+            a.status[row] = st;
+            if (!(wrow && st == kRejected)) a.iters[row] = it;
+        }
+        still = st == kActive || (wrow && st == kRejected);
+    }
+    // wave-level count of rows still iterating
+    const unsigned long long m = __ballot(still);
+    if (lane == 0) {
+        const int cnt = __popcll(m);
+        a.tile_active[tile] = cnt;
+        if (cnt) atomicAdd(a.n_active, cnt);
+    }
+}
+
+// NaN-sticky maximum of residual partials (a NaN anywhere makes the row's
+// residual NaN, so the status rule sees the non-finite step)
+__device__ __forceinline__ double res_max(double r, double v)
+{
+    return (isnan(r) || isnan(v)) ? __builtin_nan("") : fmax(r, v);
+}
+
 template <typename Real, bool PerRowY, bool WithJ>
 __global__ __launch_bounds__(kLanes* kMismatchWaves) void k_mismatch(MismatchArgs<Real> a)
 {
     const int lane = threadIdx.x & (kLanes - 1);
     // wave-uniform, so the bus and entry tables are read with scalar loads
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kLanes));
-    const size_t tile = blockIdx.x;
-    const int row = static_cast<int>(tile) * kLanes + lane;
+    // splits workgroups per tile, split-major: split s of tile t is workgroup
+    // s * T + t and takes every splits-th group of kMismatchWaves buses
+    const int splits = a.splits > 1 ? a.splits : 1;
+    const int T = static_cast<int>(gridDim.x) / splits;
+    const int split = static_cast<int>(blockIdx.x) / T;
+    const size_t tile = blockIdx.x - static_cast<size_t>(split) * T;
+    const int stride = kMismatchWaves * splits;
     __shared__ double s_res[kMismatchWaves][kLanes];
     __shared__ double s_jmax[kMismatchWaves][kLanes];
 
-    const int st0 = a.status[row];
-    const bool active = st0 == kActive;
     double rmax = 0.0, jmax = 0.0;
     if (a.tile_active[tile] > 0) {
         // The waves take the buses round-robin in the program's (RCM) order,
@@ -161,11 +208,11 @@ __global__ __launch_bounds__(kLanes* kMismatchWaves) void k_mismatch(MismatchArg
         // kMisBuses buses at once: every load of their first kMisBatch
         // entries is issued before any is used, which keeps that many
         // gathers in flight per wave instead of one bus's dependent chain.
-        for (int bb0 = wave; bb0 < a.nbus; bb0 += kMismatchWaves * kMisBuses) {
+        for (int bb0 = split * kMismatchWaves + wave; bb0 < a.nbus; bb0 += stride * kMisBuses) {
             MisBus b[kMisBuses];
 #pragma unroll
             for (int g = 0; g < kMisBuses; ++g) {
-                const int bb = bb0 + g * kMismatchWaves;
+                const int bb = bb0 + g * stride;
                 b[g].ok = bb < a.nbus;
                 const int* bus = a.k1_bus + 6 * (b[g].ok ? bb : a.nbus - 1);
                 b[g].i = bus[0];
@@ -236,39 +283,34 @@ __global__ __launch_bounds__(kLanes* kMismatchWaves) void k_mismatch(MismatchArg
     if (a.tile_active[tile] <= 0) return;
     double r = 0.0, jm = 0.0;
     for (int w = 0; w < kMismatchWaves; ++w) {
-        const double v = s_res[w][lane];
-        r = (isnan(r) || isnan(v)) ? __builtin_nan("") : fmax(r, v);
+        r = res_max(r, s_res[w][lane]);
         jm = fmax(jm, s_jmax[w][lane]);
     }
-    int still = 0;
-    if (active && row < a.rows) {
-        const double prev = a.res[row];
-        a.res[row] = r;
-        if (WithJ) a.jmax[row] = jm;
-        const int it = a.it0 ? a.iter - a.it0[row] : a.iter;  // updates this row has had
-        const int budget = a.budget ? a.budget[row] : a.max_iter;
-        // a row on its learned start that fails (non-finite, growing, or out
-        // of its budget) is marked for a restart rather than ended
-        const bool wrow = a.warm && a.warm[row];
-        const bool growth_on = a.growth > 0.0 && (!a.warm || wrow);
-        int st = kActive;
-        if (!isfinite(r)) st = wrow ? kRejected : kDiverged;
-        else if (a.freeze && r < a.tol) st = kConverged;
-        else if (growth_on && it > 0 && r > a.growth * prev) st = kRejected;
-        else if (it >= budget) st = r < a.tol ? kConverged : (wrow ? kRejected : kMaxIter);
-        if (st != kActive) {
-            a.status[row] = st;
-            if (!(wrow && st == kRejected)) a.iters[row] = it;
-        }
-        still = st == kActive || (wrow && st == kRejected);
+    if (splits > 1) {
+        // k_mis_status combines the splits and applies the status rule
+        const size_t q = static_cast<size_t>(split) * a.part_stride + tile * kLanes + lane;
+        a.part[q] = r;
+        a.part[q + static_cast<size_t>(splits) * a.part_stride] = jm;
+        return;
     }
-    // wave-level count of rows still iterating
-    const unsigned long long m = __ballot(still);
-    if (lane == 0) {
-        const int cnt = __popcll(m);
-        a.tile_active[tile] = cnt;
-        if (cnt) atomicAdd(a.n_active, cnt);
+    mis_status<Real, WithJ>(a, tile, lane, r, jm);
+}
+
+// k_mis_status: one wavefront per tile combines the splits' partials of a
+// split k_mismatch and applies the status rule
+template <typename Real, bool WithJ>
+__global__ __launch_bounds__(kLanes) void k_mis_status(MismatchArgs<Real> a)
+{
+    const int lane = threadIdx.x;
+    const size_t tile = blockIdx.x;
+    if (a.tile_active[tile] <= 0) return;
+    const size_t row = tile * kLanes + lane;
+    double r = 0.0, jm = 0.0;
+    for (int s = 0; s < a.splits; ++s) {
+        r = res_max(r, a.part[static_cast<size_t>(s) * a.part_stride + row]);
+        jm = fmax(jm, a.part[static_cast<size_t>(a.splits + s) * a.part_stride + row]);
     }
+    mis_status<Real, WithJ>(a, tile, lane, r, jm);
 }
 
 // ---------------------------------------------------------------------------
@@ -1018,17 +1060,23 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_update(UpdateArgs<Real> a
     const int lane = threadIdx.x & (kLanes - 1);
     // wave-uniform, so the bus tables are read with scalar loads
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kLanes));
-    const size_t tile = blockIdx.x;
+    // splits workgroups per tile, split-major, each a contiguous bus range
+    // (step scaling needs the whole row and runs unsplit)
+    const int splits = a.splits > 1 ? a.splits : 1;
+    const int T = static_cast<int>(gridDim.x) / splits;
+    const int split = static_cast<int>(blockIdx.x) / T;
+    const size_t tile = blockIdx.x - static_cast<size_t>(split) * T;
     if (a.tile_active[tile] <= 0) return;
     const int row = static_cast<int>(tile) * kLanes + lane;
     const bool live = row < a.rows && a.status[row] == kActive;
-    const int b0 = static_cast<int>((static_cast<long long>(a.nbus) * wave) / kMisWaves);
-    const int b1 = static_cast<int>((static_cast<long long>(a.nbus) * (wave + 1)) / kMisWaves);
+    const int part = split * kMisWaves + wave, parts = splits * kMisWaves;
+    const int b0 = static_cast<int>((static_cast<long long>(a.nbus) * part) / parts);
+    const int b1 = static_cast<int>((static_cast<long long>(a.nbus) * (part + 1)) / parts);
     // Step scaling (acpf_nr_kernels.cu:677-723): alpha = min(1, max_dva /
     // max|dVa|, max_dvm / max|dVm|) per row, the maxima NaN-sticky so that a
     // non-finite step is left unscaled for the status rule to catch.
     double alpha = 1.0;
-    if (a.max_dva > 0.0) {
+    if (a.max_dva > 0.0 && splits == 1) {
         __shared__ double s_mt[kMisWaves][kLanes];
         __shared__ double s_mv[kMisWaves][kLanes];
         double mt = 0.0, mv = 0.0;
@@ -1721,6 +1769,10 @@ template __global__ void k_backward<double, 2>(BackwardArgs<double>);
 template __global__ void k_backward<float, 2>(BackwardArgs<float>);
 template __global__ void k_backward<double, 4>(BackwardArgs<double>);
 template __global__ void k_backward<float, 4>(BackwardArgs<float>);
+template __global__ void k_mis_status<double, true>(MismatchArgs<double>);
+template __global__ void k_mis_status<double, false>(MismatchArgs<double>);
+template __global__ void k_mis_status<float, true>(MismatchArgs<float>);
+template __global__ void k_mis_status<float, false>(MismatchArgs<float>);
 template __global__ void k_update<double>(UpdateArgs<double>);
 template __global__ void k_update<float>(UpdateArgs<float>);
 template __global__ void k_lin_residual<double>(RefineArgs<double>);
