@@ -365,8 +365,27 @@ __global__ __launch_bounds__(256) void k_features_tiles_t(const double2* __restr
     const int r = tile * 64 + rr;
     if (r >= M) return;
     __bf16* x = X + static_cast<long long>(r) * ldx;
-    for (int k = 0; k < kPatchBuses / 4; ++k) {
-        const int bl = q * (kPatchBuses / 4) + k;
+    constexpr int kQ = kPatchBuses / 4;  // buses of this thread: 2 kQ contiguous features
+    const int bq = b0 + q * kQ;
+    if (bq + kQ <= n && (ldx & 7) == 0) {
+        // the 2 kQ features as 16-byte stores (row starts and bq are 16-byte aligned)
+        __bf16 o[2 * kQ];
+#pragma unroll
+        for (int k = 0; k < kQ; ++k) {
+            const int b = bq + k;
+            const double2 v = sS[q * kQ + k][rr];
+            o[2 * k] = static_cast<__bf16>(static_cast<float>((v.x - mu[2 * b]) / sigma[2 * b]));
+            o[2 * k + 1] = static_cast<__bf16>(static_cast<float>((v.y - mu[2 * b + 1]) / sigma[2 * b + 1]));
+        }
+        uint4 w[2 * kQ * sizeof(__bf16) / sizeof(uint4)];
+        __builtin_memcpy(w, o, sizeof(o));
+        uint4* dst = reinterpret_cast<uint4*>(x + 2 * bq);
+#pragma unroll
+        for (int i = 0; i < static_cast<int>(sizeof(w) / sizeof(uint4)); ++i) dst[i] = w[i];
+        return;
+    }
+    for (int k = 0; k < kQ; ++k) {
+        const int bl = q * kQ + k;
         const int b = b0 + bl;
         if (b >= n) break;
         const double2 v = sS[bl][rr];
@@ -399,10 +418,23 @@ __global__ __launch_bounds__(256) void k_warm_v_tiles_t(const float* __restrict_
         const int rr = t >> 2, q = t & 3;
         const int r = tile * 64 + rr;
         const float* y = Y + static_cast<long long>(r) * ldy;
-        for (int k = 0; k < kPatchBuses / 4; ++k) {
-            const int bl = q * (kPatchBuses / 4) + k;
-            const int b = b0 + bl;
-            sY[bl][rr] = (r < M && b < n) ? make_float2(y[2 * b], y[2 * b + 1]) : make_float2(0.f, 0.f);
+        constexpr int kQ = kPatchBuses / 4;  // buses of this thread: 2 kQ contiguous outputs
+        const int bq = b0 + q * kQ;
+        if (r < M && bq + kQ <= n && (ldy & 3) == 0) {
+            // 16-byte loads (row starts and bq are 16-byte aligned)
+            const float4* src = reinterpret_cast<const float4*>(y + 2 * bq);
+#pragma unroll
+            for (int k = 0; k < kQ / 2; ++k) {
+                const float4 f = src[k];
+                sY[q * kQ + 2 * k][rr] = make_float2(f.x, f.y);
+                sY[q * kQ + 2 * k + 1][rr] = make_float2(f.z, f.w);
+            }
+        } else {
+            for (int k = 0; k < kQ; ++k) {
+                const int bl = q * kQ + k;
+                const int b = b0 + bl;
+                sY[bl][rr] = (r < M && b < n) ? make_float2(y[2 * b], y[2 * b + 1]) : make_float2(0.f, 0.f);
+            }
         }
     }
     __syncthreads();
