@@ -127,6 +127,7 @@ std::vector<int32_t> ivec(const IArr& a) { return std::vector<int32_t>(a.data(),
 struct Grid {
     Csr Y;
     std::vector<int32_t> pv, pq;
+    std::vector<int32_t> fixed;  // buses outside pv and pq (no unknowns)
     int slack = -1;
     g2g_plan_t plan = nullptr;
     g2g_plan_info info{};
@@ -137,11 +138,21 @@ struct Grid {
         Y = csr_from(Ybus);
         pv = ivec(pv_);
         pq = ivec(pq_);
-        if (slack_ids.size() < 1) throw py::value_error("slack_ids is empty");
-        if (slack_ids.size() > 1)
-            throw py::value_error("several slack buses (distributed slack) need the lightsim2grid ledger, which the "
-                                  "array mode does not carry; pass one slack");
-        slack = slack_ids.data()[0];
+        // array mode (acpf_nr.cu:644-647,1732-1733): the solve reads no slack
+        // list; every bus in neither pv nor pq keeps its start V.  The angle
+        // anchor is the first slack id when it is such a bus, else the first one.
+        std::vector<char> has_unknowns(static_cast<size_t>(Y.n), 0);
+        for (const auto* v : {&pv, &pq})
+            for (int32_t b : *v)
+                if (b >= 0 && b < Y.n) has_unknowns[b] = 1;
+        for (int b = 0; b < Y.n; ++b)
+            if (!has_unknowns[b]) fixed.push_back(b);
+        if (fixed.empty()) throw py::value_error("every bus is pv or pq: the system has no fixed (slack) bus");
+        slack = fixed[0];
+        if (slack_ids.size() > 0) {
+            const int s0 = slack_ids.data()[0];
+            if (s0 >= 0 && s0 < Y.n && !has_unknowns[s0]) slack = s0;
+        }
         const double t0 = now_ms();
         check(g2g_plan_create(Y.n, Y.p.data(), Y.i.data(), Y.x.data(), static_cast<int32_t>(pv.size()), pv.data(),
                               static_cast<int32_t>(pq.size()), pq.data(), slack, &plan));
@@ -431,6 +442,11 @@ public:
         if (Vinit.size() != n || Sbus.size() != n) throw py::value_error("Vinit / Sbus must have n_bus entries");
         S_.assign(Sbus.data(), Sbus.data() + n);
         V_.assign(Vinit.data(), Vinit.data() + n);
+        // |V| set-points of the buses without a |V| unknown (PV and fixed),
+        // from the first start: no Newton update moves them
+        const auto qrow = q_row_of_bus();
+        for (int b = 0; b < n; ++b)
+            if (qrow.data()[b] < 0) vm_set_.emplace_back(b, std::abs(V_[b]));
         batch_ = std::make_unique<Batch>(*grid_, 1, precision_, 0);
         step.apply(batch_->h);
         timings_.t_analyze_ms = grid_->t_analyze_ms;
@@ -462,6 +478,19 @@ public:
         py::gil_scoped_release nogil;
         solve(max_iter < 0 ? max_iter_ : max_iter, tol < 0 ? tol_ : tol);
         return timings_;
+    }
+
+    // LSGrid.ac_pf's contract: a start whose |V| at the PV and fixed buses is
+    // replaced by their set-points (angles kept), solved; true if converged
+    bool solve_at_setpoints(CArr Vinit, int max_iter, double tol)
+    {
+        const int n = grid_->n();
+        if (Vinit.size() != n) throw py::value_error("Vinit must have n_bus entries");
+        V_.assign(Vinit.data(), Vinit.data() + n);
+        for (const auto& [b, vm] : vm_set_) V_[b] = std::polar(vm, std::arg(V_[b]));
+        py::gil_scoped_release nogil;
+        solve(max_iter, tol);
+        return timings_.converged;
     }
 
     CArr get_v()
@@ -550,6 +579,7 @@ private:
     std::unique_ptr<Grid> grid_;
     std::unique_ptr<Batch> batch_;
     std::vector<cplx> S_, V_;
+    std::vector<std::pair<int, double>> vm_set_;  // (bus, |V| set-point) without a |V| unknown
     int max_iter_;
     double tol_;
     int precision_;
@@ -1146,7 +1176,7 @@ public:
         bool* f = a.mutable_data();
         std::fill(f, f + n_bus(), false);
         for (int b : grid_->pv) f[b] = true;
-        f[grid_->slack] = true;
+        for (int b : grid_->fixed) f[b] = true;
         return a;
     }
 
@@ -1166,7 +1196,7 @@ protected:
         // the start voltages with the set-points applied, one row per scenario
         std::vector<char> fixed(static_cast<size_t>(n), 0);
         for (int b : grid_->pv) fixed[b] = 1;
-        fixed[grid_->slack] = 1;
+        for (int b : grid_->fixed) fixed[b] = 1;
         Vg_.resize(static_cast<size_t>(rows_) * n);
         const int ng = static_cast<int>(gb_.size());
         for (int r = 0; r < rows_; ++r) {
@@ -1522,6 +1552,8 @@ PYBIND11_MODULE(_gpusim2grid, m)
         .def_property_readonly("timings", &AcPfNrSession::timings)
         .def("resolve", &AcPfNrSession::resolve, py::arg("Vinit"), py::arg("Sbus") = py::none(),
              py::arg("max_iter") = -1, py::arg("tol") = -1.0)
+        .def("solve_at_setpoints", &AcPfNrSession::solve_at_setpoints, py::arg("Vinit"), py::arg("max_iter"),
+             py::arg("tol"))
         .def("get_v", &AcPfNrSession::get_v)
         .def("v_dlpack", [](AcPfNrSession& s) { return device_capsule(s.get_v()); })
         .def("get_F", &AcPfNrSession::get_F)

@@ -179,7 +179,7 @@ struct g2g_plan_s {
             d_Y = dupload(y);
             std::vector<int> upd;
             for (int i = 0; i < nr.n; ++i) {
-                if (i == nr.slack) continue;
+                if (nr.row_p[i] < 0) continue;  // fixed bus
                 upd.insert(upd.end(), {i, nr.row_p[i], nr.row_q[i]});
             }
             n_upd = static_cast<int>(upd.size() / 3);
@@ -1757,6 +1757,8 @@ int g2g_batch_stream_warm(g2g_batch_t b, g2g_mlp_t mlp, int32_t n, const int32_t
         // voltages back into the slots
         const unsigned g = blocks_for(static_cast<long long>(n) * N, 256);
         k_move_rows<double2><<<g, 256, 0, st>>>(b->S, Sg, d_slots, n, N, 1);
+        // the slots' start V: the fixed buses keep theirs
+        k_move_rows<double2><<<g, 256, 0, st>>>(b->V, Vg, d_slots, n, N, 1);
         HIPCHK(hipGetLastError());
         mlp_warm_start_tiles(mlp, Sg, n, N, d_mask, n_mask, Vg, st);
         k_move_rows<double2><<<g, 256, 0, st>>>(Vg, b->V, d_slots, n, N, 0);

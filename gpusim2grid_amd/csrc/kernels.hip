@@ -829,7 +829,8 @@ __global__ __launch_bounds__(kLanes * walk_waves(W)) void k_factor(FactorArgs<Re
         if (!waited)
             for (int w = 1; w < W; ++w)
                 if (!wait_flag(&flags->done[w]) && lane == 0) atomicOr(a.error, 1);
-        for (int w = 1; w < W; ++w) st.clamped += lds_load_acquire(&flags->clamped[w][lane]);
+        // per row: each lane adds the other walkers' counts of its own row
+        for (int w = 1; w < W; ++w) st.clamped += lds_load_lane(&flags->clamped[w][lane]);
     }
     if (a.clamped && row < a.rows) a.clamped[row] = st.clamped;
 }

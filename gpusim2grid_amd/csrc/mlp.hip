@@ -442,11 +442,11 @@ __global__ __launch_bounds__(256) void k_warm_v_tiles_t(const float* __restrict_
         const int bl = i >> 6, lane = i & 63;
         const int b = b0 + bl, r = tile * 64 + lane;
         if (b >= n || r >= M) continue;
+        if (kind[b] == 2) continue;  // a fixed bus keeps the row's start V
         const float2 o = sY[bl][lane];
         double va = static_cast<double>(o.x) * sigma[2 * b] + mu[2 * b];
         double vm = static_cast<double>(o.y) * sigma[2 * b + 1] + mu[2 * b + 1];
         if (kind[b] != 1) vm = vfix[b];
-        if (kind[b] == 2) va = 0.0;
         double sn, cs;
         sincos(va, &sn, &cs);
         V_tiles[(static_cast<long long>(tile) * n + b) * 64 + lane] = make_double2(vm * cs, vm * sn);

@@ -35,7 +35,7 @@ struct g2g_mlp_s {
     float* in_sigma = nullptr;
     float* out_mu = nullptr;
     float* out_sigma = nullptr;
-    // buses for the warm start: kind (0 pv, 1 pq, 2 slack), |V| where fixed
+    // buses for the warm start: kind (0 pv, 1 pq, 2 fixed), |V| of pv buses
     int* kind = nullptr;
     double* vfix = nullptr;
     int n_bus = 0;

@@ -822,9 +822,10 @@ NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int 
         if (pq[k] < 0 || pq[k] >= n || kind[pq[k]] >= 0) throw std::invalid_argument("bad or repeated pq bus");
         kind[pq[k]] = 1;
     }
+    // every bus outside pv and pq is fixed (|V| and angle, no unknowns), as
+    // the reference's array mode treats them (acpf_nr.cu:644-647,1732-1733);
+    // `slack` names the fixed bus that anchors the angles
     if (slack < 0 || slack >= n || kind[slack] >= 0) throw std::invalid_argument("bad slack bus");
-    for (int i = 0; i < n; ++i)
-        if (i != slack && kind[i] < 0) throw std::invalid_argument("every non-slack bus must be pv or pq");
     // reference layout (acpf_nr.cu:645-647, 678-685): sorted pvpq, pq as given
     P.pvpq.assign(pv, pv + npv);
     P.pvpq.insert(P.pvpq.end(), pq, pq + npq);
@@ -862,10 +863,10 @@ NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int 
     }
     P.nJ = P.Jp[d];
 
-    // bus-level minimum degree (slack excluded), weight = unknowns per bus
+    // bus-level minimum degree (fixed buses excluded), weight = unknowns per bus
     std::vector<int> bus_ids, bus_idx(static_cast<size_t>(n), -1);
     for (int i = 0; i < n; ++i)
-        if (i != slack) { bus_idx[i] = static_cast<int>(bus_ids.size()); bus_ids.push_back(i); }
+        if (kind[i] >= 0) { bus_idx[i] = static_cast<int>(bus_ids.size()); bus_ids.push_back(i); }
     const int nb = static_cast<int>(bus_ids.size());
     std::vector<int> badjp(static_cast<size_t>(nb) + 1, 0), badji, w(static_cast<size_t>(nb));
     for (int b = 0; b < nb; ++b) {
@@ -873,7 +874,7 @@ NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int 
         w[b] = kind[i] == 1 ? 2 : 1;
         for (int p = Yp[i]; p < Yp[i + 1]; ++p) {
             const int j = Yi[p];
-            if (j != i && j != slack) badji.push_back(bus_idx[j]);
+            if (j != i && kind[j] >= 0) badji.push_back(bus_idx[j]);
         }
         badjp[b + 1] = static_cast<int>(badji.size());
     }
@@ -916,7 +917,7 @@ NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int 
         P.k1_bus.clear();
         P.k1_ent.clear();
         for (int i : ro) {
-            if (i == slack) continue;
+            if (kind[i] < 0) continue;  // fixed bus: no equations
             const int e0 = static_cast<int>(P.k1_ent.size() / 6);
             int diag = -1;
             for (int p = Yp[i]; p < Yp[i + 1]; ++p) {
@@ -926,7 +927,7 @@ NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int 
                 for (int b = 0; b < 4; ++b) a[b] = P.map[b][p] >= 0 ? P.lu.a_pos_of_entry[P.map[b][p]] : -1;
                 P.k1_ent.insert(P.k1_ent.end(), {j, p, a[0], a[1], a[2], a[3]});
             }
-            if (diag < 0) throw std::invalid_argument("Ybus has no diagonal entry for a non-slack bus");
+            if (diag < 0) throw std::invalid_argument("Ybus has no diagonal entry for a pv or pq bus");
             const int e1 = static_cast<int>(P.k1_ent.size() / 6);
             const int fp = P.lu.f_pos[P.row_p[i]];
             const int fq = P.row_q[i] >= 0 ? P.lu.f_pos[P.row_q[i]] : -1;

@@ -44,6 +44,15 @@ __device__ __forceinline__ int lds_load_acquire(const int* p)
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+// a per-lane LDS value another wave published before a flag this wave has
+// seen (relaxed, no readfirstlane: every lane keeps its own value)
+__device__ __forceinline__ int lds_load_lane(const int* p)
+{
+    const int v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    return v;
+}
+
 __device__ __forceinline__ void lds_store_release(int* p, int v)
 {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);

@@ -15,7 +15,6 @@ or, with ``as_numpy=True``, a host array.
 """
 from __future__ import annotations
 
-import operator
 
 import numpy as np
 
@@ -64,25 +63,30 @@ def _tuple(grid):
             np.ascontiguousarray(pv, dtype=np.int32), np.ascontiguousarray(pq, dtype=np.int32))
 
 
+def _count(name, value):
+    """A non-negative integer option (a bool is not one)."""
+    if type(value) is bool or not hasattr(value, "__index__"):
+        raise TypeError(f"{name} must be an int, got {type(value).__name__}")
+    value = value.__index__()
+    if value < 0:
+        raise ValueError(f"{name} must be >= 0, got {value}")
+    return value
+
+
 def _check_common(ir_n_steps, scaling_max_voltage_change, max_dVa, max_dVm, use_bridge):
-    """Validated step options as session keywords.  ``ir_n_steps`` follows
-    contingency_analysis/__init__.py:157-167; ``scaling_max_voltage_change=None``
-    inherits from the grid, and an array tuple carries no configuration, so it
-    means off; the thresholds default to 0.5 rad / 0.1 pu
+    """The step options as session keywords.  Messages and exception types are
+    the reference's (contingency_analysis/__init__.py:157-167).  An array
+    tuple carries no grid configuration, so ``scaling_max_voltage_change=None``
+    (inherit) means off; the thresholds default to 0.5 rad / 0.1 pu
     (acpf_nr/gpu_facade.py:196-198)."""
     if use_bridge:
         raise ValueError("use_bridge=True requires a lightsim2grid grid object, not an explicit-array tuple.")
-    if isinstance(ir_n_steps, bool):
-        raise TypeError("ir_n_steps must be an int, got bool")
-    try:
-        n = operator.index(ir_n_steps)
-    except TypeError:
-        raise TypeError(f"ir_n_steps must be an int, got {type(ir_n_steps).__name__}") from None
-    if n < 0:
-        raise ValueError(f"ir_n_steps must be >= 0, got {n}")
-    return dict(ir_n_steps=n, scaling_max_voltage_change=bool(scaling_max_voltage_change),
-                max_dVa=0.5 if max_dVa is None else float(max_dVa),
-                max_dVm=0.1 if max_dVm is None else float(max_dVm))
+    opts = dict(ir_n_steps=_count("ir_n_steps", ir_n_steps),
+                scaling_max_voltage_change=bool(scaling_max_voltage_change), max_dVa=0.5, max_dVm=0.1)
+    for key, val in (("max_dVa", max_dVa), ("max_dVm", max_dVm)):
+        if val is not None:
+            opts[key] = float(val)
+    return opts
 
 
 def _to_device_capsule(V):
@@ -112,7 +116,6 @@ class AcPfGPU:
                  use_distributed_slack=True):
         step = _check_common(ir_n_steps, scaling_max_voltage_change, max_dVa, max_dVm, use_bridge)
         Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq = _tuple(grid)
-        self._vm_fixed = None
         self._s = _cpp.AcPfNrSession(Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq, int(max_iter), float(tol),
                                      _device(device), presolved_v=bool(init_from_n_powerflow),
                                      precision=_precision(precision), **step)
@@ -122,26 +125,21 @@ class AcPfGPU:
         return self._s.get_v()
 
     def resolve(self, Vinit, Sbus=None, *, max_iter=None, tol=None):
-        """Solve again from ``Vinit`` (and new ``Sbus``) reusing the plan."""
-        self._vm_setpoints()
-        return self._s.resolve(np.ascontiguousarray(Vinit, dtype=complex),
-                               None if Sbus is None else np.ascontiguousarray(Sbus, dtype=complex),
-                               -1 if max_iter is None else int(max_iter), -1.0 if tol is None else float(tol))
+        """Solve again from ``Vinit`` (and new ``Sbus``) reusing the plan;
+        ``max_iter`` / ``tol`` default to the constructor's."""
+        opt = (-1 if max_iter is None else int(max_iter), -1.0 if tol is None else float(tol))
+        S = None if Sbus is None else np.ascontiguousarray(Sbus, dtype=complex)
+        return self._s.resolve(np.ascontiguousarray(Vinit, dtype=complex), S, *opt)
 
     def ac_pf(self, Vinit, max_iter, tol):
-        """``LSGrid.ac_pf``'s signature: set-points written into |V| of the
-        Vm-fixed buses, then :meth:`resolve`; empty array if not converged."""
-        mask, vm = self._vm_setpoints()
-        V = np.array(Vinit, dtype=complex)
-        V[mask] = vm[mask] * np.exp(1j * np.angle(V[mask]))
-        t = self.resolve(V, max_iter=max_iter, tol=tol)
-        return self._s.get_v() if t.converged else np.zeros(0, dtype=complex)
-
-    def _vm_setpoints(self):
-        if self._vm_fixed is None:
-            mask = np.asarray(self._s.vm_col_of_bus) < 0
-            self._vm_fixed = (mask, np.abs(self._s.get_v()))
-        return self._vm_fixed
+        """``LSGrid.ac_pf``'s contract: the session solves from ``Vinit`` with
+        the PV and fixed buses' |V| reset to their set-points (taken from the
+        first start); the solved V, or an empty array when the row did not
+        converge."""
+        start = np.ascontiguousarray(Vinit, dtype=complex)
+        if not self._s.solve_at_setpoints(start, int(max_iter), float(tol)):
+            return np.empty(0, dtype=complex)
+        return self._s.get_v()
 
     def get_F(self):
         return self._s.get_F()

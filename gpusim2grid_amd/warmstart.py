@@ -18,10 +18,14 @@ from . import _capi
 PV, PQ, SLACK = 0, 1, 2
 
 
-def bus_kinds(n, pv, pq, slack):
-    kind = np.full(n, PQ, np.int32)
+def bus_kinds(n, pv, pq, slack=None):
+    """PV, PQ, or SLACK for every bus in neither list (a fixed bus: it keeps
+    the row's start V; `slack` is implied and only checked)."""
+    kind = np.full(n, SLACK, np.int32)
     kind[np.asarray(pv, np.int64)] = PV
-    kind[np.atleast_1d(np.asarray(slack, np.int64))] = SLACK
+    kind[np.asarray(pq, np.int64)] = PQ
+    if slack is not None and (kind[np.atleast_1d(np.asarray(slack, np.int64))] != SLACK).any():
+        raise ValueError("a slack bus is listed in pv or pq")
     return kind
 
 
@@ -29,7 +33,8 @@ class WarmStart:
     """An MLP warm start for one grid.
 
     n: buses; pv, pq, slack: bus indices; vset: |V| set-points (the flat
-    start's magnitudes, used at PV and slack buses); n_mask: outage-mask
+    start's magnitudes, used at PV buses; fixed buses keep the row's start V);
+    n_mask: outage-mask
     width (0: injections only).
     """
 

@@ -100,7 +100,9 @@ int g2g_set_device(int32_t device);
  * src/_cpp/contingency/strategies/cudss_batch_solver.cuh:339-440, and the own
  * LU's build_lu_plan, src/_cpp/own_lu/lu_plan.cpp:88-285).  Ybus is CSR with
  * sorted unique columns; Yx holds 2*nY doubles; pv/pq/slack are bus indices.
- * Uploads the tables to the current HIP device. */
+ * Every bus in neither pv nor pq is fixed (no unknowns: its V stays as the
+ * row's start V), as the reference's array mode treats them
+ * (acpf_nr.cu:644-647,1732-1733); `slack` is one of them. */
 int g2g_plan_create(int32_t n, const int32_t *Yp, const int32_t *Yi, const double *Yx, int32_t npv,
                     const int32_t *pv, int32_t npq, const int32_t *pq, int32_t slack, g2g_plan_t *out);
 
@@ -311,7 +313,9 @@ int g2g_mlp_set_params(g2g_mlp_t mlp, const float *in, int64_t n);
 /* feature and output normalisation: x_feat = (x - mu) / sigma, y = mu + sigma * out */
 int g2g_mlp_set_normalization(g2g_mlp_t mlp, const float *in_mu, const float *in_sigma, const float *out_mu,
                               const float *out_sigma);
-/* bus types for warm starts: kind 0 PV, 1 PQ, 2 slack; vfix = |V| setpoint of PV/slack buses */
+/* bus types for warm starts: kind 0 PV (|V| = vfix, angle predicted), 1 PQ
+ * (both predicted), 2 fixed (every bus outside pv and pq: keeps the row's
+ * start V) */
 int g2g_mlp_set_buses(g2g_mlp_t mlp, const int32_t *kind, const double *vfix, int32_t n);
 int g2g_mlp_forward(g2g_mlp_t mlp, const float *X, int32_t M, float *Y, int32_t on_device);
 /* mean-squared-error loss over all M x n_out outputs and its gradient (flat,

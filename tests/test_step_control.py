@@ -238,3 +238,41 @@ def test_gpu_facades_take_the_step_options():
     np.testing.assert_array_equal(sw.get_row_status(), str_)
     np.testing.assert_array_equal(sw.get_row_iterations(), itr)
     assert np.abs(V - Vr).max() < 1e-8
+
+
+def _saddle_blocks(nb):
+    """A block-diagonal matrix of nb saddle blocks and its block-wise identity
+    order (every block's (0, 0) pivot is 0 unless a row changes it)."""
+    blocks = sp.block_diag([SADDLE] * nb, format="csr")
+    blocks.sort_indices()
+    return blocks, np.arange(3 * nb)
+
+
+def test_saddle_blocks_plan_has_two_walkers():
+    """The per-row clamp test below runs on a plan whose walk is split over
+    two walkers (independent blocks fall into both subtree sets)."""
+    A, order = _saddle_blocks(700)
+    inf = _capi.Plan.from_matrix(A, order=order).info()
+    assert inf["walkers"] == 2 and inf["d"] == 2100
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,eps", [(_capi.FP64, 1e-12), (_capi.MIXED, 1e-5)])
+def test_gpu_raw_clamp_counts_are_per_row_on_two_walkers(prec, eps):
+    """Rows clamp different numbers of pivots, in blocks walked by both
+    walkers: each row's count is its own (own_lu/batched_lu.cu:45-53)."""
+    nb = 700
+    A, order = _saddle_blocks(nb)
+    P = _capi.Plan.from_matrix(A, order=order)
+    _say(f"raw clamp counts two walkers prec={prec} info={P.info()['walkers']}")
+    rows = 130
+    rng = np.random.default_rng(11)
+    # entry (0, 0) of each block in CSR order: the first entry of its first row
+    first = A.indptr[0:3 * nb:3]
+    vals = np.tile(A.data, (rows, 1))
+    zero = rng.random((rows, nb)) < rng.random((rows, 1))  # rows differ in how many blocks clamp
+    vals[:, first] = np.where(zero, 0.0, 5.0)
+    rhs = rng.normal(size=(rows, 3 * nb))
+    _, cl = P.lu_solve(vals, rhs, precision=prec, pivot_eps=eps)
+    np.testing.assert_array_equal(cl, zero.sum(axis=1))
+    assert len(np.unique(cl)) > 10

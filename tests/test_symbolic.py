@@ -156,8 +156,9 @@ def test_invalid_inputs_raise_value_error():
     g = syn.grid_c0()
     with pytest.raises(ValueError):
         _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, [g.pv[0]])  # slack also PV
-    with pytest.raises(ValueError):
-        _capi.Plan.from_grid(g.Ybus, g.pv[:-1], g.pq, g.slack)  # a bus without a type
+    # a bus in neither pv nor pq is fixed, as in the reference's array mode
+    inf = _capi.Plan.from_grid(g.Ybus, g.pv[:-1], g.pq, g.slack).info()
+    assert inf["npvpq"] == len(g.pv) - 1 + len(g.pq)
     Y = g.Ybus.tolil()
     Y[g.pq[0], g.pq[0]] = 0
     Y = sp.csr_matrix(Y)
