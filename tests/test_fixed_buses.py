@@ -88,7 +88,7 @@ def test_gpu_facade_takes_several_slack_ids():
     g = syn.grid_118()
     pv, extra = _two_more_fixed(g)
     _say("facade several slack ids 118")
-    slack_ids = np.concatenate([[g.slack], extra])
+    slack_ids = np.concatenate([np.atleast_1d(g.slack), extra])
     grid = (g.Ybus, g.flat_start(), g.Sbus, slack_ids, np.full(slack_ids.size, 1.0 / slack_ids.size), pv, g.pq)
     ac = AcPfGPU(grid, init_from_n_powerflow=False, max_iter=10)
     Vo, sto, ito, _ = orc.nr_batch(g.Ybus, g.flat_start(), g.Sbus, pv, g.pq, 10, 1e-8)
