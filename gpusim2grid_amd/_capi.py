@@ -98,6 +98,7 @@ def lib():
         "g2g_batch_get_results": (ctypes.c_int, [_P, _P, _P, _P]),
         "g2g_batch_eval": (ctypes.c_int, [_P, _P, _P, _P]),
         "g2g_lu_solve_batch": (ctypes.c_int, [_P, i32, i32, _P, _P, _P, dbl, i32, _P, _P]),
+        "g2g_lu_solve_batch_noalloc": (ctypes.c_int, [_P, i32, i32, _P, _P, _P, dbl, i32, _P, _P]),
         "g2g_batch_set_step_control": (ctypes.c_int, [_P, i32, i32, dbl, dbl]),
         "g2g_batch_stream_begin": (ctypes.c_int, [_P]),
         "g2g_batch_solve_adjoint": (ctypes.c_int, [_P, _P, _P]),
@@ -217,6 +218,14 @@ class Plan:
         _check(lib().g2g_lu_solve_batch(self._h, rows, precision, _ptr(vals), _ptr(rhs), _ptr(x), eps,
                                         int(ir_steps), _ptr(cl), stream))
         return x, cl
+
+
+def lu_solve_noalloc(plan, rows, precision, d_vals, d_rhs, d_x, pivot_eps=None, ir_steps=0, d_clamped=None,
+                     stream=None):
+    """g2g_lu_solve_batch_noalloc on device pointers (ints): nothing is allocated."""
+    eps = (1e-5 if precision == MIXED else 1e-12) if pivot_eps is None else pivot_eps
+    _check(lib().g2g_lu_solve_batch_noalloc(plan._h, int(rows), precision, _P(d_vals), _P(d_rhs), _P(d_x), eps,
+                                            int(ir_steps), None if d_clamped is None else _P(d_clamped), stream))
 
 
 class Batch:

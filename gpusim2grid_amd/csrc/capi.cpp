@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -27,6 +28,8 @@ template <typename Real, int W>
 __global__ void k_backward(BackwardArgs<Real> a);
 template <typename Real>
 __global__ void k_update(UpdateArgs<Real> a);
+template <typename Real>
+__global__ void k_tiles_to_rows_f64(const Real* in, int rows, int m, double* out, long long ld);
 template <typename Real>
 __global__ void k_lin_residual(RefineArgs<Real> a);
 template <typename Real>
@@ -321,6 +324,28 @@ size_t lds_limit()
     int v = 0;
     HIPCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     return static_cast<size_t>(v);
+}
+
+// The plan plan_fitting(P, fp64) returns, if it has been built already
+// (nullptr otherwise): a lookup that builds and allocates nothing.
+const g2g_plan_s* plan_fitted(const g2g_plan_s* P, bool fp64)
+{
+    const size_t limit = lds_limit();
+    auto fits = [&](const SymbolicLU& s) {
+        const size_t f = fp64 ? factor_lds<double>(s) : factor_lds<float>(s);
+        const size_t b = fp64 ? backward_lds<double>(s) : backward_lds<float>(s);
+        return f <= limit && b <= limit;
+    };
+    if (fits(P->lu())) return P;
+    const int wb = P->lu().bwalkers;
+    for (const auto& [wf, wbk] : {std::pair<int, int>{1, wb}, std::pair<int, int>{1, 1}}) {
+        const g2g_plan_s* q = nullptr;
+        for (const auto& c : P->fitted)
+            if (c->lu().walkers == wf && c->lu().bwalkers == wbk) q = c.get();
+        if (!q) return nullptr;
+        if (fits(q->lu())) return q;
+    }
+    return P->fitted.empty() ? nullptr : P->fitted.back().get();
 }
 
 // The plan whose walker kernels fit the LDS at this precision: P itself, or
@@ -965,6 +990,129 @@ void raw_solve(g2g_plan_s* P, int rows, const double* vals, const double* rhs, d
     }
 }
 
+// ---- allocation-free raw solve ------------------------------------------
+// Scratch of g2g_lu_solve_batch_noalloc: one array per device, held until the
+// process ends.  Host code never passes `scratch` itself to a kernel (on the
+// host it names a stand-in, not device memory): scratch_base() resolves the
+// current device's copy once with hipGetSymbolAddress and keeps it.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+unsigned char* scratch_base()
+{
+    static std::mutex mu;
+    static std::vector<unsigned char*> base;  // per device ordinal
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if (static_cast<size_t>(dev) >= base.size()) base.resize(static_cast<size_t>(dev) + 1, nullptr);
+    if (!base[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw std::runtime_error("hipGetSymbolAddress returned no address for the scratch array");
+        base[dev] = static_cast<unsigned char*>(p);
+    }
+    return base[dev];
+}
+
+// Bump allocation inside the scratch array (256-byte aligned pieces).
+struct ScratchCarve {
+    unsigned char* base;
+    size_t used = 0;
+    template <typename T>
+    T* take(size_t count)
+    {
+        T* p = reinterpret_cast<T*>(base + used);
+        used += (count * sizeof(T) + 255) / 256 * 256;
+        return p;
+    }
+};
+
+// Scratch bytes of a raw solve of `rows` rows (A, U and X streams, the clamp
+// scales, tile counts, the error flag, and the refinement copies).
+template <typename Real>
+size_t raw_scratch_bytes(const SymbolicLU& s, int rows, int ir_steps)
+{
+    ScratchCarve c{nullptr};
+    const int T = (rows + kLanes - 1) / kLanes;
+    const size_t rp = static_cast<size_t>(T) * kLanes;
+    c.take<Real>(rp * round_up(s.a_len, chunk_of<Real>()));
+    c.take<Real>(rp * u_stride_of(s, sizeof(Real)));
+    c.take<Real>(rp * s.d);
+    c.take<double>(rp);
+    c.take<int>(static_cast<size_t>(T));
+    c.take<int>(1);
+    if (ir_steps > 0) {
+        c.take<Real>(rp * s.d);
+        c.take<Real>(rp * s.d);
+    }
+    return c.used;
+}
+
+// The raw solve on device arrays with every intermediate in the scratch
+// array: no allocation.  Work is enqueued on `stream`; the call waits for it
+// only to read the walkers' error flag.
+template <typename Real>
+void raw_solve_noalloc(const g2g_plan_s* P0, int rows, const double* d_vals, const double* d_rhs, double* d_x,
+                       double pivot_eps, int ir_steps, int32_t* d_clamped, void* stream)
+{
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const g2g_plan_s* P = plan_fitted(P0, sizeof(Real) == sizeof(double));
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (!P || !P->uploaded || P->device != dev)
+        throw std::invalid_argument("the plan's device tables for this precision are not resident on this device: "
+                                    "run g2g_lu_solve_batch once with this plan and precision first");
+    const SymbolicLU& s = P->lu();
+    const size_t need = raw_scratch_bytes<Real>(s, rows, ir_steps);
+    if (need > kNoAllocScratchBytes) {
+        const size_t per = raw_scratch_bytes<Real>(s, kLanes, ir_steps);
+        throw std::invalid_argument(std::to_string(rows) + " rows need " + std::to_string(need >> 20) +
+                                    " MiB of scratch, more than its " + std::to_string(kNoAllocScratchBytes >> 20) +
+                                    " MiB; at most " + std::to_string(kNoAllocScratchBytes / per * kLanes) +
+                                    " rows for this plan and precision");
+    }
+    const int T = (rows + kLanes - 1) / kLanes;
+    const size_t rp = static_cast<size_t>(T) * kLanes;
+    const int nnz = P->nnz, d = s.d;
+    const int a_stride = round_up(s.a_len, chunk_of<Real>()), u_stride = u_stride_of(s, sizeof(Real));
+    ScratchCarve c{scratch_base()};
+    Real* A = c.take<Real>(rp * a_stride);
+    Real* U = c.take<Real>(rp * u_stride);
+    Real* X = c.take<Real>(rp * d);
+    double* jm = c.take<double>(rp);
+    int* ta = c.take<int>(static_cast<size_t>(T));
+    int* err = c.take<int>(1);
+    // the scratch keeps the previous call's contents: clear what the kernels
+    // accumulate into or only partly write (fill kernels, no memset nodes)
+    const long long a_words = static_cast<long long>(rp * a_stride * sizeof(Real) / 4);
+    k_fill_int<<<blocks_for(a_words, 256), 256, 0, st>>>(reinterpret_cast<int*>(A), 0, a_words);
+    const long long j_words = static_cast<long long>(rp * 2);
+    k_fill_int<<<blocks_for(j_words, 256), 256, 0, st>>>(reinterpret_cast<int*>(jm), 0, j_words);
+    k_fill_int<<<1, 64, 0, st>>>(err, 0, 1);
+    k_fill_int<<<blocks_for(T, 256), 256, 0, st>>>(ta, kLanes, T);
+    k_row_absmax<<<rows, 256, 0, st>>>(d_vals, nnz, rows, nnz, jm);
+    const long long total = static_cast<long long>(rows) * (nnz + d);
+    k_gather_astream<Real><<<blocks_for(total, 256), 256, 0, st>>>(d_vals, nnz, d_rhs, d, rows, nnz, d, P->d_apos,
+                                                                    P->d_fpos, a_stride, A);
+    HIPCHK(hipGetLastError());
+    launch_factor<Real>(P, A, U, a_stride, u_stride, ta, jm, d_clamped, err, rows, T, pivot_eps, st);
+    launch_backward<Real>(P, U, u_stride, ta, X, err, rows, T, st);
+    if (ir_steps > 0) {
+        Real* Bs = c.take<Real>(rp * d);
+        Real* Xs = c.take<Real>(rp * d);
+        refine<Real>(P, A, U, X, Bs, Xs, a_stride, u_stride, ta, jm, d_clamped, err, rows, T, pivot_eps, ir_steps,
+                     st);
+    }
+    const long long xs = static_cast<long long>(rows) * d;
+    k_tiles_to_rows_f64<Real><<<blocks_for(xs, 256), 256, 0, st>>>(X, rows, d, d_x, d);
+    HIPCHK(hipGetLastError());
+    int herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (herr) throw std::runtime_error("a stream wavefront gave up waiting for its data (results invalid)");
+}
+
 // J^T x = rhs for the rows [0, active_rows) at the batch's V: the mismatch
 // program fills the A stream with J transposed, rhs replaces its right-hand
 // side, and the factorization / backward programs of the (symmetric) pattern
@@ -1591,6 +1739,23 @@ int g2g_lu_solve_batch(g2g_plan_t P, int32_t rows, int32_t precision, const doub
         if (ir_steps < 0) throw std::invalid_argument("ir_steps must be >= 0");
         if (precision == G2G_MIXED) raw_solve<float>(P, rows, vals, rhs, x, pivot_eps, ir_steps, clamped, stream);
         else if (precision == G2G_FP64) raw_solve<double>(P, rows, vals, rhs, x, pivot_eps, ir_steps, clamped, stream);
+        else throw std::invalid_argument("unknown precision");
+    });
+}
+
+int g2g_lu_solve_batch_noalloc(g2g_plan_t P, int32_t rows, int32_t precision, const double* d_vals,
+                               const double* d_rhs, double* d_x, double pivot_eps, int32_t ir_steps,
+                               int32_t* d_clamped, void* stream)
+{
+    return guarded([&] {
+        if (!P || !d_vals || !d_rhs || !d_x) throw std::invalid_argument("null argument");
+        if (rows <= 0) throw std::invalid_argument("rows must be positive");
+        if (ir_steps < 0) throw std::invalid_argument("ir_steps must be >= 0");
+        if (P->grid) throw std::invalid_argument("a raw matrix plan (g2g_lu_plan_create) is needed");
+        if (precision == G2G_MIXED)
+            raw_solve_noalloc<float>(P, rows, d_vals, d_rhs, d_x, pivot_eps, ir_steps, d_clamped, stream);
+        else if (precision == G2G_FP64)
+            raw_solve_noalloc<double>(P, rows, d_vals, d_rhs, d_x, pivot_eps, ir_steps, d_clamped, stream);
         else throw std::invalid_argument("unknown precision");
     });
 }

@@ -1215,6 +1215,21 @@ __global__ void k_tiles_to_rows(const T* __restrict__ in, int rows, int m, T* __
     out[static_cast<long long>(r) * ld + k] = in[tidx(static_cast<size_t>(r / kLanes), m, k, r % kLanes)];
 }
 
+// tiled [T][m][64] values of the first `rows` rows into row-major FP64 rows
+// [rows][m] with row stride ld
+template <typename Real>
+__global__ void k_tiles_to_rows_f64(const Real* __restrict__ in, int rows, int m, double* __restrict__ out,
+                                    long long ld)
+{
+    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long total = static_cast<long long>(rows) * m;
+    if (gid >= total) return;
+    const int k = static_cast<int>(gid / rows);
+    const int r = static_cast<int>(gid - static_cast<long long>(k) * rows);
+    out[static_cast<long long>(r) * ld + k] =
+        static_cast<double>(in[tidx(static_cast<size_t>(r / kLanes), m, k, r % kLanes)]);
+}
+
 // Per-row Ybus values: broadcast the shared values to every row, then subtract
 // each row's outage deltas (the reference's apply_contingencies_kernel,
 // acpf_nr_kernels.cu:328-348, on the tiled layout).
@@ -1776,6 +1791,8 @@ template __global__ void k_mis_status<float, true>(MismatchArgs<float>);
 template __global__ void k_mis_status<float, false>(MismatchArgs<float>);
 template __global__ void k_update<double>(UpdateArgs<double>);
 template __global__ void k_update<float>(UpdateArgs<float>);
+template __global__ void k_tiles_to_rows_f64<double>(const double*, int, int, double*, long long);
+template __global__ void k_tiles_to_rows_f64<float>(const float*, int, int, double*, long long);
 template __global__ void k_lin_residual<double>(RefineArgs<double>);
 template __global__ void k_lin_residual<float>(RefineArgs<float>);
 template __global__ void k_axpy_tiles<double>(const double*, double*, const int*, int);

@@ -267,6 +267,22 @@ int g2g_batch_solve_adjoint(g2g_batch_t batch, const double *rhs, double *x);
 int g2g_lu_solve_batch(g2g_plan_t plan, int32_t rows, int32_t precision, const double *vals, const double *rhs,
                        double *x, double pivot_eps, int32_t ir_steps, int32_t *clamped, void *stream);
 
+/* g2g_lu_solve_batch on device arrays, with no allocation: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * d_vals (rows x nnz, CSR order), d_rhs (rows x d), d_x (rows x d) and
+ * d_clamped (rows, nullable) are device arrays; results match
+ * g2g_lu_solve_batch's bit for bit.  Every intermediate lives in a file-scope
+ * device array of 64 MiB per GPU (held until the process ends): a call whose
+ * rows need more is rejected (G2G_ERR_INVALID, the message names the row
+ * limit of this plan and precision).  The plan's device tables for this
+ * precision must already be resident (one g2g_lu_solve_batch call with the
+ * plan and precision makes them so).  Calls that overlap on one GPU share the
+ * array and race; the call returns once its work on `stream` is done. */
+int g2g_lu_solve_batch_noalloc(g2g_plan_t plan, int32_t rows, int32_t precision, const double *d_vals,
+                               const double *d_rhs, double *d_x, double pivot_eps, int32_t ir_steps,
+                               int32_t *d_clamped, void *stream);
+
 /* ---- Streamed rows (continuous batching with slot refill; replaces the
  * reference's _solve_continuous refill, contingency/batch_pf_driver.cu:2720-3100,
  * and slot_schedule.cuh:82-156).  The batch's rows become slots: a caller
