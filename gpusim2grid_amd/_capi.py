@@ -27,7 +27,7 @@ class PlanInfo(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int32) for k in (
         "n", "nY", "d", "nJ", "npvpq", "npq", "slack", "nnz", "nLU", "n_updates", "max_c", "height",
         "slots_f", "slots_b", "a_len", "u_len", "n_steps", "walkers", "n_top", "bwalkers",
-        "u_stride_f64", "u_stride_f32")]
+        "u_stride_f64", "u_stride_f32", "n_ext", "n_xs", "sa_col")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -82,6 +82,9 @@ def lib():
         "g2g_version": (ctypes.c_int, []),
         "g2g_set_device": (ctypes.c_int, [i32]),
         "g2g_plan_create": (ctypes.c_int, [i32, _P, _P, _P, i32, _P, i32, _P, i32, ctypes.POINTER(_P)]),
+        "g2g_plan_create_ext": (ctypes.c_int, [i32, _P, _P, _P, i32, _P, i32, _P, i32, _P, ctypes.POINTER(_P)]),
+        "g2g_batch_get_ext_state": (ctypes.c_int, [_P, _P]),
+        "g2g_batch_set_ext_state": (ctypes.c_int, [_P, _P]),
         "g2g_lu_plan_create": (ctypes.c_int, [i32, _P, _P, _P, ctypes.POINTER(_P)]),
         "g2g_plan_destroy": (None, [_P]),
         "g2g_plan_get_info": (ctypes.c_int, [_P, ctypes.POINTER(PlanInfo)]),
@@ -169,15 +172,24 @@ class Plan:
         self._h = handle
 
     @classmethod
-    def from_grid(cls, Ybus, pv, pq, slack):
+    def from_grid(cls, Ybus, pv, pq, slack, ext=None):
+        """ext: an ``extensions.Extensions`` (extension rows) or None."""
         import scipy.sparse as sp
         Y = sp.csr_matrix(Ybus)
         Y.sort_indices()
         Yp, Yi, Yx = _i32(Y.indptr), _i32(Y.indices), _c128(Y.data)
         pv, pq = _i32(pv), _i32(pq)
         h = _P()
-        _check(lib().g2g_plan_create(Y.shape[0], _ptr(Yp), _ptr(Yi), _ptr(Yx), pv.size, _ptr(pv), pq.size,
-                                     _ptr(pq), int(np.atleast_1d(slack)[0]), ctypes.byref(h)))
+        s0 = int(np.atleast_1d(slack)[0])
+        if ext is None or ext.empty():
+            _check(lib().g2g_plan_create(Y.shape[0], _ptr(Yp), _ptr(Yi), _ptr(Yx), pv.size, _ptr(pv), pq.size,
+                                         _ptr(pq), s0, ctypes.byref(h)))
+        else:
+            from .extensions import to_c
+            c, keep = to_c(ext, Y.shape[0])
+            _check(lib().g2g_plan_create_ext(Y.shape[0], _ptr(Yp), _ptr(Yi), _ptr(Yx), pv.size, _ptr(pv), pq.size,
+                                             _ptr(pq), s0, ctypes.byref(c), ctypes.byref(h)))
+            del keep
         return cls(h)
 
     @classmethod
@@ -423,6 +435,18 @@ class Batch:
         res = np.empty(self.rows)
         _check(lib().g2g_batch_get_results(self._h, _ptr(st), _ptr(it), _ptr(res)))
         return st, it, res
+
+    def ext_state(self):
+        """[rows, n_xs] extension state (sa, then each controller's Q)."""
+        k = self.plan.info()["n_xs"]
+        out = np.zeros((self.rows, k))
+        if k:
+            _check(lib().g2g_batch_get_ext_state(self._h, _ptr(out)))
+        return out
+
+    def set_ext_state(self, xs):
+        xs = np.ascontiguousarray(xs, dtype=np.float64)
+        _check(lib().g2g_batch_set_ext_state(self._h, _ptr(xs)))
 
     def eval(self, with_j=True):
         inf = self.plan.info()

@@ -8,6 +8,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/gpusim2grid_amd.h"
@@ -30,6 +31,14 @@ template <typename Real>
 __global__ void k_update(UpdateArgs<Real> a);
 template <typename Real>
 __global__ void k_tiles_to_rows_f64(const Real* in, int rows, int m, double* out, long long ld);
+template <typename Real>
+__global__ void k_ext_pre(ExtArgs<Real> a);
+template <typename Real>
+__global__ void k_ext_post(ExtArgs<Real> a);
+template <typename Real>
+__global__ void k_ext_update(ExtArgs<Real> a);
+template <typename Real>
+__global__ void k_ext_init(ExtArgs<Real> a);
 template <typename Real>
 __global__ void k_lin_residual(RefineArgs<Real> a);
 template <typename Real>
@@ -113,6 +122,25 @@ struct g2g_plan_s {
     std::vector<std::unique_ptr<g2g_plan_s>> fitted;
     int* d_Jp = nullptr;      // the matrix CSR in unknown labels (refinement residuals)
     int* d_Ji = nullptr;
+    // extension rows (nr.xt): device copies of its tables
+    struct {
+        int *inj_ptr = nullptr, *inj = nullptr, *jrec = nullptr, *frec = nullptr, *xcol = nullptr;
+        int *h_bus1 = nullptr, *h_bus2 = nullptr, *h_status = nullptr, *vc_reg = nullptr, *vc_ptr = nullptr;
+        int *sh_c1 = nullptr, *sh_ck = nullptr;
+        double *ds_w = nullptr, *jconst = nullptr, *h_par = nullptr, *vc_vset = nullptr, *vc_slope = nullptr,
+               *vc_w = nullptr;
+        void free_all()
+        {
+            for (void* q : {static_cast<void*>(inj_ptr), static_cast<void*>(inj), static_cast<void*>(jrec),
+                            static_cast<void*>(frec), static_cast<void*>(xcol), static_cast<void*>(h_bus1),
+                            static_cast<void*>(h_bus2), static_cast<void*>(h_status), static_cast<void*>(vc_reg),
+                            static_cast<void*>(vc_ptr), static_cast<void*>(sh_c1), static_cast<void*>(sh_ck),
+                            static_cast<void*>(ds_w), static_cast<void*>(jconst), static_cast<void*>(h_par),
+                            static_cast<void*>(vc_vset), static_cast<void*>(vc_slope), static_cast<void*>(vc_w)})
+                dfree(q);
+        }
+    } dx;
+    bool ext() const { return grid && nr.xt.on(); }
     int prog_f_stride = 0, prog_b_stride = 0;
     bool uploaded = false;
     const SymbolicLU& lu() const { return grid ? nr.lu : raw; }
@@ -121,6 +149,7 @@ struct g2g_plan_s {
         dfree(d_prog_f); dfree(d_prog_b); dfree(d_seg); dfree(d_bseg); dfree(d_k1_bus); dfree(d_k1_ent);
         dfree(d_Y); dfree(d_perm); dfree(d_apos); dfree(d_fpos); dfree(d_upd); dfree(d_Jp); dfree(d_Ji);
         dfree(d_k1_ent_t);
+        dx.free_all();
     }
     // the mismatch program writing J transposed: entry (r, c) of J goes to the
     // A position of (c, r) (the pattern is structurally symmetric)
@@ -187,6 +216,27 @@ struct g2g_plan_s {
             }
             n_upd = static_cast<int>(upd.size() / 3);
             d_upd = dupload(upd);
+            if (nr.xt.on()) {
+                const ExtTables& x = nr.xt;
+                dx.inj_ptr = dupload(x.inj_ptr);
+                dx.inj = dupload(x.inj);
+                dx.jrec = dupload(x.jrec);
+                dx.frec = dupload(x.frec);
+                dx.xcol = dupload(x.xcol);
+                dx.h_bus1 = dupload(x.h_bus1);
+                dx.h_bus2 = dupload(x.h_bus2);
+                dx.h_status = dupload(x.h_status);
+                dx.vc_reg = dupload(x.vc_reg);
+                dx.vc_ptr = dupload(x.vc_ptr);
+                dx.sh_c1 = dupload(x.sh_c1);
+                dx.sh_ck = dupload(x.sh_ck);
+                dx.ds_w = dupload(x.ds_w);
+                dx.jconst = dupload(x.jconst);
+                dx.h_par = dupload(x.h_par);
+                dx.vc_vset = dupload(x.vc_vset);
+                dx.vc_slope = dupload(x.vc_slope);
+                dx.vc_w = dupload(x.vc_w);
+            }
         }
         uploaded = true;
     }
@@ -252,6 +302,11 @@ struct g2g_batch_s {
     double2* V0k = nullptr;
     int *it0k = nullptr, *budgetk = nullptr;
     unsigned char* warmk = nullptr;
+    // extension rows: per-row state [T][n_xs][64] (sa, Q of each controller),
+    // its compaction copy, and the extension buses' residual terms [T][n_xb][64]
+    double* xs = nullptr;
+    double* xsk = nullptr;
+    double2* sx = nullptr;
     // islanded buses per row [T][n][64] (nullable), and its compaction copy
     unsigned char* mask = nullptr;
     unsigned char* maskk = nullptr;
@@ -266,7 +321,7 @@ struct g2g_batch_s {
         dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped); dfree(error);
         dfree(V0); dfree(Bs); dfree(Xs);
         dfree(Vk); dfree(Sk); dfree(Yk); dfree(stk); dfree(itk); dfree(resk); dfree(mapk); dfree(it0);
-        dfree(mask); dfree(maskk);
+        dfree(mask); dfree(maskk); dfree(xs); dfree(xsk); dfree(sx);
         dfree(budget); dfree(warm); dfree(requeued); dfree(V0k); dfree(it0k); dfree(budgetk); dfree(warmk);
         for (void*& q : stage) dfree(q);
         if (h_active) (void)hipHostFree(h_active);
@@ -379,7 +434,7 @@ g2g_plan_s* plan_fitting(g2g_plan_s* P, bool fp64)
                     if (!is_pq[b]) pv.push_back(b);
                 c->nr = build_nr_plan(nr.n, nr.Yp.data(), nr.Yi.data(), nr.Yx.empty() ? nullptr : nr.Yx.data(),
                                       static_cast<int>(pv.size()), pv.data(), nr.npq, nr.pq.data(), nr.slack, wf,
-                                      wbk);
+                                      wbk, &nr.ext);
             } else {
                 c->raw = build_symbolic(P->raw.d, P->raw_p.data(), P->raw_i.data(), P->raw_order, wf, wbk);
                 c->raw_p = P->raw_p;
@@ -477,16 +532,87 @@ void launch_update(g2g_batch_s* b)
 }
 
 template <typename Real>
+ExtArgs<Real> ext_args(g2g_batch_s* b)
+{
+    const g2g_plan_s* P = b->plan;
+    const ExtTables& x = P->nr.xt;
+    ExtArgs<Real> a{};
+    a.V = b->V;
+    a.S = b->S;
+    a.xs = b->xs;
+    a.sx = b->sx;
+    a.A = static_cast<Real*>(b->A);
+    a.X = static_cast<const Real*>(b->X);
+    a.tile_active = b->tile_active;
+    a.status = b->status;
+    a.rows = b->active_rows;
+    a.n = P->nr.n;
+    a.d = P->nr.d;
+    a.a_stride = b->a_stride;
+    a.n_xs = x.n_xs;
+    a.n_xb = static_cast<int>(x.xb_bus.size());
+    a.sa = x.sa;
+    a.q0 = x.q0;
+    a.inj_ptr = P->dx.inj_ptr;
+    a.inj = P->dx.inj;
+    a.ds_w = P->dx.ds_w;
+    a.n_jrec = static_cast<int>(x.jrec.size() / 4);
+    a.jrec = P->dx.jrec;
+    a.jconst = P->dx.jconst;
+    a.n_frec = static_cast<int>(x.frec.size() / 3);
+    a.frec = P->dx.frec;
+    a.xcol = P->dx.xcol;
+    a.h_bus1 = P->dx.h_bus1;
+    a.h_bus2 = P->dx.h_bus2;
+    a.h_status = P->dx.h_status;
+    a.h_par = P->dx.h_par;
+    a.vc_reg = P->dx.vc_reg;
+    a.vc_ptr = P->dx.vc_ptr;
+    a.vc_vset = P->dx.vc_vset;
+    a.vc_slope = P->dx.vc_slope;
+    a.vc_w = P->dx.vc_w;
+    a.sh_c1 = P->dx.sh_c1;
+    a.sh_ck = P->dx.sh_ck;
+    a.part = b->mis_part;
+    a.part_stride = b->rows_pad;
+    a.splits = b->mis_splits;
+    return a;
+}
+
+// the extension state's start: sa = sum Re(Sbus) per row, Q(c) = 0
+template <typename Real>
+void launch_ext_init(g2g_batch_s* b)
+{
+    if (!b->plan->ext()) return;
+    k_ext_init<Real><<<b->T, kLanes, 0, b->stream>>>(ext_args<Real>(b));
+    HIPCHK(hipGetLastError());
+}
+
+template <typename Real>
+void launch_ext_update(g2g_batch_s* b)
+{
+    if (!b->plan->ext() || b->plan->nr.xt.n_xs == 0) return;
+    k_ext_update<Real><<<b->T, kLanes, 0, b->stream>>>(ext_args<Real>(b));
+    HIPCHK(hipGetLastError());
+}
+
+template <typename Real>
 void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double tol, int freeze,
                      double growth = 0.0, bool transposed = false)
 {
     const NrPlan& P = b->plan->nr;
+    const bool ext = b->plan->ext();
+    if (ext) {
+        k_ext_pre<Real><<<b->T, kLanes, 0, b->stream>>>(ext_args<Real>(b));
+        HIPCHK(hipGetLastError());
+    }
     MismatchArgs<Real> a{b->plan->d_k1_bus, transposed ? b->plan->d_k1_ent_t : b->plan->d_k1_ent, P.k1_nbus, b->plan->d_Y, b->Yrow, b->V, b->S,
                          static_cast<Real*>(b->A), P.n, P.nY, b->a_stride, b->res, b->jmax, b->status, b->iters,
                          b->tile_active, b->n_active, b->active_rows, iter, max_iter, tol, freeze, growth,
                          (b->streaming || b->restart_on) ? b->it0 : nullptr, b->mask,
                          b->restart_on ? b->budget : nullptr, b->restart_on ? b->warm : nullptr,
-                         b->mis_splits, b->mis_part, b->rows_pad};
+                         b->mis_splits, (b->mis_splits > 1 || ext) ? b->mis_part : nullptr, b->rows_pad,
+                         ext ? b->sx : nullptr, static_cast<int>(P.xt.xb_bus.size())};
     const dim3 grid(b->T * b->mis_splits), block(kLanes * kMismatchWaves);
     if (b->per_row_y) {
         if (with_j) k_mismatch<Real, true, true><<<grid, block, 0, b->stream>>>(a);
@@ -496,7 +622,13 @@ void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double
         else k_mismatch<Real, false, false><<<grid, block, 0, b->stream>>>(a);
     }
     HIPCHK(hipGetLastError());
-    if (b->mis_splits > 1) {
+    if (ext) {
+        ExtArgs<Real> ea = ext_args<Real>(b);
+        if (!with_j) ea.n_jrec = 0;  // an F-only pass leaves the J values alone
+        k_ext_post<Real><<<b->T, kLanes, 0, b->stream>>>(ea);
+        HIPCHK(hipGetLastError());
+    }
+    if (a.part) {
         if (with_j) k_mis_status<Real, true><<<b->T, kLanes, 0, b->stream>>>(a);
         else k_mis_status<Real, false><<<b->T, kLanes, 0, b->stream>>>(a);
         HIPCHK(hipGetLastError());
@@ -617,6 +749,16 @@ void compact_rows(g2g_batch_s* b)
         std::swap(b->budget, b->budgetk);
         std::swap(b->warm, b->warmk);
     }
+    if (b->xs) {  // extension state moves with its rows
+        const int nxs = b->plan->nr.xt.n_xs;
+        if (!b->xsk) {
+            b->xsk = dalloc<double>(rp * nxs);
+            HIPCHK(hipMemsetAsync(b->xsk, 0, rp * nxs * sizeof(double), s));
+        }
+        k_move_rows<double><<<blocks_for(static_cast<long long>(R) * nxs, 256), 256, 0, s>>>(b->xs, b->xsk, b->mapk, R,
+                                                                                              nxs, 1);
+        std::swap(b->xs, b->xsk);
+    }
     HIPCHK(hipGetLastError());
     std::swap(b->V, b->Vk);
     std::swap(b->S, b->Sk);
@@ -636,6 +778,12 @@ void uncompact_rows(g2g_batch_s* b)
     const int R = b->compact_R;
     k_move_rows<double2><<<blocks_for(static_cast<long long>(R) * nr.n, 256), 256, 0, s>>>(b->V, b->Vk, b->mapk, R,
                                                                                           nr.n, 0);
+    if (b->xs) {
+        const int nxs = nr.xt.n_xs;
+        k_move_rows<double><<<blocks_for(static_cast<long long>(R) * nxs, 256), 256, 0, s>>>(b->xs, b->xsk, b->mapk, R,
+                                                                                              nxs, 0);
+        std::swap(b->xs, b->xsk);
+    }
     std::swap(b->V, b->Vk);
     std::swap(b->S, b->Sk);
     if (b->per_row_y) std::swap(b->Yrow, b->Yk);
@@ -729,6 +877,7 @@ int nr_loop_body(g2g_batch_s* b, int max_iter, double tol, int freeze, double gr
                          b->tile_active, b->jmax, b->clamped, b->error, b->active_rows, b->T, eps, b->ir_steps,
                          b->stream);
         launch_update<Real>(b);
+        launch_ext_update<Real>(b);
         tm.ph[2].push_back({e0, tm.mark()});
     }
     return it;
@@ -780,6 +929,7 @@ void stream_round(g2g_batch_s* b, int k, int max_iter, double tol)
                          b->tile_active, b->jmax, b->clamped, b->error, b->active_rows, b->T, eps, b->ir_steps,
                          b->stream);
         launch_update<Real>(b);
+        launch_ext_update<Real>(b);
         ++b->clock;
     }
 }
@@ -834,6 +984,7 @@ void solve_impl(g2g_batch_s* b, int max_iter, double tol, int sched, g2g_solve_s
     const int freeze = sched == G2G_SCHED_CONTINUOUS ? 1 : 0;
     b->streaming = false;
     reset_rows(b);
+    launch_ext_init<Real>(b);
     SolveTimer tm(b, 3 * (max_iter + 1) + 2);
     const int e_begin = tm.mark();
     long long rows_factored = 0;
@@ -1204,6 +1355,53 @@ int g2g_plan_create(int32_t n, const int32_t* Yp, const int32_t* Yi, const doubl
     });
 }
 
+int g2g_plan_create_ext(int32_t n, const int32_t* Yp, const int32_t* Yi, const double* Yx, int32_t npv,
+                        const int32_t* pv, int32_t npq, const int32_t* pq, int32_t slack, const g2g_extensions* ext,
+                        g2g_plan_t* out)
+{
+    return guarded([&] {
+        if (!out || !Yp || !Yi || !Yx || (npv && !pv) || (npq && !pq)) throw std::invalid_argument("null argument");
+        ExtSpec x;
+        if (ext) {
+            auto vec = [](const auto* p, int k) {
+                using T = std::remove_cv_t<std::remove_pointer_t<decltype(p)>>;
+                if (k > 0 && !p) throw std::invalid_argument("null extension array");
+                return k > 0 ? std::vector<T>(p, p + k) : std::vector<T>();
+            };
+            if (ext->slack_weights) x.slack_w = vec(ext->slack_weights, n);
+            const int nh = ext->n_hvdc;
+            if (nh < 0 || ext->n_vc_groups < 0) throw std::invalid_argument("negative extension count");
+            x.h_bus1 = vec(ext->hvdc_bus1, nh);
+            x.h_bus2 = vec(ext->hvdc_bus2, nh);
+            x.h_status = vec(ext->hvdc_status, nh);
+            x.h_p0 = vec(ext->hvdc_p0, nh);
+            x.h_k = vec(ext->hvdc_k, nh);
+            x.h_lf1 = vec(ext->hvdc_lf1, nh);
+            x.h_lf2 = vec(ext->hvdc_lf2, nh);
+            x.h_r = vec(ext->hvdc_r, nh);
+            x.h_pmax12 = vec(ext->hvdc_pmax12, nh);
+            x.h_pmax21 = vec(ext->hvdc_pmax21, nh);
+            const int ng = ext->n_vc_groups;
+            if (ng > 0) {
+                if (!ext->vc_ptr) throw std::invalid_argument("null extension array");
+                x.vc_ptr = vec(ext->vc_ptr, ng + 1);
+                const int nc = x.vc_ptr[ng];
+                if (nc < 0) throw std::invalid_argument("bad voltage-control group pointers");
+                x.vc_reg = vec(ext->vc_reg_bus, ng);
+                x.vc_vset = vec(ext->vc_v_set, ng);
+                x.vc_bus = vec(ext->vc_bus, nc);
+                x.vc_slope = vec(ext->vc_slope, nc);
+                x.vc_w = vec(ext->vc_weight, nc);
+            }
+        }
+        auto p = std::make_unique<g2g_plan_s>();
+        p->grid = true;
+        p->nr = build_nr_plan(n, Yp, Yi, Yx, npv, pv, npq, pq, slack, 0, 0, &x);
+        p->nnz = p->nr.nJ;
+        *out = p.release();
+    });
+}
+
 int g2g_lu_plan_create(int32_t d, const int32_t* Ap, const int32_t* Ai, const int32_t* order, g2g_plan_t* out)
 {
     return guarded([&] {
@@ -1254,6 +1452,13 @@ int g2g_plan_get_info(g2g_plan_t P, g2g_plan_info* info)
         info->u_len = s.u_len;
         info->u_stride_f64 = u_stride_of(s, 8);
         info->u_stride_f32 = u_stride_of(s, 4);
+        if (P->grid) {
+            info->n_ext = P->nr.n_ext;
+            info->n_xs = P->nr.xt.n_xs;
+            info->sa_col = P->nr.xt.sa_col;
+        } else {
+            info->sa_col = -1;
+        }
     });
 }
 
@@ -1348,7 +1553,14 @@ int g2g_batch_create(g2g_plan_t P, int32_t rows, int32_t precision, int32_t per_
         b->res = dalloc<double>(rp);
         b->jmax = dalloc<double>(rp);
         b->mis_splits = default_mis_splits();
-        if (b->mis_splits > 1) b->mis_part = dalloc<double>(2 * static_cast<size_t>(b->mis_splits) * rp);
+        if (b->mis_splits > 1 || P->ext())
+            b->mis_part = dalloc<double>(2 * static_cast<size_t>(b->mis_splits) * rp);
+        if (P->ext()) {
+            const size_t nxs = static_cast<size_t>(std::max(1, nr.xt.n_xs));
+            b->xs = dalloc<double>(rp * nxs);
+            HIPCHK(hipMemsetAsync(b->xs, 0, rp * nxs * sizeof(double), b->stream));
+            b->sx = dalloc<double2>(rp * std::max<size_t>(1, nr.xt.xb_bus.size()));
+        }
         b->status = dalloc<int>(rp);
         b->iters = dalloc<int>(rp);
         b->clamped = dalloc<int>(rp);
@@ -1434,6 +1646,7 @@ int g2g_batch_set_step_control(g2g_batch_t b, int32_t ir_steps, int32_t scaling,
         if (ir_steps < 0) throw std::invalid_argument("ir_steps must be >= 0");
         if (scaling && !(max_dva > 0.0 && max_dvm > 0.0))
             throw std::invalid_argument("step scaling needs positive max_dva and max_dvm");
+        if (scaling && b->plan->ext()) throw std::invalid_argument("step scaling is not available with extension rows");
         b->ir_steps = ir_steps;
         b->max_dva = scaling ? max_dva : 0.0;
         b->max_dvm = scaling ? max_dvm : 0.0;
@@ -1462,6 +1675,7 @@ int g2g_batch_solve_warm(g2g_batch_t b, g2g_mlp_t mlp, const uint8_t* mask, int3
         if (k_warm < 0 || max_iter < 0) throw std::invalid_argument("iteration budgets must be >= 0");
         if (!(tol > 0.0)) throw std::invalid_argument("tol must be positive");
         if (growth < 0.0 || growth != growth) throw std::invalid_argument("growth must be >= 0 (0: off)");
+        if (b->plan->ext()) throw std::invalid_argument("learned warm starts are not available with extension rows");
         if (b->precision == G2G_MIXED)
             solve_warm_impl<float>(b, mlp, mask, n_mask, k_warm, growth, max_iter, tol, stats);
         else solve_warm_impl<double>(b, mlp, mask, n_mask, k_warm, growth, max_iter, tol, stats);
@@ -1721,10 +1935,41 @@ int g2g_batch_eval(g2g_batch_t b, double* F, double* J, double* residual)
     });
 }
 
+int g2g_batch_get_ext_state(g2g_batch_t b, double* out)
+{
+    return guarded([&] {
+        if (!b || !out) throw std::invalid_argument("null argument");
+        const int nxs = b->plan->nr.xt.n_xs;
+        if (!b->xs || nxs == 0) return;
+        std::vector<double> h(static_cast<size_t>(b->rows_pad) * nxs);
+        HIPCHK(hipMemcpyAsync(h.data(), b->xs, h.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        for (int r = 0; r < b->rows; ++r)
+            for (int q = 0; q < nxs; ++q)
+                out[static_cast<size_t>(r) * nxs + q] = h[(static_cast<size_t>(r / kLanes) * nxs + q) * kLanes + r % kLanes];
+    });
+}
+
+int g2g_batch_set_ext_state(g2g_batch_t b, const double* in)
+{
+    return guarded([&] {
+        if (!b || !in) throw std::invalid_argument("null argument");
+        const int nxs = b->plan->nr.xt.n_xs;
+        if (!b->xs || nxs == 0) return;
+        std::vector<double> h(static_cast<size_t>(b->rows_pad) * nxs, 0.0);
+        for (int r = 0; r < b->rows; ++r)
+            for (int q = 0; q < nxs; ++q)
+                h[(static_cast<size_t>(r / kLanes) * nxs + q) * kLanes + r % kLanes] = in[static_cast<size_t>(r) * nxs + q];
+        HIPCHK(hipMemcpyAsync(b->xs, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+    });
+}
+
 int g2g_batch_solve_adjoint(g2g_batch_t b, const double* rhs, double* x)
 {
     return guarded([&] {
         if (!b || !rhs || !x) throw std::invalid_argument("null argument");
+        if (b->plan->ext()) throw std::invalid_argument("the adjoint is not available with extension rows");
         if (b->precision == G2G_MIXED) adjoint_impl<float>(b, rhs, x);
         else adjoint_impl<double>(b, rhs, x);
     });
@@ -1772,6 +2017,7 @@ int g2g_batch_set_bus_mask(g2g_batch_t b, const int32_t* row_ptr, const int32_t*
             return;
         }
         if (b->streaming) throw std::invalid_argument("streamed rows take no bus mask");
+        if (b->plan->ext()) throw std::invalid_argument("bus masks are not available with extension rows");
         if (!buses && row_ptr[rows] > row_ptr[0]) throw std::invalid_argument("null buses");
         check_rows(b, rows, 0);
         std::vector<unsigned char> h(rp * nr.n, 0);
@@ -1795,6 +2041,7 @@ int g2g_batch_stream_begin(g2g_batch_t b)
     return guarded([&] {
         if (!b) throw std::invalid_argument("null batch");
         if (b->mask) throw std::invalid_argument("streamed rows take no bus mask: clear it first");
+        if (b->plan->ext()) throw std::invalid_argument("streamed rows are not available with extension rows");
         const size_t rp = static_cast<size_t>(b->rows_pad);
         if (!b->it0) b->it0 = dalloc<int>(rp);
         k_fill_int<<<blocks_for(rp, 256), 256, 0, b->stream>>>(b->it0, 0, rp);

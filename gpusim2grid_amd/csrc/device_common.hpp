@@ -37,9 +37,49 @@ struct MismatchArgs {
                                 // marks it for a restart instead of ending it)
     int splits;                 // workgroups per tile (bus ranges); <= 1: one, which also
                                 // applies the status rule
-    double* part;               // splits > 1: per split, ||F||inf then max |J| partials
-                                // [2][splits][part_stride]
+    double* part;               // nullable: per split, ||F||inf then max |J| partials
+                                // [2][splits][part_stride]; set, the status rule is left
+                                // to k_mis_status (several splits, or extension rows)
     int part_stride;            // rows_pad
+    const double2* sx;          // nullable: extension buses' residual terms [T][n_xb][64]
+    int n_xb;
+};
+
+// Extension rows (distributed slack, HVDC droop, voltage control): one lane
+// per row, one wavefront per tile; the tables are the plan's ExtTables.
+template <typename Real>
+struct ExtArgs {
+    const double2* V;           // [T][n][64]
+    const double2* S;           // [T][n][64]
+    double* xs;                 // [T][n_xs][64] state: sa, then Q of each controller
+    double2* sx;                // [T][n_xb][64] residual terms (dF_P, dF_Q) per extension bus
+    Real* A;                    // A stream [T][a_stride][64]
+    const Real* X;              // solved step [T][d][64]
+    const int* tile_active;
+    const int* status;
+    int rows, n, d, a_stride, n_xs, n_xb, sa, q0;
+    const int* inj_ptr;         // [n_xb + 1]
+    const int* inj;             // {kind, index} per record
+    const double* ds_w;
+    int n_jrec;
+    const int* jrec;            // {A position, kind, index, add}
+    const double* jconst;
+    int n_frec;
+    const int* frec;            // {A position of F, kind, index}
+    const int* xcol;            // [n_xs] J column of each state value
+    const int* h_bus1;
+    const int* h_bus2;
+    const int* h_status;
+    const double* h_par;        // per line: p0, k, lf1, lf2, r, pmax12, pmax21
+    const int* vc_reg;
+    const int* vc_ptr;
+    const double* vc_vset;
+    const double* vc_slope;
+    const double* vc_w;
+    const int* sh_c1;
+    const int* sh_ck;
+    double* part;               // mismatch partials (see MismatchArgs)
+    int part_stride, splits;
 };
 
 // limit-violation check of solved rows (one wavefront per row)

@@ -78,7 +78,7 @@ constexpr int kMisBuses = 1;  // buses a wave works on at once
 
 // The running sums of one bus of k_mismatch.
 struct MisBus {
-    int i, e0, e1, fp, fq;
+    int i, e0, e1, fp, fq, xb;
     bool ok, masked;
     double2 vi, sb, I;
     double dr, di;
@@ -214,12 +214,13 @@ __global__ __launch_bounds__(kLanes* kMismatchWaves) void k_mismatch(MismatchArg
             for (int g = 0; g < kMisBuses; ++g) {
                 const int bb = bb0 + g * stride;
                 b[g].ok = bb < a.nbus;
-                const int* bus = a.k1_bus + 6 * (b[g].ok ? bb : a.nbus - 1);
+                const int* bus = a.k1_bus + kK1BusWords * (b[g].ok ? bb : a.nbus - 1);
                 b[g].i = bus[0];
                 b[g].e0 = bus[1];
                 b[g].e1 = bus[2];
                 b[g].fp = bus[3];
                 b[g].fq = bus[4];
+                b[g].xb = a.sx ? bus[6] : -1;
                 b[g].vi = a.V[tidx(tile, a.n, b[g].i, lane)];
                 b[g].sb = a.S[tidx(tile, a.n, b[g].i, lane)];
                 // an islanded bus (apply_bus_mask_kernel, acpf_nr_kernels.cu:219-244):
@@ -246,11 +247,14 @@ __global__ __launch_bounds__(kLanes* kMismatchWaves) void k_mismatch(MismatchArg
                 }
                 const MisBus& c = b[g];
                 const double2 sc = cmulc(c.vi, c.I);  // V conj(I)
-                const double fpv = c.masked ? 0.0 : -(sc.x - c.sb.x);
+                // an extension bus's residual terms (distributed slack share,
+                // HVDC flows, controller reactive output)
+                const double2 ex = c.xb >= 0 ? a.sx[tidx(tile, a.n_xb, c.xb, lane)] : make_double2(0.0, 0.0);
+                const double fpv = c.masked ? 0.0 : -(sc.x - c.sb.x) + ex.x;
                 rmax = nan_max(rmax, fpv);
                 stream_store(&a.A[tidx(tile, a.a_stride, c.fp, lane)], static_cast<Real>(fpv));
                 if (c.fq >= 0) {
-                    const double fqv = c.masked ? 0.0 : -(sc.y - c.sb.y);
+                    const double fqv = c.masked ? 0.0 : -(sc.y - c.sb.y) + ex.y;
                     rmax = nan_max(rmax, fqv);
                     stream_store(&a.A[tidx(tile, a.a_stride, c.fq, lane)], static_cast<Real>(fqv));
                 }
@@ -286,8 +290,9 @@ __global__ __launch_bounds__(kLanes* kMismatchWaves) void k_mismatch(MismatchArg
         r = res_max(r, s_res[w][lane]);
         jm = fmax(jm, s_jmax[w][lane]);
     }
-    if (splits > 1) {
-        // k_mis_status combines the splits and applies the status rule
+    if (a.part) {
+        // k_mis_status combines the splits (and the extension rows) and
+        // applies the status rule
         const size_t q = static_cast<size_t>(split) * a.part_stride + tile * kLanes + lane;
         a.part[q] = r;
         a.part[q + static_cast<size_t>(splits) * a.part_stride] = jm;
@@ -311,6 +316,169 @@ __global__ __launch_bounds__(kLanes) void k_mis_status(MismatchArgs<Real> a)
         jm = fmax(jm, a.part[static_cast<size_t>(a.splits + s) * a.part_stride + row]);
     }
     mis_status<Real, WithJ>(a, tile, lane, r, jm);
+}
+
+// ---------------------------------------------------------------------------
+// Extension rows (the reference's MultiSlack, HVDC droop and VoltageControl
+// features, acpf_nr_kernels.cu:793-1068, wired as nr_iter_step.cuh:302-413).
+// One lane per row, one wavefront per tile.
+//   k_ext_pre     before the mismatch: each extension bus's residual terms
+//                 dF_P = -sa w - (HVDC flows leaving the bus), dF_Q = +Q(c)
+//   k_ext_post    after it: the voltage-control rows' F, the extension J
+//                 entries (weights, HVDC slopes, +-1, SVC slopes, sharing
+//                 weights), folded into the row's residual and |J| partials
+//   k_ext_update  after the V update: sa and Q(c) take their step
+//   k_ext_init    at the start: sa = sum Re(Sbus), Q(c) = 0
+// ---------------------------------------------------------------------------
+
+// power the non-controlling end receives for |p| sent by the controlling end
+__device__ __forceinline__ double hvdc_received(double p_abs, double lf_send, double lf_recv, double r)
+{
+    const double line = (1.0 - lf_send) * p_abs;
+    return (1.0 - lf_recv) * (line - r * line * line);
+}
+
+__device__ __forceinline__ double bus_angle(double2 v) { return atan2(v.y, v.x); }
+
+// active power leaving each end bus into line e (p1 at bus1, p2 at bus2)
+template <typename Real>
+__device__ __forceinline__ void hvdc_flows(const ExtArgs<Real>& a, size_t tile, int lane, int e, double& p1,
+                                           double& p2)
+{
+    const double* q = a.h_par + 7 * e;  // p0, k, lf1, lf2, r, pmax12, pmax21
+    const int st = a.h_status[e];
+    if (st == kHvdcTripped) {
+        p1 = p2 = 0.0;
+    } else if (st > 0) {
+        p1 = q[5];
+        p2 = -hvdc_received(q[5], q[2], q[3], q[4]);
+    } else if (st < 0) {
+        p1 = -hvdc_received(q[6], q[3], q[2], q[4]);
+        p2 = q[6];
+    } else {
+        const double t1 = bus_angle(a.V[tidx(tile, a.n, a.h_bus1[e], lane)]);
+        const double t2 = bus_angle(a.V[tidx(tile, a.n, a.h_bus2[e], lane)]);
+        const double raw = q[0] + q[1] * (t1 - t2);
+        if (raw >= 0.0) {
+            p1 = raw;
+            p2 = -hvdc_received(raw, q[2], q[3], q[4]);
+        } else {
+            p1 = -hvdc_received(-raw, q[3], q[2], q[4]);
+            p2 = -raw;
+        }
+    }
+}
+
+// dp1/dtheta1 and dp2/dtheta1 of a line in its linear regime (d/dtheta2 is the negative)
+template <typename Real>
+__device__ __forceinline__ void hvdc_slopes(const ExtArgs<Real>& a, size_t tile, int lane, int e, double& s1,
+                                            double& s2)
+{
+    const double* q = a.h_par + 7 * e;
+    const double t1 = bus_angle(a.V[tidx(tile, a.n, a.h_bus1[e], lane)]);
+    const double t2 = bus_angle(a.V[tidx(tile, a.n, a.h_bus2[e], lane)]);
+    const double raw = q[0] + q[1] * (t1 - t2);
+    const bool from1 = raw >= 0.0;
+    const double line = (1.0 - (from1 ? q[2] : q[3])) * fabs(raw);
+    const double recv_slope = (1.0 - q[2]) * (1.0 - q[3]) * (1.0 - 2.0 * q[4] * line);
+    s1 = from1 ? q[1] : q[1] * recv_slope;
+    s2 = from1 ? -q[1] * recv_slope : -q[1];
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kLanes) void k_ext_pre(ExtArgs<Real> a)
+{
+    const int lane = threadIdx.x;
+    const size_t tile = blockIdx.x;
+    if (a.tile_active[tile] <= 0) return;
+    const double sa = a.sa >= 0 ? a.xs[tidx(tile, a.n_xs, a.sa, lane)] : 0.0;
+    for (int xb = 0; xb < a.n_xb; ++xb) {
+        double2 acc = make_double2(0.0, 0.0);
+        for (int r = a.inj_ptr[xb]; r < a.inj_ptr[xb + 1]; ++r) {
+            const int kind = a.inj[2 * r], idx = a.inj[2 * r + 1];
+            if (kind == kExtInjDs) {
+                acc.x -= sa * a.ds_w[idx];
+            } else if (kind == kExtInjVc) {
+                acc.y += a.xs[tidx(tile, a.n_xs, a.q0 + idx, lane)];
+            } else {
+                double p1, p2;
+                hvdc_flows(a, tile, lane, idx, p1, p2);
+                acc.x -= kind == kExtInjHvdc1 ? p1 : p2;
+            }
+        }
+        a.sx[tidx(tile, a.n_xb, xb, lane)] = acc;
+    }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kLanes) void k_ext_post(ExtArgs<Real> a)
+{
+    const int lane = threadIdx.x;
+    const size_t tile = blockIdx.x;
+    if (a.tile_active[tile] <= 0) return;
+    auto Q = [&](int c) { return a.xs[tidx(tile, a.n_xs, a.q0 + c, lane)]; };
+    double rmax = 0.0, jm = 0.0;
+    for (int f = 0; f < a.n_frec; ++f) {
+        const int* fr = a.frec + 3 * f;
+        double F;
+        if (fr[1] == kExtFVc) {
+            const int g = fr[2];
+            const double2 v = a.V[tidx(tile, a.n, a.vc_reg[g], lane)];
+            double t = sqrt(fma(v.x, v.x, v.y * v.y)) - a.vc_vset[g];
+            for (int c = a.vc_ptr[g]; c < a.vc_ptr[g + 1]; ++c) t = fma(a.vc_slope[c], Q(c), t);
+            F = -t;
+        } else {
+            const int c1 = a.sh_c1[fr[2]], ck = a.sh_ck[fr[2]];
+            F = -(a.vc_w[c1] * Q(ck) - a.vc_w[ck] * Q(c1));
+        }
+        rmax = nan_max(rmax, F);
+        a.A[tidx(tile, a.a_stride, fr[0], lane)] = static_cast<Real>(F);
+    }
+    for (int j = 0; j < a.n_jrec; ++j) {
+        const int* jr = a.jrec + 4 * j;
+        double val;
+        if (jr[1] == kExtJConst) {
+            val = a.jconst[jr[2]];
+        } else {
+            double s1, s2;
+            hvdc_slopes(a, tile, lane, jr[2] >> 2, s1, s2);
+            const int q = jr[2] & 3;  // 0: d1/d1, 1: d1/d2, 2: d2/d1, 3: d2/d2
+            val = q == 0 ? s1 : q == 1 ? -s1 : q == 2 ? s2 : -s2;
+        }
+        Real* p = &a.A[tidx(tile, a.a_stride, jr[0], lane)];
+        const Real v = jr[3] ? *p + static_cast<Real>(val) : static_cast<Real>(val);
+        *p = v;
+        jm = fmax(jm, fabs(static_cast<double>(v)));
+    }
+    const size_t row = tile * kLanes + lane;
+    const size_t jq = static_cast<size_t>(a.splits) * a.part_stride + row;
+    a.part[row] = res_max(a.part[row], rmax);
+    a.part[jq] = fmax(a.part[jq], jm);
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kLanes) void k_ext_update(ExtArgs<Real> a)
+{
+    const int lane = threadIdx.x;
+    const size_t tile = blockIdx.x;
+    if (a.tile_active[tile] <= 0) return;
+    const int row = static_cast<int>(tile) * kLanes + lane;
+    if (row >= a.rows || a.status[row] != kActive) return;
+    for (int q = 0; q < a.n_xs; ++q)
+        a.xs[tidx(tile, a.n_xs, q, lane)] += static_cast<double>(a.X[tidx(tile, a.d, a.xcol[q], lane)]);
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kLanes) void k_ext_init(ExtArgs<Real> a)
+{
+    const int lane = threadIdx.x;
+    const size_t tile = blockIdx.x;
+    if (a.sa >= 0) {
+        double s = 0.0;
+        for (int i = 0; i < a.n; ++i) s += a.S[tidx(tile, a.n, i, lane)].x;
+        a.xs[tidx(tile, a.n_xs, a.sa, lane)] = s;
+    }
+    for (int q = a.q0; q < a.n_xs; ++q) a.xs[tidx(tile, a.n_xs, q, lane)] = 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1785,6 +1953,15 @@ template __global__ void k_backward<double, 2>(BackwardArgs<double>);
 template __global__ void k_backward<float, 2>(BackwardArgs<float>);
 template __global__ void k_backward<double, 4>(BackwardArgs<double>);
 template __global__ void k_backward<float, 4>(BackwardArgs<float>);
+template __global__ void k_ext_pre<double>(ExtArgs<double>);
+template __global__ void k_ext_pre<float>(ExtArgs<float>);
+template __global__ void k_ext_post<double>(ExtArgs<double>);
+template __global__ void k_ext_post<float>(ExtArgs<float>);
+template __global__ void k_ext_update<double>(ExtArgs<double>);
+template __global__ void k_ext_update<float>(ExtArgs<float>);
+template __global__ void k_ext_init<double>(ExtArgs<double>);
+template __global__ void k_ext_init<float>(ExtArgs<float>);
+template __global__ void k_move_rows<double>(const double*, double*, const int*, int, int, int);
 template __global__ void k_mis_status<double, true>(MismatchArgs<double>);
 template __global__ void k_mis_status<double, false>(MismatchArgs<double>);
 template __global__ void k_mis_status<float, true>(MismatchArgs<float>);

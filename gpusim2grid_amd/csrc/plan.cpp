@@ -2,6 +2,7 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <numeric>
 #include <memory>
@@ -790,8 +791,59 @@ void build_backward(SymbolicLU& s, const std::vector<int>& group, int W)
 
 }  // namespace
 
+namespace {
+
+// Checks an extension spec against the grid (bus kinds: -1 fixed, 0 pv, 1 pq).
+void check_ext(const ExtSpec& x, int n, int slack, const std::vector<int>& kind)
+{
+    auto bus_ok = [&](int b) { return b >= 0 && b < n; };
+    if (x.has_ds()) {
+        if (static_cast<int>(x.slack_w.size()) != n) throw std::invalid_argument("slack_weights needs n entries");
+        for (int i = 0; i < n; ++i) {
+            if (!std::isfinite(x.slack_w[i])) throw std::invalid_argument("slack weight is not finite");
+            if (x.slack_w[i] != 0.0 && kind[i] < 0 && i != slack)
+                throw std::invalid_argument("a distributed-slack participant must be a pv or pq bus or the slack");
+        }
+    }
+    const int nh = x.n_hvdc();
+    for (const auto* v : {&x.h_bus2, &x.h_status})
+        if (static_cast<int>(v->size()) != nh) throw std::invalid_argument("hvdc arrays differ in length");
+    for (const auto* v : {&x.h_p0, &x.h_k, &x.h_lf1, &x.h_lf2, &x.h_r, &x.h_pmax12, &x.h_pmax21})
+        if (static_cast<int>(v->size()) != nh) throw std::invalid_argument("hvdc arrays differ in length");
+    for (int e = 0; e < nh; ++e) {
+        if (!bus_ok(x.h_bus1[e]) || !bus_ok(x.h_bus2[e]) || x.h_bus1[e] == x.h_bus2[e])
+            throw std::invalid_argument("bad hvdc end bus");
+        const int st = x.h_status[e];
+        if (st != 0 && st != 1 && st != -1 && st != kHvdcTripped) throw std::invalid_argument("bad hvdc status");
+    }
+    const int ng = x.n_grp(), nc = x.n_ctrl();
+    if (ng > 0) {
+        if (static_cast<int>(x.vc_vset.size()) != ng || static_cast<int>(x.vc_ptr.size()) != ng + 1 ||
+            x.vc_ptr[0] != 0 || x.vc_ptr[ng] != nc || static_cast<int>(x.vc_slope.size()) != nc ||
+            static_cast<int>(x.vc_w.size()) != nc)
+            throw std::invalid_argument("voltage-control arrays are inconsistent");
+        for (int g = 0; g < ng; ++g) {
+            if (x.vc_ptr[g + 1] <= x.vc_ptr[g]) throw std::invalid_argument("a voltage-control group has no controller");
+            if (!bus_ok(x.vc_reg[g]) || kind[x.vc_reg[g]] != 1)
+                throw std::invalid_argument("a regulated bus must be a pq bus");
+            for (int g2 = 0; g2 < g; ++g2)
+                if (x.vc_reg[g2] == x.vc_reg[g]) throw std::invalid_argument("two groups regulate one bus");
+        }
+        for (int c = 0; c < nc; ++c) {
+            if (!bus_ok(x.vc_bus[c]) || kind[x.vc_bus[c]] != 1)
+                throw std::invalid_argument("a voltage-control controller must sit at a pq bus");
+            if (!(x.vc_w[c] > 0.0) || !std::isfinite(x.vc_slope[c]))
+                throw std::invalid_argument("controller weights must be positive, slopes finite");
+        }
+    } else if (nc > 0) {
+        throw std::invalid_argument("voltage-control controllers without groups");
+    }
+}
+
+}  // namespace
+
 NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int npv, const int* pv,
-                     int npq, const int* pq, int slack, int walkers, int bwalkers)
+                     int npq, const int* pq, int slack, int walkers, int bwalkers, const ExtSpec* ext)
 {
     if (n <= 1) throw std::invalid_argument("the grid needs at least two buses");
     NrPlan P;
@@ -800,7 +852,6 @@ NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int 
     P.npv = npv;
     P.npq = npq;
     P.npvpq = npv + npq;
-    P.d = P.npvpq + npq;
     P.slack = slack;
     P.Yp.assign(Yp, Yp + n + 1);
     P.Yi.assign(Yi, Yi + P.nY);
@@ -826,64 +877,140 @@ NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int 
     // the reference's array mode treats them (acpf_nr.cu:644-647,1732-1733);
     // `slack` names the fixed bus that anchors the angles
     if (slack < 0 || slack >= n || kind[slack] >= 0) throw std::invalid_argument("bad slack bus");
-    // reference layout (acpf_nr.cu:645-647, 678-685): sorted pvpq, pq as given
+    if (ext && !ext->empty()) {
+        check_ext(*ext, n, slack, kind);
+        P.ext = *ext;
+    }
+    const ExtSpec& X = P.ext;
+    const int nc = X.n_ctrl(), ng = X.n_grp();
+    // Layout.  Rows [P of sorted pvpq | Q of pq in input order] and columns
+    // [theta of pvpq | |V| of pq] (acpf_nr.cu:645-647,678-685), then the
+    // extension rows / columns, paired by index (each pair is a diagonal
+    // entry of the static-pivot factorization):
+    //   distributed slack: the slack's P row  <->  sa (the absorbed power);
+    //   per voltage-control group with controllers c_1..c_N: the voltage row
+    //   <-> Q(c_1), then the sharing row of c_k <-> Q(c_k), k = 2..N.
     P.pvpq.assign(pv, pv + npv);
     P.pvpq.insert(P.pvpq.end(), pq, pq + npq);
     std::sort(P.pvpq.begin(), P.pvpq.end());
     P.pq.assign(pq, pq + npq);
     P.row_p.assign(static_cast<size_t>(n), -1);
     P.row_q.assign(static_cast<size_t>(n), -1);
-    for (int k = 0; k < P.npvpq; ++k) P.row_p[P.pvpq[k]] = k;
+    P.col_th.assign(static_cast<size_t>(n), -1);
+    for (int k = 0; k < P.npvpq; ++k) P.row_p[P.pvpq[k]] = P.col_th[P.pvpq[k]] = k;
     for (int k = 0; k < npq; ++k) P.row_q[pq[k]] = P.npvpq + k;
-    // reference J CSR: row r holds one coefficient per (Ybus entry, block),
-    // columns ascending
+    const std::vector<int>& col_vm = P.row_q;
+    int nxt = P.npvpq + npq;
+    ExtTables& xt = P.xt;
+    if (X.has_ds()) {
+        P.row_p[slack] = xt.sa_col = nxt++;
+        xt.sa = xt.n_xs++;
+    }
+    std::vector<int> v_row(static_cast<size_t>(ng), -1), q_col(static_cast<size_t>(nc), -1);
+    xt.q0 = xt.n_xs;
+    xt.n_xs += nc;
+    for (int g = 0; g < ng; ++g)
+        for (int c = X.vc_ptr[g]; c < X.vc_ptr[g + 1]; ++c) {
+            q_col[c] = nxt++;
+            if (c == X.vc_ptr[g]) v_row[g] = q_col[c];
+        }
+    P.n_ext = nxt - (P.npvpq + npq);
+    P.d = nxt;
     const int d = P.d;
-    struct Ent { int col, yk, blk; };
-    std::vector<std::vector<Ent>> rows(static_cast<size_t>(d));
+
+    // J pattern: every (row, col) a coefficient lands on, with its sources
+    struct Src { int col, kind, a, b; };  // kind 0: Ybus entry a, block b; 1: extension record a
+    enum { kY = 0, kX = 1 };
+    std::vector<std::vector<Src>> rows(static_cast<size_t>(d));
     for (int i = 0; i < n; ++i)
         for (int a = 0; a < 2; ++a) {
             const int r = a == 0 ? P.row_p[i] : P.row_q[i];
             if (r < 0) continue;
             for (int p = Yp[i]; p < Yp[i + 1]; ++p) {
                 const int j = Yi[p];
-                if (P.row_p[j] >= 0) rows[r].push_back({P.row_p[j], p, a == 0 ? 0 : 2});
-                if (P.row_q[j] >= 0) rows[r].push_back({P.row_q[j], p, a == 0 ? 1 : 3});
+                if (P.col_th[j] >= 0) rows[r].push_back({P.col_th[j], kY, p, a == 0 ? 0 : 2});
+                if (col_vm[j] >= 0) rows[r].push_back({col_vm[j], kY, p, a == 0 ? 1 : 3});
             }
         }
+    // extension J sources: {row, col, kind, index} (kind: kExtJConst with a
+    // value, kExtJHvdc with 4 * line + q)
+    struct XJ { int row, col, kind, idx; double val; };
+    std::vector<XJ> xj;
+    if (X.has_ds())
+        for (int i = 0; i < n; ++i)
+            if (X.slack_w[i] != 0.0 || i == slack) xj.push_back({P.row_p[i], xt.sa_col, kExtJConst, 0, X.slack_w[i]});
+    for (int e = 0; e < X.n_hvdc(); ++e) {
+        if (X.h_status[e] != 0) continue;  // saturated or tripped: constant flow, no slope
+        const int bb[2] = {X.h_bus1[e], X.h_bus2[e]};
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) {
+                const int r = P.row_p[bb[a]], c = P.col_th[bb[b]];
+                if (r >= 0 && c >= 0) xj.push_back({r, c, kExtJHvdc, 4 * e + 2 * a + b, 0.0});
+            }
+    }
+    for (int g = 0; g < ng; ++g) {
+        const int c1 = X.vc_ptr[g];
+        xj.push_back({v_row[g], col_vm[X.vc_reg[g]], kExtJConst, 0, 1.0});
+        for (int c = c1; c < X.vc_ptr[g + 1]; ++c) {
+            xj.push_back({P.row_q[X.vc_bus[c]], q_col[c], kExtJConst, 0, -1.0});
+            xj.push_back({v_row[g], q_col[c], kExtJConst, 0, X.vc_slope[c]});
+            if (c > c1) {  // sharing row of c: w_1 Q_c - w_c Q_1 = 0
+                xj.push_back({q_col[c], q_col[c], kExtJConst, 0, X.vc_w[c1]});
+                xj.push_back({q_col[c], q_col[c1], kExtJConst, 0, -X.vc_w[c]});
+            }
+        }
+    }
+    for (size_t q = 0; q < xj.size(); ++q) rows[xj[q].row].push_back({xj[q].col, kX, static_cast<int>(q), 0});
     P.Jp.assign(static_cast<size_t>(d) + 1, 0);
     for (int b = 0; b < 4; ++b) P.map[b].assign(static_cast<size_t>(P.nY), -1);
+    std::vector<int> xj_pos(xj.size(), -1);
+    std::vector<char> pos_has_y;  // per CSR position: a Ybus source writes it
     for (int r = 0; r < d; ++r) {
         auto& v = rows[r];
-        std::sort(v.begin(), v.end(), [](const Ent& x, const Ent& y) { return x.col < y.col; });
-        for (const Ent& e : v) {
-            P.map[e.blk][e.yk] = static_cast<int>(P.Ji.size());
-            P.Ji.push_back(e.col);
+        std::stable_sort(v.begin(), v.end(), [](const Src& x, const Src& y) { return x.col < y.col; });
+        for (size_t q = 0; q < v.size(); ++q) {
+            if (q == 0 || v[q].col != v[q - 1].col) {
+                P.Ji.push_back(v[q].col);
+                pos_has_y.push_back(0);
+            }
+            const int pos = static_cast<int>(P.Ji.size()) - 1;
+            if (v[q].kind == kY) {
+                P.map[v[q].b][v[q].a] = pos;
+                pos_has_y[pos] = 1;
+            } else {
+                xj_pos[v[q].a] = pos;
+            }
         }
         P.Jp[r + 1] = static_cast<int>(P.Ji.size());
     }
     P.nJ = P.Jp[d];
 
-    // bus-level minimum degree (fixed buses excluded), weight = unknowns per bus
+    // bus-level minimum degree over the buses with unknowns (P and Q of a
+    // bus eliminated together), weight = unknowns per bus; the extension
+    // couplings (HVDC ends, controllers and their regulated bus) are edges
     std::vector<int> bus_ids, bus_idx(static_cast<size_t>(n), -1);
     for (int i = 0; i < n; ++i)
         if (kind[i] >= 0) { bus_idx[i] = static_cast<int>(bus_ids.size()); bus_ids.push_back(i); }
     const int nb = static_cast<int>(bus_ids.size());
     std::vector<int> badjp(static_cast<size_t>(nb) + 1, 0), badji, w(static_cast<size_t>(nb));
-    for (int b = 0; b < nb; ++b) {
-        const int i = bus_ids[b];
-        w[b] = kind[i] == 1 ? 2 : 1;
-        for (int p = Yp[i]; p < Yp[i + 1]; ++p) {
-            const int j = Yi[p];
-            if (j != i && kind[j] >= 0) badji.push_back(bus_idx[j]);
-        }
-        badjp[b + 1] = static_cast<int>(badji.size());
-    }
-    // symmetrise the bus graph (Ybus patterns are symmetric, this guards others)
     {
         std::vector<std::vector<int>> nbv(static_cast<size_t>(nb));
-        for (int b = 0; b < nb; ++b)
-            for (int p = badjp[b]; p < badjp[b + 1]; ++p) { nbv[b].push_back(badji[p]); nbv[badji[p]].push_back(b); }
-        badji.clear();
+        auto edge = [&](int i, int j) {
+            if (i == j || bus_idx[i] < 0 || bus_idx[j] < 0) return;
+            nbv[bus_idx[i]].push_back(bus_idx[j]);
+            nbv[bus_idx[j]].push_back(bus_idx[i]);
+        };
+        for (int b = 0; b < nb; ++b) {
+            const int i = bus_ids[b];
+            w[b] = kind[i] == 1 ? 2 : 1;
+            for (int p = Yp[i]; p < Yp[i + 1]; ++p) edge(i, Yi[p]);
+        }
+        for (int e = 0; e < X.n_hvdc(); ++e) edge(X.h_bus1[e], X.h_bus2[e]);
+        for (int g = 0; g < ng; ++g)
+            for (int c = X.vc_ptr[g]; c < X.vc_ptr[g + 1]; ++c) {
+                edge(X.vc_bus[c], X.vc_reg[g]);
+                edge(X.vc_bus[c], X.vc_bus[X.vc_ptr[g]]);
+            }
         for (int b = 0; b < nb; ++b) {
             auto& v = nbv[b];
             std::sort(v.begin(), v.end());
@@ -900,13 +1027,91 @@ NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int 
         order.push_back(P.row_p[i]);
         if (P.row_q[i] >= 0) order.push_back(P.row_q[i]);
     }
-    // walkers: two for large systems (the kernels keep one otherwise);
-    // G2G_WALKERS=1|2 overrides
+    // extension pivots last: each group's sharing rows, then its voltage row
+    // (its pivot is the group's weighted sensitivity once the buses and the
+    // sharing rows are eliminated), the distributed slack at the very end
+    for (int g = 0; g < ng; ++g) {
+        for (int c = X.vc_ptr[g] + 1; c < X.vc_ptr[g + 1]; ++c) order.push_back(q_col[c]);
+        order.push_back(v_row[g]);
+    }
+    if (X.has_ds()) order.push_back(xt.sa_col);
+    if (static_cast<int>(order.size()) != d) throw std::logic_error("plan: the pivot order misses unknowns");
     P.lu = build_symbolic(d, P.Jp.data(), P.Ji.data(), order, walkers > 0 ? walkers : default_walkers(d),
                           bwalkers > 0 ? bwalkers : default_bwalkers(d));
 
-    // mismatch/Jacobian program: buses in RCM order of the Ybus graph
+    // extension tables in A-stream positions
+    if (!X.empty()) {
+        std::vector<int> xb_of(static_cast<size_t>(n), -1);
+        std::vector<std::vector<std::pair<int, int>>> inj;
+        auto add_inj = [&](int bus, int k, int idx) {
+            if (xb_of[bus] < 0) {
+                xb_of[bus] = static_cast<int>(xt.xb_bus.size());
+                xt.xb_bus.push_back(bus);
+                inj.emplace_back();
+            }
+            inj[xb_of[bus]].push_back({k, idx});
+        };
+        if (X.has_ds())
+            for (int i = 0; i < n; ++i)
+                if (X.slack_w[i] != 0.0) {
+                    add_inj(i, kExtInjDs, static_cast<int>(xt.ds_w.size()));
+                    xt.ds_w.push_back(X.slack_w[i]);
+                }
+        for (int e = 0; e < X.n_hvdc(); ++e) {
+            if (P.row_p[X.h_bus1[e]] >= 0) add_inj(X.h_bus1[e], kExtInjHvdc1, e);
+            if (P.row_p[X.h_bus2[e]] >= 0) add_inj(X.h_bus2[e], kExtInjHvdc2, e);
+        }
+        for (int c = 0; c < nc; ++c) add_inj(X.vc_bus[c], kExtInjVc, c);
+        xt.inj_ptr.assign(1, 0);
+        for (const auto& v : inj) {
+            for (const auto& [k, idx] : v) xt.inj.insert(xt.inj.end(), {k, idx});
+            xt.inj_ptr.push_back(static_cast<int>(xt.inj.size() / 2));
+        }
+        // J records, the first of a position without a Ybus source stores
+        std::vector<char> seen(static_cast<size_t>(P.nJ), 0);
+        for (size_t q = 0; q < xj.size(); ++q) {
+            const int pos = xj_pos[q];
+            const int apos = P.lu.a_pos_of_entry[pos];
+            if (pos < 0 || apos < 0 || apos >= P.lu.a_len) throw std::logic_error("plan: extension entry off the A stream");
+            const int add = (pos_has_y[pos] || seen[pos]) ? 1 : 0;
+            seen[pos] = 1;
+            int idx = xj[q].idx;
+            if (xj[q].kind == kExtJConst) {
+                idx = static_cast<int>(xt.jconst.size());
+                xt.jconst.push_back(xj[q].val);
+            }
+            xt.jrec.insert(xt.jrec.end(), {apos, xj[q].kind, idx, add});
+        }
+        for (int r = P.npvpq + npq; r < d; ++r)
+            if (P.lu.f_pos[r] < 0 || P.lu.f_pos[r] >= P.lu.a_len) throw std::logic_error("plan: extension row off the A stream");
+        for (int g = 0; g < ng; ++g) {
+            xt.frec.insert(xt.frec.end(), {P.lu.f_pos[v_row[g]], kExtFVc, g});
+            for (int c = X.vc_ptr[g] + 1; c < X.vc_ptr[g + 1]; ++c) {
+                xt.frec.insert(xt.frec.end(), {P.lu.f_pos[q_col[c]], kExtFShare, static_cast<int>(xt.sh_c1.size())});
+                xt.sh_c1.push_back(X.vc_ptr[g]);
+                xt.sh_ck.push_back(c);
+            }
+        }
+        xt.xcol.assign(static_cast<size_t>(xt.n_xs), -1);
+        if (xt.sa >= 0) xt.xcol[xt.sa] = xt.sa_col;
+        for (int c = 0; c < nc; ++c) xt.xcol[xt.q0 + c] = q_col[c];
+        xt.h_bus1 = X.h_bus1;
+        xt.h_bus2 = X.h_bus2;
+        xt.h_status = X.h_status;
+        for (int e = 0; e < X.n_hvdc(); ++e)
+            xt.h_par.insert(xt.h_par.end(), {X.h_p0[e], X.h_k[e], X.h_lf1[e], X.h_lf2[e], X.h_r[e], X.h_pmax12[e],
+                                             X.h_pmax21[e]});
+        xt.vc_reg = X.vc_reg;
+        xt.vc_ptr = X.vc_ptr;
+        xt.vc_vset = X.vc_vset;
+        xt.vc_slope = X.vc_slope;
+        xt.vc_w = X.vc_w;
+    }
+
+    // mismatch/Jacobian program: buses with a P equation in RCM order of the Ybus graph
     {
+        std::vector<int> xb_of(static_cast<size_t>(n), -1);
+        for (size_t q = 0; q < xt.xb_bus.size(); ++q) xb_of[xt.xb_bus[q]] = static_cast<int>(q);
         std::vector<int> yadjp(static_cast<size_t>(n) + 1, 0), yadji;
         for (int i = 0; i < n; ++i) {
             for (int p = Yp[i]; p < Yp[i + 1]; ++p)
@@ -917,7 +1122,7 @@ NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int 
         P.k1_bus.clear();
         P.k1_ent.clear();
         for (int i : ro) {
-            if (kind[i] < 0) continue;  // fixed bus: no equations
+            if (P.row_p[i] < 0) continue;  // fixed bus: no equations
             const int e0 = static_cast<int>(P.k1_ent.size() / 6);
             int diag = -1;
             for (int p = Yp[i]; p < Yp[i + 1]; ++p) {
@@ -927,13 +1132,13 @@ NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int 
                 for (int b = 0; b < 4; ++b) a[b] = P.map[b][p] >= 0 ? P.lu.a_pos_of_entry[P.map[b][p]] : -1;
                 P.k1_ent.insert(P.k1_ent.end(), {j, p, a[0], a[1], a[2], a[3]});
             }
-            if (diag < 0) throw std::invalid_argument("Ybus has no diagonal entry for a pv or pq bus");
+            if (diag < 0) throw std::invalid_argument("Ybus has no diagonal entry for a bus with equations");
             const int e1 = static_cast<int>(P.k1_ent.size() / 6);
             const int fp = P.lu.f_pos[P.row_p[i]];
             const int fq = P.row_q[i] >= 0 ? P.lu.f_pos[P.row_q[i]] : -1;
-            P.k1_bus.insert(P.k1_bus.end(), {i, e0, e1, fp, fq, diag});
+            P.k1_bus.insert(P.k1_bus.end(), {i, e0, e1, fp, fq, diag, xb_of[i], 0});
         }
-        P.k1_nbus = static_cast<int>(P.k1_bus.size() / 6);
+        P.k1_nbus = static_cast<int>(P.k1_bus.size() / kK1BusWords);
     }
     return P;
 }

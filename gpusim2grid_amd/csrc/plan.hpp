@@ -100,8 +100,56 @@ std::vector<int> rcm_order(int n, const std::vector<int>& adjp, const std::vecto
 // nonzero op (program_format.hpp) completes the bus: with the partner slot's
 // x it applies V <- V ((|V| + dVm) / |V|) exp(j dVa).
 
+// Extension rows of the Newton system (host copy of g2g_extensions; the
+// reference's MultiSlack / HVDC droop / VoltageControl features,
+// acpf_nr_kernels.cu:793-1068, ledger_data.hpp:61-155).  Empty vectors = off.
+struct ExtSpec {
+    std::vector<double> slack_w;  // [n] distributed-slack weights (empty: off)
+    // HVDC lines in angle droop: p = p0 + k (theta1 - theta2), losses and a frozen regime
+    std::vector<int> h_bus1, h_bus2, h_status;  // status 0 linear, +1 / -1 saturated, 2 tripped
+    std::vector<double> h_p0, h_k, h_lf1, h_lf2, h_r, h_pmax12, h_pmax21;
+    // voltage control: group g regulates |V(vc_reg[g])| at vc_vset[g] with
+    // controllers [vc_ptr[g], vc_ptr[g+1]) of vc_bus (their reactive output is
+    // an unknown), SVC slopes and sharing weights
+    std::vector<int> vc_reg, vc_ptr, vc_bus;
+    std::vector<double> vc_vset, vc_slope, vc_w;
+    bool has_ds() const { return !slack_w.empty(); }
+    int n_hvdc() const { return static_cast<int>(h_bus1.size()); }
+    int n_grp() const { return static_cast<int>(vc_reg.size()); }
+    int n_ctrl() const { return static_cast<int>(vc_bus.size()); }
+    bool empty() const { return !has_ds() && n_hvdc() == 0 && n_grp() == 0; }
+};
+
+// Device-side description of the extension rows of a plan (built with the
+// plan; see program_format.hpp for the record kinds).
+struct ExtTables {
+    int n_xs = 0;                 // per-row state values: [sa (distributed slack)] [Q_c per controller]
+    int sa = -1;                  // state index of sa, -1: no distributed slack
+    int q0 = 0;                   // state index of the first controller's Q
+    int sa_col = -1;              // J column of sa (also the slack's P row)
+    std::vector<int> xb_bus;      // buses whose mismatch takes an extension injection
+    std::vector<int> inj_ptr;     // [n_xb + 1] records of ext bus xb
+    std::vector<int> inj;         // records {kind, index} (kExtInj*)
+    std::vector<double> ds_w;     // weight per distributed-slack record
+    std::vector<int> jrec;        // J records {A position, kind, index, add}
+    std::vector<double> jconst;   // values of the kExtJConst records
+    std::vector<int> frec;        // custom-row records {A position of F, kind, group / share index}
+    std::vector<int> xcol;        // per state value, its J column (the step's unknown)
+    // HVDC lines [n_hvdc] and VC groups / shares (copied from the spec)
+    std::vector<int> h_bus1, h_bus2, h_status;
+    std::vector<double> h_par;    // per line: p0, k, lf1, lf2, r, pmax12, pmax21
+    std::vector<int> vc_reg, vc_ptr;
+    std::vector<double> vc_vset, vc_slope, vc_w;
+    std::vector<int> sh_c1, sh_ck;  // per share row: its group's first controller and controller k
+    bool on() const { return n_xs > 0 || !h_bus1.empty(); }
+};
+
 struct NrPlan {
     int n = 0, nY = 0, npv = 0, npq = 0, npvpq = 0, d = 0, nJ = 0, slack = -1;
+    int n_ext = 0;                       // extension rows (= columns) after [P(pvpq) | Q(pq)]
+    ExtSpec ext;
+    ExtTables xt;
+    std::vector<int> col_th;             // per bus: J column of its angle, -1 if none
     std::vector<int> Yp, Yi;
     std::vector<double> Yx;  // complex, interleaved
     std::vector<int> pvpq, pq;           // reference layout
@@ -109,9 +157,10 @@ struct NrPlan {
     std::vector<int> Jp, Ji;             // reference J CSR
     std::vector<int> map[4];             // per Ybus entry: J CSR position (11,12,21,22) or -1
     SymbolicLU lu;
-    // mismatch/Jacobian program: buses in RCM order (slack excluded);
-    // k1_bus[i] = {bus, entry begin, entry end, fpos_p, fpos_q, diag entry}
-    std::vector<int> k1_bus;             // 6 ints per bus
+    // mismatch/Jacobian program: buses with a P equation in RCM order;
+    // k1_bus[i] = {bus, entry begin, entry end, fpos_p, fpos_q, diag entry,
+    // extension bus index or -1, 0}
+    std::vector<int> k1_bus;             // kK1BusWords ints per bus
     // per Ybus entry (in k1 order): {col j, ybus k, a11, a12, a21, a22}
     std::vector<int> k1_ent;             // 6 ints per entry
     int k1_nbus = 0;
@@ -125,6 +174,7 @@ int default_walkers(int d);
 // backward walkers: four when d >= kWalkersMinD, else one; G2G_BWALKERS=1|2|4 overrides
 int default_bwalkers(int d);
 NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int npv, const int* pv,
-                     int npq, const int* pq, int slack, int walkers = 0, int bwalkers = 0);
+                     int npq, const int* pq, int slack, int walkers = 0, int bwalkers = 0,
+                     const ExtSpec* ext = nullptr);
 
 }  // namespace g2g

@@ -101,6 +101,25 @@ __host__ __device__ constexpr int walk_ring_bytes(int real_size, int W)
            kWalkFlagBytes;
 }
 
+// mismatch program: ints per bus record
+constexpr int kK1BusWords = 8;
+
+// Extension rows.  Injection records (an extension bus's residual terms):
+//   kExtInjDs    dF_P -= sa w        (index: distributed-slack record -> weight)
+//   kExtInjHvdc1 dF_P -= p1 flow of line `index`;  kExtInjHvdc2: p2 flow
+//   kExtInjVc    dF_Q += Q of controller `index`
+// J records {A position, kind, index, add}: add = 1 adds onto a value the
+// mismatch wrote (or an earlier record of the same position), 0 stores:
+//   kExtJConst   the value ext_const[index] (weights, slopes, +-1 of the VC rows)
+//   kExtJHvdc    dP1/dtheta1 of line index (4 * line + q: q = 0 d1/d1, 1 d1/d2,
+//                2 d2/d1, 3 d2/d2)
+// F records {A position, kind, index}: kExtFVc  -(|V(reg)| + sum s Q - v_set)
+// of group index; kExtFShare  -(w_1 Q_k - w_k Q_1) of share row index.
+constexpr int kExtInjDs = 0, kExtInjHvdc1 = 1, kExtInjHvdc2 = 2, kExtInjVc = 3;
+constexpr int kExtJConst = 0, kExtJHvdc = 1;
+constexpr int kExtFVc = 0, kExtFShare = 1;
+constexpr int kHvdcTripped = 2;
+
 // row status (gpusim2grid contingency/row_status.hpp:33-43) plus "still iterating"
 constexpr int kActive = -1;
 constexpr int kConverged = 0;

@@ -70,6 +70,9 @@ typedef struct {
     int32_t walkers, n_top;                  /* walker wavefronts per tile; pivots of the top part */
     int32_t bwalkers;                        /* walker wavefronts per tile of the backward substitution */
     int32_t u_stride_f64, u_stride_f32;      /* U positions a tile holds in a batch of this precision */
+    int32_t n_ext;                           /* extension rows (= columns) after [P(pvpq) | Q(pq)] */
+    int32_t n_xs;                            /* extension state values per row (sa, then each controller's Q) */
+    int32_t sa_col;                          /* J column of sa (= the slack's P row), -1 without distributed slack */
 } g2g_plan_info;
 
 typedef struct {
@@ -105,6 +108,53 @@ int g2g_set_device(int32_t device);
  * (acpf_nr.cu:644-647,1732-1733); `slack` is one of them. */
 int g2g_plan_create(int32_t n, const int32_t *Yp, const int32_t *Yi, const double *Yx, int32_t npv,
                     const int32_t *pv, int32_t npq, const int32_t *pq, int32_t slack, g2g_plan_t *out);
+
+/* Extension rows of the Newton system (the reference's MultiSlack, HVDC
+ * angle-droop and VoltageControl features of its augmented ledger,
+ * ledger_data.hpp:61-155, acpf_nr_kernels.cu:793-1068, nr_iter_step.cuh:302-413).
+ * Each part is off when its arrays are NULL / its count is 0.
+ *
+ * Layout: rows [P of sorted pvpq | Q of pq | extension rows] and columns
+ * [theta of pvpq | |V| of pq | extension columns]; the extension block is,
+ * in this order: the distributed slack (the slack's P row, column sa), then
+ * per voltage-control group its voltage row (column Q of its first
+ * controller) and one sharing row per further controller (column Q of that
+ * controller).  Per row the solve carries the state sa (starting at
+ * sum Re(Sbus)) and each controller's Q (starting at 0), all FP64.
+ *
+ * Distributed slack: F_P(i) = -(P_i(V) - Re S_i) - sa w_i at every bus with a
+ * P row, J(P_i, sa) = w_i.  Participants (w_i != 0) are pv / pq buses or the
+ * slack, which gets a P row.
+ * HVDC line e (bus1 -> bus2): raw = p0 + k (theta1 - theta2); linear regime
+ * (status 0): the sending end (raw >= 0: bus1) draws |raw| and the other end
+ * receives (1 - lf_r)(L - r L^2) with L = (1 - lf_s)|raw|; status +1 / -1
+ * fixes the sent power at pmax12 / pmax21, 2 trips the line.  The flows
+ * leaving each end enter its P residual; in the linear regime J gets the
+ * exact slopes at (P1, theta1), (P1, theta2), (P2, theta1), (P2, theta2).
+ * Voltage control group g: |V(reg)| + sum_c s_c Q_c = v_set (voltage row),
+ * w_1 Q_c - w_c Q_1 = 0 (sharing rows), and each controller's Q enters the Q
+ * residual of its bus (J(Q_bus, Q_c) = -1).  Regulated and controller buses
+ * are pq buses; weights are positive. */
+typedef struct g2g_extensions {
+    const double *slack_weights;                     /* [n] or NULL */
+    int32_t n_hvdc;
+    const int32_t *hvdc_bus1, *hvdc_bus2, *hvdc_status;
+    const double *hvdc_p0, *hvdc_k, *hvdc_lf1, *hvdc_lf2, *hvdc_r, *hvdc_pmax12, *hvdc_pmax21;
+    int32_t n_vc_groups;
+    const int32_t *vc_reg_bus;                       /* [groups] */
+    const double *vc_v_set;                          /* [groups] */
+    const int32_t *vc_ptr;                           /* [groups + 1] controllers of each group */
+    const int32_t *vc_bus;                           /* [controllers] */
+    const double *vc_slope, *vc_weight;              /* [controllers] */
+} g2g_extensions;
+
+/* g2g_plan_create with extension rows (ext nullable: the bare system).
+ * Batches of such a plan run the plain solve (g2g_batch_solve), evaluation
+ * and results; step scaling, warm starts, streamed rows, bus masks and the
+ * adjoint reject them. */
+int g2g_plan_create_ext(int32_t n, const int32_t *Yp, const int32_t *Yi, const double *Yx, int32_t npv,
+                        const int32_t *pv, int32_t npq, const int32_t *pq, int32_t slack, const g2g_extensions *ext,
+                        g2g_plan_t *out);
 
 /* Plan of a bare square matrix pattern for the raw batched LU (replaces
  * CudssBatchSolver::initialize, cudss_batch_solver.cuh:273-440, behind
@@ -242,6 +292,10 @@ int g2g_batch_limit_violations(g2g_batch_t batch, const g2g_limit_table *table, 
                                g2g_violations *out);
 
 int g2g_batch_get_v(g2g_batch_t batch, double *V, int64_t ld, int32_t on_device);
+/* Extension state of every row (rows x n_xs host arrays: sa, then each
+ * controller's Q).  A solve starts it afresh; the setter serves evaluation. */
+int g2g_batch_get_ext_state(g2g_batch_t batch, double *out);
+int g2g_batch_set_ext_state(g2g_batch_t batch, const double *in);
 int g2g_batch_get_results(g2g_batch_t batch, int32_t *status, int32_t *iters, double *residual);
 
 /* Evaluates, at the current V, F (reference row layout, rows x d) and, when

@@ -243,8 +243,11 @@ def test_gpu_facades_take_the_step_options():
 def _saddle_blocks(nb):
     """A block-diagonal matrix of nb saddle blocks and its block-wise identity
     order (every block's (0, 0) pivot is 0 unless a row changes it)."""
-    blocks = sp.block_diag([SADDLE] * nb, format="csr")
-    blocks.sort_indices()
+    # built by hand: a block_diag would drop SADDLE's explicit zero at (0, 0)
+    indptr = np.arange(3 * nb + 1) * 3
+    indices = (np.repeat(np.arange(nb) * 3, 9) + np.tile(SADDLE.indices, nb)).astype(np.int32)
+    data = np.tile(SADDLE.data, nb)
+    blocks = sp.csr_matrix((data, indices, indptr), shape=(3 * nb, 3 * nb))
     return blocks, np.arange(3 * nb)
 
 
@@ -252,6 +255,7 @@ def test_saddle_blocks_plan_has_two_walkers():
     """The per-row clamp test below runs on a plan whose walk is split over
     two walkers (independent blocks fall into both subtree sets)."""
     A, order = _saddle_blocks(700)
+    assert A.nnz == 9 * 700 and A[3, 3] == 0 and A[3, 4] == 1  # (0, 0) of every block stays in the pattern
     inf = _capi.Plan.from_matrix(A, order=order).info()
     assert inf["walkers"] == 2 and inf["d"] == 2100
 
