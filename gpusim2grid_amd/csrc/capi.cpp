@@ -211,8 +211,8 @@ struct g2g_plan_s {
             d_Y = dupload(y);
             std::vector<int> upd;
             for (int i = 0; i < nr.n; ++i) {
-                if (nr.row_p[i] < 0) continue;  // fixed bus
-                upd.insert(upd.end(), {i, nr.row_p[i], nr.row_q[i]});
+                if (nr.col_th[i] < 0) continue;  // no angle unknown: a fixed bus (or the slack's P row)
+                upd.insert(upd.end(), {i, nr.col_th[i], nr.row_q[i]});
             }
             n_upd = static_cast<int>(upd.size() / 3);
             d_upd = dupload(upd);

@@ -275,7 +275,7 @@ def test_gpu_raw_clamp_counts_are_per_row_on_two_walkers(prec, eps):
     first = A.indptr[0:3 * nb:3]
     vals = np.tile(A.data, (rows, 1))
     zero = rng.random((rows, nb)) < rng.random((rows, 1))  # rows differ in how many blocks clamp
-    vals[:, first] = np.where(zero, 0.0, 5.0)
+    vals[:, first] = np.where(zero, 0.0, 3.0)  # 0: the first pivot clamps; 3: no pivot does
     rhs = rng.normal(size=(rows, 3 * nb))
     _, cl = P.lu_solve(vals, rhs, precision=prec, pivot_eps=eps)
     np.testing.assert_array_equal(cl, zero.sum(axis=1))

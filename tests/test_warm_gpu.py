@@ -58,6 +58,7 @@ def test_warm_start_kernel_matches_oracle_forward(trained):
     plan = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
     B = _capi.Batch(plan, rows)
     B.set_sbus(S)
+    B.set_v(g.flat_start())  # fixed buses keep this start V
     B.warm_start(ws.mlp)
     Vw = B.get_v()
     p = {k: v.astype(np.float64) for k, v in ws.mlp.get_params().items()}
@@ -88,6 +89,7 @@ def test_streamed_slots_take_the_device_warm_start(trained):
     plan = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
     B = _capi.Batch(plan, rows, precision=_capi.MIXED)
     B.set_sbus(S)
+    B.set_v(g.flat_start())
     B.warm_start(ws.mlp)
     Vw = B.get_v()
     Bs = _capi.Batch(plan, slots, precision=_capi.MIXED, per_row_ybus=True)
