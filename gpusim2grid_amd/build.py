@@ -67,10 +67,13 @@ def build_module(verbose=False):
     ext = sysconfig.get_config_var("EXT_SUFFIX")
     out = os.path.join(HERE, "_gpusim2grid" + ext)
     tmp = out + ".tmp"
-    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    # host code only: the HIP runtime API for the sessions' device buffers
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-D__HIP_PLATFORM_AMD__",
            f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}",
-           f"-I{os.path.join(ROOT, 'include')}", src, "-o", tmp,
-           f"-L{HERE}", "-lgpusim2grid_amd", "-Wl,-rpath,$ORIGIN"]
+           f"-I{os.path.join(ROOT, 'include')}", f"-I{rocm}/include", src, "-o", tmp,
+           f"-L{HERE}", "-lgpusim2grid_amd", f"-L{rocm}/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+           f"-Wl,-rpath,{rocm}/lib"]
     _run(cmd, verbose)
     os.replace(tmp, out)
     return out

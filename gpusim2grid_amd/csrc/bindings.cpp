@@ -11,6 +11,7 @@
 #include <pybind11/stl.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -21,6 +22,8 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+
+#include <hip/hip_runtime_api.h>
 
 #include "gpusim2grid_amd.h"
 
@@ -37,6 +40,132 @@ void check(int rc)
     if (rc == G2G_OK) return;
     if (rc == G2G_ERR_INVALID) throw py::value_error(g2g_last_error());
     throw std::runtime_error(g2g_last_error());
+}
+
+void hipchk(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// A device buffer shared by a session and the DLPack tensors aliasing it: a
+// session that needs a larger one allocates anew, so an exported tensor
+// never points to freed memory (it sees no further runs, though).
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int device = 0;
+    explicit DevBuf(size_t b) : bytes(b)
+    {
+        hipchk(hipGetDevice(&device), "hipGetDevice");
+        hipchk(hipMalloc(&p, b ? b : 1), "hipMalloc");
+    }
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    cplx* c() const { return static_cast<cplx*>(p); }
+};
+using DevBufPtr = std::shared_ptr<DevBuf>;
+
+void ensure(DevBufPtr& b, size_t bytes)
+{
+    if (!b || b->bytes < bytes) b = std::make_shared<DevBuf>(bytes);
+}
+
+void upload(DevBufPtr& b, const void* host, size_t bytes)
+{
+    ensure(b, bytes);
+    if (bytes) hipchk(hipMemcpy(b->p, host, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+// pinned host staging of downloads (grow-only)
+struct Pinned {
+    void* p = nullptr;
+    size_t bytes = 0;
+    Pinned() = default;
+    Pinned(const Pinned&) = delete;
+    Pinned& operator=(const Pinned&) = delete;
+    ~Pinned()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+    void* get(size_t b)
+    {
+        if (b > bytes) {
+            if (p) (void)hipHostFree(p);
+            p = nullptr;
+            bytes = 0;
+            hipchk(hipHostMalloc(&p, b, hipHostMallocDefault), "hipHostMalloc");
+            bytes = b;
+        }
+        return p;
+    }
+};
+
+// DLPack (the dlpack.h v0.8 ABI): a managed tensor of complex128 values on a
+// ROCm device aliasing a session buffer
+struct DLDevice_ {
+    int32_t device_type, device_id;
+};
+struct DLDataType_ {
+    uint8_t code, bits;
+    uint16_t lanes;
+};
+struct DLTensor_ {
+    void* data;
+    DLDevice_ device;
+    int32_t ndim;
+    DLDataType_ dtype;
+    int64_t* shape;
+    int64_t* strides;
+    uint64_t byte_offset;
+};
+struct DLManagedTensor_ {
+    DLTensor_ dl_tensor;
+    void* manager_ctx;
+    void (*deleter)(DLManagedTensor_*);
+};
+constexpr int32_t kDLROCM_ = 10;
+constexpr uint8_t kDLComplex_ = 5;
+
+struct AliasCtx {
+    DevBufPtr buf;
+    int64_t shape[2];
+};
+
+// a "dltensor" capsule aliasing buf as (rows, n) (ndim 2) or (n,) (ndim 1);
+// the tensor keeps the buffer alive, and the session's next run overwrites it
+py::object alias_capsule(const DevBufPtr& buf, int64_t rows, int64_t n, int ndim)
+{
+    if (static_cast<size_t>(rows * n) * sizeof(cplx) > buf->bytes) throw std::logic_error("capsule beyond its buffer");
+    auto* ctx = new AliasCtx{buf, {rows, n}};
+    auto* mt = new DLManagedTensor_{};
+    mt->dl_tensor.data = buf->p;
+    mt->dl_tensor.device = DLDevice_{kDLROCM_, buf->device};
+    mt->dl_tensor.ndim = ndim;
+    mt->dl_tensor.dtype = DLDataType_{kDLComplex_, 128, 1};
+    mt->dl_tensor.shape = ndim == 2 ? ctx->shape : ctx->shape + 1;
+    mt->dl_tensor.strides = nullptr;
+    mt->dl_tensor.byte_offset = 0;
+    mt->manager_ctx = ctx;
+    mt->deleter = [](DLManagedTensor_* self) {
+        delete static_cast<AliasCtx*>(self->manager_ctx);
+        delete self;
+    };
+    PyObject* cap = PyCapsule_New(mt, "dltensor", [](PyObject* c) {
+        // an importer renames the capsule and owns the tensor; otherwise it is ours
+        if (PyCapsule_IsValid(c, "dltensor")) {
+            auto* m = static_cast<DLManagedTensor_*>(PyCapsule_GetPointer(c, "dltensor"));
+            if (m && m->deleter) m->deleter(m);
+        }
+    });
+    if (!cap) {
+        mt->deleter(mt);
+        throw py::error_already_set();
+    }
+    return py::reinterpret_steal<py::object>(cap);
 }
 
 double now_ms()
@@ -133,7 +262,9 @@ struct Grid {
     g2g_plan_info info{};
     double t_analyze_ms = 0.0;
 
-    Grid(py::object Ybus, const IArr& slack_ids, const IArr& pv_, const IArr& pq_)
+    int n_xs = 0;  // extension state values per row
+
+    Grid(py::object Ybus, const IArr& slack_ids, const IArr& pv_, const IArr& pq_, py::object ext = py::none())
     {
         Y = csr_from(Ybus);
         pv = ivec(pv_);
@@ -154,10 +285,68 @@ struct Grid {
             if (s0 >= 0 && s0 < Y.n && !has_unknowns[s0]) slack = s0;
         }
         const double t0 = now_ms();
-        check(g2g_plan_create(Y.n, Y.p.data(), Y.i.data(), Y.x.data(), static_cast<int32_t>(pv.size()), pv.data(),
-                              static_cast<int32_t>(pq.size()), pq.data(), slack, &plan));
+        if (ext.is_none()) {
+            check(g2g_plan_create(Y.n, Y.p.data(), Y.i.data(), Y.x.data(), static_cast<int32_t>(pv.size()),
+                                  pv.data(), static_cast<int32_t>(pq.size()), pq.data(), slack, &plan));
+        } else {
+            // extension rows: a dict {"slack_weights": [n] | None, "hvdc": {bus1, bus2, status, p0, k, lf1,
+            // lf2, r, pmax12, pmax21} | None, "vc": {reg_bus, v_set, ptr, bus, slope, weight} | None}
+            py::dict d = ext.cast<py::dict>();
+            std::vector<DArr> keep_d;
+            std::vector<IArr> keep_i;
+            auto darr = [&](py::handle h) { keep_d.push_back(h.cast<DArr>()); return keep_d.back().data(); };
+            auto iarr = [&](py::handle h) { keep_i.push_back(h.cast<IArr>()); return keep_i.back().data(); };
+            g2g_extensions x{};
+            if (d.contains("slack_weights") && !d["slack_weights"].is_none()) {
+                x.slack_weights = darr(d["slack_weights"]);
+                if (keep_d.back().size() != Y.n) throw py::value_error("slack_weights must have n_bus entries");
+            }
+            if (d.contains("hvdc") && !d["hvdc"].is_none()) {
+                py::dict h = d["hvdc"].cast<py::dict>();
+                x.hvdc_bus1 = iarr(h["bus1"]);
+                x.n_hvdc = static_cast<int32_t>(keep_i.back().size());
+                auto hi = [&](const char* k) {
+                    const int32_t* q = iarr(h[k]);
+                    if (keep_i.back().size() != x.n_hvdc) throw py::value_error("hvdc arrays differ in length");
+                    return q;
+                };
+                auto hd = [&](const char* k) {
+                    const double* q = darr(h[k]);
+                    if (keep_d.back().size() != x.n_hvdc) throw py::value_error("hvdc arrays differ in length");
+                    return q;
+                };
+                x.hvdc_bus2 = hi("bus2");
+                x.hvdc_status = hi("status");
+                x.hvdc_p0 = hd("p0");
+                x.hvdc_k = hd("k");
+                x.hvdc_lf1 = hd("lf1");
+                x.hvdc_lf2 = hd("lf2");
+                x.hvdc_r = hd("r");
+                x.hvdc_pmax12 = hd("pmax12");
+                x.hvdc_pmax21 = hd("pmax21");
+            }
+            if (d.contains("vc") && !d["vc"].is_none()) {
+                py::dict v = d["vc"].cast<py::dict>();
+                x.vc_reg_bus = iarr(v["reg_bus"]);
+                x.n_vc_groups = static_cast<int32_t>(keep_i.back().size());
+                x.vc_v_set = darr(v["v_set"]);
+                if (keep_d.back().size() != x.n_vc_groups) throw py::value_error("v_set needs one entry per group");
+                x.vc_ptr = iarr(v["ptr"]);
+                if (keep_i.back().size() != x.n_vc_groups + 1) throw py::value_error("ptr needs groups + 1 entries");
+                const int nc = x.vc_ptr[x.n_vc_groups];
+                x.vc_bus = iarr(v["bus"]);
+                if (keep_i.back().size() != nc) throw py::value_error("bus needs one entry per controller");
+                x.vc_slope = darr(v["slope"]);
+                if (keep_d.back().size() != nc) throw py::value_error("slope needs one entry per controller");
+                x.vc_weight = darr(v["weight"]);
+                if (keep_d.back().size() != nc) throw py::value_error("weight needs one entry per controller");
+            }
+            check(g2g_plan_create_ext(Y.n, Y.p.data(), Y.i.data(), Y.x.data(), static_cast<int32_t>(pv.size()),
+                                      pv.data(), static_cast<int32_t>(pq.size()), pq.data(), slack, &x, &plan));
+        }
         t_analyze_ms = now_ms() - t0;
         check(g2g_plan_get_info(plan, &info));
+        n_xs = info.n_xs;
     }
     ~Grid() { g2g_plan_destroy(plan); }
     Grid(const Grid&) = delete;
@@ -248,10 +437,13 @@ struct ViolResults {
 
 // ---- the solve of many rows, chunked by batch_size -----------------------
 struct RowJob {
-    const cplx* S = nullptr;       // rows x n, or one shared row (S_stride 0)
+    const cplx* S = nullptr;       // host: rows x n, or one shared row (S_stride 0)
     int64_t S_stride = 0;
-    const cplx* V0 = nullptr;      // rows x n, or one shared row
+    const cplx* V0 = nullptr;      // host: rows x n, or one shared row
     int64_t V_stride = 0;
+    const cplx* dS = nullptr;      // the same on the device (filled by the session)
+    const cplx* dV0 = nullptr;
+    cplx* dV = nullptr;            // device results: rows x n
     // per-row Ybus edits (CSR-like, nullable)
     const std::vector<int32_t>* edit_ptr = nullptr;
     const std::vector<int32_t>* edit_ent = nullptr;
@@ -271,11 +463,23 @@ struct RowJob {
 };
 
 struct RowResults {
-    std::vector<cplx> V;
     std::vector<int32_t> status, iters;
     std::vector<double> res;
+    std::vector<double> xs;  // extension state, rows x n_xs
     ViolResults viol;
 };
+
+// contiguous runs [first row, length, chunk offset] of the chunk rows live[c0 .. c0 + m)
+std::vector<std::array<int, 3>> row_runs(const std::vector<int>& live, size_t c0, int m)
+{
+    std::vector<std::array<int, 3>> runs;
+    for (int k = 0; k < m; ++k) {
+        const int r = live[c0 + k];
+        if (!runs.empty() && runs.back()[0] + runs.back()[1] == r) ++runs.back()[1];
+        else runs.push_back({r, 1, k});
+    }
+    return runs;
+}
 
 // the limit checks of chunk rows live[c0 .. c0 + m), scattered to their rows
 void check_violations(g2g_batch_t h, const ViolJob& vj, const std::vector<int>& live, size_t c0, int m,
@@ -314,14 +518,19 @@ void check_violations(g2g_batch_t h, const ViolJob& vj, const std::vector<int>& 
     }
 }
 
+// The rows' solve, chunked by batch_size, device-resident: every chunk's
+// injections and start voltages are gathered on the device from the
+// session's device rows (job.dS / job.dV0) and its voltages go straight to the
+// session's device results (job.dV); only statuses, counts and residuals
+// (and the limit-check records) come to the host.
 void run_rows(const Grid& g, std::unique_ptr<Batch>& batch, int precision, int batch_size, int n_rows,
               const RowJob& job, RowResults& out, BatchTimings& tm)
 {
     const int n = g.n();
-    out.V.assign(static_cast<size_t>(n_rows) * n, cplx(std::nan(""), std::nan("")));
     out.status.assign(n_rows, static_cast<int32_t>(RowStatus::NotSimulated));
     out.iters.assign(n_rows, 0);
     out.res.assign(n_rows, std::nan(""));
+    out.xs.assign(static_cast<size_t>(n_rows) * g.n_xs, std::nan(""));
     if (job.viol) out.viol.reset(n_rows, job.viol->K);
     else out.viol = ViolResults{};
     // simulated rows, in order
@@ -339,23 +548,29 @@ void run_rows(const Grid& g, std::unique_ptr<Batch>& batch, int precision, int b
         tm.t_alloc_ms += now_ms() - t0;
     }
     const int B = batch->rows;
-    std::vector<cplx> Sbuf(static_cast<size_t>(B) * n), Vbuf(static_cast<size_t>(B) * n);
     std::vector<int32_t> st(B), it(B);
-    std::vector<double> rs(B);
+    std::vector<double> rs(B), xs(static_cast<size_t>(B) * g.n_xs);
     std::vector<int32_t> eptr, eent, mptr, mbus;
     std::vector<cplx> edel;
     if (!job.mask_ptr) check(g2g_batch_set_bus_mask(batch->h, nullptr, nullptr, 0));
+    auto dptr = [](const cplx* p) { return reinterpret_cast<const double*>(p); };
     for (size_t c0 = 0; c0 < live.size(); c0 += B) {
         const int m = static_cast<int>(std::min<size_t>(B, live.size() - c0));
-        for (int k = 0; k < m; ++k) {
-            const int r = live[c0 + k];
-            std::copy_n(job.S + r * job.S_stride, n, Sbuf.begin() + static_cast<size_t>(k) * n);
-            std::copy_n(job.V0 + r * job.V_stride, n, Vbuf.begin() + static_cast<size_t>(k) * n);
-        }
+        const auto runs = row_runs(live, c0, m);
         check(g2g_batch_set_active_rows(batch->h, m));
         job.step.apply(batch->h);
-        check(g2g_batch_set_sbus(batch->h, reinterpret_cast<const double*>(Sbuf.data()), n, m, 0, 0));
-        check(g2g_batch_set_v(batch->h, reinterpret_cast<const double*>(Vbuf.data()), n, m, 0, 0));
+        if (job.S_stride == 0) {
+            check(g2g_batch_set_sbus(batch->h, dptr(job.dS), 0, m, 0, 1));
+        } else {
+            for (const auto& rn : runs)
+                check(g2g_batch_set_sbus(batch->h, dptr(job.dS + rn[0] * job.S_stride), job.S_stride, rn[1], rn[2], 1));
+        }
+        if (job.V_stride == 0) {
+            check(g2g_batch_set_v(batch->h, dptr(job.dV0), 0, m, 0, 1));
+        } else {
+            for (const auto& rn : runs)
+                check(g2g_batch_set_v(batch->h, dptr(job.dV0 + rn[0] * job.V_stride), job.V_stride, rn[1], rn[2], 1));
+        }
         if (per_row_y) {
             eptr.assign(1, 0);
             eent.clear();
@@ -394,15 +609,19 @@ void run_rows(const Grid& g, std::unique_ptr<Batch>& batch, int precision, int b
         tm.n_requeued += s.n_requeued;
         ++tm.n_chunks;
         const double t0 = now_ms();
-        check(g2g_batch_get_v(batch->h, reinterpret_cast<double*>(Vbuf.data()), n, 0));
+        for (const auto& rn : runs)
+            check(g2g_batch_get_v_rows(batch->h, reinterpret_cast<double*>(job.dV + static_cast<size_t>(rn[0]) * n), n,
+                                       rn[1], rn[2], 1));
         tm.t_copy_V_to_host_ms += now_ms() - t0;
         check(g2g_batch_get_results(batch->h, st.data(), it.data(), rs.data()));
+        if (g.n_xs) check(g2g_batch_get_ext_state(batch->h, xs.data()));
         for (int k = 0; k < m; ++k) {
             const int r = live[c0 + k];
-            std::copy_n(Vbuf.begin() + static_cast<size_t>(k) * n, n, out.V.begin() + static_cast<size_t>(r) * n);
             out.status[r] = st[k];
             out.iters[r] = it[k];
             out.res[r] = rs[k];
+            std::copy_n(xs.begin() + static_cast<size_t>(k) * g.n_xs, g.n_xs,
+                        out.xs.begin() + static_cast<size_t>(r) * g.n_xs);
         }
         if (job.viol) {
             const double tv = now_ms();
@@ -432,12 +651,13 @@ void set_device(int device)
 class AcPfNrSession {
 public:
     AcPfNrSession(py::object Ybus, CArr Vinit, CArr Sbus, IArr slack_ids, DArr /*slack_weights*/, IArr pv, IArr pq,
-                  int max_iter, double tol, int device, bool presolved_v, StepControl step, Precision precision)
+                  int max_iter, double tol, int device, bool presolved_v, StepControl step, Precision precision,
+                  py::object extensions = py::none())
         : max_iter_(max_iter), tol_(tol), precision_(static_cast<int>(precision))
     {
         set_device(device);
         const double t0 = now_ms();
-        grid_ = std::make_unique<Grid>(Ybus, slack_ids, pv, pq);
+        grid_ = std::make_unique<Grid>(Ybus, slack_ids, pv, pq, extensions);
         const int n = grid_->n();
         if (Vinit.size() != n || Sbus.size() != n) throw py::value_error("Vinit / Sbus must have n_bus entries");
         S_.assign(Sbus.data(), Sbus.data() + n);
@@ -460,6 +680,7 @@ public:
                                          " is not below tol");
             timings_.converged = true;
             timings_.residual = r;
+            keep_v();
         } else {
             solve(max_iter_, tol_);
         }
@@ -497,9 +718,19 @@ public:
     {
         const double t0 = now_ms();
         CArr out(grid_->n());
-        check(g2g_batch_get_v(batch_->h, reinterpret_cast<double*>(out.mutable_data()), grid_->n(), 0));
+        hipchk(hipMemcpy(out.mutable_data(), dV_->p, grid_->n() * sizeof(cplx), hipMemcpyDeviceToHost),
+               "hipMemcpy D2H");
         timings_.t_copy_v_to_host_ms = now_ms() - t0;
         return out;
+    }
+    // a DLPack (n_bus,) complex128 tensor aliasing the session's device V:
+    // resolve() overwrites it (acpf_nr/gpu_facade.py's v_dlpack contract)
+    py::object v_dlpack() { return alias_capsule(dV_, 1, grid_->n(), 1); }
+    py::array_t<double> get_ext_state() const
+    {
+        std::vector<double> xs(static_cast<size_t>(grid_->n_xs));
+        if (!xs.empty()) check(g2g_batch_get_ext_state(batch_->h, xs.data()));
+        return to_array(xs);
     }
 
     DArr get_F()
@@ -574,12 +805,20 @@ private:
         timings_.n_refactorize = s.n_factor_launches;
         timings_.converged = st == 0;
         timings_.residual = r;
+        keep_v();
+    }
+    // the solved V into the session's device vector (what v_dlpack aliases)
+    void keep_v()
+    {
+        ensure(dV_, static_cast<size_t>(grid_->n()) * sizeof(cplx));
+        check(g2g_batch_get_v_rows(batch_->h, reinterpret_cast<double*>(dV_->p), grid_->n(), 1, 0, 1));
     }
 
     std::unique_ptr<Grid> grid_;
     std::unique_ptr<Batch> batch_;
     std::vector<cplx> S_, V_;
     std::vector<std::pair<int, double>> vm_set_;  // (bus, |V| set-point) without a |V| unknown
+    DevBufPtr dV_;                                 // the solved V on the device
     int max_iter_;
     double tol_;
     int precision_;
@@ -776,14 +1015,14 @@ class BatchSession {
 public:
     BatchSession(py::object Ybus, CArr Vinit, CArr Sbus, IArr slack_ids, IArr pv, IArr pq, int batch_size,
                  int nb_iter, int max_iter_base, double tol_base, int device, bool presolved_v, StepControl step,
-                 Precision precision)
+                 Precision precision, py::object extensions = py::none())
         : batch_size_(batch_size), nb_iter_(nb_iter), precision_(static_cast<int>(precision)), step_(step)
     {
         if (batch_size <= 0) throw py::value_error("batch_size must be positive");
         if (nb_iter < 0) throw py::value_error("nb_iter must be >= 0");
         set_device(device);
         const double t0 = now_ms();
-        grid_ = std::make_unique<Grid>(Ybus, slack_ids, pv, pq);
+        grid_ = std::make_unique<Grid>(Ybus, slack_ids, pv, pq, extensions);
         timings_.t_analysis_ms = grid_->t_analyze_ms;
         const int n = grid_->n();
         if (Vinit.size() != n || Sbus.size() != n) throw py::value_error("Vinit / Sbus must have n_bus entries");
@@ -928,6 +1167,28 @@ public:
             job.viol = &vj;
         }
         const double t0 = now_ms();
+        const int n = n_bus();
+        // the rows on the device: injections once per change of the session's
+        // rows (shared rows and start voltages are small and go every run)
+        if (job.S_stride == 0) {
+            upload(dS1_, job.S, static_cast<size_t>(n) * sizeof(cplx));
+            job.dS = dS1_->c();
+        } else {
+            if (s_dirty_ || !dS_ || dS_->bytes < static_cast<size_t>(rows) * n * sizeof(cplx)) {
+                upload(dS_, job.S, static_cast<size_t>(rows) * job.S_stride * sizeof(cplx));
+                s_dirty_ = false;
+            }
+            job.dS = dS_->c();
+        }
+        upload(dV0_, job.V0, static_cast<size_t>(job.V_stride ? rows : 1) * n * sizeof(cplx));
+        job.dV0 = dV0_->c();
+        ensure(dVres_, static_cast<size_t>(rows) * n * sizeof(cplx));
+        job.dV = dVres_->c();
+        bool any_skip = false;
+        for (char c : job.skip) any_skip |= c != 0;
+        if (any_skip)  // rows not simulated read NaN (all-ones bytes)
+            hipchk(hipMemset(dVres_->p, 0xFF, static_cast<size_t>(rows) * n * sizeof(cplx)), "hipMemset");
+        res_rows_ = rows;
         run_rows(*grid_, batch_, precision_, batch_size_, rows, job, res_, t);
         t.t_run_ms = now_ms() - t0;
         timings_ = t;
@@ -937,11 +1198,32 @@ public:
         flows_ = false;
     }
 
-    CArr get_V_results() const
+    // the result voltages, downloaded through pinned staging
+    CArr get_V_results()
     {
         need_run();
-        CArr a(static_cast<py::ssize_t>(res_.V.size()));
-        std::copy(res_.V.begin(), res_.V.end(), a.mutable_data());
+        const size_t cnt = static_cast<size_t>(res_rows_) * n_bus();
+        CArr a(static_cast<py::ssize_t>(cnt));
+        if (cnt) {
+            void* st = pin_.get(cnt * sizeof(cplx));
+            hipchk(hipMemcpy(st, dVres_->p, cnt * sizeof(cplx), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+            std::memcpy(a.mutable_data(), st, cnt * sizeof(cplx));
+        }
+        return a;
+    }
+    // a DLPack (n_rows, n_bus) complex128 tensor aliasing the session's device
+    // results: the next run() overwrites it (clone to keep it)
+    py::object v_results_dlpack()
+    {
+        need_run();
+        return alias_capsule(dVres_, res_rows_, n_bus(), 2);
+    }
+    // extension state of every row (rows x n_xs: sa, then each controller's Q; NaN if not simulated)
+    py::array_t<double> get_ext_state() const
+    {
+        need_run();
+        py::array_t<double> a({static_cast<py::ssize_t>(res_rows_), static_cast<py::ssize_t>(grid_->n_xs)});
+        std::copy(res_.xs.begin(), res_.xs.end(), a.mutable_data());
         return a;
     }
     py::array_t<double> get_residuals() const { need_run(); return to_array(res_.res); }
@@ -980,7 +1262,6 @@ public:
             batch_ = std::make_unique<Batch>(*grid_, cap, precision_, 0);
         }
         const int B = batch_->rows;
-        std::vector<cplx> Vbuf(static_cast<size_t>(B) * n);
         std::vector<double> o(static_cast<size_t>(B) * m), e(static_cast<size_t>(B) * m);
         std::vector<int32_t> tp, tb;
         for (size_t c0 = 0; c0 < live.size(); c0 += B) {
@@ -989,13 +1270,14 @@ public:
             tb.clear();
             for (int q = 0; q < k; ++q) {
                 const int r = live[c0 + q];
-                std::copy_n(res_.V.begin() + static_cast<size_t>(r) * n, n, Vbuf.begin() + static_cast<size_t>(q) * n);
                 if (static_cast<size_t>(r) < trips_.size())
                     for (int l : trips_[r]) tb.push_back(l);
                 tp.push_back(static_cast<int32_t>(tb.size()));
             }
             check(g2g_batch_set_active_rows(batch_->h, k));
-            check(g2g_batch_set_v(batch_->h, reinterpret_cast<const double*>(Vbuf.data()), n, k, 0, 0));
+            for (const auto& rn : row_runs(live, c0, k))  // the rows' results, device to device
+                check(g2g_batch_set_v(batch_->h, reinterpret_cast<const double*>(dVres_->c() + static_cast<size_t>(rn[0]) * n),
+                                      n, rn[1], rn[2], 1));
             check(g2g_batch_branch_flows(batch_->h, m, br_.f.data(), br_.t.data(),
                                          reinterpret_cast<const double*>(y4.data()), br_.base_or.data(),
                                          br_.base_ex.data(), tp.data(), tb.empty() ? nullptr : tb.data(), o.data(),
@@ -1053,6 +1335,12 @@ protected:
     BatchTimings timings_;
     bool ran_ = false;
     std::vector<int32_t> disc_;
+    // device rows: injections (and their change flag), a shared injection
+    // row, start voltages, results (aliased by v_results_dlpack), staging
+    DevBufPtr dS_, dS1_, dV0_, dVres_;
+    bool s_dirty_ = true;
+    int res_rows_ = 0;
+    Pinned pin_;
     // limits of compute_limit_violations
     std::vector<double> vmin_, vmax_, lim1_, lim2_, vn_kv_;
     int n_lines_ = 0;
@@ -1127,6 +1415,7 @@ public:
         S_.resize(static_cast<size_t>(rows * n));
         for (py::ssize_t i = 0; i < rows * n; ++i) S_[i] = cplx(p_mw.data()[i], q_mvar.data()[i]) / sn_mva;
         rows_ = static_cast<int>(rows);
+        s_dirty_ = true;  // uploaded once, at the next run
         ran_ = false;
     }
     void set_sbus(CArr S)
@@ -1134,6 +1423,7 @@ public:
         if (S.ndim() != 2 || S.shape(1) != n_bus()) throw py::value_error("Sbus must be (n_scenarios, n_bus)");
         S_.assign(S.data(), S.data() + S.size());
         rows_ = static_cast<int>(S.shape(0));
+        s_dirty_ = true;
         ran_ = false;
     }
     // per-row start voltages (None: the base case), as ScenarioSweep's set_v_init_from
@@ -1300,16 +1590,6 @@ protected:
     bool topo_ = false;
     Topology tp_;
 };
-
-// a DLPack capsule of a device copy of host complex rows (the reference's
-// *_dlpack exports hand out device memory; the results here live on the host)
-py::object device_capsule(py::array arr)
-{
-    py::object torch = py::module_::import("torch");
-    py::object t = torch.attr("from_numpy")(arr).attr("to")("cuda");
-    torch.attr("cuda").attr("synchronize")();
-    return py::module_::import("torch.utils.dlpack").attr("to_dlpack")(t);
-}
 
 // ---- raw batched LU (raw_cudss_solve.hpp:31-121) --------------------------
 struct RawPlan {
@@ -1537,10 +1817,11 @@ PYBIND11_MODULE(_gpusim2grid, m)
         .def(py::init([](py::object Ybus, CArr Vinit, CArr Sbus, IArr slack_ids, DArr slack_weights, IArr pv,
                          IArr pq, int max_iter, double tol, int device, bool presolved_v, ReorderingAlg,
                          MatchingAlg, PivotEpsilonAlg, int ir_n_steps, bool, bool scaling, double max_dva,
-                         double max_dvm, Precision precision) {
+                         double max_dvm, Precision precision, py::object extensions) {
                  return std::make_shared<AcPfNrSession>(Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq,
                                                         max_iter, tol, device, presolved_v,
-                                                        StepControl(ir_n_steps, scaling, max_dva, max_dvm), precision);
+                                                        StepControl(ir_n_steps, scaling, max_dva, max_dvm), precision,
+                                                        extensions);
              }),
              py::arg("Ybus"), py::arg("Vinit"), py::arg("Sbus"), py::arg("slack_ids"), py::arg("slack_weights"),
              py::arg("pv"), py::arg("pq"), py::arg("max_iter"), py::arg("tol"), py::arg("device") = -1,
@@ -1548,14 +1829,15 @@ PYBIND11_MODULE(_gpusim2grid, m)
              py::arg("matching_alg") = MatchingAlg::None, py::arg("pivot_epsilon_alg") = PivotEpsilonAlg::Default,
              py::arg("ir_n_steps") = 0, py::arg("debug_base_case") = false,
              py::arg("scaling_max_voltage_change") = false, py::arg("max_dVa") = 0.5, py::arg("max_dVm") = 0.1,
-             py::arg("precision") = Precision::Fp64)
+             py::arg("precision") = Precision::Fp64, py::arg("extensions") = py::none())
         .def_property_readonly("timings", &AcPfNrSession::timings)
+        .def("get_ext_state", &AcPfNrSession::get_ext_state)
         .def("resolve", &AcPfNrSession::resolve, py::arg("Vinit"), py::arg("Sbus") = py::none(),
              py::arg("max_iter") = -1, py::arg("tol") = -1.0)
         .def("solve_at_setpoints", &AcPfNrSession::solve_at_setpoints, py::arg("Vinit"), py::arg("max_iter"),
              py::arg("tol"))
         .def("get_v", &AcPfNrSession::get_v)
-        .def("v_dlpack", [](AcPfNrSession& s) { return device_capsule(s.get_v()); })
+        .def("v_dlpack", &AcPfNrSession::v_dlpack)
         .def("get_F", &AcPfNrSession::get_F)
         .def("get_J", &AcPfNrSession::get_J)
         .def("solve_JT", &AcPfNrSession::solve_JT, py::arg("rhs"))
@@ -1615,12 +1897,8 @@ PYBIND11_MODULE(_gpusim2grid, m)
             .def("get_ex_amps", &S::get_ex_amps)
             .def("clear_warm_start", &S::clear_warm_start)
             .def_readwrite("handle_disconnected_grid", &S::handle_disconnected_grid)
-            .def("v_results_dlpack",
-                 [](const S& s) {
-                     CArr V = s.get_V_results();
-                     return device_capsule(V.reshape({static_cast<py::ssize_t>(s.n_rows()),
-                                                      static_cast<py::ssize_t>(s.n_bus())}));
-                 })
+            .def("v_results_dlpack", &S::v_results_dlpack)
+            .def("get_ext_state", &S::get_ext_state)
             .def_readwrite("compute_limit_violations", &S::compute_limit_violations)
             .def_readwrite("violation_tol", &S::violation_tol)
             .def_property("violation_rel_tol", &S::get_violation_rel_tol, &S::set_violation_rel_tol)
@@ -1645,10 +1923,10 @@ PYBIND11_MODULE(_gpusim2grid, m)
         return py::init([](py::object Ybus, CArr Vinit, CArr Sbus, IArr slack_ids, DArr, IArr pv, IArr pq,
                            int batch_size, int nb_iter, int max_iter_base, double tol_base, int device,
                            bool presolved_v, ReorderingAlg, MatchingAlg, PivotEpsilonAlg, int ir_n_steps, bool,
-                           bool scaling, double max_dva, double max_dvm, Precision precision) {
+                           bool scaling, double max_dva, double max_dvm, Precision precision, py::object extensions) {
             return std::make_shared<S>(Ybus, Vinit, Sbus, slack_ids, pv, pq, batch_size, nb_iter, max_iter_base,
                                        tol_base, device, presolved_v, StepControl(ir_n_steps, scaling, max_dva, max_dvm),
-                                       precision);
+                                       precision, extensions);
         });
     };
 #define G2G_BATCH_ARGS                                                                                          \
@@ -1658,7 +1936,8 @@ PYBIND11_MODULE(_gpusim2grid, m)
         py::arg("reordering_alg") = ReorderingAlg::Default, py::arg("matching_alg") = MatchingAlg::None,        \
         py::arg("pivot_epsilon_alg") = PivotEpsilonAlg::Default, py::arg("ir_n_steps") = 0,                     \
         py::arg("debug_base_case") = false, py::arg("scaling_max_voltage_change") = false,                      \
-        py::arg("max_dVa") = 0.5, py::arg("max_dVm") = 0.1, py::arg("precision") = Precision::Fp64
+        py::arg("max_dVa") = 0.5, py::arg("max_dVm") = 0.1, py::arg("precision") = Precision::Fp64,             \
+        py::arg("extensions") = py::none()
 
     py::class_<InjectionSweepSession, std::shared_ptr<InjectionSweepSession>> inj(m, "InjectionSweepSession");
     inj.def(batch_init(static_cast<InjectionSweepSession*>(nullptr)), G2G_BATCH_ARGS)

@@ -31,6 +31,7 @@ template <typename Real>
 __global__ void k_update(UpdateArgs<Real> a);
 template <typename Real>
 __global__ void k_tiles_to_rows_f64(const Real* in, int rows, int m, double* out, long long ld);
+__global__ void k_slots_range_to_rows(const double2* in, int row0, int rows, int m, double2* out, long long ld);
 template <typename Real>
 __global__ void k_ext_pre(ExtArgs<Real> a);
 template <typename Real>
@@ -1882,6 +1883,27 @@ int g2g_batch_get_v(g2g_batch_t b, double* V, int64_t ld, int32_t on_device)
             HIPCHK(hipStreamSynchronize(b->stream));
             dfree(tmp);
         }
+    });
+}
+
+int g2g_batch_get_v_rows(g2g_batch_t b, double* V, int64_t ld, int32_t rows, int32_t row0, int32_t on_device)
+{
+    return guarded([&] {
+        if (!b || !V) throw std::invalid_argument("null argument");
+        check_rows(b, rows, row0);
+        const int n = b->plan->nr.n;
+        if (ld < n) throw std::invalid_argument("ld smaller than n");
+        if (rows == 0) return;
+        const long long total = static_cast<long long>(rows) * n;
+        double2* dst = reinterpret_cast<double2*>(V);
+        DeviceScratch tmp;
+        if (!on_device) dst = tmp.get<double2>(static_cast<size_t>(rows) * ld);
+        k_slots_range_to_rows<<<blocks_for(total, 256), 256, 0, b->stream>>>(b->V, row0, rows, n, dst, ld);
+        HIPCHK(hipGetLastError());
+        if (!on_device)
+            HIPCHK(hipMemcpyAsync(V, dst, static_cast<size_t>(rows) * ld * sizeof(double2), hipMemcpyDeviceToHost,
+                                  b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
     });
 }
 

@@ -1383,6 +1383,20 @@ __global__ void k_tiles_to_rows(const T* __restrict__ in, int rows, int m, T* __
     out[static_cast<long long>(r) * ld + k] = in[tidx(static_cast<size_t>(r / kLanes), m, k, r % kLanes)];
 }
 
+// tiled rows [row0, row0 + rows) of [T][m][64] into row-major rows [rows][m]
+// (row stride ld)
+__global__ void k_slots_range_to_rows(const double2* __restrict__ in, int row0, int rows, int m,
+                                      double2* __restrict__ out, long long ld)
+{
+    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long total = static_cast<long long>(rows) * m;
+    if (gid >= total) return;
+    const int k = static_cast<int>(gid / rows);
+    const int r = static_cast<int>(gid - static_cast<long long>(k) * rows);
+    const int b = row0 + r;
+    out[static_cast<long long>(r) * ld + k] = in[tidx(static_cast<size_t>(b / kLanes), m, k, b % kLanes)];
+}
+
 // tiled [T][m][64] values of the first `rows` rows into row-major FP64 rows
 // [rows][m] with row stride ld
 template <typename Real>

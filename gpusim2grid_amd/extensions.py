@@ -64,6 +64,27 @@ class Extensions:
     def n_state(self):
         return (self.slack_weights is not None) + sum(len(g.buses) for g in self.voltage)
 
+    def as_dict(self):
+        """The ``extensions=`` argument of the ``_gpusim2grid`` sessions."""
+        d = {"slack_weights": None if self.slack_weights is None
+             else np.ascontiguousarray(self.slack_weights, dtype=np.float64), "hvdc": None, "vc": None}
+        if self.hvdc:
+            h = self.hvdc
+            d["hvdc"] = {"bus1": np.array([x.bus1 for x in h], np.int32),
+                         "bus2": np.array([x.bus2 for x in h], np.int32),
+                         "status": np.array([x.status for x in h], np.int32)}
+            for name in ("p0", "k", "lf1", "lf2", "r", "pmax12", "pmax21"):
+                d["hvdc"][name] = np.array([getattr(x, name) for x in h], np.float64)
+        if self.voltage:
+            g = self.voltage
+            d["vc"] = {"reg_bus": np.array([x.reg_bus for x in g], np.int32),
+                       "v_set": np.array([x.v_set for x in g], np.float64),
+                       "ptr": np.concatenate([[0], np.cumsum([len(x.buses) for x in g])]).astype(np.int32),
+                       "bus": np.array([b for x in g for b in x.buses], np.int32),
+                       "slope": np.array([v for x in g for v in x.slopes], np.float64),
+                       "weight": np.array([v for x in g for v in x.weights], np.float64)}
+        return d
+
 
 class CExtensions(ctypes.Structure):
     """g2g_extensions"""

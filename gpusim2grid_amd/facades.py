@@ -9,9 +9,14 @@ run the HIP kernels.  Settings take effect on the next run, as in the
 reference (``CLAUDE.md:130-138``).  A lightsim2grid grid object is not
 accepted: the bridge is out of scope.
 
-``compute()`` returns the voltages as a DLPack capsule of a device tensor
-(the reference's ``compute`` contract, ``injection_sweep/gpu_facade.py:515-524``),
-or, with ``as_numpy=True``, a host array.
+``compute()`` returns the voltages as a DLPack capsule aliasing the session's
+device results (the reference's ``compute`` contract,
+``injection_sweep/gpu_facade.py:515-524``): the next run overwrites them, so
+clone a tensor to keep it.  With ``as_numpy=True`` it returns a host array.
+
+``extensions=`` (an ``extensions.Extensions``) adds the distributed-slack,
+HVDC-droop and voltage-control rows to the Newton system; the reference takes
+these from a lightsim2grid grid, which array mode does not carry.
 """
 from __future__ import annotations
 
@@ -89,11 +94,10 @@ def _check_common(ir_n_steps, scaling_max_voltage_change, max_dVa, max_dVm, use_
     return opts
 
 
-def _to_device_capsule(V):
-    """(rows, n) complex host array -> DLPack capsule of a device tensor."""
-    import torch
-    from torch.utils import dlpack
-    return dlpack.to_dlpack(torch.from_numpy(np.ascontiguousarray(V)).to("cuda"))
+def _ext(extensions):
+    if extensions is None:
+        return None
+    return extensions.as_dict() if hasattr(extensions, "as_dict") else extensions
 
 
 class _HostResult:
@@ -113,12 +117,12 @@ class AcPfGPU:
     def __init__(self, grid, *, precision="fp64", max_iter=10, tol=1e-8, device=None, init_from_n_powerflow=True,
                  use_bridge=None, reordering_alg="default", matching_alg="none", pivot_epsilon_alg="default",
                  ir_n_steps=0, debug_base_case=False, scaling_max_voltage_change=None, max_dVa=None, max_dVm=None,
-                 use_distributed_slack=True):
+                 use_distributed_slack=True, extensions=None):
         step = _check_common(ir_n_steps, scaling_max_voltage_change, max_dVa, max_dVm, use_bridge)
         Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq = _tuple(grid)
         self._s = _cpp.AcPfNrSession(Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq, int(max_iter), float(tol),
                                      _device(device), presolved_v=bool(init_from_n_powerflow),
-                                     precision=_precision(precision), **step)
+                                     precision=_precision(precision), extensions=_ext(extensions), **step)
 
     def solve(self):
         """The solved complex voltage vector (host copy)."""
@@ -143,6 +147,16 @@ class AcPfGPU:
 
     def get_F(self):
         return self._s.get_F()
+
+    def v_dlpack(self):
+        """DLPack (n_bus,) complex128 capsule aliasing the session's device V
+        (the next resolve overwrites it)."""
+        return self._s.v_dlpack()
+
+    def ext_state(self):
+        """Extension state of the solve: sa (distributed slack), then each
+        voltage controller's Q (per unit)."""
+        return self._s.get_ext_state()
 
     def solve_JT_dlpack(self, rhs):
         """Solve J^T x = rhs at the solved V (the adjoint) -> DLPack capsule of
@@ -175,7 +189,8 @@ class _BatchFacade:
     def __init__(self, grid, *, init_from_n_powerflow=True, precision="fp64", nb_iter=4, max_iter_base=10,
                  tol_base=1e-8, device=None, use_bridge=None, ir_n_steps=0, scaling_max_voltage_change=None,
                  max_dVa=None, max_dVm=None, handle_disconnected_grid=False, compute_limit_violations=False,
-                 scheduling=None, nb_iter_per_round=1, tol=None, linear_solver="own_lu", **_ignored):
+                 scheduling=None, nb_iter_per_round=1, tol=None, linear_solver="own_lu", extensions=None,
+                 **_ignored):
         step = _check_common(ir_n_steps, scaling_max_voltage_change, max_dVa, max_dVm, use_bridge)
         if nb_iter_per_round != 1:
             raise ValueError("nb_iter_per_round: rows are checked after every iteration in this build (1)")
@@ -185,7 +200,7 @@ class _BatchFacade:
         self._s = self._session_cls(Ybus, Vinit, Sbus, slack_ids, slack_weights, pv, pq, 100, int(nb_iter),
                                     int(max_iter_base), float(tol_base), _device(device),
                                     presolved_v=bool(init_from_n_powerflow), precision=_precision(precision),
-                                    **step)
+                                    extensions=_ext(extensions), **step)
         self.scheduling = scheduling or "chunked"
         self._s.handle_disconnected_grid = bool(handle_disconnected_grid)
         self._s.compute_limit_violations = bool(compute_limit_violations)
@@ -351,13 +366,20 @@ class _BatchFacade:
 
     # running ----------------------------------------------------------------
     def compute(self, batch_size=512, as_numpy=False):
-        """Solve every row; DLPack (n_rows, n_bus) complex of a device tensor
-        (``as_numpy``: host array).  Residuals go to :meth:`last_residuals`."""
+        """Solve every row; a DLPack (n_rows, n_bus) complex128 capsule aliasing
+        the session's device results, which the next run overwrites
+        (``as_numpy``: a host copy).  Residuals go to :meth:`last_residuals`."""
         self._s.batch_size = int(batch_size)
         self._s.run()
         self._last_residuals = self._s.get_residuals()
-        V = self._s.get_V_results().reshape(-1, self._s.n_bus)
-        return V if as_numpy else _to_device_capsule(V)
+        if as_numpy:
+            return self._s.get_V_results().reshape(-1, self._s.n_bus)
+        return self._s.v_results_dlpack()
+
+    def ext_state(self):
+        """[n_rows, n_state] extension state of the last run (sa, then each
+        voltage controller's Q; NaN for rows not simulated)."""
+        return self._s.get_ext_state()
 
     def last_residuals(self):
         if self._last_residuals is None:

@@ -292,6 +292,9 @@ int g2g_batch_limit_violations(g2g_batch_t batch, const g2g_limit_table *table, 
                                g2g_violations *out);
 
 int g2g_batch_get_v(g2g_batch_t batch, double *V, int64_t ld, int32_t on_device);
+/* Rows [row0, row0 + rows) of the batch's V into V (row stride ld complex;
+ * on_device = 1: V is device memory, no host round trip). */
+int g2g_batch_get_v_rows(g2g_batch_t batch, double *V, int64_t ld, int32_t rows, int32_t row0, int32_t on_device);
 /* Extension state of every row (rows x n_xs host arrays: sa, then each
  * controller's Q).  A solve starts it afresh; the setter serves evaluation. */
 int g2g_batch_get_ext_state(g2g_batch_t batch, double *out);
