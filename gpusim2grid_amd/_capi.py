@@ -20,14 +20,16 @@ FP64, MIXED = 0, 1
 CHUNKED, CONTINUOUS = 0, 1
 CONVERGED, MAXITER, DIVERGED, NOTSIMULATED = 0, 1, 2, 3
 ARR = dict(jp=0, ji=1, maps=2, pvpq=3, perm=4, colp=5, coli=6, prog_f=7, prog_b=8, apos=9, fpos=10,
-           uoff=11, segs=12, bsegs=13)
+           uoff=11, segs=12, bsegs=13, prog_fs=14, prog_bs=15, split_segs=16, split_bsegs=17,
+           split_apos=18, split_fpos=19)
 
 
 class PlanInfo(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int32) for k in (
         "n", "nY", "d", "nJ", "npvpq", "npq", "slack", "nnz", "nLU", "n_updates", "max_c", "height",
         "slots_f", "slots_b", "a_len", "u_len", "n_steps", "walkers", "n_top", "bwalkers",
-        "u_stride_f64", "u_stride_f32", "n_ext", "n_xs", "sa_col")]
+        "u_stride_f64", "u_stride_f32", "n_ext", "n_xs", "sa_col", "n_split", "slots_fs", "slots_bs",
+        "pend_len", "split_a_len", "split_u_len")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -23,9 +23,9 @@ template <typename Real, bool PerRowY, bool WithJ>
 __global__ void k_mismatch(MismatchArgs<Real> a);
 template <typename Real, bool WithJ>
 __global__ void k_mis_status(MismatchArgs<Real> a);
-template <typename Real, int W>
+template <typename Real, int W, bool SPLIT>
 __global__ void k_factor(FactorArgs<Real> a);
-template <typename Real, int W>
+template <typename Real, int W, bool SPLIT>
 __global__ void k_backward(BackwardArgs<Real> a);
 template <typename Real>
 __global__ void k_update(UpdateArgs<Real> a);
@@ -123,6 +123,15 @@ struct g2g_plan_s {
     std::vector<std::unique_ptr<g2g_plan_s>> fitted;
     int* d_Jp = nullptr;      // the matrix CSR in unknown labels (refinement residuals)
     int* d_Ji = nullptr;
+    // split walk: programs, segments, and the mismatch tables of its layout
+    int* d_prog_fs = nullptr;
+    int* d_prog_bs = nullptr;
+    int* d_split_seg = nullptr;
+    int* d_split_bseg = nullptr;
+    int* d_k1_bus_s = nullptr;
+    int* d_k1_ent_s = nullptr;
+    int prog_fs_stride = 0, prog_bs_stride = 0;
+    bool split() const { return lu().n_split > 0 && (!grid || nr.split_ok()); }
     // extension rows (nr.xt): device copies of its tables
     struct {
         int *inj_ptr = nullptr, *inj = nullptr, *jrec = nullptr, *frec = nullptr, *xcol = nullptr;
@@ -151,6 +160,8 @@ struct g2g_plan_s {
         dfree(d_Y); dfree(d_perm); dfree(d_apos); dfree(d_fpos); dfree(d_upd); dfree(d_Jp); dfree(d_Ji);
         dfree(d_k1_ent_t);
         dx.free_all();
+        dfree(d_prog_fs); dfree(d_prog_bs); dfree(d_split_seg); dfree(d_split_bseg); dfree(d_k1_bus_s);
+        dfree(d_k1_ent_s);
     }
     // the mismatch program writing J transposed: entry (r, c) of J goes to the
     // A position of (c, r) (the pattern is structurally symmetric)
@@ -204,6 +215,16 @@ struct g2g_plan_s {
         d_fpos = dupload(s.f_pos);
         d_Jp = dupload(grid ? nr.Jp : raw_p);
         d_Ji = dupload(grid ? nr.Ji : raw_i);
+        if (split()) {
+            d_prog_fs = dupload(padded(s.prog_fs, prog_fs_stride));
+            d_prog_bs = dupload(padded(s.prog_bs, prog_bs_stride));
+            d_split_seg = dupload(seg_words(s.split_seg));
+            d_split_bseg = dupload(seg_words(s.split_bseg));
+            if (grid) {
+                d_k1_bus_s = dupload(nr.k1_bus_s);
+                d_k1_ent_s = dupload(nr.k1_ent_s);
+            }
+        }
         if (grid) {
             d_k1_bus = dupload(nr.k1_bus);
             d_k1_ent = dupload(nr.k1_ent);
@@ -303,6 +324,11 @@ struct g2g_batch_s {
     double2* V0k = nullptr;
     int *it0k = nullptr, *budgetk = nullptr;
     unsigned char* warmk = nullptr;
+    // split walk: the pending buffer [T][pend_len][64], whether the A / U
+    // streams currently hold the split layout, and the tiles a launch covers
+    void* pend = nullptr;
+    bool split_layout = false;
+    int split_T = 0;
     // extension rows: per-row state [T][n_xs][64] (sa, Q of each controller),
     // its compaction copy, and the extension buses' residual terms [T][n_xb][64]
     double* xs = nullptr;
@@ -322,7 +348,7 @@ struct g2g_batch_s {
         dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped); dfree(error);
         dfree(V0); dfree(Bs); dfree(Xs);
         dfree(Vk); dfree(Sk); dfree(Yk); dfree(stk); dfree(itk); dfree(resk); dfree(mapk); dfree(it0);
-        dfree(mask); dfree(maskk); dfree(xs); dfree(xsk); dfree(sx);
+        dfree(mask); dfree(maskk); dfree(xs); dfree(xsk); dfree(sx); dfree(pend);
         dfree(budget); dfree(warm); dfree(requeued); dfree(V0k); dfree(it0k); dfree(budgetk); dfree(warmk);
         for (void*& q : stage) dfree(q);
         if (h_active) (void)hipHostFree(h_active);
@@ -349,6 +375,19 @@ int u_stride_of(const SymbolicLU& s, int real_size)
     int end = s.u_len;
     for (const auto& g : s.seg) end = std::max(end, g.u_off + round_up(g.u_len, value_chunk(real_size, s.walkers)));
     for (const auto& g : s.bseg) end = std::max(end, g.u_off + round_up(g.u_len, value_chunk(real_size, s.bwalkers)));
+    for (const auto& g : s.split_seg) end = std::max(end, g.u_off + round_up(g.u_len, value_chunk(real_size, kSplitRingF)));
+    for (const auto& g : s.split_bseg) end = std::max(end, g.u_off + round_up(g.u_len, value_chunk(real_size, kSplitRingB)));
+    return round_up(end, ch);
+}
+
+// A positions one tile holds: the main stream, and the split walk's, each
+// read in whole chunks from every segment's start
+int a_stride_of(const SymbolicLU& s, int real_size)
+{
+    const int ch = real_size == 8 ? kChunkF64 : kChunkF32;
+    int end = s.a_len;
+    for (const auto& g : s.seg) end = std::max(end, g.a_off + round_up(g.a_len, value_chunk(real_size, s.walkers)));
+    for (const auto& g : s.split_seg) end = std::max(end, g.a_off + round_up(g.a_len, value_chunk(real_size, kSplitRingF)));
     return round_up(end, ch);
 }
 
@@ -457,13 +496,13 @@ void launch_factor(const g2g_plan_s* P, const Real* A, Real* U, int a_stride, in
     const SymbolicLU& s = P->lu();
     const size_t lds = factor_lds<Real>(s);
     if (lds > lds_limit()) throw std::runtime_error("factorization needs " + std::to_string(lds) + " bytes of LDS");
-    FactorArgs<Real> a{A, U, P->d_prog_f, tile_active, jmax, clamped, error, s.d, a_stride, u_stride,
-                       P->prog_f_stride, s.slots_f, rows, s.max_c, eps, P->d_seg};
-    const void* fn = s.walkers == 2 ? reinterpret_cast<const void*>(&k_factor<Real, 2>)
-                                    : reinterpret_cast<const void*>(&k_factor<Real, 1>);
+    FactorArgs<Real> a{A,     U,    P->d_prog_f, tile_active, jmax,    clamped, error,  s.d, a_stride, u_stride,
+                       P->prog_f_stride, s.slots_f, rows, s.max_c, eps, P->d_seg, T, 0, nullptr, 0};
+    const void* fn = s.walkers == 2 ? reinterpret_cast<const void*>(&k_factor<Real, 2, false>)
+                                    : reinterpret_cast<const void*>(&k_factor<Real, 1, false>);
     HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    if (s.walkers == 2) k_factor<Real, 2><<<T, kLanes * walk_waves(2), lds, st>>>(a);
-    else k_factor<Real, 1><<<T, kLanes * walk_waves(1), lds, st>>>(a);
+    if (s.walkers == 2) k_factor<Real, 2, false><<<T, kLanes * walk_waves(2), lds, st>>>(a);
+    else k_factor<Real, 1, false><<<T, kLanes * walk_waves(1), lds, st>>>(a);
     HIPCHK(hipGetLastError());
 }
 
@@ -475,15 +514,82 @@ void launch_backward(const g2g_plan_s* P, const Real* U, int u_stride, const int
     const size_t lds = backward_lds<Real>(s);
     if (lds > lds_limit()) throw std::runtime_error("backward substitution needs too much LDS");
     BackwardArgs<Real> a{U, P->d_prog_b, tile_active, X, error, s.d, s.u_len, u_stride, P->prog_b_stride, rows,
-                         s.max_c, P->d_bseg, s.n_top};
-    const void* fn = s.bwalkers == 4   ? reinterpret_cast<const void*>(&k_backward<Real, 4>)
-                     : s.bwalkers == 2 ? reinterpret_cast<const void*>(&k_backward<Real, 2>)
-                                       : reinterpret_cast<const void*>(&k_backward<Real, 1>);
+                         s.max_c, P->d_bseg, s.n_top, T, 0};
+    const void* fn = s.bwalkers == 4   ? reinterpret_cast<const void*>(&k_backward<Real, 4, false>)
+                     : s.bwalkers == 2 ? reinterpret_cast<const void*>(&k_backward<Real, 2, false>)
+                                       : reinterpret_cast<const void*>(&k_backward<Real, 1, false>);
     HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    if (s.bwalkers == 4) k_backward<Real, 4><<<T, kLanes * walk_waves(4), lds, st>>>(a);
-    else if (s.bwalkers == 2) k_backward<Real, 2><<<T, kLanes * walk_waves(2), lds, st>>>(a);
-    else k_backward<Real, 1><<<T, kLanes * walk_waves(1), lds, st>>>(a);
+    if (s.bwalkers == 4) k_backward<Real, 4, false><<<T, kLanes * walk_waves(4), lds, st>>>(a);
+    else if (s.bwalkers == 2) k_backward<Real, 2, false><<<T, kLanes * walk_waves(2), lds, st>>>(a);
+    else k_backward<Real, 1, false><<<T, kLanes * walk_waves(1), lds, st>>>(a);
     HIPCHK(hipGetLastError());
+}
+
+// LDS of a split-walk workgroup: its rings, then its slots (+ zero and trash)
+// and long-column scratch rows
+template <typename Real>
+size_t split_factor_lds(const SymbolicLU& s)
+{
+    const int scratch = s.max_c > kFastC ? 2 * s.max_c : 0;
+    return static_cast<size_t>(split_ring_bytes(sizeof(Real), kSplitRingF)) +
+           static_cast<size_t>(s.slots_fs + 2 + scratch) * kLanes * sizeof(Real);
+}
+
+template <typename Real>
+size_t split_backward_lds(const SymbolicLU& s)
+{
+    return static_cast<size_t>(split_ring_bytes(sizeof(Real), kSplitRingB)) +
+           static_cast<size_t>(s.slots_bs + 2) * kLanes * sizeof(Real);
+}
+
+// The split walk's factorization: the fine sets, then the middle parts, then
+// the top part, one launch each over the first T tiles.  clamped counts are
+// added per row, so they start at zero.
+template <typename Real>
+void launch_factor_split(const g2g_plan_s* P, const Real* A, Real* U, int a_stride, int u_stride,
+                         const int* tile_active, const double* jmax, int* clamped, int* error, int rows, int T,
+                         double eps, Real* pend, hipStream_t st)
+{
+    const SymbolicLU& s = P->lu();
+    const size_t lds = split_factor_lds<Real>(s);
+    if (lds > lds_limit()) throw std::runtime_error("split factorization needs " + std::to_string(lds) + " bytes of LDS");
+    if (clamped) {
+        k_fill_int<<<blocks_for(static_cast<long long>(T) * kLanes, 256), 256, 0, st>>>(clamped, 0,
+                                                                                       static_cast<long long>(T) * kLanes);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_factor<Real, 1, true>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    const int levels[3][2] = {{0, s.split_fine}, {s.split_fine, s.split_mid}, {s.split_fine + s.split_mid, 1}};
+    for (const auto& lv : levels) {
+        if (lv[1] <= 0) continue;
+        FactorArgs<Real> a{A,   U,        P->d_prog_fs, tile_active, jmax, clamped, error, s.d, a_stride,
+                           u_stride, P->prog_fs_stride, s.slots_fs, rows, s.max_c, eps, P->d_split_seg,
+                           T,   lv[0],    pend,         s.pend_len};
+        k_factor<Real, 1, true><<<T * lv[1], kLanes * (2 + kSplitProducers), lds, st>>>(a);
+        HIPCHK(hipGetLastError());
+    }
+}
+
+// The split walk's backward substitution: the top part, then the middle
+// parts, then the fine sets.
+template <typename Real>
+void launch_backward_split(const g2g_plan_s* P, const Real* U, int u_stride, const int* tile_active, Real* X,
+                           int* error, int rows, int T, hipStream_t st)
+{
+    const SymbolicLU& s = P->lu();
+    const size_t lds = split_backward_lds<Real>(s);
+    if (lds > lds_limit()) throw std::runtime_error("split backward substitution needs too much LDS");
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_backward<Real, 1, true>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    const int levels[3][2] = {{s.split_fine + s.split_mid, 1}, {s.split_fine, s.split_mid}, {0, s.split_fine}};
+    for (const auto& lv : levels) {
+        if (lv[1] <= 0) continue;
+        BackwardArgs<Real> a{U, P->d_prog_bs, tile_active, X, error, s.d, s.split_u_len, u_stride,
+                             P->prog_bs_stride, rows, s.max_c, P->d_split_bseg, 0, T, lv[0]};
+        k_backward<Real, 1, true><<<T * lv[1], kLanes * (2 + kSplitProducers), lds, st>>>(a);
+        HIPCHK(hipGetLastError());
+    }
 }
 
 // `steps` rounds of x += (LU)^-1 (b - A x) on the tiles' A streams: the
@@ -607,7 +713,9 @@ void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double
         k_ext_pre<Real><<<b->T, kLanes, 0, b->stream>>>(ext_args<Real>(b));
         HIPCHK(hipGetLastError());
     }
-    MismatchArgs<Real> a{b->plan->d_k1_bus, transposed ? b->plan->d_k1_ent_t : b->plan->d_k1_ent, P.k1_nbus, b->plan->d_Y, b->Yrow, b->V, b->S,
+    const int* k1b = b->split_layout ? b->plan->d_k1_bus_s : b->plan->d_k1_bus;
+    const int* k1e = b->split_layout ? b->plan->d_k1_ent_s : transposed ? b->plan->d_k1_ent_t : b->plan->d_k1_ent;
+    MismatchArgs<Real> a{k1b, k1e, P.k1_nbus, b->plan->d_Y, b->Yrow, b->V, b->S,
                          static_cast<Real*>(b->A), P.n, P.nY, b->a_stride, b->res, b->jmax, b->status, b->iters,
                          b->tile_active, b->n_active, b->active_rows, iter, max_iter, tol, freeze, growth,
                          (b->streaming || b->restart_on) ? b->it0 : nullptr, b->mask,
@@ -809,6 +917,40 @@ void uncompact_rows(g2g_batch_s* b)
     HIPCHK(hipGetLastError());
 }
 
+// Whether the rest of a solve takes the split walk: when few tiles are left
+// (a launch over a tile costs the whole walk of its rows, the split walk cuts
+// it into dozens of independent programs); G2G_SPLIT=0 / 1 forces it off / on.
+int split_env()
+{
+    if (const char* e = std::getenv("G2G_SPLIT")) return std::atoi(e) ? 1 : 0;
+    return -1;
+}
+
+constexpr int kSplitMaxTiles = 128;  // active tiles up to which the split walk is faster
+
+bool use_split(const g2g_batch_s* b, int active)
+{
+    if (!b->plan->split() || !b->pend || b->ir_steps > 0) return false;
+    const int env = split_env();
+    if (env >= 0) return env == 1;
+    const int tiles = b->compact_R > 0 ? (active + kLanes - 1) / kLanes : std::min(b->T, active);
+    return tiles <= kSplitMaxTiles;
+}
+
+// tiles a split launch covers: the compacted rows' tiles, or all
+int split_tiles(const g2g_batch_s* b) { return b->compact_R > 0 ? (b->compact_R + kLanes - 1) / kLanes : b->T; }
+
+// The A stream of tiles [0, T) to zero.  Each layout leaves its fill
+// positions (factor entries with no J value) at zero, and the two layouts'
+// fill positions differ, so a switch of layout starts from zeros.
+void zero_a_tiles(g2g_batch_s* b, int T)
+{
+    const long long words = static_cast<long long>(T) * b->a_stride * kLanes * static_cast<long long>(b->real_size()) / 4;
+    k_fill_int<<<blocks_for(words, 256), 256, 0, b->stream>>>(static_cast<int*>(b->A), 0, words);
+    HIPCHK(hipGetLastError());
+    b->split_T = std::max(b->split_T, T);
+}
+
 template <typename Real>
 int nr_loop_body(g2g_batch_s* b, int max_iter, double tol, int freeze, double growth, SolveTimer& tm,
                  long long& rows_factored, bool& compacted)
@@ -863,15 +1005,35 @@ int nr_loop_body(g2g_batch_s* b, int max_iter, double tol, int freeze, double gr
             launch_mismatch<Real>(b, true, it, max_iter, tol, freeze, growth);
             tm.ph[0].push_back({e0, tm.mark()});
         }
+        if (!b->split_layout && use_split(b, *b->h_active)) {
+            // the rest of the solve takes the split walk: the A stream again,
+            // in its layout (same V, same decisions), over zeros (its fill
+            // positions are not the main layout's)
+            zero_a_tiles(b, split_tiles(b));
+            b->split_layout = true;
+            k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
+            e0 = tm.mark();
+            launch_mismatch<Real>(b, true, it, max_iter, tol, freeze, growth);
+            tm.ph[0].push_back({e0, tm.mark()});
+        }
         e0 = tm.mark();
-        launch_factor<Real>(b->plan, static_cast<const Real*>(b->A), static_cast<Real*>(b->U), b->a_stride,
-                            b->u_stride, b->tile_active, b->jmax, b->clamped, b->error, b->active_rows, b->T, eps,
-                            b->stream);
+        if (b->split_layout)
+            launch_factor_split<Real>(b->plan, static_cast<const Real*>(b->A), static_cast<Real*>(b->U), b->a_stride,
+                                      b->u_stride, b->tile_active, b->jmax, b->clamped, b->error, b->active_rows,
+                                      split_tiles(b), eps, static_cast<Real*>(b->pend), b->stream);
+        else
+            launch_factor<Real>(b->plan, static_cast<const Real*>(b->A), static_cast<Real*>(b->U), b->a_stride,
+                                b->u_stride, b->tile_active, b->jmax, b->clamped, b->error, b->active_rows, b->T,
+                                eps, b->stream);
         tm.ph[1].push_back({e0, tm.mark()});
         rows_factored += *b->h_active;
         e0 = tm.mark();
-        launch_backward<Real>(b->plan, static_cast<const Real*>(b->U), b->u_stride, b->tile_active,
-                              static_cast<Real*>(b->X), b->error, b->active_rows, b->T, b->stream);
+        if (b->split_layout)
+            launch_backward_split<Real>(b->plan, static_cast<const Real*>(b->U), b->u_stride, b->tile_active,
+                                        static_cast<Real*>(b->X), b->error, b->active_rows, split_tiles(b), b->stream);
+        else
+            launch_backward<Real>(b->plan, static_cast<const Real*>(b->U), b->u_stride, b->tile_active,
+                                  static_cast<Real*>(b->X), b->error, b->active_rows, b->T, b->stream);
         if (b->ir_steps > 0)
             refine<Real>(b->plan, static_cast<Real*>(b->A), static_cast<Real*>(b->U), static_cast<Real*>(b->X),
                          static_cast<Real*>(b->Bs), static_cast<Real*>(b->Xs), b->a_stride, b->u_stride,
@@ -893,11 +1055,21 @@ int nr_loop(g2g_batch_s* b, int max_iter, double tol, int freeze, double growth,
             long long& rows_factored)
 {
     bool compacted = false;
+    b->split_layout = false;
+    b->split_T = 0;
+    // back to the main layout: the tiles the split walk used go back to zeros
+    auto leave_split = [&] {
+        if (b->split_layout) zero_a_tiles(b, b->split_T);
+        b->split_layout = false;
+        b->split_T = 0;
+    };
     try {
         const int it = nr_loop_body<Real>(b, max_iter, tol, freeze, growth, tm, rows_factored, compacted);
+        leave_split();
         if (compacted) uncompact_rows(b);
         return it;
     } catch (...) {
+        leave_split();
         if (compacted) uncompact_rows(b);
         throw;
     }
@@ -1096,7 +1268,7 @@ void raw_solve(g2g_plan_s* P, int rows, const double* vals, const double* rhs, d
     const int T = (rows + kLanes - 1) / kLanes;
     const size_t rp = static_cast<size_t>(T) * kLanes;
     const int nnz = P->nnz, d = s.d;
-    const int a_stride = round_up(s.a_len, chunk_of<Real>()), u_stride = u_stride_of(s, sizeof(Real));
+    const int a_stride = a_stride_of(s, sizeof(Real)), u_stride = u_stride_of(s, sizeof(Real));
     DeviceScratch B;
     double* d_vals = B.get<double>(static_cast<size_t>(rows) * nnz);
     double* d_rhs = B.get<double>(static_cast<size_t>(rows) * d);
@@ -1188,7 +1360,7 @@ size_t raw_scratch_bytes(const SymbolicLU& s, int rows, int ir_steps)
     ScratchCarve c{nullptr};
     const int T = (rows + kLanes - 1) / kLanes;
     const size_t rp = static_cast<size_t>(T) * kLanes;
-    c.take<Real>(rp * round_up(s.a_len, chunk_of<Real>()));
+    c.take<Real>(rp * a_stride_of(s, sizeof(Real)));
     c.take<Real>(rp * u_stride_of(s, sizeof(Real)));
     c.take<Real>(rp * s.d);
     c.take<double>(rp);
@@ -1227,7 +1399,7 @@ void raw_solve_noalloc(const g2g_plan_s* P0, int rows, const double* d_vals, con
     const int T = (rows + kLanes - 1) / kLanes;
     const size_t rp = static_cast<size_t>(T) * kLanes;
     const int nnz = P->nnz, d = s.d;
-    const int a_stride = round_up(s.a_len, chunk_of<Real>()), u_stride = u_stride_of(s, sizeof(Real));
+    const int a_stride = a_stride_of(s, sizeof(Real)), u_stride = u_stride_of(s, sizeof(Real));
     ScratchCarve c{scratch_base()};
     Real* A = c.take<Real>(rp * a_stride);
     Real* U = c.take<Real>(rp * u_stride);
@@ -1453,6 +1625,12 @@ int g2g_plan_get_info(g2g_plan_t P, g2g_plan_info* info)
         info->u_len = s.u_len;
         info->u_stride_f64 = u_stride_of(s, 8);
         info->u_stride_f32 = u_stride_of(s, 4);
+        info->n_split = s.n_split;
+        info->slots_fs = s.slots_fs;
+        info->slots_bs = s.slots_bs;
+        info->pend_len = s.pend_len;
+        info->split_a_len = s.split_a_len;
+        info->split_u_len = s.split_u_len;
         if (P->grid) {
             info->n_ext = P->nr.n_ext;
             info->n_xs = P->nr.xt.n_xs;
@@ -1495,6 +1673,18 @@ int g2g_plan_get_array(g2g_plan_t P, int32_t which, int32_t* out, int64_t capaci
             tmp = seg_words(s.bseg);
             v = &tmp;
             break;
+        case G2G_ARR_PROG_FS: v = &s.prog_fs; break;
+        case G2G_ARR_PROG_BS: v = &s.prog_bs; break;
+        case G2G_ARR_SPLIT_SEGS:
+            tmp = seg_words(s.split_seg);
+            v = &tmp;
+            break;
+        case G2G_ARR_SPLIT_BSEGS:
+            tmp = seg_words(s.split_bseg);
+            v = &tmp;
+            break;
+        case G2G_ARR_SPLIT_APOS: v = &s.split_apos; break;
+        case G2G_ARR_SPLIT_FPOS: v = &s.split_fpos; break;
         default: throw std::invalid_argument("unknown array id");
         }
         if (length) *length = static_cast<int64_t>(v->size());
@@ -1544,8 +1734,7 @@ int g2g_batch_create(g2g_plan_t P, int32_t rows, int32_t precision, int32_t per_
                 P->d_Y, nr.nY, b->rows_pad, b->Yrow);
             HIPCHK(hipGetLastError());
         }
-        const int ch = precision == G2G_MIXED ? kChunkF32 : kChunkF64;
-        b->a_stride = round_up(nr.lu.a_len, ch);
+        b->a_stride = a_stride_of(nr.lu, static_cast<int>(b->real_size()));
         b->u_stride = u_stride_of(nr.lu, static_cast<int>(b->real_size()));
         b->A = dalloc<unsigned char>(rp * b->a_stride * b->real_size());
         b->U = dalloc<unsigned char>(rp * b->u_stride * b->real_size());
@@ -1556,6 +1745,8 @@ int g2g_batch_create(g2g_plan_t P, int32_t rows, int32_t precision, int32_t per_
         b->mis_splits = default_mis_splits();
         if (b->mis_splits > 1 || P->ext())
             b->mis_part = dalloc<double>(2 * static_cast<size_t>(b->mis_splits) * rp);
+        if (P->split())
+            b->pend = dalloc<unsigned char>(rp * static_cast<size_t>(std::max(1, nr.lu.pend_len)) * b->real_size());
         if (P->ext()) {
             const size_t nxs = static_cast<size_t>(std::max(1, nr.xt.n_xs));
             b->xs = dalloc<double>(rp * nxs);

@@ -128,6 +128,11 @@ struct FactorArgs {
     int max_c;              // longest column (sizes the long-column scratch rows)
     double pivot_eps;
     const int* seg;         // per walker: prog_f off/len, prog_b off/len, A off/len, U off/len, pivots
+    // split walk: the launch covers programs seg0 .. seg0 + gridDim.x / T - 1 (program-major,
+    // T tiles each); pending update targets go through pend [T][pend_stride][64]
+    int T, seg0;
+    Real* pend;
+    int pend_stride;
 };
 
 template <typename Real>
@@ -141,6 +146,7 @@ struct BackwardArgs {
     int max_c;              // longest column (a longer record header means a corrupt walk)
     const int* seg;         // per walker segments (as FactorArgs::seg)
     int n_top;              // pivots walker 0 walks before the others start
+    int T, seg0;            // split walk (as FactorArgs)
 };
 
 template <typename Real>

@@ -869,24 +869,84 @@ __device__ int factor_merge(ProgReader& P, FactorState<Real>& st)
     return next;
 }
 
-__device__ __forceinline__ bool head_ok(int h, int max_c) { return h == kMergeHead || (h & 0x7FFF) <= max_c; }
+// split walk: dump [kDumpHead | next << 16, n, first, ref x n]: pend[first + i] =
+// slot ref_i (then cleared); gather [kGatherHead | next << 16, n, (dst ref,
+// index) x n]: dst += pend[index].  Both return the next record's head.
+template <typename Real>
+__device__ int factor_dump(ProgReader& P, FactorState<Real>& st, Real* pend_lane, int pend_stride)
+{
+    P.window(st.p, st.p + 2);
+    const int n = __builtin_amdgcn_readfirstlane(P.at(st.p + 1));
+    const int first = __builtin_amdgcn_readfirstlane(P.at(st.p + 2));
+    const int m = n > kSplitRecItems ? kSplitRecItems : (n < 0 ? 0 : n);
+    const int nw = dump_words(m);
+    P.window(st.p, st.p + nw - 1);
+    for (int i = 0; i < m; ++i) {
+        const unsigned ref = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(st.p + 3 + i)));
+        const int q = first + i;
+        Real* ps = slot_at<Real>(st.lb, ref);
+        if (static_cast<unsigned>(q) < static_cast<unsigned>(pend_stride))
+            pend_lane[static_cast<size_t>(q) * kLanes] = *ps;
+        *ps = Real(0);
+    }
+    const int next = static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(st.p))) >> 16);
+    st.p += nw;
+    return next;
+}
+
+template <typename Real>
+__device__ int factor_gather(ProgReader& P, FactorState<Real>& st, const Real* pend_lane, int pend_stride)
+{
+    P.window(st.p, st.p + 1);
+    const int n = __builtin_amdgcn_readfirstlane(P.at(st.p + 1));
+    const int m = n > kSplitRecItems ? kSplitRecItems : (n < 0 ? 0 : n);
+    const int nw = pair_rec_words(m);
+    P.window(st.p, st.p + nw - 1);
+    for (int i = 0; i < m; ++i) {
+        const unsigned dst = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(st.p + 2 + 2 * i)));
+        const int q = __builtin_amdgcn_readfirstlane(P.at(st.p + 3 + 2 * i));
+        if (static_cast<unsigned>(q) < static_cast<unsigned>(pend_stride))
+            *slot_at<Real>(st.lb, dst) += pend_lane[static_cast<size_t>(q) * kLanes];
+    }
+    const int next = static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(st.p))) >> 16);
+    st.p += nw;
+    return next;
+}
+
+__device__ __forceinline__ bool head_ok(int h, int max_c)
+{
+    return h == kMergeHead || h == kDumpHead || h == kGatherHead || (h & 0x7FFF) <= max_c;
+}
 
 // Workgroup over one tile of 64 rows: W walker wavefronts walk their
 // program segments (LDS only, plus the U-stream stores); value_producers(W)
 // waves stream the walkers' A segments into their LDS rings (split evenly
 // between the walkers) and W waves stream the program segments.
-template <typename Real, int W>
-__global__ __launch_bounds__(kLanes * walk_waves(W)) void k_factor(FactorArgs<Real> a)
+// SPLIT: one walker per workgroup over one program of the split walk (the
+// launch covers programs seg0 .., T tiles each, program-major), smaller rings,
+// dump and gather records; clamp counts are added per row.
+template <int W, bool SPLIT>
+__host__ __device__ constexpr int factor_waves()
 {
-    constexpr int kCh = value_chunk(sizeof(Real), W), kNch = value_ring_slots(sizeof(Real), W);
+    return SPLIT ? 2 + kSplitProducers : walk_waves(W);
+}
+
+template <typename Real, int W, bool SPLIT>
+__global__ __launch_bounds__((kLanes * factor_waves<W, SPLIT>())) void k_factor(FactorArgs<Real> a)
+{
+    static_assert(!SPLIT || W == 1, "a split-walk workgroup has one walker");
+    constexpr int RW = SPLIT ? kSplitRingF : W;  // ring geometry
+    constexpr int kCh = value_chunk(sizeof(Real), RW), kNch = value_ring_slots(sizeof(Real), RW);
+    static_assert((kNch - 2) * kCh >= 6 + 4 * kFastC2 - 1, "the ring window spans a two-column step's values");
     using ARing = Ring<Real, kCh, kNch, false, true>;
-    constexpr int kNP = value_producers(W), kP = kNP / W;
+    constexpr int kNP = SPLIT ? kSplitProducers : value_producers(W), kP = kNP / W;
     static_assert(kP * W == kNP, "producers split evenly");
     static_assert(sizeof(typename ARing::Ctl) + sizeof(ProgRing::Ctl) <= kWalkerCtlBytes, "control blocks");
     const int lane = threadIdx.x & (kLanes - 1);
     // wave-uniform (scalar) role, so the segment table is read with scalar loads
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kLanes));
-    const size_t tile = blockIdx.x;
+    const size_t tile = SPLIT ? blockIdx.x % a.T : blockIdx.x;
+    const int seg_base = SPLIT ? a.seg0 + static_cast<int>(blockIdx.x / a.T) : 0;  // this workgroup's program
     if (a.tile_active[tile] <= 0) return;
     extern __shared__ __align__(16) unsigned char g2g_smem[];
     auto ctlA = [&](int w) { return reinterpret_cast<typename ARing::Ctl*>(g2g_smem + w * kWalkerCtlBytes); };
@@ -914,20 +974,20 @@ __global__ __launch_bounds__(kLanes * walk_waves(W)) void k_factor(FactorArgs<Re
     const Real* Atile = a.A + tile * static_cast<size_t>(a.a_stride) * kLanes;
     if (wave >= W && wave < W + kNP) {
         const int pw = (wave - W) / kP;
-        const int* sg = a.seg + 9 * pw;
+        const int* sg = a.seg + 9 * (seg_base + pw);
         ARing::produce(Atile + static_cast<size_t>(sg[4]) * kLanes, (sg[5] + kCh - 1) / kCh, aring(pw), ctlA(pw),
                        lane, (wave - W) % kP, kP);
         return;
     }
     if (wave >= W + kNP) {
         const int pw = wave - W - kNP;
-        const int* sg = a.seg + 9 * pw;
+        const int* sg = a.seg + 9 * (seg_base + pw);
         ProgRing::produce(a.prog + sg[0], sg[1] / kProgChunk, pring(pw), ctlP(pw), lane, 0, 1);
         return;
     }
     // walker `wave`: its program segment
     const int wk = wave;
-    const int* sg = a.seg + 9 * wk;
+    const int* sg = a.seg + 9 * (seg_base + wk);
     const int n_piv = sg[8];
     const int row = static_cast<int>(tile) * kLanes + lane;
     // long-column scratch rows of this walker (long columns only)
@@ -943,10 +1003,16 @@ __global__ __launch_bounds__(kLanes * walk_waves(W)) void k_factor(FactorArgs<Re
     st.eps = static_cast<Real>(a.pivot_eps) * static_cast<Real>(a.jmax ? a.jmax[row] : 1.0);
     st.clamped = 0;
     st.p = st.ap = st.up = 0;
-    int h = n_piv > 0 ? prog_word(P, 0) & 0xFFFF : 0;  // an empty segment has no record
+    Real* pend_lane = SPLIT ? a.pend + tile * static_cast<size_t>(a.pend_stride) * kLanes + lane : nullptr;
+    int h = (n_piv > 0 || SPLIT) && sg[1] > 0 ? prog_word(P, 0) & 0xFFFF : 0;  // an empty segment has no record
     bool bad = !head_ok(h, a.max_c);
     bool waited = W == 1 || wk != 0;
     for (int k = 0; k < n_piv && !bad;) {
+        if (SPLIT && h == kGatherHead) {
+            h = factor_gather(P, st, pend_lane, a.pend_stride) & 0xFFFF;
+            if (P.failed || !head_ok(h, a.max_c)) bad = true;
+            continue;
+        }
         if (h == kMergeHead) {
             // the other walkers' first phase is over: their pending
             // contributions to the top entries are final
@@ -983,6 +1049,12 @@ __global__ __launch_bounds__(kLanes * walk_waves(W)) void k_factor(FactorArgs<Re
             break;
         }
     }
+    if (SPLIT) {  // the pending update targets left by this program
+        while (!bad && h == kDumpHead) {
+            h = factor_dump(P, st, pend_lane, a.pend_stride) & 0xFFFF;
+            if (P.failed) bad = true;
+        }
+    }
     if (bad) {  // release the producers at once
         if (lane == 0) atomicOr(a.error, 1);
         lds_store_release(&ctlA(wk)->abort, 1);
@@ -1000,7 +1072,13 @@ __global__ __launch_bounds__(kLanes * walk_waves(W)) void k_factor(FactorArgs<Re
         // per row: each lane adds the other walkers' counts of its own row
         for (int w = 1; w < W; ++w) st.clamped += lds_load_lane(&flags->clamped[w][lane]);
     }
-    if (a.clamped && row < a.rows) a.clamped[row] = st.clamped;
+    if (a.clamped && row < a.rows) {
+        if (SPLIT) {  // every program of the walk adds its count
+            if (st.clamped) atomicAdd(&a.clamped[row], st.clamped);
+        } else {
+            a.clamped[row] = st.clamped;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1117,21 +1195,53 @@ __device__ __forceinline__ int backward_dispatch(int c, UR& U, ProgReader& P, Ba
     }
 }
 
+// split walk: preload [kPreloadHead | next << 16, n, (dst ref, x index) x n]:
+// dst = x[index], a higher program's solution; returns the next head
+template <typename Real>
+__device__ int backward_preload(ProgReader& P, BackState<Real>& st)
+{
+    P.window(st.p, st.p + 1);
+    const int n = __builtin_amdgcn_readfirstlane(P.at(st.p + 1));
+    const int m = n > kSplitRecItems ? kSplitRecItems : (n < 0 ? 0 : n);
+    const int nw = pair_rec_words(m);
+    P.window(st.p, st.p + nw - 1);
+    for (int i = 0; i < m; ++i) {
+        const unsigned dst = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(st.p + 2 + 2 * i)));
+        const int q = __builtin_amdgcn_readfirstlane(P.at(st.p + 3 + 2 * i));
+        if (static_cast<unsigned>(q) < static_cast<unsigned>(st.d))
+            *slot_at<Real>(st.lb, dst) = st.X[static_cast<size_t>(q) * kLanes];
+    }
+    const int next = static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(st.p))) >> 16);
+    st.p += nw;
+    return next;
+}
+
 // Same roles as k_factor.  Walker 0 walks the top part first and then
 // signals the others, which walk their subtrees once the top's x are in
-// their slots.
-template <typename Real, int W>
-__global__ __launch_bounds__(kLanes * walk_waves(W)) void k_backward(BackwardArgs<Real> a)
+// their slots.  SPLIT: one walker per workgroup over one program of the
+// split walk, whose preload records bring the x of higher programs.
+template <int W, bool SPLIT>
+__host__ __device__ constexpr int backward_waves()
 {
-    constexpr int kCh = value_chunk(sizeof(Real), W), kNch = value_ring_slots(sizeof(Real), W);
+    return SPLIT ? 2 + kSplitProducers : walk_waves(W);
+}
+
+template <typename Real, int W, bool SPLIT>
+__global__ __launch_bounds__((kLanes * backward_waves<W, SPLIT>())) void k_backward(BackwardArgs<Real> a)
+{
+    static_assert(!SPLIT || W == 1, "a split-walk workgroup has one walker");
+    constexpr int RW = SPLIT ? kSplitRingB : W;
+    constexpr int kCh = value_chunk(sizeof(Real), RW), kNch = value_ring_slots(sizeof(Real), RW);
+    static_assert((kNch - 2) * kCh >= 5 + 2 * kFastC2 - 1, "the ring window spans a two-column record");
     using URing = Ring<Real, kCh, kNch, true, true>;
-    constexpr int kNP = value_producers(W), kP = kNP / W;
+    constexpr int kNP = SPLIT ? kSplitProducers : value_producers(W), kP = kNP / W;
     static_assert(kP * W == kNP, "producers split evenly");
     static_assert(sizeof(typename URing::Ctl) + sizeof(ProgRing::Ctl) <= kWalkerCtlBytes, "control blocks");
     const int lane = threadIdx.x & (kLanes - 1);
     // wave-uniform (scalar) role, so the segment table is read with scalar loads
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kLanes));
-    const size_t tile = blockIdx.x;
+    const size_t tile = SPLIT ? blockIdx.x % a.T : blockIdx.x;
+    const int seg_base = SPLIT ? a.seg0 + static_cast<int>(blockIdx.x / a.T) : 0;
     if (a.tile_active[tile] <= 0) return;
     extern __shared__ __align__(16) unsigned char g2g_smem[];
     auto ctlU = [&](int w) { return reinterpret_cast<typename URing::Ctl*>(g2g_smem + w * kWalkerCtlBytes); };
@@ -1156,19 +1266,19 @@ __global__ __launch_bounds__(kLanes * walk_waves(W)) void k_backward(BackwardArg
     const Real* Utile = a.U + tile * static_cast<size_t>(a.u_stride) * kLanes;
     if (wave >= W && wave < W + kNP) {
         const int pw = (wave - W) / kP;
-        const int* sg = a.seg + 9 * pw;
+        const int* sg = a.seg + 9 * (seg_base + pw);
         URing::produce(Utile + static_cast<size_t>(sg[6]) * kLanes, (sg[7] + kCh - 1) / kCh, uring(pw), ctlU(pw),
                        lane, (wave - W) % kP, kP);
         return;
     }
     if (wave >= W + kNP) {
         const int pw = wave - W - kNP;
-        const int* sg = a.seg + 9 * pw;
+        const int* sg = a.seg + 9 * (seg_base + pw);
         ProgRing::produce(a.prog + sg[2], sg[3] / kProgChunk, pring(pw), ctlP(pw), lane, 0, 1);
         return;
     }
     const int wk = wave;
-    const int* sg = a.seg + 9 * wk;
+    const int* sg = a.seg + 9 * (seg_base + wk);
     const int n_piv = sg[8];
     typename URing::Reader U;
     U.init(uring(wk), ctlU(wk), (sg[7] + kCh - 1) / kCh, lane);
@@ -1184,8 +1294,13 @@ __global__ __launch_bounds__(kLanes * walk_waves(W)) void k_backward(BackwardArg
     bool signaled = W == 1 || wk != 0;
     if (W > 1 && wk != 0 && !wait_flag(&flags->top_done)) bad = true;
     int h = n_piv > 0 && !bad ? prog_word(P, 0) & 0xFFFF : 0;  // an empty segment has no record
-    if ((h & 0x7FFF) > a.max_c) bad = true;
+    if ((h & 0x7FFF) > a.max_c && !(SPLIT && h == kPreloadHead)) bad = true;
     for (int q = 0; q < n_piv && !bad;) {
+        if (SPLIT && h == kPreloadHead) {
+            h = backward_preload(P, st) & 0xFFFF;
+            if (P.failed || ((h & 0x7FFF) > a.max_c && h != kPreloadHead)) bad = true;
+            continue;
+        }
         if (!signaled && q >= a.n_top) {
             signaled = true;
             lds_store_release(&flags->top_done, 1);
@@ -1957,16 +2072,20 @@ template __global__ void k_slots_to_rows<double2>(const double2*, int, int, cons
 template __global__ void k_move_vals<int>(const int*, int*, const int*, int, int, int);
 template __global__ void k_move_vals<double>(const double*, double*, const int*, int, int, int);
 template __global__ void k_move_vals<unsigned char>(const unsigned char*, unsigned char*, const int*, int, int, int);
-template __global__ void k_factor<double, 1>(FactorArgs<double>);
-template __global__ void k_factor<float, 1>(FactorArgs<float>);
-template __global__ void k_factor<double, 2>(FactorArgs<double>);
-template __global__ void k_factor<float, 2>(FactorArgs<float>);
-template __global__ void k_backward<double, 1>(BackwardArgs<double>);
-template __global__ void k_backward<float, 1>(BackwardArgs<float>);
-template __global__ void k_backward<double, 2>(BackwardArgs<double>);
-template __global__ void k_backward<float, 2>(BackwardArgs<float>);
-template __global__ void k_backward<double, 4>(BackwardArgs<double>);
-template __global__ void k_backward<float, 4>(BackwardArgs<float>);
+template __global__ void k_factor<double, 1, false>(FactorArgs<double>);
+template __global__ void k_factor<float, 1, false>(FactorArgs<float>);
+template __global__ void k_factor<double, 2, false>(FactorArgs<double>);
+template __global__ void k_factor<float, 2, false>(FactorArgs<float>);
+template __global__ void k_factor<double, 1, true>(FactorArgs<double>);
+template __global__ void k_factor<float, 1, true>(FactorArgs<float>);
+template __global__ void k_backward<double, 1, false>(BackwardArgs<double>);
+template __global__ void k_backward<float, 1, false>(BackwardArgs<float>);
+template __global__ void k_backward<double, 2, false>(BackwardArgs<double>);
+template __global__ void k_backward<float, 2, false>(BackwardArgs<float>);
+template __global__ void k_backward<double, 4, false>(BackwardArgs<double>);
+template __global__ void k_backward<float, 4, false>(BackwardArgs<float>);
+template __global__ void k_backward<double, 1, true>(BackwardArgs<double>);
+template __global__ void k_backward<float, 1, true>(BackwardArgs<float>);
 template __global__ void k_ext_pre<double>(ExtArgs<double>);
 template __global__ void k_ext_pre<float>(ExtArgs<float>);
 template __global__ void k_ext_post<double>(ExtArgs<double>);

@@ -201,10 +201,11 @@ inline int round_up_to(int v, int m) { return (v + m - 1) / m * m; }
 // A record of a factorization program: a walk step (single pivot or
 // 2-column supernode) or a merge of another walker's pending values.
 struct Rec {
-    enum { kStep = 0, kMerge = 1 };
+    enum { kStep = 0, kMerge = 1, kDump = 2, kGather = 3 };
     int kind = kStep;
-    int k = 0, w = 1, c = 0;  // step: first pivot, width, |struct| of its last pivot
-    std::vector<int> r;       // step: sources then targets (slot | kFirst, -1 none); merge: (dst, src) pairs
+    int k = 0, w = 1, c = 0;  // step: first pivot, width, |struct| of its last pivot; dump: first pend index
+    std::vector<int> r;       // step: sources then targets (slot | kFirst, -1 none); merge: (dst, src) pairs;
+                              // dump: slots; gather: (dst slot, pend index) pairs
 };
 
 // One walker's factorization program under construction: its live update
@@ -214,6 +215,12 @@ struct Rec {
 struct Emitter {
     SymbolicLU& s;
     const std::unordered_map<long long, int>& a_entry;
+    bool record = true;  // false: writes no stream table
+    // the stream tables a recording emitter fills (the main layout's by default)
+    std::vector<int>* apos_tab = &s.a_pos_of_entry;
+    std::vector<int>* fpos_tab = &s.f_pos;
+    std::vector<int>* uoff_tab = &s.u_off;
+    bool count = true;   // adds to nA / n_updates (the main layout only)
     std::unordered_map<long long, int> slot_of;  // live update targets
     std::vector<int> bslot;                      // live forward-substitution targets
     SlotAlloc sa;
@@ -236,10 +243,12 @@ struct Emitter {
     }
     void place(int i, int j)  // dense A stream: every entry has a position
     {
-        auto it = a_entry.find(key(i, j, s.d));
-        if (it != a_entry.end()) {
-            s.a_pos_of_entry[it->second] = apos;
-            ++s.nA;
+        if (record) {
+            auto it = a_entry.find(key(i, j, s.d));
+            if (it != a_entry.end()) {
+                (*apos_tab)[it->second] = apos;
+                if (count) ++s.nA;
+            }
         }
         ++apos;
     }
@@ -267,7 +276,8 @@ struct Emitter {
     void sources(int k, std::vector<int>& r)  // diagonal, b(k), L(struct,k), U(k,struct)
     {
         const int c0 = s.colp[k], c = s.colp[k + 1] - c0;
-        s.f_pos[s.perm[k]] = apos++;
+        if (record) (*fpos_tab)[s.perm[k]] = apos;
+        ++apos;
         place(k, k);
         for (int p = c0; p < c0 + c; ++p) place(s.coli[p], k);
         for (int p = c0; p < c0 + c; ++p) place(k, s.coli[p]);
@@ -281,7 +291,7 @@ struct Emitter {
         r.push_back(bref);
         for (int p = c0; p < c0 + c; ++p) r.push_back(take(s.coli[p], k));
         for (int p = c0; p < c0 + c; ++p) r.push_back(take(k, s.coli[p]));
-        s.u_off[k] = upos;
+        if (record) (*uoff_tab)[k] = upos;
         upos += c + 2;
     }
     void step(int k)
@@ -299,7 +309,7 @@ struct Emitter {
         for (int pa = l0; pa < l0 + cl; ++pa)
             for (int pb = l0; pb < l0 + cl; ++pb) alloc_target(s.coli[pa], s.coli[pb], st.r);
         for (int p = l0; p < l0 + cl; ++p) alloc_b(s.coli[p], st.r);
-        s.n_updates += w == 2 ? 2 * cl * cl : cl * cl;
+        if (record && count) s.n_updates += w == 2 ? 2 * cl * cl : cl * cl;
         for (int sl : consumed) sa.put(sl);
         consumed.clear();
         recs.push_back(std::move(st));
@@ -311,12 +321,37 @@ struct Emitter {
     {
         const auto enc = [](int slot) { return slot * kRefUnit; };
         auto head = [](const Rec& x) {
-            return x.kind == Rec::kMerge ? kMergeHead : rec_head(x.c, x.w);
+            return x.kind == Rec::kMerge    ? kMergeHead
+                   : x.kind == Rec::kDump   ? kDumpHead
+                   : x.kind == Rec::kGather ? kGatherHead
+                                            : rec_head(x.c, x.w);
         };
         for (size_t q = 0; q < recs.size(); ++q) {
             const Rec& st = recs[q];
             const int next = q + 1 < recs.size() ? head(recs[q + 1]) : 0;
             const size_t at = out.size();
+            if (st.kind == Rec::kDump) {
+                const int n = static_cast<int>(st.r.size());
+                out.resize(at + static_cast<size_t>(dump_words(n)), enc(zero));
+                int* wd = out.data() + at;
+                wd[0] = kDumpHead | (next << 16);
+                wd[1] = n;
+                wd[2] = st.k;
+                for (int i = 0; i < n; ++i) wd[3 + i] = enc(st.r[i] + off);
+                continue;
+            }
+            if (st.kind == Rec::kGather) {
+                const int n = static_cast<int>(st.r.size() / 2);
+                out.resize(at + static_cast<size_t>(pair_rec_words(n)), enc(zero));
+                int* wd = out.data() + at;
+                wd[0] = kGatherHead | (next << 16);
+                wd[1] = n;
+                for (int i = 0; i < n; ++i) {
+                    wd[2 + 2 * i] = enc(st.r[2 * i] + off);
+                    wd[3 + 2 * i] = st.r[2 * i + 1];
+                }
+                continue;
+            }
             if (st.kind == Rec::kMerge) {
                 const int n = static_cast<int>(st.r.size() / 2);
                 out.resize(at + static_cast<size_t>(mrec_words(n)), enc(zero));
@@ -341,6 +376,15 @@ struct Emitter {
 std::vector<int> partition_steps(const SymbolicLU& s, const std::vector<int>& parent,
                                  const std::vector<int>& step_first, int W);
 void build_backward(SymbolicLU& s, const std::vector<int>& group, int W);
+
+// where each step of the main programs landed, and each split set's range
+struct SplitLayout {
+    int n = 0;                      // programs: G * F fine sets, G middles, the top (0: no split walk)
+    int n_fine = 0, n_mid = 0;      // G * F fine sets, then G middle parts, then the top part
+    std::vector<int> fine;          // per pivot: its program
+};
+void build_split(SymbolicLU& s, const SplitLayout& L, const std::vector<int>& step_first,
+                 const std::unordered_map<long long, int>& a_entry);
 
 }  // namespace
 
@@ -482,7 +526,35 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
     // group G is the top part
     int G = bwalkers > 0 ? std::min(bwalkers, kMaxWalkers) : W;
     if (G < W || G % W != 0) G = W;
+    // The walker sets: G sets of whole subtrees (largest first into the
+    // lightest) below a small top part.
     const std::vector<int> group = partition_steps(s, parent, step_first, G);
+    // Split walk (large matrices): each walker set is cut once more into F
+    // fine sets of whole subtrees and a middle part above them; the split
+    // walk has stream positions of its own (build_split).
+    SplitLayout SL;
+    if (d >= kSplitMinD) {
+        const int F = std::max(2, kSplitSets / G);
+        SL.n_fine = G * F;
+        SL.n_mid = G;
+        SL.n = SL.n_fine + SL.n_mid + 1;
+        SL.fine.assign(static_cast<size_t>(d), SL.n - 1);
+        for (int g = 0; g < G; ++g) {
+            std::vector<int> sub;
+            for (int k : step_first)
+                if (group[k] == g) sub.push_back(k);
+            if (sub.empty()) continue;
+            const std::vector<int> fg = partition_steps(s, parent, sub, F);  // F: the middle part
+            for (int k : sub) {
+                const int f = fg[k];
+                const int prog = f == F ? SL.n_fine + g : g * F + f;
+                SL.fine[k] = prog;
+                if (s.step_w[k] == 2) SL.fine[k + 1] = prog;
+            }
+        }
+        for (int k = 0; k < d; ++k)
+            if (group[k] == G) SL.fine[k] = SL.n - 1;
+    }
     const int gpw = G / W;
     // U-stream range and pivots of every set (and of the top part)
     std::vector<int> gu_off(static_cast<size_t>(G) + 1, 0), gu_end(static_cast<size_t>(G) + 1, 0),
@@ -621,6 +693,7 @@ SymbolicLU build_symbolic(int d, const int* Ap, const int* Ai, std::vector<int> 
         s.seg[w].prog_f_len = static_cast<int>(s.prog_f.size()) - s.seg[w].prog_f_off;
     }
     build_backward(s, group, G);
+    if (SL.n) build_split(s, SL, step_first, a_entry);
     return s;
 }
 
@@ -638,7 +711,7 @@ std::vector<int> partition_steps(const SymbolicLU& s, const std::vector<int>& pa
     std::vector<int> group(static_cast<size_t>(d), 0);
     if (W <= 1) return group;
     const int ns = static_cast<int>(step_first.size());
-    std::vector<int> step_of(static_cast<size_t>(d));
+    std::vector<int> step_of(static_cast<size_t>(d), -1);  // -1: a step outside the given ones
     for (int q = 0; q < ns; ++q) {
         const int k = step_first[q];
         step_of[k] = q;
@@ -651,7 +724,7 @@ std::vector<int> partition_steps(const SymbolicLU& s, const std::vector<int>& pa
         const int last = s.step_w[k] == 2 ? k + 1 : k;
         const int c = s.colp[last + 1] - s.colp[last];
         cost[q] = 3.0 + c + 0.25 * c * c;  // a step's walk time, roughly
-        if (parent[last] >= 0) sparent[q] = step_of[parent[last]];
+        if (parent[last] >= 0) sparent[q] = step_of[parent[last]];  // -1 when the parent is not given
     }
     for (int q = 0; q < ns; ++q) sub[q] += cost[q];  // children precede parents
     for (int q = 0; q < ns; ++q)
@@ -696,7 +769,8 @@ std::vector<int> partition_steps(const SymbolicLU& s, const std::vector<int>& pa
     }
     for (int q = ns - 1; q >= 0; --q)  // parents precede children in reverse order
         if (sg[q] < 0) sg[q] = sg[sparent[q]];
-    for (int k = 0; k < d; ++k) group[k] = sg[step_of[k]];
+    for (int k = 0; k < d; ++k)
+        if (step_of[k] >= 0) group[k] = sg[step_of[k]];
     return group;
 }
 
@@ -786,6 +860,261 @@ void build_backward(SymbolicLU& s, const std::vector<int>& group, int W)
         }
         s.prog_b.resize(static_cast<size_t>(round_up_to(static_cast<int>(s.prog_b.size()), kProgChunk)), enc(-1));
         s.bseg[w].prog_b_len = static_cast<int>(s.prog_b.size()) - s.bseg[w].prog_b_off;
+    }
+}
+
+// The split walk's programs (see SymbolicLU): fine sets (level 0), middle
+// parts (level 1), the top part (level 2), each with its own slot numbering
+// (all sharing the zero and trash slots slots_fs / slots_fs + 1), laid out
+// at the A / U positions the main programs gave the same steps.  A program
+// dumps the update targets it leaves pending; a higher program gathers each
+// pending value right before the step that consumes it, so its slots stay
+// as few as the main walk's.
+int split_level(const SplitLayout& L, int j) { return j < L.n_fine ? 0 : j < L.n_fine + L.n_mid ? 1 : 2; }
+
+void build_split(SymbolicLU& s, const SplitLayout& L, const std::vector<int>& step_first,
+                 const std::unordered_map<long long, int>& a_entry)
+{
+    const int d = s.d, n = L.n;
+    s.n_split = n;
+    s.split_fine = L.n_fine;
+    s.split_mid = L.n_mid;
+    // the split layout: the programs' A / U ranges one after the other, each
+    // starting at a multiple of kSegAlign positions
+    s.split_apos.assign(s.a_pos_of_entry.size(), -1);
+    s.split_fpos.assign(static_cast<size_t>(d), -1);
+    std::vector<int> split_uoff(static_cast<size_t>(d) + 1, 0);
+    std::vector<int> a0(static_cast<size_t>(n)), a1(static_cast<size_t>(n)), u0(static_cast<size_t>(n)),
+        u1(static_cast<size_t>(n)), piv(static_cast<size_t>(n));
+    int a_cur = 0, u_cur = 0;
+    // ---- factorization ----
+    std::vector<std::unique_ptr<Emitter>> em;
+    const long long bkey0 = static_cast<long long>(d) * d;  // keys of forward-substitution targets: bkey0 + i
+    std::unordered_map<long long, std::vector<int>> pending;  // key -> pend indices not gathered yet
+    int pend_len = 0;
+    for (int j = 0; j < n; ++j) {
+        em.push_back(std::make_unique<Emitter>(s, a_entry));
+        Emitter& e = *em.back();
+        e.apos_tab = &s.split_apos;
+        e.fpos_tab = &s.split_fpos;
+        e.uoff_tab = &split_uoff;
+        e.count = false;
+        e.apos = a_cur;
+        e.upos = u_cur;
+        a0[j] = a_cur;
+        u0[j] = u_cur;
+        for (int k : step_first) {
+            if (L.fine[k] != j) continue;
+            // the pending values this step consumes as sources, gathered first
+            std::vector<int> gat;
+            auto gather = [&](long long kk, bool bk, int i) {
+                auto it = pending.find(kk);
+                if (it == pending.end()) return;
+                int dst;
+                if (bk) {
+                    if (e.bslot[i] < 0) e.bslot[i] = e.sa.get();
+                    dst = e.bslot[i];
+                } else {
+                    auto jt = e.slot_of.find(kk);
+                    if (jt == e.slot_of.end()) jt = e.slot_of.emplace(kk, e.sa.get()).first;
+                    dst = jt->second;
+                }
+                for (int q : it->second) {
+                    gat.push_back(dst);
+                    gat.push_back(q);
+                }
+                pending.erase(it);
+            };
+            const int w = s.step_w[k] == 2 ? 2 : 1;
+            for (int pv = k; pv < k + w; ++pv) {
+                gather(key(pv, pv, d), false, 0);
+                gather(bkey0 + pv, true, pv);
+                for (int p = s.colp[pv]; p < s.colp[pv + 1]; ++p) {
+                    gather(key(s.coli[p], pv, d), false, 0);
+                    gather(key(pv, s.coli[p], d), false, 0);
+                }
+            }
+            for (size_t q = 0; q < gat.size(); q += 2 * kSplitRecItems) {
+                Rec r;
+                r.kind = Rec::kGather;
+                r.r.assign(gat.begin() + static_cast<long>(q),
+                           gat.begin() + static_cast<long>(std::min(gat.size(), q + 2 * kSplitRecItems)));
+                e.recs.push_back(std::move(r));
+            }
+            e.step(k);
+        }
+        a1[j] = e.apos;
+        u1[j] = e.upos;
+        piv[j] = e.n_piv;
+        a_cur = round_up_to(e.apos, kSegAlign);
+        u_cur = round_up_to(e.upos, kSegAlign);
+        if (j == n - 1) break;
+        // what this program leaves pending goes to the pend buffer
+        std::vector<std::pair<long long, int>> pv;
+        for (const auto& kv : e.slot_of) pv.push_back(kv);
+        for (int i = 0; i < d; ++i)
+            if (e.bslot[i] >= 0) pv.push_back({bkey0 + i, e.bslot[i]});
+        std::sort(pv.begin(), pv.end());
+        for (size_t q = 0; q < pv.size(); q += kSplitRecItems) {
+            Rec r;
+            r.kind = Rec::kDump;
+            r.k = pend_len + static_cast<int>(q);
+            for (size_t i = q; i < std::min(pv.size(), q + kSplitRecItems); ++i) r.r.push_back(pv[i].second);
+            e.recs.push_back(std::move(r));
+        }
+        for (size_t q = 0; q < pv.size(); ++q) pending[pv[q].first].push_back(pend_len + static_cast<int>(q));
+        pend_len += static_cast<int>(pv.size());
+        e.slot_of.clear();
+    }
+    s.pend_len = pend_len;
+    s.split_a_len = a1[n - 1];
+    s.split_u_len = u1[n - 1];
+    int covered = 0;
+    for (int j = 0; j < n; ++j) covered += piv[j];
+    if (covered != d) throw std::logic_error("split: the programs do not cover every pivot");
+    for (int q : s.split_fpos)
+        if (q < 0) throw std::logic_error("split: an equation without a right-hand side position");
+    Emitter& et = *em.back();
+    if (!pending.empty()) throw std::logic_error("split: pending values never gathered");
+    if (!et.slot_of.empty()) throw std::logic_error("split: update targets left unconsumed");
+    for (int i = 0; i < d; ++i)
+        if (et.bslot[i] >= 0) throw std::logic_error("split: forward-substitution targets left unconsumed");
+    int S = 0;
+    for (const auto& e : em) S = std::max(S, e->sa.hi);
+    s.slots_fs = S;
+    s.prog_fs.clear();
+    s.split_seg.assign(static_cast<size_t>(n), SymbolicLU::Seg{});
+    for (int j = 0; j < n; ++j) {
+        SymbolicLU::Seg& g = s.split_seg[j];
+        g.prog_f_off = static_cast<int>(s.prog_fs.size());
+        em[j]->encode(s.prog_fs, 0, S);
+        s.prog_fs.resize(static_cast<size_t>(round_up_to(static_cast<int>(s.prog_fs.size()), kProgChunk)), S * kRefUnit);
+        g.prog_f_len = static_cast<int>(s.prog_fs.size()) - g.prog_f_off;
+        g.a_off = a0[j];
+        g.a_len = a1[j] - a0[j];
+        g.u_off = u0[j];
+        g.u_len = u1[j] - u0[j];
+        g.n_piv = piv[j];
+    }
+
+    // ---- backward substitution: the top part, then the middle parts, then the fine sets ----
+    std::vector<int> set_of(static_cast<size_t>(d));
+    for (int k = 0; k < d; ++k) set_of[k] = L.fine[s.step_w[k] == 0 ? k - 1 : k];
+    struct BRec {
+        int kind, k, w, c;  // kind 0: step, 1: preload (r = (dst, x index) pairs)
+        std::vector<int> r;
+    };
+    std::vector<std::vector<BRec>> brecs(static_cast<size_t>(n));
+    std::vector<int> bslots(static_cast<size_t>(n), 0);
+    for (int j = 0; j < n; ++j) {
+        // the last pivot of this program reading x(i)
+        std::vector<int> last(static_cast<size_t>(d), d);
+        std::vector<char> need(static_cast<size_t>(d), 0);  // top x a set reads
+        for (int k = 0; k < d; ++k) {
+            if (set_of[k] != j) continue;
+            for (int p = s.colp[k]; p < s.colp[k + 1]; ++p) {
+                const int i = s.coli[p];
+                last[i] = std::min(last[i], k);
+                if (set_of[i] != j) {  // an ancestor's x, solved by a program of a higher level
+                    if (split_level(L, set_of[i]) <= split_level(L, j))
+                        throw std::logic_error("split: a program reads the x of a program of its own level");
+                    need[i] = 1;
+                }
+            }
+        }
+        SlotAlloc sa;
+        std::vector<int> xslot(static_cast<size_t>(d), -1);
+        auto& rv = brecs[static_cast<size_t>(j)];
+        std::vector<int> pre;
+        for (int i = 0; i < d; ++i)
+            if (need[i]) {
+                xslot[i] = sa.get();
+                pre.push_back(xslot[i]);
+                pre.push_back(s.perm[i]);
+            }
+        for (size_t q = 0; q < pre.size(); q += 2 * kSplitRecItems) {
+            BRec r{1, 0, 0, 0, {}};
+            r.r.assign(pre.begin() + static_cast<long>(q), pre.begin() + static_cast<long>(std::min(pre.size(), q + 2 * kSplitRecItems)));
+            rv.push_back(std::move(r));
+        }
+        int npiv = 0;
+        for (int kk = d - 1; kk >= 0; --kk) {
+            const bool pair = s.step_w[kk] == 0;
+            const int k = pair ? kk - 1 : kk;
+            kk = k;
+            if (set_of[k] != j) continue;
+            const int last_p = pair ? k + 1 : k;
+            const int c0 = s.colp[last_p], c = s.colp[last_p + 1] - c0;
+            BRec rec{0, k, pair ? 2 : 1, c, {}};
+            std::vector<int> to_free;
+            for (int p = c0; p < c0 + c; ++p) {
+                const int i = s.coli[p];
+                if (xslot[i] < 0) throw std::logic_error("split backward: x(i) read before it is computed");
+                rec.r.push_back(xslot[i]);
+                if (last[i] == k) to_free.push_back(xslot[i]);
+            }
+            for (int sl : to_free) sa.put(sl);
+            auto out_slot = [&](int i) {
+                int out = -1;
+                if (last[i] < k) {
+                    out = sa.get();
+                    xslot[i] = out;
+                }
+                rec.r.push_back(out);
+            };
+            if (pair) out_slot(k + 1);
+            out_slot(k);
+            rv.push_back(std::move(rec));
+            npiv += pair ? 2 : 1;
+        }
+        if (npiv != piv[j]) throw std::logic_error("split backward: pivot count");
+        bslots[j] = sa.hi;
+    }
+    const int SB = *std::max_element(bslots.begin(), bslots.end());
+    s.slots_bs = SB;
+    const auto enc = [&](int slot) { return (slot < 0 ? SB + 1 : slot) * kRefUnit; };
+    auto head = [](const BRec& r) { return r.kind == 1 ? kPreloadHead : rec_head(r.c, r.w); };
+    s.prog_bs.clear();
+    s.split_bseg.assign(static_cast<size_t>(n), SymbolicLU::Seg{});
+    for (int j = 0; j < n; ++j) {
+        SymbolicLU::Seg& g = s.split_bseg[j];
+        g.prog_b_off = static_cast<int>(s.prog_bs.size());
+        const auto& rv = brecs[static_cast<size_t>(j)];
+        for (size_t q = 0; q < rv.size(); ++q) {
+            const BRec& rec = rv[q];
+            const int next = q + 1 < rv.size() ? head(rv[q + 1]) : 0;
+            const size_t at = s.prog_bs.size();
+            if (rec.kind == 1) {
+                const int m = static_cast<int>(rec.r.size() / 2);
+                s.prog_bs.resize(at + static_cast<size_t>(pair_rec_words(m)), enc(-1));
+                int* wd = s.prog_bs.data() + at;
+                wd[0] = kPreloadHead | (next << 16);
+                wd[1] = m;
+                for (int i = 0; i < m; ++i) {
+                    wd[2 + 2 * i] = enc(rec.r[2 * i]);
+                    wd[3 + 2 * i] = rec.r[2 * i + 1];
+                }
+                continue;
+            }
+            const int c = rec.c;
+            const int nw = rec.w == 2 ? brec2_words(c) : brec_words(c);
+            s.prog_bs.resize(at + static_cast<size_t>(nw), enc(-1));
+            int* wd = s.prog_bs.data() + at;
+            wd[0] = rec_head(c, rec.w) | (next << 16);
+            if (rec.w == 2) {
+                wd[1] = s.perm[rec.k + 1];
+                wd[2] = s.perm[rec.k];
+                for (size_t i = 0; i < rec.r.size(); ++i) wd[3 + i] = enc(rec.r[i]);
+            } else {
+                wd[1] = s.perm[rec.k];
+                for (size_t i = 0; i < rec.r.size(); ++i) wd[2 + i] = enc(rec.r[i]);
+            }
+        }
+        s.prog_bs.resize(static_cast<size_t>(round_up_to(static_cast<int>(s.prog_bs.size()), kProgChunk)), enc(-1));
+        g.prog_b_len = static_cast<int>(s.prog_bs.size()) - g.prog_b_off;
+        g.u_off = u0[j];
+        g.u_len = u1[j] - u0[j];
+        g.n_piv = piv[j];
     }
 }
 
@@ -1119,26 +1448,34 @@ NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int 
             yadjp[i + 1] = static_cast<int>(yadji.size());
         }
         std::vector<int> ro = rcm_order(n, yadjp, yadji);
-        P.k1_bus.clear();
-        P.k1_ent.clear();
-        for (int i : ro) {
-            if (P.row_p[i] < 0) continue;  // fixed bus: no equations
-            const int e0 = static_cast<int>(P.k1_ent.size() / 6);
-            int diag = -1;
-            for (int p = Yp[i]; p < Yp[i + 1]; ++p) {
-                const int j = Yi[p];
-                if (j == i) diag = p;
-                int a[4];
-                for (int b = 0; b < 4; ++b) a[b] = P.map[b][p] >= 0 ? P.lu.a_pos_of_entry[P.map[b][p]] : -1;
-                P.k1_ent.insert(P.k1_ent.end(), {j, p, a[0], a[1], a[2], a[3]});
+        // the tables for one stream layout (A-stream positions of the J
+        // entries and of the right-hand sides)
+        auto k1_tables = [&](const std::vector<int>& apos, const std::vector<int>& fpos, std::vector<int>& kb,
+                             std::vector<int>& ke) {
+            kb.clear();
+            ke.clear();
+            for (int i : ro) {
+                if (P.row_p[i] < 0) continue;  // fixed bus: no equations
+                const int e0 = static_cast<int>(ke.size() / 6);
+                int diag = -1;
+                for (int p = Yp[i]; p < Yp[i + 1]; ++p) {
+                    const int j = Yi[p];
+                    if (j == i) diag = p;
+                    int a[4];
+                    for (int b = 0; b < 4; ++b) a[b] = P.map[b][p] >= 0 ? apos[P.map[b][p]] : -1;
+                    ke.insert(ke.end(), {j, p, a[0], a[1], a[2], a[3]});
+                }
+                if (diag < 0) throw std::invalid_argument("Ybus has no diagonal entry for a bus with equations");
+                const int e1 = static_cast<int>(ke.size() / 6);
+                const int fp = fpos[P.row_p[i]];
+                const int fq = P.row_q[i] >= 0 ? fpos[P.row_q[i]] : -1;
+                kb.insert(kb.end(), {i, e0, e1, fp, fq, diag, xb_of[i], 0});
             }
-            if (diag < 0) throw std::invalid_argument("Ybus has no diagonal entry for a bus with equations");
-            const int e1 = static_cast<int>(P.k1_ent.size() / 6);
-            const int fp = P.lu.f_pos[P.row_p[i]];
-            const int fq = P.row_q[i] >= 0 ? P.lu.f_pos[P.row_q[i]] : -1;
-            P.k1_bus.insert(P.k1_bus.end(), {i, e0, e1, fp, fq, diag, xb_of[i], 0});
-        }
+        };
+        k1_tables(P.lu.a_pos_of_entry, P.lu.f_pos, P.k1_bus, P.k1_ent);
         P.k1_nbus = static_cast<int>(P.k1_bus.size() / kK1BusWords);
+        // the split walk's layout (plans without extension rows)
+        if (P.lu.n_split > 0 && X.empty()) k1_tables(P.lu.split_apos, P.lu.split_fpos, P.k1_bus_s, P.k1_ent_s);
     }
     return P;
 }

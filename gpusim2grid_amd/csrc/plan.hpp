@@ -77,6 +77,27 @@ struct SymbolicLU {
     //   op, and for op != 0 two further ints (see below)
     std::vector<int> prog_b;
     int slots_b = 0;
+    // Split walk (tail launches): every walker set cut into fine sets of
+    // whole subtrees and a middle part above them; each fine set (level 0),
+    // middle part (level 1) and the top part (level 2) is a program walked
+    // by a workgroup of its own, a level per launch.  Factorization: a
+    // program ends with dump records writing the update targets it leaves
+    // pending to a per-tile buffer (pend_len positions); a higher program
+    // gathers each pending value right before the step that consumes it.
+    // Backward: the top part first, then the middles, then the fine sets;
+    // a program starts with preload records reading the higher programs' x
+    // it needs.  split_seg / split_bseg hold one segment per program (fine
+    // sets, middles, top), in a stream layout of the split walk's own.
+    int n_split = 0;
+    int split_fine = 0, split_mid = 0;  // programs of levels 0 and 1 (the top is the last)
+    std::vector<Seg> split_seg, split_bseg;
+    std::vector<int> prog_fs, prog_bs;
+    int slots_fs = 0, slots_bs = 0;
+    int pend_len = 0;
+    // the split walk's own stream layout (its programs one after the other)
+    int split_a_len = 0, split_u_len = 0;
+    std::vector<int> split_apos;  // input CSR position -> A-stream position
+    std::vector<int> split_fpos;  // equation -> A-stream position of its right-hand side
 };
 
 // Builds the symbolic factorization of a square CSR pattern (structurally
@@ -164,6 +185,9 @@ struct NrPlan {
     // per Ybus entry (in k1 order): {col j, ybus k, a11, a12, a21, a22}
     std::vector<int> k1_ent;             // 6 ints per entry
     int k1_nbus = 0;
+    // the same tables in the split walk's stream layout (empty: no split walk)
+    std::vector<int> k1_bus_s, k1_ent_s;
+    bool split_ok() const { return !k1_bus_s.empty(); }
 };
 
 // walkers <= 0: default_walkers(d)

@@ -75,6 +75,16 @@ constexpr int kSegAlign = 32;
 constexpr int kMergeHead = 0x7FFE;
 constexpr int kMergePairs = 31;
 __host__ __device__ constexpr int mrec_words(int n) { return round4(2 + 2 * n); }
+// split walk records (their n is at most kSplitRecItems):
+//   dump    [kDumpHead | next << 16, n, first, ref x n]: pend[first + i] = slot ref_i
+//   gather  [kGatherHead | next << 16, n, (dst ref, pend index) x n]: dst += pend[index]
+//   preload [kPreloadHead | next << 16, n, (dst ref, x index) x n] (backward): dst = x[index]
+constexpr int kDumpHead = 0x7FFD;
+constexpr int kGatherHead = 0x7FFC;
+constexpr int kPreloadHead = 0x7FFB;
+constexpr int kSplitRecItems = 30;
+__host__ __device__ constexpr int dump_words(int n) { return round4(3 + n); }
+__host__ __device__ constexpr int pair_rec_words(int n) { return round4(2 + 2 * n); }
 // walker kernels' LDS layout: per walker a 256-byte control area (value and
 // program ring), then kWalkFlagBytes of cross-walker flags, the W value
 // rings, the W program rings, and the slots
@@ -99,6 +109,21 @@ __host__ __device__ constexpr int walk_ring_bytes(int real_size, int W)
     return W * (kWalkerCtlBytes + value_chunk(real_size, W) * value_ring_slots(real_size, W) * kLanes * real_size +
                 kProgChunk * kProgSlots * 4) +
            kWalkFlagBytes;
+}
+
+// split walk: sets per tile (from kSplitMinD unknowns up), the walker
+// workgroup's value producers, and its ring geometry (as for RW walkers): the
+// factorization's full chunks, whose window spans a two-column step's 30
+// values; the backward substitution's half chunks (a record spans at most 17)
+constexpr int kSplitSets = 32;
+constexpr int kSplitMinD = 2000;
+constexpr int kSplitProducers = 2;
+constexpr int kSplitRingF = 2;
+constexpr int kSplitRingB = 4;
+__host__ __device__ constexpr int split_ring_bytes(int real_size, int RW)
+{
+    return kWalkerCtlBytes + value_chunk(real_size, RW) * value_ring_slots(real_size, RW) * kLanes * real_size +
+           kProgChunk * kProgSlots * 4 + kWalkFlagBytes;
 }
 
 // mismatch program: ints per bus record

@@ -56,6 +56,12 @@ extern "C" {
 #define G2G_ARR_UOFF 11     /* U-stream record start per pivot [d+1]          */
 #define G2G_ARR_SEGS 12     /* per walker: prog_f off/len, prog_b off/len, A off/len, U off/len, pivots [9 walkers] */
 #define G2G_ARR_BSEGS 13    /* the same per backward-substitution walker */
+#define G2G_ARR_PROG_FS 14  /* split walk: factorization programs of the sets and the top part */
+#define G2G_ARR_PROG_BS 15  /* split walk: backward programs */
+#define G2G_ARR_SPLIT_SEGS 16  /* split walk: factorization segments (9 ints per set, then the top) */
+#define G2G_ARR_SPLIT_BSEGS 17 /* split walk: backward segments */
+#define G2G_ARR_SPLIT_APOS 18  /* split walk: A-stream position of each input CSR entry */
+#define G2G_ARR_SPLIT_FPOS 19  /* split walk: A-stream position of each rhs entry */
 
 typedef struct g2g_plan_s *g2g_plan_t;
 typedef struct g2g_batch_s *g2g_batch_t;
@@ -73,6 +79,9 @@ typedef struct {
     int32_t n_ext;                           /* extension rows (= columns) after [P(pvpq) | Q(pq)] */
     int32_t n_xs;                            /* extension state values per row (sa, then each controller's Q) */
     int32_t sa_col;                          /* J column of sa (= the slack's P row), -1 without distributed slack */
+    int32_t n_split;                         /* split walk: subtree sets per tile (0: none) */
+    int32_t slots_fs, slots_bs, pend_len;    /* split walk: slots and pending contributions per tile */
+    int32_t split_a_len, split_u_len;        /* split walk: its A- and U-stream lengths */
 } g2g_plan_info;
 
 typedef struct {

@@ -52,6 +52,9 @@ def _slot(ref):
 
 
 MERGE_HEAD = 0x7FFE
+DUMP_HEAD = 0x7FFD
+GATHER_HEAD = 0x7FFC
+PRELOAD_HEAD = 0x7FFB
 
 
 def mrec_words(n):
@@ -62,7 +65,7 @@ def _segs(plan):
     return plan.array("segs").reshape(-1, 9)
 
 
-def factor_forward(plan, A, pivot_eps=0.0, scale=None):
+def factor_forward(plan, A, pivot_eps=0.0, scale=None, split=False):
     """A: [rows, a_len] dense stream -> (U stream [rows, u_len], clamped [rows]).
 
     Slot semantics of the device kernel: a source slot is read and then
@@ -73,12 +76,14 @@ def factor_forward(plan, A, pivot_eps=0.0, scale=None):
     walkers are independent until walker 0's first merge record), then
     walker 0 with its merges (dst += src, src = 0) and the top part."""
     info = plan.info()
-    prog = np.asarray(plan.array("prog_f"), dtype=np.int64)
-    segs = _segs(plan)
+    prog = np.asarray(plan.array("prog_fs" if split else "prog_f"), dtype=np.int64)
+    segs = plan.array("split_segs").reshape(-1, 9) if split else _segs(plan)
     rows = A.shape[0]
-    S = info["slots_f"]
+    S = info["slots_fs" if split else "slots_f"]
+    u_len = info["split_u_len" if split else "u_len"]
     sl = np.zeros((S + 2, rows))
-    U = np.zeros((rows, info["u_len"]))
+    pend = np.full((info["pend_len"], rows), np.nan)
+    U = np.zeros((rows, u_len))
     eps = pivot_eps * (np.ones(rows) if scale is None else scale)
     clamped = np.zeros(rows, np.int64)
 
@@ -98,6 +103,13 @@ def factor_forward(plan, A, pivot_eps=0.0, scale=None):
                     dst, src = _slot(int(prog[p + 2 + 2 * i])), _slot(int(prog[p + 3 + 2 * i]))
                     sl[dst] += sl[src]
                     sl[src] = 0.0
+                p += mrec_words(n)
+                continue
+            if h == GATHER_HEAD:  # split walk: dst += pend[index]
+                n = int(prog[p + 1])
+                for i in range(n):
+                    dst, src = _slot(int(prog[p + 2 + 2 * i])), int(prog[p + 3 + 2 * i])
+                    sl[dst] += pend[src]
                 p += mrec_words(n)
                 continue
             c, wdt = h & 0x7FFF, 1 + (h >> 15)
@@ -172,32 +184,55 @@ def factor_forward(plan, A, pivot_eps=0.0, scale=None):
                 t = _slot(int(prog[base + a]))
                 for lv, _, f in upd:
                     sl[t] += -lv[a] * f
+        # split walk: the set's pending contributions go to pend (dump records)
+        while p < sg[0] + sg[1] and int(prog[p] & 0xFFFF) == DUMP_HEAD:
+            n, first = int(prog[p + 1]), int(prog[p + 2])
+            for i in range(n):
+                t = _slot(int(prog[p + 3 + i]))
+                pend[first + i] = sl[t]
+                sl[t] = 0.0
+            p += _r4(3 + n)
         assert k == sg[8] and p <= sg[0] + sg[1]
         assert ap == sg[4] + sg[5] and up == sg[6] + sg[7]
 
+    if split:  # every program on fresh slots: fine sets, middle parts, the top part
+        for w in range(len(segs)):
+            sl[:] = 0.0
+            run(segs[w])
+            assert np.all(sl[:S] == 0.0)
+        assert not np.isnan(pend).any()
+        return U, clamped
     for w in range(len(segs) - 1, -1, -1):
         run(segs[w])
     assert np.all(sl[:S] == 0.0)  # every slot consumed
     return U, clamped
 
 
-def backward(plan, U):
+def backward(plan, U, split=False):
     """U stream -> x [rows, d] in input labels (U(k,k) is stored as its
     reciprocal; width-2 records solve x(k+1) then x(k)).  Walker 0 runs
     first (its segment starts with the top part), then the others."""
     info = plan.info()
-    prog = np.asarray(plan.array("prog_b"), dtype=np.int64)
-    segs = plan.array("bsegs").reshape(-1, 9)
+    prog = np.asarray(plan.array("prog_bs" if split else "prog_b"), dtype=np.int64)
+    segs = plan.array("split_bsegs" if split else "bsegs").reshape(-1, 9)
     rows = U.shape[0]
     d = info["d"]
-    S = info["slots_b"]
+    S = info["slots_bs" if split else "slots_b"]
     sl = np.full((S + 2, rows), np.nan)
-    x_out = np.zeros((rows, d))
+    x_out = np.full((rows, d), np.nan) if split else np.zeros((rows, d))
 
     def run(sg):
         p, up, q = int(sg[2]), int(sg[6] + sg[7]), 0
         while q < sg[8]:
             h = int(prog[p] & 0xFFFF)
+            if h == PRELOAD_HEAD:  # split walk: a top x the set reads
+                n = int(prog[p + 1])
+                for i in range(n):
+                    v = x_out[:, int(prog[p + 3 + 2 * i])]
+                    assert not np.isnan(v).any()
+                    sl[_slot(int(prog[p + 2 + 2 * i]))] = v
+                p += mrec_words(n)
+                continue
             c, wdt = h & 0x7FFF, 1 + (h >> 15)
             if wdt == 1:
                 perm_k = int(prog[p + 1])
@@ -232,6 +267,11 @@ def backward(plan, U):
                 q += 2
         assert q == sg[8] and p <= sg[2] + sg[3] and up == sg[6]
 
+    if split:  # the top part first (its x in x_out), then the middles, then the fine sets
+        for w in range(len(segs) - 1, -1, -1):
+            sl[:] = np.nan
+            run(segs[w])
+        return x_out
     for w in range(len(segs)):
         run(segs[w])
     return x_out

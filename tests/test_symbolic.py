@@ -271,3 +271,37 @@ def test_u_stride_covers_every_whole_chunk_read(monkeypatch, wf, wb):
                 kch = chunk // (2 if W >= 4 else 1)
                 for s in P.array(name).reshape(-1, 9):
                     assert s[6] + -(-s[7] // kch) * kch <= stride
+
+
+def test_split_walk_programs_solve_like_the_main_walk():
+    """The split walk (fine sets, middle parts, top part, each on slots of its
+    own, pending targets through the dump / gather records, top x through the
+    preload records) solves the 2,000-bus Jacobian like scipy, in its own
+    stream layout; its slots fit a workgroup's LDS in FP32 and FP64."""
+    g = syn.grid_c1()
+    P = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+    inf = P.info()
+    assert inf["n_split"] > 8 and inf["pend_len"] > 0
+    F, J, _ = orc.f_and_j(g.Ybus, _perturbed(g, 3), g.Sbus, g.pv, g.pq)
+    inf2 = dict(inf, a_len=inf["split_a_len"])
+    A = pi.astream_from_csr(inf2, P.array("split_apos"), P.array("split_fpos"), J.data[None], F[None])
+    U, cl = pi.factor_forward(P, A, split=True)
+    assert cl[0] == 0
+    x = pi.backward(P, U, split=True)[0]
+    xs = spla.spsolve(J.tocsc(), F)
+    assert np.abs(x - xs).max() < 1e-9 * max(1.0, np.abs(xs).max())
+    # every equation and entry has a position in the split layout, all distinct
+    sa, sf = P.array("split_apos"), P.array("split_fpos")
+    pos = np.concatenate([sa[sa >= 0], sf])
+    assert (sf >= 0).all() and np.unique(pos).size == pos.size and pos.max() < inf["split_a_len"]
+    for rs in (4, 8):
+        chunk = 32 if rs == 4 else 16  # full chunks (factorization), half chunks (backward)
+        ring_f = 256 + chunk * 4 * 64 * rs + 512 * 4 * 4 + 1280
+        ring_b = 256 + chunk // 2 * 4 * 64 * rs + 512 * 4 * 4 + 1280
+        assert ring_f + (inf["slots_fs"] + 2) * 64 * rs <= LDS_BYTES
+        assert ring_b + (inf["slots_bs"] + 2) * 64 * rs <= LDS_BYTES
+
+
+def test_small_plans_have_no_split_walk():
+    g = syn.grid_118()
+    assert _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack).info()["n_split"] == 0
