@@ -113,6 +113,7 @@ def lib():
         "g2g_batch_stream_warm": (ctypes.c_int, [_P, _P, i32, _P, _P, i32]),
         "g2g_batch_device_arrays": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
         "g2g_batch_warm_start": (ctypes.c_int, [_P, _P, _P, i32]),
+        "g2g_batch_set_warm_contraction": (ctypes.c_int, [_P, dbl]),
         "g2g_batch_branch_flows": (ctypes.c_int, [_P, i32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
         "g2g_batch_limit_violations": (ctypes.c_int, [_P, ctypes.POINTER(LimitTable), _P, _P, ctypes.c_double,
                                                       ctypes.c_double, i32, ctypes.POINTER(Violations)]),
@@ -128,6 +129,7 @@ def lib():
         "g2g_mlp_forward": (ctypes.c_int, [_P, _P, i32, _P, i32]),
         "g2g_mlp_loss_grad": (ctypes.c_int, [_P, _P, _P, i32, _P, _P, i32]),
         "g2g_mlp_train_step": (ctypes.c_int, [_P, _P, _P, i32, ctypes.c_float, _P, i32]),
+        "g2g_mlp_set_grad_clip": (ctypes.c_int, [_P, ctypes.c_float]),
         "g2g_gemm_bf16": (ctypes.c_int, [i32, i32, i32, i32, i32, _P, i64, _P, i64, _P, i64]),
     }
     for name, (res, args) in sig.items():
@@ -355,6 +357,9 @@ class Batch:
         _check(lib().g2g_batch_solve(self._h, int(max_iter), float(tol), int(scheduling), ctypes.byref(st)))
         return st.as_dict()
 
+    def set_warm_contraction(self, q):
+        _check(lib().g2g_batch_set_warm_contraction(self._h, float(q)))
+
     def solve_warm(self, mlp, k_warm=3, growth=2.0, max_iter=10, tol=1e-8, mask=None):
         """Warm-started solve with flat-start fallback (g2g_batch_solve_warm);
         the batch's V (set_v) is the fallback start."""
@@ -543,6 +548,9 @@ class Mlp:
         loss = np.empty(1, np.float32)
         _check(lib().g2g_mlp_train_step(self._h, _ptr(X), _ptr(T), X.shape[0], float(lr), _ptr(loss), 0))
         return float(loss[0])
+
+    def set_grad_clip(self, max_norm):
+        _check(lib().g2g_mlp_set_grad_clip(self._h, float(max_norm)))
 
     def train_step_device(self, x_ptr, t_ptr, M, lr):
         loss = np.empty(1, np.float32)

@@ -318,6 +318,7 @@ struct g2g_batch_s {
     // still on their learned start, the restart count, and compaction copies
     bool restart_on = false;
     int restart_max_iter = 0;
+    double contraction = 0.0;  // the quadratic-contraction rejection of learned starts (0: off)
     int* budget = nullptr;
     unsigned char* warm = nullptr;
     int* requeued = nullptr;
@@ -718,6 +719,7 @@ void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double
     MismatchArgs<Real> a{k1b, k1e, P.k1_nbus, b->plan->d_Y, b->Yrow, b->V, b->S,
                          static_cast<Real*>(b->A), P.n, P.nY, b->a_stride, b->res, b->jmax, b->status, b->iters,
                          b->tile_active, b->n_active, b->active_rows, iter, max_iter, tol, freeze, growth,
+                         growth > 0.0 ? b->contraction : 0.0,
                          (b->streaming || b->restart_on) ? b->it0 : nullptr, b->mask,
                          b->restart_on ? b->budget : nullptr, b->restart_on ? b->warm : nullptr,
                          b->mis_splits, (b->mis_splits > 1 || ext) ? b->mis_part : nullptr, b->rows_pad,
@@ -999,10 +1001,10 @@ int nr_loop_body(g2g_batch_s* b, int max_iter, double tol, int freeze, double gr
             compact_rows(b);
             compacted = true;
             // the A stream was written in the old row order: evaluate again
-            // (same V, same decisions) in the compacted one
+            // (same V, same decisions; no rejection test) in the compacted one
             k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
             e0 = tm.mark();
-            launch_mismatch<Real>(b, true, it, max_iter, tol, freeze, growth);
+            launch_mismatch<Real>(b, true, it, max_iter, tol, freeze, 0.0);
             tm.ph[0].push_back({e0, tm.mark()});
         }
         if (!b->split_layout && use_split(b, *b->h_active)) {
@@ -1013,7 +1015,7 @@ int nr_loop_body(g2g_batch_s* b, int max_iter, double tol, int freeze, double gr
             b->split_layout = true;
             k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
             e0 = tm.mark();
-            launch_mismatch<Real>(b, true, it, max_iter, tol, freeze, growth);
+            launch_mismatch<Real>(b, true, it, max_iter, tol, freeze, 0.0);
             tm.ph[0].push_back({e0, tm.mark()});
         }
         e0 = tm.mark();
@@ -1871,6 +1873,15 @@ int g2g_batch_solve_warm(g2g_batch_t b, g2g_mlp_t mlp, const uint8_t* mask, int3
         if (b->precision == G2G_MIXED)
             solve_warm_impl<float>(b, mlp, mask, n_mask, k_warm, growth, max_iter, tol, stats);
         else solve_warm_impl<double>(b, mlp, mask, n_mask, k_warm, growth, max_iter, tol, stats);
+    });
+}
+
+int g2g_batch_set_warm_contraction(g2g_batch_t b, double q)
+{
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        if (!(q >= 0.0)) throw std::invalid_argument("the contraction bound must be >= 0 (0: off)");
+        b->contraction = q;
     });
 }
 

@@ -29,6 +29,8 @@ struct MismatchArgs {
     double tol;
     int freeze;             // 1: a converged row stops (continuous); 0: fixed count (chunked)
     double growth;          // > 0: a row whose ||F||inf exceeds growth x its previous one is rejected
+    double contraction;     // > 0: a row on its learned start is rejected from its second update on
+                            // when ||F||inf exceeds contraction x its previous value squared
     const int* it0;         // nullable: per row, the iteration its solve started at (streamed rows)
     const unsigned char* mask;  // nullable: [T][n][64], 1 = bus islanded in this row (its equations
                                 // become identity rows: F = 0, J row = unit diagonal)

@@ -161,6 +161,10 @@ __device__ __forceinline__ void mis_status(const MismatchArgs<Real>& a, size_t t
         if (!isfinite(r)) st = wrow ? kRejected : kDiverged;
         else if (a.freeze && r < a.tol) st = kConverged;
         else if (growth_on && it > 0 && r > a.growth * prev) st = kRejected;
+        // not in Newton's quadratic regime by now: on its learned start, the
+        // row would take too many updates
+        else if (wrow && a.contraction > 0.0 && it >= 2 && r >= a.tol && r > a.contraction * prev * prev)
+            st = kRejected;
         else if (it >= budget) st = r < a.tol ? kConverged : (wrow ? kRejected : kMaxIter);
         if (st != kActive) {
             a.status[row] = st;

@@ -314,14 +314,36 @@ __global__ __launch_bounds__(256) void k_mse_grad(const float* __restrict__ Y, c
     if (threadIdx.x == 0) atomicAdd(loss, (part[0] + part[1] + part[2] + part[3]) * scale);
 }
 
-// Adam on fp32 master weights, refreshing the bf16 copy (nullable)
+// sum of squares of x[0..n) added into *out (the gradient's norm for clipping)
+__global__ __launch_bounds__(256) void k_sumsq(const float* __restrict__ x, long long n, float* __restrict__ out)
+{
+    __shared__ float s[256];
+    float a = 0.0f;
+    for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<long long>(gridDim.x) * blockDim.x)
+        a = fmaf(x[i], x[i], a);
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, s[0]);
+}
+
+// Adam on fp32 master weights, refreshing the bf16 copy (nullable).  With
+// gnorm2 (nullable) the gradient is scaled by min(1, clip / ||g||) first.
 __global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
                        const float* __restrict__ g, long long n, float lr, float b1, float b2, float eps,
-                       float c1, float c2, __bf16* __restrict__ wb)
+                       float c1, float c2, __bf16* __restrict__ wb, const float* __restrict__ gnorm2, float clip)
 {
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float gi = g[i];
+    float gi = g[i];
+    if (gnorm2) {
+        const float nrm = sqrtf(*gnorm2);
+        if (nrm > clip) gi *= clip / nrm;
+    }
     const float mi = b1 * m[i] + (1.0f - b1) * gi;
     const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
     m[i] = mi;

@@ -39,7 +39,9 @@ __global__ void k_colsum(const float* X, long long ldx, int M, int N, float* out
 __global__ void k_mse_grad(const float* Y, const float* T, long long n_el, float scale, float* dY, __bf16* dYb,
                            float* loss);
 __global__ void k_adam(float* w, float* m, float* v, const float* g, long long n, float lr, float b1, float b2,
-                       float eps, float c1, float c2, __bf16* wb);
+                       float eps, float c1, float c2, __bf16* wb, const float* gnorm2, float clip);
+__global__ void k_sumsq(const float* x, long long n, float* out);
+__global__ void k_fill_int(int* p, int v, long long n);
 __global__ void k_f32_to_bf16(const float* x, long long n, __bf16* y);
 constexpr int kPatchBuses = 32;  // buses per block of the tiled-layout conversions
 __global__ void k_features_tiles_t(const double2* S_tiles, int M, int n, const float* mu, const float* sigma,

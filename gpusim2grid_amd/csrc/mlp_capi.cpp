@@ -55,6 +55,8 @@ struct g2g_mlp_s {
     __bf16* dZ1b = nullptr;
     __bf16* dZ2b = nullptr;
     float* loss = nullptr;
+    float clip = 0.0f;          // gradient-norm clip of the training steps (0: off)
+    float* gnorm2 = nullptr;    // its squared-norm accumulator
     DeviceScratch ws;   // workspace buffers (freed on growth / destruction)
     DeviceScratch own;  // parameters and state
 
@@ -414,13 +416,28 @@ int g2g_mlp_train_step(g2g_mlp_t p, const float* X, const float* T, int32_t M, f
         const float c1 = 1.0f / (1.0f - std::pow(b1, static_cast<float>(p->step)));
         const float c2 = 1.0f / (1.0f - std::pow(b2, static_cast<float>(p->step)));
         const long long n = static_cast<long long>(p->n_params);
-        k_adam<<<blocks_for(n, 256), 256, 0, p->stream>>>(p->w, p->m, p->v, p->g, n, lr, b1, b2, 1e-8f, c1, c2, p->wb);
+        if (p->clip > 0.0f) {
+            if (!p->gnorm2) p->gnorm2 = p->own.get<float>(1);
+            k_fill_int<<<1, 64, 0, p->stream>>>(reinterpret_cast<int*>(p->gnorm2), 0, 1);  // 0.0f
+            k_sumsq<<<std::min<long long>(1024, blocks_for(n, 256)), 256, 0, p->stream>>>(p->g, n, p->gnorm2);
+        }
+        k_adam<<<blocks_for(n, 256), 256, 0, p->stream>>>(p->w, p->m, p->v, p->g, n, lr, b1, b2, 1e-8f, c1, c2, p->wb,
+                                                           p->clip > 0.0f ? p->gnorm2 : nullptr, p->clip);
         HIPCHK(hipGetLastError());
         p->transpose_weights(p->stream);
         if (loss) {
             HIPCHK(hipMemcpyAsync(loss, p->loss, sizeof(float), hipMemcpyDeviceToHost, p->stream));
             HIPCHK(hipStreamSynchronize(p->stream));
         }
+    });
+}
+
+int g2g_mlp_set_grad_clip(g2g_mlp_t p, float max_norm)
+{
+    return guarded([&] {
+        if (!p) throw std::invalid_argument("null argument");
+        if (!(max_norm >= 0.0f)) throw std::invalid_argument("max_norm must be >= 0 (0: off)");
+        p->clip = max_norm;
     });
 }
 

@@ -104,10 +104,13 @@ class WarmStart:
         return X, T
 
     # -- training ----------------------------------------------------------
-    def train(self, S, V, mask=None, epochs=20, batch=1024, lr=1e-3, lr_final=1e-5, seed=0, log=None):
-        """Adam on (features(S), targets(V)) with a cosine step-size decay from
-        lr to lr_final; S, V: device complex128 tensors of converged training
-        rows.  Returns the per-epoch mean losses."""
+    def train(self, S, V, mask=None, epochs=20, batch=1024, lr=1e-3, lr_final=1e-5, seed=0, log=None,
+              warmup=0.05, clip=1.0):
+        """Adam on (features(S), targets(V)): the step size rises linearly over
+        the first ``warmup`` of the steps, then decays along a cosine from lr to
+        lr_final; every gradient is clipped to norm ``clip`` (0: off).  S, V:
+        device complex128 tensors of converged training rows.  Returns the
+        per-epoch mean losses."""
         import math
         import torch
         if self.in_mu is None:
@@ -117,6 +120,8 @@ class WarmStart:
         g = torch.Generator(device="cpu").manual_seed(seed)
         hist = []
         steps = max(1, epochs * (rows // batch))
+        n_warm = int(warmup * steps)
+        self.mlp.set_grad_clip(clip)
         k = 0
         for ep in range(epochs):
             perm = torch.randperm(rows, generator=g).to(X.device)
@@ -126,7 +131,11 @@ class WarmStart:
             for r0 in range(0, rows - batch + 1, batch):
                 xp = Xs.data_ptr() + r0 * self.n_in * 4
                 tp = Ts.data_ptr() + r0 * self.n_out * 4
-                lr_k = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * k / steps))
+                if k < n_warm:
+                    lr_k = lr * (k + 1) / n_warm
+                else:
+                    t = (k - n_warm) / max(1, steps - n_warm)
+                    lr_k = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * t))
                 tot += self.mlp.train_step_device(xp, tp, batch, lr_k)
                 cnt += 1
                 k += 1

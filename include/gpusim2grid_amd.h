@@ -240,6 +240,12 @@ int g2g_batch_set_step_control(g2g_batch_t batch, int32_t ir_steps, int32_t scal
 int g2g_batch_solve_warm(g2g_batch_t batch, g2g_mlp_t mlp, const uint8_t *mask, int32_t n_mask, int32_t k_warm,
                          double growth, int32_t max_iter, double tol, g2g_solve_stats *stats);
 
+/* Warm solves' early rejection: a row on its learned start whose ||F||inf,
+ * from its second update on, exceeds q x the previous one squared (it is
+ * not converging quadratically) restarts at once, as a failed row does
+ * (0, the default: off; applies when growth > 0). */
+int g2g_batch_set_warm_contraction(g2g_batch_t batch, double q);
+
 /* The MLP's start voltages for the active rows, written as the batch's V
  * (no counterpart in the reference; same mask convention as above). */
 int g2g_batch_warm_start(g2g_batch_t batch, g2g_mlp_t mlp, const uint8_t *mask, int32_t n_mask);
@@ -407,6 +413,9 @@ int g2g_mlp_loss_grad(g2g_mlp_t mlp, const float *X, const float *T, int32_t M, 
 /* one Adam step (beta 0.9 / 0.999, eps 1e-8) on a minibatch */
 int g2g_mlp_train_step(g2g_mlp_t mlp, const float *X, const float *T, int32_t M, float lr, float *loss,
                        int32_t on_device);
+/* Training steps scale the gradient by min(1, max_norm / ||g||2) before the
+ * Adam update (0: off, the default). */
+int g2g_mlp_set_grad_clip(g2g_mlp_t mlp, float max_norm);
 /* C = op(A) op(B) with bf16 operands (raw bit patterns) and fp32 result, on the
  * MLP's MFMA kernels (test entry; host arrays).  op(A) = A, op(B) = B^T with
  * K, lda and ldb multiples of 8 runs the forward pass's K-contiguous kernel. */

@@ -17,7 +17,10 @@ against it with fp32-accumulation tolerances.
 
 Pinning (tests/test_mlp_oracle.py): ``grad`` equals central differences of
 ``loss``; ``gelu`` equals scipy's ``erf`` form and its known values; ``bf16``
-rounding equals the IEEE round-to-nearest-even of the upper 16 bits.
+rounding equals the IEEE round-to-nearest-even of the upper 16 bits;
+``adam_step`` reduces to the sign update lr g / (|g| + eps) whenever the first
+and second moments are those of one repeated gradient (Kingma & Ba 2015,
+Algorithm 1), and its clipping keeps the direction and caps the norm.
 """
 from __future__ import annotations
 
@@ -134,3 +137,20 @@ def warm_v(Y, mu, sigma, kind, vfix):
     vm[:, kind != 1] = np.asarray(vfix)[kind != 1]
     va[:, kind == 2] = 0.0
     return vm * np.exp(1j * va)
+
+
+def adam_step(w, m, v, g, step, lr, clip=0.0, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update (Kingma & Ba 2015, Algorithm 1) of the flat parameter
+    vector w with moments m, v at step number ``step`` (1-based), after
+    scaling g to norm ``clip`` when its norm exceeds it (0: no clipping).
+    Returns (w, m, v), new arrays."""
+    g = np.asarray(g, np.float64)
+    if clip > 0.0:
+        nrm = float(np.sqrt(np.sum(g * g)))
+        if nrm > clip:
+            g = g * (clip / nrm)
+    m = b1 * m + (1.0 - b1) * g
+    v = b2 * v + (1.0 - b2) * g * g
+    mh = m / (1.0 - b1 ** step)
+    vh = v / (1.0 - b2 ** step)
+    return w - lr * mh / (np.sqrt(vh) + eps), m, v

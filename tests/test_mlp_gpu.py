@@ -96,6 +96,49 @@ def test_mlp_training_reduces_loss():
     assert abs(Ldev - Lref) < 0.05 * Lref + 1e-6
 
 
+def _flat(net, d):
+    return net.flatten(d).astype(np.float64)
+
+
+@pytest.mark.parametrize("clip_frac", [0.0, 0.3])
+def test_adam_steps_with_clipping_against_oracle(clip_frac):
+    """Two training steps equal the oracle's Adam applied to the device's own
+    gradients (fp32 update arithmetic); the clip norm sits below both
+    gradients' norms, which differ, so an unclipped update would not match."""
+    _say(f"mlp adam clip_frac={clip_frac}")
+    net, p = _mlp(48, 64, 40, seed=4)
+    rng = np.random.default_rng(9)
+    X1 = rng.normal(size=(96, 48)).astype(np.float32)
+    T1 = rng.normal(size=(96, 40)).astype(np.float32)
+    X2 = rng.normal(size=(96, 48)).astype(np.float32)
+    T2 = (4.0 * rng.normal(size=(96, 40))).astype(np.float32)
+    lr = 1e-3
+    w0 = _flat(net, p)
+    _, g1 = net.loss_grad(X1, T1)
+    g1 = _flat(net, g1)
+    clip = clip_frac * np.linalg.norm(g1)
+    net.set_grad_clip(clip)
+    net.train_step(X1, T1, lr)
+    w1 = _flat(net, net.get_params())
+    z = np.zeros_like(w0)
+    r1, m1, v1 = M.adam_step(w0, z, z, g1, 1, lr, clip=clip)
+    d, dr = w1 - w0, r1 - w0
+    assert np.linalg.norm(d - dr) < 1e-3 * np.linalg.norm(dr)
+    _, g2 = net.loss_grad(X2, T2)
+    g2 = _flat(net, g2)
+    if clip_frac:
+        assert np.linalg.norm(g2) > 1.5 * np.linalg.norm(g1)
+    net.train_step(X2, T2, lr)
+    w2 = _flat(net, net.get_params())
+    r2, _, _ = M.adam_step(w1, m1, v1, g2, 2, lr, clip=clip)
+    d, dr = w2 - w1, r2 - w1
+    assert np.linalg.norm(d - dr) < 1e-3 * np.linalg.norm(dr)
+    if clip_frac:  # the unclipped update is measurably different
+        _, mu, vu = M.adam_step(w0, z, z, g1, 1, lr)
+        ru, _, _ = M.adam_step(w1, mu, vu, g2, 2, lr)
+        assert np.linalg.norm((ru - w1) - dr) > 0.05 * np.linalg.norm(dr)
+
+
 def test_mlp_rejects_bad_widths():
     with pytest.raises(ValueError):
         _capi.Mlp(0, 64, 8)

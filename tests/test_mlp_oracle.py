@@ -82,3 +82,29 @@ def test_features_targets_warm_v_round_trip():
     assert F.shape == (3, 2 * n + 2)
     np.testing.assert_array_equal(F[:, 0:2 * n:2], S.real)
     np.testing.assert_array_equal(F[:, 1:2 * n:2], S.imag)
+
+
+def test_adam_sign_update_and_clipping():
+    rng = np.random.default_rng(4)
+    n, lr = 50, 1e-3
+    w0 = rng.normal(size=n)
+    g = rng.normal(size=n) * 10.0 ** rng.uniform(-3, 3, size=n)
+    z = np.zeros(n)
+    # step 1: the bias-corrected moments are g and g^2
+    w1, m1, v1 = M.adam_step(w0, z, z, g, 1, lr)
+    np.testing.assert_allclose(w0 - w1, lr * g / (np.abs(g) + 1e-8), rtol=1e-12)
+    # the same gradient again: still the sign update
+    w2, _, _ = M.adam_step(w1, m1, v1, g, 2, lr)
+    np.testing.assert_allclose(w1 - w2, lr * g / (np.abs(g) + 1e-8), rtol=1e-9)
+    # clipping keeps the direction and caps the norm; at or above the norm it
+    # is a no-op
+    nrm = np.linalg.norm(g)
+    _, mc, _ = M.adam_step(w0, z, z, g, 1, lr, clip=0.25 * nrm)
+    np.testing.assert_allclose(mc / 0.1, 0.25 * g, rtol=1e-12)
+    _, mn, _ = M.adam_step(w0, z, z, g, 1, lr, clip=nrm * (1 + 1e-12))
+    np.testing.assert_array_equal(mn, m1)
+    # a second, different gradient: b1 weighs the first moment, b2 the second
+    h = rng.normal(size=n)
+    _, m2, v2 = M.adam_step(w1, m1, v1, h, 2, lr)
+    np.testing.assert_allclose(m2, 0.9 * 0.1 * g + 0.1 * h, rtol=1e-12)
+    np.testing.assert_allclose(v2, 0.999 * 0.001 * g * g + 0.001 * h * h, rtol=1e-12)

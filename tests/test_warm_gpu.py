@@ -172,6 +172,54 @@ def test_untrained_mlp_falls_back_to_flat_start():
     assert np.max(np.abs(V[re] - Vf[re])) < 1e-12
 
 
+def test_contraction_rule_restarts_rows_at_their_second_update():
+    """With the quadratic-contraction rejection at its tightest (q tiny), the
+    growth rule and the update budget out of the way, every row that has not
+    converged after two updates on its learned start restarts right then:
+    its count is 2 + the flat-start run's, and it ends where the flat start
+    ends.  Off (q = 0), no row restarts before its budget."""
+    g = syn.grid_118()
+    rows = 300
+    _say("warm contraction")
+    S = _sbus(g, rows, seed=3)
+    ws = WarmStart(g.n_bus, g.pv, g.pq, g.slack, np.abs(g.flat_start()), n_hidden=64, seed=5)
+    n2 = 2 * g.n_bus
+    ws.in_mu = np.zeros(n2, np.float32)
+    ws.in_sigma = np.ones(n2, np.float32)
+    ws.out_mu = np.tile(np.array([0.0, 1.0], np.float32), g.n_bus)
+    ws.out_sigma = np.tile(np.array([0.02, 0.002], np.float32), g.n_bus)  # a fair start
+    ws.mlp.set_normalization(ws.in_mu, ws.in_sigma, ws.out_mu, ws.out_sigma)
+    plan = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+    Bf = _capi.Batch(plan, rows)
+    Bf.set_sbus(S)
+    Bf.set_v(g.flat_start())
+    Bf.solve(max_iter=10, tol=1e-8)
+    stf, itf, _ = Bf.results()
+    Vf = Bf.get_v()
+    assert (stf == 0).all()
+    out = {}
+    for q in (0.0, 1e-300):
+        B = _capi.Batch(plan, rows)
+        B.set_sbus(S)
+        B.set_v(g.flat_start())
+        B.set_warm_contraction(q)
+        stats = B.solve_warm(ws.mlp, k_warm=8, growth=1e300, max_iter=10, tol=1e-8)
+        st, it, _ = B.results()
+        out[q] = (stats, st, it, B.get_v())
+    stats, st, it, V = out[1e-300]
+    assert (st == 0).all()
+    re = it > 2
+    assert re.sum() == stats["n_requeued"] > 0
+    np.testing.assert_array_equal(it[re], 2 + itf[re])
+    assert np.max(np.abs(V[re] - Vf[re])) < 1e-12
+    stats0, st0, it0, _ = out[0.0]
+    assert (st0 == 0).all()
+    # off: the rows that converge on their learned start within the budget
+    # never restart, so fewer rows restart than with the rule at its tightest
+    assert stats0["n_requeued"] < stats["n_requeued"]
+    assert (it0[~re] == it[~re]).all()
+
+
 def test_warm_start_with_outages_gains_over_flat_start():
     """The learned-warm-start workload in small: the 2,000-bus grid, load
     scaling plus 0-2 random non-islanding outages per row, the outage mask
