@@ -19,8 +19,9 @@ precision); every method's figure is in ``methods``.
 ``--workload c3`` runs the learned-warm-start workload instead: the 2,000-bus
 grid, 16,384 rows of the same load scaling plus 0-2 random non-islanding
 branch outages each (per-row Ybus value edits, and a 0/1 outage mask among
-the network's inputs), the warm start trained on solver-generated solutions
-and a flat-start fallback after 4 iterations.
+the network's inputs, resident in HBM like the injections), the warm start
+trained on solver-generated solutions.  A row on its learned start falls back
+to the base case's solution when its mismatch grows or after 6 updates.
 """
 from __future__ import annotations
 
@@ -57,12 +58,13 @@ def parse():
     ap.add_argument("--hidden", type=int, default=1024)
     ap.add_argument("--train-batch", type=int, default=512)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--k-warm", type=int, default=4, help="warm start: updates before the fallback")
-    ap.add_argument("--growth", type=float, default=1.0, help="warm start: fall back when ||F|| grows this much")
-    ap.add_argument("--contraction", type=float, default=100.0,
+    ap.add_argument("--k-warm", type=int, default=6, help="warm start: updates before the fallback")
+    ap.add_argument("--growth", type=float, default=1.0, help="warm start: fall back when ||F|| grows (x this)")
+    ap.add_argument("--contraction", type=float, default=0.0,
                     help="warm start: fall back from the second update on when ||F|| > q ||F_prev||^2 (0: off)")
-    ap.add_argument("--fallback", choices=("flat", "base"), default="flat",
+    ap.add_argument("--fallback", choices=("flat", "base"), default="base",
                     help="warm start: the fallback start (flat start, or the base case's solution)")
+    ap.add_argument("--clip", type=float, default=0.3, help="warm start: gradient-norm clip of the training")
     ap.add_argument("--cpu-rows", type=int, default=512, help="oracle sample size (0: skip)")
     ap.add_argument("--workload", choices=("c2", "c3", "c4"), default="c2")
     ap.add_argument("--rate", type=float, default=50000.0, help="c4: offered requests per second")
@@ -151,7 +153,7 @@ def train_warm(args, wl, torch, capi, syn):
     ws = WarmStart(g.n_bus, g.pv, g.pq, g.slack, np.abs(g.flat_start()), n_hidden=args.hidden, seed=2,
                    n_mask=0 if mask is None else g.n_branch)
     t1 = time.perf_counter()
-    hist = ws.train(S, V, mask=mask, epochs=args.epochs, batch=args.train_batch, lr=args.lr,
+    hist = ws.train(S, V, mask=mask, epochs=args.epochs, batch=args.train_batch, lr=args.lr, clip=args.clip,
                     log=(lambda m: log(args, m)))
     lr = args.lr
     if not hist[-1] < 0.5 * hist[0]:
@@ -160,8 +162,10 @@ def train_warm(args, wl, torch, capi, syn):
     t_train = time.perf_counter() - t1
     n_train = int(S.shape[0])
     del S, V
-    return ws, dict(train_rows=n_train, epochs=args.epochs, hidden=args.hidden, lr=lr, loss_first=hist[0],
-                    loss_last=hist[-1], data_s=t_data, train_s=t_train)
+    rise = max([hist[i] / hist[i - 1] for i in range(4, len(hist))], default=None)
+    return ws, dict(train_rows=n_train, epochs=args.epochs, hidden=args.hidden, lr=lr, clip=args.clip,
+                    loss_first=hist[0], loss_last=hist[-1], max_epoch_rise_after_3=rise, data_s=t_data,
+                    train_s=t_train)
 
 
 def run_method(args, name, wl, torch, capi, ws=None):
@@ -191,13 +195,15 @@ def run_method(args, name, wl, torch, capi, ws=None):
     V0 = torch.from_numpy(V0h).to("cuda")
     if ws is not None:
         B.set_warm_contraction(args.contraction)
+        if wl.mask is not None:
+            B.set_warm_mask(wl.mask)  # the outage mask resident in HBM, like the injections
     torch.cuda.synchronize()
 
     def step():
         B.set_v_device(V0.data_ptr(), 0, args.rows)  # flat start (broadcast), inside the step
         if ws is not None:
             return B.solve_warm(ws.mlp, k_warm=args.k_warm, growth=args.growth, max_iter=args.max_iter,
-                                tol=args.tol, mask=wl.mask)
+                                tol=args.tol)
         return B.solve(max_iter=args.max_iter, tol=args.tol, scheduling=capi.CONTINUOUS)
 
     for _ in range(args.warmup):

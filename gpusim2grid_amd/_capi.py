@@ -114,6 +114,7 @@ def lib():
         "g2g_batch_device_arrays": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
         "g2g_batch_warm_start": (ctypes.c_int, [_P, _P, _P, i32]),
         "g2g_batch_set_warm_contraction": (ctypes.c_int, [_P, dbl]),
+        "g2g_batch_set_warm_mask": (ctypes.c_int, [_P, _P, i32]),
         "g2g_batch_branch_flows": (ctypes.c_int, [_P, i32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
         "g2g_batch_limit_violations": (ctypes.c_int, [_P, ctypes.POINTER(LimitTable), _P, _P, ctypes.c_double,
                                                       ctypes.c_double, i32, ctypes.POINTER(Violations)]),
@@ -174,6 +175,7 @@ class Plan:
 
     def __init__(self, handle):
         self._h = handle
+        self._destroy = lib().g2g_plan_destroy
 
     @classmethod
     def from_grid(cls, Ybus, pv, pq, slack, ext=None):
@@ -208,9 +210,12 @@ class Plan:
         return cls(h)
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().g2g_plan_destroy(self._h)
-            self._h = None
+        # the destroy function is looked up while the module is alive (at
+        # interpreter exit its globals may already be gone)
+        h, self._h = getattr(self, "_h", None), None
+        destroy = getattr(self, "_destroy", None)
+        if h and destroy:
+            destroy(h)
 
     def info(self):
         inf = PlanInfo()
@@ -255,11 +260,15 @@ class Batch:
         _check(lib().g2g_batch_create(plan._h, self.rows, precision, int(bool(per_row_ybus)), stream,
                                       ctypes.byref(h)))
         self._h = h
+        self._destroy = lib().g2g_batch_destroy
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().g2g_batch_destroy(self._h)
-            self._h = None
+        # the destroy function is looked up while the module is alive (at
+        # interpreter exit its globals may already be gone)
+        h, self._h = getattr(self, "_h", None), None
+        destroy = getattr(self, "_destroy", None)
+        if h and destroy:
+            destroy(h)
 
     def set_v(self, V, row0=0):
         V = _c128(V)
@@ -356,6 +365,17 @@ class Batch:
         st = SolveStats()
         _check(lib().g2g_batch_solve(self._h, int(max_iter), float(tol), int(scheduling), ctypes.byref(st)))
         return st.as_dict()
+
+    def set_warm_mask(self, mask):
+        """Resident outage mask for warm starts (rows x n_mask, 0/1), used
+        when solve_warm / warm_start get no mask; None drops it."""
+        if mask is None:
+            _check(lib().g2g_batch_set_warm_mask(self._h, None, 0))
+            return
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        if m.ndim != 2 or m.shape[0] != self.rows:
+            raise ValueError("the mask needs one row per batch row")
+        _check(lib().g2g_batch_set_warm_mask(self._h, _ptr(m), m.shape[1]))
 
     def set_warm_contraction(self, q):
         _check(lib().g2g_batch_set_warm_contraction(self._h, float(q)))
@@ -484,15 +504,19 @@ class Mlp:
         h = _P()
         _check(lib().g2g_mlp_create(self.n_in, self.H, self.n_out, int(seed), stream, ctypes.byref(h)))
         self._h = h
+        self._destroy = lib().g2g_mlp_destroy
         self.n_params = int(lib().g2g_mlp_num_params(h))
         off = np.empty(6, np.int64)
         _check(lib().g2g_mlp_param_offsets(h, _ptr(off)))
         self.offsets = off
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().g2g_mlp_destroy(self._h)
-            self._h = None
+        # the destroy function is looked up while the module is alive (at
+        # interpreter exit its globals may already be gone)
+        h, self._h = getattr(self, "_h", None), None
+        destroy = getattr(self, "_destroy", None)
+        if h and destroy:
+            destroy(h)
 
     def shapes(self):
         n_in, H, n_out = self.n_in, self.H, self.n_out

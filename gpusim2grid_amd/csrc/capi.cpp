@@ -58,8 +58,7 @@ template <typename T>
 __global__ void k_slots_to_rows(const T* src, int R, int m, const int* slots, T* dst, long long ld);
 __global__ void k_set_slots(const int* slots, int R, int* v, int value);
 __global__ void k_mask_v_nan(const unsigned char* mask, long long total, double2* V);
-__global__ void k_restart_v(const int* status, const unsigned char* warm, int T, int n, const double2* V0,
-                            double2* V);
+__global__ void k_restart_v(const int* status, const unsigned char* warm, int n, const double2* V0, double2* V);
 __global__ void k_restart_rows(int* status, unsigned char* warm, const int* it0, int* budget, int rows, int iter,
                                int max_iter, int* requeued);
 template <typename Real>
@@ -280,17 +279,20 @@ struct g2g_batch_s {
     int a_stride = 0, u_stride = 0;
     double* res = nullptr;
     double* jmax = nullptr;
-    // workgroups per tile of the mismatch and update kernels, and the
-    // mismatch's per-split residual partials [2][mis_splits][rows_pad]
+    // workgroups per tile of the mismatch and update kernels on a full
+    // batch, and the mismatch's per-split residual partials
+    // [2][splits][tiles x 64], room for part_cap (split, tile) pairs (a
+    // launch over fewer tiles may split each into more workgroups)
     int mis_splits = 1;
+    int part_cap = 1;
     double* mis_part = nullptr;
     int* status = nullptr;
     int* iters = nullptr;
     int* tile_active = nullptr;
-    int* n_active = nullptr;
+    int* n_active = nullptr;  // rows and tiles still iterating (2 ints)
     int* clamped = nullptr;
     int* error = nullptr;     // stream kernels' give-up flag
-    int* h_active = nullptr;  // pinned: rows still iterating, rows restarted
+    int* h_active = nullptr;  // pinned: rows and tiles still iterating, rows restarted
     // warm solves: the caller's start voltages (the fallback start)
     double2* V0 = nullptr;
     // step control (settings of the next solve)
@@ -319,6 +321,9 @@ struct g2g_batch_s {
     bool restart_on = false;
     int restart_max_iter = 0;
     double contraction = 0.0;  // the quadratic-contraction rejection of learned starts (0: off)
+    // the warm start's resident outage mask [rows][wmask_n] (nullable)
+    unsigned char* wmask = nullptr;
+    int wmask_n = 0;
     int* budget = nullptr;
     unsigned char* warm = nullptr;
     int* requeued = nullptr;
@@ -347,7 +352,7 @@ struct g2g_batch_s {
     {
         dfree(V); dfree(S); dfree(Yrow); dfree(A); dfree(U); dfree(X); dfree(res); dfree(jmax); dfree(mis_part);
         dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped); dfree(error);
-        dfree(V0); dfree(Bs); dfree(Xs);
+        dfree(V0); dfree(Bs); dfree(Xs); dfree(wmask);
         dfree(Vk); dfree(Sk); dfree(Yk); dfree(stk); dfree(itk); dfree(resk); dfree(mapk); dfree(it0);
         dfree(mask); dfree(maskk); dfree(xs); dfree(xsk); dfree(sx); dfree(pend);
         dfree(budget); dfree(warm); dfree(requeued); dfree(V0k); dfree(it0k); dfree(budgetk); dfree(warmk);
@@ -628,14 +633,34 @@ int default_mis_splits()
     return 4;
 }
 
+constexpr int kRestartChunks = 16;  // workgroups per tile of a restart copy
+
+// The tiles that can hold iterating rows: the compacted rows' tiles, or all.
+int live_tiles(const g2g_batch_s* b) { return b->compact_R > 0 ? (b->compact_R + kLanes - 1) / kLanes : b->T; }
+
+// Workgroups per tile of a mismatch or update launch over `tiles` tiles: the
+// batch's setting, doubled (up to 32) while the launch stays within one
+// round of resident workgroups and the partials within their buffer.  A
+// tile's buses form a latency-bound chain per wave, so a launch over a few
+// tiles runs as many times faster as it has workgroups per tile.
+int launch_splits(const g2g_batch_s* b, int tiles)
+{
+    constexpr int kMaxSplits = 32, kResidentWgs = 512;
+    int s = b->mis_splits;
+    while (s < kMaxSplits && 2 * s * tiles <= kResidentWgs && 2 * s * tiles <= b->part_cap) s *= 2;
+    return s;
+}
+
 template <typename Real>
 void launch_update(g2g_batch_s* b)
 {
     const g2g_plan_s* P = b->plan;
-    const int splits = b->max_dva > 0.0 ? 1 : b->mis_splits;  // step scaling: the row's maxima in one workgroup
+    const int T = live_tiles(b);
+    // step scaling: the row's maxima in one workgroup
+    const int splits = b->max_dva > 0.0 ? 1 : launch_splits(b, T);
     UpdateArgs<Real> a{static_cast<const Real*>(b->X), P->d_upd, P->n_upd, b->tile_active, b->status, b->V,
                        P->nr.d, P->nr.n, b->active_rows, b->max_dva, b->max_dvm, splits};
-    k_update<Real><<<b->T * splits, kLanes * kMisWaves, 0, b->stream>>>(a);
+    k_update<Real><<<T * splits, kLanes * kMisWaves, 0, b->stream>>>(a);
     HIPCHK(hipGetLastError());
 }
 
@@ -681,9 +706,9 @@ ExtArgs<Real> ext_args(g2g_batch_s* b)
     a.vc_w = P->dx.vc_w;
     a.sh_c1 = P->dx.sh_c1;
     a.sh_ck = P->dx.sh_ck;
-    a.part = b->mis_part;
+    a.part = b->mis_part;  // launch_mismatch sets the split count and the partials' stride
     a.part_stride = b->rows_pad;
-    a.splits = b->mis_splits;
+    a.splits = 1;
     return a;
 }
 
@@ -700,7 +725,7 @@ template <typename Real>
 void launch_ext_update(g2g_batch_s* b)
 {
     if (!b->plan->ext() || b->plan->nr.xt.n_xs == 0) return;
-    k_ext_update<Real><<<b->T, kLanes, 0, b->stream>>>(ext_args<Real>(b));
+    k_ext_update<Real><<<live_tiles(b), kLanes, 0, b->stream>>>(ext_args<Real>(b));
     HIPCHK(hipGetLastError());
 }
 
@@ -710,8 +735,11 @@ void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double
 {
     const NrPlan& P = b->plan->nr;
     const bool ext = b->plan->ext();
+    const int T = live_tiles(b);
+    const int splits = launch_splits(b, T);
+    const int part_stride = T * kLanes;
     if (ext) {
-        k_ext_pre<Real><<<b->T, kLanes, 0, b->stream>>>(ext_args<Real>(b));
+        k_ext_pre<Real><<<T, kLanes, 0, b->stream>>>(ext_args<Real>(b));
         HIPCHK(hipGetLastError());
     }
     const int* k1b = b->split_layout ? b->plan->d_k1_bus_s : b->plan->d_k1_bus;
@@ -722,9 +750,9 @@ void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double
                          growth > 0.0 ? b->contraction : 0.0,
                          (b->streaming || b->restart_on) ? b->it0 : nullptr, b->mask,
                          b->restart_on ? b->budget : nullptr, b->restart_on ? b->warm : nullptr,
-                         b->mis_splits, (b->mis_splits > 1 || ext) ? b->mis_part : nullptr, b->rows_pad,
+                         splits, (splits > 1 || ext) ? b->mis_part : nullptr, part_stride,
                          ext ? b->sx : nullptr, static_cast<int>(P.xt.xb_bus.size())};
-    const dim3 grid(b->T * b->mis_splits), block(kLanes * kMismatchWaves);
+    const dim3 grid(T * splits), block(kLanes * kMismatchWaves);
     if (b->per_row_y) {
         if (with_j) k_mismatch<Real, true, true><<<grid, block, 0, b->stream>>>(a);
         else k_mismatch<Real, true, false><<<grid, block, 0, b->stream>>>(a);
@@ -736,12 +764,14 @@ void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double
     if (ext) {
         ExtArgs<Real> ea = ext_args<Real>(b);
         if (!with_j) ea.n_jrec = 0;  // an F-only pass leaves the J values alone
-        k_ext_post<Real><<<b->T, kLanes, 0, b->stream>>>(ea);
+        ea.splits = splits;
+        ea.part_stride = part_stride;
+        k_ext_post<Real><<<T, kLanes, 0, b->stream>>>(ea);
         HIPCHK(hipGetLastError());
     }
     if (a.part) {
-        if (with_j) k_mis_status<Real, true><<<b->T, kLanes, 0, b->stream>>>(a);
-        else k_mis_status<Real, false><<<b->T, kLanes, 0, b->stream>>>(a);
+        if (with_j) k_mis_status<Real, true><<<T, kLanes, 0, b->stream>>>(a);
+        else k_mis_status<Real, false><<<T, kLanes, 0, b->stream>>>(a);
         HIPCHK(hipGetLastError());
     }
 }
@@ -899,11 +929,15 @@ void uncompact_rows(g2g_batch_s* b)
     std::swap(b->S, b->Sk);
     if (b->per_row_y) std::swap(b->Yrow, b->Yk);
     if (b->mask) std::swap(b->mask, b->maskk);
-    if (b->restart_on) {  // the fallback tables go back to row order unread
+    if (b->restart_on) {  // the fallback start and first counts go back unread
         std::swap(b->V0, b->V0k);
         std::swap(b->it0, b->it0k);
         std::swap(b->budget, b->budgetk);
         std::swap(b->warm, b->warmk);
+        // restarts while compacted changed the budgets and the warm flags
+        const unsigned g = blocks_for(R, 256);
+        k_move_vals<int><<<g, 256, 0, s>>>(b->budgetk, b->budget, b->mapk, R, 0, 0);
+        k_move_vals<unsigned char><<<g, 256, 0, s>>>(b->warmk, b->warm, b->mapk, R, 0, 0);
     }
     const unsigned gr = blocks_for(R, 256);
     k_move_vals<int><<<gr, 256, 0, s>>>(b->status, b->stk, b->mapk, R, 0, 0);
@@ -930,17 +964,27 @@ int split_env()
 
 constexpr int kSplitMaxTiles = 128;  // active tiles up to which the split walk is faster
 
-bool use_split(const g2g_batch_s* b, int active)
+bool use_split(const g2g_batch_s* b, int active_tiles)
 {
     if (!b->plan->split() || !b->pend || b->ir_steps > 0) return false;
     const int env = split_env();
     if (env >= 0) return env == 1;
-    const int tiles = b->compact_R > 0 ? (active + kLanes - 1) / kLanes : std::min(b->T, active);
-    return tiles <= kSplitMaxTiles;
+    return active_tiles <= kSplitMaxTiles;
 }
 
 // tiles a split launch covers: the compacted rows' tiles, or all
-int split_tiles(const g2g_batch_s* b) { return b->compact_R > 0 ? (b->compact_R + kLanes - 1) / kLanes : b->T; }
+int split_tiles(const g2g_batch_s* b) { return live_tiles(b); }
+
+// Whether to (re)compact: the active rows, packed, would free at least
+// kCompactMinTiles tiles and a quarter of the active ones.  Every kernel of
+// an iteration skips inactive tiles, so what compaction buys is the launch's
+// active tiles, at the cost of two row moves and a status download.
+constexpr int kCompactMinTiles = 16;
+bool worth_compacting(int active_rows, int active_tiles)
+{
+    const int packed = (active_rows + kLanes - 1) / kLanes;
+    return active_tiles - packed >= kCompactMinTiles && 4 * packed <= 3 * active_tiles;
+}
 
 // The A stream of tiles [0, T) to zero.  Each layout leaves its fill
 // positions (factor entries with no J value) at zero, and the two layouts'
@@ -966,7 +1010,7 @@ int nr_loop_body(g2g_batch_s* b, int max_iter, double tol, int freeze, double gr
     int it = 0;
     int restarts = 0;
     for (;; ++it) {
-        k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
+        k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 2);
         int e0 = tm.mark();
         launch_mismatch<Real>(b, true, it, max_iter, tol, freeze, growth);
         tm.ph[0].push_back({e0, tm.mark()});
@@ -975,45 +1019,47 @@ int nr_loop_body(g2g_batch_s* b, int max_iter, double tol, int freeze, double gr
             // rows whose learned start failed at this mismatch take their
             // fallback start now, with max_iter more updates
             const NrPlan& nr = b->plan->nr;
-            k_restart_v<<<blocks_for(static_cast<long long>(b->T) * nr.n * kLanes, 256), 256, 0, b->stream>>>(
-                b->status, b->warm, b->T, nr.n, b->V0, b->V);
+            k_restart_v<<<dim3(live_tiles(b), kRestartChunks), 256, 0, b->stream>>>(b->status, b->warm, nr.n, b->V0,
+                                                                                  b->V);
             k_restart_rows<<<blocks_for(b->active_rows, 256), 256, 0, b->stream>>>(
                 b->status, b->warm, b->it0, b->budget, b->active_rows, it, b->restart_max_iter, b->requeued);
             HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(b->h_active + 1, b->requeued, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+            HIPCHK(hipMemcpyAsync(b->h_active + 2, b->requeued, sizeof(int), hipMemcpyDeviceToHost, b->stream));
         }
-        HIPCHK(hipMemcpyAsync(b->h_active, b->n_active, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipMemcpyAsync(b->h_active, b->n_active, 2 * sizeof(int), hipMemcpyDeviceToHost, b->stream));
         HIPCHK(hipStreamSynchronize(b->stream));
-        if (b->restart_on && b->h_active[1] > restarts) {
+        if (b->restart_on && b->h_active[2] > restarts) {
             // evaluate again so the restarted rows' J and F (at their new
             // start) are in the A stream; the other rows' decisions were made
             // (no growth test this time) and come out the same
-            restarts = b->h_active[1];
-            k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
+            restarts = b->h_active[2];
+            k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 2);
             e0 = tm.mark();
             launch_mismatch<Real>(b, true, it, max_iter, tol, freeze, 0.0);
             tm.ph[0].push_back({e0, tm.mark()});
-            HIPCHK(hipMemcpyAsync(b->h_active, b->n_active, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+            HIPCHK(hipMemcpyAsync(b->h_active, b->n_active, 2 * sizeof(int), hipMemcpyDeviceToHost, b->stream));
             HIPCHK(hipStreamSynchronize(b->stream));
         }
         if (*b->h_active == 0) break;
-        if (!compacted && *b->h_active * 4 <= b->active_rows && *b->h_active + 4 * kLanes <= b->active_rows) {
+        if (worth_compacting(b->h_active[0], b->h_active[1])) {
+            // (again: the rows go back to their batch places first)
+            if (compacted) uncompact_rows(b);
             compact_rows(b);
             compacted = true;
             // the A stream was written in the old row order: evaluate again
             // (same V, same decisions; no rejection test) in the compacted one
-            k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
+            k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 2);
             e0 = tm.mark();
             launch_mismatch<Real>(b, true, it, max_iter, tol, freeze, 0.0);
             tm.ph[0].push_back({e0, tm.mark()});
         }
-        if (!b->split_layout && use_split(b, *b->h_active)) {
+        if (!b->split_layout && use_split(b, b->h_active[1])) {
             // the rest of the solve takes the split walk: the A stream again,
             // in its layout (same V, same decisions), over zeros (its fill
             // positions are not the main layout's)
             zero_a_tiles(b, split_tiles(b));
             b->split_layout = true;
-            k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
+            k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 2);
             e0 = tm.mark();
             launch_mismatch<Real>(b, true, it, max_iter, tol, freeze, 0.0);
             tm.ph[0].push_back({e0, tm.mark()});
@@ -1091,7 +1137,7 @@ void stream_round(g2g_batch_s* b, int k, int max_iter, double tol)
         b->Xs = dalloc<unsigned char>(bytes);
     }
     for (int q = 0; q < k; ++q) {
-        k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
+        k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 2);
         launch_mismatch<Real>(b, true, b->clock, max_iter, tol, 1, 0.0);
         launch_factor<Real>(b->plan, static_cast<const Real*>(b->A), static_cast<Real*>(b->U), b->a_stride,
                             b->u_stride, b->tile_active, b->jmax, b->clamped, b->error, b->active_rows, b->T, eps,
@@ -1177,6 +1223,20 @@ void solve_impl(g2g_batch_s* b, int max_iter, double tol, int sched, g2g_solve_s
 // at once and gets the full budget of max_iter updates on top, while the
 // other rows carry on, so a failure costs its own iterations only.  A row's
 // iteration count is the sum of its updates from both starts.
+// The warm start's outage mask on the device: the caller's host mask for
+// this call (uploaded into `tmp`), else the batch's resident one, else none.
+unsigned char* warm_mask(g2g_batch_s* b, const unsigned char* mask, int& n_mask, DeviceScratch& tmp)
+{
+    if (n_mask > 0) {
+        unsigned char* d = tmp.get<unsigned char>(static_cast<size_t>(b->active_rows) * n_mask);
+        HIPCHK(hipMemcpyAsync(d, mask, static_cast<size_t>(b->active_rows) * n_mask, hipMemcpyHostToDevice,
+                              b->stream));
+        return d;
+    }
+    n_mask = b->wmask ? b->wmask_n : 0;
+    return b->wmask;
+}
+
 template <typename Real>
 void solve_warm_impl(g2g_batch_s* b, g2g_mlp_s* mlp, const unsigned char* mask, int n_mask, int k_warm,
                      double growth, int max_iter, double tol, g2g_solve_stats* stats)
@@ -1193,12 +1253,7 @@ void solve_warm_impl(g2g_batch_s* b, g2g_mlp_s* mlp, const unsigned char* mask, 
     }
     if (!b->it0) b->it0 = dalloc<int>(rp);
     DeviceScratch tmp;
-    unsigned char* d_mask = nullptr;
-    if (n_mask > 0) {
-        d_mask = tmp.get<unsigned char>(static_cast<size_t>(b->active_rows) * n_mask);
-        HIPCHK(hipMemcpyAsync(d_mask, mask, static_cast<size_t>(b->active_rows) * n_mask, hipMemcpyHostToDevice,
-                              b->stream));
-    }
+    unsigned char* d_mask = warm_mask(b, mask, n_mask, tmp);
     reset_rows(b);
     SolveTimer tm(b, 3 * (k_warm + max_iter + 2) + 8);
     const int e_begin = tm.mark();
@@ -1464,7 +1519,7 @@ void adjoint_impl(g2g_batch_s* b, const double* rhs, double* x)
     b->streaming = false;
     try {
         reset_rows(b);
-        k_fill_int<<<1, 64, 0, st>>>(b->n_active, 0, 1);
+        k_fill_int<<<1, 64, 0, st>>>(b->n_active, 0, 2);
         launch_mismatch<Real>(b, true, 0, 1 << 30, 1e-300, 0, 0.0, true);
         k_gather_astream<Real><<<blocks_for(static_cast<long long>(rows) * nr.d, 256), 256, 0, st>>>(
             nullptr, 0, d_rhs, nr.d, rows, 0, nr.d, b->plan->d_apos, b->plan->d_fpos, b->a_stride,
@@ -1745,8 +1800,8 @@ int g2g_batch_create(g2g_plan_t P, int32_t rows, int32_t precision, int32_t per_
         b->res = dalloc<double>(rp);
         b->jmax = dalloc<double>(rp);
         b->mis_splits = default_mis_splits();
-        if (b->mis_splits > 1 || P->ext())
-            b->mis_part = dalloc<double>(2 * static_cast<size_t>(b->mis_splits) * rp);
+        b->part_cap = std::max(std::max(b->mis_splits, 4) * b->T, 128);
+        b->mis_part = dalloc<double>(2 * static_cast<size_t>(b->part_cap) * kLanes);
         if (P->split())
             b->pend = dalloc<unsigned char>(rp * static_cast<size_t>(std::max(1, nr.lu.pend_len)) * b->real_size());
         if (P->ext()) {
@@ -1759,10 +1814,10 @@ int g2g_batch_create(g2g_plan_t P, int32_t rows, int32_t precision, int32_t per_
         b->iters = dalloc<int>(rp);
         b->clamped = dalloc<int>(rp);
         b->tile_active = dalloc<int>(b->T);
-        b->n_active = dalloc<int>(1);
+        b->n_active = dalloc<int>(2);
         b->error = dalloc<int>(1);
         HIPCHK(hipMemsetAsync(b->error, 0, sizeof(int), b->stream));
-        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->h_active), 2 * sizeof(int), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->h_active), 4 * sizeof(int), hipHostMallocDefault));
         k_fill_int<<<blocks_for(rp, 256), 256, 0, b->stream>>>(b->status, kNotSimulated, rp);
         k_fill_int<<<blocks_for(rp, 256), 256, 0, b->stream>>>(b->iters, 0, rp);
         HIPCHK(hipGetLastError());
@@ -1876,6 +1931,22 @@ int g2g_batch_solve_warm(g2g_batch_t b, g2g_mlp_t mlp, const uint8_t* mask, int3
     });
 }
 
+int g2g_batch_set_warm_mask(g2g_batch_t b, const uint8_t* mask, int32_t n_mask)
+{
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        if (n_mask < 0 || (n_mask > 0 && !mask)) throw std::invalid_argument("mask missing");
+        HIPCHK(hipStreamSynchronize(b->stream));
+        DeviceScratch::free_one(b->wmask);
+        b->wmask_n = 0;
+        if (n_mask == 0) return;
+        const size_t bytes = static_cast<size_t>(b->rows) * n_mask;
+        b->wmask = dalloc<unsigned char>(bytes);
+        HIPCHK(hipMemcpy(b->wmask, mask, bytes, hipMemcpyHostToDevice));
+        b->wmask_n = n_mask;
+    });
+}
+
 int g2g_batch_set_warm_contraction(g2g_batch_t b, double q)
 {
     return guarded([&] {
@@ -1891,12 +1962,7 @@ int g2g_batch_warm_start(g2g_batch_t b, g2g_mlp_t mlp, const uint8_t* mask, int3
         if (!b || !mlp) throw std::invalid_argument("null argument");
         if (n_mask < 0 || (n_mask > 0 && !mask)) throw std::invalid_argument("mask missing");
         DeviceScratch tmp;
-        unsigned char* d_mask = nullptr;
-        if (n_mask > 0) {
-            d_mask = tmp.get<unsigned char>(static_cast<size_t>(b->active_rows) * n_mask);
-            HIPCHK(hipMemcpyAsync(d_mask, mask, static_cast<size_t>(b->active_rows) * n_mask,
-                                  hipMemcpyHostToDevice, b->stream));
-        }
+        unsigned char* d_mask = warm_mask(b, mask, n_mask, tmp);
         mlp_warm_start_tiles(mlp, b->S, b->active_rows, b->plan->nr.n, d_mask, n_mask, b->V, b->stream);
         HIPCHK(hipStreamSynchronize(b->stream));
     });
@@ -2133,7 +2199,7 @@ int g2g_batch_eval(g2g_batch_t b, double* F, double* J, double* residual)
         HIPCHK(hipMemcpyAsync(saved, b->status, rp * sizeof(int), hipMemcpyDeviceToDevice, b->stream));
         HIPCHK(hipMemcpyAsync(saved_it, b->iters, rp * sizeof(int), hipMemcpyDeviceToDevice, b->stream));
         reset_rows(b);
-        k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 1);
+        k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 2);
         if (b->precision == G2G_MIXED) launch_mismatch<float>(b, J != nullptr, 0, 1 << 30, 1e-300, 0);
         else launch_mismatch<double>(b, J != nullptr, 0, 1 << 30, 1e-300, 0);
         const size_t rs = b->real_size();

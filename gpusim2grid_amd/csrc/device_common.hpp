@@ -23,7 +23,7 @@ struct MismatchArgs {
     int* status;            // [rows_pad]
     int* iters;             // [rows_pad]
     int* tile_active;       // [T]
-    int* n_active;          // rows still iterating (accumulated)
+    int* n_active;          // rows, then tiles, still iterating (accumulated)
     int rows;
     int iter, max_iter;
     double tol;
@@ -42,7 +42,7 @@ struct MismatchArgs {
     double* part;               // nullable: per split, ||F||inf then max |J| partials
                                 // [2][splits][part_stride]; set, the status rule is left
                                 // to k_mis_status (several splits, or extension rows)
-    int part_stride;            // rows_pad
+    int part_stride;            // the launch's tiles x 64
     const double2* sx;          // nullable: extension buses' residual terms [T][n_xb][64]
     int n_xb;
 };

@@ -177,7 +177,10 @@ __device__ __forceinline__ void mis_status(const MismatchArgs<Real>& a, size_t t
     if (lane == 0) {
         const int cnt = __popcll(m);
         a.tile_active[tile] = cnt;
-        if (cnt) atomicAdd(a.n_active, cnt);
+        if (cnt) {
+            atomicAdd(a.n_active, cnt);
+            atomicAdd(a.n_active + 1, 1);
+        }
     }
 }
 
@@ -1607,15 +1610,27 @@ __global__ void k_mask_v_nan(const unsigned char* __restrict__ mask, long long t
 
 // Rows marked for a restart (status kRejected while on their learned start)
 // take their fallback start V0 back, bus by bus ...
-__global__ void k_restart_v(const int* __restrict__ status, const unsigned char* __restrict__ warm, int T, int n,
-                            const double2* __restrict__ V0, double2* __restrict__ V)
+__global__ __launch_bounds__(256) void k_restart_v(const int* __restrict__ status,
+                                                   const unsigned char* __restrict__ warm, int n,
+                                                   const double2* __restrict__ V0, double2* __restrict__ V)
 {
-    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gid >= static_cast<long long>(T) * n * kLanes) return;
-    const int lane = static_cast<int>(gid & (kLanes - 1));
-    const long long tp = gid / kLanes;
-    const int row = static_cast<int>(tp / n) * kLanes + lane;
-    if (status[row] == kRejected && warm[row]) V[gid] = V0[gid];
+    // workgroup (tile, chunk): a chunk of the tile's buses, for the lanes
+    // whose row restarts; a tile with none leaves at once
+    const size_t tile = blockIdx.x;
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int q = threadIdx.x / kLanes, nq = blockDim.x / kLanes;
+    const int row = static_cast<int>(tile) * kLanes + lane;
+    const bool mine = status[row] == kRejected && warm[row];
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    if (mine) any = 1;
+    __syncthreads();
+    if (!any || !mine) return;
+    const int chunk = (n + static_cast<int>(gridDim.y) - 1) / static_cast<int>(gridDim.y);
+    const int b0 = static_cast<int>(blockIdx.y) * chunk;
+    const int b1 = b0 + chunk < n ? b0 + chunk : n;
+    for (int bus = b0 + q; bus < b1; bus += nq) V[tidx(tile, n, bus, lane)] = V0[tidx(tile, n, bus, lane)];
 }
 
 // ... then continue from it in the same iteration: the fallback gets max_iter

@@ -283,6 +283,9 @@ int g2g_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_out, uint64_t seed,
         HIPCHK(hipMemcpy(p->w, h.data(), o * sizeof(float), hipMemcpyHostToDevice));
         HIPCHK(hipMemset(p->m, 0, o * sizeof(float)));
         HIPCHK(hipMemset(p->v, 0, o * sizeof(float)));
+        // the layers' gradients never write the padding between them, which
+        // the gradient norm and the Adam update read: zero it once
+        HIPCHK(hipMemset(p->g, 0, o * sizeof(float)));
         k_f32_to_bf16<<<blocks_for(static_cast<long long>(o), 256), 256>>>(p->w, static_cast<long long>(o), p->wb);
         HIPCHK(hipGetLastError());
         p->transpose_weights(nullptr);

@@ -105,7 +105,7 @@ class WarmStart:
 
     # -- training ----------------------------------------------------------
     def train(self, S, V, mask=None, epochs=20, batch=1024, lr=1e-3, lr_final=1e-5, seed=0, log=None,
-              warmup=0.05, clip=1.0):
+              warmup=0.05, clip=0.3):
         """Adam on (features(S), targets(V)): the step size rises linearly over
         the first ``warmup`` of the steps, then decays along a cosine from lr to
         lr_final; every gradient is clipped to norm ``clip`` (0: off).  S, V:

@@ -240,6 +240,12 @@ int g2g_batch_set_step_control(g2g_batch_t batch, int32_t ir_steps, int32_t scal
 int g2g_batch_solve_warm(g2g_batch_t batch, g2g_mlp_t mlp, const uint8_t *mask, int32_t n_mask, int32_t k_warm,
                          double growth, int32_t max_iter, double tol, g2g_solve_stats *stats);
 
+/* A resident outage mask for the batch's warm starts: rows x n_mask bytes
+ * (0/1, one row per batch row), uploaded once; g2g_batch_solve_warm and
+ * g2g_batch_warm_start called with no mask (NULL, 0) use it.  NULL / 0 drops
+ * it.  No counterpart in the reference (its starts are not learned). */
+int g2g_batch_set_warm_mask(g2g_batch_t batch, const uint8_t *mask, int32_t n_mask);
+
 /* Warm solves' early rejection: a row on its learned start whose ||F||inf,
  * from its second update on, exceeds q x the previous one squared (it is
  * not converging quadratically) restarts at once, as a failed row does

@@ -266,3 +266,16 @@ def test_warm_start_with_outages_gains_over_flat_start():
     _say(f"warm outages: mean updates {it[conv].mean():.2f} vs flat {ito[conv].mean():.2f}, "
          f"re-queued {stats['n_requeued']}")
     assert it[conv].mean() <= ito[conv].mean() - 0.75
+    # the same solve with the mask resident on the device
+    _say("warm outages: resident mask")
+    Bt.set_v(g.flat_start())
+    Bt.set_warm_mask(syn.outage_mask(g, outs_t))
+    stats2 = Bt.solve_warm(ws.mlp, k_warm=4, growth=2.0, max_iter=10, tol=1e-8)
+    st2, it2, _ = Bt.results()
+    np.testing.assert_array_equal(st2, st)
+    np.testing.assert_array_equal(it2, it)
+    assert np.abs(Bt.get_v() - Vw).max() <= 1e-12
+    assert stats2["n_requeued"] == stats["n_requeued"]
+    Bt.set_warm_mask(None)
+    with pytest.raises(ValueError):  # the network needs its mask again
+        Bt.solve_warm(ws.mlp, k_warm=4, growth=2.0, max_iter=10, tol=1e-8)
