@@ -58,7 +58,9 @@ template <typename T>
 __global__ void k_slots_to_rows(const T* src, int R, int m, const int* slots, T* dst, long long ld);
 __global__ void k_set_slots(const int* slots, int R, int* v, int value);
 __global__ void k_mask_v_nan(const unsigned char* mask, long long total, double2* V);
-__global__ void k_restart_v(const int* status, const unsigned char* warm, int n, const double2* V0, double2* V);
+__global__ void k_restart_v(const int* status, const unsigned char* warm, int n, const double2* V0, int v0_row,
+                            double2* V);
+__global__ void k_v_uniform(const double2* V, int rows, int n, int* differs);
 __global__ void k_restart_rows(int* status, unsigned char* warm, const int* it0, int* budget, int rows, int iter,
                                int max_iter, int* requeued);
 template <typename Real>
@@ -295,6 +297,8 @@ struct g2g_batch_s {
     int* h_active = nullptr;  // pinned: rows and tiles still iterating, rows restarted
     // warm solves: the caller's start voltages (the fallback start)
     double2* V0 = nullptr;
+    bool v0_row = false;  // V0 holds one row [n] (every row's fallback start is the same)
+    int* differs = nullptr;
     // step control (settings of the next solve)
     int ir_steps = 0;                 // refinement steps of every Newton step
     double max_dva = 0.0, max_dvm = 0.0;  // step scaling bounds (max_dva <= 0: off)
@@ -321,6 +325,7 @@ struct g2g_batch_s {
     bool restart_on = false;
     int restart_max_iter = 0;
     double contraction = 0.0;  // the quadratic-contraction rejection of learned starts (0: off)
+    int n_cu = 256;            // the device's compute units
     // the warm start's resident outage mask [rows][wmask_n] (nullable)
     unsigned char* wmask = nullptr;
     int wmask_n = 0;
@@ -352,7 +357,7 @@ struct g2g_batch_s {
     {
         dfree(V); dfree(S); dfree(Yrow); dfree(A); dfree(U); dfree(X); dfree(res); dfree(jmax); dfree(mis_part);
         dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped); dfree(error);
-        dfree(V0); dfree(Bs); dfree(Xs); dfree(wmask);
+        dfree(V0); dfree(Bs); dfree(Xs); dfree(wmask); dfree(differs);
         dfree(Vk); dfree(Sk); dfree(Yk); dfree(stk); dfree(itk); dfree(resk); dfree(mapk); dfree(it0);
         dfree(mask); dfree(maskk); dfree(xs); dfree(xsk); dfree(sx); dfree(pend);
         dfree(budget); dfree(warm); dfree(requeued); dfree(V0k); dfree(it0k); dfree(budgetk); dfree(warmk);
@@ -881,11 +886,11 @@ void compact_rows(g2g_batch_s* b)
             b->budgetk = dalloc<int>(rp);
             b->warmk = dalloc<unsigned char>(rp);
         }
-        k_move_rows<double2><<<gv, 256, 0, s>>>(b->V0, b->V0k, b->mapk, R, nr.n, 1);
+        if (!b->v0_row) k_move_rows<double2><<<gv, 256, 0, s>>>(b->V0, b->V0k, b->mapk, R, nr.n, 1);
         k_move_vals<int><<<gr, 256, 0, s>>>(b->it0, b->it0k, b->mapk, R, 1, 0);
         k_move_vals<int><<<gr, 256, 0, s>>>(b->budget, b->budgetk, b->mapk, R, 1, 0);
         k_move_vals<unsigned char><<<gr, 256, 0, s>>>(b->warm, b->warmk, b->mapk, R, 1, 0);
-        std::swap(b->V0, b->V0k);
+        if (!b->v0_row) std::swap(b->V0, b->V0k);
         std::swap(b->it0, b->it0k);
         std::swap(b->budget, b->budgetk);
         std::swap(b->warm, b->warmk);
@@ -930,7 +935,7 @@ void uncompact_rows(g2g_batch_s* b)
     if (b->per_row_y) std::swap(b->Yrow, b->Yk);
     if (b->mask) std::swap(b->mask, b->maskk);
     if (b->restart_on) {  // the fallback start and first counts go back unread
-        std::swap(b->V0, b->V0k);
+        if (!b->v0_row) std::swap(b->V0, b->V0k);
         std::swap(b->it0, b->it0k);
         std::swap(b->budget, b->budgetk);
         std::swap(b->warm, b->warmk);
@@ -976,14 +981,24 @@ bool use_split(const g2g_batch_s* b, int active_tiles)
 int split_tiles(const g2g_batch_s* b) { return live_tiles(b); }
 
 // Whether to (re)compact: the active rows, packed, would free at least
-// kCompactMinTiles tiles and a quarter of the active ones.  Every kernel of
-// an iteration skips inactive tiles, so what compaction buys is the launch's
-// active tiles, at the cost of two row moves and a status download.
+// kCompactMinTiles tiles and a quarter of the active ones, and the LU walk
+// gains from it.  A tile's walk is one workgroup, a latency-bound chain, so
+// a launch costs about one walk per round of workgroups over the CUs: fewer
+// tiles help once they drop a round, or fit the split walk.  (The mismatch
+// and update kernels skip inactive tiles; their saving alone does not repay
+// the row moves, which for per-row Ybus batches carry the whole Ybus.)
+// G2G_COMPACT=0 / 1: never / whenever packing frees a quarter of the active
+// tiles.
 constexpr int kCompactMinTiles = 16;
-bool worth_compacting(int active_rows, int active_tiles)
+bool worth_compacting(const g2g_batch_s* b, int active_rows, int active_tiles)
 {
     const int packed = (active_rows + kLanes - 1) / kLanes;
-    return active_tiles - packed >= kCompactMinTiles && 4 * packed <= 3 * active_tiles;
+    if (4 * packed > 3 * active_tiles) return false;
+    if (const char* e = std::getenv("G2G_COMPACT")) return std::atoi(e) != 0 && packed < active_tiles;
+    if (active_tiles - packed < kCompactMinTiles) return false;
+    const int rounds_now = (active_tiles + b->n_cu - 1) / b->n_cu;
+    const int rounds_packed = (packed + b->n_cu - 1) / b->n_cu;
+    return rounds_packed < rounds_now || (b->plan->split() && packed <= kSplitMaxTiles);
 }
 
 // The A stream of tiles [0, T) to zero.  Each layout leaves its fill
@@ -1020,7 +1035,7 @@ int nr_loop_body(g2g_batch_s* b, int max_iter, double tol, int freeze, double gr
             // fallback start now, with max_iter more updates
             const NrPlan& nr = b->plan->nr;
             k_restart_v<<<dim3(live_tiles(b), kRestartChunks), 256, 0, b->stream>>>(b->status, b->warm, nr.n, b->V0,
-                                                                                  b->V);
+                                                                                  b->v0_row ? 1 : 0, b->V);
             k_restart_rows<<<blocks_for(b->active_rows, 256), 256, 0, b->stream>>>(
                 b->status, b->warm, b->it0, b->budget, b->active_rows, it, b->restart_max_iter, b->requeued);
             HIPCHK(hipGetLastError());
@@ -1041,7 +1056,7 @@ int nr_loop_body(g2g_batch_s* b, int max_iter, double tol, int freeze, double gr
             HIPCHK(hipStreamSynchronize(b->stream));
         }
         if (*b->h_active == 0) break;
-        if (worth_compacting(b->h_active[0], b->h_active[1])) {
+        if (worth_compacting(b, b->h_active[0], b->h_active[1])) {
             // (again: the rows go back to their batch places first)
             if (compacted) uncompact_rows(b);
             compact_rows(b);
@@ -1258,7 +1273,20 @@ void solve_warm_impl(g2g_batch_s* b, g2g_mlp_s* mlp, const unsigned char* mask, 
     SolveTimer tm(b, 3 * (k_warm + max_iter + 2) + 8);
     const int e_begin = tm.mark();
     hipStream_t s = b->stream;
-    HIPCHK(hipMemcpyAsync(b->V0, b->V, nv * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    // the fallback start: one row when every row's V is the same (a
+    // broadcast start), else the whole V
+    if (!b->differs) b->differs = dalloc<int>(1);
+    k_fill_int<<<1, 64, 0, s>>>(b->differs, 0, 1);
+    const long long vt = (b->active_rows + kLanes - 1) / kLanes;
+    if (vt > 0)
+        k_v_uniform<<<blocks_for(vt * nr.n * kLanes, 256), 256, 0, s>>>(b->V, b->active_rows, nr.n, b->differs);
+    HIPCHK(hipGetLastError());
+    int differs = 0;
+    HIPCHK(hipMemcpyAsync(&differs, b->differs, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    b->v0_row = differs == 0;
+    if (b->v0_row) k_tiles_to_rows<double2><<<blocks_for(nr.n, 256), 256, 0, s>>>(b->V, 1, nr.n, b->V0, nr.n);
+    else HIPCHK(hipMemcpyAsync(b->V0, b->V, nv * sizeof(double2), hipMemcpyDeviceToDevice, s));
     mlp_warm_start_tiles(mlp, b->S, b->active_rows, nr.n, d_mask, n_mask, b->V, s);
     tm.ph[3].push_back({e_begin, tm.mark()});
     k_fill_int<<<blocks_for(rp, 256), 256, 0, s>>>(b->it0, 0, rp);
@@ -1799,6 +1827,12 @@ int g2g_batch_create(g2g_plan_t P, int32_t rows, int32_t precision, int32_t per_
         HIPCHK(hipMemsetAsync(b->A, 0, rp * b->a_stride * b->real_size(), b->stream));
         b->res = dalloc<double>(rp);
         b->jmax = dalloc<double>(rp);
+        {
+            int dev = 0, cu = 0;
+            HIPCHK(hipGetDevice(&dev));
+            HIPCHK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+            if (cu > 0) b->n_cu = cu;
+        }
         b->mis_splits = default_mis_splits();
         b->part_cap = std::max(std::max(b->mis_splits, 4) * b->T, 128);
         b->mis_part = dalloc<double>(2 * static_cast<size_t>(b->part_cap) * kLanes);

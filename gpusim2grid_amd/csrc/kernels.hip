@@ -1612,8 +1612,10 @@ __global__ void k_mask_v_nan(const unsigned char* __restrict__ mask, long long t
 // take their fallback start V0 back, bus by bus ...
 __global__ __launch_bounds__(256) void k_restart_v(const int* __restrict__ status,
                                                    const unsigned char* __restrict__ warm, int n,
-                                                   const double2* __restrict__ V0, double2* __restrict__ V)
+                                                   const double2* __restrict__ V0, int v0_row,
+                                                   double2* __restrict__ V)
 {
+    // V0: tiled like V, or (v0_row) one row [n] for every row
     // workgroup (tile, chunk): a chunk of the tile's buses, for the lanes
     // whose row restarts; a tile with none leaves at once
     const size_t tile = blockIdx.x;
@@ -1630,7 +1632,25 @@ __global__ __launch_bounds__(256) void k_restart_v(const int* __restrict__ statu
     const int chunk = (n + static_cast<int>(gridDim.y) - 1) / static_cast<int>(gridDim.y);
     const int b0 = static_cast<int>(blockIdx.y) * chunk;
     const int b1 = b0 + chunk < n ? b0 + chunk : n;
-    for (int bus = b0 + q; bus < b1; bus += nq) V[tidx(tile, n, bus, lane)] = V0[tidx(tile, n, bus, lane)];
+    for (int bus = b0 + q; bus < b1; bus += nq)
+        V[tidx(tile, n, bus, lane)] = v0_row ? V0[bus] : V0[tidx(tile, n, bus, lane)];
+}
+
+// *differs = 1 if some row of [0, rows) has a V other than row 0's
+__global__ __launch_bounds__(256) void k_v_uniform(const double2* __restrict__ V, int rows, int n,
+                                                   int* __restrict__ differs)
+{
+    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long tiles = (rows + kLanes - 1) / kLanes;
+    if (gid >= tiles * n * kLanes) return;
+    // gid enumerates (tile, bus, lane) in the tiled layout's order
+    const int lane = static_cast<int>(gid % kLanes);
+    const long long tb = gid / kLanes;
+    const size_t tile = static_cast<size_t>(tb / n);
+    const int bus = static_cast<int>(tb % n);
+    if (static_cast<int>(tile) * kLanes + lane >= rows) return;
+    const double2 v = V[tidx(tile, n, bus, lane)], r = V[tidx(0, n, bus, 0)];
+    if (v.x != r.x || v.y != r.y) *differs = 1;
 }
 
 // ... then continue from it in the same iteration: the fallback gets max_iter

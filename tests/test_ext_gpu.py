@@ -111,9 +111,10 @@ def test_distributed_slack_on_the_2000_bus_grid():
         assert abs(xs[k, 0] - xso[0]) < 1e-8
 
 
-def test_batch_size_does_not_change_the_rows():
+def test_batch_size_does_not_change_the_rows(monkeypatch):
     """Rows solved in a batch of 320 (tail compaction moves the extension
     state with its rows) equal the same rows solved alone."""
+    monkeypatch.setenv("G2G_COMPACT", "1")  # compact even this small a batch
     g = syn.grid_118()
     ext = features(g)
     rows = 320

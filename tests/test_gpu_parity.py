@@ -203,10 +203,11 @@ def test_raw_lu_solve_matches_scipy(prec):
 
 
 @pytest.mark.parametrize("prec", [_capi.FP64, _capi.MIXED])
-def test_tail_compaction_keeps_every_row(prec):
+def test_tail_compaction_keeps_every_row(prec, monkeypatch):
     """A batch whose rows mostly start converged: after the first update few
     rows are left, they are compacted into dense tiles and scattered back;
     statuses, iteration counts and voltages are the oracle's, row by row."""
+    monkeypatch.setenv("G2G_COMPACT", "1")  # compact even this small a batch
     g = syn.grid_118()
     rows = 1000
     _say(f"tail compaction prec={prec} rows={rows}")

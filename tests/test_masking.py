@@ -178,9 +178,10 @@ def test_masked_rows_match_oracle(prec):
 
 
 @pytest.mark.gpu
-def test_masked_rows_through_tail_compaction():
+def test_masked_rows_through_tail_compaction(monkeypatch):
     """Most rows start solved, so the few still iterating are compacted into
     dense tiles; their masks travel with them."""
+    monkeypatch.setenv("G2G_COMPACT", "1")  # compact even this small a batch
     from gpusim2grid_amd import _capi
     g = syn.grid_118()
     rows = 900

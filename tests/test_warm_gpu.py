@@ -133,10 +133,16 @@ def test_warm_solve_matches_flat_start_oracle(trained, prec):
     assert stats["n_converged"] == rows
 
 
-def test_untrained_mlp_falls_back_to_flat_start():
+@pytest.mark.parametrize("start,compact", [("flat", None), ("flat", "1"), ("per_row", None), ("per_row", "1")])
+def test_untrained_mlp_falls_back_to_flat_start(start, compact, monkeypatch):
+    """Rows whose learned start fails restart from the batch's V: one start
+    for every row, or a start per row; with compaction forced, the restarts
+    happen in compacted batches too (compacted again as rows finish)."""
+    if compact is not None:
+        monkeypatch.setenv("G2G_COMPACT", compact)
     g = syn.grid_118()
     rows = 300
-    _say("warm fallback (untrained)")
+    _say(f"warm fallback (untrained) start={start} compact={compact}")
     S = _sbus(g, rows, seed=3)
     ws = WarmStart(g.n_bus, g.pv, g.pq, g.slack, np.abs(g.flat_start()), n_hidden=64, seed=5)
     n2 = 2 * g.n_bus
@@ -147,21 +153,27 @@ def test_untrained_mlp_falls_back_to_flat_start():
     ws.out_mu = np.tile(np.array([0.0, 1.0], np.float32), g.n_bus)
     ws.out_sigma = np.tile(np.array([1.0, 0.05], np.float32), g.n_bus)
     ws.mlp.set_normalization(ws.in_mu, ws.in_sigma, ws.out_mu, ws.out_sigma)
+    V0 = g.flat_start()
+    if start == "per_row":  # angles perturbed row by row
+        rng = np.random.default_rng(6)
+        V0 = V0[None, :] * np.exp(1j * rng.normal(0.0, 0.01, (rows, g.n_bus)))
+        V0[:, g.slack] = g.flat_start()[g.slack]
     plan = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
     B = _capi.Batch(plan, rows)
     B.set_sbus(S)
-    B.set_v(g.flat_start())
+    B.set_v(V0)
     stats = B.solve_warm(ws.mlp, k_warm=2, growth=2.0, max_iter=10, tol=1e-8)
     st, it, _ = B.results()
     V = B.get_v()
-    Vo, sto, ito, _ = orc.nr_batch(g.Ybus, g.flat_start(), S, g.pv, g.pq, 10, 1e-8, n_threads=16)
+    Vo, sto, ito, _ = orc.nr_batch(g.Ybus, V0, S, g.pv, g.pq, 10, 1e-8, n_threads=16)
     np.testing.assert_array_equal(st, sto)
     assert stats["n_requeued"] > 0
-    # re-queued rows restart from the flat start: their second pass is the
-    # oracle's flat-start run, iteration for iteration, on top of the warm pass
+    # re-queued rows restart from the batch's V: their second pass is the
+    # oracle's run from that start, iteration for iteration, on top of the
+    # warm pass
     Bf = _capi.Batch(plan, rows)
     Bf.set_sbus(S)
-    Bf.set_v(g.flat_start())
+    Bf.set_v(V0)
     Bf.solve(max_iter=10, tol=1e-8)
     stf, itf, _ = Bf.results()
     Vf = Bf.get_v()
