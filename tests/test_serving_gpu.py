@@ -94,3 +94,71 @@ def test_serving_loop_two_grids():
         V = np.stack([r.V for r in rs])
         assert np.abs(V[ok] - Vo[ok]).max() < 1e-6
     assert rep["p50_ms"] > 0 and rep["p99_ms"] >= rep["p50_ms"]
+
+
+def _train_warm(g, rows=4096, epochs=8, hidden=256):
+    """A warm start for grid g trained on outage rows solved by the device
+    (0-2 non-islanding outages, the mask among the inputs)."""
+    import torch
+    from gpusim2grid_amd.warmstart import WarmStart
+    p, q = syn.injection_scenarios(g, rows, seed=2)
+    S = (p + 1j * q) / g.sn_mva
+    outs = syn.random_outages(g, rows, max_k=2, seed=4)
+    plan = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+    B = _capi.Batch(plan, rows, precision=_capi.MIXED, per_row_ybus=True)
+    B.set_sbus(S)
+    B.set_v(g.flat_start())
+    B.set_ybus_edits(*syn.outage_edits(g, outs))
+    B.solve(max_iter=10, tol=1e-8)
+    st, _, _ = B.results()
+    V = B.get_v()
+    del B
+    ok = st == 0
+    ws = WarmStart(g.n_bus, g.pv, g.pq, g.slack, np.abs(g.flat_start()), n_hidden=hidden, seed=2,
+                   n_mask=g.n_branch)
+    ws.train(torch.as_tensor(S[ok], device="cuda"), torch.as_tensor(V[ok], device="cuda"),
+             mask=torch.as_tensor(syn.outage_mask(g, outs)[ok], device="cuda"), epochs=epochs, batch=256, lr=1e-3)
+    return ws
+
+
+def test_serving_three_grids_outages_and_warm_start_against_oracle():
+    """The serving workload in small: Poisson requests over the 118-, 2,000-
+    and 6,500-bus grids, 0-3 tripped branches each, resident per-grid batches
+    with slot refill, learned warm starts with the flat-start fallback, mixed
+    precision.  Every request comes back; per grid, a sample of requests is
+    solved by the oracle from the flat start on the request's own outage
+    Ybus: where the oracle converges the server converged to the same V, and
+    every converged answer satisfies the oracle's mismatch test."""
+    grids = {"g118": syn.grid_118(), "c1": syn.grid_c1(), "c2": syn.grid_c2()}
+    warm = {}
+    for name, g in grids.items():
+        _say(f"serving c4: training the warm start of {name}")
+        warm[name] = _train_warm(g)
+    reqs = serving.poisson_requests(grids, rate=30000.0, n=3000, max_outages=3, seed=7)
+    _say(f"serving c4: {len(reqs)} requests")
+    srv = serving.PowerFlowServer(grids, slots={"g118": 512, "c1": 1024, "c2": 1024}, precision=_capi.MIXED,
+                                  warm=warm, hbm_budget=64e9)
+    done, rep = srv.run(reqs, timeout=300.0)
+    _say(str(rep))
+    assert rep["requests"] == len(reqs) and len({r.rid for r in done}) == len(reqs)
+    by = {}
+    for r in done:
+        by.setdefault(r.grid, []).append(r)
+    assert set(by) == set(grids)
+    for name, rs in by.items():
+        g = grids[name]
+        rs = sorted(rs, key=lambda r: r.rid)[:200]
+        _say(f"serving c4: oracle on {len(rs)} requests of {name}")
+        S = np.stack([r.S for r in rs])
+        Yrows = np.stack([orc.ybus_values_on_pattern(g.Ybus, orc.outage_ybus(g, r.outages)) for r in rs])
+        Vo, sto, _, _ = orc.nr_batch(g.Ybus, g.flat_start(), S, g.pv, g.pq, 10, 1e-8, n_threads=16, Yx_rows=Yrows)
+        st = np.array([r.status for r in rs])
+        V = np.stack([r.V for r in rs])
+        ok = sto == 0
+        assert ok.mean() > 0.95
+        assert (st[ok] == 0).all()
+        assert np.abs(V[ok] - Vo[ok]).max() < 1e-6
+        for k in np.flatnonzero(st == 0):
+            F = orc.mismatch_np(orc.outage_ybus(g, rs[k].outages), V[k], S[k], g.pv, g.pq)
+            assert np.max(np.abs(F)) < 2e-8  # the device's own test was < 1e-8
+    assert rep["p50_ms"] > 0 and rep["p99_ms"] >= rep["p50_ms"]
