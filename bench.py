@@ -10,7 +10,10 @@ injections are resident in HBM before the timed region.  Methods:
 * ``fp64``  plain FP64 flat-start NR (the reference method);
 * ``mixed`` FP32 Jacobian / LU / triangular solves, FP64 mismatch, test and
   state;
-* ``warm``  learned warm start + mixed precision (when the MLP is available).
+* ``warm``  learned warm start + mixed precision (when the MLP is available);
+* ``facade_mixed`` / ``facade_warm``: ``mixed`` / ``warm`` through the
+  drop-in Python surface (InjectionSweepGPU.compute), to show what the
+  facade path costs (``vs_capi``).
 
 Prints one JSON line.  ``value`` is the converged PF/s of the best method
 measured (the headline method: warm start + mixed precision, else mixed
@@ -52,7 +55,8 @@ def parse():
     ap.add_argument("--rows", type=int, default=16384)
     ap.add_argument("--max-iter", type=int, default=10)
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--methods", default="fp64,mixed,warm")
+    ap.add_argument("--methods", default=None,
+                    help="comma list (default: fp64,mixed,warm plus, on c2, facade_mixed,facade_warm)")
     ap.add_argument("--train-rows", type=int, default=65536, help="warm start: solved training scenarios")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--hidden", type=int, default=1024)
@@ -65,7 +69,8 @@ def parse():
     ap.add_argument("--fallback", choices=("flat", "base"), default="base",
                     help="warm start: the fallback start (flat start, or the base case's solution)")
     ap.add_argument("--clip", type=float, default=0.3, help="warm start: gradient-norm clip of the training")
-    ap.add_argument("--cpu-rows", type=int, default=512, help="oracle sample size (0: skip)")
+    ap.add_argument("--cpu-rows", type=int, default=0, help="oracle sample rows (0: sized by --cpu-seconds)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="oracle sample's CPU time (0: skip)")
     ap.add_argument("--workload", choices=("c2", "c3", "c4"), default="c2")
     ap.add_argument("--rate", type=float, default=50000.0, help="c4: offered requests per second")
     ap.add_argument("--requests", type=int, default=100000, help="c4: requests replayed")
@@ -75,6 +80,8 @@ def parse():
                          "rocprofv3 PMC byte counters)")
     ap.add_argument("--quiet", action="store_true")
     args = ap.parse_args()
+    if args.methods is None:
+        args.methods = "fp64,mixed,warm" + (",facade_mixed,facade_warm" if args.workload == "c2" else "")
     if args.workload == "c3":  # the learned-warm-start workload's defaults: 200k training rows
         for k, v in (("train_rows", 200_000),):
             if getattr(args, k) == ap.get_default(k):
@@ -240,31 +247,84 @@ def run_method(args, name, wl, torch, capi, ws=None):
     V = B.get_v()
     del B, S_dev, V0
     torch.cuda.synchronize()
-    return out, V, st
+    return out, V, st, it
 
 
-def cpu_baseline(args, wl, V_gpu, st_gpu):
+def run_facade(args, name, wl, torch, ws=None):
+    """The same rows through the drop-in Python surface: InjectionSweepGPU
+    (array-tuple grid, set_injections in MW / MVAr, compute() returning the
+    aliasing DLPack capsule), mixed precision, continuous scheduling; with the
+    learned warm start (falling back to the base case) or from the flat
+    start."""
+    from gpusim2grid_amd.facades import InjectionSweepGPU
+    g, S = wl.g, wl.S
+    grid = (g.Ybus, g.flat_start(), g.Sbus, np.atleast_1d(g.slack), np.ones(np.atleast_1d(g.slack).size), g.pv,
+            g.pq)
+    f = InjectionSweepGPU(grid, init_from_n_powerflow=False, precision="mixed", nb_iter=args.max_iter,
+                          scheduling="continuous", tol=args.tol)
+    f.set_injections(S.real * g.sn_mva, S.imag * g.sn_mva, g.sn_mva)
+    if ws is not None:
+        f.set_warm_start(ws, k_warm=args.k_warm, growth=args.growth)
+        f.set_v_init_from(None)  # the fallback: the base case
+    else:
+        f.set_v_init_from(g.flat_start())
+    for _ in range(args.warmup):
+        f.compute(batch_size=args.rows)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cap = f.compute(batch_size=args.rows)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    Vd = torch.from_dlpack(cap)  # aliases the session's results
+    st = np.asarray(f.get_row_status())
+    it = np.asarray(f.get_row_iterations())
+    V = Vd.cpu().numpy().reshape(-1, g.n_bus)
+    conv = int((st == 0).sum())
+    out = dict(method=name, pfps=conv * args.steps / dt, ms_per_step=1e3 * dt / args.steps, converged=conv,
+               rows=args.rows, mean_row_iters=float(it[st == 0].mean()) if conv else None,
+               path="InjectionSweepGPU.compute (DLPack result)")
+    del f
+    return out, V, st, it
+
+
+def cpu_baseline(args, wl, keep):
+    """The oracle's C NR timed on the host's cores over a bounded sample of
+    the workload's rows (about --cpu-seconds of work, sized from a pilot),
+    with KLU-style refactorizations after each row's first factorization;
+    and every GPU method's answers against it on that sample."""
     from oracle import oracle as orc
     g, S = wl.g, wl.S
-    n = min(args.cpu_rows, S.shape[0])
-    Yrows = None
-    if wl.outages is not None:
-        Yrows = np.stack([orc.ybus_values_on_pattern(g.Ybus, orc.outage_ybus(g, wl.outages[r])) for r in range(n)])
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
-    t0 = time.perf_counter()
-    Vo, sto, ito, reso = orc.nr_batch(g.Ybus, g.flat_start(), S[:n], g.pv, g.pq, max_iter=args.max_iter,
-                                      tol=args.tol, n_threads=cores, Yx_rows=Yrows)
-    dt = time.perf_counter() - t0
+
+    def run(r0, r1):
+        Yrows = None if wl.outages is None else orc.outage_ybus_rows(g, wl.outages[r0:r1])
+        t0 = time.perf_counter()
+        out = orc.nr_batch(g.Ybus, g.flat_start(), S[r0:r1], g.pv, g.pq, max_iter=args.max_iter, tol=args.tol,
+                           n_threads=cores, Yx_rows=Yrows, refactor=True)
+        return out, time.perf_counter() - t0
+
+    pilot = min(S.shape[0], 4 * cores)
+    _, dt_p = run(0, pilot)
+    n = args.cpu_rows or int(args.cpu_seconds * pilot / max(dt_p, 1e-6))
+    n = max(pilot, min(S.shape[0], n))
+    (Vo, sto, ito, _), dt = run(0, n)
     conv = int((sto == 0).sum())
-    parity = None
-    if V_gpu is not None:
-        ok = (sto == 0) & (st_gpu[:n] == 0)
-        parity = dict(status_equal=bool(np.array_equal(sto, st_gpu[:n])),
-                      max_dV=float(np.abs(V_gpu[:n][ok] - Vo[ok]).max()) if ok.any() else None)
+    parity = dict(rows=n)
+    for k, (Vk, stk, itk) in keep.items():
+        ok = (sto == 0) & (stk[:n] == 0)
+        dv = np.abs(Vk[:n][ok] - Vo[ok]).max(axis=1) if ok.any() else np.zeros(0)
+        parity[k] = dict(status_equal=bool(np.array_equal(sto, stk[:n])),
+                         iters_equal=float((ito[ok] == itk[:n][ok]).mean()) if ok.any() else None,
+                         mean_iters=float(itk[:n][ok].mean()) if ok.any() else None,
+                         oracle_mean_iters=float(ito[ok].mean()) if ok.any() else None,
+                         dV_p50=float(np.percentile(dv, 50)) if dv.size else None,
+                         dV_p99=float(np.percentile(dv, 99)) if dv.size else None,
+                         dV_max=float(dv.max()) if dv.size else None)
     cpu = "unknown CPU"
     try:
         with open("/proc/cpuinfo") as fh:
@@ -275,10 +335,11 @@ def cpu_baseline(args, wl, V_gpu, st_gpu):
     except OSError:
         pass
     return dict(value=conv / dt, unit="PF/s", cores=cores, kind="port", per_core=conv / dt / cores,
-                label="own CPU NR (KLU-equivalent), not lightsim2grid", host_cpu=cpu,
+                label="own CPU NR (KLU-style refactor), not lightsim2grid", host_cpu=cpu,
                 host_nproc=os.cpu_count(),
-                sample=f"first {n} of the {S.shape[0]} rows, oracle C NR (min-degree + partial-pivot sparse LU), "
-                       f"{cores} threads, {dt:.1f} s"), parity
+                sample=f"first {n} of the {S.shape[0]} rows, oracle C NR (min-degree order, partial-pivot sparse LU "
+                       f"on each row's first iteration, refactorization on its pivots after), {cores} threads, "
+                       f"{dt:.1f} s"), parity
 
 
 def run_serving(args, torch, capi, syn):
@@ -362,27 +423,38 @@ def main():
     methods = {}
     keep = {}
     warm_info = None
-    for name in args.methods.split(","):
+    names = args.methods.split(",")
+    ws_trained = None
+    for name in names:
         t1 = time.perf_counter()
         ws = None
-        if name == "warm":
-            ws, warm_info = train_warm(args, wl, torch, capi, syn)
-            log(args, "warm-start training", warm_info)
-        methods[name], V, st = run_method(args, name, wl, torch, capi, ws)
+        if name in ("warm", "facade_warm"):
+            if ws_trained is None:
+                ws_trained, warm_info = train_warm(args, wl, torch, capi, syn)
+                log(args, "warm-start training", warm_info)
+            ws = ws_trained
+        if name.startswith("facade_"):
+            if args.workload != "c2":
+                raise SystemExit("the facade methods run the injection sweep (--workload c2)")
+            methods[name], V, st, it = run_facade(args, name, wl, torch, ws)
+            base = methods.get(name[len("facade_"):])
+            if base:
+                methods[name]["vs_capi"] = methods[name]["pfps"] / base["pfps"]
+        else:
+            methods[name], V, st, it = run_method(args, name, wl, torch, capi, ws)
         if name == "warm":
             methods[name]["training"] = warm_info
-        keep[name] = (V, st)
+        keep[name] = (V, st, it)
         log(args, f"{name}: {methods[name]['pfps']:.0f} PF/s ({time.perf_counter() - t1:.1f} s)",
             json.dumps({k: v for k, v in methods[name].items() if k != 'info'}))
     cpu, parity = (None, None)
-    if args.cpu_rows > 0:
-        V, st = keep.get("fp64", (None, None))
-        cpu, parity = cpu_baseline(args, wl, V, st)
+    if args.cpu_seconds > 0:
+        cpu, parity = cpu_baseline(args, wl, keep)
         log(args, "cpu", cpu, parity)
     # every method's answers against the plain FP64 method's (same rows)
     if "fp64" in keep:
-        V64, st64 = keep["fp64"]
-        for k, (Vk, stk) in keep.items():
+        V64, st64, _ = keep["fp64"]
+        for k, (Vk, stk, _) in keep.items():
             if k == "fp64":
                 continue
             ok = (stk == 0) & (st64 == 0)

@@ -24,8 +24,10 @@
  *
  * The linear solver is its own: a minimum-degree column order on A+A^T and a
  * left-looking (Gilbert-Peierls) sparse LU with threshold partial pivoting
- * that prefers the diagonal, KLU's method.  Rows of a batch run on POSIX
- * threads.
+ * that prefers the diagonal, KLU's method; optionally (the CPU baseline's
+ * setting) a row's later iterations refactorize numerically on its first
+ * factorization's pivots and pattern, as KLU's klu_refactor does.  Rows of a
+ * batch run on POSIX threads.
  *
  * Status codes: 0 Converged, 1 MaxIter, 2 Diverged (row_status.hpp:33-43).
  */
@@ -426,6 +428,46 @@ static int orc_lu_factor(orc_lu *f, const double *Jx)
     return 0;
 }
 
+/* Numeric refactorization on the last factorization's pivots and L / U
+ * pattern (KLU's klu_refactor: no reach, no pivot search).  The U entries
+ * of a column are stored in the reach's topological order, so applying
+ * them in stored order is the left-looking triangular solve.  Returns -1
+ * (the caller factorizes afresh) when a pivot is zero, non-finite, or under
+ * 1e-3 of its column's largest L entry, the full factorization's threshold. */
+static int orc_lu_refactor(orc_lu *f, const double *Jx)
+{
+    const int d = f->d;
+    double *x = f->x;  /* indexed by pivot position, all zero between columns */
+    for (int k = 0; k < d; ++k) {
+        const int col = f->q[k];
+        for (int p = f->Ap[col]; p < f->Ap[col + 1]; ++p) x[f->pinv[f->Ai[p]]] = Jx[f->Amap[p]];
+        const int pd = f->Up[k + 1] - 1;  /* the diagonal, last */
+        for (int t = f->Up[k]; t < pd; ++t) {
+            const int kk = f->Ui[t];
+            const double xr = x[kk];
+            x[kk] = 0.0;
+            f->Ux[t] = xr;
+            for (int t2 = f->Lp[kk] + 1; t2 < f->Lp[kk + 1]; ++t2) x[f->Li[t2]] -= f->Lx[t2] * xr;
+        }
+        const double piv = x[k];
+        x[k] = 0.0;
+        double lmax = 0.0;
+        for (int t = f->Lp[k] + 1; t < f->Lp[k + 1]; ++t) lmax = fmax(lmax, fabs(x[f->Li[t]]));
+        if (!(fabs(piv) > 0.0) || !isfinite(piv) || fabs(piv) < 1e-3 * lmax) {
+            for (int t = f->Lp[k] + 1; t < f->Lp[k + 1]; ++t) x[f->Li[t]] = 0.0;
+            for (int kk = k + 1; kk < d; ++kk) x[kk] = 0.0;
+            return -1;
+        }
+        f->Ux[pd] = piv;
+        for (int t = f->Lp[k] + 1; t < f->Lp[k + 1]; ++t) {
+            const int r = f->Li[t];
+            f->Lx[t] = x[r] / piv;
+            x[r] = 0.0;
+        }
+    }
+    return 0;
+}
+
 /* x = A^-1 b (x, b may alias) */
 static void orc_lu_solve(const orc_lu *f, const double *b, double *x, double *w)
 {
@@ -452,6 +494,7 @@ typedef struct {
     orc_sys s;
     orc_lu lu;
     double *I, *F, *Jx, *dx, *w;
+    int refactor;  /* iterations after a row's first refactor on its pivots */
 } orc_solver;
 
 static void orc_solver_init(orc_solver *o, int n, const int *Yp, const int *Yi, const double *Yx,
@@ -510,7 +553,8 @@ static int orc_solver_run(orc_solver *o, const double *Yx, double *V, const doub
         if (r < tol) return 0;
         if (it >= max_iter) return 1;
         orc_jacobian(s, V, o->I, o->Jx);
-        if (orc_lu_factor(&o->lu, o->Jx) != 0) { *res = NAN; return 2; }
+        if (!(o->refactor && it > 0 && orc_lu_refactor(&o->lu, o->Jx) == 0))
+            if (orc_lu_factor(&o->lu, o->Jx) != 0) { *res = NAN; return 2; }
         orc_lu_solve(&o->lu, o->F, o->dx, o->w);
         orc_scale_step(s, o->dx, max_dva, max_dvm);
         /* theta of pvpq buses, then |V| of pq buses */
@@ -590,6 +634,25 @@ int orc_sparse_solve(int d, const int *Ap, const int *Ai, const double *Ax, cons
     return rc;
 }
 
+/* Factorizes A1 (CSR Ap/Ai/Ax1), refactorizes on its pivots with A2's
+ * values (same pattern) and solves A2 x = b.  Returns the refactorization's
+ * code (-1: a weak pivot, x untouched) or the first factorization's. */
+int orc_sparse_refactor_solve(int d, const int *Ap, const int *Ai, const double *Ax1, const double *Ax2,
+                              const double *b, double *x)
+{
+    orc_lu f;
+    orc_lu_init(&f, d, Ap, Ai);
+    int rc = orc_lu_factor(&f, Ax1);
+    if (rc == 0) rc = orc_lu_refactor(&f, Ax2);
+    if (rc == 0) {
+        double *w = (double *)malloc(sizeof(double) * (d + 1));
+        orc_lu_solve(&f, b, x, w);
+        free(w);
+    }
+    orc_lu_free(&f);
+    return rc;
+}
+
 typedef struct {
     int n; const int *Yp, *Yi; const double *Yx; long long y_stride; /* 0: shared */
     int npv; const int *pv; int npq; const int *pq;
@@ -597,6 +660,7 @@ typedef struct {
     int max_iter; double tol; double max_dva, max_dvm;
     int *status; int *iters; double *res;
     int row0, row1;
+    int refactor;
 } orc_job;
 
 static void *orc_worker(void *arg)
@@ -604,6 +668,7 @@ static void *orc_worker(void *arg)
     orc_job *j = (orc_job *)arg;
     orc_solver o;
     orc_solver_init(&o, j->n, j->Yp, j->Yi, j->Yx, j->npv, j->pv, j->npq, j->pq);
+    o.refactor = j->refactor;
     for (int r = j->row0; r < j->row1; ++r) {
         const double *Yx = j->Yx + (size_t)r * (size_t)j->y_stride;
         j->status[r] = orc_solver_run(&o, Yx, j->V + (size_t)r * (size_t)j->v_stride,
@@ -618,11 +683,13 @@ static void *orc_worker(void *arg)
  * S: [rows, n] complex, or shared when s_stride == 0; Yx: [rows, nY] complex,
  * or shared when y_stride == 0 (strides counted in complex numbers).
  * Runs on n_threads POSIX threads.  max_dva > 0 turns on step scaling
- * with bounds max_dva (angles, rad) and max_dvm (magnitudes, pu). */
+ * with bounds max_dva (angles, rad) and max_dvm (magnitudes, pu).  refactor:
+ * each row's iterations after its first refactorize on the first one's
+ * pivots (KLU-style), falling back to a fresh factorization on a weak pivot. */
 void orc_nr_batch(int n, const int *Yp, const int *Yi, const double *Yx, long long y_stride, int npv,
                   const int *pv, int npq, const int *pq, int n_rows, double *V, const double *S,
                   long long s_stride, int max_iter, double tol, int *status, int *iters, double *res,
-                  int n_threads, double max_dva, double max_dvm)
+                  int n_threads, double max_dva, double max_dvm, int refactor)
 {
     if (n_threads < 1) n_threads = 1;
     if (n_threads > n_rows) n_threads = n_rows > 0 ? n_rows : 1;
@@ -634,7 +701,7 @@ void orc_nr_batch(int n, const int *Yp, const int *Yi, const double *Yx, long lo
         j->npv = npv; j->pv = pv; j->npq = npq; j->pq = pq;
         j->V = V; j->v_stride = 2LL * n; j->S = S; j->s_stride = 2 * s_stride;
         j->max_iter = max_iter; j->tol = tol; j->max_dva = max_dva; j->max_dvm = max_dvm;
-        j->status = status; j->iters = iters; j->res = res;
+        j->status = status; j->iters = iters; j->res = res; j->refactor = refactor;
         j->row0 = (int)((long long)n_rows * t / n_threads);
         j->row1 = (int)((long long)n_rows * (t + 1) / n_threads);
         if (n_threads == 1) orc_worker(j);

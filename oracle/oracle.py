@@ -59,11 +59,13 @@ def lib():
                                   ctypes.c_int, _i32p, _f64p, _f64p, _f64p, ctypes.c_void_p]
         L.orc_sparse_solve.restype = ctypes.c_int
         L.orc_sparse_solve.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+        L.orc_sparse_refactor_solve.restype = ctypes.c_int
+        L.orc_sparse_refactor_solve.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p]
         L.orc_nr_batch.restype = None
         L.orc_nr_batch.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, ctypes.c_longlong,
                                    ctypes.c_int, _i32p, ctypes.c_int, _i32p, ctypes.c_int, _f64p,
                                    _f64p, ctypes.c_longlong, ctypes.c_int, ctypes.c_double, _i32p,
-                                   _i32p, _f64p, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+                                   _i32p, _f64p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         _LIB = L
     return _LIB
 
@@ -123,9 +125,28 @@ def sparse_solve(A, b):
     return x
 
 
+def sparse_refactor_solve(A1, A2, b):
+    """Factorizes A1, refactorizes on its pivots with A2's values (same
+    pattern), returns A2^-1 b; None when the refactorization meets a weak
+    pivot (the NR then factorizes afresh)."""
+    A1 = sp.csr_matrix(A1)
+    A1.sort_indices()
+    A2 = sp.csr_matrix(A2)
+    A2.sort_indices()
+    if not (np.array_equal(A1.indptr, A2.indptr) and np.array_equal(A1.indices, A2.indices)):
+        raise ValueError("A1 and A2 must share the pattern")
+    x = np.empty(A1.shape[0])
+    rc = lib().orc_sparse_refactor_solve(A1.shape[0], A1.indptr.astype(np.int32), A1.indices.astype(np.int32),
+                                         np.ascontiguousarray(A1.data, dtype=np.float64),
+                                         np.ascontiguousarray(A2.data, dtype=np.float64),
+                                         np.ascontiguousarray(b, dtype=np.float64), x)
+    return x if rc == 0 else None
+
+
 def nr_batch(Ybus, V0, Sbus, pv, pq, max_iter=10, tol=1e-8, n_threads=1, Yx_rows=None, max_dva=0.0,
-             max_dvm=0.0):
-    """Plain FP64 NR over rows (step scaling when max_dva > 0).
+             max_dvm=0.0, refactor=False):
+    """Plain FP64 NR over rows (step scaling when max_dva > 0; refactor: a
+    row's later iterations refactorize on its first factorization's pivots).
 
     V0 : (n,) shared start or (rows, n); Sbus : (n,) shared or (rows, n);
     Yx_rows : optional (rows, nY) per-row Ybus values on Ybus's pattern.
@@ -152,7 +173,7 @@ def nr_batch(Ybus, V0, Sbus, pv, pq, max_iter=10, tol=1e-8, n_threads=1, Yx_rows
     Vf = np.ascontiguousarray(V).view(np.float64)
     lib().orc_nr_batch(n, Yp, Yi, _cplx(Yall), y_stride, pv.size, pv, pq.size, pq, rows, Vf,
                        _cplx(np.ascontiguousarray(S)), s_stride, int(max_iter), float(tol), status,
-                       iters, res, int(n_threads), float(max_dva), float(max_dvm))
+                       iters, res, int(n_threads), float(max_dva), float(max_dvm), int(bool(refactor)))
     return Vf.view(np.complex128).reshape(rows, n), status, iters, res
 
 
@@ -307,14 +328,44 @@ def outage_ybus(grid, branch_ids):
     return Y
 
 
+def _pattern_keys(Y):
+    """Sorted CSR Y -> int64 keys row * ncols + col of its entries."""
+    rows = np.repeat(np.arange(Y.shape[0], dtype=np.int64), np.diff(Y.indptr))
+    return rows * Y.shape[1] + Y.indices.astype(np.int64)
+
+
 def ybus_values_on_pattern(Ybase, Y):
     """Values of Y on the (superset) pattern of Ybase, zeros where Y has none."""
     Ybase = sp.csr_matrix(Ybase)
     Ybase.sort_indices()
     Yd = sp.csr_matrix(Y)
+    Yd.sum_duplicates()
+    Yd.sort_indices()
+    kb, ky = _pattern_keys(Ybase), _pattern_keys(Yd)
     out = np.zeros(Ybase.nnz, np.complex128)
-    for i in range(Ybase.shape[0]):
-        cols = Ybase.indices[Ybase.indptr[i]:Ybase.indptr[i + 1]]
-        row = Yd[i].toarray().ravel()
-        out[Ybase.indptr[i]:Ybase.indptr[i + 1]] = row[cols]
+    if ky.size:
+        pos = np.minimum(np.searchsorted(ky, kb), ky.size - 1)
+        hit = ky[pos] == kb
+        out[hit] = Yd.data[pos[hit]]
     return out
+
+
+def outage_ybus_rows(grid, outages):
+    """Per-row Ybus values on grid.Ybus's pattern with each row's branches
+    tripped: the base values minus each tripped branch's four terms (yff at
+    (f, f), yft at (f, t), ytf at (t, f), ytt at (t, t)), the closed form
+    outage_ybus rebuilds from the branch list.  [rows, nY] complex."""
+    Yb = sp.csr_matrix(grid.Ybus)
+    Yb.sort_indices()
+    kb = _pattern_keys(Yb)
+    n = Yb.shape[1]
+    f = np.asarray(grid.branch_from, np.int64)
+    t = np.asarray(grid.branch_to, np.int64)
+    at = [np.searchsorted(kb, a * n + b) for a, b in ((f, f), (f, t), (t, f), (t, t))]
+    val = [grid.yff, grid.yft, grid.ytf, grid.ytt]
+    Y = np.tile(Yb.data.astype(np.complex128), (len(outages), 1))
+    for r, out in enumerate(outages):
+        for l in out:
+            for a, v in zip(at, val):
+                Y[r, a[l]] -= v[l]
+    return Y

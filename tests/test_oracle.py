@@ -82,6 +82,40 @@ def test_nr_converges_quadratically_and_solves_the_equations(grid):
     assert st1[0] == 1 and it1[0] == it[0] - 1 and res1[0] >= 1e-8
 
 
+def test_refactor_on_first_pivots_solves_the_new_matrix():
+    """The KLU-style refactorization: the same pattern with new values, on
+    the first factorization's pivots, solves the new system (scipy's
+    solution); a pivot that collapses under the new values is refused."""
+    rng = np.random.default_rng(3)
+    d = 60
+    A = sp.random(d, d, density=0.08, random_state=4, format="csr") + sp.eye(d) * 5.0
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    A2 = A.copy()
+    A2.data = A.data * (1.0 + 0.3 * rng.standard_normal(A.data.size))
+    b = rng.standard_normal(d)
+    x = orc.sparse_refactor_solve(A, A2, b)
+    np.testing.assert_allclose(x, spla.spsolve(A2.tocsc(), b), rtol=1e-10, atol=1e-12)
+    # the diagonal of the first pivot's row goes to zero: refused
+    A3 = A.copy()
+    i0 = A3.indptr[0] + int(np.flatnonzero(A3.indices[A3.indptr[0]:A3.indptr[1]] == 0)[0])
+    A3.data[i0] = 0.0
+    assert orc.sparse_refactor_solve(A, A3, b) is None
+
+
+def test_nr_with_refactor_matches_fresh_factorizations(grid):
+    """The CPU baseline's NR (refactorizations after a row's first) ends
+    where the fresh-factorization NR ends, row by row."""
+    p, q = syn.injection_scenarios(grid, 64, seed=5)
+    S = (p + 1j * q) / grid.sn_mva
+    V1, st1, it1, _ = orc.nr_batch(grid.Ybus, grid.flat_start(), S, grid.pv, grid.pq, n_threads=4)
+    V2, st2, it2, _ = orc.nr_batch(grid.Ybus, grid.flat_start(), S, grid.pv, grid.pq, n_threads=4, refactor=True)
+    np.testing.assert_array_equal(st1, st2)
+    assert (np.abs(it1 - it2) <= 1).all() and (it1 == it2).mean() > 0.95
+    ok = st1 == 0
+    assert np.abs(V1[ok] - V2[ok]).max() < 1e-8
+
+
 def test_nr_status_rules():
     g = syn.grid_118()
     S = np.stack([g.Sbus, g.Sbus * np.nan, g.Sbus])
@@ -90,6 +124,24 @@ def test_nr_status_rules():
     # already converged at V0: zero iterations
     V2, st2, it2, _ = orc.nr_batch(g.Ybus, V[0], g.Sbus, g.pv, g.pq)
     assert st2[0] == 0 and it2[0] == 0
+
+
+def test_outage_rows_equal_the_rebuilt_ybus():
+    """The oracle's per-row outage values (base minus the tripped branches'
+    terms) equal the Ybus rebuilt from the branch list without them, on the
+    base pattern; and the pattern lookup equals a dense restriction."""
+    for g in (syn.grid_118(), syn.grid_c0()):
+        outs = syn.random_outages(g, 40, max_k=3, seed=9) + [[]]
+        Y = orc.outage_ybus_rows(g, outs)
+        for r, o in enumerate(outs):
+            Yr = orc.outage_ybus(g, o)
+            np.testing.assert_allclose(Y[r], orc.ybus_values_on_pattern(g.Ybus, Yr), rtol=0, atol=1e-12)
+        Yb = sp.csr_matrix(g.Ybus)
+        Yb.sort_indices()
+        D = orc.outage_ybus(g, outs[0]).toarray()
+        rows = np.repeat(np.arange(g.n_bus), np.diff(Yb.indptr))
+        np.testing.assert_array_equal(orc.ybus_values_on_pattern(g.Ybus, orc.outage_ybus(g, outs[0])),
+                                      D[rows, Yb.indices])
 
 
 def test_outage_ybus_closed_form():
