@@ -308,10 +308,19 @@ def cpu_baseline(args, wl, keep):
                            n_threads=cores, Yx_rows=Yrows, refactor=True)
         return out, time.perf_counter() - t0
 
-    pilot = min(S.shape[0], 4 * cores)
-    _, dt_p = run(0, pilot)
-    n = args.cpu_rows or int(args.cpu_seconds * pilot / max(dt_p, 1e-6))
-    n = max(pilot, min(S.shape[0], n))
+    # sized from pilots (a small one carries each thread's set-up, so a
+    # second, larger one sets the rate)
+    n = args.cpu_rows
+    if not n:
+        n = min(S.shape[0], 16 * cores)
+        for _ in range(2):
+            _, dt_p = run(0, n)
+            n_next = max(n, min(S.shape[0], int(args.cpu_seconds * n / max(dt_p, 1e-6))))
+            if n_next <= 2 * n:
+                n = n_next
+                break
+            n = min(n_next, 8 * n)
+    n = max(1, min(S.shape[0], n))
     (Vo, sto, ito, _), dt = run(0, n)
     conv = int((sto == 0).sum())
     parity = dict(rows=n)
