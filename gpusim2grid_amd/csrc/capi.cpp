@@ -734,6 +734,17 @@ void launch_ext_update(g2g_batch_s* b)
     HIPCHK(hipGetLastError());
 }
 
+// Whether a mismatch launch groups each tile's split workgroups on one XCD
+// (G2G_MIS_XCD=0: split-major instead).
+bool mis_xcd_group()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("G2G_MIS_XCD");
+        return !(e && std::atoi(e) == 0);
+    }();
+    return on;
+}
+
 template <typename Real>
 void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double tol, int freeze,
                      double growth = 0.0, bool transposed = false)
@@ -756,8 +767,10 @@ void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double
                          (b->streaming || b->restart_on) ? b->it0 : nullptr, b->mask,
                          b->restart_on ? b->budget : nullptr, b->restart_on ? b->warm : nullptr,
                          splits, (splits > 1 || ext) ? b->mis_part : nullptr, part_stride,
-                         ext ? b->sx : nullptr, static_cast<int>(P.xt.xb_bus.size())};
-    const dim3 grid(T * splits), block(kLanes * kMismatchWaves);
+                         ext ? b->sx : nullptr, static_cast<int>(P.xt.xb_bus.size()), T,
+                         splits > 1 && mis_xcd_group() ? 1 : 0};
+    const int grid_tiles = a.xcd_group ? (T + kXcds - 1) / kXcds * kXcds : T;
+    const dim3 grid(grid_tiles * splits), block(kLanes * kMismatchWaves);
     if (b->per_row_y) {
         if (with_j) k_mismatch<Real, true, true><<<grid, block, 0, b->stream>>>(a);
         else k_mismatch<Real, true, false><<<grid, block, 0, b->stream>>>(a);

@@ -45,6 +45,8 @@ struct MismatchArgs {
     int part_stride;            // the launch's tiles x 64
     const double2* sx;          // nullable: extension buses' residual terms [T][n_xb][64]
     int n_xb;
+    int tiles;                  // tiles of the launch
+    int xcd_group;              // 1: a tile's split workgroups grouped on one XCD
 };
 
 // Extension rows (distributed slack, HVDC droop, voltage control): one lane

@@ -197,12 +197,26 @@ __global__ __launch_bounds__(kLanes* kMismatchWaves) void k_mismatch(MismatchArg
     const int lane = threadIdx.x & (kLanes - 1);
     // wave-uniform, so the bus and entry tables are read with scalar loads
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kLanes));
-    // splits workgroups per tile, split-major: split s of tile t is workgroup
-    // s * T + t and takes every splits-th group of kMismatchWaves buses
+    // splits workgroups per tile; split s takes every splits-th group of
+    // kMismatchWaves buses.  The launch's workgroups go to the XCDs round
+    // robin (workgroup b to XCD b % 8); with xcd_group, the splits of tile t
+    // are workgroups ((t / 8) splits + s) 8 + t % 8, so they run on one XCD
+    // at about the same time and the neighbour V they gather is shared in its
+    // L2.  Else split-major: split s of tile t is workgroup s T + t.
     const int splits = a.splits > 1 ? a.splits : 1;
-    const int T = static_cast<int>(gridDim.x) / splits;
-    const int split = static_cast<int>(blockIdx.x) / T;
-    const size_t tile = blockIdx.x - static_cast<size_t>(split) * T;
+    int split;
+    size_t tile;
+    if (a.xcd_group) {
+        const int b = static_cast<int>(blockIdx.x);
+        const int q = b / kXcds;
+        split = q % splits;
+        tile = static_cast<size_t>(q / splits) * kXcds + b % kXcds;
+        if (tile >= static_cast<size_t>(a.tiles)) return;  // the grid is padded to whole XCD groups
+    } else {
+        const int T = static_cast<int>(gridDim.x) / splits;
+        split = static_cast<int>(blockIdx.x) / T;
+        tile = blockIdx.x - static_cast<size_t>(split) * T;
+    }
     const int stride = kMismatchWaves * splits;
     __shared__ double s_res[kMismatchWaves][kLanes];
     __shared__ double s_jmax[kMismatchWaves][kLanes];

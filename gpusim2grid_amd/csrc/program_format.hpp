@@ -126,6 +126,9 @@ __host__ __device__ constexpr int split_ring_bytes(int real_size, int RW)
            kProgChunk * kProgSlots * 4 + kWalkFlagBytes;
 }
 
+// XCDs of the device: a launch's workgroups go to them round robin
+constexpr int kXcds = 8;
+
 // mismatch program: ints per bus record
 constexpr int kK1BusWords = 8;
 
