@@ -23,7 +23,7 @@ template <typename Real, bool PerRowY, bool WithJ>
 __global__ void k_mismatch(MismatchArgs<Real> a);
 template <typename Real, bool WithJ>
 __global__ void k_mis_status(MismatchArgs<Real> a);
-template <typename Real, int W, bool SPLIT>
+template <typename Real, int W, bool SPLIT, int RG>
 __global__ void k_factor(FactorArgs<Real> a);
 template <typename Real, int W, bool SPLIT>
 __global__ void k_backward(BackwardArgs<Real> a);
@@ -409,11 +409,34 @@ constexpr int ring_slots_of() { return sizeof(Real) == sizeof(double) ? kRingSlo
 // (plus the zero and trash slots) and each walker's long-column scratch rows
 // (kernels.hip's LDS layout)
 template <typename Real>
-size_t factor_lds(const SymbolicLU& s)
+size_t factor_lds(const SymbolicLU& s, int rg = 0)
 {
     const int scratch = s.max_c > kFastC ? 2 * s.max_c * s.walkers : 0;
-    return static_cast<size_t>(walk_ring_bytes(sizeof(Real), s.walkers)) +
+    return static_cast<size_t>(walk_ring_bytes(sizeof(Real), s.walkers, rg > 0 ? rg : s.walkers)) +
            static_cast<size_t>(s.slots_f + 2 + scratch) * kLanes * sizeof(Real);
+}
+
+// The factorization's ring geometry: its own walkers' (0), or, for two FP32
+// walkers whose slots do not fit beside full rings, half-size chunks
+// (G2G_HALF_RINGS=1 takes them whenever they fit).
+template <typename Real>
+int factor_ring_geometry(const SymbolicLU& s, size_t limit)
+{
+    if (sizeof(Real) != 4 || s.walkers != 2) return 0;
+    static const int force = [] {
+        const char* e = std::getenv("G2G_HALF_RINGS");
+        return e ? std::atoi(e) : -1;
+    }();
+    if (force == 0) return 0;
+    const bool half_fits = factor_lds<Real>(s, kHalfRingGeometry) <= limit;
+    if (force == 1) return half_fits ? kHalfRingGeometry : 0;
+    return factor_lds<Real>(s) > limit && half_fits ? kHalfRingGeometry : 0;
+}
+
+template <typename Real>
+size_t factor_lds_fit(const SymbolicLU& s, size_t limit)
+{
+    return factor_lds<Real>(s, factor_ring_geometry<Real>(s, limit));
 }
 
 template <typename Real>
@@ -438,7 +461,7 @@ const g2g_plan_s* plan_fitted(const g2g_plan_s* P, bool fp64)
 {
     const size_t limit = lds_limit();
     auto fits = [&](const SymbolicLU& s) {
-        const size_t f = fp64 ? factor_lds<double>(s) : factor_lds<float>(s);
+        const size_t f = fp64 ? factor_lds_fit<double>(s, limit) : factor_lds_fit<float>(s, limit);
         const size_t b = fp64 ? backward_lds<double>(s) : backward_lds<float>(s);
         return f <= limit && b <= limit;
     };
@@ -460,7 +483,7 @@ g2g_plan_s* plan_fitting(g2g_plan_s* P, bool fp64)
 {
     const size_t limit = lds_limit();
     auto fits = [&](const SymbolicLU& s) {
-        const size_t f = fp64 ? factor_lds<double>(s) : factor_lds<float>(s);
+        const size_t f = fp64 ? factor_lds_fit<double>(s, limit) : factor_lds_fit<float>(s, limit);
         const size_t b = fp64 ? backward_lds<double>(s) : backward_lds<float>(s);
         return f <= limit && b <= limit;
     };
@@ -505,15 +528,29 @@ void launch_factor(const g2g_plan_s* P, const Real* A, Real* U, int a_stride, in
                    const double* jmax, int* clamped, int* error, int rows, int T, double eps, hipStream_t st)
 {
     const SymbolicLU& s = P->lu();
-    const size_t lds = factor_lds<Real>(s);
-    if (lds > lds_limit()) throw std::runtime_error("factorization needs " + std::to_string(lds) + " bytes of LDS");
+    const size_t limit = lds_limit();
+    const int rg = factor_ring_geometry<Real>(s, limit);
+    const size_t lds = factor_lds<Real>(s, rg);
+    if (lds > limit) throw std::runtime_error("factorization needs " + std::to_string(lds) + " bytes of LDS");
     FactorArgs<Real> a{A,     U,    P->d_prog_f, tile_active, jmax,    clamped, error,  s.d, a_stride, u_stride,
                        P->prog_f_stride, s.slots_f, rows, s.max_c, eps, P->d_seg, T, 0, nullptr, 0};
-    const void* fn = s.walkers == 2 ? reinterpret_cast<const void*>(&k_factor<Real, 2, false>)
-                                    : reinterpret_cast<const void*>(&k_factor<Real, 1, false>);
+    const void* fn = nullptr;
+    if constexpr (sizeof(Real) == 4) {
+        if (rg == kHalfRingGeometry) fn = reinterpret_cast<const void*>(&k_factor<Real, 2, false, kHalfRingGeometry>);
+    }
+    if (!fn)
+        fn = s.walkers == 2 ? reinterpret_cast<const void*>(&k_factor<Real, 2, false, 0>)
+                            : reinterpret_cast<const void*>(&k_factor<Real, 1, false, 0>);
     HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    if (s.walkers == 2) k_factor<Real, 2, false><<<T, kLanes * walk_waves(2), lds, st>>>(a);
-    else k_factor<Real, 1, false><<<T, kLanes * walk_waves(1), lds, st>>>(a);
+    if constexpr (sizeof(Real) == 4) {
+        if (rg == kHalfRingGeometry) {
+            k_factor<Real, 2, false, kHalfRingGeometry><<<T, kLanes * walk_waves(2), lds, st>>>(a);
+            HIPCHK(hipGetLastError());
+            return;
+        }
+    }
+    if (s.walkers == 2) k_factor<Real, 2, false, 0><<<T, kLanes * walk_waves(2), lds, st>>>(a);
+    else k_factor<Real, 1, false, 0><<<T, kLanes * walk_waves(1), lds, st>>>(a);
     HIPCHK(hipGetLastError());
 }
 
@@ -569,7 +606,7 @@ void launch_factor_split(const g2g_plan_s* P, const Real* A, Real* U, int a_stri
                                                                                        static_cast<long long>(T) * kLanes);
         HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_factor<Real, 1, true>),
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_factor<Real, 1, true, 0>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     const int levels[3][2] = {{0, s.split_fine}, {s.split_fine, s.split_mid}, {s.split_fine + s.split_mid, 1}};
     for (const auto& lv : levels) {
@@ -577,7 +614,7 @@ void launch_factor_split(const g2g_plan_s* P, const Real* A, Real* U, int a_stri
         FactorArgs<Real> a{A,   U,        P->d_prog_fs, tile_active, jmax, clamped, error, s.d, a_stride,
                            u_stride, P->prog_fs_stride, s.slots_fs, rows, s.max_c, eps, P->d_split_seg,
                            T,   lv[0],    pend,         s.pend_len};
-        k_factor<Real, 1, true><<<T * lv[1], kLanes * (2 + kSplitProducers), lds, st>>>(a);
+        k_factor<Real, 1, true, 0><<<T * lv[1], kLanes * (2 + kSplitProducers), lds, st>>>(a);
         HIPCHK(hipGetLastError());
     }
 }

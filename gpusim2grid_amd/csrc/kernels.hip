@@ -952,11 +952,11 @@ __host__ __device__ constexpr int factor_waves()
     return SPLIT ? 2 + kSplitProducers : walk_waves(W);
 }
 
-template <typename Real, int W, bool SPLIT>
+template <typename Real, int W, bool SPLIT, int RG>
 __global__ __launch_bounds__((kLanes * factor_waves<W, SPLIT>())) void k_factor(FactorArgs<Real> a)
 {
     static_assert(!SPLIT || W == 1, "a split-walk workgroup has one walker");
-    constexpr int RW = SPLIT ? kSplitRingF : W;  // ring geometry
+    constexpr int RW = SPLIT ? kSplitRingF : RG > 0 ? RG : W;  // ring geometry (RG: another walker count's)
     constexpr int kCh = value_chunk(sizeof(Real), RW), kNch = value_ring_slots(sizeof(Real), RW);
     static_assert((kNch - 2) * kCh >= 6 + 4 * kFastC2 - 1, "the ring window spans a two-column step's values");
     using ARing = Ring<Real, kCh, kNch, false, true>;
@@ -2125,12 +2125,13 @@ template __global__ void k_slots_to_rows<double2>(const double2*, int, int, cons
 template __global__ void k_move_vals<int>(const int*, int*, const int*, int, int, int);
 template __global__ void k_move_vals<double>(const double*, double*, const int*, int, int, int);
 template __global__ void k_move_vals<unsigned char>(const unsigned char*, unsigned char*, const int*, int, int, int);
-template __global__ void k_factor<double, 1, false>(FactorArgs<double>);
-template __global__ void k_factor<float, 1, false>(FactorArgs<float>);
-template __global__ void k_factor<double, 2, false>(FactorArgs<double>);
-template __global__ void k_factor<float, 2, false>(FactorArgs<float>);
-template __global__ void k_factor<double, 1, true>(FactorArgs<double>);
-template __global__ void k_factor<float, 1, true>(FactorArgs<float>);
+template __global__ void k_factor<double, 1, false, 0>(FactorArgs<double>);
+template __global__ void k_factor<float, 1, false, 0>(FactorArgs<float>);
+template __global__ void k_factor<double, 2, false, 0>(FactorArgs<double>);
+template __global__ void k_factor<float, 2, false, 0>(FactorArgs<float>);
+template __global__ void k_factor<float, 2, false, kHalfRingGeometry>(FactorArgs<float>);
+template __global__ void k_factor<double, 1, true, 0>(FactorArgs<double>);
+template __global__ void k_factor<float, 1, true, 0>(FactorArgs<float>);
 template __global__ void k_backward<double, 1, false>(BackwardArgs<double>);
 template __global__ void k_backward<float, 1, false>(BackwardArgs<float>);
 template __global__ void k_backward<double, 2, false>(BackwardArgs<double>);

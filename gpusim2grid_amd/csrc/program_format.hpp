@@ -104,12 +104,17 @@ __host__ __device__ constexpr int value_ring_slots(int real_size, int W)
 {
     return W == 1 ? (real_size == 8 ? kRingSlotsF64 : kRingSlotsF32) : 4;
 }
-__host__ __device__ constexpr int walk_ring_bytes(int real_size, int W)
+// (the rings of W walkers streaming with the geometry of RW walkers)
+__host__ __device__ constexpr int walk_ring_bytes(int real_size, int W, int RW)
 {
-    return W * (kWalkerCtlBytes + value_chunk(real_size, W) * value_ring_slots(real_size, W) * kLanes * real_size +
+    return W * (kWalkerCtlBytes + value_chunk(real_size, RW) * value_ring_slots(real_size, RW) * kLanes * real_size +
                 kProgChunk * kProgSlots * 4) +
            kWalkFlagBytes;
 }
+__host__ __device__ constexpr int walk_ring_bytes(int real_size, int W) { return walk_ring_bytes(real_size, W, W); }
+// two FP32 factorization walkers may stream half-size chunks (the four
+// walkers' geometry: 16 KB rings), for slot sets too large beside full rings
+constexpr int kHalfRingGeometry = 4;
 
 // split walk: sets per tile (from kSplitMinD unknowns up), the walker
 // workgroup's value producers, and its ring geometry (as for RW walkers): the
