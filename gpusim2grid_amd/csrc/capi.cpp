@@ -62,7 +62,7 @@ __global__ void k_restart_v(const int* status, const unsigned char* warm, int n,
                             double2* V);
 __global__ void k_v_uniform(const double2* V, int rows, int n, int* differs);
 __global__ void k_restart_rows(int* status, unsigned char* warm, const int* it0, int* budget, int rows, int iter,
-                               int max_iter, int* requeued);
+                               int max_iter, int* requeued, int* tile_restart);
 template <typename Real>
 __global__ void k_gather_astream(const double* vals, long long ldv, const double* rhs, long long ldb, int rows,
                                  int nnz, int d, const int* a_pos, const int* f_pos, int a_len, Real* A);
@@ -298,6 +298,7 @@ struct g2g_batch_s {
     // warm solves: the caller's start voltages (the fallback start)
     double2* V0 = nullptr;
     bool v0_row = false;  // V0 holds one row [n] (every row's fallback start is the same)
+    int* tile_restart = nullptr;  // [T] tiles with a row restarted at this iteration
     int* differs = nullptr;
     // step control (settings of the next solve)
     int ir_steps = 0;                 // refinement steps of every Newton step
@@ -357,7 +358,7 @@ struct g2g_batch_s {
     {
         dfree(V); dfree(S); dfree(Yrow); dfree(A); dfree(U); dfree(X); dfree(res); dfree(jmax); dfree(mis_part);
         dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped); dfree(error);
-        dfree(V0); dfree(Bs); dfree(Xs); dfree(wmask); dfree(differs);
+        dfree(V0); dfree(Bs); dfree(Xs); dfree(wmask); dfree(differs); dfree(tile_restart);
         dfree(Vk); dfree(Sk); dfree(Yk); dfree(stk); dfree(itk); dfree(resk); dfree(mapk); dfree(it0);
         dfree(mask); dfree(maskk); dfree(xs); dfree(xsk); dfree(sx); dfree(pend);
         dfree(budget); dfree(warm); dfree(requeued); dfree(V0k); dfree(it0k); dfree(budgetk); dfree(warmk);
@@ -782,9 +783,11 @@ bool mis_xcd_group()
     return on;
 }
 
+// only (nullable): evaluate only the tiles with only[tile] > 0; their
+// counts go to a spare counter (the caller keeps the full pass's)
 template <typename Real>
 void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double tol, int freeze,
-                     double growth = 0.0, bool transposed = false)
+                     double growth = 0.0, bool transposed = false, const int* only = nullptr)
 {
     const NrPlan& P = b->plan->nr;
     const bool ext = b->plan->ext();
@@ -799,13 +802,14 @@ void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double
     const int* k1e = b->split_layout ? b->plan->d_k1_ent_s : transposed ? b->plan->d_k1_ent_t : b->plan->d_k1_ent;
     MismatchArgs<Real> a{k1b, k1e, P.k1_nbus, b->plan->d_Y, b->Yrow, b->V, b->S,
                          static_cast<Real*>(b->A), P.n, P.nY, b->a_stride, b->res, b->jmax, b->status, b->iters,
-                         b->tile_active, b->n_active, b->active_rows, iter, max_iter, tol, freeze, growth,
+                         b->tile_active, only ? b->n_active + 2 : b->n_active, b->active_rows, iter, max_iter, tol,
+                         freeze, growth,
                          growth > 0.0 ? b->contraction : 0.0,
                          (b->streaming || b->restart_on) ? b->it0 : nullptr, b->mask,
                          b->restart_on ? b->budget : nullptr, b->restart_on ? b->warm : nullptr,
                          splits, (splits > 1 || ext) ? b->mis_part : nullptr, part_stride,
                          ext ? b->sx : nullptr, static_cast<int>(P.xt.xb_bus.size()), T,
-                         splits > 1 && mis_xcd_group() ? 1 : 0};
+                         splits > 1 && mis_xcd_group() ? 1 : 0, only};
     const int grid_tiles = a.xcd_group ? (T + kXcds - 1) / kXcds * kXcds : T;
     const dim3 grid(grid_tiles * splits), block(kLanes * kMismatchWaves);
     if (b->per_row_y) {
@@ -1086,24 +1090,25 @@ int nr_loop_body(g2g_batch_s* b, int max_iter, double tol, int freeze, double gr
             const NrPlan& nr = b->plan->nr;
             k_restart_v<<<dim3(live_tiles(b), kRestartChunks), 256, 0, b->stream>>>(b->status, b->warm, nr.n, b->V0,
                                                                                   b->v0_row ? 1 : 0, b->V);
+            k_fill_int<<<blocks_for(live_tiles(b), 256), 256, 0, b->stream>>>(b->tile_restart, 0, live_tiles(b));
             k_restart_rows<<<blocks_for(b->active_rows, 256), 256, 0, b->stream>>>(
-                b->status, b->warm, b->it0, b->budget, b->active_rows, it, b->restart_max_iter, b->requeued);
+                b->status, b->warm, b->it0, b->budget, b->active_rows, it, b->restart_max_iter, b->requeued,
+                b->tile_restart);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(b->h_active + 2, b->requeued, sizeof(int), hipMemcpyDeviceToHost, b->stream));
         }
         HIPCHK(hipMemcpyAsync(b->h_active, b->n_active, 2 * sizeof(int), hipMemcpyDeviceToHost, b->stream));
         HIPCHK(hipStreamSynchronize(b->stream));
         if (b->restart_on && b->h_active[2] > restarts) {
-            // evaluate again so the restarted rows' J and F (at their new
-            // start) are in the A stream; the other rows' decisions were made
-            // (no growth test this time) and come out the same
+            // evaluate the tiles of the restarted rows again, so their J and
+            // F at the new start are in the A stream; the other rows of those
+            // tiles come out as before (no rejection test this time), and the
+            // counts stay the first pass's (a row marked for a restart counted
+            // as iterating)
             restarts = b->h_active[2];
-            k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 2);
             e0 = tm.mark();
-            launch_mismatch<Real>(b, true, it, max_iter, tol, freeze, 0.0);
+            launch_mismatch<Real>(b, true, it, max_iter, tol, freeze, 0.0, false, b->tile_restart);
             tm.ph[0].push_back({e0, tm.mark()});
-            HIPCHK(hipMemcpyAsync(b->h_active, b->n_active, 2 * sizeof(int), hipMemcpyDeviceToHost, b->stream));
-            HIPCHK(hipStreamSynchronize(b->stream));
         }
         if (*b->h_active == 0) break;
         if (worth_compacting(b, b->h_active[0], b->h_active[1])) {
@@ -1315,6 +1320,7 @@ void solve_warm_impl(g2g_batch_s* b, g2g_mlp_s* mlp, const unsigned char* mask, 
         b->budget = dalloc<int>(rp);
         b->warm = dalloc<unsigned char>(rp);
         b->requeued = dalloc<int>(1);
+        b->tile_restart = dalloc<int>(b->T);
     }
     if (!b->it0) b->it0 = dalloc<int>(rp);
     DeviceScratch tmp;
@@ -1898,7 +1904,7 @@ int g2g_batch_create(g2g_plan_t P, int32_t rows, int32_t precision, int32_t per_
         b->iters = dalloc<int>(rp);
         b->clamped = dalloc<int>(rp);
         b->tile_active = dalloc<int>(b->T);
-        b->n_active = dalloc<int>(2);
+        b->n_active = dalloc<int>(4);  // rows, tiles; a partial pass's spare pair
         b->error = dalloc<int>(1);
         HIPCHK(hipMemsetAsync(b->error, 0, sizeof(int), b->stream));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->h_active), 4 * sizeof(int), hipHostMallocDefault));

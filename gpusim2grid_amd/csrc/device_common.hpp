@@ -47,6 +47,7 @@ struct MismatchArgs {
     int n_xb;
     int tiles;                  // tiles of the launch
     int xcd_group;              // 1: a tile's split workgroups grouped on one XCD
+    const int* only;            // nullable: evaluate only the tiles with only[tile] > 0
 };
 
 // Extension rows (distributed slack, HVDC droop, voltage control): one lane

@@ -221,6 +221,9 @@ __global__ __launch_bounds__(kLanes* kMismatchWaves) void k_mismatch(MismatchArg
     __shared__ double s_res[kMismatchWaves][kLanes];
     __shared__ double s_jmax[kMismatchWaves][kLanes];
 
+    // a pass over some tiles only (the rows restarted at this iteration):
+    // the others keep what the iteration's evaluation wrote
+    if (a.only && a.only[tile] <= 0) return;
     double rmax = 0.0, jmax = 0.0;
     if (a.tile_active[tile] > 0) {
         // The waves take the buses round-robin in the program's (RCM) order,
@@ -329,7 +332,7 @@ __global__ __launch_bounds__(kLanes) void k_mis_status(MismatchArgs<Real> a)
 {
     const int lane = threadIdx.x;
     const size_t tile = blockIdx.x;
-    if (a.tile_active[tile] <= 0) return;
+    if (a.tile_active[tile] <= 0 || (a.only && a.only[tile] <= 0)) return;
     const size_t row = tile * kLanes + lane;
     double r = 0.0, jm = 0.0;
     for (int s = 0; s < a.splits; ++s) {
@@ -1670,7 +1673,8 @@ __global__ __launch_bounds__(256) void k_v_uniform(const double2* __restrict__ V
 // ... then continue from it in the same iteration: the fallback gets max_iter
 // updates on top of the ones the row had, and the row counts as re-queued.
 __global__ void k_restart_rows(int* __restrict__ status, unsigned char* __restrict__ warm, const int* __restrict__ it0,
-                               int* __restrict__ budget, int rows, int iter, int max_iter, int* __restrict__ requeued)
+                               int* __restrict__ budget, int rows, int iter, int max_iter, int* __restrict__ requeued,
+                               int* __restrict__ tile_restart)
 {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= rows || status[row] != kRejected || !warm[row]) return;
@@ -1678,6 +1682,7 @@ __global__ void k_restart_rows(int* __restrict__ status, unsigned char* __restri
     warm[row] = 0;
     status[row] = kActive;
     atomicAdd(requeued, 1);
+    tile_restart[row / kLanes] = 1;  // the evaluation at the new start covers this tile
 }
 
 // per-slot scalars: v[slots[r]] = value
