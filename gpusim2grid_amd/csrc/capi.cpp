@@ -298,6 +298,10 @@ struct g2g_batch_s {
     // warm solves: the caller's start voltages (the fallback start)
     double2* V0 = nullptr;
     bool v0_row = false;  // V0 holds one row [n] (every row's fallback start is the same)
+    // V as last set: every row the same vector, kept in v_row [n] (the
+    // calls that change V clear the flag)
+    bool v_bcast = false;
+    double2* v_row = nullptr;
     int* tile_restart = nullptr;  // [T] tiles with a row restarted at this iteration
     int* differs = nullptr;
     // step control (settings of the next solve)
@@ -358,7 +362,7 @@ struct g2g_batch_s {
     {
         dfree(V); dfree(S); dfree(Yrow); dfree(A); dfree(U); dfree(X); dfree(res); dfree(jmax); dfree(mis_part);
         dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped); dfree(error);
-        dfree(V0); dfree(Bs); dfree(Xs); dfree(wmask); dfree(differs); dfree(tile_restart);
+        dfree(V0); dfree(Bs); dfree(Xs); dfree(wmask); dfree(differs); dfree(tile_restart); dfree(v_row);
         dfree(Vk); dfree(Sk); dfree(Yk); dfree(stk); dfree(itk); dfree(resk); dfree(mapk); dfree(it0);
         dfree(mask); dfree(maskk); dfree(xs); dfree(xsk); dfree(sx); dfree(pend);
         dfree(budget); dfree(warm); dfree(requeued); dfree(V0k); dfree(it0k); dfree(budgetk); dfree(warmk);
@@ -1116,6 +1120,7 @@ int nr_loop_body(g2g_batch_s* b, int max_iter, double tol, int freeze, double gr
             if (compacted) uncompact_rows(b);
             compact_rows(b);
             compacted = true;
+            b->h_active[1] = (b->h_active[0] + kLanes - 1) / kLanes;  // the packed rows' tiles
             // the A stream was written in the old row order: evaluate again
             // (same V, same decisions; no rejection test) in the compacted one
             k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 2);
@@ -1330,7 +1335,12 @@ void solve_warm_impl(g2g_batch_s* b, g2g_mlp_s* mlp, const unsigned char* mask, 
     const int e_begin = tm.mark();
     hipStream_t s = b->stream;
     // the fallback start: one row when every row's V is the same (a
-    // broadcast start), else the whole V
+    // broadcast start, known from g2g_batch_set_v or checked), else the whole V
+    if (b->v_bcast) {
+        b->v0_row = true;
+        HIPCHK(hipMemcpyAsync(b->V0, b->v_row, static_cast<size_t>(nr.n) * sizeof(double2), hipMemcpyDeviceToDevice,
+                              s));
+    } else {
     if (!b->differs) b->differs = dalloc<int>(1);
     k_fill_int<<<1, 64, 0, s>>>(b->differs, 0, 1);
     const long long vt = (b->active_rows + kLanes - 1) / kLanes;
@@ -1343,6 +1353,8 @@ void solve_warm_impl(g2g_batch_s* b, g2g_mlp_s* mlp, const unsigned char* mask, 
     b->v0_row = differs == 0;
     if (b->v0_row) k_tiles_to_rows<double2><<<blocks_for(nr.n, 256), 256, 0, s>>>(b->V, 1, nr.n, b->V0, nr.n);
     else HIPCHK(hipMemcpyAsync(b->V0, b->V, nv * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    }
+    b->v_bcast = false;  // the solve changes V
     mlp_warm_start_tiles(mlp, b->S, b->active_rows, nr.n, d_mask, n_mask, b->V, s);
     tm.ph[3].push_back({e_begin, tm.mark()});
     k_fill_int<<<blocks_for(rp, 256), 256, 0, s>>>(b->it0, 0, rp);
@@ -1923,7 +1935,15 @@ int g2g_batch_set_v(g2g_batch_t b, const double* V, int64_t ld, int32_t rows, in
     return guarded([&] {
         if (!b || !V) throw std::invalid_argument("null argument");
         check_rows(b, rows, row0);
-        to_tiles(V, ld, rows, b->plan->nr.n, row0, on_device, b->V, b->stream);
+        const int n = b->plan->nr.n;
+        to_tiles(V, ld, rows, n, row0, on_device, b->V, b->stream);
+        b->v_bcast = ld == 0 && row0 == 0 && rows == b->rows;
+        if (b->v_bcast) {  // keep the one row (a later warm solve's fallback start)
+            if (!b->v_row) b->v_row = dalloc<double2>(static_cast<size_t>(n));
+            HIPCHK(hipMemcpyAsync(b->v_row, V, static_cast<size_t>(n) * sizeof(double2),
+                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, b->stream));
+            if (!on_device) HIPCHK(hipStreamSynchronize(b->stream));
+        }
     });
 }
 
@@ -1995,6 +2015,7 @@ int g2g_batch_set_step_control(g2g_batch_t b, int32_t ir_steps, int32_t scaling,
 int g2g_batch_solve(g2g_batch_t b, int32_t max_iter, double tol, int32_t scheduling, g2g_solve_stats* stats)
 {
     return guarded([&] {
+        if (b) b->v_bcast = false;  // V changes (or is handed out)
         if (!b) throw std::invalid_argument("null batch");
         if (max_iter < 0) throw std::invalid_argument("max_iter must be >= 0");
         if (!(tol > 0.0)) throw std::invalid_argument("tol must be positive");
@@ -2049,6 +2070,7 @@ int g2g_batch_set_warm_contraction(g2g_batch_t b, double q)
 int g2g_batch_warm_start(g2g_batch_t b, g2g_mlp_t mlp, const uint8_t* mask, int32_t n_mask)
 {
     return guarded([&] {
+        if (b) b->v_bcast = false;  // V changes (or is handed out)
         if (!b || !mlp) throw std::invalid_argument("null argument");
         if (n_mask < 0 || (n_mask > 0 && !mask)) throw std::invalid_argument("mask missing");
         DeviceScratch tmp;
@@ -2419,6 +2441,7 @@ int g2g_batch_set_bus_mask(g2g_batch_t b, const int32_t* row_ptr, const int32_t*
 int g2g_batch_stream_begin(g2g_batch_t b)
 {
     return guarded([&] {
+        if (b) b->v_bcast = false;  // V changes (or is handed out)
         if (!b) throw std::invalid_argument("null batch");
         if (b->mask) throw std::invalid_argument("streamed rows take no bus mask: clear it first");
         if (b->plan->ext()) throw std::invalid_argument("streamed rows are not available with extension rows");
@@ -2465,6 +2488,7 @@ int g2g_batch_stream_fill(g2g_batch_t b, int32_t n, const int32_t* slots, const 
                           const int32_t* edit_ent, const double* edit_delta)
 {
     return guarded([&] {
+        if (b) b->v_bcast = false;  // V changes (or is handed out)
         if (!b || (n > 0 && (!slots || !S || !V0))) throw std::invalid_argument("null argument");
         if (!b->streaming) throw std::invalid_argument("call g2g_batch_stream_begin first");
         if (n <= 0) return;
@@ -2514,6 +2538,7 @@ int g2g_batch_stream_fill(g2g_batch_t b, int32_t n, const int32_t* slots, const 
 int g2g_batch_stream_round(g2g_batch_t b, int32_t k, int32_t max_iter, double tol)
 {
     return guarded([&] {
+        if (b) b->v_bcast = false;  // V changes (or is handed out)
         if (!b) throw std::invalid_argument("null batch");
         if (!b->streaming) throw std::invalid_argument("call g2g_batch_stream_begin first");
         if (k <= 0 || max_iter < 0) throw std::invalid_argument("k must be positive and max_iter >= 0");
@@ -2527,6 +2552,7 @@ int g2g_batch_stream_warm(g2g_batch_t b, g2g_mlp_t mlp, int32_t n, const int32_t
                           int32_t n_mask)
 {
     return guarded([&] {
+        if (b) b->v_bcast = false;  // V changes (or is handed out)
         if (!b || !mlp || (n > 0 && !slots)) throw std::invalid_argument("null argument");
         if (!b->streaming) throw std::invalid_argument("call g2g_batch_stream_begin first");
         if (n_mask < 0 || (n_mask > 0 && !mask)) throw std::invalid_argument("mask missing");
@@ -2603,6 +2629,7 @@ int g2g_batch_stream_take(g2g_batch_t b, int32_t n, const int32_t* slots, double
 int g2g_batch_device_arrays(g2g_batch_t b, void** V_tiles, void** S_tiles)
 {
     return guarded([&] {
+        if (b) b->v_bcast = false;  // V changes (or is handed out)
         if (!b) throw std::invalid_argument("null batch");
         if (V_tiles) *V_tiles = b->V;
         if (S_tiles) *S_tiles = b->S;

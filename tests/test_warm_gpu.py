@@ -133,7 +133,8 @@ def test_warm_solve_matches_flat_start_oracle(trained, prec):
     assert stats["n_converged"] == rows
 
 
-@pytest.mark.parametrize("start,compact", [("flat", None), ("flat", "1"), ("per_row", None), ("per_row", "1")])
+@pytest.mark.parametrize("start,compact", [("flat", None), ("flat", "1"), ("per_row", None), ("per_row", "1"),
+                                           ("tiled", None)])
 def test_untrained_mlp_falls_back_to_flat_start(start, compact, monkeypatch):
     """Rows whose learned start fails restart from the batch's V: one start
     for every row, or a start per row; with compaction forced, the restarts
@@ -153,7 +154,9 @@ def test_untrained_mlp_falls_back_to_flat_start(start, compact, monkeypatch):
     ws.out_mu = np.tile(np.array([0.0, 1.0], np.float32), g.n_bus)
     ws.out_sigma = np.tile(np.array([1.0, 0.05], np.float32), g.n_bus)
     ws.mlp.set_normalization(ws.in_mu, ws.in_sigma, ws.out_mu, ws.out_sigma)
-    V0 = g.flat_start()
+    V0 = g.flat_start()  # one row: the batch keeps it as its fallback start
+    if start == "tiled":  # the same start written row by row: found uniform on the device
+        V0 = np.tile(V0, (rows, 1))
     if start == "per_row":  # angles perturbed row by row
         rng = np.random.default_rng(6)
         V0 = V0[None, :] * np.exp(1j * rng.normal(0.0, 0.01, (rows, g.n_bus)))
