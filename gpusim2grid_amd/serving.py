@@ -34,6 +34,10 @@ class Request:
     status: int = -1
     iterations: int = 0
     V: np.ndarray | None = None
+    # where the payload sits: row `row` of `payload` ([rows, n] complex, the
+    # receive buffer S is a view of); None for a standalone S
+    payload: np.ndarray | None = None
+    row: int = -1
 
 
 def batch_bytes(plan_info, rows, precision, per_row_ybus):
@@ -72,43 +76,85 @@ class GridState:
         self._pin = [torch.empty((self.stage_rows, grid.n_bus), dtype=torch.complex128, pin_memory=True)
                      for _ in range(2)]
         self.S_pin, self.V_pin = (t.numpy() for t in self._pin)
+        self.t = collections.Counter()  # host seconds per phase
+        self.results = {}  # id(payload buffer) -> (payload, response buffer)
 
     def admit(self):
         """Fills free slots from the queue; returns how many."""
         k = min(len(self.free), len(self.queue), self.stage_rows)
         if k == 0:
             return 0
+        t0 = time.perf_counter()
         reqs = [self.queue.popleft() for _ in range(k)]
         slots = [self.free.pop() for _ in range(k)]
-        S = np.stack([r.S for r in reqs], out=self.S_pin[:k])
-        edits = syn.outage_edits(self.g, [r.outages for r in reqs])
+        p0 = reqs[0].payload
+        if p0 is not None and all(r.payload is p0 for r in reqs) and reqs[-1].row - reqs[0].row == k - 1 and \
+                all(reqs[i].row + 1 == reqs[i + 1].row for i in range(k - 1)):
+            S = p0[reqs[0].row:reqs[0].row + k]  # consecutive rows of the receive buffer: no staging copy
+        else:
+            S = np.stack([r.S for r in reqs], out=self.S_pin[:k])
+        t1 = time.perf_counter()
+        outs = [r.outages for r in reqs]
+        edits = syn.outage_edits(self.g, outs)
+        t2 = time.perf_counter()
         self.batch.stream_fill(slots, S, self.flat[None], edits=edits)
+        t3 = time.perf_counter()
         if self.warm is not None:  # start voltages predicted on the device, into the slots
-            mask = syn.outage_mask(self.g, [r.outages for r in reqs]) if self.warm.n_mask else None
+            mask = syn.outage_mask(self.g, outs) if self.warm.n_mask else None
             self.batch.stream_warm(self.warm.mlp, slots, mask=mask)
         for s, r in zip(slots, reqs):
             self.busy[s] = r
+        t4 = time.perf_counter()
+        self.t["admit_stack"] += t1 - t0
+        self.t["admit_edits"] += t2 - t1
+        self.t["admit_fill"] += t3 - t2
+        self.t["admit_warm"] += t4 - t3
         return k
 
     def round(self, k=1):
+        t0 = time.perf_counter()
         self.batch.stream_round(k=k, max_iter=self.max_iter, tol=self.tol)
+        self.t["round"] += time.perf_counter() - t0
 
     def retire(self, now):
         """Takes the finished slots' results out; returns the finished requests."""
+        t0 = time.perf_counter()
         st, it, _ = self.batch.results()
+        t1 = time.perf_counter()
         busy = np.fromiter(self.busy.keys(), dtype=np.int64, count=len(self.busy))
         fin = (st[busy] != -1) & (st[busy] != _capi.NOTSIMULATED)
         done = busy[fin].tolist()
         out = []
+        self.t["retire_results"] += t1 - t0
+        t2 = time.perf_counter()
         for c0 in range(0, len(done), self.stage_rows):
             part = done[c0:c0 + self.stage_rows]
             V = self.batch.stream_take(part, out=self.V_pin)
-            for q, s in enumerate(part):
-                r = self.busy.pop(s)
-                r.status, r.iterations, r.V, r.t_done = int(st[s]), int(it[s]), V[q].copy(), now
-                self.free.append(s)
-                out.append(r)
+            reqs = [self.busy.pop(s) for s in part]
+            res = self._results(reqs[0].payload)
+            if res is not None and all(r.payload is reqs[0].payload for r in reqs):
+                # one copy into the response buffer (same rows as the payloads);
+                # each request's V is a view of its row
+                rows = np.fromiter((r.row for r in reqs), dtype=np.int64, count=len(reqs))
+                res[rows] = V[:len(reqs)]
+                for r, s in zip(reqs, part):
+                    r.status, r.iterations, r.V, r.t_done = int(st[s]), int(it[s]), res[r.row], now
+            else:
+                for q, (r, s) in enumerate(zip(reqs, part)):
+                    r.status, r.iterations, r.V, r.t_done = int(st[s]), int(it[s]), V[q].copy(), now
+            self.free.extend(part)
+            out.extend(reqs)
+        self.t["retire_take"] += time.perf_counter() - t2
         return out
+
+    def _results(self, payload):
+        """The response buffer of a payload buffer (same shape), made on first use."""
+        if payload is None:
+            return None
+        key = id(payload)
+        if key not in self.results:
+            self.results[key] = (payload, np.empty_like(payload))
+        return self.results[key][1]
 
 
 class PowerFlowServer:
@@ -159,11 +205,33 @@ class PowerFlowServer:
         wall = time.perf_counter() - t0
         lat = np.array([r.t_done - r.t_arrival for r in done])
         conv = sum(r.status == 0 for r in done)
+        t_last = reqs[-1].t_arrival
+        # while requests still arrive: results per second up to the last arrival
+        steady = sum(r.t_done <= t_last for r in done) / t_last if t_last > 0 else None
         report = dict(requests=len(done), converged=int(conv), wall_s=wall,
-                      throughput_rps=len(done) / wall, offered_rps=len(reqs) / max(reqs[-1].t_arrival, 1e-9),
+                      throughput_rps=len(done) / wall, served_rps_while_arriving=steady,
+                      offered_rps=len(reqs) / max(t_last, 1e-9),
                       p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
-                      mean_iterations=float(np.mean([r.iterations for r in done])), slots=self.slots)
+                      mean_iterations=float(np.mean([r.iterations for r in done])), slots=self.slots,
+                      host_s={f"{name}.{k}": round(v, 4) for name, st in self.state.items() for k, v in st.t.items()})
+        for st in self.state.values():
+            st.t.clear()
         return done, report
+
+
+def _receive_buffer(S):
+    """The grid's request payloads in one page-locked [rows, n] complex
+    buffer, rows in arrival order: what a receiving front end would fill, and
+    what the device reads by DMA without staging."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            buf = torch.empty(S.shape, dtype=torch.complex128, pin_memory=True).numpy()
+            buf[...] = S
+            return buf
+    except ImportError:
+        pass
+    return np.ascontiguousarray(S)
 
 
 def poisson_requests(grids, rate, n, max_outages=3, seed=5):
@@ -185,7 +253,9 @@ def poisson_requests(grids, rate, n, max_outages=3, seed=5):
             extra = syn.random_outages(g, max(1, idx.size), max_k=1, seed=seed + 21 + name_i)
             outs = [sorted(set(a) | set(b)) if syn._connected_without(g, sorted(set(a) | set(b))) else a
                     for a, b in zip(outs, extra)]
+        buf = _receive_buffer(S)
         for q_i, k in enumerate(idx):
-            reqs.append(Request(rid=int(k), grid=name, S=S[q_i], outages=list(outs[q_i]), t_arrival=float(t[k])))
+            reqs.append(Request(rid=int(k), grid=name, S=buf[q_i], outages=list(outs[q_i]), t_arrival=float(t[k]),
+                                payload=buf, row=q_i))
     reqs.sort(key=lambda r: r.t_arrival)
     return reqs
