@@ -115,6 +115,8 @@ def lib():
         "g2g_batch_warm_start": (ctypes.c_int, [_P, _P, _P, i32]),
         "g2g_batch_set_warm_contraction": (ctypes.c_int, [_P, dbl]),
         "g2g_batch_set_warm_mask": (ctypes.c_int, [_P, _P, i32]),
+        "g2g_batch_set_stream_graph": (ctypes.c_int, [_P, i32]),
+        "g2g_batch_stream_graph_info": (ctypes.c_int, [_P, ctypes.POINTER(i32), ctypes.POINTER(i64)]),
         "g2g_batch_branch_flows": (ctypes.c_int, [_P, i32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
         "g2g_batch_limit_violations": (ctypes.c_int, [_P, ctypes.POINTER(LimitTable), _P, _P, ctypes.c_double,
                                                       ctypes.c_double, i32, ctypes.POINTER(Violations)]),
@@ -341,6 +343,15 @@ class Batch:
 
     def stream_round(self, k=1, max_iter=10, tol=1e-8):
         _check(lib().g2g_batch_stream_round(self._h, int(k), int(max_iter), float(tol)))
+
+    def set_stream_graph(self, on):
+        """Streamed rounds captured as a HIP graph and replayed (default on)."""
+        _check(lib().g2g_batch_set_stream_graph(self._h, int(bool(on))))
+
+    def stream_graph_info(self):
+        on, rep = ctypes.c_int32(), ctypes.c_int64()
+        _check(lib().g2g_batch_stream_graph_info(self._h, ctypes.byref(on), ctypes.byref(rep)))
+        return bool(on.value), int(rep.value)
 
     def stream_take(self, slots, out=None):
         """V of the given slots (host [k, n], or into `out`, a C-contiguous

@@ -68,6 +68,7 @@ __global__ void k_gather_astream(const double* vals, long long ldv, const double
                                  int nnz, int d, const int* a_pos, const int* f_pos, int a_len, Real* A);
 __global__ void k_row_absmax(const double* vals, long long ldv, int rows, int nnz, double* out);
 __global__ void k_fill_int(int* p, int v, long long n);
+__global__ void k_add_int(int* p, int v);
 __global__ void k_count_tiles(const int* status, int rows_pad, int* tile_active);
 __global__ void k_branch_flows(const double2* V, int n, const int* from, const int* to, const double2* y4,
                                const double* base_or, const double* base_ex, int m, int T, double* or_t,
@@ -321,6 +322,15 @@ struct g2g_batch_s {
     int* mapk = nullptr;
     int compact_R = 0, compact_saved = 0;
     // streamed rows (continuous batching): per-row start iteration and the
+    // streamed rounds: the iteration clock on the device (the kernels read
+    // it, so a round is the same launches every time), and the round
+    // captured as a HIP graph (per round shape), replayed
+    int* clock_dev = nullptr;
+    bool stream_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    long long graph_key[6] = {0, 0, 0, 0, 0, 0};
+    long long graph_replays = 0;
     // batch's iteration clock
     int* it0 = nullptr;
     int clock = 0;
@@ -363,6 +373,9 @@ struct g2g_batch_s {
         dfree(V); dfree(S); dfree(Yrow); dfree(A); dfree(U); dfree(X); dfree(res); dfree(jmax); dfree(mis_part);
         dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped); dfree(error);
         dfree(V0); dfree(Bs); dfree(Xs); dfree(wmask); dfree(differs); dfree(tile_restart); dfree(v_row);
+        dfree(clock_dev);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
         dfree(Vk); dfree(Sk); dfree(Yk); dfree(stk); dfree(itk); dfree(resk); dfree(mapk); dfree(it0);
         dfree(mask); dfree(maskk); dfree(xs); dfree(xsk); dfree(sx); dfree(pend);
         dfree(budget); dfree(warm); dfree(requeued); dfree(V0k); dfree(it0k); dfree(budgetk); dfree(warmk);
@@ -788,10 +801,12 @@ bool mis_xcd_group()
 }
 
 // only (nullable): evaluate only the tiles with only[tile] > 0; their
-// counts go to a spare counter (the caller keeps the full pass's)
+// counts go to a spare counter (the caller keeps the full pass's).
+// clock (nullable): the iteration is *clock + iter.
 template <typename Real>
 void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double tol, int freeze,
-                     double growth = 0.0, bool transposed = false, const int* only = nullptr)
+                     double growth = 0.0, bool transposed = false, const int* only = nullptr,
+                     const int* clock = nullptr)
 {
     const NrPlan& P = b->plan->nr;
     const bool ext = b->plan->ext();
@@ -813,7 +828,7 @@ void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double
                          b->restart_on ? b->budget : nullptr, b->restart_on ? b->warm : nullptr,
                          splits, (splits > 1 || ext) ? b->mis_part : nullptr, part_stride,
                          ext ? b->sx : nullptr, static_cast<int>(P.xt.xb_bus.size()), T,
-                         splits > 1 && mis_xcd_group() ? 1 : 0, only};
+                         splits > 1 && mis_xcd_group() ? 1 : 0, only, clock};
     const int grid_tiles = a.xcd_group ? (T + kXcds - 1) / kXcds * kXcds : T;
     const dim3 grid(grid_tiles * splits), block(kLanes * kMismatchWaves);
     if (b->per_row_y) {
@@ -1203,17 +1218,12 @@ int nr_loop(g2g_batch_s* b, int max_iter, double tol, int freeze, double growth,
 // last update brought it below tol is found by the next round's first
 // mismatch (its count is then the number of updates it had).
 template <typename Real>
-void stream_round(g2g_batch_s* b, int k, int max_iter, double tol)
+void stream_round_launches(g2g_batch_s* b, int k, int max_iter, double tol)
 {
     const double eps = b->precision == G2G_MIXED ? 1e-5 : 1e-12;  // own_lu/batched_lu.cuh:50-56
-    if (b->ir_steps > 0 && !b->Bs) {
-        const size_t bytes = static_cast<size_t>(b->rows_pad) * b->plan->nr.d * b->real_size();
-        b->Bs = dalloc<unsigned char>(bytes);
-        b->Xs = dalloc<unsigned char>(bytes);
-    }
     for (int q = 0; q < k; ++q) {
         k_fill_int<<<1, 64, 0, b->stream>>>(b->n_active, 0, 2);
-        launch_mismatch<Real>(b, true, b->clock, max_iter, tol, 1, 0.0);
+        launch_mismatch<Real>(b, true, q, max_iter, tol, 1, 0.0, false, nullptr, b->clock_dev);
         launch_factor<Real>(b->plan, static_cast<const Real*>(b->A), static_cast<Real*>(b->U), b->a_stride,
                             b->u_stride, b->tile_active, b->jmax, b->clamped, b->error, b->active_rows, b->T, eps,
                             b->stream);
@@ -1226,8 +1236,59 @@ void stream_round(g2g_batch_s* b, int k, int max_iter, double tol)
                          b->stream);
         launch_update<Real>(b);
         launch_ext_update<Real>(b);
-        ++b->clock;
     }
+    k_add_int<<<1, 64, 0, b->stream>>>(b->clock_dev, k);
+    HIPCHK(hipGetLastError());
+}
+
+// A round's launches depend only on (k, max_iter, tol, refinement steps,
+// step scaling): with graphs on, they are captured once per such shape and
+// replayed.
+template <typename Real>
+void stream_round(g2g_batch_s* b, int k, int max_iter, double tol)
+{
+    if (b->ir_steps > 0 && !b->Bs) {
+        const size_t bytes = static_cast<size_t>(b->rows_pad) * b->plan->nr.d * b->real_size();
+        b->Bs = dalloc<unsigned char>(bytes);
+        b->Xs = dalloc<unsigned char>(bytes);
+    }
+    if (!b->stream_graph) {
+        stream_round_launches<Real>(b, k, max_iter, tol);
+    } else {
+        auto bits = [](double x) {
+            long long v = 0;
+            std::memcpy(&v, &x, sizeof(x));
+            return v;
+        };
+        // everything a round's launches take that can change between rounds
+        const long long key[6] = {k, max_iter, bits(tol), b->ir_steps, bits(b->max_dva), bits(b->max_dvm)};
+        if (!b->graph_exec || !std::equal(key, key + 6, b->graph_key)) {
+            if (b->graph_exec) HIPCHK(hipGraphExecDestroy(b->graph_exec));
+            if (b->graph) HIPCHK(hipGraphDestroy(b->graph));
+            b->graph_exec = nullptr;
+            b->graph = nullptr;
+            // function attributes (dynamic LDS) are set by an eager round
+            // first, so the capture records kernel launches only
+            stream_round_launches<Real>(b, k, max_iter, tol);
+            b->clock += k;
+            HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+            try {
+                stream_round_launches<Real>(b, k, max_iter, tol);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(b->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
+            HIPCHK(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+            std::copy(key, key + 6, b->graph_key);
+            return;
+        }
+        HIPCHK(hipGraphLaunch(b->graph_exec, b->stream));
+        ++b->graph_replays;
+    }
+    b->clock += k;
 }
 
 // islanded buses of the solved rows to NaN
@@ -2453,6 +2514,10 @@ int g2g_batch_stream_begin(g2g_batch_t b)
         k_count_tiles<<<b->T, kLanes, 0, b->stream>>>(b->status, b->rows_pad, b->tile_active);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(b->stream));
+        if (!b->clock_dev) b->clock_dev = dalloc<int>(1);
+        k_fill_int<<<1, 64, 0, b->stream>>>(b->clock_dev, 0, 1);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(b->stream));
         b->clock = 0;
         b->active_rows = b->rows;
         b->streaming = true;
@@ -2545,6 +2610,23 @@ int g2g_batch_stream_round(g2g_batch_t b, int32_t k, int32_t max_iter, double to
         if (b->precision == G2G_MIXED) stream_round<float>(b, k, max_iter, tol);
         else stream_round<double>(b, k, max_iter, tol);
         check_stream_error(b);
+    });
+}
+
+int g2g_batch_set_stream_graph(g2g_batch_t b, int32_t on)
+{
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        b->stream_graph = on != 0;
+    });
+}
+
+int g2g_batch_stream_graph_info(g2g_batch_t b, int32_t* on, int64_t* replays)
+{
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        if (on) *on = b->stream_graph ? 1 : 0;
+        if (replays) *replays = b->graph_replays;
     });
 }
 

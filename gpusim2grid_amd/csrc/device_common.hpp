@@ -48,6 +48,7 @@ struct MismatchArgs {
     int tiles;                  // tiles of the launch
     int xcd_group;              // 1: a tile's split workgroups grouped on one XCD
     const int* only;            // nullable: evaluate only the tiles with only[tile] > 0
+    const int* clock;           // nullable: the iteration is *clock + iter (a graph-replayed round)
 };
 
 // Extension rows (distributed slack, HVDC droop, voltage control): one lane

@@ -151,7 +151,8 @@ __device__ __forceinline__ void mis_status(const MismatchArgs<Real>& a, size_t t
         const double prev = a.res[row];
         a.res[row] = r;
         if (WithJ) a.jmax[row] = jm;
-        const int it = a.it0 ? a.iter - a.it0[row] : a.iter;  // updates this row has had
+        const int iter = a.clock ? *a.clock + a.iter : a.iter;
+        const int it = a.it0 ? iter - a.it0[row] : iter;  // updates this row has had
         const int budget = a.budget ? a.budget[row] : a.max_iter;
         // a row on its learned start that fails (non-finite, growing, or out
         // of its budget) is marked for a restart rather than ended
@@ -1683,6 +1684,12 @@ __global__ void k_restart_rows(int* __restrict__ status, unsigned char* __restri
     status[row] = kActive;
     atomicAdd(requeued, 1);
     tile_restart[row / kLanes] = 1;  // the evaluation at the new start covers this tile
+}
+
+// *p += v (one thread)
+__global__ void k_add_int(int* p, int v)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p += v;
 }
 
 // per-slot scalars: v[slots[r]] = value

@@ -373,6 +373,17 @@ int g2g_lu_solve_batch_noalloc(g2g_plan_t plan, int32_t rows, int32_t precision,
  * round.  Rows are host or device arrays (on_device), row-major n complex
  * with row strides lds / ldv in complex numbers (0 broadcasts one row). */
 int g2g_batch_stream_begin(g2g_batch_t batch);
+/* Streamed rounds as HIP graphs (default on): a round's launches (k
+ * iterations of mismatch, factorization, substitution and update, and the
+ * iteration clock's step, which lives on the device) are captured on the
+ * first round of a shape (k, max_iter, tol, refinement, step scaling) and
+ * replayed after; 0 launches them one by one.  No counterpart in the
+ * reference, whose continuous rounds launch kernels each time
+ * (batch_pf_driver.cu:2720-3100).  The info call reports the setting and the
+ * number of replays so far. */
+int g2g_batch_set_stream_graph(g2g_batch_t batch, int32_t on);
+int g2g_batch_stream_graph_info(g2g_batch_t batch, int32_t *on, int64_t *replays);
+
 /* The learned start voltages of n freshly filled slots, predicted on the
  * device from the slots' Sbus (and a host 0/1 outage mask [n][n_mask], or
  * NULL) and written as their V; no counterpart in the reference. */
