@@ -329,6 +329,11 @@ struct g2g_batch_s {
     bool stream_graph = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // where the graph is captured and replayed: the batch's stream, or for a
+    // batch on the null stream (which cannot be captured) a blocking stream
+    // of its own, ordered against the null stream both ways
+    hipStream_t graph_stream = nullptr;
+    bool own_graph_stream = false;
     long long graph_key[6] = {0, 0, 0, 0, 0, 0};
     long long graph_replays = 0;
     // batch's iteration clock
@@ -376,6 +381,7 @@ struct g2g_batch_s {
         dfree(clock_dev);
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
+        if (own_graph_stream) (void)hipStreamDestroy(graph_stream);
         dfree(Vk); dfree(Sk); dfree(Yk); dfree(stk); dfree(itk); dfree(resk); dfree(mapk); dfree(it0);
         dfree(mask); dfree(maskk); dfree(xs); dfree(xsk); dfree(sx); dfree(pend);
         dfree(budget); dfree(warm); dfree(requeued); dfree(V0k); dfree(it0k); dfree(budgetk); dfree(warmk);
@@ -1271,6 +1277,16 @@ void stream_round(g2g_batch_s* b, int k, int max_iter, double tol)
             // first, so the capture records kernel launches only
             stream_round_launches<Real>(b, k, max_iter, tol);
             b->clock += k;
+            if (!b->graph_stream) {
+                if (b->stream) {
+                    b->graph_stream = b->stream;
+                } else {
+                    HIPCHK(hipStreamCreate(&b->graph_stream));  // default flags and priority
+                    b->own_graph_stream = true;
+                }
+            }
+            hipStream_t keep = b->stream;
+            b->stream = b->graph_stream;  // the launches below are recorded on it
             HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
             try {
                 stream_round_launches<Real>(b, k, max_iter, tol);
@@ -1278,14 +1294,17 @@ void stream_round(g2g_batch_s* b, int k, int max_iter, double tol)
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(b->stream, &g);
                 if (g) (void)hipGraphDestroy(g);
+                b->stream = keep;
                 throw;
             }
-            HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
+            const hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
+            b->stream = keep;
+            HIPCHK(e);
             HIPCHK(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
             std::copy(key, key + 6, b->graph_key);
             return;
         }
-        HIPCHK(hipGraphLaunch(b->graph_exec, b->stream));
+        HIPCHK(hipGraphLaunch(b->graph_exec, b->graph_stream));
         ++b->graph_replays;
     }
     b->clock += k;
