@@ -330,10 +330,12 @@ struct g2g_batch_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     // where the graph is captured and replayed: the batch's stream, or for a
-    // batch on the null stream (which cannot be captured) a blocking stream
-    // of its own, ordered against the null stream both ways
+    // batch on the null stream (which cannot be captured) a stream of its
+    // own, ordered after the batch's earlier work and before its later work
+    // by two events
     hipStream_t graph_stream = nullptr;
     bool own_graph_stream = false;
+    hipEvent_t graph_ev[2] = {nullptr, nullptr};
     long long graph_key[6] = {0, 0, 0, 0, 0, 0};
     long long graph_replays = 0;
     // batch's iteration clock
@@ -382,6 +384,8 @@ struct g2g_batch_s {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (own_graph_stream) (void)hipStreamDestroy(graph_stream);
+        for (hipEvent_t e : graph_ev)
+            if (e) (void)hipEventDestroy(e);
         dfree(Vk); dfree(Sk); dfree(Yk); dfree(stk); dfree(itk); dfree(resk); dfree(mapk); dfree(it0);
         dfree(mask); dfree(maskk); dfree(xs); dfree(xsk); dfree(sx); dfree(pend);
         dfree(budget); dfree(warm); dfree(requeued); dfree(V0k); dfree(it0k); dfree(budgetk); dfree(warmk);
@@ -1283,6 +1287,7 @@ void stream_round(g2g_batch_s* b, int k, int max_iter, double tol)
                 } else {
                     HIPCHK(hipStreamCreate(&b->graph_stream));  // default flags and priority
                     b->own_graph_stream = true;
+                    for (hipEvent_t& e : b->graph_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
                 }
             }
             hipStream_t keep = b->stream;
@@ -1304,7 +1309,15 @@ void stream_round(g2g_batch_s* b, int k, int max_iter, double tol)
             std::copy(key, key + 6, b->graph_key);
             return;
         }
+        if (b->own_graph_stream) {  // after the batch's earlier work on its stream
+            HIPCHK(hipEventRecord(b->graph_ev[0], b->stream));
+            HIPCHK(hipStreamWaitEvent(b->graph_stream, b->graph_ev[0], 0));
+        }
         HIPCHK(hipGraphLaunch(b->graph_exec, b->graph_stream));
+        if (b->own_graph_stream) {  // and before its later work
+            HIPCHK(hipEventRecord(b->graph_ev[1], b->graph_stream));
+            HIPCHK(hipStreamWaitEvent(b->stream, b->graph_ev[1], 0));
+        }
         ++b->graph_replays;
     }
     b->clock += k;
