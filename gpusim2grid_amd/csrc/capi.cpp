@@ -1988,6 +1988,8 @@ int g2g_batch_create(g2g_plan_t P, int32_t rows, int32_t precision, int32_t per_
         HIPCHK(hipMemsetAsync(b->A, 0, rp * b->a_stride * b->real_size(), b->stream));
         b->res = dalloc<double>(rp);
         b->jmax = dalloc<double>(rp);
+        HIPCHK(hipMemsetAsync(b->res, 0, rp * sizeof(double), b->stream));  // rows never simulated read 0
+        HIPCHK(hipMemsetAsync(b->jmax, 0, rp * sizeof(double), b->stream));
         {
             int dev = 0, cu = 0;
             HIPCHK(hipGetDevice(&dev));

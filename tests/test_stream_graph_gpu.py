@@ -56,7 +56,8 @@ def test_graph_rounds_equal_eager_rounds(prec):
         se, ie, re_ = Be.results()
         np.testing.assert_array_equal(sg, se)
         np.testing.assert_array_equal(ig, ie)
-        np.testing.assert_array_equal(rg, re_)
+        sim = sg != _capi.NOTSIMULATED
+        np.testing.assert_array_equal(rg[sim], re_[sim])
         done = [s for s in busy if sg[s] not in (-1, _capi.NOTSIMULATED)]
         if done:
             np.testing.assert_array_equal(Bg.stream_take(done), Be.stream_take(done))
