@@ -116,6 +116,7 @@ def lib():
         "g2g_batch_set_warm_contraction": (ctypes.c_int, [_P, dbl]),
         "g2g_batch_set_warm_mask": (ctypes.c_int, [_P, _P, i32]),
         "g2g_batch_set_stream_graph": (ctypes.c_int, [_P, i32]),
+        "g2g_batch_set_refinement_residual": (ctypes.c_int, [_P, i32]),
         "g2g_batch_stream_graph_info": (ctypes.c_int, [_P, ctypes.POINTER(i32), ctypes.POINTER(i64)]),
         "g2g_batch_branch_flows": (ctypes.c_int, [_P, i32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
         "g2g_batch_limit_violations": (ctypes.c_int, [_P, ctypes.POINTER(LimitTable), _P, _P, ctypes.c_double,
@@ -376,6 +377,10 @@ class Batch:
         st = SolveStats()
         _check(lib().g2g_batch_solve(self._h, int(max_iter), float(tol), int(scheduling), ctypes.byref(st)))
         return st.as_dict()
+
+    def set_refinement_residual(self, fp64):
+        """Refinement rounds' residual: FP64 from V (True) or the solve's precision."""
+        _check(lib().g2g_batch_set_refinement_residual(self._h, int(bool(fp64))))
 
     def set_warm_mask(self, mask):
         """Resident outage mask for warm starts (rows x n_mask, 0/1), used

@@ -42,6 +42,8 @@ template <typename Real>
 __global__ void k_ext_init(ExtArgs<Real> a);
 template <typename Real>
 __global__ void k_lin_residual(RefineArgs<Real> a);
+template <typename Real, bool PerRowY>
+__global__ void k_ir_residual(IrArgs<Real> a);
 template <typename Real>
 __global__ void k_axpy_tiles(const Real* Xs, Real* X, const int* tile_active, int d);
 template <typename T>
@@ -117,6 +119,7 @@ struct g2g_plan_s {
     int* d_fpos = nullptr;
     int* d_upd = nullptr;     // per bus {bus, theta unknown, |V| unknown}
     int n_upd = 0;
+    int* d_colmap = nullptr;  // [n][2]: each bus's theta and |V| unknown, -1 for none
     int* d_k1_ent_t = nullptr;  // k1_ent with the A positions of the transposed entries (adjoint)
     std::vector<int> raw_p, raw_i;  // a raw plan's matrix pattern (CSR)
     std::vector<int> raw_order;     // and its pivot order (empty: minimum degree)
@@ -160,6 +163,7 @@ struct g2g_plan_s {
     {
         dfree(d_prog_f); dfree(d_prog_b); dfree(d_seg); dfree(d_bseg); dfree(d_k1_bus); dfree(d_k1_ent);
         dfree(d_Y); dfree(d_perm); dfree(d_apos); dfree(d_fpos); dfree(d_upd); dfree(d_Jp); dfree(d_Ji);
+        dfree(d_colmap);
         dfree(d_k1_ent_t);
         dx.free_all();
         dfree(d_prog_fs); dfree(d_prog_bs); dfree(d_split_seg); dfree(d_split_bseg); dfree(d_k1_bus_s);
@@ -240,6 +244,12 @@ struct g2g_plan_s {
             }
             n_upd = static_cast<int>(upd.size() / 3);
             d_upd = dupload(upd);
+            std::vector<int> colmap(2 * static_cast<size_t>(nr.n));
+            for (int i = 0; i < nr.n; ++i) {
+                colmap[2 * i] = nr.col_th[i];
+                colmap[2 * i + 1] = nr.row_q[i];
+            }
+            d_colmap = dupload(colmap);
             if (nr.xt.on()) {
                 const ExtTables& x = nr.xt;
                 dx.inj_ptr = dupload(x.inj_ptr);
@@ -347,6 +357,7 @@ struct g2g_batch_s {
     bool restart_on = false;
     int restart_max_iter = 0;
     double contraction = 0.0;  // the quadratic-contraction rejection of learned starts (0: off)
+    bool ir_fp64 = false;      // refinement residuals in FP64 from V (else the solve's precision)
     int n_cu = 256;            // the device's compute units
     // the warm start's resident outage mask [rows][wmask_n] (nullable)
     unsigned char* wmask = nullptr;
@@ -688,6 +699,49 @@ void refine(const g2g_plan_s* P, Real* A, Real* U, Real* X, Real* Bs, Real* Xs, 
         k_axpy_tiles<Real><<<dim3(std::min(64, (per_tile + 255) / 256), T), 256, 0, st>>>(Xs, X, tile_active, s.d);
         HIPCHK(hipGetLastError());
     }
+}
+
+// Refinement rounds of a Newton step with the residual in FP64 from V
+// (k_ir_residual), against the solve's factors: r = F(V) - J(V) x over the
+// A stream's rhs, the factorization again (bitwise the same factors), the
+// substitution, x += the correction.
+template <typename Real>
+void refine_fp64(g2g_batch_s* b, double eps, int T)
+{
+    const g2g_plan_s* P = b->plan;
+    const NrPlan& nr = P->nr;
+    Real* A = static_cast<Real*>(b->A);
+    Real* X = static_cast<Real*>(b->X);
+    Real* Xs = static_cast<Real*>(b->Xs);
+    for (int k = 0; k < b->ir_steps; ++k) {
+        IrArgs<Real> a{P->d_k1_bus, P->d_k1_ent, nr.k1_nbus, P->d_Y, b->Yrow, b->V, b->S, b->mask, P->d_colmap,
+                       A, X, Xs, b->tile_active, nr.n, nr.nY, nr.d, b->a_stride};
+        if (b->per_row_y) k_ir_residual<Real, true><<<T, kLanes * kMismatchWaves, 0, b->stream>>>(a);
+        else k_ir_residual<Real, false><<<T, kLanes * kMismatchWaves, 0, b->stream>>>(a);
+        HIPCHK(hipGetLastError());
+        launch_factor<Real>(P, A, static_cast<Real*>(b->U), b->a_stride, b->u_stride, b->tile_active, b->jmax,
+                            b->clamped, b->error, b->active_rows, T, eps, b->stream);
+        launch_backward<Real>(P, static_cast<const Real*>(b->U), b->u_stride, b->tile_active, X, b->error,
+                              b->active_rows, T, b->stream);
+        const int per_tile = nr.d * kLanes;
+        k_axpy_tiles<Real><<<dim3(std::min(64, (per_tile + 255) / 256), T), 256, 0, b->stream>>>(Xs, X,
+                                                                                              b->tile_active, nr.d);
+        HIPCHK(hipGetLastError());
+    }
+}
+
+// A Newton step's refinement: FP64 residual or the solve's own (the
+// reference's, BatchedLu::refine, batched_lu.cu:391-411).
+template <typename Real>
+void refine_step(g2g_batch_s* b, double eps, int T)
+{
+    if (b->ir_fp64) {
+        refine_fp64<Real>(b, eps, T);
+        return;
+    }
+    refine<Real>(b->plan, static_cast<Real*>(b->A), static_cast<Real*>(b->U), static_cast<Real*>(b->X),
+                 static_cast<Real*>(b->Bs), static_cast<Real*>(b->Xs), b->a_stride, b->u_stride, b->tile_active,
+                 b->jmax, b->clamped, b->error, b->active_rows, T, eps, b->ir_steps, b->stream);
 }
 
 // Workgroups per tile of the mismatch and update kernels (G2G_MIS_SPLITS:
@@ -1182,11 +1236,7 @@ int nr_loop_body(g2g_batch_s* b, int max_iter, double tol, int freeze, double gr
         else
             launch_backward<Real>(b->plan, static_cast<const Real*>(b->U), b->u_stride, b->tile_active,
                                   static_cast<Real*>(b->X), b->error, b->active_rows, b->T, b->stream);
-        if (b->ir_steps > 0)
-            refine<Real>(b->plan, static_cast<Real*>(b->A), static_cast<Real*>(b->U), static_cast<Real*>(b->X),
-                         static_cast<Real*>(b->Bs), static_cast<Real*>(b->Xs), b->a_stride, b->u_stride,
-                         b->tile_active, b->jmax, b->clamped, b->error, b->active_rows, b->T, eps, b->ir_steps,
-                         b->stream);
+        if (b->ir_steps > 0) refine_step<Real>(b, eps, b->T);
         launch_update<Real>(b);
         launch_ext_update<Real>(b);
         tm.ph[2].push_back({e0, tm.mark()});
@@ -1239,11 +1289,7 @@ void stream_round_launches(g2g_batch_s* b, int k, int max_iter, double tol)
                             b->stream);
         launch_backward<Real>(b->plan, static_cast<const Real*>(b->U), b->u_stride, b->tile_active,
                               static_cast<Real*>(b->X), b->error, b->active_rows, b->T, b->stream);
-        if (b->ir_steps > 0)
-            refine<Real>(b->plan, static_cast<Real*>(b->A), static_cast<Real*>(b->U), static_cast<Real*>(b->X),
-                         static_cast<Real*>(b->Bs), static_cast<Real*>(b->Xs), b->a_stride, b->u_stride,
-                         b->tile_active, b->jmax, b->clamped, b->error, b->active_rows, b->T, eps, b->ir_steps,
-                         b->stream);
+        if (b->ir_steps > 0) refine_step<Real>(b, eps, b->T);
         launch_update<Real>(b);
         launch_ext_update<Real>(b);
     }
@@ -2134,6 +2180,16 @@ int g2g_batch_solve_warm(g2g_batch_t b, g2g_mlp_t mlp, const uint8_t* mask, int3
         if (b->precision == G2G_MIXED)
             solve_warm_impl<float>(b, mlp, mask, n_mask, k_warm, growth, max_iter, tol, stats);
         else solve_warm_impl<double>(b, mlp, mask, n_mask, k_warm, growth, max_iter, tol, stats);
+    });
+}
+
+int g2g_batch_set_refinement_residual(g2g_batch_t b, int32_t fp64)
+{
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        if (fp64 && b->plan->ext())
+            throw std::invalid_argument("the FP64 refinement residual is not available with extension rows");
+        b->ir_fp64 = fp64 != 0;
     });
 }
 

@@ -168,6 +168,27 @@ struct UpdateArgs {
     int splits;               // workgroups per tile (bus ranges); > 1 only with step scaling off
 };
 
+// Refinement with an FP64 residual: r = F(V) - J(V) dx, both from V in FP64
+// (J dx matrix-free: dS_i = dV_i conj(I_i) + V_i conj(sum_j Y_ij dV_j) with
+// dV_j = V_j (j dtheta_j + dVm_j / |V_j|)), written over the A stream's rhs.
+template <typename Real>
+struct IrArgs {
+    const int* k1_bus;          // the mismatch's bus and entry tables
+    const int* k1_ent;
+    int nbus;
+    const double2* Y;           // shared Ybus values, or
+    const double2* Yrow;        // per-row values [T][nY][64]
+    const double2* V;           // [T][n][64]
+    const double2* S;           // [T][n][64]
+    const unsigned char* mask;  // nullable: islanded buses [T][n][64] (identity rows)
+    const int* colmap;          // [n][2]: each bus's theta and |V| unknown (-1: none)
+    Real* A;                    // rhs positions overwritten with r
+    const Real* X;              // the current step [T][d][64]
+    Real* Xs;                   // its copy
+    const int* tile_active;
+    int n, nY, d, a_stride;
+};
+
 template <typename Real>
 struct RefineArgs {
     Real* A;                // A stream [T][a_stride][64]: J values read, rhs overwritten

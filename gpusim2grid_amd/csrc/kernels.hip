@@ -1452,6 +1452,59 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_update(UpdateArgs<Real> a
 //   stream, x is copied to Xs on every step.
 // ---------------------------------------------------------------------------
 
+// k_ir_residual: one workgroup per tile, its waves over the buses; also
+// keeps a copy of the step (the refinement adds its correction to it).
+template <typename Real, bool PerRowY>
+__global__ __launch_bounds__(kLanes* kMismatchWaves) void k_ir_residual(IrArgs<Real> a)
+{
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kLanes));
+    const size_t tile = blockIdx.x;
+    if (a.tile_active[tile] <= 0) return;
+    const size_t m = static_cast<size_t>(a.d) * kLanes;
+    for (size_t k = threadIdx.x; k < m; k += blockDim.x) a.Xs[tile * m + k] = a.X[tile * m + k];
+    // dV of bus b at V = v: V (j dtheta + dVm / |V|)
+    auto dv = [&](int bus, double2 v) {
+        const int kt = a.colmap[2 * bus], kv = a.colmap[2 * bus + 1];
+        const double th = kt >= 0 ? static_cast<double>(a.X[tidx(tile, a.d, kt, lane)]) : 0.0;
+        const double vm = kv >= 0 ? static_cast<double>(a.X[tidx(tile, a.d, kv, lane)]) : 0.0;
+        const double s = vm / sqrt(fma(v.x, v.x, v.y * v.y));
+        return make_double2(fma(v.x, s, -v.y * th), fma(v.y, s, v.x * th));
+    };
+    for (int bb = wave; bb < a.nbus; bb += kMismatchWaves) {
+        const int* bus = a.k1_bus + kK1BusWords * bb;
+        const int i = bus[0], e0 = bus[1], e1 = bus[2], fp = bus[3], fq = bus[4];
+        const double2 vi = a.V[tidx(tile, a.n, i, lane)];
+        const double2 sb = a.S[tidx(tile, a.n, i, lane)];
+        double2 I = make_double2(0.0, 0.0), dI = make_double2(0.0, 0.0);
+        for (int e = e0; e < e1; ++e) {
+            const int* en = a.k1_ent + 6 * e;
+            const int j = en[0];
+            const double2 y = PerRowY ? a.Yrow[tidx(tile, a.nY, en[1], lane)] : a.Y[en[1]];
+            const double2 vj = j == i ? vi : a.V[tidx(tile, a.n, j, lane)];
+            const double2 t = cmul(y, vj), u = cmul(y, dv(j, vj));
+            I.x += t.x;
+            I.y += t.y;
+            dI.x += u.x;
+            dI.y += u.y;
+        }
+        double rp, rq;
+        if (a.mask && a.mask[tidx(tile, a.n, i, lane)]) {
+            // an islanded bus: identity rows, zero mismatch
+            const int kt = a.colmap[2 * i], kv = a.colmap[2 * i + 1];
+            rp = kt >= 0 ? -static_cast<double>(a.X[tidx(tile, a.d, kt, lane)]) : 0.0;
+            rq = kv >= 0 ? -static_cast<double>(a.X[tidx(tile, a.d, kv, lane)]) : 0.0;
+        } else {
+            const double2 sc = cmulc(vi, I);  // V conj(I)
+            const double2 d1 = cmulc(dv(i, vi), I), d2 = cmulc(vi, dI);
+            rp = -(sc.x - sb.x) - (d1.x + d2.x);
+            rq = -(sc.y - sb.y) - (d1.y + d2.y);
+        }
+        a.A[tidx(tile, a.a_stride, fp, lane)] = static_cast<Real>(rp);
+        if (fq >= 0) a.A[tidx(tile, a.a_stride, fq, lane)] = static_cast<Real>(rq);
+    }
+}
+
 template <typename Real>
 __global__ __launch_bounds__(kLanes* kMisWaves) void k_lin_residual(RefineArgs<Real> a)
 {
@@ -2171,6 +2224,10 @@ template __global__ void k_tiles_to_rows_f64<double>(const double*, int, int, do
 template __global__ void k_tiles_to_rows_f64<float>(const float*, int, int, double*, long long);
 template __global__ void k_lin_residual<double>(RefineArgs<double>);
 template __global__ void k_lin_residual<float>(RefineArgs<float>);
+template __global__ void k_ir_residual<double, false>(IrArgs<double>);
+template __global__ void k_ir_residual<double, true>(IrArgs<double>);
+template __global__ void k_ir_residual<float, false>(IrArgs<float>);
+template __global__ void k_ir_residual<float, true>(IrArgs<float>);
 template __global__ void k_axpy_tiles<double>(const double*, double*, const int*, int);
 template __global__ void k_axpy_tiles<float>(const float*, float*, const int*, int);
 template __global__ void k_rows_to_tiles<double2>(const double2*, long long, int, int, int, double2*, int);

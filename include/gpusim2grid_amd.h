@@ -240,6 +240,14 @@ int g2g_batch_set_step_control(g2g_batch_t batch, int32_t ir_steps, int32_t scal
 int g2g_batch_solve_warm(g2g_batch_t batch, g2g_mlp_t mlp, const uint8_t *mask, int32_t n_mask, int32_t k_warm,
                          double growth, int32_t max_iter, double tol, g2g_solve_stats *stats);
 
+/* The residual of the refinement rounds of Newton steps (ir_n_steps of
+ * g2g_batch_set_step_control): 0, the default, in the solve's precision from
+ * the A stream's J values (BatchedLu::refine, own_lu/batched_lu.cu:391-411);
+ * 1, in FP64 from V: r = F(V) - J(V) dx with J dx formed matrix-free, the
+ * correction solved on the FP32 factors in mixed precision.  Not available
+ * with extension rows. */
+int g2g_batch_set_refinement_residual(g2g_batch_t batch, int32_t fp64);
+
 /* A resident outage mask for the batch's warm starts: rows x n_mask bytes
  * (0/1, one row per batch row), uploaded once; g2g_batch_solve_warm and
  * g2g_batch_warm_start called with no mask (NULL, 0) use it.  NULL / 0 drops
