@@ -10,6 +10,8 @@ injections are resident in HBM before the timed region.  Methods:
 * ``fp64``  plain FP64 flat-start NR (the reference method);
 * ``mixed`` FP32 Jacobian / LU / triangular solves, FP64 mismatch, test and
   state;
+* ``mixed_ir`` mixed precision with one refinement round per Newton step whose
+  residual is formed in FP64 from V, against the FP32 factors;
 * ``warm``  learned warm start + mixed precision (when the MLP is available);
 * ``facade_mixed`` / ``facade_warm``: ``mixed`` / ``warm`` through the
   drop-in Python surface (InjectionSweepGPU.compute), to show what the
@@ -57,7 +59,7 @@ def parse():
     ap.add_argument("--max-iter", type=int, default=10)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--methods", default=None,
-                    help="comma list (default: fp64,mixed,warm plus, on c2, facade_mixed,facade_warm)")
+                    help="comma list (default: fp64,mixed,mixed_ir,warm plus, on c2, facade_mixed,facade_warm)")
     ap.add_argument("--train-rows", type=int, default=65536, help="warm start: solved training scenarios")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--hidden", type=int, default=1024)
@@ -82,7 +84,7 @@ def parse():
     ap.add_argument("--quiet", action="store_true")
     args = ap.parse_args()
     if args.methods is None:
-        args.methods = "fp64,mixed,warm" + (",facade_mixed,facade_warm" if args.workload == "c2" else "")
+        args.methods = "fp64,mixed,mixed_ir,warm" + (",facade_mixed,facade_warm" if args.workload == "c2" else "")
     if args.workload == "c3":  # the learned-warm-start workload's defaults: 200k training rows
         for k, v in (("train_rows", 200_000),):
             if getattr(args, k) == ap.get_default(k):
@@ -179,6 +181,7 @@ def train_warm(args, wl, torch, capi, syn):
 def run_method(args, name, wl, torch, capi, ws=None):
     g, S = wl.g, wl.S
     prec = capi.FP64 if name == "fp64" else capi.MIXED
+    ir = 1 if name == "mixed_ir" else 0  # one refinement round per Newton step, FP64 residual
     t_plan = time.perf_counter()
     plan = capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
     info = plan.info()
@@ -186,6 +189,9 @@ def run_method(args, name, wl, torch, capi, ws=None):
     B = capi.Batch(plan, args.rows, precision=prec, per_row_ybus=wl.edits is not None)
     if wl.edits is not None:
         B.set_ybus_edits(*wl.edits)
+    if ir:
+        B.set_step_control(ir_steps=ir)
+        B.set_refinement_residual(True)
     # injections resident in HBM: one upload before any timing
     S_dev = torch.from_numpy(np.ascontiguousarray(S)).to("cuda")
     B.set_sbus_device(S_dev.data_ptr(), S.shape[1], args.rows)

@@ -713,16 +713,26 @@ void refine_fp64(g2g_batch_s* b, double eps, int T)
     Real* A = static_cast<Real*>(b->A);
     Real* X = static_cast<Real*>(b->X);
     Real* Xs = static_cast<Real*>(b->Xs);
+    const bool sp = b->split_layout;  // the tail's split walk: its layout's tables and launches
     for (int k = 0; k < b->ir_steps; ++k) {
-        IrArgs<Real> a{P->d_k1_bus, P->d_k1_ent, nr.k1_nbus, P->d_Y, b->Yrow, b->V, b->S, b->mask, P->d_colmap,
-                       A, X, Xs, b->tile_active, nr.n, nr.nY, nr.d, b->a_stride};
+        IrArgs<Real> a{sp ? P->d_k1_bus_s : P->d_k1_bus, sp ? P->d_k1_ent_s : P->d_k1_ent, nr.k1_nbus, P->d_Y,
+                       b->Yrow, b->V, b->S, b->mask, P->d_colmap, A, X, Xs, b->tile_active, nr.n, nr.nY, nr.d,
+                       b->a_stride};
         if (b->per_row_y) k_ir_residual<Real, true><<<T, kLanes * kMismatchWaves, 0, b->stream>>>(a);
         else k_ir_residual<Real, false><<<T, kLanes * kMismatchWaves, 0, b->stream>>>(a);
         HIPCHK(hipGetLastError());
-        launch_factor<Real>(P, A, static_cast<Real*>(b->U), b->a_stride, b->u_stride, b->tile_active, b->jmax,
-                            b->clamped, b->error, b->active_rows, T, eps, b->stream);
-        launch_backward<Real>(P, static_cast<const Real*>(b->U), b->u_stride, b->tile_active, X, b->error,
-                              b->active_rows, T, b->stream);
+        if (sp) {
+            launch_factor_split<Real>(P, A, static_cast<Real*>(b->U), b->a_stride, b->u_stride, b->tile_active,
+                                      b->jmax, b->clamped, b->error, b->active_rows, T, eps,
+                                      static_cast<Real*>(b->pend), b->stream);
+            launch_backward_split<Real>(P, static_cast<const Real*>(b->U), b->u_stride, b->tile_active, X,
+                                        b->error, b->active_rows, T, b->stream);
+        } else {
+            launch_factor<Real>(P, A, static_cast<Real*>(b->U), b->a_stride, b->u_stride, b->tile_active, b->jmax,
+                                b->clamped, b->error, b->active_rows, T, eps, b->stream);
+            launch_backward<Real>(P, static_cast<const Real*>(b->U), b->u_stride, b->tile_active, X, b->error,
+                                  b->active_rows, T, b->stream);
+        }
         const int per_tile = nr.d * kLanes;
         k_axpy_tiles<Real><<<dim3(std::min(64, (per_tile + 255) / 256), T), 256, 0, b->stream>>>(Xs, X,
                                                                                               b->tile_active, nr.d);
@@ -1108,7 +1118,8 @@ constexpr int kSplitMaxTiles = 128;  // active tiles up to which the split walk 
 
 bool use_split(const g2g_batch_s* b, int active_tiles)
 {
-    if (!b->plan->split() || !b->pend || b->ir_steps > 0) return false;
+    // refinement from the A stream's J reads the main layout's positions
+    if (!b->plan->split() || !b->pend || (b->ir_steps > 0 && !b->ir_fp64)) return false;
     const int env = split_env();
     if (env >= 0) return env == 1;
     return active_tiles <= kSplitMaxTiles;
@@ -1236,7 +1247,7 @@ int nr_loop_body(g2g_batch_s* b, int max_iter, double tol, int freeze, double gr
         else
             launch_backward<Real>(b->plan, static_cast<const Real*>(b->U), b->u_stride, b->tile_active,
                                   static_cast<Real*>(b->X), b->error, b->active_rows, b->T, b->stream);
-        if (b->ir_steps > 0) refine_step<Real>(b, eps, b->T);
+        if (b->ir_steps > 0) refine_step<Real>(b, eps, b->split_layout ? split_tiles(b) : b->T);
         launch_update<Real>(b);
         launch_ext_update<Real>(b);
         tm.ph[2].push_back({e0, tm.mark()});
