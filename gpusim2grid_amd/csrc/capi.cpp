@@ -50,10 +50,12 @@ template <typename T>
 __global__ void k_rows_to_tiles(const T* in, long long ld, int rows, int m, int row_offset, T* out, int broadcast);
 template <typename T>
 __global__ void k_tiles_to_rows(const T* in, int rows, int m, T* out, long long ld);
-__global__ void k_ybus_broadcast(const double2* Y, int nY, int rows_pad, double2* Yrow);
-__global__ void k_ybus_edits(const int* row_ptr, const int* ent, const double2* delta, int rows, int nY, double2* Yrow,
-                             const int* slots);
-__global__ void k_ybus_reset_slots(const int* slots, int R, const double2* Y, int nY, double2* Yrow);
+__global__ void k_set_row_edits(const int* row_ptr, const int* ent, const double2* delta, int R, const int* slots,
+                                int cap, int* n_ed, int* ed_ent, double2* ed_val);
+__global__ void k_edit_flags(const int* n_ed, const int* ed_ent, int cap, int rows, const int* entry_row, int n,
+                             int* flag);
+template <typename T>
+__global__ void k_regrow_edits(const T* src, int cap_old, T* dst, int cap_new, int tiles);
 template <typename T>
 __global__ void k_rows_to_slots(const T* src, long long ld, int R, int m, const int* slots, T* dst);
 template <typename T>
@@ -120,6 +122,7 @@ struct g2g_plan_s {
     int* d_upd = nullptr;     // per bus {bus, theta unknown, |V| unknown}
     int n_upd = 0;
     int* d_colmap = nullptr;  // [n][2]: each bus's theta and |V| unknown, -1 for none
+    int* d_entry_row = nullptr;  // [nY]: the Ybus row of each entry
     int* d_k1_ent_t = nullptr;  // k1_ent with the A positions of the transposed entries (adjoint)
     std::vector<int> raw_p, raw_i;  // a raw plan's matrix pattern (CSR)
     std::vector<int> raw_order;     // and its pivot order (empty: minimum degree)
@@ -163,7 +166,7 @@ struct g2g_plan_s {
     {
         dfree(d_prog_f); dfree(d_prog_b); dfree(d_seg); dfree(d_bseg); dfree(d_k1_bus); dfree(d_k1_ent);
         dfree(d_Y); dfree(d_perm); dfree(d_apos); dfree(d_fpos); dfree(d_upd); dfree(d_Jp); dfree(d_Ji);
-        dfree(d_colmap);
+        dfree(d_colmap); dfree(d_entry_row);
         dfree(d_k1_ent_t);
         dx.free_all();
         dfree(d_prog_fs); dfree(d_prog_bs); dfree(d_split_seg); dfree(d_split_bseg); dfree(d_k1_bus_s);
@@ -250,6 +253,10 @@ struct g2g_plan_s {
                 colmap[2 * i + 1] = nr.row_q[i];
             }
             d_colmap = dupload(colmap);
+            std::vector<int> entry_row(static_cast<size_t>(nr.nY));
+            for (int i = 0; i < nr.n; ++i)
+                for (int q = nr.Yp[i]; q < nr.Yp[i + 1]; ++q) entry_row[q] = i;
+            d_entry_row = dupload(entry_row);
             if (nr.xt.on()) {
                 const ExtTables& x = nr.xt;
                 dx.inj_ptr = dupload(x.inj_ptr);
@@ -285,7 +292,12 @@ struct g2g_batch_s {
     hipStream_t stream = nullptr;
     double2* V = nullptr;
     double2* S = nullptr;
-    double2* Yrow = nullptr;
+    // outage batches: each row's Ybus edits (RowEdits), capacity ed_cap per row
+    int ed_cap = 0;
+    int* ed_n = nullptr;
+    int* ed_ent = nullptr;
+    double2* ed_val = nullptr;
+    int* ed_flag = nullptr;  // [T][n]
     void* A = nullptr;
     void* U = nullptr;
     void* X = nullptr;         // solved Newton step [T][d][64]
@@ -325,7 +337,10 @@ struct g2g_batch_s {
     // dense tiles, and the row map (compacted row -> batch row)
     double2* Vk = nullptr;
     double2* Sk = nullptr;
-    double2* Yk = nullptr;
+    int* edk_n = nullptr;  // the edits' compaction copies
+    int* edk_ent = nullptr;
+    double2* edk_val = nullptr;
+    int* edk_flag = nullptr;
     int* stk = nullptr;
     int* itk = nullptr;
     double* resk = nullptr;
@@ -388,7 +403,7 @@ struct g2g_batch_s {
     std::vector<hipEvent_t> ev;
     ~g2g_batch_s()
     {
-        dfree(V); dfree(S); dfree(Yrow); dfree(A); dfree(U); dfree(X); dfree(res); dfree(jmax); dfree(mis_part);
+        dfree(V); dfree(S); dfree(ed_n); dfree(ed_ent); dfree(ed_val); dfree(ed_flag); dfree(A); dfree(U); dfree(X); dfree(res); dfree(jmax); dfree(mis_part);
         dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped); dfree(error);
         dfree(V0); dfree(Bs); dfree(Xs); dfree(wmask); dfree(differs); dfree(tile_restart); dfree(v_row);
         dfree(clock_dev);
@@ -397,7 +412,7 @@ struct g2g_batch_s {
         if (own_graph_stream) (void)hipStreamDestroy(graph_stream);
         for (hipEvent_t e : graph_ev)
             if (e) (void)hipEventDestroy(e);
-        dfree(Vk); dfree(Sk); dfree(Yk); dfree(stk); dfree(itk); dfree(resk); dfree(mapk); dfree(it0);
+        dfree(Vk); dfree(Sk); dfree(edk_n); dfree(edk_ent); dfree(edk_val); dfree(edk_flag); dfree(stk); dfree(itk); dfree(resk); dfree(mapk); dfree(it0);
         dfree(mask); dfree(maskk); dfree(xs); dfree(xsk); dfree(sx); dfree(pend);
         dfree(budget); dfree(warm); dfree(requeued); dfree(V0k); dfree(it0k); dfree(budgetk); dfree(warmk);
         for (void*& q : stage) dfree(q);
@@ -701,6 +716,49 @@ void refine(const g2g_plan_s* P, Real* A, Real* U, Real* X, Real* Bs, Real* Xs, 
     }
 }
 
+// The batch's row edits as the kernels take them (all null for a batch
+// sharing one Ybus).
+RowEdits row_edits(const g2g_batch_s* b)
+{
+    return RowEdits{b->ed_n, b->ed_ent, b->ed_val, b->ed_flag, b->ed_cap};
+}
+
+// flags of the Ybus rows that rows [0, rows) edit, afresh
+void edit_flags(g2g_batch_s* b, const int* n_ed, const int* ent, int rows, int* flag)
+{
+    const NrPlan& nr = b->plan->nr;
+    const long long nf = static_cast<long long>(b->T) * nr.n;
+    k_fill_int<<<blocks_for(nf, 256), 256, 0, b->stream>>>(flag, 0, nf);
+    if (rows > 0)
+        k_edit_flags<<<blocks_for(rows, 256), 256, 0, b->stream>>>(n_ed, ent, b->ed_cap, rows, b->plan->d_entry_row,
+                                                                   nr.n, flag);
+    HIPCHK(hipGetLastError());
+}
+
+// room for cap edits per row (the lists kept when keep is set)
+void ensure_edit_cap(g2g_batch_s* b, int cap, bool keep)
+{
+    if (cap <= b->ed_cap) return;
+    cap = (cap + 7) / 8 * 8;
+    const size_t n = static_cast<size_t>(b->rows_pad) * cap;
+    int* ent = dalloc<int>(n);
+    double2* val = dalloc<double2>(n);
+    if (keep && b->ed_cap > 0) {
+        const long long m = static_cast<long long>(b->T) * b->ed_cap * kLanes;
+        k_regrow_edits<int><<<blocks_for(m, 256), 256, 0, b->stream>>>(b->ed_ent, b->ed_cap, ent, cap, b->T);
+        k_regrow_edits<double2><<<blocks_for(m, 256), 256, 0, b->stream>>>(b->ed_val, b->ed_cap, val, cap, b->T);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
+    DeviceScratch::free_one(b->ed_ent);
+    DeviceScratch::free_one(b->ed_val);
+    DeviceScratch::free_one(b->edk_ent);  // compaction copies follow the capacity
+    DeviceScratch::free_one(b->edk_val);
+    b->ed_ent = ent;
+    b->ed_val = val;
+    b->ed_cap = cap;
+}
+
 // Refinement rounds of a Newton step with the residual in FP64 from V
 // (k_ir_residual), against the solve's factors: r = F(V) - J(V) x over the
 // A stream's rhs, the factorization again (bitwise the same factors), the
@@ -716,7 +774,7 @@ void refine_fp64(g2g_batch_s* b, double eps, int T)
     const bool sp = b->split_layout;  // the tail's split walk: its layout's tables and launches
     for (int k = 0; k < b->ir_steps; ++k) {
         IrArgs<Real> a{sp ? P->d_k1_bus_s : P->d_k1_bus, sp ? P->d_k1_ent_s : P->d_k1_ent, nr.k1_nbus, P->d_Y,
-                       b->Yrow, b->V, b->S, b->mask, P->d_colmap, A, X, Xs, b->tile_active, nr.n, nr.nY, nr.d,
+                       row_edits(b), b->V, b->S, b->mask, P->d_colmap, A, X, Xs, b->tile_active, nr.n, nr.nY, nr.d,
                        b->a_stride};
         if (b->per_row_y) k_ir_residual<Real, true><<<T, kLanes * kMismatchWaves, 0, b->stream>>>(a);
         else k_ir_residual<Real, false><<<T, kLanes * kMismatchWaves, 0, b->stream>>>(a);
@@ -768,6 +826,7 @@ int default_mis_splits()
 }
 
 constexpr int kRestartChunks = 16;  // workgroups per tile of a restart copy
+
 
 // The tiles that can hold iterating rows: the compacted rows' tiles, or all.
 int live_tiles(const g2g_batch_s* b) { return b->compact_R > 0 ? (b->compact_R + kLanes - 1) / kLanes : b->T; }
@@ -893,7 +952,7 @@ void launch_mismatch(g2g_batch_s* b, bool with_j, int iter, int max_iter, double
     }
     const int* k1b = b->split_layout ? b->plan->d_k1_bus_s : b->plan->d_k1_bus;
     const int* k1e = b->split_layout ? b->plan->d_k1_ent_s : transposed ? b->plan->d_k1_ent_t : b->plan->d_k1_ent;
-    MismatchArgs<Real> a{k1b, k1e, P.k1_nbus, b->plan->d_Y, b->Yrow, b->V, b->S,
+    MismatchArgs<Real> a{k1b, k1e, P.k1_nbus, b->plan->d_Y, row_edits(b), b->V, b->S,
                          static_cast<Real*>(b->A), P.n, P.nY, b->a_stride, b->res, b->jmax, b->status, b->iters,
                          b->tile_active, only ? b->n_active + 2 : b->n_active, b->active_rows, iter, max_iter, tol,
                          freeze, growth,
@@ -984,7 +1043,6 @@ void compact_rows(g2g_batch_s* b)
     if (!b->Vk) {
         b->Vk = dalloc<double2>(rp * nr.n);
         b->Sk = dalloc<double2>(rp * nr.n);
-        if (b->per_row_y) b->Yk = dalloc<double2>(rp * nr.nY);
         b->stk = dalloc<int>(rp);
         b->itk = dalloc<int>(rp);
         b->resk = dalloc<double>(rp);
@@ -992,7 +1050,13 @@ void compact_rows(g2g_batch_s* b)
         // lanes beyond the compacted rows stay finite (|V| = 1)
         HIPCHK(hipMemcpyAsync(b->Vk, b->V, rp * nr.n * sizeof(double2), hipMemcpyDeviceToDevice, s));
         HIPCHK(hipMemsetAsync(b->Sk, 0, rp * nr.n * sizeof(double2), s));
-        if (b->Yk) HIPCHK(hipMemcpyAsync(b->Yk, b->Yrow, rp * nr.nY * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    }
+    if (b->per_row_y && !b->edk_ent) {
+        const size_t m = rp * b->ed_cap;
+        if (!b->edk_n) b->edk_n = dalloc<int>(rp);
+        if (!b->edk_flag) b->edk_flag = dalloc<int>(static_cast<size_t>(b->T) * nr.n);
+        b->edk_ent = dalloc<int>(m);
+        b->edk_val = dalloc<double2>(m);
     }
     const int rows = b->active_rows;
     std::vector<int> st(static_cast<size_t>(rows));
@@ -1012,9 +1076,15 @@ void compact_rows(g2g_batch_s* b)
     const unsigned gv = blocks_for(static_cast<long long>(R) * nr.n, 256);
     k_move_rows<double2><<<gv, 256, 0, s>>>(b->V, b->Vk, b->mapk, R, nr.n, 1);
     k_move_rows<double2><<<gv, 256, 0, s>>>(b->S, b->Sk, b->mapk, R, nr.n, 1);
-    if (b->per_row_y)
-        k_move_rows<double2><<<blocks_for(static_cast<long long>(R) * nr.nY, 256), 256, 0, s>>>(b->Yrow, b->Yk,
-                                                                                               b->mapk, R, nr.nY, 1);
+    if (b->per_row_y) {  // the rows' edits move with them (lanes beyond: none)
+        k_fill_int<<<blocks_for(rp, 256), 256, 0, s>>>(b->edk_n, 0, rp);
+        k_move_vals<int><<<blocks_for(R, 256), 256, 0, s>>>(b->ed_n, b->edk_n, b->mapk, R, 1, 0);
+        const unsigned ge = blocks_for(static_cast<long long>(R) * b->ed_cap, 256);
+        k_move_rows<int><<<ge, 256, 0, s>>>(b->ed_ent, b->edk_ent, b->mapk, R, b->ed_cap, 1);
+        k_move_rows<double2><<<ge, 256, 0, s>>>(b->ed_val, b->edk_val, b->mapk, R, b->ed_cap, 1);
+        HIPCHK(hipGetLastError());
+        edit_flags(b, b->edk_n, b->edk_ent, R, b->edk_flag);
+    }
     if (b->mask) {
         if (!b->maskk) b->maskk = dalloc<unsigned char>(rp * nr.n);
         HIPCHK(hipMemsetAsync(b->maskk, 0, rp * nr.n, s));
@@ -1055,7 +1125,12 @@ void compact_rows(g2g_batch_s* b)
     HIPCHK(hipGetLastError());
     std::swap(b->V, b->Vk);
     std::swap(b->S, b->Sk);
-    if (b->per_row_y) std::swap(b->Yrow, b->Yk);
+    if (b->per_row_y) {
+        std::swap(b->ed_n, b->edk_n);
+        std::swap(b->ed_ent, b->edk_ent);
+        std::swap(b->ed_val, b->edk_val);
+        std::swap(b->ed_flag, b->edk_flag);
+    }
     b->compact_saved = rows;
     b->compact_R = R;
     b->active_rows = R;
@@ -1079,7 +1154,12 @@ void uncompact_rows(g2g_batch_s* b)
     }
     std::swap(b->V, b->Vk);
     std::swap(b->S, b->Sk);
-    if (b->per_row_y) std::swap(b->Yrow, b->Yk);
+    if (b->per_row_y) {
+        std::swap(b->ed_n, b->edk_n);
+        std::swap(b->ed_ent, b->edk_ent);
+        std::swap(b->ed_val, b->edk_val);
+        std::swap(b->ed_flag, b->edk_flag);
+    }
     if (b->mask) std::swap(b->mask, b->maskk);
     if (b->restart_on) {  // the fallback start and first counts go back unread
         if (!b->v0_row) std::swap(b->V0, b->V0k);
@@ -2031,11 +2111,12 @@ int g2g_batch_create(g2g_plan_t P, int32_t rows, int32_t precision, int32_t per_
             HIPCHK(hipStreamSynchronize(b->stream));
             dfree(d1);
         }
-        if (b->per_row_y) {
-            b->Yrow = dalloc<double2>(rp * nr.nY);
-            k_ybus_broadcast<<<blocks_for(static_cast<long long>(rp) * nr.nY, 256), 256, 0, b->stream>>>(
-                P->d_Y, nr.nY, b->rows_pad, b->Yrow);
-            HIPCHK(hipGetLastError());
+        if (b->per_row_y) {  // no edits yet: every row on the shared Ybus
+            b->ed_n = dalloc<int>(rp);
+            k_fill_int<<<blocks_for(rp, 256), 256, 0, b->stream>>>(b->ed_n, 0, rp);
+            b->ed_flag = dalloc<int>(static_cast<size_t>(b->T) * nr.n);
+            ensure_edit_cap(b.get(), 16, false);
+            edit_flags(b.get(), b->ed_n, b->ed_ent, 0, b->ed_flag);
         }
         b->a_stride = a_stride_of(nr.lu, static_cast<int>(b->real_size()));
         b->u_stride = u_stride_of(nr.lu, static_cast<int>(b->real_size()));
@@ -2121,8 +2202,10 @@ int g2g_batch_set_ybus_edits(g2g_batch_t b, const int32_t* row_ptr, const int32_
             if (row_ptr[r + 1] < row_ptr[r]) throw std::invalid_argument("row_ptr is not monotone");
         for (int q = 0; q < ne; ++q)
             if (entry[q] < 0 || entry[q] >= nr.nY) throw std::invalid_argument("edit entry out of range");
-        k_ybus_broadcast<<<blocks_for(static_cast<long long>(b->rows_pad) * nr.nY, 256), 256, 0, b->stream>>>(
-            b->plan->d_Y, nr.nY, b->rows_pad, b->Yrow);
+        int most = 0;
+        for (int r = 0; r < rows; ++r) most = std::max(most, row_ptr[r + 1] - row_ptr[r]);
+        ensure_edit_cap(b, most, false);
+        k_fill_int<<<blocks_for(b->rows_pad, 256), 256, 0, b->stream>>>(b->ed_n, 0, b->rows_pad);
         if (ne > 0) {
             std::vector<int> rp(row_ptr, row_ptr + rows + 1), en(entry, entry + ne);
             std::vector<double2> de(static_cast<size_t>(ne));
@@ -2130,14 +2213,16 @@ int g2g_batch_set_ybus_edits(g2g_batch_t b, const int32_t* row_ptr, const int32_
             int* d_rp = dupload(rp);
             int* d_en = dupload(en);
             double2* d_de = dupload(de);
-            k_ybus_edits<<<blocks_for(rows, 256), 256, 0, b->stream>>>(d_rp, d_en, d_de, rows, nr.nY, b->Yrow,
-                                                                       nullptr);
+            k_set_row_edits<<<blocks_for(rows, 256), 256, 0, b->stream>>>(d_rp, d_en, d_de, rows, nullptr, b->ed_cap,
+                                                                          b->ed_n, b->ed_ent, b->ed_val);
             HIPCHK(hipGetLastError());
             HIPCHK(hipStreamSynchronize(b->stream));
             dfree(d_rp);
             dfree(d_en);
             dfree(d_de);
         }
+        edit_flags(b, b->ed_n, b->ed_ent, b->rows_pad, b->ed_flag);
+        HIPCHK(hipStreamSynchronize(b->stream));
     });
 }
 
@@ -2675,21 +2760,34 @@ int g2g_batch_stream_fill(g2g_batch_t b, int32_t n, const int32_t* slots, const 
         const unsigned g = blocks_for(static_cast<long long>(n) * nr.n, 256);
         k_rows_to_slots<double2><<<g, 256, 0, st>>>(rows_src(S, lds, kStS), lds, n, nr.n, d_slots, b->S);
         k_rows_to_slots<double2><<<g, 256, 0, st>>>(rows_src(V0, ldv, kStV), ldv, n, nr.n, d_slots, b->V);
-        if (b->per_row_y) {
-            k_ybus_reset_slots<<<blocks_for(static_cast<long long>(n) * nr.nY, 256), 256, 0, st>>>(
-                d_slots, n, b->plan->d_Y, nr.nY, b->Yrow);
+        if (b->per_row_y) {  // the slots' edits (none without edit lists), then the tiles' flags
+            const int* d_rp = nullptr;
+            const int* d_en = nullptr;
+            const double2* d_de = nullptr;
             if (edit_ptr && edit_ptr[n] > 0) {
                 const int ne = edit_ptr[n];
+                int most = 0;
+                for (int r = 0; r < n; ++r) {
+                    if (edit_ptr[r + 1] < edit_ptr[r]) throw std::invalid_argument("edit_ptr is not monotone");
+                    most = std::max(most, edit_ptr[r + 1] - edit_ptr[r]);
+                }
                 for (int q = 0; q < ne; ++q)
                     if (edit_ent[q] < 0 || edit_ent[q] >= nr.nY) throw std::invalid_argument("edit entry out of range");
-                int* d_rp = stage_buf<int>(b, kStEdPtr, n + 1);
-                int* d_en = stage_buf<int>(b, kStEdEnt, ne);
-                double2* d_de = stage_buf<double2>(b, kStEdDelta, ne);
-                HIPCHK(hipMemcpyAsync(d_rp, edit_ptr, (n + 1) * sizeof(int), hipMemcpyHostToDevice, st));
-                HIPCHK(hipMemcpyAsync(d_en, edit_ent, ne * sizeof(int), hipMemcpyHostToDevice, st));
-                HIPCHK(hipMemcpyAsync(d_de, edit_delta, ne * sizeof(double2), hipMemcpyHostToDevice, st));
-                k_ybus_edits<<<blocks_for(n, 256), 256, 0, st>>>(d_rp, d_en, d_de, n, nr.nY, b->Yrow, d_slots);
+                ensure_edit_cap(b, most, true);
+                int* rp_ = stage_buf<int>(b, kStEdPtr, n + 1);
+                int* en_ = stage_buf<int>(b, kStEdEnt, ne);
+                double2* de_ = stage_buf<double2>(b, kStEdDelta, ne);
+                HIPCHK(hipMemcpyAsync(rp_, edit_ptr, (n + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(en_, edit_ent, ne * sizeof(int), hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(de_, edit_delta, ne * sizeof(double2), hipMemcpyHostToDevice, st));
+                d_rp = rp_;
+                d_en = en_;
+                d_de = de_;
             }
+            k_set_row_edits<<<blocks_for(n, 256), 256, 0, st>>>(d_rp, d_en, d_de, n, d_slots, b->ed_cap, b->ed_n,
+                                                                b->ed_ent, b->ed_val);
+            HIPCHK(hipGetLastError());
+            edit_flags(b, b->ed_n, b->ed_ent, b->rows_pad, b->ed_flag);
         }
         const unsigned gs = blocks_for(n, 256);
         k_set_slots<<<gs, 256, 0, st>>>(d_slots, n, b->status, kActive);

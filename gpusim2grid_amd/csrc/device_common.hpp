@@ -7,13 +7,25 @@
 
 namespace g2g {
 
+// Each row's Ybus value edits (outage batches): row r's Ybus is Y minus its
+// deltas, the tripped branches' terms.  Tiled like the rows: n_ed[row],
+// ent / val [T][cap][64].  flag[tile][bus] is set when some row of the tile
+// edits an entry of Ybus row `bus`: only those buses look the edits up.
+struct RowEdits {
+    const int* n_ed;
+    const int* ent;
+    const double2* val;
+    const int* flag;
+    int cap;
+};
+
 template <typename Real>
 struct MismatchArgs {
     const int* k1_bus;      // 6 ints per bus (mismatch program)
     const int* k1_ent;      // 6 ints per Ybus entry
     int nbus;
     const double2* Y;       // shared Ybus values [nY]
-    const double2* Yrow;    // per-row Ybus values [T][nY][64] (outage batches)
+    RowEdits ed;            // outage batches: each row's value edits on Y
     const double2* V;       // [T][n][64]
     const double2* S;       // [T][n][64]
     Real* A;                // A stream [T][a_stride][64]
@@ -176,8 +188,8 @@ struct IrArgs {
     const int* k1_bus;          // the mismatch's bus and entry tables
     const int* k1_ent;
     int nbus;
-    const double2* Y;           // shared Ybus values, or
-    const double2* Yrow;        // per-row values [T][nY][64]
+    const double2* Y;           // shared Ybus values
+    RowEdits ed;                // outage batches: each row's edits
     const double2* V;           // [T][n][64]
     const double2* S;           // [T][n][64]
     const unsigned char* mask;  // nullable: islanded buses [T][n][64] (identity rows)

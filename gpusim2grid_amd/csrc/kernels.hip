@@ -76,9 +76,23 @@ __device__ __forceinline__ double nan_max(double m, double v)
 constexpr int kMisBatch = 4;  // Ybus entries of a bus gathered together
 constexpr int kMisBuses = 1;  // buses a wave works on at once
 
+// Ybus entry k of this lane's row: the shared value minus the row's edits of
+// entry k (an edited bus's entries only)
+__device__ __forceinline__ double2 edited_y(const RowEdits& ed, size_t tile, int lane, int ned, int k, double2 y)
+{
+    for (int t = 0; t < ned; ++t)
+        if (ed.ent[tidx(tile, ed.cap, t, lane)] == k) {
+            const double2 d = ed.val[tidx(tile, ed.cap, t, lane)];
+            y.x -= d.x;
+            y.y -= d.y;
+        }
+    return y;
+}
+
 // The running sums of one bus of k_mismatch.
 struct MisBus {
     int i, e0, e1, fp, fq, xb;
+    int ned;  // the lane row's edit count when some row of the tile edits this bus, else 0
     bool ok, masked;
     double2 vi, sb, I;
     double dr, di;
@@ -135,7 +149,8 @@ __device__ __forceinline__ void mis_gather(const MismatchArgs<Real>& a, size_t t
         const int* en = a.k1_ent + 6 * e;
         j[q] = eb + q < b.e1 ? en[0] : -1;
         const int yk = en[1];
-        y[q] = PerRowY ? a.Yrow[tidx(tile, a.nY, yk, lane)] : a.Y[yk];
+        y[q] = a.Y[yk];
+        if (PerRowY && b.ned > 0) y[q] = edited_y(a.ed, tile, lane, b.ned, yk, y[q]);
         vj[q] = (j[q] >= 0 && j[q] != b.i) ? a.V[tidx(tile, a.n, j[q], lane)] : b.vi;
     }
 }
@@ -251,6 +266,7 @@ __global__ __launch_bounds__(kLanes* kMismatchWaves) void k_mismatch(MismatchArg
                 // an islanded bus (apply_bus_mask_kernel, acpf_nr_kernels.cu:219-244):
                 // identity rows in J, zero mismatch, out of the residual
                 b[g].masked = a.mask && a.mask[tidx(tile, a.n, b[g].i, lane)];
+                b[g].ned = PerRowY && a.ed.flag[tile * a.n + b[g].i] ? a.ed.n_ed[tile * kLanes + lane] : 0;
                 b[g].I = make_double2(0.0, 0.0);
                 b[g].dr = b[g].di = 0.0;
                 b[g].dent = -1;
@@ -1476,11 +1492,13 @@ __global__ __launch_bounds__(kLanes* kMismatchWaves) void k_ir_residual(IrArgs<R
         const int i = bus[0], e0 = bus[1], e1 = bus[2], fp = bus[3], fq = bus[4];
         const double2 vi = a.V[tidx(tile, a.n, i, lane)];
         const double2 sb = a.S[tidx(tile, a.n, i, lane)];
+        const int ned = PerRowY && a.ed.flag[tile * a.n + i] ? a.ed.n_ed[tile * kLanes + lane] : 0;
         double2 I = make_double2(0.0, 0.0), dI = make_double2(0.0, 0.0);
         for (int e = e0; e < e1; ++e) {
             const int* en = a.k1_ent + 6 * e;
             const int j = en[0];
-            const double2 y = PerRowY ? a.Yrow[tidx(tile, a.nY, en[1], lane)] : a.Y[en[1]];
+            double2 y = a.Y[en[1]];
+            if (PerRowY && ned > 0) y = edited_y(a.ed, tile, lane, ned, en[1], y);
             const double2 vj = j == i ? vi : a.V[tidx(tile, a.n, j, lane)];
             const double2 t = cmul(y, vj), u = cmul(y, dv(j, vj));
             I.x += t.x;
@@ -1605,46 +1623,49 @@ __global__ void k_tiles_to_rows_f64(const Real* __restrict__ in, int rows, int m
         static_cast<double>(in[tidx(static_cast<size_t>(r / kLanes), m, k, r % kLanes)]);
 }
 
-// Per-row Ybus values: broadcast the shared values to every row, then subtract
-// each row's outage deltas (the reference's apply_contingencies_kernel,
-// acpf_nr_kernels.cu:328-348, on the tiled layout).
-__global__ void k_ybus_broadcast(const double2* __restrict__ Y, int nY, int rows_pad, double2* __restrict__ Yrow)
-{
-    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const long long total = static_cast<long long>(rows_pad) * nY;
-    if (gid >= total) return;
-    const int lane = static_cast<int>(gid & (kLanes - 1));
-    const long long rest = gid >> 6;
-    const int k = static_cast<int>(rest % nY);
-    const long long tile = rest / nY;
-    Yrow[tidx(static_cast<size_t>(tile), nY, k, lane)] = Y[k];
-}
-
-// subtracts row r's edits from the per-row values of batch row slots[r]
-// (slots nullable: row r itself)
-__global__ void k_ybus_edits(const int* __restrict__ row_ptr, const int* __restrict__ ent,
-                             const double2* __restrict__ delta, int rows, int nY, double2* __restrict__ Yrow,
-                             const int* __restrict__ slots)
+// Row edits (outage batches; the reference's apply_contingencies_kernel,
+// acpf_nr_kernels.cu:328-348, kept as edits instead of a per-row Ybus copy).
+// Rows slots[r] (slots nullable: row r) take row r's edits of the CSR lists.
+__global__ void k_set_row_edits(const int* __restrict__ row_ptr, const int* __restrict__ ent,
+                                const double2* __restrict__ delta, int R, const int* __restrict__ slots, int cap,
+                                int* __restrict__ n_ed, int* __restrict__ ed_ent, double2* __restrict__ ed_val)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
+    if (r >= R) return;
     const int b = slots ? slots[r] : r;
-    for (int q = row_ptr[r]; q < row_ptr[r + 1]; ++q) {
-        double2* y = Yrow + tidx(static_cast<size_t>(b / kLanes), nY, ent[q], b % kLanes);
-        const double2 v = *y;
-        *y = make_double2(v.x - delta[q].x, v.y - delta[q].y);
+    const size_t tile = static_cast<size_t>(b / kLanes);
+    const int lane = b % kLanes;
+    const int q0 = row_ptr ? row_ptr[r] : 0, cnt = row_ptr ? row_ptr[r + 1] - q0 : 0;
+    n_ed[b] = cnt;
+    for (int t = 0; t < cnt; ++t) {
+        ed_ent[tidx(tile, cap, t, lane)] = ent[q0 + t];
+        ed_val[tidx(tile, cap, t, lane)] = delta[q0 + t];
     }
 }
 
-// the shared Ybus values into the per-row values of batch rows slots[0..R)
-__global__ void k_ybus_reset_slots(const int* __restrict__ slots, int R, const double2* __restrict__ Y, int nY,
-                                   double2* __restrict__ Yrow)
+// flag[tile][bus] = 1 for the Ybus rows the edits of rows [0, rows) touch
+// (entry_row: each Ybus entry's row); the flags start at zero
+__global__ void k_edit_flags(const int* __restrict__ n_ed, const int* __restrict__ ed_ent, int cap, int rows,
+                             const int* __restrict__ entry_row, int n, int* __restrict__ flag)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const size_t tile = static_cast<size_t>(r / kLanes);
+    const int lane = r % kLanes;
+    for (int t = 0; t < n_ed[r]; ++t) flag[tile * n + entry_row[ed_ent[tidx(tile, cap, t, lane)]]] = 1;
+}
+
+// edit lists of capacity cap_old into capacity cap_new (same rows)
+template <typename T>
+__global__ void k_regrow_edits(const T* __restrict__ src, int cap_old, T* __restrict__ dst, int cap_new, int tiles)
 {
     const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gid >= static_cast<long long>(R) * nY) return;
-    const int r = static_cast<int>(gid / nY), k = static_cast<int>(gid - static_cast<long long>(r) * nY);
-    const int b = slots[r];
-    Yrow[tidx(static_cast<size_t>(b / kLanes), nY, k, b % kLanes)] = Y[k];
+    if (gid >= static_cast<long long>(tiles) * cap_old * kLanes) return;
+    const int lane = static_cast<int>(gid % kLanes);
+    const long long q = gid / kLanes;
+    const int t = static_cast<int>(q % cap_old);
+    const size_t tile = static_cast<size_t>(q / cap_old);
+    dst[tidx(tile, cap_new, t, lane)] = src[tidx(tile, cap_old, t, lane)];
 }
 
 // row-major rows [R][m] (row stride ld; ld = 0 broadcasts row 0) into the
@@ -2184,6 +2205,9 @@ template __global__ void k_mismatch<float, true, true>(MismatchArgs<float>);
 template __global__ void k_mismatch<float, false, false>(MismatchArgs<float>);
 template __global__ void k_mismatch<float, true, false>(MismatchArgs<float>);
 template __global__ void k_move_rows<double2>(const double2*, double2*, const int*, int, int, int);
+template __global__ void k_move_rows<int>(const int*, int*, const int*, int, int, int);
+template __global__ void k_regrow_edits<int>(const int*, int, int*, int, int);
+template __global__ void k_regrow_edits<double2>(const double2*, int, double2*, int, int);
 template __global__ void k_move_rows<unsigned char>(const unsigned char*, unsigned char*, const int*, int, int, int);
 template __global__ void k_rows_to_slots<double2>(const double2*, long long, int, int, const int*, double2*);
 template __global__ void k_slots_to_rows<double2>(const double2*, int, int, const int*, double2*, long long);

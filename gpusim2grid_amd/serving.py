@@ -49,8 +49,8 @@ def batch_bytes(plan_info, rows, precision, per_row_ybus):
     b = rp * n * 16 * 2                      # V, Sbus
     b += rp * (plan_info["a_len"] + plan_info["u_len"] + d) * rs
     b += rp * 8 * 8                          # per-row scalars
-    if per_row_ybus:
-        b += rp * nY * 16
+    if per_row_ybus:  # each row's outage edits (16 to start) and the tiles' edited-bus flags
+        b += rp * 16 * 20 + rp // 64 * n * 4
     return int(b * 1.1)
 
 
