@@ -735,6 +735,35 @@ void edit_flags(g2g_batch_s* b, const int* n_ed, const int* ent, int rows, int* 
     HIPCHK(hipGetLastError());
 }
 
+// CSR edit lists with each row's repeated entries merged (their deltas
+// summed), in place of (row_ptr, ent, delta of interleaved pairs)
+struct EditLists {
+    std::vector<int> rp, ent;
+    std::vector<double2> val;
+    int most = 0;
+    EditLists(const int32_t* row_ptr, const int32_t* entry, const double* delta, int rows)
+    {
+        rp.assign(1, 0);
+        for (int r = 0; r < rows; ++r) {
+            const size_t start = ent.size();
+            for (int q = row_ptr[r]; q < row_ptr[r + 1]; ++q) {
+                const double2 d = make_double2(delta[2 * q], delta[2 * q + 1]);
+                size_t k = start;
+                while (k < ent.size() && ent[k] != entry[q]) ++k;
+                if (k < ent.size()) {
+                    val[k].x += d.x;
+                    val[k].y += d.y;
+                } else {
+                    ent.push_back(entry[q]);
+                    val.push_back(d);
+                }
+            }
+            most = std::max(most, static_cast<int>(ent.size() - start));
+            rp.push_back(static_cast<int>(ent.size()));
+        }
+    }
+};
+
 // room for cap edits per row (the lists kept when keep is set)
 void ensure_edit_cap(g2g_batch_s* b, int cap, bool keep)
 {
@@ -2202,17 +2231,13 @@ int g2g_batch_set_ybus_edits(g2g_batch_t b, const int32_t* row_ptr, const int32_
             if (row_ptr[r + 1] < row_ptr[r]) throw std::invalid_argument("row_ptr is not monotone");
         for (int q = 0; q < ne; ++q)
             if (entry[q] < 0 || entry[q] >= nr.nY) throw std::invalid_argument("edit entry out of range");
-        int most = 0;
-        for (int r = 0; r < rows; ++r) most = std::max(most, row_ptr[r + 1] - row_ptr[r]);
-        ensure_edit_cap(b, most, false);
+        const EditLists el(row_ptr, entry, delta, rows);
+        ensure_edit_cap(b, el.most, false);
         k_fill_int<<<blocks_for(b->rows_pad, 256), 256, 0, b->stream>>>(b->ed_n, 0, b->rows_pad);
         if (ne > 0) {
-            std::vector<int> rp(row_ptr, row_ptr + rows + 1), en(entry, entry + ne);
-            std::vector<double2> de(static_cast<size_t>(ne));
-            for (int q = 0; q < ne; ++q) de[q] = make_double2(delta[2 * q], delta[2 * q + 1]);
-            int* d_rp = dupload(rp);
-            int* d_en = dupload(en);
-            double2* d_de = dupload(de);
+            int* d_rp = dupload(el.rp);
+            int* d_en = dupload(el.ent);
+            double2* d_de = dupload(el.val);
             k_set_row_edits<<<blocks_for(rows, 256), 256, 0, b->stream>>>(d_rp, d_en, d_de, rows, nullptr, b->ed_cap,
                                                                           b->ed_n, b->ed_ent, b->ed_val);
             HIPCHK(hipGetLastError());
@@ -2765,21 +2790,19 @@ int g2g_batch_stream_fill(g2g_batch_t b, int32_t n, const int32_t* slots, const 
             const int* d_en = nullptr;
             const double2* d_de = nullptr;
             if (edit_ptr && edit_ptr[n] > 0) {
-                const int ne = edit_ptr[n];
-                int most = 0;
-                for (int r = 0; r < n; ++r) {
+                for (int r = 0; r < n; ++r)
                     if (edit_ptr[r + 1] < edit_ptr[r]) throw std::invalid_argument("edit_ptr is not monotone");
-                    most = std::max(most, edit_ptr[r + 1] - edit_ptr[r]);
-                }
-                for (int q = 0; q < ne; ++q)
+                for (int q = 0; q < edit_ptr[n]; ++q)
                     if (edit_ent[q] < 0 || edit_ent[q] >= nr.nY) throw std::invalid_argument("edit entry out of range");
-                ensure_edit_cap(b, most, true);
+                const EditLists el(edit_ptr, edit_ent, edit_delta, n);
+                const int ne = el.rp[n];
+                ensure_edit_cap(b, el.most, true);
                 int* rp_ = stage_buf<int>(b, kStEdPtr, n + 1);
                 int* en_ = stage_buf<int>(b, kStEdEnt, ne);
                 double2* de_ = stage_buf<double2>(b, kStEdDelta, ne);
-                HIPCHK(hipMemcpyAsync(rp_, edit_ptr, (n + 1) * sizeof(int), hipMemcpyHostToDevice, st));
-                HIPCHK(hipMemcpyAsync(en_, edit_ent, ne * sizeof(int), hipMemcpyHostToDevice, st));
-                HIPCHK(hipMemcpyAsync(de_, edit_delta, ne * sizeof(double2), hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(rp_, el.rp.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(en_, el.ent.data(), ne * sizeof(int), hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(de_, el.val.data(), ne * sizeof(double2), hipMemcpyHostToDevice, st));
                 d_rp = rp_;
                 d_en = en_;
                 d_de = de_;

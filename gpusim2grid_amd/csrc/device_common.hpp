@@ -21,7 +21,7 @@ struct RowEdits {
 
 template <typename Real>
 struct MismatchArgs {
-    const int* k1_bus;      // 6 ints per bus (mismatch program)
+    const int* k1_bus;      // kK1BusWords ints per bus (mismatch program)
     const int* k1_ent;      // 6 ints per Ybus entry
     int nbus;
     const double2* Y;       // shared Ybus values [nY]

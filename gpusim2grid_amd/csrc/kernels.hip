@@ -92,7 +92,11 @@ __device__ __forceinline__ double2 edited_y(const RowEdits& ed, size_t tile, int
 // The running sums of one bus of k_mismatch.
 struct MisBus {
     int i, e0, e1, fp, fq, xb;
-    int ned;  // the lane row's edit count when some row of the tile edits this bus, else 0
+    // the lane row's edits of this bus's Ybus row: up to two kept in
+    // registers (entry ke, delta kv; -1: none), or, with more, ned > 0 and
+    // every entry scans the row's list
+    int ke0, ke1, ned;
+    double2 kv0, kv1;
     bool ok, masked;
     double2 vi, sb, I;
     double dr, di;
@@ -150,7 +154,14 @@ __device__ __forceinline__ void mis_gather(const MismatchArgs<Real>& a, size_t t
         j[q] = eb + q < b.e1 ? en[0] : -1;
         const int yk = en[1];
         y[q] = a.Y[yk];
-        if (PerRowY && b.ned > 0) y[q] = edited_y(a.ed, tile, lane, b.ned, yk, y[q]);
+        if (PerRowY) {
+            if (b.ned > 0) {
+                y[q] = edited_y(a.ed, tile, lane, b.ned, yk, y[q]);
+            } else {
+                if (b.ke0 == yk) y[q] = make_double2(y[q].x - b.kv0.x, y[q].y - b.kv0.y);
+                if (b.ke1 == yk) y[q] = make_double2(y[q].x - b.kv1.x, y[q].y - b.kv1.y);
+            }
+        }
         vj[q] = (j[q] >= 0 && j[q] != b.i) ? a.V[tidx(tile, a.n, j[q], lane)] : b.vi;
     }
 }
@@ -266,7 +277,27 @@ __global__ __launch_bounds__(kLanes* kMismatchWaves) void k_mismatch(MismatchArg
                 // an islanded bus (apply_bus_mask_kernel, acpf_nr_kernels.cu:219-244):
                 // identity rows in J, zero mismatch, out of the residual
                 b[g].masked = a.mask && a.mask[tidx(tile, a.n, b[g].i, lane)];
-                b[g].ned = PerRowY && a.ed.flag[tile * a.n + b[g].i] ? a.ed.n_ed[tile * kLanes + lane] : 0;
+                b[g].ke0 = b[g].ke1 = -1;
+                b[g].ned = 0;
+                if (PerRowY && a.ed.flag[tile * a.n + b[g].i]) {
+                    // the lane row's edits within this bus's Ybus row, scanned once
+                    const int yp0 = bus[7], yp1 = yp0 + (b[g].e1 - b[g].e0);
+                    const int cnt = a.ed.n_ed[tile * kLanes + lane];
+                    int nk = 0;
+                    for (int t = 0; t < cnt; ++t) {
+                        const int en = a.ed.ent[tidx(tile, a.ed.cap, t, lane)];
+                        if (en < yp0 || en >= yp1) continue;
+                        if (nk == 0) {
+                            b[g].ke0 = en;
+                            b[g].kv0 = a.ed.val[tidx(tile, a.ed.cap, t, lane)];
+                        } else if (nk == 1) {
+                            b[g].ke1 = en;
+                            b[g].kv1 = a.ed.val[tidx(tile, a.ed.cap, t, lane)];
+                        }
+                        ++nk;
+                    }
+                    if (nk > 2) b[g].ned = cnt;
+                }
                 b[g].I = make_double2(0.0, 0.0);
                 b[g].dr = b[g].di = 0.0;
                 b[g].dent = -1;

@@ -1469,7 +1469,7 @@ NrPlan build_nr_plan(int n, const int* Yp, const int* Yi, const double* Yx, int 
                 const int e1 = static_cast<int>(ke.size() / 6);
                 const int fp = fpos[P.row_p[i]];
                 const int fq = P.row_q[i] >= 0 ? fpos[P.row_q[i]] : -1;
-                kb.insert(kb.end(), {i, e0, e1, fp, fq, diag, xb_of[i], 0});
+                kb.insert(kb.end(), {i, e0, e1, fp, fq, diag, xb_of[i], Yp[i]});  // its entries: Ybus row i
             }
         };
         k1_tables(P.lu.a_pos_of_entry, P.lu.f_pos, P.k1_bus, P.k1_ent);

@@ -180,7 +180,7 @@ struct NrPlan {
     SymbolicLU lu;
     // mismatch/Jacobian program: buses with a P equation in RCM order;
     // k1_bus[i] = {bus, entry begin, entry end, fpos_p, fpos_q, diag entry,
-    // extension bus index or -1, 0}
+    // extension bus index or -1, first Ybus entry of the bus's row}
     std::vector<int> k1_bus;             // kK1BusWords ints per bus
     // per Ybus entry (in k1 order): {col j, ybus k, a11, a12, a21, a22}
     std::vector<int> k1_ent;             // 6 ints per entry
