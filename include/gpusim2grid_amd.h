@@ -196,7 +196,8 @@ int g2g_batch_set_sbus(g2g_batch_t batch, const double *S, int64_t ld, int32_t r
  * entry[q] for q in [row_ptr[r], row_ptr[r+1]) (replaces the flat patches and
  * apply_contingencies_kernel, contingency_analysis_helper.cpp:163 and
  * acpf_nr_kernels.cu:328-348).  Needs per_row_ybus; resets every row to the
- * plan's base values first.  Host arrays. */
+ * plan's base values first.  A row's repeated entries are summed into one
+ * edit.  Host arrays. */
 int g2g_batch_set_ybus_edits(g2g_batch_t batch, const int32_t *row_ptr, const int32_t *entry, const double *delta,
                              int32_t rows);
 /* Islanded buses (handle_disconnected_grid; replaces build_mask_entries,
