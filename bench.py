@@ -488,6 +488,9 @@ def main():
                 frac=(dom["factor_achieved_gbs"] or 0.0) / HBM_PEAK_GBS,
                 traffic=(FACTOR_TRAFFIC_PER_ROW[dom["real"]] * dom["factor_rows_per_launch"]
                          if args.workload == "c2" else None),
+                traffic_source=("rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, on full k_factor launches "
+                                "(profiles/hbm_traffic_c2_16384.md): bytes per factored row x this run's "
+                                "rows per launch" if args.workload == "c2" else None),
                 alg_bytes_per_launch=dom["factor_alg_bytes_per_launch"], launch_ms=dom["factor_launch_ms"])
     line = dict(
         metric=METRIC, value=head["pfps"], unit="PF/s", n_gpus=1, steps=args.steps, warmup=args.warmup,

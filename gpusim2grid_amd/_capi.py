@@ -29,7 +29,7 @@ class PlanInfo(ctypes.Structure):
         "n", "nY", "d", "nJ", "npvpq", "npq", "slack", "nnz", "nLU", "n_updates", "max_c", "height",
         "slots_f", "slots_b", "a_len", "u_len", "n_steps", "walkers", "n_top", "bwalkers",
         "u_stride_f64", "u_stride_f32", "n_ext", "n_xs", "sa_col", "n_split", "slots_fs", "slots_bs",
-        "pend_len", "split_a_len", "split_u_len")]
+        "pend_len", "split_a_len", "split_u_len", "a_stride_f64", "a_stride_f32")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -2043,6 +2043,8 @@ int g2g_plan_get_info(g2g_plan_t P, g2g_plan_info* info)
         info->pend_len = s.pend_len;
         info->split_a_len = s.split_a_len;
         info->split_u_len = s.split_u_len;
+        info->a_stride_f64 = a_stride_of(s, 8);
+        info->a_stride_f32 = a_stride_of(s, 4);
         if (P->grid) {
             info->n_ext = P->nr.n_ext;
             info->n_xs = P->nr.xt.n_xs;

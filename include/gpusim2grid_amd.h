@@ -82,6 +82,7 @@ typedef struct {
     int32_t n_split;                         /* split walk: subtree sets per tile (0: none) */
     int32_t slots_fs, slots_bs, pend_len;    /* split walk: slots and pending contributions per tile */
     int32_t split_a_len, split_u_len;        /* split walk: its A- and U-stream lengths */
+    int32_t a_stride_f64, a_stride_f32;      /* A positions a tile holds in a batch of this precision */
 } g2g_plan_info;
 
 typedef struct {

@@ -273,6 +273,37 @@ def test_u_stride_covers_every_whole_chunk_read(monkeypatch, wf, wb):
                     assert s[6] + -(-s[7] // kch) * kch <= stride
 
 
+@pytest.mark.parametrize("wf", [1, 2])
+def test_a_stride_covers_every_whole_chunk_read(monkeypatch, wf):
+    """The factorization's A producers stream each walker's segment in whole
+    chunks from its start (full chunks, or half chunks in the half-ring
+    geometry two FP32 walkers may take), the program producers stream whole
+    program chunks, and the split walk's producers do the same over its own
+    segments: a batch's A stride, and each program's length, reach past every
+    such read."""
+    monkeypatch.setenv("G2G_WALKERS", str(wf))
+    for g in (syn.grid_c1(), syn.grid_c2()):
+        P = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+        inf = P.info()
+        assert inf["walkers"] == wf
+        for chunk, stride in ((16, inf["a_stride_f64"]), (32, inf["a_stride_f32"])):
+            assert stride % chunk == 0 and stride >= max(inf["a_len"], inf["split_a_len"])
+            for kch in (chunk, chunk // 2):  # own geometry, half rings
+                for s in P.array("segs").reshape(-1, 9):
+                    assert s[4] + -(-s[5] // kch) * kch <= stride
+            if inf["n_split"] > 0:  # the split walk's factorization rings (two-walker geometry)
+                for s in P.array("split_segs").reshape(-1, 9):
+                    assert s[4] + -(-s[5] // chunk) * chunk <= stride
+        for name, prog in (("segs", "prog_f"), ("bsegs", "prog_b"), ("split_segs", "prog_fs"),
+                           ("split_bsegs", "prog_bs")):
+            if name.startswith("split") and inf["n_split"] == 0:
+                continue
+            n_words = P.array(prog).size
+            for s in P.array(name).reshape(-1, 9):
+                off, ln = (s[0], s[1]) if "bsegs" not in name else (s[2], s[3])
+                assert ln % 512 == 0 and off + ln <= n_words
+
+
 def test_split_walk_programs_solve_like_the_main_walk():
     """The split walk (fine sets, middle parts, top part, each on slots of its
     own, pending targets through the dump / gather records, top x through the
