@@ -598,13 +598,13 @@ void launch_factor(const g2g_plan_s* P, const Real* A, Real* U, int a_stride, in
     HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     if constexpr (sizeof(Real) == 4) {
         if (rg == kHalfRingGeometry) {
-            k_factor<Real, 2, false, kHalfRingGeometry><<<T, kLanes * walk_waves(2), lds, st>>>(a);
+            k_factor<Real, 2, false, kHalfRingGeometry><<<T, kLanes * factor_walk_waves(2), lds, st>>>(a);
             HIPCHK(hipGetLastError());
             return;
         }
     }
-    if (s.walkers == 2) k_factor<Real, 2, false, 0><<<T, kLanes * walk_waves(2), lds, st>>>(a);
-    else k_factor<Real, 1, false, 0><<<T, kLanes * walk_waves(1), lds, st>>>(a);
+    if (s.walkers == 2) k_factor<Real, 2, false, 0><<<T, kLanes * factor_walk_waves(2), lds, st>>>(a);
+    else k_factor<Real, 1, false, 0><<<T, kLanes * factor_walk_waves(1), lds, st>>>(a);
     HIPCHK(hipGetLastError());
 }
 

@@ -571,16 +571,19 @@ template <>
 struct StreamShape<float> {
     static constexpr int kCh = kChunkF32, kNch = kRingSlotsF32;
 };
-using ProgRing = Ring<int, kProgChunk, kProgSlots, false, false>;
+using ProgRing = Ring<int, kProgChunk, kProgSlots, false, false, kProgMirror>;
 using ProgReader = ProgRing::Reader;
 
-// NW program words from position p (4-aligned) as 16-byte broadcast reads
+// NW program words from position p (4-aligned) as 16-byte broadcast reads:
+// one base address and immediate offsets (a record running past the ring's
+// end reads on into its mirror)
 template <int NW>
 __device__ __forceinline__ void read_words(const ProgReader& P, int p, v4u (&w)[NW / 4])
 {
-    const v4u* base = reinterpret_cast<const v4u*>(P.ring);
+    static_assert(NW <= kProgMirror, "a record read at once fits the ring's mirror");
+    const v4u* base = reinterpret_cast<const v4u*>(P.ring) + ((p >> 2) & (ProgRing::kRing / 4 - 1));
 #pragma unroll
-    for (int i = 0; i < NW / 4; ++i) w[i] = base[((p >> 2) + i) & (ProgRing::kRing / 4 - 1)];
+    for (int i = 0; i < NW / 4; ++i) w[i] = base[i];
 }
 
 __device__ __forceinline__ int prog_word(ProgReader& P, int p)
@@ -991,7 +994,7 @@ __device__ __forceinline__ bool head_ok(int h, int max_c)
 }
 
 // Workgroup over one tile of 64 rows: W walker wavefronts walk their
-// program segments (LDS only, plus the U-stream stores); value_producers(W)
+// program segments (LDS only, plus the U-stream stores); factor_value_producers(W)
 // waves stream the walkers' A segments into their LDS rings (split evenly
 // between the walkers) and W waves stream the program segments.
 // SPLIT: one walker per workgroup over one program of the split walk (the
@@ -1000,7 +1003,7 @@ __device__ __forceinline__ bool head_ok(int h, int max_c)
 template <int W, bool SPLIT>
 __host__ __device__ constexpr int factor_waves()
 {
-    return SPLIT ? 2 + kSplitProducers : walk_waves(W);
+    return SPLIT ? 2 + kSplitProducers : factor_walk_waves(W);
 }
 
 template <typename Real, int W, bool SPLIT, int RG>
@@ -1011,7 +1014,7 @@ __global__ __launch_bounds__((kLanes * factor_waves<W, SPLIT>())) void k_factor(
     constexpr int kCh = value_chunk(sizeof(Real), RW), kNch = value_ring_slots(sizeof(Real), RW);
     static_assert((kNch - 2) * kCh >= 6 + 4 * kFastC2 - 1, "the ring window spans a two-column step's values");
     using ARing = Ring<Real, kCh, kNch, false, true>;
-    constexpr int kNP = SPLIT ? kSplitProducers : value_producers(W), kP = kNP / W;
+    constexpr int kNP = SPLIT ? kSplitProducers : factor_value_producers(W), kP = kNP / W;
     static_assert(kP * W == kNP, "producers split evenly");
     static_assert(sizeof(typename ARing::Ctl) + sizeof(ProgRing::Ctl) <= kWalkerCtlBytes, "control blocks");
     const int lane = threadIdx.x & (kLanes - 1);

@@ -23,6 +23,7 @@ constexpr int kRingSlotsF32 = 8;    // 64 KB ring
 constexpr int kValueProducers = 6;  // wavefronts streaming a value stream (one chunk in flight each)
 constexpr int kProgChunk = 512;     // program words per chunk (2 KB)
 constexpr int kProgSlots = 4;       // 8 KB ring
+constexpr int kProgMirror = 128;    // program words copied after the ring (the longest record read at once)
 
 // Programs (one record per pivot, padded to 4 words so the kernels read a
 // record with 16-byte loads):
@@ -94,6 +95,12 @@ constexpr int kWalkFlagBytes = 1280;
 // for two (eight waves either way, which leaves 256 VGPRs per wave)
 __host__ __device__ constexpr int value_producers(int W) { return W == 1 ? kValueProducers : 2 * W; }
 __host__ __device__ constexpr int walk_waves(int W) { return 2 * W + value_producers(W); }
+// value-stream producers of a factorization workgroup: as above, except two
+// walkers, which may take more (their walk runs at up to 168 VGPRs, so twelve
+// waves fit a CU)
+constexpr int kFactorProducers2 = 4;
+__host__ __device__ constexpr int factor_value_producers(int W) { return W == 2 ? kFactorProducers2 : value_producers(W); }
+__host__ __device__ constexpr int factor_walk_waves(int W) { return 2 * W + factor_value_producers(W); }
 // value ring geometry of a walker: positions per chunk and ring slots (four
 // walkers stream half-size chunks, so their rings fit the LDS together)
 __host__ __device__ constexpr int value_chunk(int real_size, int W = 1)
@@ -108,7 +115,7 @@ __host__ __device__ constexpr int value_ring_slots(int real_size, int W)
 __host__ __device__ constexpr int walk_ring_bytes(int real_size, int W, int RW)
 {
     return W * (kWalkerCtlBytes + value_chunk(real_size, RW) * value_ring_slots(real_size, RW) * kLanes * real_size +
-                kProgChunk * kProgSlots * 4) +
+                (kProgChunk * kProgSlots + kProgMirror) * 4) +
            kWalkFlagBytes;
 }
 __host__ __device__ constexpr int walk_ring_bytes(int real_size, int W) { return walk_ring_bytes(real_size, W, W); }
@@ -128,7 +135,7 @@ constexpr int kSplitRingB = 4;
 __host__ __device__ constexpr int split_ring_bytes(int real_size, int RW)
 {
     return kWalkerCtlBytes + value_chunk(real_size, RW) * value_ring_slots(real_size, RW) * kLanes * real_size +
-           kProgChunk * kProgSlots * 4 + kWalkFlagBytes;
+           (kProgChunk * kProgSlots + kProgMirror) * 4 + kWalkFlagBytes;
 }
 
 // XCDs of the device: a launch's workgroups go to them round robin

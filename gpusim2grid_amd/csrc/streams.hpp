@@ -70,22 +70,31 @@ struct RingCtl {
 // PerLane: element (pos, lane) stored [pos][64] (a tile's value stream); else
 // one element per position shared by every lane (the program).
 // Reverse: the stream is consumed from its end towards position 0.
+// MIRROR (shared forward streams): the ring is followed by a copy of its
+// first MIRROR positions, written with the chunk of slot 0, so a run of up
+// to MIRROR positions starting anywhere in the ring reads contiguously.
 // The global buffer must be readable up to nch * CH positions.
-template <typename T, int CH, int NCH, bool Reverse, bool PerLane>
+template <typename T, int CH, int NCH, bool Reverse, bool PerLane, int MIRROR = 0>
 struct Ring {
     static constexpr int kElems = PerLane ? CH * kLanes : CH;
     static constexpr int kBytes = kElems * static_cast<int>(sizeof(T));
     static constexpr int kV4 = kBytes / 16 / kLanes;  // 16-byte loads per lane per chunk
     static_assert(kV4 >= 1 && kBytes % (16 * kLanes) == 0, "chunk must be a whole number of wave loads");
     static_assert((CH & (CH - 1)) == 0 && (NCH & (NCH - 1)) == 0, "powers of two");
+    static_assert(MIRROR == 0 || (!PerLane && !Reverse && MIRROR <= CH && MIRROR * sizeof(T) % 16 == 0),
+                  "a mirror follows a shared forward ring and copies part of one chunk");
     static constexpr int kRing = CH * NCH;
-    static constexpr int kLdsBytes = kBytes * NCH;
+    static constexpr int kLdsBytes = kBytes * NCH + MIRROR * static_cast<int>(sizeof(T));
     using Ctl = RingCtl<NCH>;
 
     // global chunk of consumption-order chunk c
     __device__ static __forceinline__ int phys(int c, int nch) { return Reverse ? nch - 1 - c : c; }
-    // consumption-order chunk of global position p
-    __device__ static __forceinline__ int chunk_of(int p, int nch) { return Reverse ? nch - 1 - p / CH : p / CH; }
+    // consumption-order chunk of global position p (p >= 0)
+    __device__ static __forceinline__ int chunk_of(int p, int nch)
+    {
+        const int c = static_cast<int>(static_cast<unsigned>(p) / CH);
+        return Reverse ? nch - 1 - c : c;
+    }
 
     __device__ static void init_ctl(Ctl* ctl, int lane)
     {
@@ -120,9 +129,15 @@ struct Ring {
             }
             // data slot by global chunk (what Reader::at addresses), tag by
             // consumption-order chunk (what Reader::wait checks)
-            v4u* dst = r + static_cast<size_t>(phys(c, nch) & (NCH - 1)) * (kBytes / 16) + lane;
+            const int slot = phys(c, nch) & (NCH - 1);
+            v4u* dst = r + static_cast<size_t>(slot) * (kBytes / 16) + lane;
 #pragma unroll
             for (int i = 0; i < kV4; ++i) dst[i * kLanes] = st[i];
+            if (MIRROR > 0 && slot == 0) {
+#pragma unroll
+                for (int i = 0; i < kV4; ++i)
+                    if (i * kLanes + lane < MIRROR * static_cast<int>(sizeof(T)) / 16) dst[kRing * sizeof(T) / 16 + i * kLanes] = st[i];
+            }
             lds_store_release(&ctl->tag[c & (NCH - 1)], c + 1);
         }
     }
@@ -135,6 +150,8 @@ struct Ring {
         int nch;
         int ready;       // chunks [lo, ready) known to be in the ring
         int lo;          // published lo
+        int ready_end;   // forward streams: positions below ready * CH are readable
+        int lo_next;     // forward streams: a window from (lo + 2) * CH on frees a chunk
         bool failed;
 
         __device__ __forceinline__ void init(T* r, Ctl* c, int n, int lane)
@@ -144,13 +161,20 @@ struct Ring {
             nch = n;
             ready = 0;
             lo = 0;
+            ready_end = 0;
+            lo_next = 2 * CH;
             failed = false;
         }
 
         // makes the positions from `first` to `last` (consumption order, at
-        // most (NCH - 2) * CH apart) readable; frees the chunks before first's
+        // most (NCH - 2) * CH apart) readable; frees the chunks before first's.
+        // A forward stream first compares against two position thresholds: a
+        // window inside the known-ready chunks that frees no chunk does nothing.
         __device__ __forceinline__ void window(int first, int last)
         {
+            if constexpr (!Reverse) {
+                if (last < ready_end && first < lo_next) return;
+            }
             const int cf = chunk_of(first, nch);
             const int cl = chunk_of(last, nch);
             if (cf - 1 > lo) {
@@ -158,8 +182,11 @@ struct Ring {
                 lds_store_release(&ctl->lo, lo);
             }
             if (ready < cf) ready = cf;  // chunks before the window are not needed
-            if (ready > cl) return;
-            wait_for(cl);
+            if (ready <= cl) wait_for(cl);
+            if constexpr (!Reverse) {
+                ready_end = ready * CH;
+                lo_next = (lo + 2) * CH;
+            }
         }
 
         // Makes chunks up to cl readable.  One poll reads every slot's tag at
