@@ -648,13 +648,18 @@ struct FactorState {
 // round trips.  Sources are cleared after the read; targets accumulate.
 // Longer columns read their update targets row by row, one row ahead of the
 // writes.  The U record stores y(k), 1 / U(k,k) and the U row.
-constexpr int kAllReadC = 5;
+// Longest columns read in one batch (pivot, 2-column supernode), per value
+// size: the FP32 walk has the registers for more.
+template <typename Real>
+constexpr int kAllReadC = sizeof(Real) == 4 ? 5 : 5;
+template <typename Real>
+constexpr int kAllRead2C = sizeof(Real) == 4 ? 4 : 3;
 
 template <int C, typename Real, typename AR>
 __device__ __forceinline__ int factor_pivot(AR& A, ProgReader& P, FactorState<Real>& st)
 {
     constexpr int NS = frec_src_words(C), NW = frec_words(C), NV = 2 + 2 * C;
-    constexpr bool kAll = C <= kAllReadC;
+    constexpr bool kAll = C <= kAllReadC<Real>;
     P.window(st.p, st.p + NW - 1);
     A.window(st.ap, st.ap + NV - 1);
     v4u w[NW / 4];
@@ -733,7 +738,7 @@ template <int C, typename Real, typename AR>
 __device__ __forceinline__ int factor_pivot2(AR& A, ProgReader& P, FactorState<Real>& st)
 {
     constexpr int NS = frec2_src_words(C), NW = frec2_words(C), NV = 6 + 4 * C;
-    constexpr bool kAll = C <= 3;
+    constexpr bool kAll = C <= kAllRead2C<Real>;
     P.window(st.p, st.p + NW - 1);
     A.window(st.ap, st.ap + NV - 1);
     v4u w[NW / 4];
