@@ -849,7 +849,7 @@ __device__ __forceinline__ int factor_dispatch2(int c, AR& A, ProgReader& P, Fac
 
 // Any column length (up to kMaxC), one word at a time: the rare long columns.
 template <typename Real, typename AR>
-__device__ int factor_pivot_any(int c, AR& A, ProgReader& P, FactorState<Real>& st, Real* lsc, Real* usc)
+__device__ __forceinline__ int factor_pivot_any(int c, AR& A, ProgReader& P, FactorState<Real>& st, Real* lsc, Real* usc)
 {
     const int ns = frec_src_words(c), rw = frec_row_words(c), nw = frec_words(c);
     P.window(st.p, st.p + nw - 1);
@@ -929,7 +929,7 @@ __device__ __forceinline__ bool wait_flag(int* flag)
 // src = 0 (walker 0, after the other walkers' first phase); returns the
 // next record's head
 template <typename Real>
-__device__ int factor_merge(ProgReader& P, FactorState<Real>& st)
+__device__ __forceinline__ int factor_merge(ProgReader& P, FactorState<Real>& st)
 {
     P.window(st.p, st.p + 1);
     const int n = __builtin_amdgcn_readfirstlane(P.at(st.p + 1));
@@ -953,7 +953,7 @@ __device__ int factor_merge(ProgReader& P, FactorState<Real>& st)
 // slot ref_i (then cleared); gather [kGatherHead | next << 16, n, (dst ref,
 // index) x n]: dst += pend[index].  Both return the next record's head.
 template <typename Real>
-__device__ int factor_dump(ProgReader& P, FactorState<Real>& st, Real* pend_lane, int pend_stride)
+__device__ __forceinline__ int factor_dump(ProgReader& P, FactorState<Real>& st, Real* pend_lane, int pend_stride)
 {
     P.window(st.p, st.p + 2);
     const int n = __builtin_amdgcn_readfirstlane(P.at(st.p + 1));
@@ -961,13 +961,24 @@ __device__ int factor_dump(ProgReader& P, FactorState<Real>& st, Real* pend_lane
     const int m = n > kSplitRecItems ? kSplitRecItems : (n < 0 ? 0 : n);
     const int nw = dump_words(m);
     P.window(st.p, st.p + nw - 1);
-    for (int i = 0; i < m; ++i) {
-        const unsigned ref = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(st.p + 3 + i)));
-        const int q = first + i;
-        Real* ps = slot_at<Real>(st.lb, ref);
-        if (static_cast<unsigned>(q) < static_cast<unsigned>(pend_stride))
-            pend_lane[static_cast<size_t>(q) * kLanes] = *ps;
-        *ps = Real(0);
+    // every slot is read before any is cleared or stored, so the record pays
+    // one LDS round trip instead of one per item
+    // (items past m read the first item's slot again and are dropped)
+    unsigned ref[kSplitRecItems];
+    Real v[kSplitRecItems];
+#pragma unroll
+    for (int i = 0; i < kSplitRecItems; ++i) {
+        ref[i] = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(st.p + 3 + (i < m ? i : 0))));
+        v[i] = *slot_at<Real>(st.lb, ref[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < kSplitRecItems; ++i) {
+        if (i < m) {
+            const int q = first + i;
+            if (static_cast<unsigned>(q) < static_cast<unsigned>(pend_stride))
+                pend_lane[static_cast<size_t>(q) * kLanes] = v[i];
+            *slot_at<Real>(st.lb, ref[i]) = Real(0);
+        }
     }
     const int next = static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(st.p))) >> 16);
     st.p += nw;
@@ -975,18 +986,41 @@ __device__ int factor_dump(ProgReader& P, FactorState<Real>& st, Real* pend_lane
 }
 
 template <typename Real>
-__device__ int factor_gather(ProgReader& P, FactorState<Real>& st, const Real* pend_lane, int pend_stride)
+__device__ __forceinline__ int factor_gather(ProgReader& P, FactorState<Real>& st, const Real* pend_lane, int pend_stride)
 {
     P.window(st.p, st.p + 1);
     const int n = __builtin_amdgcn_readfirstlane(P.at(st.p + 1));
     const int m = n > kSplitRecItems ? kSplitRecItems : (n < 0 ? 0 : n);
     const int nw = pair_rec_words(m);
     P.window(st.p, st.p + nw - 1);
-    for (int i = 0; i < m; ++i) {
-        const unsigned dst = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(st.p + 2 + 2 * i)));
-        const int q = __builtin_amdgcn_readfirstlane(P.at(st.p + 3 + 2 * i));
-        if (static_cast<unsigned>(q) < static_cast<unsigned>(pend_stride))
-            *slot_at<Real>(st.lb, dst) += pend_lane[static_cast<size_t>(q) * kLanes];
+    // Every pending value is loaded, and every target slot read, before any
+    // is used: one global-memory and one LDS round trip per record instead of
+    // one each per item.  Items of one target are consecutive (the planner
+    // emits a target's contributions together) and distinct targets are
+    // distinct slots, so each run of equal targets sums its values onto the
+    // slot's old value in item order and writes once.
+    // (items past m repeat the first item's reads and are dropped)
+    unsigned dst[kSplitRecItems];
+    Real v[kSplitRecItems], t[kSplitRecItems];
+#pragma unroll
+    for (int i = 0; i < kSplitRecItems; ++i) {
+        const int w = st.p + 2 + 2 * (i < m ? i : 0);
+        dst[i] = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(w)));
+        const int q = __builtin_amdgcn_readfirstlane(P.at(w + 1));
+        const bool ok = static_cast<unsigned>(q) < static_cast<unsigned>(pend_stride);
+        v[i] = pend_lane[static_cast<size_t>(ok ? q : 0) * kLanes];
+        if (!ok) v[i] = Real(0);
+        t[i] = *slot_at<Real>(st.lb, dst[i]);
+    }
+    Real acc = Real(0);
+#pragma unroll
+    for (int i = 0; i < kSplitRecItems; ++i) {
+        if (i < m) {
+            if (i == 0 || dst[i] != dst[i - 1]) acc = t[i];
+            acc += v[i];
+            const bool last = i + 1 >= m || (i + 1 < kSplitRecItems && dst[i + 1] != dst[i]);
+            if (last) *slot_at<Real>(st.lb, dst[i]) = acc;
+        }
     }
     const int next = static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(st.p))) >> 16);
     st.p += nw;
@@ -1245,7 +1279,7 @@ __device__ __forceinline__ int backward_dispatch2(int c, UR& U, ProgReader& P, B
 }
 
 template <typename Real, typename UR>
-__device__ int backward_pivot_any(int c, UR& U, ProgReader& P, BackState<Real>& st)
+__device__ __forceinline__ int backward_pivot_any(int c, UR& U, ProgReader& P, BackState<Real>& st)
 {
     const int nw = brec_words(c);
     P.window(st.p, st.p + nw - 1);
@@ -1278,19 +1312,30 @@ __device__ __forceinline__ int backward_dispatch(int c, UR& U, ProgReader& P, Ba
 // split walk: preload [kPreloadHead | next << 16, n, (dst ref, x index) x n]:
 // dst = x[index], a higher program's solution; returns the next head
 template <typename Real>
-__device__ int backward_preload(ProgReader& P, BackState<Real>& st)
+__device__ __forceinline__ int backward_preload(ProgReader& P, BackState<Real>& st)
 {
     P.window(st.p, st.p + 1);
     const int n = __builtin_amdgcn_readfirstlane(P.at(st.p + 1));
     const int m = n > kSplitRecItems ? kSplitRecItems : (n < 0 ? 0 : n);
     const int nw = pair_rec_words(m);
     P.window(st.p, st.p + nw - 1);
-    for (int i = 0; i < m; ++i) {
-        const unsigned dst = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(st.p + 2 + 2 * i)));
-        const int q = __builtin_amdgcn_readfirstlane(P.at(st.p + 3 + 2 * i));
-        if (static_cast<unsigned>(q) < static_cast<unsigned>(st.d))
-            *slot_at<Real>(st.lb, dst) = st.X[static_cast<size_t>(q) * kLanes];
+    // every x is loaded before any is stored: one global-memory round trip
+    // per record instead of one per item
+    // (items past m repeat the first item's reads and are dropped)
+    unsigned dst[kSplitRecItems];
+    Real v[kSplitRecItems];
+    bool ok[kSplitRecItems];
+#pragma unroll
+    for (int i = 0; i < kSplitRecItems; ++i) {
+        const int w = st.p + 2 + 2 * (i < m ? i : 0);
+        dst[i] = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(w)));
+        const int q = __builtin_amdgcn_readfirstlane(P.at(w + 1));
+        ok[i] = static_cast<unsigned>(q) < static_cast<unsigned>(st.d);
+        v[i] = st.X[static_cast<size_t>(ok[i] ? q : 0) * kLanes];
     }
+#pragma unroll
+    for (int i = 0; i < kSplitRecItems; ++i)
+        if (i < m && ok[i]) *slot_at<Real>(st.lb, dst[i]) = v[i];
     const int next = static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(P.at(st.p))) >> 16);
     st.p += nw;
     return next;

@@ -96,7 +96,7 @@ struct Ring {
         return Reverse ? nch - 1 - c : c;
     }
 
-    __device__ static void init_ctl(Ctl* ctl, int lane)
+    __device__ static __forceinline__ void init_ctl(Ctl* ctl, int lane)
     {
         if (lane == 0) {
             ctl->lo = 0;
@@ -106,7 +106,7 @@ struct Ring {
     }
 
     // ---- producer wavefront w of W ---------------------------------------
-    __device__ static void produce(const void* base, int nch, T* ring, Ctl* ctl, int lane, int w, int W)
+    __device__ static __forceinline__ void produce(const void* base, int nch, T* ring, Ctl* ctl, int lane, int w, int W)
     {
         const v4u* g = static_cast<const v4u*>(base);
         v4u* r = reinterpret_cast<v4u*>(ring);
@@ -192,7 +192,7 @@ struct Ring {
         // Makes chunks up to cl readable.  One poll reads every slot's tag at
         // once and advances `ready` over all consecutive chunks already
         // written, so a poll also covers the chunks the next windows need.
-        __device__ void wait_for(int cl)
+        __device__ __forceinline__ void wait_for(int cl)
         {
             if (failed) return;
             int spins = 0;
@@ -219,7 +219,7 @@ struct Ring {
             }
         }
 
-        __device__ void wait(int c)
+        __device__ __forceinline__ void wait(int c)
         {
             if (failed) return;
             int spins = 0;

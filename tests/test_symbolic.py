@@ -304,6 +304,61 @@ def test_a_stride_covers_every_whole_chunk_read(monkeypatch, wf):
                 assert ln % 512 == 0 and off + ln <= n_words
 
 
+def _split_records(prog, start, length, n_piv):
+    """(kind, items) of every record of one split-walk factorization program:
+    kind 'gather' with its (dst ref, pend index) pairs, 'dump' with its refs,
+    'step' otherwise (no items)."""
+    r4 = lambda v: (v + 3) // 4 * 4
+    p, end, k, out = start, start + length, 0, []
+    h = int(prog[p]) & 0xFFFF
+    while p < end and (k < n_piv or h in (0x7FFC, 0x7FFD)):
+        c = h & 0x7FFF
+        if c == 0x7FFC:
+            m = int(prog[p + 1])
+            out.append(("gather", [(int(prog[p + 2 + 2 * i]), int(prog[p + 3 + 2 * i])) for i in range(m)]))
+            nw = r4(2 + 2 * m)
+        elif c == 0x7FFD:
+            m = int(prog[p + 1])
+            out.append(("dump", [int(prog[p + 3 + i]) for i in range(m)]))
+            nw = r4(3 + m)
+        elif c == 0x7FFE:
+            nw = r4(2 + 2 * int(prog[p + 1]))
+            out.append(("merge", []))
+        elif h >> 15:
+            nw, k = r4(7 + 4 * c + (c + 1) * c), k + 2
+            out.append(("step", []))
+        else:
+            nw, k = r4(3 + 2 * c + (c + 1) * c), k + 1
+            out.append(("step", []))
+        h = (int(prog[p]) >> 16) & 0xFFFF
+        p += nw
+    return out
+
+
+def test_split_walk_gathers_keep_each_target_together():
+    """The factorization kernel loads a gather record's pending values and
+    target slots in one batch and writes each target once, after summing the
+    run of items that share it: that needs every target's items consecutive
+    within a record and no slot shared by two runs.  Dump records name
+    distinct slots."""
+    for g in (syn.grid_c1(), syn.grid_c2()):
+        P = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+        inf = P.info()
+        assert inf["n_split"] > 0
+        prog = P.array("prog_fs").astype(np.int64) & 0xFFFFFFFF
+        n_gather = 0
+        for s in P.array("split_segs").reshape(-1, 9):
+            for kind, items in _split_records(prog, s[0], s[1], s[8]):
+                if kind == "gather":
+                    n_gather += 1
+                    assert 0 < len(items) <= 30
+                    runs = [d for i, (d, _) in enumerate(items) if i == 0 or items[i - 1][0] != d]
+                    assert len(runs) == len(set(runs))
+                elif kind == "dump":
+                    assert 0 < len(items) <= 30 and len(items) == len(set(items))
+        assert n_gather > 10
+
+
 def test_split_walk_programs_solve_like_the_main_walk():
     """The split walk (fine sets, middle parts, top part, each on slots of its
     own, pending targets through the dump / gather records, top x through the
