@@ -13,6 +13,8 @@ injections are resident in HBM before the timed region.  Methods:
 * ``mixed_ir`` mixed precision with one refinement round per Newton step whose
   residual is formed in FP64 from V, against the FP32 factors;
 * ``warm``  learned warm start + mixed precision (when the MLP is available);
+* ``warm_ir`` ``warm`` with ``mixed_ir``'s FP64-residual refinement round on
+  every Newton step;
 * ``facade_mixed`` / ``facade_warm``: ``mixed`` / ``warm`` through the
   drop-in Python surface (InjectionSweepGPU.compute), to show what the
   facade path costs (``vs_capi``).
@@ -84,7 +86,7 @@ def parse():
     ap.add_argument("--quiet", action="store_true")
     args = ap.parse_args()
     if args.methods is None:
-        args.methods = "fp64,mixed,mixed_ir,warm" + (",facade_mixed,facade_warm" if args.workload == "c2" else "")
+        args.methods = "fp64,mixed,mixed_ir,warm,warm_ir" + (",facade_mixed,facade_warm" if args.workload == "c2" else "")
     if args.workload == "c3":  # the learned-warm-start workload's defaults: 200k training rows
         for k, v in (("train_rows", 200_000),):
             if getattr(args, k) == ap.get_default(k):
@@ -181,7 +183,7 @@ def train_warm(args, wl, torch, capi, syn):
 def run_method(args, name, wl, torch, capi, ws=None):
     g, S = wl.g, wl.S
     prec = capi.FP64 if name == "fp64" else capi.MIXED
-    ir = 1 if name == "mixed_ir" else 0  # one refinement round per Newton step, FP64 residual
+    ir = 1 if name in ("mixed_ir", "warm_ir") else 0  # one refinement round per Newton step, FP64 residual
     t_plan = time.perf_counter()
     plan = capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
     info = plan.info()
@@ -448,7 +450,7 @@ def main():
     for name in names:
         t1 = time.perf_counter()
         ws = None
-        if name in ("warm", "facade_warm"):
+        if name in ("warm", "warm_ir", "facade_warm"):
             if ws_trained is None:
                 ws_trained, warm_info = train_warm(args, wl, torch, capi, syn)
                 log(args, "warm-start training", warm_info)

@@ -23,7 +23,7 @@ template <typename Real, bool PerRowY, bool WithJ>
 __global__ void k_mismatch(MismatchArgs<Real> a);
 template <typename Real, bool WithJ>
 __global__ void k_mis_status(MismatchArgs<Real> a);
-template <typename Real, int W, bool SPLIT, int RG>
+template <typename Real, int W, bool SPLIT, int RG, int MODE>
 __global__ void k_factor(FactorArgs<Real> a);
 template <typename Real, int W, bool SPLIT>
 __global__ void k_backward(BackwardArgs<Real> a);
@@ -299,6 +299,10 @@ struct g2g_batch_s {
     double2* ed_val = nullptr;
     int* ed_flag = nullptr;  // [T][n]
     void* A = nullptr;
+    // mixed precision with FP64-residual refinement: the factorization's L
+    // multipliers and the refinement's right-hand sides, in the A stream's
+    // layout (allocated on first use)
+    void* Lm = nullptr;
     void* U = nullptr;
     void* X = nullptr;         // solved Newton step [T][d][64]
     int a_stride = 0, u_stride = 0;
@@ -405,7 +409,7 @@ struct g2g_batch_s {
     {
         dfree(V); dfree(S); dfree(ed_n); dfree(ed_ent); dfree(ed_val); dfree(ed_flag); dfree(A); dfree(U); dfree(X); dfree(res); dfree(jmax); dfree(mis_part);
         dfree(status); dfree(iters); dfree(tile_active); dfree(n_active); dfree(clamped); dfree(error);
-        dfree(V0); dfree(Bs); dfree(Xs); dfree(wmask); dfree(differs); dfree(tile_restart); dfree(v_row);
+        dfree(V0); dfree(Bs); dfree(Xs); dfree(Lm); dfree(wmask); dfree(differs); dfree(tile_restart); dfree(v_row);
         dfree(clock_dev);
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -577,9 +581,39 @@ g2g_plan_s* plan_fitting(g2g_plan_s* P, bool fp64)
     return P->fitted.back().get();  // one walker each: the launch reports what does not fit
 }
 
+template <typename Real, int MODE>
+void launch_factor_mode(const SymbolicLU& s, int rg, size_t lds, const FactorArgs<Real>& a, int T, hipStream_t st)
+{
+    const void* fn = nullptr;
+    if constexpr (sizeof(Real) == 4) {
+        if (rg == kHalfRingGeometry)
+            fn = reinterpret_cast<const void*>(&k_factor<Real, 2, false, kHalfRingGeometry, MODE>);
+    }
+    if (!fn)
+        fn = s.walkers == 2 ? reinterpret_cast<const void*>(&k_factor<Real, 2, false, 0, MODE>)
+                            : reinterpret_cast<const void*>(&k_factor<Real, 1, false, 0, MODE>);
+    HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    if constexpr (sizeof(Real) == 4) {
+        if (rg == kHalfRingGeometry) {
+            k_factor<Real, 2, false, kHalfRingGeometry, MODE><<<T, kLanes * factor_walk_waves(2), lds, st>>>(a);
+            HIPCHK(hipGetLastError());
+            return;
+        }
+    }
+    if (s.walkers == 2) k_factor<Real, 2, false, 0, MODE><<<T, kLanes * factor_walk_waves(2), lds, st>>>(a);
+    else k_factor<Real, 1, false, 0, MODE><<<T, kLanes * factor_walk_waves(1), lds, st>>>(a);
+    HIPCHK(hipGetLastError());
+}
+
+// The main-layout walk over the A stream: the factorization (mode
+// kFactorFull), the factorization storing its L multipliers in the A stream
+// (kFactorStoreL), or the forward substitution alone over such a stream
+// (kForwardOnly: y into the U stream, nothing else written).  The stored
+// modes are FP32 only.
 template <typename Real>
 void launch_factor(const g2g_plan_s* P, const Real* A, Real* U, int a_stride, int u_stride, const int* tile_active,
-                   const double* jmax, int* clamped, int* error, int rows, int T, double eps, hipStream_t st)
+                   const double* jmax, int* clamped, int* error, int rows, int T, double eps, hipStream_t st,
+                   int mode = kFactorFull, Real* Lw = nullptr)
 {
     const SymbolicLU& s = P->lu();
     const size_t limit = lds_limit();
@@ -588,24 +622,20 @@ void launch_factor(const g2g_plan_s* P, const Real* A, Real* U, int a_stride, in
     if (lds > limit) throw std::runtime_error("factorization needs " + std::to_string(lds) + " bytes of LDS");
     FactorArgs<Real> a{A,     U,    P->d_prog_f, tile_active, jmax,    clamped, error,  s.d, a_stride, u_stride,
                        P->prog_f_stride, s.slots_f, rows, s.max_c, eps, P->d_seg, T, 0, nullptr, 0};
-    const void* fn = nullptr;
     if constexpr (sizeof(Real) == 4) {
-        if (rg == kHalfRingGeometry) fn = reinterpret_cast<const void*>(&k_factor<Real, 2, false, kHalfRingGeometry>);
-    }
-    if (!fn)
-        fn = s.walkers == 2 ? reinterpret_cast<const void*>(&k_factor<Real, 2, false, 0>)
-                            : reinterpret_cast<const void*>(&k_factor<Real, 1, false, 0>);
-    HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    if constexpr (sizeof(Real) == 4) {
-        if (rg == kHalfRingGeometry) {
-            k_factor<Real, 2, false, kHalfRingGeometry><<<T, kLanes * factor_walk_waves(2), lds, st>>>(a);
-            HIPCHK(hipGetLastError());
-            return;
+        if (mode == kFactorStoreL) {
+            if (!Lw) throw std::logic_error("a multiplier-storing factorization needs its stream");
+            a.Lw = Lw;
+            return launch_factor_mode<Real, kFactorStoreL>(s, rg, lds, a, T, st);
         }
+        if (mode == kForwardOnly) {
+            a.clamped = nullptr;  // no pivots: the factorization's counts stay
+            return launch_factor_mode<Real, kForwardOnly>(s, rg, lds, a, T, st);
+        }
+    } else {
+        if (mode != kFactorFull) throw std::logic_error("stored-multiplier walks are FP32 only");
     }
-    if (s.walkers == 2) k_factor<Real, 2, false, 0><<<T, kLanes * factor_walk_waves(2), lds, st>>>(a);
-    else k_factor<Real, 1, false, 0><<<T, kLanes * factor_walk_waves(1), lds, st>>>(a);
-    HIPCHK(hipGetLastError());
+    launch_factor_mode<Real, kFactorFull>(s, rg, lds, a, T, st);
 }
 
 template <typename Real>
@@ -660,7 +690,7 @@ void launch_factor_split(const g2g_plan_s* P, const Real* A, Real* U, int a_stri
                                                                                        static_cast<long long>(T) * kLanes);
         HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_factor<Real, 1, true, 0>),
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_factor<Real, 1, true, 0, kFactorFull>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     const int levels[3][2] = {{0, s.split_fine}, {s.split_fine, s.split_mid}, {s.split_fine + s.split_mid, 1}};
     for (const auto& lv : levels) {
@@ -668,7 +698,7 @@ void launch_factor_split(const g2g_plan_s* P, const Real* A, Real* U, int a_stri
         FactorArgs<Real> a{A,   U,        P->d_prog_fs, tile_active, jmax, clamped, error, s.d, a_stride,
                            u_stride, P->prog_fs_stride, s.slots_fs, rows, s.max_c, eps, P->d_split_seg,
                            T,   lv[0],    pend,         s.pend_len};
-        k_factor<Real, 1, true, 0><<<T * lv[1], kLanes * (2 + kSplitProducers), lds, st>>>(a);
+        k_factor<Real, 1, true, 0, kFactorFull><<<T * lv[1], kLanes * (2 + kSplitProducers), lds, st>>>(a);
         HIPCHK(hipGetLastError());
     }
 }
@@ -788,10 +818,38 @@ void ensure_edit_cap(g2g_batch_s* b, int cap, bool keep)
     b->ed_cap = cap;
 }
 
+// The walk mode of a Newton step's main-layout factorization.  With
+// FP64-residual refinement in mixed precision it stores its L multipliers in
+// the A stream, so each refinement round solves with a forward-only walk
+// instead of a second factorization (G2G_IR_REFACTOR=1: the second
+// factorization, for comparison).
+template <typename Real>
+int factor_walk_mode(const g2g_batch_s* b)
+{
+    if (sizeof(Real) != 4 || b->ir_steps <= 0 || !b->ir_fp64) return kFactorFull;
+    const char* e = std::getenv("G2G_IR_REFACTOR");
+    return e && std::atoi(e) == 1 ? kFactorFull : kFactorStoreL;
+}
+
+// the multipliers' stream of kFactorStoreL (zeroed once: positions no walk
+// writes are read and dropped by the forward walk)
+template <typename Real>
+Real* multiplier_stream(g2g_batch_s* b)
+{
+    if (factor_walk_mode<Real>(b) != kFactorStoreL) return nullptr;
+    if (!b->Lm) {
+        const size_t bytes = static_cast<size_t>(b->rows_pad) * b->a_stride * sizeof(Real);
+        b->Lm = dalloc<unsigned char>(bytes);
+        HIPCHK(hipMemsetAsync(b->Lm, 0, bytes, b->stream));
+    }
+    return static_cast<Real*>(b->Lm);
+}
+
 // Refinement rounds of a Newton step with the residual in FP64 from V
 // (k_ir_residual), against the solve's factors: r = F(V) - J(V) x over the
-// A stream's rhs, the factorization again (bitwise the same factors), the
-// substitution, x += the correction.
+// A stream's rhs, its forward substitution (a forward-only walk over the
+// stored multipliers, or the factorization again: bitwise the same factors),
+// the backward substitution, x += the correction.
 template <typename Real>
 void refine_fp64(g2g_batch_s* b, double eps, int T)
 {
@@ -801,12 +859,18 @@ void refine_fp64(g2g_batch_s* b, double eps, int T)
     Real* X = static_cast<Real*>(b->X);
     Real* Xs = static_cast<Real*>(b->Xs);
     const bool sp = b->split_layout;  // the tail's split walk: its layout's tables and launches
+    // workgroups per tile: the mismatch's setting, doubled (up to 32) while
+    // the launch stays within one round of resident workgroups
+    int splits = std::max(1, b->mis_splits);
+    while (splits < 32 && 2 * splits * T <= 512) splits *= 2;
+    // the forward walk's stream: the rhs goes into the multipliers' stream
+    Real* L = sp ? nullptr : (factor_walk_mode<Real>(b) == kFactorStoreL ? static_cast<Real*>(b->Lm) : nullptr);
     for (int k = 0; k < b->ir_steps; ++k) {
         IrArgs<Real> a{sp ? P->d_k1_bus_s : P->d_k1_bus, sp ? P->d_k1_ent_s : P->d_k1_ent, nr.k1_nbus, P->d_Y,
-                       row_edits(b), b->V, b->S, b->mask, P->d_colmap, A, X, Xs, b->tile_active, nr.n, nr.nY, nr.d,
-                       b->a_stride};
-        if (b->per_row_y) k_ir_residual<Real, true><<<T, kLanes * kMismatchWaves, 0, b->stream>>>(a);
-        else k_ir_residual<Real, false><<<T, kLanes * kMismatchWaves, 0, b->stream>>>(a);
+                       row_edits(b), b->V, b->S, b->mask, P->d_colmap, L ? L : A, X, Xs, b->tile_active, nr.n, nr.nY,
+                       nr.d, b->a_stride, splits};
+        if (b->per_row_y) k_ir_residual<Real, true><<<T * splits, kLanes * kMismatchWaves, 0, b->stream>>>(a);
+        else k_ir_residual<Real, false><<<T * splits, kLanes * kMismatchWaves, 0, b->stream>>>(a);
         HIPCHK(hipGetLastError());
         if (sp) {
             launch_factor_split<Real>(P, A, static_cast<Real*>(b->U), b->a_stride, b->u_stride, b->tile_active,
@@ -815,8 +879,9 @@ void refine_fp64(g2g_batch_s* b, double eps, int T)
             launch_backward_split<Real>(P, static_cast<const Real*>(b->U), b->u_stride, b->tile_active, X,
                                         b->error, b->active_rows, T, b->stream);
         } else {
-            launch_factor<Real>(P, A, static_cast<Real*>(b->U), b->a_stride, b->u_stride, b->tile_active, b->jmax,
-                                b->clamped, b->error, b->active_rows, T, eps, b->stream);
+            launch_factor<Real>(P, L ? L : A, static_cast<Real*>(b->U), b->a_stride, b->u_stride, b->tile_active,
+                                b->jmax, b->clamped, b->error, b->active_rows, T, eps, b->stream,
+                                L ? kForwardOnly : kFactorFull);
             launch_backward<Real>(P, static_cast<const Real*>(b->U), b->u_stride, b->tile_active, X, b->error,
                                   b->active_rows, T, b->stream);
         }
@@ -1346,7 +1411,7 @@ int nr_loop_body(g2g_batch_s* b, int max_iter, double tol, int freeze, double gr
         else
             launch_factor<Real>(b->plan, static_cast<const Real*>(b->A), static_cast<Real*>(b->U), b->a_stride,
                                 b->u_stride, b->tile_active, b->jmax, b->clamped, b->error, b->active_rows, b->T,
-                                eps, b->stream);
+                                eps, b->stream, factor_walk_mode<Real>(b), multiplier_stream<Real>(b));
         tm.ph[1].push_back({e0, tm.mark()});
         rows_factored += *b->h_active;
         e0 = tm.mark();
@@ -1406,7 +1471,7 @@ void stream_round_launches(g2g_batch_s* b, int k, int max_iter, double tol)
         launch_mismatch<Real>(b, true, q, max_iter, tol, 1, 0.0, false, nullptr, b->clock_dev);
         launch_factor<Real>(b->plan, static_cast<const Real*>(b->A), static_cast<Real*>(b->U), b->a_stride,
                             b->u_stride, b->tile_active, b->jmax, b->clamped, b->error, b->active_rows, b->T, eps,
-                            b->stream);
+                            b->stream, factor_walk_mode<Real>(b), multiplier_stream<Real>(b));
         launch_backward<Real>(b->plan, static_cast<const Real*>(b->U), b->u_stride, b->tile_active,
                               static_cast<Real*>(b->X), b->error, b->active_rows, b->T, b->stream);
         if (b->ir_steps > 0) refine_step<Real>(b, eps, b->T);
@@ -1417,9 +1482,9 @@ void stream_round_launches(g2g_batch_s* b, int k, int max_iter, double tol)
     HIPCHK(hipGetLastError());
 }
 
-// A round's launches depend only on (k, max_iter, tol, refinement steps,
-// step scaling): with graphs on, they are captured once per such shape and
-// replayed.
+// A round's launches depend only on (k, max_iter, tol, refinement steps and
+// kind, step scaling): with graphs on, they are captured once per such shape
+// and replayed.
 template <typename Real>
 void stream_round(g2g_batch_s* b, int k, int max_iter, double tol)
 {
@@ -1428,6 +1493,7 @@ void stream_round(g2g_batch_s* b, int k, int max_iter, double tol)
         b->Bs = dalloc<unsigned char>(bytes);
         b->Xs = dalloc<unsigned char>(bytes);
     }
+    multiplier_stream<Real>(b);  // allocated (and cleared) outside any graph capture
     if (!b->stream_graph) {
         stream_round_launches<Real>(b, k, max_iter, tol);
     } else {
@@ -1437,7 +1503,8 @@ void stream_round(g2g_batch_s* b, int k, int max_iter, double tol)
             return v;
         };
         // everything a round's launches take that can change between rounds
-        const long long key[6] = {k, max_iter, bits(tol), b->ir_steps, bits(b->max_dva), bits(b->max_dvm)};
+        const long long refine = 4LL * b->ir_steps + (b->ir_fp64 ? 2 : 0) + (factor_walk_mode<Real>(b) == kFactorStoreL);
+        const long long key[6] = {k, max_iter, bits(tol), refine, bits(b->max_dva), bits(b->max_dvm)};
         if (!b->graph_exec || !std::equal(key, key + 6, b->graph_key)) {
             if (b->graph_exec) HIPCHK(hipGraphExecDestroy(b->graph_exec));
             if (b->graph) HIPCHK(hipGraphDestroy(b->graph));

@@ -151,6 +151,7 @@ struct FactorArgs {
     int T, seg0;
     Real* pend;
     int pend_stride;
+    Real* Lw;               // kFactorStoreL: the multipliers' stream (the A stream's layout)
 };
 
 template <typename Real>
@@ -199,6 +200,7 @@ struct IrArgs {
     Real* Xs;                   // its copy
     const int* tile_active;
     int n, nY, d, a_stride;
+    int splits;                 // workgroups per tile (split-major), each over every splits-th bus group
 };
 
 template <typename Real>

@@ -636,10 +636,21 @@ template <typename Real>
 struct FactorState {
     unsigned char* lb;  // this lane's slot base
     Real* U;            // U stream of this lane
+    Real* Aw;           // kFactorStoreL: this walker's segment of the multipliers' stream, this lane
     Real eps;
     int clamped;
     int p, ap, up;
 };
+
+// Walk modes of k_factor.  kFactorStoreL also writes each pivot's final L
+// multipliers into a second stream of the A stream's layout (Lw), at the
+// positions of the L values they came from.  kForwardOnly then walks the
+// same program over that stream, with a right-hand side in its F positions,
+// as a forward substitution alone: y(k) = F(k) + b(k), b(S) += L(S,k) y(k),
+// y into the U stream.  The operations and their order are the
+// factorization's own, so y comes out bitwise the same as from a second full
+// factorization with that right-hand side; the Schur updates, the pivots and
+// the U rows are skipped.  (kFactorFull: the factorization alone.)
 
 // One pivot with C off-diagonal entries.  The whole record is read at once;
 // then every slot the pivot reads -- its sources and, for short columns, the
@@ -655,7 +666,7 @@ constexpr int kAllReadC = sizeof(Real) == 4 ? 5 : 5;
 template <typename Real>
 constexpr int kAllRead2C = sizeof(Real) == 4 ? 4 : 3;
 
-template <int C, typename Real, typename AR>
+template <int C, int MODE, typename Real, typename AR>
 __device__ __forceinline__ int factor_pivot(AR& A, ProgReader& P, FactorState<Real>& st)
 {
     constexpr int NS = frec_src_words(C), NW = frec_words(C), NV = 2 + 2 * C;
@@ -667,6 +678,23 @@ __device__ __forceinline__ int factor_pivot(AR& A, ProgReader& P, FactorState<Re
     Real av[NV];
     load_values<NV>(A, st.ap, av);
     auto at = [&](int q) -> Real* { return slot_at<Real>(st.lb, word(w, q)); };
+    if constexpr (MODE == kForwardOnly) {
+        // sources [diag, b(k), ...]; A stream [F, diag, -L / U(k,k) (stored), ...]
+        Real tb[C > 0 ? C : 1];
+#pragma unroll
+        for (int a = 0; a < C; ++a) tb[a] = *at(NS + C * C + a);
+        const Real bk = *at(2);
+        *at(2) = Real(0);
+        const Real fk = av[0] + bk;
+        stream_store(&st.U[static_cast<size_t>(st.up) * kLanes], fk);
+#pragma unroll
+        for (int a = 0; a < C; ++a) *at(NS + C * C + a) = fma(av[2 + a], fk, tb[a]);
+        st.up += 2 + C;
+        st.ap += NV;
+        st.p += NW;
+        return static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(word(w, 0))) >> 16);
+    }
+    const int ap0 = st.ap;
     Real sv[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) sv[q] = *at(1 + q);
@@ -694,6 +722,10 @@ __device__ __forceinline__ int factor_pivot(AR& A, ProgReader& P, FactorState<Re
     const Real inv = pivot_inv(dv);
 #pragma unroll
     for (int q = 0; q < C; ++q) lv[q] *= -inv;  // negated multipliers
+    if constexpr (MODE == kFactorStoreL) {
+#pragma unroll
+        for (int q = 0; q < C; ++q) stream_store(&st.Aw[static_cast<size_t>(ap0 + 2 + q) * kLanes], lv[q]);
+    }
     Real* u = st.U + static_cast<size_t>(st.up) * kLanes;
     stream_store(&u[0], fk);
     stream_store(&u[kLanes], inv);
@@ -734,11 +766,12 @@ __device__ __forceinline__ int factor_pivot(AR& A, ProgReader& P, FactorState<Re
 // would write them.  The A stream holds pivot k's then pivot k+1's values:
 //   [F(k), a(k,k), a(k+1,k), L(S,k)[C], a(k,k+1), U(k,S)[C],
 //    F(k+1), a(k+1,k+1), L(S,k+1)[C], U(k+1,S)[C]]
-template <int C, typename Real, typename AR>
+template <int C, int MODE, typename Real, typename AR>
 __device__ __forceinline__ int factor_pivot2(AR& A, ProgReader& P, FactorState<Real>& st)
 {
     constexpr int NS = frec2_src_words(C), NW = frec2_words(C), NV = 6 + 4 * C;
     constexpr bool kAll = C <= kAllRead2C<Real>;
+    constexpr int K1 = 4 + 2 * C;  // pivot k+1's first A-stream value
     P.window(st.p, st.p + NW - 1);
     A.window(st.ap, st.ap + NV - 1);
     v4u w[NW / 4];
@@ -746,6 +779,28 @@ __device__ __forceinline__ int factor_pivot2(AR& A, ProgReader& P, FactorState<R
     Real av[NV];
     load_values<NV>(A, st.ap, av);
     auto at = [&](int q) -> Real* { return slot_at<Real>(st.lb, word(w, q)); };
+    if constexpr (MODE == kForwardOnly) {
+        // stored: l at 2, L(S,k) / U(k,k) at 3 .., L(S,k+1) / U(k+1,k+1) at K1 + 2 ..
+        Real tb[C > 0 ? C : 1];
+#pragma unroll
+        for (int a = 0; a < C; ++a) tb[a] = *at(NS + C * C + a);
+        const Real b0 = *at(2), b1 = *at(2 + K1);
+        *at(2) = Real(0);
+        *at(2 + K1) = Real(0);
+        const Real fk = av[0] + b0;
+        const Real fk1 = fma(-av[2], fk, av[K1] + b1);
+        Real* u = st.U + static_cast<size_t>(st.up) * kLanes;
+        stream_store(&u[0], fk);
+        stream_store(&u[static_cast<size_t>(3 + C) * kLanes], fk1);
+#pragma unroll
+        for (int a = 0; a < C; ++a)
+            *at(NS + C * C + a) = fma(-av[K1 + 2 + a], fk1, fma(-av[3 + a], fk, tb[a]));
+        st.up += 5 + 2 * C;
+        st.ap += NV;
+        st.p += NW;
+        return static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(word(w, 0))) >> 16);
+    }
+    const int ap0 = st.ap;
     Real sv[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) sv[q] = *at(1 + q);
@@ -759,7 +814,6 @@ __device__ __forceinline__ int factor_pivot2(AR& A, ProgReader& P, FactorState<R
 #pragma unroll
     for (int q = 0; q < NV; ++q) *at(1 + q) = Real(0);
     // A-stream order per pivot is [F, diag, ...] and source order [diag, b, ...]
-    constexpr int K1 = 4 + 2 * C;
     Real x[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
@@ -800,6 +854,14 @@ __device__ __forceinline__ int factor_pivot2(AR& A, ProgReader& P, FactorState<R
     for (int i = 0; i < C; ++i) mk1[i] = fma(-mk[i], b, x[K1 + 2 + i]) * inv2;
 #pragma unroll
     for (int j = 0; j < C; ++j) uk1[j] = fma(-l, uk[j], x[K1 + 2 + C + j]);
+    if constexpr (MODE == kFactorStoreL) {
+        Real* aw = st.Aw + static_cast<size_t>(ap0) * kLanes;
+        stream_store(&aw[2 * kLanes], l);
+#pragma unroll
+        for (int i = 0; i < C; ++i) stream_store(&aw[static_cast<size_t>(3 + i) * kLanes], mk[i]);
+#pragma unroll
+        for (int i = 0; i < C; ++i) stream_store(&aw[static_cast<size_t>(K1 + 2 + i) * kLanes], mk1[i]);
+    }
     Real* u1 = u + static_cast<size_t>(3 + C) * kLanes;
     stream_store(&u1[0], fk1);
     stream_store(&u1[kLanes], inv2);
@@ -836,19 +898,19 @@ __device__ __forceinline__ int factor_pivot2(AR& A, ProgReader& P, FactorState<R
     return static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(word(w, 0))) >> 16);
 }
 
-template <int C, typename Real, typename AR>
+template <int C, int MODE, typename Real, typename AR>
 __device__ __forceinline__ int factor_dispatch2(int c, AR& A, ProgReader& P, FactorState<Real>& st)
 {
     if constexpr (C > kFastC2) {
         return -1;
     } else {
-        if (c == C) return factor_pivot2<C>(A, P, st);
-        return factor_dispatch2<C + 1>(c, A, P, st);
+        if (c == C) return factor_pivot2<C, MODE>(A, P, st);
+        return factor_dispatch2<C + 1, MODE>(c, A, P, st);
     }
 }
 
 // Any column length (up to kMaxC), one word at a time: the rare long columns.
-template <typename Real, typename AR>
+template <int MODE, typename Real, typename AR>
 __device__ __forceinline__ int factor_pivot_any(int c, AR& A, ProgReader& P, FactorState<Real>& st, Real* lsc, Real* usc)
 {
     const int ns = frec_src_words(c), rw = frec_row_words(c), nw = frec_words(c);
@@ -862,6 +924,20 @@ __device__ __forceinline__ int factor_pivot_any(int c, AR& A, ProgReader& P, Fac
     };
     A.window(st.ap, st.ap + 1 + 2 * c);
     auto aval = [&](int i) { return A.at(st.ap + i); };
+    if constexpr (MODE == kForwardOnly) {
+        const Real fk = aval(0) + take(1);
+        stream_store(&st.U[static_cast<size_t>(st.up) * kLanes], fk);
+        for (int a = 0; a < c; ++a) {
+            Real* t = slot_at<Real>(st.lb, ref(ns + c * rw + a));
+            *t = fma(aval(2 + a), fk, *t);
+        }
+        const int c_next = __builtin_amdgcn_readfirstlane(static_cast<int>(ref(0) >> 16));
+        st.up += 2 + c;
+        st.ap += 2 + 2 * c;
+        st.p += nw;
+        return c_next;
+    }
+    const int ap0 = st.ap;
     Real dv = aval(1) + take(0);
     const Real fk = aval(0) + take(1);
     for (int q = 0; q < c; ++q) lsc[q * kLanes] = aval(2 + q) + take(2 + q);
@@ -873,6 +949,9 @@ __device__ __forceinline__ int factor_pivot_any(int c, AR& A, ProgReader& P, Fac
     }
     const Real inv = pivot_inv(dv);
     for (int q = 0; q < c; ++q) lsc[q * kLanes] *= -inv;
+    if constexpr (MODE == kFactorStoreL) {
+        for (int q = 0; q < c; ++q) stream_store(&st.Aw[static_cast<size_t>(ap0 + 2 + q) * kLanes], lsc[q * kLanes]);
+    }
     Real* u = st.U + static_cast<size_t>(st.up) * kLanes;
     stream_store(&u[0], fk);
     stream_store(&u[kLanes], inv);
@@ -893,14 +972,14 @@ __device__ __forceinline__ int factor_pivot_any(int c, AR& A, ProgReader& P, Fac
 }
 
 // the pivot of column length c; returns the next record's c (-1: no fast path)
-template <int C, typename Real, typename AR>
+template <int C, int MODE, typename Real, typename AR>
 __device__ __forceinline__ int factor_dispatch(int c, AR& A, ProgReader& P, FactorState<Real>& st)
 {
     if constexpr (C > kFastC) {
         return -1;
     } else {
-        if (c == C) return factor_pivot<C>(A, P, st);
-        return factor_dispatch<C + 1>(c, A, P, st);
+        if (c == C) return factor_pivot<C, MODE>(A, P, st);
+        return factor_dispatch<C + 1, MODE>(c, A, P, st);
     }
 }
 
@@ -1045,10 +1124,11 @@ __host__ __device__ constexpr int factor_waves()
     return SPLIT ? 2 + kSplitProducers : factor_walk_waves(W);
 }
 
-template <typename Real, int W, bool SPLIT, int RG>
+template <typename Real, int W, bool SPLIT, int RG, int MODE>
 __global__ __launch_bounds__((kLanes * factor_waves<W, SPLIT>())) void k_factor(FactorArgs<Real> a)
 {
     static_assert(!SPLIT || W == 1, "a split-walk workgroup has one walker");
+    static_assert(!SPLIT || MODE == kFactorFull, "the split walk factorizes in full");
     constexpr int RW = SPLIT ? kSplitRingF : RG > 0 ? RG : W;  // ring geometry (RG: another walker count's)
     constexpr int kCh = value_chunk(sizeof(Real), RW), kNch = value_ring_slots(sizeof(Real), RW);
     static_assert((kNch - 2) * kCh >= 6 + 4 * kFastC2 - 1, "the ring window spans a two-column step's values");
@@ -1114,6 +1194,9 @@ __global__ __launch_bounds__((kLanes * factor_waves<W, SPLIT>())) void k_factor(
     FactorState<Real> st;
     st.lb = reinterpret_cast<unsigned char*>(sl + lane);
     st.U = a.U + tile * static_cast<size_t>(a.u_stride) * kLanes + static_cast<size_t>(sg[6]) * kLanes + lane;
+    st.Aw = MODE == kFactorStoreL
+                ? a.Lw + (tile * static_cast<size_t>(a.a_stride) + static_cast<size_t>(sg[4])) * kLanes + lane
+                : nullptr;
     st.eps = static_cast<Real>(a.pivot_eps) * static_cast<Real>(a.jmax ? a.jmax[row] : 1.0);
     st.clamped = 0;
     st.p = st.ap = st.up = 0;
@@ -1146,15 +1229,15 @@ __global__ __launch_bounds__((kLanes * factor_waves<W, SPLIT>())) void k_factor(
         const int c = h & 0x7FFF;
         int h_next;
         if (h >> 15) {
-            h_next = factor_dispatch2<0>(c, A, P, st);
+            h_next = factor_dispatch2<0, MODE>(c, A, P, st);
             if (h_next < 0) {  // never emitted: the host pairs columns up to kFastC2 only
                 bad = true;
                 break;
             }
             k += 2;
         } else {
-            h_next = factor_dispatch<0>(c, A, P, st);
-            if (h_next < 0) h_next = factor_pivot_any(c, A, P, st, lsc + lane, usc + lane);
+            h_next = factor_dispatch<0, MODE>(c, A, P, st);
+            if (h_next < 0) h_next = factor_pivot_any<MODE>(c, A, P, st, lsc + lane, usc + lane);
             k += 1;
         }
         h = h_next & 0xFFFF;
@@ -1552,17 +1635,24 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_update(UpdateArgs<Real> a
 //   stream, x is copied to Xs on every step.
 // ---------------------------------------------------------------------------
 
-// k_ir_residual: one workgroup per tile, its waves over the buses; also
-// keeps a copy of the step (the refinement adds its correction to it).
+// k_ir_residual: splits workgroups per tile (split-major, split s taking
+// every splits-th group of kMismatchWaves buses, its waves one bus each), as
+// a tile's buses are a latency-bound chain per wave; also keeps a copy of
+// the step (the refinement adds its correction to it).
 template <typename Real, bool PerRowY>
 __global__ __launch_bounds__(kLanes* kMismatchWaves) void k_ir_residual(IrArgs<Real> a)
 {
     const int lane = threadIdx.x & (kLanes - 1);
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kLanes));
-    const size_t tile = blockIdx.x;
+    const int splits = a.splits > 1 ? a.splits : 1;
+    const int T = static_cast<int>(gridDim.x) / splits;
+    const int split = static_cast<int>(blockIdx.x) / T;
+    const size_t tile = blockIdx.x - static_cast<size_t>(split) * T;
     if (a.tile_active[tile] <= 0) return;
     const size_t m = static_cast<size_t>(a.d) * kLanes;
-    for (size_t k = threadIdx.x; k < m; k += blockDim.x) a.Xs[tile * m + k] = a.X[tile * m + k];
+    for (size_t k = static_cast<size_t>(split) * blockDim.x + threadIdx.x; k < m;
+         k += static_cast<size_t>(blockDim.x) * splits)
+        a.Xs[tile * m + k] = a.X[tile * m + k];
     // dV of bus b at V = v: V (j dtheta + dVm / |V|)
     auto dv = [&](int bus, double2 v) {
         const int kt = a.colmap[2 * bus], kv = a.colmap[2 * bus + 1];
@@ -1571,7 +1661,7 @@ __global__ __launch_bounds__(kLanes* kMismatchWaves) void k_ir_residual(IrArgs<R
         const double s = vm / sqrt(fma(v.x, v.x, v.y * v.y));
         return make_double2(fma(v.x, s, -v.y * th), fma(v.y, s, v.x * th));
     };
-    for (int bb = wave; bb < a.nbus; bb += kMismatchWaves) {
+    for (int bb = split * kMismatchWaves + wave; bb < a.nbus; bb += kMismatchWaves * splits) {
         const int* bus = a.k1_bus + kK1BusWords * bb;
         const int i = bus[0], e0 = bus[1], e1 = bus[2], fp = bus[3], fq = bus[4];
         const double2 vi = a.V[tidx(tile, a.n, i, lane)];
@@ -2298,13 +2388,19 @@ template __global__ void k_slots_to_rows<double2>(const double2*, int, int, cons
 template __global__ void k_move_vals<int>(const int*, int*, const int*, int, int, int);
 template __global__ void k_move_vals<double>(const double*, double*, const int*, int, int, int);
 template __global__ void k_move_vals<unsigned char>(const unsigned char*, unsigned char*, const int*, int, int, int);
-template __global__ void k_factor<double, 1, false, 0>(FactorArgs<double>);
-template __global__ void k_factor<float, 1, false, 0>(FactorArgs<float>);
-template __global__ void k_factor<double, 2, false, 0>(FactorArgs<double>);
-template __global__ void k_factor<float, 2, false, 0>(FactorArgs<float>);
-template __global__ void k_factor<float, 2, false, kHalfRingGeometry>(FactorArgs<float>);
-template __global__ void k_factor<double, 1, true, 0>(FactorArgs<double>);
-template __global__ void k_factor<float, 1, true, 0>(FactorArgs<float>);
+template __global__ void k_factor<double, 1, false, 0, kFactorFull>(FactorArgs<double>);
+template __global__ void k_factor<float, 1, false, 0, kFactorFull>(FactorArgs<float>);
+template __global__ void k_factor<double, 2, false, 0, kFactorFull>(FactorArgs<double>);
+template __global__ void k_factor<float, 2, false, 0, kFactorFull>(FactorArgs<float>);
+template __global__ void k_factor<float, 2, false, kHalfRingGeometry, kFactorFull>(FactorArgs<float>);
+template __global__ void k_factor<double, 1, true, 0, kFactorFull>(FactorArgs<double>);
+template __global__ void k_factor<float, 1, true, 0, kFactorFull>(FactorArgs<float>);
+template __global__ void k_factor<float, 1, false, 0, kFactorStoreL>(FactorArgs<float>);
+template __global__ void k_factor<float, 2, false, 0, kFactorStoreL>(FactorArgs<float>);
+template __global__ void k_factor<float, 2, false, kHalfRingGeometry, kFactorStoreL>(FactorArgs<float>);
+template __global__ void k_factor<float, 1, false, 0, kForwardOnly>(FactorArgs<float>);
+template __global__ void k_factor<float, 2, false, 0, kForwardOnly>(FactorArgs<float>);
+template __global__ void k_factor<float, 2, false, kHalfRingGeometry, kForwardOnly>(FactorArgs<float>);
 template __global__ void k_backward<double, 1, false>(BackwardArgs<double>);
 template __global__ void k_backward<float, 1, false>(BackwardArgs<float>);
 template __global__ void k_backward<double, 2, false>(BackwardArgs<double>);

@@ -138,6 +138,11 @@ __host__ __device__ constexpr int split_ring_bytes(int real_size, int RW)
            (kProgChunk * kProgSlots + kProgMirror) * 4 + kWalkFlagBytes;
 }
 
+// k_factor's walk modes: the factorization; the factorization that also
+// stores its L multipliers over the A stream's L values; the forward
+// substitution alone over such a stream
+constexpr int kFactorFull = 0, kFactorStoreL = 1, kForwardOnly = 2;
+
 // XCDs of the device: a launch's workgroups go to them round robin
 constexpr int kXcds = 8;
 

@@ -81,3 +81,36 @@ def test_fp64_residual_rejected_with_extension_rows():
     B = _capi.Batch(P, 4)
     with pytest.raises(ValueError):
         B.set_refinement_residual(True)
+
+
+@pytest.mark.parametrize("grid,ir", [("c1", 1), ("c1", 2), ("118", 1)])
+def test_forward_walk_refinement_equals_a_second_factorization(monkeypatch, grid, ir):
+    """The refinement's correction solve walks the factorization's program
+    forward only, over the multipliers the factorization stored in the A
+    stream; a second full factorization (G2G_IR_REFACTOR=1) gives bitwise the
+    same solves.  The 2,000-bus plan has two walkers and columns longer than
+    the kernels' unrolled cases; the 118-bus plan one walker.  (The split walk,
+    which a batch of a few tiles would take, refactorizes.)"""
+    g = syn.grid_c1() if grid == "c1" else syn.grid_118()
+    rows = 192
+    _say(f"forward-walk refinement {grid} ir={ir}")
+    p, q = syn.injection_scenarios(g, rows, seed=35)
+    S = (p + 1j * q) / g.sn_mva
+    outs = syn.random_outages(g, rows, seed=36)
+    plan = _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack)
+    if grid == "c1":
+        assert plan.info()["walkers"] == 2 and plan.info()["max_c"] > 7
+    monkeypatch.setenv("G2G_SPLIT", "0")  # the main walk (a few tiles would take the split walk)
+    got = {}
+    for refactor in ("0", "1"):
+        monkeypatch.setenv("G2G_IR_REFACTOR", refactor)
+        B = _batch(plan, S, g.flat_start(), _capi.MIXED, ir, True, outs, g)
+        B.solve(max_iter=10, tol=1e-8)
+        got[refactor] = (B.get_v(), *B.results())
+    V0, st0, it0, r0 = got["0"]
+    V1, st1, it1, r1 = got["1"]
+    np.testing.assert_array_equal(st0, st1)
+    np.testing.assert_array_equal(it0, it1)
+    assert (st0 == 0).sum() > rows // 2
+    np.testing.assert_array_equal(V0, V1)
+    np.testing.assert_array_equal(r0, r1)
