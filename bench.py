@@ -482,7 +482,8 @@ def main():
             ok = (stk == 0) & (st64 == 0)
             methods[k]["vs_fp64"] = dict(status_equal=bool(np.array_equal(stk, st64)),
                                          max_dV=float(np.abs(Vk[ok] - V64[ok]).max()) if ok.any() else None)
-    head = methods.get("warm") or methods.get("mixed") or methods.get("fp64")
+    head = (methods.get("warm") or methods.get("mixed") or methods.get("fp64")
+            or next(v for k, v in methods.items() if not k.startswith("facade_")))
     fp64 = methods.get("fp64")
     dom = head
     roof = dict(bound="hbm", kernel="k_factor",

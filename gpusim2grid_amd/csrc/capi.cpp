@@ -595,13 +595,13 @@ void launch_factor_mode(const SymbolicLU& s, int rg, size_t lds, const FactorArg
     HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     if constexpr (sizeof(Real) == 4) {
         if (rg == kHalfRingGeometry) {
-            k_factor<Real, 2, false, kHalfRingGeometry, MODE><<<T, kLanes * factor_walk_waves(2), lds, st>>>(a);
+            k_factor<Real, 2, false, kHalfRingGeometry, MODE><<<T, kLanes * factor_walk_waves(2, MODE), lds, st>>>(a);
             HIPCHK(hipGetLastError());
             return;
         }
     }
-    if (s.walkers == 2) k_factor<Real, 2, false, 0, MODE><<<T, kLanes * factor_walk_waves(2), lds, st>>>(a);
-    else k_factor<Real, 1, false, 0, MODE><<<T, kLanes * factor_walk_waves(1), lds, st>>>(a);
+    if (s.walkers == 2) k_factor<Real, 2, false, 0, MODE><<<T, kLanes * factor_walk_waves(2, MODE), lds, st>>>(a);
+    else k_factor<Real, 1, false, 0, MODE><<<T, kLanes * factor_walk_waves(1, MODE), lds, st>>>(a);
     HIPCHK(hipGetLastError());
 }
 
@@ -885,10 +885,12 @@ void refine_fp64(g2g_batch_s* b, double eps, int T)
             launch_backward<Real>(P, static_cast<const Real*>(b->U), b->u_stride, b->tile_active, X, b->error,
                                   b->active_rows, T, b->stream);
         }
-        const int per_tile = nr.d * kLanes;
-        k_axpy_tiles<Real><<<dim3(std::min(64, (per_tile + 255) / 256), T), 256, 0, b->stream>>>(Xs, X,
-                                                                                              b->tile_active, nr.d);
-        HIPCHK(hipGetLastError());
+        if (k + 1 < b->ir_steps) {  // the last correction is added by the update
+            const int per_tile = nr.d * kLanes;
+            k_axpy_tiles<Real><<<dim3(std::min(64, (per_tile + 255) / 256), T), 256, 0, b->stream>>>(
+                Xs, X, b->tile_active, nr.d);
+            HIPCHK(hipGetLastError());
+        }
     }
 }
 
@@ -945,8 +947,11 @@ void launch_update(g2g_batch_s* b)
     const int T = live_tiles(b);
     // step scaling: the row's maxima in one workgroup
     const int splits = b->max_dva > 0.0 ? 1 : launch_splits(b, T);
+    // FP64-residual refinement leaves its last correction in X and the step
+    // before it in Xs: the update adds them
+    const Real* add = b->ir_steps > 0 && b->ir_fp64 ? static_cast<const Real*>(b->Xs) : nullptr;
     UpdateArgs<Real> a{static_cast<const Real*>(b->X), P->d_upd, P->n_upd, b->tile_active, b->status, b->V,
-                       P->nr.d, P->nr.n, b->active_rows, b->max_dva, b->max_dvm, splits};
+                       P->nr.d, P->nr.n, b->active_rows, b->max_dva, b->max_dvm, splits, add};
     k_update<Real><<<T * splits, kLanes * kMisWaves, 0, b->stream>>>(a);
     HIPCHK(hipGetLastError());
 }

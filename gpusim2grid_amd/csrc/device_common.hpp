@@ -179,6 +179,7 @@ struct UpdateArgs {
     int d, n, rows;
     double max_dva, max_dvm;  // step scaling bounds; max_dva <= 0: off
     int splits;               // workgroups per tile (bus ranges); > 1 only with step scaling off
+    const Real* Xadd;         // nullable: the step is X + Xadd (added in Real, the refinement's last correction)
 };
 
 // Refinement with an FP64 residual: r = F(V) - J(V) dx, both from V in FP64

@@ -1118,14 +1118,14 @@ __device__ __forceinline__ bool head_ok(int h, int max_c)
 // SPLIT: one walker per workgroup over one program of the split walk (the
 // launch covers programs seg0 .., T tiles each, program-major), smaller rings,
 // dump and gather records; clamp counts are added per row.
-template <int W, bool SPLIT>
+template <int W, bool SPLIT, int MODE>
 __host__ __device__ constexpr int factor_waves()
 {
-    return SPLIT ? 2 + kSplitProducers : factor_walk_waves(W);
+    return SPLIT ? 2 + kSplitProducers : factor_walk_waves(W, MODE);
 }
 
 template <typename Real, int W, bool SPLIT, int RG, int MODE>
-__global__ __launch_bounds__((kLanes * factor_waves<W, SPLIT>())) void k_factor(FactorArgs<Real> a)
+__global__ __launch_bounds__((kLanes * factor_waves<W, SPLIT, MODE>())) void k_factor(FactorArgs<Real> a)
 {
     static_assert(!SPLIT || W == 1, "a split-walk workgroup has one walker");
     static_assert(!SPLIT || MODE == kFactorFull, "the split walk factorizes in full");
@@ -1133,7 +1133,7 @@ __global__ __launch_bounds__((kLanes * factor_waves<W, SPLIT>())) void k_factor(
     constexpr int kCh = value_chunk(sizeof(Real), RW), kNch = value_ring_slots(sizeof(Real), RW);
     static_assert((kNch - 2) * kCh >= 6 + 4 * kFastC2 - 1, "the ring window spans a two-column step's values");
     using ARing = Ring<Real, kCh, kNch, false, true>;
-    constexpr int kNP = SPLIT ? kSplitProducers : factor_value_producers(W), kP = kNP / W;
+    constexpr int kNP = SPLIT ? kSplitProducers : factor_value_producers(W, MODE), kP = kNP / W;
     static_assert(kP * W == kNP, "producers split evenly");
     static_assert(sizeof(typename ARing::Ctl) + sizeof(ProgRing::Ctl) <= kWalkerCtlBytes, "control blocks");
     const int lane = threadIdx.x & (kLanes - 1);
@@ -1564,6 +1564,11 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_update(UpdateArgs<Real> a
     const int part = split * kMisWaves + wave, parts = splits * kMisWaves;
     const int b0 = static_cast<int>((static_cast<long long>(a.nbus) * part) / parts);
     const int b1 = static_cast<int>((static_cast<long long>(a.nbus) * (part + 1)) / parts);
+    // the step of unknown k (with the refinement's last correction added in Real)
+    auto step = [&](int k) -> Real {
+        const size_t q = tidx(tile, a.d, k, lane);
+        return a.Xadd ? a.X[q] + a.Xadd[q] : a.X[q];
+    };
     // Step scaling (acpf_nr_kernels.cu:677-723): alpha = min(1, max_dva /
     // max|dVa|, max_dvm / max|dVm|) per row, the maxima NaN-sticky so that a
     // non-finite step is left unscaled for the status rule to catch.
@@ -1574,10 +1579,10 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_update(UpdateArgs<Real> a
         double mt = 0.0, mv = 0.0;
         for (int bb = b0; bb < b1; ++bb) {
             const int kt = a.upd[3 * bb + 1], kv = a.upd[3 * bb + 2];
-            const double at = fabs(static_cast<double>(a.X[tidx(tile, a.d, kt, lane)]));
+            const double at = fabs(static_cast<double>(step(kt)));
             if (!(mt >= at)) mt = at;
             if (kv >= 0) {
-                const double av = fabs(static_cast<double>(a.X[tidx(tile, a.d, kv, lane)]));
+                const double av = fabs(static_cast<double>(step(kv)));
                 if (!(mv >= av)) mv = av;
             }
         }
@@ -1604,8 +1609,8 @@ __global__ __launch_bounds__(kLanes* kMisWaves) void k_update(UpdateArgs<Real> a
             const int bb = bq + q < b1 ? bq + q : b1 - 1;
             const int kt = a.upd[3 * bb + 1], kv = a.upd[3 * bb + 2];
             bus[q] = bq + q < b1 ? a.upd[3 * bb] : -1;
-            dva[q] = static_cast<double>(a.X[tidx(tile, a.d, kt, lane)]);
-            dvm[q] = kv >= 0 ? static_cast<double>(a.X[tidx(tile, a.d, kv, lane)]) : 0.0;
+            dva[q] = static_cast<double>(step(kt));
+            dvm[q] = kv >= 0 ? static_cast<double>(step(kv)) : 0.0;
             v[q] = a.V[tidx(tile, a.n, a.upd[3 * bb], lane)];
         }
 #pragma unroll

@@ -95,12 +95,23 @@ constexpr int kWalkFlagBytes = 1280;
 // for two (eight waves either way, which leaves 256 VGPRs per wave)
 __host__ __device__ constexpr int value_producers(int W) { return W == 1 ? kValueProducers : 2 * W; }
 __host__ __device__ constexpr int walk_waves(int W) { return 2 * W + value_producers(W); }
+// k_factor's walk modes: the factorization; the factorization that also
+// stores its L multipliers in a stream of the A stream's layout; the forward
+// substitution alone over such a stream
+constexpr int kFactorFull = 0, kFactorStoreL = 1, kForwardOnly = 2;
 // value-stream producers of a factorization workgroup: as above, except two
 // walkers, which may take more (their walk runs at up to 168 VGPRs, so twelve
-// waves fit a CU)
+// waves fit a CU), and the forward-only walk (about 40 VGPRs), which consumes
+// its stream several times faster than a factorization: twelve waves
 constexpr int kFactorProducers2 = 4;
-__host__ __device__ constexpr int factor_value_producers(int W) { return W == 2 ? kFactorProducers2 : value_producers(W); }
-__host__ __device__ constexpr int factor_walk_waves(int W) { return 2 * W + factor_value_producers(W); }
+__host__ __device__ constexpr int factor_value_producers(int W, int mode = kFactorFull)
+{
+    return mode == kForwardOnly ? 12 - 2 * W : W == 2 ? kFactorProducers2 : value_producers(W);
+}
+__host__ __device__ constexpr int factor_walk_waves(int W, int mode = kFactorFull)
+{
+    return 2 * W + factor_value_producers(W, mode);
+}
 // value ring geometry of a walker: positions per chunk and ring slots (four
 // walkers stream half-size chunks, so their rings fit the LDS together)
 __host__ __device__ constexpr int value_chunk(int real_size, int W = 1)
@@ -137,11 +148,6 @@ __host__ __device__ constexpr int split_ring_bytes(int real_size, int RW)
     return kWalkerCtlBytes + value_chunk(real_size, RW) * value_ring_slots(real_size, RW) * kLanes * real_size +
            (kProgChunk * kProgSlots + kProgMirror) * 4 + kWalkFlagBytes;
 }
-
-// k_factor's walk modes: the factorization; the factorization that also
-// stores its L multipliers over the A stream's L values; the forward
-// substitution alone over such a stream
-constexpr int kFactorFull = 0, kFactorStoreL = 1, kForwardOnly = 2;
 
 // XCDs of the device: a launch's workgroups go to them round robin
 constexpr int kXcds = 8;
