@@ -361,7 +361,7 @@ def cpu_baseline(args, wl, keep):
 
 
 def run_serving(args, torch, capi, syn):
-    """C4: Poisson arrivals over the 118-, 2,000- and 6,500-bus grids, 0-3
+    """The serving workload: Poisson arrivals over the 118-, 2,000- and 6,500-bus grids, 0-3
     tripped branches per request, resident per-grid batches (continuous
     batching with slot refill), warm start + mixed precision; throughput and
     p50 / p99 latency from arrival to result."""
