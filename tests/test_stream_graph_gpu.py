@@ -18,11 +18,13 @@ def _say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("prec", [_capi.FP64, _capi.MIXED])
-def test_graph_rounds_equal_eager_rounds(prec):
+@pytest.mark.parametrize("prec,refine", [(_capi.FP64, False), (_capi.MIXED, False), (_capi.MIXED, True)])
+def test_graph_rounds_equal_eager_rounds(prec, refine):
+    """refine: one FP64-residual refinement round per update, whose
+    multiplier-storing factorization and forward-only walk are captured too."""
     g = syn.grid_118()
     rows, slots = 500, 128
-    _say(f"stream graph prec={prec}")
+    _say(f"stream graph prec={prec} refine={refine}")
     p, q = syn.injection_scenarios(g, rows, seed=21)
     S = (p + 1j * q) / g.sn_mva
     outs = syn.random_outages(g, rows, seed=22)
@@ -31,6 +33,9 @@ def test_graph_rounds_equal_eager_rounds(prec):
     Be = _capi.Batch(plan, slots, precision=prec, per_row_ybus=True)
     Be.set_stream_graph(False)
     for B in (Bg, Be):
+        if refine:
+            B.set_step_control(ir_steps=1)
+            B.set_refinement_residual(True)
         B.stream_begin()
     free = list(range(slots))
     busy = {}
