@@ -679,20 +679,28 @@ __device__ __forceinline__ int factor_pivot(AR& A, ProgReader& P, FactorState<Re
     load_values<NV>(A, st.ap, av);
     auto at = [&](int q) -> Real* { return slot_at<Real>(st.lb, word(w, q)); };
     if constexpr (MODE == kForwardOnly) {
-        // sources [diag, b(k), ...]; A stream [F, diag, -L / U(k,k) (stored), ...]
+        // sources [diag, b(k), ...]; A stream [F, diag, -L / U(k,k) (stored), ...];
+        // the refs it needs read one word each
+        const unsigned* wr = reinterpret_cast<const unsigned*>(P.ring) + (st.p & (ProgRing::kRing - 1));
+        unsigned rt[C > 0 ? C : 1];
+#pragma unroll
+        for (int a = 0; a < C; ++a) rt[a] = wr[NS + C * C + a];
+        const unsigned rb = wr[2], head = wr[0];
+        __builtin_amdgcn_sched_barrier(0);  // every ref's read issued before any slot read
+        auto at = [&](unsigned ref) -> Real* { return slot_at<Real>(st.lb, ref); };
         Real tb[C > 0 ? C : 1];
 #pragma unroll
-        for (int a = 0; a < C; ++a) tb[a] = *at(NS + C * C + a);
-        const Real bk = *at(2);
-        *at(2) = Real(0);
+        for (int a = 0; a < C; ++a) tb[a] = *at(rt[a]);
+        const Real bk = *at(rb);
+        *at(rb) = Real(0);
         const Real fk = av[0] + bk;
         stream_store(&st.U[static_cast<size_t>(st.up) * kLanes], fk);
 #pragma unroll
-        for (int a = 0; a < C; ++a) *at(NS + C * C + a) = fma(av[2 + a], fk, tb[a]);
+        for (int a = 0; a < C; ++a) *at(rt[a]) = fma(av[2 + a], fk, tb[a]);
         st.up += 2 + C;
         st.ap += NV;
         st.p += NW;
-        return static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(word(w, 0))) >> 16);
+        return static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(head)) >> 16);
     }
     const int ap0 = st.ap;
     Real sv[NV];
@@ -780,25 +788,33 @@ __device__ __forceinline__ int factor_pivot2(AR& A, ProgReader& P, FactorState<R
     load_values<NV>(A, st.ap, av);
     auto at = [&](int q) -> Real* { return slot_at<Real>(st.lb, word(w, q)); };
     if constexpr (MODE == kForwardOnly) {
-        // stored: l at 2, L(S,k) / U(k,k) at 3 .., L(S,k+1) / U(k+1,k+1) at K1 + 2 ..
+        // stored: l at 2, L(S,k) / U(k,k) at 3 .., L(S,k+1) / U(k+1,k+1) at K1 + 2 ..;
+        // the refs it needs read one word each (a wide read whose other words
+        // are dead lets the compiler reuse their registers and wait in between)
+        const unsigned* wr = reinterpret_cast<const unsigned*>(P.ring) + (st.p & (ProgRing::kRing - 1));
+        unsigned rt[C > 0 ? C : 1];
+#pragma unroll
+        for (int a = 0; a < C; ++a) rt[a] = wr[NS + C * C + a];
+        const unsigned r0 = wr[2], r1 = wr[2 + K1], head = wr[0];
+        __builtin_amdgcn_sched_barrier(0);  // every ref's read issued before any slot read
+        auto at = [&](unsigned ref) -> Real* { return slot_at<Real>(st.lb, ref); };
         Real tb[C > 0 ? C : 1];
 #pragma unroll
-        for (int a = 0; a < C; ++a) tb[a] = *at(NS + C * C + a);
-        const Real b0 = *at(2), b1 = *at(2 + K1);
-        *at(2) = Real(0);
-        *at(2 + K1) = Real(0);
+        for (int a = 0; a < C; ++a) tb[a] = *at(rt[a]);
+        const Real b0 = *at(r0), b1 = *at(r1);
+        *at(r0) = Real(0);
+        *at(r1) = Real(0);
         const Real fk = av[0] + b0;
         const Real fk1 = fma(-av[2], fk, av[K1] + b1);
         Real* u = st.U + static_cast<size_t>(st.up) * kLanes;
         stream_store(&u[0], fk);
         stream_store(&u[static_cast<size_t>(3 + C) * kLanes], fk1);
 #pragma unroll
-        for (int a = 0; a < C; ++a)
-            *at(NS + C * C + a) = fma(-av[K1 + 2 + a], fk1, fma(-av[3 + a], fk, tb[a]));
+        for (int a = 0; a < C; ++a) *at(rt[a]) = fma(-av[K1 + 2 + a], fk1, fma(-av[3 + a], fk, tb[a]));
         st.up += 5 + 2 * C;
         st.ap += NV;
         st.p += NW;
-        return static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(word(w, 0))) >> 16);
+        return static_cast<int>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(head)) >> 16);
     }
     const int ap0 = st.ap;
     Real sv[NV];
