@@ -235,6 +235,12 @@ def run_method(args, name, wl, torch, capi, ws=None):
     alg_bytes = sum((info["nJ"] + info["nLU"]) * s_r * s["rows_factored"] for s in stats)
     launches = sum(s["n_factor_launches"] for s in stats)
     ms_factor = sum(s["ms_factor"] for s in stats)
+    # the main-layout walk alone (whole-tile launches of the dominant kernel;
+    # the split walk's tail launches are another kernel)
+    n_main = sum(s["n_factor_main"] for s in stats)
+    ms_main = sum(s["ms_factor_main"] for s in stats)
+    rows_main = sum(s["rows_factored_main"] for s in stats)
+    main_bytes = (info["nJ"] + info["nLU"]) * s_r * rows_main
     out = dict(
         method=name, pfps=conv * args.steps / dt, ms_per_step=1e3 * dt / args.steps,
         analysis_s=t_plan, pfps_incl_analysis=conv / (dt / args.steps + t_plan),
@@ -250,6 +256,11 @@ def run_method(args, name, wl, torch, capi, ws=None):
         factor_alg_bytes_per_launch=alg_bytes / max(launches, 1),
         factor_achieved_gbs=alg_bytes / (ms_factor * 1e-3) / 1e9 if ms_factor > 0 else None,
         factor_rows_per_launch=sum(s["rows_factored"] for s in stats) / max(launches, 1),
+        main_launches=n_main / args.steps,
+        main_launch_ms=ms_main / max(n_main, 1),
+        main_rows_per_launch=rows_main / max(n_main, 1),
+        main_alg_bytes_per_launch=main_bytes / max(n_main, 1),
+        main_achieved_gbs=main_bytes / (ms_main * 1e-3) / 1e9 if ms_main > 0 else None,
         real="fp64" if prec == capi.FP64 else "f32",
         info=info,
     )
@@ -486,15 +497,23 @@ def main():
             or next(v for k, v in methods.items() if not k.startswith("facade_")))
     fp64 = methods.get("fp64")
     dom = head
-    roof = dict(bound="hbm", kernel="k_factor",
-                achieved=dom["factor_achieved_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=(dom["factor_achieved_gbs"] or 0.0) / HBM_PEAK_GBS,
-                traffic=(FACTOR_TRAFFIC_PER_ROW[dom["real"]] * dom["factor_rows_per_launch"]
+    # the dominant kernel: the main-layout factorization walk (k_factor over
+    # whole tiles); every factorization of the method, the split walk's tails
+    # included, beside it
+    roof = dict(bound="hbm", kernel="k_factor (main-layout walk)",
+                achieved=dom["main_achieved_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=(dom["main_achieved_gbs"] or 0.0) / HBM_PEAK_GBS,
+                traffic=(FACTOR_TRAFFIC_PER_ROW[dom["real"]] * dom["main_rows_per_launch"]
                          if args.workload == "c2" else None),
                 traffic_source=("rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, on full k_factor launches "
                                 "(profiles/hbm_traffic_c2_16384.md): bytes per factored row x this run's "
                                 "rows per launch" if args.workload == "c2" else None),
-                alg_bytes_per_launch=dom["factor_alg_bytes_per_launch"], launch_ms=dom["factor_launch_ms"])
+                alg_bytes_per_launch=dom["main_alg_bytes_per_launch"], launch_ms=dom["main_launch_ms"],
+                launches_per_step=dom["main_launches"],
+                all_factorizations=dict(achieved=dom["factor_achieved_gbs"],
+                                        frac=(dom["factor_achieved_gbs"] or 0.0) / HBM_PEAK_GBS,
+                                        launch_ms=dom["factor_launch_ms"],
+                                        alg_bytes_per_launch=dom["factor_alg_bytes_per_launch"]))
     line = dict(
         metric=METRIC, value=head["pfps"], unit="PF/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
         ms_per_step=head["ms_per_step"], higher_is_better=True, scaling="weak", vs_baseline=None,

@@ -60,7 +60,8 @@ class SolveStats(ctypes.Structure):
                 ("ms_factor", ctypes.c_double), ("ms_backward", ctypes.c_double),
                 ("bytes_factor", ctypes.c_int64), ("rows_factored", ctypes.c_int64),
                 ("n_factor_launches", ctypes.c_int32), ("n_requeued", ctypes.c_int32),
-                ("ms_warm_start", ctypes.c_double)]
+                ("ms_warm_start", ctypes.c_double), ("ms_factor_main", ctypes.c_double),
+                ("rows_factored_main", ctypes.c_int64), ("n_factor_main", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

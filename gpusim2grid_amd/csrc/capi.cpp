@@ -1103,6 +1103,8 @@ struct SolveTimer {
     g2g_batch_s* b;
     int nev = 0;
     std::vector<std::pair<int, int>> ph[4];  // mismatch, factor, backward + update, warm start
+    std::vector<char> factor_main;          // per factorization: a main-layout walk (not the split walk)
+    long long rows_main = 0;                // rows of the main-layout factorizations
     explicit SolveTimer(g2g_batch_s* bb, int max_launches) : b(bb)
     {
         while (static_cast<int>(b->ev.size()) < 2 * max_launches + 4) {
@@ -1117,12 +1119,13 @@ struct SolveTimer {
         HIPCHK(hipEventRecord(b->ev[nev], b->stream));
         return nev++;
     }
-    double ms(int k) const
+    double ms(int k, bool main_only = false) const
     {
         double t = 0.0;
-        for (auto& pr : ph[k]) {
+        for (size_t q = 0; q < ph[k].size(); ++q) {
+            if (main_only && !(q < factor_main.size() && factor_main[q])) continue;
             float m = 0.f;
-            HIPCHK(hipEventElapsedTime(&m, b->ev[pr.first], b->ev[pr.second]));
+            HIPCHK(hipEventElapsedTime(&m, b->ev[ph[k][q].first], b->ev[ph[k][q].second]));
             t += m;
         }
         return t;
@@ -1418,6 +1421,8 @@ int nr_loop_body(g2g_batch_s* b, int max_iter, double tol, int freeze, double gr
                                 b->u_stride, b->tile_active, b->jmax, b->clamped, b->error, b->active_rows, b->T,
                                 eps, b->stream, factor_walk_mode<Real>(b), multiplier_stream<Real>(b));
         tm.ph[1].push_back({e0, tm.mark()});
+        tm.factor_main.push_back(b->split_layout ? 0 : 1);
+        if (!b->split_layout) tm.rows_main += *b->h_active;
         rows_factored += *b->h_active;
         e0 = tm.mark();
         if (b->split_layout)
@@ -1596,6 +1601,9 @@ void fill_stats(g2g_batch_s* b, const SolveTimer& tm, int e_begin, int e_end, in
                           static_cast<int64_t>(b->active_rows);
     stats->rows_factored = rows_factored;
     stats->n_factor_launches = static_cast<int32_t>(tm.ph[1].size());
+    stats->ms_factor_main = tm.ms(1, true);
+    stats->rows_factored_main = tm.rows_main;
+    for (char m : tm.factor_main) stats->n_factor_main += m ? 1 : 0;
     std::vector<int> st(static_cast<size_t>(b->active_rows));
     if (!st.empty()) HIPCHK(hipMemcpy(st.data(), b->status, st.size() * sizeof(int), hipMemcpyDeviceToHost));
     for (int v : st) {

@@ -97,6 +97,11 @@ typedef struct {
     int32_t n_factor_launches;
     int32_t n_requeued;     /* warm solves: rows restarted from the caller's V   */
     double ms_warm_start;   /* warm solves: MLP inference                        */
+    /* the main-layout factorization launches alone (whole-tile walks; the
+     * split walk's tail launches excluded): time, rows, launches */
+    double ms_factor_main;
+    int64_t rows_factored_main;
+    int32_t n_factor_main;
 } g2g_solve_stats;
 
 const char *g2g_last_error(void);
