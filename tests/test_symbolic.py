@@ -172,11 +172,11 @@ LDS_BYTES = 160 * 1024
 
 def ring_bytes(W, real_size):
     """program_format.hpp walk_ring_bytes: per walker a control area, a value
-    ring (four walkers stream half-size chunks) and a program ring, then the
-    cross-walker flags."""
+    ring (four walkers stream half-size chunks) and a program ring with its
+    128-word mirror, then the cross-walker flags."""
     chunk = (16 if real_size == 8 else 32) // (2 if W >= 4 else 1)
     ring_slots = (8 if real_size == 8 else 4) if W == 1 else 4
-    return W * (256 + chunk * ring_slots * 64 * real_size + 512 * 4 * 4) + 1280
+    return W * (256 + chunk * ring_slots * 64 * real_size + (512 * 4 + 128) * 4) + 1280
 
 
 def walk_lds(inf, real_size):
@@ -382,8 +382,8 @@ def test_split_walk_programs_solve_like_the_main_walk():
     assert (sf >= 0).all() and np.unique(pos).size == pos.size and pos.max() < inf["split_a_len"]
     for rs in (4, 8):
         chunk = 32 if rs == 4 else 16  # full chunks (factorization), half chunks (backward)
-        ring_f = 256 + chunk * 4 * 64 * rs + 512 * 4 * 4 + 1280
-        ring_b = 256 + chunk // 2 * 4 * 64 * rs + 512 * 4 * 4 + 1280
+        ring_f = 256 + chunk * 4 * 64 * rs + (512 * 4 + 128) * 4 + 1280
+        ring_b = 256 + chunk // 2 * 4 * 64 * rs + (512 * 4 + 128) * 4 + 1280
         assert ring_f + (inf["slots_fs"] + 2) * 64 * rs <= LDS_BYTES
         assert ring_b + (inf["slots_bs"] + 2) * 64 * rs <= LDS_BYTES
 
@@ -391,3 +391,4 @@ def test_split_walk_programs_solve_like_the_main_walk():
 def test_small_plans_have_no_split_walk():
     g = syn.grid_118()
     assert _capi.Plan.from_grid(g.Ybus, g.pv, g.pq, g.slack).info()["n_split"] == 0
+
