@@ -62,7 +62,8 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--methods", default=None,
                     help="comma list (default: fp64,mixed,mixed_ir,warm plus, on c2, facade_mixed,facade_warm)")
-    ap.add_argument("--train-rows", type=int, default=65536, help="warm start: solved training scenarios")
+    ap.add_argument("--train-rows", type=int, default=200_000,
+                    help="warm start: solved training scenarios (the workload text's 200k solver solutions)")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--hidden", type=int, default=1024)
     ap.add_argument("--train-batch", type=int, default=512)
@@ -87,10 +88,6 @@ def parse():
     args = ap.parse_args()
     if args.methods is None:
         args.methods = "fp64,mixed,mixed_ir,warm,warm_ir" + (",facade_mixed,facade_warm" if args.workload == "c2" else "")
-    if args.workload == "c3":  # the learned-warm-start workload's defaults: 200k training rows
-        for k, v in (("train_rows", 200_000),):
-            if getattr(args, k) == ap.get_default(k):
-                setattr(args, k, v)
     return args
 
 
