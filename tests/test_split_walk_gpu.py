@@ -39,7 +39,13 @@ def test_split_walk_matches_oracle(monkeypatch, prec):
     monkeypatch.setenv("G2G_SPLIT", "1")
     V, st, it, stats = _solve(g, S, rows, prec)
     monkeypatch.setenv("G2G_SPLIT", "0")
-    Vm, stm, itm, _ = _solve(g, S, rows, prec)
+    Vm, stm, itm, stats_m = _solve(g, S, rows, prec)
+    # the solve stats split the factorizations by walk: none is a main-layout
+    # walk when the split walk is forced, all are when it is off
+    assert stats["n_factor_main"] == 0 and stats["ms_factor_main"] == 0.0 and stats["rows_factored_main"] == 0
+    assert stats_m["n_factor_main"] == stats_m["n_factor_launches"] > 0
+    assert stats_m["rows_factored_main"] == stats_m["rows_factored"]
+    assert 0.0 < stats_m["ms_factor_main"] <= stats_m["ms_factor"] * 1.0001
     Vo, sto, ito, _ = orc.nr_batch(g.Ybus, g.flat_start(), S, g.pv, g.pq, 10, 1e-8, n_threads=16)
     np.testing.assert_array_equal(st, sto)
     np.testing.assert_array_equal(stm, sto)
