@@ -574,16 +574,32 @@ struct StreamShape<float> {
 using ProgRing = Ring<int, kProgChunk, kProgSlots, false, false, kProgMirror>;
 using ProgReader = ProgRing::Reader;
 
-// NW program words from position p (4-aligned) as 16-byte broadcast reads:
-// one base address and immediate offsets (a record running past the ring's
-// end reads on into its mirror)
-template <int NW>
+// The first NU of a record's NW program words from position p (4-aligned)
+// as broadcast reads: one base address and immediate offsets (a record
+// running past the ring's end reads on into its mirror).  Only the used
+// words are read: a padding word loaded into a register is dead at once, and
+// the register allocator may give it to a later load or address, which then
+// waits for this read and every LDS read issued before it.
+template <int NW, int NU = NW>
 __device__ __forceinline__ void read_words(const ProgReader& P, int p, v4u (&w)[NW / 4])
 {
     static_assert(NW <= kProgMirror, "a record read at once fits the ring's mirror");
+    static_assert(NU <= NW && NW - NU < 4, "the unused words are the record's padding");
     const v4u* base = reinterpret_cast<const v4u*>(P.ring) + ((p >> 2) & (ProgRing::kRing / 4 - 1));
+    constexpr int NF = NU / 4, R = NU % 4;
 #pragma unroll
-    for (int i = 0; i < NW / 4; ++i) w[i] = base[i];
+    for (int i = 0; i < NF; ++i) w[i] = base[i];
+    if constexpr (R > 0) {
+        const unsigned* t = reinterpret_cast<const unsigned*>(base + NF);
+        if constexpr (R >= 2) {
+            const v2u t2 = *reinterpret_cast<const v2u*>(t);
+            w[NF][0] = t2[0];
+            w[NF][1] = t2[1];
+        } else {
+            w[NF][0] = t[0];
+        }
+        if constexpr (R == 3) w[NF][2] = t[2];
+    }
 }
 
 __device__ __forceinline__ int prog_word(ProgReader& P, int p)
@@ -612,13 +628,13 @@ template <int N, typename RD, typename T>
 __device__ __forceinline__ void load_values(const RD& R, int p, T (&v)[N])
 {
     const int b = p & (RD::kRingLen - 1);
-    if (b + N <= RD::kRingLen) {
+    if (b + N > RD::kRingLen) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = R.at(p + i);
+    } else {
         const T* q = R.ring + static_cast<size_t>(b) * kLanes;
 #pragma unroll
         for (int i = 0; i < N; ++i) v[i] = q[i * kLanes];
-    } else {
-#pragma unroll
-        for (int i = 0; i < N; ++i) v[i] = R.at(p + i);
     }
 }
 
@@ -674,7 +690,7 @@ __device__ __forceinline__ int factor_pivot(AR& A, ProgReader& P, FactorState<Re
     P.window(st.p, st.p + NW - 1);
     A.window(st.ap, st.ap + NV - 1);
     v4u w[NW / 4];
-    read_words<NW>(P, st.p, w);
+    read_words<NW, NS + (C + 1) * C>(P, st.p, w);
     Real av[NV];
     load_values<NV>(A, st.ap, av);
     auto at = [&](int q) -> Real* { return slot_at<Real>(st.lb, word(w, q)); };
@@ -783,7 +799,7 @@ __device__ __forceinline__ int factor_pivot2(AR& A, ProgReader& P, FactorState<R
     P.window(st.p, st.p + NW - 1);
     A.window(st.ap, st.ap + NV - 1);
     v4u w[NW / 4];
-    read_words<NW>(P, st.p, w);
+    read_words<NW, NS + (C + 1) * C>(P, st.p, w);
     Real av[NV];
     load_values<NV>(A, st.ap, av);
     auto at = [&](int q) -> Real* { return slot_at<Real>(st.lb, word(w, q)); };
@@ -1312,7 +1328,7 @@ __device__ __forceinline__ int backward_pivot(UR& U, ProgReader& P, BackState<Re
     constexpr int NW = brec_words(C);
     P.window(st.p, st.p + NW - 1);
     v4u w[NW / 4];
-    read_words<NW>(P, st.p, w);
+    read_words<NW, 3 + C>(P, st.p, w);
     st.up -= C + 2;
     U.window(st.up + C + 1, st.up);  // the record is consumed from its top
     Real uv[C + 2];
@@ -1340,7 +1356,7 @@ __device__ __forceinline__ int backward_pivot2(UR& U, ProgReader& P, BackState<R
     constexpr int NW = brec2_words(C), NU = 5 + 2 * C;
     P.window(st.p, st.p + NW - 1);
     v4u w[NW / 4];
-    read_words<NW>(P, st.p, w);
+    read_words<NW, 5 + C>(P, st.p, w);
     st.up -= NU;
     U.window(st.up + NU - 1, st.up);
     Real uv[NU];

@@ -24,6 +24,7 @@ namespace g2g {
 
 // 16-byte register type (a native vector, so arrays of it stay in VGPRs)
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
 
 constexpr int kSpinLimit = 1 << 22;  // polls (each after s_sleep 2) before a wave gives up
 
