@@ -1317,14 +1317,21 @@ int split_tiles(const g2g_batch_s* b) { return live_tiles(b); }
 // tiles help once they drop a round, or fit the split walk.  (The mismatch
 // and update kernels skip inactive tiles; their saving alone does not repay
 // the row moves, which for per-row Ybus batches carry the whole Ybus.)
+// A batch in its own row order about to take the split walk is packed
+// whatever the counts: that walk covers every live tile, which is all of
+// such a batch's tiles, and its layout switch zeroes their A stream (twice:
+// on and off), a cost in proportion to the batch, not to the few rows left.
 // G2G_COMPACT=0 / 1: never / whenever packing frees a quarter of the active
-// tiles.
+// tiles (or precedes the split walk).
 constexpr int kCompactMinTiles = 16;
 bool worth_compacting(const g2g_batch_s* b, int active_rows, int active_tiles)
 {
     const int packed = (active_rows + kLanes - 1) / kLanes;
+    const char* e = std::getenv("G2G_COMPACT");
+    if (e && std::atoi(e) == 0) return false;
+    if (b->compact_R == 0 && !b->split_layout && packed < b->T && use_split(b, active_tiles)) return true;
     if (4 * packed > 3 * active_tiles) return false;
-    if (const char* e = std::getenv("G2G_COMPACT")) return std::atoi(e) != 0 && packed < active_tiles;
+    if (e) return packed < active_tiles;
     if (active_tiles - packed < kCompactMinTiles) return false;
     const int rounds_now = (active_tiles + b->n_cu - 1) / b->n_cu;
     const int rounds_packed = (packed + b->n_cu - 1) / b->n_cu;
